@@ -1,0 +1,225 @@
+// Workload scenes shared by the product, its tests and benchmarks, and the
+// CPU checkers. A scene is a plain list of body and fixture definitions in
+// creation order; every backend replays it through its own API, so the same
+// bodies, fixtures and proxy creation order reach every engine.
+//
+// Scenes (sizes and parameters are the workloads the engine is measured on):
+//   hello   : HelloWorld (HelloWorld/HelloWorld.cpp:30-107): ground box
+//             50x10 at (0,-10), one dynamic 1x1 box at (0,4), density 1,
+//             friction 0.3.
+//   pyramid : 20-row box pyramid, 210 dynamic boxes of half-extent 0.5,
+//             density 1, friction 0.6, static ground edge (-40,0)-(40,0).
+//   pile    : N bodies from seed on a 300x600 m grid, 50% circles
+//             (r ~ U[0.25,0.75]), 50% convex polygons (3..8 points on a
+//             circle of R ~ U[0.3,0.8] with random phase, hulled by
+//             b2PolygonShape::Set), density 1, friction 0.6, restitution 0,
+//             inside a static U-shaped container of three edges.
+//   worlds  : many independent piles of log-uniform size in [16,1024],
+//             each with its own container (one scene per world).
+#pragma once
+#include <stdint.h>
+#include <math.h>
+#include <vector>
+#include <random>
+#include <algorithm>
+#include <string>
+
+namespace b2scene {
+
+enum ShapeKind { kCircle = 0, kEdge = 1, kPolygon = 2 };
+enum PolyMode { kPolySet = 0, kPolyBox = 1, kPolyBoxRotated = 2 };
+
+struct BodyDef {
+    int type;              // 0 static, 1 kinematic, 2 dynamic (b2BodyType order)
+    float px, py, angle;
+    float vx, vy, w;
+    float linearDamping, angularDamping, gravityScale;
+    int allowSleep, awake, fixedRotation, bullet;
+    int firstFixture, fixtureCount;
+};
+
+struct FixtureDef {
+    int shape;             // ShapeKind
+    int polyMode;          // PolyMode, polygons only
+    int n;                 // polygon: number of input points
+    float pts[16];         // polygon input points (x0,y0,x1,y1,...) or box params
+    float radius, cx, cy;  // circle: radius and local centre
+    float e[8];            // edge: v0, v1, v2, v3 (x,y pairs)
+    int has0, has3;        // edge: ghost vertex flags
+    float density, friction, restitution;
+    int sensor;
+    int categoryBits, maskBits, groupIndex;
+};
+
+struct Scene {
+    float gx = 0.0f, gy = -10.0f;
+    std::vector<BodyDef> bodies;
+    std::vector<FixtureDef> fixtures;
+    int dynamicCount() const {
+        int n = 0;
+        for (auto& b : bodies) n += (b.type == 2);
+        return n;
+    }
+};
+
+inline BodyDef defaultBody(int type, float x, float y) {
+    BodyDef b{};
+    b.type = type; b.px = x; b.py = y; b.angle = 0.0f;
+    b.vx = b.vy = b.w = 0.0f;
+    b.linearDamping = 0.0f; b.angularDamping = 0.0f; b.gravityScale = 1.0f;
+    b.allowSleep = 1; b.awake = 1; b.fixedRotation = 0; b.bullet = 0;
+    b.firstFixture = 0; b.fixtureCount = 0;
+    return b;
+}
+
+inline FixtureDef defaultFixture(int shape) {
+    FixtureDef f{};
+    f.shape = shape; f.polyMode = kPolySet; f.n = 0;
+    f.radius = 0.0f; f.cx = f.cy = 0.0f;
+    f.has0 = f.has3 = 0;
+    f.density = 0.0f; f.friction = 0.2f; f.restitution = 0.0f;
+    f.sensor = 0; f.categoryBits = 1; f.maskBits = 0xFFFF; f.groupIndex = 0;
+    return f;
+}
+
+inline void addBody(Scene& s, const BodyDef& b) {
+    s.bodies.push_back(b);
+    s.bodies.back().firstFixture = (int)s.fixtures.size();
+    s.bodies.back().fixtureCount = 0;
+}
+
+inline void addFixture(Scene& s, const FixtureDef& f) {
+    s.fixtures.push_back(f);
+    s.bodies.back().fixtureCount++;
+}
+
+inline FixtureDef boxFixture(float hx, float hy, float density, float friction) {
+    FixtureDef f = defaultFixture(kPolygon);
+    f.polyMode = kPolyBox; f.pts[0] = hx; f.pts[1] = hy;
+    f.density = density; f.friction = friction;
+    return f;
+}
+
+inline FixtureDef edgeFixture(float x1, float y1, float x2, float y2) {
+    FixtureDef f = defaultFixture(kEdge);
+    f.e[2] = x1; f.e[3] = y1; f.e[4] = x2; f.e[5] = y2;
+    return f;
+}
+
+// HelloWorld/HelloWorld.cpp:30-107
+inline Scene hello() {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, -10.0f));
+    addFixture(s, boxFixture(50.0f, 10.0f, 0.0f, 0.2f));
+    addBody(s, defaultBody(2, 0.0f, 4.0f));
+    addFixture(s, boxFixture(1.0f, 1.0f, 1.0f, 0.3f));
+    return s;
+}
+
+// 20-row pyramid on a ground edge; the layout accumulates the row offsets
+// in FP32 exactly as a b2Vec2 loop would.
+inline Scene pyramid(int rows = 20, float density = 1.0f, float friction = 0.6f) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-40.0f, 0.0f, 40.0f, 0.0f));
+    float xx = -7.0f, xy = 0.75f;
+    const float dxx = 0.5625f, dxy = 1.25f;
+    const float dyx = 1.125f, dyy = 0.0f;
+    for (int i = 0; i < rows; ++i) {
+        float yx = xx, yy = xy;
+        for (int j = i; j < rows; ++j) {
+            addBody(s, defaultBody(2, yx, yy));
+            addFixture(s, boxFixture(0.5f, 0.5f, density, friction));
+            yx += dyx; yy += dyy;
+        }
+        xx += dxx; xy += dxy;
+    }
+    return s;
+}
+
+// Uniform float in [0,1) from 24 high bits of one MT19937 draw.
+struct Uniform {
+    std::mt19937 mt;
+    explicit Uniform(uint32_t seed) : mt(seed) {}
+    float operator()() { return (float)(mt() >> 8) * (1.0f / 16777216.0f); }
+};
+
+// Random pile: N dynamic bodies from `seed` on a width x height grid with
+// its lower edge at y = 5, in a U container of half-width width/2 + 5 and
+// wall height height + 50. `scale` shrinks the whole layout for the small
+// per-world piles of the batched scene.
+inline Scene pile(int n, uint32_t seed, float width = 300.0f, float height = 600.0f) {
+    Scene s;
+    const float half = 0.5f * width + 5.0f;
+    const float wallTop = height + 50.0f;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-half, 0.0f, half, 0.0f));
+    addFixture(s, edgeFixture(-half, 0.0f, -half, wallTop));
+    addFixture(s, edgeFixture(half, 0.0f, half, wallTop));
+    for (auto& f : s.fixtures) f.friction = 0.6f;
+
+    Uniform U(seed);
+    const int cols = (int)ceil(sqrt((double)n * (double)width / (double)height));
+    const int rows = (n + cols - 1) / cols;
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < n; ++i) {
+        const int c = i % cols, r = i / cols;
+        const float x = (float)(-0.5 * width + (c + 0.5) * (double)width / cols);
+        const float y = (float)(5.0 + (r + 0.5) * (double)height / rows);
+        addBody(s, defaultBody(2, x, y));
+        FixtureDef f;
+        if (i & 1) {
+            f = defaultFixture(kCircle);
+            f.radius = 0.25f + 0.5f * U();
+        } else {
+            f = defaultFixture(kPolygon);
+            const int k = 3 + std::min(5, (int)(6.0f * U()));
+            const float R = 0.3f + 0.5f * U();
+            const double phase = 2.0 * pi * (double)U();
+            f.polyMode = kPolySet; f.n = k;
+            for (int j = 0; j < k; ++j) {
+                const double t = phase + 2.0 * pi * j / k;
+                f.pts[2 * j + 0] = (float)((double)R * cos(t));
+                f.pts[2 * j + 1] = (float)((double)R * sin(t));
+            }
+        }
+        f.density = 1.0f; f.friction = 0.6f; f.restitution = 0.0f;
+        addFixture(s, f);
+    }
+    return s;
+}
+
+// Body count of world w in the batched scene: floor(16 * 64^U), U from seed.
+inline std::vector<int> worldSizes(int nWorlds, uint32_t seed, int lo = 16, int hi = 1024) {
+    Uniform U(seed);
+    std::vector<int> sizes(nWorlds);
+    const double ratio = (double)hi / (double)lo;
+    for (int w = 0; w < nWorlds; ++w) {
+        int v = (int)floor((double)lo * pow(ratio, (double)U()));
+        sizes[w] = std::min(hi, std::max(lo, v));
+    }
+    return sizes;
+}
+
+// One world of the batched scene: a pile whose grid is sized so bodies start
+// at roughly the same spacing as the large pile (about 1.34 m).
+inline Scene batchedWorld(int n, uint32_t seed) {
+    const float spacing = 1.3416f;
+    const int cols = std::max(4, (int)ceil(sqrt((double)n * 0.5)));
+    const int rows = (n + cols - 1) / cols;
+    return pile(n, seed, cols * spacing, rows * spacing);
+}
+
+// Scene by name: "hello", "pyramid" (n = rows, 0 -> 20), "pile" (n bodies,
+// seed), "world" (one batched-scene world of n bodies, seed). Unknown names
+// give an empty scene.
+inline Scene byName(const char* kind, int n, uint32_t seed) {
+    const std::string k(kind ? kind : "");
+    if (k == "hello") return hello();
+    if (k == "pyramid") return pyramid(n > 0 ? n : 20);
+    if (k == "pile") return pile(n, seed);
+    if (k == "world") return batchedWorld(n, seed);
+    return Scene();
+}
+
+}  // namespace b2scene

@@ -1,0 +1,1948 @@
+// ORACLE TEST INFRASTRUCTURE - not part of the product. See cpu_ref.h.
+//
+// Each function names the upstream file:line it restates (REF =
+// Regression/box2d-ref/Box2DRef). Arithmetic is written out in the same
+// order as upstream so the results are bit-identical; the data structures
+// are this file's own (index-linked lists instead of pointers, one flat
+// pool per entity).
+#include "cpu_ref.h"
+
+#include <math.h>
+#include <float.h>
+#include <string.h>
+#include <vector>
+#include <algorithm>
+
+#include "scenes/scenes.h"
+
+namespace orc {
+
+// ------------------------------------------------------------ constants
+// REF/Common/b2Settings.h:40-150
+constexpr float kPi = 3.14159265359f;
+constexpr float kLinearSlop = 0.005f;
+constexpr float kAngularSlop = 2.0f / 180.0f * kPi;
+constexpr float kPolygonRadius = 2.0f * kLinearSlop;
+constexpr float kAabbExtension = 0.1f;
+constexpr float kAabbMultiplier = 2.0f;
+constexpr float kVelocityThreshold = 1.0f;
+constexpr float kMaxLinearCorrection = 0.2f;
+constexpr float kMaxTranslation = 2.0f;
+constexpr float kMaxTranslationSq = kMaxTranslation * kMaxTranslation;
+constexpr float kMaxRotation = 0.5f * kPi;
+constexpr float kMaxRotationSq = kMaxRotation * kMaxRotation;
+constexpr float kBaumgarte = 0.2f;
+constexpr float kTimeToSleep = 0.5f;
+constexpr float kLinearSleepTol = 0.01f;
+constexpr float kAngularSleepTol = 2.0f / 180.0f * kPi;
+
+// ------------------------------------------------------------ FP32 math
+// REF/Common/b2Math.h: the operators below evaluate in the same order.
+struct V2 { float x, y; };
+inline V2 mk(float x, float y) { V2 v; v.x = x; v.y = y; return v; }
+inline V2 operator+(V2 a, V2 b) { return mk(a.x + b.x, a.y + b.y); }
+inline V2 operator-(V2 a, V2 b) { return mk(a.x - b.x, a.y - b.y); }
+inline V2 operator-(V2 a) { return mk(-a.x, -a.y); }
+inline V2 operator*(float s, V2 a) { return mk(s * a.x, s * a.y); }
+inline void operator+=(V2& a, V2 b) { a.x += b.x; a.y += b.y; }
+inline void operator-=(V2& a, V2 b) { a.x -= b.x; a.y -= b.y; }
+inline void operator*=(V2& a, float s) { a.x *= s; a.y *= s; }
+inline float dot(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
+inline float cross(V2 a, V2 b) { return a.x * b.y - a.y * b.x; }
+inline V2 cross(V2 a, float s) { return mk(s * a.y, -s * a.x); }
+inline V2 cross(float s, V2 a) { return mk(-s * a.y, s * a.x); }
+inline float lengthSq(V2 a) { return a.x * a.x + a.y * a.y; }
+inline float length(V2 a) { return sqrtf(a.x * a.x + a.y * a.y); }
+inline float distSq(V2 a, V2 b) { V2 c = a - b; return dot(c, c); }
+inline float vmin(float a, float b) { return a < b ? a : b; }
+inline float vmax(float a, float b) { return a > b ? a : b; }
+inline V2 vmin(V2 a, V2 b) { return mk(vmin(a.x, b.x), vmin(a.y, b.y)); }
+inline V2 vmax(V2 a, V2 b) { return mk(vmax(a.x, b.x), vmax(a.y, b.y)); }
+inline float clampf(float a, float lo, float hi) { return vmax(lo, vmin(a, hi)); }
+inline float absf(float a) { return a > 0.0f ? a : -a; }
+
+// b2Vec2::Normalize (b2Math.h): zero-length vectors stay as they are.
+inline float normalize(V2& v) {
+    float len = length(v);
+    if (len < FLT_EPSILON) return 0.0f;
+    float inv = 1.0f / len;
+    v.x *= inv;
+    v.y *= inv;
+    return len;
+}
+
+struct Rot { float s, c; };
+inline Rot rotOf(float angle) { Rot q; q.s = sinf(angle); q.c = cosf(angle); return q; }
+struct Xf { V2 p; Rot q; };
+inline V2 mul(Rot q, V2 v) { return mk(q.c * v.x - q.s * v.y, q.s * v.x + q.c * v.y); }
+inline V2 mulT(Rot q, V2 v) { return mk(q.c * v.x + q.s * v.y, -q.s * v.x + q.c * v.y); }
+inline V2 mul(const Xf& T, V2 v) {
+    float x = (T.q.c * v.x - T.q.s * v.y) + T.p.x;
+    float y = (T.q.s * v.x + T.q.c * v.y) + T.p.y;
+    return mk(x, y);
+}
+inline V2 mulT(const Xf& T, V2 v) {
+    float px = v.x - T.p.x;
+    float py = v.y - T.p.y;
+    float x = (T.q.c * px + T.q.s * py);
+    float y = (-T.q.s * px + T.q.c * py);
+    return mk(x, y);
+}
+inline Rot mulT(Rot q, Rot r) {
+    Rot o;
+    o.s = q.c * r.s - q.s * r.c;
+    o.c = q.c * r.c + q.s * r.s;
+    return o;
+}
+inline Xf mulT(const Xf& A, const Xf& B) {
+    Xf C;
+    C.q = mulT(A.q, B.q);
+    C.p = mulT(A.q, B.p - A.p);
+    return C;
+}
+
+struct AABB { V2 lo, hi; };
+// b2TestOverlap(b2AABB, b2AABB), REF/Collision/b2Collision.h:247-259
+inline bool overlap(const AABB& a, const AABB& b) {
+    V2 d1 = b.lo - a.hi;
+    V2 d2 = a.lo - b.hi;
+    if (d1.x > 0.0f || d1.y > 0.0f) return false;
+    if (d2.x > 0.0f || d2.y > 0.0f) return false;
+    return true;
+}
+// b2AABB::Contains (b2Collision.h:200-208)
+inline bool contains(const AABB& a, const AABB& b) {
+    return a.lo.x <= b.lo.x && a.lo.y <= b.lo.y && b.hi.x <= a.hi.x && b.hi.y <= a.hi.y;
+}
+
+inline V2 v(const b2g_vec2& a) { return mk(a.x, a.y); }
+inline b2g_vec2 g(V2 a) { b2g_vec2 o; o.x = a.x; o.y = a.y; return o; }
+inline Xf xfOf(const b2g_transform* t) { Xf x; x.p = mk(t->px, t->py); x.q.s = t->s; x.q.c = t->c; return x; }
+
+// ------------------------------------------------------------ shapes
+// REF/Collision/Shapes/b2PolygonShape.cpp:42-61 (SetAsBox)
+void setAsBox(b2g_shape* s, float hx, float hy) {
+    memset(s, 0, sizeof(*s));
+    s->type = B2G_POLYGON;
+    s->radius = kPolygonRadius;
+    s->count = 4;
+    s->verts[0] = g(mk(-hx, -hy)); s->verts[1] = g(mk(hx, -hy));
+    s->verts[2] = g(mk(hx, hy));   s->verts[3] = g(mk(-hx, hy));
+    s->normals[0] = g(mk(0.0f, -1.0f)); s->normals[1] = g(mk(1.0f, 0.0f));
+    s->normals[2] = g(mk(0.0f, 1.0f));  s->normals[3] = g(mk(-1.0f, 0.0f));
+}
+
+// REF/Collision/Shapes/b2PolygonShape.cpp:63-89 (SetAsBox with centre and angle)
+void setAsBoxRotated(b2g_shape* s, float hx, float hy, V2 center, float angle) {
+    setAsBox(s, hx, hy);
+    s->centroid = g(center);
+    Xf xf;
+    xf.p = center;
+    xf.q = rotOf(angle);
+    for (int i = 0; i < 4; ++i) {
+        s->verts[i] = g(mul(xf, v(s->verts[i])));
+        s->normals[i] = g(mul(xf.q, v(s->normals[i])));
+    }
+}
+
+// REF/Collision/Shapes/b2PolygonShape.cpp:96-142 (ComputeCentroid, origin reference)
+V2 polygonCentroid(const V2* vs, int count) {
+    V2 c = mk(0.0f, 0.0f);
+    float area = 0.0f;
+    const V2 pRef = mk(0.0f, 0.0f);
+    const float inv3 = 1.0f / 3.0f;
+    for (int i = 0; i < count; ++i) {
+        V2 p1 = pRef, p2 = vs[i], p3 = i + 1 < count ? vs[i + 1] : vs[0];
+        V2 e1 = p2 - p1, e2 = p3 - p1;
+        float D = cross(e1, e2);
+        float tri = 0.5f * D;
+        area += tri;
+        c += tri * inv3 * (p1 + p2 + p3);
+    }
+    c *= 1.0f / area;
+    return c;
+}
+
+// REF/Collision/Shapes/b2PolygonShape.cpp:144-245 (Set: weld, gift-wrap hull,
+// normals, centroid). Degenerate input falls back to a 1x1 box as upstream.
+void polygonSet(b2g_shape* s, const V2* input, int count) {
+    if (count < 3) { setAsBox(s, 1.0f, 1.0f); return; }
+    int n = count < 8 ? count : 8;
+    V2 ps[8];
+    int tc = 0;
+    for (int i = 0; i < n; ++i) {
+        V2 p = input[i];
+        bool unique = true;
+        for (int j = 0; j < tc; ++j) {
+            if (distSq(p, ps[j]) < 0.5f * kLinearSlop) { unique = false; break; }
+        }
+        if (unique) ps[tc++] = p;
+    }
+    n = tc;
+    if (n < 3) { setAsBox(s, 1.0f, 1.0f); return; }
+    int i0 = 0;
+    float x0 = ps[0].x;
+    for (int i = 1; i < n; ++i) {
+        float x = ps[i].x;
+        if (x > x0 || (x == x0 && ps[i].y < ps[i0].y)) { i0 = i; x0 = x; }
+    }
+    int hull[8];
+    int m = 0, ih = i0;
+    for (;;) {
+        hull[m] = ih;
+        int ie = 0;
+        for (int j = 1; j < n; ++j) {
+            if (ie == ih) { ie = j; continue; }
+            V2 r = ps[ie] - ps[hull[m]];
+            V2 w = ps[j] - ps[hull[m]];
+            float c = cross(r, w);
+            if (c < 0.0f) ie = j;
+            if (c == 0.0f && lengthSq(w) > lengthSq(r)) ie = j;
+        }
+        ++m;
+        ih = ie;
+        if (ie == i0) break;
+    }
+    if (m < 3) { setAsBox(s, 1.0f, 1.0f); return; }
+    memset(s, 0, sizeof(*s));
+    s->type = B2G_POLYGON;
+    s->radius = kPolygonRadius;
+    s->count = m;
+    V2 vs[8];
+    for (int i = 0; i < m; ++i) { vs[i] = ps[hull[i]]; s->verts[i] = g(vs[i]); }
+    for (int i = 0; i < m; ++i) {
+        V2 e = vs[i + 1 < m ? i + 1 : 0] - vs[i];
+        V2 nrm = cross(e, 1.0f);
+        normalize(nrm);
+        s->normals[i] = g(nrm);
+    }
+    s->centroid = g(polygonCentroid(vs, m));
+}
+
+void circleShape(b2g_shape* s, V2 p, float r) {
+    memset(s, 0, sizeof(*s));
+    s->type = B2G_CIRCLE;
+    s->radius = r;
+    s->centroid = g(p);
+}
+
+void edgeShape(b2g_shape* s, V2 v0, V2 v1, V2 v2, V2 v3, bool has0, bool has3) {
+    memset(s, 0, sizeof(*s));
+    s->type = B2G_EDGE;
+    s->radius = kPolygonRadius;
+    s->count = 2;
+    s->verts[0] = g(v1); s->verts[1] = g(v2);
+    s->v0 = g(v0); s->v3 = g(v3);
+    s->flags = (has0 ? 1 : 0) | (has3 ? 2 : 0);
+}
+
+// ComputeAABB: polygon b2PolygonShape.cpp:335-352, circle b2CircleShape.cpp:85-94,
+// edge b2EdgeShape.cpp:118-131.
+AABB shapeAABB(const b2g_shape* s, const Xf& xf) {
+    AABB a;
+    if (s->type == B2G_CIRCLE) {
+        V2 p = xf.p + mul(xf.q, v(s->centroid));
+        a.lo = mk(p.x - s->radius, p.y - s->radius);
+        a.hi = mk(p.x + s->radius, p.y + s->radius);
+        return a;
+    }
+    const int n = s->type == B2G_EDGE ? 2 : s->count;
+    V2 lo = mul(xf, v(s->verts[0]));
+    V2 hi = lo;
+    for (int i = 1; i < n; ++i) {
+        V2 w = mul(xf, v(s->verts[i]));
+        lo = vmin(lo, w);
+        hi = vmax(hi, w);
+    }
+    V2 r = mk(s->radius, s->radius);
+    a.lo = lo - r;
+    a.hi = hi + r;
+    return a;
+}
+
+struct MassData { float mass; V2 center; float I; };
+
+// ComputeMass: polygon b2PolygonShape.cpp:354-437, circle b2CircleShape.cpp:96-103,
+// edge b2EdgeShape.cpp:133-139.
+MassData shapeMass(const b2g_shape* s, float density) {
+    MassData md;
+    if (s->type == B2G_CIRCLE) {
+        md.mass = density * kPi * s->radius * s->radius;
+        md.center = v(s->centroid);
+        md.I = md.mass * (0.5f * s->radius * s->radius + dot(md.center, md.center));
+        return md;
+    }
+    if (s->type == B2G_EDGE) {
+        md.mass = 0.0f;
+        md.center = 0.5f * (v(s->verts[0]) + v(s->verts[1]));
+        md.I = 0.0f;
+        return md;
+    }
+    const int n = s->count;
+    V2 center = mk(0.0f, 0.0f);
+    float area = 0.0f, I = 0.0f;
+    V2 sref = mk(0.0f, 0.0f);
+    for (int i = 0; i < n; ++i) sref += v(s->verts[i]);
+    sref *= 1.0f / n;
+    const float k_inv3 = 1.0f / 3.0f;
+    for (int i = 0; i < n; ++i) {
+        V2 e1 = v(s->verts[i]) - sref;
+        V2 e2 = i + 1 < n ? v(s->verts[i + 1]) - sref : v(s->verts[0]) - sref;
+        float D = cross(e1, e2);
+        float tri = 0.5f * D;
+        area += tri;
+        center += tri * k_inv3 * (e1 + e2);
+        float ex1 = e1.x, ey1 = e1.y, ex2 = e2.x, ey2 = e2.y;
+        float intx2 = ex1 * ex1 + ex2 * ex1 + ex2 * ex2;
+        float inty2 = ey1 * ey1 + ey2 * ey1 + ey2 * ey2;
+        I += (0.25f * k_inv3 * D) * (intx2 + inty2);
+    }
+    md.mass = density * area;
+    center *= 1.0f / area;
+    md.center = center + sref;
+    md.I = density * I;
+    md.I += md.mass * (dot(md.center, md.center) - dot(center, center));
+    return md;
+}
+
+// ------------------------------------------------------------ narrowphase
+struct ClipVertex { V2 v; uint32_t id; };
+
+inline uint32_t makeId(int indexA, int indexB, int typeA, int typeB) {
+    return (uint32_t)(indexA & 0xFF) | ((uint32_t)(indexB & 0xFF) << 8) |
+           ((uint32_t)(typeA & 0xFF) << 16) | ((uint32_t)(typeB & 0xFF) << 24);
+}
+inline uint32_t swapId(uint32_t id) {
+    uint32_t iA = id & 0xFF, iB = (id >> 8) & 0xFF, tA = (id >> 16) & 0xFF, tB = (id >> 24) & 0xFF;
+    return iB | (iA << 8) | (tB << 16) | (tA << 24);
+}
+constexpr int kVertex = 0, kFace = 1;
+
+void clearManifold(b2g_manifold* m) { memset(m, 0, sizeof(*m)); }
+
+// b2ClipSegmentToLine, REF/Collision/b2Collision.cpp:201-232
+int clipSegment(ClipVertex out[2], const ClipVertex in[2], V2 normal, float offset, int vertexIndexA) {
+    int n = 0;
+    float d0 = dot(normal, in[0].v) - offset;
+    float d1 = dot(normal, in[1].v) - offset;
+    if (d0 <= 0.0f) out[n++] = in[0];
+    if (d1 <= 0.0f) out[n++] = in[1];
+    if (d0 * d1 < 0.0f) {
+        float interp = d0 / (d0 - d1);
+        out[n].v = in[0].v + interp * (in[1].v - in[0].v);
+        out[n].id = makeId(vertexIndexA, (int)((in[0].id >> 8) & 0xFF), kVertex, kFace);
+        ++n;
+    }
+    return n;
+}
+
+// b2FindMaxSeparation, REF/Collision/b2CollidePolygon.cpp:28-66
+float findMaxSeparation(int* edge, const b2g_shape* p1, const Xf& xf1, const b2g_shape* p2, const Xf& xf2) {
+    Xf xf = mulT(xf2, xf1);
+    int best = 0;
+    float maxSep = -FLT_MAX;
+    for (int i = 0; i < p1->count; ++i) {
+        V2 n = mul(xf.q, v(p1->normals[i]));
+        V2 v1 = mul(xf, v(p1->verts[i]));
+        float si = FLT_MAX;
+        for (int j = 0; j < p2->count; ++j) {
+            float sij = dot(n, v(p2->verts[j]) - v1);
+            if (sij < si) si = sij;
+        }
+        if (si > maxSep) { maxSep = si; best = i; }
+    }
+    *edge = best;
+    return maxSep;
+}
+
+// b2FindIncidentEdge, REF/Collision/b2CollidePolygon.cpp:69-112
+void findIncidentEdge(ClipVertex c[2], const b2g_shape* p1, const Xf& xf1, int edge1,
+                      const b2g_shape* p2, const Xf& xf2) {
+    V2 normal1 = mulT(xf2.q, mul(xf1.q, v(p1->normals[edge1])));
+    int index = 0;
+    float minDot = FLT_MAX;
+    for (int i = 0; i < p2->count; ++i) {
+        float d = dot(normal1, v(p2->normals[i]));
+        if (d < minDot) { minDot = d; index = i; }
+    }
+    int i1 = index, i2 = i1 + 1 < p2->count ? i1 + 1 : 0;
+    c[0].v = mul(xf2, v(p2->verts[i1]));
+    c[0].id = makeId(edge1, i1, kFace, kVertex);
+    c[1].v = mul(xf2, v(p2->verts[i2]));
+    c[1].id = makeId(edge1, i2, kFace, kVertex);
+}
+
+// b2CollidePolygons, REF/Collision/b2CollidePolygon.cpp:121-247
+void collidePolygons(b2g_manifold* m, const b2g_shape* A, const Xf& xfA, const b2g_shape* B, const Xf& xfB) {
+    m->pointCount = 0;
+    float totalRadius = A->radius + B->radius;
+    int edgeA = 0;
+    float sepA = findMaxSeparation(&edgeA, A, xfA, B, xfB);
+    if (sepA > totalRadius) return;
+    int edgeB = 0;
+    float sepB = findMaxSeparation(&edgeB, B, xfB, A, xfA);
+    if (sepB > totalRadius) return;
+    const b2g_shape *p1, *p2;
+    Xf xf1, xf2;
+    int edge1;
+    bool flip;
+    const float k_tol = 0.1f * kLinearSlop;
+    if (sepB > sepA + k_tol) {
+        p1 = B; p2 = A; xf1 = xfB; xf2 = xfA; edge1 = edgeB;
+        m->type = B2G_MANIFOLD_FACE_B; flip = true;
+    } else {
+        p1 = A; p2 = B; xf1 = xfA; xf2 = xfB; edge1 = edgeA;
+        m->type = B2G_MANIFOLD_FACE_A; flip = false;
+    }
+    ClipVertex incident[2];
+    findIncidentEdge(incident, p1, xf1, edge1, p2, xf2);
+    int count1 = p1->count;
+    int iv1 = edge1, iv2 = edge1 + 1 < count1 ? edge1 + 1 : 0;
+    V2 v11 = v(p1->verts[iv1]), v12 = v(p1->verts[iv2]);
+    V2 localTangent = v12 - v11;
+    normalize(localTangent);
+    V2 localNormal = cross(localTangent, 1.0f);
+    V2 planePoint = 0.5f * (v11 + v12);
+    V2 tangent = mul(xf1.q, localTangent);
+    V2 normal = cross(tangent, 1.0f);
+    v11 = mul(xf1, v11);
+    v12 = mul(xf1, v12);
+    float frontOffset = dot(normal, v11);
+    float sideOffset1 = -dot(tangent, v11) + totalRadius;
+    float sideOffset2 = dot(tangent, v12) + totalRadius;
+    ClipVertex cp1[2], cp2[2];
+    if (clipSegment(cp1, incident, -tangent, sideOffset1, iv1) < 2) return;
+    if (clipSegment(cp2, cp1, tangent, sideOffset2, iv2) < 2) return;
+    m->localNormal = g(localNormal);
+    m->localPoint = g(planePoint);
+    int pc = 0;
+    for (int i = 0; i < 2; ++i) {
+        float separation = dot(normal, cp2[i].v) - frontOffset;
+        if (separation <= totalRadius) {
+            b2g_manifold_point* mp = m->points + pc;
+            mp->localPoint = g(mulT(xf2, cp2[i].v));
+            mp->id = flip ? swapId(cp2[i].id) : cp2[i].id;
+            ++pc;
+        }
+    }
+    m->pointCount = pc;
+}
+
+// b2CollideCircles, REF/Collision/b2CollideCircle.cpp:25-50
+void collideCircles(b2g_manifold* m, const b2g_shape* A, const Xf& xfA, const b2g_shape* B, const Xf& xfB) {
+    m->pointCount = 0;
+    V2 pA = mul(xfA, v(A->centroid));
+    V2 pB = mul(xfB, v(B->centroid));
+    V2 d = pB - pA;
+    float distSqr = dot(d, d);
+    float radius = A->radius + B->radius;
+    if (distSqr > radius * radius) return;
+    m->type = B2G_MANIFOLD_CIRCLES;
+    m->localPoint = A->centroid;
+    m->localNormal = g(mk(0.0f, 0.0f));
+    m->pointCount = 1;
+    m->points[0].localPoint = B->centroid;
+    m->points[0].id = 0;
+}
+
+// b2CollidePolygonAndCircle, REF/Collision/b2CollideCircle.cpp:53-157
+void collidePolygonCircle(b2g_manifold* m, const b2g_shape* A, const Xf& xfA, const b2g_shape* B, const Xf& xfB) {
+    m->pointCount = 0;
+    V2 c = mul(xfB, v(B->centroid));
+    V2 cLocal = mulT(xfA, c);
+    int normalIndex = 0;
+    float separation = -FLT_MAX;
+    float radius = A->radius + B->radius;
+    const int n = A->count;
+    for (int i = 0; i < n; ++i) {
+        float s = dot(v(A->normals[i]), cLocal - v(A->verts[i]));
+        if (s > radius) return;
+        if (s > separation) { separation = s; normalIndex = i; }
+    }
+    int vi1 = normalIndex, vi2 = vi1 + 1 < n ? vi1 + 1 : 0;
+    V2 v1 = v(A->verts[vi1]), v2 = v(A->verts[vi2]);
+    if (separation < FLT_EPSILON) {
+        m->pointCount = 1;
+        m->type = B2G_MANIFOLD_FACE_A;
+        m->localNormal = A->normals[normalIndex];
+        m->localPoint = g(0.5f * (v1 + v2));
+        m->points[0].localPoint = B->centroid;
+        m->points[0].id = 0;
+        return;
+    }
+    float u1 = dot(cLocal - v1, v2 - v1);
+    float u2 = dot(cLocal - v2, v1 - v2);
+    if (u1 <= 0.0f) {
+        if (distSq(cLocal, v1) > radius * radius) return;
+        m->pointCount = 1;
+        m->type = B2G_MANIFOLD_FACE_A;
+        V2 nrm = cLocal - v1;
+        normalize(nrm);
+        m->localNormal = g(nrm);
+        m->localPoint = g(v1);
+        m->points[0].localPoint = B->centroid;
+        m->points[0].id = 0;
+    } else if (u2 <= 0.0f) {
+        if (distSq(cLocal, v2) > radius * radius) return;
+        m->pointCount = 1;
+        m->type = B2G_MANIFOLD_FACE_A;
+        V2 nrm = cLocal - v2;
+        normalize(nrm);
+        m->localNormal = g(nrm);
+        m->localPoint = g(v2);
+        m->points[0].localPoint = B->centroid;
+        m->points[0].id = 0;
+    } else {
+        V2 faceCenter = 0.5f * (v1 + v2);
+        float sep = dot(cLocal - faceCenter, v(A->normals[vi1]));
+        if (sep > radius) return;
+        m->pointCount = 1;
+        m->type = B2G_MANIFOLD_FACE_A;
+        m->localNormal = A->normals[vi1];
+        m->localPoint = g(faceCenter);
+        m->points[0].localPoint = B->centroid;
+        m->points[0].id = 0;
+    }
+}
+
+// b2CollideEdgeAndCircle, REF/Collision/b2CollideEdge.cpp:27-151
+void collideEdgeCircle(b2g_manifold* m, const b2g_shape* E, const Xf& xfA, const b2g_shape* C, const Xf& xfB) {
+    m->pointCount = 0;
+    V2 Q = mulT(xfA, mul(xfB, v(C->centroid)));
+    V2 A = v(E->verts[0]), B = v(E->verts[1]);
+    V2 e = B - A;
+    float u = dot(e, B - Q);
+    float w = dot(e, Q - A);
+    float radius = E->radius + C->radius;
+    if (w <= 0.0f) {
+        V2 P = A;
+        V2 d = Q - P;
+        if (dot(d, d) > radius * radius) return;
+        if (E->flags & 1) {
+            V2 A1 = v(E->v0), B1 = A;
+            V2 e1 = B1 - A1;
+            float u1 = dot(e1, B1 - Q);
+            if (u1 > 0.0f) return;
+        }
+        m->pointCount = 1;
+        m->type = B2G_MANIFOLD_CIRCLES;
+        m->localNormal = g(mk(0.0f, 0.0f));
+        m->localPoint = g(P);
+        m->points[0].id = makeId(0, 0, kVertex, kVertex);
+        m->points[0].localPoint = C->centroid;
+        return;
+    }
+    if (u <= 0.0f) {
+        V2 P = B;
+        V2 d = Q - P;
+        if (dot(d, d) > radius * radius) return;
+        if (E->flags & 2) {
+            V2 B2 = v(E->v3), A2 = B;
+            V2 e2 = B2 - A2;
+            float v2 = dot(e2, Q - A2);
+            if (v2 > 0.0f) return;
+        }
+        m->pointCount = 1;
+        m->type = B2G_MANIFOLD_CIRCLES;
+        m->localNormal = g(mk(0.0f, 0.0f));
+        m->localPoint = g(P);
+        m->points[0].id = makeId(1, 0, kVertex, kVertex);
+        m->points[0].localPoint = C->centroid;
+        return;
+    }
+    float den = dot(e, e);
+    V2 P = (1.0f / den) * (u * A + w * B);
+    V2 d = Q - P;
+    if (dot(d, d) > radius * radius) return;
+    V2 n = mk(-e.y, e.x);
+    if (dot(n, Q - A) < 0.0f) n = mk(-n.x, -n.y);
+    normalize(n);
+    m->pointCount = 1;
+    m->type = B2G_MANIFOLD_FACE_A;
+    m->localNormal = g(n);
+    m->localPoint = g(A);
+    m->points[0].id = makeId(0, 0, kFace, kVertex);
+    m->points[0].localPoint = C->centroid;
+}
+
+// b2EPCollider (edge versus polygon with adjacency),
+// REF/Collision/b2CollideEdge.cpp:153-700. The fork's copy of this function
+// has two transcription bugs (Box2D/Collision/b2CollideEdge.cpp:443, :597);
+// the upstream arithmetic is the one restated here.
+struct EPAxis { int type; int index; float separation; };  // type: 0 unknown, 1 edgeA, 2 edgeB
+void collideEdgePolygon(b2g_manifold* m, const b2g_shape* E, const Xf& xfA, const b2g_shape* P, const Xf& xfB) {
+    Xf xf = mulT(xfA, xfB);
+    V2 centroidB = mul(xf, v(P->centroid));
+    V2 v0 = v(E->v0), v1 = v(E->verts[0]), v2 = v(E->verts[1]), v3 = v(E->v3);
+    bool has0 = (E->flags & 1) != 0, has3 = (E->flags & 2) != 0;
+    V2 edge1 = v2 - v1;
+    normalize(edge1);
+    V2 normal0 = mk(0.0f, 0.0f), normal1 = mk(edge1.y, -edge1.x), normal2 = mk(0.0f, 0.0f);
+    float offset1 = dot(normal1, centroidB - v1);
+    float offset0 = 0.0f, offset2 = 0.0f;
+    bool convex1 = false, convex2 = false;
+    if (has0) {
+        V2 edge0 = v1 - v0;
+        normalize(edge0);
+        normal0 = mk(edge0.y, -edge0.x);
+        convex1 = cross(edge0, edge1) >= 0.0f;
+        offset0 = dot(normal0, centroidB - v0);
+    }
+    if (has3) {
+        V2 edge2 = v3 - v2;
+        normalize(edge2);
+        normal2 = mk(edge2.y, -edge2.x);
+        convex2 = cross(edge1, edge2) > 0.0f;
+        offset2 = dot(normal2, centroidB - v2);
+    }
+    bool front;
+    V2 normal, lower, upper;
+    if (has0 && has3) {
+        if (convex1 && convex2) {
+            front = offset0 >= 0.0f || offset1 >= 0.0f || offset2 >= 0.0f;
+            if (front) { normal = normal1; lower = normal0; upper = normal2; }
+            else { normal = -normal1; lower = -normal1; upper = -normal1; }
+        } else if (convex1) {
+            front = offset0 >= 0.0f || (offset1 >= 0.0f && offset2 >= 0.0f);
+            if (front) { normal = normal1; lower = normal0; upper = normal1; }
+            else { normal = -normal1; lower = -normal2; upper = -normal1; }
+        } else if (convex2) {
+            front = offset2 >= 0.0f || (offset0 >= 0.0f && offset1 >= 0.0f);
+            if (front) { normal = normal1; lower = normal1; upper = normal2; }
+            else { normal = -normal1; lower = -normal1; upper = -normal0; }
+        } else {
+            front = offset0 >= 0.0f && offset1 >= 0.0f && offset2 >= 0.0f;
+            if (front) { normal = normal1; lower = normal1; upper = normal1; }
+            else { normal = -normal1; lower = -normal2; upper = -normal0; }
+        }
+    } else if (has0) {
+        if (convex1) {
+            front = offset0 >= 0.0f || offset1 >= 0.0f;
+            if (front) { normal = normal1; lower = normal0; upper = -normal1; }
+            else { normal = -normal1; lower = normal1; upper = -normal1; }
+        } else {
+            front = offset0 >= 0.0f && offset1 >= 0.0f;
+            if (front) { normal = normal1; lower = normal1; upper = -normal1; }
+            else { normal = -normal1; lower = normal1; upper = -normal0; }
+        }
+    } else if (has3) {
+        if (convex2) {
+            front = offset1 >= 0.0f || offset2 >= 0.0f;
+            if (front) { normal = normal1; lower = -normal1; upper = normal2; }
+            else { normal = -normal1; lower = -normal1; upper = normal1; }
+        } else {
+            front = offset1 >= 0.0f && offset2 >= 0.0f;
+            if (front) { normal = normal1; lower = -normal1; upper = normal1; }
+            else { normal = -normal1; lower = -normal2; upper = normal1; }
+        }
+    } else {
+        front = offset1 >= 0.0f;
+        if (front) { normal = normal1; lower = -normal1; upper = -normal1; }
+        else { normal = -normal1; lower = normal1; upper = normal1; }
+    }
+    // Polygon B in frame A.
+    const int count = P->count;
+    V2 pv[8], pn[8];
+    for (int i = 0; i < count; ++i) {
+        pv[i] = mul(xf, v(P->verts[i]));
+        pn[i] = mul(xf.q, v(P->normals[i]));
+    }
+    const float radius = 2.0f * kPolygonRadius;
+    m->pointCount = 0;
+    // ComputeEdgeSeparation (REF :606-623)
+    EPAxis edgeAxis;
+    edgeAxis.type = 1;
+    edgeAxis.index = front ? 0 : 1;
+    edgeAxis.separation = FLT_MAX;
+    for (int i = 0; i < count; ++i) {
+        float s = dot(normal, pv[i] - v1);
+        if (s < edgeAxis.separation) edgeAxis.separation = s;
+    }
+    if (edgeAxis.separation > radius) return;
+    // ComputePolygonSeparation (REF :625-676)
+    EPAxis polyAxis;
+    polyAxis.type = 0;
+    polyAxis.index = -1;
+    polyAxis.separation = -FLT_MAX;
+    {
+        V2 perp = mk(-normal.y, normal.x);
+        for (int i = 0; i < count; ++i) {
+            V2 n = -pn[i];
+            float s1 = dot(n, pv[i] - v1);
+            float s2 = dot(n, pv[i] - v2);
+            float s = vmin(s1, s2);
+            if (s > radius) {
+                polyAxis.type = 2; polyAxis.index = i; polyAxis.separation = s;
+                break;
+            }
+            if (dot(n, perp) >= 0.0f) {
+                if (dot(n - upper, normal) < -kAngularSlop) continue;
+            } else {
+                if (dot(n - lower, normal) < -kAngularSlop) continue;
+            }
+            if (s > polyAxis.separation) {
+                polyAxis.type = 2; polyAxis.index = i; polyAxis.separation = s;
+            }
+        }
+    }
+    if (polyAxis.type != 0 && polyAxis.separation > radius) return;
+    const float k_relativeTol = 0.98f, k_absoluteTol = 0.001f;
+    EPAxis primary;
+    if (polyAxis.type == 0) primary = edgeAxis;
+    else if (polyAxis.separation > k_relativeTol * edgeAxis.separation + k_absoluteTol) primary = polyAxis;
+    else primary = edgeAxis;
+
+    ClipVertex ie[2];
+    int rfi1, rfi2;
+    V2 rfv1, rfv2, rfn;
+    if (primary.type == 1) {
+        m->type = B2G_MANIFOLD_FACE_A;
+        int best = 0;
+        float bestValue = dot(normal, pn[0]);
+        for (int i = 1; i < count; ++i) {
+            float value = dot(normal, pn[i]);
+            if (value < bestValue) { bestValue = value; best = i; }
+        }
+        int i1 = best, i2 = i1 + 1 < count ? i1 + 1 : 0;
+        ie[0].v = pv[i1]; ie[0].id = makeId(0, i1, kFace, kVertex);
+        ie[1].v = pv[i2]; ie[1].id = makeId(0, i2, kFace, kVertex);
+        if (front) { rfi1 = 0; rfi2 = 1; rfv1 = v1; rfv2 = v2; rfn = normal1; }
+        else { rfi1 = 1; rfi2 = 0; rfv1 = v2; rfv2 = v1; rfn = -normal1; }
+    } else {
+        m->type = B2G_MANIFOLD_FACE_B;
+        ie[0].v = v1; ie[0].id = makeId(0, primary.index, kVertex, kFace);
+        ie[1].v = v2; ie[1].id = makeId(0, primary.index, kVertex, kFace);
+        rfi1 = primary.index;
+        rfi2 = rfi1 + 1 < count ? rfi1 + 1 : 0;
+        rfv1 = pv[rfi1]; rfv2 = pv[rfi2]; rfn = pn[rfi1];
+    }
+    V2 side1 = mk(rfn.y, -rfn.x);
+    V2 side2 = -side1;
+    float sideOffset1 = dot(side1, rfv1);
+    float sideOffset2 = dot(side2, rfv2);
+    ClipVertex cp1[2], cp2[2];
+    if (clipSegment(cp1, ie, side1, sideOffset1, rfi1) < 2) return;
+    if (clipSegment(cp2, cp1, side2, sideOffset2, rfi2) < 2) return;
+    if (primary.type == 1) {
+        m->localNormal = g(rfn);
+        m->localPoint = g(rfv1);
+    } else {
+        m->localNormal = P->normals[rfi1];
+        m->localPoint = P->verts[rfi1];
+    }
+    int pc = 0;
+    for (int i = 0; i < 2; ++i) {
+        float separation = dot(rfn, cp2[i].v - rfv1);
+        if (separation <= radius) {
+            b2g_manifold_point* mp = m->points + pc;
+            if (primary.type == 1) {
+                mp->localPoint = g(mulT(xf, cp2[i].v));
+                mp->id = cp2[i].id;
+            } else {
+                mp->localPoint = g(cp2[i].v);
+                mp->id = swapId(cp2[i].id);
+            }
+            ++pc;
+        }
+    }
+    m->pointCount = pc;
+}
+
+// Dispatch for a contact whose shapes are already in registry order
+// (REF/Dynamics/Contacts/b2Contact.cpp:34-43 and each Evaluate override).
+bool collide(b2g_manifold* m, const b2g_shape* A, const Xf& xfA, const b2g_shape* B, const Xf& xfB) {
+    if (A->type == B2G_POLYGON && B->type == B2G_POLYGON) collidePolygons(m, A, xfA, B, xfB);
+    else if (A->type == B2G_CIRCLE && B->type == B2G_CIRCLE) collideCircles(m, A, xfA, B, xfB);
+    else if (A->type == B2G_POLYGON && B->type == B2G_CIRCLE) collidePolygonCircle(m, A, xfA, B, xfB);
+    else if (A->type == B2G_EDGE && B->type == B2G_CIRCLE) collideEdgeCircle(m, A, xfA, B, xfB);
+    else if (A->type == B2G_EDGE && B->type == B2G_POLYGON) collideEdgePolygon(m, A, xfA, B, xfB);
+    else return false;
+    return true;
+}
+
+// b2WorldManifold::Initialize, REF/Collision/b2Collision.cpp:22-86
+struct WorldManifold { V2 normal; V2 points[2]; float separations[2]; };
+void worldManifold(WorldManifold* wm, const b2g_manifold* m, const Xf& xfA, float radiusA, const Xf& xfB, float radiusB) {
+    if (m->pointCount == 0) return;
+    if (m->type == B2G_MANIFOLD_CIRCLES) {
+        wm->normal = mk(1.0f, 0.0f);
+        V2 pointA = mul(xfA, v(m->localPoint));
+        V2 pointB = mul(xfB, v(m->points[0].localPoint));
+        if (distSq(pointA, pointB) > FLT_EPSILON * FLT_EPSILON) {
+            wm->normal = pointB - pointA;
+            normalize(wm->normal);
+        }
+        V2 cA = pointA + radiusA * wm->normal;
+        V2 cB = pointB - radiusB * wm->normal;
+        wm->points[0] = 0.5f * (cA + cB);
+        wm->separations[0] = dot(cB - cA, wm->normal);
+    } else if (m->type == B2G_MANIFOLD_FACE_A) {
+        wm->normal = mul(xfA.q, v(m->localNormal));
+        V2 planePoint = mul(xfA, v(m->localPoint));
+        for (int i = 0; i < m->pointCount; ++i) {
+            V2 clipPoint = mul(xfB, v(m->points[i].localPoint));
+            V2 cA = clipPoint + (radiusA - dot(clipPoint - planePoint, wm->normal)) * wm->normal;
+            V2 cB = clipPoint - radiusB * wm->normal;
+            wm->points[i] = 0.5f * (cA + cB);
+            wm->separations[i] = dot(cB - cA, wm->normal);
+        }
+    } else {
+        wm->normal = mul(xfB.q, v(m->localNormal));
+        V2 planePoint = mul(xfB, v(m->localPoint));
+        for (int i = 0; i < m->pointCount; ++i) {
+            V2 clipPoint = mul(xfA, v(m->points[i].localPoint));
+            V2 cB = clipPoint + (radiusB - dot(clipPoint - planePoint, wm->normal)) * wm->normal;
+            V2 cA = clipPoint - radiusA * wm->normal;
+            wm->points[i] = 0.5f * (cA + cB);
+            wm->separations[i] = dot(cA - cB, wm->normal);
+        }
+        wm->normal = -wm->normal;
+    }
+}
+
+// ------------------------------------------------------------ world
+enum BodyType { kStatic = 0, kKinematic = 1, kDynamic = 2 };
+enum BodyFlag { kIsland = 1, kAwake = 2, kAutoSleep = 4, kBullet = 8, kFixedRotation = 16, kActive = 32 };
+enum ContactFlag { cIsland = 1, cTouching = 2, cEnabled = 4, cFilter = 8 };
+
+struct Body {
+    int type;
+    int flags;
+    Xf xf;
+    V2 localCenter, c0, c;
+    float a0, a;
+    V2 v;
+    float w;
+    V2 force;
+    float torque;
+    float mass, invMass, I, invI;
+    float linearDamping, angularDamping, gravityScale;
+    float sleepTime;
+    int islandIndex;
+    std::vector<int> fixtures;  // creation order; iterate reversed (newest first)
+    int edgeHead;               // contact edge list head: contact*2 + side
+    bool alive;
+};
+
+struct Fixture {
+    int body;
+    b2g_shape shape;
+    float density, friction, restitution;
+    bool sensor;
+    uint16_t category, mask;
+    int16_t group;
+    AABB aabb;   // swept tight AABB of the last synchronize
+    AABB fat;    // fat AABB in the broad-phase tree
+    bool proxy;  // has a broad-phase proxy
+};
+
+struct Edge { int prev, next; };  // contact-edge links (edge ids)
+
+struct Contact {
+    int fA, fB;
+    int flags;
+    float friction, restitution, tangentSpeed;
+    b2g_manifold manifold;
+    int prev, next;  // world list
+    Edge node[2];    // node[0] on body A (other = B), node[1] on body B
+    bool alive;
+};
+
+// b2ContactVelocityConstraint / b2ContactPositionConstraint,
+// REF/Dynamics/Contacts/b2ContactSolver.h:33-84 and b2ContactSolver.cpp:35-50
+struct VelPoint { V2 rA, rB; float normalImpulse, tangentImpulse, normalMass, tangentMass, velocityBias; };
+struct VelConstraint {
+    VelPoint points[2];
+    V2 normal;
+    float K[4];           // ex.x ex.y ey.x ey.y
+    float normalMass[4];
+    int indexA, indexB;
+    float invMassA, invMassB, invIA, invIB;
+    float friction, restitution, tangentSpeed;
+    int pointCount, contactIndex;
+};
+struct PosConstraint {
+    V2 localPoints[2];
+    V2 localNormal, localPoint;
+    int indexA, indexB;
+    float invMassA, invMassB;
+    V2 localCenterA, localCenterB;
+    float invIA, invIB;
+    int type;
+    float radiusA, radiusB;
+    int pointCount;
+};
+
+struct Events { long begin = 0, end = 0; };
+
+struct World {
+    V2 gravity;
+    std::vector<Body> bodies;
+    std::vector<Fixture> fixtures;
+    std::vector<Contact> contacts;
+    std::vector<int> freeContacts;
+    int contactHead = -1;
+    int contactCount = 0;
+    std::vector<int> moveBuffer;
+    bool newFixture = false;
+    bool allowSleep = true, warmStarting = true, continuous = false, subStepping = false;
+    bool autoClearForces = true;
+    bool stepComplete = true;
+    float inv_dt0 = 0.0f;
+    int unsupported = 0;
+    Events events;
+
+    // island scratch
+    std::vector<int> islandBodies, islandContacts, stack;
+    std::vector<V2> posC, velV;
+    std::vector<float> posA, velW;
+    std::vector<VelConstraint> vcs;
+    std::vector<PosConstraint> pcs;
+
+    // ---- body state (REF/Dynamics/b2Body.h:635-654)
+    void setAwake(Body& b, bool flag) {
+        if (flag) {
+            if ((b.flags & kAwake) == 0) { b.flags |= kAwake; b.sleepTime = 0.0f; }
+        } else {
+            b.flags &= ~kAwake;
+            b.sleepTime = 0.0f;
+            b.v = mk(0.0f, 0.0f);
+            b.w = 0.0f;
+            b.force = mk(0.0f, 0.0f);
+            b.torque = 0.0f;
+        }
+    }
+
+    // b2Body::ResetMassData, REF/Dynamics/b2Body.cpp:285-352
+    void resetMassData(Body& b) {
+        b.mass = 0.0f; b.invMass = 0.0f; b.I = 0.0f; b.invI = 0.0f;
+        b.localCenter = mk(0.0f, 0.0f);
+        if (b.type == kStatic || b.type == kKinematic) {
+            b.c0 = b.xf.p; b.c = b.xf.p; b.a0 = b.a;
+            return;
+        }
+        V2 localCenter = mk(0.0f, 0.0f);
+        for (int k = (int)b.fixtures.size() - 1; k >= 0; --k) {
+            const Fixture& f = fixtures[b.fixtures[k]];
+            if (f.density == 0.0f) continue;
+            MassData md = shapeMass(&f.shape, f.density);
+            b.mass += md.mass;
+            localCenter += md.mass * md.center;
+            b.I += md.I;
+        }
+        if (b.mass > 0.0f) {
+            b.invMass = 1.0f / b.mass;
+            localCenter *= b.invMass;
+        } else {
+            b.mass = 1.0f;
+            b.invMass = 1.0f;
+        }
+        if (b.I > 0.0f && (b.flags & kFixedRotation) == 0) {
+            b.I -= b.mass * dot(localCenter, localCenter);
+            b.invI = 1.0f / b.I;
+        } else {
+            b.I = 0.0f;
+            b.invI = 0.0f;
+        }
+        V2 oldCenter = b.c;
+        b.localCenter = localCenter;
+        b.c0 = b.c = mul(b.xf, b.localCenter);
+        b.v += cross(b.w, b.c - oldCenter);
+    }
+
+    // b2World::CreateBody + b2Body::b2Body, REF/Dynamics/b2World.cpp:109-127,
+    // REF/Dynamics/b2Body.cpp:25-90
+    int createBody(const b2scene::BodyDef& d) {
+        Body b;
+        b.flags = 0;
+        if (d.bullet) b.flags |= kBullet;
+        if (d.fixedRotation) b.flags |= kFixedRotation;
+        if (d.allowSleep) b.flags |= kAutoSleep;
+        if (d.awake) b.flags |= kAwake;
+        b.flags |= kActive;
+        b.xf.p = mk(d.px, d.py);
+        b.xf.q = rotOf(d.angle);
+        b.localCenter = mk(0.0f, 0.0f);
+        b.c0 = b.xf.p; b.c = b.xf.p;
+        b.a0 = d.angle; b.a = d.angle;
+        b.v = mk(d.vx, d.vy);
+        b.w = d.w;
+        b.linearDamping = d.linearDamping;
+        b.angularDamping = d.angularDamping;
+        b.gravityScale = d.gravityScale;
+        b.force = mk(0.0f, 0.0f);
+        b.torque = 0.0f;
+        b.sleepTime = 0.0f;
+        b.type = d.type;
+        if (b.type == kDynamic) { b.mass = 1.0f; b.invMass = 1.0f; }
+        else { b.mass = 0.0f; b.invMass = 0.0f; }
+        b.I = 0.0f; b.invI = 0.0f;
+        b.islandIndex = 0;
+        b.edgeHead = -1;
+        b.alive = true;
+        bodies.push_back(b);
+        return (int)bodies.size() - 1;
+    }
+
+    // b2Body::CreateFixture + b2Fixture::CreateProxies + b2BroadPhase::CreateProxy,
+    // REF/Dynamics/b2Body.cpp:164-203, b2Fixture.cpp:124-139, b2DynamicTree.cpp:112-126
+    int createFixture(int bi, const b2g_shape& shape, const b2scene::FixtureDef& d) {
+        Fixture f;
+        f.body = bi;
+        f.shape = shape;
+        f.density = d.density;
+        f.friction = d.friction;
+        f.restitution = d.restitution;
+        f.sensor = d.sensor != 0;
+        f.category = (uint16_t)d.categoryBits;
+        f.mask = (uint16_t)d.maskBits;
+        f.group = (int16_t)d.groupIndex;
+        if (f.sensor) unsupported |= 2;
+        Body& b = bodies[bi];
+        f.proxy = (b.flags & kActive) != 0;
+        if (f.proxy) {
+            f.aabb = shapeAABB(&f.shape, b.xf);
+            V2 r = mk(kAabbExtension, kAabbExtension);
+            f.fat.lo = f.aabb.lo - r;
+            f.fat.hi = f.aabb.hi + r;
+        }
+        fixtures.push_back(f);
+        const int fi = (int)fixtures.size() - 1;
+        if (f.proxy) moveBuffer.push_back(fi);
+        b.fixtures.push_back(fi);
+        if (f.density > 0.0f) resetMassData(b);
+        newFixture = true;
+        return fi;
+    }
+
+    // b2Body::ShouldCollide, REF/Dynamics/b2Body.cpp:396-417 (no joints here)
+    bool shouldCollide(const Body& b, const Body& other) const {
+        return !(b.type != kDynamic && other.type != kDynamic);
+    }
+    // b2ContactFilter::ShouldCollide, REF/Dynamics/b2WorldCallbacks.cpp:26-38
+    static bool filter(const Fixture& a, const Fixture& b) {
+        if (a.group == b.group && a.group != 0) return a.group > 0;
+        return (a.mask & b.category) != 0 && (a.category & b.mask) != 0;
+    }
+
+    // ---- contact-edge lists (edge id = contact*2 + side)
+    Edge& edgeRef(int e) { return contacts[e >> 1].node[e & 1]; }
+    int edgeOther(int e) {
+        const Contact& c = contacts[e >> 1];
+        return (e & 1) == 0 ? fixtures[c.fB].body : fixtures[c.fA].body;
+    }
+    void edgePrepend(int bi, int e) {
+        Edge& n = edgeRef(e);
+        n.prev = -1;
+        n.next = bodies[bi].edgeHead;
+        if (n.next != -1) edgeRef(n.next).prev = e;
+        bodies[bi].edgeHead = e;
+    }
+    void edgeUnlink(int bi, int e) {
+        Edge& n = edgeRef(e);
+        if (n.prev != -1) edgeRef(n.prev).next = n.next;
+        if (n.next != -1) edgeRef(n.next).prev = n.prev;
+        if (bodies[bi].edgeHead == e) bodies[bi].edgeHead = n.next;
+    }
+
+    // b2ContactManager::AddPair + b2Contact::Create,
+    // REF/Dynamics/b2ContactManager.cpp:179-296, b2Contact.cpp:64-95, b2Contact.cpp:136-168
+    void addPair(int fa, int fb) {
+        const int ba = fixtures[fa].body, bb = fixtures[fb].body;
+        if (ba == bb) return;
+        for (int e = bodies[bb].edgeHead; e != -1; e = edgeRef(e).next) {
+            if (edgeOther(e) == ba) {
+                const Contact& c = contacts[e >> 1];
+                if (c.fA == fa && c.fB == fb) return;
+                if (c.fA == fb && c.fB == fa) return;
+            }
+        }
+        if (!shouldCollide(bodies[bb], bodies[ba])) return;
+        if (!filter(fixtures[fa], fixtures[fb])) return;
+        // Registry: circle-circle, polygon-circle, polygon-polygon, edge-circle,
+        // edge-polygon are primary; their mirrors swap the fixtures.
+        int ta = fixtures[fa].shape.type, tb = fixtures[fb].shape.type;
+        int A = fa, B = fb;
+        auto primary = [](int x, int y) {
+            return (x == B2G_CIRCLE && y == B2G_CIRCLE) || (x == B2G_POLYGON && y == B2G_CIRCLE) ||
+                   (x == B2G_POLYGON && y == B2G_POLYGON) || (x == B2G_EDGE && y == B2G_CIRCLE) ||
+                   (x == B2G_EDGE && y == B2G_POLYGON);
+        };
+        if (!primary(ta, tb)) {
+            if (!primary(tb, ta)) return;
+            A = fb; B = fa;
+        }
+        int ci;
+        if (!freeContacts.empty()) { ci = freeContacts.back(); freeContacts.pop_back(); }
+        else { contacts.push_back(Contact()); ci = (int)contacts.size() - 1; }
+        Contact& c = contacts[ci];
+        c.fA = A; c.fB = B;
+        c.flags = cEnabled;
+        clearManifold(&c.manifold);
+        c.friction = sqrtf(fixtures[A].friction * fixtures[B].friction);
+        c.restitution = fixtures[A].restitution > fixtures[B].restitution ? fixtures[A].restitution
+                                                                          : fixtures[B].restitution;
+        c.tangentSpeed = 0.0f;
+        c.alive = true;
+        c.prev = -1;
+        c.next = contactHead;
+        if (contactHead != -1) contacts[contactHead].prev = ci;
+        contactHead = ci;
+        const int bodyA = fixtures[A].body, bodyB = fixtures[B].body;
+        edgePrepend(bodyA, ci * 2 + 0);
+        edgePrepend(bodyB, ci * 2 + 1);
+        if (!fixtures[A].sensor && !fixtures[B].sensor) {
+            setAwake(bodies[bodyA], true);
+            setAwake(bodies[bodyB], true);
+        }
+        ++contactCount;
+    }
+
+    // b2ContactManager::Destroy + b2Contact::Destroy,
+    // REF/Dynamics/b2ContactManager.cpp:38-103, b2Contact.cpp:112-134
+    void destroyContact(int ci) {
+        Contact& c = contacts[ci];
+        const int bodyA = fixtures[c.fA].body, bodyB = fixtures[c.fB].body;
+        if (c.flags & cTouching) events.end++;
+        if (c.prev != -1) contacts[c.prev].next = c.next;
+        if (c.next != -1) contacts[c.next].prev = c.prev;
+        if (contactHead == ci) contactHead = c.next;
+        edgeUnlink(bodyA, ci * 2 + 0);
+        edgeUnlink(bodyB, ci * 2 + 1);
+        if (c.manifold.pointCount > 0 && !fixtures[c.fA].sensor && !fixtures[c.fB].sensor) {
+            setAwake(bodies[bodyA], true);
+            setAwake(bodies[bodyB], true);
+        }
+        c.alive = false;
+        freeContacts.push_back(ci);
+        --contactCount;
+    }
+
+    // b2Contact::Update, REF/Dynamics/Contacts/b2Contact.cpp:171-251
+    void updateContact(int ci) {
+        Contact& c = contacts[ci];
+        b2g_manifold old = c.manifold;
+        c.flags |= cEnabled;
+        bool wasTouching = (c.flags & cTouching) != 0;
+        Body& bA = bodies[fixtures[c.fA].body];
+        Body& bB = bodies[fixtures[c.fB].body];
+        collide(&c.manifold, &fixtures[c.fA].shape, bA.xf, &fixtures[c.fB].shape, bB.xf);
+        bool touching = c.manifold.pointCount > 0;
+        for (int i = 0; i < c.manifold.pointCount; ++i) {
+            b2g_manifold_point* mp2 = c.manifold.points + i;
+            mp2->normalImpulse = 0.0f;
+            mp2->tangentImpulse = 0.0f;
+            for (int j = 0; j < old.pointCount; ++j) {
+                if (old.points[j].id == mp2->id) {
+                    mp2->normalImpulse = old.points[j].normalImpulse;
+                    mp2->tangentImpulse = old.points[j].tangentImpulse;
+                    break;
+                }
+            }
+        }
+        if (touching != wasTouching) {
+            setAwake(bA, true);
+            setAwake(bB, true);
+        }
+        if (touching) c.flags |= cTouching;
+        else c.flags &= ~cTouching;
+        if (!wasTouching && touching) events.begin++;
+        if (wasTouching && !touching) events.end++;
+    }
+
+    // b2ContactManager::Collide, REF/Dynamics/b2ContactManager.cpp:107-172
+    void collideAll() {
+        int ci = contactHead;
+        while (ci != -1) {
+            Contact& c = contacts[ci];
+            const Fixture& fA = fixtures[c.fA];
+            const Fixture& fB = fixtures[c.fB];
+            Body& bA = bodies[fA.body];
+            Body& bB = bodies[fB.body];
+            if (c.flags & cFilter) {
+                if (!shouldCollide(bB, bA) || !filter(fA, fB)) {
+                    int nx = c.next;
+                    destroyContact(ci);
+                    ci = nx;
+                    continue;
+                }
+                c.flags &= ~cFilter;
+            }
+            bool activeA = (bA.flags & kAwake) && bA.type != kStatic;
+            bool activeB = (bB.flags & kAwake) && bB.type != kStatic;
+            if (!activeA && !activeB) { ci = c.next; continue; }
+            if (!overlap(fA.fat, fB.fat)) {
+                int nx = c.next;
+                destroyContact(ci);
+                ci = nx;
+                continue;
+            }
+            updateContact(ci);
+            ci = contacts[ci].next;
+        }
+    }
+
+    // b2BroadPhase::UpdatePairs + QueryCallback, REF/Collision/b2BroadPhase.h:188-241,
+    // b2BroadPhase.cpp:98-119. The pair set is every proxy whose fat AABB overlaps a
+    // moved proxy's fat AABB; pairs are keyed by proxy id. Proxy ids of the dynamic
+    // tree grow with creation order when no proxy is destroyed (0, 1, 3, 5, ...), so
+    // sorting by fixture creation index gives the same order.
+    void findNewContacts() {
+        std::vector<std::pair<int, int>> pairs;
+        const int nf = (int)fixtures.size();
+        // Candidate index: proxies sorted by fat lower x; proxies wider than
+        // `wide` are checked against every query.
+        std::vector<int> order, wideList;
+        std::vector<float> widths;
+        for (int i = 0; i < nf; ++i) {
+            if (!fixtures[i].proxy) continue;
+            widths.push_back(fixtures[i].fat.hi.x - fixtures[i].fat.lo.x);
+        }
+        float wide = 1e30f;
+        if (!widths.empty()) {
+            std::vector<float> w2 = widths;
+            std::nth_element(w2.begin(), w2.begin() + w2.size() / 2, w2.end());
+            wide = 8.0f * w2[w2.size() / 2] + 1.0f;
+        }
+        float maxNarrow = 0.0f;
+        for (int i = 0; i < nf; ++i) {
+            if (!fixtures[i].proxy) continue;
+            float wd = fixtures[i].fat.hi.x - fixtures[i].fat.lo.x;
+            if (wd > wide) wideList.push_back(i);
+            else { order.push_back(i); maxNarrow = vmax(maxNarrow, wd); }
+        }
+        std::sort(order.begin(), order.end(), [&](int a, int b) {
+            return fixtures[a].fat.lo.x < fixtures[b].fat.lo.x || (fixtures[a].fat.lo.x == fixtures[b].fat.lo.x && a < b);
+        });
+        std::vector<float> keys(order.size());
+        for (size_t i = 0; i < order.size(); ++i) keys[i] = fixtures[order[i]].fat.lo.x;
+        for (int q : moveBuffer) {
+            if (q < 0) continue;
+            const AABB& fq = fixtures[q].fat;
+            auto consider = [&](int p) {
+                if (p == q) return;
+                if (overlap(fq, fixtures[p].fat)) pairs.push_back(std::make_pair(std::min(p, q), std::max(p, q)));
+            };
+            const float qw = fq.hi.x - fq.lo.x;
+            if (qw > wide) {
+                for (int p = 0; p < nf; ++p) if (fixtures[p].proxy) consider(p);
+                continue;
+            }
+            const float lo = fq.lo.x - maxNarrow;
+            size_t k = std::lower_bound(keys.begin(), keys.end(), lo) - keys.begin();
+            for (; k < order.size() && keys[k] <= fq.hi.x; ++k) consider(order[k]);
+            for (int p : wideList) consider(p);
+        }
+        moveBuffer.clear();
+        std::sort(pairs.begin(), pairs.end());
+        pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+        for (auto& pr : pairs) addPair(pr.first, pr.second);
+    }
+
+    // b2DynamicTree::MoveProxy fat-AABB rule, REF/Collision/b2DynamicTree.cpp:133-181
+    void moveProxy(int fi, const AABB& aabb, V2 displacement) {
+        Fixture& f = fixtures[fi];
+        if (contains(f.fat, aabb)) return;
+        AABB b = aabb;
+        V2 r = mk(kAabbExtension, kAabbExtension);
+        b.lo = b.lo - r;
+        b.hi = b.hi + r;
+        V2 d = kAabbMultiplier * displacement;
+        if (d.x < 0.0f) b.lo.x += d.x; else b.hi.x += d.x;
+        if (d.y < 0.0f) b.lo.y += d.y; else b.hi.y += d.y;
+        f.fat = b;
+        moveBuffer.push_back(fi);
+    }
+
+    // b2Body::SynchronizeFixtures + b2Fixture::Synchronize,
+    // REF/Dynamics/b2Body.cpp:443-454, REF/Dynamics/b2Fixture.cpp:154-176
+    void synchronizeFixtures(Body& b) {
+        Xf xf1;
+        xf1.q = rotOf(b.a0);
+        xf1.p = b.c0 - mul(xf1.q, b.localCenter);
+        for (int k = (int)b.fixtures.size() - 1; k >= 0; --k) {
+            const int fi = b.fixtures[k];
+            Fixture& f = fixtures[fi];
+            if (!f.proxy) continue;
+            AABB a1 = shapeAABB(&f.shape, xf1);
+            AABB a2 = shapeAABB(&f.shape, b.xf);
+            f.aabb.lo = vmin(a1.lo, a2.lo);
+            f.aabb.hi = vmax(a1.hi, a2.hi);
+            V2 disp = b.xf.p - xf1.p;
+            moveProxy(fi, f.aabb, disp);
+        }
+    }
+
+    // b2Body::SynchronizeTransform, REF/Dynamics/b2Body.h:836-840
+    void synchronizeTransform(Body& b) {
+        b.xf.q = rotOf(b.a);
+        b.xf.p = b.c - mul(b.xf.q, b.localCenter);
+    }
+
+    // b2Island::Solve and the b2ContactSolver it drives,
+    // REF/Dynamics/b2Island.cpp:182-383, REF/Dynamics/Contacts/b2ContactSolver.cpp:52-744
+    void solveIsland(float h, float inv_dt, float dtRatio, int velIters, int posIters) {
+        (void)inv_dt;
+        const int nb = (int)islandBodies.size();
+        const int nc = (int)islandContacts.size();
+        posC.resize(nb); posA.resize(nb); velV.resize(nb); velW.resize(nb);
+        for (int i = 0; i < nb; ++i) {
+            Body& b = bodies[islandBodies[i]];
+            V2 c = b.c;
+            float a = b.a;
+            V2 vv = b.v;
+            float w = b.w;
+            b.c0 = b.c;
+            b.a0 = b.a;
+            if (b.type == kDynamic) {
+                vv += h * (b.gravityScale * gravity + b.invMass * b.force);
+                w += h * b.invI * b.torque;
+                vv *= 1.0f / (1.0f + h * b.linearDamping);
+                w *= 1.0f / (1.0f + h * b.angularDamping);
+            }
+            posC[i] = c; posA[i] = a; velV[i] = vv; velW[i] = w;
+        }
+        // b2ContactSolver ctor (REF b2ContactSolver.cpp:52-140)
+        vcs.resize(nc); pcs.resize(nc);
+        for (int i = 0; i < nc; ++i) {
+            Contact& ct = contacts[islandContacts[i]];
+            const Fixture& fA = fixtures[ct.fA];
+            const Fixture& fB = fixtures[ct.fB];
+            const Body& bA = bodies[fA.body];
+            const Body& bB = bodies[fB.body];
+            const b2g_manifold& m = ct.manifold;
+            VelConstraint& vc = vcs[i];
+            vc.friction = ct.friction;
+            vc.restitution = ct.restitution;
+            vc.tangentSpeed = ct.tangentSpeed;
+            vc.indexA = bA.islandIndex;
+            vc.indexB = bB.islandIndex;
+            vc.invMassA = bA.invMass; vc.invMassB = bB.invMass;
+            vc.invIA = bA.invI; vc.invIB = bB.invI;
+            vc.contactIndex = i;
+            vc.pointCount = m.pointCount;
+            for (int k = 0; k < 4; ++k) { vc.K[k] = 0.0f; vc.normalMass[k] = 0.0f; }
+            PosConstraint& pc = pcs[i];
+            pc.indexA = bA.islandIndex; pc.indexB = bB.islandIndex;
+            pc.invMassA = bA.invMass; pc.invMassB = bB.invMass;
+            pc.localCenterA = bA.localCenter; pc.localCenterB = bB.localCenter;
+            pc.invIA = bA.invI; pc.invIB = bB.invI;
+            pc.localNormal = v(m.localNormal);
+            pc.localPoint = v(m.localPoint);
+            pc.pointCount = m.pointCount;
+            pc.radiusA = fA.shape.radius;
+            pc.radiusB = fB.shape.radius;
+            pc.type = m.type;
+            for (int j = 0; j < m.pointCount; ++j) {
+                VelPoint& vp = vc.points[j];
+                if (warmStarting) {
+                    vp.normalImpulse = dtRatio * m.points[j].normalImpulse;
+                    vp.tangentImpulse = dtRatio * m.points[j].tangentImpulse;
+                } else {
+                    vp.normalImpulse = 0.0f;
+                    vp.tangentImpulse = 0.0f;
+                }
+                vp.rA = mk(0.0f, 0.0f); vp.rB = mk(0.0f, 0.0f);
+                vp.normalMass = 0.0f; vp.tangentMass = 0.0f; vp.velocityBias = 0.0f;
+                pc.localPoints[j] = v(m.points[j].localPoint);
+            }
+        }
+        // InitializeVelocityConstraints (REF :143-252)
+        for (int i = 0; i < nc; ++i) {
+            VelConstraint& vc = vcs[i];
+            PosConstraint& pc = pcs[i];
+            const b2g_manifold& m = contacts[islandContacts[vc.contactIndex]].manifold;
+            const int iA = vc.indexA, iB = vc.indexB;
+            const float mA = vc.invMassA, mB = vc.invMassB, IA = vc.invIA, IB = vc.invIB;
+            V2 cA = posC[iA]; float aA = posA[iA];
+            V2 vA = velV[iA]; float wA = velW[iA];
+            V2 cB = posC[iB]; float aB = posA[iB];
+            V2 vB = velV[iB]; float wB = velW[iB];
+            Xf xfA, xfB;
+            xfA.q = rotOf(aA);
+            xfB.q = rotOf(aB);
+            xfA.p = cA - mul(xfA.q, pc.localCenterA);
+            xfB.p = cB - mul(xfB.q, pc.localCenterB);
+            WorldManifold wm;
+            worldManifold(&wm, &m, xfA, pc.radiusA, xfB, pc.radiusB);
+            vc.normal = wm.normal;
+            for (int j = 0; j < vc.pointCount; ++j) {
+                VelPoint& vp = vc.points[j];
+                vp.rA = wm.points[j] - cA;
+                vp.rB = wm.points[j] - cB;
+                float rnA = cross(vp.rA, vc.normal);
+                float rnB = cross(vp.rB, vc.normal);
+                float kNormal = mA + mB + IA * rnA * rnA + IB * rnB * rnB;
+                vp.normalMass = kNormal > 0.0f ? 1.0f / kNormal : 0.0f;
+                V2 tangent = cross(vc.normal, 1.0f);
+                float rtA = cross(vp.rA, tangent);
+                float rtB = cross(vp.rB, tangent);
+                float kTangent = mA + mB + IA * rtA * rtA + IB * rtB * rtB;
+                vp.tangentMass = kTangent > 0.0f ? 1.0f / kTangent : 0.0f;
+                vp.velocityBias = 0.0f;
+                float vRel = dot(vc.normal, vB + cross(wB, vp.rB) - vA - cross(wA, vp.rA));
+                if (vRel < -kVelocityThreshold) vp.velocityBias = -vc.restitution * vRel;
+            }
+            if (vc.pointCount == 2) {
+                VelPoint& p1 = vc.points[0];
+                VelPoint& p2 = vc.points[1];
+                float rn1A = cross(p1.rA, vc.normal), rn1B = cross(p1.rB, vc.normal);
+                float rn2A = cross(p2.rA, vc.normal), rn2B = cross(p2.rB, vc.normal);
+                float k11 = mA + mB + IA * rn1A * rn1A + IB * rn1B * rn1B;
+                float k22 = mA + mB + IA * rn2A * rn2A + IB * rn2B * rn2B;
+                float k12 = mA + mB + IA * rn1A * rn2A + IB * rn1B * rn2B;
+                const float k_maxConditionNumber = 1000.0f;
+                if (k11 * k11 < k_maxConditionNumber * (k11 * k22 - k12 * k12)) {
+                    vc.K[0] = k11; vc.K[1] = k12; vc.K[2] = k12; vc.K[3] = k22;
+                    // b2Mat22::GetInverse (REF b2Math.h:198-212)
+                    float a = vc.K[0], b = vc.K[2], c = vc.K[1], d = vc.K[3];
+                    float det = a * d - b * c;
+                    if (det != 0.0f) det = 1.0f / det;
+                    vc.normalMass[0] = det * d;
+                    vc.normalMass[2] = -det * b;
+                    vc.normalMass[1] = -det * c;
+                    vc.normalMass[3] = det * a;
+                } else {
+                    vc.pointCount = 1;
+                }
+            }
+        }
+        // WarmStart (REF :254-292)
+        if (warmStarting) {
+            for (int i = 0; i < nc; ++i) {
+                VelConstraint& vc = vcs[i];
+                const int iA = vc.indexA, iB = vc.indexB;
+                const float mA = vc.invMassA, IA = vc.invIA, mB = vc.invMassB, IB = vc.invIB;
+                V2 vA = velV[iA]; float wA = velW[iA];
+                V2 vB = velV[iB]; float wB = velW[iB];
+                V2 normal = vc.normal;
+                V2 tangent = cross(normal, 1.0f);
+                for (int j = 0; j < vc.pointCount; ++j) {
+                    VelPoint& vp = vc.points[j];
+                    V2 P = vp.normalImpulse * normal + vp.tangentImpulse * tangent;
+                    wA -= IA * cross(vp.rA, P);
+                    vA -= mA * P;
+                    wB += IB * cross(vp.rB, P);
+                    vB += mB * P;
+                }
+                velV[iA] = vA; velW[iA] = wA;
+                velV[iB] = vB; velW[iB] = wB;
+            }
+        }
+        // SolveVelocityConstraints (REF :294-597)
+        for (int it = 0; it < velIters; ++it) {
+            for (int i = 0; i < nc; ++i) solveVelocity(vcs[i]);
+        }
+        // StoreImpulses (REF :599-612)
+        for (int i = 0; i < nc; ++i) {
+            VelConstraint& vc = vcs[i];
+            b2g_manifold& m = contacts[islandContacts[vc.contactIndex]].manifold;
+            for (int j = 0; j < vc.pointCount; ++j) {
+                m.points[j].normalImpulse = vc.points[j].normalImpulse;
+                m.points[j].tangentImpulse = vc.points[j].tangentImpulse;
+            }
+        }
+        // Integrate positions (REF b2Island.cpp:276-306)
+        for (int i = 0; i < nb; ++i) {
+            V2 c = posC[i]; float a = posA[i];
+            V2 vv = velV[i]; float w = velW[i];
+            V2 translation = h * vv;
+            if (dot(translation, translation) > kMaxTranslationSq) {
+                float ratio = kMaxTranslation / length(translation);
+                vv *= ratio;
+            }
+            float rotation = h * w;
+            if (rotation * rotation > kMaxRotationSq) {
+                float ratio = kMaxRotation / absf(rotation);
+                w *= ratio;
+            }
+            c += h * vv;
+            a += h * w;
+            posC[i] = c; posA[i] = a; velV[i] = vv; velW[i] = w;
+        }
+        // Position iterations with early exit (REF b2Island.cpp:308-328)
+        bool positionSolved = false;
+        for (int it = 0; it < posIters; ++it) {
+            if (solvePositions()) { positionSolved = true; break; }
+        }
+        // Copy back (REF b2Island.cpp:330-339)
+        for (int i = 0; i < nb; ++i) {
+            Body& b = bodies[islandBodies[i]];
+            b.c = posC[i]; b.a = posA[i];
+            b.v = velV[i]; b.w = velW[i];
+            synchronizeTransform(b);
+        }
+        // Sleep (REF b2Island.cpp:345-382)
+        if (allowSleep) {
+            float minSleepTime = FLT_MAX;
+            const float linTolSqr = kLinearSleepTol * kLinearSleepTol;
+            const float angTolSqr = kAngularSleepTol * kAngularSleepTol;
+            for (int i = 0; i < nb; ++i) {
+                Body& b = bodies[islandBodies[i]];
+                if (b.type == kStatic) continue;
+                if ((b.flags & kAutoSleep) == 0 || b.w * b.w > angTolSqr || dot(b.v, b.v) > linTolSqr) {
+                    b.sleepTime = 0.0f;
+                    minSleepTime = 0.0f;
+                } else {
+                    b.sleepTime += h;
+                    minSleepTime = vmin(minSleepTime, b.sleepTime);
+                }
+            }
+            if (minSleepTime >= kTimeToSleep && positionSolved) {
+                for (int i = 0; i < nb; ++i) setAwake(bodies[islandBodies[i]], false);
+            }
+        }
+    }
+
+    void solveVelocity(VelConstraint& vc) {
+        const int iA = vc.indexA, iB = vc.indexB;
+        const float mA = vc.invMassA, IA = vc.invIA, mB = vc.invMassB, IB = vc.invIB;
+        const int pointCount = vc.pointCount;
+        V2 vA = velV[iA]; float wA = velW[iA];
+        V2 vB = velV[iB]; float wB = velW[iB];
+        V2 normal = vc.normal;
+        V2 tangent = cross(normal, 1.0f);
+        const float friction = vc.friction;
+        for (int j = 0; j < pointCount; ++j) {
+            VelPoint& vp = vc.points[j];
+            V2 dv = vB + cross(wB, vp.rB) - vA - cross(wA, vp.rA);
+            float vt = dot(dv, tangent) - vc.tangentSpeed;
+            float lambda = vp.tangentMass * (-vt);
+            float maxFriction = friction * vp.normalImpulse;
+            float newImpulse = clampf(vp.tangentImpulse + lambda, -maxFriction, maxFriction);
+            lambda = newImpulse - vp.tangentImpulse;
+            vp.tangentImpulse = newImpulse;
+            V2 P = lambda * tangent;
+            vA -= mA * P;
+            wA -= IA * cross(vp.rA, P);
+            vB += mB * P;
+            wB += IB * cross(vp.rB, P);
+        }
+        if (pointCount == 1) {
+            VelPoint& vp = vc.points[0];
+            V2 dv = vB + cross(wB, vp.rB) - vA - cross(wA, vp.rA);
+            float vn = dot(dv, normal);
+            float lambda = -vp.normalMass * (vn - vp.velocityBias);
+            float newImpulse = vmax(vp.normalImpulse + lambda, 0.0f);
+            lambda = newImpulse - vp.normalImpulse;
+            vp.normalImpulse = newImpulse;
+            V2 P = lambda * normal;
+            vA -= mA * P;
+            wA -= IA * cross(vp.rA, P);
+            vB += mB * P;
+            wB += IB * cross(vp.rB, P);
+        } else {
+            // Block solver by total enumeration (REF :386-589)
+            VelPoint& cp1 = vc.points[0];
+            VelPoint& cp2 = vc.points[1];
+            V2 a = mk(cp1.normalImpulse, cp2.normalImpulse);
+            V2 dv1 = vB + cross(wB, cp1.rB) - vA - cross(wA, cp1.rA);
+            V2 dv2 = vB + cross(wB, cp2.rB) - vA - cross(wA, cp2.rA);
+            float vn1 = dot(dv1, normal);
+            float vn2 = dot(dv2, normal);
+            V2 b;
+            b.x = vn1 - cp1.velocityBias;
+            b.y = vn2 - cp2.velocityBias;
+            // b -= K * a  (b2Mul(b2Mat22, b2Vec2), REF b2Math.h:389-392)
+            b -= mk(vc.K[0] * a.x + vc.K[2] * a.y, vc.K[1] * a.x + vc.K[3] * a.y);
+            auto apply = [&](V2 x) {
+                V2 d = x - a;
+                V2 P1 = d.x * normal;
+                V2 P2 = d.y * normal;
+                vA -= mA * (P1 + P2);
+                wA -= IA * (cross(cp1.rA, P1) + cross(cp2.rA, P2));
+                vB += mB * (P1 + P2);
+                wB += IB * (cross(cp1.rB, P1) + cross(cp2.rB, P2));
+                cp1.normalImpulse = x.x;
+                cp2.normalImpulse = x.y;
+            };
+            for (;;) {
+                // x = -M * b
+                V2 Mb = mk(vc.normalMass[0] * b.x + vc.normalMass[2] * b.y,
+                           vc.normalMass[1] * b.x + vc.normalMass[3] * b.y);
+                V2 x = -Mb;
+                if (x.x >= 0.0f && x.y >= 0.0f) { apply(x); break; }
+                x.x = -cp1.normalMass * b.x;
+                x.y = 0.0f;
+                vn1 = 0.0f;
+                vn2 = vc.K[1] * x.x + b.y;
+                if (x.x >= 0.0f && vn2 >= 0.0f) { apply(x); break; }
+                x.x = 0.0f;
+                x.y = -cp2.normalMass * b.y;
+                vn1 = vc.K[2] * x.y + b.x;
+                vn2 = 0.0f;
+                if (x.y >= 0.0f && vn1 >= 0.0f) { apply(x); break; }
+                x.x = 0.0f;
+                x.y = 0.0f;
+                vn1 = b.x;
+                vn2 = b.y;
+                if (vn1 >= 0.0f && vn2 >= 0.0f) { apply(x); break; }
+                break;
+            }
+        }
+        velV[iA] = vA; velW[iA] = wA;
+        velV[iB] = vB; velW[iB] = wB;
+    }
+
+    // SolvePositionConstraints + b2PositionSolverManifold (REF :614-744)
+    bool solvePositions() {
+        float minSeparation = 0.0f;
+        for (size_t i = 0; i < pcs.size(); ++i) {
+            PosConstraint& pc = pcs[i];
+            const int iA = pc.indexA, iB = pc.indexB;
+            V2 lcA = pc.localCenterA, lcB = pc.localCenterB;
+            const float mA = pc.invMassA, IA = pc.invIA, mB = pc.invMassB, IB = pc.invIB;
+            V2 cA = posC[iA]; float aA = posA[iA];
+            V2 cB = posC[iB]; float aB = posA[iB];
+            for (int j = 0; j < pc.pointCount; ++j) {
+                Xf xfA, xfB;
+                xfA.q = rotOf(aA);
+                xfB.q = rotOf(aB);
+                xfA.p = cA - mul(xfA.q, lcA);
+                xfB.p = cB - mul(xfB.q, lcB);
+                V2 normal, point;
+                float separation;
+                if (pc.type == B2G_MANIFOLD_CIRCLES) {
+                    V2 pointA = mul(xfA, pc.localPoint);
+                    V2 pointB = mul(xfB, pc.localPoints[0]);
+                    normal = pointB - pointA;
+                    normalize(normal);
+                    point = 0.5f * (pointA + pointB);
+                    separation = dot(pointB - pointA, normal) - pc.radiusA - pc.radiusB;
+                } else if (pc.type == B2G_MANIFOLD_FACE_A) {
+                    normal = mul(xfA.q, pc.localNormal);
+                    V2 planePoint = mul(xfA, pc.localPoint);
+                    V2 clipPoint = mul(xfB, pc.localPoints[j]);
+                    separation = dot(clipPoint - planePoint, normal) - pc.radiusA - pc.radiusB;
+                    point = clipPoint;
+                } else {
+                    normal = mul(xfB.q, pc.localNormal);
+                    V2 planePoint = mul(xfB, pc.localPoint);
+                    V2 clipPoint = mul(xfA, pc.localPoints[j]);
+                    separation = dot(clipPoint - planePoint, normal) - pc.radiusA - pc.radiusB;
+                    point = clipPoint;
+                    normal = -normal;
+                }
+                V2 rA = point - cA;
+                V2 rB = point - cB;
+                minSeparation = vmin(minSeparation, separation);
+                float C = clampf(kBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, 0.0f);
+                float rnA = cross(rA, normal);
+                float rnB = cross(rB, normal);
+                float K = mA + mB + IA * rnA * rnA + IB * rnB * rnB;
+                float impulse = K > 0.0f ? -C / K : 0.0f;
+                V2 P = impulse * normal;
+                cA -= mA * P;
+                aA -= IA * cross(rA, P);
+                cB += mB * P;
+                aB += IB * cross(rB, P);
+            }
+            posC[iA] = cA; posA[iA] = aA;
+            posC[iB] = cB; posA[iB] = aB;
+        }
+        return minSeparation >= -3.0f * kLinearSlop;
+    }
+
+    // b2World::Solve, REF/Dynamics/b2World.cpp:388-577
+    void solve(float dt, float inv_dt, float dtRatio, int velIters, int posIters) {
+        const int nb = (int)bodies.size();
+        for (auto& b : bodies) b.flags &= ~kIsland;
+        for (int ci = contactHead; ci != -1; ci = contacts[ci].next) contacts[ci].flags &= ~cIsland;
+        stack.resize(nb > 0 ? nb : 1);
+        for (int seed = nb - 1; seed >= 0; --seed) {
+            Body& sb = bodies[seed];
+            if (!sb.alive) continue;
+            if (sb.flags & kIsland) continue;
+            if ((sb.flags & kAwake) == 0 || (sb.flags & kActive) == 0) continue;
+            if (sb.type == kStatic) continue;
+            islandBodies.clear();
+            islandContacts.clear();
+            int sc = 0;
+            stack[sc++] = seed;
+            sb.flags |= kIsland;
+            while (sc > 0) {
+                const int bi = stack[--sc];
+                Body& b = bodies[bi];
+                b.islandIndex = (int)islandBodies.size();
+                islandBodies.push_back(bi);
+                setAwake(b, true);
+                if (b.type == kStatic) continue;
+                for (int e = b.edgeHead; e != -1; e = edgeRef(e).next) {
+                    Contact& c = contacts[e >> 1];
+                    if (c.flags & cIsland) continue;
+                    if ((c.flags & cEnabled) == 0 || (c.flags & cTouching) == 0) continue;
+                    if (fixtures[c.fA].sensor || fixtures[c.fB].sensor) continue;
+                    islandContacts.push_back(e >> 1);
+                    c.flags |= cIsland;
+                    const int other = edgeOther(e);
+                    if (bodies[other].flags & kIsland) continue;
+                    stack[sc++] = other;
+                    bodies[other].flags |= kIsland;
+                }
+            }
+            solveIsland(dt, inv_dt, dtRatio, velIters, posIters);
+            for (int bi : islandBodies) {
+                if (bodies[bi].type == kStatic) bodies[bi].flags &= ~kIsland;
+            }
+        }
+        for (int bi = nb - 1; bi >= 0; --bi) {
+            Body& b = bodies[bi];
+            if (!b.alive) continue;
+            if ((b.flags & kIsland) == 0) continue;
+            if (b.type == kStatic) continue;
+            synchronizeFixtures(b);
+        }
+        findNewContacts();
+    }
+
+    // b2World::Step, REF/Dynamics/b2World.cpp:899-964 (SolveTOI not restated)
+    void step(float dt, int velIters, int posIters) {
+        if (newFixture) {
+            findNewContacts();
+            newFixture = false;
+        }
+        float inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
+        float dtRatio = inv_dt0 * dt;
+        collideAll();
+        if (stepComplete && dt > 0.0f) solve(dt, inv_dt, dtRatio, velIters, posIters);
+        if (continuous && dt > 0.0f) unsupported |= 1;
+        if (dt > 0.0f) inv_dt0 = inv_dt;
+        if (autoClearForces) {
+            for (auto& b : bodies) { b.force = mk(0.0f, 0.0f); b.torque = 0.0f; }
+        }
+    }
+
+    int load(const b2scene::BodyDef* bd, int nb, const b2scene::FixtureDef* fd, int nf) {
+        for (int i = 0; i < nb; ++i) {
+            const int bi = createBody(bd[i]);
+            for (int k = 0; k < bd[i].fixtureCount; ++k) {
+                const int fi = bd[i].firstFixture + k;
+                if (fi < 0 || fi >= nf) return B2G_ERR_ARG;
+                const b2scene::FixtureDef& f = fd[fi];
+                b2g_shape s;
+                if (f.shape == b2scene::kPolygon) {
+                    if (f.polyMode == b2scene::kPolyBox) setAsBox(&s, f.pts[0], f.pts[1]);
+                    else if (f.polyMode == b2scene::kPolyBoxRotated)
+                        setAsBoxRotated(&s, f.pts[0], f.pts[1], mk(f.pts[2], f.pts[3]), f.pts[4]);
+                    else {
+                        V2 p[8];
+                        for (int j = 0; j < f.n && j < 8; ++j) p[j] = mk(f.pts[2 * j], f.pts[2 * j + 1]);
+                        polygonSet(&s, p, f.n);
+                    }
+                } else if (f.shape == b2scene::kCircle) {
+                    circleShape(&s, mk(f.cx, f.cy), f.radius);
+                } else {
+                    edgeShape(&s, mk(f.e[0], f.e[1]), mk(f.e[2], f.e[3]), mk(f.e[4], f.e[5]), mk(f.e[6], f.e[7]),
+                              f.has0 != 0, f.has3 != 0);
+                }
+                createFixture(bi, s, f);
+            }
+        }
+        return B2G_OK;
+    }
+};
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+void* orc_world_create(float gx, float gy) {
+    World* w = new World;
+    w->gravity = mk(gx, gy);
+    return w;
+}
+void orc_world_destroy(void* h) { delete (World*)h; }
+
+// b2World::SetAllowSleeping (REF b2World.cpp:370-386) wakes every body when
+// sleeping is switched off.
+void orc_world_set_flags(void* h, int allowSleep, int warmStarting, int continuous, int subStepping) {
+    World* w = (World*)h;
+    bool s = allowSleep != 0;
+    if (s != w->allowSleep) {
+        w->allowSleep = s;
+        if (!s) for (auto& b : w->bodies) w->setAwake(b, true);
+    }
+    w->warmStarting = warmStarting != 0;
+    w->continuous = continuous != 0;
+    w->subStepping = subStepping != 0;
+}
+
+int orc_world_load(void* h, const b2scene::BodyDef* bd, int nb, const b2scene::FixtureDef* fd, int nf) {
+    return ((World*)h)->load(bd, nb, fd, nf);
+}
+
+int orc_world_load_scene(void* h, const char* kind, int n, unsigned seed) {
+    b2scene::Scene s = b2scene::byName(kind, n, seed);
+    return orc_world_load(h, s.bodies.data(), (int)s.bodies.size(), s.fixtures.data(), (int)s.fixtures.size());
+}
+
+int orc_world_step(void* h, float dt, int vel, int pos, int nsteps) {
+    World* w = (World*)h;
+    for (int i = 0; i < nsteps; ++i) w->step(dt, vel, pos);
+    return w->unsupported ? B2G_ERR_UNSUPPORTED : B2G_OK;
+}
+
+int orc_world_body_count(void* h) { return (int)((World*)h)->bodies.size(); }
+int orc_world_fixture_count(void* h) { return (int)((World*)h)->fixtures.size(); }
+
+void orc_world_get_bodies(void* h, float* out) {
+    World* w = (World*)h;
+    for (size_t i = 0; i < w->bodies.size(); ++i) {
+        const Body& b = w->bodies[i];
+        float* o = out + 12 * i;
+        o[0] = b.xf.p.x; o[1] = b.xf.p.y; o[2] = b.xf.q.s; o[3] = b.xf.q.c;
+        o[4] = b.c.x; o[5] = b.c.y; o[6] = b.a;
+        o[7] = b.v.x; o[8] = b.v.y; o[9] = b.w;
+        o[10] = (b.flags & kAwake) ? 1.0f : 0.0f;
+        o[11] = b.sleepTime;
+    }
+}
+
+int orc_world_contact_count(void* h) { return ((World*)h)->contactCount; }
+
+int orc_world_get_contacts(void* h, int32_t* meta, b2g_manifold* manifolds, int cap) {
+    World* w = (World*)h;
+    int n = 0;
+    for (int ci = w->contactHead; ci != -1; ci = w->contacts[ci].next) {
+        const Contact& c = w->contacts[ci];
+        if (n < cap) {
+            meta[4 * n + 0] = c.fA;
+            meta[4 * n + 1] = c.fB;
+            meta[4 * n + 2] = ((c.flags & cTouching) ? 1 : 0) | ((c.flags & cEnabled) ? 2 : 0);
+            meta[4 * n + 3] = 0;
+            b2g_manifold m = c.manifold;
+            for (int i = m.pointCount; i < 2; ++i) memset(&m.points[i], 0, sizeof(m.points[i]));
+            if (m.pointCount == 0) { m.type = 0; m.localNormal = g(mk(0, 0)); m.localPoint = g(mk(0, 0)); }
+            manifolds[n] = m;
+        }
+        ++n;
+    }
+    return n;
+}
+
+void orc_world_get_fat_aabbs(void* h, b2g_aabb* out) {
+    World* w = (World*)h;
+    for (size_t i = 0; i < w->fixtures.size(); ++i) {
+        const AABB& a = w->fixtures[i].fat;
+        out[i] = {a.lo.x, a.lo.y, a.hi.x, a.hi.y};
+    }
+}
+
+void orc_world_get_shapes(void* h, b2g_shape* out) {
+    World* w = (World*)h;
+    for (size_t i = 0; i < w->fixtures.size(); ++i) out[i] = w->fixtures[i].shape;
+}
+
+void orc_world_penetration(void* h, double* out) {
+    World* w = (World*)h;
+    double sum = 0.0, mx = 0.0;
+    long np = 0;
+    for (int ci = w->contactHead; ci != -1; ci = w->contacts[ci].next) {
+        const Contact& c = w->contacts[ci];
+        if (!(c.flags & cTouching)) continue;
+        const Fixture& fA = w->fixtures[c.fA];
+        const Fixture& fB = w->fixtures[c.fB];
+        WorldManifold wm;
+        worldManifold(&wm, &c.manifold, w->bodies[fA.body].xf, fA.shape.radius, w->bodies[fB.body].xf,
+                      fB.shape.radius);
+        for (int i = 0; i < c.manifold.pointCount; ++i) {
+            double p = wm.separations[i] < 0.0f ? -(double)wm.separations[i] : 0.0;
+            sum += p;
+            mx = p > mx ? p : mx;
+            ++np;
+        }
+    }
+    out[0] = np ? sum / np : 0.0;
+    out[1] = mx;
+    out[2] = (double)np;
+}
+
+void orc_world_events(void* h, long* out) {
+    World* w = (World*)h;
+    out[0] = w->events.begin;
+    out[1] = w->events.end;
+}
+
+int orc_collide(const b2g_shape* a, const b2g_transform* xfA, const b2g_shape* b, const b2g_transform* xfB,
+                b2g_manifold* out) {
+    clearManifold(out);
+    return collide(out, a, xfOf(xfA), b, xfOf(xfB)) ? 0 : -1;
+}
+
+void orc_world_manifold(const b2g_manifold* m, const b2g_transform* xfA, float radiusA, const b2g_transform* xfB,
+                        float radiusB, float* out) {
+    for (int i = 0; i < 8; ++i) out[i] = 0.0f;
+    if (m->pointCount == 0) return;
+    WorldManifold wm;
+    worldManifold(&wm, m, xfOf(xfA), radiusA, xfOf(xfB), radiusB);
+    out[0] = wm.normal.x; out[1] = wm.normal.y;
+    for (int i = 0; i < m->pointCount; ++i) {
+        out[2 + 2 * i] = wm.points[i].x;
+        out[3 + 2 * i] = wm.points[i].y;
+        out[6 + i] = wm.separations[i];
+    }
+}
+
+void orc_polygon_set(const float* pts, int n, b2g_shape* out) {
+    V2 p[8];
+    for (int i = 0; i < n && i < 8; ++i) p[i] = mk(pts[2 * i], pts[2 * i + 1]);
+    polygonSet(out, p, n);
+}
+
+void orc_polygon_box(float hx, float hy, float cx, float cy, float angle, int rotated, b2g_shape* out) {
+    if (rotated) setAsBoxRotated(out, hx, hy, mk(cx, cy), angle);
+    else setAsBox(out, hx, hy);
+}
+
+void orc_shape_mass(const b2g_shape* s, float density, float* out) {
+    MassData md = shapeMass(s, density);
+    out[0] = md.mass; out[1] = md.center.x; out[2] = md.center.y; out[3] = md.I;
+}
+
+void orc_shape_aabb(const b2g_shape* s, const b2g_transform* xf, b2g_aabb* out) {
+    AABB a = shapeAABB(s, xfOf(xf));
+    *out = {a.lo.x, a.lo.y, a.hi.x, a.hi.y};
+}
+
+// glibc sinf/cosf, the functions b2Rot::Set calls (REF b2Math.h:302-311).
+void orc_sincos(const float* a, int n, float* s, float* c) {
+    for (int i = 0; i < n; ++i) { s[i] = sinf(a[i]); c[i] = cosf(a[i]); }
+}
+
+// Broad-phase checker: the pair set UpdatePairs reports for the given fat
+// AABBs and moved flags, by brute force over all proxies.
+int orc_pairs_bruteforce(const b2g_aabb* fat, const uint8_t* moved, int n, int32_t* out, int cap) {
+    std::vector<std::pair<int, int>> pairs;
+    for (int q = 0; q < n; ++q) {
+        if (!moved[q]) continue;
+        AABB a = {mk(fat[q].lx, fat[q].ly), mk(fat[q].ux, fat[q].uy)};
+        for (int p = 0; p < n; ++p) {
+            if (p == q) continue;
+            AABB b = {mk(fat[p].lx, fat[p].ly), mk(fat[p].ux, fat[p].uy)};
+            if (overlap(a, b)) pairs.push_back(std::make_pair(std::min(p, q), std::max(p, q)));
+        }
+    }
+    std::sort(pairs.begin(), pairs.end());
+    pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+    const int np = (int)pairs.size();
+    for (int i = 0; i < np && i < cap; ++i) { out[2 * i] = pairs[i].first; out[2 * i + 1] = pairs[i].second; }
+    return np;
+}
+
+// The fat-AABB rule alone: returns 1 and writes the new fat AABB when the
+// proxy must be reinserted (moved), 0 when the old one still contains it.
+int orc_fat_rule(const b2g_aabb* fat, const b2g_aabb* tight, float dx, float dy, b2g_aabb* out) {
+    World w;
+    Fixture f;
+    f.fat = {mk(fat->lx, fat->ly), mk(fat->ux, fat->uy)};
+    w.fixtures.push_back(f);
+    AABB t = {mk(tight->lx, tight->ly), mk(tight->ux, tight->uy)};
+    w.moveProxy(0, t, mk(dx, dy));
+    const AABB& r = w.fixtures[0].fat;
+    *out = {r.lo.x, r.lo.y, r.hi.x, r.hi.y};
+    return (int)w.moveBuffer.size();
+}
+
+}  // extern "C"

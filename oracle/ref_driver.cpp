@@ -1,0 +1,460 @@
+// ORACLE TEST INFRASTRUCTURE - not part of the product.
+//
+// C-ABI driver over the upstream Box2D 2.3.2 sources that the reference
+// vendors (Regression/box2d-ref/Box2DRef, namespace box2dref). oracle/Makefile
+// compiles those sources where they lie, together with this file, into
+// oracle/_ref/libb2ref.so. Tests use it to pin the CPU restatement
+// (oracle/cpu_ref.cpp), to write golden fixtures, and bench.py times it as the
+// CPU baseline ("kind": "reference"). Only tests/, the smoke check and the
+// benchmark's CPU-baseline leg may load it.
+//
+// Every entry point mirrors one exported by the restatement with the prefix
+// ref_ instead of orc_, so tests treat both checkers alike.
+// The driver reads a few private members (proxy ids, sleep time) for dumps;
+// it opens class access for the reference headers only.
+#define private public
+#define protected public
+#include <Box2DRef/Box2D.h>
+#undef private
+#undef protected
+#include <stdint.h>
+#include <string.h>
+#include <vector>
+#include <algorithm>
+
+#include "b2g.h"
+#include "scenes/scenes.h"
+
+using namespace box2dref;
+
+namespace {
+
+struct RefWorld {
+    b2World* world;
+    std::vector<b2Body*> bodies;
+    std::vector<b2Fixture*> fixtures;
+};
+
+int fixtureIndex(const b2Fixture* f) { return (int)(intptr_t)f->GetUserData(); }
+
+void shapeToRec(const b2Shape* s, b2g_shape* out) {
+    memset(out, 0, sizeof(*out));
+    out->type = (int32_t)s->GetType();
+    out->radius = s->m_radius;
+    if (s->GetType() == b2Shape::e_polygon) {
+        const b2PolygonShape* p = (const b2PolygonShape*)s;
+        out->count = p->m_count;
+        out->centroid = {p->m_centroid.x, p->m_centroid.y};
+        for (int i = 0; i < p->m_count; ++i) {
+            out->verts[i] = {p->m_vertices[i].x, p->m_vertices[i].y};
+            out->normals[i] = {p->m_normals[i].x, p->m_normals[i].y};
+        }
+    } else if (s->GetType() == b2Shape::e_circle) {
+        const b2CircleShape* c = (const b2CircleShape*)s;
+        out->centroid = {c->m_p.x, c->m_p.y};
+    } else if (s->GetType() == b2Shape::e_edge) {
+        const b2EdgeShape* e = (const b2EdgeShape*)s;
+        out->verts[0] = {e->m_vertex1.x, e->m_vertex1.y};
+        out->verts[1] = {e->m_vertex2.x, e->m_vertex2.y};
+        out->v0 = {e->m_vertex0.x, e->m_vertex0.y};
+        out->v3 = {e->m_vertex3.x, e->m_vertex3.y};
+        out->flags = (e->m_hasVertex0 ? 1 : 0) | (e->m_hasVertex3 ? 2 : 0);
+        out->count = 2;
+    }
+}
+
+void recToPolygon(const b2g_shape* r, b2PolygonShape* p) {
+    p->m_count = r->count;
+    p->m_radius = r->radius;
+    p->m_centroid.Set(r->centroid.x, r->centroid.y);
+    for (int i = 0; i < r->count; ++i) {
+        p->m_vertices[i].Set(r->verts[i].x, r->verts[i].y);
+        p->m_normals[i].Set(r->normals[i].x, r->normals[i].y);
+    }
+}
+
+void recToCircle(const b2g_shape* r, b2CircleShape* c) {
+    c->m_radius = r->radius;
+    c->m_p.Set(r->centroid.x, r->centroid.y);
+}
+
+void recToEdge(const b2g_shape* r, b2EdgeShape* e) {
+    e->m_radius = r->radius;
+    e->m_vertex1.Set(r->verts[0].x, r->verts[0].y);
+    e->m_vertex2.Set(r->verts[1].x, r->verts[1].y);
+    e->m_vertex0.Set(r->v0.x, r->v0.y);
+    e->m_vertex3.Set(r->v3.x, r->v3.y);
+    e->m_hasVertex0 = (r->flags & 1) != 0;
+    e->m_hasVertex3 = (r->flags & 2) != 0;
+}
+
+b2Transform toXf(const b2g_transform* t) {
+    b2Transform xf;
+    xf.p.Set(t->px, t->py);
+    xf.q.s = t->s;
+    xf.q.c = t->c;
+    return xf;
+}
+
+void manifoldOut(const b2Manifold& m, b2g_manifold* out) {
+    memset(out, 0, sizeof(*out));
+    out->pointCount = m.pointCount;
+    if (m.pointCount == 0) return;  // other fields are stale for empty manifolds
+    out->type = (int32_t)m.type;
+    out->localNormal = {m.localNormal.x, m.localNormal.y};
+    out->localPoint = {m.localPoint.x, m.localPoint.y};
+    for (int i = 0; i < m.pointCount; ++i) {
+        out->points[i].localPoint = {m.points[i].localPoint.x, m.points[i].localPoint.y};
+        out->points[i].normalImpulse = m.points[i].normalImpulse;
+        out->points[i].tangentImpulse = m.points[i].tangentImpulse;
+        out->points[i].id = m.points[i].id.key;
+    }
+}
+
+// Builds one fixture's shape from a scene definition through the public API.
+void buildShape(const b2scene::FixtureDef& fd, b2PolygonShape* poly, b2CircleShape* circle,
+                b2EdgeShape* edge, const b2Shape** shape) {
+    if (fd.shape == b2scene::kPolygon) {
+        if (fd.polyMode == b2scene::kPolyBox) {
+            poly->SetAsBox(fd.pts[0], fd.pts[1]);
+        } else if (fd.polyMode == b2scene::kPolyBoxRotated) {
+            poly->SetAsBox(fd.pts[0], fd.pts[1], b2Vec2(fd.pts[2], fd.pts[3]), fd.pts[4]);
+        } else {
+            b2Vec2 pts[b2_maxPolygonVertices];
+            for (int i = 0; i < fd.n; ++i) pts[i].Set(fd.pts[2 * i], fd.pts[2 * i + 1]);
+            poly->Set(pts, fd.n);
+        }
+        *shape = poly;
+    } else if (fd.shape == b2scene::kCircle) {
+        circle->m_radius = fd.radius;
+        circle->m_p.Set(fd.cx, fd.cy);
+        *shape = circle;
+    } else {
+        edge->Set(b2Vec2(fd.e[2], fd.e[3]), b2Vec2(fd.e[4], fd.e[5]));
+        edge->m_vertex0.Set(fd.e[0], fd.e[1]);
+        edge->m_vertex3.Set(fd.e[6], fd.e[7]);
+        edge->m_hasVertex0 = fd.has0 != 0;
+        edge->m_hasVertex3 = fd.has3 != 0;
+        *shape = edge;
+    }
+}
+
+struct PairCollector {
+    std::vector<int32_t> pairs;
+    void AddPair(void* a, void* b) {
+        pairs.push_back((int32_t)(intptr_t)a - 1);
+        pairs.push_back((int32_t)(intptr_t)b - 1);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+void* ref_world_create(float gx, float gy) {
+    RefWorld* w = new RefWorld;
+    w->world = new b2World(b2Vec2(gx, gy));
+    return w;
+}
+
+void ref_world_destroy(void* h) {
+    RefWorld* w = (RefWorld*)h;
+    delete w->world;
+    delete w;
+}
+
+void ref_world_set_flags(void* h, int allowSleep, int warmStarting, int continuous, int subStepping) {
+    RefWorld* w = (RefWorld*)h;
+    w->world->SetAllowSleeping(allowSleep != 0);
+    w->world->SetWarmStarting(warmStarting != 0);
+    w->world->SetContinuousPhysics(continuous != 0);
+    w->world->SetSubStepping(subStepping != 0);
+}
+
+int ref_world_load(void* h, const b2scene::BodyDef* bodies, int nb, const b2scene::FixtureDef* fx, int nf) {
+    RefWorld* w = (RefWorld*)h;
+    for (int i = 0; i < nb; ++i) {
+        const b2scene::BodyDef& bd = bodies[i];
+        b2BodyDef def;
+        def.type = (b2BodyType)bd.type;
+        def.position.Set(bd.px, bd.py);
+        def.angle = bd.angle;
+        def.linearVelocity.Set(bd.vx, bd.vy);
+        def.angularVelocity = bd.w;
+        def.linearDamping = bd.linearDamping;
+        def.angularDamping = bd.angularDamping;
+        def.gravityScale = bd.gravityScale;
+        def.allowSleep = bd.allowSleep != 0;
+        def.awake = bd.awake != 0;
+        def.fixedRotation = bd.fixedRotation != 0;
+        def.bullet = bd.bullet != 0;
+        b2Body* body = w->world->CreateBody(&def);
+        w->bodies.push_back(body);
+        for (int k = 0; k < bd.fixtureCount; ++k) {
+            const int fi = bd.firstFixture + k;
+            if (fi < 0 || fi >= nf) return -1;
+            const b2scene::FixtureDef& fd = fx[fi];
+            b2PolygonShape poly;
+            b2CircleShape circle;
+            b2EdgeShape edge;
+            const b2Shape* shape = nullptr;
+            buildShape(fd, &poly, &circle, &edge, &shape);
+            b2FixtureDef fdef;
+            fdef.shape = shape;
+            fdef.density = fd.density;
+            fdef.friction = fd.friction;
+            fdef.restitution = fd.restitution;
+            fdef.isSensor = fd.sensor != 0;
+            fdef.filter.categoryBits = (uint16)fd.categoryBits;
+            fdef.filter.maskBits = (uint16)fd.maskBits;
+            fdef.filter.groupIndex = (int16)fd.groupIndex;
+            fdef.userData = (void*)(intptr_t)w->fixtures.size();
+            w->fixtures.push_back(body->CreateFixture(&fdef));
+        }
+    }
+    return 0;
+}
+
+int ref_world_load_scene(void* h, const char* kind, int n, unsigned seed) {
+    b2scene::Scene s = b2scene::byName(kind, n, seed);
+    return ref_world_load(h, s.bodies.data(), (int)s.bodies.size(), s.fixtures.data(), (int)s.fixtures.size());
+}
+
+void ref_world_step(void* h, float dt, int vel, int pos, int nsteps) {
+    RefWorld* w = (RefWorld*)h;
+    for (int i = 0; i < nsteps; ++i) w->world->Step(dt, vel, pos);
+}
+
+int ref_world_body_count(void* h) { return (int)((RefWorld*)h)->bodies.size(); }
+int ref_world_fixture_count(void* h) { return (int)((RefWorld*)h)->fixtures.size(); }
+
+// 12 floats per body in creation order:
+// px py s c cx cy a vx vy w awake sleepTime
+void ref_world_get_bodies(void* h, float* out) {
+    RefWorld* w = (RefWorld*)h;
+    for (size_t i = 0; i < w->bodies.size(); ++i) {
+        const b2Body* b = w->bodies[i];
+        const b2Transform& xf = b->GetTransform();
+        float* o = out + 12 * i;
+        o[0] = xf.p.x; o[1] = xf.p.y; o[2] = xf.q.s; o[3] = xf.q.c;
+        o[4] = b->GetWorldCenter().x; o[5] = b->GetWorldCenter().y; o[6] = b->GetAngle();
+        o[7] = b->GetLinearVelocity().x; o[8] = b->GetLinearVelocity().y;
+        o[9] = b->GetAngularVelocity();
+        o[10] = b->IsAwake() ? 1.0f : 0.0f;
+        o[11] = b->m_sleepTime;
+    }
+}
+
+int ref_world_contact_count(void* h) { return ((RefWorld*)h)->world->GetContactCount(); }
+
+// Contacts in world-list order. meta: fixtureA, fixtureB, flags
+// (bit0 touching, bit1 enabled), reserved.
+int ref_world_get_contacts(void* h, int32_t* meta, b2g_manifold* manifolds, int cap) {
+    RefWorld* w = (RefWorld*)h;
+    int n = 0;
+    for (b2Contact* c = w->world->GetContactList(); c; c = c->GetNext()) {
+        if (n < cap) {
+            meta[4 * n + 0] = fixtureIndex(c->GetFixtureA());
+            meta[4 * n + 1] = fixtureIndex(c->GetFixtureB());
+            meta[4 * n + 2] = (c->IsTouching() ? 1 : 0) | (c->IsEnabled() ? 2 : 0);
+            meta[4 * n + 3] = 0;
+            manifoldOut(*c->GetManifold(), manifolds + n);
+        }
+        ++n;
+    }
+    return n;
+}
+
+// Fat AABB of each fixture's proxy, fixture creation order.
+void ref_world_get_fat_aabbs(void* h, b2g_aabb* out) {
+    RefWorld* w = (RefWorld*)h;
+    const b2BroadPhase& bp = w->world->GetContactManager().m_broadPhase;
+    for (size_t i = 0; i < w->fixtures.size(); ++i) {
+        const b2AABB& a = bp.GetFatAABB(w->fixtures[i]->m_proxies[0].proxyId);
+        out[i] = {a.lowerBound.x, a.lowerBound.y, a.upperBound.x, a.upperBound.y};
+    }
+}
+
+void ref_world_get_shapes(void* h, b2g_shape* out) {
+    RefWorld* w = (RefWorld*)h;
+    for (size_t i = 0; i < w->fixtures.size(); ++i) shapeToRec(w->fixtures[i]->GetShape(), out + i);
+}
+
+// Mean and max penetration (metres) over touching manifold points, computed
+// from b2WorldManifold separations: mean of max(0, -separation).
+void ref_world_penetration(void* h, double* out /* mean, max, points */) {
+    RefWorld* w = (RefWorld*)h;
+    double sum = 0.0, mx = 0.0;
+    long np = 0;
+    for (b2Contact* c = w->world->GetContactList(); c; c = c->GetNext()) {
+        if (!c->IsTouching()) continue;
+        b2WorldManifold wm;
+        c->GetWorldManifold(&wm);
+        for (int i = 0; i < c->GetManifold()->pointCount; ++i) {
+            double p = wm.separations[i] < 0.0f ? -(double)wm.separations[i] : 0.0;
+            sum += p;
+            mx = std::max(mx, p);
+            ++np;
+        }
+    }
+    out[0] = np ? sum / np : 0.0;
+    out[1] = mx;
+    out[2] = (double)np;
+}
+
+// ----------------------------------------------------------- per function
+int ref_collide(const b2g_shape* a, const b2g_transform* xfA, const b2g_shape* b,
+                const b2g_transform* xfB, b2g_manifold* out) {
+    b2Manifold m;
+    m.pointCount = 0;
+    b2Transform ta = toXf(xfA), tb = toXf(xfB);
+    if (a->type == B2G_POLYGON && b->type == B2G_POLYGON) {
+        b2PolygonShape pa, pb;
+        recToPolygon(a, &pa);
+        recToPolygon(b, &pb);
+        b2CollidePolygons(&m, &pa, ta, &pb, tb);
+    } else if (a->type == B2G_CIRCLE && b->type == B2G_CIRCLE) {
+        b2CircleShape ca, cb;
+        recToCircle(a, &ca);
+        recToCircle(b, &cb);
+        b2CollideCircles(&m, &ca, ta, &cb, tb);
+    } else if (a->type == B2G_POLYGON && b->type == B2G_CIRCLE) {
+        b2PolygonShape pa;
+        b2CircleShape cb;
+        recToPolygon(a, &pa);
+        recToCircle(b, &cb);
+        b2CollidePolygonAndCircle(&m, &pa, ta, &cb, tb);
+    } else if (a->type == B2G_EDGE && b->type == B2G_CIRCLE) {
+        b2EdgeShape ea;
+        b2CircleShape cb;
+        recToEdge(a, &ea);
+        recToCircle(b, &cb);
+        b2CollideEdgeAndCircle(&m, &ea, ta, &cb, tb);
+    } else if (a->type == B2G_EDGE && b->type == B2G_POLYGON) {
+        b2EdgeShape ea;
+        b2PolygonShape pb;
+        recToEdge(a, &ea);
+        recToPolygon(b, &pb);
+        b2CollideEdgeAndPolygon(&m, &ea, ta, &pb, tb);
+    } else {
+        return -1;
+    }
+    manifoldOut(m, out);
+    return 0;
+}
+
+// b2WorldManifold::Initialize; out = normal(2), points(4), separations(2).
+void ref_world_manifold(const b2g_manifold* mf, const b2g_transform* xfA, float radiusA,
+                        const b2g_transform* xfB, float radiusB, float* out) {
+    b2Manifold m;
+    m.type = (b2Manifold::Type)mf->type;
+    m.pointCount = mf->pointCount;
+    m.localNormal.Set(mf->localNormal.x, mf->localNormal.y);
+    m.localPoint.Set(mf->localPoint.x, mf->localPoint.y);
+    for (int i = 0; i < mf->pointCount; ++i) {
+        m.points[i].localPoint.Set(mf->points[i].localPoint.x, mf->points[i].localPoint.y);
+        m.points[i].id.key = mf->points[i].id;
+    }
+    b2WorldManifold wm;
+    for (int i = 0; i < 8; ++i) out[i] = 0.0f;
+    wm.Initialize(&m, toXf(xfA), radiusA, toXf(xfB), radiusB);
+    if (m.pointCount == 0) return;
+    out[0] = wm.normal.x; out[1] = wm.normal.y;
+    for (int i = 0; i < m.pointCount; ++i) {
+        out[2 + 2 * i] = wm.points[i].x;
+        out[3 + 2 * i] = wm.points[i].y;
+        out[6 + i] = wm.separations[i];
+    }
+}
+
+// b2PolygonShape::Set on raw points (hull, normals, centroid).
+void ref_polygon_set(const float* pts, int n, b2g_shape* out) {
+    b2PolygonShape p;
+    b2Vec2 v[b2_maxPolygonVertices];
+    for (int i = 0; i < n; ++i) v[i].Set(pts[2 * i], pts[2 * i + 1]);
+    p.Set(v, n);
+    shapeToRec(&p, out);
+}
+
+void ref_polygon_box(float hx, float hy, float cx, float cy, float angle, int rotated, b2g_shape* out) {
+    b2PolygonShape p;
+    if (rotated) p.SetAsBox(hx, hy, b2Vec2(cx, cy), angle);
+    else p.SetAsBox(hx, hy);
+    shapeToRec(&p, out);
+}
+
+// b2Shape::ComputeMass; out = mass, center.x, center.y, I.
+void ref_shape_mass(const b2g_shape* s, float density, float* out) {
+    b2MassData md;
+    if (s->type == B2G_POLYGON) {
+        b2PolygonShape p;
+        recToPolygon(s, &p);
+        p.ComputeMass(&md, density);
+    } else if (s->type == B2G_CIRCLE) {
+        b2CircleShape c;
+        recToCircle(s, &c);
+        c.ComputeMass(&md, density);
+    } else {
+        b2EdgeShape e;
+        recToEdge(s, &e);
+        e.ComputeMass(&md, density);
+    }
+    out[0] = md.mass; out[1] = md.center.x; out[2] = md.center.y; out[3] = md.I;
+}
+
+void ref_shape_aabb(const b2g_shape* s, const b2g_transform* xf, b2g_aabb* out) {
+    b2AABB a;
+    b2Transform t = toXf(xf);
+    if (s->type == B2G_POLYGON) {
+        b2PolygonShape p;
+        recToPolygon(s, &p);
+        p.ComputeAABB(&a, t, 0);
+    } else if (s->type == B2G_CIRCLE) {
+        b2CircleShape c;
+        recToCircle(s, &c);
+        c.ComputeAABB(&a, t, 0);
+    } else {
+        b2EdgeShape e;
+        recToEdge(s, &e);
+        e.ComputeAABB(&a, t, 0);
+    }
+    *out = {a.lowerBound.x, a.lowerBound.y, a.upperBound.x, a.upperBound.y};
+}
+
+// ------------------------------------------------------------ broad-phase
+void* ref_bp_create() { return new b2BroadPhase; }
+void ref_bp_destroy(void* h) { delete (b2BroadPhase*)h; }
+
+// Proxy user data is (index in creation order) + 1.
+int ref_bp_create_proxy(void* h, const b2g_aabb* a, int userIndex) {
+    b2AABB box;
+    box.lowerBound.Set(a->lx, a->ly);
+    box.upperBound.Set(a->ux, a->uy);
+    return ((b2BroadPhase*)h)->CreateProxy(box, (void*)(intptr_t)(userIndex + 1));
+}
+
+void ref_bp_move_proxy(void* h, int proxyId, const b2g_aabb* a, float dx, float dy) {
+    b2AABB box;
+    box.lowerBound.Set(a->lx, a->ly);
+    box.upperBound.Set(a->ux, a->uy);
+    ((b2BroadPhase*)h)->MoveProxy(proxyId, box, b2Vec2(dx, dy));
+}
+
+void ref_bp_get_fat(void* h, int proxyId, b2g_aabb* out) {
+    const b2AABB& a = ((b2BroadPhase*)h)->GetFatAABB(proxyId);
+    *out = {a.lowerBound.x, a.lowerBound.y, a.upperBound.x, a.upperBound.y};
+}
+
+// Pairs in the order UpdatePairs hands them to AddPair, as user indices.
+int ref_bp_update_pairs(void* h, int32_t* out, int cap) {
+    PairCollector pc;
+    ((b2BroadPhase*)h)->UpdatePairs(&pc);
+    const int n = (int)pc.pairs.size() / 2;
+    for (int i = 0; i < n && i < cap; ++i) {
+        out[2 * i] = pc.pairs[2 * i];
+        out[2 * i + 1] = pc.pairs[2 * i + 1];
+    }
+    return n;
+}
+
+}  // extern "C"
