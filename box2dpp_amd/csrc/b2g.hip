@@ -1,0 +1,2255 @@
+// libb2g: the MI355X (gfx950) Box2D step behind the C ABI of include/b2g.h.
+//
+// One context holds one world in HBM as structure-of-arrays. A step is a
+// sequence of kernels on the context's stream:
+//
+//   [new fixtures] hash + broad-phase pairs -> new contacts
+//   collide       per contact: fat-AABB overlap test (destroy), narrowphase,
+//                 feature-id warm-start matching, touching flag, wakes
+//   mark          constraint set of this step; release colours of contacts
+//                 that stopped being constraints
+//   compact       stable removal of destroyed contacts; hash of live pairs
+//   colour        greedy edge colouring of new constraints (persistent
+//                 colours per contact, one winner per body per round)
+//   order         constraint list sorted by colour
+//   integrate v   gravity, forces, damping
+//   init          velocity/position constraints (world manifold, masses,
+//                 restitution bias, 2-point block matrix), warm-start scale
+//   warm start, velocity iterations, store impulses   (per colour)
+//   integrate x   translation/rotation clamps
+//   position iterations (per colour, early exit on min separation)
+//   sync          transforms from sweeps
+//   fat AABBs     swept AABB and the fat-box reinsert rule -> move flags
+//   broad-phase   uniform hash grid over fat AABBs, moved-proxy queries,
+//                 existing-contact filter, new contacts in pair-key order
+//
+// Contacts are created only from moved proxies and destroyed only when their
+// fat AABBs stop overlapping, exactly as b2ContactManager does, so the
+// contact set equals the reference's at every step.
+#include "b2g_device.h"
+#include "collide.h"
+#include "prims.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+#include <stdio.h>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+using namespace b2g;
+
+// ====================================================================
+// Device counters shared by the step kernels.
+// ====================================================================
+namespace {
+
+constexpr int kMaxColours = 64;       // colours with a body-mask bit
+constexpr int kOverflowColour = 64;   // solved serially when masks are full
+constexpr int kColourSlots = 65;
+constexpr int kMaxPosIters = 32;
+constexpr int kMaxCellsPerProxy = 16;
+
+struct Counters {
+    int nContacts;
+    int nNewPairs;
+    int nConstraints;
+    int remaining;
+    int overflow;
+    int nBig;
+    int beginEvents;
+    int endEvents;
+    int points;
+    int pad[3];
+    int colourCount[kColourSlots + 3];
+    int minSep[kMaxPosIters];
+};
+
+// Constraint arrays (b2ContactVelocityConstraint / b2ContactPositionConstraint,
+// REF/Dynamics/Contacts/b2ContactSolver.h:33-84, b2ContactSolver.cpp:35-50),
+// one slot per constraint in colour order.
+struct ConView {
+    int* contact;     // contact table slot
+    int2* bodies;     // body a, body b
+    float4* mass;     // invMassA invIA invMassB invIB
+    float4* normal;   // normal.x normal.y friction tangentSpeed
+    float4* r0;       // point 0: rA.x rA.y rB.x rB.y
+    float4* r1;       // point 1
+    float4* pmass;    // normalMass0 tangentMass0 normalMass1 tangentMass1
+    float2* bias;     // velocityBias0 velocityBias1
+    float4* imp;      // normalImpulse0 tangentImpulse0 normalImpulse1 tangentImpulse1
+    float4* K;        // block matrix ex.x ex.y ey.x ey.y
+    float4* NM;       // its inverse
+    int* info;        // bits 0-1 velocity point count, 2-3 position point count,
+                      // 4 body A dynamic, 5 body B dynamic, 8-9 manifold type
+    float4* lp;       // localPoints[0], localPoints[1]
+    float4* lnp;      // localNormal, localPoint
+    float4* lc;       // localCenterA, localCenterB
+    float2* rad;      // radiusA radiusB
+};
+
+__device__ inline int encodeOrdered(float f) {
+    int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__host__ __device__ inline float decodeOrdered(int i) {
+    const int b = i >= 0 ? i : i ^ 0x7FFFFFFF;
+    union { int i; float f; } u;
+    u.i = b;
+    return u.f;
+}
+
+__host__ __device__ inline void zeroManifold(b2g_manifold& m) {
+    for (int i = 0; i < 2; ++i) {
+        m.points[i].localPoint.x = m.points[i].localPoint.y = 0.0f;
+        m.points[i].normalImpulse = m.points[i].tangentImpulse = 0.0f;
+        m.points[i].id = 0;
+    }
+    m.localNormal.x = m.localNormal.y = 0.0f;
+    m.localPoint.x = m.localPoint.y = 0.0f;
+    m.type = 0;
+    m.pointCount = 0;
+}
+
+__device__ inline unsigned long long pairKey(int a, int b) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    return ((unsigned long long)(unsigned)lo << 32) | (unsigned)hi;
+}
+
+__device__ inline uint32_t hashU64(unsigned long long k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdULL;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ULL;
+    k ^= k >> 33;
+    return (uint32_t)k;
+}
+
+// Sets the awake bit; a body that was asleep restarts its sleep timer
+// (b2Body::SetAwake(true), REF/Dynamics/b2Body.h:635-644).
+__device__ inline void wakeBody(const WorldView& w, int b) {
+    const int old = atomicOr(&w.bflags[b], BF_AWAKE);
+    if ((old & BF_AWAKE) == 0) w.gsleep[b].y = 0.0f;
+}
+
+__device__ inline bool sensorOf(const WorldView& w, int f) { return w.fmat[f].w != 0.0f; }
+
+// b2ContactFilter::ShouldCollide, REF/Dynamics/b2WorldCallbacks.cpp:26-38.
+__device__ inline bool filterPass(const WorldView& w, int fa, int fb) {
+    const int ga = w.fgroup[fa], gb = w.fgroup[fb];
+    if (ga == gb && ga != 0) return ga > 0;
+    const uint32_t a = w.ffilter[fa], b = w.ffilter[fb];
+    const uint32_t catA = a & 0xFFFF, maskA = a >> 16, catB = b & 0xFFFF, maskB = b >> 16;
+    return (maskA & catB) != 0 && (catA & maskB) != 0;
+}
+
+// ====================================================================
+// Collide (b2ContactManager::Collide + b2Contact::Update,
+// REF/Dynamics/b2ContactManager.cpp:107-172, REF/Dynamics/Contacts/b2Contact.cpp:171-251)
+// ====================================================================
+__global__ void k_collide(WorldView w, Counters* cnt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    int flags = w.cflags[c];
+    const int fA = w.cfA[c], fB = w.cfB[c];
+    const int bA = w.fbody[fA], bB = w.fbody[fB];
+    if (flags & CF_FILTER) {
+        const bool dynAny = w.btype[bA] == BT_DYNAMIC || w.btype[bB] == BT_DYNAMIC;
+        if (!dynAny || !filterPass(w, fA, fB)) {
+            if (flags & CF_TOUCHING) atomicAdd(&cnt->endEvents, 1);
+            if (w.cman[c].pointCount > 0 && !sensorOf(w, fA) && !sensorOf(w, fB)) {
+                wakeBody(w, bA);
+                wakeBody(w, bB);
+            }
+            w.cflags[c] = flags | CF_DEAD;
+            return;
+        }
+        flags &= ~CF_FILTER;
+    }
+    const bool activeA = (w.bflags[bA] & BF_AWAKE) && w.btype[bA] != BT_STATIC;
+    const bool activeB = (w.bflags[bB] & BF_AWAKE) && w.btype[bB] != BT_STATIC;
+    if (!activeA && !activeB) {
+        w.cflags[c] = flags;
+        return;
+    }
+    if (!overlaps(w.fat[fA], w.fat[fB])) {
+        if (flags & CF_TOUCHING) atomicAdd(&cnt->endEvents, 1);
+        if (w.cman[c].pointCount > 0 && !sensorOf(w, fA) && !sensorOf(w, fB)) {
+            wakeBody(w, bA);
+            wakeBody(w, bB);
+        }
+        w.cflags[c] = flags | CF_DEAD;
+        return;
+    }
+    const b2g_manifold old = w.cman[c];
+    b2g_manifold m;
+    zeroManifold(m);
+    const Xf xfA = xfOf(w.xf[bA]);
+    const Xf xfB = xfOf(w.xf[bB]);
+    collideShapes(m, w.shape[fA], xfA, w.shape[fB], xfB);
+    const bool wasTouching = (flags & CF_TOUCHING) != 0;
+    const bool touching = m.pointCount > 0;
+    for (int i = 0; i < 2; ++i) {
+        m.points[i].normalImpulse = 0.0f;
+        m.points[i].tangentImpulse = 0.0f;
+        if (i >= m.pointCount) continue;
+        for (int j = 0; j < old.pointCount; ++j) {
+            if (old.points[j].id == m.points[i].id) {
+                m.points[i].normalImpulse = old.points[j].normalImpulse;
+                m.points[i].tangentImpulse = old.points[j].tangentImpulse;
+                break;
+            }
+        }
+    }
+    if (touching != wasTouching) {
+        wakeBody(w, bA);
+        wakeBody(w, bB);
+    }
+    flags |= CF_ENABLED;
+    if (touching) flags |= CF_TOUCHING;
+    else flags &= ~CF_TOUCHING;
+    if (!wasTouching && touching) atomicAdd(&cnt->beginEvents, 1);
+    if (wasTouching && !touching) atomicAdd(&cnt->endEvents, 1);
+    w.cman[c] = m;
+    w.cflags[c] = flags;
+}
+
+// Constraint set of the step: touching, enabled, non-sensor contacts with an
+// awake non-static body (the contacts b2World::Solve's island search adds,
+// REF/Dynamics/b2World.cpp:451-475). Contacts leaving the set give their
+// colour back to both bodies' masks.
+__global__ void k_mark(WorldView w, int* keep, Counters* cnt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    int flags = w.cflags[c] & ~CF_CONSTRAINT;
+    const int fA = w.cfA[c], fB = w.cfB[c];
+    const int bA = w.fbody[fA], bB = w.fbody[fB];
+    bool isCons = false;
+    if ((flags & CF_DEAD) == 0 && (flags & CF_TOUCHING) && (flags & CF_ENABLED) && !sensorOf(w, fA) &&
+        !sensorOf(w, fB)) {
+        const bool liveA = (w.bflags[bA] & BF_AWAKE) && w.btype[bA] != BT_STATIC;
+        const bool liveB = (w.bflags[bB] & BF_AWAKE) && w.btype[bB] != BT_STATIC;
+        isCons = liveA || liveB;
+    }
+    const int col = w.ccolour[c];
+    if (col >= 0 && !isCons) {
+        if (col < kMaxColours) {
+            const unsigned long long bit = 1ull << col;
+            if (w.btype[bA] == BT_DYNAMIC) atomicAnd(&w.colourMask[bA], ~bit);
+            if (w.btype[bB] == BT_DYNAMIC) atomicAnd(&w.colourMask[bB], ~bit);
+        }
+        w.ccolour[c] = -1;
+    }
+    if (isCons) {
+        flags |= CF_CONSTRAINT;
+        if (w.ccolour[c] < 0) atomicAdd(&cnt->remaining, 1);
+    }
+    w.cflags[c] = flags;
+    keep[c] = (flags & CF_DEAD) ? 0 : 1;
+}
+
+struct ContactArrays {
+    int* cfA;
+    int* cfB;
+    int* cflags;
+    float2* cmat;
+    b2g_manifold* cman;
+    int* ccolour;
+};
+
+__global__ void k_compact(ContactArrays src, ContactArrays dst, const int* keep, const int* idx, int n,
+                          Counters* cnt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    if (c == n - 1) cnt->nContacts = idx[c] + keep[c];
+    if (!keep[c]) return;
+    const int d = idx[c];
+    dst.cfA[d] = src.cfA[c];
+    dst.cfB[d] = src.cfB[c];
+    dst.cflags[d] = src.cflags[c];
+    dst.cmat[d] = src.cmat[c];
+    dst.cman[d] = src.cman[c];
+    dst.ccolour[d] = src.ccolour[c];
+}
+
+template <typename T>
+__global__ void k_fill(T* p, int n, T v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Open-addressing set of live contact pair keys (the existence test of
+// b2ContactManager::AddPair, REF/Dynamics/b2ContactManager.cpp:196-221).
+__global__ void k_hash_insert(WorldView w, unsigned long long* table, uint32_t mask) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    const unsigned long long key = pairKey(w.cfA[c], w.cfB[c]);
+    uint32_t h = hashU64(key) & mask;
+    for (;;) {
+        const unsigned long long prev = atomicCAS(&table[h], ~0ull, key);
+        if (prev == ~0ull || prev == key) return;
+        h = (h + 1) & mask;
+    }
+}
+
+__device__ inline bool hashContains(const unsigned long long* table, uint32_t mask, unsigned long long key) {
+    uint32_t h = hashU64(key) & mask;
+    for (;;) {
+        const unsigned long long v = table[h];
+        if (v == key) return true;
+        if (v == ~0ull) return false;
+        h = (h + 1) & mask;
+    }
+}
+
+// ====================================================================
+// Colouring: a new constraint takes the lowest colour free at both of its
+// dynamic bodies. Per round, each body admits at most one new constraint
+// (the highest priority one claiming it), so no two constraints of a colour
+// share a dynamic body. Priorities are a hash of the contact's pair key, so
+// the result does not depend on thread scheduling.
+// ====================================================================
+__device__ inline unsigned long long claimValue(uint32_t tag, const WorldView& w, int c) {
+    const unsigned long long key = pairKey(w.cfA[c], w.cfB[c]);
+    const unsigned long long prio = ((unsigned long long)(hashU64(key) & 0xFFFFFFu) << 24) | (unsigned)(c & 0xFFFFFF);
+    return ((unsigned long long)tag << 48) | (0xFFFFFFFFFFFFull - prio);
+}
+
+__global__ void k_colour_claim(WorldView w, unsigned long long* claim, uint32_t tag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    if (!(w.cflags[c] & CF_CONSTRAINT) || w.ccolour[c] >= 0) return;
+    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
+    const unsigned long long v = claimValue(tag, w, c);
+    if (w.btype[bA] == BT_DYNAMIC) atomicMax(&claim[bA], v);
+    if (w.btype[bB] == BT_DYNAMIC) atomicMax(&claim[bB], v);
+}
+
+__global__ void k_colour_commit(WorldView w, const unsigned long long* claim, uint32_t tag, Counters* cnt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    if (!(w.cflags[c] & CF_CONSTRAINT) || w.ccolour[c] >= 0) return;
+    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
+    const bool dA = w.btype[bA] == BT_DYNAMIC, dB = w.btype[bB] == BT_DYNAMIC;
+    const unsigned long long v = claimValue(tag, w, c);
+    const bool won = (!dA || claim[bA] == v) && (!dB || claim[bB] == v);
+    if (!won) {
+        atomicAdd(&cnt->remaining, 1);
+        return;
+    }
+    const unsigned long long used = (dA ? w.colourMask[bA] : 0ull) | (dB ? w.colourMask[bB] : 0ull);
+    const unsigned long long freeBits = ~used;
+    if (freeBits == 0ull) {
+        w.ccolour[c] = kOverflowColour;
+        return;
+    }
+    const int col = __ffsll((long long)freeBits) - 1;
+    const unsigned long long bit = 1ull << col;
+    if (dA) atomicOr(&w.colourMask[bA], bit);
+    if (dB) atomicOr(&w.colourMask[bB], bit);
+    w.ccolour[c] = col;
+}
+
+// Gather constraint contacts with their colour as sort key.
+__global__ void k_cons_flags(WorldView w, int* flag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    flag[c] = (w.cflags[c] & CF_CONSTRAINT) ? 1 : 0;
+}
+
+__global__ void k_cons_gather(WorldView w, const int* flag, const int* idx, uint32_t* keys, int* vals, Counters* cnt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    if (c == w.nContacts - 1) cnt->nConstraints = idx[c] + flag[c];
+    if (!flag[c]) return;
+    keys[idx[c]] = (uint32_t)w.ccolour[c];
+    vals[idx[c]] = c;
+}
+
+__global__ void k_colour_hist(const uint32_t* keys, int n, Counters* cnt) {
+    __shared__ int h[kColourSlots];
+    for (int i = threadIdx.x; i < kColourSlots; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) atomicAdd(&h[keys[k]], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kColourSlots; i += blockDim.x)
+        if (h[i]) atomicAdd(&cnt->colourCount[i], h[i]);
+}
+
+// ====================================================================
+// Island solve stages (b2Island::Solve, REF/Dynamics/b2Island.cpp:182-383)
+// ====================================================================
+__device__ inline bool islandBody(const WorldView& w, int b) {
+    return w.btype[b] != BT_STATIC && (w.bflags[b] & BF_AWAKE) && (w.bflags[b] & BF_ACTIVE);
+}
+
+__global__ void k_integrate_velocities(WorldView w, V2 gravity, float h) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies || !islandBody(w, b)) return;
+    const float4 p = w.pos[b];
+    w.pos0[b] = make_float4(p.x, p.y, p.z, w.pos0[b].w);
+    if (w.btype[b] != BT_DYNAMIC) return;
+    float4 vv = w.vel[b];
+    V2 v = mk(vv.x, vv.y);
+    float wv = vv.z;
+    const float4 ms = w.mass[b];
+    const float4 f = w.force[b];
+    const float gs = w.gsleep[b].x;
+    v += h * (gs * gravity + ms.x * mk(f.x, f.y));
+    wv += h * ms.y * f.z;
+    v *= 1.0f / (1.0f + h * ms.z);
+    wv *= 1.0f / (1.0f + h * ms.w);
+    w.vel[b] = make_float4(v.x, v.y, wv, 0.0f);
+}
+
+// b2ContactSolver ctor + InitializeVelocityConstraints,
+// REF/Dynamics/Contacts/b2ContactSolver.cpp:52-252.
+__global__ void k_init_constraints(WorldView w, ConView cv, const int* order, int n, float dtRatio, int warm,
+                                   Counters* cnt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = order[k];
+    const int fA = w.cfA[c], fB = w.cfB[c];
+    const int bA = w.fbody[fA], bB = w.fbody[fB];
+    const b2g_manifold& m = w.cman[c];
+    const float4 msA = w.mass[bA], msB = w.mass[bB];
+    const float mA = msA.x, iA = msA.y, mB = msB.x, iB = msB.y;
+    const float radiusA = w.shape[fA].radius, radiusB = w.shape[fB].radius;
+    const float2 lcA2 = w.lcenter[bA], lcB2 = w.lcenter[bB];
+    const V2 lcA = mk(lcA2.x, lcA2.y), lcB = mk(lcB2.x, lcB2.y);
+    const float4 pA = w.pos[bA], pB = w.pos[bB];
+    const float4 vA4 = w.vel[bA], vB4 = w.vel[bB];
+    const V2 cA = mk(pA.x, pA.y), cB = mk(pB.x, pB.y);
+    const V2 vA = mk(vA4.x, vA4.y), vB = mk(vB4.x, vB4.y);
+    const float wA = vA4.z, wB = vB4.z;
+    const float2 mat = w.cmat[c];
+    const int pc = m.pointCount;
+    atomicAdd(&cnt->points, pc);
+
+    Xf xfA, xfB;
+    xfA.q = rotOf(pA.z);
+    xfB.q = rotOf(pB.z);
+    xfA.p = cA - mul(xfA.q, lcA);
+    xfB.p = cB - mul(xfB.q, lcB);
+    WorldManifold wm;
+    worldManifold(wm, m, xfA, radiusA, xfB, radiusB);
+    const V2 normal = wm.normal;
+    const V2 tangent = cross(normal, 1.0f);
+
+    float nm[2] = {0.0f, 0.0f}, tm[2] = {0.0f, 0.0f}, vb[2] = {0.0f, 0.0f};
+    V2 rA[2] = {mk(0.0f, 0.0f), mk(0.0f, 0.0f)}, rB[2] = {mk(0.0f, 0.0f), mk(0.0f, 0.0f)};
+    float ni[2] = {0.0f, 0.0f}, ti[2] = {0.0f, 0.0f};
+    for (int j = 0; j < pc; ++j) {
+        if (warm) {
+            ni[j] = dtRatio * m.points[j].normalImpulse;
+            ti[j] = dtRatio * m.points[j].tangentImpulse;
+        }
+        rA[j] = wm.points[j] - cA;
+        rB[j] = wm.points[j] - cB;
+        const float rnA = cross(rA[j], normal);
+        const float rnB = cross(rB[j], normal);
+        const float kNormal = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+        nm[j] = kNormal > 0.0f ? 1.0f / kNormal : 0.0f;
+        const float rtA = cross(rA[j], tangent);
+        const float rtB = cross(rB[j], tangent);
+        const float kTangent = mA + mB + iA * rtA * rtA + iB * rtB * rtB;
+        tm[j] = kTangent > 0.0f ? 1.0f / kTangent : 0.0f;
+        const float vRel = dot(normal, vB + cross(wB, rB[j]) - vA - cross(wA, rA[j]));
+        if (vRel < -kVelocityThreshold) vb[j] = -mat.y * vRel;
+    }
+    int vpc = pc;
+    float4 K = make_float4(0.0f, 0.0f, 0.0f, 0.0f), NM = K;
+    if (pc == 2) {
+        const float rn1A = cross(rA[0], normal), rn1B = cross(rB[0], normal);
+        const float rn2A = cross(rA[1], normal), rn2B = cross(rB[1], normal);
+        const float k11 = mA + mB + iA * rn1A * rn1A + iB * rn1B * rn1B;
+        const float k22 = mA + mB + iA * rn2A * rn2A + iB * rn2B * rn2B;
+        const float k12 = mA + mB + iA * rn1A * rn2A + iB * rn1B * rn2B;
+        if (k11 * k11 < 1000.0f * (k11 * k22 - k12 * k12)) {
+            K = make_float4(k11, k12, k12, k22);
+            // b2Mat22::GetInverse: a = ex.x, b = ey.x, c = ex.y, d = ey.y
+            float det = k11 * k22 - k12 * k12;
+            if (det != 0.0f) det = 1.0f / det;
+            NM = make_float4(det * k22, -det * k12, -det * k12, det * k11);
+        } else {
+            vpc = 1;
+        }
+    }
+    const bool dynA = w.btype[bA] == BT_DYNAMIC, dynB = w.btype[bB] == BT_DYNAMIC;
+    cv.contact[k] = c;
+    cv.bodies[k] = make_int2(bA, bB);
+    cv.mass[k] = make_float4(mA, iA, mB, iB);
+    cv.normal[k] = make_float4(normal.x, normal.y, mat.x, 0.0f);
+    cv.r0[k] = make_float4(rA[0].x, rA[0].y, rB[0].x, rB[0].y);
+    cv.r1[k] = make_float4(rA[1].x, rA[1].y, rB[1].x, rB[1].y);
+    cv.pmass[k] = make_float4(nm[0], tm[0], nm[1], tm[1]);
+    cv.bias[k] = make_float2(vb[0], vb[1]);
+    cv.imp[k] = make_float4(ni[0], ti[0], ni[1], ti[1]);
+    cv.K[k] = K;
+    cv.NM[k] = NM;
+    cv.info[k] = vpc | (pc << 2) | (dynA ? 16 : 0) | (dynB ? 32 : 0) | (m.type << 8);
+    cv.lp[k] = make_float4(m.points[0].localPoint.x, m.points[0].localPoint.y, m.points[1].localPoint.x,
+                           m.points[1].localPoint.y);
+    cv.lnp[k] = make_float4(m.localNormal.x, m.localNormal.y, m.localPoint.x, m.localPoint.y);
+    cv.lc[k] = make_float4(lcA.x, lcA.y, lcB.x, lcB.y);
+    cv.rad[k] = make_float2(radiusA, radiusB);
+}
+
+// b2ContactSolver::WarmStart, REF/Dynamics/Contacts/b2ContactSolver.cpp:254-292.
+__global__ void k_warm_start(WorldView w, ConView cv, int begin, int end) {
+    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    const int2 bi = cv.bodies[k];
+    const float4 ms = cv.mass[k];
+    const int info = cv.info[k];
+    const int pc = info & 3;
+    const float4 va = w.vel[bi.x], vbv = w.vel[bi.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const float4 nf = cv.normal[k];
+    const V2 normal = mk(nf.x, nf.y);
+    const V2 tangent = cross(normal, 1.0f);
+    const float4 imp = cv.imp[k];
+    const float4 r0 = cv.r0[k], r1 = cv.r1[k];
+    for (int j = 0; j < pc; ++j) {
+        const float ni = j == 0 ? imp.x : imp.z;
+        const float ti = j == 0 ? imp.y : imp.w;
+        const float4 r = j == 0 ? r0 : r1;
+        const V2 P = ni * normal + ti * tangent;
+        wA -= ms.y * cross(mk(r.x, r.y), P);
+        vA -= ms.x * P;
+        wB += ms.w * cross(mk(r.z, r.w), P);
+        vB += ms.z * P;
+    }
+    if (info & 16) w.vel[bi.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (info & 32) w.vel[bi.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2ContactSolver::SolveVelocityConstraints, REF/Dynamics/Contacts/b2ContactSolver.cpp:294-597:
+// friction per point clamped to the friction cone, then the normal impulse,
+// by a 2x2 linear complementarity solve (total enumeration) for two points.
+__global__ void k_solve_velocity(WorldView w, ConView cv, int begin, int end) {
+    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    const int2 bi = cv.bodies[k];
+    const float4 ms = cv.mass[k];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const int info = cv.info[k];
+    const int pc = info & 3;
+    const float4 va = w.vel[bi.x], vbv = w.vel[bi.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const float4 nf = cv.normal[k];
+    const V2 normal = mk(nf.x, nf.y);
+    const V2 tangent = cross(normal, 1.0f);
+    const float friction = nf.z;
+    const float tangentSpeed = nf.w;
+    float4 imp = cv.imp[k];
+    const float4 pm = cv.pmass[k];
+    const float4 r0 = cv.r0[k], r1 = cv.r1[k];
+    const V2 rA0 = mk(r0.x, r0.y), rB0 = mk(r0.z, r0.w);
+    const V2 rA1 = mk(r1.x, r1.y), rB1 = mk(r1.z, r1.w);
+
+    // Tangent constraints.
+    {
+        const V2 dv = vB + cross(wB, rB0) - vA - cross(wA, rA0);
+        const float vt = dot(dv, tangent) - tangentSpeed;
+        float lambda = pm.y * (-vt);
+        const float maxF = friction * imp.x;
+        const float ni = clampB(imp.y + lambda, -maxF, maxF);
+        lambda = ni - imp.y;
+        imp.y = ni;
+        const V2 P = lambda * tangent;
+        vA -= mA * P;
+        wA -= iA * cross(rA0, P);
+        vB += mB * P;
+        wB += iB * cross(rB0, P);
+    }
+    if (pc == 2) {
+        const V2 dv = vB + cross(wB, rB1) - vA - cross(wA, rA1);
+        const float vt = dot(dv, tangent) - tangentSpeed;
+        float lambda = pm.w * (-vt);
+        const float maxF = friction * imp.z;
+        const float ni = clampB(imp.w + lambda, -maxF, maxF);
+        lambda = ni - imp.w;
+        imp.w = ni;
+        const V2 P = lambda * tangent;
+        vA -= mA * P;
+        wA -= iA * cross(rA1, P);
+        vB += mB * P;
+        wB += iB * cross(rB1, P);
+    }
+    // Normal constraints.
+    if (pc == 1) {
+        const V2 dv = vB + cross(wB, rB0) - vA - cross(wA, rA0);
+        const float vn = dot(dv, normal);
+        float lambda = -pm.x * (vn - cv.bias[k].x);
+        const float ni = fmaxB(imp.x + lambda, 0.0f);
+        lambda = ni - imp.x;
+        imp.x = ni;
+        const V2 P = lambda * normal;
+        vA -= mA * P;
+        wA -= iA * cross(rA0, P);
+        vB += mB * P;
+        wB += iB * cross(rB0, P);
+    } else {
+        const float2 vbias = cv.bias[k];
+        const float4 K = cv.K[k], NM = cv.NM[k];
+        const V2 a = mk(imp.x, imp.z);
+        const V2 dv1 = vB + cross(wB, rB0) - vA - cross(wA, rA0);
+        const V2 dv2 = vB + cross(wB, rB1) - vA - cross(wA, rA1);
+        const float vn1 = dot(dv1, normal);
+        const float vn2 = dot(dv2, normal);
+        V2 b;
+        b.x = vn1 - vbias.x;
+        b.y = vn2 - vbias.y;
+        b -= mk(K.x * a.x + K.z * a.y, K.y * a.x + K.w * a.y);
+        V2 x;
+        bool found = false;
+        // Case 1: both points active.
+        {
+            const V2 Mb = mk(NM.x * b.x + NM.z * b.y, NM.y * b.x + NM.w * b.y);
+            x = -Mb;
+            if (x.x >= 0.0f && x.y >= 0.0f) found = true;
+        }
+        // Case 2: point 1 active, point 2 separating.
+        if (!found) {
+            x.x = -pm.x * b.x;
+            x.y = 0.0f;
+            const float vn2b = K.y * x.x + b.y;
+            if (x.x >= 0.0f && vn2b >= 0.0f) found = true;
+        }
+        // Case 3: point 2 active, point 1 separating.
+        if (!found) {
+            x.x = 0.0f;
+            x.y = -pm.z * b.y;
+            const float vn1b = K.z * x.y + b.x;
+            if (x.y >= 0.0f && vn1b >= 0.0f) found = true;
+        }
+        // Case 4: both separating.
+        if (!found) {
+            x.x = 0.0f;
+            x.y = 0.0f;
+            if (b.x >= 0.0f && b.y >= 0.0f) found = true;
+        }
+        if (found) {
+            const V2 d = x - a;
+            const V2 P1 = d.x * normal;
+            const V2 P2 = d.y * normal;
+            vA -= mA * (P1 + P2);
+            wA -= iA * (cross(rA0, P1) + cross(rA1, P2));
+            vB += mB * (P1 + P2);
+            wB += iB * (cross(rB0, P1) + cross(rB1, P2));
+            imp.x = x.x;
+            imp.z = x.y;
+        }
+    }
+    cv.imp[k] = imp;
+    if (info & 16) w.vel[bi.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (info & 32) w.vel[bi.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2ContactSolver::StoreImpulses, REF/Dynamics/Contacts/b2ContactSolver.cpp:599-612.
+__global__ void k_store_impulses(WorldView w, ConView cv, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = cv.contact[k];
+    const int vpc = cv.info[k] & 3;
+    const float4 imp = cv.imp[k];
+    w.cman[c].points[0].normalImpulse = imp.x;
+    w.cman[c].points[0].tangentImpulse = imp.y;
+    if (vpc == 2) {
+        w.cman[c].points[1].normalImpulse = imp.z;
+        w.cman[c].points[1].tangentImpulse = imp.w;
+    }
+}
+
+// Position integration with translation and rotation clamps,
+// REF/Dynamics/b2Island.cpp:276-306.
+__global__ void k_integrate_positions(WorldView w, float h) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies || !islandBody(w, b)) return;
+    float4 p = w.pos[b];
+    const float4 vv = w.vel[b];
+    V2 c = mk(p.x, p.y);
+    float a = p.z;
+    V2 v = mk(vv.x, vv.y);
+    float wv = vv.z;
+    const V2 translation = h * v;
+    if (dot(translation, translation) > kMaxTranslationSq) {
+        const float ratio = kMaxTranslation / lengthOf(translation);
+        v *= ratio;
+    }
+    const float rotation = h * wv;
+    if (rotation * rotation > kMaxRotationSq) {
+        const float ratio = kMaxRotation / absB(rotation);
+        wv *= ratio;
+    }
+    c += h * v;
+    a += h * wv;
+    w.pos[b] = make_float4(c.x, c.y, a, 0.0f);
+    w.vel[b] = make_float4(v.x, v.y, wv, 0.0f);
+}
+
+// b2ContactSolver::SolvePositionConstraints with b2PositionSolverManifold,
+// REF/Dynamics/Contacts/b2ContactSolver.cpp:614-744. Skips itself when the
+// previous iteration already met the early-exit test of b2Island::Solve.
+__global__ void k_solve_position(WorldView w, ConView cv, int begin, int end, Counters* cnt, int it) {
+    if (it > 0 && decodeOrdered(cnt->minSep[it - 1]) >= -3.0f * kLinearSlop) return;
+    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    const int2 bi = cv.bodies[k];
+    const float4 ms = cv.mass[k];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const int info = cv.info[k];
+    const int pc = (info >> 2) & 3;
+    const int type = (info >> 8) & 3;
+    const float4 lc = cv.lc[k];
+    const V2 lcA = mk(lc.x, lc.y), lcB = mk(lc.z, lc.w);
+    const float4 lp = cv.lp[k];
+    const float4 lnp = cv.lnp[k];
+    const float2 rad = cv.rad[k];
+    const float4 pA = w.pos[bi.x], pB = w.pos[bi.y];
+    V2 cA = mk(pA.x, pA.y), cB = mk(pB.x, pB.y);
+    float aA = pA.z, aB = pB.z;
+    float minSep = 0.0f;
+    for (int j = 0; j < pc; ++j) {
+        Xf xfA, xfB;
+        xfA.q = rotOf(aA);
+        xfB.q = rotOf(aB);
+        xfA.p = cA - mul(xfA.q, lcA);
+        xfB.p = cB - mul(xfB.q, lcB);
+        V2 normal, point;
+        float separation;
+        if (type == B2G_MANIFOLD_CIRCLES) {
+            const V2 pointA = mul(xfA, mk(lnp.z, lnp.w));
+            const V2 pointB = mul(xfB, mk(lp.x, lp.y));
+            normal = pointB - pointA;
+            normalizeV(normal);
+            point = 0.5f * (pointA + pointB);
+            separation = dot(pointB - pointA, normal) - rad.x - rad.y;
+        } else if (type == B2G_MANIFOLD_FACE_A) {
+            normal = mul(xfA.q, mk(lnp.x, lnp.y));
+            const V2 planePoint = mul(xfA, mk(lnp.z, lnp.w));
+            const V2 clipPoint = mul(xfB, j == 0 ? mk(lp.x, lp.y) : mk(lp.z, lp.w));
+            separation = dot(clipPoint - planePoint, normal) - rad.x - rad.y;
+            point = clipPoint;
+        } else {
+            normal = mul(xfB.q, mk(lnp.x, lnp.y));
+            const V2 planePoint = mul(xfB, mk(lnp.z, lnp.w));
+            const V2 clipPoint = mul(xfA, j == 0 ? mk(lp.x, lp.y) : mk(lp.z, lp.w));
+            separation = dot(clipPoint - planePoint, normal) - rad.x - rad.y;
+            point = clipPoint;
+            normal = -normal;
+        }
+        const V2 rA = point - cA;
+        const V2 rB = point - cB;
+        minSep = fminB(minSep, separation);
+        const float C = clampB(kBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, 0.0f);
+        const float rnA = cross(rA, normal);
+        const float rnB = cross(rB, normal);
+        const float Kn = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+        const float impulse = Kn > 0.0f ? -C / Kn : 0.0f;
+        const V2 P = impulse * normal;
+        cA -= mA * P;
+        aA -= iA * cross(rA, P);
+        cB += mB * P;
+        aB += iB * cross(rB, P);
+    }
+    if (info & 16) w.pos[bi.x] = make_float4(cA.x, cA.y, aA, 0.0f);
+    if (info & 32) w.pos[bi.y] = make_float4(cB.x, cB.y, aB, 0.0f);
+    if (minSep < 0.0f) atomicMin(&cnt->minSep[it], encodeOrdered(minSep));
+}
+
+// Copy-back transform of every solved body (b2Body::SynchronizeTransform,
+// REF/Dynamics/b2Body.h:836-840). Static bodies keep their creation-time
+// transform, which the reference's recomputation reproduces exactly.
+__global__ void k_sync_transforms(WorldView w) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies || !islandBody(w, b)) return;
+    const float4 p = w.pos[b];
+    const float2 lc = w.lcenter[b];
+    Xf xf;
+    xf.q = rotOf(p.z);
+    xf.p = mk(p.x, p.y) - mul(xf.q, mk(lc.x, lc.y));
+    w.xf[b] = f4(xf);
+}
+
+__device__ inline float4 shapeAABB(const b2g_shape& s, const Xf& xf) {
+    if (s.type == B2G_CIRCLE) {
+        const V2 p = xf.p + mul(xf.q, ld(s.centroid));
+        return make_float4(p.x - s.radius, p.y - s.radius, p.x + s.radius, p.y + s.radius);
+    }
+    const int n = s.type == B2G_EDGE ? 2 : s.count;
+    V2 lo = mul(xf, ld(s.verts[0]));
+    V2 hi = lo;
+    for (int i = 1; i < n; ++i) {
+        const V2 v = mul(xf, ld(s.verts[i]));
+        lo = vmin(lo, v);
+        hi = vmax(hi, v);
+    }
+    const V2 r = mk(s.radius, s.radius);
+    const V2 l = lo - r, u = hi + r;
+    return make_float4(l.x, l.y, u.x, u.y);
+}
+
+// b2DynamicTree::MoveProxy's fat-box rule (REF/Collision/b2DynamicTree.cpp:133-181):
+// keep the fat box while it contains the tight one, else re-fatten by the
+// AABB extension and stretch by twice the displacement toward the motion.
+__device__ inline bool fatRule(float4& fat, float4 tight, V2 disp) {
+    if (containsBox(fat, tight)) return false;
+    float4 b = make_float4(tight.x - kAabbExtension, tight.y - kAabbExtension, tight.z + kAabbExtension,
+                           tight.w + kAabbExtension);
+    const V2 d = kAabbMultiplier * disp;
+    if (d.x < 0.0f) b.x += d.x; else b.z += d.x;
+    if (d.y < 0.0f) b.y += d.y; else b.w += d.y;
+    fat = b;
+    return true;
+}
+
+// b2Body::SynchronizeFixtures + b2Fixture::Synchronize,
+// REF/Dynamics/b2Body.cpp:443-454, REF/Dynamics/b2Fixture.cpp:154-176.
+__global__ void k_sync_fixtures(WorldView w) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= w.nFixtures) return;
+    const int b = w.fbody[f];
+    if (!islandBody(w, b)) return;
+    const float4 p0 = w.pos0[b];
+    const float2 lc = w.lcenter[b];
+    Xf xf1;
+    xf1.q = rotOf(p0.z);
+    xf1.p = mk(p0.x, p0.y) - mul(xf1.q, mk(lc.x, lc.y));
+    const Xf xf2 = xfOf(w.xf[b]);
+    const b2g_shape& s = w.shape[f];
+    const float4 a1 = shapeAABB(s, xf1);
+    const float4 a2 = shapeAABB(s, xf2);
+    const float4 tight = make_float4(fminB(a1.x, a2.x), fminB(a1.y, a2.y), fmaxB(a1.z, a2.z), fmaxB(a1.w, a2.w));
+    float4 fat = w.fat[f];
+    if (fatRule(fat, tight, xf2.p - xf1.p)) {
+        w.fat[f] = fat;
+        w.moved[f] = 1;
+    }
+}
+
+// ====================================================================
+// Broad-phase: uniform hash grid over fat AABBs. Every proxy is binned into
+// the cells its fat box covers; proxies covering more than
+// kMaxCellsPerProxy cells go to a short "wide" list tested by brute force.
+// Each moved proxy reports the pairs b2BroadPhase::UpdatePairs would
+// (REF/Collision/b2BroadPhase.h:188-241): every other proxy whose fat box
+// overlaps its own. A pair of two moved proxies is reported by the lower id
+// only, and a pair seen in several cells only in the cell holding the lower
+// corner of the two boxes' intersection, so each pair appears once.
+// ====================================================================
+struct GridView {
+    float invCell;
+    uint32_t bucketMask;
+    int* counts;     // nBuckets + 1
+    int* starts;     // exclusive scan of counts
+    int* cursor;     // fill cursor per bucket
+    int* entries;    // proxy ids by bucket
+    unsigned long long* ecell;  // the cell each entry was binned for (buckets mix cells)
+    int* big;        // wide proxy ids
+    int4* range;     // per proxy cell range (x0, y0, x1, y1); x0 > x1 marks a wide proxy
+};
+
+__device__ inline int cellCoord(float x, float inv) {
+    float c = floorf(x * inv);
+    c = fminB(fmaxB(c, -1.0e9f), 1.0e9f);
+    return (int)c;
+}
+__device__ inline unsigned long long cellKey(int ix, int iy) {
+    return ((unsigned long long)(uint32_t)ix << 32) | (uint32_t)iy;
+}
+__device__ inline uint32_t cellHash(int ix, int iy) {
+    return ((uint32_t)ix * 0x8DA6B343u) ^ ((uint32_t)iy * 0xD8163841u);
+}
+
+__global__ void k_grid_count(const float4* fat, int n, const uint8_t* hasProxy, GridView g, Counters* cnt) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    if (hasProxy && !hasProxy[f]) {
+        g.range[f] = make_int4(1, 0, 0, 0);
+        return;
+    }
+    const float4 a = fat[f];
+    if (!(a.x <= a.z && a.y <= a.w)) {
+        g.range[f] = make_int4(1, 0, 0, -1);  // removed proxy: not binned, never queried
+        return;
+    }
+    const int x0 = cellCoord(a.x, g.invCell), y0 = cellCoord(a.y, g.invCell);
+    const int x1 = cellCoord(a.z, g.invCell), y1 = cellCoord(a.w, g.invCell);
+    const long long cells = (long long)(x1 - x0 + 1) * (long long)(y1 - y0 + 1);
+    if (cells > kMaxCellsPerProxy) {
+        g.range[f] = make_int4(1, 0, 0, 0);
+        const int slot = atomicAdd(&cnt->nBig, 1);
+        g.big[slot] = f;
+        return;
+    }
+    g.range[f] = make_int4(x0, y0, x1, y1);
+    for (int iy = y0; iy <= y1; ++iy)
+        for (int ix = x0; ix <= x1; ++ix) atomicAdd(&g.counts[cellHash(ix, iy) & g.bucketMask], 1);
+}
+
+__global__ void k_grid_scatter(int n, GridView g) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    const int4 r = g.range[f];
+    if (r.x > r.z) return;
+    for (int iy = r.y; iy <= r.w; ++iy)
+        for (int ix = r.x; ix <= r.z; ++ix) {
+            const uint32_t bkt = cellHash(ix, iy) & g.bucketMask;
+            const int slot = atomicAdd(&g.cursor[bkt], 1);
+            g.entries[g.starts[bkt] + slot] = f;
+            g.ecell[g.starts[bkt] + slot] = cellKey(ix, iy);
+        }
+}
+
+// Pair acceptance of b2ContactManager::AddPair (REF/Dynamics/b2ContactManager.cpp:179-230):
+// distinct bodies, no existing contact, at least one dynamic body, filter
+// pass, and a contact type for the two shapes.
+__device__ inline bool acceptPair(const WorldView& w, int a, int b, const unsigned long long* table, uint32_t hmask) {
+    const int ba = w.fbody[a], bb = w.fbody[b];
+    if (ba == bb) return false;
+    if (table && hashContains(table, hmask, pairKey(a, b))) return false;
+    if (w.btype[ba] != BT_DYNAMIC && w.btype[bb] != BT_DYNAMIC) return false;
+    if (!filterPass(w, a, b)) return false;
+    const int ta = w.shape[a].type, tb = w.shape[b].type;
+    return primaryPair(ta, tb) || primaryPair(tb, ta);
+}
+
+struct PairSink {
+    unsigned long long* keys;
+    int cap;
+};
+
+__device__ inline void emitPair(PairSink sink, Counters* cnt, int a, int b) {
+    const int slot = atomicAdd(&cnt->nNewPairs, 1);
+    if (slot < sink.cap) sink.keys[slot] = pairKey(a, b);
+    else atomicOr(&cnt->overflow, 1);
+}
+
+// One thread per proxy; moved proxies query. `world` is null for the bare
+// pair-set computation (no contact filtering).
+__global__ void k_pairs_query(const float4* fat, const uint8_t* moved, int n, GridView g, WorldView w, bool world,
+                              const unsigned long long* table, uint32_t hmask, PairSink sink, Counters* cnt) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n || !moved[q]) return;
+    const float4 fq = fat[q];
+    const int4 r = g.range[q];
+    if (r.w < 0 && r.x > r.z) return;  // removed proxy
+    auto consider = [&](int p, bool checkCell, int ix, int iy) {
+        if (p == q) return;
+        if (moved[p] && p < q) return;  // reported by p's own query
+        const float4 fp = fat[p];
+        if (!overlaps(fq, fp)) return;
+        if (checkCell) {
+            const int ox = cellCoord(fmaxB(fq.x, fp.x), g.invCell);
+            const int oy = cellCoord(fmaxB(fq.y, fp.y), g.invCell);
+            if (ox != ix || oy != iy) return;
+        }
+        const int a = q < p ? q : p, b = q < p ? p : q;
+        if (world && !acceptPair(w, a, b, table, hmask)) return;
+        emitPair(sink, cnt, a, b);
+    };
+    const int nBig = cnt->nBig;
+    if (r.x > r.z) {
+        // A wide proxy is not binned; it tests every proxy (removed proxies
+        // fail every overlap test).
+        for (int p = 0; p < n; ++p) consider(p, false, 0, 0);
+        return;
+    }
+    for (int iy = r.y; iy <= r.w; ++iy)
+        for (int ix = r.x; ix <= r.z; ++ix) {
+            const uint32_t bkt = cellHash(ix, iy) & g.bucketMask;
+            const int s0 = g.starts[bkt], s1 = g.starts[bkt + 1];
+            const unsigned long long ck = cellKey(ix, iy);
+            for (int e = s0; e < s1; ++e)
+                if (g.ecell[e] == ck) consider(g.entries[e], true, ix, iy);
+        }
+    for (int i = 0; i < nBig; ++i) consider(g.big[i], false, 0, 0);
+}
+
+// New contacts from the sorted pair keys (b2Contact::Create with the
+// registry's fixture swap, b2Contact ctor, AddPair's wake-up;
+// REF/Dynamics/Contacts/b2Contact.cpp:64-95, 136-168, REF/Dynamics/b2ContactManager.cpp:284-293).
+__global__ void k_append_contacts(WorldView w, const unsigned long long* keys, int nNew, int base) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nNew) return;
+    const unsigned long long key = keys[k];
+    int a = (int)(key >> 32), b = (int)(key & 0xFFFFFFFFu);
+    if (!primaryPair(w.shape[a].type, w.shape[b].type)) {
+        const int t = a;
+        a = b;
+        b = t;
+    }
+    const int c = base + k;
+    w.cfA[c] = a;
+    w.cfB[c] = b;
+    w.cflags[c] = CF_ENABLED;
+    const float4 ma = w.fmat[a], mb = w.fmat[b];
+    w.cmat[c] = make_float2(sqrtf(ma.x * mb.x), ma.y > mb.y ? ma.y : mb.y);
+    b2g_manifold m;
+    zeroManifold(m);
+    w.cman[c] = m;
+    w.ccolour[c] = -1;
+    if (ma.w == 0.0f && mb.w == 0.0f) {
+        wakeBody(w, w.fbody[a]);
+        wakeBody(w, w.fbody[b]);
+    }
+}
+
+// Proxy removal: an inverted fat box overlaps nothing.
+__global__ void k_remove_fixtures(WorldView w, const int* ids, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int f = ids[i];
+    w.fat[f] = make_float4(FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX);
+    w.moved[f] = 0;
+}
+
+// b2Fixture::Refilter (REF/Dynamics/b2Fixture.cpp:183-213): flag contacts of
+// the given fixtures for filtering.
+__global__ void k_flag_filter(WorldView w, const uint8_t* mark) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    if (mark[w.cfA[c]] || mark[w.cfB[c]]) w.cflags[c] |= CF_FILTER;
+}
+
+// ====================================================================
+// Batch kernels for per-function parity
+// ====================================================================
+__global__ void k_collide_batch(const b2g_shape* A, const b2g_transform* xa, const b2g_shape* B,
+                                const b2g_transform* xb, int n, b2g_manifold* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    b2g_manifold m;
+    zeroManifold(m);
+    Xf xfA, xfB;
+    xfA.p = mk(xa[i].px, xa[i].py); xfA.q.s = xa[i].s; xfA.q.c = xa[i].c;
+    xfB.p = mk(xb[i].px, xb[i].py); xfB.q.s = xb[i].s; xfB.q.c = xb[i].c;
+    collideShapes(m, A[i], xfA, B[i], xfB);
+    if (m.pointCount == 0) zeroManifold(m);
+    out[i] = m;
+}
+
+__global__ void k_world_manifold_batch(const b2g_manifold* m, const b2g_transform* xa, const float* ra,
+                                       const b2g_transform* xb, const float* rb, int n, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    WorldManifold wm;
+    wm.normal = mk(0.0f, 0.0f);
+    wm.points[0] = wm.points[1] = mk(0.0f, 0.0f);
+    wm.separations[0] = wm.separations[1] = 0.0f;
+    Xf xfA, xfB;
+    xfA.p = mk(xa[i].px, xa[i].py); xfA.q.s = xa[i].s; xfA.q.c = xa[i].c;
+    xfB.p = mk(xb[i].px, xb[i].py); xfB.q.s = xb[i].s; xfB.q.c = xb[i].c;
+    worldManifold(wm, m[i], xfA, ra[i], xfB, rb[i]);
+    float* o = out + 8 * i;
+    o[0] = wm.normal.x; o[1] = wm.normal.y;
+    o[2] = wm.points[0].x; o[3] = wm.points[0].y; o[4] = wm.points[1].x; o[5] = wm.points[1].y;
+    o[6] = wm.separations[0]; o[7] = wm.separations[1];
+}
+
+__global__ void k_fat_rule_batch(const float4* fat, const float4* tight, const float2* disp, int n, float4* out,
+                                 uint8_t* moved) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 f = fat[i];
+    moved[i] = fatRule(f, tight[i], mk(disp[i].x, disp[i].y)) ? 1 : 0;
+    out[i] = f;
+}
+
+__global__ void k_sincos_batch(const float* a, int n, float* s, float* c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Rot q = rotOf(a[i]);
+    s[i] = q.s;
+    c[i] = q.c;
+}
+
+__global__ void k_penetration(WorldView w, double* acc, int* npts, unsigned long long* maxBits) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    if (!(w.cflags[c] & CF_TOUCHING)) return;
+    const int fA = w.cfA[c], fB = w.cfB[c];
+    const b2g_manifold& m = w.cman[c];
+    WorldManifold wm;
+    worldManifold(wm, m, xfOf(w.xf[w.fbody[fA]]), w.shape[fA].radius, xfOf(w.xf[w.fbody[fB]]), w.shape[fB].radius);
+    for (int i = 0; i < m.pointCount; ++i) {
+        const double p = wm.separations[i] < 0.0f ? -(double)wm.separations[i] : 0.0;
+        atomicAdd(acc, p);
+        atomicMax(maxBits, (unsigned long long)__double_as_longlong(p));
+        atomicAdd(npts, 1);
+    }
+}
+
+inline int blocks(int n, int bs = 256) { return (n + bs - 1) / bs; }
+
+}  // namespace
+
+// ====================================================================
+// Host side
+// ====================================================================
+struct b2g_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // capacities
+    int capBodies = 0, capFixtures = 0, capContacts = 0;
+    int nBodies = 0, nFixtures = 0, nContacts = 0;
+
+    // body arrays
+    int* btype = nullptr;
+    int* bflags = nullptr;
+    float4 *xf = nullptr, *pos = nullptr, *pos0 = nullptr, *vel = nullptr, *force = nullptr, *mass = nullptr;
+    float2 *lcenter = nullptr, *gsleep = nullptr;
+    unsigned long long* colourMask = nullptr;
+    unsigned long long* claim = nullptr;
+    int* islandOf = nullptr;
+
+    // fixture arrays
+    int* fbody = nullptr;
+    b2g_shape* shape = nullptr;
+    float4* fmat = nullptr;
+    uint32_t* ffilter = nullptr;
+    int* fgroup = nullptr;
+    float4* fat = nullptr;
+    uint8_t* moved = nullptr;
+    int4* range = nullptr;
+
+    // contacts (two sets for compaction)
+    ContactArrays cur{}, alt{};
+    int* keep = nullptr;
+    int* idx = nullptr;
+    unsigned long long* hashTable = nullptr;
+    uint32_t hashMask = 0;
+    int hashCap = 0;
+    unsigned long long* newKeys = nullptr;
+    unsigned long long* newKeysSorted = nullptr;
+
+    // constraints
+    ConView cv{};
+    uint32_t* consKeys = nullptr;
+    uint32_t* consKeysSorted = nullptr;
+    int* consVals = nullptr;
+    int* consOrder = nullptr;
+
+    // grid
+    int nBuckets = 0, capEntries = 0;
+    int *gCounts = nullptr, *gStarts = nullptr, *gCursor = nullptr, *gEntries = nullptr, *gBig = nullptr;
+    unsigned long long* gCell = nullptr;
+    float cellSize = 2.0f;
+    bool cellSizeSet = false;
+
+    // scratch
+    void* scratch = nullptr;
+    size_t scratchBytes = 0;
+    Counters* cnt = nullptr;       // device
+    Counters* hcnt = nullptr;      // pinned host mirror
+    double* penAcc = nullptr;
+
+    // world switches
+    V2 gravity = mk(0.0f, -10.0f);
+    bool allowSleep = true, warmStarting = true, continuous = false, subStepping = false, autoClearForces = true;
+    bool newFixture = false;
+    float inv_dt0 = 0.0f;
+    uint32_t claimTag = 0;
+    b2g_step_stats stats{};
+
+    // profiling (HIP events around launches)
+    bool profiling = false;
+    std::vector<hipEvent_t> evPool;
+    int evUsed = 0;
+    std::vector<std::pair<int, int>> evMarks;  // (phase, start event index)
+    double profMs[32] = {};
+    int profLaunches[32] = {};
+
+    WorldView view() const {
+        WorldView w;
+        w.nBodies = nBodies;
+        w.btype = btype; w.bflags = bflags; w.xf = xf; w.pos = pos; w.pos0 = pos0; w.lcenter = lcenter;
+        w.vel = vel; w.force = force; w.mass = mass; w.gsleep = gsleep; w.colourMask = colourMask;
+        w.islandOf = islandOf;
+        w.nFixtures = nFixtures;
+        w.fbody = fbody; w.shape = shape; w.fmat = fmat; w.ffilter = ffilter; w.fgroup = fgroup; w.fat = fat;
+        w.moved = moved;
+        w.nContacts = nContacts;
+        w.cfA = cur.cfA; w.cfB = cur.cfB; w.cflags = cur.cflags; w.cmat = cur.cmat; w.cman = cur.cman;
+        w.ccolour = cur.ccolour;
+        return w;
+    }
+};
+
+namespace {
+
+int fail(b2g_context* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(ctx, B2G_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+#define LAUNCHCHK(ctx)                                                                             \
+    do {                                                                                           \
+        hipError_t e_ = hipGetLastError();                                                         \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(ctx, B2G_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+hipError_t growArray(T*& p, int oldCount, int newCap, hipStream_t s) {
+    T* q = nullptr;
+    hipError_t e = hipMalloc((void**)&q, sizeof(T) * (size_t)(newCap > 0 ? newCap : 1));
+    if (e != hipSuccess) return e;
+    if (p && oldCount > 0) {
+        e = hipMemcpyAsync(q, p, sizeof(T) * (size_t)oldCount, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+    }
+    if (p) (void)hipFree(p);
+    p = q;
+    return hipSuccess;
+}
+
+template <typename T>
+void freeArray(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int growBodies(b2g_context* ctx, int need) {
+    if (need <= ctx->capBodies) return B2G_OK;
+    const int cap = std::max(need, ctx->capBodies * 2 + 64);
+    const int n = ctx->nBodies;
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, growArray(ctx->btype, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->bflags, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->xf, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->pos, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->pos0, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->vel, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->force, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->mass, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->lcenter, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->gsleep, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->colourMask, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->claim, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->islandOf, n, cap, s));
+    // claims start at zero (tag 0 never wins)
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(cap - n)), dim3(256), 0, s, ctx->claim + n, cap - n,
+                       0ull);
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(cap - n)), dim3(256), 0, s, ctx->colourMask + n,
+                       cap - n, 0ull);
+    LAUNCHCHK(ctx);
+    ctx->capBodies = cap;
+    return B2G_OK;
+}
+
+int growContacts(b2g_context* ctx, int need);
+
+// Scan/sort scratch for the largest array the step scans or sorts.
+int sizeScratch(b2g_context* ctx) {
+    const int sn = std::max(std::max(ctx->capContacts, ctx->nBuckets + 1), 1024);
+    const size_t need = prims_scratch_bytes(sn);
+    if (need > ctx->scratchBytes) {
+        freeArray(ctx->scratch);
+        HIPCHK(ctx, hipMalloc(&ctx->scratch, need));
+        ctx->scratchBytes = need;
+    }
+    return B2G_OK;
+}
+
+int growFixtures(b2g_context* ctx, int need) {
+    if (need <= ctx->capFixtures) return B2G_OK;
+    const int cap = std::max(need, ctx->capFixtures * 2 + 64);
+    const int n = ctx->nFixtures;
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, growArray(ctx->fbody, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->shape, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->fmat, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->ffilter, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->fgroup, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->fat, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->moved, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->range, 0, cap, s));
+    ctx->capFixtures = cap;
+    // grid sized for the fixture capacity
+    int nb = 4096;
+    while (nb < 4 * cap) nb <<= 1;
+    freeArray(ctx->gCounts); freeArray(ctx->gStarts); freeArray(ctx->gCursor); freeArray(ctx->gEntries);
+    freeArray(ctx->gBig); freeArray(ctx->gCell);
+    ctx->nBuckets = nb;
+    ctx->capEntries = kMaxCellsPerProxy * cap;
+    HIPCHK(ctx, hipMalloc((void**)&ctx->gCounts, sizeof(int) * (nb + 1)));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->gStarts, sizeof(int) * (nb + 1)));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->gCursor, sizeof(int) * (nb + 1)));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->gEntries, sizeof(int) * ctx->capEntries));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->gCell, sizeof(unsigned long long) * ctx->capEntries));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->gBig, sizeof(int) * cap));
+    const int rc = growContacts(ctx, 8 * cap + 4096);
+    if (rc != B2G_OK) return rc;
+    return sizeScratch(ctx);
+}
+
+int growContacts(b2g_context* ctx, int need) {
+    if (need <= ctx->capContacts) return B2G_OK;
+    const int cap = std::max(need, ctx->capContacts * 2);
+    const int n = ctx->nContacts;
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, growArray(ctx->cur.cfA, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->cur.cfB, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->cur.cflags, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->cur.cmat, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->cur.cman, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->cur.ccolour, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->alt.cfA, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->alt.cfB, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->alt.cflags, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->alt.cmat, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->alt.cman, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->alt.ccolour, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->keep, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->idx, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->newKeys, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->newKeysSorted, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->consKeys, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->consKeysSorted, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->consVals, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->consOrder, 0, cap, s));
+    ConView& c = ctx->cv;
+    HIPCHK(ctx, growArray(c.contact, 0, cap, s));
+    HIPCHK(ctx, growArray(c.bodies, 0, cap, s));
+    HIPCHK(ctx, growArray(c.mass, 0, cap, s));
+    HIPCHK(ctx, growArray(c.normal, 0, cap, s));
+    HIPCHK(ctx, growArray(c.r0, 0, cap, s));
+    HIPCHK(ctx, growArray(c.r1, 0, cap, s));
+    HIPCHK(ctx, growArray(c.pmass, 0, cap, s));
+    HIPCHK(ctx, growArray(c.bias, 0, cap, s));
+    HIPCHK(ctx, growArray(c.imp, 0, cap, s));
+    HIPCHK(ctx, growArray(c.K, 0, cap, s));
+    HIPCHK(ctx, growArray(c.NM, 0, cap, s));
+    HIPCHK(ctx, growArray(c.info, 0, cap, s));
+    HIPCHK(ctx, growArray(c.lp, 0, cap, s));
+    HIPCHK(ctx, growArray(c.lnp, 0, cap, s));
+    HIPCHK(ctx, growArray(c.lc, 0, cap, s));
+    HIPCHK(ctx, growArray(c.rad, 0, cap, s));
+    int hc = 1024;
+    while (hc < 2 * cap) hc <<= 1;
+    freeArray(ctx->hashTable);
+    HIPCHK(ctx, hipMalloc((void**)&ctx->hashTable, sizeof(unsigned long long) * hc));
+    ctx->hashCap = hc;
+    ctx->hashMask = (uint32_t)(hc - 1);
+    ctx->capContacts = cap;
+    return sizeScratch(ctx);
+}
+
+// Cell size of the broad-phase grid: the largest proxy extent among
+// non-static fixtures plus the fat margins, so a typical fat box spans at
+// most 2x2 cells.
+void chooseCellSize(b2g_context* ctx, const b2g_fixture_def* defs, int n, const std::vector<int>& types) {
+    float ext = ctx->cellSizeSet ? ctx->cellSize - 2.0f * kAabbExtension : 0.0f;
+    for (int i = 0; i < n; ++i) {
+        const b2g_shape& s = defs[i].shape;
+        const int b = defs[i].body;
+        if (b < 0 || b >= (int)types.size() || types[b] == BT_STATIC) continue;
+        float r = 0.0f;
+        if (s.type == B2G_CIRCLE) {
+            r = sqrtf(s.centroid.x * s.centroid.x + s.centroid.y * s.centroid.y) + s.radius;
+        } else {
+            const int nv = s.type == B2G_EDGE ? 2 : s.count;
+            for (int k = 0; k < nv; ++k)
+                r = std::max(r, sqrtf(s.verts[k].x * s.verts[k].x + s.verts[k].y * s.verts[k].y));
+            r += s.radius;
+        }
+        ext = std::max(ext, 2.0f * r);
+    }
+    if (ext > 0.0f) {
+        ctx->cellSize = ext + 2.0f * kAabbExtension;
+        ctx->cellSizeSet = true;
+    }
+}
+
+// Runs the broad-phase over the current fat boxes and appends the new
+// contacts. Requires the hash of live contacts to be current.
+int runBroadphase(b2g_context* ctx) {
+    hipStream_t s = ctx->stream;
+    const int nf = ctx->nFixtures;
+    if (nf == 0) return B2G_OK;
+    GridView g;
+    g.invCell = 1.0f / ctx->cellSize;
+    g.bucketMask = (uint32_t)(ctx->nBuckets - 1);
+    g.counts = ctx->gCounts; g.starts = ctx->gStarts; g.cursor = ctx->gCursor; g.entries = ctx->gEntries;
+    g.ecell = ctx->gCell;
+    g.big = ctx->gBig; g.range = ctx->range;
+    hipLaunchKernelGGL(k_fill<int>, dim3(blocks(ctx->nBuckets + 1)), dim3(256), 0, s, ctx->gCounts,
+                       ctx->nBuckets + 1, 0);
+    hipLaunchKernelGGL(k_fill<int>, dim3(blocks(ctx->nBuckets + 1)), dim3(256), 0, s, ctx->gCursor,
+                       ctx->nBuckets + 1, 0);
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nBig, 1, 0);
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nNewPairs, 1, 0);
+    hipLaunchKernelGGL(k_grid_count, dim3(blocks(nf)), dim3(256), 0, s, ctx->fat, nf, (const uint8_t*)nullptr, g,
+                       ctx->cnt);
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->gCounts, ctx->gStarts, ctx->nBuckets + 1, s));
+    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(nf)), dim3(256), 0, s, nf, g);
+    PairSink sink{ctx->newKeys, ctx->capContacts};
+    WorldView w = ctx->view();
+    hipLaunchKernelGGL(k_pairs_query, dim3(blocks(nf)), dim3(256), 0, s, ctx->fat, ctx->moved, nf, g, w, true,
+                       ctx->hashTable, ctx->hashMask, sink, ctx->cnt);
+    hipLaunchKernelGGL(k_fill<uint8_t>, dim3(blocks(nf)), dim3(256), 0, s, ctx->moved, nf, (uint8_t)0);
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->hcnt, ctx->cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    int nNew = ctx->hcnt->nNewPairs;
+    if (ctx->hcnt->overflow || ctx->nContacts + nNew > ctx->capContacts) {
+        return fail(ctx, B2G_ERR_CAPACITY, "contact capacity exceeded in the broad-phase");
+    }
+    if (nNew > 0) {
+        HIPCHK(ctx, prims_sort_keys_u64(ctx->scratch, ctx->scratchBytes, ctx->newKeys, ctx->newKeysSorted, nNew, s));
+        hipLaunchKernelGGL(k_append_contacts, dim3(blocks(nNew)), dim3(256), 0, s, w, ctx->newKeysSorted, nNew,
+                           ctx->nContacts);
+        LAUNCHCHK(ctx);
+        ctx->nContacts += nNew;
+    }
+    ctx->stats.newPairs += nNew;
+    return B2G_OK;
+}
+
+int rebuildHash(b2g_context* ctx) {
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(ctx->hashCap)), dim3(256), 0, s, ctx->hashTable,
+                       ctx->hashCap, ~0ull);
+    if (ctx->nContacts > 0)
+        hipLaunchKernelGGL(k_hash_insert, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), ctx->hashTable,
+                           ctx->hashMask);
+    LAUNCHCHK(ctx);
+    return B2G_OK;
+}
+
+int readCounters(b2g_context* ctx) {
+    HIPCHK(ctx, hipMemcpyAsync(ctx->hcnt, ctx->cnt, sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
+// ---------------------------------------------------------------- profiling
+enum Phase {
+    PH_COLLIDE, PH_COMPACT, PH_COLOUR, PH_ORDER, PH_INTEGRATE_V, PH_INIT, PH_WARM, PH_SOLVE_V, PH_STORE,
+    PH_INTEGRATE_X, PH_SOLVE_X, PH_SYNC, PH_BROADPHASE, PH_COUNT
+};
+const char* kPhaseNames[PH_COUNT] = {"collide", "compact", "colour", "order", "integrate_velocities",
+                                     "init_constraints", "warm_start", "solve_velocity", "store_impulses",
+                                     "integrate_positions", "solve_position", "sync", "broadphase"};
+
+int profStart(b2g_context* ctx) {
+    if (!ctx->profiling) return -1;
+    while ((int)ctx->evPool.size() < ctx->evUsed + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        ctx->evPool.push_back(e);
+    }
+    const int i = ctx->evUsed;
+    ctx->evUsed += 2;
+    (void)hipEventRecord(ctx->evPool[i], ctx->stream);
+    return i;
+}
+
+void profEnd(b2g_context* ctx, int phase, int i) {
+    if (i < 0) return;
+    (void)hipEventRecord(ctx->evPool[i + 1], ctx->stream);
+    ctx->evMarks.push_back(std::make_pair(phase, i));
+}
+
+void profCollect(b2g_context* ctx) {
+    if (!ctx->profiling || ctx->evMarks.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& m : ctx->evMarks) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ctx->evPool[m.second], ctx->evPool[m.second + 1]) == hipSuccess) {
+            ctx->profMs[m.first] += ms;
+            ctx->profLaunches[m.first] += 1;
+        }
+    }
+    ctx->evMarks.clear();
+    ctx->evUsed = 0;
+}
+
+int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
+    hipStream_t s = ctx->stream;
+    const float dt = p->dt;
+    const float inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
+    const float dtRatio = ctx->inv_dt0 * dt;
+    int rc;
+    memset(&ctx->stats, 0, sizeof(ctx->stats));
+    // reset per-step counters
+    Counters zero;
+    memset(&zero, 0, sizeof(zero));
+    for (int i = 0; i < kMaxPosIters; ++i) {
+        float z = 0.0f;
+        int b;
+        memcpy(&b, &z, 4);
+        zero.minSep[i] = b;  // encodeOrdered(0.0f) == 0
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->cnt, &zero, sizeof(Counters), hipMemcpyHostToDevice, s));
+
+    if (ctx->newFixture) {
+        if ((rc = rebuildHash(ctx)) != B2G_OK) return rc;
+        if ((rc = runBroadphase(ctx)) != B2G_OK) return rc;
+        ctx->newFixture = false;
+    }
+    // Collide: narrowphase, destruction of non-overlapping contacts.
+    if (ctx->nContacts > 0) {
+        int pe = profStart(ctx);
+        hipLaunchKernelGGL(k_collide, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), ctx->cnt);
+        profEnd(ctx, PH_COLLIDE, pe);
+        pe = profStart(ctx);
+        hipLaunchKernelGGL(k_mark, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), ctx->keep, ctx->cnt);
+        LAUNCHCHK(ctx);
+        HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->keep, ctx->idx, ctx->nContacts, s));
+        hipLaunchKernelGGL(k_compact, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->cur, ctx->alt, ctx->keep,
+                           ctx->idx, ctx->nContacts, ctx->cnt);
+        profEnd(ctx, PH_COMPACT, pe);
+        LAUNCHCHK(ctx);
+        std::swap(ctx->cur, ctx->alt);
+        if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+        ctx->nContacts = ctx->hcnt->nContacts;
+    } else {
+        if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+    }
+    ctx->stats.beginEvents = ctx->hcnt->beginEvents;
+    ctx->stats.endEvents = ctx->hcnt->endEvents;
+    if (dt > 0.0f) {
+        WorldView w = ctx->view();
+        // Colour new constraints.
+        int remaining = ctx->nContacts > 0 ? ctx->hcnt->remaining : 0;
+        int rounds = 0;
+        while (remaining > 0) {
+            if (ctx->claimTag >= 0xFFFF) {
+                hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(ctx->nBodies)), dim3(256), 0, s,
+                                   ctx->claim, ctx->nBodies, 0ull);
+                ctx->claimTag = 0;
+            }
+            const uint32_t tag = ++ctx->claimTag;
+            const int pe = profStart(ctx);
+            hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->remaining, 1, 0);
+            hipLaunchKernelGGL(k_colour_claim, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->claim, tag);
+            hipLaunchKernelGGL(k_colour_commit, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->claim, tag,
+                               ctx->cnt);
+            profEnd(ctx, PH_COLOUR, pe);
+            LAUNCHCHK(ctx);
+            if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+            remaining = ctx->hcnt->remaining;
+            if (++rounds > 100000) return fail(ctx, B2G_ERR_STATE, "colouring did not converge");
+        }
+        ctx->stats.colouringRounds = rounds;
+        // Constraint order by colour.
+        int nCons = 0;
+        int colourStart[kColourSlots + 1];
+        memset(colourStart, 0, sizeof(colourStart));
+        if (ctx->nContacts > 0) {
+            const int pe = profStart(ctx);
+            hipLaunchKernelGGL(k_cons_flags, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->keep);
+            HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->keep, ctx->idx, ctx->nContacts, s));
+            hipLaunchKernelGGL(k_cons_gather, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->keep, ctx->idx,
+                               ctx->consKeys, ctx->consVals, ctx->cnt);
+            LAUNCHCHK(ctx);
+            if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+            nCons = ctx->hcnt->nConstraints;
+            if (nCons > 0) {
+                HIPCHK(ctx, prims_sort_pairs_u32(ctx->scratch, ctx->scratchBytes, ctx->consKeys, ctx->consKeysSorted,
+                                                 ctx->consVals, ctx->consOrder, nCons, 7, s));
+                hipLaunchKernelGGL(k_colour_hist, dim3(blocks(nCons)), dim3(256), 0, s, ctx->consKeysSorted, nCons,
+                                   ctx->cnt);
+                profEnd(ctx, PH_ORDER, pe);
+                LAUNCHCHK(ctx);
+                if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+            }
+            for (int c = 0; c < kColourSlots; ++c)
+                colourStart[c + 1] = colourStart[c] + (nCons > 0 ? ctx->hcnt->colourCount[c] : 0);
+        }
+        int nColours = 0;
+        for (int c = 0; c < kColourSlots; ++c)
+            if (colourStart[c + 1] > colourStart[c]) nColours = c + 1;
+        ctx->stats.constraints = nCons;
+        ctx->stats.colours = nColours;
+
+        // Island solve.
+        int pe = profStart(ctx);
+        hipLaunchKernelGGL(k_integrate_velocities, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, ctx->gravity, dt);
+        profEnd(ctx, PH_INTEGRATE_V, pe);
+        if (nCons > 0) {
+            pe = profStart(ctx);
+            hipLaunchKernelGGL(k_init_constraints, dim3(blocks(nCons)), dim3(256), 0, s, w, ctx->cv, ctx->consOrder,
+                               nCons, dtRatio, ctx->warmStarting ? 1 : 0, ctx->cnt);
+            profEnd(ctx, PH_INIT, pe);
+        }
+        LAUNCHCHK(ctx);
+        auto perColour = [&](int phase, auto kernelLaunch) {
+            for (int c = 0; c < nColours; ++c) {
+                const int b0 = colourStart[c], b1 = colourStart[c + 1];
+                if (b1 <= b0) continue;
+                const int pc = profStart(ctx);
+                if (c == kOverflowColour) {
+                    for (int k = b0; k < b1; ++k) kernelLaunch(k, k + 1);
+                } else {
+                    kernelLaunch(b0, b1);
+                }
+                profEnd(ctx, phase, pc);
+            }
+        };
+        if (nCons > 0 && ctx->warmStarting) {
+            perColour(PH_WARM, [&](int b0, int b1) {
+                hipLaunchKernelGGL(k_warm_start, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, ctx->cv, b0, b1);
+            });
+        }
+        for (int it = 0; it < p->velocityIterations && nCons > 0; ++it) {
+            perColour(PH_SOLVE_V, [&](int b0, int b1) {
+                hipLaunchKernelGGL(k_solve_velocity, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, ctx->cv, b0, b1);
+            });
+        }
+        if (nCons > 0) {
+            pe = profStart(ctx);
+            hipLaunchKernelGGL(k_store_impulses, dim3(blocks(nCons)), dim3(256), 0, s, w, ctx->cv, nCons);
+            profEnd(ctx, PH_STORE, pe);
+        }
+        pe = profStart(ctx);
+        hipLaunchKernelGGL(k_integrate_positions, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, dt);
+        profEnd(ctx, PH_INTEGRATE_X, pe);
+        LAUNCHCHK(ctx);
+        const int posIters = std::min(p->positionIterations, kMaxPosIters);
+        for (int it = 0; it < posIters && nCons > 0; ++it) {
+            perColour(PH_SOLVE_X, [&](int b0, int b1) {
+                hipLaunchKernelGGL(k_solve_position, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, ctx->cv, b0, b1,
+                                   ctx->cnt, it);
+            });
+        }
+        pe = profStart(ctx);
+        hipLaunchKernelGGL(k_sync_transforms, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w);
+        hipLaunchKernelGGL(k_sync_fixtures, dim3(blocks(ctx->nFixtures)), dim3(256), 0, s, w);
+        profEnd(ctx, PH_SYNC, pe);
+        LAUNCHCHK(ctx);
+        // Broad-phase against the compacted contact set.
+        pe = profStart(ctx);
+        if ((rc = rebuildHash(ctx)) != B2G_OK) return rc;
+        if ((rc = runBroadphase(ctx)) != B2G_OK) return rc;
+        profEnd(ctx, PH_BROADPHASE, pe);
+        // Stats of the position iterations.
+        int ran = 0;
+        float lastMin = 0.0f;
+        for (int it = 0; it < posIters && nCons > 0; ++it) {
+            ++ran;
+            lastMin = decodeOrdered(ctx->hcnt->minSep[it]);
+            if (lastMin >= -3.0f * kLinearSlop) break;
+        }
+        ctx->stats.positionIterations = ran;
+        ctx->stats.points = ctx->hcnt->points;
+        ctx->stats.minSeparation = lastMin;
+        ctx->inv_dt0 = inv_dt;
+    }
+    if (ctx->autoClearForces && ctx->nBodies > 0)
+        hipLaunchKernelGGL(k_fill<float4>, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, ctx->force, ctx->nBodies,
+                           make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    LAUNCHCHK(ctx);
+    profCollect(ctx);
+    ctx->stats.bodies = ctx->nBodies;
+    ctx->stats.fixtures = ctx->nFixtures;
+    ctx->stats.contacts = ctx->nContacts;
+    ctx->stats.overflow = ctx->hcnt->overflow;
+    ctx->stats.cellSize = ctx->cellSize;
+    return B2G_OK;
+}
+
+void bodyToDevice(const b2g_body_state& b, int& type, int& flags, float4& xf, float4& pos, float4& pos0,
+                  float2& lc, float4& vel, float4& force, float4& mass, float2& gs) {
+    type = b.type;
+    flags = b.flags;
+    xf = make_float4(b.xf[0], b.xf[1], b.xf[2], b.xf[3]);
+    pos = make_float4(b.c[0], b.c[1], b.a, 0.0f);
+    pos0 = make_float4(b.c0[0], b.c0[1], b.a0, 0.0f);
+    lc = make_float2(b.lc[0], b.lc[1]);
+    vel = make_float4(b.v[0], b.v[1], b.w, 0.0f);
+    force = make_float4(b.force[0], b.force[1], b.torque, 0.0f);
+    mass = make_float4(b.invMass, b.invI, b.linearDamping, b.angularDamping);
+    gs = make_float2(b.gravityScale, b.sleepTime);
+}
+
+int uploadBodies(b2g_context* ctx, int first, int count, const b2g_body_state* bodies) {
+    std::vector<int> type(count), flags(count);
+    std::vector<float4> xf(count), pos(count), pos0(count), vel(count), force(count), mass(count);
+    std::vector<float2> lc(count), gs(count);
+    for (int i = 0; i < count; ++i)
+        bodyToDevice(bodies[i], type[i], flags[i], xf[i], pos[i], pos0[i], lc[i], vel[i], force[i], mass[i], gs[i]);
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->btype + first, type.data(), sizeof(int) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->bflags + first, flags.data(), sizeof(int) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->xf + first, xf.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->pos + first, pos.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->pos0 + first, pos0.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->vel + first, vel.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->force + first, force.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->mass + first, mass.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->lcenter + first, lc.data(), sizeof(float2) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->gsleep + first, gs.data(), sizeof(float2) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+}  // namespace
+
+// ====================================================================
+// C ABI
+// ====================================================================
+extern "C" {
+
+int b2g_create(b2g_context** out, int device) {
+    if (!out) return B2G_ERR_ARG;
+    *out = nullptr;
+    b2g_context* ctx = new b2g_context;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete ctx;
+        return B2G_ERR_HIP;
+    }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return B2G_ERR_HIP;
+    }
+    if (hipMalloc((void**)&ctx->cnt, sizeof(Counters)) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->hcnt, sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&ctx->penAcc, 64) != hipSuccess) {
+        delete ctx;
+        return B2G_ERR_HIP;
+    }
+    memset(ctx->hcnt, 0, sizeof(Counters));
+    int rc = growBodies(ctx, 64);
+    if (rc == B2G_OK) rc = growFixtures(ctx, 64);
+    if (rc != B2G_OK) {
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return B2G_OK;
+}
+
+int b2g_destroy(b2g_context* ctx) {
+    if (!ctx) return B2G_ERR_ARG;
+    (void)hipStreamSynchronize(ctx->stream);
+    freeArray(ctx->btype); freeArray(ctx->bflags); freeArray(ctx->xf); freeArray(ctx->pos); freeArray(ctx->pos0);
+    freeArray(ctx->vel); freeArray(ctx->force); freeArray(ctx->mass); freeArray(ctx->lcenter);
+    freeArray(ctx->gsleep); freeArray(ctx->colourMask); freeArray(ctx->claim); freeArray(ctx->islandOf);
+    freeArray(ctx->fbody); freeArray(ctx->shape); freeArray(ctx->fmat); freeArray(ctx->ffilter);
+    freeArray(ctx->fgroup); freeArray(ctx->fat); freeArray(ctx->moved); freeArray(ctx->range);
+    for (ContactArrays* a : {&ctx->cur, &ctx->alt}) {
+        freeArray(a->cfA); freeArray(a->cfB); freeArray(a->cflags); freeArray(a->cmat); freeArray(a->cman);
+        freeArray(a->ccolour);
+    }
+    freeArray(ctx->keep); freeArray(ctx->idx); freeArray(ctx->hashTable); freeArray(ctx->newKeys);
+    freeArray(ctx->newKeysSorted); freeArray(ctx->consKeys); freeArray(ctx->consKeysSorted);
+    freeArray(ctx->consVals); freeArray(ctx->consOrder);
+    ConView& c = ctx->cv;
+    freeArray(c.contact); freeArray(c.bodies); freeArray(c.mass); freeArray(c.normal); freeArray(c.r0);
+    freeArray(c.r1); freeArray(c.pmass); freeArray(c.bias); freeArray(c.imp); freeArray(c.K); freeArray(c.NM);
+    freeArray(c.info); freeArray(c.lp); freeArray(c.lnp); freeArray(c.lc); freeArray(c.rad);
+    freeArray(ctx->gCounts); freeArray(ctx->gStarts); freeArray(ctx->gCursor); freeArray(ctx->gEntries);
+    freeArray(ctx->gBig); freeArray(ctx->gCell);
+    for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->cnt) (void)hipFree(ctx->cnt);
+    if (ctx->penAcc) (void)hipFree(ctx->penAcc);
+    if (ctx->hcnt) (void)hipHostFree(ctx->hcnt);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return B2G_OK;
+}
+
+const char* b2g_last_error(const b2g_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int b2g_reserve(b2g_context* ctx, int maxBodies, int maxFixtures, int maxContacts) {
+    if (!ctx || maxBodies < 0 || maxFixtures < 0 || maxContacts < 0) return B2G_ERR_ARG;
+    int rc = growBodies(ctx, maxBodies);
+    if (rc == B2G_OK) rc = growFixtures(ctx, maxFixtures);
+    if (rc == B2G_OK) rc = growContacts(ctx, maxContacts);
+    return rc;
+}
+
+int b2g_set_world(b2g_context* ctx, const b2g_world_def* def) {
+    if (!ctx || !def) return B2G_ERR_ARG;
+    if (def->continuous || def->subStepping)
+        return fail(ctx, B2G_ERR_UNSUPPORTED, "continuous collision (TOI) and sub-stepping are not on the GPU path");
+    ctx->gravity = mk(def->gravity[0], def->gravity[1]);
+    ctx->allowSleep = def->allowSleep != 0;
+    ctx->warmStarting = def->warmStarting != 0;
+    ctx->continuous = false;
+    ctx->subStepping = false;
+    ctx->autoClearForces = def->autoClearForces != 0;
+    return B2G_OK;
+}
+
+int b2g_add_bodies(b2g_context* ctx, const b2g_body_state* bodies, int count, int* first) {
+    if (!ctx || (count > 0 && !bodies) || count < 0) return B2G_ERR_ARG;
+    int rc = growBodies(ctx, ctx->nBodies + count);
+    if (rc != B2G_OK) return rc;
+    const int f = ctx->nBodies;
+    ctx->nBodies += count;
+    if (count > 0) {
+        rc = uploadBodies(ctx, f, count, bodies);
+        if (rc != B2G_OK) return rc;
+        hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(count)), dim3(256), 0, ctx->stream,
+                           ctx->colourMask + f, count, 0ull);
+        LAUNCHCHK(ctx);
+    }
+    if (first) *first = f;
+    return B2G_OK;
+}
+
+int b2g_set_bodies(b2g_context* ctx, int first, int count, const b2g_body_state* bodies) {
+    if (!ctx || first < 0 || count < 0 || first + count > ctx->nBodies || (count > 0 && !bodies)) return B2G_ERR_ARG;
+    if (count == 0) return B2G_OK;
+    return uploadBodies(ctx, first, count, bodies);
+}
+
+int b2g_get_bodies(b2g_context* ctx, int first, int count, b2g_body_state* out) {
+    if (!ctx || first < 0 || count < 0 || first + count > ctx->nBodies || (count > 0 && !out)) return B2G_ERR_ARG;
+    if (count == 0) return B2G_OK;
+    std::vector<int> type(count), flags(count);
+    std::vector<float4> xf(count), pos(count), pos0(count), vel(count), force(count), mass(count);
+    std::vector<float2> lc(count), gs(count);
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(type.data(), ctx->btype + first, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(flags.data(), ctx->bflags + first, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(xf.data(), ctx->xf + first, sizeof(float4) * count, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(pos.data(), ctx->pos + first, sizeof(float4) * count, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(pos0.data(), ctx->pos0 + first, sizeof(float4) * count, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(vel.data(), ctx->vel + first, sizeof(float4) * count, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(force.data(), ctx->force + first, sizeof(float4) * count, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(mass.data(), ctx->mass + first, sizeof(float4) * count, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(lc.data(), ctx->lcenter + first, sizeof(float2) * count, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(gs.data(), ctx->gsleep + first, sizeof(float2) * count, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    for (int i = 0; i < count; ++i) {
+        b2g_body_state& b = out[i];
+        b.type = type[i];
+        b.flags = flags[i];
+        b.xf[0] = xf[i].x; b.xf[1] = xf[i].y; b.xf[2] = xf[i].z; b.xf[3] = xf[i].w;
+        b.lc[0] = lc[i].x; b.lc[1] = lc[i].y;
+        b.c0[0] = pos0[i].x; b.c0[1] = pos0[i].y; b.a0 = pos0[i].z;
+        b.c[0] = pos[i].x; b.c[1] = pos[i].y; b.a = pos[i].z;
+        b.v[0] = vel[i].x; b.v[1] = vel[i].y; b.w = vel[i].z;
+        b.force[0] = force[i].x; b.force[1] = force[i].y; b.torque = force[i].z;
+        b.invMass = mass[i].x; b.invI = mass[i].y; b.linearDamping = mass[i].z; b.angularDamping = mass[i].w;
+        b.gravityScale = gs[i].x; b.sleepTime = gs[i].y;
+    }
+    return B2G_OK;
+}
+
+int b2g_add_fixtures(b2g_context* ctx, const b2g_fixture_def* defs, int count, int* first) {
+    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
+    for (int i = 0; i < count; ++i) {
+        if (defs[i].body < 0 || defs[i].body >= ctx->nBodies) return fail(ctx, B2G_ERR_ARG, "fixture body index");
+        if (defs[i].sensor) return fail(ctx, B2G_ERR_UNSUPPORTED, "sensor fixtures are not on the GPU path");
+        const int t = defs[i].shape.type;
+        if (t != B2G_CIRCLE && t != B2G_EDGE && t != B2G_POLYGON)
+            return fail(ctx, B2G_ERR_UNSUPPORTED, "shape type");
+        if (t == B2G_POLYGON && (defs[i].shape.count < 3 || defs[i].shape.count > 8))
+            return fail(ctx, B2G_ERR_ARG, "polygon vertex count");
+    }
+    int rc = growFixtures(ctx, ctx->nFixtures + count);
+    if (rc != B2G_OK) return rc;
+    const int f = ctx->nFixtures;
+    if (count > 0) {
+        // body types and transforms for the initial AABBs
+        std::vector<b2g_body_state> bodies(ctx->nBodies);
+        rc = b2g_get_bodies(ctx, 0, ctx->nBodies, bodies.data());
+        if (rc != B2G_OK) return rc;
+        std::vector<int> types(ctx->nBodies);
+        for (int i = 0; i < ctx->nBodies; ++i) types[i] = bodies[i].type;
+        std::vector<int> fbody(count), fgroup(count);
+        std::vector<uint32_t> ffilter(count);
+        std::vector<float4> fmat(count), fat(count);
+        std::vector<uint8_t> moved(count, 0);
+        for (int i = 0; i < count; ++i) {
+            const b2g_fixture_def& d = defs[i];
+            fbody[i] = d.body;
+            fgroup[i] = d.groupIndex;
+            ffilter[i] = (d.categoryBits & 0xFFFF) | ((d.maskBits & 0xFFFF) << 16);
+            fmat[i] = make_float4(d.friction, d.restitution, d.density, d.sensor ? 1.0f : 0.0f);
+            const b2g_body_state& b = bodies[d.body];
+            const bool active = (b.flags & BF_ACTIVE) != 0;
+            // b2Fixture::CreateProxies: AABB at the body transform, fattened
+            // by the AABB extension (REF/Collision/b2DynamicTree.cpp:112-126).
+            Xf xf;
+            xf.p = mk(b.xf[0], b.xf[1]);
+            xf.q.s = b.xf[2];
+            xf.q.c = b.xf[3];
+            const b2g_shape& s = d.shape;
+            float lx, ly, ux, uy;
+            if (s.type == B2G_CIRCLE) {
+                const V2 p = xf.p + mul(xf.q, ld(s.centroid));
+                lx = p.x - s.radius; ly = p.y - s.radius; ux = p.x + s.radius; uy = p.y + s.radius;
+            } else {
+                const int nv = s.type == B2G_EDGE ? 2 : s.count;
+                V2 lo = mul(xf, ld(s.verts[0])), hi = lo;
+                for (int k = 1; k < nv; ++k) {
+                    const V2 v = mul(xf, ld(s.verts[k]));
+                    lo = vmin(lo, v);
+                    hi = vmax(hi, v);
+                }
+                const V2 r = mk(s.radius, s.radius);
+                const V2 l = lo - r, u = hi + r;
+                lx = l.x; ly = l.y; ux = u.x; uy = u.y;
+            }
+            fat[i] = make_float4(lx - kAabbExtension, ly - kAabbExtension, ux + kAabbExtension, uy + kAabbExtension);
+            moved[i] = active ? 1 : 0;
+        }
+        std::vector<b2g_shape> shapes(count);
+        for (int i = 0; i < count; ++i) shapes[i] = defs[i].shape;
+        hipStream_t s = ctx->stream;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->fbody + f, fbody.data(), sizeof(int) * count, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->shape + f, shapes.data(), sizeof(b2g_shape) * count, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->fmat + f, fmat.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->ffilter + f, ffilter.data(), sizeof(uint32_t) * count, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->fgroup + f, fgroup.data(), sizeof(int) * count, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->fat + f, fat.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->moved + f, moved.data(), count, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        chooseCellSize(ctx, defs, count, types);
+        ctx->nFixtures += count;
+        ctx->newFixture = true;
+        rc = growContacts(ctx, 8 * ctx->nFixtures + 4096);
+        if (rc != B2G_OK) return rc;
+    }
+    if (first) *first = f;
+    return B2G_OK;
+}
+
+int b2g_remove_fixtures(b2g_context* ctx, const int32_t* ids, int count) {
+    if (!ctx || count < 0 || (count > 0 && !ids)) return B2G_ERR_ARG;
+    if (count == 0) return B2G_OK;
+    for (int i = 0; i < count; ++i)
+        if (ids[i] < 0 || ids[i] >= ctx->nFixtures) return fail(ctx, B2G_ERR_ARG, "fixture index");
+    int* d = nullptr;
+    HIPCHK(ctx, hipMalloc((void**)&d, sizeof(int) * count));
+    hipError_t e = hipMemcpyAsync(d, ids, sizeof(int) * count, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_remove_fixtures, dim3(blocks(count)), dim3(256), 0, ctx->stream, ctx->view(), d, count);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(ctx, B2G_ERR_HIP, hipGetErrorString(e));
+    return B2G_OK;
+}
+
+int b2g_update_fixtures(b2g_context* ctx, const int32_t* ids, const b2g_fixture_def* defs, int count) {
+    if (!ctx || count < 0 || (count > 0 && (!ids || !defs))) return B2G_ERR_ARG;
+    if (count == 0) return B2G_OK;
+    std::vector<uint8_t> mark(ctx->nFixtures, 0);
+    hipStream_t s = ctx->stream;
+    for (int i = 0; i < count; ++i) {
+        const int f = ids[i];
+        if (f < 0 || f >= ctx->nFixtures) return fail(ctx, B2G_ERR_ARG, "fixture index");
+        const b2g_fixture_def& d = defs[i];
+        const float4 mat = make_float4(d.friction, d.restitution, d.density, d.sensor ? 1.0f : 0.0f);
+        const uint32_t filt = (d.categoryBits & 0xFFFF) | ((d.maskBits & 0xFFFF) << 16);
+        const int group = d.groupIndex;
+        const uint8_t one = 1;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->fmat + f, &mat, sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->ffilter + f, &filt, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->fgroup + f, &group, sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->moved + f, &one, 1, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        mark[f] = 1;
+    }
+    if (ctx->nContacts > 0) {
+        uint8_t* dm = nullptr;
+        HIPCHK(ctx, hipMalloc((void**)&dm, mark.size()));
+        hipError_t e = hipMemcpyAsync(dm, mark.data(), mark.size(), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_flag_filter, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), dm);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(dm);
+        if (e != hipSuccess) return fail(ctx, B2G_ERR_HIP, hipGetErrorString(e));
+    }
+    ctx->newFixture = true;  // touched proxies are queried at the next step
+    return B2G_OK;
+}
+
+int b2g_step(b2g_context* ctx, const b2g_step_params* p) {
+    if (!ctx || !p) return B2G_ERR_ARG;
+    if (p->velocityIterations < 0 || p->positionIterations < 0) return fail(ctx, B2G_ERR_ARG, "iterations");
+    if (p->mode != B2G_MODE_COLOURED) return fail(ctx, B2G_ERR_UNSUPPORTED, "solver mode");
+    if (p->learnedWarmStart) return fail(ctx, B2G_ERR_UNSUPPORTED, "learned warm start without a model");
+    (void)hipSetDevice(ctx->device);
+    return stepColoured(ctx, p);
+}
+
+int b2g_sync(b2g_context* ctx) {
+    if (!ctx) return B2G_ERR_ARG;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
+int b2g_set_profiling(b2g_context* ctx, int on) {
+    if (!ctx) return B2G_ERR_ARG;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->profiling = on != 0;
+    ctx->evMarks.clear();
+    ctx->evUsed = 0;
+    for (int i = 0; i < 32; ++i) {
+        ctx->profMs[i] = 0.0;
+        ctx->profLaunches[i] = 0;
+    }
+    return B2G_OK;
+}
+
+int b2g_get_profile(b2g_context* ctx, b2g_phase_profile* out, int cap, int* count) {
+    if (!ctx || cap < 0 || (cap > 0 && !out)) return B2G_ERR_ARG;
+    if (count) *count = PH_COUNT;
+    for (int i = 0; i < PH_COUNT && i < cap; ++i) {
+        memset(&out[i], 0, sizeof(out[i]));
+        strncpy(out[i].name, kPhaseNames[i], sizeof(out[i].name) - 1);
+        out[i].launches = ctx->profLaunches[i];
+        out[i].ms = ctx->profMs[i];
+    }
+    return B2G_OK;
+}
+
+int b2g_get_stats(b2g_context* ctx, b2g_step_stats* out) {
+    if (!ctx || !out) return B2G_ERR_ARG;
+    *out = ctx->stats;
+    return B2G_OK;
+}
+
+int b2g_get_contacts(b2g_context* ctx, int32_t* meta, b2g_manifold* manifolds, int cap, int* count) {
+    if (!ctx || cap < 0) return B2G_ERR_ARG;
+    const int n = ctx->nContacts;
+    if (count) *count = n;
+    const int m = std::min(n, cap);
+    if (m == 0) return B2G_OK;
+    std::vector<int> fa(m), fb(m), fl(m);
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(fa.data(), ctx->cur.cfA, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(fb.data(), ctx->cur.cfB, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(fl.data(), ctx->cur.cflags, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+    if (manifolds)
+        HIPCHK(ctx, hipMemcpyAsync(manifolds, ctx->cur.cman, sizeof(b2g_manifold) * m, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    if (meta) {
+        for (int i = 0; i < m; ++i) {
+            meta[4 * i + 0] = fa[i];
+            meta[4 * i + 1] = fb[i];
+            meta[4 * i + 2] = ((fl[i] & CF_TOUCHING) ? 1 : 0) | ((fl[i] & CF_ENABLED) ? 2 : 0);
+            meta[4 * i + 3] = 0;
+        }
+    }
+    return B2G_OK;
+}
+
+int b2g_get_fat_aabbs(b2g_context* ctx, b2g_aabb* out, int cap) {
+    if (!ctx || !out) return B2G_ERR_ARG;
+    const int m = std::min(cap, ctx->nFixtures);
+    if (m <= 0) return B2G_OK;
+    HIPCHK(ctx, hipMemcpyAsync(out, ctx->fat, sizeof(float4) * m, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
+int b2g_penetration(b2g_context* ctx, double* out) {
+    if (!ctx || !out) return B2G_ERR_ARG;
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemsetAsync(ctx->penAcc, 0, 64, s));
+    double* acc = ctx->penAcc;
+    int* np = (int*)(ctx->penAcc + 1);
+    unsigned long long* mx = (unsigned long long*)(ctx->penAcc + 2);
+    if (ctx->nContacts > 0)
+        hipLaunchKernelGGL(k_penetration, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), acc, np, mx);
+    LAUNCHCHK(ctx);
+    double h[3];
+    HIPCHK(ctx, hipMemcpyAsync(h, ctx->penAcc, 24, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    int n;
+    memcpy(&n, &h[1], 4);
+    out[0] = n ? h[0] / n : 0.0;
+    memcpy(&out[1], &h[2], 8);
+    out[2] = (double)n;
+    return B2G_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batches
+namespace {
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    hipError_t alloc(size_t n) { return hipMalloc((void**)&p, sizeof(T) * (n ? n : 1)); }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+}  // namespace
+
+extern "C" {
+
+int b2g_collide_batch(b2g_context* ctx, const b2g_shape* a, const b2g_transform* xfA, const b2g_shape* b,
+                      const b2g_transform* xfB, int n, b2g_manifold* out) {
+    if (!ctx || n < 0 || (n > 0 && (!a || !b || !xfA || !xfB || !out))) return B2G_ERR_ARG;
+    if (n == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<b2g_shape> dA, dB;
+    DevBuf<b2g_transform> dxa, dxb;
+    DevBuf<b2g_manifold> dm;
+    HIPCHK(ctx, dA.alloc(n)); HIPCHK(ctx, dB.alloc(n)); HIPCHK(ctx, dxa.alloc(n)); HIPCHK(ctx, dxb.alloc(n));
+    HIPCHK(ctx, dm.alloc(n));
+    HIPCHK(ctx, hipMemcpyAsync(dA.p, a, sizeof(b2g_shape) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dB.p, b, sizeof(b2g_shape) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dxa.p, xfA, sizeof(b2g_transform) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dxb.p, xfB, sizeof(b2g_transform) * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_collide_batch, dim3(blocks(n, 64)), dim3(64), 0, s, dA.p, dxa.p, dB.p, dxb.p, n, dm.p);
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, hipMemcpyAsync(out, dm.p, sizeof(b2g_manifold) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+int b2g_world_manifold_batch(b2g_context* ctx, const b2g_manifold* m, const b2g_transform* xfA, const float* radiusA,
+                             const b2g_transform* xfB, const float* radiusB, int n, float* out) {
+    if (!ctx || n < 0 || (n > 0 && (!m || !xfA || !xfB || !radiusA || !radiusB || !out))) return B2G_ERR_ARG;
+    if (n == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<b2g_manifold> dm;
+    DevBuf<b2g_transform> dxa, dxb;
+    DevBuf<float> dra, drb, dout;
+    HIPCHK(ctx, dm.alloc(n)); HIPCHK(ctx, dxa.alloc(n)); HIPCHK(ctx, dxb.alloc(n)); HIPCHK(ctx, dra.alloc(n));
+    HIPCHK(ctx, drb.alloc(n)); HIPCHK(ctx, dout.alloc(8 * (size_t)n));
+    HIPCHK(ctx, hipMemcpyAsync(dm.p, m, sizeof(b2g_manifold) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dxa.p, xfA, sizeof(b2g_transform) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dxb.p, xfB, sizeof(b2g_transform) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dra.p, radiusA, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(drb.p, radiusB, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_world_manifold_batch, dim3(blocks(n)), dim3(256), 0, s, dm.p, dxa.p, dra.p, dxb.p, drb.p, n,
+                       dout.p);
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, hipMemcpyAsync(out, dout.p, sizeof(float) * 8 * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+int b2g_pairs_batch(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved, int n, int32_t* pairs, int cap,
+                    int* count) {
+    if (!ctx || n < 0 || cap < 0 || (n > 0 && (!fat || !moved))) return B2G_ERR_ARG;
+    if (count) *count = 0;
+    if (n == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    // Cell size from the median proxy extent.
+    std::vector<float> ext(n);
+    for (int i = 0; i < n; ++i) ext[i] = std::max(fat[i].ux - fat[i].lx, fat[i].uy - fat[i].ly);
+    std::nth_element(ext.begin(), ext.begin() + n / 2, ext.end());
+    const float cell = std::max(ext[n / 2], 1e-3f);
+    int nb = 1024;
+    while (nb < 2 * n) nb <<= 1;
+    const int capPairs = std::max(cap, 1 << 16);
+    DevBuf<float4> dfat;
+    DevBuf<uint8_t> dmoved;
+    DevBuf<int> counts, starts, cursor, entries, big;
+    DevBuf<unsigned long long> ecell;
+    DevBuf<int4> range;
+    DevBuf<unsigned long long> keys, sorted;
+    DevBuf<char> scratch;
+    DevBuf<Counters> dc;
+    const size_t sb = prims_scratch_bytes(std::max(std::max(nb + 1, capPairs), n));
+    HIPCHK(ctx, dfat.alloc(n)); HIPCHK(ctx, dmoved.alloc(n)); HIPCHK(ctx, counts.alloc(nb + 1));
+    HIPCHK(ctx, starts.alloc(nb + 1)); HIPCHK(ctx, cursor.alloc(nb + 1));
+    HIPCHK(ctx, entries.alloc((size_t)kMaxCellsPerProxy * n)); HIPCHK(ctx, big.alloc(n)); HIPCHK(ctx, range.alloc(n));
+    HIPCHK(ctx, ecell.alloc((size_t)kMaxCellsPerProxy * n));
+    HIPCHK(ctx, keys.alloc(capPairs)); HIPCHK(ctx, sorted.alloc(capPairs)); HIPCHK(ctx, scratch.alloc(sb));
+    HIPCHK(ctx, dc.alloc(1));
+    HIPCHK(ctx, hipMemcpyAsync(dfat.p, fat, sizeof(float4) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dmoved.p, moved, n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemsetAsync(dc.p, 0, sizeof(Counters), s));
+    HIPCHK(ctx, hipMemsetAsync(counts.p, 0, sizeof(int) * (nb + 1), s));
+    HIPCHK(ctx, hipMemsetAsync(cursor.p, 0, sizeof(int) * (nb + 1), s));
+    GridView g;
+    g.invCell = 1.0f / cell;
+    g.bucketMask = (uint32_t)(nb - 1);
+    g.counts = counts.p; g.starts = starts.p; g.cursor = cursor.p; g.entries = entries.p; g.big = big.p;
+    g.ecell = ecell.p;
+    g.range = range.p;
+    hipLaunchKernelGGL(k_grid_count, dim3(blocks(n)), dim3(256), 0, s, dfat.p, n, (const uint8_t*)nullptr, g, dc.p);
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, prims_exclusive_sum(scratch.p, sb, counts.p, starts.p, nb + 1, s));
+    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(n)), dim3(256), 0, s, n, g);
+    WorldView w;
+    memset(&w, 0, sizeof(w));
+    PairSink sink{keys.p, capPairs};
+    hipLaunchKernelGGL(k_pairs_query, dim3(blocks(n)), dim3(256), 0, s, dfat.p, dmoved.p, n, g, w, false,
+                       (const unsigned long long*)nullptr, 0u, sink, dc.p);
+    LAUNCHCHK(ctx);
+    Counters hc;
+    HIPCHK(ctx, hipMemcpyAsync(&hc, dc.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    const int np = hc.nNewPairs;
+    if (count) *count = np;
+    if (hc.overflow || np > capPairs) return fail(ctx, B2G_ERR_CAPACITY, "pair buffer");
+    if (np > 0) {
+        HIPCHK(ctx, prims_sort_keys_u64(scratch.p, sb, keys.p, sorted.p, np, s));
+        std::vector<unsigned long long> hk(np);
+        HIPCHK(ctx, hipMemcpyAsync(hk.data(), sorted.p, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        for (int i = 0; i < np && i < cap; ++i) {
+            pairs[2 * i] = (int32_t)(hk[i] >> 32);
+            pairs[2 * i + 1] = (int32_t)(hk[i] & 0xFFFFFFFFu);
+        }
+    }
+    return B2G_OK;
+}
+
+int b2g_fat_rule_batch(b2g_context* ctx, const b2g_aabb* fat, const b2g_aabb* tight, const float* disp, int n,
+                       b2g_aabb* fatOut, uint8_t* moved) {
+    if (!ctx || n < 0 || (n > 0 && (!fat || !tight || !disp || !fatOut || !moved))) return B2G_ERR_ARG;
+    if (n == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<float4> df, dt, dout;
+    DevBuf<float2> dd;
+    DevBuf<uint8_t> dm;
+    HIPCHK(ctx, df.alloc(n)); HIPCHK(ctx, dt.alloc(n)); HIPCHK(ctx, dout.alloc(n)); HIPCHK(ctx, dd.alloc(n));
+    HIPCHK(ctx, dm.alloc(n));
+    HIPCHK(ctx, hipMemcpyAsync(df.p, fat, sizeof(float4) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dt.p, tight, sizeof(float4) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dd.p, disp, sizeof(float2) * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fat_rule_batch, dim3(blocks(n)), dim3(256), 0, s, df.p, dt.p, dd.p, n, dout.p, dm.p);
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, hipMemcpyAsync(fatOut, dout.p, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(moved, dm.p, n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+int b2g_sincos_batch(b2g_context* ctx, const float* angles, int n, float* so, float* co) {
+    if (!ctx || n < 0 || (n > 0 && (!angles || !so || !co))) return B2G_ERR_ARG;
+    if (n == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<float> da, ds, dc;
+    HIPCHK(ctx, da.alloc(n)); HIPCHK(ctx, ds.alloc(n)); HIPCHK(ctx, dc.alloc(n));
+    HIPCHK(ctx, hipMemcpyAsync(da.p, angles, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_sincos_batch, dim3(blocks(n)), dim3(256), 0, s, da.p, n, ds.p, dc.p);
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, hipMemcpyAsync(so, ds.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(co, dc.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+// C entry points over the box2d:: API for the Python tests and benchmark:
+// build a scene through b2World/b2Body/b2Fixture, step it, and read bodies,
+// contacts and penetration back. The entry points mirror the CPU checkers'
+// (orc_* / ref_*) with the prefix b2pp_, so tests compare all three alike.
+#include "box2d/Box2D.h"
+#include "b2g.h"
+#include "scenes/scenes.h"
+
+#include <vector>
+#include <string>
+
+using namespace box2d;
+
+namespace {
+
+struct Runner {
+    b2World world;
+    std::vector<b2Body*> bodies;
+    std::vector<b2Fixture*> fixtures;
+    explicit Runner(float gx, float gy) : world(b2Vec2(gx, gy)) {}
+};
+
+int loadScene(Runner* r, const b2scene::BodyDef* bodies, int nb, const b2scene::FixtureDef* fx, int nf) {
+    for (int i = 0; i < nb; ++i) {
+        const b2scene::BodyDef& d = bodies[i];
+        b2BodyDef bd;
+        bd.type = (b2BodyType)d.type;
+        bd.position = b2Vec2(d.px, d.py);
+        bd.angle = d.angle;
+        bd.linearVelocity = b2Vec2(d.vx, d.vy);
+        bd.angularVelocity = d.w;
+        bd.linearDamping = d.linearDamping;
+        bd.angularDamping = d.angularDamping;
+        bd.gravityScale = d.gravityScale;
+        bd.allowSleep = d.allowSleep != 0;
+        bd.awake = d.awake != 0;
+        bd.fixedRotation = d.fixedRotation != 0;
+        bd.bullet = d.bullet != 0;
+        b2Body* body = r->world.CreateBody(&bd);
+        r->bodies.push_back(body);
+        for (int k = 0; k < d.fixtureCount; ++k) {
+            const int fi = d.firstFixture + k;
+            if (fi < 0 || fi >= nf) return B2G_ERR_ARG;
+            const b2scene::FixtureDef& f = fx[fi];
+            b2PolygonShape poly;
+            b2CircleShape circle;
+            b2EdgeShape edge;
+            const b2Shape* shape = nullptr;
+            if (f.shape == b2scene::kPolygon) {
+                if (f.polyMode == b2scene::kPolyBox) poly.SetAsBox(f.pts[0], f.pts[1]);
+                else if (f.polyMode == b2scene::kPolyBoxRotated)
+                    poly.SetAsBox(f.pts[0], f.pts[1], b2Vec2(f.pts[2], f.pts[3]), f.pts[4]);
+                else {
+                    b2Vec2 p[MAX_POLYGON_VERTICES];
+                    for (int j = 0; j < f.n && j < MAX_POLYGON_VERTICES; ++j) p[j] = b2Vec2(f.pts[2 * j], f.pts[2 * j + 1]);
+                    poly.Set(p, f.n);
+                }
+                shape = &poly;
+            } else if (f.shape == b2scene::kCircle) {
+                circle.m_radius = f.radius;
+                circle.m_p = b2Vec2(f.cx, f.cy);
+                shape = &circle;
+            } else {
+                edge.Set(b2Vec2(f.e[2], f.e[3]), b2Vec2(f.e[4], f.e[5]));
+                edge.m_vertex0 = b2Vec2(f.e[0], f.e[1]);
+                edge.m_vertex3 = b2Vec2(f.e[6], f.e[7]);
+                edge.m_hasVertex0 = f.has0 != 0;
+                edge.m_hasVertex3 = f.has3 != 0;
+                shape = &edge;
+            }
+            b2FixtureDef fd;
+            fd.shape = shape;
+            fd.density = f.density;
+            fd.friction = f.friction;
+            fd.restitution = f.restitution;
+            fd.isSensor = f.sensor != 0;
+            fd.filter.categoryBits = (uint16_t)f.categoryBits;
+            fd.filter.maskBits = (uint16_t)f.maskBits;
+            fd.filter.groupIndex = (int16_t)f.groupIndex;
+            r->fixtures.push_back(body->CreateFixture(&fd));
+        }
+    }
+    return r->world.Flush();
+}
+
+}  // namespace
+
+extern "C" {
+
+void* b2pp_world_create(float gx, float gy) { return new Runner(gx, gy); }
+void b2pp_world_destroy(void* h) { delete (Runner*)h; }
+
+void b2pp_world_set_flags(void* h, int allowSleep, int warmStarting, int continuous, int subStepping) {
+    Runner* r = (Runner*)h;
+    r->world.SetAllowSleeping(allowSleep != 0);
+    r->world.SetWarmStarting(warmStarting != 0);
+    r->world.SetContinuousPhysics(continuous != 0);
+    r->world.SetSubStepping(subStepping != 0);
+}
+
+void b2pp_world_set_mode(void* h, int mode) {
+    Runner* r = (Runner*)h;
+    b2SolverOptions o = r->world.GetSolverOptions();
+    o.mode = mode;
+    r->world.SetSolverOptions(o);
+}
+
+int b2pp_world_load(void* h, const b2scene::BodyDef* bodies, int nb, const b2scene::FixtureDef* fx, int nf) {
+    return loadScene((Runner*)h, bodies, nb, fx, nf);
+}
+
+int b2pp_world_load_scene(void* h, const char* kind, int n, unsigned seed) {
+    b2scene::Scene s = b2scene::byName(kind, n, seed);
+    return loadScene((Runner*)h, s.bodies.data(), (int)s.bodies.size(), s.fixtures.data(), (int)s.fixtures.size());
+}
+
+int b2pp_world_step(void* h, float dt, int vel, int pos, int nsteps) {
+    Runner* r = (Runner*)h;
+    for (int i = 0; i < nsteps; ++i) {
+        r->world.Step(dt, vel, pos);
+        if (r->world.GetLastStatus() != 0) return r->world.GetLastStatus();
+    }
+    return 0;
+}
+
+int b2pp_world_sync(void* h) { return b2g_sync(((Runner*)h)->world.GetDeviceContext()); }
+
+const char* b2pp_world_last_error(void* h) { return ((Runner*)h)->world.GetLastError().c_str(); }
+
+int b2pp_world_body_count(void* h) { return (int)((Runner*)h)->bodies.size(); }
+int b2pp_world_fixture_count(void* h) { return (int)((Runner*)h)->fixtures.size(); }
+
+void b2pp_world_get_bodies(void* h, float* out) {
+    Runner* r = (Runner*)h;
+    for (size_t i = 0; i < r->bodies.size(); ++i) {
+        const b2Body* b = r->bodies[i];
+        const b2Transform& xf = b->GetTransform();
+        float* o = out + 12 * i;
+        o[0] = xf.p[0]; o[1] = xf.p[1]; o[2] = xf.q.s; o[3] = xf.q.c;
+        o[4] = b->GetWorldCenter()[0]; o[5] = b->GetWorldCenter()[1]; o[6] = b->GetAngle();
+        o[7] = b->GetLinearVelocity()[0]; o[8] = b->GetLinearVelocity()[1];
+        o[9] = b->GetAngularVelocity();
+        o[10] = b->IsAwake() ? 1.0f : 0.0f;
+        o[11] = 0.0f;
+    }
+}
+
+int b2pp_world_contact_count(void* h) { return ((Runner*)h)->world.GetContactCount(); }
+
+// Device table order (not the reference's list order).
+int b2pp_world_get_contacts(void* h, int32_t* meta, b2g_manifold* manifolds, int cap) {
+    Runner* r = (Runner*)h;
+    int n = 0;
+    b2g_get_contacts(r->world.GetDeviceContext(), meta, manifolds, cap, &n);
+    return n;
+}
+
+void b2pp_world_get_fat_aabbs(void* h, b2g_aabb* out) {
+    Runner* r = (Runner*)h;
+    b2g_get_fat_aabbs(r->world.GetDeviceContext(), out, (int)r->fixtures.size());
+}
+
+void b2pp_world_penetration(void* h, double* out) { ((Runner*)h)->world.GetPenetration(out); }
+
+void* b2pp_world_device_context(void* h) { return ((Runner*)h)->world.GetDeviceContext(); }
+
+int b2pp_world_stats(void* h, b2g_step_stats* out) {
+    return b2g_get_stats(((Runner*)h)->world.GetDeviceContext(), out);
+}
+
+}  // extern "C"

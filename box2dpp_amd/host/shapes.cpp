@@ -1,0 +1,232 @@
+// Host-side geometry of the box2d:: API: polygon hull construction, boxes,
+// AABBs, mass properties and world manifolds. These run once per shape at
+// creation (or on demand for queries); their FP32 arithmetic follows
+// upstream Box2D 2.3.2 so shapes built here are bit-identical to the
+// reference's.
+#include "box2d/b2Collision.h"
+
+namespace box2d {
+
+// b2PolygonShape::SetAsBox, REF/Collision/Shapes/b2PolygonShape.cpp:32-61
+void b2PolygonShape::SetAsBox(float hx, float hy) {
+    m_count = 4;
+    m_vertices[0] = b2Vec2(-hx, -hy);
+    m_vertices[1] = b2Vec2(hx, -hy);
+    m_vertices[2] = b2Vec2(hx, hy);
+    m_vertices[3] = b2Vec2(-hx, hy);
+    m_normals[0] = b2Vec2(0.0f, -1.0f);
+    m_normals[1] = b2Vec2(1.0f, 0.0f);
+    m_normals[2] = b2Vec2(0.0f, 1.0f);
+    m_normals[3] = b2Vec2(-1.0f, 0.0f);
+    m_centroid.SetZero();
+}
+
+void b2PolygonShape::SetAsBox(float hx, float hy, const b2Vec2& center, float angle) {
+    SetAsBox(hx, hy);
+    m_centroid = center;
+    b2Transform xf;
+    xf.p = center;
+    xf.q.Set(angle);
+    for (int32_t i = 0; i < m_count; ++i) {
+        m_vertices[i] = b2Mul(xf, m_vertices[i]);
+        m_normals[i] = b2Mul(xf.q, m_normals[i]);
+    }
+}
+
+namespace {
+// Area-weighted centroid about the origin (b2PolygonShape.cpp:96-142).
+b2Vec2 centroidOf(const b2Vec2* vs, int32_t count) {
+    b2Vec2 c(0.0f, 0.0f);
+    float area = 0.0f;
+    const b2Vec2 ref(0.0f, 0.0f);
+    const float third = 1.0f / 3.0f;
+    for (int32_t i = 0; i < count; ++i) {
+        const b2Vec2 p2 = vs[i];
+        const b2Vec2 p3 = i + 1 < count ? vs[i + 1] : vs[0];
+        const b2Vec2 e1 = p2 - ref, e2 = p3 - ref;
+        const float half = 0.5f * b2Cross(e1, e2);
+        area += half;
+        c += half * third * (ref + p2 + p3);
+    }
+    c *= 1.0f / area;
+    return c;
+}
+}  // namespace
+
+// b2PolygonShape::Set, REF/Collision/Shapes/b2PolygonShape.cpp:144-245
+void b2PolygonShape::Set(const b2Vec2* points, int32_t count) {
+    if (count < 3) {
+        SetAsBox(1.0f, 1.0f);
+        return;
+    }
+    const int32_t n0 = b2Min(count, MAX_POLYGON_VERTICES);
+    b2Vec2 ps[MAX_POLYGON_VERTICES];
+    int32_t n = 0;
+    for (int32_t i = 0; i < n0; ++i) {
+        bool unique = true;
+        for (int32_t j = 0; j < n && unique; ++j)
+            if (b2DistanceSquared(points[i], ps[j]) < 0.5f * LINEAR_SLOP) unique = false;
+        if (unique) ps[n++] = points[i];
+    }
+    if (n < 3) {
+        SetAsBox(1.0f, 1.0f);
+        return;
+    }
+    // Rightmost point (lowest y on ties) starts the gift wrap.
+    int32_t start = 0;
+    float x0 = ps[0][0];
+    for (int32_t i = 1; i < n; ++i) {
+        const float x = ps[i][0];
+        if (x > x0 || (x == x0 && ps[i][1] < ps[start][1])) {
+            start = i;
+            x0 = x;
+        }
+    }
+    int32_t hull[MAX_POLYGON_VERTICES];
+    int32_t m = 0;
+    int32_t cur = start;
+    for (;;) {
+        hull[m] = cur;
+        int32_t next = 0;
+        for (int32_t j = 1; j < n; ++j) {
+            if (next == cur) {
+                next = j;
+                continue;
+            }
+            const b2Vec2 r = ps[next] - ps[hull[m]];
+            const b2Vec2 v = ps[j] - ps[hull[m]];
+            const float c = b2Cross(r, v);
+            if (c < 0.0f) next = j;
+            if (c == 0.0f && v.LengthSquared() > r.LengthSquared()) next = j;
+        }
+        ++m;
+        cur = next;
+        if (next == start) break;
+    }
+    if (m < 3) {
+        SetAsBox(1.0f, 1.0f);
+        return;
+    }
+    m_count = m;
+    for (int32_t i = 0; i < m; ++i) m_vertices[i] = ps[hull[i]];
+    for (int32_t i = 0; i < m; ++i) {
+        const b2Vec2 edge = m_vertices[i + 1 < m ? i + 1 : 0] - m_vertices[i];
+        m_normals[i] = b2Cross(edge, 1.0f);
+        b2Normalize(m_normals[i]);
+    }
+    m_centroid = centroidOf(m_vertices, m);
+}
+
+void b2PolygonShape::ComputeAABB(b2AABB* aabb, const b2Transform& xf, int32_t childIndex) const {
+    B2_NOT_USED(childIndex);
+    b2Vec2 lower = b2Mul(xf, m_vertices[0]);
+    b2Vec2 upper = lower;
+    for (int32_t i = 1; i < m_count; ++i) {
+        const b2Vec2 v = b2Mul(xf, m_vertices[i]);
+        lower = b2Min(lower, v);
+        upper = b2Max(upper, v);
+    }
+    const b2Vec2 r(m_radius, m_radius);
+    aabb->lowerBound = lower - r;
+    aabb->upperBound = upper + r;
+}
+
+// b2PolygonShape::ComputeMass, REF/Collision/Shapes/b2PolygonShape.cpp:354-437:
+// triangle fan about the vertex mean.
+void b2PolygonShape::ComputeMass(b2MassData* md, float density) const {
+    b2Vec2 center(0.0f, 0.0f);
+    float area = 0.0f;
+    float I = 0.0f;
+    b2Vec2 s(0.0f, 0.0f);
+    for (int32_t i = 0; i < m_count; ++i) s += m_vertices[i];
+    s *= 1.0f / m_count;
+    const float third = 1.0f / 3.0f;
+    for (int32_t i = 0; i < m_count; ++i) {
+        const b2Vec2 e1 = m_vertices[i] - s;
+        const b2Vec2 e2 = i + 1 < m_count ? m_vertices[i + 1] - s : m_vertices[0] - s;
+        const float D = b2Cross(e1, e2);
+        const float tri = 0.5f * D;
+        area += tri;
+        center += tri * third * (e1 + e2);
+        const float ex1 = e1[0], ey1 = e1[1], ex2 = e2[0], ey2 = e2[1];
+        const float intx2 = ex1 * ex1 + ex2 * ex1 + ex2 * ex2;
+        const float inty2 = ey1 * ey1 + ey2 * ey1 + ey2 * ey2;
+        I += (0.25f * third * D) * (intx2 + inty2);
+    }
+    md->mass = density * area;
+    center *= 1.0f / area;
+    md->center = center + s;
+    md->I = density * I;
+    md->I += md->mass * (b2Dot(md->center, md->center) - b2Dot(center, center));
+}
+
+void b2CircleShape::ComputeAABB(b2AABB* aabb, const b2Transform& xf, int32_t childIndex) const {
+    B2_NOT_USED(childIndex);
+    const b2Vec2 p = xf.p + b2Mul(xf.q, m_p);
+    aabb->lowerBound = b2Vec2(p[0] - m_radius, p[1] - m_radius);
+    aabb->upperBound = b2Vec2(p[0] + m_radius, p[1] + m_radius);
+}
+
+void b2CircleShape::ComputeMass(b2MassData* md, float density) const {
+    md->mass = density * B2_PI * m_radius * m_radius;
+    md->center = m_p;
+    md->I = md->mass * (0.5f * m_radius * m_radius + b2Dot(m_p, m_p));
+}
+
+void b2EdgeShape::ComputeAABB(b2AABB* aabb, const b2Transform& xf, int32_t childIndex) const {
+    B2_NOT_USED(childIndex);
+    const b2Vec2 v1 = b2Mul(xf, m_vertex1);
+    const b2Vec2 v2 = b2Mul(xf, m_vertex2);
+    const b2Vec2 r(m_radius, m_radius);
+    aabb->lowerBound = b2Min(v1, v2) - r;
+    aabb->upperBound = b2Max(v1, v2) + r;
+}
+
+void b2EdgeShape::ComputeMass(b2MassData* md, float density) const {
+    B2_NOT_USED(density);
+    md->mass = 0.0f;
+    md->center = 0.5f * (m_vertex1 + m_vertex2);
+    md->I = 0.0f;
+}
+
+// b2WorldManifold::Initialize, REF/Collision/b2Collision.cpp:22-86
+void b2WorldManifold::Initialize(const b2Manifold* m, const b2Transform& xfA, float radiusA, const b2Transform& xfB,
+                                 float radiusB) {
+    if (m->pointCount == 0) return;
+    if (m->type == b2Manifold::e_circles) {
+        normal = b2Vec2(1.0f, 0.0f);
+        const b2Vec2 pointA = b2Mul(xfA, m->localPoint);
+        const b2Vec2 pointB = b2Mul(xfB, m->points[0].localPoint);
+        if (b2DistanceSquared(pointA, pointB) > FLT_EPSILON * FLT_EPSILON) {
+            normal = pointB - pointA;
+            b2Normalize(normal);
+        }
+        const b2Vec2 cA = pointA + radiusA * normal;
+        const b2Vec2 cB = pointB - radiusB * normal;
+        points[0] = 0.5f * (cA + cB);
+        separations[0] = b2Dot(cB - cA, normal);
+    } else if (m->type == b2Manifold::e_faceA) {
+        normal = b2Mul(xfA.q, m->localNormal);
+        const b2Vec2 planePoint = b2Mul(xfA, m->localPoint);
+        for (int32_t i = 0; i < m->pointCount; ++i) {
+            const b2Vec2 clipPoint = b2Mul(xfB, m->points[i].localPoint);
+            const b2Vec2 cA = clipPoint + (radiusA - b2Dot(clipPoint - planePoint, normal)) * normal;
+            const b2Vec2 cB = clipPoint - radiusB * normal;
+            points[i] = 0.5f * (cA + cB);
+            separations[i] = b2Dot(cB - cA, normal);
+        }
+    } else {
+        normal = b2Mul(xfB.q, m->localNormal);
+        const b2Vec2 planePoint = b2Mul(xfB, m->localPoint);
+        for (int32_t i = 0; i < m->pointCount; ++i) {
+            const b2Vec2 clipPoint = b2Mul(xfA, m->points[i].localPoint);
+            const b2Vec2 cB = clipPoint + (radiusB - b2Dot(clipPoint - planePoint, normal)) * normal;
+            const b2Vec2 cA = clipPoint - radiusA * normal;
+            points[i] = 0.5f * (cA + cB);
+            separations[i] = b2Dot(cA - cB, normal);
+        }
+        normal = -normal;
+    }
+}
+
+}  // namespace box2d

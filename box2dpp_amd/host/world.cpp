@@ -1,0 +1,704 @@
+// Host side of the box2d:: dynamics API. b2World keeps host mirrors of its
+// bodies and fixtures and hands the per-step work to the GPU through the
+// C ABI of b2g.h:
+//   * bodies and fixtures created since the last Step are appended to the
+//     device world in creation order (body index = creation order, proxy
+//     index = fixture creation order, as the reference's proxy ids grow);
+//   * host edits (SetTransform, velocities, forces, SetAwake, mass) mark a
+//     body dirty and are uploaded before the next Step;
+//   * after Step the mirrors are refreshed from the device on first read,
+//     so getters that return references (GetTransform, GetPosition, ...) are
+//     current, and contact listeners are replayed.
+#include "box2d/b2World.h"
+#include "b2g.h"
+
+#include <algorithm>
+#include <chrono>
+#include <unordered_set>
+
+namespace box2d {
+
+namespace {
+
+b2g_shape toDeviceShape(const b2Shape* s) {
+    b2g_shape r{};
+    r.type = (int32_t)s->GetType();
+    r.radius = s->m_radius;
+    if (s->GetType() == b2Shape::e_polygon) {
+        const b2PolygonShape* p = static_cast<const b2PolygonShape*>(s);
+        r.count = p->m_count;
+        r.centroid = {p->m_centroid[0], p->m_centroid[1]};
+        for (int i = 0; i < p->m_count; ++i) {
+            r.verts[i] = {p->m_vertices[i][0], p->m_vertices[i][1]};
+            r.normals[i] = {p->m_normals[i][0], p->m_normals[i][1]};
+        }
+    } else if (s->GetType() == b2Shape::e_circle) {
+        const b2CircleShape* c = static_cast<const b2CircleShape*>(s);
+        r.centroid = {c->m_p[0], c->m_p[1]};
+    } else if (s->GetType() == b2Shape::e_edge) {
+        const b2EdgeShape* e = static_cast<const b2EdgeShape*>(s);
+        r.count = 2;
+        r.verts[0] = {e->m_vertex1[0], e->m_vertex1[1]};
+        r.verts[1] = {e->m_vertex2[0], e->m_vertex2[1]};
+        r.v0 = {e->m_vertex0[0], e->m_vertex0[1]};
+        r.v3 = {e->m_vertex3[0], e->m_vertex3[1]};
+        r.flags = (e->m_hasVertex0 ? 1 : 0) | (e->m_hasVertex3 ? 2 : 0);
+    }
+    return r;
+}
+
+uint64_t pairKeyOf(int a, int b) {
+    const uint32_t lo = (uint32_t)std::min(a, b), hi = (uint32_t)std::max(a, b);
+    return ((uint64_t)lo << 32) | hi;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ callbacks
+// b2ContactFilter::ShouldCollide, REF/Dynamics/b2WorldCallbacks.cpp:26-38
+bool b2ContactFilter::ShouldCollide(b2Fixture* fixtureA, b2Fixture* fixtureB) {
+    const b2Filter& a = fixtureA->GetFilterData();
+    const b2Filter& b = fixtureB->GetFilterData();
+    if (a.groupIndex == b.groupIndex && a.groupIndex != 0) return a.groupIndex > 0;
+    return (a.maskBits & b.categoryBits) != 0 && (a.categoryBits & b.maskBits) != 0;
+}
+
+// ------------------------------------------------------------ fixture
+void b2Fixture::SetFilterData(const b2Filter& filter) {
+    m_filter = filter;
+    m_body->m_world->m_updatedFixtures.push_back(m_index);
+}
+void b2Fixture::SetFriction(float f) {
+    m_friction = f;
+    m_body->m_world->m_updatedFixtures.push_back(m_index);
+}
+void b2Fixture::SetRestitution(float r) {
+    m_restitution = r;
+    m_body->m_world->m_updatedFixtures.push_back(m_index);
+}
+
+// ------------------------------------------------------------ body
+// b2Body::b2Body, REF/Dynamics/b2Body.cpp:25-90
+b2Body::b2Body(const b2BodyDef* bd, b2World* world, int32_t index) : m_world(world) {
+    m_index = index;
+    m_flags = 0;
+    if (bd->bullet) m_flags |= e_bulletFlag;
+    if (bd->fixedRotation) m_flags |= e_fixedRotationFlag;
+    if (bd->allowSleep) m_flags |= e_autoSleepFlag;
+    if (bd->awake) m_flags |= e_awakeFlag;
+    if (bd->active) m_flags |= e_activeFlag;
+    m_xf.p = bd->position;
+    m_xf.q.Set(bd->angle);
+    m_sweep.localCenter.SetZero();
+    m_sweep.c0 = m_xf.p;
+    m_sweep.c = m_xf.p;
+    m_sweep.a0 = bd->angle;
+    m_sweep.a = bd->angle;
+    m_sweep.alpha0 = 0.0f;
+    m_linearVelocity = bd->linearVelocity;
+    m_angularVelocity = bd->angularVelocity;
+    m_linearDamping = bd->linearDamping;
+    m_angularDamping = bd->angularDamping;
+    m_gravityScale = bd->gravityScale;
+    m_force.SetZero();
+    m_torque = 0.0f;
+    m_sleepTime = 0.0f;
+    m_type = bd->type;
+    if (m_type == b2BodyType::DYNAMIC_BODY) {
+        m_mass = 1.0f;
+        m_invMass = 1.0f;
+    } else {
+        m_mass = 0.0f;
+        m_invMass = 0.0f;
+    }
+    m_I = 0.0f;
+    m_invI = 0.0f;
+    m_userData = bd->userData;
+}
+
+void b2Body::Touch() const { m_world->Download(); }
+
+void b2Body::MarkDirty() {
+    if (m_index < m_world->m_uploadedBodies) {
+        m_dirty = true;
+        m_world->m_anyDirty = true;
+    }
+}
+
+// b2Body::CreateFixture, REF/Dynamics/b2Body.cpp:164-203
+b2Fixture* b2Body::CreateFixture(const b2FixtureDef* def) {
+    if (m_world->IsLocked() || def == nullptr || def->shape == nullptr) return nullptr;
+    Touch();
+    b2Fixture* f = new b2Fixture;
+    f->m_shape.reset(def->shape->Clone());
+    f->m_body = this;
+    f->m_userData = def->userData;
+    f->m_density = def->density;
+    f->m_friction = def->friction;
+    f->m_restitution = def->restitution;
+    f->m_isSensor = def->isSensor;
+    f->m_filter = def->filter;
+    f->m_index = (int32_t)m_world->m_fixtures.size();
+    m_world->m_fixtures.push_back(f);
+    f->m_next = m_fixtureList;
+    m_fixtureList = f;
+    ++m_fixtureCount;
+    if (f->m_density > 0.0f) ResetMassData();
+    return f;
+}
+
+b2Fixture* b2Body::CreateFixture(const b2Shape* shape, float density) {
+    b2FixtureDef def;
+    def.shape = shape;
+    def.density = density;
+    return CreateFixture(&def);
+}
+
+// b2Body::ResetMassData, REF/Dynamics/b2Body.cpp:285-352
+void b2Body::ResetMassData() {
+    Touch();
+    m_mass = 0.0f;
+    m_invMass = 0.0f;
+    m_I = 0.0f;
+    m_invI = 0.0f;
+    m_sweep.localCenter.SetZero();
+    if (m_type == b2BodyType::STATIC_BODY || m_type == b2BodyType::KINEMATIC_BODY) {
+        m_sweep.c0 = m_xf.p;
+        m_sweep.c = m_xf.p;
+        m_sweep.a0 = m_sweep.a;
+        MarkDirty();
+        return;
+    }
+    b2Vec2 localCenter(0.0f, 0.0f);
+    for (b2Fixture* f = m_fixtureList; f; f = f->m_next) {
+        if (f->m_density == 0.0f) continue;
+        b2MassData md;
+        f->GetMassData(&md);
+        m_mass += md.mass;
+        localCenter += md.mass * md.center;
+        m_I += md.I;
+    }
+    if (m_mass > 0.0f) {
+        m_invMass = 1.0f / m_mass;
+        localCenter *= m_invMass;
+    } else {
+        m_mass = 1.0f;
+        m_invMass = 1.0f;
+    }
+    if (m_I > 0.0f && (m_flags & e_fixedRotationFlag) == 0) {
+        m_I -= m_mass * b2Dot(localCenter, localCenter);
+        m_invI = 1.0f / m_I;
+    } else {
+        m_I = 0.0f;
+        m_invI = 0.0f;
+    }
+    const b2Vec2 oldCenter = m_sweep.c;
+    m_sweep.localCenter = localCenter;
+    m_sweep.c0 = m_sweep.c = b2Mul(m_xf, m_sweep.localCenter);
+    m_linearVelocity += b2Cross(m_angularVelocity, m_sweep.c - oldCenter);
+    MarkDirty();
+}
+
+const b2Transform& b2Body::GetTransform() const { Touch(); return m_xf; }
+const b2Vec2& b2Body::GetPosition() const { Touch(); return m_xf.p; }
+float b2Body::GetAngle() const { Touch(); return m_sweep.a; }
+const b2Vec2& b2Body::GetWorldCenter() const { Touch(); return m_sweep.c; }
+const b2Vec2& b2Body::GetLocalCenter() const { return m_sweep.localCenter; }
+const b2Vec2& b2Body::GetLinearVelocity() const { Touch(); return m_linearVelocity; }
+float b2Body::GetAngularVelocity() const { Touch(); return m_angularVelocity; }
+bool b2Body::IsAwake() const { Touch(); return (m_flags & e_awakeFlag) != 0; }
+float b2Body::GetInertia() const { return m_I + m_mass * b2Dot(m_sweep.localCenter, m_sweep.localCenter); }
+
+void b2Body::GetMassData(b2MassData* data) const {
+    data->mass = m_mass;
+    data->I = GetInertia();
+    data->center = m_sweep.localCenter;
+}
+
+b2Vec2 b2Body::GetWorldPoint(const b2Vec2& p) const { return b2Mul(GetTransform(), p); }
+b2Vec2 b2Body::GetWorldVector(const b2Vec2& v) const { return b2Mul(GetTransform().q, v); }
+b2Vec2 b2Body::GetLocalPoint(const b2Vec2& p) const { return b2MulT(GetTransform(), p); }
+b2Vec2 b2Body::GetLocalVector(const b2Vec2& v) const { return b2MulT(GetTransform().q, v); }
+b2Vec2 b2Body::GetLinearVelocityFromWorldPoint(const b2Vec2& p) const {
+    Touch();
+    return m_linearVelocity + b2Cross(m_angularVelocity, p - m_sweep.c);
+}
+
+// b2Body::SetTransform, REF/Dynamics/b2Body.cpp:419-440 (the proxies are
+// re-synchronised on the device at the next Step through the move flags).
+void b2Body::SetTransform(const b2Vec2& position, float angle) {
+    if (m_world->IsLocked()) return;
+    Touch();
+    m_xf.q.Set(angle);
+    m_xf.p = position;
+    m_sweep.c = b2Mul(m_xf, m_sweep.localCenter);
+    m_sweep.a = angle;
+    m_sweep.c0 = m_sweep.c;
+    m_sweep.a0 = angle;
+    MarkDirty();
+}
+
+void b2Body::SetLinearVelocity(const b2Vec2& v) {
+    if (m_type == b2BodyType::STATIC_BODY) return;
+    Touch();
+    if (b2Dot(v, v) > 0.0f) SetAwake(true);
+    m_linearVelocity = v;
+    MarkDirty();
+}
+
+void b2Body::SetAngularVelocity(float w) {
+    if (m_type == b2BodyType::STATIC_BODY) return;
+    Touch();
+    if (w * w > 0.0f) SetAwake(true);
+    m_angularVelocity = w;
+    MarkDirty();
+}
+
+void b2Body::ApplyForce(const b2Vec2& force, const b2Vec2& point, bool wake) {
+    if (m_type != b2BodyType::DYNAMIC_BODY) return;
+    Touch();
+    if (wake && (m_flags & e_awakeFlag) == 0) SetAwake(true);
+    if (m_flags & e_awakeFlag) {
+        m_force += force;
+        m_torque += b2Cross(point - m_sweep.c, force);
+        MarkDirty();
+    }
+}
+
+void b2Body::ApplyForceToCenter(const b2Vec2& force, bool wake) {
+    if (m_type != b2BodyType::DYNAMIC_BODY) return;
+    Touch();
+    if (wake && (m_flags & e_awakeFlag) == 0) SetAwake(true);
+    if (m_flags & e_awakeFlag) {
+        m_force += force;
+        MarkDirty();
+    }
+}
+
+void b2Body::ApplyTorque(float torque, bool wake) {
+    if (m_type != b2BodyType::DYNAMIC_BODY) return;
+    Touch();
+    if (wake && (m_flags & e_awakeFlag) == 0) SetAwake(true);
+    if (m_flags & e_awakeFlag) {
+        m_torque += torque;
+        MarkDirty();
+    }
+}
+
+void b2Body::ApplyLinearImpulse(const b2Vec2& impulse, const b2Vec2& point, bool wake) {
+    if (m_type != b2BodyType::DYNAMIC_BODY) return;
+    Touch();
+    if (wake && (m_flags & e_awakeFlag) == 0) SetAwake(true);
+    if (m_flags & e_awakeFlag) {
+        m_linearVelocity += m_invMass * impulse;
+        m_angularVelocity += m_invI * b2Cross(point - m_sweep.c, impulse);
+        MarkDirty();
+    }
+}
+
+void b2Body::ApplyAngularImpulse(float impulse, bool wake) {
+    if (m_type != b2BodyType::DYNAMIC_BODY) return;
+    Touch();
+    if (wake && (m_flags & e_awakeFlag) == 0) SetAwake(true);
+    if (m_flags & e_awakeFlag) {
+        m_angularVelocity += m_invI * impulse;
+        MarkDirty();
+    }
+}
+
+void b2Body::SetLinearDamping(float d) { Touch(); m_linearDamping = d; MarkDirty(); }
+void b2Body::SetAngularDamping(float d) { Touch(); m_angularDamping = d; MarkDirty(); }
+void b2Body::SetGravityScale(float s) { Touch(); m_gravityScale = s; MarkDirty(); }
+
+void b2Body::SetSleepingAllowed(bool flag) {
+    Touch();
+    if (flag) m_flags |= e_autoSleepFlag;
+    else {
+        m_flags &= ~e_autoSleepFlag;
+        SetAwake(true);
+    }
+    MarkDirty();
+}
+
+// b2Body::SetAwake, REF/Dynamics/b2Body.h:635-654
+void b2Body::SetAwake(bool flag) {
+    Touch();
+    if (flag) {
+        if ((m_flags & e_awakeFlag) == 0) {
+            m_flags |= e_awakeFlag;
+            m_sleepTime = 0.0f;
+        }
+    } else {
+        m_flags &= ~e_awakeFlag;
+        m_sleepTime = 0.0f;
+        m_linearVelocity.SetZero();
+        m_angularVelocity = 0.0f;
+        m_force.SetZero();
+        m_torque = 0.0f;
+    }
+    MarkDirty();
+}
+
+// ------------------------------------------------------------ world
+b2World::b2World(const b2Vec2& gravity) : m_gravity(gravity) {
+    if (b2g_create(&m_ctx, 0) != B2G_OK) {
+        m_ctx = nullptr;
+        m_lastStatus = B2G_ERR_HIP;
+        m_lastError = "no HIP device: the GPU step library could not create a context";
+    }
+}
+
+b2World::~b2World() {
+    for (b2Body* b : m_bodies) {
+        if (!b) continue;
+        b2Fixture* f = b->m_fixtureList;
+        while (f) {
+            b2Fixture* n = f->m_next;
+            delete f;
+            f = n;
+        }
+        delete b;
+    }
+    if (m_ctx) b2g_destroy(m_ctx);
+}
+
+// b2World::CreateBody, REF/Dynamics/b2World.cpp:109-127: new bodies go to
+// the head of the body list.
+b2Body* b2World::CreateBody(const b2BodyDef* def) {
+    if (IsLocked() || def == nullptr) return nullptr;
+    b2Body* b = new b2Body(def, this, (int32_t)m_bodies.size());
+    m_bodies.push_back(b);
+    b->m_prev = nullptr;
+    b->m_next = m_bodyList;
+    if (m_bodyList) m_bodyList->m_prev = b;
+    m_bodyList = b;
+    ++m_bodyCount;
+    return b;
+}
+
+// Destroyed bodies become inactive static bodies on the device; their
+// proxies leave the broad-phase, so their contacts end at the next Step.
+void b2World::DestroyBody(b2Body* b) {
+    if (IsLocked() || b == nullptr || b->m_world != this) return;
+    b->Touch();
+    if (b->m_prev) b->m_prev->m_next = b->m_next;
+    if (b->m_next) b->m_next->m_prev = b->m_prev;
+    if (b == m_bodyList) m_bodyList = b->m_next;
+    b->m_flags = 0;
+    b->m_type = b2BodyType::STATIC_BODY;
+    b->m_linearVelocity.SetZero();
+    b->m_angularVelocity = 0.0f;
+    b->m_invMass = 0.0f;
+    b->m_invI = 0.0f;
+    b->MarkDirty();
+    for (b2Fixture* f = b->m_fixtureList; f; f = f->m_next) m_removedFixtures.push_back(f->m_index);
+    --m_bodyCount;
+    m_contactsStale = true;
+}
+
+void b2World::SetGravity(const b2Vec2& g) {
+    m_gravity = g;
+    m_worldDirty = true;
+}
+
+// b2World::SetAllowSleeping, REF/Dynamics/b2World.cpp:370-386
+void b2World::SetAllowSleeping(bool flag) {
+    if (flag == m_allowSleep) return;
+    m_allowSleep = flag;
+    m_worldDirty = true;
+    if (!flag) {
+        for (b2Body* b = m_bodyList; b; b = b->m_next) b->SetAwake(true);
+    }
+}
+
+void b2World::ClearForces() {
+    for (b2Body* b = m_bodyList; b; b = b->m_next) {
+        b->Touch();
+        b->m_force.SetZero();
+        b->m_torque = 0.0f;
+        b->MarkDirty();
+    }
+}
+
+static b2g_body_state stateOf(const b2Body* b, int type, int flags, const b2Transform& xf, const b2Sweep& sw,
+                              const b2Vec2& v, float w, const b2Vec2& f, float t, float invMass, float invI,
+                              float ld, float ad, float gs, float sleepTime) {
+    b2g_body_state s{};
+    (void)b;
+    s.type = type;
+    s.flags = flags;
+    s.xf[0] = xf.p[0]; s.xf[1] = xf.p[1]; s.xf[2] = xf.q.s; s.xf[3] = xf.q.c;
+    s.lc[0] = sw.localCenter[0]; s.lc[1] = sw.localCenter[1];
+    s.c0[0] = sw.c0[0]; s.c0[1] = sw.c0[1]; s.a0 = sw.a0;
+    s.c[0] = sw.c[0]; s.c[1] = sw.c[1]; s.a = sw.a;
+    s.v[0] = v[0]; s.v[1] = v[1]; s.w = w;
+    s.force[0] = f[0]; s.force[1] = f[1]; s.torque = t;
+    s.invMass = invMass; s.invI = invI;
+    s.linearDamping = ld; s.angularDamping = ad; s.gravityScale = gs; s.sleepTime = sleepTime;
+    return s;
+}
+
+int32_t b2World::Upload() {
+    if (!m_ctx) return B2G_ERR_HIP;
+    int rc = B2G_OK;
+    if (m_worldDirty) {
+        b2g_world_def wd{};
+        wd.gravity[0] = m_gravity[0];
+        wd.gravity[1] = m_gravity[1];
+        wd.allowSleep = m_allowSleep;
+        wd.warmStarting = m_warmStarting;
+        wd.continuous = 0;
+        wd.subStepping = 0;
+        wd.autoClearForces = m_autoClearForces;
+        rc = b2g_set_world(m_ctx, &wd);
+        if (rc != B2G_OK) return rc;
+        m_worldDirty = false;
+    }
+    auto toState = [](const b2Body* b) {
+        return stateOf(b, (int)b->m_type, b->m_flags, b->m_xf, b->m_sweep, b->m_linearVelocity,
+                       b->m_angularVelocity, b->m_force, b->m_torque, b->m_invMass, b->m_invI, b->m_linearDamping,
+                       b->m_angularDamping, b->m_gravityScale, b->m_sleepTime);
+    };
+    if (m_anyDirty) {
+        for (int i = 0; i < m_uploadedBodies; ++i) {
+            b2Body* b = m_bodies[i];
+            if (!b->m_dirty) continue;
+            // contiguous dirty runs in one call
+            int j = i;
+            while (j < m_uploadedBodies && m_bodies[j]->m_dirty) ++j;
+            std::vector<b2g_body_state> st;
+            st.reserve(j - i);
+            for (int k = i; k < j; ++k) {
+                st.push_back(toState(m_bodies[k]));
+                m_bodies[k]->m_dirty = false;
+            }
+            rc = b2g_set_bodies(m_ctx, i, j - i, st.data());
+            if (rc != B2G_OK) return rc;
+            i = j - 1;
+        }
+        m_anyDirty = false;
+    }
+    const int nb = (int)m_bodies.size();
+    if (nb > m_uploadedBodies) {
+        std::vector<b2g_body_state> st;
+        st.reserve(nb - m_uploadedBodies);
+        for (int i = m_uploadedBodies; i < nb; ++i) st.push_back(toState(m_bodies[i]));
+        int first = 0;
+        rc = b2g_add_bodies(m_ctx, st.data(), (int)st.size(), &first);
+        if (rc != B2G_OK) return rc;
+        m_uploadedBodies = nb;
+    }
+    const int nf = (int)m_fixtures.size();
+    if (nf > m_uploadedFixtures) {
+        std::vector<b2g_fixture_def> defs;
+        defs.reserve(nf - m_uploadedFixtures);
+        for (int i = m_uploadedFixtures; i < nf; ++i) {
+            const b2Fixture* f = m_fixtures[i];
+            b2g_fixture_def d{};
+            d.body = f->m_body->m_index;
+            d.sensor = f->m_isSensor ? 1 : 0;
+            d.friction = f->m_friction;
+            d.restitution = f->m_restitution;
+            d.density = f->m_density;
+            d.categoryBits = f->m_filter.categoryBits;
+            d.maskBits = f->m_filter.maskBits;
+            d.groupIndex = f->m_filter.groupIndex;
+            d.shape = toDeviceShape(f->m_shape.get());
+            defs.push_back(d);
+        }
+        int first = 0;
+        rc = b2g_add_fixtures(m_ctx, defs.data(), (int)defs.size(), &first);
+        if (rc != B2G_OK) return rc;
+        m_uploadedFixtures = nf;
+    }
+    if (!m_updatedFixtures.empty()) {
+        std::vector<int32_t> ids;
+        std::vector<b2g_fixture_def> defs;
+        for (int32_t id : m_updatedFixtures) {
+            if (id >= m_uploadedFixtures) continue;
+            const b2Fixture* f = m_fixtures[id];
+            b2g_fixture_def d{};
+            d.body = f->m_body->m_index;
+            d.sensor = f->m_isSensor ? 1 : 0;
+            d.friction = f->m_friction;
+            d.restitution = f->m_restitution;
+            d.density = f->m_density;
+            d.categoryBits = f->m_filter.categoryBits;
+            d.maskBits = f->m_filter.maskBits;
+            d.groupIndex = f->m_filter.groupIndex;
+            ids.push_back(id);
+            defs.push_back(d);
+        }
+        rc = b2g_update_fixtures(m_ctx, ids.data(), defs.data(), (int)ids.size());
+        if (rc != B2G_OK) return rc;
+        m_updatedFixtures.clear();
+    }
+    if (!m_removedFixtures.empty()) {
+        std::vector<int32_t> ids;
+        for (int32_t id : m_removedFixtures)
+            if (id < m_uploadedFixtures) ids.push_back(id);
+        rc = b2g_remove_fixtures(m_ctx, ids.data(), (int)ids.size());
+        if (rc != B2G_OK) return rc;
+        m_removedFixtures.clear();
+    }
+    return B2G_OK;
+}
+
+int32_t b2World::Flush() {
+    const int rc = Upload();
+    if (rc != B2G_OK && m_ctx) m_lastError = b2g_last_error(m_ctx);
+    return rc;
+}
+
+void b2World::Download() const {
+    if (!m_hostStale || !m_ctx) return;
+    m_hostStale = false;
+    const int n = m_uploadedBodies;
+    if (n == 0) return;
+    std::vector<b2g_body_state> st(n);
+    if (b2g_get_bodies(m_ctx, 0, n, st.data()) != B2G_OK) return;
+    for (int i = 0; i < n; ++i) {
+        b2Body* b = m_bodies[i];
+        if (b->m_dirty) continue;
+        const b2g_body_state& s = st[i];
+        b->m_flags = (b->m_flags & ~0x2) | (s.flags & 0x2);
+        b->m_xf.p = b2Vec2(s.xf[0], s.xf[1]);
+        b->m_xf.q.s = s.xf[2];
+        b->m_xf.q.c = s.xf[3];
+        b->m_sweep.c0 = b2Vec2(s.c0[0], s.c0[1]);
+        b->m_sweep.a0 = s.a0;
+        b->m_sweep.c = b2Vec2(s.c[0], s.c[1]);
+        b->m_sweep.a = s.a;
+        b->m_linearVelocity = b2Vec2(s.v[0], s.v[1]);
+        b->m_angularVelocity = s.w;
+        b->m_force = b2Vec2(s.force[0], s.force[1]);
+        b->m_torque = s.torque;
+        b->m_sleepTime = s.sleepTime;
+    }
+}
+
+// b2World::Step, Box2D/Dynamics/b2World.cpp:888-954, on the GPU.
+void b2World::Step(float dt, int32_t velocityIterations, int32_t positionIterations) {
+    const auto t0 = std::chrono::steady_clock::now();
+    m_lastStatus = B2G_OK;
+    m_lastError.clear();
+    if (!m_ctx) {
+        m_lastStatus = B2G_ERR_HIP;
+        m_lastError = "no HIP device";
+        return;
+    }
+    if (m_continuousPhysics || m_subStepping) {
+        // The reference runs SolveTOI here; the GPU path has no TOI phase.
+        m_lastError = "continuous physics is not executed on the GPU path; stepping without it";
+    }
+    m_locked = true;
+    int rc = Upload();
+    if (rc == B2G_OK) {
+        b2g_step_params p{};
+        p.dt = dt;
+        p.velocityIterations = velocityIterations;
+        p.positionIterations = positionIterations;
+        p.mode = m_options.mode;
+        p.learnedWarmStart = m_options.learnedWarmStart ? 1 : 0;
+        p.residualThreshold = m_options.residualThreshold;
+        rc = b2g_step(m_ctx, &p);
+    }
+    m_locked = false;
+    if (rc != B2G_OK) {
+        m_lastStatus = rc;
+        m_lastError = b2g_last_error(m_ctx);
+        return;
+    }
+    m_hostStale = true;
+    m_contactsStale = true;
+    b2g_step_stats st{};
+    b2g_get_stats(m_ctx, &st);
+    m_contactCount = st.contacts;
+    if (m_contactListener) ReplayEvents();
+    const auto t1 = std::chrono::steady_clock::now();
+    m_profile.step = std::chrono::duration<float, std::milli>(t1 - t0).count();
+}
+
+b2Contact* b2World::GetContactList() {
+    if (m_contactsStale && m_ctx) {
+        int n = 0;
+        b2g_get_contacts(m_ctx, nullptr, nullptr, 0, &n);
+        std::vector<int32_t> meta(4 * (size_t)n);
+        std::vector<b2g_manifold> man(n);
+        b2g_get_contacts(m_ctx, meta.data(), man.data(), n, &n);
+        m_contacts.assign(n, b2Contact());
+        // Newest contacts first, as the reference's contact list.
+        for (int i = 0; i < n; ++i) {
+            const int src = n - 1 - i;
+            b2Contact& c = m_contacts[i];
+            c.m_fixtureA = m_fixtures[meta[4 * src]];
+            c.m_fixtureB = m_fixtures[meta[4 * src + 1]];
+            c.m_touching = (meta[4 * src + 2] & 1) != 0;
+            c.m_enabled = (meta[4 * src + 2] & 2) != 0;
+            c.m_friction = sqrtf(c.m_fixtureA->m_friction * c.m_fixtureB->m_friction);
+            c.m_restitution = std::max(c.m_fixtureA->m_restitution, c.m_fixtureB->m_restitution);
+            const b2g_manifold& m = man[src];
+            c.m_manifold.pointCount = m.pointCount;
+            c.m_manifold.type = (b2Manifold::Type)m.type;
+            c.m_manifold.localNormal = b2Vec2(m.localNormal.x, m.localNormal.y);
+            c.m_manifold.localPoint = b2Vec2(m.localPoint.x, m.localPoint.y);
+            for (int k = 0; k < 2; ++k) {
+                c.m_manifold.points[k].localPoint = b2Vec2(m.points[k].localPoint.x, m.points[k].localPoint.y);
+                c.m_manifold.points[k].normalImpulse = m.points[k].normalImpulse;
+                c.m_manifold.points[k].tangentImpulse = m.points[k].tangentImpulse;
+                c.m_manifold.points[k].id.key = m.points[k].id;
+            }
+            c.m_next = i + 1 < n ? &m_contacts[i + 1] : nullptr;
+        }
+        m_contactsStale = false;
+    }
+    return m_contacts.empty() ? nullptr : &m_contacts[0];
+}
+
+void b2Contact::GetWorldManifold(b2WorldManifold* wm) const {
+    const b2Body* bA = m_fixtureA->GetBody();
+    const b2Body* bB = m_fixtureB->GetBody();
+    wm->Initialize(&m_manifold, bA->GetTransform(), m_fixtureA->GetShape()->m_radius, bB->GetTransform(),
+                   m_fixtureB->GetShape()->m_radius);
+}
+
+// Contact events after a Step: BeginContact for fixture pairs that started
+// touching, EndContact for pairs that stopped (or whose contact ended), and
+// PostSolve with each touching contact's stored impulses.
+void b2World::ReplayEvents() {
+    b2Contact* list = GetContactList();
+    std::vector<uint64_t> now;
+    for (b2Contact* c = list; c; c = c->GetNext())
+        if (c->IsTouching()) now.push_back(pairKeyOf(c->m_fixtureA->m_index, c->m_fixtureB->m_index));
+    std::sort(now.begin(), now.end());
+    std::unordered_set<uint64_t> before(m_touchingKeys.begin(), m_touchingKeys.end());
+    std::unordered_set<uint64_t> after(now.begin(), now.end());
+    for (uint64_t k : m_touchingKeys) {
+        if (after.count(k)) continue;
+        b2Contact ended;
+        ended.m_fixtureA = m_fixtures[(int)(k >> 32)];
+        ended.m_fixtureB = m_fixtures[(int)(k & 0xFFFFFFFFu)];
+        ended.m_touching = false;
+        m_contactListener->EndContact(&ended);
+    }
+    for (b2Contact* c = list; c; c = c->GetNext()) {
+        if (!c->IsTouching()) continue;
+        const uint64_t k = pairKeyOf(c->m_fixtureA->m_index, c->m_fixtureB->m_index);
+        if (!before.count(k)) m_contactListener->BeginContact(c);
+        b2ContactImpulse imp{};
+        imp.count = c->m_manifold.pointCount;
+        for (int j = 0; j < imp.count; ++j) {
+            imp.normalImpulses[j] = c->m_manifold.points[j].normalImpulse;
+            imp.tangentImpulses[j] = c->m_manifold.points[j].tangentImpulse;
+        }
+        m_contactListener->PostSolve(c, &imp);
+    }
+    m_touchingKeys.swap(now);
+}
+
+void b2World::GetPenetration(double* out) const {
+    out[0] = out[1] = out[2] = 0.0;
+    if (m_ctx) b2g_penetration(m_ctx, out);
+}
+
+}  // namespace box2d

@@ -2,7 +2,7 @@
  * b2g.h - C ABI of the MI355X (gfx950) Box2D step library, libb2g.so.
  *
  * This is the thin boundary between the host C++ `box2d::` API
- * (include/box2d/*.h) and the HIP kernels. Only plain C types cross it:
+ * (headers under include/box2d) and the HIP kernels. Only plain C types cross it:
  * pointers, sizes, POD structs. Every call returns an int status
  * (B2G_OK = 0, negative on error); b2g_last_error() gives the message.
  * Device memory is owned by the context. One context per host thread;
@@ -86,6 +86,198 @@ typedef struct b2g_manifold {
 
 /* Fixture-level AABB {lower, upper}. */
 typedef struct b2g_aabb { float lx, ly, ux, uy; } b2g_aabb;
+
+/* Full simulation state of one body: b2Body's transform, sweep, velocity,
+ * accumulated force and mass data (REF/Dynamics/b2Body.h:430-470). The host
+ * computes mass data (ResetMassData) at creation; the device owns the rest
+ * between steps. flags use b2Body::m_flags bits: 0x2 awake, 0x4 auto-sleep,
+ * 0x8 bullet, 0x10 fixed rotation, 0x20 active. type: 0 static,
+ * 1 kinematic, 2 dynamic (b2BodyType order). */
+typedef struct b2g_body_state {
+    int32_t type;
+    int32_t flags;
+    float xf[4];      /* p.x p.y q.s q.c */
+    float lc[2];      /* sweep localCenter */
+    float c0[2];
+    float a0;
+    float c[2];
+    float a;
+    float v[2];
+    float w;
+    float force[2];
+    float torque;
+    float invMass;
+    float invI;
+    float linearDamping;
+    float angularDamping;
+    float gravityScale;
+    float sleepTime;
+} b2g_body_state;
+
+/* One fixture: body index, material, filter and shape
+ * (b2FixtureDef, Box2D/Dynamics/b2Fixture.h:59-94). Fixtures get broad-phase
+ * proxy ids in the order they are added. */
+typedef struct b2g_fixture_def {
+    int32_t body;
+    int32_t sensor;
+    float friction;
+    float restitution;
+    float density;
+    uint32_t categoryBits;
+    uint32_t maskBits;
+    int32_t groupIndex;
+    b2g_shape shape;
+} b2g_fixture_def;
+
+/* World-level switches (b2World::SetAllowSleeping / SetWarmStarting /
+ * SetContinuousPhysics / SetSubStepping / SetAutoClearForces,
+ * Box2D/Dynamics/b2World.h:146-217). continuous and subStepping must be 0:
+ * the GPU step has no time-of-impact phase. */
+typedef struct b2g_world_def {
+    float gravity[2];
+    int32_t allowSleep;
+    int32_t warmStarting;
+    int32_t continuous;
+    int32_t subStepping;
+    int32_t autoClearForces;
+} b2g_world_def;
+
+/* Solver modes. COLOURED: graph-coloured sequential impulses over all
+ * contacts at once (the throughput path). REFERENCE: one wavefront per
+ * island visiting constraints in Box2D's island order. */
+#define B2G_MODE_COLOURED 0
+#define B2G_MODE_REFERENCE 1
+
+/* b2World::Step arguments (Box2D/Dynamics/b2World.h:96) plus the solver
+ * mode and the learned warm-start switches. */
+typedef struct b2g_step_params {
+    float dt;
+    int32_t velocityIterations;
+    int32_t positionIterations;
+    int32_t mode;
+    int32_t learnedWarmStart;   /* 0 off, 1 on (requires a loaded model) */
+    float residualThreshold;    /* fallback threshold for the learned start */
+} b2g_step_params;
+
+/* Counters of the last step. */
+typedef struct b2g_step_stats {
+    int32_t bodies;
+    int32_t fixtures;
+    int32_t contacts;          /* contacts alive after the step (b2World::GetContactCount) */
+    int32_t constraints;       /* touching contacts solved this step */
+    int32_t points;            /* manifold points of those contacts */
+    int32_t colours;
+    int32_t colouringRounds;
+    int32_t newPairs;          /* contacts created by the broad-phase this step */
+    int32_t positionIterations;/* position iterations actually run */
+    int32_t beginEvents;       /* BeginContact count this step */
+    int32_t endEvents;         /* EndContact count this step */
+    int32_t overflow;          /* nonzero if a device buffer overflowed */
+    float minSeparation;       /* of the last position iteration run */
+    float cellSize;            /* broad-phase grid cell */
+} b2g_step_stats;
+
+typedef struct b2g_context b2g_context;
+
+/* ---------------------------------------------------------- world calls */
+
+/* Creates a context on HIP device `device`. Replaces the construction of
+ * b2World's device-side state (Box2D/Dynamics/b2World.h:48). */
+int b2g_create(b2g_context** out, int device);
+int b2g_destroy(b2g_context* ctx);
+/* Message of the last failing call on ctx (static storage owned by ctx). */
+const char* b2g_last_error(const b2g_context* ctx);
+
+/* Pre-sizes device arrays (grows automatically otherwise). */
+int b2g_reserve(b2g_context* ctx, int maxBodies, int maxFixtures, int maxContacts);
+
+/* World switches and gravity; see b2g_world_def. */
+int b2g_set_world(b2g_context* ctx, const b2g_world_def* def);
+
+/* Appends bodies (b2World::CreateBody, Box2D/Dynamics/b2World.h:74);
+ * *first receives the index of the first new body. */
+int b2g_add_bodies(b2g_context* ctx, const b2g_body_state* bodies, int count, int* first);
+/* Overwrites bodies [first, first+count) (SetTransform, SetLinearVelocity,
+ * ApplyForce, SetAwake, mass changes: Box2D/Dynamics/b2Body.h:163-330). */
+int b2g_set_bodies(b2g_context* ctx, int first, int count, const b2g_body_state* bodies);
+/* Reads bodies [first, first+count) back to the host. */
+int b2g_get_bodies(b2g_context* ctx, int first, int count, b2g_body_state* out);
+
+/* Appends fixtures with broad-phase proxies (b2Body::CreateFixture,
+ * Box2D/Dynamics/b2Body.h:138); new proxies join the move buffer so the next
+ * step finds their pairs (b2World::Step's e_newFixture path,
+ * REF/Dynamics/b2World.cpp:903-907). */
+int b2g_add_fixtures(b2g_context* ctx, const b2g_fixture_def* fixtures, int count, int* first);
+
+/* Removes fixtures' proxies from the broad-phase (b2Body::DestroyFixture /
+ * b2World::DestroyBody, Box2D/Dynamics/b2Body.h:156, b2World.h:80): their
+ * contacts end at the next step when their fat AABBs no longer overlap. */
+int b2g_remove_fixtures(b2g_context* ctx, const int32_t* ids, int count);
+/* Updates material and filter of fixtures (b2Fixture::SetFriction /
+ * SetRestitution / SetFilterData, Box2D/Dynamics/b2Fixture.h:131-170); a
+ * filter change flags the fixture's contacts for filtering and re-queries
+ * its proxy, as b2Fixture::Refilter does. */
+int b2g_update_fixtures(b2g_context* ctx, const int32_t* ids, const b2g_fixture_def* defs, int count);
+
+/* One time step: b2World::Step(dt, velocityIterations, positionIterations)
+ * (Box2D/Dynamics/b2World.cpp:888-954; REF/Dynamics/b2World.cpp:899)
+ * without the continuous (TOI) phase. Asynchronous on the context's stream:
+ * returns once the work is queued; b2g_get_* and b2g_sync wait for it. */
+int b2g_step(b2g_context* ctx, const b2g_step_params* params);
+/* Blocks until queued work on ctx has finished. */
+int b2g_sync(b2g_context* ctx);
+int b2g_get_stats(b2g_context* ctx, b2g_step_stats* out);
+
+/* Contacts in table order: meta[4*i] = fixtureA, [4*i+1] = fixtureB (after
+ * the registry's shape-order swap), [4*i+2] = flags (bit0 touching,
+ * bit1 enabled). *count receives the number of contacts (may exceed cap). */
+int b2g_get_contacts(b2g_context* ctx, int32_t* meta, b2g_manifold* manifolds, int cap, int* count);
+/* Fat AABB of every proxy, fixture order. */
+int b2g_get_fat_aabbs(b2g_context* ctx, b2g_aabb* out, int cap);
+/* Mean and max penetration over touching manifold points (from
+ * b2WorldManifold separations), and the point count: out[0..2]. */
+int b2g_penetration(b2g_context* ctx, double* out);
+
+/* Per-phase GPU time of the steps run while profiling is on, measured with
+ * HIP events on the context's stream around every launch of the phase. */
+typedef struct b2g_phase_profile {
+    char name[24];
+    int32_t launches;
+    int32_t pad;
+    double ms;        /* summed launch durations */
+} b2g_phase_profile;
+/* on != 0 starts profiling and clears the accumulated times. */
+int b2g_set_profiling(b2g_context* ctx, int on);
+int b2g_get_profile(b2g_context* ctx, b2g_phase_profile* out, int cap, int* count);
+
+/* ---------------------------------------------- per-function batch calls
+ * Run one hot-path stage over host arrays (copied in and out); used for
+ * per-function parity. */
+
+/* Narrowphase for n shape pairs already in registry order
+ * (b2CollidePolygons / b2CollideCircles / b2CollidePolygonAndCircle /
+ * b2CollideEdgeAndCircle / b2CollideEdgeAndPolygon,
+ * Box2D/Collision/b2Collision.h:237-260). Unsupported pairs give
+ * pointCount 0. */
+int b2g_collide_batch(b2g_context* ctx, const b2g_shape* a, const b2g_transform* xfA, const b2g_shape* b,
+                      const b2g_transform* xfB, int n, b2g_manifold* out);
+/* b2WorldManifold::Initialize (REF/Collision/b2Collision.cpp:22) for n
+ * manifolds; out: 8 floats each (normal, 2 points, 2 separations). */
+int b2g_world_manifold_batch(b2g_context* ctx, const b2g_manifold* m, const b2g_transform* xfA,
+                             const float* radiusA, const b2g_transform* xfB, const float* radiusB, int n,
+                             float* out);
+/* b2BroadPhase::UpdatePairs (Box2D/Collision/b2BroadPhase.h:193-247): the
+ * pair set of proxies whose fat AABBs overlap a moved proxy's, as sorted
+ * unique (min, max) proxy index pairs. *count may exceed cap. */
+int b2g_pairs_batch(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved, int n, int32_t* pairs, int cap,
+                    int* count);
+/* b2DynamicTree::MoveProxy's fat-AABB rule (Box2D/Collision/b2DynamicTree.cpp:137-181):
+ * per proxy, fatOut = fat unchanged and moved = 0 when fat contains tight,
+ * else the re-fattened, displacement-extended box and moved = 1. */
+int b2g_fat_rule_batch(b2g_context* ctx, const b2g_aabb* fat, const b2g_aabb* tight, const float* disp, int n,
+                       b2g_aabb* fatOut, uint8_t* moved);
+/* Rotation used inside the step for n angles (sine, cosine). */
+int b2g_sincos_batch(b2g_context* ctx, const float* angles, int n, float* s, float* c);
 
 #ifdef __cplusplus
 }

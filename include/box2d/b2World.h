@@ -1,0 +1,353 @@
+// box2d:: dynamics API of the host side (the fork's Box2D/Dynamics/b2World.h,
+// b2Body.h, b2Fixture.h, b2WorldCallbacks.h, Contacts/b2Contact.h and
+// b2TimeStep.h surface). b2World::Step runs on the GPU through the C ABI in
+// b2g.h; the objects here are host mirrors that stay current after every
+// Step (their getters return references into host members, as the
+// reference's do).
+//
+// Differences from the reference, by design of the GPU path:
+//  * continuous collision (SetContinuousPhysics(true)) and sub-stepping are
+//    not executed on the GPU; Step reports them through GetLastError().
+//  * sensors and joints are rejected at creation.
+//  * b2ContactListener::PreSolve is not called (no mid-step host hook);
+//    BeginContact, EndContact and PostSolve are replayed after Step.
+#pragma once
+#include <vector>
+#include <string>
+#include <memory>
+#include "b2Math.h"
+#include "b2Collision.h"
+
+struct b2g_context;
+
+namespace box2d {
+
+class b2World;
+class b2Body;
+class b2Fixture;
+class b2Contact;
+
+// Box2D/Dynamics/b2Body.h:41-49
+enum class b2BodyType { STATIC_BODY = 0, KINEMATIC_BODY, DYNAMIC_BODY };
+
+// Box2D/Dynamics/b2Body.h:53-127
+struct b2BodyDef {
+    void* userData = nullptr;
+    b2Vec2 position{0.0f, 0.0f};
+    float angle = 0.0f;
+    b2Vec2 linearVelocity{0.0f, 0.0f};
+    float angularVelocity = 0.0f;
+    float linearDamping = 0.0f;
+    float angularDamping = 0.0f;
+    bool allowSleep = true;
+    bool awake = true;
+    bool fixedRotation = false;
+    bool bullet = false;
+    b2BodyType type = b2BodyType::STATIC_BODY;
+    bool active = true;
+    float gravityScale = 1.0f;
+};
+
+// Box2D/Dynamics/b2Fixture.h:34-57
+struct b2Filter {
+    uint16_t categoryBits = 0x0001;
+    uint16_t maskBits = 0xFFFF;
+    int16_t groupIndex = 0;
+};
+
+// Box2D/Dynamics/b2Fixture.h:59-94
+struct b2FixtureDef {
+    const b2Shape* shape = nullptr;
+    void* userData = nullptr;
+    float friction = 0.2f;
+    float restitution = 0.0f;
+    float density = 0.0f;
+    bool isSensor = false;
+    b2Filter filter;
+};
+
+// Box2D/Dynamics/b2WorldCallbacks.h:74-79
+struct b2ContactImpulse {
+    float normalImpulses[MAX_MANIFOLD_POINTS];
+    float tangentImpulses[MAX_MANIFOLD_POINTS];
+    int32_t count;
+};
+
+// Box2D/Dynamics/b2WorldCallbacks.h:59-71
+class b2ContactFilter {
+public:
+    virtual ~b2ContactFilter() {}
+    virtual bool ShouldCollide(b2Fixture* fixtureA, b2Fixture* fixtureB);
+};
+
+// Box2D/Dynamics/b2WorldCallbacks.h:90-136
+class b2ContactListener {
+public:
+    virtual ~b2ContactListener() {}
+    virtual void BeginContact(b2Contact* contact) { B2_NOT_USED(contact); }
+    virtual void EndContact(b2Contact* contact) { B2_NOT_USED(contact); }
+    virtual void PreSolve(b2Contact* contact, const b2Manifold* oldManifold) {
+        B2_NOT_USED(contact);
+        B2_NOT_USED(oldManifold);
+    }
+    virtual void PostSolve(b2Contact* contact, const b2ContactImpulse* impulse) {
+        B2_NOT_USED(contact);
+        B2_NOT_USED(impulse);
+    }
+};
+
+// Box2D/Dynamics/b2TimeStep.h:27-37 (milliseconds)
+struct b2Profile {
+    float step = 0.0f;
+    float collide = 0.0f;
+    float solve = 0.0f;
+    float solveInit = 0.0f;
+    float solveVelocity = 0.0f;
+    float solvePosition = 0.0f;
+    float broadphase = 0.0f;
+    float solveTOI = 0.0f;
+};
+
+class b2Fixture {
+public:
+    b2Shape::Type GetType() const { return m_shape->GetType(); }
+    b2Shape* GetShape() { return m_shape.get(); }
+    const b2Shape* GetShape() const { return m_shape.get(); }
+    bool IsSensor() const { return m_isSensor; }
+    const b2Filter& GetFilterData() const { return m_filter; }
+    void SetFilterData(const b2Filter& filter);
+    b2Body* GetBody() { return m_body; }
+    const b2Body* GetBody() const { return m_body; }
+    b2Fixture* GetNext() { return m_next; }
+    const b2Fixture* GetNext() const { return m_next; }
+    void* GetUserData() const { return m_userData; }
+    void SetUserData(void* d) { m_userData = d; }
+    float GetDensity() const { return m_density; }
+    float GetFriction() const { return m_friction; }
+    void SetFriction(float f);
+    float GetRestitution() const { return m_restitution; }
+    void SetRestitution(float r);
+    void GetMassData(b2MassData* md) const { m_shape->ComputeMass(md, m_density); }
+    // Broad-phase proxy index of this fixture (its creation order in the world).
+    int32_t GetProxyIndex() const { return m_index; }
+
+private:
+    friend class b2Body;
+    friend class b2World;
+    std::unique_ptr<b2Shape> m_shape;
+    b2Body* m_body = nullptr;
+    b2Fixture* m_next = nullptr;
+    void* m_userData = nullptr;
+    float m_density = 0.0f, m_friction = 0.2f, m_restitution = 0.0f;
+    bool m_isSensor = false;
+    b2Filter m_filter;
+    int32_t m_index = -1;
+};
+
+class b2Body {
+public:
+    b2Fixture* CreateFixture(const b2FixtureDef* def);
+    b2Fixture* CreateFixture(const b2Shape* shape, float density);
+
+    void SetTransform(const b2Vec2& position, float angle);
+    const b2Transform& GetTransform() const;
+    const b2Vec2& GetPosition() const;
+    float GetAngle() const;
+    const b2Vec2& GetWorldCenter() const;
+    const b2Vec2& GetLocalCenter() const;
+    void SetLinearVelocity(const b2Vec2& v);
+    const b2Vec2& GetLinearVelocity() const;
+    void SetAngularVelocity(float omega);
+    float GetAngularVelocity() const;
+    void ApplyForce(const b2Vec2& force, const b2Vec2& point, bool wake);
+    void ApplyForceToCenter(const b2Vec2& force, bool wake);
+    void ApplyTorque(float torque, bool wake);
+    void ApplyLinearImpulse(const b2Vec2& impulse, const b2Vec2& point, bool wake);
+    void ApplyAngularImpulse(float impulse, bool wake);
+    float GetMass() const { return m_mass; }
+    float GetInertia() const;
+    void GetMassData(b2MassData* data) const;
+    void ResetMassData();
+    b2Vec2 GetWorldPoint(const b2Vec2& localPoint) const;
+    b2Vec2 GetWorldVector(const b2Vec2& localVector) const;
+    b2Vec2 GetLocalPoint(const b2Vec2& worldPoint) const;
+    b2Vec2 GetLocalVector(const b2Vec2& worldVector) const;
+    b2Vec2 GetLinearVelocityFromWorldPoint(const b2Vec2& worldPoint) const;
+    float GetLinearDamping() const { return m_linearDamping; }
+    void SetLinearDamping(float d);
+    float GetAngularDamping() const { return m_angularDamping; }
+    void SetAngularDamping(float d);
+    float GetGravityScale() const { return m_gravityScale; }
+    void SetGravityScale(float s);
+    b2BodyType GetType() const { return m_type; }
+    bool IsBullet() const { return (m_flags & e_bulletFlag) != 0; }
+    void SetSleepingAllowed(bool flag);
+    bool IsSleepingAllowed() const { return (m_flags & e_autoSleepFlag) != 0; }
+    void SetAwake(bool flag);
+    bool IsAwake() const;
+    bool IsActive() const { return (m_flags & e_activeFlag) != 0; }
+    bool IsFixedRotation() const { return (m_flags & e_fixedRotationFlag) != 0; }
+    b2Fixture* GetFixtureList() { return m_fixtureList; }
+    const b2Fixture* GetFixtureList() const { return m_fixtureList; }
+    b2Body* GetNext() { return m_next; }
+    const b2Body* GetNext() const { return m_next; }
+    void* GetUserData() const { return m_userData; }
+    void SetUserData(void* d) { m_userData = d; }
+    b2World* GetWorld() { return m_world; }
+    const b2World* GetWorld() const { return m_world; }
+    // Device index of this body (its creation order in the world).
+    int32_t GetIndex() const { return m_index; }
+
+private:
+    friend class b2World;
+    friend class b2Fixture;
+    enum {
+        e_islandFlag = 0x0001, e_awakeFlag = 0x0002, e_autoSleepFlag = 0x0004, e_bulletFlag = 0x0008,
+        e_fixedRotationFlag = 0x0010, e_activeFlag = 0x0020,
+    };
+    b2Body(const b2BodyDef* def, b2World* world, int32_t index);
+    void Touch() const;   // reads the device state if the host copy is stale
+    void MarkDirty();     // host edits to upload before the next Step
+    b2BodyType m_type;
+    int32_t m_flags = 0;
+    int32_t m_index = -1;
+    b2Transform m_xf;
+    b2Sweep m_sweep;
+    b2Vec2 m_linearVelocity;
+    float m_angularVelocity = 0.0f;
+    b2Vec2 m_force;
+    float m_torque = 0.0f;
+    b2World* m_world;
+    b2Body* m_prev = nullptr;
+    b2Body* m_next = nullptr;
+    b2Fixture* m_fixtureList = nullptr;
+    int32_t m_fixtureCount = 0;
+    float m_mass = 0.0f, m_invMass = 0.0f;
+    float m_I = 0.0f, m_invI = 0.0f;
+    float m_linearDamping = 0.0f, m_angularDamping = 0.0f, m_gravityScale = 1.0f;
+    float m_sleepTime = 0.0f;
+    void* m_userData = nullptr;
+    bool m_dirty = false;
+};
+
+// Host view of one device contact, valid until the next Step.
+class b2Contact {
+public:
+    b2Manifold* GetManifold() { return &m_manifold; }
+    const b2Manifold* GetManifold() const { return &m_manifold; }
+    void GetWorldManifold(b2WorldManifold* worldManifold) const;
+    bool IsTouching() const { return m_touching; }
+    bool IsEnabled() const { return m_enabled; }
+    b2Contact* GetNext() { return m_next; }
+    const b2Contact* GetNext() const { return m_next; }
+    b2Fixture* GetFixtureA() { return m_fixtureA; }
+    const b2Fixture* GetFixtureA() const { return m_fixtureA; }
+    int32_t GetChildIndexA() const { return 0; }
+    b2Fixture* GetFixtureB() { return m_fixtureB; }
+    const b2Fixture* GetFixtureB() const { return m_fixtureB; }
+    int32_t GetChildIndexB() const { return 0; }
+    float GetFriction() const { return m_friction; }
+    float GetRestitution() const { return m_restitution; }
+    float GetTangentSpeed() const { return 0.0f; }
+
+private:
+    friend class b2World;
+    b2Fixture* m_fixtureA = nullptr;
+    b2Fixture* m_fixtureB = nullptr;
+    b2Manifold m_manifold;
+    bool m_touching = false, m_enabled = true;
+    float m_friction = 0.0f, m_restitution = 0.0f;
+    b2Contact* m_next = nullptr;
+};
+
+// Per-step solver options of the GPU path (no reference counterpart).
+struct b2SolverOptions {
+    int32_t mode = 0;                 // 0 coloured, 1 reference order
+    bool learnedWarmStart = false;
+    float residualThreshold = 1e-4f;
+};
+
+class b2World {
+public:
+    explicit b2World(const b2Vec2& gravity);
+    ~b2World();
+    b2World(const b2World&) = delete;
+    b2World& operator=(const b2World&) = delete;
+
+    void SetContactFilter(b2ContactFilter* filter) { m_contactFilter = filter; }
+    void SetContactListener(b2ContactListener* listener) { m_contactListener = listener; }
+    b2Body* CreateBody(const b2BodyDef* def);
+    void DestroyBody(b2Body* body);
+    void Step(float timeStep, int32_t velocityIterations, int32_t positionIterations);
+    void ClearForces();
+    b2Body* GetBodyList() { return m_bodyList; }
+    const b2Body* GetBodyList() const { return m_bodyList; }
+    b2Contact* GetContactList();
+    void SetAllowSleeping(bool flag);
+    bool GetAllowSleeping() const { return m_allowSleep; }
+    void SetWarmStarting(bool flag) { m_warmStarting = flag; }
+    bool GetWarmStarting() const { return m_warmStarting; }
+    void SetContinuousPhysics(bool flag) { m_continuousPhysics = flag; }
+    bool GetContinuousPhysics() const { return m_continuousPhysics; }
+    void SetSubStepping(bool flag) { m_subStepping = flag; }
+    bool GetSubStepping() const { return m_subStepping; }
+    int32_t GetProxyCount() const { return (int32_t)m_fixtures.size(); }
+    int32_t GetBodyCount() const { return m_bodyCount; }
+    int32_t GetJointCount() const { return 0; }
+    int32_t GetContactCount() const { return m_contactCount; }
+    void SetGravity(const b2Vec2& gravity);
+    b2Vec2 GetGravity() const { return m_gravity; }
+    bool IsLocked() const { return m_locked; }
+    void SetAutoClearForces(bool flag) { m_autoClearForces = flag; }
+    bool GetAutoClearForces() const { return m_autoClearForces; }
+    const b2Profile& GetProfile() const { return m_profile; }
+
+    // GPU-path extensions.
+    void SetSolverOptions(const b2SolverOptions& o) { m_options = o; }
+    const b2SolverOptions& GetSolverOptions() const { return m_options; }
+    // Status of the last Step (0 = ok, b2g.h error codes otherwise).
+    int32_t GetLastStatus() const { return m_lastStatus; }
+    const std::string& GetLastError() const { return m_lastError; }
+    b2g_context* GetDeviceContext() { return m_ctx; }
+    // Uploads pending bodies and fixtures without stepping.
+    int32_t Flush();
+    // Mean and max penetration over touching points and the point count,
+    // from device world manifolds.
+    void GetPenetration(double* meanMax3) const;
+
+private:
+    friend class b2Body;
+    friend class b2Fixture;
+    int32_t Upload();
+    void Download() const;
+    void ReplayEvents();
+
+    b2g_context* m_ctx = nullptr;
+    b2Vec2 m_gravity;
+    b2Body* m_bodyList = nullptr;
+    std::vector<b2Body*> m_bodies;        // creation order = device index
+    std::vector<b2Fixture*> m_fixtures;   // creation order = proxy index
+    int32_t m_bodyCount = 0;
+    int32_t m_uploadedBodies = 0;
+    int32_t m_uploadedFixtures = 0;
+    int32_t m_contactCount = 0;
+    bool m_allowSleep = true, m_warmStarting = true, m_continuousPhysics = true, m_subStepping = false;
+    bool m_autoClearForces = true;
+    bool m_locked = false;
+    bool m_worldDirty = true;
+    mutable bool m_hostStale = false;
+    bool m_anyDirty = false;
+    b2ContactFilter* m_contactFilter = nullptr;
+    b2ContactListener* m_contactListener = nullptr;
+    b2Profile m_profile;
+    b2SolverOptions m_options;
+    int32_t m_lastStatus = 0;
+    std::string m_lastError;
+    std::vector<b2Contact> m_contacts;
+    bool m_contactsStale = true;
+    std::vector<uint64_t> m_touchingKeys;  // touching fixture pairs after the last Step
+    std::vector<int32_t> m_removedFixtures;  // proxies to drop at the next upload
+    std::vector<int32_t> m_updatedFixtures;  // material/filter changes to upload
+};
+
+}  // namespace box2d

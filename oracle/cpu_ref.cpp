@@ -71,8 +71,18 @@ inline float normalize(V2& v) {
     return len;
 }
 
+// Trig used inside Step. 0: glibc sinf/cosf, as b2Rot::Set calls them
+// (REF/Common/b2Math.h:302-311); the reference build agrees bit for bit.
+// 1: the FP32 rounding of the double-precision sine/cosine, which is what the
+// GPU step evaluates (glibc sinf is not correctly rounded, so the two differ
+// in the last bit for about 1% of angles). Body creation always uses glibc.
+static int g_trigMode = 0;
+inline float stepSin(float a) { return g_trigMode ? (float)sin((double)a) : sinf(a); }
+inline float stepCos(float a) { return g_trigMode ? (float)cos((double)a) : cosf(a); }
+
 struct Rot { float s, c; };
 inline Rot rotOf(float angle) { Rot q; q.s = sinf(angle); q.c = cosf(angle); return q; }
+inline Rot stepRot(float angle) { Rot q; q.s = stepSin(angle); q.c = stepCos(angle); return q; }
 struct Xf { V2 p; Rot q; };
 inline V2 mul(Rot q, V2 v) { return mk(q.c * v.x - q.s * v.y, q.s * v.x + q.c * v.y); }
 inline V2 mulT(Rot q, V2 v) { return mk(q.c * v.x + q.s * v.y, -q.s * v.x + q.c * v.y); }
@@ -1258,7 +1268,7 @@ struct World {
     // REF/Dynamics/b2Body.cpp:443-454, REF/Dynamics/b2Fixture.cpp:154-176
     void synchronizeFixtures(Body& b) {
         Xf xf1;
-        xf1.q = rotOf(b.a0);
+        xf1.q = stepRot(b.a0);
         xf1.p = b.c0 - mul(xf1.q, b.localCenter);
         for (int k = (int)b.fixtures.size() - 1; k >= 0; --k) {
             const int fi = b.fixtures[k];
@@ -1275,7 +1285,7 @@ struct World {
 
     // b2Body::SynchronizeTransform, REF/Dynamics/b2Body.h:836-840
     void synchronizeTransform(Body& b) {
-        b.xf.q = rotOf(b.a);
+        b.xf.q = stepRot(b.a);
         b.xf.p = b.c - mul(b.xf.q, b.localCenter);
     }
 
@@ -1359,8 +1369,8 @@ struct World {
             V2 cB = posC[iB]; float aB = posA[iB];
             V2 vB = velV[iB]; float wB = velW[iB];
             Xf xfA, xfB;
-            xfA.q = rotOf(aA);
-            xfB.q = rotOf(aB);
+            xfA.q = stepRot(aA);
+            xfB.q = stepRot(aB);
             xfA.p = cA - mul(xfA.q, pc.localCenterA);
             xfB.p = cB - mul(xfB.q, pc.localCenterB);
             WorldManifold wm;
@@ -1596,8 +1606,8 @@ struct World {
             V2 cB = posC[iB]; float aB = posA[iB];
             for (int j = 0; j < pc.pointCount; ++j) {
                 Xf xfA, xfB;
-                xfA.q = rotOf(aA);
-                xfB.q = rotOf(aB);
+                xfA.q = stepRot(aA);
+                xfB.q = stepRot(aB);
                 xfA.p = cA - mul(xfA.q, lcA);
                 xfB.p = cB - mul(xfB.q, lcB);
                 V2 normal, point;
@@ -1871,6 +1881,25 @@ int orc_collide(const b2g_shape* a, const b2g_transform* xfA, const b2g_shape* b
     return collide(out, a, xfOf(xfA), b, xfOf(xfB)) ? 0 : -1;
 }
 
+float orc_find_max_separation(const b2g_shape* a, const b2g_transform* xfA, const b2g_shape* b,
+                              const b2g_transform* xfB, int* edge) {
+    return findMaxSeparation(edge, a, xfOf(xfA), b, xfOf(xfB));
+}
+
+void orc_find_incident_edge(const b2g_shape* a, const b2g_transform* xfA, int edge, const b2g_shape* b,
+                            const b2g_transform* xfB, float* out, uint32_t* ids) {
+    ClipVertex c[2];
+    findIncidentEdge(c, a, xfOf(xfA), edge, b, xfOf(xfB));
+    out[0] = c[0].v.x; out[1] = c[0].v.y; out[2] = c[1].v.x; out[3] = c[1].v.y;
+    ids[0] = c[0].id;
+    ids[1] = c[1].id;
+}
+
+void orc_transform(float x, float y, float angle, b2g_transform* out) {
+    Rot q = rotOf(angle);
+    out->px = x; out->py = y; out->s = q.s; out->c = q.c;
+}
+
 void orc_world_manifold(const b2g_manifold* m, const b2g_transform* xfA, float radiusA, const b2g_transform* xfB,
                         float radiusB, float* out) {
     for (int i = 0; i < 8; ++i) out[i] = 0.0f;
@@ -1905,6 +1934,9 @@ void orc_shape_aabb(const b2g_shape* s, const b2g_transform* xf, b2g_aabb* out) 
     AABB a = shapeAABB(s, xfOf(xf));
     *out = {a.lo.x, a.lo.y, a.hi.x, a.hi.y};
 }
+
+// Selects the trig used inside Step (0 glibc, 1 rounded double); process-wide.
+void orc_set_trig_mode(int mode) { g_trigMode = mode; }
 
 // glibc sinf/cosf, the functions b2Rot::Set calls (REF b2Math.h:302-311).
 void orc_sincos(const float* a, int n, float* s, float* c) {

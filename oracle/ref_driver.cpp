@@ -433,11 +433,48 @@ int ref_bp_create_proxy(void* h, const b2g_aabb* a, int userIndex) {
     return ((b2BroadPhase*)h)->CreateProxy(box, (void*)(intptr_t)(userIndex + 1));
 }
 
-void ref_bp_move_proxy(void* h, int proxyId, const b2g_aabb* a, float dx, float dy) {
+// b2BroadPhase::MoveProxy; returns whether the tree re-inserted the proxy
+// (the value b2DynamicTree::MoveProxy hands to BufferMove).
+int ref_bp_move_proxy(void* h, int proxyId, const b2g_aabb* a, float dx, float dy) {
     b2AABB box;
     box.lowerBound.Set(a->lx, a->ly);
     box.upperBound.Set(a->ux, a->uy);
-    ((b2BroadPhase*)h)->MoveProxy(proxyId, box, b2Vec2(dx, dy));
+    b2BroadPhase* bp = (b2BroadPhase*)h;
+    const bool moved = bp->m_tree.MoveProxy(proxyId, box, b2Vec2(dx, dy));
+    if (moved) bp->BufferMove(proxyId);
+    return moved ? 1 : 0;
+}
+
+// b2FindMaxSeparation (REF/Collision/b2CollidePolygon.cpp:28).
+float ref_find_max_separation(const b2g_shape* a, const b2g_transform* xfA, const b2g_shape* b,
+                              const b2g_transform* xfB, int* edge) {
+    b2PolygonShape pa, pb;
+    recToPolygon(a, &pa);
+    recToPolygon(b, &pb);
+    int32 e = 0;
+    const float s = b2FindMaxSeparation(&e, &pa, toXf(xfA), &pb, toXf(xfB));
+    *edge = e;
+    return s;
+}
+
+// b2FindIncidentEdge (REF/Collision/b2CollidePolygon.cpp:69); out: v0.x v0.y v1.x v1.y, ids[2].
+void ref_find_incident_edge(const b2g_shape* a, const b2g_transform* xfA, int edge, const b2g_shape* b,
+                            const b2g_transform* xfB, float* out, uint32_t* ids) {
+    b2PolygonShape pa, pb;
+    recToPolygon(a, &pa);
+    recToPolygon(b, &pb);
+    b2ClipVertex c[2];
+    b2FindIncidentEdge(c, &pa, toXf(xfA), edge, &pb, toXf(xfB));
+    out[0] = c[0].v.x; out[1] = c[0].v.y; out[2] = c[1].v.x; out[3] = c[1].v.y;
+    ids[0] = c[0].id.key;
+    ids[1] = c[1].id.key;
+}
+
+// b2Transform::Set(position, angle) on the host (glibc sinf/cosf).
+void ref_transform(float x, float y, float angle, b2g_transform* out) {
+    b2Transform xf;
+    xf.Set(b2Vec2(x, y), angle);
+    out->px = xf.p.x; out->py = xf.p.y; out->s = xf.q.s; out->c = xf.q.c;
 }
 
 void ref_bp_get_fat(void* h, int proxyId, b2g_aabb* out) {

@@ -1,0 +1,256 @@
+"""Pins the CPU oracle (oracle/cpu_ref.cpp) against the upstream Box2D 2.3.2
+build of the reference's own sources (oracle/_ref), against the reference's
+known-answer collision cases (Regression/tests/helloworld.cpp:268-418), and
+against closed forms. Everything here is bit-exact unless stated."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import corpus
+import engines as E
+
+F = ctypes.c_float
+
+
+# ------------------------------------------------------------ whole step
+@pytest.mark.parametrize("kind,n,seed,steps,sleep,vel,pos", [
+    ("hello", 0, 0, 60, True, 6, 2),           # helloworld.cpp:420-432 (6/2 iterations)
+    ("pyramid", 20, 0, 600, True, 8, 3),       # sleeps at step ~284
+    ("pyramid", 20, 0, 300, False, 8, 3),
+    ("pyramid", 6, 0, 200, True, 4, 1),
+    ("pile", 300, 1, 400, False, 8, 3),
+    ("pile", 1500, 7, 150, True, 8, 3),
+])
+def test_world_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, vel, pos):
+    a = ref.world(kind, n, seed, sleep=sleep)
+    b = orc.world(kind, n, seed, sleep=sleep)
+    for s in range(steps):
+        a.step(1, vel=vel, pos=pos)
+        b.step(1, vel=vel, pos=pos)
+        ba, bb = a.bodies(), b.bodies()
+        assert np.array_equal(ba.view(np.uint32), bb.view(np.uint32)), f"{kind} bodies differ at step {s + 1}"
+    ma, na = a.contacts()
+    mb, nb = b.contacts()
+    assert np.array_equal(ma, mb), "contact lists differ (order or fixtures or flags)"
+    assert np.array_equal(E.manifold_bits(na), E.manifold_bits(nb)), "manifolds differ"
+    assert a.penetration() == b.penetration()
+    a.close(); b.close()
+
+
+def test_pyramid_reference_facts(orc):
+    """Pyramid with sleeping on: 590 contacts, about 800 manifold points (799
+    with continuous collision on, 800 without), asleep by the end, mean
+    penetration about 0.7 cm (the reference's values)."""
+    w = orc.world("pyramid", 20, 0, sleep=True)
+    w.step(600)
+    bodies = w.bodies()
+    mean, mx, npts = w.penetration()
+    assert w.e.fn("world_contact_count")(w.h) == 590
+    assert npts == 800
+    assert np.all(bodies[1:, E.AWAKE] == 0)
+    assert 0.004 < mean < 0.01 and mx < 0.02
+
+
+def test_hello_world_closed_form(orc):
+    """HelloWorld: the box falls freely for the first steps (v = -g t in FP32)
+    and comes to rest on the ground at y = 1 within the polygon skin."""
+    w = orc.world("hello", sleep=True)
+    h = np.float32(1.0 / 60.0)
+    v = np.float32(0.0)
+    y = np.float32(4.0)
+    for s in range(10):
+        w.step(1, vel=6, pos=2)
+        v = np.float32(v + h * np.float32(np.float32(1.0) * np.float32(-10.0)))
+        y = np.float32(y + h * v)
+        b = w.bodies()[1]
+        assert b[E.VY] == v and b[E.CY] == y
+    w.step(50, vel=6, pos=2)
+    b = w.bodies()[1]
+    assert abs(b[E.PY] - 1.0) < 0.02 and abs(b[E.ANG]) < 1e-3
+
+
+# ------------------------------------------------------------ narrowphase
+def _compare_collide(reflib, orclib, A, XA, B, XB):
+    # b2Collide* leave the impulse fields unwritten, so compare geometry and ids.
+    mr = corpus.collide_all(reflib, "ref_", A, XA, B, XB)
+    mo = corpus.collide_all(orclib, "orc_", A, XA, B, XB)
+    bad = np.where(~np.all(E.geometry_bits(mr) == E.geometry_bits(mo), axis=1))[0]
+    assert len(bad) == 0, f"{len(bad)} manifolds differ, first case {bad[:5]}"
+    return mr
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_collide_corpus_bitexact(reflib, orclib, seed):
+    A, XA, B, XB = corpus.random_corpus(reflib, "ref_", 400, seed)
+    m = _compare_collide(reflib, orclib, A, XA, B, XB)
+    counts = np.bincount(m["count"], minlength=3)
+    assert counts[0] > 0 and counts[1] > 0 and counts[2] > 0, counts
+
+
+def test_collide_special_cases_bitexact(reflib, orclib):
+    A, XA, B, XB = corpus.special_corpus(reflib, "ref_")
+    _compare_collide(reflib, orclib, A, XA, B, XB)
+
+
+def _xf(lib, pre, x, y, a):
+    t = np.zeros(1, E.XF)
+    getattr(lib, pre + "transform")(F(x), F(y), F(a), E.ptr(t))
+    return t
+
+
+def test_reference_known_answers(reflib, orclib):
+    """The reference's collision tests (Regression/tests/helloworld.cpp:268-418),
+    with closed forms for what they compute."""
+    for lib, pre in ((reflib, "ref_"), (orclib, "orc_")):
+        box = np.array([E.box(lib, pre, 1.0, 1.0)], E.SHAPE)
+        # Testb2CollideCirclesTouching (REF placement: centres 1 apart, radii 1)
+        c = np.array([E.circle(1.0)], E.SHAPE)
+        m = np.zeros(1, E.MANIFOLD)
+        getattr(lib, pre + "collide")(E.ptr(c), E.ptr(_xf(lib, pre, 0, 0, 1)), E.ptr(c), E.ptr(_xf(lib, pre, -1, 0, 1)), E.ptr(m))
+        assert m[0]["count"] == 1 and m[0]["type"] == 0 and m[0]["points"][0]["id"] == 0
+        # Testb2FindMaxSeparation: unit boxes 3 apart, both turned 1 rad:
+        # separation = 3 sin(1) - 2 along the face normals.
+        edge = ctypes.c_int(0)
+        s = getattr(lib, pre + "find_max_separation")(E.ptr(box), E.ptr(_xf(lib, pre, 2, 0, 1)), E.ptr(box),
+                                                      E.ptr(_xf(lib, pre, -1, 0, 1)), ctypes.byref(edge))
+        assert abs(s - (3 * np.sin(1.0) - 2)) < 1e-5
+        # overlap variant: centres 2 apart -> 2 sin(1) - 2 < 0
+        s2 = getattr(lib, pre + "find_max_separation")(E.ptr(box), E.ptr(_xf(lib, pre, 1, 0, 1)), E.ptr(box),
+                                                       E.ptr(_xf(lib, pre, -1, 0, 1)), ctypes.byref(edge))
+        assert abs(s2 - (2 * np.sin(1.0) - 2)) < 1e-5
+        # Testb2FindIncidentEdge: axis-aligned boxes touching at x = 0: the
+        # incident edge of box 2 is its right face x = 0, from y = -1 to 1.
+        s3 = getattr(lib, pre + "find_max_separation")(E.ptr(box), E.ptr(_xf(lib, pre, 1, 0, 0)), E.ptr(box),
+                                                       E.ptr(_xf(lib, pre, -1, 0, 0)), ctypes.byref(edge))
+        assert s3 == 0.0 and edge.value == 3
+        out = np.zeros(4, np.float32)
+        ids = np.zeros(2, np.uint32)
+        getattr(lib, pre + "find_incident_edge")(E.ptr(box), E.ptr(_xf(lib, pre, 1, 0, 0)), edge.value, E.ptr(box),
+                                                 E.ptr(_xf(lib, pre, -1, 0, 0)), E.ptr(out), E.ptr(ids))
+        assert np.allclose(out, [0, -1, 0, 1]) or np.allclose(out, [0, 1, 0, -1])
+        # Testb2CollidePolygonsTouching: two points, normal along the face.
+        getattr(lib, pre + "collide")(E.ptr(box), E.ptr(_xf(lib, pre, 1, 0, 1)), E.ptr(box),
+                                      E.ptr(_xf(lib, pre, -1, 0, 1)), E.ptr(m))
+        assert m[0]["count"] == 2
+    # and both checkers agree bit for bit on all of them (compared in the corpus tests too)
+
+
+def test_box_on_box_closed_form(orclib):
+    """Box resting on a box: two points at the clipped corners, face normal
+    (0, 1); the side planes are pushed out by the two skins (0.02), so the
+    corner beyond the reference face clips at x = 0.5 + 0.02."""
+    box = np.array([E.box(orclib, "orc_", 0.5, 0.5)], E.SHAPE)
+    xa = np.array([E.transform(0, 0, 0)], E.XF)
+    xb = np.array([E.transform(0.25, 0.995, 0)], E.XF)
+    m = np.zeros(1, E.MANIFOLD)
+    orclib.orc_collide(E.ptr(box), E.ptr(xa), E.ptr(box), E.ptr(xb), E.ptr(m))
+    assert m[0]["count"] == 2 and m[0]["type"] in (1, 2)
+    wm = np.zeros(8, np.float32)
+    ra = np.float32(0.01)
+    orclib.orc_world_manifold(E.ptr(m), E.ptr(xa), F(ra), E.ptr(xb), F(ra), E.ptr(wm))
+    assert np.allclose(wm[:2], [0, 1], atol=1e-6)
+    xs = sorted([wm[2], wm[4]])
+    assert abs(xs[0] - (-0.25)) < 1e-5 and abs(xs[1] - 0.52) < 1e-5
+    assert np.allclose(wm[6:8], 0.995 - 1.0 - 0.02, atol=1e-6)
+
+
+# ------------------------------------------------------------ shapes
+def test_polygon_hull_and_mass(reflib, orclib):
+    rng = np.random.default_rng(5)
+    for i in range(300):
+        k = int(rng.integers(3, 9))
+        pts = rng.uniform(-2, 2, (k, 2)).astype(np.float32)
+        if i % 5 == 0:
+            pts[1] = pts[0] + 1e-3  # welded duplicate
+        a = E.polygon_from_points(reflib, "ref_", pts)
+        b = E.polygon_from_points(orclib, "orc_", pts)
+        assert a.tobytes() == b.tobytes(), i
+        ma, mb = np.zeros(4, np.float32), np.zeros(4, np.float32)
+        sa = np.array([a], E.SHAPE)
+        reflib.ref_shape_mass(E.ptr(sa), F(1.3), E.ptr(ma))
+        orclib.orc_shape_mass(E.ptr(sa), F(1.3), E.ptr(mb))
+        assert ma.tobytes() == mb.tobytes(), i
+    for (hx, hy, cx, cy, ang, rot) in ((0.5, 0.5, 0, 0, 0, 0), (2, 0.3, 0.4, -0.2, 0.7, 1), (1, 1, 0, 0, 1.2, 1)):
+        a = E.box(reflib, "ref_", hx, hy, cx, cy, ang, rot)
+        b = E.box(orclib, "orc_", hx, hy, cx, cy, ang, rot)
+        assert a.tobytes() == b.tobytes()
+        mb = np.zeros(4, np.float32)
+        orclib.orc_shape_mass(E.ptr(np.array([b], E.SHAPE)), F(2.0), E.ptr(mb))
+        mass = 2.0 * 4 * hx * hy
+        inertia_c = mass * (hx * hx + hy * hy) / 3.0
+        assert abs(mb[0] - mass) < 1e-5 * mass
+        assert abs(mb[3] - (inertia_c + mass * (cx * cx + cy * cy))) < 1e-4 * mb[3]
+    # circle mass: pi r^2 rho, I = m (r^2 / 2 + |p|^2)
+    c = np.array([E.circle(0.7, 0.1, -0.2)], E.SHAPE)
+    mc = np.zeros(4, np.float32)
+    orclib.orc_shape_mass(E.ptr(c), F(1.0), E.ptr(mc))
+    m = np.pi * 0.49
+    assert abs(mc[0] - m) < 1e-6 and abs(mc[3] - m * (0.245 + 0.05)) < 1e-6
+
+
+# ------------------------------------------------------------ broad-phase
+def _ref_pairs(reflib, bp, cap=1 << 20):
+    out = np.zeros(2 * cap, np.int32)
+    n = reflib.ref_bp_update_pairs(ctypes.c_void_p(bp), E.ptr(out), cap)
+    return out[:2 * n].reshape(-1, 2)
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_update_pairs_vs_bruteforce(reflib, orclib, seed):
+    """b2BroadPhase::UpdatePairs (tree queries) equals the O(n^2) pair set on
+    the same fat AABBs and move set, in the same sorted order."""
+    rng = np.random.default_rng(seed)
+    n = 600
+    lo = rng.uniform(-40, 40, (n, 2)).astype(np.float32)
+    size = rng.uniform(0.2, 3.0, (n, 2)).astype(np.float32)
+    size[::97] *= 20  # a few wide proxies
+    tight = np.zeros(n, E.AABB)
+    tight["lx"], tight["ly"] = lo[:, 0], lo[:, 1]
+    tight["ux"], tight["uy"] = lo[:, 0] + size[:, 0], lo[:, 1] + size[:, 1]
+    bp = reflib.ref_bp_create()
+    for i in range(n):
+        reflib.ref_bp_create_proxy(ctypes.c_void_p(bp), E.ptr(tight[i:i + 1]), i)
+    ids = np.arange(n)  # proxy k gets tree node 0, 1, 3, 5, ...: order preserved, mapped by user index
+    got = _ref_pairs(reflib, bp)
+    fat = np.zeros(n, E.AABB)
+    node = np.where(ids == 0, 0, 2 * ids - 1)
+    for i in range(n):
+        reflib.ref_bp_get_fat(ctypes.c_void_p(bp), int(node[i]), E.ptr(fat[i:i + 1]))
+    moved = np.ones(n, np.uint8)
+    want = np.zeros(2 * (1 << 20), np.int32)
+    cnt = orclib.orc_pairs_bruteforce(E.ptr(fat), E.ptr(moved), n, E.ptr(want), 1 << 20)
+    assert np.array_equal(got, want[:2 * cnt].reshape(-1, 2))
+    # second round: move a subset (fat-box rule decides which re-enter the move buffer)
+    moved = np.zeros(n, np.uint8)
+    for i in rng.choice(n, 150, replace=False):
+        d = rng.uniform(-0.4, 0.4, 2).astype(np.float32)
+        nt = tight[i:i + 1].copy()
+        nt["lx"] += d[0]; nt["ux"] += d[0]; nt["ly"] += d[1]; nt["uy"] += d[1]
+        fat_before = fat[i:i + 1].copy()
+        exp = np.zeros(1, E.AABB)
+        em = orclib.orc_fat_rule(E.ptr(fat_before), E.ptr(nt), F(d[0]), F(d[1]), E.ptr(exp))
+        m = reflib.ref_bp_move_proxy(ctypes.c_void_p(bp), int(node[i]), E.ptr(nt), F(d[0]), F(d[1]))
+        reflib.ref_bp_get_fat(ctypes.c_void_p(bp), int(node[i]), E.ptr(fat[i:i + 1]))
+        assert m == em and fat[i:i + 1].tobytes() == exp.tobytes()
+        moved[i] = max(moved[i], m)
+    got = _ref_pairs(reflib, bp)
+    cnt = orclib.orc_pairs_bruteforce(E.ptr(fat), E.ptr(moved), n, E.ptr(want), 1 << 20)
+    assert np.array_equal(got, want[:2 * cnt].reshape(-1, 2))
+    reflib.ref_bp_destroy(ctypes.c_void_p(bp))
+
+
+def test_trig_modes(orclib):
+    """Mode 0 is glibc sinf/cosf; mode 1 the rounded double sin/cos; they agree
+    to within one ulp and differ for about 1% of angles."""
+    a = np.linspace(-60, 60, 200001).astype(np.float32)
+    s0, c0 = np.zeros_like(a), np.zeros_like(a)
+    orclib.orc_sincos(E.ptr(a), len(a), E.ptr(s0), E.ptr(c0))
+    s1 = np.sin(a.astype(np.float64)).astype(np.float32)
+    c1 = np.cos(a.astype(np.float64)).astype(np.float32)
+    ulp = lambda x, y: np.abs(x.view(np.int32).astype(np.int64) - y.view(np.int32).astype(np.int64))
+    assert ulp(s0, s1).max() <= 1 and ulp(c0, c1).max() <= 1
+    frac = (ulp(s0, s1) > 0).mean()
+    assert 0.0 < frac < 0.05
