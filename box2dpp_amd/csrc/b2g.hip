@@ -61,7 +61,9 @@ struct Counters {
     int beginEvents;
     int endEvents;
     int points;
-    int pad[3];
+    int nEdges;
+    int nIslands;
+    int pad;
     int colourCount[kColourSlots + 3];
     int minSep[kMaxPosIters];
 };
@@ -87,6 +89,7 @@ struct ConView {
     float4* lnp;      // localNormal, localPoint
     float4* lc;       // localCenterA, localCenterB
     float2* rad;      // radiusA radiusB
+    int* island;      // island root of the constraint's non-static bodies
 };
 
 __device__ inline int encodeOrdered(float f) {
@@ -219,34 +222,123 @@ __global__ void k_collide(WorldView w, Counters* cnt) {
 // awake non-static body (the contacts b2World::Solve's island search adds,
 // REF/Dynamics/b2World.cpp:451-475). Contacts leaving the set give their
 // colour back to both bodies' masks.
-__global__ void k_mark(WorldView w, int* keep, Counters* cnt) {
+__device__ inline void releaseColour(const WorldView& w, int c, int bA, int bB) {
+    const int col = w.ccolour[c];
+    if (col < 0) return;
+    if (col < kMaxColours) {
+        const unsigned long long bit = 1ull << col;
+        if (w.btype[bA] == BT_DYNAMIC) atomicAnd(&w.colourMask[bA], ~bit);
+        if (w.btype[bB] == BT_DYNAMIC) atomicAnd(&w.colourMask[bB], ~bit);
+    }
+    w.ccolour[c] = -1;
+}
+
+// Destroyed contacts give their colour back and leave the table.
+__global__ void k_mark_dead(WorldView w, int* keep) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    const int flags = w.cflags[c];
+    if (flags & CF_DEAD) releaseColour(w, c, w.fbody[w.cfA[c]], w.fbody[w.cfB[c]]);
+    keep[c] = (flags & CF_DEAD) ? 0 : 1;
+}
+
+// A contact joins an island when it is touching, enabled and not a sensor
+// (the contact filter of b2World::Solve's search, REF/Dynamics/b2World.cpp:451-469).
+__device__ inline bool islandContact(const WorldView& w, int c) {
+    const int f = w.cflags[c];
+    return (f & CF_TOUCHING) && (f & CF_ENABLED) && !(f & CF_DEAD) && !sensorOf(w, w.cfA[c]) &&
+           !sensorOf(w, w.cfB[c]);
+}
+
+// ---------------------------------------------------------------- islands
+// b2World::Solve grows islands by depth-first search from every awake,
+// active, non-static body through island contacts, without crossing static
+// bodies, and wakes every body it reaches (REF/Dynamics/b2World.cpp:408-519).
+// Here the same components come from a lock-free union-find over the island
+// contacts: roots are hooked under the smaller index, so each component's
+// root is its smallest body index whatever the thread schedule.
+__device__ inline int ufFind(int* parent, int x) {
+    for (;;) {
+        const int p = parent[x];
+        if (p == x) return x;
+        const int gp = parent[p];
+        if (gp == p) return p;
+        parent[x] = gp;  // path halving: parents only move toward the root
+        x = gp;
+    }
+}
+
+__device__ inline void ufUnion(int* parent, int a, int b) {
+    for (;;) {
+        a = ufFind(parent, a);
+        b = ufFind(parent, b);
+        if (a == b) return;
+        if (a > b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        const int old = atomicCAS(&parent[b], b, a);
+        if (old == b) return;
+    }
+}
+
+__global__ void k_island_init(WorldView w) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies) return;
+    w.islandOf[b] = b;
+    w.islandAwake[b] = 0;
+    w.islandSleep[b] = 0;
+}
+
+__global__ void k_island_union(WorldView w) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts || !islandContact(w, c)) return;
+    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
+    if (w.btype[bA] == BT_STATIC || w.btype[bB] == BT_STATIC) return;
+    if (!(w.bflags[bA] & BF_ACTIVE) || !(w.bflags[bB] & BF_ACTIVE)) return;
+    ufUnion(w.islandOf, bA, bB);
+}
+
+__global__ void k_island_roots(WorldView w) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies) return;
+    const int r = ufFind(w.islandOf, b);
+    if (w.btype[b] != BT_STATIC && (w.bflags[b] & BF_ACTIVE) && (w.bflags[b] & BF_AWAKE)) w.islandAwake[r] = 1;
+}
+
+__global__ void k_island_compress(WorldView w) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies) return;
+    w.islandOf[b] = ufFind(w.islandOf, b);
+}
+
+// Bodies of awake islands are woken and take part in this step's solve.
+__global__ void k_island_mark(WorldView w) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies) return;
+    int flags = w.bflags[b] & ~BF_ISLAND;
+    if (w.btype[b] != BT_STATIC && (flags & BF_ACTIVE) && w.islandAwake[w.islandOf[b]]) {
+        if ((flags & BF_AWAKE) == 0) w.gsleep[b].y = 0.0f;
+        flags |= BF_AWAKE | BF_ISLAND;
+    }
+    w.bflags[b] = flags;
+}
+
+// Constraint set of the step: island contacts of awake islands. Contacts
+// leaving the set give their colour back.
+__global__ void k_mark_constraints(WorldView w, Counters* cnt) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= w.nContacts) return;
     int flags = w.cflags[c] & ~CF_CONSTRAINT;
-    const int fA = w.cfA[c], fB = w.cfB[c];
-    const int bA = w.fbody[fA], bB = w.fbody[fB];
-    bool isCons = false;
-    if ((flags & CF_DEAD) == 0 && (flags & CF_TOUCHING) && (flags & CF_ENABLED) && !sensorOf(w, fA) &&
-        !sensorOf(w, fB)) {
-        const bool liveA = (w.bflags[bA] & BF_AWAKE) && w.btype[bA] != BT_STATIC;
-        const bool liveB = (w.bflags[bB] & BF_AWAKE) && w.btype[bB] != BT_STATIC;
-        isCons = liveA || liveB;
-    }
-    const int col = w.ccolour[c];
-    if (col >= 0 && !isCons) {
-        if (col < kMaxColours) {
-            const unsigned long long bit = 1ull << col;
-            if (w.btype[bA] == BT_DYNAMIC) atomicAnd(&w.colourMask[bA], ~bit);
-            if (w.btype[bB] == BT_DYNAMIC) atomicAnd(&w.colourMask[bB], ~bit);
-        }
-        w.ccolour[c] = -1;
-    }
+    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
+    const bool isCons = islandContact(w, c) && ((w.bflags[bA] & BF_ISLAND) || (w.bflags[bB] & BF_ISLAND));
+    if (!isCons) releaseColour(w, c, bA, bB);
     if (isCons) {
         flags |= CF_CONSTRAINT;
         if (w.ccolour[c] < 0) atomicAdd(&cnt->remaining, 1);
     }
     w.cflags[c] = flags;
-    keep[c] = (flags & CF_DEAD) ? 0 : 1;
 }
 
 struct ContactArrays {
@@ -381,13 +473,9 @@ __global__ void k_colour_hist(const uint32_t* keys, int n, Counters* cnt) {
 // ====================================================================
 // Island solve stages (b2Island::Solve, REF/Dynamics/b2Island.cpp:182-383)
 // ====================================================================
-__device__ inline bool islandBody(const WorldView& w, int b) {
-    return w.btype[b] != BT_STATIC && (w.bflags[b] & BF_AWAKE) && (w.bflags[b] & BF_ACTIVE);
-}
+__device__ inline bool islandBody(const WorldView& w, int b) { return (w.bflags[b] & BF_ISLAND) != 0; }
 
-__global__ void k_integrate_velocities(WorldView w, V2 gravity, float h) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= w.nBodies || !islandBody(w, b)) return;
+__device__ inline void integrateVelocityOne(const WorldView& w, int b, V2 gravity, float h) {
     const float4 p = w.pos[b];
     w.pos0[b] = make_float4(p.x, p.y, p.z, w.pos0[b].w);
     if (w.btype[b] != BT_DYNAMIC) return;
@@ -404,13 +492,16 @@ __global__ void k_integrate_velocities(WorldView w, V2 gravity, float h) {
     w.vel[b] = make_float4(v.x, v.y, wv, 0.0f);
 }
 
+__global__ void k_integrate_velocities(WorldView w, V2 gravity, float h) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies || !islandBody(w, b)) return;
+    integrateVelocityOne(w, b, gravity, h);
+}
+
 // b2ContactSolver ctor + InitializeVelocityConstraints,
 // REF/Dynamics/Contacts/b2ContactSolver.cpp:52-252.
-__global__ void k_init_constraints(WorldView w, ConView cv, const int* order, int n, float dtRatio, int warm,
-                                   Counters* cnt) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int c = order[k];
+__device__ inline void initConstraintOne(const WorldView& w, const ConView& cv, int k, int c, float dtRatio, int warm,
+                                         Counters* cnt) {
     const int fA = w.cfA[c], fB = w.cfB[c];
     const int bA = w.fbody[fA], bB = w.fbody[fB];
     const b2g_manifold& m = w.cman[c];
@@ -495,12 +586,18 @@ __global__ void k_init_constraints(WorldView w, ConView cv, const int* order, in
     cv.lnp[k] = make_float4(m.localNormal.x, m.localNormal.y, m.localPoint.x, m.localPoint.y);
     cv.lc[k] = make_float4(lcA.x, lcA.y, lcB.x, lcB.y);
     cv.rad[k] = make_float2(radiusA, radiusB);
+    cv.island[k] = w.islandOf[w.btype[bA] != BT_STATIC ? bA : bB];
+}
+
+__global__ void k_init_constraints(WorldView w, ConView cv, const int* order, int n, float dtRatio, int warm,
+                                   Counters* cnt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    initConstraintOne(w, cv, k, order[k], dtRatio, warm, cnt);
 }
 
 // b2ContactSolver::WarmStart, REF/Dynamics/Contacts/b2ContactSolver.cpp:254-292.
-__global__ void k_warm_start(WorldView w, ConView cv, int begin, int end) {
-    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= end) return;
+__device__ inline void warmStartOne(const WorldView& w, const ConView& cv, int k) {
     const int2 bi = cv.bodies[k];
     const float4 ms = cv.mass[k];
     const int info = cv.info[k];
@@ -527,12 +624,16 @@ __global__ void k_warm_start(WorldView w, ConView cv, int begin, int end) {
     if (info & 32) w.vel[bi.y] = make_float4(vB.x, vB.y, wB, 0.0f);
 }
 
+__global__ void k_warm_start(WorldView w, ConView cv, int begin, int end) {
+    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    warmStartOne(w, cv, k);
+}
+
 // b2ContactSolver::SolveVelocityConstraints, REF/Dynamics/Contacts/b2ContactSolver.cpp:294-597:
 // friction per point clamped to the friction cone, then the normal impulse,
 // by a 2x2 linear complementarity solve (total enumeration) for two points.
-__global__ void k_solve_velocity(WorldView w, ConView cv, int begin, int end) {
-    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= end) return;
+__device__ inline void solveVelocityOne(const WorldView& w, const ConView& cv, int k) {
     const int2 bi = cv.bodies[k];
     const float4 ms = cv.mass[k];
     const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
@@ -651,10 +752,14 @@ __global__ void k_solve_velocity(WorldView w, ConView cv, int begin, int end) {
     if (info & 32) w.vel[bi.y] = make_float4(vB.x, vB.y, wB, 0.0f);
 }
 
+__global__ void k_solve_velocity(WorldView w, ConView cv, int begin, int end) {
+    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    solveVelocityOne(w, cv, k);
+}
+
 // b2ContactSolver::StoreImpulses, REF/Dynamics/Contacts/b2ContactSolver.cpp:599-612.
-__global__ void k_store_impulses(WorldView w, ConView cv, int n) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+__device__ inline void storeImpulsesOne(const WorldView& w, const ConView& cv, int k) {
     const int c = cv.contact[k];
     const int vpc = cv.info[k] & 3;
     const float4 imp = cv.imp[k];
@@ -666,11 +771,15 @@ __global__ void k_store_impulses(WorldView w, ConView cv, int n) {
     }
 }
 
+__global__ void k_store_impulses(WorldView w, ConView cv, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    storeImpulsesOne(w, cv, k);
+}
+
 // Position integration with translation and rotation clamps,
 // REF/Dynamics/b2Island.cpp:276-306.
-__global__ void k_integrate_positions(WorldView w, float h) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= w.nBodies || !islandBody(w, b)) return;
+__device__ inline void integratePositionOne(const WorldView& w, int b, float h) {
     float4 p = w.pos[b];
     const float4 vv = w.vel[b];
     V2 c = mk(p.x, p.y);
@@ -693,13 +802,16 @@ __global__ void k_integrate_positions(WorldView w, float h) {
     w.vel[b] = make_float4(v.x, v.y, wv, 0.0f);
 }
 
+__global__ void k_integrate_positions(WorldView w, float h) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies || !islandBody(w, b)) return;
+    integratePositionOne(w, b, h);
+}
+
 // b2ContactSolver::SolvePositionConstraints with b2PositionSolverManifold,
 // REF/Dynamics/Contacts/b2ContactSolver.cpp:614-744. Skips itself when the
 // previous iteration already met the early-exit test of b2Island::Solve.
-__global__ void k_solve_position(WorldView w, ConView cv, int begin, int end, Counters* cnt, int it) {
-    if (it > 0 && decodeOrdered(cnt->minSep[it - 1]) >= -3.0f * kLinearSlop) return;
-    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= end) return;
+__device__ inline float solvePositionOne(const WorldView& w, const ConView& cv, int k) {
     const int2 bi = cv.bodies[k];
     const float4 ms = cv.mass[k];
     const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
@@ -760,21 +872,89 @@ __global__ void k_solve_position(WorldView w, ConView cv, int begin, int end, Co
     }
     if (info & 16) w.pos[bi.x] = make_float4(cA.x, cA.y, aA, 0.0f);
     if (info & 32) w.pos[bi.y] = make_float4(cB.x, cB.y, aB, 0.0f);
-    if (minSep < 0.0f) atomicMin(&cnt->minSep[it], encodeOrdered(minSep));
+    return minSep;
+}
+
+// Early exit per island (b2Island::Solve, REF/Dynamics/b2Island.cpp:310-328):
+// a pass leaves an island solved when none of its points is deeper than
+// 3 * linearSlop, so per island and pass one "failed" flag suffices. Lanes
+// set it only while it is still clear, which keeps the writes to a large
+// island's flag few.
+__global__ void k_solve_position(WorldView w, ConView cv, int begin, int end, Counters* cnt, int it) {
+    __shared__ int blockMin;
+    if (threadIdx.x == 0) blockMin = 0;
+    __syncthreads();
+    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < end) {
+        const int root = cv.island[k];
+        const bool done = it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0;
+        if (!done) {
+            const float minSep = solvePositionOne(w, cv, k);
+            if (minSep < -3.0f * kLinearSlop) {
+                int* f = &w.islandMinSep[it * w.nBodies + root];
+                if (*f == 0) *f = 1;
+            }
+            if (minSep < 0.0f) atomicMin(&blockMin, encodeOrdered(minSep));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && blockMin < 0) atomicMin(&cnt->minSep[it], blockMin);
 }
 
 // Copy-back transform of every solved body (b2Body::SynchronizeTransform,
 // REF/Dynamics/b2Body.h:836-840). Static bodies keep their creation-time
 // transform, which the reference's recomputation reproduces exactly.
-__global__ void k_sync_transforms(WorldView w) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= w.nBodies || !islandBody(w, b)) return;
+__device__ inline void syncTransformOne(const WorldView& w, int b) {
     const float4 p = w.pos[b];
     const float2 lc = w.lcenter[b];
     Xf xf;
     xf.q = rotOf(p.z);
     xf.p = mk(p.x, p.y) - mul(xf.q, mk(lc.x, lc.y));
     w.xf[b] = f4(xf);
+}
+
+__global__ void k_sync_transforms(WorldView w) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies || !islandBody(w, b)) return;
+    syncTransformOne(w, b);
+}
+
+// Sleep (b2Island::Solve, REF/Dynamics/b2Island.cpp:345-382): per island,
+// the minimum over its bodies of the time spent below the sleep tolerances;
+// an island that has rested for b2_timeToSleep and whose positions were
+// solved goes to sleep as a whole.
+__global__ void k_sleep_accum(WorldView w, float h) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies || !islandBody(w, b)) return;
+    const float4 v = w.vel[b];
+    const float linTolSqr = kLinearSleepTol * kLinearSleepTol;
+    const float angTolSqr = kAngularSleepTol * kAngularSleepTol;
+    float t;
+    if ((w.bflags[b] & BF_AUTOSLEEP) == 0 || v.z * v.z > angTolSqr || dot(mk(v.x, v.y), mk(v.x, v.y)) > linTolSqr) {
+        t = 0.0f;
+    } else {
+        t = w.gsleep[b].y + h;
+    }
+    w.gsleep[b].y = t;
+    if (t < kTimeToSleep) {
+        int* f = &w.islandSleep[w.islandOf[b]];
+        if (*f == 0) *f = 1;  // an island body has not rested long enough
+    }
+}
+
+__global__ void k_sleep_apply(WorldView w, int posIters) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies || !islandBody(w, b)) return;
+    const int root = w.islandOf[b];
+    bool solved = false;
+    for (int it = 0; it < posIters; ++it)
+        if (w.islandMinSep[it * w.nBodies + root] == 0) solved = true;
+    if (!(w.islandSleep[root] == 0 && solved)) return;
+    // b2Body::SetAwake(false)
+    w.bflags[b] &= ~BF_AWAKE;
+    w.gsleep[b].y = 0.0f;
+    w.vel[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    w.force[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 __device__ inline float4 shapeAABB(const b2g_shape& s, const Xf& xf) {
@@ -1019,6 +1199,278 @@ __global__ void k_flag_filter(WorldView w, const uint8_t* mark) {
 }
 
 // ====================================================================
+// Reference-order mode: Box2D's own visiting order, for parity
+// (b2ContactManager::Collide over the newest-first contact list, islands by
+// depth-first search from the newest-first body list, constraints solved
+// sequentially in island order, REF/Dynamics/b2World.cpp:388-577,
+// REF/Dynamics/b2Island.cpp:182-383). The contact table holds contacts in
+// creation order (new pairs are appended in pair-key order and compaction
+// keeps order), so "newest first" is descending table index.
+// ====================================================================
+struct RefView {
+    b2g_manifold* tmpMan;   // narrowphase result per contact
+    uint8_t* overlap;       // fat AABBs overlap
+    int* edgeCount;         // per body (+1)
+    int* edgeStart;         // exclusive scan of edgeCount
+    int* edgeCursor;
+    unsigned long long* edgeKeys;
+    unsigned long long* edgeSorted;
+    int* edgeContact;
+    int* bvisited;
+    int* cvisited;
+    int* stack;
+    int* islandBodies;
+    int* islandContacts;
+    int4* islandRange;      // bodyStart bodyCount contactStart contactCount
+    int* islandSlept;       // per island: went to sleep this step
+    int* lastIsland;        // per static body: last island (in search order) holding it
+};
+
+__global__ void k_collide_compute(WorldView w, RefView r) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    const int fA = w.cfA[c], fB = w.cfB[c];
+    const int bA = w.fbody[fA], bB = w.fbody[fB];
+    const bool ov = overlaps(w.fat[fA], w.fat[fB]);
+    r.overlap[c] = ov ? 1 : 0;
+    b2g_manifold m;
+    zeroManifold(m);
+    if (ov) collideShapes(m, w.shape[fA], xfOf(w.xf[bA]), w.shape[fB], xfOf(w.xf[bB]));
+    r.tmpMan[c] = m;
+}
+
+// Sequential commit in the reference's list order: activity is read as
+// earlier contacts' wake-ups leave it.
+__global__ void k_collide_serial(WorldView w, RefView r, Counters* cnt) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int c = w.nContacts - 1; c >= 0; --c) {
+        int flags = w.cflags[c];
+        const int fA = w.cfA[c], fB = w.cfB[c];
+        const int bA = w.fbody[fA], bB = w.fbody[fB];
+        const bool destroyWake = w.cman[c].pointCount > 0 && !sensorOf(w, fA) && !sensorOf(w, fB);
+        if (flags & CF_FILTER) {
+            const bool dynAny = w.btype[bA] == BT_DYNAMIC || w.btype[bB] == BT_DYNAMIC;
+            if (!dynAny || !filterPass(w, fA, fB)) {
+                if (flags & CF_TOUCHING) cnt->endEvents++;
+                if (destroyWake) { wakeBody(w, bA); wakeBody(w, bB); }
+                w.cflags[c] = flags | CF_DEAD;
+                continue;
+            }
+            flags &= ~CF_FILTER;
+        }
+        const bool activeA = (w.bflags[bA] & BF_AWAKE) && w.btype[bA] != BT_STATIC;
+        const bool activeB = (w.bflags[bB] & BF_AWAKE) && w.btype[bB] != BT_STATIC;
+        if (!activeA && !activeB) {
+            w.cflags[c] = flags;
+            continue;
+        }
+        if (!r.overlap[c]) {
+            if (flags & CF_TOUCHING) cnt->endEvents++;
+            if (destroyWake) { wakeBody(w, bA); wakeBody(w, bB); }
+            w.cflags[c] = flags | CF_DEAD;
+            continue;
+        }
+        const b2g_manifold old = w.cman[c];
+        b2g_manifold m = r.tmpMan[c];
+        const bool wasTouching = (flags & CF_TOUCHING) != 0;
+        const bool touching = m.pointCount > 0;
+        for (int i = 0; i < m.pointCount; ++i) {
+            for (int j = 0; j < old.pointCount; ++j) {
+                if (old.points[j].id == m.points[i].id) {
+                    m.points[i].normalImpulse = old.points[j].normalImpulse;
+                    m.points[i].tangentImpulse = old.points[j].tangentImpulse;
+                    break;
+                }
+            }
+        }
+        if (touching != wasTouching) { wakeBody(w, bA); wakeBody(w, bB); }
+        flags |= CF_ENABLED;
+        if (touching) flags |= CF_TOUCHING;
+        else flags &= ~CF_TOUCHING;
+        if (!wasTouching && touching) cnt->beginEvents++;
+        if (wasTouching && !touching) cnt->endEvents++;
+        w.cman[c] = m;
+        w.cflags[c] = flags;
+    }
+}
+
+__global__ void k_clear_island_flags(WorldView w, RefView r) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies) return;
+    w.bflags[b] &= ~BF_ISLAND;
+    r.bvisited[b] = 0;
+    r.edgeCount[b] = 0;
+    r.edgeCursor[b] = 0;
+    if (b == 0) r.edgeCount[w.nBodies] = 0;
+}
+
+__global__ void k_edge_count(WorldView w, RefView r, Counters* cnt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    r.cvisited[c] = 0;
+    if (!islandContact(w, c)) return;
+    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
+    if (w.btype[bA] != BT_STATIC) { atomicAdd(&r.edgeCount[bA], 1); atomicAdd(&cnt->nEdges, 1); }
+    if (w.btype[bB] != BT_STATIC) { atomicAdd(&r.edgeCount[bB], 1); atomicAdd(&cnt->nEdges, 1); }
+}
+
+__global__ void k_edge_fill(WorldView w, RefView r) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts || !islandContact(w, c)) return;
+    const int bs[2] = {w.fbody[w.cfA[c]], w.fbody[w.cfB[c]]};
+    for (int k = 0; k < 2; ++k) {
+        const int b = bs[k];
+        if (w.btype[b] == BT_STATIC) continue;
+        const int slot = r.edgeStart[b] + atomicAdd(&r.edgeCursor[b], 1);
+        r.edgeKeys[slot] = ((unsigned long long)(unsigned)b << 32) | (0xFFFFFFFFu - (unsigned)c);
+    }
+}
+
+__global__ void k_edge_extract(RefView r, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    r.edgeContact[i] = (int)(0xFFFFFFFFu - (unsigned)(r.edgeSorted[i] & 0xFFFFFFFFull));
+}
+
+// Island search of b2World::Solve, one thread.
+__global__ void k_dfs(WorldView w, RefView r, Counters* cnt) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int nI = 0, nb = 0, nc = 0;
+    for (int seed = w.nBodies - 1; seed >= 0; --seed) {
+        if (r.bvisited[seed]) continue;
+        const int sf = w.bflags[seed];
+        if (!(sf & BF_AWAKE) || !(sf & BF_ACTIVE)) continue;
+        if (w.btype[seed] == BT_STATIC) continue;
+        const int bStart = nb, cStart = nc;
+        int sp = 0;
+        r.stack[sp++] = seed;
+        r.bvisited[seed] = 1;
+        while (sp > 0) {
+            const int b = r.stack[--sp];
+            r.islandBodies[nb++] = b;
+            if ((w.bflags[b] & BF_AWAKE) == 0) {
+                w.bflags[b] |= BF_AWAKE;
+                w.gsleep[b].y = 0.0f;
+            }
+            if (w.btype[b] == BT_STATIC) continue;
+            w.bflags[b] |= BF_ISLAND;
+            for (int e = r.edgeStart[b]; e < r.edgeStart[b + 1]; ++e) {
+                const int c = r.edgeContact[e];
+                if (r.cvisited[c]) continue;
+                r.islandContacts[nc++] = c;
+                r.cvisited[c] = 1;
+                const int bA = w.fbody[w.cfA[c]];
+                const int other = bA == b ? w.fbody[w.cfB[c]] : bA;
+                if (r.bvisited[other]) continue;
+                r.stack[sp++] = other;
+                r.bvisited[other] = 1;
+            }
+        }
+        r.islandRange[nI++] = make_int4(bStart, nb - bStart, cStart, nc - cStart);
+        for (int i = bStart; i < nb; ++i)
+            if (w.btype[r.islandBodies[i]] == BT_STATIC) r.bvisited[r.islandBodies[i]] = 0;
+    }
+    cnt->nIslands = nI;
+    cnt->nConstraints = nc;
+}
+
+// One wavefront per island: parallel per-body and per-constraint set-up,
+// the sequential passes of b2ContactSolver on lane 0.
+__global__ void k_island_solve(WorldView w, ConView cv, RefView r, V2 gravity, float h, float dtRatio, int warm,
+                               int velIters, int posIters, int allowSleep, Counters* cnt) {
+    const int4 rg = r.islandRange[blockIdx.x];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int* bodies = r.islandBodies + rg.x;
+    const int k0 = rg.z, k1 = rg.z + rg.w;
+    for (int i = tid; i < rg.y; i += nt) integrateVelocityOne(w, bodies[i], gravity, h);
+    __syncthreads();
+    for (int k = k0 + tid; k < k1; k += nt) initConstraintOne(w, cv, k, r.islandContacts[k], dtRatio, warm, cnt);
+    __syncthreads();
+    if (tid == 0) {
+        if (warm)
+            for (int k = k0; k < k1; ++k) warmStartOne(w, cv, k);
+        for (int it = 0; it < velIters; ++it)
+            for (int k = k0; k < k1; ++k) solveVelocityOne(w, cv, k);
+    }
+    __syncthreads();
+    for (int k = k0 + tid; k < k1; k += nt) storeImpulsesOne(w, cv, k);
+    for (int i = tid; i < rg.y; i += nt)
+        if (w.btype[bodies[i]] != BT_STATIC) integratePositionOne(w, bodies[i], h);
+    __syncthreads();
+    __shared__ int solved;
+    __shared__ int minSleep;
+    if (tid == 0) {
+        solved = 0;
+        minSleep = encodeOrdered(FLT_MAX);
+        for (int it = 0; it < posIters; ++it) {
+            float minSep = 0.0f;
+            for (int k = k0; k < k1; ++k) minSep = fminB(minSep, solvePositionOne(w, cv, k));
+            if (minSep >= -3.0f * kLinearSlop) {
+                solved = 1;
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < rg.y; i += nt)
+        if (w.btype[bodies[i]] != BT_STATIC) syncTransformOne(w, bodies[i]);
+    if (tid == 0) r.islandSlept[blockIdx.x] = 0;
+    if (!allowSleep) return;
+    const float linTolSqr = kLinearSleepTol * kLinearSleepTol;
+    const float angTolSqr = kAngularSleepTol * kAngularSleepTol;
+    for (int i = tid; i < rg.y; i += nt) {
+        const int b = bodies[i];
+        if (w.btype[b] == BT_STATIC) continue;
+        const float4 v = w.vel[b];
+        float t;
+        if ((w.bflags[b] & BF_AUTOSLEEP) == 0 || v.z * v.z > angTolSqr ||
+            dot(mk(v.x, v.y), mk(v.x, v.y)) > linTolSqr) {
+            t = 0.0f;
+        } else {
+            t = w.gsleep[b].y + h;
+        }
+        w.gsleep[b].y = t;
+        atomicMin(&minSleep, encodeOrdered(t));
+    }
+    __syncthreads();
+    const bool sleeps = decodeOrdered(minSleep) >= kTimeToSleep && solved;
+    if (tid == 0) r.islandSlept[blockIdx.x] = sleeps ? 1 : 0;
+    if (sleeps) {
+        for (int i = tid; i < rg.y; i += nt) {
+            const int b = bodies[i];
+            if (w.btype[b] == BT_STATIC) continue;  // shared static bodies: see k_static_awake
+            w.bflags[b] &= ~BF_AWAKE;
+            w.gsleep[b].y = 0.0f;
+            w.vel[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            w.force[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+}
+
+// A static body joins every island that touches it; the reference wakes it
+// when each island is built and puts it to sleep with its island, one island
+// after another, so its final awake flag is that of the last island in
+// search order that holds it.
+__global__ void k_static_last_island(WorldView w, RefView r, int nIslands) {
+    const int isl = blockIdx.x;
+    if (isl >= nIslands) return;
+    const int4 rg = r.islandRange[isl];
+    for (int i = threadIdx.x; i < rg.y; i += blockDim.x) {
+        const int b = r.islandBodies[rg.x + i];
+        if (w.btype[b] == BT_STATIC) atomicMax(&r.lastIsland[b], isl);
+    }
+}
+
+__global__ void k_static_awake(WorldView w, RefView r) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies || w.btype[b] != BT_STATIC) return;
+    const int isl = r.lastIsland[b];
+    if (isl < 0) return;
+    if (r.islandSlept[isl]) w.bflags[b] &= ~BF_AWAKE;
+    r.lastIsland[b] = -1;
+}
+
+// ====================================================================
 // Batch kernels for per-function parity
 // ====================================================================
 __global__ void k_collide_batch(const b2g_shape* A, const b2g_transform* xa, const b2g_shape* B,
@@ -1110,6 +1562,10 @@ struct b2g_context {
     unsigned long long* colourMask = nullptr;
     unsigned long long* claim = nullptr;
     int* islandOf = nullptr;
+    int* islandAwake = nullptr;
+    int* islandSleep = nullptr;
+    int* islandMinSep = nullptr;
+    int islandIterCap = 0;
 
     // fixture arrays
     int* fbody = nullptr;
@@ -1160,6 +1616,10 @@ struct b2g_context {
     uint32_t claimTag = 0;
     b2g_step_stats stats{};
 
+    // reference-order mode buffers
+    RefView ref{};
+    int refCapBodies = 0, refCapContacts = 0;
+
     // profiling (HIP events around launches)
     bool profiling = false;
     std::vector<hipEvent_t> evPool;
@@ -1174,6 +1634,9 @@ struct b2g_context {
         w.btype = btype; w.bflags = bflags; w.xf = xf; w.pos = pos; w.pos0 = pos0; w.lcenter = lcenter;
         w.vel = vel; w.force = force; w.mass = mass; w.gsleep = gsleep; w.colourMask = colourMask;
         w.islandOf = islandOf;
+        w.islandAwake = islandAwake;
+        w.islandSleep = islandSleep;
+        w.islandMinSep = islandMinSep;
         w.nFixtures = nFixtures;
         w.fbody = fbody; w.shape = shape; w.fmat = fmat; w.ffilter = ffilter; w.fgroup = fgroup; w.fat = fat;
         w.moved = moved;
@@ -1245,6 +1708,10 @@ int growBodies(b2g_context* ctx, int need) {
     HIPCHK(ctx, growArray(ctx->colourMask, n, cap, s));
     HIPCHK(ctx, growArray(ctx->claim, n, cap, s));
     HIPCHK(ctx, growArray(ctx->islandOf, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->islandAwake, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->islandSleep, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->islandMinSep, 0, cap * std::max(ctx->islandIterCap, 4), s));
+    ctx->islandIterCap = std::max(ctx->islandIterCap, 4);
     // claims start at zero (tag 0 never wins)
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(cap - n)), dim3(256), 0, s, ctx->claim + n, cap - n,
                        0ull);
@@ -1343,6 +1810,7 @@ int growContacts(b2g_context* ctx, int need) {
     HIPCHK(ctx, growArray(c.lnp, 0, cap, s));
     HIPCHK(ctx, growArray(c.lc, 0, cap, s));
     HIPCHK(ctx, growArray(c.rad, 0, cap, s));
+    HIPCHK(ctx, growArray(c.island, 0, cap, s));
     int hc = 1024;
     while (hc < 2 * cap) hc <<= 1;
     freeArray(ctx->hashTable);
@@ -1379,6 +1847,15 @@ void chooseCellSize(b2g_context* ctx, const b2g_fixture_def* defs, int n, const 
     }
 }
 
+int rebuildHash(b2g_context* ctx);
+
+// Keeps a third of the contact capacity free ahead of the step.
+int reserveContacts(b2g_context* ctx) {
+    if (3 * (long long)ctx->nContacts > 2 * (long long)ctx->capContacts)
+        return growContacts(ctx, 2 * ctx->capContacts);
+    return B2G_OK;
+}
+
 // Runs the broad-phase over the current fat boxes and appends the new
 // contacts. Requires the hash of live contacts to be current.
 int runBroadphase(b2g_context* ctx) {
@@ -1402,18 +1879,27 @@ int runBroadphase(b2g_context* ctx) {
     LAUNCHCHK(ctx);
     HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->gCounts, ctx->gStarts, ctx->nBuckets + 1, s));
     hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(nf)), dim3(256), 0, s, nf, g);
-    PairSink sink{ctx->newKeys, ctx->capContacts};
-    WorldView w = ctx->view();
-    hipLaunchKernelGGL(k_pairs_query, dim3(blocks(nf)), dim3(256), 0, s, ctx->fat, ctx->moved, nf, g, w, true,
-                       ctx->hashTable, ctx->hashMask, sink, ctx->cnt);
-    hipLaunchKernelGGL(k_fill<uint8_t>, dim3(blocks(nf)), dim3(256), 0, s, ctx->moved, nf, (uint8_t)0);
-    LAUNCHCHK(ctx);
-    HIPCHK(ctx, hipMemcpyAsync(ctx->hcnt, ctx->cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipStreamSynchronize(s));
-    int nNew = ctx->hcnt->nNewPairs;
-    if (ctx->hcnt->overflow || ctx->nContacts + nNew > ctx->capContacts) {
-        return fail(ctx, B2G_ERR_CAPACITY, "contact capacity exceeded in the broad-phase");
+    // The query only reads the grid, the fat boxes and the move flags, so it
+    // is re-run after growing the contact arrays when its pairs do not fit.
+    int nNew = 0;
+    for (int attempt = 0;; ++attempt) {
+        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nNewPairs, 1, 0);
+        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->overflow, 1, 0);
+        PairSink sink{ctx->newKeys, ctx->capContacts};
+        hipLaunchKernelGGL(k_pairs_query, dim3(blocks(nf)), dim3(256), 0, s, ctx->fat, ctx->moved, nf, g, ctx->view(),
+                           true, ctx->hashTable, ctx->hashMask, sink, ctx->cnt);
+        LAUNCHCHK(ctx);
+        HIPCHK(ctx, hipMemcpyAsync(ctx->hcnt, ctx->cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        nNew = ctx->hcnt->nNewPairs;
+        if (!ctx->hcnt->overflow && ctx->nContacts + nNew <= ctx->capContacts) break;
+        if (attempt >= 2) return fail(ctx, B2G_ERR_CAPACITY, "contact capacity exceeded in the broad-phase");
+        int rc = growContacts(ctx, std::max(2 * ctx->capContacts, ctx->nContacts + nNew + 4096));
+        if (rc != B2G_OK) return rc;
+        if ((rc = rebuildHash(ctx)) != B2G_OK) return rc;
     }
+    hipLaunchKernelGGL(k_fill<uint8_t>, dim3(blocks(nf)), dim3(256), 0, s, ctx->moved, nf, (uint8_t)0);
+    WorldView w = ctx->view();
     if (nNew > 0) {
         HIPCHK(ctx, prims_sort_keys_u64(ctx->scratch, ctx->scratchBytes, ctx->newKeys, ctx->newKeysSorted, nNew, s));
         hipLaunchKernelGGL(k_append_contacts, dim3(blocks(nNew)), dim3(256), 0, s, w, ctx->newKeysSorted, nNew,
@@ -1445,11 +1931,11 @@ int readCounters(b2g_context* ctx) {
 // ---------------------------------------------------------------- profiling
 enum Phase {
     PH_COLLIDE, PH_COMPACT, PH_COLOUR, PH_ORDER, PH_INTEGRATE_V, PH_INIT, PH_WARM, PH_SOLVE_V, PH_STORE,
-    PH_INTEGRATE_X, PH_SOLVE_X, PH_SYNC, PH_BROADPHASE, PH_COUNT
+    PH_INTEGRATE_X, PH_SOLVE_X, PH_SYNC, PH_BROADPHASE, PH_ISLANDS, PH_COUNT
 };
 const char* kPhaseNames[PH_COUNT] = {"collide", "compact", "colour", "order", "integrate_velocities",
                                      "init_constraints", "warm_start", "solve_velocity", "store_impulses",
-                                     "integrate_positions", "solve_position", "sync", "broadphase"};
+                                     "integrate_positions", "solve_position", "sync", "broadphase", "islands"};
 
 int profStart(b2g_context* ctx) {
     if (!ctx->profiling) return -1;
@@ -1490,6 +1976,7 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
     const float inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
     const float dtRatio = ctx->inv_dt0 * dt;
     int rc;
+    if ((rc = reserveContacts(ctx)) != B2G_OK) return rc;
     memset(&ctx->stats, 0, sizeof(ctx->stats));
     // reset per-step counters
     Counters zero;
@@ -1513,7 +2000,7 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
         hipLaunchKernelGGL(k_collide, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), ctx->cnt);
         profEnd(ctx, PH_COLLIDE, pe);
         pe = profStart(ctx);
-        hipLaunchKernelGGL(k_mark, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), ctx->keep, ctx->cnt);
+        hipLaunchKernelGGL(k_mark_dead, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), ctx->keep);
         LAUNCHCHK(ctx);
         HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->keep, ctx->idx, ctx->nContacts, s));
         hipLaunchKernelGGL(k_compact, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->cur, ctx->alt, ctx->keep,
@@ -1529,7 +2016,32 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
     ctx->stats.beginEvents = ctx->hcnt->beginEvents;
     ctx->stats.endEvents = ctx->hcnt->endEvents;
     if (dt > 0.0f) {
+        const int posIters = std::min(p->positionIterations, kMaxPosIters);
+        if (posIters > ctx->islandIterCap) {
+            freeArray(ctx->islandMinSep);
+            HIPCHK(ctx, hipMalloc((void**)&ctx->islandMinSep, sizeof(int) * (size_t)ctx->capBodies * posIters));
+            ctx->islandIterCap = posIters;
+        }
         WorldView w = ctx->view();
+        // Islands of awake bodies and the constraint set.
+        {
+            const int pe = profStart(ctx);
+            const int nb = ctx->nBodies;
+            hipLaunchKernelGGL(k_fill<int>, dim3(blocks(nb * std::max(posIters, 1))), dim3(256), 0, s,
+                               ctx->islandMinSep, nb * std::max(posIters, 1), 0);
+            hipLaunchKernelGGL(k_island_init, dim3(blocks(nb)), dim3(256), 0, s, w);
+            if (ctx->nContacts > 0)
+                hipLaunchKernelGGL(k_island_union, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w);
+            hipLaunchKernelGGL(k_island_roots, dim3(blocks(nb)), dim3(256), 0, s, w);
+            hipLaunchKernelGGL(k_island_compress, dim3(blocks(nb)), dim3(256), 0, s, w);
+            hipLaunchKernelGGL(k_island_mark, dim3(blocks(nb)), dim3(256), 0, s, w);
+            hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->remaining, 1, 0);
+            if (ctx->nContacts > 0)
+                hipLaunchKernelGGL(k_mark_constraints, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->cnt);
+            profEnd(ctx, PH_ISLANDS, pe);
+            LAUNCHCHK(ctx);
+            if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+        }
         // Colour new constraints.
         int remaining = ctx->nContacts > 0 ? ctx->hcnt->remaining : 0;
         int rounds = 0;
@@ -1626,7 +2138,6 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
         hipLaunchKernelGGL(k_integrate_positions, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, dt);
         profEnd(ctx, PH_INTEGRATE_X, pe);
         LAUNCHCHK(ctx);
-        const int posIters = std::min(p->positionIterations, kMaxPosIters);
         for (int it = 0; it < posIters && nCons > 0; ++it) {
             perColour(PH_SOLVE_X, [&](int b0, int b1) {
                 hipLaunchKernelGGL(k_solve_position, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, ctx->cv, b0, b1,
@@ -1635,6 +2146,10 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
         }
         pe = profStart(ctx);
         hipLaunchKernelGGL(k_sync_transforms, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w);
+        if (ctx->allowSleep) {
+            hipLaunchKernelGGL(k_sleep_accum, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, dt);
+            hipLaunchKernelGGL(k_sleep_apply, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, posIters);
+        }
         hipLaunchKernelGGL(k_sync_fixtures, dim3(blocks(ctx->nFixtures)), dim3(256), 0, s, w);
         profEnd(ctx, PH_SYNC, pe);
         LAUNCHCHK(ctx);
@@ -1661,6 +2176,136 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
                            make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     LAUNCHCHK(ctx);
     profCollect(ctx);
+    ctx->stats.bodies = ctx->nBodies;
+    ctx->stats.fixtures = ctx->nFixtures;
+    ctx->stats.contacts = ctx->nContacts;
+    ctx->stats.overflow = ctx->hcnt->overflow;
+    ctx->stats.cellSize = ctx->cellSize;
+    return B2G_OK;
+}
+
+void freeRef(b2g_context* ctx) {
+    RefView& r = ctx->ref;
+    freeArray(r.tmpMan); freeArray(r.overlap); freeArray(r.edgeCount); freeArray(r.edgeStart);
+    freeArray(r.edgeCursor); freeArray(r.edgeKeys); freeArray(r.edgeSorted); freeArray(r.edgeContact);
+    freeArray(r.bvisited); freeArray(r.cvisited); freeArray(r.stack); freeArray(r.islandBodies);
+    freeArray(r.islandContacts); freeArray(r.islandRange); freeArray(r.islandSlept); freeArray(r.lastIsland);
+    ctx->refCapBodies = ctx->refCapContacts = 0;
+}
+
+int ensureRefBuffers(b2g_context* ctx) {
+    if (ctx->refCapBodies >= ctx->capBodies && ctx->refCapContacts >= ctx->capContacts) return B2G_OK;
+    freeRef(ctx);
+    RefView& r = ctx->ref;
+    const size_t nb = (size_t)ctx->capBodies, nc = (size_t)ctx->capContacts;
+    HIPCHK(ctx, hipMalloc((void**)&r.tmpMan, sizeof(b2g_manifold) * nc));
+    HIPCHK(ctx, hipMalloc((void**)&r.overlap, nc));
+    HIPCHK(ctx, hipMalloc((void**)&r.edgeCount, sizeof(int) * (nb + 1)));
+    HIPCHK(ctx, hipMalloc((void**)&r.edgeStart, sizeof(int) * (nb + 1)));
+    HIPCHK(ctx, hipMalloc((void**)&r.edgeCursor, sizeof(int) * nb));
+    HIPCHK(ctx, hipMalloc((void**)&r.edgeKeys, sizeof(unsigned long long) * 2 * nc));
+    HIPCHK(ctx, hipMalloc((void**)&r.edgeSorted, sizeof(unsigned long long) * 2 * nc));
+    HIPCHK(ctx, hipMalloc((void**)&r.edgeContact, sizeof(int) * 2 * nc));
+    HIPCHK(ctx, hipMalloc((void**)&r.bvisited, sizeof(int) * nb));
+    HIPCHK(ctx, hipMalloc((void**)&r.cvisited, sizeof(int) * nc));
+    HIPCHK(ctx, hipMalloc((void**)&r.stack, sizeof(int) * nb));
+    HIPCHK(ctx, hipMalloc((void**)&r.islandBodies, sizeof(int) * (nb + nc)));
+    HIPCHK(ctx, hipMalloc((void**)&r.islandContacts, sizeof(int) * nc));
+    HIPCHK(ctx, hipMalloc((void**)&r.islandRange, sizeof(int4) * nb));
+    HIPCHK(ctx, hipMalloc((void**)&r.islandSlept, sizeof(int) * nb));
+    HIPCHK(ctx, hipMalloc((void**)&r.lastIsland, sizeof(int) * nb));
+    hipLaunchKernelGGL(k_fill<int>, dim3(blocks((int)nb)), dim3(256), 0, ctx->stream, r.lastIsland, (int)nb, -1);
+    ctx->refCapBodies = (int)nb;
+    ctx->refCapContacts = (int)nc;
+    // sorting 2 edges per contact
+    const size_t need = prims_scratch_bytes((int)std::max<size_t>(2 * nc, 1024));
+    if (need > ctx->scratchBytes) {
+        freeArray(ctx->scratch);
+        HIPCHK(ctx, hipMalloc(&ctx->scratch, need));
+        ctx->scratchBytes = need;
+    }
+    return B2G_OK;
+}
+
+// b2World::Step in the reference's visiting order (see the reference-order
+// kernels above). Same collide/broad-phase as the coloured path; islands
+// and constraint order follow Box2D exactly.
+int stepReference(b2g_context* ctx, const b2g_step_params* p) {
+    hipStream_t s = ctx->stream;
+    const float dt = p->dt;
+    const float inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
+    const float dtRatio = ctx->inv_dt0 * dt;
+    int rc;
+    if ((rc = reserveContacts(ctx)) != B2G_OK) return rc;
+    if ((rc = ensureRefBuffers(ctx)) != B2G_OK) return rc;
+    memset(&ctx->stats, 0, sizeof(ctx->stats));
+    Counters zero;
+    memset(&zero, 0, sizeof(zero));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->cnt, &zero, sizeof(Counters), hipMemcpyHostToDevice, s));
+    if (ctx->newFixture) {
+        if ((rc = rebuildHash(ctx)) != B2G_OK) return rc;
+        if ((rc = runBroadphase(ctx)) != B2G_OK) return rc;
+        ctx->newFixture = false;
+    }
+    RefView r = ctx->ref;
+    if (ctx->nContacts > 0) {
+        WorldView w = ctx->view();
+        hipLaunchKernelGGL(k_collide_compute, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, r);
+        hipLaunchKernelGGL(k_collide_serial, dim3(1), dim3(64), 0, s, w, r, ctx->cnt);
+        hipLaunchKernelGGL(k_mark_dead, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->keep);
+        LAUNCHCHK(ctx);
+        HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->keep, ctx->idx, ctx->nContacts, s));
+        hipLaunchKernelGGL(k_compact, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->cur, ctx->alt, ctx->keep,
+                           ctx->idx, ctx->nContacts, ctx->cnt);
+        LAUNCHCHK(ctx);
+        std::swap(ctx->cur, ctx->alt);
+        if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+        ctx->nContacts = ctx->hcnt->nContacts;
+    } else {
+        if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+    }
+    ctx->stats.beginEvents = ctx->hcnt->beginEvents;
+    ctx->stats.endEvents = ctx->hcnt->endEvents;
+    if (dt > 0.0f) {
+        WorldView w = ctx->view();
+        const int nb = ctx->nBodies, nc = ctx->nContacts;
+        hipLaunchKernelGGL(k_clear_island_flags, dim3(blocks(nb + 1)), dim3(256), 0, s, w, r);
+        if (nc > 0) hipLaunchKernelGGL(k_edge_count, dim3(blocks(nc)), dim3(256), 0, s, w, r, ctx->cnt);
+        LAUNCHCHK(ctx);
+        HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, r.edgeCount, r.edgeStart, nb + 1, s));
+        if (nc > 0) hipLaunchKernelGGL(k_edge_fill, dim3(blocks(nc)), dim3(256), 0, s, w, r);
+        LAUNCHCHK(ctx);
+        if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+        const int nEdges = ctx->hcnt->nEdges;
+        if (nEdges > 0) {
+            HIPCHK(ctx, prims_sort_keys_u64(ctx->scratch, ctx->scratchBytes, r.edgeKeys, r.edgeSorted, nEdges, s));
+            hipLaunchKernelGGL(k_edge_extract, dim3(blocks(nEdges)), dim3(256), 0, s, r, nEdges);
+        }
+        hipLaunchKernelGGL(k_dfs, dim3(1), dim3(64), 0, s, w, r, ctx->cnt);
+        LAUNCHCHK(ctx);
+        if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+        const int nIslands = ctx->hcnt->nIslands;
+        const int posIters = std::min(p->positionIterations, kMaxPosIters);
+        if (nIslands > 0)
+            hipLaunchKernelGGL(k_island_solve, dim3(nIslands), dim3(64), 0, s, w, ctx->cv, r, ctx->gravity, dt, dtRatio,
+                               ctx->warmStarting ? 1 : 0, p->velocityIterations, posIters, ctx->allowSleep ? 1 : 0,
+                               ctx->cnt);
+        if (nIslands > 0 && ctx->allowSleep) {
+            hipLaunchKernelGGL(k_static_last_island, dim3(nIslands), dim3(64), 0, s, w, r, nIslands);
+            hipLaunchKernelGGL(k_static_awake, dim3(blocks(nb)), dim3(256), 0, s, w, r);
+        }
+        hipLaunchKernelGGL(k_sync_fixtures, dim3(blocks(ctx->nFixtures)), dim3(256), 0, s, w);
+        LAUNCHCHK(ctx);
+        if ((rc = rebuildHash(ctx)) != B2G_OK) return rc;
+        if ((rc = runBroadphase(ctx)) != B2G_OK) return rc;
+        ctx->stats.constraints = ctx->hcnt->nConstraints;
+        ctx->stats.points = ctx->hcnt->points;
+        ctx->inv_dt0 = inv_dt;
+    }
+    if (ctx->autoClearForces && ctx->nBodies > 0)
+        hipLaunchKernelGGL(k_fill<float4>, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, ctx->force, ctx->nBodies,
+                           make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    LAUNCHCHK(ctx);
     ctx->stats.bodies = ctx->nBodies;
     ctx->stats.fixtures = ctx->nFixtures;
     ctx->stats.contacts = ctx->nContacts;
@@ -1747,6 +2392,7 @@ int b2g_destroy(b2g_context* ctx) {
     freeArray(ctx->btype); freeArray(ctx->bflags); freeArray(ctx->xf); freeArray(ctx->pos); freeArray(ctx->pos0);
     freeArray(ctx->vel); freeArray(ctx->force); freeArray(ctx->mass); freeArray(ctx->lcenter);
     freeArray(ctx->gsleep); freeArray(ctx->colourMask); freeArray(ctx->claim); freeArray(ctx->islandOf);
+    freeArray(ctx->islandAwake); freeArray(ctx->islandSleep); freeArray(ctx->islandMinSep);
     freeArray(ctx->fbody); freeArray(ctx->shape); freeArray(ctx->fmat); freeArray(ctx->ffilter);
     freeArray(ctx->fgroup); freeArray(ctx->fat); freeArray(ctx->moved); freeArray(ctx->range);
     for (ContactArrays* a : {&ctx->cur, &ctx->alt}) {
@@ -1759,9 +2405,10 @@ int b2g_destroy(b2g_context* ctx) {
     ConView& c = ctx->cv;
     freeArray(c.contact); freeArray(c.bodies); freeArray(c.mass); freeArray(c.normal); freeArray(c.r0);
     freeArray(c.r1); freeArray(c.pmass); freeArray(c.bias); freeArray(c.imp); freeArray(c.K); freeArray(c.NM);
-    freeArray(c.info); freeArray(c.lp); freeArray(c.lnp); freeArray(c.lc); freeArray(c.rad);
+    freeArray(c.info); freeArray(c.lp); freeArray(c.lnp); freeArray(c.lc); freeArray(c.rad); freeArray(c.island);
     freeArray(ctx->gCounts); freeArray(ctx->gStarts); freeArray(ctx->gCursor); freeArray(ctx->gEntries);
     freeArray(ctx->gBig); freeArray(ctx->gCell);
+    freeRef(ctx);
     for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->cnt) (void)hipFree(ctx->cnt);
@@ -1989,10 +2636,11 @@ int b2g_update_fixtures(b2g_context* ctx, const int32_t* ids, const b2g_fixture_
 int b2g_step(b2g_context* ctx, const b2g_step_params* p) {
     if (!ctx || !p) return B2G_ERR_ARG;
     if (p->velocityIterations < 0 || p->positionIterations < 0) return fail(ctx, B2G_ERR_ARG, "iterations");
-    if (p->mode != B2G_MODE_COLOURED) return fail(ctx, B2G_ERR_UNSUPPORTED, "solver mode");
+    if (p->mode != B2G_MODE_COLOURED && p->mode != B2G_MODE_REFERENCE)
+        return fail(ctx, B2G_ERR_ARG, "solver mode");
     if (p->learnedWarmStart) return fail(ctx, B2G_ERR_UNSUPPORTED, "learned warm start without a model");
     (void)hipSetDevice(ctx->device);
-    return stepColoured(ctx, p);
+    return p->mode == B2G_MODE_REFERENCE ? stepReference(ctx, p) : stepColoured(ctx, p);
 }
 
 int b2g_sync(b2g_context* ctx) {

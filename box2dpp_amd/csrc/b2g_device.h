@@ -169,7 +169,10 @@ struct WorldView {
     float4* mass;      // invMass invI linearDamping angularDamping
     float2* gsleep;    // gravityScale sleepTime
     unsigned long long* colourMask;
-    int* islandOf;     // island id (union-find root) per body, -1 none
+    int* islandOf;     // island root (smallest body index of the island) per body
+    int* islandAwake;  // per root: the island holds an awake body
+    int* islandSleep;  // per root: 1 if a body has rested less than the time to sleep
+    int* islandMinSep; // per position pass and root: 1 if a point is deeper than 3 * linearSlop
     // fixtures / proxies
     int nFixtures;
     int* fbody;
