@@ -13,8 +13,13 @@ Also reported on the same JSON line:
   roofline      the dominant kernel (velocity solve) against HBM bandwidth:
                 algorithmic bytes per launch from the byte model
                 (48 B per constraint + 72 B per manifold point per velocity
-                iteration) over its mean launch time measured with HIP events
-                on the step stream;
+                iteration) over its mean execution time, measured in a second
+                pass over the same workload (fresh pile, same steps) with HIP
+                events that its dispatch packets record on the step stream
+                (hipExtLaunchKernelGGL); traffic is the HBM bytes per launch
+                measured by rocprofv3 counters (given with --traffic-bytes);
+                stream_us_per_launch is the time per launch seen on the
+                stream, dispatch gaps included;
   cpu_baseline  the reference b2World::Step (upstream Box2D 2.3.2 built from
                 the reference's sources, oracle/_ref) on the same pile on one
                 host core, over the first steps that fit the time budget.
@@ -36,6 +41,12 @@ import numpy as np  # noqa: E402
 import box2dpp_amd  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_solve_velocity launch on the default workload (100k pile,
+# seed 1, steps 1-1000, 8/3 iterations), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE averaged over all launches and scaled by the counters' readings
+# for a plain 256 MiB copy (x2.000 and x1.000); the summary with the commands
+# is under profiles/.
+DEFAULT_TRAFFIC_BYTES = 4.929e6
 
 
 class PhaseProfile(ctypes.Structure):
@@ -110,10 +121,18 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--vel", type=int, default=8)
     ap.add_argument("--pos", type=int, default=3)
-    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--profile-steps", type=int, default=None,
+                    help="steps of the timing pass (default: the same as --steps)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-bytes", type=float, default=None,
+                    help="HBM bytes per velocity-solve launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE")
     args = ap.parse_args()
+    if args.profile_steps is None:
+        args.profile_steps = args.steps
+    default_workload = (args.bodies, args.seed, args.steps, args.vel, args.pos) == (100000, 1, 1000, 8, 3)
+    if args.traffic_bytes is None and default_workload:
+        args.traffic_bytes = DEFAULT_TRAFFIC_BYTES
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
                                    "(batched worlds shard across GPUs by world; a single world by slab, "
@@ -140,8 +159,13 @@ def main():
     pen = np.zeros(3, np.float64)
     lib.b2pp_world_penetration(h, pen.ctypes.data_as(ctypes.c_void_p))
 
-    # live per-kernel timing on the step stream (HIP events), a few more steps
+    lib.b2pp_world_destroy(h)
+
+    # Timing pass: the same workload again from a fresh pile, with per-phase
+    # events on the step stream and the velocity kernel's own execution time
+    # from events carried by its dispatch packets.
     b2g = box2dpp_amd.b2g()
+    h = make_world(lib, args.bodies, args.seed)
     ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
     b2g.b2g_set_profiling(ctx, 1)
     prof_bytes = 0.0
@@ -149,13 +173,17 @@ def main():
         step(lib, h, 1, args.vel, args.pos)
         s2 = stats_of(lib, h)
         prof_bytes += args.vel * (48.0 * s2.constraints + 72.0 * s2.points)
-    prof = (PhaseProfile * 16)()
+    prof = (PhaseProfile * 32)()
     cnt = ctypes.c_int(0)
-    b2g.b2g_get_profile(ctx, prof, 16, ctypes.byref(cnt))
+    b2g.b2g_get_profile(ctx, prof, 32, ctypes.byref(cnt))
     b2g.b2g_set_profiling(ctx, 0)
-    phases = {prof[i].name.decode(): (prof[i].launches, prof[i].ms) for i in range(cnt.value)}
-    sv_launches, sv_ms = phases.get("solve_velocity", (0, 0.0))
+    entries = {prof[i].name.decode(): (prof[i].launches, prof[i].ms) for i in range(min(cnt.value, 32))}
+    phases = {k: v for k, v in entries.items() if not k.startswith("k_")}
+    kernels = {k: v for k, v in entries.items() if k.startswith("k_")}
+    # kernel execution time from events carried by the dispatch packets
+    sv_launches, sv_ms = kernels.get("k_solve_velocity", (0, 0.0))
     avg_launch_ms = sv_ms / max(sv_launches, 1)
+    ph_launches, ph_ms = phases.get("solve_velocity", (0, 0.0))
     bytes_per_launch = prof_bytes / max(sv_launches, 1)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
     lib.b2pp_world_destroy(h)
@@ -182,13 +210,16 @@ def main():
                                f"{args.vel} vel / {args.pos} pos iterations, coloured solver, sleep and CCD off",
                    "bodies": args.bodies, "vel_iters": args.vel, "pos_iters": args.pos, "solver": "coloured"},
         "roofline": {"bound": "hbm", "kernel": "k_solve_velocity", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_ms * 1e3},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": args.traffic_bytes,
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_ms * 1e3,
+                     "launches_per_step": sv_launches / max(args.profile_steps, 1),
+                     "stream_us_per_launch": 1e3 * ph_ms / max(ph_launches, 1)},
         "cpu_baseline": cpu,
         "final_state": {"contacts": st.contacts, "constraints": st.constraints, "points": st.points,
                         "colours": st.colours, "mean_penetration_m": float(pen[0]), "max_penetration_m": float(pen[1])},
         "phase_ms_per_step": {k: round(v[1] / max(args.profile_steps, 1), 4) for k, v in phases.items()},
         "phase_share": {k: round(v[1] / total_phase_ms, 4) for k, v in phases.items()} if total_phase_ms else {},
+        "kernel_us_per_launch": {k: round(1e3 * v[1] / max(v[0], 1), 3) for k, v in kernels.items()},
     }
     print(json.dumps(out))
     return 0

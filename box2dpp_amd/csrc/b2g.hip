@@ -29,13 +29,17 @@
 #include "b2g_device.h"
 #include "collide.h"
 #include "prims.h"
+#include "mlp.h"
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <math.h>
 #include <string.h>
 #include <stdio.h>
 #include <string>
 #include <vector>
+#include <limits>
+#include <random>
 #include <algorithm>
 
 using namespace b2g;
@@ -63,7 +67,10 @@ struct Counters {
     int points;
     int nEdges;
     int nIslands;
-    int pad;
+    int islandChanged;     // set by a hooking pass that lowered a label
+    int nFresh;            // fresh contact points of the step (learned warm start, log, trace)
+    int residualPoints;
+    float residualSum;     // sum of squared complementarity residuals (k_residual)
     int colourCount[kColourSlots + 3];
     int minSep[kMaxPosIters];
 };
@@ -254,35 +261,15 @@ __device__ inline bool islandContact(const WorldView& w, int c) {
 // b2World::Solve grows islands by depth-first search from every awake,
 // active, non-static body through island contacts, without crossing static
 // bodies, and wakes every body it reaches (REF/Dynamics/b2World.cpp:408-519).
-// Here the same components come from a lock-free union-find over the island
-// contacts: roots are hooked under the smaller index, so each component's
-// root is its smallest body index whatever the thread schedule.
-__device__ inline int ufFind(int* parent, int x) {
-    for (;;) {
-        const int p = parent[x];
-        if (p == x) return x;
-        const int gp = parent[p];
-        if (gp == p) return p;
-        parent[x] = gp;  // path halving: parents only move toward the root
-        x = gp;
-    }
-}
-
-__device__ inline void ufUnion(int* parent, int a, int b) {
-    for (;;) {
-        a = ufFind(parent, a);
-        b = ufFind(parent, b);
-        if (a == b) return;
-        if (a > b) {
-            const int t = a;
-            a = b;
-            b = t;
-        }
-        const int old = atomicCAS(&parent[b], b, a);
-        if (old == b) return;
-    }
-}
-
+// Here the same components come from minimum-label propagation with pointer
+// jumping: every body starts labelled with its own index; a hooking pass
+// lowers, with atomicMin only, the label of the larger of the two labels of
+// every island contact's bodies to the smaller one, and a jumping pass points
+// every body at the end of its label chain. Threads of one pass share data
+// only through atomicMin; each pass reads what the previous launch wrote, so
+// the result -- each body labelled with the smallest body index of its
+// component -- is the same for any thread schedule. Passes repeat until a
+// hooking pass changes nothing.
 __global__ void k_island_init(WorldView w) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= w.nBodies) return;
@@ -291,26 +278,36 @@ __global__ void k_island_init(WorldView w) {
     w.islandSleep[b] = 0;
 }
 
-__global__ void k_island_union(WorldView w) {
+__global__ void k_island_hook(WorldView w, int* changed) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= w.nContacts || !islandContact(w, c)) return;
     const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
     if (w.btype[bA] == BT_STATIC || w.btype[bB] == BT_STATIC) return;
     if (!(w.bflags[bA] & BF_ACTIVE) || !(w.bflags[bB] & BF_ACTIVE)) return;
-    ufUnion(w.islandOf, bA, bB);
+    const int la = w.islandOf[bA], lb = w.islandOf[bB];
+    if (la == lb) return;
+    atomicMin(&w.islandOf[la > lb ? la : lb], la < lb ? la : lb);
+    *changed = 1;
+}
+
+// Chains only lead to smaller indices, so a chain ends within nBodies hops.
+__global__ void k_island_jump(WorldView w) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies) return;
+    int l = w.islandOf[b];
+    for (;;) {
+        const int n = w.islandOf[l];
+        if (n == l) break;
+        l = n;
+    }
+    if (l != w.islandOf[b]) w.islandOf[b] = l;
 }
 
 __global__ void k_island_roots(WorldView w) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= w.nBodies) return;
-    const int r = ufFind(w.islandOf, b);
-    if (w.btype[b] != BT_STATIC && (w.bflags[b] & BF_ACTIVE) && (w.bflags[b] & BF_AWAKE)) w.islandAwake[r] = 1;
-}
-
-__global__ void k_island_compress(WorldView w) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= w.nBodies) return;
-    w.islandOf[b] = ufFind(w.islandOf, b);
+    if (w.btype[b] != BT_STATIC && (w.bflags[b] & BF_ACTIVE) && (w.bflags[b] & BF_AWAKE))
+        w.islandAwake[w.islandOf[b]] = 1;
 }
 
 // Bodies of awake islands are woken and take part in this step's solve.
@@ -498,10 +495,20 @@ __global__ void k_integrate_velocities(WorldView w, V2 gravity, float h) {
     integrateVelocityOne(w, b, gravity, h);
 }
 
+// Fresh contact points: points that start the step without a carried
+// impulse (new contacts, new feature ids, or warm starting off). Their
+// features feed the warm-start network and the training log.
+constexpr int kFeatures = 16;
+struct FreshView {
+    int* mask;    // per constraint: bit j set if point j is fresh
+    int* num;     // popcount of mask
+    float* feat;  // [2 * constraints][kFeatures], written for fresh points only
+};
+
 // b2ContactSolver ctor + InitializeVelocityConstraints,
 // REF/Dynamics/Contacts/b2ContactSolver.cpp:52-252.
 __device__ inline void initConstraintOne(const WorldView& w, const ConView& cv, int k, int c, float dtRatio, int warm,
-                                         Counters* cnt) {
+                                         Counters* cnt, const FreshView& fv) {
     const int fA = w.cfA[c], fB = w.cfB[c];
     const int bA = w.fbody[fA], bB = w.fbody[fB];
     const b2g_manifold& m = w.cman[c];
@@ -587,13 +594,108 @@ __device__ inline void initConstraintOne(const WorldView& w, const ConView& cv, 
     cv.lc[k] = make_float4(lcA.x, lcA.y, lcB.x, lcB.y);
     cv.rad[k] = make_float2(radiusA, radiusB);
     cv.island[k] = w.islandOf[w.btype[bA] != BT_STATIC ? bA : bB];
+    if (fv.mask) {
+        // Features of the fresh points: relative normal and tangent velocity,
+        // normal, separation, inverse masses and inertias, point count,
+        // normal and tangent mass, friction, r x n of both bodies,
+        // restitution bias.
+        int fm = 0;
+        for (int j = 0; j < vpc; ++j) {
+            if (ni[j] != 0.0f || ti[j] != 0.0f) continue;
+            fm |= 1 << j;
+            const V2 dv = vB + cross(wB, rB[j]) - vA - cross(wA, rA[j]);
+            float4* f = (float4*)(fv.feat + (size_t)(2 * k + j) * kFeatures);
+            f[0] = make_float4(dot(dv, normal), dot(dv, tangent), normal.x, normal.y);
+            f[1] = make_float4(wm.separations[j], mA, mB, iA);
+            f[2] = make_float4(iB, (float)vpc, nm[j], tm[j]);
+            f[3] = make_float4(mat.x, cross(rA[j], normal), cross(rB[j], normal), vb[j]);
+        }
+        fv.mask[k] = fm;
+        fv.num[k] = __popc(fm);
+    }
 }
 
 __global__ void k_init_constraints(WorldView w, ConView cv, const int* order, int n, float dtRatio, int warm,
-                                   Counters* cnt) {
+                                   Counters* cnt, FreshView fv) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    initConstraintOne(w, cv, k, order[k], dtRatio, warm, cnt);
+    initConstraintOne(w, cv, k, order[k], dtRatio, warm, cnt, fv);
+}
+
+// Compacts the fresh points in constraint order: ref = (k << 1) | j.
+__global__ void k_fresh_gather(int n, FreshView fv, const int* off, float* feat, int* ref, Counters* cnt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int m = fv.mask[k];
+    if (k == n - 1) cnt->nFresh = off[k] + __popc(m);
+    int o = off[k];
+    for (int j = 0; j < 2; ++j) {
+        if (!((m >> j) & 1)) continue;
+        const float4* src = (const float4*)(fv.feat + (size_t)(2 * k + j) * kFeatures);
+        float4* dst = (float4*)(feat + (size_t)o * kFeatures);
+        for (int q = 0; q < kFeatures / 4; ++q) dst[q] = src[q];
+        ref[o] = 2 * k + j;
+        ++o;
+    }
+}
+
+// Appends the fresh points of the step to the training log: features and
+// the impulses the solver gave them, in velocity-change units (the normal
+// impulse over the point's normal mass, the tangent impulse over its tangent
+// mass), so that an impact's target is about its approach speed.
+__global__ void k_log_append(int nFresh, const int* ref, const float* feat, const float4* imp, float* logFeat,
+                             float* logTarget, int base, int cap) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nFresh || base + e >= cap) return;
+    const size_t d = (size_t)(base + e);
+    const float* f = feat + (size_t)e * kFeatures;
+    const float4* src = (const float4*)f;
+    float4* dst = (float4*)(logFeat + d * kFeatures);
+    for (int q = 0; q < kFeatures / 4; ++q) dst[q] = src[q];
+    const int r = ref[e];
+    const float4 v = imp[r >> 1];
+    const float nm = f[10], tm = f[11];
+    const float ln = (r & 1) ? v.z : v.x, lt = (r & 1) ? v.w : v.y;
+    logTarget[2 * d] = nm > 0.0f ? ln / nm : 0.0f;
+    logTarget[2 * d + 1] = tm > 0.0f ? lt / tm : 0.0f;
+}
+
+// Complementarity residual of the velocity constraints, summed over the
+// solved points: r_n = vn - bias where the normal impulse is positive, else
+// min(vn - bias, 0); r_t = vt where the tangent impulse is inside the
+// friction cone, else 0.
+__global__ void k_residual(WorldView w, ConView cv, int n, Counters* cnt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    float sum = 0.0f;
+    int pts = 0;
+    if (k < n) {
+        const int2 bi = cv.bodies[k];
+        const int vpc = cv.info[k] & 3;
+        const float4 va = w.vel[bi.x], vbv = w.vel[bi.y];
+        const V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+        const float4 nf = cv.normal[k];
+        const V2 normal = mk(nf.x, nf.y), tangent = cross(normal, 1.0f);
+        const float4 imp = cv.imp[k];
+        const float2 bias = cv.bias[k];
+        for (int j = 0; j < vpc; ++j) {
+            const float4 r = j == 0 ? cv.r0[k] : cv.r1[k];
+            const V2 dv = vB + cross(vbv.z, mk(r.z, r.w)) - vA - cross(va.z, mk(r.x, r.y));
+            const float ln = j == 0 ? imp.x : imp.z, lt = j == 0 ? imp.y : imp.w;
+            const float vn = dot(dv, normal) - (j == 0 ? bias.x : bias.y);
+            const float rn = ln > 0.0f ? vn : fminB(vn, 0.0f);
+            const float rt = absB(lt) < nf.z * ln ? dot(dv, tangent) : 0.0f;
+            sum += rn * rn + rt * rt;
+            ++pts;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sum += __shfl_down(sum, off);
+        pts += __shfl_down(pts, off);
+    }
+    if ((threadIdx.x & 63) == 0 && pts > 0) {
+        atomicAdd(&cnt->residualSum, sum);
+        atomicAdd(&cnt->residualPoints, pts);
+    }
 }
 
 // b2ContactSolver::WarmStart, REF/Dynamics/Contacts/b2ContactSolver.cpp:254-292.
@@ -1135,12 +1237,7 @@ __global__ void k_pairs_query(const float4* fat, const uint8_t* moved, int n, Gr
         emitPair(sink, cnt, a, b);
     };
     const int nBig = cnt->nBig;
-    if (r.x > r.z) {
-        // A wide proxy is not binned; it tests every proxy (removed proxies
-        // fail every overlap test).
-        for (int p = 0; p < n; ++p) consider(p, false, 0, 0);
-        return;
-    }
+    if (r.x > r.z) return;  // a wide proxy's own query runs in k_pairs_wide
     for (int iy = r.y; iy <= r.w; ++iy)
         for (int ix = r.x; ix <= r.z; ++ix) {
             const uint32_t bkt = cellHash(ix, iy) & g.bucketMask;
@@ -1150,6 +1247,29 @@ __global__ void k_pairs_query(const float4* fat, const uint8_t* moved, int n, Gr
                 if (g.ecell[e] == ck) consider(g.entries[e], true, ix, iy);
         }
     for (int i = 0; i < nBig; ++i) consider(g.big[i], false, 0, 0);
+}
+
+// Queries of moved wide proxies (those not binned in the grid), one thread
+// per candidate proxy: the same pairs as the wide proxy testing every proxy
+// itself, under the same rule for two moved proxies (the lower id reports).
+// Removed proxies fail every overlap test.
+__global__ void k_pairs_wide(const float4* fat, const uint8_t* moved, int n, GridView g, WorldView w, bool world,
+                             const unsigned long long* table, uint32_t hmask, PairSink sink, Counters* cnt) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int nBig = cnt->nBig;
+    if (nBig == 0) return;
+    const float4 fp = fat[p];
+    const bool movedP = moved[p] != 0;
+    for (int i = 0; i < nBig; ++i) {
+        const int q = g.big[i];
+        if (q == p || !moved[q]) continue;
+        if (movedP && p < q) continue;  // reported by p's own query
+        if (!overlaps(fat[q], fp)) continue;
+        const int a = q < p ? q : p, b = q < p ? p : q;
+        if (world && !acceptPair(w, a, b, table, hmask)) continue;
+        emitPair(sink, cnt, a, b);
+    }
 }
 
 // New contacts from the sorted pair keys (b2Contact::Create with the
@@ -1384,7 +1504,9 @@ __global__ void k_island_solve(WorldView w, ConView cv, RefView r, V2 gravity, f
     const int k0 = rg.z, k1 = rg.z + rg.w;
     for (int i = tid; i < rg.y; i += nt) integrateVelocityOne(w, bodies[i], gravity, h);
     __syncthreads();
-    for (int k = k0 + tid; k < k1; k += nt) initConstraintOne(w, cv, k, r.islandContacts[k], dtRatio, warm, cnt);
+    const FreshView noFresh{nullptr, nullptr, nullptr};
+    for (int k = k0 + tid; k < k1; k += nt)
+        initConstraintOne(w, cv, k, r.islandContacts[k], dtRatio, warm, cnt, noFresh);
     __syncthreads();
     if (tid == 0) {
         if (warm)
@@ -1619,6 +1741,32 @@ struct b2g_context {
     // reference-order mode buffers
     RefView ref{};
     int refCapBodies = 0, refCapContacts = 0;
+
+    // learned warm start: network, fresh points of the step, training log,
+    // snapshots for the fallback and the residual trace
+    b2mlp::Net mlp;
+    bool mlpAllocated = false;
+    int capFresh = 0;               // constraint capacity of the fresh-point arrays
+    int* freshMask = nullptr;
+    int* freshNum = nullptr;
+    int* freshOff = nullptr;
+    float* freshAll = nullptr;      // [2 * capFresh][16]
+    float* freshFeat = nullptr;     // compacted [2 * capFresh][16]
+    int* freshRef = nullptr;        // compacted (k << 1) | j
+    float4* impSnap = nullptr;      // [capFresh]
+    float4* impAlt = nullptr;       // [capFresh]
+    float4* velSnap = nullptr;
+    int capVelSnap = 0;
+    float* logFeat = nullptr;
+    float* logTarget = nullptr;
+    int logCap = 0, logCount = 0;
+    void* trainScratch = nullptr;
+    size_t trainScratchBytes = 0;
+    int traceIters = 0;
+    std::vector<double> trace;      // [3][traceIters + 1]
+    int traceStepIters = 0;         // iterations of the last traced step
+    int lastColourStart[kColourSlots + 1] = {};  // colour ranges of the last coloured step
+    int lastConstraints = 0;
 
     // profiling (HIP events around launches)
     bool profiling = false;
@@ -1888,6 +2036,8 @@ int runBroadphase(b2g_context* ctx) {
         PairSink sink{ctx->newKeys, ctx->capContacts};
         hipLaunchKernelGGL(k_pairs_query, dim3(blocks(nf)), dim3(256), 0, s, ctx->fat, ctx->moved, nf, g, ctx->view(),
                            true, ctx->hashTable, ctx->hashMask, sink, ctx->cnt);
+        hipLaunchKernelGGL(k_pairs_wide, dim3(blocks(nf)), dim3(256), 0, s, ctx->fat, ctx->moved, nf, g, ctx->view(),
+                           true, ctx->hashTable, ctx->hashMask, sink, ctx->cnt);
         LAUNCHCHK(ctx);
         HIPCHK(ctx, hipMemcpyAsync(ctx->hcnt, ctx->cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
         HIPCHK(ctx, hipStreamSynchronize(s));
@@ -1929,13 +2079,20 @@ int readCounters(b2g_context* ctx) {
 }
 
 // ---------------------------------------------------------------- profiling
+// Phases are timed with events recorded on the step stream around their
+// launches (so they include dispatch gaps); entries named after a kernel
+// ("k_...") are that kernel's own execution time, from events that the
+// dispatch packet itself records (hipExtLaunchKernelGGL).
 enum Phase {
     PH_COLLIDE, PH_COMPACT, PH_COLOUR, PH_ORDER, PH_INTEGRATE_V, PH_INIT, PH_WARM, PH_SOLVE_V, PH_STORE,
-    PH_INTEGRATE_X, PH_SOLVE_X, PH_SYNC, PH_BROADPHASE, PH_ISLANDS, PH_COUNT
+    PH_INTEGRATE_X, PH_SOLVE_X, PH_SYNC, PH_BROADPHASE, PH_ISLANDS, PH_LEARNED, PH_K_SOLVE_V, PH_K_SOLVE_X,
+    PH_K_MLP_INFER, PH_K_MLP_GRAD, PH_COUNT
 };
 const char* kPhaseNames[PH_COUNT] = {"collide", "compact", "colour", "order", "integrate_velocities",
                                      "init_constraints", "warm_start", "solve_velocity", "store_impulses",
-                                     "integrate_positions", "solve_position", "sync", "broadphase", "islands"};
+                                     "integrate_positions", "solve_position", "sync", "broadphase", "islands",
+                                     "fresh_points", "k_solve_velocity", "k_solve_position", "k_mlp_infer",
+                                     "k_mlp_grad"};
 
 int profStart(b2g_context* ctx) {
     if (!ctx->profiling) return -1;
@@ -1956,6 +2113,28 @@ void profEnd(b2g_context* ctx, int phase, int i) {
     ctx->evMarks.push_back(std::make_pair(phase, i));
 }
 
+// Event pair for a kernel-level entry: the caller passes both events to
+// hipExtLaunchKernelGGL, then calls profKernel.
+int profKernelEvents(b2g_context* ctx, hipEvent_t* start, hipEvent_t* stop) {
+    *start = nullptr;
+    *stop = nullptr;
+    if (!ctx->profiling) return -1;
+    while ((int)ctx->evPool.size() < ctx->evUsed + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        ctx->evPool.push_back(e);
+    }
+    const int i = ctx->evUsed;
+    ctx->evUsed += 2;
+    *start = ctx->evPool[i];
+    *stop = ctx->evPool[i + 1];
+    return i;
+}
+
+void profKernel(b2g_context* ctx, int phase, int i) {
+    if (i >= 0) ctx->evMarks.push_back(std::make_pair(phase, i));
+}
+
 void profCollect(b2g_context* ctx) {
     if (!ctx->profiling || ctx->evMarks.empty()) return;
     (void)hipStreamSynchronize(ctx->stream);
@@ -1970,6 +2149,152 @@ void profCollect(b2g_context* ctx) {
     ctx->evUsed = 0;
 }
 
+// ---------------------------------------------------------------- learned warm start
+int ensureFresh(b2g_context* ctx) {
+    const int cap = ctx->capContacts;
+    if (cap <= ctx->capFresh) return B2G_OK;
+    freeArray(ctx->freshMask); freeArray(ctx->freshNum); freeArray(ctx->freshOff); freeArray(ctx->freshAll);
+    freeArray(ctx->freshFeat); freeArray(ctx->freshRef); freeArray(ctx->impSnap); freeArray(ctx->impAlt);
+    HIPCHK(ctx, hipMalloc((void**)&ctx->freshMask, sizeof(int) * cap));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->freshNum, sizeof(int) * cap));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->freshOff, sizeof(int) * cap));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->freshAll, sizeof(float) * kFeatures * 2 * (size_t)cap));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->freshFeat, sizeof(float) * kFeatures * 2 * (size_t)cap));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->freshRef, sizeof(int) * 2 * (size_t)cap));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->impSnap, sizeof(float4) * cap));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->impAlt, sizeof(float4) * cap));
+    ctx->capFresh = cap;
+    return B2G_OK;
+}
+
+int ensureVelSnap(b2g_context* ctx) {
+    if (ctx->capVelSnap >= ctx->capBodies) return B2G_OK;
+    freeArray(ctx->velSnap);
+    HIPCHK(ctx, hipMalloc((void**)&ctx->velSnap, sizeof(float4) * std::max(ctx->capBodies, 1)));
+    ctx->capVelSnap = ctx->capBodies;
+    return B2G_OK;
+}
+
+int ensureMlp(b2g_context* ctx) {
+    if (ctx->mlpAllocated) return B2G_OK;
+    HIPCHK(ctx, b2mlp::mlpAlloc(ctx->mlp));
+    ctx->mlpAllocated = true;
+    return B2G_OK;
+}
+
+// RMS complementarity residual of the current velocities and impulses
+// (k_residual, FP32 on the device).
+int deviceResidual(b2g_context* ctx, const WorldView& w, int nCons, double* rms) {
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->residualPoints, 1, 0);
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, (int*)&ctx->cnt->residualSum, 1, 0);
+    hipLaunchKernelGGL(k_residual, dim3(blocks(nCons)), dim3(256), 0, s, w, ctx->cv, nCons, ctx->cnt);
+    LAUNCHCHK(ctx);
+    int rc = readCounters(ctx);
+    if (rc != B2G_OK) return rc;
+    const int n = ctx->hcnt->residualPoints;
+    *rms = n > 0 ? sqrt((double)ctx->hcnt->residualSum / n) : 0.0;
+    return B2G_OK;
+}
+
+// Host copy of the velocity-constraint geometry for the FP64 residual.
+struct HostConstraints {
+    std::vector<int2> bodies;
+    std::vector<float4> normal, r0, r1, imp, vel;
+    std::vector<float2> bias;
+    std::vector<int> info;
+};
+
+// RMS over the solved points of the complementarity residual, in FP64:
+// r_n = vn - bias where the normal impulse is positive, else min(vn - bias, 0);
+// r_t = vt where |tangent impulse| < friction * normal impulse, else 0.
+double hostResidual(const HostConstraints& h, int nCons) {
+    double sum = 0.0;
+    long long pts = 0;
+    for (int k = 0; k < nCons; ++k) {
+        const int vpc = h.info[k] & 3;
+        const float4 va = h.vel[h.bodies[k].x], vb = h.vel[h.bodies[k].y];
+        const double nx = h.normal[k].x, ny = h.normal[k].y, mu = h.normal[k].z;
+        const double tx = ny, ty = -nx;
+        for (int j = 0; j < vpc; ++j) {
+            const float4 r = j == 0 ? h.r0[k] : h.r1[k];
+            const double dvx = (double)vb.x - (double)vb.z * r.w - (double)va.x + (double)va.z * r.y;
+            const double dvy = (double)vb.y + (double)vb.z * r.z - (double)va.y - (double)va.z * r.x;
+            const double bias = j == 0 ? h.bias[k].x : h.bias[k].y;
+            const double vn = dvx * nx + dvy * ny - bias;
+            const double vt = dvx * tx + dvy * ty;
+            const double ln = j == 0 ? h.imp[k].x : h.imp[k].z;
+            const double lt = j == 0 ? h.imp[k].y : h.imp[k].w;
+            const double rn = ln > 0.0 ? vn : std::min(vn, 0.0);
+            const double rt = fabs(lt) < mu * ln ? vt : 0.0;
+            sum += rn * rn + rt * rt;
+            ++pts;
+        }
+    }
+    return pts > 0 ? sqrt(sum / (double)pts) : 0.0;
+}
+
+// Residual curves of the three starts (cold, warm, learned) for the
+// current step; velocities and impulses are restored afterwards.
+template <typename WarmFn, typename SolveFn, typename PredictFn>
+int traceResiduals(b2g_context* ctx, const WorldView& w, int nCons, int nFresh, WarmFn& warmStart,
+                   SolveFn& solveVelocity, PredictFn& predict) {
+    hipStream_t s = ctx->stream;
+    const int T = ctx->traceIters, nb = ctx->nBodies;
+    int rc;
+    if ((rc = ensureFresh(ctx)) != B2G_OK) return rc;
+    if ((rc = ensureVelSnap(ctx)) != B2G_OK) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->velSnap, ctx->vel, sizeof(float4) * nb, hipMemcpyDeviceToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->impSnap, ctx->cv.imp, sizeof(float4) * nCons, hipMemcpyDeviceToDevice, s));
+    const bool haveLearned = ctx->mlp.ready;
+    if (haveLearned) {
+        HIPCHK(ctx, hipMemcpyAsync(ctx->impAlt, ctx->impSnap, sizeof(float4) * nCons, hipMemcpyDeviceToDevice, s));
+        if (nFresh > 0 && (rc = predict(ctx->impAlt)) != B2G_OK) return rc;
+    }
+    HostConstraints h;
+    h.bodies.resize(nCons); h.normal.resize(nCons); h.r0.resize(nCons); h.r1.resize(nCons); h.imp.resize(nCons);
+    h.bias.resize(nCons); h.info.resize(nCons); h.vel.resize(nb);
+    const ConView& cv = ctx->cv;
+    HIPCHK(ctx, hipMemcpyAsync(h.bodies.data(), cv.bodies, sizeof(int2) * nCons, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(h.normal.data(), cv.normal, sizeof(float4) * nCons, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(h.r0.data(), cv.r0, sizeof(float4) * nCons, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(h.r1.data(), cv.r1, sizeof(float4) * nCons, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(h.bias.data(), cv.bias, sizeof(float2) * nCons, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(h.info.data(), cv.info, sizeof(int) * nCons, hipMemcpyDeviceToHost, s));
+    auto sample = [&](double* out) -> int {
+        HIPCHK(ctx, hipMemcpyAsync(h.vel.data(), ctx->vel, sizeof(float4) * nb, hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipMemcpyAsync(h.imp.data(), cv.imp, sizeof(float4) * nCons, hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        *out = hostResidual(h, nCons);
+        return B2G_OK;
+    };
+    ctx->trace.assign(3 * (size_t)(T + 1), std::numeric_limits<double>::quiet_NaN());
+    for (int v = 0; v < 3; ++v) {
+        if (v == 2 && !haveLearned) continue;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->vel, ctx->velSnap, sizeof(float4) * nb, hipMemcpyDeviceToDevice, s));
+        if (v == 0) {
+            hipLaunchKernelGGL(k_fill<float4>, dim3(blocks(nCons)), dim3(256), 0, s, cv.imp, nCons,
+                               make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        } else {
+            HIPCHK(ctx, hipMemcpyAsync(cv.imp, v == 1 ? ctx->impSnap : ctx->impAlt, sizeof(float4) * nCons,
+                                       hipMemcpyDeviceToDevice, s));
+            warmStart();
+        }
+        LAUNCHCHK(ctx);
+        double* curve = &ctx->trace[(size_t)v * (T + 1)];
+        if ((rc = sample(&curve[0])) != B2G_OK) return rc;
+        for (int it = 1; it <= T; ++it) {
+            solveVelocity(1);
+            LAUNCHCHK(ctx);
+            if ((rc = sample(&curve[it])) != B2G_OK) return rc;
+        }
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->vel, ctx->velSnap, sizeof(float4) * nb, hipMemcpyDeviceToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(cv.imp, ctx->impSnap, sizeof(float4) * nCons, hipMemcpyDeviceToDevice, s));
+    ctx->traceStepIters = T;
+    return B2G_OK;
+}
+
 int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
     hipStream_t s = ctx->stream;
     const float dt = p->dt;
@@ -1978,6 +2303,7 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
     int rc;
     if ((rc = reserveContacts(ctx)) != B2G_OK) return rc;
     memset(&ctx->stats, 0, sizeof(ctx->stats));
+    ctx->stats.residual = -1.0f;
     // reset per-step counters
     Counters zero;
     memset(&zero, 0, sizeof(zero));
@@ -2030,10 +2356,23 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
             hipLaunchKernelGGL(k_fill<int>, dim3(blocks(nb * std::max(posIters, 1))), dim3(256), 0, s,
                                ctx->islandMinSep, nb * std::max(posIters, 1), 0);
             hipLaunchKernelGGL(k_island_init, dim3(blocks(nb)), dim3(256), 0, s, w);
-            if (ctx->nContacts > 0)
-                hipLaunchKernelGGL(k_island_union, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w);
+            for (int pass = 0; ctx->nContacts > 0; ++pass) {
+                // two hook/jump rounds between checks of the change flag
+                hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->islandChanged, 1, 0);
+                for (int r = 0; r < 2; ++r) {
+                    hipLaunchKernelGGL(k_island_hook, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w,
+                                       &ctx->cnt->islandChanged);
+                    hipLaunchKernelGGL(k_island_jump, dim3(blocks(nb)), dim3(256), 0, s, w);
+                }
+                LAUNCHCHK(ctx);
+                HIPCHK(ctx, hipMemcpyAsync(&ctx->hcnt->islandChanged, &ctx->cnt->islandChanged, sizeof(int),
+                                           hipMemcpyDeviceToHost, s));
+                HIPCHK(ctx, hipStreamSynchronize(s));
+                if (!ctx->hcnt->islandChanged) break;
+                // every pass that changes something lowers a label, so this is a safeguard only
+                if (pass >= 1000000) return fail(ctx, B2G_ERR_STATE, "island labelling did not converge");
+            }
             hipLaunchKernelGGL(k_island_roots, dim3(blocks(nb)), dim3(256), 0, s, w);
-            hipLaunchKernelGGL(k_island_compress, dim3(blocks(nb)), dim3(256), 0, s, w);
             hipLaunchKernelGGL(k_island_mark, dim3(blocks(nb)), dim3(256), 0, s, w);
             hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->remaining, 1, 0);
             if (ctx->nContacts > 0)
@@ -2092,20 +2431,41 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
         int nColours = 0;
         for (int c = 0; c < kColourSlots; ++c)
             if (colourStart[c + 1] > colourStart[c]) nColours = c + 1;
+        memcpy(ctx->lastColourStart, colourStart, sizeof(colourStart));
+        ctx->lastConstraints = nCons;
         ctx->stats.constraints = nCons;
         ctx->stats.colours = nColours;
 
         // Island solve.
+        const bool learned = p->learnedWarmStart != 0;
+        if (learned && !ctx->mlp.ready)
+            return fail(ctx, B2G_ERR_STATE, "learned warm start requested without a warm-start network");
+        const bool wantFresh = nCons > 0 && (learned || ctx->logCap > 0 || ctx->traceIters > 0);
+        if (wantFresh && (rc = ensureFresh(ctx)) != B2G_OK) return rc;
+        const FreshView fv{wantFresh ? ctx->freshMask : nullptr, ctx->freshNum, ctx->freshAll};
         int pe = profStart(ctx);
         hipLaunchKernelGGL(k_integrate_velocities, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, ctx->gravity, dt);
         profEnd(ctx, PH_INTEGRATE_V, pe);
         if (nCons > 0) {
             pe = profStart(ctx);
             hipLaunchKernelGGL(k_init_constraints, dim3(blocks(nCons)), dim3(256), 0, s, w, ctx->cv, ctx->consOrder,
-                               nCons, dtRatio, ctx->warmStarting ? 1 : 0, ctx->cnt);
+                               nCons, dtRatio, ctx->warmStarting ? 1 : 0, ctx->cnt, fv);
             profEnd(ctx, PH_INIT, pe);
         }
         LAUNCHCHK(ctx);
+        int nFresh = 0;
+        if (wantFresh) {
+            pe = profStart(ctx);
+            HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->freshNum, ctx->freshOff, nCons, s));
+            hipLaunchKernelGGL(k_fresh_gather, dim3(blocks(nCons)), dim3(256), 0, s, nCons, fv, ctx->freshOff,
+                               ctx->freshFeat, ctx->freshRef, ctx->cnt);
+            profEnd(ctx, PH_LEARNED, pe);
+            LAUNCHCHK(ctx);
+            if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+            nFresh = ctx->hcnt->nFresh;
+        }
+        ctx->stats.freshPoints = nFresh;
+        ctx->stats.residual = -1.0f;
         auto perColour = [&](int phase, auto kernelLaunch) {
             for (int c = 0; c < nColours; ++c) {
                 const int b0 = colourStart[c], b1 = colourStart[c + 1];
@@ -2119,15 +2479,64 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
                 profEnd(ctx, phase, pc);
             }
         };
-        if (nCons > 0 && ctx->warmStarting) {
+        auto warmStart = [&]() {
             perColour(PH_WARM, [&](int b0, int b1) {
                 hipLaunchKernelGGL(k_warm_start, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, ctx->cv, b0, b1);
             });
+        };
+        auto solveVelocity = [&](int iters) {
+            for (int it = 0; it < iters; ++it) {
+                perColour(PH_SOLVE_V, [&](int b0, int b1) {
+                    hipEvent_t e0, e1;
+                    const int pk = profKernelEvents(ctx, &e0, &e1);
+                    hipExtLaunchKernelGGL(k_solve_velocity, dim3(blocks(b1 - b0)), dim3(256), 0, s, e0, e1, 0, w,
+                                          ctx->cv, b0, b1);
+                    profKernel(ctx, PH_K_SOLVE_V, pk);
+                });
+            }
+        };
+        auto predict = [&](float4* imp) -> int {
+            hipEvent_t e0, e1;
+            const int pk = profKernelEvents(ctx, &e0, &e1);
+            HIPCHK(ctx, b2mlp::mlpPredictImpulses(ctx->mlp, ctx->freshFeat, ctx->freshRef, &ctx->cnt->nFresh, nFresh,
+                                                  ctx->cv.normal, imp, s, e0, e1));
+            profKernel(ctx, PH_K_MLP_INFER, pk);
+            return B2G_OK;
+        };
+        if (nCons > 0 && ctx->traceIters > 0) {
+            if ((rc = traceResiduals(ctx, w, nCons, nFresh, warmStart, solveVelocity, predict)) != B2G_OK) return rc;
         }
-        for (int it = 0; it < p->velocityIterations && nCons > 0; ++it) {
-            perColour(PH_SOLVE_V, [&](int b0, int b1) {
-                hipLaunchKernelGGL(k_solve_velocity, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, ctx->cv, b0, b1);
-            });
+        const bool usePrediction = learned && nFresh > 0;
+        const bool checkResidual = usePrediction && p->residualThreshold > 0.0f;
+        if (checkResidual) {
+            if ((rc = ensureVelSnap(ctx)) != B2G_OK) return rc;
+            HIPCHK(ctx, hipMemcpyAsync(ctx->velSnap, ctx->vel, sizeof(float4) * ctx->nBodies, hipMemcpyDeviceToDevice, s));
+            HIPCHK(ctx, hipMemcpyAsync(ctx->impSnap, ctx->cv.imp, sizeof(float4) * nCons, hipMemcpyDeviceToDevice, s));
+        }
+        if (usePrediction && (rc = predict(ctx->cv.imp)) != B2G_OK) return rc;
+        if (nCons > 0 && (ctx->warmStarting || usePrediction)) warmStart();
+        if (nCons > 0) solveVelocity(p->velocityIterations);
+        if (checkResidual) {
+            double rms = 0.0;
+            if ((rc = deviceResidual(ctx, w, nCons, &rms)) != B2G_OK) return rc;
+            ctx->stats.residual = (float)rms;
+            if (!(rms <= (double)p->residualThreshold)) {
+                // The plain start judges the learned one: solve again from it.
+                HIPCHK(ctx, hipMemcpyAsync(ctx->vel, ctx->velSnap, sizeof(float4) * ctx->nBodies,
+                                           hipMemcpyDeviceToDevice, s));
+                HIPCHK(ctx, hipMemcpyAsync(ctx->cv.imp, ctx->impSnap, sizeof(float4) * nCons, hipMemcpyDeviceToDevice,
+                                           s));
+                if (ctx->warmStarting) warmStart();
+                solveVelocity(p->velocityIterations);
+                ctx->stats.fallback = 1;
+            }
+        }
+        LAUNCHCHK(ctx);
+        if (ctx->logCap > 0 && nFresh > 0 && ctx->logCount < ctx->logCap) {
+            hipLaunchKernelGGL(k_log_append, dim3(blocks(nFresh)), dim3(256), 0, s, nFresh, ctx->freshRef,
+                               ctx->freshFeat, ctx->cv.imp, ctx->logFeat, ctx->logTarget, ctx->logCount, ctx->logCap);
+            LAUNCHCHK(ctx);
+            ctx->logCount = std::min(ctx->logCap, ctx->logCount + nFresh);
         }
         if (nCons > 0) {
             pe = profStart(ctx);
@@ -2140,8 +2549,11 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
         LAUNCHCHK(ctx);
         for (int it = 0; it < posIters && nCons > 0; ++it) {
             perColour(PH_SOLVE_X, [&](int b0, int b1) {
-                hipLaunchKernelGGL(k_solve_position, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, ctx->cv, b0, b1,
-                                   ctx->cnt, it);
+                hipEvent_t e0, e1;
+                const int pk = profKernelEvents(ctx, &e0, &e1);
+                hipExtLaunchKernelGGL(k_solve_position, dim3(blocks(b1 - b0)), dim3(256), 0, s, e0, e1, 0, w, ctx->cv,
+                                      b0, b1, ctx->cnt, it);
+                profKernel(ctx, PH_K_SOLVE_X, pk);
             });
         }
         pe = profStart(ctx);
@@ -2409,6 +2821,11 @@ int b2g_destroy(b2g_context* ctx) {
     freeArray(ctx->gCounts); freeArray(ctx->gStarts); freeArray(ctx->gCursor); freeArray(ctx->gEntries);
     freeArray(ctx->gBig); freeArray(ctx->gCell);
     freeRef(ctx);
+    freeArray(ctx->freshMask); freeArray(ctx->freshNum); freeArray(ctx->freshOff); freeArray(ctx->freshAll);
+    freeArray(ctx->freshFeat); freeArray(ctx->freshRef); freeArray(ctx->impSnap); freeArray(ctx->impAlt);
+    freeArray(ctx->velSnap); freeArray(ctx->logFeat); freeArray(ctx->logTarget);
+    if (ctx->trainScratch) (void)hipFree(ctx->trainScratch);
+    if (ctx->mlpAllocated) b2mlp::mlpFree(ctx->mlp);
     for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->cnt) (void)hipFree(ctx->cnt);
@@ -2638,7 +3055,8 @@ int b2g_step(b2g_context* ctx, const b2g_step_params* p) {
     if (p->velocityIterations < 0 || p->positionIterations < 0) return fail(ctx, B2G_ERR_ARG, "iterations");
     if (p->mode != B2G_MODE_COLOURED && p->mode != B2G_MODE_REFERENCE)
         return fail(ctx, B2G_ERR_ARG, "solver mode");
-    if (p->learnedWarmStart) return fail(ctx, B2G_ERR_UNSUPPORTED, "learned warm start without a model");
+    if (p->learnedWarmStart && p->mode == B2G_MODE_REFERENCE)
+        return fail(ctx, B2G_ERR_UNSUPPORTED, "the learned warm start runs with the coloured solver");
     (void)hipSetDevice(ctx->device);
     return p->mode == B2G_MODE_REFERENCE ? stepReference(ctx, p) : stepColoured(ctx, p);
 }
@@ -2686,11 +3104,12 @@ int b2g_get_contacts(b2g_context* ctx, int32_t* meta, b2g_manifold* manifolds, i
     if (count) *count = n;
     const int m = std::min(n, cap);
     if (m == 0) return B2G_OK;
-    std::vector<int> fa(m), fb(m), fl(m);
+    std::vector<int> fa(m), fb(m), fl(m), col(m);
     hipStream_t s = ctx->stream;
     HIPCHK(ctx, hipMemcpyAsync(fa.data(), ctx->cur.cfA, sizeof(int) * m, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipMemcpyAsync(fb.data(), ctx->cur.cfB, sizeof(int) * m, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipMemcpyAsync(fl.data(), ctx->cur.cflags, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(col.data(), ctx->cur.ccolour, sizeof(int) * m, hipMemcpyDeviceToHost, s));
     if (manifolds)
         HIPCHK(ctx, hipMemcpyAsync(manifolds, ctx->cur.cman, sizeof(b2g_manifold) * m, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
@@ -2699,7 +3118,7 @@ int b2g_get_contacts(b2g_context* ctx, int32_t* meta, b2g_manifold* manifolds, i
             meta[4 * i + 0] = fa[i];
             meta[4 * i + 1] = fb[i];
             meta[4 * i + 2] = ((fl[i] & CF_TOUCHING) ? 1 : 0) | ((fl[i] & CF_ENABLED) ? 2 : 0);
-            meta[4 * i + 3] = 0;
+            meta[4 * i + 3] = col[i];
         }
     }
     return B2G_OK;
@@ -2844,6 +3263,8 @@ int b2g_pairs_batch(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved,
     PairSink sink{keys.p, capPairs};
     hipLaunchKernelGGL(k_pairs_query, dim3(blocks(n)), dim3(256), 0, s, dfat.p, dmoved.p, n, g, w, false,
                        (const unsigned long long*)nullptr, 0u, sink, dc.p);
+    hipLaunchKernelGGL(k_pairs_wide, dim3(blocks(n)), dim3(256), 0, s, dfat.p, dmoved.p, n, g, w, false,
+                       (const unsigned long long*)nullptr, 0u, sink, dc.p);
     LAUNCHCHK(ctx);
     Counters hc;
     HIPCHK(ctx, hipMemcpyAsync(&hc, dc.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
@@ -2897,6 +3318,387 @@ int b2g_sincos_batch(b2g_context* ctx, const float* angles, int n, float* so, fl
     HIPCHK(ctx, hipMemcpyAsync(so, ds.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipMemcpyAsync(co, dc.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ learned warm start (C ABI)
+namespace {
+
+// Per-block partial sums over the log: pass 0 sums of the 16 features and 2
+// targets, pass 1 sums of squared deviations from `mean`.
+__global__ void k_log_moments(const float* feat, const float* tgt, int n, const float* mean, float* partial) {
+    constexpr int kQ = kFeatures + 2;
+    float acc[kQ];
+    for (int q = 0; q < kQ; ++q) acc[q] = 0.0f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        for (int q = 0; q < kQ; ++q) {
+            float v = q < kFeatures ? feat[(size_t)i * kFeatures + q] : tgt[2 * (size_t)i + (q - kFeatures)];
+            if (mean) {
+                v -= mean[q];
+                v *= v;
+            }
+            acc[q] += v;
+        }
+    }
+    __shared__ float red[256];
+    for (int q = 0; q < kQ; ++q) {
+        red[threadIdx.x] = acc[q];
+        __syncthreads();
+        for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[blockIdx.x * kQ + q] = red[0];
+        __syncthreads();
+    }
+}
+
+// Epoch order of the log: sort keys from a hash of (seed, epoch, index).
+__global__ void k_perm_keys(int n, uint32_t seed, uint32_t epoch, uint32_t* keys, int* vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t h = (uint32_t)i * 0x9E3779B1u ^ seed * 0x85EBCA77u ^ epoch * 0xC2B2AE3Du;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    keys[i] = h;
+    vals[i] = i;
+}
+
+void paramsToDevice(const b2g_mlp_params& p, std::vector<float>& dev, float* norm) {
+    using namespace b2mlp;
+    dev.assign(kParams, 0.0f);
+    memcpy(&dev[kW1], p.W1, sizeof(p.W1));
+    memcpy(&dev[kB1], p.b1, sizeof(p.b1));
+    memcpy(&dev[kW2], p.W2, sizeof(p.W2));
+    memcpy(&dev[kB2], p.b2, sizeof(p.b2));
+    memcpy(&dev[kW3], p.W3, sizeof(p.W3));
+    memcpy(&dev[kB3], p.b3, sizeof(p.b3));
+    memcpy(&dev[kW4], p.W4, sizeof(p.W4));
+    memcpy(&dev[kB4], p.b4, sizeof(p.b4));
+    if (norm) {
+        for (int f = 0; f < kIn; ++f) {
+            norm[kNormXMean + f] = p.xmean[f];
+            norm[kNormXInv + f] = p.xstd[f] > 0.0f ? 1.0f / p.xstd[f] : 0.0f;
+        }
+        for (int o = 0; o < kOut; ++o) {
+            norm[kNormYMean + o] = p.ymean[o];
+            norm[kNormYStd + o] = p.ystd[o];
+        }
+    }
+}
+
+void paramsFromDevice(const std::vector<float>& dev, const float* norm, b2g_mlp_params& p) {
+    using namespace b2mlp;
+    memset(&p, 0, sizeof(p));
+    memcpy(p.W1, &dev[kW1], sizeof(p.W1));
+    memcpy(p.b1, &dev[kB1], sizeof(p.b1));
+    memcpy(p.W2, &dev[kW2], sizeof(p.W2));
+    memcpy(p.b2, &dev[kB2], sizeof(p.b2));
+    memcpy(p.W3, &dev[kW3], sizeof(p.W3));
+    memcpy(p.b3, &dev[kB3], sizeof(p.b3));
+    memcpy(p.W4, &dev[kW4], sizeof(p.W4));
+    memcpy(p.b4, &dev[kB4], sizeof(p.b4));
+    if (norm) {
+        for (int f = 0; f < kIn; ++f) {
+            p.xmean[f] = norm[kNormXMean + f];
+            p.xstd[f] = norm[kNormXInv + f] > 0.0f ? 1.0f / norm[kNormXInv + f] : 0.0f;
+        }
+        for (int o = 0; o < kOut; ++o) {
+            p.ymean[o] = norm[kNormYMean + o];
+            p.ystd[o] = norm[kNormYStd + o];
+        }
+    }
+}
+
+int uploadNet(b2g_context* ctx, const std::vector<float>& dev, const float* norm) {
+    using namespace b2mlp;
+    int rc = ensureMlp(ctx);
+    if (rc != B2G_OK) return rc;
+    hipStream_t s = ctx->stream;
+    Net& n = ctx->mlp;
+    HIPCHK(ctx, hipMemcpyAsync(n.params, dev.data(), sizeof(float) * kParams, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(n.norm, norm, sizeof(float) * kNorm, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemsetAsync(n.grad, 0, sizeof(float) * kParams, s));
+    HIPCHK(ctx, hipMemsetAsync(n.m, 0, sizeof(float) * kParams, s));
+    HIPCHK(ctx, hipMemsetAsync(n.v, 0, sizeof(float) * kParams, s));
+    HIPCHK(ctx, mlpPrepare(n, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    n.adamStep = 0;
+    n.ready = true;
+    return B2G_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int b2g_log_fresh(b2g_context* ctx, int capacity) {
+    if (!ctx || capacity < 0) return B2G_ERR_ARG;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    freeArray(ctx->logFeat);
+    freeArray(ctx->logTarget);
+    ctx->logCap = 0;
+    ctx->logCount = 0;
+    if (capacity == 0) return B2G_OK;
+    HIPCHK(ctx, hipMalloc((void**)&ctx->logFeat, sizeof(float) * kFeatures * (size_t)capacity));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->logTarget, sizeof(float) * 2 * (size_t)capacity));
+    ctx->logCap = capacity;
+    return B2G_OK;
+}
+
+int b2g_log_size(b2g_context* ctx, int* count) {
+    if (!ctx || !count) return B2G_ERR_ARG;
+    *count = ctx->logCount;
+    return B2G_OK;
+}
+
+int b2g_get_log(b2g_context* ctx, int first, int count, float* features, float* targets) {
+    if (!ctx || first < 0 || count < 0 || first + count > ctx->logCount) return B2G_ERR_ARG;
+    if (count == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    if (features)
+        HIPCHK(ctx, hipMemcpyAsync(features, ctx->logFeat + (size_t)first * kFeatures,
+                                   sizeof(float) * kFeatures * (size_t)count, hipMemcpyDeviceToHost, s));
+    if (targets)
+        HIPCHK(ctx, hipMemcpyAsync(targets, ctx->logTarget + 2 * (size_t)first, sizeof(float) * 2 * (size_t)count,
+                                   hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+int b2g_mlp_init(b2g_context* ctx, uint32_t seed) {
+    using namespace b2mlp;
+    if (!ctx) return B2G_ERR_ARG;
+    b2g_mlp_params p;
+    memset(&p, 0, sizeof(p));
+    std::mt19937 rng(seed);
+    auto fill = [&](float* w, int fanOut, int fanIn) {
+        const float a = sqrtf(6.0f / (float)(fanIn + fanOut));
+        std::uniform_real_distribution<float> u(-a, a);
+        for (int i = 0; i < fanOut * fanIn; ++i) w[i] = u(rng);
+    };
+    fill(p.W1, kHidden, kIn);
+    fill(p.W2, kHidden, kHidden);
+    fill(p.W3, kHidden, kHidden);
+    fill(p.W4, kOut, kHidden);
+    for (int f = 0; f < kIn; ++f) p.xstd[f] = 1.0f;
+    for (int o = 0; o < kOut; ++o) p.ystd[o] = 1.0f;
+    const int n = ctx->logCount;
+    if (n > 0) {
+        // Normalisation from the log: means, then standard deviations (two
+        // passes; FP32 partial sums per block, summed in FP64 here).
+        constexpr int kQ = kFeatures + 2;
+        const int grid = std::min(256, std::max(1, blocks(n)));
+        hipStream_t s = ctx->stream;
+        DevBuf<float> partial, mean;
+        HIPCHK(ctx, partial.alloc((size_t)grid * kQ));
+        HIPCHK(ctx, mean.alloc(kQ));
+        std::vector<float> hp((size_t)grid * kQ);
+        double mu[kQ], var[kQ];
+        for (int pass = 0; pass < 2; ++pass) {
+            hipLaunchKernelGGL(k_log_moments, dim3(grid), dim3(256), 0, s, ctx->logFeat, ctx->logTarget, n,
+                               pass ? (const float*)mean.p : (const float*)nullptr, partial.p);
+            LAUNCHCHK(ctx);
+            HIPCHK(ctx, hipMemcpyAsync(hp.data(), partial.p, sizeof(float) * hp.size(), hipMemcpyDeviceToHost, s));
+            HIPCHK(ctx, hipStreamSynchronize(s));
+            for (int q = 0; q < kQ; ++q) {
+                double acc = 0.0;
+                for (int b = 0; b < grid; ++b) acc += hp[(size_t)b * kQ + q];
+                if (pass == 0) mu[q] = acc / n; else var[q] = acc / n;
+            }
+            if (pass == 0) {
+                float mf[kQ];
+                for (int q = 0; q < kQ; ++q) mf[q] = (float)mu[q];
+                HIPCHK(ctx, hipMemcpyAsync(mean.p, mf, sizeof(mf), hipMemcpyHostToDevice, s));
+            }
+        }
+        for (int f = 0; f < kIn; ++f) {
+            p.xmean[f] = (float)mu[f];
+            const double sd = sqrt(var[f]);
+            p.xstd[f] = sd > 1e-12 * std::max(1.0, fabs(mu[f])) ? (float)sd : 0.0f;
+        }
+        for (int o = 0; o < kOut; ++o) {
+            p.ymean[o] = (float)mu[kIn + o];
+            const double sd = sqrt(var[kIn + o]);
+            p.ystd[o] = sd > 1e-12 ? (float)sd : 1.0f;
+        }
+    }
+    return b2g_mlp_set_params(ctx, &p);
+}
+
+int b2g_mlp_set_params(b2g_context* ctx, const b2g_mlp_params* p) {
+    if (!ctx || !p) return B2G_ERR_ARG;
+    std::vector<float> dev;
+    float norm[b2mlp::kNorm];
+    paramsToDevice(*p, dev, norm);
+    return uploadNet(ctx, dev, norm);
+}
+
+int b2g_mlp_get_params(b2g_context* ctx, b2g_mlp_params* p) {
+    using namespace b2mlp;
+    if (!ctx || !p) return B2G_ERR_ARG;
+    if (!ctx->mlp.ready) return fail(ctx, B2G_ERR_STATE, "no warm-start network");
+    std::vector<float> dev(kParams);
+    float norm[kNorm];
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(dev.data(), ctx->mlp.params, sizeof(float) * kParams, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(norm, ctx->mlp.norm, sizeof(norm), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    paramsFromDevice(dev, norm, *p);
+    return B2G_OK;
+}
+
+int b2g_mlp_train(b2g_context* ctx, int epochs, int batch, float lr, uint32_t seed, double* loss) {
+    using namespace b2mlp;
+    if (!ctx || epochs < 0 || batch <= 0 || !(lr > 0.0f)) return B2G_ERR_ARG;
+    if (!ctx->mlp.ready) return fail(ctx, B2G_ERR_STATE, "no warm-start network (b2g_mlp_init first)");
+    const int n = ctx->logCount;
+    if (n == 0) return fail(ctx, B2G_ERR_STATE, "the training log is empty");
+    hipStream_t s = ctx->stream;
+    // order buffers: keys, sorted keys, values, order, then sort scratch
+    const size_t sortBytes = prims_scratch_bytes(n);
+    const size_t need = 4 * sizeof(int) * (size_t)n + sortBytes + 1024;
+    if (need > ctx->trainScratchBytes) {
+        if (ctx->trainScratch) (void)hipFree(ctx->trainScratch);
+        ctx->trainScratch = nullptr;
+        ctx->trainScratchBytes = 0;
+        HIPCHK(ctx, hipMalloc(&ctx->trainScratch, need));
+        ctx->trainScratchBytes = need;
+    }
+    char* base = (char*)ctx->trainScratch;
+    uint32_t* keys = (uint32_t*)base;
+    uint32_t* keysSorted = keys + n;
+    int* vals = (int*)(keysSorted + n);
+    int* order = vals + n;
+    void* sortScratch = (void*)(((uintptr_t)(order + n) + 255) & ~(uintptr_t)255);
+    Net& net = ctx->mlp;
+    const int batches = (n + batch - 1) / batch;
+    for (int e = 0; e < epochs; ++e) {
+        hipLaunchKernelGGL(k_perm_keys, dim3(blocks(n)), dim3(256), 0, s, n, seed, (uint32_t)e, keys, vals);
+        LAUNCHCHK(ctx);
+        HIPCHK(ctx, prims_sort_pairs_u32(sortScratch, sortBytes, keys, keysSorted, vals, order, n, 32, s));
+        HIPCHK(ctx, hipMemsetAsync(net.loss, 0, sizeof(float), s));
+        for (int b = 0; b < batches; ++b) {
+            const int first = b * batch;
+            const int count = std::min(batch, n - first);
+            hipEvent_t e0, e1;
+            const int pk = profKernelEvents(ctx, &e0, &e1);
+            HIPCHK(ctx, mlpGradient(net, ctx->logFeat, ctx->logTarget, order, first, count, s, e0, e1));
+            profKernel(ctx, PH_K_MLP_GRAD, pk);
+            HIPCHK(ctx, mlpAdam(net, lr, s));
+        }
+        float l = 0.0f;
+        HIPCHK(ctx, hipMemcpyAsync(&l, net.loss, sizeof(float), hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        profCollect(ctx);
+        if (loss) loss[e] = (double)l / batches;
+    }
+    return B2G_OK;
+}
+
+int b2g_mlp_forward(b2g_context* ctx, const float* x, int n, float* y) {
+    if (!ctx || n < 0 || (n > 0 && (!x || !y))) return B2G_ERR_ARG;
+    if (!ctx->mlp.ready) return fail(ctx, B2G_ERR_STATE, "no warm-start network");
+    if (n == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<float> dx, dy;
+    HIPCHK(ctx, dx.alloc((size_t)n * kFeatures));
+    HIPCHK(ctx, dy.alloc((size_t)n * 2));
+    HIPCHK(ctx, hipMemcpyAsync(dx.p, x, sizeof(float) * kFeatures * (size_t)n, hipMemcpyHostToDevice, s));
+    hipEvent_t e0, e1;
+    const int pk = profKernelEvents(ctx, &e0, &e1);
+    HIPCHK(ctx, b2mlp::mlpForward(ctx->mlp, dx.p, n, dy.p, s, e0, e1));
+    profKernel(ctx, PH_K_MLP_INFER, pk);
+    HIPCHK(ctx, hipMemcpyAsync(y, dy.p, sizeof(float) * 2 * (size_t)n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    profCollect(ctx);
+    return B2G_OK;
+}
+
+int b2g_mlp_gradient(b2g_context* ctx, const float* x, const float* t, int n, b2g_mlp_params* grad, double* loss) {
+    using namespace b2mlp;
+    if (!ctx || n <= 0 || !x || !t || !grad) return B2G_ERR_ARG;
+    if (!ctx->mlp.ready) return fail(ctx, B2G_ERR_STATE, "no warm-start network");
+    hipStream_t s = ctx->stream;
+    DevBuf<float> dx, dt;
+    HIPCHK(ctx, dx.alloc((size_t)n * kFeatures));
+    HIPCHK(ctx, dt.alloc((size_t)n * 2));
+    HIPCHK(ctx, hipMemcpyAsync(dx.p, x, sizeof(float) * kFeatures * (size_t)n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dt.p, t, sizeof(float) * 2 * (size_t)n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemsetAsync(ctx->mlp.grad, 0, sizeof(float) * kParams, s));
+    HIPCHK(ctx, hipMemsetAsync(ctx->mlp.loss, 0, sizeof(float), s));
+    HIPCHK(ctx, mlpGradient(ctx->mlp, dx.p, dt.p, nullptr, 0, n, s));
+    std::vector<float> g(kParams);
+    float l = 0.0f;
+    HIPCHK(ctx, hipMemcpyAsync(g.data(), ctx->mlp.grad, sizeof(float) * kParams, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(&l, ctx->mlp.loss, sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemsetAsync(ctx->mlp.grad, 0, sizeof(float) * kParams, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    paramsFromDevice(g, nullptr, *grad);
+    if (loss) *loss = l;
+    return B2G_OK;
+}
+
+int b2g_set_trace(b2g_context* ctx, int maxIters) {
+    if (!ctx || maxIters < 0 || maxIters > 4096) return B2G_ERR_ARG;
+    ctx->traceIters = maxIters;  // the last recorded trace stays readable
+    return B2G_OK;
+}
+
+int b2g_get_trace(b2g_context* ctx, double* curves, int cap, int* iters) {
+    if (!ctx || cap < 0 || (cap > 0 && !curves)) return B2G_ERR_ARG;
+    if (iters) *iters = ctx->trace.empty() ? 0 : ctx->traceStepIters;
+    const int m = std::min(cap, (int)ctx->trace.size());
+    if (m > 0) memcpy(curves, ctx->trace.data(), sizeof(double) * m);
+    return B2G_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int b2g_check_colouring(b2g_context* ctx, int* conflicts, int* maskErrors) {
+    if (!ctx || !conflicts || !maskErrors) return B2G_ERR_ARG;
+    *conflicts = 0;
+    *maskErrors = 0;
+    const int n = ctx->lastConstraints;
+    hipStream_t s = ctx->stream;
+    std::vector<int2> bodies(n);
+    std::vector<int> contact(n), btype(ctx->nBodies), colour(ctx->nContacts);
+    std::vector<unsigned long long> mask(ctx->nBodies);
+    if (n > 0) {
+        HIPCHK(ctx, hipMemcpyAsync(bodies.data(), ctx->cv.bodies, sizeof(int2) * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipMemcpyAsync(contact.data(), ctx->cv.contact, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    }
+    if (ctx->nBodies > 0) {
+        HIPCHK(ctx, hipMemcpyAsync(btype.data(), ctx->btype, sizeof(int) * ctx->nBodies, hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipMemcpyAsync(mask.data(), ctx->colourMask, sizeof(unsigned long long) * ctx->nBodies,
+                                   hipMemcpyDeviceToHost, s));
+    }
+    if (ctx->nContacts > 0)
+        HIPCHK(ctx, hipMemcpyAsync(colour.data(), ctx->cur.ccolour, sizeof(int) * ctx->nContacts,
+                                   hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    // no dynamic body twice in one colour
+    std::vector<int> seen(ctx->nBodies, -1);
+    std::vector<unsigned long long> rebuilt(ctx->nBodies, 0ull);
+    for (int c = 0; c < kMaxColours; ++c) {
+        for (int k = ctx->lastColourStart[c]; k < ctx->lastColourStart[c + 1] && k < n; ++k) {
+            for (int b : {bodies[k].x, bodies[k].y}) {
+                if (b < 0 || b >= ctx->nBodies || btype[b] != BT_DYNAMIC) continue;
+                if (seen[b] == c) ++*conflicts;
+                seen[b] = c;
+                rebuilt[b] |= 1ull << c;
+            }
+        }
+    }
+    for (int b = 0; b < ctx->nBodies; ++b)
+        if (btype[b] == BT_DYNAMIC && rebuilt[b] != mask[b]) ++*maskErrors;
     return B2G_OK;
 }
 

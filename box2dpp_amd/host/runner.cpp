@@ -105,6 +105,15 @@ void b2pp_world_set_mode(void* h, int mode) {
     r->world.SetSolverOptions(o);
 }
 
+// Learned warm start on/off and its fallback threshold (b2SolverOptions).
+void b2pp_world_set_learned(void* h, int on, float residualThreshold) {
+    Runner* r = (Runner*)h;
+    b2SolverOptions o = r->world.GetSolverOptions();
+    o.learnedWarmStart = on != 0;
+    o.residualThreshold = residualThreshold;
+    r->world.SetSolverOptions(o);
+}
+
 int b2pp_world_load(void* h, const b2scene::BodyDef* bodies, int nb, const b2scene::FixtureDef* fx, int nf) {
     return loadScene((Runner*)h, bodies, nb, fx, nf);
 }

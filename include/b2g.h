@@ -155,8 +155,13 @@ typedef struct b2g_step_params {
     int32_t velocityIterations;
     int32_t positionIterations;
     int32_t mode;
-    int32_t learnedWarmStart;   /* 0 off, 1 on (requires a loaded model) */
-    float residualThreshold;    /* fallback threshold for the learned start */
+    int32_t learnedWarmStart;   /* 0 off, 1 on: fresh contact points start from the
+                                   warm-start network's impulses (needs a network,
+                                   b2g_mlp_init / b2g_mlp_set_params) */
+    float residualThreshold;    /* > 0: if the learned start leaves an RMS
+                                   complementarity residual above this (m/s) after
+                                   the velocity iterations, the step is solved again
+                                   from the plain start; <= 0: no check */
 } b2g_step_params;
 
 /* Counters of the last step. */
@@ -175,6 +180,10 @@ typedef struct b2g_step_stats {
     int32_t overflow;          /* nonzero if a device buffer overflowed */
     float minSeparation;       /* of the last position iteration run */
     float cellSize;            /* broad-phase grid cell */
+    int32_t freshPoints;       /* points without a carried impulse (when the learned
+                                  start, the training log or a trace is on) */
+    int32_t fallback;          /* 1 if the learned start was replaced by the plain one */
+    float residual;            /* RMS residual checked for the fallback, -1 if none */
 } b2g_step_stats;
 
 typedef struct b2g_context b2g_context;
@@ -230,7 +239,8 @@ int b2g_get_stats(b2g_context* ctx, b2g_step_stats* out);
 
 /* Contacts in table order: meta[4*i] = fixtureA, [4*i+1] = fixtureB (after
  * the registry's shape-order swap), [4*i+2] = flags (bit0 touching,
- * bit1 enabled). *count receives the number of contacts (may exceed cap). */
+ * bit1 enabled), [4*i+3] = solver colour (-1: none; coloured mode).
+ * *count receives the number of contacts (may exceed cap). */
 int b2g_get_contacts(b2g_context* ctx, int32_t* meta, b2g_manifold* manifolds, int cap, int* count);
 /* Fat AABB of every proxy, fixture order. */
 int b2g_get_fat_aabbs(b2g_context* ctx, b2g_aabb* out, int cap);
@@ -249,6 +259,70 @@ typedef struct b2g_phase_profile {
 /* on != 0 starts profiling and clears the accumulated times. */
 int b2g_set_profiling(b2g_context* ctx, int on);
 int b2g_get_profile(b2g_context* ctx, b2g_phase_profile* out, int cap, int* count);
+
+/* ------------------------------------------------ learned warm start
+ * No counterpart in the reference: its solver starts every new contact
+ * point from zero impulse (b2ContactSolver ctor, REF/Dynamics/Contacts/
+ * b2ContactSolver.cpp:100-109, with b2Contact::Update's id matching,
+ * REF/Dynamics/Contacts/b2Contact.cpp:190-205, carrying the rest).
+ *
+ * A fresh point is a solved manifold point that starts the step without a
+ * carried impulse. Its 16 features: relative normal and tangent velocity,
+ * normal (x, y), separation, inverse mass A, B, inverse inertia A, B, point
+ * count, normal mass, tangent mass, friction, cross(rA, n), cross(rB, n),
+ * restitution bias. The network (16-64-64-64-2, GELU, BF16 weights on the
+ * matrix cores, FP32 accumulation) maps them to the normal and tangent
+ * impulse in velocity-change units (impulse over the point's normal or
+ * tangent mass); the point starts from those times its masses, clamped to
+ * the friction cone. Log targets are in the same units. */
+#define B2G_MLP_FEATURES 16
+#define B2G_MLP_HIDDEN 64
+typedef struct b2g_mlp_params {
+    float W1[64 * 16];  /* row-major [out][in] */
+    float b1[64];
+    float W2[64 * 64];
+    float b2[64];
+    float W3[64 * 64];
+    float b3[64];
+    float W4[2 * 64];
+    float b4[2];
+    float xmean[16];    /* feature normalisation; xstd 0 ignores a feature */
+    float xstd[16];
+    float ymean[2];     /* target normalisation (normal, tangent impulse) */
+    float ystd[2];
+} b2g_mlp_params;
+
+/* Logs the fresh points of the following steps with the impulses the solver
+ * leaves on them, up to `capacity` points (0 stops and frees the log). */
+int b2g_log_fresh(b2g_context* ctx, int capacity);
+int b2g_log_size(b2g_context* ctx, int* count);
+int b2g_get_log(b2g_context* ctx, int first, int count, float* features, float* targets);
+/* Fresh network with seeded random weights; the normalisation is taken from
+ * the log (identity if the log is empty). Resets the optimiser. */
+int b2g_mlp_init(b2g_context* ctx, uint32_t seed);
+int b2g_mlp_set_params(b2g_context* ctx, const b2g_mlp_params* p);
+int b2g_mlp_get_params(b2g_context* ctx, b2g_mlp_params* p);
+/* Trains on the log with Adam (mean squared error on normalised targets),
+ * minibatches of `batch` points in a seeded order per epoch; loss[e] gets
+ * epoch e's mean loss (may be null). */
+int b2g_mlp_train(b2g_context* ctx, int epochs, int batch, float lr, uint32_t seed, double* loss);
+/* Network outputs for n feature rows (host arrays; y: 2 per row). */
+int b2g_mlp_forward(b2g_context* ctx, const float* x, int n, float* y);
+/* Gradient of L = 1/(2n) sum |y - t|^2 (normalised targets) with respect to
+ * the weights and biases, in b2g_mlp_params layout (normalisation fields
+ * zero); *loss receives L. */
+int b2g_mlp_gradient(b2g_context* ctx, const float* x, const float* t, int n, b2g_mlp_params* grad, double* loss);
+
+/* Residual trace: while maxIters > 0, each step also solves its velocity
+ * constraints from three starts (cold: all impulses zero; warm: carried
+ * impulses, fresh points zero; learned: carried impulses, fresh points from
+ * the network) for maxIters iterations each, recording the RMS
+ * complementarity residual (FP64 on the host) after the warm start and after
+ * every iteration; the step's own result is unchanged. */
+int b2g_set_trace(b2g_context* ctx, int maxIters);
+/* curves: 3 x (iters + 1) values (cold, warm, learned; NaN where a start
+ * was not available); cap in doubles. */
+int b2g_get_trace(b2g_context* ctx, double* curves, int cap, int* iters);
 
 /* ---------------------------------------------- per-function batch calls
  * Run one hot-path stage over host arrays (copied in and out); used for
@@ -278,6 +352,13 @@ int b2g_fat_rule_batch(b2g_context* ctx, const b2g_aabb* fat, const b2g_aabb* ti
                        b2g_aabb* fatOut, uint8_t* moved);
 /* Rotation used inside the step for n angles (sine, cosine). */
 int b2g_sincos_batch(b2g_context* ctx, const float* angles, int n, float* s, float* c);
+
+/* Checks the colouring of the last coloured step: *conflicts counts dynamic
+ * bodies met twice within one colour (must be 0: a colour is solved in
+ * parallel), *maskErrors bodies whose colour mask differs from the colours
+ * of their constraints. No reference counterpart (the reference solves
+ * sequentially). */
+int b2g_check_colouring(b2g_context* ctx, int* conflicts, int* maskErrors);
 
 #ifdef __cplusplus
 }

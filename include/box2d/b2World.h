@@ -263,8 +263,9 @@ private:
 // Per-step solver options of the GPU path (no reference counterpart).
 struct b2SolverOptions {
     int32_t mode = 0;                 // 0 coloured, 1 reference order
-    bool learnedWarmStart = false;
-    float residualThreshold = 1e-4f;
+    bool learnedWarmStart = false;    // fresh contact points start from the warm-start network
+    float residualThreshold = 0.0f;   // > 0: re-solve from the plain start when the learned
+                                      // start leaves a larger RMS residual (m/s); 0: no check
 };
 
 class b2World {

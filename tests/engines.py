@@ -39,7 +39,7 @@ STATS = np.dtype([
     ("bodies", "<i4"), ("fixtures", "<i4"), ("contacts", "<i4"), ("constraints", "<i4"), ("points", "<i4"),
     ("colours", "<i4"), ("colouringRounds", "<i4"), ("newPairs", "<i4"), ("positionIterations", "<i4"),
     ("beginEvents", "<i4"), ("endEvents", "<i4"), ("overflow", "<i4"), ("minSeparation", "<f4"),
-    ("cellSize", "<f4"),
+    ("cellSize", "<f4"), ("freshPoints", "<i4"), ("fallback", "<i4"), ("residual", "<f4"),
 ])
 
 # body row columns of *_world_get_bodies
@@ -132,6 +132,14 @@ class World:
         out = np.zeros(3, np.float64)
         self.e.fn("world_penetration")(self.h, ptr(out))
         return float(out[0]), float(out[1]), int(out[2])
+
+    def device_context(self) -> VP:
+        """The world's b2g_context (C ABI of include/b2g.h)."""
+        self.e.lib.b2pp_world_device_context.restype = VP
+        return VP(self.e.lib.b2pp_world_device_context(self.h))
+
+    def set_learned(self, on: bool, threshold: float = 0.0) -> None:
+        self.e.lib.b2pp_world_set_learned(self.h, int(on), ctypes.c_float(threshold))
 
     def stats(self):
         if self.e.name != "gpu":
@@ -237,3 +245,16 @@ def geometry_bits(m: np.ndarray) -> np.ndarray:
     m["points"]["ni"] = 0
     m["points"]["ti"] = 0
     return manifold_bits(m)
+
+
+# ---------------------------------------------------------------- C ABI
+def b2g_lib():
+    """libb2g.so with the return types of the calls the tests make."""
+    lib = ctypes.CDLL(GPU_B2G_LIB, mode=ctypes.RTLD_GLOBAL)
+    lib.b2g_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+def b2g_check(lib, ctx, rc) -> None:
+    if rc != 0:
+        raise RuntimeError(f"b2g call failed rc={rc}: {lib.b2g_last_error(ctx).decode()}")

@@ -130,3 +130,29 @@ def test_pile_invariants(gpu):
     st = g.stats()
     assert st["overflow"] == 0 and st["colours"] > 1
     g.close()
+
+
+@pytest.mark.parametrize("seed", [6, 9])
+def test_coloured_step_is_deterministic(gpu, seed):
+    """Two copies of a dense pile step to the same bits: colours, islands and
+    every parallel stage give results independent of the thread schedule.
+    The colouring invariant (no dynamic body twice in one colour, masks equal
+    to the colours of each body's constraints) holds after every step."""
+    import ctypes
+    lib = E.b2g_lib()
+    a = gpu.world("world", 1500, seed, sleep=False)
+    b = gpu.world("world", 1500, seed, sleep=False)
+    ca = a.device_context()
+    conf, mask = ctypes.c_int(), ctypes.c_int()
+    for s in range(150):
+        a.step(1)
+        b.step(1)
+        E.b2g_check(lib, ca, lib.b2g_check_colouring(ca, ctypes.byref(conf), ctypes.byref(mask)))
+        assert conf.value == 0 and mask.value == 0, (s, conf.value, mask.value)
+    ba, bb = a.bodies(), b.bodies()
+    assert np.array_equal(ba.view(np.uint32), bb.view(np.uint32))
+    ma, na = a.contacts()
+    mb, nb = b.contacts()
+    assert np.array_equal(ma, mb)                      # pairs, flags and colours
+    assert np.array_equal(E.manifold_bits(na), E.manifold_bits(nb))
+    a.close(); b.close()
