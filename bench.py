@@ -112,6 +112,127 @@ def cpu_reference(n, seed, vel, pos, budget_s):
                       f"{el:.1f} s; early steps are cheaper than the settled pile"}
 
 
+MLP_PARAM_FLOATS = 64 * 16 + 64 + 2 * (64 * 64 + 64) + 2 * 64 + 2 + 16 + 16 + 2 + 2
+MLP_TRAIN_FLOPS = 2.0 * 2.0 * (16 * 64 + 2 * 64 * 64 + 64 * 16) + 2.0 * (16 * 64 + 2 * 64 * 64)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def _check(b2g, ctx, rc):
+    if rc != 0:
+        raise RuntimeError(f"b2g call failed rc={rc}: {b2g.b2g_last_error(ctx).decode()}")
+
+
+def learned_warm_start(lib, args):
+    """Second half of the metric: velocity iterations to reach a residual,
+    plain (cold, warm) against the learned warm start.
+
+    1. log: fresh contact points and their solved impulses from the plain
+       solver on a pile of the same size (seed args.log_seed), every step;
+    2. train the warm-start network on the log with its own kernels;
+    3. evaluate on an unseen pile (seed args.eval_seed): at every
+       args.trace_every-th step, the same velocity constraints are solved
+       from the three starts for args.trace_iters iterations, recording the
+       RMS complementarity residual (FP64, host) after each;
+    4. time the learned step (network inference for fresh points inside the
+       step) over the same steps as the plain timed run.
+    """
+    b2g = box2dpp_amd.b2g()
+    b2g.b2g_last_error.restype = ctypes.c_char_p
+    out = {}
+    # 1. log
+    h = make_world(lib, args.bodies, args.log_seed)
+    ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
+    _check(b2g, ctx, b2g.b2g_log_fresh(ctx, args.log_capacity))
+    t0 = time.perf_counter()
+    step(lib, h, args.steps, args.vel, args.pos)
+    lib.b2pp_world_sync(h)
+    n = ctypes.c_int()
+    b2g.b2g_log_size(ctx, ctypes.byref(n))
+    out["log"] = {"seed": args.log_seed, "steps": args.steps, "points": n.value,
+                  "seconds": round(time.perf_counter() - t0, 2)}
+    # 2. train (kernel time of the gradient kernel from dispatch-packet events)
+    _check(b2g, ctx, b2g.b2g_mlp_init(ctx, 5))
+    losses = np.zeros(args.epochs, np.float64)
+    b2g.b2g_set_profiling(ctx, 1)
+    t0 = time.perf_counter()
+    _check(b2g, ctx, b2g.b2g_mlp_train(ctx, args.epochs, args.batch, ctypes.c_float(args.lr), 9,
+                                       losses.ctypes.data_as(ctypes.c_void_p)))
+    train_s = time.perf_counter() - t0
+    prof = (PhaseProfile * 32)()
+    cnt = ctypes.c_int(0)
+    b2g.b2g_get_profile(ctx, prof, 32, ctypes.byref(cnt))
+    b2g.b2g_set_profiling(ctx, 0)
+    entries = {prof[i].name.decode(): (prof[i].launches, prof[i].ms) for i in range(min(cnt.value, 32))}
+    g_launches, g_ms = entries.get("k_mlp_grad", (0, 0.0))
+    flops = MLP_TRAIN_FLOPS * n.value * args.epochs
+    tflops = flops / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
+    params = (ctypes.c_float * MLP_PARAM_FLOATS)()
+    _check(b2g, ctx, b2g.b2g_mlp_get_params(ctx, params))
+    lib.b2pp_world_destroy(h)
+    out["train"] = {"epochs": args.epochs, "batch": args.batch, "lr": args.lr,
+                    "loss_first_epoch": round(float(losses[0]), 4), "loss_last_epoch": round(float(losses[-1]), 4),
+                    "seconds": round(train_s, 2)}
+    out["mfma"] = {"kernel": "k_mlp_grad", "dtype": "bf16", "launches": g_launches,
+                   "avg_launch_us": round(1e3 * g_ms / max(g_launches, 1), 2),
+                   "achieved_tflops": round(tflops, 2), "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+                   "frac": tflops / MFMA_BF16_PEAK_TFLOPS,
+                   "flops_per_sample": MLP_TRAIN_FLOPS}
+    # 3. residual traces on an unseen pile
+    h = make_world(lib, args.bodies, args.eval_seed)
+    ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
+    _check(b2g, ctx, b2g.b2g_mlp_set_params(ctx, params))
+    T = args.trace_iters
+    curves = []
+    buf = np.zeros(3 * (T + 1), np.float64)
+    it = ctypes.c_int()
+    for s in range(1, args.steps + 1):
+        traced = s % args.trace_every == 0
+        if traced:
+            b2g.b2g_set_trace(ctx, T)
+        step(lib, h, 1, args.vel, args.pos)
+        if traced:
+            b2g.b2g_set_trace(ctx, 0)
+            _check(b2g, ctx, b2g.b2g_get_trace(ctx, buf.ctypes.data_as(ctypes.c_void_p), len(buf), ctypes.byref(it)))
+            if it.value == T:
+                curves.append(buf.reshape(3, T + 1).copy())
+    lib.b2pp_world_destroy(h)
+    cur = np.array(curves).mean(axis=0)
+    names = ("cold", "warm", "learned")
+
+    def iters_to(row, target):
+        hit = np.nonzero(row <= target)[0]
+        return int(hit[0]) if len(hit) else None
+
+    plain8 = float(cur[1, min(8, T)])
+    out["residual"] = {
+        "definition": "RMS over solved points of r_n (vn - bias if normal impulse > 0 else min(vn - bias, 0)) "
+                      "and r_t (vt if |tangent impulse| < friction * normal impulse else 0), m/s, FP64 on host",
+        "traced_steps": len(curves), "trace_every": args.trace_every, "eval_seed": args.eval_seed,
+        "mean_at_iteration": {nm: {str(k): round(float(cur[i, k]), 5) for k in (0, 1, 2, 4, 8, T) if k <= T}
+                              for i, nm in enumerate(names)},
+        "iterations_to_1e-4": {nm: iters_to(cur[i], 1e-4) for i, nm in enumerate(names)},
+        "plain_warm_residual_at_8": round(plain8, 5),
+        "iterations_to_plain_warm_residual_at_8": {nm: iters_to(cur[i], plain8) for i, nm in enumerate(names)},
+    }
+    # 4. learned step throughput and penetration on the evaluation pile
+    h = make_world(lib, args.bodies, args.eval_seed)
+    ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
+    _check(b2g, ctx, b2g.b2g_mlp_set_params(ctx, params))
+    lib.b2pp_world_set_learned(h, 1, ctypes.c_float(0.0))
+    lib.b2pp_world_sync(h)
+    t0 = time.perf_counter()
+    step(lib, h, args.steps, args.vel, args.pos)
+    lib.b2pp_world_sync(h)
+    wall = time.perf_counter() - t0
+    pen = np.zeros(3, np.float64)
+    lib.b2pp_world_penetration(h, pen.ctypes.data_as(ctypes.c_void_p))
+    lib.b2pp_world_destroy(h)
+    out["learned_step"] = {"value": args.bodies * args.steps / wall, "unit": "body-steps/s",
+                           "ms_per_step": 1e3 * wall / args.steps, "mean_penetration_m": float(pen[0]),
+                           "max_penetration_m": float(pen[1])}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +246,15 @@ def main():
                     help="steps of the timing pass (default: the same as --steps)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start evaluation")
+    ap.add_argument("--log-seed", type=int, default=2)
+    ap.add_argument("--eval-seed", type=int, default=1001)
+    ap.add_argument("--log-capacity", type=int, default=10_000_000)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--trace-every", type=int, default=100)
+    ap.add_argument("--trace-iters", type=int, default=16)
     ap.add_argument("--traffic-bytes", type=float, default=None,
                     help="HBM bytes per velocity-solve launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE")
     args = ap.parse_args()
@@ -188,6 +318,8 @@ def main():
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
     lib.b2pp_world_destroy(h)
 
+    learned = None if args.no_learned else learned_warm_start(lib, args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_reference(args.bodies, args.seed, args.vel, args.pos, args.cpu_seconds)
@@ -215,6 +347,7 @@ def main():
                      "launches_per_step": sv_launches / max(args.profile_steps, 1),
                      "stream_us_per_launch": 1e3 * ph_ms / max(ph_launches, 1)},
         "cpu_baseline": cpu,
+        "learned_warm_start": learned,
         "final_state": {"contacts": st.contacts, "constraints": st.constraints, "points": st.points,
                         "colours": st.colours, "mean_penetration_m": float(pen[0]), "max_penetration_m": float(pen[1])},
         "phase_ms_per_step": {k: round(v[1] / max(args.profile_steps, 1), 4) for k, v in phases.items()},

@@ -440,7 +440,10 @@ hipError_t mlpPredictImpulses(const Net& n, const float* feat, const int* ref, c
 hipError_t mlpGradient(const Net& n, const float* x, const float* t, const int* order, int first, int count,
                        hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (count <= 0) return hipSuccess;
-    int grid = (count + 16 * kWaves * 4 - 1) / (16 * kWaves * 4);  // about four tiles per wavefront
+    // one workgroup per CU at most (the kernel's LDS allows one), each
+    // wavefront taking as many 16-sample tiles as needed
+    const int tiles = (count + 15) / 16;
+    int grid = (tiles + kWaves - 1) / kWaves;
     if (grid > 256) grid = 256;
     if (grid < 1) grid = 1;
     hipExtLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(kThreads), 0, s, e0, e1, 0, n.wh, n.params, n.norm, x, t,
