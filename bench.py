@@ -57,8 +57,15 @@ class PhaseProfile(ctypes.Structure):
 class StepStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "bodies", "fixtures", "contacts", "constraints", "points", "colours", "colouringRounds", "newPairs",
-        "positionIterations", "beginEvents", "endEvents", "overflow")] + [("minSeparation", ctypes.c_float),
-                                                                           ("cellSize", ctypes.c_float)]
+        "positionIterations", "beginEvents", "endEvents", "overflow")] + [
+        ("minSeparation", ctypes.c_float), ("cellSize", ctypes.c_float), ("freshPoints", ctypes.c_int32),
+        ("fallback", ctypes.c_int32), ("residual", ctypes.c_float)]
+
+
+def progress(msg):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    sys.stderr.write(f"[bench {time.strftime('%H:%M:%S')}] {msg}\n")
+    sys.stderr.flush()
 
 
 def make_world(lib, n, seed):
@@ -140,6 +147,7 @@ def learned_warm_start(lib, args):
     b2g.b2g_last_error.restype = ctypes.c_char_p
     out = {}
     # 1. log
+    progress("learned warm start: logging fresh points")
     h = make_world(lib, args.bodies, args.log_seed)
     ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
     _check(b2g, ctx, b2g.b2g_log_fresh(ctx, args.log_capacity))
@@ -151,6 +159,7 @@ def learned_warm_start(lib, args):
     out["log"] = {"seed": args.log_seed, "steps": args.steps, "points": n.value,
                   "seconds": round(time.perf_counter() - t0, 2)}
     # 2. train (kernel time of the gradient kernel from dispatch-packet events)
+    progress(f"learned warm start: training on {n.value} points")
     _check(b2g, ctx, b2g.b2g_mlp_init(ctx, 5))
     losses = np.zeros(args.epochs, np.float64)
     b2g.b2g_set_profiling(ctx, 1)
@@ -178,6 +187,7 @@ def learned_warm_start(lib, args):
                    "frac": tflops / MFMA_BF16_PEAK_TFLOPS,
                    "flops_per_sample": MLP_TRAIN_FLOPS}
     # 3. residual traces on an unseen pile
+    progress("learned warm start: residual traces")
     h = make_world(lib, args.bodies, args.eval_seed)
     ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
     _check(b2g, ctx, b2g.b2g_mlp_set_params(ctx, params))
@@ -215,6 +225,7 @@ def learned_warm_start(lib, args):
         "iterations_to_plain_warm_residual_at_8": {nm: iters_to(cur[i], plain8) for i, nm in enumerate(names)},
     }
     # 4. learned step throughput and penetration on the evaluation pile
+    progress("learned warm start: learned step")
     h = make_world(lib, args.bodies, args.eval_seed)
     ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
     _check(b2g, ctx, b2g.b2g_mlp_set_params(ctx, params))
@@ -233,6 +244,81 @@ def learned_warm_start(lib, args):
     return out
 
 
+def batched_worlds(lib, args):
+    """Batched worlds: args.batch_worlds independent worlds (dynamic body
+    counts log-uniform in [16, 1024], world w built from seed + w, each in its
+    own container) in sub-worlds of one b2World, advanced together by one
+    Step; body-steps/s over args.batch_steps steps, state resident on the
+    device before timing. CPU leg: the reference b2World::Step, one world per
+    thread over the host's cores, on a bounded sample of the worlds."""
+    h = ctypes.c_void_p(lib.b2pp_world_create(0.0, -10.0))
+    lib.b2pp_world_set_flags(h, 0, 1, 0, 0)
+    sizes = np.zeros(args.batch_worlds, np.int32)
+    progress(f"batched worlds: loading {args.batch_worlds} worlds")
+    t0 = time.perf_counter()
+    rc = lib.b2pp_world_load_batch(h, args.batch_worlds, ctypes.c_uint(args.batch_seed), 16, 1024,
+                                   sizes.ctypes.data_as(ctypes.c_void_p))
+    if rc != 0:
+        raise RuntimeError(f"batch load failed rc={rc}: {lib.b2pp_world_last_error(h).decode()}")
+    lib.b2pp_world_sync(h)
+    load_s = time.perf_counter() - t0
+    bodies = int(sizes.sum())
+    progress(f"batched worlds: {bodies} bodies loaded in {load_s:.1f} s; stepping")
+    t0 = time.perf_counter()
+    step(lib, h, args.batch_steps, args.vel, args.pos)
+    lib.b2pp_world_sync(h)
+    wall = time.perf_counter() - t0
+    st = stats_of(lib, h)
+    pen = np.zeros(3, np.float64)
+    lib.b2pp_world_penetration(h, pen.ctypes.data_as(ctypes.c_void_p))
+    lib.b2pp_world_destroy(h)
+    out = {"value": bodies * args.batch_steps / wall, "unit": "body-steps/s", "worlds": args.batch_worlds,
+           "bodies": bodies, "steps": args.batch_steps, "ms_per_step": 1e3 * wall / args.batch_steps,
+           "load_seconds": round(load_s, 2), "contacts": st.contacts, "colours": st.colours,
+           "mean_penetration_m": float(pen[0]),
+           "workload": f"{args.batch_worlds} worlds, bodies log-uniform in [16, 1024] (seed {args.batch_seed}), "
+                       f"{args.vel}/{args.pos} iterations, sleep and CCD off, one context"}
+    if not args.no_cpu:
+        progress("batched worlds: CPU reference sample")
+        out["cpu_baseline"] = cpu_batched(sizes, args)
+    return out
+
+
+def cpu_batched(sizes, args):
+    """Reference b2World::Step on host threads, one world per task."""
+    import concurrent.futures as cf
+    ref = os.path.join(ROOT, "oracle", "_ref", "libb2ref.so")
+    kind = "reference"
+    if os.path.exists(ref):
+        lib, pre = ctypes.CDLL(ref), "ref_"
+    else:
+        lib, pre = ctypes.CDLL(os.path.join(ROOT, "oracle", "_build", "liborc.so")), "orc_"
+        kind = "port"
+    f = lambda name: getattr(lib, pre + name)
+    f("world_create").restype = ctypes.c_void_p
+    f("world_create").argtypes = [ctypes.c_float, ctypes.c_float]
+    threads = max(1, min(16, os.cpu_count() or 1))
+    rng = np.random.default_rng(0)
+    sample = np.sort(rng.choice(len(sizes), size=min(args.batch_cpu_worlds, len(sizes)), replace=False))
+
+    def run(w):
+        hw = ctypes.c_void_p(f("world_create")(0.0, -10.0))
+        f("world_set_flags")(hw, 0, 1, 0, 0)
+        f("world_load_scene")(hw, b"world", int(sizes[w]), ctypes.c_uint(args.batch_seed + int(w)))
+        for _ in range(args.batch_cpu_steps):
+            f("world_step")(hw, ctypes.c_float(1.0 / 60.0), args.vel, args.pos, 1)
+        f("world_destroy")(hw)
+        return int(sizes[w]) * args.batch_cpu_steps
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        total = sum(ex.map(run, sample))
+    el = time.perf_counter() - t0
+    return {"value": total / el, "unit": "body-steps/s", "cores": threads, "kind": kind,
+            "sample": f"{len(sample)} of the {len(sizes)} worlds (random, seed 0), steps 1-{args.batch_cpu_steps} "
+                      f"of {args.batch_steps}, one world per thread, {threads} threads, {el:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,6 +333,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start evaluation")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-worlds run")
+    ap.add_argument("--batch-worlds", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--batch-seed", type=int, default=1)
+    ap.add_argument("--batch-cpu-worlds", type=int, default=1024)
+    ap.add_argument("--batch-cpu-steps", type=int, default=500)
     ap.add_argument("--log-seed", type=int, default=2)
     ap.add_argument("--eval-seed", type=int, default=1001)
     ap.add_argument("--log-capacity", type=int, default=10_000_000)
@@ -279,6 +371,7 @@ def main():
 
     h = make_world(lib, args.bodies, args.seed)
     lib.b2pp_world_sync(h)
+    progress(f"timed run: {args.bodies} bodies, {args.steps} steps")
     t0 = time.perf_counter()
     step(lib, h, args.steps, args.vel, args.pos)
     lib.b2pp_world_sync(h)
@@ -295,6 +388,7 @@ def main():
     # events on the step stream and the velocity kernel's own execution time
     # from events carried by its dispatch packets.
     b2g = box2dpp_amd.b2g()
+    progress(f"timed run done ({1e3 * wall / args.steps:.3f} ms/step); timing pass")
     h = make_world(lib, args.bodies, args.seed)
     ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
     b2g.b2g_set_profiling(ctx, 1)
@@ -319,9 +413,11 @@ def main():
     lib.b2pp_world_destroy(h)
 
     learned = None if args.no_learned else learned_warm_start(lib, args)
+    batched = None if args.no_batched else batched_worlds(lib, args)
 
     cpu = None
     if not args.no_cpu:
+        progress("CPU reference sample")
         cpu = cpu_reference(args.bodies, args.seed, args.vel, args.pos, args.cpu_seconds)
 
     total_phase_ms = sum(ms for _, ms in phases.values())
@@ -348,6 +444,7 @@ def main():
                      "stream_us_per_launch": 1e3 * ph_ms / max(ph_launches, 1)},
         "cpu_baseline": cpu,
         "learned_warm_start": learned,
+        "batched_worlds": batched,
         "final_state": {"contacts": st.contacts, "constraints": st.constraints, "points": st.points,
                         "colours": st.colours, "mean_penetration_m": float(pen[0]), "max_penetration_m": float(pen[1])},
         "phase_ms_per_step": {k: round(v[1] / max(args.profile_steps, 1), 4) for k, v in phases.items()},

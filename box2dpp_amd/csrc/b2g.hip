@@ -1133,20 +1133,50 @@ struct GridView {
     int* cursor;     // fill cursor per bucket
     int* entries;    // proxy ids by bucket
     unsigned long long* ecell;  // the cell each entry was binned for (buckets mix cells)
-    int* big;        // wide proxy ids
+    unsigned long long* big;  // wide proxies: (sub-world << 32) | proxy id, sorted when bigSorted
+    int bigSorted;
     int4* range;     // per proxy cell range (x0, y0, x1, y1); x0 > x1 marks a wide proxy
+    const int* fbody;   // proxy -> body, with bworld: the proxy's sub-world
+    const int* bworld;  // null: every proxy in sub-world 0
 };
+
+// Sub-world of a proxy: proxies of different sub-worlds share no cell.
+__device__ inline int proxyWorld(const GridView& g, int f) { return g.bworld ? g.bworld[g.fbody[f]] : 0; }
+
+// Entries [b0, b1) of the wide-proxy list that may belong to `world`: the
+// whole list when unsorted, else the sub-world's run found by binary search.
+__device__ inline void bigRange(const GridView& g, int nBig, int world, int& b0, int& b1) {
+    if (!g.bigSorted) {
+        b0 = 0;
+        b1 = nBig;
+        return;
+    }
+    auto lower = [&](unsigned long long key) {
+        int lo = 0, hi = nBig;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (g.big[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    b0 = lower((unsigned long long)world << 32);
+    b1 = lower((unsigned long long)(world + 1) << 32);
+}
 
 __device__ inline int cellCoord(float x, float inv) {
     float c = floorf(x * inv);
     c = fminB(fmaxB(c, -1.0e9f), 1.0e9f);
     return (int)c;
 }
-__device__ inline unsigned long long cellKey(int ix, int iy) {
-    return ((unsigned long long)(uint32_t)ix << 32) | (uint32_t)iy;
+// Cell identity within a sub-world: 24 bits of sub-world, 20 bits of each
+// coordinate (coordinates that alias only cost extra overlap tests).
+__device__ inline unsigned long long cellKey(int ix, int iy, int world) {
+    return ((unsigned long long)(uint32_t)(world & 0xFFFFFF) << 40) |
+           ((unsigned long long)((uint32_t)ix & 0xFFFFFu) << 20) | ((uint32_t)iy & 0xFFFFFu);
 }
-__device__ inline uint32_t cellHash(int ix, int iy) {
-    return ((uint32_t)ix * 0x8DA6B343u) ^ ((uint32_t)iy * 0xD8163841u);
+__device__ inline uint32_t cellHash(int ix, int iy, int world) {
+    return ((uint32_t)ix * 0x8DA6B343u) ^ ((uint32_t)iy * 0xD8163841u) ^ ((uint32_t)world * 0xCB1AB31Fu);
 }
 
 __global__ void k_grid_count(const float4* fat, int n, const uint8_t* hasProxy, GridView g, Counters* cnt) {
@@ -1167,12 +1197,13 @@ __global__ void k_grid_count(const float4* fat, int n, const uint8_t* hasProxy, 
     if (cells > kMaxCellsPerProxy) {
         g.range[f] = make_int4(1, 0, 0, 0);
         const int slot = atomicAdd(&cnt->nBig, 1);
-        g.big[slot] = f;
+        g.big[slot] = ((unsigned long long)proxyWorld(g, f) << 32) | (uint32_t)f;
         return;
     }
     g.range[f] = make_int4(x0, y0, x1, y1);
+    const int wf = proxyWorld(g, f);
     for (int iy = y0; iy <= y1; ++iy)
-        for (int ix = x0; ix <= x1; ++ix) atomicAdd(&g.counts[cellHash(ix, iy) & g.bucketMask], 1);
+        for (int ix = x0; ix <= x1; ++ix) atomicAdd(&g.counts[cellHash(ix, iy, wf) & g.bucketMask], 1);
 }
 
 __global__ void k_grid_scatter(int n, GridView g) {
@@ -1180,12 +1211,13 @@ __global__ void k_grid_scatter(int n, GridView g) {
     if (f >= n) return;
     const int4 r = g.range[f];
     if (r.x > r.z) return;
+    const int wf = proxyWorld(g, f);
     for (int iy = r.y; iy <= r.w; ++iy)
         for (int ix = r.x; ix <= r.z; ++ix) {
-            const uint32_t bkt = cellHash(ix, iy) & g.bucketMask;
+            const uint32_t bkt = cellHash(ix, iy, wf) & g.bucketMask;
             const int slot = atomicAdd(&g.cursor[bkt], 1);
             g.entries[g.starts[bkt] + slot] = f;
-            g.ecell[g.starts[bkt] + slot] = cellKey(ix, iy);
+            g.ecell[g.starts[bkt] + slot] = cellKey(ix, iy, wf);
         }
 }
 
@@ -1195,6 +1227,7 @@ __global__ void k_grid_scatter(int n, GridView g) {
 __device__ inline bool acceptPair(const WorldView& w, int a, int b, const unsigned long long* table, uint32_t hmask) {
     const int ba = w.fbody[a], bb = w.fbody[b];
     if (ba == bb) return false;
+    if (w.bworld && w.bworld[ba] != w.bworld[bb]) return false;
     if (table && hashContains(table, hmask, pairKey(a, b))) return false;
     if (w.btype[ba] != BT_DYNAMIC && w.btype[bb] != BT_DYNAMIC) return false;
     if (!filterPass(w, a, b)) return false;
@@ -1238,15 +1271,21 @@ __global__ void k_pairs_query(const float4* fat, const uint8_t* moved, int n, Gr
     };
     const int nBig = cnt->nBig;
     if (r.x > r.z) return;  // a wide proxy's own query runs in k_pairs_wide
+    const int wq = proxyWorld(g, q);
     for (int iy = r.y; iy <= r.w; ++iy)
         for (int ix = r.x; ix <= r.z; ++ix) {
-            const uint32_t bkt = cellHash(ix, iy) & g.bucketMask;
+            const uint32_t bkt = cellHash(ix, iy, wq) & g.bucketMask;
             const int s0 = g.starts[bkt], s1 = g.starts[bkt + 1];
-            const unsigned long long ck = cellKey(ix, iy);
+            const unsigned long long ck = cellKey(ix, iy, wq);
             for (int e = s0; e < s1; ++e)
                 if (g.ecell[e] == ck) consider(g.entries[e], true, ix, iy);
         }
-    for (int i = 0; i < nBig; ++i) consider(g.big[i], false, 0, 0);
+    int b0, b1;
+    bigRange(g, nBig, wq, b0, b1);
+    for (int i = b0; i < b1; ++i) {
+        const unsigned long long e = g.big[i];
+        if ((int)(e >> 32) == wq) consider((int)(e & 0xFFFFFFFFu), false, 0, 0);
+    }
 }
 
 // Queries of moved wide proxies (those not binned in the grid), one thread
@@ -1261,9 +1300,13 @@ __global__ void k_pairs_wide(const float4* fat, const uint8_t* moved, int n, Gri
     if (nBig == 0) return;
     const float4 fp = fat[p];
     const bool movedP = moved[p] != 0;
-    for (int i = 0; i < nBig; ++i) {
-        const int q = g.big[i];
-        if (q == p || !moved[q]) continue;
+    const int wp = proxyWorld(g, p);
+    int b0, b1;
+    bigRange(g, nBig, wp, b0, b1);
+    for (int i = b0; i < b1; ++i) {
+        const unsigned long long e = g.big[i];
+        const int q = (int)(e & 0xFFFFFFFFu);
+        if (q == p || !moved[q] || (int)(e >> 32) != wp) continue;
         if (movedP && p < q) continue;  // reported by p's own query
         if (!overlaps(fat[q], fp)) continue;
         const int a = q < p ? q : p, b = q < p ? p : q;
@@ -1688,6 +1731,8 @@ struct b2g_context {
     int* islandSleep = nullptr;
     int* islandMinSep = nullptr;
     int islandIterCap = 0;
+    int* bworld = nullptr;          // sub-world of each body (batched worlds)
+    bool subWorlds = false;         // some body is outside sub-world 0
 
     // fixture arrays
     int* fbody = nullptr;
@@ -1718,7 +1763,8 @@ struct b2g_context {
 
     // grid
     int nBuckets = 0, capEntries = 0;
-    int *gCounts = nullptr, *gStarts = nullptr, *gCursor = nullptr, *gEntries = nullptr, *gBig = nullptr;
+    int *gCounts = nullptr, *gStarts = nullptr, *gCursor = nullptr, *gEntries = nullptr;
+    unsigned long long *gBig = nullptr, *gBigSorted = nullptr;
     unsigned long long* gCell = nullptr;
     float cellSize = 2.0f;
     bool cellSizeSet = false;
@@ -1785,6 +1831,7 @@ struct b2g_context {
         w.islandAwake = islandAwake;
         w.islandSleep = islandSleep;
         w.islandMinSep = islandMinSep;
+        w.bworld = subWorlds ? bworld : nullptr;
         w.nFixtures = nFixtures;
         w.fbody = fbody; w.shape = shape; w.fmat = fmat; w.ffilter = ffilter; w.fgroup = fgroup; w.fat = fat;
         w.moved = moved;
@@ -1855,6 +1902,7 @@ int growBodies(b2g_context* ctx, int need) {
     HIPCHK(ctx, growArray(ctx->gsleep, n, cap, s));
     HIPCHK(ctx, growArray(ctx->colourMask, n, cap, s));
     HIPCHK(ctx, growArray(ctx->claim, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->bworld, n, cap, s));
     HIPCHK(ctx, growArray(ctx->islandOf, n, cap, s));
     HIPCHK(ctx, growArray(ctx->islandAwake, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->islandSleep, 0, cap, s));
@@ -1865,6 +1913,7 @@ int growBodies(b2g_context* ctx, int need) {
                        0ull);
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(cap - n)), dim3(256), 0, s, ctx->colourMask + n,
                        cap - n, 0ull);
+    hipLaunchKernelGGL(k_fill<int>, dim3(blocks(cap - n)), dim3(256), 0, s, ctx->bworld + n, cap - n, 0);
     LAUNCHCHK(ctx);
     ctx->capBodies = cap;
     return B2G_OK;
@@ -1902,7 +1951,7 @@ int growFixtures(b2g_context* ctx, int need) {
     int nb = 4096;
     while (nb < 4 * cap) nb <<= 1;
     freeArray(ctx->gCounts); freeArray(ctx->gStarts); freeArray(ctx->gCursor); freeArray(ctx->gEntries);
-    freeArray(ctx->gBig); freeArray(ctx->gCell);
+    freeArray(ctx->gBig); freeArray(ctx->gBigSorted); freeArray(ctx->gCell);
     ctx->nBuckets = nb;
     ctx->capEntries = kMaxCellsPerProxy * cap;
     HIPCHK(ctx, hipMalloc((void**)&ctx->gCounts, sizeof(int) * (nb + 1)));
@@ -1910,7 +1959,8 @@ int growFixtures(b2g_context* ctx, int need) {
     HIPCHK(ctx, hipMalloc((void**)&ctx->gCursor, sizeof(int) * (nb + 1)));
     HIPCHK(ctx, hipMalloc((void**)&ctx->gEntries, sizeof(int) * ctx->capEntries));
     HIPCHK(ctx, hipMalloc((void**)&ctx->gCell, sizeof(unsigned long long) * ctx->capEntries));
-    HIPCHK(ctx, hipMalloc((void**)&ctx->gBig, sizeof(int) * cap));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->gBig, sizeof(unsigned long long) * cap));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->gBigSorted, sizeof(unsigned long long) * cap));
     const int rc = growContacts(ctx, 8 * cap + 4096);
     if (rc != B2G_OK) return rc;
     return sizeScratch(ctx);
@@ -2016,6 +2066,9 @@ int runBroadphase(b2g_context* ctx) {
     g.counts = ctx->gCounts; g.starts = ctx->gStarts; g.cursor = ctx->gCursor; g.entries = ctx->gEntries;
     g.ecell = ctx->gCell;
     g.big = ctx->gBig; g.range = ctx->range;
+    g.bigSorted = 0;
+    g.fbody = ctx->fbody;
+    g.bworld = ctx->subWorlds ? ctx->bworld : nullptr;
     hipLaunchKernelGGL(k_fill<int>, dim3(blocks(ctx->nBuckets + 1)), dim3(256), 0, s, ctx->gCounts,
                        ctx->nBuckets + 1, 0);
     hipLaunchKernelGGL(k_fill<int>, dim3(blocks(ctx->nBuckets + 1)), dim3(256), 0, s, ctx->gCursor,
@@ -2027,6 +2080,18 @@ int runBroadphase(b2g_context* ctx) {
     LAUNCHCHK(ctx);
     HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->gCounts, ctx->gStarts, ctx->nBuckets + 1, s));
     hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(nf)), dim3(256), 0, s, nf, g);
+    if (ctx->subWorlds) {
+        // Wide proxies sorted by sub-world, so a query scans its own world's only.
+        int nBig = 0;
+        HIPCHK(ctx, hipMemcpyAsync(&ctx->hcnt->nBig, &ctx->cnt->nBig, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        nBig = ctx->hcnt->nBig;
+        if (nBig > 0) {
+            HIPCHK(ctx, prims_sort_keys_u64(ctx->scratch, ctx->scratchBytes, ctx->gBig, ctx->gBigSorted, nBig, s));
+            g.big = ctx->gBigSorted;
+            g.bigSorted = 1;
+        }
+    }
     // The query only reads the grid, the fat boxes and the move flags, so it
     // is re-run after growing the contact arrays when its pairs do not fit.
     int nNew = 0;
@@ -2804,6 +2869,7 @@ int b2g_destroy(b2g_context* ctx) {
     freeArray(ctx->btype); freeArray(ctx->bflags); freeArray(ctx->xf); freeArray(ctx->pos); freeArray(ctx->pos0);
     freeArray(ctx->vel); freeArray(ctx->force); freeArray(ctx->mass); freeArray(ctx->lcenter);
     freeArray(ctx->gsleep); freeArray(ctx->colourMask); freeArray(ctx->claim); freeArray(ctx->islandOf);
+    freeArray(ctx->bworld);
     freeArray(ctx->islandAwake); freeArray(ctx->islandSleep); freeArray(ctx->islandMinSep);
     freeArray(ctx->fbody); freeArray(ctx->shape); freeArray(ctx->fmat); freeArray(ctx->ffilter);
     freeArray(ctx->fgroup); freeArray(ctx->fat); freeArray(ctx->moved); freeArray(ctx->range);
@@ -2819,7 +2885,7 @@ int b2g_destroy(b2g_context* ctx) {
     freeArray(c.r1); freeArray(c.pmass); freeArray(c.bias); freeArray(c.imp); freeArray(c.K); freeArray(c.NM);
     freeArray(c.info); freeArray(c.lp); freeArray(c.lnp); freeArray(c.lc); freeArray(c.rad); freeArray(c.island);
     freeArray(ctx->gCounts); freeArray(ctx->gStarts); freeArray(ctx->gCursor); freeArray(ctx->gEntries);
-    freeArray(ctx->gBig); freeArray(ctx->gCell);
+    freeArray(ctx->gBig); freeArray(ctx->gBigSorted); freeArray(ctx->gCell);
     freeRef(ctx);
     freeArray(ctx->freshMask); freeArray(ctx->freshNum); freeArray(ctx->freshOff); freeArray(ctx->freshAll);
     freeArray(ctx->freshFeat); freeArray(ctx->freshRef); freeArray(ctx->impSnap); freeArray(ctx->impAlt);
@@ -3230,7 +3296,8 @@ int b2g_pairs_batch(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved,
     const int capPairs = std::max(cap, 1 << 16);
     DevBuf<float4> dfat;
     DevBuf<uint8_t> dmoved;
-    DevBuf<int> counts, starts, cursor, entries, big;
+    DevBuf<int> counts, starts, cursor, entries;
+    DevBuf<unsigned long long> big;
     DevBuf<unsigned long long> ecell;
     DevBuf<int4> range;
     DevBuf<unsigned long long> keys, sorted;
@@ -3252,8 +3319,11 @@ int b2g_pairs_batch(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved,
     g.invCell = 1.0f / cell;
     g.bucketMask = (uint32_t)(nb - 1);
     g.counts = counts.p; g.starts = starts.p; g.cursor = cursor.p; g.entries = entries.p; g.big = big.p;
+    g.bigSorted = 0;
     g.ecell = ecell.p;
     g.range = range.p;
+    g.fbody = nullptr;
+    g.bworld = nullptr;
     hipLaunchKernelGGL(k_grid_count, dim3(blocks(n)), dim3(256), 0, s, dfat.p, n, (const uint8_t*)nullptr, g, dc.p);
     LAUNCHCHK(ctx);
     HIPCHK(ctx, prims_exclusive_sum(scratch.p, sb, counts.p, starts.p, nb + 1, s));
@@ -3700,6 +3770,46 @@ int b2g_check_colouring(b2g_context* ctx, int* conflicts, int* maskErrors) {
     for (int b = 0; b < ctx->nBodies; ++b)
         if (btype[b] == BT_DYNAMIC && rebuilt[b] != mask[b]) ++*maskErrors;
     return B2G_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int b2g_set_body_worlds(b2g_context* ctx, int first, int count, const int32_t* worlds) {
+    if (!ctx || first < 0 || count < 0 || first + count > ctx->nBodies || (count > 0 && !worlds))
+        return B2G_ERR_ARG;
+    if (count == 0) return B2G_OK;
+    bool any = false;
+    for (int i = 0; i < count; ++i) {
+        if (worlds[i] < 0 || worlds[i] > 0xFFFFFF) return fail(ctx, B2G_ERR_ARG, "sub-world id out of range");
+        any = any || worlds[i] != 0;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->bworld + first, worlds, sizeof(int32_t) * count, hipMemcpyHostToDevice,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (any && !ctx->subWorlds) {
+        ctx->subWorlds = true;
+        ctx->newFixture = true;  // re-run the broad-phase with sub-world cells
+    }
+    return B2G_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int b2g_abi_sizes(int32_t* out, int cap) {
+    const int32_t sizes[] = {
+        (int32_t)sizeof(b2g_shape),       (int32_t)sizeof(b2g_manifold),     (int32_t)sizeof(b2g_body_state),
+        (int32_t)sizeof(b2g_fixture_def), (int32_t)sizeof(b2g_world_def),    (int32_t)sizeof(b2g_step_params),
+        (int32_t)sizeof(b2g_step_stats),  (int32_t)sizeof(b2g_phase_profile), (int32_t)sizeof(b2g_mlp_params),
+        (int32_t)sizeof(b2g_aabb),        (int32_t)sizeof(b2g_transform),
+    };
+    const int n = (int)(sizeof(sizes) / sizeof(sizes[0]));
+    if (!out) return n;
+    for (int i = 0; i < n && i < cap; ++i) out[i] = sizes[i];
+    return n;
 }
 
 }  // extern "C"

@@ -173,6 +173,7 @@ struct WorldView {
     int* islandAwake;  // per root: the island holds an awake body
     int* islandSleep;  // per root: 1 if a body has rested less than the time to sleep
     int* islandMinSep; // per position pass and root: 1 if a point is deeper than 3 * linearSlop
+    int* bworld;       // sub-world per body (null: a single world)
     // fixtures / proxies
     int nFixtures;
     int* fbody;

@@ -20,7 +20,8 @@ struct Runner {
     explicit Runner(float gx, float gy) : world(b2Vec2(gx, gy)) {}
 };
 
-int loadScene(Runner* r, const b2scene::BodyDef* bodies, int nb, const b2scene::FixtureDef* fx, int nf) {
+int loadScene(Runner* r, const b2scene::BodyDef* bodies, int nb, const b2scene::FixtureDef* fx, int nf,
+              bool flush = true) {
     for (int i = 0; i < nb; ++i) {
         const b2scene::BodyDef& d = bodies[i];
         b2BodyDef bd;
@@ -80,7 +81,7 @@ int loadScene(Runner* r, const b2scene::BodyDef* bodies, int nb, const b2scene::
             r->fixtures.push_back(body->CreateFixture(&fd));
         }
     }
-    return r->world.Flush();
+    return flush ? r->world.Flush() : 0;
 }
 
 }  // namespace
@@ -121,6 +122,26 @@ int b2pp_world_load(void* h, const b2scene::BodyDef* bodies, int nb, const b2sce
 int b2pp_world_load_scene(void* h, const char* kind, int n, unsigned seed) {
     b2scene::Scene s = b2scene::byName(kind, n, seed);
     return loadScene((Runner*)h, s.bodies.data(), (int)s.bodies.size(), s.fixtures.data(), (int)s.fixtures.size());
+}
+
+// Batched worlds: nWorlds worlds of the batched scene, each in its own
+// sub-world of this b2World (dynamic body counts log-uniform in [lo, hi]
+// from the seed, world w built from seed + w); sizes[w] receives world w's
+// dynamic body count. World w's bodies are contiguous: its container, then
+// its dynamic bodies.
+int b2pp_world_load_batch(void* h, int nWorlds, unsigned seed, int lo, int hi, int* sizes) {
+    Runner* r = (Runner*)h;
+    if (nWorlds <= 0 || lo <= 0 || hi < lo) return B2G_ERR_ARG;
+    const std::vector<int> n = b2scene::worldSizes(nWorlds, seed, lo, hi);
+    for (int w = 0; w < nWorlds; ++w) {
+        if (w > 0) r->world.BeginSubWorld();
+        const b2scene::Scene s = b2scene::batchedWorld(n[w], seed + (unsigned)w);
+        const int rc = loadScene(r, s.bodies.data(), (int)s.bodies.size(), s.fixtures.data(), (int)s.fixtures.size(),
+                                 false);
+        if (rc != 0) return rc;
+        if (sizes) sizes[w] = n[w];
+    }
+    return r->world.Flush();  // one upload of every world
 }
 
 int b2pp_world_step(void* h, float dt, int vel, int pos, int nsteps) {

@@ -367,6 +367,7 @@ b2World::~b2World() {
 b2Body* b2World::CreateBody(const b2BodyDef* def) {
     if (IsLocked() || def == nullptr) return nullptr;
     b2Body* b = new b2Body(def, this, (int32_t)m_bodies.size());
+    b->m_subWorld = m_currentSubWorld;
     m_bodies.push_back(b);
     b->m_prev = nullptr;
     b->m_next = m_bodyList;
@@ -486,6 +487,13 @@ int32_t b2World::Upload() {
         int first = 0;
         rc = b2g_add_bodies(m_ctx, st.data(), (int)st.size(), &first);
         if (rc != B2G_OK) return rc;
+        if (m_subWorldCount > 0) {
+            std::vector<int32_t> ids;
+            ids.reserve(nb - m_uploadedBodies);
+            for (int i = m_uploadedBodies; i < nb; ++i) ids.push_back(m_bodies[i]->m_subWorld);
+            rc = b2g_set_body_worlds(m_ctx, first, (int)ids.size(), ids.data());
+            if (rc != B2G_OK) return rc;
+        }
         m_uploadedBodies = nb;
     }
     const int nf = (int)m_fixtures.size();

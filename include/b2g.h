@@ -353,6 +353,21 @@ int b2g_fat_rule_batch(b2g_context* ctx, const b2g_aabb* fat, const b2g_aabb* ti
 /* Rotation used inside the step for n angles (sine, cosine). */
 int b2g_sincos_batch(b2g_context* ctx, const float* angles, int n, float* s, float* c);
 
+/* Batched worlds: sets the sub-world of bodies [first, first + count) (ids
+ * 0 .. 2^24 - 1; every body starts in sub-world 0). Fixtures of bodies in
+ * different sub-worlds never form contacts, and the broad-phase keys its
+ * cells by sub-world, so one context advances many independent worlds in one
+ * step. Set before the bodies' first step. No reference counterpart (the
+ * reference steps one b2World at a time, Box2D/Dynamics/b2World.h:96). */
+int b2g_set_body_worlds(b2g_context* ctx, int first, int count, const int32_t* worlds);
+
+/* Sizes in bytes of the ABI structs, in this order: b2g_shape,
+ * b2g_manifold, b2g_body_state, b2g_fixture_def, b2g_world_def,
+ * b2g_step_params, b2g_step_stats, b2g_phase_profile, b2g_mlp_params,
+ * b2g_aabb, b2g_transform. Returns the count (out may be null). Needs no
+ * device; lets bindings check their struct layouts. */
+int b2g_abi_sizes(int32_t* out, int cap);
+
 /* Checks the colouring of the last coloured step: *conflicts counts dynamic
  * bodies met twice within one colour (must be 0: a colour is solved in
  * parallel), *maskErrors bodies whose colour mask differs from the colours

@@ -197,6 +197,8 @@ public:
     const b2World* GetWorld() const { return m_world; }
     // Device index of this body (its creation order in the world).
     int32_t GetIndex() const { return m_index; }
+    // Sub-world this body was created in (b2World::BeginSubWorld).
+    int32_t GetSubWorld() const { return m_subWorld; }
 
 private:
     friend class b2World;
@@ -228,6 +230,7 @@ private:
     float m_sleepTime = 0.0f;
     void* m_userData = nullptr;
     bool m_dirty = false;
+    int32_t m_subWorld = 0;
 };
 
 // Host view of one device contact, valid until the next Step.
@@ -315,6 +318,13 @@ public:
     // Mean and max penetration over touching points and the point count,
     // from device world manifolds.
     void GetPenetration(double* meanMax3) const;
+    // Batched worlds: bodies created after BeginSubWorld() belong to a new
+    // sub-world (ids 1, 2, ...; bodies created before are in sub-world 0).
+    // Fixtures of different sub-worlds never collide, so one b2World holds
+    // many independent worlds that Step advances together on one device
+    // context. Sub-worlds are set before the bodies' first Step.
+    int32_t BeginSubWorld() { return ++m_subWorldCount, m_currentSubWorld = m_subWorldCount; }
+    int32_t GetSubWorldCount() const { return m_subWorldCount + 1; }
 
 private:
     friend class b2Body;
@@ -349,6 +359,8 @@ private:
     std::vector<uint64_t> m_touchingKeys;  // touching fixture pairs after the last Step
     std::vector<int32_t> m_removedFixtures;  // proxies to drop at the next upload
     std::vector<int32_t> m_updatedFixtures;  // material/filter changes to upload
+    int32_t m_subWorldCount = 0;
+    int32_t m_currentSubWorld = 0;
 };
 
 }  // namespace box2d

@@ -86,9 +86,22 @@ class World:
         self.e = eng
         self.h = VP(eng.fn("world_create")(gravity[0], gravity[1]))
         eng.fn("world_set_flags")(self.h, int(sleep), int(warm), 0, 0)
+        if kind is None:
+            return
         rc = eng.fn("world_load_scene")(self.h, kind.encode(), int(n), ctypes.c_uint(seed))
         if rc != 0:
             raise RuntimeError(f"{eng.name}: load {kind} failed rc={rc} {self.error()}")
+
+    @classmethod
+    def batch(cls, eng: "Engine", n_worlds: int, seed: int, lo: int, hi: int, sleep: bool = False):
+        """Batched worlds in sub-worlds of one GPU world; returns (world,
+        dynamic body count per sub-world)."""
+        w = cls(eng, None, 0, 0, sleep, True, (0.0, -10.0))
+        sizes = np.zeros(n_worlds, np.int32)
+        rc = eng.lib.b2pp_world_load_batch(w.h, int(n_worlds), ctypes.c_uint(seed), int(lo), int(hi), ptr(sizes))
+        if rc != 0:
+            raise RuntimeError(f"batch load failed rc={rc} {w.error()}")
+        return w, sizes
 
     def error(self) -> str:
         if self.e.name == "gpu":
