@@ -99,6 +99,28 @@ struct ConView {
     int* island;      // island root of the constraint's non-static bodies
 };
 
+// Revolute joints (b2RevoluteJoint, REF/Dynamics/Joints/b2RevoluteJoint.h),
+// indexed by creation order; `order` lists them by colour.
+enum : int {
+    JF_LIMIT = 1, JF_MOTOR = 2, JF_COLLIDE = 4, JF_REMOVED = 8,
+    JL_INACTIVE = 0, JL_LOWER = 1, JL_UPPER = 2, JL_EQUAL = 3,  // limit state in bits 8-9
+};
+struct JointView {
+    int n;
+    const int* order;  // joint per slot, colour by colour
+    int2* bodies;
+    float4* anchors;   // localAnchorA, localAnchorB
+    float4* limits;    // referenceAngle, lowerAngle, upperAngle, maxMotorTorque
+    float* motorSpeed;
+    int* flags;        // JF_* | limit state << 8
+    float4* impulse;   // impulse x y z, motor impulse
+    float4* r;         // rA, rB of the step
+    float4* masses;    // invMassA invIA invMassB invIB
+    float4* K0;        // mass matrix m11 m12 m13 m22
+    float2* K1;        // m23 m33
+    float* motorMass;
+};
+
 __device__ inline int encodeOrdered(float f) {
     int i = __float_as_int(f);
     return i >= 0 ? i : i ^ 0x7FFFFFFF;
@@ -136,6 +158,23 @@ __device__ inline uint32_t hashU64(unsigned long long k) {
     return (uint32_t)k;
 }
 
+// Open-addressing hash set of 64-bit keys (~0 marks an empty slot).
+__device__ inline bool hashContains(const unsigned long long* table, uint32_t mask, unsigned long long key) {
+    uint32_t h = hashU64(key) & mask;
+    for (;;) {
+        const unsigned long long v = table[h];
+        if (v == key) return true;
+        if (v == ~0ull) return false;
+        h = (h + 1) & mask;
+    }
+}
+
+// b2Body::ShouldCollide's joint test (REF/Dynamics/b2Body.cpp:404-414): a
+// joint with collideConnected off keeps its two bodies from colliding.
+__device__ inline bool jointKeepsApart(const WorldView& w, int bA, int bB) {
+    return w.jointPairs && hashContains(w.jointPairs, w.jointPairMask, pairKey(bA, bB));
+}
+
 // Sets the awake bit; a body that was asleep restarts its sleep timer
 // (b2Body::SetAwake(true), REF/Dynamics/b2Body.h:635-644).
 __device__ inline void wakeBody(const WorldView& w, int b) {
@@ -166,7 +205,7 @@ __global__ void k_collide(WorldView w, Counters* cnt) {
     const int bA = w.fbody[fA], bB = w.fbody[fB];
     if (flags & CF_FILTER) {
         const bool dynAny = w.btype[bA] == BT_DYNAMIC || w.btype[bB] == BT_DYNAMIC;
-        if (!dynAny || !filterPass(w, fA, fB)) {
+        if (!dynAny || jointKeepsApart(w, bA, bB) || !filterPass(w, fA, fB)) {
             if (flags & CF_TOUCHING) atomicAdd(&cnt->endEvents, 1);
             if (w.cman[c].pointCount > 0 && !sensorOf(w, fA) && !sensorOf(w, fB)) {
                 wakeBody(w, bA);
@@ -378,16 +417,6 @@ __global__ void k_hash_insert(WorldView w, unsigned long long* table, uint32_t m
     for (;;) {
         const unsigned long long prev = atomicCAS(&table[h], ~0ull, key);
         if (prev == ~0ull || prev == key) return;
-        h = (h + 1) & mask;
-    }
-}
-
-__device__ inline bool hashContains(const unsigned long long* table, uint32_t mask, unsigned long long key) {
-    uint32_t h = hashU64(key) & mask;
-    for (;;) {
-        const unsigned long long v = table[h];
-        if (v == key) return true;
-        if (v == ~0ull) return false;
         h = (h + 1) & mask;
     }
 }
@@ -1003,6 +1032,290 @@ __global__ void k_solve_position(WorldView w, ConView cv, int begin, int end, Co
     if (threadIdx.x == 0 && blockMin < 0) atomicMin(&cnt->minSep[it], blockMin);
 }
 
+// ---------------------------------------------------------------- joints
+// A joint is solved when one of its bodies is in an awake island and both
+// are active (the joints b2World::Solve's island search adds,
+// REF/Dynamics/b2World.cpp:502-529).
+__device__ inline bool jointActive(const WorldView& w, int2 b) {
+    return ((w.bflags[b.x] & BF_ISLAND) || (w.bflags[b.y] & BF_ISLAND)) && (w.bflags[b.x] & BF_ACTIVE) &&
+           (w.bflags[b.y] & BF_ACTIVE);
+}
+
+// b2Mat33::Solve33 / Solve22 (REF/Common/b2Math.cpp:28-56) for the symmetric
+// mass matrix ex = (m11, m12, m13), ey = (m12, m22, m23), ez = (m13, m23, m33).
+struct Sym33 { float m11, m12, m13, m22, m23, m33; };
+__device__ inline float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return ax * bx + ay * by + az * bz;
+}
+__device__ inline void solve33(const Sym33& m, float bx, float by, float bz, float& x, float& y, float& z) {
+    // ey x ez, b x ez, ey x b (b2Cross of b2Vec3)
+    const float c1x = m.m22 * m.m33 - m.m23 * m.m23, c1y = m.m23 * m.m13 - m.m12 * m.m33,
+                c1z = m.m12 * m.m23 - m.m22 * m.m13;
+    float det = dot3(m.m11, m.m12, m.m13, c1x, c1y, c1z);
+    if (det != 0.0f) det = 1.0f / det;
+    const float c2x = by * m.m33 - bz * m.m23, c2y = bz * m.m13 - bx * m.m33, c2z = bx * m.m23 - by * m.m13;
+    const float c3x = m.m22 * bz - m.m23 * by, c3y = m.m23 * bx - m.m12 * bz, c3z = m.m12 * by - m.m22 * bx;
+    x = det * dot3(bx, by, bz, c1x, c1y, c1z);
+    y = det * dot3(m.m11, m.m12, m.m13, c2x, c2y, c2z);
+    z = det * dot3(m.m11, m.m12, m.m13, c3x, c3y, c3z);
+}
+__device__ inline V2 solve22(const Sym33& m, V2 b) {
+    const float a11 = m.m11, a12 = m.m12, a21 = m.m12, a22 = m.m22;
+    float det = a11 * a22 - a12 * a21;
+    if (det != 0.0f) det = 1.0f / det;
+    return mk(det * (a22 * b.x - a12 * b.y), det * (a11 * b.y - a21 * b.x));
+}
+
+// b2RevoluteJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2RevoluteJoint.cpp:66-198
+__device__ inline void jointInitOne(const WorldView& w, const JointView& jv, int j, float dtRatio, int warm) {
+    const int2 b = jv.bodies[j];
+    if (!jointActive(w, b)) return;
+    const float4 msA = w.mass[b.x], msB = w.mass[b.y];
+    const float mA = msA.x, iA = msA.y, mB = msB.x, iB = msB.y;
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float aA = w.pos[b.x].z, aB = w.pos[b.y].z;
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const Rot qA = rotOf(aA), qB = rotOf(aB);
+    const float4 an = jv.anchors[j];
+    const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    const bool fixedRotation = (iA + iB == 0.0f);
+    Sym33 K;
+    K.m11 = mA + mB + rA.y * rA.y * iA + rB.y * rB.y * iB;
+    K.m12 = -rA.y * rA.x * iA - rB.y * rB.x * iB;
+    K.m13 = -rA.y * iA - rB.y * iB;
+    K.m22 = mA + mB + rA.x * rA.x * iA + rB.x * rB.x * iB;
+    K.m23 = rA.x * iA + rB.x * iB;
+    K.m33 = iA + iB;
+    float motorMass = iA + iB;
+    if (motorMass > 0.0f) motorMass = 1.0f / motorMass;
+    int flags = jv.flags[j];
+    float4 imp = jv.impulse[j];
+    if (!(flags & JF_MOTOR) || fixedRotation) imp.w = 0.0f;
+    int state = (flags >> 8) & 3;
+    const float4 lim = jv.limits[j];
+    if ((flags & JF_LIMIT) && !fixedRotation) {
+        const float jointAngle = aB - aA - lim.x;
+        if (absB(lim.z - lim.y) < 2.0f * kAngularSlop) {
+            state = JL_EQUAL;
+        } else if (jointAngle <= lim.y) {
+            if (state != JL_LOWER) imp.z = 0.0f;
+            state = JL_LOWER;
+        } else if (jointAngle >= lim.z) {
+            if (state != JL_UPPER) imp.z = 0.0f;
+            state = JL_UPPER;
+        } else {
+            state = JL_INACTIVE;
+            imp.z = 0.0f;
+        }
+    } else {
+        state = JL_INACTIVE;
+    }
+    if (warm) {
+        imp.x *= dtRatio;
+        imp.y *= dtRatio;
+        imp.z *= dtRatio;
+        imp.w *= dtRatio;
+        const V2 P = mk(imp.x, imp.y);
+        vA -= mA * P;
+        wA -= iA * (cross(rA, P) + imp.w + imp.z);
+        vB += mB * P;
+        wB += iB * (cross(rB, P) + imp.w + imp.z);
+    } else {
+        imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    jv.flags[j] = (flags & 0xFF) | (state << 8);
+    jv.impulse[j] = imp;
+    jv.r[j] = make_float4(rA.x, rA.y, rB.x, rB.y);
+    jv.masses[j] = make_float4(mA, iA, mB, iB);
+    jv.K0[j] = make_float4(K.m11, K.m12, K.m13, K.m22);
+    jv.K1[j] = make_float2(K.m23, K.m33);
+    jv.motorMass[j] = motorMass;
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+__global__ void k_joint_init(WorldView w, JointView jv, int begin, int end, float dtRatio, int warm) {
+    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    jointInitOne(w, jv, jv.order[k], dtRatio, warm);
+}
+
+// b2RevoluteJoint::SolveVelocityConstraints, REF b2RevoluteJoint.cpp:200-292
+__device__ inline void jointSolveVelocityOne(const WorldView& w, const JointView& jv, int j, float h) {
+    const int2 b = jv.bodies[j];
+    if (!jointActive(w, b)) return;
+    const float4 ms = jv.masses[j];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const float4 r = jv.r[j];
+    const V2 rA = mk(r.x, r.y), rB = mk(r.z, r.w);
+    const int flags = jv.flags[j];
+    const int state = (flags >> 8) & 3;
+    float4 imp = jv.impulse[j];
+    const bool fixedRotation = (iA + iB == 0.0f);
+    if ((flags & JF_MOTOR) && state != JL_EQUAL && !fixedRotation) {
+        const float Cdot = wB - wA - jv.motorSpeed[j];
+        float impulse = -jv.motorMass[j] * Cdot;
+        const float oldImpulse = imp.w;
+        const float maxImpulse = h * jv.limits[j].w;
+        imp.w = clampB(imp.w + impulse, -maxImpulse, maxImpulse);
+        impulse = imp.w - oldImpulse;
+        wA -= iA * impulse;
+        wB += iB * impulse;
+    }
+    const float4 k0 = jv.K0[j];
+    const float2 k1 = jv.K1[j];
+    const Sym33 K{k0.x, k0.y, k0.z, k0.w, k1.x, k1.y};
+    if ((flags & JF_LIMIT) && state != JL_INACTIVE && !fixedRotation) {
+        const V2 Cdot1 = vB + cross(wB, rB) - vA - cross(wA, rA);
+        const float Cdot2 = wB - wA;
+        float ix, iy, iz;
+        solve33(K, Cdot1.x, Cdot1.y, Cdot2, ix, iy, iz);
+        ix = -ix;
+        iy = -iy;
+        iz = -iz;
+        if (state == JL_EQUAL) {
+            imp.x += ix;
+            imp.y += iy;
+            imp.z += iz;
+        } else {
+            const float newImpulse = imp.z + iz;
+            const bool reduce = state == JL_LOWER ? newImpulse < 0.0f : newImpulse > 0.0f;
+            if (reduce) {
+                const V2 rhs = -Cdot1 + imp.z * mk(K.m13, K.m23);
+                const V2 reduced = solve22(K, rhs);
+                ix = reduced.x;
+                iy = reduced.y;
+                iz = -imp.z;
+                imp.x += reduced.x;
+                imp.y += reduced.y;
+                imp.z = 0.0f;
+            } else {
+                imp.x += ix;
+                imp.y += iy;
+                imp.z += iz;
+            }
+        }
+        const V2 P = mk(ix, iy);
+        vA -= mA * P;
+        wA -= iA * (cross(rA, P) + iz);
+        vB += mB * P;
+        wB += iB * (cross(rB, P) + iz);
+    } else {
+        const V2 Cdot = vB + cross(wB, rB) - vA - cross(wA, rA);
+        const V2 impulse = solve22(K, -Cdot);
+        imp.x += impulse.x;
+        imp.y += impulse.y;
+        vA -= mA * impulse;
+        wA -= iA * cross(rA, impulse);
+        vB += mB * impulse;
+        wB += iB * cross(rB, impulse);
+    }
+    jv.impulse[j] = imp;
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+__global__ void k_joint_solve_velocity(WorldView w, JointView jv, int begin, int end, float h) {
+    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    jointSolveVelocityOne(w, jv, jv.order[k], h);
+}
+
+// b2RevoluteJoint::SolvePositionConstraints, REF b2RevoluteJoint.cpp:294-378.
+// A joint left with an error above the slops marks its island's pass as
+// failed (the jointsOkay of b2Island::Solve, REF/Dynamics/b2Island.cpp:310-328).
+__device__ inline void jointSolvePositionOne(const WorldView& w, const JointView& jv, int j, int it) {
+    const int2 b = jv.bodies[j];
+    if (!jointActive(w, b)) return;
+    const int root = w.islandOf[w.btype[b.x] != BT_STATIC ? b.x : b.y];
+    if (it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0) return;  // island done
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    V2 cA = mk(pA.x, pA.y), cB = mk(pB.x, pB.y);
+    float aA = pA.z, aB = pB.z;
+    const float4 ms = jv.masses[j];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const int flags = jv.flags[j];
+    const int state = (flags >> 8) & 3;
+    const float4 lim = jv.limits[j];
+    float angularError = 0.0f, positionError = 0.0f;
+    const bool fixedRotation = (iA + iB == 0.0f);
+    if ((flags & JF_LIMIT) && state != JL_INACTIVE && !fixedRotation) {
+        const float angle = aB - aA - lim.x;
+        float limitImpulse = 0.0f;
+        const float motorMass = jv.motorMass[j];
+        if (state == JL_EQUAL) {
+            const float C = clampB(angle - lim.y, -kMaxAngularCorrection, kMaxAngularCorrection);
+            limitImpulse = -motorMass * C;
+            angularError = absB(C);
+        } else if (state == JL_LOWER) {
+            float C = angle - lim.y;
+            angularError = -C;
+            C = clampB(C + kAngularSlop, -kMaxAngularCorrection, 0.0f);
+            limitImpulse = -motorMass * C;
+        } else if (state == JL_UPPER) {
+            float C = angle - lim.z;
+            angularError = C;
+            C = clampB(C - kAngularSlop, 0.0f, kMaxAngularCorrection);
+            limitImpulse = -motorMass * C;
+        }
+        aA -= iA * limitImpulse;
+        aB += iB * limitImpulse;
+    }
+    {
+        const Rot qA = rotOf(aA), qB = rotOf(aB);
+        const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+        const float4 an = jv.anchors[j];
+        const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+        const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+        const V2 C = cB + rB - cA - rA;
+        positionError = lengthOf(C);
+        const float k11 = mA + mB + iA * rA.y * rA.y + iB * rB.y * rB.y;
+        const float k21 = -iA * rA.x * rA.y - iB * rB.x * rB.y;
+        const float k12 = k21;
+        const float k22 = mA + mB + iA * rA.x * rA.x + iB * rB.x * rB.x;
+        // b2Mat22::Solve (REF/Common/b2Math.h:242-254)
+        float det = k11 * k22 - k12 * k21;
+        if (det != 0.0f) det = 1.0f / det;
+        const V2 impulse = -mk(det * (k22 * C.x - k12 * C.y), det * (k11 * C.y - k21 * C.x));
+        cA -= mA * impulse;
+        aA -= iA * cross(rA, impulse);
+        cB += mB * impulse;
+        aB += iB * cross(rB, impulse);
+    }
+    if (w.btype[b.x] == BT_DYNAMIC) w.pos[b.x] = make_float4(cA.x, cA.y, aA, pA.w);
+    if (w.btype[b.y] == BT_DYNAMIC) w.pos[b.y] = make_float4(cB.x, cB.y, aB, pB.w);
+    if (!(positionError <= kLinearSlop && angularError <= kAngularSlop)) {
+        int* f = &w.islandMinSep[it * w.nBodies + root];
+        if (*f == 0) *f = 1;
+    }
+}
+
+__global__ void k_joint_solve_position(WorldView w, JointView jv, int begin, int end, int it) {
+    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    jointSolvePositionOne(w, jv, jv.order[k], it);
+}
+
+// Island labelling through joints whose bodies are both non-static and active.
+__global__ void k_island_hook_joints(WorldView w, JointView jv, int* changed) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= jv.n) return;
+    const int2 b = jv.bodies[j];
+    if (jv.flags[j] & JF_REMOVED) return;
+    if (w.btype[b.x] == BT_STATIC || w.btype[b.y] == BT_STATIC) return;
+    if (!(w.bflags[b.x] & BF_ACTIVE) || !(w.bflags[b.y] & BF_ACTIVE)) return;
+    const int la = w.islandOf[b.x], lb = w.islandOf[b.y];
+    if (la == lb) return;
+    atomicMin(&w.islandOf[la > lb ? la : lb], la < lb ? la : lb);
+    *changed = 1;
+}
+
 // Copy-back transform of every solved body (b2Body::SynchronizeTransform,
 // REF/Dynamics/b2Body.h:836-840). Static bodies keep their creation-time
 // transform, which the reference's recomputation reproduces exactly.
@@ -1229,6 +1542,7 @@ __device__ inline bool acceptPair(const WorldView& w, int a, int b, const unsign
     if (ba == bb) return false;
     if (w.bworld && w.bworld[ba] != w.bworld[bb]) return false;
     if (table && hashContains(table, hmask, pairKey(a, b))) return false;
+    if (jointKeepsApart(w, ba, bb)) return false;
     if (w.btype[ba] != BT_DYNAMIC && w.btype[bb] != BT_DYNAMIC) return false;
     if (!filterPass(w, a, b)) return false;
     const int ta = w.shape[a].type, tb = w.shape[b].type;
@@ -1344,6 +1658,26 @@ __global__ void k_append_contacts(WorldView w, const unsigned long long* keys, i
     }
 }
 
+// Contacts between bodies that a new joint keeps apart are flagged for
+// filtering (b2World::CreateJoint, REF/Dynamics/b2World.cpp:250-266).
+__global__ void k_flag_joint_contacts(WorldView w) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    if (jointKeepsApart(w, w.fbody[w.cfA[c]], w.fbody[w.cfB[c]])) w.cflags[c] |= CF_FILTER;
+}
+
+__global__ void k_hash_insert_keys(const unsigned long long* keys, int n, unsigned long long* table, uint32_t mask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long key = keys[i];
+    uint32_t h = hashU64(key) & mask;
+    for (;;) {
+        const unsigned long long prev = atomicCAS(&table[h], ~0ull, key);
+        if (prev == ~0ull || prev == key) return;
+        h = (h + 1) & mask;
+    }
+}
+
 // Proxy removal: an inverted fat box overlaps nothing.
 __global__ void k_remove_fixtures(WorldView w, const int* ids, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1413,7 +1747,7 @@ __global__ void k_collide_serial(WorldView w, RefView r, Counters* cnt) {
         const bool destroyWake = w.cman[c].pointCount > 0 && !sensorOf(w, fA) && !sensorOf(w, fB);
         if (flags & CF_FILTER) {
             const bool dynAny = w.btype[bA] == BT_DYNAMIC || w.btype[bB] == BT_DYNAMIC;
-            if (!dynAny || !filterPass(w, fA, fB)) {
+            if (!dynAny || jointKeepsApart(w, bA, bB) || !filterPass(w, fA, fB)) {
                 if (flags & CF_TOUCHING) cnt->endEvents++;
                 if (destroyWake) { wakeBody(w, bA); wakeBody(w, bB); }
                 w.cflags[c] = flags | CF_DEAD;
@@ -1788,6 +2122,24 @@ struct b2g_context {
     RefView ref{};
     int refCapBodies = 0, refCapContacts = 0;
 
+    // revolute joints
+    int nJoints = 0, capJoints = 0;
+    int2* jBodies = nullptr;
+    float4 *jAnchors = nullptr, *jLimits = nullptr, *jImpulse = nullptr, *jR = nullptr, *jMasses = nullptr,
+           *jK0 = nullptr, *jImpulseSnap = nullptr;
+    float2* jK1 = nullptr;
+    float *jMotorSpeed = nullptr, *jMotorMass = nullptr;
+    int *jFlags = nullptr, *jFlagsSnap = nullptr, *jOrder = nullptr;
+    std::vector<int2> hJointBodies;
+    std::vector<int> hJointKeepApart;
+    std::vector<int> hJointRemoved;
+    std::vector<int> jointColourStart;  // colour c: slots [start[c], start[c + 1]) of jOrder
+    bool jointsDirty = false;
+    unsigned long long* jointPairs = nullptr;
+    uint32_t jointPairMask = 0;
+    int jointPairCap = 0;
+    bool anyJointPair = false;
+
     // learned warm start: network, fresh points of the step, training log,
     // snapshots for the fallback and the residual trace
     b2mlp::Net mlp;
@@ -1822,6 +2174,16 @@ struct b2g_context {
     double profMs[32] = {};
     int profLaunches[32] = {};
 
+    JointView jview() const {
+        JointView j;
+        j.n = nJoints;
+        j.order = jOrder;
+        j.bodies = jBodies; j.anchors = jAnchors; j.limits = jLimits; j.motorSpeed = jMotorSpeed;
+        j.flags = jFlags; j.impulse = jImpulse; j.r = jR; j.masses = jMasses; j.K0 = jK0; j.K1 = jK1;
+        j.motorMass = jMotorMass;
+        return j;
+    }
+
     WorldView view() const {
         WorldView w;
         w.nBodies = nBodies;
@@ -1832,6 +2194,8 @@ struct b2g_context {
         w.islandSleep = islandSleep;
         w.islandMinSep = islandMinSep;
         w.bworld = subWorlds ? bworld : nullptr;
+        w.jointPairs = anyJointPair ? jointPairs : nullptr;
+        w.jointPairMask = jointPairMask;
         w.nFixtures = nFixtures;
         w.fbody = fbody; w.shape = shape; w.fmat = fmat; w.ffilter = ffilter; w.fgroup = fgroup; w.fat = fat;
         w.moved = moved;
@@ -2214,6 +2578,109 @@ void profCollect(b2g_context* ctx) {
     ctx->evUsed = 0;
 }
 
+// ---------------------------------------------------------------- joints
+int growJoints(b2g_context* ctx, int need) {
+    if (need <= ctx->capJoints) return B2G_OK;
+    const int cap = std::max(need, ctx->capJoints * 2 + 64);
+    const int n = ctx->nJoints;
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, growArray(ctx->jBodies, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->jAnchors, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->jLimits, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->jMotorSpeed, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->jFlags, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->jImpulse, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->jR, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->jMasses, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->jK0, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->jK1, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->jMotorMass, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->jOrder, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->jImpulseSnap, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->jFlagsSnap, 0, cap, s));
+    ctx->capJoints = cap;
+    ctx->jointsDirty = true;
+    return B2G_OK;
+}
+
+// Joint colours: joints in creation order take the lowest colour free at
+// their dynamic bodies (joint topology changes only when joints are added
+// or removed, so this runs on the host then). Colour 64 collects joints
+// whose bodies have no free colour left; it is solved one joint at a time.
+// Removed joints take no slot.
+int recolourJoints(b2g_context* ctx) {
+    const int nj = ctx->nJoints, nb = ctx->nBodies;
+    std::vector<int> btype(nb);
+    if (nb > 0) {
+        HIPCHK(ctx, hipMemcpyAsync(btype.data(), ctx->btype, sizeof(int) * nb, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<unsigned long long> used(nb, 0ull);
+    std::vector<int> colour(nj);
+    std::vector<int> count(kColourSlots, 0);
+    for (int j = 0; j < nj; ++j) {
+        colour[j] = -1;
+        if (ctx->hJointRemoved[j]) continue;
+        const int2 b = ctx->hJointBodies[j];
+        const bool dA = btype[b.x] == BT_DYNAMIC, dB = btype[b.y] == BT_DYNAMIC;
+        const unsigned long long u = (dA ? used[b.x] : 0ull) | (dB ? used[b.y] : 0ull);
+        int c = kOverflowColour;
+        if (~u != 0ull) {
+            c = __builtin_ctzll(~u);
+            if (dA) used[b.x] |= 1ull << c;
+            if (dB) used[b.y] |= 1ull << c;
+        }
+        colour[j] = c;
+        ++count[c];
+    }
+    ctx->jointColourStart.assign(kColourSlots + 1, 0);
+    for (int c = 0; c < kColourSlots; ++c) ctx->jointColourStart[c + 1] = ctx->jointColourStart[c] + count[c];
+    std::vector<int> order(nj), cursor(ctx->jointColourStart.begin(), ctx->jointColourStart.end() - 1);
+    for (int j = 0; j < nj; ++j)
+        if (colour[j] >= 0) order[cursor[colour[j]]++] = j;
+    if (nj > 0)
+        HIPCHK(ctx, hipMemcpyAsync(ctx->jOrder, order.data(), sizeof(int) * nj, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->jointsDirty = false;
+    return B2G_OK;
+}
+
+// Hash set of body pairs that joints keep from colliding.
+int rebuildJointPairs(b2g_context* ctx) {
+    std::vector<unsigned long long> keys;
+    for (int j = 0; j < ctx->nJoints; ++j) {
+        if (!ctx->hJointKeepApart[j] || ctx->hJointRemoved[j]) continue;
+        const int2 b = ctx->hJointBodies[j];
+        const unsigned lo = (unsigned)std::min(b.x, b.y), hi = (unsigned)std::max(b.x, b.y);
+        keys.push_back(((unsigned long long)lo << 32) | hi);
+    }
+    ctx->anyJointPair = !keys.empty();
+    if (keys.empty()) return B2G_OK;
+    int cap = 1024;
+    while (cap < 2 * (int)keys.size()) cap <<= 1;
+    hipStream_t s = ctx->stream;
+    if (cap > ctx->jointPairCap) {
+        freeArray(ctx->jointPairs);
+        HIPCHK(ctx, hipMalloc((void**)&ctx->jointPairs, sizeof(unsigned long long) * cap));
+        ctx->jointPairCap = cap;
+    }
+    ctx->jointPairMask = (uint32_t)(ctx->jointPairCap - 1);
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(ctx->jointPairCap)), dim3(256), 0, s, ctx->jointPairs,
+                       ctx->jointPairCap, ~0ull);
+    unsigned long long* dk = nullptr;
+    HIPCHK(ctx, hipMalloc((void**)&dk, sizeof(unsigned long long) * keys.size()));
+    HIPCHK(ctx, hipMemcpyAsync(dk, keys.data(), sizeof(unsigned long long) * keys.size(), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_hash_insert_keys, dim3(blocks((int)keys.size())), dim3(256), 0, s, dk, (int)keys.size(),
+                       ctx->jointPairs, ctx->jointPairMask);
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    (void)hipFree(dk);
+    if (ctx->nContacts > 0)
+        hipLaunchKernelGGL(k_flag_joint_contacts, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view());
+    LAUNCHCHK(ctx);
+    return B2G_OK;
+}
+
 // ---------------------------------------------------------------- learned warm start
 int ensureFresh(b2g_context* ctx) {
     const int cap = ctx->capContacts;
@@ -2301,6 +2768,8 @@ double hostResidual(const HostConstraints& h, int nCons) {
 
 // Residual curves of the three starts (cold, warm, learned) for the
 // current step; velocities and impulses are restored afterwards.
+// warmStart(bool) starts the velocity phase (contact warm start when true,
+// then the joints' initialisation).
 template <typename WarmFn, typename SolveFn, typename PredictFn>
 int traceResiduals(b2g_context* ctx, const WorldView& w, int nCons, int nFresh, WarmFn& warmStart,
                    SolveFn& solveVelocity, PredictFn& predict) {
@@ -2340,10 +2809,11 @@ int traceResiduals(b2g_context* ctx, const WorldView& w, int nCons, int nFresh, 
         if (v == 0) {
             hipLaunchKernelGGL(k_fill<float4>, dim3(blocks(nCons)), dim3(256), 0, s, cv.imp, nCons,
                                make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+            warmStart(false);
         } else {
             HIPCHK(ctx, hipMemcpyAsync(cv.imp, v == 1 ? ctx->impSnap : ctx->impAlt, sizeof(float4) * nCons,
                                        hipMemcpyDeviceToDevice, s));
-            warmStart();
+            warmStart(true);
         }
         LAUNCHCHK(ctx);
         double* curve = &ctx->trace[(size_t)v * (T + 1)];
@@ -2421,12 +2891,17 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
             hipLaunchKernelGGL(k_fill<int>, dim3(blocks(nb * std::max(posIters, 1))), dim3(256), 0, s,
                                ctx->islandMinSep, nb * std::max(posIters, 1), 0);
             hipLaunchKernelGGL(k_island_init, dim3(blocks(nb)), dim3(256), 0, s, w);
-            for (int pass = 0; ctx->nContacts > 0; ++pass) {
+            const JointView jvw = ctx->jview();
+            for (int pass = 0; ctx->nContacts > 0 || ctx->nJoints > 0; ++pass) {
                 // two hook/jump rounds between checks of the change flag
                 hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->islandChanged, 1, 0);
                 for (int r = 0; r < 2; ++r) {
-                    hipLaunchKernelGGL(k_island_hook, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w,
-                                       &ctx->cnt->islandChanged);
+                    if (ctx->nContacts > 0)
+                        hipLaunchKernelGGL(k_island_hook, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w,
+                                           &ctx->cnt->islandChanged);
+                    if (ctx->nJoints > 0)
+                        hipLaunchKernelGGL(k_island_hook_joints, dim3(blocks(ctx->nJoints)), dim3(256), 0, s, w, jvw,
+                                           &ctx->cnt->islandChanged);
                     hipLaunchKernelGGL(k_island_jump, dim3(blocks(nb)), dim3(256), 0, s, w);
                 }
                 LAUNCHCHK(ctx);
@@ -2502,6 +2977,7 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
         ctx->stats.colours = nColours;
 
         // Island solve.
+        if (ctx->jointsDirty && (rc = recolourJoints(ctx)) != B2G_OK) return rc;
         const bool learned = p->learnedWarmStart != 0;
         if (learned && !ctx->mlp.ready)
             return fail(ctx, B2G_ERR_STATE, "learned warm start requested without a warm-start network");
@@ -2544,13 +3020,40 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
                 profEnd(ctx, phase, pc);
             }
         };
-        auto warmStart = [&]() {
-            perColour(PH_WARM, [&](int b0, int b1) {
-                hipLaunchKernelGGL(k_warm_start, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, ctx->cv, b0, b1);
+        // Joints: colour by colour; joints of the overflow colour one at a time.
+        const JointView jv = ctx->jview();
+        auto perJointColour = [&](auto kernelLaunch) {
+            const int nc = ctx->nJoints > 0 ? kColourSlots : 0;
+            for (int c = 0; c < nc; ++c) {
+                const int b0 = ctx->jointColourStart[c], b1 = ctx->jointColourStart[c + 1];
+                if (b1 <= b0) continue;
+                if (c == kOverflowColour) {
+                    for (int k = b0; k < b1; ++k) kernelLaunch(k, k + 1);
+                } else {
+                    kernelLaunch(b0, b1);
+                }
+            }
+        };
+        // b2Island::Solve's start of the velocity phase (REF/Dynamics/b2Island.cpp:246-255):
+        // contact warm start, then the joints' InitVelocityConstraints.
+        auto startSolve = [&](bool contactWarm) {
+            if (contactWarm) {
+                perColour(PH_WARM, [&](int b0, int b1) {
+                    hipLaunchKernelGGL(k_warm_start, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, ctx->cv, b0, b1);
+                });
+            }
+            perJointColour([&](int b0, int b1) {
+                hipLaunchKernelGGL(k_joint_init, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, jv, b0, b1, dtRatio,
+                                   ctx->warmStarting ? 1 : 0);
             });
         };
+        // Each iteration: joints, then contacts (REF/Dynamics/b2Island.cpp:261-270).
         auto solveVelocity = [&](int iters) {
             for (int it = 0; it < iters; ++it) {
+                perJointColour([&](int b0, int b1) {
+                    hipLaunchKernelGGL(k_joint_solve_velocity, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, jv, b0, b1,
+                                       dt);
+                });
                 perColour(PH_SOLVE_V, [&](int b0, int b1) {
                     hipEvent_t e0, e1;
                     const int pk = profKernelEvents(ctx, &e0, &e1);
@@ -2568,8 +3071,30 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
             profKernel(ctx, PH_K_MLP_INFER, pk);
             return B2G_OK;
         };
+        auto saveJoints = [&]() -> int {
+            if (ctx->nJoints == 0) return B2G_OK;
+            HIPCHK(ctx, hipMemcpyAsync(ctx->jImpulseSnap, ctx->jImpulse, sizeof(float4) * ctx->nJoints,
+                                       hipMemcpyDeviceToDevice, s));
+            HIPCHK(ctx, hipMemcpyAsync(ctx->jFlagsSnap, ctx->jFlags, sizeof(int) * ctx->nJoints,
+                                       hipMemcpyDeviceToDevice, s));
+            return B2G_OK;
+        };
+        auto restoreJoints = [&]() -> int {
+            if (ctx->nJoints == 0) return B2G_OK;
+            HIPCHK(ctx, hipMemcpyAsync(ctx->jImpulse, ctx->jImpulseSnap, sizeof(float4) * ctx->nJoints,
+                                       hipMemcpyDeviceToDevice, s));
+            HIPCHK(ctx, hipMemcpyAsync(ctx->jFlags, ctx->jFlagsSnap, sizeof(int) * ctx->nJoints,
+                                       hipMemcpyDeviceToDevice, s));
+            return B2G_OK;
+        };
         if (nCons > 0 && ctx->traceIters > 0) {
-            if ((rc = traceResiduals(ctx, w, nCons, nFresh, warmStart, solveVelocity, predict)) != B2G_OK) return rc;
+            if ((rc = saveJoints()) != B2G_OK) return rc;
+            auto traceStart = [&](bool contactWarm) {
+                (void)restoreJoints();
+                startSolve(contactWarm);
+            };
+            if ((rc = traceResiduals(ctx, w, nCons, nFresh, traceStart, solveVelocity, predict)) != B2G_OK) return rc;
+            if ((rc = restoreJoints()) != B2G_OK) return rc;
         }
         const bool usePrediction = learned && nFresh > 0;
         const bool checkResidual = usePrediction && p->residualThreshold > 0.0f;
@@ -2577,10 +3102,11 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
             if ((rc = ensureVelSnap(ctx)) != B2G_OK) return rc;
             HIPCHK(ctx, hipMemcpyAsync(ctx->velSnap, ctx->vel, sizeof(float4) * ctx->nBodies, hipMemcpyDeviceToDevice, s));
             HIPCHK(ctx, hipMemcpyAsync(ctx->impSnap, ctx->cv.imp, sizeof(float4) * nCons, hipMemcpyDeviceToDevice, s));
+            if ((rc = saveJoints()) != B2G_OK) return rc;
         }
         if (usePrediction && (rc = predict(ctx->cv.imp)) != B2G_OK) return rc;
-        if (nCons > 0 && (ctx->warmStarting || usePrediction)) warmStart();
-        if (nCons > 0) solveVelocity(p->velocityIterations);
+        startSolve(nCons > 0 && (ctx->warmStarting || usePrediction));
+        solveVelocity(p->velocityIterations);
         if (checkResidual) {
             double rms = 0.0;
             if ((rc = deviceResidual(ctx, w, nCons, &rms)) != B2G_OK) return rc;
@@ -2591,7 +3117,8 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
                                            hipMemcpyDeviceToDevice, s));
                 HIPCHK(ctx, hipMemcpyAsync(ctx->cv.imp, ctx->impSnap, sizeof(float4) * nCons, hipMemcpyDeviceToDevice,
                                            s));
-                if (ctx->warmStarting) warmStart();
+                if ((rc = restoreJoints()) != B2G_OK) return rc;
+                startSolve(ctx->warmStarting);
                 solveVelocity(p->velocityIterations);
                 ctx->stats.fallback = 1;
             }
@@ -2612,13 +3139,17 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
         hipLaunchKernelGGL(k_integrate_positions, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, dt);
         profEnd(ctx, PH_INTEGRATE_X, pe);
         LAUNCHCHK(ctx);
-        for (int it = 0; it < posIters && nCons > 0; ++it) {
+        // Each position iteration: contacts, then joints (REF/Dynamics/b2Island.cpp:310-328).
+        for (int it = 0; it < posIters && (nCons > 0 || ctx->nJoints > 0); ++it) {
             perColour(PH_SOLVE_X, [&](int b0, int b1) {
                 hipEvent_t e0, e1;
                 const int pk = profKernelEvents(ctx, &e0, &e1);
                 hipExtLaunchKernelGGL(k_solve_position, dim3(blocks(b1 - b0)), dim3(256), 0, s, e0, e1, 0, w, ctx->cv,
                                       b0, b1, ctx->cnt, it);
                 profKernel(ctx, PH_K_SOLVE_X, pk);
+            });
+            perJointColour([&](int b0, int b1) {
+                hipLaunchKernelGGL(k_joint_solve_position, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, jv, b0, b1, it);
             });
         }
         pe = profStart(ctx);
@@ -2890,6 +3421,10 @@ int b2g_destroy(b2g_context* ctx) {
     freeArray(ctx->freshMask); freeArray(ctx->freshNum); freeArray(ctx->freshOff); freeArray(ctx->freshAll);
     freeArray(ctx->freshFeat); freeArray(ctx->freshRef); freeArray(ctx->impSnap); freeArray(ctx->impAlt);
     freeArray(ctx->velSnap); freeArray(ctx->logFeat); freeArray(ctx->logTarget);
+    freeArray(ctx->jBodies); freeArray(ctx->jAnchors); freeArray(ctx->jLimits); freeArray(ctx->jImpulse);
+    freeArray(ctx->jR); freeArray(ctx->jMasses); freeArray(ctx->jK0); freeArray(ctx->jK1);
+    freeArray(ctx->jMotorSpeed); freeArray(ctx->jMotorMass); freeArray(ctx->jFlags); freeArray(ctx->jOrder);
+    freeArray(ctx->jImpulseSnap); freeArray(ctx->jFlagsSnap); freeArray(ctx->jointPairs);
     if (ctx->trainScratch) (void)hipFree(ctx->trainScratch);
     if (ctx->mlpAllocated) b2mlp::mlpFree(ctx->mlp);
     for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
@@ -3123,6 +3658,8 @@ int b2g_step(b2g_context* ctx, const b2g_step_params* p) {
         return fail(ctx, B2G_ERR_ARG, "solver mode");
     if (p->learnedWarmStart && p->mode == B2G_MODE_REFERENCE)
         return fail(ctx, B2G_ERR_UNSUPPORTED, "the learned warm start runs with the coloured solver");
+    if (ctx->nJoints > 0 && p->mode == B2G_MODE_REFERENCE)
+        return fail(ctx, B2G_ERR_UNSUPPORTED, "joints run with the coloured solver");
     (void)hipSetDevice(ctx->device);
     return p->mode == B2G_MODE_REFERENCE ? stepReference(ctx, p) : stepColoured(ctx, p);
 }
@@ -3810,6 +4347,107 @@ int b2g_abi_sizes(int32_t* out, int cap) {
     if (!out) return n;
     for (int i = 0; i < n && i < cap; ++i) out[i] = sizes[i];
     return n;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int b2g_add_joints(b2g_context* ctx, const b2g_revolute_def* defs, int count, int* first) {
+    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
+    for (int i = 0; i < count; ++i) {
+        const b2g_revolute_def& d = defs[i];
+        if (d.bodyA < 0 || d.bodyB < 0 || d.bodyA >= ctx->nBodies || d.bodyB >= ctx->nBodies || d.bodyA == d.bodyB)
+            return fail(ctx, B2G_ERR_ARG, "joint bodies out of range or equal");
+    }
+    if (first) *first = ctx->nJoints;
+    if (count == 0) return B2G_OK;
+    int rc = growJoints(ctx, ctx->nJoints + count);
+    if (rc != B2G_OK) return rc;
+    std::vector<int2> b(count);
+    std::vector<float4> anchors(count), limits(count), imp(count, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    std::vector<float> speed(count);
+    std::vector<int> flags(count);
+    bool keepApart = false;
+    for (int i = 0; i < count; ++i) {
+        const b2g_revolute_def& d = defs[i];
+        b[i] = make_int2(d.bodyA, d.bodyB);
+        anchors[i] = make_float4(d.localAnchorA[0], d.localAnchorA[1], d.localAnchorB[0], d.localAnchorB[1]);
+        limits[i] = make_float4(d.referenceAngle, d.lowerAngle, d.upperAngle, d.maxMotorTorque);
+        speed[i] = d.motorSpeed;
+        flags[i] = (d.enableLimit ? JF_LIMIT : 0) | (d.enableMotor ? JF_MOTOR : 0) | (d.collideConnected ? JF_COLLIDE : 0);
+        ctx->hJointBodies.push_back(b[i]);
+        ctx->hJointKeepApart.push_back(d.collideConnected ? 0 : 1);
+        ctx->hJointRemoved.push_back(0);
+        keepApart = keepApart || !d.collideConnected;
+    }
+    hipStream_t s = ctx->stream;
+    const int n0 = ctx->nJoints;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jBodies + n0, b.data(), sizeof(int2) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jAnchors + n0, anchors.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jLimits + n0, limits.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jMotorSpeed + n0, speed.data(), sizeof(float) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jFlags + n0, flags.data(), sizeof(int) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jImpulse + n0, imp.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    ctx->nJoints += count;
+    ctx->jointsDirty = true;
+    if (keepApart) return rebuildJointPairs(ctx);
+    return B2G_OK;
+}
+
+int b2g_get_joints(b2g_context* ctx, int first, int count, float* out) {
+    if (!ctx || first < 0 || count < 0 || first + count > ctx->nJoints || (count > 0 && !out)) return B2G_ERR_ARG;
+    if (count == 0) return B2G_OK;
+    HIPCHK(ctx, hipMemcpyAsync(out, ctx->jImpulse + first, sizeof(float4) * count, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
+int b2g_set_joint(b2g_context* ctx, int joint, const b2g_revolute_def* d) {
+    if (!ctx || !d || joint < 0 || joint >= ctx->nJoints) return B2G_ERR_ARG;
+    if (ctx->hJointRemoved[joint]) return fail(ctx, B2G_ERR_ARG, "joint was removed");
+    hipStream_t s = ctx->stream;
+    int flags = 0;
+    float4 lim0, imp;
+    HIPCHK(ctx, hipMemcpyAsync(&flags, ctx->jFlags + joint, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(&lim0, ctx->jLimits + joint, sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(&imp, ctx->jImpulse + joint, sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    // EnableLimit and SetLimits drop the limit impulse when they change
+    // something (REF/Dynamics/Joints/b2RevoluteJoint.cpp:440-471).
+    const bool limitSwitched = ((flags & JF_LIMIT) != 0) != (d->enableLimit != 0);
+    const bool limitsMoved = lim0.y != d->lowerAngle || lim0.z != d->upperAngle;
+    if (limitSwitched || limitsMoved) imp.z = 0.0f;
+    flags = (flags & ~(JF_LIMIT | JF_MOTOR)) | (d->enableLimit ? JF_LIMIT : 0) | (d->enableMotor ? JF_MOTOR : 0);
+    const float4 lim = make_float4(lim0.x, d->lowerAngle, d->upperAngle, d->maxMotorTorque);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jFlags + joint, &flags, sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jLimits + joint, &lim, sizeof(float4), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jMotorSpeed + joint, &d->motorSpeed, sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jImpulse + joint, &imp, sizeof(float4), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+int b2g_remove_joints(b2g_context* ctx, const int* ids, int count) {
+    if (!ctx || count < 0 || (count > 0 && !ids)) return B2G_ERR_ARG;
+    for (int i = 0; i < count; ++i)
+        if (ids[i] < 0 || ids[i] >= ctx->nJoints) return fail(ctx, B2G_ERR_ARG, "joint index out of range");
+    if (count == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    bool keptApart = false;
+    for (int i = 0; i < count; ++i) {
+        const int j = ids[i];
+        if (ctx->hJointRemoved[j]) continue;
+        keptApart = keptApart || ctx->hJointKeepApart[j];
+        ctx->hJointRemoved[j] = 1;
+        const int flags = JF_REMOVED;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->jFlags + j, &flags, sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    ctx->jointsDirty = true;
+    if (keptApart) return rebuildJointPairs(ctx);
+    return B2G_OK;
 }
 
 }  // extern "C"

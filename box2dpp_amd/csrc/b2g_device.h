@@ -18,6 +18,7 @@ namespace b2g {
 constexpr float kPi = 3.14159265359f;
 constexpr float kLinearSlop = 0.005f;
 constexpr float kAngularSlop = 2.0f / 180.0f * kPi;
+constexpr float kMaxAngularCorrection = 8.0f / 180.0f * kPi;
 constexpr float kPolygonRadius = 2.0f * kLinearSlop;
 constexpr float kAabbExtension = 0.1f;
 constexpr float kAabbMultiplier = 2.0f;
@@ -174,6 +175,8 @@ struct WorldView {
     int* islandSleep;  // per root: 1 if a body has rested less than the time to sleep
     int* islandMinSep; // per position pass and root: 1 if a point is deeper than 3 * linearSlop
     int* bworld;       // sub-world per body (null: a single world)
+    const unsigned long long* jointPairs;  // hash set of body pairs a joint keeps apart (null: none)
+    uint32_t jointPairMask;
     // fixtures / proxies
     int nFixtures;
     int* fbody;

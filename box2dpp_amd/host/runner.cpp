@@ -17,6 +17,7 @@ struct Runner {
     b2World world;
     std::vector<b2Body*> bodies;
     std::vector<b2Fixture*> fixtures;
+    std::vector<b2RevoluteJoint*> joints;
     explicit Runner(float gx, float gy) : world(b2Vec2(gx, gy)) {}
 };
 
@@ -84,6 +85,32 @@ int loadScene(Runner* r, const b2scene::BodyDef* bodies, int nb, const b2scene::
     return flush ? r->world.Flush() : 0;
 }
 
+// Scene joints through b2World::CreateJoint (local anchors and reference
+// angle given directly; body indices relative to `base`).
+int loadJoints(Runner* r, int base, const b2scene::JointDef* jd, int nj) {
+    for (int k = 0; k < nj; ++k) {
+        const b2scene::JointDef& d = jd[k];
+        if (base + d.bodyA >= (int)r->bodies.size() || base + d.bodyB >= (int)r->bodies.size()) return B2G_ERR_ARG;
+        b2RevoluteJointDef def;
+        def.bodyA = r->bodies[base + d.bodyA];
+        def.bodyB = r->bodies[base + d.bodyB];
+        def.collideConnected = d.collideConnected != 0;
+        def.localAnchorA = b2Vec2(d.ax, d.ay);
+        def.localAnchorB = b2Vec2(d.bx, d.by);
+        def.referenceAngle = d.referenceAngle;
+        def.enableLimit = d.enableLimit != 0;
+        def.lowerAngle = d.lowerAngle;
+        def.upperAngle = d.upperAngle;
+        def.enableMotor = d.enableMotor != 0;
+        def.motorSpeed = d.motorSpeed;
+        def.maxMotorTorque = d.maxMotorTorque;
+        b2Joint* j = r->world.CreateJoint(&def);
+        if (j == nullptr) return B2G_ERR_ARG;
+        r->joints.push_back(static_cast<b2RevoluteJoint*>(j));
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -120,8 +147,56 @@ int b2pp_world_load(void* h, const b2scene::BodyDef* bodies, int nb, const b2sce
 }
 
 int b2pp_world_load_scene(void* h, const char* kind, int n, unsigned seed) {
+    Runner* r = (Runner*)h;
     b2scene::Scene s = b2scene::byName(kind, n, seed);
-    return loadScene((Runner*)h, s.bodies.data(), (int)s.bodies.size(), s.fixtures.data(), (int)s.fixtures.size());
+    const int base = (int)r->bodies.size();
+    int rc = loadScene(r, s.bodies.data(), (int)s.bodies.size(), s.fixtures.data(), (int)s.fixtures.size(), false);
+    if (rc != 0) return rc;
+    if ((rc = loadJoints(r, base, s.joints.data(), (int)s.joints.size())) != 0) return rc;
+    return r->world.Flush();
+}
+
+int b2pp_world_joint_count(void* h) { return (int)((Runner*)h)->joints.size(); }
+
+// Per joint in creation order: impulse x, y, z and motor impulse (the
+// reaction force, reaction torque and motor torque at inv_dt = 1).
+void b2pp_world_get_joints(void* h, float* out) {
+    Runner* r = (Runner*)h;
+    for (size_t i = 0; i < r->joints.size(); ++i) {
+        const b2RevoluteJoint* j = r->joints[i];
+        if (j == nullptr) {  // destroyed
+            for (int k = 0; k < 4; ++k) out[4 * i + k] = 0.0f;
+            continue;
+        }
+        const b2Vec2 f = j->GetReactionForce(1.0f);
+        out[4 * i] = f[0];
+        out[4 * i + 1] = f[1];
+        out[4 * i + 2] = j->GetReactionTorque(1.0f);
+        out[4 * i + 3] = j->GetMotorTorque(1.0f);
+    }
+}
+
+// Joint switches through the box2d:: setters (b2RevoluteJoint::EnableMotor,
+// SetMotorSpeed, SetMaxMotorTorque, EnableLimit, SetLimits).
+int b2pp_world_set_joint(void* h, int joint, int enableLimit, float lower, float upper, int enableMotor, float speed,
+                         float maxTorque) {
+    Runner* r = (Runner*)h;
+    if (joint < 0 || joint >= (int)r->joints.size() || r->joints[joint] == nullptr) return B2G_ERR_ARG;
+    b2RevoluteJoint* j = r->joints[joint];
+    j->EnableLimit(enableLimit != 0);
+    j->SetLimits(lower, upper);
+    j->EnableMotor(enableMotor != 0);
+    j->SetMotorSpeed(speed);
+    j->SetMaxMotorTorque(maxTorque);
+    return 0;
+}
+
+int b2pp_world_destroy_joint(void* h, int joint) {
+    Runner* r = (Runner*)h;
+    if (joint < 0 || joint >= (int)r->joints.size() || r->joints[joint] == nullptr) return B2G_ERR_ARG;
+    r->world.DestroyJoint(r->joints[joint]);
+    r->joints[joint] = nullptr;
+    return 0;
 }
 
 // Batched worlds: nWorlds worlds of the batched scene, each in its own

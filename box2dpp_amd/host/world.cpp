@@ -349,6 +349,7 @@ b2World::b2World(const b2Vec2& gravity) : m_gravity(gravity) {
 }
 
 b2World::~b2World() {
+    for (b2Joint* j : m_joints) delete j;
     for (b2Body* b : m_bodies) {
         if (!b) continue;
         b2Fixture* f = b->m_fixtureList;
@@ -381,6 +382,8 @@ b2Body* b2World::CreateBody(const b2BodyDef* def) {
 // proxies leave the broad-phase, so their contacts end at the next Step.
 void b2World::DestroyBody(b2Body* b) {
     if (IsLocked() || b == nullptr || b->m_world != this) return;
+    // Attached joints go first (REF/Dynamics/b2World.cpp:134-147).
+    while (b->m_jointList) DestroyJoint(b->m_jointList->joint);
     b->Touch();
     if (b->m_prev) b->m_prev->m_next = b->m_next;
     if (b->m_next) b->m_next->m_prev = b->m_prev;
@@ -395,6 +398,199 @@ void b2World::DestroyBody(b2Body* b) {
     for (b2Fixture* f = b->m_fixtureList; f; f = f->m_next) m_removedFixtures.push_back(f->m_index);
     --m_bodyCount;
     m_contactsStale = true;
+}
+
+// ------------------------------------------------------------ joints
+b2Joint::b2Joint(const b2JointDef* def, b2World* world, int32_t index)
+    : m_type(def->type), m_bodyA(def->bodyA), m_bodyB(def->bodyB), m_world(world), m_index(index),
+      m_collideConnected(def->collideConnected), m_userData(def->userData) {}
+
+bool b2Joint::IsActive() const { return m_bodyA->IsActive() && m_bodyB->IsActive(); }
+
+// b2RevoluteJointDef::Initialize, REF/Dynamics/Joints/b2RevoluteJoint.cpp:46-53
+void b2RevoluteJointDef::Initialize(b2Body* bA, b2Body* bB, const b2Vec2& anchor) {
+    bodyA = bA;
+    bodyB = bB;
+    localAnchorA = bodyA->GetLocalPoint(anchor);
+    localAnchorB = bodyB->GetLocalPoint(anchor);
+    referenceAngle = bodyB->GetAngle() - bodyA->GetAngle();
+}
+
+b2RevoluteJoint::b2RevoluteJoint(const b2RevoluteJointDef* def, b2World* world, int32_t index)
+    : b2Joint(def, world, index), m_localAnchorA(def->localAnchorA), m_localAnchorB(def->localAnchorB),
+      m_referenceAngle(def->referenceAngle), m_enableLimit(def->enableLimit), m_lowerAngle(def->lowerAngle),
+      m_upperAngle(def->upperAngle), m_enableMotor(def->enableMotor), m_motorSpeed(def->motorSpeed),
+      m_maxMotorTorque(def->maxMotorTorque) {}
+
+b2Vec2 b2RevoluteJoint::GetAnchorA() const { return m_bodyA->GetWorldPoint(m_localAnchorA); }
+b2Vec2 b2RevoluteJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localAnchorB); }
+
+// REF/Dynamics/Joints/b2RevoluteJoint.cpp:389-412
+b2Vec2 b2RevoluteJoint::GetReactionForce(float inv_dt) const {
+    m_world->DownloadJoints();
+    return b2Vec2(inv_dt * m_impulse[0], inv_dt * m_impulse[1]);
+}
+float b2RevoluteJoint::GetReactionTorque(float inv_dt) const {
+    m_world->DownloadJoints();
+    return inv_dt * m_impulse[2];
+}
+float b2RevoluteJoint::GetMotorTorque(float inv_dt) const {
+    m_world->DownloadJoints();
+    return inv_dt * m_impulse[3];
+}
+float b2RevoluteJoint::GetJointAngle() const { return m_bodyB->GetAngle() - m_bodyA->GetAngle() - m_referenceAngle; }
+float b2RevoluteJoint::GetJointSpeed() const {
+    return m_bodyB->GetAngularVelocity() - m_bodyA->GetAngularVelocity();
+}
+
+void b2RevoluteJoint::MarkDirty() {
+    if (m_dirty) return;
+    m_dirty = true;
+    m_world->m_dirtyJoints.push_back(this);
+}
+
+// The setters wake both bodies and, where the limit changes, drop the limit
+// impulse (REF/Dynamics/Joints/b2RevoluteJoint.cpp:419-471).
+void b2RevoluteJoint::EnableMotor(bool flag) {
+    m_bodyA->SetAwake(true);
+    m_bodyB->SetAwake(true);
+    m_enableMotor = flag;
+    MarkDirty();
+}
+void b2RevoluteJoint::SetMotorSpeed(float speed) {
+    m_bodyA->SetAwake(true);
+    m_bodyB->SetAwake(true);
+    m_motorSpeed = speed;
+    MarkDirty();
+}
+void b2RevoluteJoint::SetMaxMotorTorque(float torque) {
+    m_bodyA->SetAwake(true);
+    m_bodyB->SetAwake(true);
+    m_maxMotorTorque = torque;
+    MarkDirty();
+}
+void b2RevoluteJoint::EnableLimit(bool flag) {
+    if (flag == m_enableLimit) return;
+    m_world->DownloadJoints();
+    m_bodyA->SetAwake(true);
+    m_bodyB->SetAwake(true);
+    m_enableLimit = flag;
+    m_impulse[2] = 0.0f;
+    MarkDirty();
+}
+void b2RevoluteJoint::SetLimits(float lower, float upper) {
+    if (lower == m_lowerAngle && upper == m_upperAngle) return;
+    m_world->DownloadJoints();
+    m_bodyA->SetAwake(true);
+    m_bodyB->SetAwake(true);
+    m_impulse[2] = 0.0f;
+    m_lowerAngle = lower;
+    m_upperAngle = upper;
+    MarkDirty();
+}
+
+// b2World::CreateJoint, REF/Dynamics/b2World.cpp:205-262: the joint goes to
+// the head of the world's list and of both bodies' joint lists. Contacts
+// that a collideConnected = false joint filters out end at the next Step.
+b2Joint* b2World::CreateJoint(const b2JointDef* def) {
+    if (IsLocked() || def == nullptr || def->bodyA == nullptr || def->bodyB == nullptr) return nullptr;
+    if (def->type != b2JointType::REVOLUTE_JOINT) {
+        m_lastError = "only revolute joints run on the GPU path";
+        return nullptr;
+    }
+    if (def->bodyA == def->bodyB || def->bodyA->m_world != this || def->bodyB->m_world != this) return nullptr;
+    if (def->bodyA->m_subWorld != def->bodyB->m_subWorld) {
+        m_lastError = "a joint connects bodies of one sub-world";
+        return nullptr;
+    }
+    b2Joint* j = new b2RevoluteJoint(static_cast<const b2RevoluteJointDef*>(def), this, (int32_t)m_joints.size());
+    m_joints.push_back(j);
+    j->m_prev = nullptr;
+    j->m_next = m_jointList;
+    if (m_jointList) m_jointList->m_prev = j;
+    m_jointList = j;
+    ++m_jointCount;
+    j->m_edgeA.joint = j;
+    j->m_edgeA.other = j->m_bodyB;
+    j->m_edgeA.prev = nullptr;
+    j->m_edgeA.next = j->m_bodyA->m_jointList;
+    if (j->m_bodyA->m_jointList) j->m_bodyA->m_jointList->prev = &j->m_edgeA;
+    j->m_bodyA->m_jointList = &j->m_edgeA;
+    j->m_edgeB.joint = j;
+    j->m_edgeB.other = j->m_bodyA;
+    j->m_edgeB.prev = nullptr;
+    j->m_edgeB.next = j->m_bodyB->m_jointList;
+    if (j->m_bodyB->m_jointList) j->m_bodyB->m_jointList->prev = &j->m_edgeB;
+    j->m_bodyB->m_jointList = &j->m_edgeB;
+    m_contactsStale = true;
+    return j;
+}
+
+// b2World::DestroyJoint, REF/Dynamics/b2World.cpp:267-345: unlinked, both
+// bodies woken; on the device the joint stops acting at the next Step.
+void b2World::DestroyJoint(b2Joint* j) {
+    if (IsLocked() || j == nullptr || j->m_world != this) return;
+    if (j->m_prev) j->m_prev->m_next = j->m_next;
+    if (j->m_next) j->m_next->m_prev = j->m_prev;
+    if (j == m_jointList) m_jointList = j->m_next;
+    b2Body* bA = j->m_bodyA;
+    b2Body* bB = j->m_bodyB;
+    bA->SetAwake(true);
+    bB->SetAwake(true);
+    if (j->m_edgeA.prev) j->m_edgeA.prev->next = j->m_edgeA.next;
+    if (j->m_edgeA.next) j->m_edgeA.next->prev = j->m_edgeA.prev;
+    if (&j->m_edgeA == bA->m_jointList) bA->m_jointList = j->m_edgeA.next;
+    if (j->m_edgeB.prev) j->m_edgeB.prev->next = j->m_edgeB.next;
+    if (j->m_edgeB.next) j->m_edgeB.next->prev = j->m_edgeB.prev;
+    if (&j->m_edgeB == bB->m_jointList) bB->m_jointList = j->m_edgeB.next;
+    --m_jointCount;
+    m_removedJoints.push_back(j->m_index);
+    auto& dirty = m_dirtyJoints;
+    dirty.erase(std::remove(dirty.begin(), dirty.end(), static_cast<b2RevoluteJoint*>(j)), dirty.end());
+    // A joint not yet on the device keeps its slot until the upload adds it
+    // (device indices are creation order) and removes it again.
+    if (j->m_index < m_uploadedJoints) {
+        m_joints[j->m_index] = nullptr;
+        delete j;
+    } else {
+        j->m_world = nullptr;
+    }
+    m_contactsStale = true;
+}
+
+void b2World::DownloadJoints() const {
+    if (!m_jointsStale || !m_ctx) return;
+    m_jointsStale = false;
+    const int n = m_uploadedJoints;
+    if (n == 0) return;
+    std::vector<float> st(4 * (size_t)n);
+    if (b2g_get_joints(m_ctx, 0, n, st.data()) != B2G_OK) return;
+    for (int i = 0; i < n; ++i) {
+        b2Joint* j = m_joints[i];
+        if (j == nullptr) continue;
+        b2RevoluteJoint* r = static_cast<b2RevoluteJoint*>(j);
+        if (r->m_dirty) continue;
+        for (int k = 0; k < 4; ++k) r->m_impulse[k] = st[4 * (size_t)i + k];
+    }
+}
+
+static b2g_revolute_def revoluteDef(const b2RevoluteJoint* r, int32_t bodyA, int32_t bodyB) {
+    b2g_revolute_def d{};
+    d.bodyA = bodyA;
+    d.bodyB = bodyB;
+    d.localAnchorA[0] = r->GetLocalAnchorA()[0];
+    d.localAnchorA[1] = r->GetLocalAnchorA()[1];
+    d.localAnchorB[0] = r->GetLocalAnchorB()[0];
+    d.localAnchorB[1] = r->GetLocalAnchorB()[1];
+    d.referenceAngle = r->GetReferenceAngle();
+    d.lowerAngle = r->GetLowerLimit();
+    d.upperAngle = r->GetUpperLimit();
+    d.maxMotorTorque = r->GetMaxMotorTorque();
+    d.motorSpeed = r->GetMotorSpeed();
+    d.enableLimit = r->IsLimitEnabled() ? 1 : 0;
+    d.enableMotor = r->IsMotorEnabled() ? 1 : 0;
+    d.collideConnected = r->GetCollideConnected() ? 1 : 0;
+    return d;
 }
 
 void b2World::SetGravity(const b2Vec2& g) {
@@ -549,6 +745,40 @@ int32_t b2World::Upload() {
         if (rc != B2G_OK) return rc;
         m_removedFixtures.clear();
     }
+    const int nj = (int)m_joints.size();
+    if (nj > m_uploadedJoints) {
+        std::vector<b2g_revolute_def> defs;
+        defs.reserve(nj - m_uploadedJoints);
+        for (int i = m_uploadedJoints; i < nj; ++i) {
+            const b2RevoluteJoint* r = static_cast<const b2RevoluteJoint*>(m_joints[i]);
+            defs.push_back(revoluteDef(r, r->m_bodyA->m_index, r->m_bodyB->m_index));
+        }
+        int first = 0;
+        rc = b2g_add_joints(m_ctx, defs.data(), (int)defs.size(), &first);
+        if (rc != B2G_OK) return rc;
+        for (int i = m_uploadedJoints; i < nj; ++i) {
+            if (m_joints[i]->m_world == nullptr) {  // destroyed before its upload
+                delete m_joints[i];
+                m_joints[i] = nullptr;
+            }
+        }
+        m_uploadedJoints = nj;
+    }
+    if (!m_removedJoints.empty()) {
+        rc = b2g_remove_joints(m_ctx, m_removedJoints.data(), (int)m_removedJoints.size());
+        if (rc != B2G_OK) return rc;
+        m_removedJoints.clear();
+    }
+    if (!m_dirtyJoints.empty()) {
+        for (b2RevoluteJoint* r : m_dirtyJoints) {
+            r->m_dirty = false;
+            if (r->m_index >= m_uploadedJoints) continue;
+            const b2g_revolute_def d = revoluteDef(r, r->m_bodyA->m_index, r->m_bodyB->m_index);
+            rc = b2g_set_joint(m_ctx, r->m_index, &d);
+            if (rc != B2G_OK) return rc;
+        }
+        m_dirtyJoints.clear();
+    }
     return B2G_OK;
 }
 
@@ -619,6 +849,7 @@ void b2World::Step(float dt, int32_t velocityIterations, int32_t positionIterati
     }
     m_hostStale = true;
     m_contactsStale = true;
+    m_jointsStale = true;
     b2g_step_stats st{};
     b2g_get_stats(m_ctx, &st);
     m_contactCount = st.contacts;

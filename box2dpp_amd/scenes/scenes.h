@@ -16,6 +16,13 @@
 //             inside a static U-shaped container of three edges.
 //   worlds  : many independent piles of log-uniform size in [16,1024],
 //             each with its own container (one scene per world).
+//   pendulums : N single-box pendulums (box 0.5x0.5, density 1) hinged by a
+//             revolute joint to one static body at y = 20, 5 m apart, each
+//             started with its own horizontal speed; they never touch.
+//   chains  : N chains of 32 boxes 0.5x0.1 m (density 1, friction 0.2)
+//             joined end to end by revolute joints, the first hinged to a
+//             static anchor, laid out horizontally 4 m apart above a ground
+//             edge so they fall, swing and hit the ground.
 #pragma once
 #include <stdint.h>
 #include <math.h>
@@ -51,10 +58,22 @@ struct FixtureDef {
     int categoryBits, maskBits, groupIndex;
 };
 
+// b2RevoluteJointDef (Box2D/Dynamics/Joints/b2RevoluteJoint.h:37-90) with
+// body indices in place of body pointers.
+struct JointDef {
+    int bodyA, bodyB;
+    float ax, ay, bx, by;  // local anchors on A and B
+    float referenceAngle;
+    float lowerAngle, upperAngle;
+    float maxMotorTorque, motorSpeed;
+    int enableLimit, enableMotor, collideConnected;
+};
+
 struct Scene {
     float gx = 0.0f, gy = -10.0f;
     std::vector<BodyDef> bodies;
     std::vector<FixtureDef> fixtures;
+    std::vector<JointDef> joints;
     int dynamicCount() const {
         int n = 0;
         for (auto& b : bodies) n += (b.type == 2);
@@ -210,6 +229,61 @@ inline Scene batchedWorld(int n, uint32_t seed) {
     return pile(n, seed, cols * spacing, rows * spacing);
 }
 
+// Revolute joint at a world anchor between bodies at angle zero
+// (b2RevoluteJointDef::Initialize: local anchor = anchor - body origin,
+// reference angle = angleB - angleA = 0).
+inline JointDef hinge(const Scene& s, int a, int b, float x, float y) {
+    JointDef j{};
+    j.bodyA = a; j.bodyB = b;
+    j.ax = x - s.bodies[a].px; j.ay = y - s.bodies[a].py;
+    j.bx = x - s.bodies[b].px; j.by = y - s.bodies[b].py;
+    j.referenceAngle = 0.0f;
+    j.lowerAngle = j.upperAngle = 0.0f;
+    j.maxMotorTorque = j.motorSpeed = 0.0f;
+    j.enableLimit = j.enableMotor = j.collideConnected = 0;
+    return j;
+}
+
+inline Scene pendulums(int n) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 20.0f));
+    for (int i = 0; i < n; ++i) {
+        const float x = 5.0f * (float)i;
+        const float len = 1.0f + 0.05f * (float)(i % 20);
+        BodyDef b = defaultBody(2, x, 20.0f - len);
+        b.vx = 0.5f + 0.25f * (float)(i % 13);
+        addBody(s, b);
+        addFixture(s, boxFixture(0.25f, 0.25f, 1.0f, 0.2f));
+        JointDef j = hinge(s, 0, (int)s.bodies.size() - 1, x, 20.0f);
+        if (i % 3 == 1) {  // some with a limit, some with a motor
+            j.enableLimit = 1; j.lowerAngle = -0.4f; j.upperAngle = 0.3f;
+        } else if (i % 3 == 2) {
+            j.enableMotor = 1; j.motorSpeed = 1.5f; j.maxMotorTorque = 2.0f;
+        }
+        s.joints.push_back(j);
+    }
+    return s;
+}
+
+inline Scene chains(int n, int links = 32) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-40.0f, 0.0f, 4.0f * (float)n + 40.0f, 0.0f));
+    for (int c = 0; c < n; ++c) {
+        const float x0 = 4.0f * (float)c, y = 10.0f + 0.25f * (float)(c % 4);
+        addBody(s, defaultBody(0, x0, y));  // anchor
+        int prev = (int)s.bodies.size() - 1;
+        for (int i = 0; i < links; ++i) {
+            addBody(s, defaultBody(2, x0 + 0.25f + 0.5f * (float)i, y));
+            addFixture(s, boxFixture(0.25f, 0.05f, 1.0f, 0.2f));
+            const int cur = (int)s.bodies.size() - 1;
+            s.joints.push_back(hinge(s, prev, cur, x0 + 0.5f * (float)i, y));
+            prev = cur;
+        }
+    }
+    return s;
+}
+
 // Scene by name: "hello", "pyramid" (n = rows, 0 -> 20), "pile" (n bodies,
 // seed), "world" (one batched-scene world of n bodies, seed). Unknown names
 // give an empty scene.
@@ -219,6 +293,8 @@ inline Scene byName(const char* kind, int n, uint32_t seed) {
     if (k == "pyramid") return pyramid(n > 0 ? n : 20);
     if (k == "pile") return pile(n, seed);
     if (k == "world") return batchedWorld(n, seed);
+    if (k == "pendulums") return pendulums(n > 0 ? n : 16);
+    if (k == "chains") return chains(n > 0 ? n : 4, seed > 0 ? (int)seed : 32);
     return Scene();
 }
 

@@ -260,6 +260,38 @@ typedef struct b2g_phase_profile {
 int b2g_set_profiling(b2g_context* ctx, int on);
 int b2g_get_profile(b2g_context* ctx, b2g_phase_profile* out, int cap, int* count);
 
+/* ------------------------------------------------------------- joints
+ * b2RevoluteJointDef (Box2D/Dynamics/Joints/b2RevoluteJoint.h:37-90) with
+ * body indices; joints connect bodies of the same sub-world. */
+typedef struct b2g_revolute_def {
+    int32_t bodyA, bodyB;
+    float localAnchorA[2];
+    float localAnchorB[2];
+    float referenceAngle;
+    float lowerAngle, upperAngle;
+    float maxMotorTorque, motorSpeed;
+    int32_t enableLimit, enableMotor, collideConnected;
+} b2g_revolute_def;
+
+/* Appends revolute joints (b2World::CreateJoint, Box2D/Dynamics/b2World.h:85);
+ * *first receives the index of the first new joint. Contacts between bodies
+ * that a joint keeps from colliding end at the next step. */
+int b2g_add_joints(b2g_context* ctx, const b2g_revolute_def* defs, int count, int* first);
+/* Joint state for joints [first, first + count): impulse x, y, z and motor
+ * impulse, 4 floats each (b2RevoluteJoint::GetReactionForce / Torque /
+ * GetMotorTorque read these, Box2D/Dynamics/Joints/b2RevoluteJoint.h:158-166). */
+int b2g_get_joints(b2g_context* ctx, int first, int count, float* out);
+/* Motor and limit switches of one joint (b2RevoluteJoint::EnableMotor,
+ * SetMotorSpeed, SetMaxMotorTorque, EnableLimit, SetLimits,
+ * Box2D/Dynamics/Joints/b2RevoluteJoint.cpp:419-471); bodies, anchors and
+ * the reference angle keep their creation values. A changed limit switch or
+ * limit range drops the limit impulse, as those setters do. */
+int b2g_set_joint(b2g_context* ctx, int joint, const b2g_revolute_def* def);
+/* Removes joints (b2World::DestroyJoint, Box2D/Dynamics/b2World.cpp:267):
+ * they stop constraining and connecting islands at the next step; indices of
+ * the other joints do not change. */
+int b2g_remove_joints(b2g_context* ctx, const int* ids, int count);
+
 /* ------------------------------------------------ learned warm start
  * No counterpart in the reference: its solver starts every new contact
  * point from zero impulse (b2ContactSolver ctor, REF/Dynamics/Contacts/

@@ -8,7 +8,8 @@
 // Differences from the reference, by design of the GPU path:
 //  * continuous collision (SetContinuousPhysics(true)) and sub-stepping are
 //    not executed on the GPU; Step reports them through GetLastError().
-//  * sensors and joints are rejected at creation.
+//  * sensors are rejected at creation; of the joint types, revolute joints
+//    run (with the coloured solver).
 //  * b2ContactListener::PreSolve is not called (no mid-step host hook);
 //    BeginContact, EndContact and PostSolve are replayed after Step.
 #pragma once
@@ -26,6 +27,8 @@ class b2World;
 class b2Body;
 class b2Fixture;
 class b2Contact;
+class b2Joint;
+struct b2JointEdge;
 
 // Box2D/Dynamics/b2Body.h:41-49
 enum class b2BodyType { STATIC_BODY = 0, KINEMATIC_BODY, DYNAMIC_BODY };
@@ -189,6 +192,8 @@ public:
     bool IsFixedRotation() const { return (m_flags & e_fixedRotationFlag) != 0; }
     b2Fixture* GetFixtureList() { return m_fixtureList; }
     const b2Fixture* GetFixtureList() const { return m_fixtureList; }
+    b2JointEdge* GetJointList() { return m_jointList; }
+    const b2JointEdge* GetJointList() const { return m_jointList; }
     b2Body* GetNext() { return m_next; }
     const b2Body* GetNext() const { return m_next; }
     void* GetUserData() const { return m_userData; }
@@ -224,6 +229,7 @@ private:
     b2Body* m_next = nullptr;
     b2Fixture* m_fixtureList = nullptr;
     int32_t m_fixtureCount = 0;
+    b2JointEdge* m_jointList = nullptr;
     float m_mass = 0.0f, m_invMass = 0.0f;
     float m_I = 0.0f, m_invI = 0.0f;
     float m_linearDamping = 0.0f, m_angularDamping = 0.0f, m_gravityScale = 1.0f;
@@ -263,6 +269,123 @@ private:
     b2Contact* m_next = nullptr;
 };
 
+// Box2D/Dynamics/Joints/b2Joint.h:31-53
+enum class b2JointType {
+    UNKNOWN_JOINT, REVOLUTE_JOINT, PRISMATIC_JOINT, DISTANCE_JOINT, PULLEY_JOINT, MOUSE_JOINT, GEAR_JOINT,
+    WHEEL_JOINT, WELD_JOINT, FRICTION_JOINT, ROPE_JOINT, MOTOR_JOINT
+};
+enum class b2LimitState { INACTIVE_LIMIT, AT_LOWER_LIMIT, AT_UPPER_LIMIT, EQUAL_LIMITS };
+
+// Box2D/Dynamics/Joints/b2Joint.h:62-68: a joint's entry in a body's joint list.
+struct b2JointEdge {
+    b2Body* other = nullptr;
+    b2Joint* joint = nullptr;
+    b2JointEdge* prev = nullptr;
+    b2JointEdge* next = nullptr;
+};
+
+// Box2D/Dynamics/Joints/b2Joint.h:71-101
+struct b2JointDef {
+    b2JointType type = b2JointType::UNKNOWN_JOINT;
+    void* userData = nullptr;
+    b2Body* bodyA = nullptr;
+    b2Body* bodyB = nullptr;
+    bool collideConnected = false;
+};
+
+// Box2D/Dynamics/Joints/b2RevoluteJoint.h:37-90
+struct b2RevoluteJointDef : public b2JointDef {
+    b2RevoluteJointDef() { type = b2JointType::REVOLUTE_JOINT; }
+    // Bodies, local anchors and reference angle from a world anchor point.
+    void Initialize(b2Body* bodyA, b2Body* bodyB, const b2Vec2& anchor);
+    b2Vec2 localAnchorA{0.0f, 0.0f};
+    b2Vec2 localAnchorB{0.0f, 0.0f};
+    float referenceAngle = 0.0f;
+    bool enableLimit = false;
+    float lowerAngle = 0.0f;
+    float upperAngle = 0.0f;
+    bool enableMotor = false;
+    float motorSpeed = 0.0f;
+    float maxMotorTorque = 0.0f;
+};
+
+// Box2D/Dynamics/Joints/b2Joint.h:105-190. Joints are created by
+// b2World::CreateJoint and live on the device; the getters below read the
+// device state of the last Step.
+class b2Joint {
+public:
+    virtual ~b2Joint() = default;
+    b2JointType GetType() const { return m_type; }
+    b2Body* GetBodyA() { return m_bodyA; }
+    b2Body* GetBodyB() { return m_bodyB; }
+    virtual b2Vec2 GetAnchorA() const = 0;
+    virtual b2Vec2 GetAnchorB() const = 0;
+    virtual b2Vec2 GetReactionForce(float inv_dt) const = 0;
+    virtual float GetReactionTorque(float inv_dt) const = 0;
+    b2Joint* GetNext() { return m_next; }
+    const b2Joint* GetNext() const { return m_next; }
+    void* GetUserData() const { return m_userData; }
+    void SetUserData(void* data) { m_userData = data; }
+    bool IsActive() const;
+    bool GetCollideConnected() const { return m_collideConnected; }
+    // Device index of this joint (its creation order in the world).
+    int32_t GetIndex() const { return m_index; }
+
+protected:
+    friend class b2World;
+    friend class b2Body;
+    b2Joint(const b2JointDef* def, b2World* world, int32_t index);
+    b2JointType m_type;
+    b2Joint* m_prev = nullptr;
+    b2Joint* m_next = nullptr;
+    b2JointEdge m_edgeA, m_edgeB;
+    b2Body* m_bodyA;
+    b2Body* m_bodyB;
+    b2World* m_world;
+    int32_t m_index;
+    bool m_collideConnected;
+    void* m_userData;
+};
+
+// Box2D/Dynamics/Joints/b2RevoluteJoint.h:97-200
+class b2RevoluteJoint : public b2Joint {
+public:
+    b2Vec2 GetAnchorA() const override;
+    b2Vec2 GetAnchorB() const override;
+    const b2Vec2& GetLocalAnchorA() const { return m_localAnchorA; }
+    const b2Vec2& GetLocalAnchorB() const { return m_localAnchorB; }
+    float GetReferenceAngle() const { return m_referenceAngle; }
+    float GetJointAngle() const;
+    float GetJointSpeed() const;
+    bool IsLimitEnabled() const { return m_enableLimit; }
+    void EnableLimit(bool flag);
+    float GetLowerLimit() const { return m_lowerAngle; }
+    float GetUpperLimit() const { return m_upperAngle; }
+    void SetLimits(float lower, float upper);
+    bool IsMotorEnabled() const { return m_enableMotor; }
+    void EnableMotor(bool flag);
+    void SetMotorSpeed(float speed);
+    float GetMotorSpeed() const { return m_motorSpeed; }
+    void SetMaxMotorTorque(float torque);
+    float GetMaxMotorTorque() const { return m_maxMotorTorque; }
+    b2Vec2 GetReactionForce(float inv_dt) const override;
+    float GetReactionTorque(float inv_dt) const override;
+    float GetMotorTorque(float inv_dt) const;
+
+private:
+    friend class b2World;
+    b2RevoluteJoint(const b2RevoluteJointDef* def, b2World* world, int32_t index);
+    void MarkDirty();  // switches to upload before the next Step
+    b2Vec2 m_localAnchorA, m_localAnchorB;
+    float m_referenceAngle;
+    bool m_enableLimit;
+    float m_lowerAngle, m_upperAngle;
+    bool m_enableMotor;
+    float m_motorSpeed, m_maxMotorTorque;
+    mutable float m_impulse[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // x, y, limit, motor
+    bool m_dirty = false;
+};
+
 // Per-step solver options of the GPU path (no reference counterpart).
 struct b2SolverOptions {
     int32_t mode = 0;                 // 0 coloured, 1 reference order
@@ -282,6 +405,12 @@ public:
     void SetContactListener(b2ContactListener* listener) { m_contactListener = listener; }
     b2Body* CreateBody(const b2BodyDef* def);
     void DestroyBody(b2Body* body);
+    // Box2D/Dynamics/b2World.h:85-92. Revolute joints only; other types
+    // return nullptr with GetLastError() set.
+    b2Joint* CreateJoint(const b2JointDef* def);
+    void DestroyJoint(b2Joint* joint);
+    b2Joint* GetJointList() { return m_jointList; }
+    const b2Joint* GetJointList() const { return m_jointList; }
     void Step(float timeStep, int32_t velocityIterations, int32_t positionIterations);
     void ClearForces();
     b2Body* GetBodyList() { return m_bodyList; }
@@ -297,7 +426,7 @@ public:
     bool GetSubStepping() const { return m_subStepping; }
     int32_t GetProxyCount() const { return (int32_t)m_fixtures.size(); }
     int32_t GetBodyCount() const { return m_bodyCount; }
-    int32_t GetJointCount() const { return 0; }
+    int32_t GetJointCount() const { return m_jointCount; }
     int32_t GetContactCount() const { return m_contactCount; }
     void SetGravity(const b2Vec2& gravity);
     b2Vec2 GetGravity() const { return m_gravity; }
@@ -329,8 +458,10 @@ public:
 private:
     friend class b2Body;
     friend class b2Fixture;
+    friend class b2RevoluteJoint;
     int32_t Upload();
     void Download() const;
+    void DownloadJoints() const;
     void ReplayEvents();
 
     b2g_context* m_ctx = nullptr;
@@ -361,6 +492,13 @@ private:
     std::vector<int32_t> m_updatedFixtures;  // material/filter changes to upload
     int32_t m_subWorldCount = 0;
     int32_t m_currentSubWorld = 0;
+    b2Joint* m_jointList = nullptr;
+    int32_t m_jointCount = 0;
+    std::vector<b2Joint*> m_joints;        // creation order = device index
+    int32_t m_uploadedJoints = 0;
+    std::vector<int32_t> m_removedJoints;  // device joints to drop at the next upload
+    std::vector<b2RevoluteJoint*> m_dirtyJoints;
+    mutable bool m_jointsStale = false;
 };
 
 }  // namespace box2d

@@ -22,6 +22,7 @@ namespace orc {
 constexpr float kPi = 3.14159265359f;
 constexpr float kLinearSlop = 0.005f;
 constexpr float kAngularSlop = 2.0f / 180.0f * kPi;
+constexpr float kMaxAngularCorrection = 8.0f / 180.0f * kPi;
 constexpr float kPolygonRadius = 2.0f * kLinearSlop;
 constexpr float kAabbExtension = 0.1f;
 constexpr float kAabbMultiplier = 2.0f;
@@ -831,6 +832,7 @@ struct Body {
     int islandIndex;
     std::vector<int> fixtures;  // creation order; iterate reversed (newest first)
     int edgeHead;               // contact edge list head: contact*2 + side
+    int jointHead;              // joint edge list head: joint*2 + side
     bool alive;
 };
 
@@ -883,6 +885,51 @@ struct PosConstraint {
     int pointCount;
 };
 
+// b2RevoluteJoint, REF/Dynamics/Joints/b2RevoluteJoint.h:92-200 and
+// b2Joint.h (edges, island flag, collideConnected).
+struct V3 { float x, y, z; };
+inline float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross3(V3 a, V3 b) { return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+struct Mat33 { V3 ex, ey, ez; };
+// b2Mat33::Solve33 / Solve22, REF/Common/b2Math.cpp:28-56
+inline V3 solve33(const Mat33& m, V3 b) {
+    float det = dot3(m.ex, cross3(m.ey, m.ez));
+    if (det != 0.0f) det = 1.0f / det;
+    V3 x;
+    x.x = det * dot3(b, cross3(m.ey, m.ez));
+    x.y = det * dot3(m.ex, cross3(b, m.ez));
+    x.z = det * dot3(m.ex, cross3(m.ey, b));
+    return x;
+}
+inline V2 solve22(const Mat33& m, V2 b) {
+    float a11 = m.ex.x, a12 = m.ey.x, a21 = m.ex.y, a22 = m.ey.y;
+    float det = a11 * a22 - a12 * a21;
+    if (det != 0.0f) det = 1.0f / det;
+    return mk(det * (a22 * b.x - a12 * b.y), det * (a11 * b.y - a21 * b.x));
+}
+enum { kInactiveLimit = 0, kAtLowerLimit = 1, kAtUpperLimit = 2, kEqualLimits = 3 };
+struct Joint {
+    int bodyA, bodyB;
+    bool collideConnected;
+    bool islandFlag;
+    int prev, next;        // world list (newest first)
+    Edge node[2];          // node[0] on body A (other = B), node[1] on body B
+    V2 localAnchorA, localAnchorB;
+    float referenceAngle;
+    V3 impulse;
+    float motorImpulse;
+    bool enableMotor, enableLimit;
+    float maxMotorTorque, motorSpeed, lowerAngle, upperAngle;
+    int limitState;
+    bool removed;
+    // solver temporaries
+    int indexA, indexB;
+    V2 rA, rB, localCenterA, localCenterB;
+    float invMassA, invMassB, invIA, invIB;
+    Mat33 mass;
+    float motorMass;
+};
+
 struct Events { long begin = 0, end = 0; };
 
 struct World {
@@ -902,8 +949,11 @@ struct World {
     int unsupported = 0;
     Events events;
 
+    std::vector<Joint> joints;
+    int jointHead = -1;
+
     // island scratch
-    std::vector<int> islandBodies, islandContacts, stack;
+    std::vector<int> islandBodies, islandContacts, islandJoints, stack;
     std::vector<V2> posC, velV;
     std::vector<float> posA, velW;
     std::vector<VelConstraint> vcs;
@@ -989,6 +1039,7 @@ struct World {
         b.I = 0.0f; b.invI = 0.0f;
         b.islandIndex = 0;
         b.edgeHead = -1;
+        b.jointHead = -1;
         b.alive = true;
         bodies.push_back(b);
         return (int)bodies.size() - 1;
@@ -1025,9 +1076,105 @@ struct World {
         return fi;
     }
 
-    // b2Body::ShouldCollide, REF/Dynamics/b2Body.cpp:396-417 (no joints here)
-    bool shouldCollide(const Body& b, const Body& other) const {
-        return !(b.type != kDynamic && other.type != kDynamic);
+    // b2Body::ShouldCollide, REF/Dynamics/b2Body.cpp:396-417
+    bool shouldCollide(int bi, int oi) const {
+        const Body& b = bodies[bi];
+        const Body& other = bodies[oi];
+        if (b.type != kDynamic && other.type != kDynamic) return false;
+        for (int e = b.jointHead; e != -1; e = joints[e >> 1].node[e & 1].next) {
+            const Joint& j = joints[e >> 1];
+            const int o = (e & 1) ? j.bodyA : j.bodyB;
+            if (o == oi && !j.collideConnected) return false;
+        }
+        return true;
+    }
+
+    // b2World::CreateJoint + b2RevoluteJoint ctor,
+    // REF/Dynamics/b2World.cpp:212-268, REF/Dynamics/Joints/b2RevoluteJoint.cpp:47-64
+    int createJoint(const b2scene::JointDef& d) {
+        Joint j{};
+        j.bodyA = d.bodyA; j.bodyB = d.bodyB;
+        j.collideConnected = d.collideConnected != 0;
+        j.islandFlag = false;
+        j.localAnchorA = mk(d.ax, d.ay); j.localAnchorB = mk(d.bx, d.by);
+        j.referenceAngle = d.referenceAngle;
+        j.impulse = V3{0.0f, 0.0f, 0.0f};
+        j.motorImpulse = 0.0f;
+        j.lowerAngle = d.lowerAngle; j.upperAngle = d.upperAngle;
+        j.maxMotorTorque = d.maxMotorTorque; j.motorSpeed = d.motorSpeed;
+        j.enableLimit = d.enableLimit != 0; j.enableMotor = d.enableMotor != 0;
+        j.limitState = kInactiveLimit;
+        const int ji = (int)joints.size();
+        j.prev = -1;
+        j.next = jointHead;
+        joints.push_back(j);
+        if (jointHead != -1) joints[jointHead].prev = ji;
+        jointHead = ji;
+        for (int side = 0; side < 2; ++side) {
+            Body& b = bodies[side == 0 ? d.bodyA : d.bodyB];
+            Edge& e = joints[ji].node[side];
+            e.prev = -1;
+            e.next = b.jointHead;
+            if (b.jointHead != -1) joints[b.jointHead >> 1].node[b.jointHead & 1].prev = ji * 2 + side;
+            b.jointHead = ji * 2 + side;
+        }
+        if (!j.collideConnected) {
+            // flag contacts between the two bodies for filtering (bodyB's list)
+            for (int e = bodies[d.bodyB].edgeHead; e != -1; e = edgeRef(e).next)
+                if (edgeOther(e) == d.bodyA) contacts[e >> 1].flags |= cFilter;
+        }
+        return ji;
+    }
+
+    // b2World::DestroyJoint, REF/Dynamics/b2World.cpp:267-358: off the world
+    // list and both bodies' joint lists, both bodies woken. The slot stays
+    // (marked removed) so joint indices keep their creation order.
+    void destroyJoint(int ji) {
+        Joint& j = joints[ji];
+        if (j.removed) return;
+        if (j.prev != -1) joints[j.prev].next = j.next;
+        if (j.next != -1) joints[j.next].prev = j.prev;
+        if (jointHead == ji) jointHead = j.next;
+        setAwake(bodies[j.bodyA], true);
+        setAwake(bodies[j.bodyB], true);
+        for (int side = 0; side < 2; ++side) {
+            Edge& e = j.node[side];
+            Body& b = bodies[side == 0 ? j.bodyA : j.bodyB];
+            if (e.prev != -1) joints[e.prev >> 1].node[e.prev & 1].next = e.next;
+            if (e.next != -1) joints[e.next >> 1].node[e.next & 1].prev = e.prev;
+            if (b.jointHead == ji * 2 + side) b.jointHead = e.next;
+            e.prev = e.next = -1;
+        }
+        j.removed = true;
+        if (!j.collideConnected) {
+            for (int e = bodies[j.bodyB].edgeHead; e != -1; e = edgeRef(e).next)
+                if (edgeOther(e) == j.bodyA) contacts[e >> 1].flags |= cFilter;
+        }
+    }
+
+    // b2RevoluteJoint::EnableLimit, SetLimits, EnableMotor, SetMotorSpeed,
+    // SetMaxMotorTorque in that order, REF/Dynamics/Joints/b2RevoluteJoint.cpp:419-471.
+    void setJoint(int ji, bool enableLimit, float lower, float upper, bool enableMotor, float speed, float torque) {
+        Joint& j = joints[ji];
+        if (j.removed) return;
+        if (enableLimit != j.enableLimit) {
+            setAwake(bodies[j.bodyA], true);
+            setAwake(bodies[j.bodyB], true);
+            j.enableLimit = enableLimit;
+            j.impulse.z = 0.0f;
+        }
+        if (lower != j.lowerAngle || upper != j.upperAngle) {
+            setAwake(bodies[j.bodyA], true);
+            setAwake(bodies[j.bodyB], true);
+            j.impulse.z = 0.0f;
+            j.lowerAngle = lower;
+            j.upperAngle = upper;
+        }
+        setAwake(bodies[j.bodyA], true);
+        setAwake(bodies[j.bodyB], true);
+        j.enableMotor = enableMotor;
+        j.motorSpeed = speed;
+        j.maxMotorTorque = torque;
     }
     // b2ContactFilter::ShouldCollide, REF/Dynamics/b2WorldCallbacks.cpp:26-38
     static bool filter(const Fixture& a, const Fixture& b) {
@@ -1067,7 +1214,7 @@ struct World {
                 if (c.fA == fb && c.fB == fa) return;
             }
         }
-        if (!shouldCollide(bodies[bb], bodies[ba])) return;
+        if (!shouldCollide(bb, ba)) return;
         if (!filter(fixtures[fa], fixtures[fb])) return;
         // Registry: circle-circle, polygon-circle, polygon-polygon, edge-circle,
         // edge-polygon are primary; their mirrors swap the fixtures.
@@ -1170,7 +1317,7 @@ struct World {
             Body& bA = bodies[fA.body];
             Body& bB = bodies[fB.body];
             if (c.flags & cFilter) {
-                if (!shouldCollide(bB, bA) || !filter(fA, fB)) {
+                if (!shouldCollide(fB.body, fA.body) || !filter(fA, fB)) {
                     int nx = c.next;
                     destroyContact(ci);
                     ci = nx;
@@ -1439,8 +1586,11 @@ struct World {
                 velV[iB] = vB; velW[iB] = wB;
             }
         }
-        // SolveVelocityConstraints (REF :294-597)
+        // Joint velocity constraints (REF b2Island.cpp:252-255), then the
+        // iterations: joints, then contacts (REF b2Island.cpp:261-270)
+        for (int ji : islandJoints) jointInitVelocity(joints[ji], h, dtRatio);
         for (int it = 0; it < velIters; ++it) {
+            for (int ji : islandJoints) jointSolveVelocity(joints[ji], h);
             for (int i = 0; i < nc; ++i) solveVelocity(vcs[i]);
         }
         // StoreImpulses (REF :599-612)
@@ -1473,7 +1623,13 @@ struct World {
         // Position iterations with early exit (REF b2Island.cpp:308-328)
         bool positionSolved = false;
         for (int it = 0; it < posIters; ++it) {
-            if (solvePositions()) { positionSolved = true; break; }
+            const bool contactsOkay = solvePositions();
+            bool jointsOkay = true;
+            for (int ji : islandJoints) {
+                const bool jointOkay = jointSolvePosition(joints[ji]);
+                jointsOkay = jointsOkay && jointOkay;
+            }
+            if (contactsOkay && jointsOkay) { positionSolved = true; break; }
         }
         // Copy back (REF b2Island.cpp:330-339)
         for (int i = 0; i < nb; ++i) {
@@ -1502,6 +1658,181 @@ struct World {
                 for (int i = 0; i < nb; ++i) setAwake(bodies[islandBodies[i]], false);
             }
         }
+    }
+
+    // b2RevoluteJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2RevoluteJoint.cpp:66-198
+    void jointInitVelocity(Joint& j, float h, float dtRatio) {
+        (void)h;
+        const Body& bA = bodies[j.bodyA];
+        const Body& bB = bodies[j.bodyB];
+        j.indexA = bA.islandIndex; j.indexB = bB.islandIndex;
+        j.localCenterA = bA.localCenter; j.localCenterB = bB.localCenter;
+        j.invMassA = bA.invMass; j.invMassB = bB.invMass;
+        j.invIA = bA.invI; j.invIB = bB.invI;
+        const float aA = posA[j.indexA];
+        V2 vA = velV[j.indexA]; float wA = velW[j.indexA];
+        const float aB = posA[j.indexB];
+        V2 vB = velV[j.indexB]; float wB = velW[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        j.rA = mul(qA, j.localAnchorA - j.localCenterA);
+        j.rB = mul(qB, j.localAnchorB - j.localCenterB);
+        const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+        const bool fixedRotation = (iA + iB == 0.0f);
+        j.mass.ex.x = mA + mB + j.rA.y * j.rA.y * iA + j.rB.y * j.rB.y * iB;
+        j.mass.ey.x = -j.rA.y * j.rA.x * iA - j.rB.y * j.rB.x * iB;
+        j.mass.ez.x = -j.rA.y * iA - j.rB.y * iB;
+        j.mass.ex.y = j.mass.ey.x;
+        j.mass.ey.y = mA + mB + j.rA.x * j.rA.x * iA + j.rB.x * j.rB.x * iB;
+        j.mass.ez.y = j.rA.x * iA + j.rB.x * iB;
+        j.mass.ex.z = j.mass.ez.x;
+        j.mass.ey.z = j.mass.ez.y;
+        j.mass.ez.z = iA + iB;
+        j.motorMass = iA + iB;
+        if (j.motorMass > 0.0f) j.motorMass = 1.0f / j.motorMass;
+        if (!j.enableMotor || fixedRotation) j.motorImpulse = 0.0f;
+        if (j.enableLimit && !fixedRotation) {
+            const float jointAngle = aB - aA - j.referenceAngle;
+            if (absf(j.upperAngle - j.lowerAngle) < 2.0f * kAngularSlop) {
+                j.limitState = kEqualLimits;
+            } else if (jointAngle <= j.lowerAngle) {
+                if (j.limitState != kAtLowerLimit) j.impulse.z = 0.0f;
+                j.limitState = kAtLowerLimit;
+            } else if (jointAngle >= j.upperAngle) {
+                if (j.limitState != kAtUpperLimit) j.impulse.z = 0.0f;
+                j.limitState = kAtUpperLimit;
+            } else {
+                j.limitState = kInactiveLimit;
+                j.impulse.z = 0.0f;
+            }
+        } else {
+            j.limitState = kInactiveLimit;
+        }
+        if (warmStarting) {
+            j.impulse.x *= dtRatio; j.impulse.y *= dtRatio; j.impulse.z *= dtRatio;
+            j.motorImpulse *= dtRatio;
+            const V2 P = mk(j.impulse.x, j.impulse.y);
+            vA -= mA * P;
+            wA -= iA * (cross(j.rA, P) + j.motorImpulse + j.impulse.z);
+            vB += mB * P;
+            wB += iB * (cross(j.rB, P) + j.motorImpulse + j.impulse.z);
+        } else {
+            j.impulse = V3{0.0f, 0.0f, 0.0f};
+            j.motorImpulse = 0.0f;
+        }
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2RevoluteJoint::SolveVelocityConstraints, REF b2RevoluteJoint.cpp:200-292
+    void jointSolveVelocity(Joint& j, float h) {
+        V2 vA = velV[j.indexA]; float wA = velW[j.indexA];
+        V2 vB = velV[j.indexB]; float wB = velW[j.indexB];
+        const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+        const bool fixedRotation = (iA + iB == 0.0f);
+        if (j.enableMotor && j.limitState != kEqualLimits && !fixedRotation) {
+            const float Cdot = wB - wA - j.motorSpeed;
+            float impulse = -j.motorMass * Cdot;
+            const float oldImpulse = j.motorImpulse;
+            const float maxImpulse = h * j.maxMotorTorque;
+            j.motorImpulse = clampf(j.motorImpulse + impulse, -maxImpulse, maxImpulse);
+            impulse = j.motorImpulse - oldImpulse;
+            wA -= iA * impulse;
+            wB += iB * impulse;
+        }
+        if (j.enableLimit && j.limitState != kInactiveLimit && !fixedRotation) {
+            const V2 Cdot1 = vB + cross(wB, j.rB) - vA - cross(wA, j.rA);
+            const float Cdot2 = wB - wA;
+            const V3 Cdot = V3{Cdot1.x, Cdot1.y, Cdot2};
+            V3 impulse = solve33(j.mass, Cdot);
+            impulse = V3{-impulse.x, -impulse.y, -impulse.z};
+            if (j.limitState == kEqualLimits) {
+                j.impulse.x += impulse.x; j.impulse.y += impulse.y; j.impulse.z += impulse.z;
+            } else if (j.limitState == kAtLowerLimit || j.limitState == kAtUpperLimit) {
+                const float newImpulse = j.impulse.z + impulse.z;
+                const bool reduce = j.limitState == kAtLowerLimit ? newImpulse < 0.0f : newImpulse > 0.0f;
+                if (reduce) {
+                    const V2 rhs = -Cdot1 + j.impulse.z * mk(j.mass.ez.x, j.mass.ez.y);
+                    const V2 reduced = solve22(j.mass, rhs);
+                    impulse.x = reduced.x;
+                    impulse.y = reduced.y;
+                    impulse.z = -j.impulse.z;
+                    j.impulse.x += reduced.x;
+                    j.impulse.y += reduced.y;
+                    j.impulse.z = 0.0f;
+                } else {
+                    j.impulse.x += impulse.x; j.impulse.y += impulse.y; j.impulse.z += impulse.z;
+                }
+            }
+            const V2 P = mk(impulse.x, impulse.y);
+            vA -= mA * P;
+            wA -= iA * (cross(j.rA, P) + impulse.z);
+            vB += mB * P;
+            wB += iB * (cross(j.rB, P) + impulse.z);
+        } else {
+            const V2 Cdot = vB + cross(wB, j.rB) - vA - cross(wA, j.rA);
+            const V2 impulse = solve22(j.mass, -Cdot);
+            j.impulse.x += impulse.x;
+            j.impulse.y += impulse.y;
+            vA -= mA * impulse;
+            wA -= iA * cross(j.rA, impulse);
+            vB += mB * impulse;
+            wB += iB * cross(j.rB, impulse);
+        }
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2RevoluteJoint::SolvePositionConstraints, REF b2RevoluteJoint.cpp:294-378
+    bool jointSolvePosition(Joint& j) {
+        V2 cA = posC[j.indexA]; float aA = posA[j.indexA];
+        V2 cB = posC[j.indexB]; float aB = posA[j.indexB];
+        float angularError = 0.0f, positionError = 0.0f;
+        const bool fixedRotation = (j.invIA + j.invIB == 0.0f);
+        if (j.enableLimit && j.limitState != kInactiveLimit && !fixedRotation) {
+            const float angle = aB - aA - j.referenceAngle;
+            float limitImpulse = 0.0f;
+            if (j.limitState == kEqualLimits) {
+                const float C = clampf(angle - j.lowerAngle, -kMaxAngularCorrection, kMaxAngularCorrection);
+                limitImpulse = -j.motorMass * C;
+                angularError = absf(C);
+            } else if (j.limitState == kAtLowerLimit) {
+                float C = angle - j.lowerAngle;
+                angularError = -C;
+                C = clampf(C + kAngularSlop, -kMaxAngularCorrection, 0.0f);
+                limitImpulse = -j.motorMass * C;
+            } else if (j.limitState == kAtUpperLimit) {
+                float C = angle - j.upperAngle;
+                angularError = C;
+                C = clampf(C - kAngularSlop, 0.0f, kMaxAngularCorrection);
+                limitImpulse = -j.motorMass * C;
+            }
+            aA -= j.invIA * limitImpulse;
+            aB += j.invIB * limitImpulse;
+        }
+        {
+            const Rot qA = stepRot(aA), qB = stepRot(aB);
+            const V2 rA = mul(qA, j.localAnchorA - j.localCenterA);
+            const V2 rB = mul(qB, j.localAnchorB - j.localCenterB);
+            const V2 C = cB + rB - cA - rA;
+            positionError = length(C);
+            const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+            float k11 = mA + mB + iA * rA.y * rA.y + iB * rB.y * rB.y;
+            float k21 = -iA * rA.x * rA.y - iB * rB.x * rB.y;
+            float k12 = k21;
+            float k22 = mA + mB + iA * rA.x * rA.x + iB * rB.x * rB.x;
+            // b2Mat22::Solve (REF b2Math.h:242-254): ex = (k11, k21), ey = (k12, k22)
+            float det = k11 * k22 - k12 * k21;
+            if (det != 0.0f) det = 1.0f / det;
+            V2 impulse = mk(det * (k22 * C.x - k12 * C.y), det * (k11 * C.y - k21 * C.x));
+            impulse = -impulse;
+            cA -= mA * impulse;
+            aA -= iA * cross(rA, impulse);
+            cB += mB * impulse;
+            aB += iB * cross(rB, impulse);
+        }
+        posC[j.indexA] = cA; posA[j.indexA] = aA;
+        posC[j.indexB] = cB; posA[j.indexB] = aB;
+        return positionError <= kLinearSlop && angularError <= kAngularSlop;
     }
 
     void solveVelocity(VelConstraint& vc) {
@@ -1658,6 +1989,7 @@ struct World {
         const int nb = (int)bodies.size();
         for (auto& b : bodies) b.flags &= ~kIsland;
         for (int ci = contactHead; ci != -1; ci = contacts[ci].next) contacts[ci].flags &= ~cIsland;
+        for (auto& j : joints) j.islandFlag = false;
         stack.resize(nb > 0 ? nb : 1);
         for (int seed = nb - 1; seed >= 0; --seed) {
             Body& sb = bodies[seed];
@@ -1667,6 +1999,7 @@ struct World {
             if (sb.type == kStatic) continue;
             islandBodies.clear();
             islandContacts.clear();
+            islandJoints.clear();
             int sc = 0;
             stack[sc++] = seed;
             sb.flags |= kIsland;
@@ -1685,6 +2018,18 @@ struct World {
                     islandContacts.push_back(e >> 1);
                     c.flags |= cIsland;
                     const int other = edgeOther(e);
+                    if (bodies[other].flags & kIsland) continue;
+                    stack[sc++] = other;
+                    bodies[other].flags |= kIsland;
+                }
+                // joints of this body (REF b2World.cpp:502-529)
+                for (int e = b.jointHead; e != -1; e = joints[e >> 1].node[e & 1].next) {
+                    Joint& j = joints[e >> 1];
+                    if (j.islandFlag) continue;
+                    const int other = (e & 1) ? j.bodyA : j.bodyB;
+                    if ((bodies[other].flags & kActive) == 0) continue;
+                    islandJoints.push_back(e >> 1);
+                    j.islandFlag = true;
                     if (bodies[other].flags & kIsland) continue;
                     stack[sc++] = other;
                     bodies[other].flags |= kIsland;
@@ -1722,7 +2067,9 @@ struct World {
         }
     }
 
-    int load(const b2scene::BodyDef* bd, int nb, const b2scene::FixtureDef* fd, int nf) {
+    int load(const b2scene::BodyDef* bd, int nb, const b2scene::FixtureDef* fd, int nf,
+             const b2scene::JointDef* jd = nullptr, int nj = 0) {
+        const int base = (int)bodies.size();
         for (int i = 0; i < nb; ++i) {
             const int bi = createBody(bd[i]);
             for (int k = 0; k < bd[i].fixtureCount; ++k) {
@@ -1747,6 +2094,14 @@ struct World {
                 }
                 createFixture(bi, s, f);
             }
+        }
+        for (int k = 0; k < nj; ++k) {
+            b2scene::JointDef d = jd[k];
+            if (d.bodyA < 0 || d.bodyB < 0 || d.bodyA >= nb || d.bodyB >= nb || d.bodyA == d.bodyB)
+                return B2G_ERR_ARG;
+            d.bodyA += base;
+            d.bodyB += base;
+            createJoint(d);
         }
         return B2G_OK;
     }
@@ -1785,7 +2140,41 @@ int orc_world_load(void* h, const b2scene::BodyDef* bd, int nb, const b2scene::F
 
 int orc_world_load_scene(void* h, const char* kind, int n, unsigned seed) {
     b2scene::Scene s = b2scene::byName(kind, n, seed);
-    return orc_world_load(h, s.bodies.data(), (int)s.bodies.size(), s.fixtures.data(), (int)s.fixtures.size());
+    return ((World*)h)->load(s.bodies.data(), (int)s.bodies.size(), s.fixtures.data(), (int)s.fixtures.size(),
+                             s.joints.data(), (int)s.joints.size());
+}
+
+int orc_world_joint_count(void* h) { return (int)((World*)h)->joints.size(); }
+
+int orc_world_set_joint(void* h, int joint, int enableLimit, float lower, float upper, int enableMotor, float speed,
+                        float maxTorque) {
+    World* w = (World*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint].removed) return -1;
+    w->setJoint(joint, enableLimit != 0, lower, upper, enableMotor != 0, speed, maxTorque);
+    return 0;
+}
+
+int orc_world_destroy_joint(void* h, int joint) {
+    World* w = (World*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint].removed) return -1;
+    w->destroyJoint(joint);
+    return 0;
+}
+
+// Per joint in creation order: impulse x, y, z and motor impulse.
+void orc_world_get_joints(void* h, float* out) {
+    World* w = (World*)h;
+    for (size_t i = 0; i < w->joints.size(); ++i) {
+        const Joint& j = w->joints[i];
+        if (j.removed) {  // destroyed
+            for (int k = 0; k < 4; ++k) out[4 * i + k] = 0.0f;
+            continue;
+        }
+        out[4 * i] = j.impulse.x;
+        out[4 * i + 1] = j.impulse.y;
+        out[4 * i + 2] = j.impulse.z;
+        out[4 * i + 3] = j.motorImpulse;
+    }
 }
 
 int orc_world_step(void* h, float dt, int vel, int pos, int nsteps) {

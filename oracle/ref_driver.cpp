@@ -33,7 +33,32 @@ struct RefWorld {
     b2World* world;
     std::vector<b2Body*> bodies;
     std::vector<b2Fixture*> fixtures;
+    std::vector<b2RevoluteJoint*> joints;
 };
+
+// Scene joints as b2RevoluteJoints (local anchors and reference angle given
+// directly, as b2RevoluteJointDef::Initialize computes them for bodies at
+// angle zero).
+int loadJoints(RefWorld* w, int base, const b2scene::JointDef* jd, int nj) {
+    for (int k = 0; k < nj; ++k) {
+        const b2scene::JointDef& d = jd[k];
+        b2RevoluteJointDef def;
+        def.bodyA = w->bodies[base + d.bodyA];
+        def.bodyB = w->bodies[base + d.bodyB];
+        def.collideConnected = d.collideConnected != 0;
+        def.localAnchorA.Set(d.ax, d.ay);
+        def.localAnchorB.Set(d.bx, d.by);
+        def.referenceAngle = d.referenceAngle;
+        def.enableLimit = d.enableLimit != 0;
+        def.lowerAngle = d.lowerAngle;
+        def.upperAngle = d.upperAngle;
+        def.enableMotor = d.enableMotor != 0;
+        def.motorSpeed = d.motorSpeed;
+        def.maxMotorTorque = d.maxMotorTorque;
+        w->joints.push_back((b2RevoluteJoint*)w->world->CreateJoint(&def));
+    }
+    return 0;
+}
 
 int fixtureIndex(const b2Fixture* f) { return (int)(intptr_t)f->GetUserData(); }
 
@@ -217,7 +242,50 @@ int ref_world_load(void* h, const b2scene::BodyDef* bodies, int nb, const b2scen
 
 int ref_world_load_scene(void* h, const char* kind, int n, unsigned seed) {
     b2scene::Scene s = b2scene::byName(kind, n, seed);
-    return ref_world_load(h, s.bodies.data(), (int)s.bodies.size(), s.fixtures.data(), (int)s.fixtures.size());
+    RefWorld* w = (RefWorld*)h;
+    const int base = (int)w->bodies.size();
+    int rc = ref_world_load(h, s.bodies.data(), (int)s.bodies.size(), s.fixtures.data(), (int)s.fixtures.size());
+    if (rc != 0) return rc;
+    return loadJoints(w, base, s.joints.data(), (int)s.joints.size());
+}
+
+int ref_world_joint_count(void* h) { return (int)((RefWorld*)h)->joints.size(); }
+
+int ref_world_set_joint(void* h, int joint, int enableLimit, float lower, float upper, int enableMotor, float speed,
+                        float maxTorque) {
+    RefWorld* w = (RefWorld*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
+    b2RevoluteJoint* j = w->joints[joint];
+    j->EnableLimit(enableLimit != 0);
+    j->SetLimits(lower, upper);
+    j->EnableMotor(enableMotor != 0);
+    j->SetMotorSpeed(speed);
+    j->SetMaxMotorTorque(maxTorque);
+    return 0;
+}
+
+int ref_world_destroy_joint(void* h, int joint) {
+    RefWorld* w = (RefWorld*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
+    w->world->DestroyJoint(w->joints[joint]);
+    w->joints[joint] = nullptr;
+    return 0;
+}
+
+// Per joint in creation order: impulse x, y, z and motor impulse.
+void ref_world_get_joints(void* h, float* out) {
+    RefWorld* w = (RefWorld*)h;
+    for (size_t i = 0; i < w->joints.size(); ++i) {
+        const b2RevoluteJoint* j = w->joints[i];
+        if (j == nullptr) {  // destroyed
+            for (int k = 0; k < 4; ++k) out[4 * i + k] = 0.0f;
+            continue;
+        }
+        out[4 * i] = j->m_impulse.x;
+        out[4 * i + 1] = j->m_impulse.y;
+        out[4 * i + 2] = j->m_impulse.z;
+        out[4 * i + 3] = j->m_motorImpulse;
+    }
 }
 
 void ref_world_step(void* h, float dt, int vel, int pos, int nsteps) {

@@ -1,0 +1,158 @@
+"""Revolute joints on the GPU step (coloured solver) against the oracle.
+
+* Pendulums (one joint per island, no contacts): the coloured solver's
+  order inside such an island is the reference's, so trajectories and joint
+  impulses equal the oracle's bit for bit (trig as the device evaluates it),
+  also across mid-run edits through the box2d:: setters (limits switched and
+  moved, motors retuned, a joint destroyed).
+* Chains (joints and contacts in one island, colliding chains): the
+  coloured order differs from the reference's, so the checks are the
+  constraint's own: every hinge holds within the slop the oracle itself
+  leaves, no contact joins two bodies that a joint keeps apart, and the
+  bodies stay finite.
+* The reference-order mode rejects worlds with joints.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import engines as E
+
+pytestmark = pytest.mark.gpu
+F = ctypes.c_float
+
+
+@pytest.fixture
+def wide_trig(orclib):
+    orclib.orc_set_trig_mode(1)
+    yield
+    orclib.orc_set_trig_mode(0)
+
+
+def _joints(w):
+    nj = w.e.fn("world_joint_count")(w.h)
+    out = np.zeros(4 * nj, np.float32)
+    w.e.fn("world_get_joints")(w.h, E.ptr(out))
+    return out.reshape(-1, 4)
+
+
+def _bits(a):
+    return (a + np.float32(0)).view(np.uint32)  # -0 == +0
+
+
+def test_pendulums_bitexact(gpu, orc, wide_trig):
+    g = gpu.world("pendulums", 24, 0, sleep=True)
+    o = orc.world("pendulums", 24, 0, sleep=True)
+    for s in range(400):
+        g.step(1)
+        o.step(1)
+        if s % 50 == 49 or s < 3:
+            gb, ob = g.bodies(), o.bodies()
+            bad = np.where(~np.all(_bits(gb[:, :10]) == _bits(ob[:, :10]), axis=1))[0]
+            assert len(bad) == 0, f"step {s + 1}: bodies {bad[:8]} differ"
+    assert np.array_equal(_bits(_joints(g)), _bits(_joints(o)))
+    st = g.stats()
+    assert st["contacts"] == 0
+    g.close()
+    o.close()
+
+
+def _edit(w):
+    w.e.fn("world_set_joint")(w.h, 0, 1, F(-0.2), F(0.25), 0, F(0.0), F(0.0))
+    w.e.fn("world_set_joint")(w.h, 1, 1, F(-0.1), F(0.1), 0, F(0.0), F(0.0))
+    w.e.fn("world_set_joint")(w.h, 2, 0, F(0.0), F(0.0), 1, F(-2.0), F(5.0))
+    w.e.fn("world_set_joint")(w.h, 4, 0, F(-0.4), F(0.3), 0, F(0.0), F(0.0))
+    w.e.fn("world_set_joint")(w.h, 6, 1, F(0.0), F(0.0), 1, F(0.5), F(10.0))
+    assert w.e.fn("world_destroy_joint")(w.h, 3) == 0
+
+
+def test_joint_edits_bitexact(gpu, orc, wide_trig):
+    g = gpu.world("pendulums", 24, 0, sleep=True)
+    o = orc.world("pendulums", 24, 0, sleep=True)
+    g.step(90)
+    o.step(90)
+    _edit(g)
+    _edit(o)
+    for s in range(240):
+        g.step(1)
+        o.step(1)
+        if s % 40 == 39 or s < 3:
+            gb, ob = g.bodies(), o.bodies()
+            bad = np.where(~np.all(_bits(gb[:, :10]) == _bits(ob[:, :10]), axis=1))[0]
+            assert len(bad) == 0, f"step {s + 91}: bodies {bad[:8]} differ"
+    jg, jo = _joints(g), _joints(o)
+    assert np.array_equal(_bits(jg), _bits(jo))
+    assert np.all(jg[3] == 0.0)
+    g.close()
+    o.close()
+
+
+def _chain_gaps(b, n, links):
+    """Hinge gaps |anchor on A - anchor on B| of the chains scene: per chain a
+    static anchor at the chain's first hinge, then `links` boxes 0.5 m long
+    joined end to end."""
+    px, py, qs, qc = b[:, E.PX], b[:, E.PY], b[:, E.QS], b[:, E.QC]
+
+    def world_point(i, lx):
+        return np.array([px[i] + qc[i] * lx, py[i] + qs[i] * lx])
+
+    gaps = []
+    for c in range(n):
+        anchor = 1 + c * (links + 1)
+        for i in range(links):
+            cur = anchor + 1 + i
+            pa = world_point(anchor, 0.0) if i == 0 else world_point(cur - 1, 0.25)
+            pb = world_point(cur, -0.25)
+            gaps.append(np.hypot(*(pa - pb)))
+    return np.array(gaps)
+
+
+@pytest.mark.parametrize("n,links,steps,sleep,vel,pos", [
+    (3, 12, 400, False, 8, 3),
+    (6, 16, 300, True, 4, 2),
+])
+def test_chains_hold(gpu, orc, n, links, steps, sleep, vel, pos):
+    g = gpu.world("chains", n, links, sleep=sleep)
+    o = orc.world("chains", n, links, sleep=sleep)
+    worst_g = worst_o = 0.0
+    # fixture -> body: fixture 0 is the ground edge, then one box per link
+    fb = np.concatenate([[0], [1 + c * (links + 1) + 1 + i for c in range(n) for i in range(links)]])
+    joined = set()
+    for c in range(n):
+        anchor = 1 + c * (links + 1)
+        for i in range(links):
+            a, b = (anchor if i == 0 else anchor + i), anchor + 1 + i
+            joined.add((min(a, b), max(a, b)))
+    touched = 0
+    for s in range(steps):
+        g.step(1, vel=vel, pos=pos)
+        o.step(1, vel=vel, pos=pos)
+        if s % 20 == 19:
+            gb = g.bodies()
+            assert np.all(np.isfinite(gb)), f"step {s + 1}"
+            worst_g = max(worst_g, _chain_gaps(gb, n, links).max())
+            worst_o = max(worst_o, _chain_gaps(o.bodies(), n, links).max())
+            meta, _ = g.contacts()
+            touched += len(meta)
+            for fa, fbb in meta[:, :2]:
+                ba, bb = fb[fa], fb[fbb]
+                assert (min(ba, bb), max(ba, bb)) not in joined, f"contact between jointed bodies {ba} {bb}"
+    assert touched > 0  # the chains do hit each other or the ground
+    assert worst_g <= max(0.02, 3.0 * worst_o), (worst_g, worst_o)
+    j = _joints(g)
+    assert np.all(np.isfinite(j))
+    g.close()
+    o.close()
+
+
+def test_reference_mode_rejects_joints(gpu):
+    g = gpu.world("pendulums", 4, 0)
+    g.set_mode(1)
+    with pytest.raises(RuntimeError, match="joints"):
+        g.step(1)
+    g.set_mode(0)
+    g.step(1)
+    g.close()

@@ -23,6 +23,9 @@ F = ctypes.c_float
     ("pyramid", 6, 0, 200, True, 4, 1),
     ("pile", 300, 1, 400, False, 8, 3),
     ("pile", 1500, 7, 150, True, 8, 3),
+    ("pendulums", 24, 0, 400, True, 8, 3),     # revolute joints: free, limited, motorised
+    ("chains", 3, 12, 400, False, 8, 3),       # 3 chains of 12 links hitting the ground
+    ("chains", 2, 32, 250, True, 4, 2),
 ])
 def test_world_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, vel, pos):
     a = ref.world(kind, n, seed, sleep=sleep)
@@ -37,7 +40,80 @@ def test_world_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, vel, p
     assert np.array_equal(ma, mb), "contact lists differ (order or fixtures or flags)"
     assert np.array_equal(E.manifold_bits(na), E.manifold_bits(nb)), "manifolds differ"
     assert a.penetration() == b.penetration()
+    nj = a.e.fn("world_joint_count")(a.h)
+    assert nj == b.e.fn("world_joint_count")(b.h)
+    if nj:
+        ja, jb = np.zeros(4 * nj, np.float32), np.zeros(4 * nj, np.float32)
+        a.e.fn("world_get_joints")(a.h, E.ptr(ja))
+        b.e.fn("world_get_joints")(b.h, E.ptr(jb))
+        assert np.array_equal(ja.view(np.uint32), jb.view(np.uint32)), "joint impulses differ"
     a.close(); b.close()
+
+
+def _edit_joints(w, kind):
+    """Mid-run joint edits through each engine's setters: limits switched
+    and moved, motors switched and retuned, joints destroyed."""
+    if kind == "pendulums":
+        w.e.fn("world_set_joint")(w.h, 0, 1, F(-0.2), F(0.25), 0, F(0.0), F(0.0))     # limit on
+        w.e.fn("world_set_joint")(w.h, 1, 1, F(-0.1), F(0.1), 0, F(0.0), F(0.0))      # limits moved
+        w.e.fn("world_set_joint")(w.h, 2, 0, F(0.0), F(0.0), 1, F(-2.0), F(5.0))      # motor retuned
+        w.e.fn("world_set_joint")(w.h, 4, 0, F(-0.4), F(0.3), 0, F(0.0), F(0.0))      # limit off
+        w.e.fn("world_set_joint")(w.h, 6, 1, F(0.0), F(0.0), 1, F(0.5), F(10.0))      # equal limits + motor
+        assert w.e.fn("world_destroy_joint")(w.h, 3) == 0
+    else:
+        assert w.e.fn("world_destroy_joint")(w.h, 5) == 0                             # chain torn apart
+        assert w.e.fn("world_destroy_joint")(w.h, 17) == 0
+
+
+@pytest.mark.parametrize("kind,n,seed,sleep", [
+    ("pendulums", 24, 0, True),
+    ("chains", 2, 12, False),
+])
+def test_joint_edits_bitexact(ref, orc, kind, n, seed, sleep):
+    a = ref.world(kind, n, seed, sleep=sleep)
+    b = orc.world(kind, n, seed, sleep=sleep)
+    a.step(90)
+    b.step(90)
+    _edit_joints(a, kind)
+    _edit_joints(b, kind)
+    for s in range(240):
+        a.step(1)
+        b.step(1)
+        assert np.array_equal(a.bodies().view(np.uint32), b.bodies().view(np.uint32)), f"{kind} step {s + 91}"
+    nj = a.e.fn("world_joint_count")(a.h)
+    ja, jb = np.zeros(4 * nj, np.float32), np.zeros(4 * nj, np.float32)
+    a.e.fn("world_get_joints")(a.h, E.ptr(ja))
+    b.e.fn("world_get_joints")(b.h, E.ptr(jb))
+    assert np.array_equal(ja.view(np.uint32), jb.view(np.uint32))
+    ma, _ = a.contacts()
+    mb, _ = b.contacts()
+    assert np.array_equal(ma, mb)
+    c = orc.world(kind, n, seed, sleep=sleep)  # the edits change the motion
+    c.step(330)
+    assert not np.array_equal(c.bodies(), b.bodies())
+    if kind == "pendulums":
+        jb = jb.reshape(-1, 4)
+        assert np.all(jb[3] == 0.0)
+        cb, bb = c.bodies(), b.bodies()
+        assert cb[4, E.CY] > bb[4, E.CY] + 10.0  # the box of the destroyed joint falls
+    a.close(); b.close(); c.close()
+
+
+def test_pendulum_closed_forms(orc):
+    """Free pendulums: the hinge holds each box at its length from the
+    anchor (position error below the linear slop) and the motorised joints
+    spin their box at the motor speed while the torque suffices."""
+    w = orc.world("pendulums", 24, 0, sleep=False)
+    b0 = w.bodies()
+    w.step(240)
+    b = w.bodies()
+    anchor_y = 20.0
+    for i in range(24):
+        x = 5.0 * i
+        length = 1.0 + 0.05 * (i % 20)
+        r = np.hypot(b[1 + i, E.CX] - x, b[1 + i, E.CY] - anchor_y)
+        assert abs(r - length) < 0.005, (i, r, length)
+    w.close()
 
 
 def test_pyramid_reference_facts(orc):
