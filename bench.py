@@ -241,6 +241,62 @@ def learned_warm_start(lib, args):
     out["learned_step"] = {"value": args.bodies * args.steps / wall, "unit": "body-steps/s",
                            "ms_per_step": 1e3 * wall / args.steps, "mean_penetration_m": float(pen[0]),
                            "max_penetration_m": float(pen[1])}
+    return out, params
+
+
+def stress(lib, args, params):
+    """Stress: args.stress_worlds worlds, each a random pile of
+    args.stress_bodies bodies plus args.stress_bodies / 50 free chains of 32
+    boxes 0.5 x 0.1 m joined by revolute joints (the links are bodies in
+    addition to the pile's), in sub-worlds of one context. The plain solver
+    at 8/3 iterations against the learned warm start at 4/2, each from a
+    fresh load over args.stress_steps steps: throughput, penetration, joint
+    drift and energy at the end."""
+    b2g = box2dpp_amd.b2g()
+    b2g.b2g_last_error.restype = ctypes.c_char_p
+    variants = [("plain", args.vel, args.pos, False)]
+    if params is not None:
+        variants.append(("learned", 4, 2, True))
+    out = {"workload": f"{args.stress_worlds} worlds x (random pile of {args.stress_bodies} bodies + "
+                       f"{args.stress_bodies // 50} chains of 32 links), seed {args.stress_seed}, "
+                       f"steps 1-{args.stress_steps}, sleep and CCD off, one context"}
+    for name, vel, pos, learned in variants:
+        progress(f"stress: loading ({name})")
+        h = ctypes.c_void_p(lib.b2pp_world_create(0.0, -10.0))
+        lib.b2pp_world_set_flags(h, 0, 1, 0, 0)
+        t0 = time.perf_counter()
+        rc = lib.b2pp_world_load_worlds(h, b"stress", args.stress_worlds, args.stress_bodies,
+                                        ctypes.c_uint(args.stress_seed))
+        if rc != 0:
+            raise RuntimeError(f"stress load failed rc={rc}: {lib.b2pp_world_last_error(h).decode()}")
+        ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
+        if learned:
+            _check(b2g, ctx, b2g.b2g_mlp_set_params(ctx, params))
+            lib.b2pp_world_set_learned(h, 1, ctypes.c_float(0.0))
+        lib.b2pp_world_sync(h)
+        load_s = time.perf_counter() - t0
+        bodies = lib.b2pp_world_body_count(h)
+        joints = lib.b2pp_world_joint_count(h)
+        progress(f"stress: {bodies} bodies, {joints} joints loaded in {load_s:.1f} s; {vel}/{pos} iterations")
+        t0 = time.perf_counter()
+        for k in range(0, args.stress_steps, 100):
+            step(lib, h, min(100, args.stress_steps - k), vel, pos)
+        lib.b2pp_world_sync(h)
+        wall = time.perf_counter() - t0
+        pen = np.zeros(3, np.float64)
+        lib.b2pp_world_penetration(h, pen.ctypes.data_as(ctypes.c_void_p))
+        drift = np.zeros(3, np.float64)
+        _check(b2g, ctx, b2g.b2g_joint_drift(ctx, drift.ctypes.data_as(ctypes.c_void_p)))
+        energy = np.zeros(2, np.float64)
+        _check(b2g, ctx, b2g.b2g_energy(ctx, energy.ctypes.data_as(ctypes.c_void_p)))
+        st = stats_of(lib, h)
+        lib.b2pp_world_destroy(h)
+        out[name] = {"value": bodies * args.stress_steps / wall, "unit": "body-steps/s", "bodies": bodies,
+                     "joints": joints, "vel_iters": vel, "pos_iters": pos, "ms_per_step": 1e3 * wall / args.stress_steps,
+                     "load_seconds": round(load_s, 2), "contacts": st.contacts, "colours": st.colours,
+                     "mean_penetration_m": float(pen[0]), "max_penetration_m": float(pen[1]),
+                     "joint_drift_mean_m": float(drift[0]), "joint_drift_max_m": float(drift[1]),
+                     "kinetic_energy_J": float(energy[0]), "potential_energy_J": float(energy[1])}
     return out
 
 
@@ -339,6 +395,11 @@ def main():
     ap.add_argument("--batch-seed", type=int, default=1)
     ap.add_argument("--batch-cpu-worlds", type=int, default=1024)
     ap.add_argument("--batch-cpu-steps", type=int, default=500)
+    ap.add_argument("--no-stress", action="store_true", help="skip the stress run (piles with jointed chains)")
+    ap.add_argument("--stress-worlds", type=int, default=4)
+    ap.add_argument("--stress-bodies", type=int, default=125000)
+    ap.add_argument("--stress-steps", type=int, default=600)
+    ap.add_argument("--stress-seed", type=int, default=1)
     ap.add_argument("--log-seed", type=int, default=2)
     ap.add_argument("--eval-seed", type=int, default=1001)
     ap.add_argument("--log-capacity", type=int, default=10_000_000)
@@ -412,8 +473,9 @@ def main():
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
     lib.b2pp_world_destroy(h)
 
-    learned = None if args.no_learned else learned_warm_start(lib, args)
+    learned, params = (None, None) if args.no_learned else learned_warm_start(lib, args)
     batched = None if args.no_batched else batched_worlds(lib, args)
+    stressed = None if args.no_stress else stress(lib, args, params)
 
     cpu = None
     if not args.no_cpu:
@@ -445,6 +507,7 @@ def main():
         "cpu_baseline": cpu,
         "learned_warm_start": learned,
         "batched_worlds": batched,
+        "stress": stressed,
         "final_state": {"contacts": st.contacts, "constraints": st.constraints, "points": st.points,
                         "colours": st.colours, "mean_penetration_m": float(pen[0]), "max_penetration_m": float(pen[1])},
         "phase_ms_per_step": {k: round(v[1] / max(args.profile_steps, 1), 4) for k, v in phases.items()},

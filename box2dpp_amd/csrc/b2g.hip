@@ -761,6 +761,12 @@ __global__ void k_warm_start(WorldView w, ConView cv, int begin, int end) {
     warmStartOne(w, cv, k);
 }
 
+// The overflow colour holds constraints that share bodies: one lane solves
+// them in order (launched as a single one-lane workgroup).
+__global__ void k_warm_start_serial(WorldView w, ConView cv, int begin, int end) {
+    for (int k = begin; k < end; ++k) warmStartOne(w, cv, k);
+}
+
 // b2ContactSolver::SolveVelocityConstraints, REF/Dynamics/Contacts/b2ContactSolver.cpp:294-597:
 // friction per point clamped to the friction cone, then the normal impulse,
 // by a 2x2 linear complementarity solve (total enumeration) for two points.
@@ -887,6 +893,10 @@ __global__ void k_solve_velocity(WorldView w, ConView cv, int begin, int end) {
     const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= end) return;
     solveVelocityOne(w, cv, k);
+}
+
+__global__ void k_solve_velocity_serial(WorldView w, ConView cv, int begin, int end) {
+    for (int k = begin; k < end; ++k) solveVelocityOne(w, cv, k);
 }
 
 // b2ContactSolver::StoreImpulses, REF/Dynamics/Contacts/b2ContactSolver.cpp:599-612.
@@ -1032,6 +1042,18 @@ __global__ void k_solve_position(WorldView w, ConView cv, int begin, int end, Co
     if (threadIdx.x == 0 && blockMin < 0) atomicMin(&cnt->minSep[it], blockMin);
 }
 
+__global__ void k_solve_position_serial(WorldView w, ConView cv, int begin, int end, Counters* cnt, int it) {
+    int lowest = 0;
+    for (int k = begin; k < end; ++k) {
+        const int root = cv.island[k];
+        if (it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0) continue;
+        const float minSep = solvePositionOne(w, cv, k);
+        if (minSep < -3.0f * kLinearSlop) w.islandMinSep[it * w.nBodies + root] = 1;
+        if (minSep < 0.0f) lowest = min(lowest, encodeOrdered(minSep));
+    }
+    if (lowest < 0) atomicMin(&cnt->minSep[it], lowest);
+}
+
 // ---------------------------------------------------------------- joints
 // A joint is solved when one of its bodies is in an awake island and both
 // are active (the joints b2World::Solve's island search adds,
@@ -1143,6 +1165,10 @@ __global__ void k_joint_init(WorldView w, JointView jv, int begin, int end, floa
     jointInitOne(w, jv, jv.order[k], dtRatio, warm);
 }
 
+__global__ void k_joint_init_serial(WorldView w, JointView jv, int begin, int end, float dtRatio, int warm) {
+    for (int k = begin; k < end; ++k) jointInitOne(w, jv, jv.order[k], dtRatio, warm);
+}
+
 // b2RevoluteJoint::SolveVelocityConstraints, REF b2RevoluteJoint.cpp:200-292
 __device__ inline void jointSolveVelocityOne(const WorldView& w, const JointView& jv, int j, float h) {
     const int2 b = jv.bodies[j];
@@ -1227,6 +1253,10 @@ __global__ void k_joint_solve_velocity(WorldView w, JointView jv, int begin, int
     jointSolveVelocityOne(w, jv, jv.order[k], h);
 }
 
+__global__ void k_joint_solve_velocity_serial(WorldView w, JointView jv, int begin, int end, float h) {
+    for (int k = begin; k < end; ++k) jointSolveVelocityOne(w, jv, jv.order[k], h);
+}
+
 // b2RevoluteJoint::SolvePositionConstraints, REF b2RevoluteJoint.cpp:294-378.
 // A joint left with an error above the slops marks its island's pass as
 // failed (the jointsOkay of b2Island::Solve, REF/Dynamics/b2Island.cpp:310-328).
@@ -1302,6 +1332,10 @@ __global__ void k_joint_solve_position(WorldView w, JointView jv, int begin, int
     jointSolvePositionOne(w, jv, jv.order[k], it);
 }
 
+__global__ void k_joint_solve_position_serial(WorldView w, JointView jv, int begin, int end, int it) {
+    for (int k = begin; k < end; ++k) jointSolvePositionOne(w, jv, jv.order[k], it);
+}
+
 // Island labelling through joints whose bodies are both non-static and active.
 __global__ void k_island_hook_joints(WorldView w, JointView jv, int* changed) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1314,6 +1348,38 @@ __global__ void k_island_hook_joints(WorldView w, JointView jv, int* changed) {
     if (la == lb) return;
     atomicMin(&w.islandOf[la > lb ? la : lb], la < lb ? la : lb);
     *changed = 1;
+}
+
+// Position error of each joint, |cB + rB - cA - rA| as
+// b2RevoluteJoint::SolvePositionConstraints measures it
+// (REF/Dynamics/Joints/b2RevoluteJoint.cpp:336-343); -1 for removed joints.
+__global__ void k_joint_drift(WorldView w, JointView jv, float* out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= jv.n) return;
+    if (jv.flags[j] & JF_REMOVED) {
+        out[j] = -1.0f;
+        return;
+    }
+    const int2 b = jv.bodies[j];
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float4 an = jv.anchors[j];
+    const V2 rA = mul(rotOf(pA.z), mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(rotOf(pB.z), mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    out[j] = lengthOf(mk(pB.x, pB.y) + rB - mk(pA.x, pA.y) - rA);
+}
+
+// Kinetic (0.5 m v.v + 0.5 I w^2) and potential (-m g.c) energy per body.
+__global__ void k_body_energy(WorldView w, V2 g, double2* out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies) return;
+    const float4 ms = w.mass[b];
+    const double m = ms.x > 0.0f ? 1.0 / (double)ms.x : 0.0;
+    const double I = ms.y > 0.0f ? 1.0 / (double)ms.y : 0.0;
+    const float4 v = w.vel[b], p = w.pos[b];
+    const double ke = 0.5 * m * ((double)v.x * v.x + (double)v.y * v.y) + 0.5 * I * (double)v.z * v.z;
+    const double pe = -m * ((double)g.x * p.x + (double)g.y * p.y);
+    out[b] = make_double2(ke, pe);
 }
 
 // Copy-back transform of every solved body (b2Body::SynchronizeTransform,
@@ -3012,49 +3078,54 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
                 const int b0 = colourStart[c], b1 = colourStart[c + 1];
                 if (b1 <= b0) continue;
                 const int pc = profStart(ctx);
-                if (c == kOverflowColour) {
-                    for (int k = b0; k < b1; ++k) kernelLaunch(k, k + 1);
-                } else {
-                    kernelLaunch(b0, b1);
-                }
+                kernelLaunch(b0, b1, c == kOverflowColour);
                 profEnd(ctx, phase, pc);
             }
         };
-        // Joints: colour by colour; joints of the overflow colour one at a time.
+        // Joints: colour by colour; the overflow colour by one lane in order.
         const JointView jv = ctx->jview();
         auto perJointColour = [&](auto kernelLaunch) {
             const int nc = ctx->nJoints > 0 ? kColourSlots : 0;
             for (int c = 0; c < nc; ++c) {
                 const int b0 = ctx->jointColourStart[c], b1 = ctx->jointColourStart[c + 1];
-                if (b1 <= b0) continue;
-                if (c == kOverflowColour) {
-                    for (int k = b0; k < b1; ++k) kernelLaunch(k, k + 1);
-                } else {
-                    kernelLaunch(b0, b1);
-                }
+                if (b1 > b0) kernelLaunch(b0, b1, c == kOverflowColour);
             }
         };
         // b2Island::Solve's start of the velocity phase (REF/Dynamics/b2Island.cpp:246-255):
         // contact warm start, then the joints' InitVelocityConstraints.
         auto startSolve = [&](bool contactWarm) {
             if (contactWarm) {
-                perColour(PH_WARM, [&](int b0, int b1) {
-                    hipLaunchKernelGGL(k_warm_start, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, ctx->cv, b0, b1);
+                perColour(PH_WARM, [&](int b0, int b1, bool serial) {
+                    if (serial)
+                        hipLaunchKernelGGL(k_warm_start_serial, dim3(1), dim3(1), 0, s, w, ctx->cv, b0, b1);
+                    else
+                        hipLaunchKernelGGL(k_warm_start, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, ctx->cv, b0, b1);
                 });
             }
-            perJointColour([&](int b0, int b1) {
-                hipLaunchKernelGGL(k_joint_init, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, jv, b0, b1, dtRatio,
-                                   ctx->warmStarting ? 1 : 0);
+            perJointColour([&](int b0, int b1, bool serial) {
+                if (serial)
+                    hipLaunchKernelGGL(k_joint_init_serial, dim3(1), dim3(1), 0, s, w, jv, b0, b1, dtRatio,
+                                       ctx->warmStarting ? 1 : 0);
+                else
+                    hipLaunchKernelGGL(k_joint_init, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, jv, b0, b1, dtRatio,
+                                       ctx->warmStarting ? 1 : 0);
             });
         };
         // Each iteration: joints, then contacts (REF/Dynamics/b2Island.cpp:261-270).
         auto solveVelocity = [&](int iters) {
             for (int it = 0; it < iters; ++it) {
-                perJointColour([&](int b0, int b1) {
-                    hipLaunchKernelGGL(k_joint_solve_velocity, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, jv, b0, b1,
-                                       dt);
+                perJointColour([&](int b0, int b1, bool serial) {
+                    if (serial)
+                        hipLaunchKernelGGL(k_joint_solve_velocity_serial, dim3(1), dim3(1), 0, s, w, jv, b0, b1, dt);
+                    else
+                        hipLaunchKernelGGL(k_joint_solve_velocity, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, jv, b0,
+                                           b1, dt);
                 });
-                perColour(PH_SOLVE_V, [&](int b0, int b1) {
+                perColour(PH_SOLVE_V, [&](int b0, int b1, bool serial) {
+                    if (serial) {
+                        hipLaunchKernelGGL(k_solve_velocity_serial, dim3(1), dim3(1), 0, s, w, ctx->cv, b0, b1);
+                        return;
+                    }
                     hipEvent_t e0, e1;
                     const int pk = profKernelEvents(ctx, &e0, &e1);
                     hipExtLaunchKernelGGL(k_solve_velocity, dim3(blocks(b1 - b0)), dim3(256), 0, s, e0, e1, 0, w,
@@ -3141,15 +3212,24 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
         LAUNCHCHK(ctx);
         // Each position iteration: contacts, then joints (REF/Dynamics/b2Island.cpp:310-328).
         for (int it = 0; it < posIters && (nCons > 0 || ctx->nJoints > 0); ++it) {
-            perColour(PH_SOLVE_X, [&](int b0, int b1) {
+            perColour(PH_SOLVE_X, [&](int b0, int b1, bool serial) {
+                if (serial) {
+                    hipLaunchKernelGGL(k_solve_position_serial, dim3(1), dim3(1), 0, s, w, ctx->cv, b0, b1, ctx->cnt,
+                                       it);
+                    return;
+                }
                 hipEvent_t e0, e1;
                 const int pk = profKernelEvents(ctx, &e0, &e1);
                 hipExtLaunchKernelGGL(k_solve_position, dim3(blocks(b1 - b0)), dim3(256), 0, s, e0, e1, 0, w, ctx->cv,
                                       b0, b1, ctx->cnt, it);
                 profKernel(ctx, PH_K_SOLVE_X, pk);
             });
-            perJointColour([&](int b0, int b1) {
-                hipLaunchKernelGGL(k_joint_solve_position, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, jv, b0, b1, it);
+            perJointColour([&](int b0, int b1, bool serial) {
+                if (serial)
+                    hipLaunchKernelGGL(k_joint_solve_position_serial, dim3(1), dim3(1), 0, s, w, jv, b0, b1, it);
+                else
+                    hipLaunchKernelGGL(k_joint_solve_position, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, jv, b0, b1,
+                                       it);
             });
         }
         pe = profStart(ctx);
@@ -3754,6 +3834,61 @@ int b2g_penetration(b2g_context* ctx, double* out) {
     out[0] = n ? h[0] / n : 0.0;
     memcpy(&out[1], &h[2], 8);
     out[2] = (double)n;
+    return B2G_OK;
+}
+
+int b2g_joint_drift(b2g_context* ctx, double* out) {
+    if (!ctx || !out) return B2G_ERR_ARG;
+    out[0] = out[1] = out[2] = 0.0;
+    const int nj = ctx->nJoints;
+    if (nj == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    float* d = nullptr;
+    HIPCHK(ctx, hipMalloc((void**)&d, sizeof(float) * nj));
+    hipLaunchKernelGGL(k_joint_drift, dim3(blocks(nj)), dim3(256), 0, s, ctx->view(), ctx->jview(), d);
+    std::vector<float> h(nj);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipMemcpyAsync(h.data(), d, sizeof(float) * nj, hipMemcpyDeviceToHost, s);
+    const hipError_t e3 = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    HIPCHK(ctx, e1);
+    HIPCHK(ctx, e2);
+    HIPCHK(ctx, e3);
+    double sum = 0.0, mx = 0.0;
+    int n = 0;
+    for (float x : h) {
+        if (x < 0.0f) continue;
+        sum += x;
+        mx = std::max(mx, (double)x);
+        ++n;
+    }
+    out[0] = n ? sum / n : 0.0;
+    out[1] = mx;
+    out[2] = (double)n;
+    return B2G_OK;
+}
+
+int b2g_energy(b2g_context* ctx, double* out) {
+    if (!ctx || !out) return B2G_ERR_ARG;
+    out[0] = out[1] = 0.0;
+    const int nb = ctx->nBodies;
+    if (nb == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    double2* d = nullptr;
+    HIPCHK(ctx, hipMalloc((void**)&d, sizeof(double2) * nb));
+    hipLaunchKernelGGL(k_body_energy, dim3(blocks(nb)), dim3(256), 0, s, ctx->view(), ctx->gravity, d);
+    std::vector<double2> h(nb);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipMemcpyAsync(h.data(), d, sizeof(double2) * nb, hipMemcpyDeviceToHost, s);
+    const hipError_t e3 = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    HIPCHK(ctx, e1);
+    HIPCHK(ctx, e2);
+    HIPCHK(ctx, e3);
+    for (const double2& x : h) {
+        out[0] += x.x;
+        out[1] += x.y;
+    }
     return B2G_OK;
 }
 

@@ -219,6 +219,24 @@ int b2pp_world_load_batch(void* h, int nWorlds, unsigned seed, int lo, int hi, i
     return r->world.Flush();  // one upload of every world
 }
 
+// nWorlds copies of a named scene (world w built from seed + w), each in its
+// own sub-world, joints included; one upload at the end. World w's bodies
+// are contiguous, in scene order.
+int b2pp_world_load_worlds(void* h, const char* kind, int nWorlds, int n, unsigned seed) {
+    Runner* r = (Runner*)h;
+    if (nWorlds <= 0) return B2G_ERR_ARG;
+    for (int w = 0; w < nWorlds; ++w) {
+        if (w > 0 || r->bodies.size() > 0) r->world.BeginSubWorld();
+        const b2scene::Scene s = b2scene::byName(kind, n, seed + (unsigned)w);
+        const int base = (int)r->bodies.size();
+        int rc = loadScene(r, s.bodies.data(), (int)s.bodies.size(), s.fixtures.data(), (int)s.fixtures.size(),
+                           false);
+        if (rc != 0) return rc;
+        if ((rc = loadJoints(r, base, s.joints.data(), (int)s.joints.size())) != 0) return rc;
+    }
+    return r->world.Flush();
+}
+
 int b2pp_world_step(void* h, float dt, int vel, int pos, int nsteps) {
     Runner* r = (Runner*)h;
     for (int i = 0; i < nsteps; ++i) {

@@ -167,10 +167,10 @@ struct Uniform {
 // its lower edge at y = 5, in a U container of half-width width/2 + 5 and
 // wall height height + 50. `scale` shrinks the whole layout for the small
 // per-world piles of the batched scene.
-inline Scene pile(int n, uint32_t seed, float width = 300.0f, float height = 600.0f) {
+inline Scene pile(int n, uint32_t seed, float width = 300.0f, float height = 600.0f, float base = 5.0f) {
     Scene s;
     const float half = 0.5f * width + 5.0f;
-    const float wallTop = height + 50.0f;
+    const float wallTop = base - 5.0f + height + 50.0f;
     addBody(s, defaultBody(0, 0.0f, 0.0f));
     addFixture(s, edgeFixture(-half, 0.0f, half, 0.0f));
     addFixture(s, edgeFixture(-half, 0.0f, -half, wallTop));
@@ -184,7 +184,7 @@ inline Scene pile(int n, uint32_t seed, float width = 300.0f, float height = 600
     for (int i = 0; i < n; ++i) {
         const int c = i % cols, r = i / cols;
         const float x = (float)(-0.5 * width + (c + 0.5) * (double)width / cols);
-        const float y = (float)(5.0 + (r + 0.5) * (double)height / rows);
+        const float y = (float)((double)base + (r + 0.5) * (double)height / rows);
         addBody(s, defaultBody(2, x, y));
         FixtureDef f;
         if (i & 1) {
@@ -284,9 +284,59 @@ inline Scene chains(int n, int links = 32) {
     return s;
 }
 
+// Stress world: the random pile of n bodies (seed) dropped onto a mat of
+// `nChains` chains of `links` boxes 0.5 x 0.1 m (density 1, friction 0.6)
+// joined end to end by revolute joints that keep neighbours from colliding.
+// The mat lies on the container floor in rows 0.25 m apart (at most 18
+// chains of 16 m per row across the 300 m floor); the pile's drop grid
+// starts 5 m above the mat. The links are bodies in addition to the pile's n.
+inline Scene stress(int n, uint32_t seed, int nChains, int links = 32) {
+    const float len = 0.5f * (float)links;
+    const int perRow = std::max(1, (int)(300.0f / (len + 0.5f)));
+    const int rows = (nChains + perRow - 1) / perRow;
+    const float matTop = 0.3f + 0.25f * (float)rows;
+    Scene s = pile(n, seed, 300.0f, 600.0f, matTop + 5.0f);
+    for (int c = 0; c < nChains; ++c) {
+        const int r = c / perRow, k = c % perRow;
+        const float x0 = -150.0f + 0.25f + (float)k * (len + 0.5f);
+        const float y = 0.3f + 0.25f * (float)r;
+        int prev = -1;
+        for (int i = 0; i < links; ++i) {
+            addBody(s, defaultBody(2, x0 + 0.25f + 0.5f * (float)i, y));
+            addFixture(s, boxFixture(0.25f, 0.05f, 1.0f, 0.6f));
+            const int cur = (int)s.bodies.size() - 1;
+            if (prev >= 0) s.joints.push_back(hinge(s, prev, cur, x0 + 0.5f * (float)i, y));
+            prev = cur;
+        }
+    }
+    return s;
+}
+
+// A plank (a dynamic box 12 x 0.5 m) on the ground carrying n small circles
+// (r = 0.05 m) in a row: the plank takes part in n + 1 contacts, more than
+// the 64 colours of a body's mask when n > 63.
+inline Scene plank(int n) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-20.0f, 0.0f, 20.0f, 0.0f));
+    addBody(s, defaultBody(2, 0.0f, 0.25f));
+    addFixture(s, boxFixture(6.0f, 0.25f, 1.0f, 0.6f));
+    const float step = 11.8f / (float)std::max(n, 1);
+    for (int i = 0; i < n; ++i) {
+        addBody(s, defaultBody(2, -5.9f + step * ((float)i + 0.5f), 0.55f));
+        FixtureDef f = defaultFixture(kCircle);
+        f.radius = 0.05f;
+        f.density = 1.0f; f.friction = 0.6f;
+        addFixture(s, f);
+    }
+    return s;
+}
+
 // Scene by name: "hello", "pyramid" (n = rows, 0 -> 20), "pile" (n bodies,
-// seed), "world" (one batched-scene world of n bodies, seed). Unknown names
-// give an empty scene.
+// seed), "world" (one batched-scene world of n bodies, seed), "pendulums"
+// (n pendulums), "chains" (n chains of `seed` links), "stress" (pile of n
+// bodies, seed, on a mat of n / 50 chains of 32 links), "plank" (n circles
+// on a plank). Unknown names give an empty scene.
 inline Scene byName(const char* kind, int n, uint32_t seed) {
     const std::string k(kind ? kind : "");
     if (k == "hello") return hello();
@@ -295,6 +345,8 @@ inline Scene byName(const char* kind, int n, uint32_t seed) {
     if (k == "world") return batchedWorld(n, seed);
     if (k == "pendulums") return pendulums(n > 0 ? n : 16);
     if (k == "chains") return chains(n > 0 ? n : 4, seed > 0 ? (int)seed : 32);
+    if (k == "stress") return stress(n, seed, n / 50);
+    if (k == "plank") return plank(n > 0 ? n : 100);
     return Scene();
 }
 

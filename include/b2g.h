@@ -247,6 +247,15 @@ int b2g_get_fat_aabbs(b2g_context* ctx, b2g_aabb* out, int cap);
 /* Mean and max penetration over touching manifold points (from
  * b2WorldManifold separations), and the point count: out[0..2]. */
 int b2g_penetration(b2g_context* ctx, double* out);
+/* Joint drift: mean and max position error |cB + rB - cA - rA| over the
+ * joints (the error b2RevoluteJoint::SolvePositionConstraints measures,
+ * Box2D/Dynamics/Joints/b2RevoluteJoint.cpp:336-343), and the joint count.
+ * out[3], FP64 sums on the host; no counterpart in the reference. */
+int b2g_joint_drift(b2g_context* ctx, double* out);
+/* Kinetic (0.5 m v.v + 0.5 I w^2) and gravitational potential (-m g.c)
+ * energy summed over the bodies in FP64; out[2]. No counterpart in the
+ * reference. */
+int b2g_energy(b2g_context* ctx, double* out);
 
 /* Per-phase GPU time of the steps run while profiling is on, measured with
  * HIP events on the context's stream around every launch of the phase. */

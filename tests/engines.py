@@ -103,6 +103,23 @@ class World:
             raise RuntimeError(f"batch load failed rc={rc} {w.error()}")
         return w, sizes
 
+    @classmethod
+    def worlds(cls, eng: "Engine", kind: str, n_worlds: int, n: int, seed: int, sleep: bool = False):
+        """n_worlds copies of a named scene (world w from seed + w) in
+        sub-worlds of one GPU world, joints included."""
+        w = cls(eng, None, 0, 0, sleep, True, (0.0, -10.0))
+        rc = eng.lib.b2pp_world_load_worlds(w.h, kind.encode(), int(n_worlds), int(n), ctypes.c_uint(seed))
+        if rc != 0:
+            raise RuntimeError(f"worlds load failed rc={rc} {w.error()}")
+        return w
+
+    def joints(self) -> np.ndarray:
+        """Per joint: impulse x, y, z and motor impulse."""
+        nj = self.e.fn("world_joint_count")(self.h)
+        out = np.zeros(4 * nj, np.float32)
+        self.e.fn("world_get_joints")(self.h, ptr(out))
+        return out.reshape(-1, 4)
+
     def error(self) -> str:
         if self.e.name == "gpu":
             return self.e.fn("world_last_error")(self.h).decode()

@@ -156,3 +156,99 @@ def test_reference_mode_rejects_joints(gpu):
     g.set_mode(0)
     g.step(1)
     g.close()
+
+
+def _b2g(gpu):
+    lib = ctypes.CDLL(E.GPU_B2G_LIB)
+    return lib
+
+
+def test_energy_and_drift_closed_forms(gpu):
+    """b2g_energy at load against the pendulums' closed form (boxes 0.5 x 0.5
+    m, density 1: m = 0.25, I = m (0.5^2 + 0.5^2) / 12; v0 = 0.5 + 0.25 (i mod
+    13); height 20 - length), and b2g_joint_drift against the hinge gaps
+    computed from the body poses."""
+    lib = _b2g(gpu)
+    g = gpu.world("pendulums", 24, 0, sleep=False)
+    ctx = g.device_context()
+    out = np.zeros(3, np.float64)
+    assert lib.b2g_energy(ctx, E.ptr(out)) == 0
+    m, inertia = 0.25, 0.25 * 0.5 / 12.0
+    i = np.arange(24)
+    ke = np.sum(0.5 * m * np.float32(0.5 + 0.25 * (i % 13)).astype(np.float64) ** 2)
+    length = np.float32(1.0 + 0.05 * (i % 20))
+    pe = np.sum(m * 10.0 * np.float32(20.0 - length).astype(np.float64))
+    assert abs(out[0] - ke) <= 1e-6 * ke and abs(out[1] - pe) <= 1e-6 * pe, (out, ke, pe)
+    assert inertia > 0
+    g.step(120)
+    assert lib.b2g_joint_drift(ctx, E.ptr(out)) == 0
+    b = g.bodies()[1:].astype(np.float64)
+    x = 5.0 * i
+    ax = b[:, E.PX] - b[:, E.QS] * length
+    ay = b[:, E.PY] + b[:, E.QC] * length
+    gaps = np.hypot(ax - x, ay - 20.0)
+    assert out[2] == 24
+    assert abs(out[0] - gaps.mean()) < 2e-5 and abs(out[1] - gaps.max()) < 2e-5, (out, gaps.mean(), gaps.max())
+    g.close()
+
+
+def _stress_gaps(b, base, n, links):
+    """Hinge gaps of the stress scene's chains, whose links follow the
+    container and the n pile bodies from body `base` on, end to end."""
+    nc = n // 50
+    idx = base + 1 + n + np.arange(nc * links).reshape(nc, links)
+    a, c = idx[:, :-1].ravel(), idx[:, 1:].ravel()
+    b = b.astype(np.float64)
+    pa = np.stack([b[a, E.PX] + b[a, E.QC] * 0.25, b[a, E.PY] + b[a, E.QS] * 0.25], 1)
+    pb = np.stack([b[c, E.PX] - b[c, E.QC] * 0.25, b[c, E.PY] - b[c, E.QS] * 0.25], 1)
+    return np.hypot(*(pa - pb).T)
+
+
+def test_stress_worlds_small(gpu, orc):
+    """Four stress worlds (a pile of 1500 bodies dropped on a mat of 30
+    chains of 32 links each) in sub-worlds of one context: no contact joins
+    two worlds or two jointed links, b2g_joint_drift agrees with the gaps
+    computed from the poses, and world 0's hinge gaps stay within a small
+    factor of the oracle's on the same world (fast bodies tear links in
+    both; the solver order differs)."""
+    lib = _b2g(gpu)
+    n, worlds, links, seed = 1500, 4, 32, 11
+    g = E.World.worlds(gpu, "stress", worlds, n, seed, sleep=False)
+    o = orc.world("stress", n, seed, sleep=False)
+    per_world = 1 + n + (n // 50) * links
+    nb = g.e.fn("world_body_count")(g.h)
+    assert nb == worlds * per_world
+    assert g.e.fn("world_joint_count")(g.h) == worlds * (n // 50) * (links - 1)
+    # fixture -> body: the container's 3 edges on body 0 of each world, then one per body
+    fb = []
+    for w in range(worlds):
+        base = w * per_world
+        fb += [base] * 3 + list(range(base + 1, base + per_world))
+    fb = np.array(fb)
+    ctx = g.device_context()
+    out = np.zeros(3, np.float64)
+    for s in range(3):
+        g.step(100)
+        o.step(100)
+        meta, _ = g.contacts()
+        ba, bb = fb[meta[:, 0]], fb[meta[:, 1]]
+        assert np.all(ba // per_world == bb // per_world), "a contact joins two worlds"
+        local_a, local_b = ba % per_world, bb % per_world
+        lo, hi = np.minimum(local_a, local_b), np.maximum(local_a, local_b)
+        chain_lo = lo - (1 + n)
+        jointed = (chain_lo >= 0) & (hi == lo + 1) & (chain_lo % links != links - 1)
+        assert not np.any(jointed), "contact between jointed links"
+    gb = g.bodies()
+    assert np.all(np.isfinite(gb))
+    gaps = np.concatenate([_stress_gaps(gb, w * per_world, n, links) for w in range(worlds)])
+    assert lib.b2g_joint_drift(ctx, E.ptr(out)) == 0
+    assert out[2] == len(gaps)
+    assert abs(out[0] - gaps.mean()) < 1e-5 and abs(out[1] - gaps.max()) < 1e-5, (out, gaps.mean(), gaps.max())
+    g0 = _stress_gaps(gb, 0, n, links)
+    o0 = _stress_gaps(o.bodies(), 0, n, links)
+    assert g0.mean() <= 2.0 * o0.mean() + 0.002, (g0.mean(), o0.mean())
+    assert np.percentile(g0, 99) <= 3.0 * np.percentile(o0, 99) + 0.01, (np.percentile(g0, 99), np.percentile(o0, 99))
+    pen = g.penetration()
+    assert pen[2] > 0 and pen[0] < 0.1
+    g.close()
+    o.close()

@@ -156,3 +156,34 @@ def test_coloured_step_is_deterministic(gpu, seed):
     assert np.array_equal(ma, mb)                      # pairs, flags and colours
     assert np.array_equal(E.manifold_bits(na), E.manifold_bits(nb))
     a.close(); b.close()
+
+
+def test_overflow_colour_plank(gpu, orc):
+    """A plank carrying 100 circles takes part in 101 contacts, more than a
+    body's 64 colours: the constraints past the 64th go to the overflow
+    colour, which one lane solves in order. The colouring invariant holds,
+    two copies step to the same bits, and the circles come to rest on the
+    plank where the oracle's do (within 5 mm; the solver order differs)."""
+    import ctypes
+    lib = E.b2g_lib()
+    a = gpu.world("plank", 100, 0, sleep=False)
+    b = gpu.world("plank", 100, 0, sleep=False)
+    o = orc.world("plank", 100, 0, sleep=False)
+    ca = a.device_context()
+    conf, mask = ctypes.c_int(), ctypes.c_int()
+    for s in range(180):
+        a.step(1)
+        b.step(1)
+        o.step(1)
+        if s % 30 == 29:
+            E.b2g_check(lib, ca, lib.b2g_check_colouring(ca, ctypes.byref(conf), ctypes.byref(mask)))
+            assert conf.value == 0 and mask.value == 0, (s, conf.value, mask.value)
+    st = a.stats()
+    assert st["colours"] == 65, st["colours"]
+    ga, gb, ob = a.bodies(), b.bodies(), o.bodies()
+    assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32))
+    assert np.all(np.isfinite(ga))
+    assert np.abs(ga[:, E.CY] - ob[:, E.CY]).max() < 0.005
+    assert np.abs(ga[:, E.CX] - ob[:, E.CX]).max() < 0.005
+    assert np.abs(ga[2:, E.VY]).max() < 0.05
+    a.close(); b.close(); o.close()

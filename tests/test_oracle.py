@@ -26,6 +26,8 @@ F = ctypes.c_float
     ("pendulums", 24, 0, 400, True, 8, 3),     # revolute joints: free, limited, motorised
     ("chains", 3, 12, 400, False, 8, 3),       # 3 chains of 12 links hitting the ground
     ("chains", 2, 32, 250, True, 4, 2),
+    ("stress", 1000, 3, 200, False, 8, 3),     # pile dropped on a mat of 20 chains of 32 links
+    ("plank", 100, 0, 200, True, 8, 3),        # one body in 101 contacts
 ])
 def test_world_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, vel, pos):
     a = ref.world(kind, n, seed, sleep=sleep)
