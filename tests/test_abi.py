@@ -36,7 +36,7 @@ def test_struct_layouts_match_the_library():
     n = lib.b2g_abi_sizes(None, 0)
     sizes = np.zeros(n, np.int32)
     lib.b2g_abi_sizes(E.ptr(sizes), n)
-    shape, manifold, body, fixture, world, params, stats, profile, mlp, aabb, xf = sizes[:11]
+    shape, manifold, body, fixture, world, params, stats, profile, mlp, aabb, xf, revolute = sizes[:12]
     assert E.SHAPE.itemsize == shape
     assert E.MANIFOLD.itemsize == manifold
     assert E.STATS.itemsize == stats
@@ -46,3 +46,4 @@ def test_struct_layouts_match_the_library():
     assert ctypes.sizeof(bench.StepStats) == stats
     assert ctypes.sizeof(bench.PhaseProfile) == profile
     assert bench.MLP_PARAM_FLOATS * 4 == mlp
+    assert revolute == 56  # 2 body indices, 9 floats, 3 switches
