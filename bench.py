@@ -47,6 +47,13 @@ HBM_PEAK_GBS = 8000.0
 # for a plain 256 MiB copy (x2.000 and x1.000); the summary with the commands
 # is under profiles/.
 DEFAULT_TRAFFIC_BYTES = 4.929e6
+DT = 1.0 / 60.0
+# b2g_schedule: island labelling method, collide order, workgroup sizes of
+# the solver, narrowphase and broad-phase query kernels
+SCHEDULE = np.dtype([("islands", "<i4"), ("collideOrder", "<i4"), ("solverBlock", "<i4"), ("collideBlock", "<i4"),
+                     ("queryBlock", "<i4")])
+# the schedule the benchmark's worlds run with (set by the autotuner)
+_schedule = None
 
 
 class PhaseProfile(ctypes.Structure):
@@ -62,10 +69,21 @@ class StepStats(ctypes.Structure):
         ("fallback", ctypes.c_int32), ("residual", ctypes.c_float)]
 
 
+def sched_dict(sc):
+    return None if sc is None else {k: int(sc[0][k]) for k in SCHEDULE.names}
+
+
 def progress(msg):
     """Progress line on stderr (the JSON result is the only stdout line)."""
     sys.stderr.write(f"[bench {time.strftime('%H:%M:%S')}] {msg}\n")
     sys.stderr.flush()
+
+
+def apply_schedule(lib, h):
+    if _schedule is not None:
+        rc = lib.b2pp_world_set_schedule(h, _schedule.ctypes.data_as(ctypes.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"schedule rejected: {lib.b2pp_world_last_error(h).decode()}")
 
 
 def make_world(lib, n, seed):
@@ -74,12 +92,25 @@ def make_world(lib, n, seed):
     rc = lib.b2pp_world_load_scene(h, b"pile", n, ctypes.c_uint(seed))
     if rc != 0:
         raise RuntimeError(f"scene load failed: {lib.b2pp_world_last_error(h)}")
+    apply_schedule(lib, h)
     lib.b2pp_world_sync(h)
     return h
 
 
+def autotune(lib, h, args, at):
+    """Schedule chosen by the autotuner on world h after `at` steps."""
+    out = np.zeros(1, SCHEDULE)
+    step(lib, h, at, args.vel, args.pos)
+    progress(f"autotuning at step {at} ({15 * args.tune_rounds} steps)")
+    rc = lib.b2pp_world_autotune(h, ctypes.c_float(DT), args.vel, args.pos, args.tune_rounds,
+                                 out.ctypes.data_as(ctypes.c_void_p))
+    if rc != 0:
+        raise RuntimeError(f"autotune failed rc={rc}: {lib.b2pp_world_last_error(h).decode()}")
+    return out
+
+
 def step(lib, h, k, vel, pos):
-    rc = lib.b2pp_world_step(h, ctypes.c_float(1.0 / 60.0), vel, pos, k)
+    rc = lib.b2pp_world_step(h, ctypes.c_float(DT), vel, pos, k)
     if rc != 0:
         raise RuntimeError(f"step failed rc={rc}: {lib.b2pp_world_last_error(h).decode()}")
 
@@ -257,18 +288,34 @@ def stress(lib, args, params):
     variants = [("plain", args.vel, args.pos, False)]
     if params is not None:
         variants.append(("learned", 4, 2, True))
+    global _schedule
+    pile_schedule = _schedule
+
+    def load():
+        h = ctypes.c_void_p(lib.b2pp_world_create(0.0, -10.0))
+        lib.b2pp_world_set_flags(h, 0, 1, 0, 0)
+        rc = lib.b2pp_world_load_worlds(h, b"stress", args.stress_worlds, args.stress_bodies,
+                                        ctypes.c_uint(args.stress_seed))
+        if rc != 0:
+            raise RuntimeError(f"stress load failed rc={rc}: {lib.b2pp_world_last_error(h).decode()}")
+        return h
+
+    stress_schedule = None
+    if args.tune_rounds > 0:
+        progress("stress: loading to autotune")
+        h = load()
+        stress_schedule = autotune(lib, h, args, 150)
+        lib.b2pp_world_destroy(h)
     out = {"workload": f"{args.stress_worlds} worlds x (random pile of {args.stress_bodies} bodies + "
                        f"{args.stress_bodies // 50} chains of 32 links), seed {args.stress_seed}, "
                        f"steps 1-{args.stress_steps}, sleep and CCD off, one context"}
     for name, vel, pos, learned in variants:
         progress(f"stress: loading ({name})")
-        h = ctypes.c_void_p(lib.b2pp_world_create(0.0, -10.0))
-        lib.b2pp_world_set_flags(h, 0, 1, 0, 0)
         t0 = time.perf_counter()
-        rc = lib.b2pp_world_load_worlds(h, b"stress", args.stress_worlds, args.stress_bodies,
-                                        ctypes.c_uint(args.stress_seed))
-        if rc != 0:
-            raise RuntimeError(f"stress load failed rc={rc}: {lib.b2pp_world_last_error(h).decode()}")
+        h = load()
+        _schedule = stress_schedule
+        apply_schedule(lib, h)
+        _schedule = pile_schedule
         ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
         if learned:
             _check(b2g, ctx, b2g.b2g_mlp_set_params(ctx, params))
@@ -297,6 +344,7 @@ def stress(lib, args, params):
                      "mean_penetration_m": float(pen[0]), "max_penetration_m": float(pen[1]),
                      "joint_drift_mean_m": float(drift[0]), "joint_drift_max_m": float(drift[1]),
                      "kinetic_energy_J": float(energy[0]), "potential_energy_J": float(energy[1])}
+    out["schedule"] = sched_dict(stress_schedule)
     return out
 
 
@@ -307,16 +355,32 @@ def batched_worlds(lib, args):
     Step; body-steps/s over args.batch_steps steps, state resident on the
     device before timing. CPU leg: the reference b2World::Step, one world per
     thread over the host's cores, on a bounded sample of the worlds."""
-    h = ctypes.c_void_p(lib.b2pp_world_create(0.0, -10.0))
-    lib.b2pp_world_set_flags(h, 0, 1, 0, 0)
-    sizes = np.zeros(args.batch_worlds, np.int32)
+    global _schedule
+    pile_schedule = _schedule
+
+    def load():
+        h = ctypes.c_void_p(lib.b2pp_world_create(0.0, -10.0))
+        lib.b2pp_world_set_flags(h, 0, 1, 0, 0)
+        sizes = np.zeros(args.batch_worlds, np.int32)
+        rc = lib.b2pp_world_load_batch(h, args.batch_worlds, ctypes.c_uint(args.batch_seed), 16, 1024,
+                                       sizes.ctypes.data_as(ctypes.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"batch load failed rc={rc}: {lib.b2pp_world_last_error(h).decode()}")
+        lib.b2pp_world_sync(h)
+        return h, sizes
+
+    batch_schedule = None
+    if args.tune_rounds > 0:
+        progress(f"batched worlds: loading {args.batch_worlds} worlds to autotune")
+        h, _ = load()
+        batch_schedule = autotune(lib, h, args, 100)
+        lib.b2pp_world_destroy(h)
     progress(f"batched worlds: loading {args.batch_worlds} worlds")
     t0 = time.perf_counter()
-    rc = lib.b2pp_world_load_batch(h, args.batch_worlds, ctypes.c_uint(args.batch_seed), 16, 1024,
-                                   sizes.ctypes.data_as(ctypes.c_void_p))
-    if rc != 0:
-        raise RuntimeError(f"batch load failed rc={rc}: {lib.b2pp_world_last_error(h).decode()}")
-    lib.b2pp_world_sync(h)
+    h, sizes = load()
+    _schedule = batch_schedule
+    apply_schedule(lib, h)
+    _schedule = pile_schedule
     load_s = time.perf_counter() - t0
     bodies = int(sizes.sum())
     progress(f"batched worlds: {bodies} bodies loaded in {load_s:.1f} s; stepping")
@@ -331,7 +395,7 @@ def batched_worlds(lib, args):
     out = {"value": bodies * args.batch_steps / wall, "unit": "body-steps/s", "worlds": args.batch_worlds,
            "bodies": bodies, "steps": args.batch_steps, "ms_per_step": 1e3 * wall / args.batch_steps,
            "load_seconds": round(load_s, 2), "contacts": st.contacts, "colours": st.colours,
-           "mean_penetration_m": float(pen[0]),
+           "mean_penetration_m": float(pen[0]), "schedule": sched_dict(batch_schedule),
            "workload": f"{args.batch_worlds} worlds, bodies log-uniform in [16, 1024] (seed {args.batch_seed}), "
                        f"{args.vel}/{args.pos} iterations, sleep and CCD off, one context"}
     if not args.no_cpu:
@@ -400,6 +464,9 @@ def main():
     ap.add_argument("--stress-bodies", type=int, default=125000)
     ap.add_argument("--stress-steps", type=int, default=600)
     ap.add_argument("--stress-seed", type=int, default=1)
+    ap.add_argument("--tune-rounds", type=int, default=3,
+                    help="autotuner rounds on the warm-up copy (0: default schedule)")
+    ap.add_argument("--tune-at", type=int, default=500, help="step of the warm-up copy at which to autotune")
     ap.add_argument("--log-seed", type=int, default=2)
     ap.add_argument("--eval-seed", type=int, default=1001)
     ap.add_argument("--log-capacity", type=int, default=10_000_000)
@@ -423,12 +490,20 @@ def main():
         return 2
 
     lib = box2dpp_amd.host()
-    # warm-up on a separate copy of the scene
-    if args.warmup > 0:
+    # warm-up on a separate copy of the scene; the autotuner then picks the
+    # step schedule on that copy at a later state of the pile (no schedule
+    # changes results, only speed)
+    global _schedule
+    pile_schedule = None
+    if args.warmup > 0 or args.tune_rounds > 0:
         hw = make_world(lib, args.bodies, args.seed)
-        step(lib, hw, args.warmup, args.vel, args.pos)
+        if args.tune_rounds > 0:
+            pile_schedule = autotune(lib, hw, args, max(args.warmup, args.tune_at))
+        else:
+            step(lib, hw, args.warmup, args.vel, args.pos)
         lib.b2pp_world_sync(hw)
         lib.b2pp_world_destroy(hw)
+    _schedule = pile_schedule
 
     h = make_world(lib, args.bodies, args.seed)
     lib.b2pp_world_sync(h)
@@ -505,6 +580,7 @@ def main():
                      "launches_per_step": sv_launches / max(args.profile_steps, 1),
                      "stream_us_per_launch": 1e3 * ph_ms / max(ph_launches, 1)},
         "cpu_baseline": cpu,
+        "schedule": sched_dict(_schedule),
         "learned_warm_start": learned,
         "batched_worlds": batched,
         "stress": stressed,

@@ -41,6 +41,7 @@
 #include <limits>
 #include <random>
 #include <algorithm>
+#include <chrono>
 
 using namespace b2g;
 
@@ -67,7 +68,7 @@ struct Counters {
     int points;
     int nEdges;
     int nIslands;
-    int islandChanged;     // set by a hooking pass that lowered a label
+    int islandChanged;     // set by a propagation round that lowered a label
     int nFresh;            // fresh contact points of the step (learned warm start, log, trace)
     int residualPoints;
     float residualSum;     // sum of squared complementarity residuals (k_residual)
@@ -197,9 +198,27 @@ __device__ inline bool filterPass(const WorldView& w, int fa, int fb) {
 // Collide (b2ContactManager::Collide + b2Contact::Update,
 // REF/Dynamics/b2ContactManager.cpp:107-172, REF/Dynamics/Contacts/b2Contact.cpp:171-251)
 // ====================================================================
-__global__ void k_collide(WorldView w, Counters* cnt) {
+// Shape-pair class of a contact (registry order puts fixture A's type
+// first: circle-circle, polygon-circle, polygon-polygon, edge-circle,
+// edge-polygon); below 16.
+__device__ inline int collideClass(const WorldView& w, int c) {
+    return (w.shape[w.cfA[c]].type << 2) | w.shape[w.cfB[c]].type;
+}
+
+// Collide order: contacts grouped by shape-pair class (a stable radix sort
+// of these keys), so that a wavefront runs one narrowphase routine. The
+// order does not change results: contacts are collided independently.
+__global__ void k_collide_keys(WorldView w, uint32_t* keys, int* vals) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= w.nContacts) return;
+    keys[c] = (uint32_t)(collideClass(w, c) & 15);
+    vals[c] = c;
+}
+
+__global__ void k_collide(WorldView w, Counters* cnt, const int* order) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w.nContacts) return;
+    const int c = order ? order[i] : i;
     int flags = w.cflags[c];
     const int fA = w.cfA[c], fB = w.cfB[c];
     const int bA = w.fbody[fA], bB = w.fbody[fB];
@@ -300,15 +319,23 @@ __device__ inline bool islandContact(const WorldView& w, int c) {
 // b2World::Solve grows islands by depth-first search from every awake,
 // active, non-static body through island contacts, without crossing static
 // bodies, and wakes every body it reaches (REF/Dynamics/b2World.cpp:408-519).
-// Here the same components come from minimum-label propagation with pointer
-// jumping: every body starts labelled with its own index; a hooking pass
-// lowers, with atomicMin only, the label of the larger of the two labels of
-// every island contact's bodies to the smaller one, and a jumping pass points
-// every body at the end of its label chain. Threads of one pass share data
-// only through atomicMin; each pass reads what the previous launch wrote, so
-// the result -- each body labelled with the smallest body index of its
-// component -- is the same for any thread schedule. Passes repeat until a
-// hooking pass changes nothing.
+// Here the same components come from one round of minimum-label
+// propagation (each island contact lowers the larger label to the smaller
+// with atomicMin, then pointer jumping), which merges most of each island
+// cheaply, and a lock-free union-find pass over the island contacts and
+// joints that finishes every island, then one more pointer-jumping pass:
+//  * every body starts as its own root, and a parent index is never larger
+//    than the body's own, so parent chains end;
+//  * a union finds both roots and hooks the larger root under the smaller
+//    with a compare-and-swap that succeeds only while it is still a root;
+//    a failed swap returns the root's new parent, from which the search
+//    goes on. Each retry lowers the sum of the two indices, so it ends;
+//  * plain loads of parents may be stale across XCDs, but every value a
+//    parent ever held is an ancestor, so a stale read only shortens the
+//    walk, and the swap itself reads the current value. Path halving stores
+//    such an ancestor, so it keeps every invariant above.
+// Roots only ever move under smaller indices, so each component ends up
+// labelled with its smallest body index, for any thread schedule.
 __global__ void k_island_init(WorldView w) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= w.nBodies) return;
@@ -317,6 +344,34 @@ __global__ void k_island_init(WorldView w) {
     w.islandSleep[b] = 0;
 }
 
+__device__ inline int islandFind(int* L, int x) {
+    for (;;) {
+        const int p = L[x];
+        if (p == x) return x;
+        const int g = L[p];
+        if (g != p) L[x] = g;  // path halving: g is an ancestor of x
+        x = p;
+    }
+}
+
+__device__ inline void islandUnion(int* L, int a, int b) {
+    for (;;) {
+        a = islandFind(L, a);
+        b = islandFind(L, b);
+        if (a == b) return;
+        if (a < b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        const int old = atomicCAS(&L[a], a, b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+// A propagation round: each island contact lowers the larger of its
+// bodies' labels to the smaller (atomicMin only).
 __global__ void k_island_hook(WorldView w, int* changed) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= w.nContacts || !islandContact(w, c)) return;
@@ -326,7 +381,16 @@ __global__ void k_island_hook(WorldView w, int* changed) {
     const int la = w.islandOf[bA], lb = w.islandOf[bB];
     if (la == lb) return;
     atomicMin(&w.islandOf[la > lb ? la : lb], la < lb ? la : lb);
-    *changed = 1;
+    if (changed) *changed = 1;
+}
+
+__global__ void k_island_union(WorldView w) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts || !islandContact(w, c)) return;
+    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
+    if (w.btype[bA] == BT_STATIC || w.btype[bB] == BT_STATIC) return;
+    if (!(w.bflags[bA] & BF_ACTIVE) || !(w.bflags[bB] & BF_ACTIVE)) return;
+    islandUnion(w.islandOf, bA, bB);
 }
 
 // Chains only lead to smaller indices, so a chain ends within nBodies hops.
@@ -1336,18 +1400,15 @@ __global__ void k_joint_solve_position_serial(WorldView w, JointView jv, int beg
     for (int k = begin; k < end; ++k) jointSolvePositionOne(w, jv, jv.order[k], it);
 }
 
-// Island labelling through joints whose bodies are both non-static and active.
-__global__ void k_island_hook_joints(WorldView w, JointView jv, int* changed) {
+// Island union through joints whose bodies are both non-static and active.
+__global__ void k_island_union_joints(WorldView w, JointView jv) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= jv.n) return;
     const int2 b = jv.bodies[j];
     if (jv.flags[j] & JF_REMOVED) return;
     if (w.btype[b.x] == BT_STATIC || w.btype[b.y] == BT_STATIC) return;
     if (!(w.bflags[b.x] & BF_ACTIVE) || !(w.bflags[b.y] & BF_ACTIVE)) return;
-    const int la = w.islandOf[b.x], lb = w.islandOf[b.y];
-    if (la == lb) return;
-    atomicMin(&w.islandOf[la > lb ? la : lb], la < lb ? la : lb);
-    *changed = 1;
+    islandUnion(w.islandOf, b.x, b.y);
 }
 
 // Position error of each joint, |cB + rB - cA - rA| as
@@ -2112,6 +2173,7 @@ inline int blocks(int n, int bs = 256) { return (n + bs - 1) / bs; }
 // ====================================================================
 struct b2g_context {
     int device = 0;
+    b2g_schedule sched{0, 0, 256, 256, 256};
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -2529,8 +2591,9 @@ int runBroadphase(b2g_context* ctx) {
         hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nNewPairs, 1, 0);
         hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->overflow, 1, 0);
         PairSink sink{ctx->newKeys, ctx->capContacts};
-        hipLaunchKernelGGL(k_pairs_query, dim3(blocks(nf)), dim3(256), 0, s, ctx->fat, ctx->moved, nf, g, ctx->view(),
-                           true, ctx->hashTable, ctx->hashMask, sink, ctx->cnt);
+        const int qb = ctx->sched.queryBlock;
+        hipLaunchKernelGGL(k_pairs_query, dim3((nf + qb - 1) / qb), dim3(qb), 0, s, ctx->fat, ctx->moved, nf, g,
+                           ctx->view(), true, ctx->hashTable, ctx->hashMask, sink, ctx->cnt);
         hipLaunchKernelGGL(k_pairs_wide, dim3(blocks(nf)), dim3(256), 0, s, ctx->fat, ctx->moved, nf, g, ctx->view(),
                            true, ctx->hashTable, ctx->hashMask, sink, ctx->cnt);
         LAUNCHCHK(ctx);
@@ -2896,6 +2959,46 @@ int traceResiduals(b2g_context* ctx, const WorldView& w, int nCons, int nFresh, 
     return B2G_OK;
 }
 
+// Island labels (each body labelled with the smallest body index of its
+// island) by the schedule's method; all three give the same labels.
+//  0: propagation rounds (two per host check of the change flag) until
+//     none lowers a label; best for a few large islands;
+//  1: one propagation round, then the union-find pass;
+//  2: the union-find pass alone; best for many small islands.
+// Joints join islands by the union-find pass in every method.
+int labelIslands(b2g_context* ctx, const WorldView& w) {
+    hipStream_t s = ctx->stream;
+    const int nb = ctx->nBodies, nc = ctx->nContacts;
+    const int method = ctx->sched.islands;
+    if (nc > 0 && method == 0) {
+        for (int pass = 0;; ++pass) {
+            hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->islandChanged, 1, 0);
+            for (int r = 0; r < 2; ++r) {
+                hipLaunchKernelGGL(k_island_hook, dim3(blocks(nc)), dim3(256), 0, s, w, &ctx->cnt->islandChanged);
+                hipLaunchKernelGGL(k_island_jump, dim3(blocks(nb)), dim3(256), 0, s, w);
+            }
+            LAUNCHCHK(ctx);
+            HIPCHK(ctx, hipMemcpyAsync(&ctx->hcnt->islandChanged, &ctx->cnt->islandChanged, sizeof(int),
+                                       hipMemcpyDeviceToHost, s));
+            HIPCHK(ctx, hipStreamSynchronize(s));
+            if (!ctx->hcnt->islandChanged) break;
+            // every round that changes something lowers a label, so this is a safeguard only
+            if (pass >= 1000000) return fail(ctx, B2G_ERR_STATE, "island labelling did not converge");
+        }
+    } else if (nc > 0) {
+        if (method == 1) {
+            hipLaunchKernelGGL(k_island_hook, dim3(blocks(nc)), dim3(256), 0, s, w, (int*)nullptr);
+            hipLaunchKernelGGL(k_island_jump, dim3(blocks(nb)), dim3(256), 0, s, w);
+        }
+        hipLaunchKernelGGL(k_island_union, dim3(blocks(nc)), dim3(256), 0, s, w);
+    }
+    if (ctx->nJoints > 0)
+        hipLaunchKernelGGL(k_island_union_joints, dim3(blocks(ctx->nJoints)), dim3(256), 0, s, w, ctx->jview());
+    if (method != 0 || ctx->nJoints > 0) hipLaunchKernelGGL(k_island_jump, dim3(blocks(nb)), dim3(256), 0, s, w);
+    LAUNCHCHK(ctx);
+    return B2G_OK;
+}
+
 int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
     hipStream_t s = ctx->stream;
     const float dt = p->dt;
@@ -2924,7 +3027,18 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
     // Collide: narrowphase, destruction of non-overlapping contacts.
     if (ctx->nContacts > 0) {
         int pe = profStart(ctx);
-        hipLaunchKernelGGL(k_collide, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), ctx->cnt);
+        const int* order = nullptr;
+        if (ctx->sched.collideOrder == 1) {
+            // the constraint-order buffers are free until the order phase
+            hipLaunchKernelGGL(k_collide_keys, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(),
+                               ctx->consKeys, ctx->consVals);
+            HIPCHK(ctx, prims_sort_pairs_u32(ctx->scratch, ctx->scratchBytes, ctx->consKeys, ctx->consKeysSorted,
+                                             ctx->consVals, ctx->consOrder, ctx->nContacts, 4, s));
+            order = ctx->consOrder;
+        }
+        const int cb = ctx->sched.collideBlock;
+        hipLaunchKernelGGL(k_collide, dim3((ctx->nContacts + cb - 1) / cb), dim3(cb), 0, s, ctx->view(), ctx->cnt,
+                           order);
         profEnd(ctx, PH_COLLIDE, pe);
         pe = profStart(ctx);
         hipLaunchKernelGGL(k_mark_dead, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), ctx->keep);
@@ -2957,27 +3071,7 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
             hipLaunchKernelGGL(k_fill<int>, dim3(blocks(nb * std::max(posIters, 1))), dim3(256), 0, s,
                                ctx->islandMinSep, nb * std::max(posIters, 1), 0);
             hipLaunchKernelGGL(k_island_init, dim3(blocks(nb)), dim3(256), 0, s, w);
-            const JointView jvw = ctx->jview();
-            for (int pass = 0; ctx->nContacts > 0 || ctx->nJoints > 0; ++pass) {
-                // two hook/jump rounds between checks of the change flag
-                hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->islandChanged, 1, 0);
-                for (int r = 0; r < 2; ++r) {
-                    if (ctx->nContacts > 0)
-                        hipLaunchKernelGGL(k_island_hook, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w,
-                                           &ctx->cnt->islandChanged);
-                    if (ctx->nJoints > 0)
-                        hipLaunchKernelGGL(k_island_hook_joints, dim3(blocks(ctx->nJoints)), dim3(256), 0, s, w, jvw,
-                                           &ctx->cnt->islandChanged);
-                    hipLaunchKernelGGL(k_island_jump, dim3(blocks(nb)), dim3(256), 0, s, w);
-                }
-                LAUNCHCHK(ctx);
-                HIPCHK(ctx, hipMemcpyAsync(&ctx->hcnt->islandChanged, &ctx->cnt->islandChanged, sizeof(int),
-                                           hipMemcpyDeviceToHost, s));
-                HIPCHK(ctx, hipStreamSynchronize(s));
-                if (!ctx->hcnt->islandChanged) break;
-                // every pass that changes something lowers a label, so this is a safeguard only
-                if (pass >= 1000000) return fail(ctx, B2G_ERR_STATE, "island labelling did not converge");
-            }
+            if ((rc = labelIslands(ctx, w)) != B2G_OK) return rc;
             hipLaunchKernelGGL(k_island_roots, dim3(blocks(nb)), dim3(256), 0, s, w);
             hipLaunchKernelGGL(k_island_mark, dim3(blocks(nb)), dim3(256), 0, s, w);
             hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->remaining, 1, 0);
@@ -3073,6 +3167,8 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
         }
         ctx->stats.freshPoints = nFresh;
         ctx->stats.residual = -1.0f;
+        const int sb = ctx->sched.solverBlock;
+        auto sgrid = [sb](int n) { return dim3((unsigned)((n + sb - 1) / sb)); };
         auto perColour = [&](int phase, auto kernelLaunch) {
             for (int c = 0; c < nColours; ++c) {
                 const int b0 = colourStart[c], b1 = colourStart[c + 1];
@@ -3099,7 +3195,7 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
                     if (serial)
                         hipLaunchKernelGGL(k_warm_start_serial, dim3(1), dim3(1), 0, s, w, ctx->cv, b0, b1);
                     else
-                        hipLaunchKernelGGL(k_warm_start, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, ctx->cv, b0, b1);
+                        hipLaunchKernelGGL(k_warm_start, sgrid(b1 - b0), dim3(sb), 0, s, w, ctx->cv, b0, b1);
                 });
             }
             perJointColour([&](int b0, int b1, bool serial) {
@@ -3107,7 +3203,7 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
                     hipLaunchKernelGGL(k_joint_init_serial, dim3(1), dim3(1), 0, s, w, jv, b0, b1, dtRatio,
                                        ctx->warmStarting ? 1 : 0);
                 else
-                    hipLaunchKernelGGL(k_joint_init, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, jv, b0, b1, dtRatio,
+                    hipLaunchKernelGGL(k_joint_init, sgrid(b1 - b0), dim3(sb), 0, s, w, jv, b0, b1, dtRatio,
                                        ctx->warmStarting ? 1 : 0);
             });
         };
@@ -3118,8 +3214,7 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
                     if (serial)
                         hipLaunchKernelGGL(k_joint_solve_velocity_serial, dim3(1), dim3(1), 0, s, w, jv, b0, b1, dt);
                     else
-                        hipLaunchKernelGGL(k_joint_solve_velocity, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, jv, b0,
-                                           b1, dt);
+                        hipLaunchKernelGGL(k_joint_solve_velocity, sgrid(b1 - b0), dim3(sb), 0, s, w, jv, b0, b1, dt);
                 });
                 perColour(PH_SOLVE_V, [&](int b0, int b1, bool serial) {
                     if (serial) {
@@ -3128,8 +3223,8 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
                     }
                     hipEvent_t e0, e1;
                     const int pk = profKernelEvents(ctx, &e0, &e1);
-                    hipExtLaunchKernelGGL(k_solve_velocity, dim3(blocks(b1 - b0)), dim3(256), 0, s, e0, e1, 0, w,
-                                          ctx->cv, b0, b1);
+                    hipExtLaunchKernelGGL(k_solve_velocity, sgrid(b1 - b0), dim3(sb), 0, s, e0, e1, 0, w, ctx->cv,
+                                          b0, b1);
                     profKernel(ctx, PH_K_SOLVE_V, pk);
                 });
             }
@@ -3220,16 +3315,15 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
                 }
                 hipEvent_t e0, e1;
                 const int pk = profKernelEvents(ctx, &e0, &e1);
-                hipExtLaunchKernelGGL(k_solve_position, dim3(blocks(b1 - b0)), dim3(256), 0, s, e0, e1, 0, w, ctx->cv,
-                                      b0, b1, ctx->cnt, it);
+                hipExtLaunchKernelGGL(k_solve_position, sgrid(b1 - b0), dim3(sb), 0, s, e0, e1, 0, w, ctx->cv, b0,
+                                      b1, ctx->cnt, it);
                 profKernel(ctx, PH_K_SOLVE_X, pk);
             });
             perJointColour([&](int b0, int b1, bool serial) {
                 if (serial)
                     hipLaunchKernelGGL(k_joint_solve_position_serial, dim3(1), dim3(1), 0, s, w, jv, b0, b1, it);
                 else
-                    hipLaunchKernelGGL(k_joint_solve_position, dim3(blocks(b1 - b0)), dim3(256), 0, s, w, jv, b0, b1,
-                                       it);
+                    hipLaunchKernelGGL(k_joint_solve_position, sgrid(b1 - b0), dim3(sb), 0, s, w, jv, b0, b1, it);
             });
         }
         pe = profStart(ctx);
@@ -4477,6 +4571,7 @@ int b2g_abi_sizes(int32_t* out, int cap) {
         (int32_t)sizeof(b2g_fixture_def), (int32_t)sizeof(b2g_world_def),    (int32_t)sizeof(b2g_step_params),
         (int32_t)sizeof(b2g_step_stats),  (int32_t)sizeof(b2g_phase_profile), (int32_t)sizeof(b2g_mlp_params),
         (int32_t)sizeof(b2g_aabb),        (int32_t)sizeof(b2g_transform),    (int32_t)sizeof(b2g_revolute_def),
+        (int32_t)sizeof(b2g_schedule),
     };
     const int n = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (!out) return n;
@@ -4487,6 +4582,83 @@ int b2g_abi_sizes(int32_t* out, int cap) {
 }  // extern "C"
 
 extern "C" {
+
+int b2g_set_schedule(b2g_context* ctx, const b2g_schedule* sc) {
+    if (!ctx || !sc) return B2G_ERR_ARG;
+    auto blockOk = [](int b) { return b >= 64 && b <= 1024 && b % 64 == 0; };
+    if (sc->islands < 0 || sc->islands > 2 || sc->collideOrder < 0 || sc->collideOrder > 1 ||
+        !blockOk(sc->solverBlock) || !blockOk(sc->collideBlock) || !blockOk(sc->queryBlock))
+        return fail(ctx, B2G_ERR_ARG, "schedule field out of range");
+    ctx->sched = *sc;
+    return B2G_OK;
+}
+
+int b2g_get_schedule(b2g_context* ctx, b2g_schedule* sc) {
+    if (!ctx || !sc) return B2G_ERR_ARG;
+    *sc = ctx->sched;
+    return B2G_OK;
+}
+
+int b2g_autotune(b2g_context* ctx, const b2g_step_params* p, int rounds, b2g_schedule* out) {
+    if (!ctx || !p || rounds <= 0) return B2G_ERR_ARG;
+    if (p->mode != B2G_MODE_COLOURED)
+        return fail(ctx, B2G_ERR_UNSUPPORTED, "the schedule applies to the coloured step");
+    // Each field is judged by the time of the phases it affects, from the
+    // events the profiler records on the step stream (dispatch gaps and
+    // host round trips inside a phase included).
+    struct Knob {
+        int32_t b2g_schedule::*field;
+        std::vector<int> values;
+        std::vector<int> phases;
+    };
+    const Knob knobs[] = {
+        {&b2g_schedule::islands, {0, 1, 2}, {PH_ISLANDS}},
+        {&b2g_schedule::collideOrder, {0, 1}, {PH_COLLIDE}},
+        {&b2g_schedule::solverBlock, {64, 128, 256, 512}, {PH_WARM, PH_SOLVE_V, PH_SOLVE_X}},
+        {&b2g_schedule::collideBlock, {64, 128, 256}, {PH_COLLIDE}},
+        {&b2g_schedule::queryBlock, {64, 128, 256}, {PH_BROADPHASE}},
+    };
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    // the caller's profile survives the tuning
+    const bool wasProfiling = ctx->profiling;
+    double savedMs[32];
+    int savedLaunches[32];
+    for (int i = 0; i < 32; ++i) {
+        savedMs[i] = ctx->profMs[i];
+        savedLaunches[i] = ctx->profLaunches[i];
+    }
+    ctx->profiling = true;
+    ctx->evMarks.clear();
+    ctx->evUsed = 0;
+    int rc = B2G_OK;
+    for (const Knob& k : knobs) {
+        const int nv = (int)k.values.size();
+        std::vector<double> total(nv, 0.0);
+        for (int r = 0; r < rounds && rc == B2G_OK; ++r) {
+            for (int j = 0; j < nv && rc == B2G_OK; ++j) {
+                const int i = (j + r) % nv;
+                ctx->sched.*k.field = k.values[i];
+                for (int ph : k.phases) ctx->profMs[ph] = 0.0;
+                rc = b2g_step(ctx, p);
+                profCollect(ctx);
+                for (int ph : k.phases) total[i] += ctx->profMs[ph];
+            }
+        }
+        if (rc != B2G_OK) break;
+        ctx->sched.*k.field = k.values[std::min_element(total.begin(), total.end()) - total.begin()];
+    }
+    ctx->profiling = wasProfiling;
+    ctx->evMarks.clear();
+    ctx->evUsed = 0;
+    for (int i = 0; i < 32; ++i) {
+        ctx->profMs[i] = savedMs[i];
+        ctx->profLaunches[i] = savedLaunches[i];
+    }
+    if (rc != B2G_OK) return rc;
+    if (out) *out = ctx->sched;
+    return B2G_OK;
+}
 
 int b2g_add_joints(b2g_context* ctx, const b2g_revolute_def* defs, int count, int* first) {
     if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;

@@ -246,6 +246,19 @@ int b2pp_world_step(void* h, float dt, int vel, int pos, int nsteps) {
     return 0;
 }
 
+// Autotunes the world's step schedule on 15 * rounds steps; *sched (5
+// int32, b2g_schedule) receives the choice when not null.
+int b2pp_world_autotune(void* h, float dt, int vel, int pos, int rounds, b2g_schedule* sched) {
+    Runner* r = (Runner*)h;
+    const int rc = r->world.Autotune(dt, vel, pos, rounds);
+    if (rc == 0 && sched) b2g_get_schedule(r->world.GetDeviceContext(), sched);
+    return rc;
+}
+
+int b2pp_world_set_schedule(void* h, const b2g_schedule* sched) {
+    return b2g_set_schedule(((Runner*)h)->world.GetDeviceContext(), sched);
+}
+
 int b2pp_world_sync(void* h) { return b2g_sync(((Runner*)h)->world.GetDeviceContext()); }
 
 const char* b2pp_world_last_error(void* h) { return ((Runner*)h)->world.GetLastError().c_str(); }

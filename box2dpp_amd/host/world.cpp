@@ -858,6 +858,35 @@ void b2World::Step(float dt, int32_t velocityIterations, int32_t positionIterati
     m_profile.step = std::chrono::duration<float, std::milli>(t1 - t0).count();
 }
 
+int32_t b2World::Autotune(float dt, int32_t velocityIterations, int32_t positionIterations, int32_t rounds) {
+    if (!m_ctx) return B2G_ERR_HIP;
+    if (IsLocked()) return B2G_ERR_STATE;
+    int rc = Upload();
+    if (rc == B2G_OK) {
+        b2g_step_params p{};
+        p.dt = dt;
+        p.velocityIterations = velocityIterations;
+        p.positionIterations = positionIterations;
+        p.mode = m_options.mode;
+        p.learnedWarmStart = m_options.learnedWarmStart ? 1 : 0;
+        p.residualThreshold = m_options.residualThreshold;
+        m_locked = true;
+        rc = b2g_autotune(m_ctx, &p, rounds, nullptr);
+        m_locked = false;
+    }
+    // the tuning steps are ordinary steps: mirrors and contacts are stale
+    m_hostStale = true;
+    m_contactsStale = true;
+    m_jointsStale = true;
+    b2g_step_stats st{};
+    if (b2g_get_stats(m_ctx, &st) == B2G_OK) m_contactCount = st.contacts;
+    if (rc != B2G_OK) {
+        m_lastStatus = rc;
+        m_lastError = b2g_last_error(m_ctx);
+    }
+    return rc;
+}
+
 b2Contact* b2World::GetContactList() {
     if (m_contactsStale && m_ctx) {
         int n = 0;

@@ -402,10 +402,33 @@ int b2g_sincos_batch(b2g_context* ctx, const float* angles, int n, float* s, flo
  * reference steps one b2World at a time, Box2D/Dynamics/b2World.h:96). */
 int b2g_set_body_worlds(b2g_context* ctx, int first, int count, const int32_t* worlds);
 
+/* ------------------------------------------------------------- schedule
+ * How the coloured step runs, not what it computes: every choice gives the
+ * same results bit for bit. No counterpart in the reference (its step is
+ * single-threaded). */
+typedef struct b2g_schedule {
+    int32_t islands;       /* island labelling: 0 label-propagation rounds, 1 one round
+                              then union-find, 2 union-find */
+    int32_t collideOrder;  /* 0 contact-table order, 1 grouped by shape-pair class */
+    int32_t solverBlock;   /* workgroup size of the per-colour solver kernels */
+    int32_t collideBlock;  /* workgroup size of the narrowphase kernel */
+    int32_t queryBlock;    /* workgroup size of the broad-phase query */
+} b2g_schedule;
+/* Workgroup sizes are multiples of 64 from 64 to 1024. */
+int b2g_set_schedule(b2g_context* ctx, const b2g_schedule* s);
+int b2g_get_schedule(b2g_context* ctx, b2g_schedule* s);
+/* Autotuner: picks each schedule field in turn by timing the candidates on
+ * real steps of this world (wall time of b2g_step, candidates interleaved
+ * and their order rotated each of `rounds` rounds), keeps the fastest, and
+ * returns the result in *out. The steps advance the world as ordinary
+ * steps do; since no choice changes results, the trajectory is the one the
+ * world would have had anyway. (3 + 2 + 4 + 3 + 3) * rounds steps. */
+int b2g_autotune(b2g_context* ctx, const b2g_step_params* p, int rounds, b2g_schedule* out);
+
 /* Sizes in bytes of the ABI structs, in this order: b2g_shape,
  * b2g_manifold, b2g_body_state, b2g_fixture_def, b2g_world_def,
  * b2g_step_params, b2g_step_stats, b2g_phase_profile, b2g_mlp_params,
- * b2g_aabb, b2g_transform, b2g_revolute_def. Returns the count (out may
+ * b2g_aabb, b2g_transform, b2g_revolute_def, b2g_schedule. Returns the count (out may
  * be null). Needs no
  * device; lets bindings check their struct layouts. */
 int b2g_abi_sizes(int32_t* out, int cap);

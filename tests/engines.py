@@ -42,6 +42,11 @@ STATS = np.dtype([
     ("cellSize", "<f4"), ("freshPoints", "<i4"), ("fallback", "<i4"), ("residual", "<f4"),
 ])
 
+SCHEDULE = np.dtype([("islands", "<i4"), ("collideOrder", "<i4"), ("solverBlock", "<i4"), ("collideBlock", "<i4"),
+                     ("queryBlock", "<i4")])
+STEP_PARAMS = np.dtype([("dt", "<f4"), ("velocityIterations", "<i4"), ("positionIterations", "<i4"), ("mode", "<i4"),
+                        ("learnedWarmStart", "<i4"), ("residualThreshold", "<f4")])
+
 # body row columns of *_world_get_bodies
 PX, PY, QS, QC, CX, CY, ANG, VX, VY, W, AWAKE, SLEEP = range(12)
 

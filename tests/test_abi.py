@@ -36,7 +36,7 @@ def test_struct_layouts_match_the_library():
     n = lib.b2g_abi_sizes(None, 0)
     sizes = np.zeros(n, np.int32)
     lib.b2g_abi_sizes(E.ptr(sizes), n)
-    shape, manifold, body, fixture, world, params, stats, profile, mlp, aabb, xf, revolute = sizes[:12]
+    shape, manifold, body, fixture, world, params, stats, profile, mlp, aabb, xf, revolute, schedule = sizes[:13]
     assert E.SHAPE.itemsize == shape
     assert E.MANIFOLD.itemsize == manifold
     assert E.STATS.itemsize == stats
@@ -47,3 +47,5 @@ def test_struct_layouts_match_the_library():
     assert ctypes.sizeof(bench.PhaseProfile) == profile
     assert bench.MLP_PARAM_FLOATS * 4 == mlp
     assert revolute == 56  # 2 body indices, 9 floats, 3 switches
+    assert schedule == E.SCHEDULE.itemsize
+    assert params == E.STEP_PARAMS.itemsize

@@ -1,0 +1,81 @@
+"""The step schedule (island labelling method, collide order, workgroup
+sizes) changes how the coloured step runs, not what it computes: worlds
+stepped under different schedules stay identical bit for bit, and the
+autotuner's trial steps are ordinary steps."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import engines as E
+
+pytestmark = pytest.mark.gpu
+
+DEFAULT = (0, 0, 256, 256, 256)
+SCHEDULES = [(1, 1, 64, 128, 64), (2, 1, 512, 64, 128), (2, 0, 128, 1024, 256), (0, 1, 192, 64, 512)]
+
+
+def _world(gpu):
+    # three stress worlds: piles, edges and jointed chains in one context
+    return E.World.worlds(gpu, "stress", 3, 1500, 5, sleep=True)
+
+
+def _state(w):
+    meta, man = w.contacts()
+    return (w.bodies() + np.float32(0)).view(np.uint32), meta, E.manifold_bits(man), (w.joints() + np.float32(0)).view(np.uint32)
+
+
+def _set(w, sched):
+    s = np.array([sched], E.SCHEDULE)
+    rc = w.e.lib.b2pp_world_set_schedule(w.h, E.ptr(s))
+    assert rc == 0, w.error()
+
+
+def _same(a, b):
+    sa, sb = _state(a), _state(b)
+    for x, y in zip(sa, sb):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("sched", SCHEDULES)
+def test_schedules_give_identical_steps(gpu, sched):
+    a, b = _world(gpu), _world(gpu)
+    _set(a, DEFAULT)
+    _set(b, sched)
+    for _ in range(4):
+        a.step(50)
+        b.step(50)
+        _same(a, b)
+    assert a.stats()["contacts"] > 0
+    a.close(); b.close()
+
+
+def test_autotune_steps_are_ordinary_steps(gpu):
+    a, b = _world(gpu), _world(gpu)
+    a.step(100)
+    b.step(100)
+    rounds = 2
+    out = np.zeros(1, E.SCHEDULE)
+    rc = b.e.lib.b2pp_world_autotune(b.h, ctypes.c_float(1.0 / 60.0), 8, 3, rounds, E.ptr(out))
+    assert rc == 0, b.error()
+    a.step(15 * rounds)
+    _same(a, b)
+    o = out[0]
+    assert o["islands"] in (0, 1, 2) and o["collideOrder"] in (0, 1)
+    assert o["solverBlock"] in (64, 128, 256, 512) and o["collideBlock"] in (64, 128, 256)
+    assert o["queryBlock"] in (64, 128, 256)
+    a.step(60)
+    b.step(60)
+    _same(a, b)
+    a.close(); b.close()
+
+
+def test_set_schedule_rejects_bad_values(gpu):
+    w = gpu.world("pyramid", 4, 0)
+    for bad in [(3, 0, 256, 256, 256), (0, 2, 256, 256, 256), (0, 0, 100, 256, 256), (0, 0, 256, 2048, 256),
+                (0, 0, 256, 256, 0)]:
+        s = np.array([bad], E.SCHEDULE)
+        assert w.e.lib.b2pp_world_set_schedule(w.h, E.ptr(s)) != 0
+    w.close()
