@@ -160,7 +160,7 @@ def _check(b2g, ctx, rc):
         raise RuntimeError(f"b2g call failed rc={rc}: {b2g.b2g_last_error(ctx).decode()}")
 
 
-def learned_warm_start(lib, args):
+def learned_warm_start(lib, args, scope=0, stride=1):
     """Second half of the metric: velocity iterations to reach a residual,
     plain (cold, warm) against the learned warm start.
 
@@ -173,6 +173,10 @@ def learned_warm_start(lib, args):
        RMS complementarity residual (FP64, host) after each;
     4. time the learned step (network inference for fresh points inside the
        step) over the same steps as the plain timed run.
+
+    scope 0 serves fresh points (newly created contact points); scope 1
+    serves every solved point, with the carried impulse as an input, and
+    logs every `stride`-th step.
     """
     b2g = box2dpp_amd.b2g()
     b2g.b2g_last_error.restype = ctypes.c_char_p
@@ -181,13 +185,14 @@ def learned_warm_start(lib, args):
     progress("learned warm start: logging fresh points")
     h = make_world(lib, args.bodies, args.log_seed)
     ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
+    _check(b2g, ctx, b2g.b2g_set_learned_scope(ctx, scope, stride))
     _check(b2g, ctx, b2g.b2g_log_fresh(ctx, args.log_capacity))
     t0 = time.perf_counter()
     step(lib, h, args.steps, args.vel, args.pos)
     lib.b2pp_world_sync(h)
     n = ctypes.c_int()
     b2g.b2g_log_size(ctx, ctypes.byref(n))
-    out["log"] = {"seed": args.log_seed, "steps": args.steps, "points": n.value,
+    out["log"] = {"seed": args.log_seed, "steps": args.steps, "step_stride": stride, "points": n.value,
                   "seconds": round(time.perf_counter() - t0, 2)}
     # 2. train (kernel time of the gradient kernel from dispatch-packet events)
     progress(f"learned warm start: training on {n.value} points")
@@ -221,6 +226,7 @@ def learned_warm_start(lib, args):
     progress("learned warm start: residual traces")
     h = make_world(lib, args.bodies, args.eval_seed)
     ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
+    _check(b2g, ctx, b2g.b2g_set_learned_scope(ctx, scope, stride))
     _check(b2g, ctx, b2g.b2g_mlp_set_params(ctx, params))
     T = args.trace_iters
     curves = []
@@ -245,6 +251,7 @@ def learned_warm_start(lib, args):
         return int(hit[0]) if len(hit) else None
 
     plain8 = float(cur[1, min(8, T)])
+
     out["residual"] = {
         "definition": "RMS over solved points of r_n (vn - bias if normal impulse > 0 else min(vn - bias, 0)) "
                       "and r_t (vt if |tangent impulse| < friction * normal impulse else 0), m/s, FP64 on host",
@@ -259,6 +266,7 @@ def learned_warm_start(lib, args):
     progress("learned warm start: learned step")
     h = make_world(lib, args.bodies, args.eval_seed)
     ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
+    _check(b2g, ctx, b2g.b2g_set_learned_scope(ctx, scope, stride))
     _check(b2g, ctx, b2g.b2g_mlp_set_params(ctx, params))
     lib.b2pp_world_set_learned(h, 1, ctypes.c_float(0.0))
     lib.b2pp_world_sync(h)
@@ -467,6 +475,8 @@ def main():
     ap.add_argument("--tune-rounds", type=int, default=3,
                     help="autotuner rounds on the warm-up copy (0: default schedule)")
     ap.add_argument("--tune-at", type=int, default=500, help="step of the warm-up copy at which to autotune")
+    ap.add_argument("--all-points-stride", type=int, default=30,
+                    help="step stride of the every-point log (0 skips the every-point evaluation)")
     ap.add_argument("--log-seed", type=int, default=2)
     ap.add_argument("--eval-seed", type=int, default=1001)
     ap.add_argument("--log-capacity", type=int, default=10_000_000)
@@ -549,6 +559,10 @@ def main():
     lib.b2pp_world_destroy(h)
 
     learned, params = (None, None) if args.no_learned else learned_warm_start(lib, args)
+    learned_all = None
+    if not args.no_learned and args.all_points_stride > 0:
+        progress("learned warm start, every point")
+        learned_all, _ = learned_warm_start(lib, args, 1, args.all_points_stride)
     batched = None if args.no_batched else batched_worlds(lib, args)
     stressed = None if args.no_stress else stress(lib, args, params)
 
@@ -582,6 +596,7 @@ def main():
         "cpu_baseline": cpu,
         "schedule": sched_dict(_schedule),
         "learned_warm_start": learned,
+        "learned_warm_start_every_point": learned_all,
         "batched_worlds": batched,
         "stress": stressed,
         "final_state": {"contacts": st.contacts, "constraints": st.constraints, "points": st.points,

@@ -593,9 +593,10 @@ __global__ void k_integrate_velocities(WorldView w, V2 gravity, float h) {
 // features feed the warm-start network and the training log.
 constexpr int kFeatures = 16;
 struct FreshView {
-    int* mask;    // per constraint: bit j set if point j is fresh
+    int* mask;    // per constraint: bit j set if point j is a network candidate
     int* num;     // popcount of mask
-    float* feat;  // [2 * constraints][kFeatures], written for fresh points only
+    float* feat;  // [2 * constraints][kFeatures], written for candidate points only
+    int scope;    // 0: fresh points; 1: every solved point, carried impulse as input
 };
 
 // b2ContactSolver ctor + InitializeVelocityConstraints,
@@ -688,20 +689,24 @@ __device__ inline void initConstraintOne(const WorldView& w, const ConView& cv, 
     cv.rad[k] = make_float2(radiusA, radiusB);
     cv.island[k] = w.islandOf[w.btype[bA] != BT_STATIC ? bA : bB];
     if (fv.mask) {
-        // Features of the fresh points: relative normal and tangent velocity,
-        // normal, separation, inverse masses and inertias, point count,
-        // normal and tangent mass, friction, r x n of both bodies,
-        // restitution bias.
+        // Features of the candidate points: relative normal and tangent
+        // velocity, normal, separation, inverse masses and inertias, point
+        // count, normal and tangent mass, friction, r x n of both bodies,
+        // restitution bias. In scope 1 (every point) features 9 and 15 are
+        // the carried normal and tangent impulse instead, in velocity-change
+        // units (impulse over the point's normal or tangent mass).
         int fm = 0;
         for (int j = 0; j < vpc; ++j) {
-            if (ni[j] != 0.0f || ti[j] != 0.0f) continue;
+            if (fv.scope == 0 && (ni[j] != 0.0f || ti[j] != 0.0f)) continue;
             fm |= 1 << j;
             const V2 dv = vB + cross(wB, rB[j]) - vA - cross(wA, rA[j]);
+            const float f9 = fv.scope == 0 ? (float)vpc : (nm[j] > 0.0f ? ni[j] / nm[j] : 0.0f);
+            const float f15 = fv.scope == 0 ? vb[j] : (tm[j] > 0.0f ? ti[j] / tm[j] : 0.0f);
             float4* f = (float4*)(fv.feat + (size_t)(2 * k + j) * kFeatures);
             f[0] = make_float4(dot(dv, normal), dot(dv, tangent), normal.x, normal.y);
             f[1] = make_float4(wm.separations[j], mA, mB, iA);
-            f[2] = make_float4(iB, (float)vpc, nm[j], tm[j]);
-            f[3] = make_float4(mat.x, cross(rA[j], normal), cross(rB[j], normal), vb[j]);
+            f[2] = make_float4(iB, f9, nm[j], tm[j]);
+            f[3] = make_float4(mat.x, cross(rA[j], normal), cross(rB[j], normal), f15);
         }
         fv.mask[k] = fm;
         fv.num[k] = __popc(fm);
@@ -732,12 +737,14 @@ __global__ void k_fresh_gather(int n, FreshView fv, const int* off, float* feat,
     }
 }
 
-// Appends the fresh points of the step to the training log: features and
-// the impulses the solver gave them, in velocity-change units (the normal
-// impulse over the point's normal mass, the tangent impulse over its tangent
-// mass), so that an impact's target is about its approach speed.
+// Appends the candidate points of the step to the training log: features
+// and the impulses the solver gave them, in velocity-change units (the
+// normal impulse over the point's normal mass, the tangent impulse over its
+// tangent mass), so that an impact's target is about its approach speed. In
+// scope 1 the target is the change from the carried impulse (features 9 and
+// 15).
 __global__ void k_log_append(int nFresh, const int* ref, const float* feat, const float4* imp, float* logFeat,
-                             float* logTarget, int base, int cap) {
+                             float* logTarget, int base, int cap, int scope) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nFresh || base + e >= cap) return;
     const size_t d = (size_t)(base + e);
@@ -749,8 +756,8 @@ __global__ void k_log_append(int nFresh, const int* ref, const float* feat, cons
     const float4 v = imp[r >> 1];
     const float nm = f[10], tm = f[11];
     const float ln = (r & 1) ? v.z : v.x, lt = (r & 1) ? v.w : v.y;
-    logTarget[2 * d] = nm > 0.0f ? ln / nm : 0.0f;
-    logTarget[2 * d + 1] = tm > 0.0f ? lt / tm : 0.0f;
+    logTarget[2 * d] = (nm > 0.0f ? ln / nm : 0.0f) - (scope == 1 ? f[9] : 0.0f);
+    logTarget[2 * d + 1] = (tm > 0.0f ? lt / tm : 0.0f) - (scope == 1 ? f[15] : 0.0f);
 }
 
 // Complementarity residual of the velocity constraints, summed over the
@@ -2286,6 +2293,9 @@ struct b2g_context {
     float* logFeat = nullptr;
     float* logTarget = nullptr;
     int logCap = 0, logCount = 0;
+    int learnedScope = 0;   // 0 fresh points, 1 every solved point
+    int logStride = 1;      // scope 1 logs every logStride-th step
+    long long logSteps = 0;
     void* trainScratch = nullptr;
     size_t trainScratchBytes = 0;
     int traceIters = 0;
@@ -3143,7 +3153,7 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
             return fail(ctx, B2G_ERR_STATE, "learned warm start requested without a warm-start network");
         const bool wantFresh = nCons > 0 && (learned || ctx->logCap > 0 || ctx->traceIters > 0);
         if (wantFresh && (rc = ensureFresh(ctx)) != B2G_OK) return rc;
-        const FreshView fv{wantFresh ? ctx->freshMask : nullptr, ctx->freshNum, ctx->freshAll};
+        const FreshView fv{wantFresh ? ctx->freshMask : nullptr, ctx->freshNum, ctx->freshAll, ctx->learnedScope};
         int pe = profStart(ctx);
         hipLaunchKernelGGL(k_integrate_velocities, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, ctx->gravity, dt);
         profEnd(ctx, PH_INTEGRATE_V, pe);
@@ -3233,7 +3243,7 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
             hipEvent_t e0, e1;
             const int pk = profKernelEvents(ctx, &e0, &e1);
             HIPCHK(ctx, b2mlp::mlpPredictImpulses(ctx->mlp, ctx->freshFeat, ctx->freshRef, &ctx->cnt->nFresh, nFresh,
-                                                  ctx->cv.normal, imp, s, e0, e1));
+                                                  ctx->cv.normal, imp, ctx->learnedScope == 1, s, e0, e1));
             profKernel(ctx, PH_K_MLP_INFER, pk);
             return B2G_OK;
         };
@@ -3290,9 +3300,12 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
             }
         }
         LAUNCHCHK(ctx);
-        if (ctx->logCap > 0 && nFresh > 0 && ctx->logCount < ctx->logCap) {
+        const bool logStep = ctx->learnedScope == 0 || ctx->logSteps % ctx->logStride == 0;
+        if (ctx->logCap > 0) ++ctx->logSteps;
+        if (ctx->logCap > 0 && logStep && nFresh > 0 && ctx->logCount < ctx->logCap) {
             hipLaunchKernelGGL(k_log_append, dim3(blocks(nFresh)), dim3(256), 0, s, nFresh, ctx->freshRef,
-                               ctx->freshFeat, ctx->cv.imp, ctx->logFeat, ctx->logTarget, ctx->logCount, ctx->logCap);
+                               ctx->freshFeat, ctx->cv.imp, ctx->logFeat, ctx->logTarget, ctx->logCount, ctx->logCap,
+                               ctx->learnedScope);
             LAUNCHCHK(ctx);
             ctx->logCount = std::min(ctx->logCap, ctx->logCount + nFresh);
         }
@@ -4284,6 +4297,14 @@ int b2g_log_fresh(b2g_context* ctx, int capacity) {
     HIPCHK(ctx, hipMalloc((void**)&ctx->logFeat, sizeof(float) * kFeatures * (size_t)capacity));
     HIPCHK(ctx, hipMalloc((void**)&ctx->logTarget, sizeof(float) * 2 * (size_t)capacity));
     ctx->logCap = capacity;
+    return B2G_OK;
+}
+
+int b2g_set_learned_scope(b2g_context* ctx, int scope, int logStride) {
+    if (!ctx || scope < 0 || scope > 1 || logStride < 1) return B2G_ERR_ARG;
+    ctx->learnedScope = scope;
+    ctx->logStride = logStride;
+    ctx->logSteps = 0;
     return B2G_OK;
 }
 

@@ -68,12 +68,14 @@ hipError_t mlpGradient(const Net& n, const float* x, const float* t, const int* 
                        hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // One Adam step from n.grad (then cleared); refreshes the BF16 image.
 hipError_t mlpAdam(Net& n, float lr, hipStream_t s);
-// Impulses for the fresh contact points of the step: rows of feat (count x 16)
-// through the network, clamped to the friction cone, written into the
-// constraint impulses imp[ref[i] >> 1] component (ref[i] & 1).
+// Impulses for the candidate contact points of the step: rows of feat
+// (count x 16) through the network, clamped to the friction cone, written
+// into the constraint impulses imp[ref[i] >> 1] component (ref[i] & 1). With
+// delta, the outputs are added to the carried impulse (features 9 and 15,
+// velocity-change units) before scaling by the point's masses.
 hipError_t mlpPredictImpulses(const Net& n, const float* feat, const int* ref, const int* countDev, int maxCount,
-                              const float4* normalFriction, float4* imp, hipStream_t s, hipEvent_t e0 = nullptr,
-                              hipEvent_t e1 = nullptr);
+                              const float4* normalFriction, float4* imp, bool delta, hipStream_t s,
+                              hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // Matrix-core operations per sample as executed (padded output layer):
 // forward 2 * (16*64 + 64*64 + 64*64 + 64*16); a training sample runs the
 // forward, the backward-data products of layers 4..2 and the four

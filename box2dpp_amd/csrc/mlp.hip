@@ -112,6 +112,8 @@ __device__ inline f32x4 outputLayer(const Smem& S, const bf16x4* a3, int g, int 
 // ------------------------------------------------------------ inference
 // MODE 0: y[2s], y[2s+1] = outputs (raw units).
 // MODE 1: impulses of fresh contact points (see mlpPredictImpulses).
+// MODE 2: impulses of any contact point as the carried impulse (features 9
+//         and 15, velocity-change units) plus the outputs.
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_mlp_infer(const uint16_t* wh, const float* params, const float* norm,
                                                         const float* x, int count, const int* countDev, float* y,
@@ -145,7 +147,9 @@ __global__ __launch_bounds__(kThreads) void k_mlp_infer(const uint16_t* wh, cons
                 const int r = ref[s];
                 const int k = r >> 1, j = r & 1;
                 const float mu = nf[k].z;
-                const float yn = y0 * x[s * kIn + 10], yt = y1 * x[s * kIn + 11];
+                const float un = MODE == 2 ? y0 + x[s * kIn + 9] : y0;
+                const float ut = MODE == 2 ? y1 + x[s * kIn + 15] : y1;
+                const float yn = un * x[s * kIn + 10], yt = ut * x[s * kIn + 11];
                 const float ln = yn > 0.0f ? yn : 0.0f;
                 const float lim = mu * ln;
                 const float lt = yt < -lim ? -lim : (yt > lim ? lim : yt);
@@ -429,11 +433,15 @@ hipError_t mlpForward(const Net& n, const float* x, int count, float* y, hipStre
 }
 
 hipError_t mlpPredictImpulses(const Net& n, const float* feat, const int* ref, const int* countDev, int maxCount,
-                              const float4* normalFriction, float4* imp, hipStream_t s, hipEvent_t e0,
+                              const float4* normalFriction, float4* imp, bool delta, hipStream_t s, hipEvent_t e0,
                               hipEvent_t e1) {
     if (maxCount <= 0) return hipSuccess;
-    hipExtLaunchKernelGGL(k_mlp_infer<1>, dim3(gridFor(maxCount)), dim3(kThreads), 0, s, e0, e1, 0, n.wh, n.params,
-                          n.norm, feat, maxCount, countDev, (float*)nullptr, ref, normalFriction, imp);
+    if (delta)
+        hipExtLaunchKernelGGL(k_mlp_infer<2>, dim3(gridFor(maxCount)), dim3(kThreads), 0, s, e0, e1, 0, n.wh,
+                              n.params, n.norm, feat, maxCount, countDev, (float*)nullptr, ref, normalFriction, imp);
+    else
+        hipExtLaunchKernelGGL(k_mlp_infer<1>, dim3(gridFor(maxCount)), dim3(kThreads), 0, s, e0, e1, 0, n.wh,
+                              n.params, n.norm, feat, maxCount, countDev, (float*)nullptr, ref, normalFriction, imp);
     return hipGetLastError();
 }
 

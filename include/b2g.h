@@ -333,9 +333,19 @@ typedef struct b2g_mlp_params {
     float ystd[2];
 } b2g_mlp_params;
 
-/* Logs the fresh points of the following steps with the impulses the solver
- * leaves on them, up to `capacity` points (0 stops and frees the log). */
+/* Logs the candidate points of the following steps with the impulses the
+ * solver leaves on them, up to `capacity` points (0 stops and frees the
+ * log). */
 int b2g_log_fresh(b2g_context* ctx, int capacity);
+/* Which points the network serves (logging, the learned start and the
+ * learned trace curve):
+ *   scope 0 (default): fresh points only, the features above;
+ *   scope 1: every solved point, the network replacing the carried impulse;
+ *            features 9 and 15 are then the carried normal and tangent
+ *            impulse in velocity-change units (0 for a fresh point), and the
+ *            log takes every logStride-th step (whole steps).
+ * A network is trained for one scope. */
+int b2g_set_learned_scope(b2g_context* ctx, int scope, int logStride);
 int b2g_log_size(b2g_context* ctx, int* count);
 int b2g_get_log(b2g_context* ctx, int first, int count, float* features, float* targets);
 /* Fresh network with seeded random weights; the normalisation is taken from
@@ -356,8 +366,9 @@ int b2g_mlp_gradient(b2g_context* ctx, const float* x, const float* t, int n, b2
 
 /* Residual trace: while maxIters > 0, each step also solves its velocity
  * constraints from three starts (cold: all impulses zero; warm: carried
- * impulses, fresh points zero; learned: carried impulses, fresh points from
- * the network) for maxIters iterations each, recording the RMS
+ * impulses, fresh points zero; learned: the candidate points of the scope
+ * from the network, the others carried) for maxIters iterations each,
+ * recording the RMS
  * complementarity residual (FP64 on the host) after the warm start and after
  * every iteration; the step's own result is unchanged. */
 int b2g_set_trace(b2g_context* ctx, int maxIters);

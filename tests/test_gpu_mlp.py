@@ -268,3 +268,63 @@ def test_learned_start_lowers_early_residual(lib, gpu):
     # start (before any iteration it can be above it: predicted impulses on
     # points that then separate count in the residual).
     assert np.all(learned[1:] < warm[1:]), (warm, learned)
+
+
+def test_every_point_scope_features_and_targets(lib, gpu):
+    """Scope 1 serves every solved point. HelloWorld's box at rest: both
+    persistent points are candidates; feature 9 is the carried normal
+    impulse over the point's normal mass (k_n = 1/m + (r x n)^2 / I with
+    r x n = -1, +1 at the corners: 0.25 + 1 / (4 * 8 / 12) = 0.625) and the
+    target is the solved impulse in the same units minus that."""
+    w = gpu.world("hello", sleep=False)
+    c = w.device_context()
+    w.step(150, vel=6, pos=2)
+    _, man = w.contacts()
+    carried = np.array([man[0]["points"][j]["ni"] for j in range(2)], np.float64)
+    assert np.all(carried > 0)
+    E.b2g_check(lib, c, lib.b2g_set_learned_scope(c, 1, 1))
+    E.b2g_check(lib, c, lib.b2g_log_fresh(c, 100))
+    w.step(1, vel=6, pos=2)
+    n = ctypes.c_int()
+    lib.b2g_log_size(c, ctypes.byref(n))
+    assert n.value == 2
+    f = np.zeros((2, 16), np.float32)
+    t = np.zeros((2, 2), np.float32)
+    E.b2g_check(lib, c, lib.b2g_get_log(c, 0, 2, E.ptr(f), E.ptr(t)))
+    _, man = w.contacts()
+    solved = np.array([man[0]["points"][j]["ni"] for j in range(2)], np.float64)
+    kn = 0.25 + 1.0 / (4.0 * 8.0 / 12.0)
+    for j in range(2):
+        jj = j if abs(f[j, 9] - carried[j] * kn) < abs(f[j, 9] - carried[1 - j] * kn) else 1 - j
+        assert f[j, 9] == pytest.approx(carried[jj] * kn, rel=1e-5)
+        assert f[j, 10] == pytest.approx(1.0 / kn, rel=1e-5)
+        assert t[j, 0] == pytest.approx(solved[jj] * kn - f[j, 9], abs=1e-5)
+    assert lib.b2g_set_learned_scope(c, 2, 1) != 0
+    assert lib.b2g_set_learned_scope(c, 1, 0) != 0
+    w.close()
+
+
+def test_every_point_zero_network_is_the_warm_start(lib, gpu):
+    """In scope 1 the network's outputs are added to the carried impulse, so
+    a network that outputs zero starts every point from its carried impulse
+    (up to the rounding of dividing and multiplying by the point's mass and
+    the friction-cone clamp): its residual curve is the warm start's."""
+    p = np.zeros(1, M.PARAMS_DTYPE)
+    p["xstd"] = 1.0
+    p["ystd"] = 1.0
+    w = gpu.world("world", 1500, 4, sleep=False)
+    c = w.device_context()
+    w.step(120)
+    E.b2g_check(lib, c, lib.b2g_set_learned_scope(c, 1, 1))
+    _set(lib, c, p[0])
+    E.b2g_check(lib, c, lib.b2g_set_trace(c, 6))
+    w.step(1)
+    E.b2g_check(lib, c, lib.b2g_set_trace(c, 0))
+    cur = _trace(lib, c)
+    assert np.all(np.isfinite(cur))
+    # the clamp moves carried tangent impulses that left the cone when the
+    # normal impulse dropped after the tangent update; the iterations then
+    # take both starts to the same residual
+    assert abs(cur[2, 0] - cur[1, 0]) <= 0.05 * cur[1, 0], cur
+    assert np.allclose(cur[2, 1:], cur[1, 1:], rtol=2e-2, atol=1e-4), cur
+    w.close()
