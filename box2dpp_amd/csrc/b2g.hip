@@ -989,6 +989,35 @@ __global__ void k_store_impulses(WorldView w, ConView cv, int n) {
     storeImpulsesOne(w, cv, k);
 }
 
+// Kernels of a captured solve: colour c's range [ranges[c], ranges[c + 1])
+// is read from device memory at run time (ranges[kColourSlots] is the
+// constraint count), and any grid covers it by striding, so one captured
+// launch sequence serves every step with the same colour layout.
+__global__ void k_warm_start_r(WorldView w, ConView cv, const int* ranges, int c) {
+    const int b1 = ranges[c + 1];
+    for (int k = ranges[c] + blockIdx.x * blockDim.x + threadIdx.x; k < b1; k += gridDim.x * blockDim.x)
+        warmStartOne(w, cv, k);
+}
+
+__global__ void k_warm_start_serial_r(WorldView w, ConView cv, const int* ranges, int c) {
+    for (int k = ranges[c]; k < ranges[c + 1]; ++k) warmStartOne(w, cv, k);
+}
+
+__global__ void k_solve_velocity_r(WorldView w, ConView cv, const int* ranges, int c) {
+    const int b1 = ranges[c + 1];
+    for (int k = ranges[c] + blockIdx.x * blockDim.x + threadIdx.x; k < b1; k += gridDim.x * blockDim.x)
+        solveVelocityOne(w, cv, k);
+}
+
+__global__ void k_solve_velocity_serial_r(WorldView w, ConView cv, const int* ranges, int c) {
+    for (int k = ranges[c]; k < ranges[c + 1]; ++k) solveVelocityOne(w, cv, k);
+}
+
+__global__ void k_store_impulses_r(WorldView w, ConView cv, const int* ranges) {
+    const int n = ranges[kColourSlots];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) storeImpulsesOne(w, cv, k);
+}
+
 // Position integration with translation and rotation clamps,
 // REF/Dynamics/b2Island.cpp:276-306.
 __device__ inline void integratePositionOne(const WorldView& w, int b, float h) {
@@ -1111,6 +1140,39 @@ __global__ void k_solve_position(WorldView w, ConView cv, int begin, int end, Co
     }
     __syncthreads();
     if (threadIdx.x == 0 && blockMin < 0) atomicMin(&cnt->minSep[it], blockMin);
+}
+
+__global__ void k_solve_position_r(WorldView w, ConView cv, const int* ranges, int c, Counters* cnt, int it) {
+    __shared__ int blockMin;
+    if (threadIdx.x == 0) blockMin = 0;
+    __syncthreads();
+    const int b1 = ranges[c + 1];
+    int lowest = 0;
+    for (int k = ranges[c] + blockIdx.x * blockDim.x + threadIdx.x; k < b1; k += gridDim.x * blockDim.x) {
+        const int root = cv.island[k];
+        if (it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0) continue;
+        const float minSep = solvePositionOne(w, cv, k);
+        if (minSep < -3.0f * kLinearSlop) {
+            int* f = &w.islandMinSep[it * w.nBodies + root];
+            if (*f == 0) *f = 1;
+        }
+        if (minSep < 0.0f) lowest = min(lowest, encodeOrdered(minSep));
+    }
+    if (lowest < 0) atomicMin(&blockMin, lowest);
+    __syncthreads();
+    if (threadIdx.x == 0 && blockMin < 0) atomicMin(&cnt->minSep[it], blockMin);
+}
+
+__global__ void k_solve_position_serial_r(WorldView w, ConView cv, const int* ranges, int c, Counters* cnt, int it) {
+    int lowest = 0;
+    for (int k = ranges[c]; k < ranges[c + 1]; ++k) {
+        const int root = cv.island[k];
+        if (it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0) continue;
+        const float minSep = solvePositionOne(w, cv, k);
+        if (minSep < -3.0f * kLinearSlop) w.islandMinSep[it * w.nBodies + root] = 1;
+        if (minSep < 0.0f) lowest = min(lowest, encodeOrdered(minSep));
+    }
+    if (lowest < 0) atomicMin(&cnt->minSep[it], lowest);
 }
 
 __global__ void k_solve_position_serial(WorldView w, ConView cv, int begin, int end, Counters* cnt, int it) {
@@ -2294,6 +2356,20 @@ struct b2g_context {
     float* logTarget = nullptr;
     int logCap = 0, logCount = 0;
     int learnedScope = 0;   // 0 fresh points, 1 every solved point
+    // Captured solves: launch sequences from the warm start to the
+    // transform sync, keyed by everything their launches depend on.
+    struct SolveGraph {
+        std::vector<unsigned long long> key;
+        hipGraphExec_t exec = nullptr;
+        long long used = 0;
+    };
+    bool graphs = true;
+    std::vector<SolveGraph> solveGraphs;
+    std::vector<std::vector<unsigned long long>> recentKeys;  // keys of the last steps not captured
+    long long graphClock = 0;
+    int* dRanges = nullptr;  // colour ranges of the step (kColourSlots + 1 ints)
+    int* hRanges = nullptr;  // pinned host staging of dRanges
+    long long graphLaunches = 0, graphCaptures = 0;
     int logStride = 1;      // scope 1 logs every logStride-th step
     long long logSteps = 0;
     void* trainScratch = nullptr;
@@ -2969,6 +3045,169 @@ int traceResiduals(b2g_context* ctx, const WorldView& w, int nCons, int nFresh, 
     return B2G_OK;
 }
 
+// The plain solve of a step -- contact warm start and joint init, the
+// velocity passes (joint colours, then contact colours), store impulses,
+// position integration, the position passes (contact colours, then joint
+// colours) and the transform sync -- as one captured graph. Contact colour
+// ranges come from ctx->dRanges, written before each launch, and each
+// colour's grid is its size class (the next power of two of the workgroups
+// it needs), so a graph stays valid while the layout key repeats: views,
+// iteration counts, time step, the non-empty colours and their size
+// classes, the joint colours. A key is captured when it has been seen in
+// one of the last three steps; until then the step launches directly. The
+// kernels are the direct path's, reading their ranges from memory, so both
+// give the same results.
+int solveByGraph(b2g_context* ctx, const WorldView& w, const JointView& jv, const int* colourStart, int nColours,
+                 int nCons, float dt, float dtRatio, int velIters, int posIters, bool* ran) {
+    *ran = false;
+    hipStream_t s = ctx->stream;
+    const int sb = ctx->sched.solverBlock;
+    auto sizeClass = [sb](int n) {
+        const int g = (n + sb - 1) / sb;
+        int c = 1;
+        while (c < g) c <<= 1;
+        return c;
+    };
+    const bool warm = nCons > 0 && ctx->warmStarting;
+    const bool positions = nCons > 0 || ctx->nJoints > 0;
+    std::vector<unsigned long long> key;
+    auto put = [&](unsigned long long v) { key.push_back(v); };
+    auto ptr = [&](const void* q) { put((unsigned long long)(uintptr_t)q); };
+    // every pointer and count the captured kernels read
+    put((unsigned)w.nBodies);
+    ptr(w.btype); ptr(w.bflags); ptr(w.xf); ptr(w.pos); ptr(w.pos0); ptr(w.lcenter); ptr(w.vel); ptr(w.force);
+    ptr(w.mass); ptr(w.gsleep); ptr(w.colourMask); ptr(w.islandOf); ptr(w.islandAwake); ptr(w.islandSleep);
+    ptr(w.islandMinSep); ptr(w.bworld); ptr(w.jointPairs); put(w.jointPairMask);
+    ptr(w.fbody); ptr(w.shape); ptr(w.fmat); ptr(w.ffilter); ptr(w.fgroup); ptr(w.fat); ptr(w.moved);
+    ptr(w.cfA); ptr(w.cfB); ptr(w.cflags); ptr(w.cmat); ptr(w.cman); ptr(w.ccolour);
+    const ConView& cv = ctx->cv;
+    ptr(cv.contact); ptr(cv.bodies); ptr(cv.mass); ptr(cv.normal); ptr(cv.r0); ptr(cv.r1); ptr(cv.pmass);
+    ptr(cv.bias); ptr(cv.imp); ptr(cv.K); ptr(cv.NM); ptr(cv.info); ptr(cv.lp); ptr(cv.lnp); ptr(cv.lc);
+    ptr(cv.rad); ptr(cv.island);
+    put((unsigned)jv.n); ptr(jv.order); ptr(jv.bodies); ptr(jv.anchors); ptr(jv.limits); ptr(jv.motorSpeed);
+    ptr(jv.flags); ptr(jv.impulse); ptr(jv.r); ptr(jv.masses); ptr(jv.K0); ptr(jv.K1); ptr(jv.motorMass);
+    ptr(ctx->cnt); ptr(ctx->dRanges);
+    unsigned dtBits, ratioBits;
+    memcpy(&dtBits, &dt, 4);
+    memcpy(&ratioBits, &dtRatio, 4);
+    put(dtBits); put(ratioBits); put((unsigned)velIters); put((unsigned)posIters); put(warm ? 1 : 0);
+    put((unsigned)sb); put((unsigned)nColours); put(sizeClass(nCons));
+    for (int c = 0; c < nColours; ++c) {
+        const int n = colourStart[c + 1] - colourStart[c];
+        put(n > 0 ? (unsigned long long)sizeClass(n) : 0ull);
+    }
+    if (ctx->nJoints > 0)
+        for (int c = 0; c <= kColourSlots; ++c) put((unsigned)ctx->jointColourStart[c]);
+
+    using SolveGraph = b2g_context::SolveGraph;
+    SolveGraph* hit = nullptr;
+    for (auto& g : ctx->solveGraphs)
+        if (g.key == key) hit = &g;
+    if (!hit) {
+        bool recent = false;
+        for (auto& k : ctx->recentKeys) recent = recent || k == key;
+        if (!recent) {
+            ctx->recentKeys.push_back(key);
+            if (ctx->recentKeys.size() > 3) ctx->recentKeys.erase(ctx->recentKeys.begin());
+            return B2G_OK;  // launched directly this time
+        }
+    }
+    if (!ctx->dRanges) HIPCHK(ctx, hipMalloc((void**)&ctx->dRanges, sizeof(int) * (kColourSlots + 1)));
+    if (!ctx->hRanges)
+        HIPCHK(ctx, hipHostMalloc((void**)&ctx->hRanges, sizeof(int) * (kColourSlots + 1), hipHostMallocDefault));
+    // the previous step's copy has completed: the step synchronises with the
+    // stream several times before reaching here
+    memcpy(ctx->hRanges, colourStart, sizeof(int) * (kColourSlots + 1));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->dRanges, ctx->hRanges, sizeof(int) * (kColourSlots + 1), hipMemcpyHostToDevice,
+                               s));
+    if (!hit) {
+        // capture the launch sequence
+        auto jointLaunch = [&](auto parallel, auto serial) {
+            for (int c = 0; ctx->nJoints > 0 && c < kColourSlots; ++c) {
+                const int b0 = ctx->jointColourStart[c], b1 = ctx->jointColourStart[c + 1];
+                if (b1 <= b0) continue;
+                if (c == kOverflowColour) serial(b0, b1);
+                else parallel(b0, b1);
+            }
+        };
+        auto colourLaunch = [&](auto parallel, auto serial) {
+            for (int c = 0; c < nColours; ++c) {
+                const int n = colourStart[c + 1] - colourStart[c];
+                if (n <= 0) continue;
+                if (c == kOverflowColour) serial(c, dim3(1));
+                else parallel(c, dim3((unsigned)sizeClass(n)));
+            }
+        };
+        const int* R = ctx->dRanges;
+        HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        if (warm)
+            colourLaunch([&](int c, dim3 g) { hipLaunchKernelGGL(k_warm_start_r, g, dim3(sb), 0, s, w, cv, R, c); },
+                         [&](int c, dim3) { hipLaunchKernelGGL(k_warm_start_serial_r, dim3(1), dim3(1), 0, s, w, cv, R, c); });
+        const int wj = ctx->warmStarting ? 1 : 0;
+        jointLaunch(
+            [&](int b0, int b1) {
+                hipLaunchKernelGGL(k_joint_init, dim3((b1 - b0 + sb - 1) / sb), dim3(sb), 0, s, w, jv, b0, b1, dtRatio, wj);
+            },
+            [&](int b0, int b1) { hipLaunchKernelGGL(k_joint_init_serial, dim3(1), dim3(1), 0, s, w, jv, b0, b1, dtRatio, wj); });
+        for (int it = 0; it < velIters; ++it) {
+            jointLaunch(
+                [&](int b0, int b1) {
+                    hipLaunchKernelGGL(k_joint_solve_velocity, dim3((b1 - b0 + sb - 1) / sb), dim3(sb), 0, s, w, jv, b0,
+                                       b1, dt);
+                },
+                [&](int b0, int b1) {
+                    hipLaunchKernelGGL(k_joint_solve_velocity_serial, dim3(1), dim3(1), 0, s, w, jv, b0, b1, dt);
+                });
+            colourLaunch([&](int c, dim3 g) { hipLaunchKernelGGL(k_solve_velocity_r, g, dim3(sb), 0, s, w, cv, R, c); },
+                         [&](int c, dim3) { hipLaunchKernelGGL(k_solve_velocity_serial_r, dim3(1), dim3(1), 0, s, w, cv, R, c); });
+        }
+        if (nCons > 0)
+            hipLaunchKernelGGL(k_store_impulses_r, dim3((unsigned)sizeClass(nCons)), dim3(sb), 0, s, w, cv, R);
+        hipLaunchKernelGGL(k_integrate_positions, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, dt);
+        for (int it = 0; it < posIters && positions; ++it) {
+            colourLaunch(
+                [&](int c, dim3 g) { hipLaunchKernelGGL(k_solve_position_r, g, dim3(sb), 0, s, w, cv, R, c, ctx->cnt, it); },
+                [&](int c, dim3) {
+                    hipLaunchKernelGGL(k_solve_position_serial_r, dim3(1), dim3(1), 0, s, w, cv, R, c, ctx->cnt, it);
+                });
+            jointLaunch(
+                [&](int b0, int b1) {
+                    hipLaunchKernelGGL(k_joint_solve_position, dim3((b1 - b0 + sb - 1) / sb), dim3(sb), 0, s, w, jv, b0,
+                                       b1, it);
+                },
+                [&](int b0, int b1) {
+                    hipLaunchKernelGGL(k_joint_solve_position_serial, dim3(1), dim3(1), 0, s, w, jv, b0, b1, it);
+                });
+        }
+        hipLaunchKernelGGL(k_sync_transforms, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w);
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(s, &graph);
+        const hipError_t le = hipGetLastError();
+        if (ce != hipSuccess || le != hipSuccess || !graph) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(ctx, B2G_ERR_HIP, std::string("solve capture: ") + hipGetErrorString(ce != hipSuccess ? ce : le));
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(ctx, ie);
+        if (ctx->solveGraphs.size() >= 6) {
+            auto lru = std::min_element(ctx->solveGraphs.begin(), ctx->solveGraphs.end(),
+                                        [](const SolveGraph& a, const SolveGraph& b) { return a.used < b.used; });
+            (void)hipGraphExecDestroy(lru->exec);
+            ctx->solveGraphs.erase(lru);
+        }
+        ctx->solveGraphs.push_back(SolveGraph{key, exec, 0});
+        hit = &ctx->solveGraphs.back();
+        ++ctx->graphCaptures;
+    }
+    hit->used = ++ctx->graphClock;
+    HIPCHK(ctx, hipGraphLaunch(hit->exec, s));
+    ++ctx->graphLaunches;
+    *ran = true;
+    return B2G_OK;
+}
+
 // Island labels (each body labelled with the smallest body index of its
 // island) by the schedule's method; all three give the same labels.
 //  0: propagation rounds (two per host check of the change flag) until
@@ -3263,6 +3502,15 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
                                        hipMemcpyDeviceToDevice, s));
             return B2G_OK;
         };
+        // The plain solve (no learned start, trace, log or profiling) runs as
+        // a captured graph once its layout has repeated.
+        bool graphRan = false;
+        if (ctx->graphs && !ctx->profiling && !learned && ctx->traceIters == 0 && ctx->logCap == 0) {
+            rc = solveByGraph(ctx, w, jv, colourStart, nColours, nCons, dt, dtRatio, p->velocityIterations, posIters,
+                              &graphRan);
+            if (rc != B2G_OK) return rc;
+        }
+        if (!graphRan) {
         if (nCons > 0 && ctx->traceIters > 0) {
             if ((rc = saveJoints()) != B2G_OK) return rc;
             auto traceStart = [&](bool contactWarm) {
@@ -3339,8 +3587,9 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
                     hipLaunchKernelGGL(k_joint_solve_position, sgrid(b1 - b0), dim3(sb), 0, s, w, jv, b0, b1, it);
             });
         }
+        }  // !graphRan
         pe = profStart(ctx);
-        hipLaunchKernelGGL(k_sync_transforms, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w);
+        if (!graphRan) hipLaunchKernelGGL(k_sync_transforms, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w);
         if (ctx->allowSleep) {
             hipLaunchKernelGGL(k_sleep_accum, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, dt);
             hipLaunchKernelGGL(k_sleep_apply, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, posIters);
@@ -3589,6 +3838,11 @@ int b2g_destroy(b2g_context* ctx) {
     freeArray(ctx->gsleep); freeArray(ctx->colourMask); freeArray(ctx->claim); freeArray(ctx->islandOf);
     freeArray(ctx->bworld);
     freeArray(ctx->islandAwake); freeArray(ctx->islandSleep); freeArray(ctx->islandMinSep);
+    for (auto& g : ctx->solveGraphs) (void)hipGraphExecDestroy(g.exec);
+    ctx->solveGraphs.clear();
+    freeArray(ctx->dRanges);
+    if (ctx->hRanges) (void)hipHostFree(ctx->hRanges);
+    ctx->hRanges = nullptr;
     freeArray(ctx->fbody); freeArray(ctx->shape); freeArray(ctx->fmat); freeArray(ctx->ffilter);
     freeArray(ctx->fgroup); freeArray(ctx->fat); freeArray(ctx->moved); freeArray(ctx->range);
     for (ContactArrays* a : {&ctx->cur, &ctx->alt}) {
@@ -4603,6 +4857,19 @@ int b2g_abi_sizes(int32_t* out, int cap) {
 }  // extern "C"
 
 extern "C" {
+
+int b2g_set_graphs(b2g_context* ctx, int on) {
+    if (!ctx) return B2G_ERR_ARG;
+    ctx->graphs = on != 0;
+    return B2G_OK;
+}
+
+int b2g_get_graph_counts(b2g_context* ctx, int64_t* launches, int64_t* captures) {
+    if (!ctx) return B2G_ERR_ARG;
+    if (launches) *launches = ctx->graphLaunches;
+    if (captures) *captures = ctx->graphCaptures;
+    return B2G_OK;
+}
 
 int b2g_set_schedule(b2g_context* ctx, const b2g_schedule* sc) {
     if (!ctx || !sc) return B2G_ERR_ARG;

@@ -79,3 +79,34 @@ def test_set_schedule_rejects_bad_values(gpu):
         s = np.array([bad], E.SCHEDULE)
         assert w.e.lib.b2pp_world_set_schedule(w.h, E.ptr(s)) != 0
     w.close()
+
+
+@pytest.mark.parametrize("scene", ["stress", "plank", "pendulums"])
+def test_graph_solve_matches_direct(gpu, scene):
+    """The solve captured as a hipGraph (replayed while its layout repeats)
+    and the directly launched solve give the same steps bit for bit: a
+    stress context (piles, edges, jointed chains, sleep), a plank whose
+    contacts overflow into the serial colour, and pendulums (joints only)."""
+    lib = E.b2g_lib()
+
+    def make():
+        if scene == "stress":
+            return _world(gpu)
+        if scene == "plank":
+            return gpu.world("plank", 100, 0, sleep=True)
+        return gpu.world("pendulums", 24, 0, sleep=True)
+
+    a, b = make(), make()
+    ca, cb = a.device_context(), b.device_context()
+    E.b2g_check(lib, ca, lib.b2g_set_graphs(ca, 1))
+    E.b2g_check(lib, cb, lib.b2g_set_graphs(cb, 0))
+    for _ in range(5):
+        a.step(40)
+        b.step(40)
+        _same(a, b)
+    launches, captures = ctypes.c_int64(), ctypes.c_int64()
+    E.b2g_check(lib, ca, lib.b2g_get_graph_counts(ca, ctypes.byref(launches), ctypes.byref(captures)))
+    assert launches.value > 50, (launches.value, captures.value)
+    E.b2g_check(lib, cb, lib.b2g_get_graph_counts(cb, ctypes.byref(launches), None))
+    assert launches.value == 0
+    a.close(); b.close()
