@@ -2363,7 +2363,7 @@ struct b2g_context {
         hipGraphExec_t exec = nullptr;
         long long used = 0;
     };
-    bool graphs = true;
+    bool graphs = false;
     std::vector<SolveGraph> solveGraphs;
     std::vector<std::vector<unsigned long long>> recentKeys;  // keys of the last steps not captured
     long long graphClock = 0;

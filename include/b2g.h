@@ -436,12 +436,14 @@ int b2g_get_schedule(b2g_context* ctx, b2g_schedule* s);
  * world would have had anyway. (3 + 2 + 4 + 3 + 3) * rounds steps. */
 int b2g_autotune(b2g_context* ctx, const b2g_step_params* p, int rounds, b2g_schedule* out);
 
-/* Captured solves (default on): the plain coloured solve of a step (warm
+/* Captured solves (default off): the plain coloured solve of a step (warm
  * start, velocity passes, store, position integration, position passes,
  * transform sync) runs as one hipGraph launch once its layout has repeated
  * (views, iterations, time step, non-empty colours and their size classes);
  * steps with the learned start, a trace, a log or profiling launch
- * directly. Results are the same either way. No reference counterpart. */
+ * directly. Results are the same either way. On a settling 100k-body pile
+ * the layout changes often enough that capturing costs more than replaying
+ * saves, so direct launches are the default. No reference counterpart. */
 int b2g_set_graphs(b2g_context* ctx, int on);
 /* Graph launches and captures since the context was created. */
 int b2g_get_graph_counts(b2g_context* ctx, int64_t* launches, int64_t* captures);
