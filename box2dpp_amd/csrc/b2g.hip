@@ -215,11 +215,23 @@ __global__ void k_collide_keys(WorldView w, uint32_t* keys, int* vals) {
     vals[c] = c;
 }
 
+// PreSolve's edits (b2Contact::SetEnabled, SetFriction, SetRestitution,
+// REF/Dynamics/Contacts/b2Contact.h:98-131): the enabled flag holds for this
+// step (Update re-enables every step), friction and restitution stay.
+__global__ void k_override_contacts(WorldView w, const int* ids, const int* enabled, const float2* mat, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = ids[i];
+    if (c < 0 || c >= w.nContacts) return;
+    w.cflags[c] = enabled[i] ? (w.cflags[c] | CF_ENABLED) : (w.cflags[c] & ~CF_ENABLED);
+    w.cmat[c] = mat[i];
+}
+
 __global__ void k_collide(WorldView w, Counters* cnt, const int* order) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= w.nContacts) return;
     const int c = order ? order[i] : i;
-    int flags = w.cflags[c];
+    int flags = w.cflags[c] & ~CF_UPDATED;
     const int fA = w.cfA[c], fB = w.cfB[c];
     const int bA = w.fbody[fA], bB = w.fbody[fB];
     if (flags & CF_FILTER) {
@@ -274,7 +286,7 @@ __global__ void k_collide(WorldView w, Counters* cnt, const int* order) {
         wakeBody(w, bA);
         wakeBody(w, bB);
     }
-    flags |= CF_ENABLED;
+    flags |= CF_ENABLED | CF_UPDATED;
     if (touching) flags |= CF_TOUCHING;
     else flags &= ~CF_TOUCHING;
     if (!wasTouching && touching) atomicAdd(&cnt->beginEvents, 1);
@@ -1553,6 +1565,83 @@ __global__ void k_sleep_accum(WorldView w, float h) {
     }
 }
 
+// Static bodies reached by the island search (REF/Dynamics/b2World.cpp:
+// 451-529): the search wakes every body it adds, static ones included
+// (without searching on from them), and an island that falls asleep puts all
+// its bodies to sleep (REF/Dynamics/b2Island.cpp:366-382). Islands are solved
+// in the order of their seeds, the body list from its newest body, so by
+// decreasing largest body index; a static body reached by several islands
+// keeps the state that the island of the smallest largest index leaves.
+// islandMax holds each island's largest body index (one atomic per
+// wavefront when the wavefront's bodies share an island); staticKey the
+// minimum over the solved islands reaching a static body of
+// (largest index << 1) | (island stays awake).
+__global__ void k_island_max(WorldView w, int* islandMax) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = b < w.nBodies && islandBody(w, b);
+    const int root = valid ? w.islandOf[b] : -1;
+    const int r0 = __shfl(root, 0);
+    const unsigned long long same = __ballot(root == r0);
+    if (r0 >= 0 && same == ~0ull) {
+        // the wavefront's bodies all belong to island r0: its last lane has the largest index
+        if (__lane_id() == 63) atomicMax(&islandMax[r0], b);
+        return;
+    }
+    if (valid) atomicMax(&islandMax[root], b);
+}
+
+__device__ inline int staticReachKey(const WorldView& w, const int* islandMax, int root, int allowSleep,
+                                     int posIters) {
+    bool solved = false;
+    for (int it = 0; it < posIters; ++it)
+        if (w.islandMinSep[it * w.nBodies + root] == 0) solved = true;
+    const bool slept = allowSleep && w.islandSleep[root] == 0 && solved;
+    return (islandMax[root] << 1) | (slept ? 0 : 1);
+}
+
+__global__ void k_static_reach(WorldView w, const int* islandMax, int* staticKey, int allowSleep, int posIters) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts || !(w.cflags[c] & CF_CONSTRAINT)) return;
+    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
+    const bool sA = w.btype[bA] == BT_STATIC, sB = w.btype[bB] == BT_STATIC;
+    if (sA == sB) return;
+    const int st = sA ? bA : bB, dyn = sA ? bB : bA;
+    atomicMin(&staticKey[st], staticReachKey(w, islandMax, w.islandOf[dyn], allowSleep, posIters));
+}
+
+__global__ void k_static_reach_joints(WorldView w, JointView jv, const int* islandMax, int* staticKey,
+                                      int allowSleep, int posIters) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= jv.n || (jv.flags[j] & JF_REMOVED)) return;
+    const int2 b = jv.bodies[j];
+    if (!jointActive(w, b)) return;
+    const bool sA = w.btype[b.x] == BT_STATIC, sB = w.btype[b.y] == BT_STATIC;
+    if (sA == sB) return;
+    const int st = sA ? b.x : b.y, dyn = sA ? b.y : b.x;
+    if (!islandBody(w, dyn)) return;
+    atomicMin(&staticKey[st], staticReachKey(w, islandMax, w.islandOf[dyn], allowSleep, posIters));
+}
+
+// b2Body::SetAwake (REF/Dynamics/b2Body.h:635-654) on the reached static bodies.
+__global__ void k_static_apply(WorldView w, int* staticKey) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies) return;
+    const int key = staticKey[b];
+    if (key == 0x7FFFFFFF) return;
+    staticKey[b] = 0x7FFFFFFF;
+    if (key & 1) {
+        if (!(w.bflags[b] & BF_AWAKE)) {
+            w.bflags[b] |= BF_AWAKE;
+            w.gsleep[b].y = 0.0f;
+        }
+    } else {
+        w.bflags[b] &= ~BF_AWAKE;
+        w.gsleep[b].y = 0.0f;
+        w.vel[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        w.force[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
 __global__ void k_sleep_apply(WorldView w, int posIters) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= w.nBodies || !islandBody(w, b)) return;
@@ -2260,6 +2349,9 @@ struct b2g_context {
     int* islandOf = nullptr;
     int* islandAwake = nullptr;
     int* islandSleep = nullptr;
+    int* islandMax = nullptr;   // largest body index per island root
+    int* staticKey = nullptr;   // per static body: see k_static_reach
+    int staticKeyCap = 0;
     int* islandMinSep = nullptr;
     int islandIterCap = 0;
     int* bworld = nullptr;          // sub-world of each body (batched worlds)
@@ -2369,6 +2461,12 @@ struct b2g_context {
     long long graphClock = 0;
     int* dRanges = nullptr;  // colour ranges of the step (kColourSlots + 1 ints)
     int* hRanges = nullptr;  // pinned host staging of dRanges
+    // split step (b2g_step_collide / b2g_step_solve) for the host's PreSolve
+    bool keepOldManifolds = false;
+    b2g_manifold* oldMan = nullptr;
+    int oldManCap = 0;
+    bool stepOpen = false;
+    b2g_step_params openParams{};
     long long graphLaunches = 0, graphCaptures = 0;
     int logStride = 1;      // scope 1 logs every logStride-th step
     long long logSteps = 0;
@@ -2484,6 +2582,16 @@ int growBodies(b2g_context* ctx, int need) {
     HIPCHK(ctx, growArray(ctx->islandOf, n, cap, s));
     HIPCHK(ctx, growArray(ctx->islandAwake, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->islandSleep, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->islandMax, 0, cap, s));
+    {
+        // staticKey starts "not reached" and k_static_apply restores that after use
+        const bool fresh = ctx->staticKeyCap < cap;
+        HIPCHK(ctx, growArray(ctx->staticKey, 0, cap, s));
+        if (fresh) {
+            hipLaunchKernelGGL(k_fill<int>, dim3(blocks(cap)), dim3(256), 0, s, ctx->staticKey, cap, 0x7FFFFFFF);
+            ctx->staticKeyCap = cap;
+        }
+    }
     HIPCHK(ctx, growArray(ctx->islandMinSep, 0, cap * std::max(ctx->islandIterCap, 4), s));
     ctx->islandIterCap = std::max(ctx->islandIterCap, 4);
     // claims start at zero (tag 0 never wins)
@@ -2897,6 +3005,14 @@ int rebuildJointPairs(b2g_context* ctx) {
 }
 
 // ---------------------------------------------------------------- learned warm start
+int ensureOldManifolds(b2g_context* ctx) {
+    if (ctx->oldManCap >= ctx->capContacts) return B2G_OK;
+    freeArray(ctx->oldMan);
+    HIPCHK(ctx, hipMalloc((void**)&ctx->oldMan, sizeof(b2g_manifold) * (size_t)ctx->capContacts));
+    ctx->oldManCap = ctx->capContacts;
+    return B2G_OK;
+}
+
 int ensureFresh(b2g_context* ctx) {
     const int cap = ctx->capContacts;
     if (cap <= ctx->capFresh) return B2G_OK;
@@ -3248,11 +3364,15 @@ int labelIslands(b2g_context* ctx, const WorldView& w) {
     return B2G_OK;
 }
 
-int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
+// First half of a coloured step: counters, pending broad-phase, and the
+// narrowphase over the contact table (b2ContactManager::Collide's Update of
+// every contact, REF/Dynamics/b2ContactManager.cpp:107-175). The table still
+// holds the contacts that Collide destroys (flagged dead) until the second
+// half compacts it. With ctx->keepOldManifolds the manifolds from before the
+// update are kept for PreSolve.
+int stepColouredCollide(b2g_context* ctx, const b2g_step_params* p) {
+    (void)p;
     hipStream_t s = ctx->stream;
-    const float dt = p->dt;
-    const float inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
-    const float dtRatio = ctx->inv_dt0 * dt;
     int rc;
     if ((rc = reserveContacts(ctx)) != B2G_OK) return rc;
     memset(&ctx->stats, 0, sizeof(ctx->stats));
@@ -3275,6 +3395,11 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
     }
     // Collide: narrowphase, destruction of non-overlapping contacts.
     if (ctx->nContacts > 0) {
+        if (ctx->keepOldManifolds) {
+            if ((rc = ensureOldManifolds(ctx)) != B2G_OK) return rc;
+            HIPCHK(ctx, hipMemcpyAsync(ctx->oldMan, ctx->cur.cman, sizeof(b2g_manifold) * ctx->nContacts,
+                                       hipMemcpyDeviceToDevice, s));
+        }
         int pe = profStart(ctx);
         const int* order = nullptr;
         if (ctx->sched.collideOrder == 1) {
@@ -3289,7 +3414,20 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
         hipLaunchKernelGGL(k_collide, dim3((ctx->nContacts + cb - 1) / cb), dim3(cb), 0, s, ctx->view(), ctx->cnt,
                            order);
         profEnd(ctx, PH_COLLIDE, pe);
-        pe = profStart(ctx);
+        LAUNCHCHK(ctx);
+    }
+    return B2G_OK;
+}
+
+// Second half: compaction, islands, colouring, the solve, broad-phase.
+int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
+    hipStream_t s = ctx->stream;
+    const float dt = p->dt;
+    const float inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
+    const float dtRatio = ctx->inv_dt0 * dt;
+    int rc;
+    if (ctx->nContacts > 0) {
+        int pe = profStart(ctx);
         hipLaunchKernelGGL(k_mark_dead, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), ctx->keep);
         LAUNCHCHK(ctx);
         HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->keep, ctx->idx, ctx->nContacts, s));
@@ -3592,6 +3730,22 @@ int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
         if (!graphRan) hipLaunchKernelGGL(k_sync_transforms, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w);
         if (ctx->allowSleep) {
             hipLaunchKernelGGL(k_sleep_accum, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, dt);
+        }
+        if (ctx->nContacts > 0 || ctx->nJoints > 0) {
+            // static bodies reached by this step's islands (before the sleep
+            // below changes the island flags the key reads)
+            hipLaunchKernelGGL(k_fill<int>, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, ctx->islandMax, ctx->nBodies,
+                               -1);
+            hipLaunchKernelGGL(k_island_max, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, ctx->islandMax);
+            if (ctx->nContacts > 0)
+                hipLaunchKernelGGL(k_static_reach, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->islandMax,
+                                   ctx->staticKey, ctx->allowSleep ? 1 : 0, posIters);
+            if (ctx->nJoints > 0)
+                hipLaunchKernelGGL(k_static_reach_joints, dim3(blocks(ctx->nJoints)), dim3(256), 0, s, w, jv,
+                                   ctx->islandMax, ctx->staticKey, ctx->allowSleep ? 1 : 0, posIters);
+            hipLaunchKernelGGL(k_static_apply, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, ctx->staticKey);
+        }
+        if (ctx->allowSleep) {
             hipLaunchKernelGGL(k_sleep_apply, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, posIters);
         }
         hipLaunchKernelGGL(k_sync_fixtures, dim3(blocks(ctx->nFixtures)), dim3(256), 0, s, w);
@@ -3674,6 +3828,11 @@ int ensureRefBuffers(b2g_context* ctx) {
 // b2World::Step in the reference's visiting order (see the reference-order
 // kernels above). Same collide/broad-phase as the coloured path; islands
 // and constraint order follow Box2D exactly.
+int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
+    const int rc = stepColouredCollide(ctx, p);
+    return rc != B2G_OK ? rc : stepColouredSolve(ctx, p);
+}
+
 int stepReference(b2g_context* ctx, const b2g_step_params* p) {
     hipStream_t s = ctx->stream;
     const float dt = p->dt;
@@ -3838,9 +3997,11 @@ int b2g_destroy(b2g_context* ctx) {
     freeArray(ctx->gsleep); freeArray(ctx->colourMask); freeArray(ctx->claim); freeArray(ctx->islandOf);
     freeArray(ctx->bworld);
     freeArray(ctx->islandAwake); freeArray(ctx->islandSleep); freeArray(ctx->islandMinSep);
+    freeArray(ctx->islandMax); freeArray(ctx->staticKey);
     for (auto& g : ctx->solveGraphs) (void)hipGraphExecDestroy(g.exec);
     ctx->solveGraphs.clear();
     freeArray(ctx->dRanges);
+    freeArray(ctx->oldMan);
     if (ctx->hRanges) (void)hipHostFree(ctx->hRanges);
     ctx->hRanges = nullptr;
     freeArray(ctx->fbody); freeArray(ctx->shape); freeArray(ctx->fmat); freeArray(ctx->ffilter);
@@ -4101,8 +4262,95 @@ int b2g_step(b2g_context* ctx, const b2g_step_params* p) {
         return fail(ctx, B2G_ERR_UNSUPPORTED, "the learned warm start runs with the coloured solver");
     if (ctx->nJoints > 0 && p->mode == B2G_MODE_REFERENCE)
         return fail(ctx, B2G_ERR_UNSUPPORTED, "joints run with the coloured solver");
+    if (ctx->stepOpen) return fail(ctx, B2G_ERR_STATE, "a split step is open: finish it with b2g_step_solve");
     (void)hipSetDevice(ctx->device);
     return p->mode == B2G_MODE_REFERENCE ? stepReference(ctx, p) : stepColoured(ctx, p);
+}
+
+int b2g_get_contact_solved(b2g_context* ctx, int32_t* out, int cap) {
+    if (!ctx || cap < 0 || (cap > 0 && !out)) return B2G_ERR_ARG;
+    const int m = std::min(ctx->nContacts, cap);
+    if (m == 0) return B2G_OK;
+    HIPCHK(ctx, hipMemcpyAsync(out, ctx->cur.cflags, sizeof(int) * m, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < m; ++i) out[i] = (out[i] & CF_CONSTRAINT) ? 1 : 0;
+    return B2G_OK;
+}
+
+int b2g_get_contact_materials(b2g_context* ctx, float* out, int cap) {
+    if (!ctx || cap < 0 || (cap > 0 && !out)) return B2G_ERR_ARG;
+    const int m = std::min(ctx->nContacts, cap);
+    if (m == 0) return B2G_OK;
+    HIPCHK(ctx, hipMemcpyAsync(out, ctx->cur.cmat, sizeof(float2) * m, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
+int b2g_step_collide(b2g_context* ctx, const b2g_step_params* p) {
+    if (!ctx || !p) return B2G_ERR_ARG;
+    if (ctx->stepOpen) return fail(ctx, B2G_ERR_STATE, "a split step is already open");
+    if (p->velocityIterations < 0 || p->positionIterations < 0) return fail(ctx, B2G_ERR_ARG, "iterations");
+    if (p->mode != B2G_MODE_COLOURED)
+        return fail(ctx, B2G_ERR_UNSUPPORTED, "the split step runs with the coloured solver");
+    (void)hipSetDevice(ctx->device);
+    ctx->keepOldManifolds = true;
+    const int rc = stepColouredCollide(ctx, p);
+    ctx->keepOldManifolds = false;
+    if (rc != B2G_OK) return rc;
+    ctx->openParams = *p;
+    ctx->stepOpen = true;
+    return B2G_OK;
+}
+
+int b2g_get_old_manifolds(b2g_context* ctx, b2g_manifold* out, int cap, int* count) {
+    if (!ctx || cap < 0 || (cap > 0 && !out)) return B2G_ERR_ARG;
+    if (!ctx->stepOpen) return fail(ctx, B2G_ERR_STATE, "no split step is open");
+    const int n = ctx->nContacts;
+    if (count) *count = n;
+    const int m = std::min(n, cap);
+    if (m == 0) return B2G_OK;
+    HIPCHK(ctx, hipMemcpyAsync(out, ctx->oldMan, sizeof(b2g_manifold) * m, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
+int b2g_override_contacts(b2g_context* ctx, const int32_t* ids, const int32_t* enabled, const float* friction,
+                          const float* restitution, int n) {
+    if (!ctx || n < 0 || (n > 0 && (!ids || !enabled || !friction || !restitution))) return B2G_ERR_ARG;
+    if (!ctx->stepOpen) return fail(ctx, B2G_ERR_STATE, "no split step is open");
+    for (int i = 0; i < n; ++i)
+        if (ids[i] < 0 || ids[i] >= ctx->nContacts) return fail(ctx, B2G_ERR_ARG, "contact index out of range");
+    if (n == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    std::vector<float2> mat(n);
+    for (int i = 0; i < n; ++i) mat[i] = make_float2(friction[i], restitution[i]);
+    int* dIds = nullptr;
+    int* dEn = nullptr;
+    float2* dMat = nullptr;
+    hipError_t e = hipMalloc((void**)&dIds, sizeof(int) * n);
+    if (e == hipSuccess) e = hipMalloc((void**)&dEn, sizeof(int) * n);
+    if (e == hipSuccess) e = hipMalloc((void**)&dMat, sizeof(float2) * n);
+    if (e == hipSuccess) e = hipMemcpyAsync(dIds, ids, sizeof(int) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dEn, enabled, sizeof(int) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dMat, mat.data(), sizeof(float2) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_override_contacts, dim3(blocks(n)), dim3(256), 0, s, ctx->view(), dIds, dEn, dMat, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(dIds);
+    (void)hipFree(dEn);
+    (void)hipFree(dMat);
+    HIPCHK(ctx, e);
+    return B2G_OK;
+}
+
+int b2g_step_solve(b2g_context* ctx) {
+    if (!ctx) return B2G_ERR_ARG;
+    if (!ctx->stepOpen) return fail(ctx, B2G_ERR_STATE, "no split step is open");
+    ctx->stepOpen = false;
+    (void)hipSetDevice(ctx->device);
+    return stepColouredSolve(ctx, &ctx->openParams);
 }
 
 int b2g_sync(b2g_context* ctx) {
@@ -4162,6 +4410,7 @@ int b2g_get_contacts(b2g_context* ctx, int32_t* meta, b2g_manifold* manifolds, i
             meta[4 * i + 0] = fa[i];
             meta[4 * i + 1] = fb[i];
             meta[4 * i + 2] = ((fl[i] & CF_TOUCHING) ? 1 : 0) | ((fl[i] & CF_ENABLED) ? 2 : 0);
+            if (ctx->stepOpen) meta[4 * i + 2] |= ((fl[i] & CF_DEAD) ? 4 : 0) | ((fl[i] & CF_UPDATED) ? 8 : 0);
             meta[4 * i + 3] = col[i];
         }
     }

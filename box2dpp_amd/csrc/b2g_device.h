@@ -45,6 +45,7 @@ enum : int {
     CF_DEAD = 0x0100,        // destroyed this step, removed by compaction
     CF_CONSTRAINT = 0x0200,  // solved this step (touching, enabled, not a sensor, awake)
     CF_NOOVERLAP = 0x0400,   // fat AABBs stopped overlapping
+    CF_UPDATED = 0x0800,     // b2Contact::Update ran on it this step
 };
 enum : int { BT_STATIC = 0, BT_KINEMATIC = 1, BT_DYNAMIC = 2 };
 

@@ -13,11 +13,44 @@ using namespace box2d;
 
 namespace {
 
+// Records the listener calls (kind 1 BeginContact, 2 EndContact, 3 PreSolve,
+// 4 PostSolve; fixture indices; for PreSolve whether the contact stays
+// enabled) and disables the contacts of a one-sided platform fixture while
+// the other body's origin is less than `clear` above the platform's top (a
+// one-sided platform after the reference testbed's OneSidedPlatform).
+struct Recorder : public b2ContactListener {
+    const std::vector<b2Fixture*>* fixtures = nullptr;
+    int platform = -1;
+    float top = 0.0f, clear = 0.0f;
+    int step = 0;
+    std::vector<int32_t> events;  // step, kind, fixture A, fixture B, flag
+    int indexOf(const b2Fixture* f) const {
+        for (size_t i = 0; i < fixtures->size(); ++i)
+            if ((*fixtures)[i] == f) return (int)i;
+        return -1;
+    }
+    void record(int kind, b2Contact* c, int flag) {
+        events.insert(events.end(), {step, kind, indexOf(c->GetFixtureA()), indexOf(c->GetFixtureB()), flag});
+    }
+    void BeginContact(b2Contact* c) override { record(1, c, 0); }
+    void EndContact(b2Contact* c) override { record(2, c, 0); }
+    void PreSolve(b2Contact* c, const b2Manifold*) override {
+        const int a = indexOf(c->GetFixtureA()), b = indexOf(c->GetFixtureB());
+        if (a == platform || b == platform) {
+            const b2Body* other = (a == platform ? c->GetFixtureB() : c->GetFixtureA())->GetBody();
+            if (other->GetPosition()[1] < top + clear) c->SetEnabled(false);
+        }
+        record(3, c, c->IsEnabled() ? 1 : 0);
+    }
+    void PostSolve(b2Contact* c, const b2ContactImpulse*) override { record(4, c, 0); }
+};
+
 struct Runner {
     b2World world;
     std::vector<b2Body*> bodies;
     std::vector<b2Fixture*> fixtures;
     std::vector<b2RevoluteJoint*> joints;
+    Recorder recorder;
     explicit Runner(float gx, float gy) : world(b2Vec2(gx, gy)) {}
 };
 
@@ -241,9 +274,30 @@ int b2pp_world_step(void* h, float dt, int vel, int pos, int nsteps) {
     Runner* r = (Runner*)h;
     for (int i = 0; i < nsteps; ++i) {
         r->world.Step(dt, vel, pos);
+        ++r->recorder.step;
         if (r->world.GetLastStatus() != 0) return r->world.GetLastStatus();
     }
     return 0;
+}
+
+// Installs the recording listener with the one-sided platform rule for
+// fixture `platform` (its top and the clearance as given); -1: record only.
+void b2pp_world_set_recorder(void* h, int platform, float top, float clear) {
+    Runner* r = (Runner*)h;
+    r->recorder.fixtures = &r->fixtures;
+    r->recorder.platform = platform;
+    r->recorder.top = top;
+    r->recorder.clear = clear;
+    r->world.SetContactListener(&r->recorder);
+}
+
+// Recorded events, 5 ints each; returns the event count.
+int b2pp_world_events(void* h, int32_t* out, int cap) {
+    Runner* r = (Runner*)h;
+    const int n = (int)r->recorder.events.size() / 5;
+    for (int i = 0; i < n && i < cap; ++i)
+        for (int k = 0; k < 5; ++k) out[5 * i + k] = r->recorder.events[5 * i + k];
+    return n;
 }
 
 // Autotunes the world's step schedule on 15 * rounds steps; *sched (5

@@ -839,7 +839,13 @@ void b2World::Step(float dt, int32_t velocityIterations, int32_t positionIterati
         p.mode = m_options.mode;
         p.learnedWarmStart = m_options.learnedWarmStart ? 1 : 0;
         p.residualThreshold = m_options.residualThreshold;
-        rc = b2g_step(m_ctx, &p);
+        if (m_contactListener && p.mode == B2G_MODE_COLOURED) {
+            // Begin/End/PreSolve inside the step, as Collide calls them
+            rc = CollideWithListener(&p);
+            if (rc == B2G_OK) rc = b2g_step_solve(m_ctx);
+        } else {
+            rc = b2g_step(m_ctx, &p);
+        }
     }
     m_locked = false;
     if (rc != B2G_OK) {
@@ -853,7 +859,7 @@ void b2World::Step(float dt, int32_t velocityIterations, int32_t positionIterati
     b2g_step_stats st{};
     b2g_get_stats(m_ctx, &st);
     m_contactCount = st.contacts;
-    if (m_contactListener) ReplayEvents();
+    if (m_contactListener) ReplayEvents(m_options.mode == B2G_MODE_COLOURED);
     const auto t1 = std::chrono::steady_clock::now();
     m_profile.step = std::chrono::duration<float, std::milli>(t1 - t0).count();
 }
@@ -894,6 +900,8 @@ b2Contact* b2World::GetContactList() {
         std::vector<int32_t> meta(4 * (size_t)n);
         std::vector<b2g_manifold> man(n);
         b2g_get_contacts(m_ctx, meta.data(), man.data(), n, &n);
+        std::vector<float> mat(2 * (size_t)n);
+        const bool haveMat = n > 0 && b2g_get_contact_materials(m_ctx, mat.data(), n) == B2G_OK;
         m_contacts.assign(n, b2Contact());
         // Newest contacts first, as the reference's contact list.
         for (int i = 0; i < n; ++i) {
@@ -903,8 +911,9 @@ b2Contact* b2World::GetContactList() {
             c.m_fixtureB = m_fixtures[meta[4 * src + 1]];
             c.m_touching = (meta[4 * src + 2] & 1) != 0;
             c.m_enabled = (meta[4 * src + 2] & 2) != 0;
-            c.m_friction = sqrtf(c.m_fixtureA->m_friction * c.m_fixtureB->m_friction);
-            c.m_restitution = std::max(c.m_fixtureA->m_restitution, c.m_fixtureB->m_restitution);
+            c.m_friction = haveMat ? mat[2 * src] : sqrtf(c.m_fixtureA->m_friction * c.m_fixtureB->m_friction);
+            c.m_restitution = haveMat ? mat[2 * src + 1]
+                                      : std::max(c.m_fixtureA->m_restitution, c.m_fixtureB->m_restitution);
             const b2g_manifold& m = man[src];
             c.m_manifold.pointCount = m.pointCount;
             c.m_manifold.type = (b2Manifold::Type)m.type;
@@ -923,6 +932,12 @@ b2Contact* b2World::GetContactList() {
     return m_contacts.empty() ? nullptr : &m_contacts[0];
 }
 
+// b2MixFriction, b2MixRestitution (REF/Dynamics/Contacts/b2Contact.h:30-41)
+void b2Contact::ResetFriction() { m_friction = sqrtf(m_fixtureA->GetFriction() * m_fixtureB->GetFriction()); }
+void b2Contact::ResetRestitution() {
+    m_restitution = std::max(m_fixtureA->GetRestitution(), m_fixtureB->GetRestitution());
+}
+
 void b2Contact::GetWorldManifold(b2WorldManifold* wm) const {
     const b2Body* bA = m_fixtureA->GetBody();
     const b2Body* bB = m_fixtureB->GetBody();
@@ -933,8 +948,88 @@ void b2Contact::GetWorldManifold(b2WorldManifold* wm) const {
 // Contact events after a Step: BeginContact for fixture pairs that started
 // touching, EndContact for pairs that stopped (or whose contact ended), and
 // PostSolve with each touching contact's stored impulses.
-void b2World::ReplayEvents() {
+// b2ContactManager::Collide's listener calls (REF/Dynamics/b2ContactManager.cpp:
+// 110-175): over the contact list, newest first, a contact destroyed by the
+// filter or by fat boxes that stopped overlapping reports EndContact if it was
+// touching (b2ContactManager::Destroy, :72-104); an updated contact reports
+// BeginContact or EndContact when its touching state changed and PreSolve
+// with its old manifold while touching (b2Contact::Update,
+// REF/Dynamics/Contacts/b2Contact.cpp:164-256). PreSolve's edits go back
+// to the device before the solve.
+int32_t b2World::CollideWithListener(const void* params) {
+    const b2g_step_params* p = static_cast<const b2g_step_params*>(params);
+    int rc = b2g_step_collide(m_ctx, p);
+    if (rc != B2G_OK) return rc;
+    m_hostStale = true;  // wakes of the narrowphase
+    int n = 0;
+    b2g_get_contacts(m_ctx, nullptr, nullptr, 0, &n);
+    std::vector<int32_t> meta(4 * (size_t)n);
+    std::vector<b2g_manifold> man(n), old(n);
+    std::vector<float> mat(2 * (size_t)n);
+    if (n > 0) {
+        if ((rc = b2g_get_contacts(m_ctx, meta.data(), man.data(), n, &n)) != B2G_OK) return rc;
+        if ((rc = b2g_get_old_manifolds(m_ctx, old.data(), n, nullptr)) != B2G_OK) return rc;
+        if ((rc = b2g_get_contact_materials(m_ctx, mat.data(), n)) != B2G_OK) return rc;
+    }
+    auto toHost = [](const b2g_manifold& m, b2Manifold& out) {
+        out.pointCount = m.pointCount;
+        out.type = (b2Manifold::Type)m.type;
+        out.localNormal = b2Vec2(m.localNormal.x, m.localNormal.y);
+        out.localPoint = b2Vec2(m.localPoint.x, m.localPoint.y);
+        for (int k = 0; k < 2; ++k) {
+            out.points[k].localPoint = b2Vec2(m.points[k].localPoint.x, m.points[k].localPoint.y);
+            out.points[k].normalImpulse = m.points[k].normalImpulse;
+            out.points[k].tangentImpulse = m.points[k].tangentImpulse;
+            out.points[k].id.key = m.points[k].id;
+        }
+    };
+    std::vector<int32_t> ids, enabled;
+    std::vector<float> fr, re;
+    for (int i = n - 1; i >= 0; --i) {
+        const int flags = meta[4 * i + 2];
+        const bool dead = (flags & 4) != 0, updated = (flags & 8) != 0, touching = (flags & 1) != 0;
+        if (!dead && !updated) continue;
+        b2Contact c;
+        c.m_fixtureA = m_fixtures[meta[4 * i]];
+        c.m_fixtureB = m_fixtures[meta[4 * i + 1]];
+        c.m_friction = mat[2 * i];
+        c.m_restitution = mat[2 * i + 1];
+        c.m_enabled = true;
+        if (dead) {
+            c.m_touching = touching;
+            toHost(old[i], c.m_manifold);
+            if (touching) m_contactListener->EndContact(&c);
+            continue;
+        }
+        const bool wasTouching = old[i].pointCount > 0;
+        c.m_touching = touching;
+        toHost(man[i], c.m_manifold);
+        if (!wasTouching && touching) m_contactListener->BeginContact(&c);
+        if (wasTouching && !touching) m_contactListener->EndContact(&c);
+        if (touching) {
+            b2Manifold oldManifold;
+            toHost(old[i], oldManifold);
+            m_contactListener->PreSolve(&c, &oldManifold);
+            if (!c.m_enabled || c.m_friction != mat[2 * i] || c.m_restitution != mat[2 * i + 1]) {
+                ids.push_back(i);
+                enabled.push_back(c.m_enabled ? 1 : 0);
+                fr.push_back(c.m_friction);
+                re.push_back(c.m_restitution);
+            }
+        }
+    }
+    if (!ids.empty())
+        rc = b2g_override_contacts(m_ctx, ids.data(), enabled.data(), fr.data(), re.data(), (int)ids.size());
+    return rc;
+}
+
+void b2World::ReplayEvents(bool collideEventsDone) {
     b2Contact* list = GetContactList();
+    // PostSolve goes to the contacts the step solved (list order is table
+    // order reversed)
+    const int nc = (int)m_contacts.size();
+    std::vector<int32_t> solved(nc, 0);
+    if (nc > 0) b2g_get_contact_solved(m_ctx, solved.data(), nc);
     std::vector<uint64_t> now;
     for (b2Contact* c = list; c; c = c->GetNext())
         if (c->IsTouching()) now.push_back(pairKeyOf(c->m_fixtureA->m_index, c->m_fixtureB->m_index));
@@ -942,7 +1037,7 @@ void b2World::ReplayEvents() {
     std::unordered_set<uint64_t> before(m_touchingKeys.begin(), m_touchingKeys.end());
     std::unordered_set<uint64_t> after(now.begin(), now.end());
     for (uint64_t k : m_touchingKeys) {
-        if (after.count(k)) continue;
+        if (collideEventsDone || after.count(k)) continue;
         b2Contact ended;
         ended.m_fixtureA = m_fixtures[(int)(k >> 32)];
         ended.m_fixtureB = m_fixtures[(int)(k & 0xFFFFFFFFu)];
@@ -952,7 +1047,8 @@ void b2World::ReplayEvents() {
     for (b2Contact* c = list; c; c = c->GetNext()) {
         if (!c->IsTouching()) continue;
         const uint64_t k = pairKeyOf(c->m_fixtureA->m_index, c->m_fixtureB->m_index);
-        if (!before.count(k)) m_contactListener->BeginContact(c);
+        if (!collideEventsDone && !before.count(k)) m_contactListener->BeginContact(c);
+        if (!solved[nc - 1 - (int)(c - &m_contacts[0])]) continue;  // not solved this step: no PostSolve
         b2ContactImpulse imp{};
         imp.count = c->m_manifold.pointCount;
         for (int j = 0; j < imp.count; ++j) {

@@ -332,11 +332,33 @@ inline Scene plank(int n) {
     return s;
 }
 
+// One-sided platform (after the reference testbed's OneSidedPlatform): a
+// ground edge, a static platform 20 x 0.5 m centred at y = 5, and n boxes
+// 0.5 x 0.5 m (fixed rotation, 1.6 m apart) launched upward at 10.5 m/s
+// from y = 1 under it. A listener that disables the platform's contacts while a
+// box is below its top lets the boxes pass up through it and land on it.
+inline Scene oneway(int n) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-20.0f, 0.0f, 20.0f, 0.0f));
+    addBody(s, defaultBody(0, 0.0f, 5.0f));
+    addFixture(s, boxFixture(10.0f, 0.25f, 0.0f, 0.6f));
+    for (int i = 0; i < n; ++i) {
+        BodyDef b = defaultBody(2, -8.0f + 1.6f * (float)i, 1.0f);
+        b.vy = 10.5f;
+        b.fixedRotation = 1;
+        addBody(s, b);
+        addFixture(s, boxFixture(0.25f, 0.25f, 1.0f, 0.6f));
+    }
+    return s;
+}
+
 // Scene by name: "hello", "pyramid" (n = rows, 0 -> 20), "pile" (n bodies,
 // seed), "world" (one batched-scene world of n bodies, seed), "pendulums"
 // (n pendulums), "chains" (n chains of `seed` links), "stress" (pile of n
 // bodies, seed, on a mat of n / 50 chains of 32 links), "plank" (n circles
-// on a plank). Unknown names give an empty scene.
+// on a plank), "oneway" (n boxes through a one-sided platform). Unknown
+// names give an empty scene.
 inline Scene byName(const char* kind, int n, uint32_t seed) {
     const std::string k(kind ? kind : "");
     if (k == "hello") return hello();
@@ -347,6 +369,7 @@ inline Scene byName(const char* kind, int n, uint32_t seed) {
     if (k == "chains") return chains(n > 0 ? n : 4, seed > 0 ? (int)seed : 32);
     if (k == "stress") return stress(n, seed, n / 50);
     if (k == "plank") return plank(n > 0 ? n : 100);
+    if (k == "oneway") return oneway(n > 0 ? n : 8);
     return Scene();
 }
 

@@ -436,6 +436,33 @@ int b2g_get_schedule(b2g_context* ctx, b2g_schedule* s);
  * world would have had anyway. (3 + 2 + 4 + 3 + 3) * rounds steps. */
 int b2g_autotune(b2g_context* ctx, const b2g_step_params* p, int rounds, b2g_schedule* out);
 
+/* ------------------------------------------------------------- split step
+ * b2g_step in two halves, so that the host can run b2ContactListener's
+ * BeginContact, EndContact and PreSolve inside the step as
+ * b2ContactManager::Collide does (Box2D/Dynamics/b2ContactManager.cpp:110-175,
+ * Contacts/b2Contact.cpp:170-256). Coloured solver only.
+ *   b2g_step_collide: the narrowphase over the contact table. Until
+ *     b2g_step_solve, b2g_get_contacts returns the updated table including
+ *     the contacts Collide destroys, with meta[2] bits 1 touching,
+ *     2 enabled, 4 destroyed this step, 8 updated this step, and
+ *     b2g_get_old_manifolds the manifolds from before the update (in the
+ *     same order);
+ *   b2g_override_contacts: PreSolve's edits per contact index: enabled for
+ *     this step (b2Contact::SetEnabled), friction and restitution (kept);
+ *   b2g_step_solve: the rest of the step. */
+int b2g_step_collide(b2g_context* ctx, const b2g_step_params* p);
+/* 1 for each contact of [0, cap) that the last step solved (an island
+ * contact of an awake island: the contacts b2Island::Report passes to
+ * PostSolve, Box2D/Dynamics/b2Island.cpp:517-540), else 0. */
+int b2g_get_contact_solved(b2g_context* ctx, int32_t* out, int cap);
+/* Friction and restitution of contacts [0, cap), 2 floats each, in the
+ * order of b2g_get_contacts (b2Contact::GetFriction / GetRestitution). */
+int b2g_get_contact_materials(b2g_context* ctx, float* out, int cap);
+int b2g_get_old_manifolds(b2g_context* ctx, b2g_manifold* out, int cap, int* count);
+int b2g_override_contacts(b2g_context* ctx, const int32_t* ids, const int32_t* enabled, const float* friction,
+                          const float* restitution, int n);
+int b2g_step_solve(b2g_context* ctx);
+
 /* Captured solves (default off): the plain coloured solve of a step (warm
  * start, velocity passes, store, position integration, position passes,
  * transform sync) runs as one hipGraph launch once its layout has repeated

@@ -258,6 +258,14 @@ public:
     float GetFriction() const { return m_friction; }
     float GetRestitution() const { return m_restitution; }
     float GetTangentSpeed() const { return 0.0f; }
+    // Box2D/Dynamics/Contacts/b2Contact.h:98-131. Edits made inside
+    // b2ContactListener::PreSolve reach the step: SetEnabled for this step,
+    // friction and restitution for the rest of the contact's life.
+    void SetEnabled(bool flag) { m_enabled = flag; }
+    void SetFriction(float friction) { m_friction = friction; }
+    void ResetFriction();
+    void SetRestitution(float restitution) { m_restitution = restitution; }
+    void ResetRestitution();
 
 private:
     friend class b2World;
@@ -467,7 +475,10 @@ private:
     int32_t Upload();
     void Download() const;
     void DownloadJoints() const;
-    void ReplayEvents();
+    int32_t CollideWithListener(const void* params);
+    // PostSolve after the Step; Begin/End too unless the split step
+    // already delivered them
+    void ReplayEvents(bool collideEventsDone);
 
     b2g_context* m_ctx = nullptr;
     b2Vec2 m_gravity;

@@ -29,11 +29,46 @@ using namespace box2dref;
 
 namespace {
 
+// Records the listener calls (kind 1 BeginContact, 2 EndContact, 3 PreSolve,
+// 4 PostSolve; fixture indices; for PreSolve whether the contact stays
+// enabled) and disables the contacts of a one-sided platform fixture while
+// the other body's origin is less than `clear` above the platform's top (a
+// one-sided platform after the testbed's OneSidedPlatform,
+// Testbed/Tests/OneSidedPlatform.h).
+struct Recorder : public b2ContactListener {
+    std::vector<b2Fixture*>* fixtures = nullptr;
+    int platform = -1;
+    float top = 0.0f, clear = 0.0f;
+    int step = 0;
+    std::vector<int32_t> events;  // step, kind, fixture A, fixture B, flag
+    int indexOf(const b2Fixture* f) const {
+        for (size_t i = 0; i < fixtures->size(); ++i)
+            if ((*fixtures)[i] == f) return (int)i;
+        return -1;
+    }
+    void record(int kind, b2Contact* c, int flag) {
+        events.insert(events.end(),
+                      {step, kind, indexOf(c->GetFixtureA()), indexOf(c->GetFixtureB()), flag});
+    }
+    void BeginContact(b2Contact* c) override { record(1, c, 0); }
+    void EndContact(b2Contact* c) override { record(2, c, 0); }
+    void PreSolve(b2Contact* c, const b2Manifold*) override {
+        const int a = indexOf(c->GetFixtureA()), b = indexOf(c->GetFixtureB());
+        if (a == platform || b == platform) {
+            const b2Body* other = (a == platform ? c->GetFixtureB() : c->GetFixtureA())->GetBody();
+            if (other->GetPosition().y < top + clear) c->SetEnabled(false);
+        }
+        record(3, c, c->IsEnabled() ? 1 : 0);
+    }
+    void PostSolve(b2Contact* c, const b2ContactImpulse*) override { record(4, c, 0); }
+};
+
 struct RefWorld {
     b2World* world;
     std::vector<b2Body*> bodies;
     std::vector<b2Fixture*> fixtures;
     std::vector<b2RevoluteJoint*> joints;
+    Recorder recorder;
 };
 
 // Scene joints as b2RevoluteJoints (local anchors and reference angle given
@@ -251,6 +286,25 @@ int ref_world_load_scene(void* h, const char* kind, int n, unsigned seed) {
 
 int ref_world_joint_count(void* h) { return (int)((RefWorld*)h)->joints.size(); }
 
+// Installs the recording listener with the one-sided platform rule for
+// fixture `platform` (its top and the clearance as given); -1: record only.
+void ref_world_set_recorder(void* h, int platform, float top, float clear) {
+    RefWorld* w = (RefWorld*)h;
+    w->recorder.fixtures = &w->fixtures;
+    w->recorder.platform = platform;
+    w->recorder.top = top;
+    w->recorder.clear = clear;
+    w->world->SetContactListener(&w->recorder);
+}
+
+// Recorded events, 5 ints each; returns the event count.
+int ref_world_events(void* h, int32_t* out, int cap) {
+    RefWorld* w = (RefWorld*)h;
+    const int n = (int)w->recorder.events.size() / 5;
+    for (int i = 0; i < n && i < cap; ++i) memcpy(out + 5 * i, &w->recorder.events[5 * i], 5 * sizeof(int32_t));
+    return n;
+}
+
 int ref_world_set_joint(void* h, int joint, int enableLimit, float lower, float upper, int enableMotor, float speed,
                         float maxTorque) {
     RefWorld* w = (RefWorld*)h;
@@ -290,7 +344,10 @@ void ref_world_get_joints(void* h, float* out) {
 
 void ref_world_step(void* h, float dt, int vel, int pos, int nsteps) {
     RefWorld* w = (RefWorld*)h;
-    for (int i = 0; i < nsteps; ++i) w->world->Step(dt, vel, pos);
+    for (int i = 0; i < nsteps; ++i) {
+        w->world->Step(dt, vel, pos);
+        ++w->recorder.step;
+    }
 }
 
 int ref_world_body_count(void* h) { return (int)((RefWorld*)h)->bodies.size(); }

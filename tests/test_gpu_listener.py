@@ -1,0 +1,95 @@
+"""b2ContactListener on the GPU path against the reference build.
+
+With a listener set, b2World::Step splits at the narrowphase
+(b2g_step_collide / b2g_step_solve) and makes the reference's calls inside
+the step: over the contact list, newest first, EndContact for destroyed
+touching contacts, BeginContact / EndContact on touching changes and
+PreSolve with the old manifold; PreSolve's SetEnabled(false) keeps the
+contact out of the step's solve. PostSolve follows the step for the solved
+contacts.
+
+The one-sided platform scene has one contact per island and bodies of fixed
+rotation, so the coloured solver's order is the reference's and no in-step
+trig rounds differently: poses and the sequence of listener calls must equal
+the reference's exactly. PostSolve calls are compared per step as a set (the
+reference reports them island by island, the GPU in list order).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import engines as E
+
+pytestmark = pytest.mark.gpu
+F = ctypes.c_float
+
+
+def _events(w):
+    n = w.e.fn("world_events")(w.h, None, 0)
+    ev = np.zeros(5 * max(n, 1), np.int32)
+    w.e.fn("world_events")(w.h, E.ptr(ev), n)
+    return ev[:5 * n].reshape(-1, 5)
+
+
+def _bits(a):
+    return (a + np.float32(0)).view(np.uint32)
+
+
+@pytest.mark.parametrize("sleep", [True, False])
+def test_one_sided_platform_matches_reference(gpu, ref, sleep):
+    g = gpu.world("oneway", 8, 0, sleep=sleep)
+    r = ref.world("oneway", 8, 0, sleep=sleep)
+    for w in (g, r):
+        w.e.fn("world_set_recorder")(w.h, 1, F(5.25), F(0.1))
+    for s in range(240):
+        g.step(1)
+        r.step(1)
+        gb, rb = g.bodies(), r.bodies()
+        assert np.array_equal(_bits(gb[:, :11]), _bits(rb[:, :11])), f"step {s + 1}"
+    ge, re_ = _events(g), _events(r)
+    inside = lambda e: e[e[:, 1] != 4]  # Begin, End, PreSolve: in call order
+    assert np.array_equal(inside(ge), inside(re_))
+    post = lambda e: sorted(map(tuple, e[e[:, 1] == 4].tolist()))
+    assert post(ge) == post(re_)
+    assert ((ge[:, 1] == 3) & (ge[:, 4] == 0)).sum() > 8  # PreSolve did disable contacts
+    assert np.all(np.abs(gb[2:, E.CY] - 5.5) < 0.02)       # the boxes landed on the platform
+    g.close()
+    r.close()
+
+
+def test_recording_listener_without_rule_matches_plain_step(gpu):
+    """A listener that only records changes nothing: the split step gives
+    the same poses as the unsplit one, and every touching contact reports
+    PostSolve while it is solved."""
+    a = gpu.world("pyramid", 10, 0, sleep=True)
+    b = gpu.world("pyramid", 10, 0, sleep=True)
+    a.e.fn("world_set_recorder")(a.h, -1, F(0.0), F(0.0))
+    for s in range(300):
+        a.step(1)
+        b.step(1)
+    assert np.array_equal(_bits(a.bodies()), _bits(b.bodies()))
+    ev = _events(a)
+    assert (ev[:, 1] == 1).sum() >= 100 and (ev[:, 1] == 3).sum() > 1000 and (ev[:, 1] == 4).sum() > 1000
+    assert np.all(ev[ev[:, 1] == 3][:, 4] == 1)
+    a.close()
+    b.close()
+
+
+def test_split_step_state_errors(gpu):
+    lib = E.b2g_lib()
+    w = gpu.world("pyramid", 4, 0)
+    c = w.device_context()
+    p = np.zeros(1, E.STEP_PARAMS)
+    p[0] = (1.0 / 60.0, 8, 3, 0, 0, 0.0)
+    assert lib.b2g_step_solve(c) != 0                      # nothing open
+    E.b2g_check(lib, c, lib.b2g_step_collide(c, E.ptr(p)))
+    assert lib.b2g_step_collide(c, E.ptr(p)) != 0          # already open
+    assert lib.b2g_step(c, E.ptr(p)) != 0                  # must finish first
+    E.b2g_check(lib, c, lib.b2g_step_solve(c))
+    E.b2g_check(lib, c, lib.b2g_step(c, E.ptr(p)))
+    p[0]["mode"] = 1
+    assert lib.b2g_step_collide(c, E.ptr(p)) != 0          # coloured solver only
+    w.close()

@@ -332,3 +332,35 @@ def test_trig_modes(orclib):
     assert ulp(s0, s1).max() <= 1 and ulp(c0, c1).max() <= 1
     frac = (ulp(s0, s1) > 0).mean()
     assert 0.0 < frac < 0.05
+
+
+def _events(w):
+    n = w.e.fn("world_events")(w.h, None, 0)
+    ev = np.zeros(5 * max(n, 1), np.int32)
+    w.e.fn("world_events")(w.h, E.ptr(ev), n)
+    return ev[:5 * n].reshape(-1, 5)
+
+
+def test_one_sided_platform_listener(ref):
+    """The listener the GPU's PreSolve path is checked with, on the
+    reference build: boxes launched at 10.5 m/s from y = 1 pass up through
+    the platform (top y = 5.25) while PreSolve disables its contacts, peak
+    near 1 + v^2 / 2g = 6.51, land on it (origin 5.25 + 0.25 above, less the
+    skin) and fall asleep. Each box begins touching the platform twice (rising
+    through, landing) and ends once; every PreSolve that disabled a contact
+    came while the box was below the clearance."""
+    w = ref.world("oneway", 8, 0, sleep=True)
+    w.e.fn("world_set_recorder")(w.h, 1, F(5.25), F(0.1))
+    peak = np.full(8, -1e9)
+    for _ in range(240):
+        w.step(1)
+        peak = np.maximum(peak, w.bodies()[2:, E.CY])
+    b = w.bodies()[2:]
+    assert np.all(np.abs(peak - (1.0 + 10.5 ** 2 / 20.0)) < 0.05), peak
+    assert np.all(np.abs(b[:, E.CY] - 5.5) < 0.02) and np.all(b[:, E.AWAKE] == 0)
+    ev = _events(w)
+    plat = (ev[:, 2] == 1) | (ev[:, 3] == 1)
+    for k, per_box in ((1, 2), (2, 1)):
+        assert ((ev[:, 1] == k) & plat).sum() == 8 * per_box, k
+    assert ((ev[:, 1] == 3) & (ev[:, 4] == 0)).sum() > 8
+    w.close()
