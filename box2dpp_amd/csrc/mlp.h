@@ -49,6 +49,7 @@ struct Net {
     uint16_t* wh = nullptr;   // hTotal BF16 weights
     float* norm = nullptr;    // kNorm
     float* loss = nullptr;    // one float: loss sum of the last mlpGradient calls
+    float* partial = nullptr; // kMaxGradBlocks x (kParams + 1): per-workgroup gradient and loss
     int adamStep = 0;
     bool ready = false;       // parameters set (initialised, trained or loaded)
 };

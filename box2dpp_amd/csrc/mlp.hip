@@ -20,6 +20,7 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;           // wavefronts per workgroup
+constexpr int kMaxGradBlocks = 256; // workgroups of one gradient launch at most
 constexpr int kThreads = 64 * kWaves;
 constexpr int kBiasCount = 3 * kHidden + kOutPad;
 
@@ -197,13 +198,15 @@ __device__ inline float groupSum16(float v) {
 
 __global__ __launch_bounds__(kThreads) void k_mlp_grad(const uint16_t* wh, const float* params, const float* norm,
                                                        const float* x, const float* tgt, const int* order, int first,
-                                                       int count, float invN, float* grad, float* lossOut) {
+                                                       int count, float invN, float* partial) {
     __shared__ Smem S;
     __shared__ float red[kParams];
+    __shared__ float lossShared;
     __shared__ float tf[kWaves][16][17];
     __shared__ uint16_t th[kWaves][16][18];
     loadSmem(S, wh, params, norm);
     for (int i = threadIdx.x; i < kParams; i += blockDim.x) red[i] = 0.0f;
+    if (threadIdx.x == 0) lossShared = 0.0f;
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, c = lane & 15;
@@ -311,38 +314,57 @@ __global__ __launch_bounds__(kThreads) void k_mlp_grad(const uint16_t* wh, const
             }
         }
     }
-    // Reduce the four wavefronts in LDS: accumulator element j of tile
-    // (mo, ni) is dW[16mo + 4g + j][16ni + c].
-    for (int mo = 0; mo < 4; ++mo)
-        for (int j = 0; j < 4; ++j) {
-            const int o = 16 * mo + 4 * g + j;
-            atomicAdd(&red[kW1 + o * kIn + c], gW1[mo][j]);
-            for (int ni = 0; ni < 4; ++ni) {
-                atomicAdd(&red[kW2 + o * kHidden + 16 * ni + c], gW2[mo][ni][j]);
-                atomicAdd(&red[kW3 + o * kHidden + 16 * ni + c], gW3[mo][ni][j]);
-            }
-        }
-    for (int ni = 0; ni < 4; ++ni)
-        for (int j = 0; j < 4; ++j) atomicAdd(&red[kW4 + (4 * g + j) * kHidden + 16 * ni + c], gW4[ni][j]);
-    for (int mt = 0; mt < 4; ++mt)
-        for (int j = 0; j < 4; ++j) {
-            const float s1 = groupSum16(gb1[mt][j]), s2 = groupSum16(gb2[mt][j]), s3 = groupSum16(gb3[mt][j]);
-            if (c == 0) {
-                atomicAdd(&red[kB1 + 16 * mt + 4 * g + j], s1);
-                atomicAdd(&red[kB2 + 16 * mt + 4 * g + j], s2);
-                atomicAdd(&red[kB3 + 16 * mt + 4 * g + j], s3);
-            }
-        }
-    for (int j = 0; j < 4; ++j) {
-        const float s4 = groupSum16(gb4[j]);
-        if (c == 0) atomicAdd(&red[kB4 + 4 * g + j], s4);
-    }
+    // Reduce the four wavefronts in LDS, one wavefront after another (each
+    // lane owns distinct entries, so plain read-add-writes): accumulator
+    // element j of tile (mo, ni) is dW[16mo + 4g + j][16ni + c]. Then the
+    // workgroup's sums go to its row of `partial`, which k_mlp_reduce adds up
+    // in a fixed order: the gradient does not depend on the schedule.
     float l = loss;
     for (int off = 1; off < 64; off <<= 1) l += __shfl_xor(l, off);
-    if (lane == 0) atomicAdd(lossOut, l);
-    __syncthreads();
-    for (int i = threadIdx.x; i < kParams; i += blockDim.x)
-        if (red[i] != 0.0f) atomicAdd(&grad[i], red[i]);
+    for (int wv = 0; wv < kWaves; ++wv) {
+        if (wave == wv) {
+            for (int mo = 0; mo < 4; ++mo)
+                for (int j = 0; j < 4; ++j) {
+                    const int o = 16 * mo + 4 * g + j;
+                    red[kW1 + o * kIn + c] += gW1[mo][j];
+                    for (int ni = 0; ni < 4; ++ni) {
+                        red[kW2 + o * kHidden + 16 * ni + c] += gW2[mo][ni][j];
+                        red[kW3 + o * kHidden + 16 * ni + c] += gW3[mo][ni][j];
+                    }
+                }
+            for (int ni = 0; ni < 4; ++ni)
+                for (int j = 0; j < 4; ++j) red[kW4 + (4 * g + j) * kHidden + 16 * ni + c] += gW4[ni][j];
+            for (int mt = 0; mt < 4; ++mt)
+                for (int j = 0; j < 4; ++j) {
+                    const float s1 = groupSum16(gb1[mt][j]), s2 = groupSum16(gb2[mt][j]), s3 = groupSum16(gb3[mt][j]);
+                    if (c == 0) {
+                        red[kB1 + 16 * mt + 4 * g + j] += s1;
+                        red[kB2 + 16 * mt + 4 * g + j] += s2;
+                        red[kB3 + 16 * mt + 4 * g + j] += s3;
+                    }
+                }
+            for (int j = 0; j < 4; ++j) {
+                const float s4 = groupSum16(gb4[j]);
+                if (c == 0) red[kB4 + 4 * g + j] += s4;
+            }
+            if (lane == 0) lossShared += l;
+        }
+        __syncthreads();
+    }
+    float* row = partial + (size_t)blockIdx.x * (kParams + 1);
+    for (int i = threadIdx.x; i < kParams; i += blockDim.x) row[i] = red[i];
+    if (threadIdx.x == 0) row[kParams] = lossShared;
+}
+
+// Sum of the workgroups' rows, in workgroup order, added to the gradient
+// and the loss.
+__global__ void k_mlp_reduce(const float* partial, int rows, float* grad, float* loss) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > kParams) return;
+    float acc = 0.0f;
+    for (int r = 0; r < rows; ++r) acc += partial[(size_t)r * (kParams + 1) + i];
+    if (i < kParams) grad[i] += acc;
+    else *loss += acc;
 }
 
 // BF16 image entries of parameter i (weights only; biases stay FP32).
@@ -402,6 +424,7 @@ hipError_t mlpAlloc(Net& n) {
     if ((e = hipMalloc((void**)&n.wh, sizeof(uint16_t) * hTotal)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&n.norm, sizeof(float) * kNorm)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&n.loss, sizeof(float))) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&n.partial, sizeof(float) * kMaxGradBlocks * (kParams + 1))) != hipSuccess) return e;
     if ((e = hipMemset(n.params, 0, sizeof(float) * kParams)) != hipSuccess) return e;
     if ((e = hipMemset(n.grad, 0, sizeof(float) * kParams)) != hipSuccess) return e;
     if ((e = hipMemset(n.m, 0, sizeof(float) * kParams)) != hipSuccess) return e;
@@ -412,7 +435,7 @@ hipError_t mlpAlloc(Net& n) {
 }
 
 void mlpFree(Net& n) {
-    void* ps[] = {n.params, n.grad, n.m, n.v, n.wh, n.norm, n.loss};
+    void* ps[] = {n.params, n.grad, n.m, n.v, n.wh, n.norm, n.loss, n.partial};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     n = Net();
@@ -452,10 +475,12 @@ hipError_t mlpGradient(const Net& n, const float* x, const float* t, const int* 
     // wavefront taking as many 16-sample tiles as needed
     const int tiles = (count + 15) / 16;
     int grid = (tiles + kWaves - 1) / kWaves;
-    if (grid > 256) grid = 256;
+    if (grid > kMaxGradBlocks) grid = kMaxGradBlocks;
     if (grid < 1) grid = 1;
     hipExtLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(kThreads), 0, s, e0, e1, 0, n.wh, n.params, n.norm, x, t,
-                          order, first, count, 1.0f / (float)count, n.grad, n.loss);
+                          order, first, count, 1.0f / (float)count, n.partial);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((kParams + 1 + 255) / 256), dim3(256), 0, s, n.partial, grid, n.grad,
+                       n.loss);
     return hipGetLastError();
 }
 

@@ -88,6 +88,23 @@ def test_gradient_matches_oracle(lib, ctx):
         assert rel_x < 5e-2 and cos > 0.998, (k, rel_x, cos)
 
 
+def test_gradient_is_deterministic(lib, ctx):
+    """Workgroup sums are added up in a fixed order and the wavefronts of a
+    workgroup in a fixed order: the same batch gives the same gradient bits
+    (enough samples for many workgroups)."""
+    rng = np.random.default_rng(21)
+    _set(lib, ctx, M.random_params(rng))
+    n = 40000
+    x, t = _inputs(rng, n)
+    out = []
+    for _ in range(3):
+        g = np.zeros(1, M.PARAMS_DTYPE)
+        loss = ctypes.c_double()
+        E.b2g_check(lib, ctx, lib.b2g_mlp_gradient(ctx, E.ptr(x), E.ptr(t), n, E.ptr(g), ctypes.byref(loss)))
+        out.append((g.view(np.uint8).copy(), loss.value))
+    assert all(np.array_equal(o[0], out[0][0]) and o[1] == out[0][1] for o in out)
+
+
 def test_adam_training_fits_a_smooth_map(lib, gpu):
     """Log fresh points from a pile, then train: the loss falls well below
     the loss of predicting the mean."""
