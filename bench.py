@@ -51,7 +51,7 @@ DT = 1.0 / 60.0
 # b2g_schedule: island labelling method, collide order, workgroup sizes of
 # the solver, narrowphase and broad-phase query kernels
 SCHEDULE = np.dtype([("islands", "<i4"), ("collideOrder", "<i4"), ("solverBlock", "<i4"), ("collideBlock", "<i4"),
-                     ("queryBlock", "<i4")])
+                     ("queryBlock", "<i4"), ("tailColours", "<i4")])
 # the schedule the benchmark's worlds run with (set by the autotuner)
 _schedule = None
 
@@ -101,7 +101,7 @@ def autotune(lib, h, args, at):
     """Schedule chosen by the autotuner on world h after `at` steps."""
     out = np.zeros(1, SCHEDULE)
     step(lib, h, at, args.vel, args.pos)
-    progress(f"autotuning at step {at} ({15 * args.tune_rounds} steps)")
+    progress(f"autotuning at step {at} ({19 * args.tune_rounds} steps)")
     rc = lib.b2pp_world_autotune(h, ctypes.c_float(DT), args.vel, args.pos, args.tune_rounds,
                                  out.ctypes.data_as(ctypes.c_void_p))
     if rc != 0:

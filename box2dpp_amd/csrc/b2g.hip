@@ -1187,6 +1187,65 @@ __global__ void k_solve_position_serial_r(WorldView w, ConView cv, const int* ra
     if (lowest < 0) atomicMin(&cnt->minSep[it], lowest);
 }
 
+// Trailing colours in one workgroup: colours [c0, c1) in order, the
+// workgroup's lanes sharing each colour and a barrier between colours (the
+// barrier orders the workgroup's global writes, which the next colour
+// reads); the overflow colour is solved by lane 0 alone, in order.
+struct ColourRanges {
+    int s[kColourSlots + 1];
+};
+
+__global__ void k_warm_start_tail(WorldView w, ConView cv, ColourRanges r, int c0, int c1) {
+    for (int c = c0; c < c1; ++c) {
+        if (c == kOverflowColour) {
+            if (threadIdx.x == 0)
+                for (int k = r.s[c]; k < r.s[c + 1]; ++k) warmStartOne(w, cv, k);
+        } else {
+            for (int k = r.s[c] + threadIdx.x; k < r.s[c + 1]; k += blockDim.x) warmStartOne(w, cv, k);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_solve_velocity_tail(WorldView w, ConView cv, ColourRanges r, int c0, int c1) {
+    for (int c = c0; c < c1; ++c) {
+        if (c == kOverflowColour) {
+            if (threadIdx.x == 0)
+                for (int k = r.s[c]; k < r.s[c + 1]; ++k) solveVelocityOne(w, cv, k);
+        } else {
+            for (int k = r.s[c] + threadIdx.x; k < r.s[c + 1]; k += blockDim.x) solveVelocityOne(w, cv, k);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_solve_position_tail(WorldView w, ConView cv, ColourRanges r, int c0, int c1, Counters* cnt,
+                                      int it) {
+    __shared__ int blockMin;
+    if (threadIdx.x == 0) blockMin = 0;
+    __syncthreads();
+    int lowest = 0;
+    for (int c = c0; c < c1; ++c) {
+        const bool serial = c == kOverflowColour;
+        if (!serial || threadIdx.x == 0) {
+            for (int k = r.s[c] + (serial ? 0 : threadIdx.x); k < r.s[c + 1]; k += serial ? 1 : blockDim.x) {
+                const int root = cv.island[k];
+                if (it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0) continue;
+                const float minSep = solvePositionOne(w, cv, k);
+                if (minSep < -3.0f * kLinearSlop) {
+                    int* f = &w.islandMinSep[it * w.nBodies + root];
+                    if (*f == 0) *f = 1;
+                }
+                if (minSep < 0.0f) lowest = min(lowest, encodeOrdered(minSep));
+            }
+        }
+        __syncthreads();
+    }
+    if (lowest < 0) atomicMin(&blockMin, lowest);
+    __syncthreads();
+    if (threadIdx.x == 0 && blockMin < 0) atomicMin(&cnt->minSep[it], blockMin);
+}
+
 __global__ void k_solve_position_serial(WorldView w, ConView cv, int begin, int end, Counters* cnt, int it) {
     int lowest = 0;
     for (int k = begin; k < end; ++k) {
@@ -2331,7 +2390,7 @@ inline int blocks(int n, int bs = 256) { return (n + bs - 1) / bs; }
 // ====================================================================
 struct b2g_context {
     int device = 0;
-    b2g_schedule sched{0, 0, 256, 256, 256};
+    b2g_schedule sched{0, 0, 256, 256, 256, 1024};
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -2352,6 +2411,7 @@ struct b2g_context {
     int* islandMax = nullptr;   // largest body index per island root
     int* staticKey = nullptr;   // per static body: see k_static_reach
     int staticKeyCap = 0;
+    bool staticAsleep = false;  // a static body was uploaded asleep
     int* islandMinSep = nullptr;
     int islandIterCap = 0;
     int* bworld = nullptr;          // sub-world of each body (batched worlds)
@@ -3556,12 +3616,27 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         ctx->stats.residual = -1.0f;
         const int sb = ctx->sched.solverBlock;
         auto sgrid = [sb](int n) { return dim3((unsigned)((n + sb - 1) / sb)); };
-        auto perColour = [&](int phase, auto kernelLaunch) {
-            for (int c = 0; c < nColours; ++c) {
+        // Trailing colours with at most sched.tailColours constraints each go to
+        // one workgroup (tailLaunch); the colours before them get a launch each.
+        int tailBegin = nColours;
+        if (ctx->sched.tailColours > 0) {
+            while (tailBegin > 0 && colourStart[tailBegin] - colourStart[tailBegin - 1] <= ctx->sched.tailColours)
+                --tailBegin;
+            if (nColours - tailBegin < 2) tailBegin = nColours;  // one colour: no gain
+        }
+        ColourRanges ranges;
+        memcpy(ranges.s, colourStart, sizeof(ranges.s));
+        auto perColour = [&](int phase, auto kernelLaunch, auto tailLaunch) {
+            for (int c = 0; c < tailBegin; ++c) {
                 const int b0 = colourStart[c], b1 = colourStart[c + 1];
                 if (b1 <= b0) continue;
                 const int pc = profStart(ctx);
                 kernelLaunch(b0, b1, c == kOverflowColour);
+                profEnd(ctx, phase, pc);
+            }
+            if (tailBegin < nColours) {
+                const int pc = profStart(ctx);
+                tailLaunch(tailBegin, nColours);
                 profEnd(ctx, phase, pc);
             }
         };
@@ -3578,12 +3653,17 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         // contact warm start, then the joints' InitVelocityConstraints.
         auto startSolve = [&](bool contactWarm) {
             if (contactWarm) {
-                perColour(PH_WARM, [&](int b0, int b1, bool serial) {
-                    if (serial)
-                        hipLaunchKernelGGL(k_warm_start_serial, dim3(1), dim3(1), 0, s, w, ctx->cv, b0, b1);
-                    else
-                        hipLaunchKernelGGL(k_warm_start, sgrid(b1 - b0), dim3(sb), 0, s, w, ctx->cv, b0, b1);
-                });
+                perColour(
+                    PH_WARM,
+                    [&](int b0, int b1, bool serial) {
+                        if (serial)
+                            hipLaunchKernelGGL(k_warm_start_serial, dim3(1), dim3(1), 0, s, w, ctx->cv, b0, b1);
+                        else
+                            hipLaunchKernelGGL(k_warm_start, sgrid(b1 - b0), dim3(sb), 0, s, w, ctx->cv, b0, b1);
+                    },
+                    [&](int c0, int c1) {
+                        hipLaunchKernelGGL(k_warm_start_tail, dim3(1), dim3(1024), 0, s, w, ctx->cv, ranges, c0, c1);
+                    });
             }
             perJointColour([&](int b0, int b1, bool serial) {
                 if (serial)
@@ -3603,17 +3683,23 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
                     else
                         hipLaunchKernelGGL(k_joint_solve_velocity, sgrid(b1 - b0), dim3(sb), 0, s, w, jv, b0, b1, dt);
                 });
-                perColour(PH_SOLVE_V, [&](int b0, int b1, bool serial) {
-                    if (serial) {
-                        hipLaunchKernelGGL(k_solve_velocity_serial, dim3(1), dim3(1), 0, s, w, ctx->cv, b0, b1);
-                        return;
-                    }
-                    hipEvent_t e0, e1;
-                    const int pk = profKernelEvents(ctx, &e0, &e1);
-                    hipExtLaunchKernelGGL(k_solve_velocity, sgrid(b1 - b0), dim3(sb), 0, s, e0, e1, 0, w, ctx->cv,
-                                          b0, b1);
-                    profKernel(ctx, PH_K_SOLVE_V, pk);
-                });
+                perColour(
+                    PH_SOLVE_V,
+                    [&](int b0, int b1, bool serial) {
+                        if (serial) {
+                            hipLaunchKernelGGL(k_solve_velocity_serial, dim3(1), dim3(1), 0, s, w, ctx->cv, b0, b1);
+                            return;
+                        }
+                        hipEvent_t e0, e1;
+                        const int pk = profKernelEvents(ctx, &e0, &e1);
+                        hipExtLaunchKernelGGL(k_solve_velocity, sgrid(b1 - b0), dim3(sb), 0, s, e0, e1, 0, w,
+                                              ctx->cv, b0, b1);
+                        profKernel(ctx, PH_K_SOLVE_V, pk);
+                    },
+                    [&](int c0, int c1) {
+                        hipLaunchKernelGGL(k_solve_velocity_tail, dim3(1), dim3(1024), 0, s, w, ctx->cv, ranges, c0,
+                                           c1);
+                    });
             }
         };
         auto predict = [&](float4* imp) -> int {
@@ -3706,18 +3792,24 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         LAUNCHCHK(ctx);
         // Each position iteration: contacts, then joints (REF/Dynamics/b2Island.cpp:310-328).
         for (int it = 0; it < posIters && (nCons > 0 || ctx->nJoints > 0); ++it) {
-            perColour(PH_SOLVE_X, [&](int b0, int b1, bool serial) {
-                if (serial) {
-                    hipLaunchKernelGGL(k_solve_position_serial, dim3(1), dim3(1), 0, s, w, ctx->cv, b0, b1, ctx->cnt,
-                                       it);
-                    return;
-                }
-                hipEvent_t e0, e1;
-                const int pk = profKernelEvents(ctx, &e0, &e1);
-                hipExtLaunchKernelGGL(k_solve_position, sgrid(b1 - b0), dim3(sb), 0, s, e0, e1, 0, w, ctx->cv, b0,
-                                      b1, ctx->cnt, it);
-                profKernel(ctx, PH_K_SOLVE_X, pk);
-            });
+            perColour(
+                PH_SOLVE_X,
+                [&](int b0, int b1, bool serial) {
+                    if (serial) {
+                        hipLaunchKernelGGL(k_solve_position_serial, dim3(1), dim3(1), 0, s, w, ctx->cv, b0, b1,
+                                           ctx->cnt, it);
+                        return;
+                    }
+                    hipEvent_t e0, e1;
+                    const int pk = profKernelEvents(ctx, &e0, &e1);
+                    hipExtLaunchKernelGGL(k_solve_position, sgrid(b1 - b0), dim3(sb), 0, s, e0, e1, 0, w, ctx->cv,
+                                          b0, b1, ctx->cnt, it);
+                    profKernel(ctx, PH_K_SOLVE_X, pk);
+                },
+                [&](int c0, int c1) {
+                    hipLaunchKernelGGL(k_solve_position_tail, dim3(1), dim3(1024), 0, s, w, ctx->cv, ranges, c0, c1,
+                                       ctx->cnt, it);
+                });
             perJointColour([&](int b0, int b1, bool serial) {
                 if (serial)
                     hipLaunchKernelGGL(k_joint_solve_position_serial, dim3(1), dim3(1), 0, s, w, jv, b0, b1, it);
@@ -3731,9 +3823,10 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         if (ctx->allowSleep) {
             hipLaunchKernelGGL(k_sleep_accum, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, dt);
         }
-        if (ctx->nContacts > 0 || ctx->nJoints > 0) {
-            // static bodies reached by this step's islands (before the sleep
-            // below changes the island flags the key reads)
+        // Static bodies reached by this step's islands (before the sleep below
+        // changes the island flags the key reads). Without sleeping, only a
+        // static body that is asleep can change (the search wakes it).
+        if ((ctx->nContacts > 0 || ctx->nJoints > 0) && (ctx->allowSleep || ctx->staticAsleep)) {
             hipLaunchKernelGGL(k_fill<int>, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, ctx->islandMax, ctx->nBodies,
                                -1);
             hipLaunchKernelGGL(k_island_max, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, ctx->islandMax);
@@ -3935,8 +4028,10 @@ int uploadBodies(b2g_context* ctx, int first, int count, const b2g_body_state* b
     std::vector<int> type(count), flags(count);
     std::vector<float4> xf(count), pos(count), pos0(count), vel(count), force(count), mass(count);
     std::vector<float2> lc(count), gs(count);
-    for (int i = 0; i < count; ++i)
+    for (int i = 0; i < count; ++i) {
         bodyToDevice(bodies[i], type[i], flags[i], xf[i], pos[i], pos0[i], lc[i], vel[i], force[i], mass[i], gs[i]);
+        if (type[i] == BT_STATIC && !(flags[i] & BF_AWAKE)) ctx->staticAsleep = true;
+    }
     hipStream_t s = ctx->stream;
     HIPCHK(ctx, hipMemcpyAsync(ctx->btype + first, type.data(), sizeof(int) * count, hipMemcpyHostToDevice, s));
     HIPCHK(ctx, hipMemcpyAsync(ctx->bflags + first, flags.data(), sizeof(int) * count, hipMemcpyHostToDevice, s));
@@ -5124,7 +5219,8 @@ int b2g_set_schedule(b2g_context* ctx, const b2g_schedule* sc) {
     if (!ctx || !sc) return B2G_ERR_ARG;
     auto blockOk = [](int b) { return b >= 64 && b <= 1024 && b % 64 == 0; };
     if (sc->islands < 0 || sc->islands > 2 || sc->collideOrder < 0 || sc->collideOrder > 1 ||
-        !blockOk(sc->solverBlock) || !blockOk(sc->collideBlock) || !blockOk(sc->queryBlock))
+        !blockOk(sc->solverBlock) || !blockOk(sc->collideBlock) || !blockOk(sc->queryBlock) ||
+        sc->tailColours < 0 || sc->tailColours > 8192)
         return fail(ctx, B2G_ERR_ARG, "schedule field out of range");
     ctx->sched = *sc;
     return B2G_OK;
@@ -5154,6 +5250,7 @@ int b2g_autotune(b2g_context* ctx, const b2g_step_params* p, int rounds, b2g_sch
         {&b2g_schedule::solverBlock, {64, 128, 256, 512}, {PH_WARM, PH_SOLVE_V, PH_SOLVE_X}},
         {&b2g_schedule::collideBlock, {64, 128, 256}, {PH_COLLIDE}},
         {&b2g_schedule::queryBlock, {64, 128, 256}, {PH_BROADPHASE}},
+        {&b2g_schedule::tailColours, {0, 512, 1024, 2048}, {PH_WARM, PH_SOLVE_V, PH_SOLVE_X}},
     };
     (void)hipSetDevice(ctx->device);
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

@@ -424,8 +424,11 @@ typedef struct b2g_schedule {
     int32_t solverBlock;   /* workgroup size of the per-colour solver kernels */
     int32_t collideBlock;  /* workgroup size of the narrowphase kernel */
     int32_t queryBlock;    /* workgroup size of the broad-phase query */
+    int32_t tailColours;   /* trailing colours of at most this many constraints are
+                              solved by one 1024-lane workgroup, colour after colour
+                              (0: every colour its own launch) */
 } b2g_schedule;
-/* Workgroup sizes are multiples of 64 from 64 to 1024. */
+/* Workgroup sizes are multiples of 64 from 64 to 1024; tailColours 0 to 8192. */
 int b2g_set_schedule(b2g_context* ctx, const b2g_schedule* s);
 int b2g_get_schedule(b2g_context* ctx, b2g_schedule* s);
 /* Autotuner: picks each schedule field in turn by timing the candidates on
@@ -433,7 +436,7 @@ int b2g_get_schedule(b2g_context* ctx, b2g_schedule* s);
  * and their order rotated each of `rounds` rounds), keeps the fastest, and
  * returns the result in *out. The steps advance the world as ordinary
  * steps do; since no choice changes results, the trajectory is the one the
- * world would have had anyway. (3 + 2 + 4 + 3 + 3) * rounds steps. */
+ * world would have had anyway. (3 + 2 + 4 + 3 + 3 + 4) * rounds steps. */
 int b2g_autotune(b2g_context* ctx, const b2g_step_params* p, int rounds, b2g_schedule* out);
 
 /* ------------------------------------------------------------- split step
