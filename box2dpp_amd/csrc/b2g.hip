@@ -5202,6 +5202,181 @@ int b2g_abi_sizes(int32_t* out, int cap) {
 
 extern "C" {
 
+}  // extern "C"
+
+// ------------------------------------------------------------ snapshot
+// A snapshot is the world state a step reads: header, then the body,
+// fixture, contact and joint arrays in a fixed order. Everything else
+// (islands, constraint rows, grids, hash sets, counters) is rebuilt by the
+// step or by the restore, so a restored context steps exactly as the
+// original would.
+namespace {
+constexpr uint32_t kSnapMagic = 0x53473242u;  // "B2GS"
+constexpr uint32_t kSnapVersion = 1;
+struct SnapHeader {
+    uint32_t magic, version;
+    int32_t nBodies, nFixtures, nContacts, nJoints;
+    float gravity[2];
+    float inv_dt0, cellSize;
+    uint32_t claimTag;
+    uint8_t allowSleep, warmStarting, continuous, subStepping, autoClearForces, newFixture, cellSizeSet, subWorlds;
+    uint8_t staticAsleep, pad[3];
+    int32_t jointColourStart[kColourSlots + 1];
+};
+
+// Walks the snapshot layout: `visit(devicePointer, bytes)` for each array.
+template <typename F>
+void snapArrays(b2g_context* ctx, int nb, int nf, int nc, int nj, bool subWorlds, F visit) {
+    visit(ctx->btype, sizeof(int) * nb);
+    visit(ctx->bflags, sizeof(int) * nb);
+    visit(ctx->xf, sizeof(float4) * nb);
+    visit(ctx->pos, sizeof(float4) * nb);
+    visit(ctx->pos0, sizeof(float4) * nb);
+    visit(ctx->vel, sizeof(float4) * nb);
+    visit(ctx->force, sizeof(float4) * nb);
+    visit(ctx->mass, sizeof(float4) * nb);
+    visit(ctx->lcenter, sizeof(float2) * nb);
+    visit(ctx->gsleep, sizeof(float2) * nb);
+    visit(ctx->colourMask, sizeof(unsigned long long) * nb);
+    if (subWorlds) visit(ctx->bworld, sizeof(int) * nb);
+    visit(ctx->fbody, sizeof(int) * nf);
+    visit(ctx->shape, sizeof(b2g_shape) * nf);
+    visit(ctx->fmat, sizeof(float4) * nf);
+    visit(ctx->ffilter, sizeof(uint32_t) * nf);
+    visit(ctx->fgroup, sizeof(int) * nf);
+    visit(ctx->fat, sizeof(float4) * nf);
+    visit(ctx->moved, sizeof(uint8_t) * nf);
+    visit(ctx->cur.cfA, sizeof(int) * nc);
+    visit(ctx->cur.cfB, sizeof(int) * nc);
+    visit(ctx->cur.cflags, sizeof(int) * nc);
+    visit(ctx->cur.cmat, sizeof(float2) * nc);
+    visit(ctx->cur.cman, sizeof(b2g_manifold) * nc);
+    visit(ctx->cur.ccolour, sizeof(int) * nc);
+    visit(ctx->jBodies, sizeof(int2) * nj);
+    visit(ctx->jAnchors, sizeof(float4) * nj);
+    visit(ctx->jLimits, sizeof(float4) * nj);
+    visit(ctx->jMotorSpeed, sizeof(float) * nj);
+    visit(ctx->jFlags, sizeof(int) * nj);
+    visit(ctx->jImpulse, sizeof(float4) * nj);
+    visit(ctx->jOrder, sizeof(int) * nj);
+}
+}  // namespace
+
+extern "C" {
+
+int b2g_snapshot(b2g_context* ctx, void* out, size_t cap, size_t* size) {
+    if (!ctx || !size) return B2G_ERR_ARG;
+    if (ctx->stepOpen) return fail(ctx, B2G_ERR_STATE, "a split step is open");
+    if (ctx->jointsDirty && ctx->nJoints > 0) {
+        const int rc = recolourJoints(ctx);
+        if (rc != B2G_OK) return rc;
+    }
+    const int nb = ctx->nBodies, nf = ctx->nFixtures, nc = ctx->nContacts, nj = ctx->nJoints;
+    size_t total = sizeof(SnapHeader) + 3 * sizeof(int32_t) * (size_t)nj;  // host joint vectors
+    snapArrays(ctx, nb, nf, nc, nj, ctx->subWorlds, [&](const void*, size_t bytes) { total += bytes; });
+    *size = total;
+    if (!out) return B2G_OK;
+    if (cap < total) return fail(ctx, B2G_ERR_CAPACITY, "snapshot buffer too small");
+    SnapHeader h{};
+    h.magic = kSnapMagic;
+    h.version = kSnapVersion;
+    h.nBodies = nb; h.nFixtures = nf; h.nContacts = nc; h.nJoints = nj;
+    h.gravity[0] = ctx->gravity.x; h.gravity[1] = ctx->gravity.y;
+    h.inv_dt0 = ctx->inv_dt0; h.cellSize = ctx->cellSize; h.claimTag = ctx->claimTag;
+    h.allowSleep = ctx->allowSleep; h.warmStarting = ctx->warmStarting; h.continuous = ctx->continuous;
+    h.subStepping = ctx->subStepping; h.autoClearForces = ctx->autoClearForces; h.newFixture = ctx->newFixture;
+    h.cellSizeSet = ctx->cellSizeSet; h.subWorlds = ctx->subWorlds; h.staticAsleep = ctx->staticAsleep;
+    for (int c = 0; c <= kColourSlots; ++c)
+        h.jointColourStart[c] = nj > 0 && (int)ctx->jointColourStart.size() > c ? ctx->jointColourStart[c] : 0;
+    uint8_t* o = (uint8_t*)out;
+    memcpy(o, &h, sizeof(h));
+    o += sizeof(h);
+    for (int j = 0; j < nj; ++j) {
+        const int32_t v[3] = {ctx->hJointKeepApart[j], ctx->hJointRemoved[j], 0};
+        memcpy(o, v, sizeof(v));
+        o += sizeof(v);
+    }
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipSuccess;
+    snapArrays(ctx, nb, nf, nc, nj, ctx->subWorlds, [&](const void* p, size_t bytes) {
+        if (e == hipSuccess && bytes > 0) e = hipMemcpyAsync(o, p, bytes, hipMemcpyDeviceToHost, s);
+        o += bytes;
+    });
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    HIPCHK(ctx, e);
+    return B2G_OK;
+}
+
+int b2g_restore(b2g_context* ctx, const void* in, size_t size) {
+    if (!ctx || !in) return B2G_ERR_ARG;
+    if (ctx->nBodies != 0 || ctx->nFixtures != 0 || ctx->nJoints != 0)
+        return fail(ctx, B2G_ERR_STATE, "restore needs a new context");
+    SnapHeader h;
+    if (size < sizeof(h)) return fail(ctx, B2G_ERR_ARG, "snapshot too short");
+    memcpy(&h, in, sizeof(h));
+    if (h.magic != kSnapMagic || h.version != kSnapVersion) return fail(ctx, B2G_ERR_ARG, "not a snapshot");
+    if (h.nBodies < 0 || h.nFixtures < 0 || h.nContacts < 0 || h.nJoints < 0)
+        return fail(ctx, B2G_ERR_ARG, "snapshot counts");
+    const int nb = h.nBodies, nf = h.nFixtures, nc = h.nContacts, nj = h.nJoints;
+    size_t total = sizeof(SnapHeader) + 3 * sizeof(int32_t) * (size_t)nj;
+    snapArrays(ctx, nb, nf, nc, nj, h.subWorlds != 0, [&](const void*, size_t bytes) { total += bytes; });
+    if (size != total) return fail(ctx, B2G_ERR_ARG, "snapshot size does not match its counts");
+    int rc;
+    if ((rc = growBodies(ctx, std::max(nb, 1))) != B2G_OK) return rc;
+    if ((rc = growFixtures(ctx, std::max(nf, 1))) != B2G_OK) return rc;
+    if ((rc = growContacts(ctx, std::max(2 * nc, 1024))) != B2G_OK) return rc;
+    if (nj > 0 && (rc = growJoints(ctx, nj)) != B2G_OK) return rc;
+    if (h.subWorlds && !ctx->bworld) HIPCHK(ctx, hipMalloc((void**)&ctx->bworld, sizeof(int) * ctx->capBodies));
+    const uint8_t* q = (const uint8_t*)in + sizeof(h);
+    ctx->hJointBodies.assign(nj, make_int2(0, 0));
+    ctx->hJointKeepApart.assign(nj, 0);
+    ctx->hJointRemoved.assign(nj, 0);
+    for (int j = 0; j < nj; ++j) {
+        int32_t v[3];
+        memcpy(v, q, sizeof(v));
+        q += sizeof(v);
+        ctx->hJointKeepApart[j] = v[0];
+        ctx->hJointRemoved[j] = v[1];
+    }
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipSuccess;
+    snapArrays(ctx, nb, nf, nc, nj, h.subWorlds != 0, [&](const void* p, size_t bytes) {
+        if (e == hipSuccess && bytes > 0) e = hipMemcpyAsync((void*)p, q, bytes, hipMemcpyHostToDevice, s);
+        q += bytes;
+    });
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    HIPCHK(ctx, e);
+    if (nj > 0)
+        HIPCHK(ctx, hipMemcpy(ctx->hJointBodies.data(), ctx->jBodies, sizeof(int2) * nj, hipMemcpyDeviceToHost));
+    ctx->nBodies = nb;
+    ctx->nFixtures = nf;
+    ctx->nContacts = nc;
+    ctx->nJoints = nj;
+    ctx->gravity = mk(h.gravity[0], h.gravity[1]);
+    ctx->inv_dt0 = h.inv_dt0;
+    ctx->cellSize = h.cellSize;
+    ctx->cellSizeSet = h.cellSizeSet != 0;
+    ctx->claimTag = h.claimTag;
+    ctx->allowSleep = h.allowSleep != 0;
+    ctx->warmStarting = h.warmStarting != 0;
+    ctx->continuous = h.continuous != 0;
+    ctx->subStepping = h.subStepping != 0;
+    ctx->autoClearForces = h.autoClearForces != 0;
+    ctx->newFixture = h.newFixture != 0;
+    ctx->subWorlds = h.subWorlds != 0;
+    ctx->staticAsleep = h.staticAsleep != 0;
+    // colour claims only compare with the current tag: zero is older than any
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(ctx->capBodies)), dim3(256), 0, s, ctx->claim,
+                       ctx->capBodies, 0ull);
+    LAUNCHCHK(ctx);
+    ctx->jointColourStart.assign(h.jointColourStart, h.jointColourStart + kColourSlots + 1);
+    ctx->jointsDirty = false;
+    if ((rc = rebuildHash(ctx)) != B2G_OK) return rc;
+    if (nj > 0 && (rc = rebuildJointPairs(ctx)) != B2G_OK) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
 int b2g_set_graphs(b2g_context* ctx, int on) {
     if (!ctx) return B2G_ERR_ARG;
     ctx->graphs = on != 0;

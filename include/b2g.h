@@ -466,6 +466,18 @@ int b2g_override_contacts(b2g_context* ctx, const int32_t* ids, const int32_t* e
                           const float* restitution, int n);
 int b2g_step_solve(b2g_context* ctx);
 
+/* ------------------------------------------------------------- snapshot
+ * The world state a step reads (bodies, fixtures with fat boxes and move
+ * flags, the contact table with manifolds, warm-start impulses and colours,
+ * joints with their impulses and order, world settings and the warm-start
+ * ratio) as one byte buffer. A context restored from it steps exactly as
+ * the original does (the reference has only a text dump, b2World::Dump,
+ * Box2D/Dynamics/b2World.cpp:1271-1328). The network and logs are not
+ * included. b2g_snapshot with out == NULL returns the size in *size;
+ * b2g_restore needs a new context. */
+int b2g_snapshot(b2g_context* ctx, void* out, size_t cap, size_t* size);
+int b2g_restore(b2g_context* ctx, const void* in, size_t size);
+
 /* Captured solves (default off): the plain coloured solve of a step (warm
  * start, velocity passes, store, position integration, position passes,
  * transform sync) runs as one hipGraph launch once its layout has repeated

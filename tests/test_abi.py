@@ -46,6 +46,7 @@ def test_struct_layouts_match_the_library():
     assert ctypes.sizeof(bench.StepStats) == stats
     assert ctypes.sizeof(bench.PhaseProfile) == profile
     assert bench.MLP_PARAM_FLOATS * 4 == mlp
+    assert body == 26 * 4  # 2 int32 + 24 float
     assert revolute == 56  # 2 body indices, 9 floats, 3 switches
     assert schedule == E.SCHEDULE.itemsize
     assert params == E.STEP_PARAMS.itemsize

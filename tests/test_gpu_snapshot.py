@@ -1,0 +1,143 @@
+"""b2g_snapshot / b2g_restore: a world restored into a new context steps
+exactly as the world it was taken from.
+
+Each case runs a scene for a while through the box2d:: world, snapshots its
+context, restores the bytes into a fresh b2g_context and then steps both
+contexts through the C ABI: bodies, the contact table (fixture pairs, flags,
+colours, manifolds with their impulses) and the joint impulses must agree
+bit for bit after every step. The scenes cover sleeping islands, joints,
+sub-worlds, a removed joint and a removed fixture.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import engines as E
+
+pytestmark = pytest.mark.gpu
+BODY_WORDS = 26  # b2g_body_state: 2 int32 + 24 float
+
+
+def _params():
+    p = np.zeros(1, E.STEP_PARAMS)
+    p[0] = (1.0 / 60.0, 8, 3, 0, 0, 0.0)
+    return p
+
+
+def _state(lib, ctx):
+    """(contact meta, manifold bits) of a context's contact table."""
+    count = ctypes.c_int(0)
+    E.b2g_check(lib, ctx, lib.b2g_get_contacts(ctx, None, None, 0, ctypes.byref(count)))
+    nc = count.value
+    meta = np.zeros(4 * max(nc, 1), np.int32)
+    man = np.zeros(max(nc, 1), E.MANIFOLD)
+    E.b2g_check(lib, ctx, lib.b2g_get_contacts(ctx, E.ptr(meta), E.ptr(man), nc, ctypes.byref(count)))
+    return meta[:4 * nc], man[:nc].view(np.uint32)
+
+
+def _bodies(lib, ctx, n):
+    out = np.zeros(BODY_WORDS * n, np.uint32)
+    E.b2g_check(lib, ctx, lib.b2g_get_bodies(ctx, 0, n, E.ptr(out)))
+    return out.reshape(n, BODY_WORDS)
+
+
+def _joints(lib, ctx, n):
+    out = np.zeros(4 * max(n, 1), np.uint32)
+    if n:
+        E.b2g_check(lib, ctx, lib.b2g_get_joints(ctx, 0, n, E.ptr(out)))
+    return out[:4 * n]
+
+
+def _snapshot(lib, ctx):
+    size = ctypes.c_size_t(0)
+    E.b2g_check(lib, ctx, lib.b2g_snapshot(ctx, None, ctypes.c_size_t(0), ctypes.byref(size)))
+    buf = np.zeros(size.value, np.uint8)
+    E.b2g_check(lib, ctx, lib.b2g_snapshot(ctx, E.ptr(buf), ctypes.c_size_t(buf.size), ctypes.byref(size)))
+    assert size.value == buf.size
+    return buf
+
+
+def _fresh(lib):
+    c = ctypes.c_void_p()
+    assert lib.b2g_create(ctypes.byref(c), 0) == 0
+    return c
+
+
+def _edit_world(lib, w, ctx, kind, p):
+    """A removed joint and a removed fixture before the snapshot, so the
+    restored tables carry holes."""
+    if kind == "chains":
+        assert w.e.fn("world_destroy_joint")(w.h, 5) == 0
+    w.step(60)
+    gone = np.array([7], np.int32)
+    E.b2g_check(lib, ctx, lib.b2g_remove_fixtures(ctx, E.ptr(gone), 1))
+    for _ in range(5):
+        E.b2g_check(lib, ctx, lib.b2g_step(ctx, E.ptr(p)))
+
+
+@pytest.mark.parametrize("kind,n,arg,sleep,worlds", [
+    ("pyramid", 20, 0, True, 0),
+    ("chains", 4, 14, True, 0),
+    ("stress", 600, 11, False, 2),
+])
+def test_restored_world_steps_identically(gpu, kind, n, arg, sleep, worlds):
+    lib = E.b2g_lib()
+    if worlds:
+        w = E.World.worlds(gpu, kind, worlds, n, arg, sleep=sleep)
+    else:
+        w = gpu.world(kind, n, arg, sleep=sleep)
+    w.step(60)
+    a = w.device_context()
+    p = _params()
+    _edit_world(lib, w, a, kind, p)
+    nb = w.e.fn("world_body_count")(w.h)
+    nj = w.e.fn("world_joint_count")(w.h)
+    snap = _snapshot(lib, a)
+    b = _fresh(lib)
+    try:
+        E.b2g_check(lib, b, lib.b2g_restore(b, E.ptr(snap), ctypes.c_size_t(snap.size)))
+        # the restored context snapshots to the same bytes
+        assert np.array_equal(_snapshot(lib, b), snap)
+        for s in range(90):
+            E.b2g_check(lib, a, lib.b2g_step(a, E.ptr(p)))
+            E.b2g_check(lib, b, lib.b2g_step(b, E.ptr(p)))
+            if s % 15 == 14 or s < 2:
+                ba, bb = _bodies(lib, a, nb), _bodies(lib, b, nb)
+                bad = np.where(~np.all(ba == bb, axis=1))[0]
+                assert len(bad) == 0, f"step {s + 1}: bodies {bad[:8]} differ"
+                ma, na = _state(lib, a)
+                mb, nbm = _state(lib, b)
+                assert np.array_equal(ma, mb), f"step {s + 1}: contact tables differ"
+                assert np.array_equal(na, nbm), f"step {s + 1}: manifolds differ"
+                assert np.array_equal(_joints(lib, a, nj), _joints(lib, b, nj)), f"step {s + 1}: joints differ"
+        assert len(ma) > 0
+    finally:
+        lib.b2g_destroy(b)
+        w.close()
+
+
+def test_snapshot_errors(gpu):
+    lib = E.b2g_lib()
+    w = gpu.world("pyramid", 6, 0)
+    w.step(10)
+    a = w.device_context()
+    snap = _snapshot(lib, a)
+    size = ctypes.c_size_t(0)
+    small = np.zeros(16, np.uint8)
+    assert lib.b2g_snapshot(a, E.ptr(small), ctypes.c_size_t(16), ctypes.byref(size)) != 0
+    assert size.value == snap.size
+    b = _fresh(lib)
+    try:
+        assert lib.b2g_restore(b, E.ptr(snap), ctypes.c_size_t(snap.size - 4)) != 0   # truncated
+        bad = snap.copy()
+        bad[0] ^= 0xFF
+        assert lib.b2g_restore(b, E.ptr(bad), ctypes.c_size_t(bad.size)) != 0      # not a snapshot
+        E.b2g_check(lib, b, lib.b2g_restore(b, E.ptr(snap), ctypes.c_size_t(snap.size)))
+        assert lib.b2g_restore(b, E.ptr(snap), ctypes.c_size_t(snap.size)) != 0    # not a new context
+        assert b"new context" in lib.b2g_last_error(b)
+    finally:
+        lib.b2g_destroy(b)
+        w.close()
