@@ -353,12 +353,49 @@ inline Scene oneway(int n) {
     return s;
 }
 
+// Fast bodies for continuous collision: n small circles and boxes fired at
+// 80-200 m/s from the left, over a ground edge, through a column of five
+// dynamic boxes, at a static wall 0.2 m thick (3 m of travel per step would
+// tunnel through it without TOI). Two bodies in three are bullets, so TOI
+// events pair bullets with dynamic boxes as well as bodies with the wall.
+inline Scene bullets(int n, uint32_t seed) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-40.0f, 0.0f, 40.0f, 0.0f));
+    addBody(s, defaultBody(0, 12.0f, 5.0f));
+    addFixture(s, boxFixture(0.1f, 5.0f, 0.0f, 0.6f));
+    for (int k = 0; k < 5; ++k) {
+        addBody(s, defaultBody(2, 8.0f, 0.5f + 1.0f * (float)k));
+        addFixture(s, boxFixture(0.5f, 0.5f, 1.0f, 0.6f));
+    }
+    Uniform U(seed);
+    for (int i = 0; i < n; ++i) {
+        BodyDef b = defaultBody(2, -15.0f + 0.5f * (float)(i % 4), 0.8f + 0.7f * (float)(i / 4 % 12));
+        b.vx = 80.0f + 120.0f * U();
+        b.vy = -5.0f + 10.0f * U();
+        b.w = -10.0f + 20.0f * U();
+        b.bullet = (i % 3 != 2) ? 1 : 0;
+        addBody(s, b);
+        FixtureDef f;
+        if (i & 1) {
+            f = defaultFixture(kCircle);
+            f.radius = 0.1f + 0.1f * U();
+        } else {
+            const float h = 0.1f + 0.1f * U();
+            f = boxFixture(h, h, 1.0f, 0.6f);
+        }
+        f.density = 1.0f; f.friction = 0.6f; f.restitution = 0.1f;
+        addFixture(s, f);
+    }
+    return s;
+}
+
 // Scene by name: "hello", "pyramid" (n = rows, 0 -> 20), "pile" (n bodies,
 // seed), "world" (one batched-scene world of n bodies, seed), "pendulums"
 // (n pendulums), "chains" (n chains of `seed` links), "stress" (pile of n
 // bodies, seed, on a mat of n / 50 chains of 32 links), "plank" (n circles
-// on a plank), "oneway" (n boxes through a one-sided platform). Unknown
-// names give an empty scene.
+// on a plank), "oneway" (n boxes through a one-sided platform), "bullets"
+// (n fast bodies, seed). Unknown names give an empty scene.
 inline Scene byName(const char* kind, int n, uint32_t seed) {
     const std::string k(kind ? kind : "");
     if (k == "hello") return hello();
@@ -370,6 +407,7 @@ inline Scene byName(const char* kind, int n, uint32_t seed) {
     if (k == "stress") return stress(n, seed, n / 50);
     if (k == "plank") return plank(n > 0 ? n : 100);
     if (k == "oneway") return oneway(n > 0 ? n : 8);
+    if (k == "bullets") return bullets(n > 0 ? n : 24, seed);
     return Scene();
 }
 

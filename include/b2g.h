@@ -46,6 +46,17 @@ typedef struct b2g_vec2 { float x, y; } b2g_vec2;
 /* b2Transform {p, q(s, c)} (REF/Common/b2Math.h, b2Transform / b2Rot). */
 typedef struct b2g_transform { float px, py, s, c; } b2g_transform;
 
+/* b2Sweep (REF/Common/b2Math.h:383-405): the motion of a body over a step,
+ * centre of mass from c0 / a0 at time alpha0 to c / a at time 1. 36 bytes. */
+typedef struct b2g_sweep {
+    float localCenter[2];
+    float c0[2];
+    float c[2];
+    float a0;
+    float a;
+    float alpha0;
+} b2g_sweep;
+
 /* One convex shape in local body coordinates, 176 bytes.
  * polygon: count vertices/normals (<= 8), centroid, radius = 2*linearSlop
  *          (b2PolygonShape, REF/Collision/Shapes/b2PolygonShape.h:80-96)

@@ -21,6 +21,9 @@ namespace orc {
 // REF/Common/b2Settings.h:40-150
 constexpr float kPi = 3.14159265359f;
 constexpr float kLinearSlop = 0.005f;
+constexpr float kToiBaumgarte = 0.75f;
+constexpr int kMaxSubSteps = 8;
+constexpr int kMaxTOIContacts = 32;
 constexpr float kAngularSlop = 2.0f / 180.0f * kPi;
 constexpr float kMaxAngularCorrection = 8.0f / 180.0f * kPi;
 constexpr float kPolygonRadius = 2.0f * kLinearSlop;
@@ -814,7 +817,343 @@ void worldManifold(WorldManifold* wm, const b2g_manifold* m, const Xf& xfA, floa
 // ------------------------------------------------------------ world
 enum BodyType { kStatic = 0, kKinematic = 1, kDynamic = 2 };
 enum BodyFlag { kIsland = 1, kAwake = 2, kAutoSleep = 4, kBullet = 8, kFixedRotation = 16, kActive = 32 };
-enum ContactFlag { cIsland = 1, cTouching = 2, cEnabled = 4, cFilter = 8 };
+enum ContactFlag { cIsland = 1, cTouching = 2, cEnabled = 4, cFilter = 8, cToi = 0x20 };
+
+// ------------------------------------------------------------ time of impact
+// Convex proxy of a shape for GJK (b2DistanceProxy::Set,
+// REF/Collision/b2Distance.cpp:31-82): a circle is its centre with its
+// radius, a polygon its vertices with the polygon skin, an edge its two ends.
+struct Proxy { V2 v[8]; int count; float radius; };
+inline Proxy proxyOf(const b2g_shape* s) {
+    Proxy p;
+    p.radius = s->radius;
+    if (s->type == B2G_CIRCLE) {
+        p.count = 1;
+        p.v[0] = v(s->centroid);
+    } else if (s->type == B2G_EDGE) {
+        p.count = 2;
+        p.v[0] = v(s->verts[0]);
+        p.v[1] = v(s->verts[1]);
+    } else {
+        p.count = s->count;
+        for (int i = 0; i < s->count; ++i) p.v[i] = v(s->verts[i]);
+    }
+    return p;
+}
+// b2DistanceProxy::GetSupport, REF/Collision/b2Distance.h:109-124: first
+// vertex of largest projection.
+inline int support(const Proxy& p, V2 d) {
+    int best = 0;
+    float bestValue = dot(p.v[0], d);
+    for (int i = 1; i < p.count; ++i) {
+        const float value = dot(p.v[i], d);
+        if (value > bestValue) { best = i; bestValue = value; }
+    }
+    return best;
+}
+
+// b2Sweep (REF/Common/b2Math.h:383-405, 695-721)
+struct Sweep { V2 localCenter, c0, c; float a0, a, alpha0; };
+inline Xf sweepXf(const Sweep& sw, float beta) {
+    Xf xf;
+    xf.p = (1.0f - beta) * sw.c0 + beta * sw.c;
+    const float angle = (1.0f - beta) * sw.a0 + beta * sw.a;
+    xf.q = stepRot(angle);
+    xf.p -= mul(xf.q, sw.localCenter);
+    return xf;
+}
+inline void sweepAdvance(Sweep& sw, float alpha) {
+    const float beta = (alpha - sw.alpha0) / (1.0f - sw.alpha0);
+    sw.c0 += beta * (sw.c - sw.c0);
+    sw.a0 += beta * (sw.a - sw.a0);
+    sw.alpha0 = alpha;
+}
+inline void sweepNormalize(Sweep& sw) {
+    const float twoPi = 2.0f * kPi;
+    const float d = twoPi * floorf(sw.a0 / twoPi);
+    sw.a0 -= d;
+    sw.a -= d;
+}
+
+// GJK distance between two proxies (b2Distance with b2Simplex and
+// b2SimplexCache, REF/Collision/b2Distance.cpp:84-606), without radii.
+struct SimplexCache { float metric; int count; int indexA[3], indexB[3]; };
+struct SimplexVertex { V2 wA, wB, w; float a; int indexA, indexB; };
+struct Simplex {
+    SimplexVertex v[3];
+    int count;
+    float metric() const {
+        if (count == 2) return length(v[0].w - v[1].w);
+        if (count == 3) return cross(v[1].w - v[0].w, v[2].w - v[0].w);
+        return 0.0f;
+    }
+};
+inline void gjk(V2* pointA, V2* pointB, float* distance, SimplexCache* cache, const Proxy& pA, const Xf& xfA,
+                const Proxy& pB, const Xf& xfB) {
+    Simplex sx;
+    // ReadCache: reuse the cached vertices unless the metric changed a lot.
+    sx.count = cache->count;
+    for (int i = 0; i < sx.count; ++i) {
+        SimplexVertex& vx = sx.v[i];
+        vx.indexA = cache->indexA[i];
+        vx.indexB = cache->indexB[i];
+        vx.wA = mul(xfA, pA.v[vx.indexA]);
+        vx.wB = mul(xfB, pB.v[vx.indexB]);
+        vx.w = vx.wB - vx.wA;
+        vx.a = 0.0f;
+    }
+    if (sx.count > 1) {
+        const float m1 = cache->metric, m2 = sx.metric();
+        if (m2 < 0.5f * m1 || 2.0f * m1 < m2 || m2 < FLT_EPSILON) sx.count = 0;
+    }
+    if (sx.count == 0) {
+        SimplexVertex& vx = sx.v[0];
+        vx.indexA = 0;
+        vx.indexB = 0;
+        vx.wA = mul(xfA, pA.v[0]);
+        vx.wB = mul(xfB, pB.v[0]);
+        vx.w = vx.wB - vx.wA;
+        vx.a = 1.0f;
+        sx.count = 1;
+    }
+    int saveA[3], saveB[3];
+    for (int iter = 0; iter < 20;) {
+        const int saveCount = sx.count;
+        for (int i = 0; i < saveCount; ++i) { saveA[i] = sx.v[i].indexA; saveB[i] = sx.v[i].indexB; }
+        if (sx.count == 2) {
+            // closest feature of a segment (Solve2)
+            const V2 w1 = sx.v[0].w, w2 = sx.v[1].w;
+            const V2 e12 = w2 - w1;
+            const float d12_2 = -dot(w1, e12);
+            const float d12_1 = dot(w2, e12);
+            if (d12_2 <= 0.0f) {
+                sx.v[0].a = 1.0f;
+                sx.count = 1;
+            } else if (d12_1 <= 0.0f) {
+                sx.v[1].a = 1.0f;
+                sx.count = 1;
+                sx.v[0] = sx.v[1];
+            } else {
+                const float inv = 1.0f / (d12_1 + d12_2);
+                sx.v[0].a = d12_1 * inv;
+                sx.v[1].a = d12_2 * inv;
+                sx.count = 2;
+            }
+        } else if (sx.count == 3) {
+            // closest feature of a triangle (Solve3)
+            const V2 w1 = sx.v[0].w, w2 = sx.v[1].w, w3 = sx.v[2].w;
+            const V2 e12 = w2 - w1, e13 = w3 - w1, e23 = w3 - w2;
+            const float d12_1 = dot(w2, e12), d12_2 = -dot(w1, e12);
+            const float d13_1 = dot(w3, e13), d13_2 = -dot(w1, e13);
+            const float d23_1 = dot(w3, e23), d23_2 = -dot(w2, e23);
+            const float n123 = cross(e12, e13);
+            const float d123_1 = n123 * cross(w2, w3);
+            const float d123_2 = n123 * cross(w3, w1);
+            const float d123_3 = n123 * cross(w1, w2);
+            if (d12_2 <= 0.0f && d13_2 <= 0.0f) {
+                sx.v[0].a = 1.0f;
+                sx.count = 1;
+            } else if (d12_1 > 0.0f && d12_2 > 0.0f && d123_3 <= 0.0f) {
+                const float inv = 1.0f / (d12_1 + d12_2);
+                sx.v[0].a = d12_1 * inv;
+                sx.v[1].a = d12_2 * inv;
+                sx.count = 2;
+            } else if (d13_1 > 0.0f && d13_2 > 0.0f && d123_2 <= 0.0f) {
+                const float inv = 1.0f / (d13_1 + d13_2);
+                sx.v[0].a = d13_1 * inv;
+                sx.v[2].a = d13_2 * inv;
+                sx.count = 2;
+                sx.v[1] = sx.v[2];
+            } else if (d12_1 <= 0.0f && d23_2 <= 0.0f) {
+                sx.v[1].a = 1.0f;
+                sx.count = 1;
+                sx.v[0] = sx.v[1];
+            } else if (d13_1 <= 0.0f && d23_1 <= 0.0f) {
+                sx.v[2].a = 1.0f;
+                sx.count = 1;
+                sx.v[0] = sx.v[2];
+            } else if (d23_1 > 0.0f && d23_2 > 0.0f && d123_1 <= 0.0f) {
+                const float inv = 1.0f / (d23_1 + d23_2);
+                sx.v[1].a = d23_1 * inv;
+                sx.v[2].a = d23_2 * inv;
+                sx.count = 2;
+                sx.v[0] = sx.v[2];
+            } else {
+                const float inv = 1.0f / (d123_1 + d123_2 + d123_3);
+                sx.v[0].a = d123_1 * inv;
+                sx.v[1].a = d123_2 * inv;
+                sx.v[2].a = d123_3 * inv;
+                sx.count = 3;
+            }
+        }
+        if (sx.count == 3) break;  // the origin is inside the triangle
+        // search direction towards the origin
+        V2 d;
+        if (sx.count == 1) {
+            d = -sx.v[0].w;
+        } else {
+            const V2 e12 = sx.v[1].w - sx.v[0].w;
+            d = cross(e12, -sx.v[0].w) > 0.0f ? cross(1.0f, e12) : cross(e12, 1.0f);
+        }
+        if (dot(d, d) < FLT_EPSILON * FLT_EPSILON) break;
+        SimplexVertex& nv = sx.v[sx.count];
+        nv.indexA = support(pA, mulT(xfA.q, -d));
+        nv.wA = mul(xfA, pA.v[nv.indexA]);
+        nv.indexB = support(pB, mulT(xfB.q, d));
+        nv.wB = mul(xfB, pB.v[nv.indexB]);
+        nv.w = nv.wB - nv.wA;
+        ++iter;
+        bool duplicate = false;
+        for (int i = 0; i < saveCount; ++i) {
+            if (nv.indexA == saveA[i] && nv.indexB == saveB[i]) { duplicate = true; break; }
+        }
+        if (duplicate) break;
+        ++sx.count;
+    }
+    // witness points
+    if (sx.count == 1) {
+        *pointA = sx.v[0].wA;
+        *pointB = sx.v[0].wB;
+    } else if (sx.count == 2) {
+        *pointA = sx.v[0].a * sx.v[0].wA + sx.v[1].a * sx.v[1].wA;
+        *pointB = sx.v[0].a * sx.v[0].wB + sx.v[1].a * sx.v[1].wB;
+    } else {
+        *pointA = sx.v[0].a * sx.v[0].wA + sx.v[1].a * sx.v[1].wA + sx.v[2].a * sx.v[2].wA;
+        *pointB = *pointA;
+    }
+    *distance = length(*pointA - *pointB);
+    cache->metric = sx.metric();
+    cache->count = sx.count;
+    for (int i = 0; i < sx.count; ++i) { cache->indexA[i] = sx.v[i].indexA; cache->indexB[i] = sx.v[i].indexB; }
+}
+
+// Separating-axis function of the TOI root finder (b2SeparationFunction,
+// REF/Collision/b2TimeOfImpact.cpp:35-253).
+struct SepFn {
+    enum { kPoints, kFaceA, kFaceB };
+    const Proxy* pA;
+    const Proxy* pB;
+    Sweep sA, sB;
+    int type;
+    V2 localPoint, axis;
+
+    float init(const SimplexCache& cache, float t1) {
+        const Xf xfA = sweepXf(sA, t1), xfB = sweepXf(sB, t1);
+        if (cache.count == 1) {
+            type = kPoints;
+            axis = mul(xfB, pB->v[cache.indexB[0]]) - mul(xfA, pA->v[cache.indexA[0]]);
+            return normalize(axis);
+        }
+        if (cache.indexA[0] == cache.indexA[1]) {
+            type = kFaceB;
+            const V2 b1 = pB->v[cache.indexB[0]], b2 = pB->v[cache.indexB[1]];
+            axis = cross(b2 - b1, 1.0f);
+            normalize(axis);
+            const V2 normal = mul(xfB.q, axis);
+            localPoint = 0.5f * (b1 + b2);
+            const V2 pointB = mul(xfB, localPoint);
+            const V2 pointA = mul(xfA, pA->v[cache.indexA[0]]);
+            float sep = dot(pointA - pointB, normal);
+            if (sep < 0.0f) { axis = -axis; sep = -sep; }
+            return sep;
+        }
+        type = kFaceA;
+        const V2 a1 = pA->v[cache.indexA[0]], a2 = pA->v[cache.indexA[1]];
+        axis = cross(a2 - a1, 1.0f);
+        normalize(axis);
+        const V2 normal = mul(xfA.q, axis);
+        localPoint = 0.5f * (a1 + a2);
+        const V2 pointA = mul(xfA, localPoint);
+        const V2 pointB = mul(xfB, pB->v[cache.indexB[0]]);
+        float sep = dot(pointB - pointA, normal);
+        if (sep < 0.0f) { axis = -axis; sep = -sep; }
+        return sep;
+    }
+    float findMin(int* iA, int* iB, float t) const {
+        const Xf xfA = sweepXf(sA, t), xfB = sweepXf(sB, t);
+        if (type == kPoints) {
+            *iA = support(*pA, mulT(xfA.q, axis));
+            *iB = support(*pB, mulT(xfB.q, -axis));
+            return dot(mul(xfB, pB->v[*iB]) - mul(xfA, pA->v[*iA]), axis);
+        }
+        if (type == kFaceA) {
+            const V2 normal = mul(xfA.q, axis);
+            const V2 pointA = mul(xfA, localPoint);
+            *iA = -1;
+            *iB = support(*pB, mulT(xfB.q, -normal));
+            return dot(mul(xfB, pB->v[*iB]) - pointA, normal);
+        }
+        const V2 normal = mul(xfB.q, axis);
+        const V2 pointB = mul(xfB, localPoint);
+        *iB = -1;
+        *iA = support(*pA, mulT(xfA.q, -normal));
+        return dot(mul(xfA, pA->v[*iA]) - pointB, normal);
+    }
+    float eval(int iA, int iB, float t) const {
+        const Xf xfA = sweepXf(sA, t), xfB = sweepXf(sB, t);
+        if (type == kPoints) return dot(mul(xfB, pB->v[iB]) - mul(xfA, pA->v[iA]), axis);
+        if (type == kFaceA) {
+            const V2 normal = mul(xfA.q, axis);
+            return dot(mul(xfB, pB->v[iB]) - mul(xfA, localPoint), normal);
+        }
+        const V2 normal = mul(xfB.q, axis);
+        return dot(mul(xfA, pA->v[iA]) - mul(xfB, localPoint), normal);
+    }
+};
+
+// b2TimeOfImpact, REF/Collision/b2TimeOfImpact.cpp:257-489 (conservative
+// advancement on separating axes with a bisection/secant root finder).
+enum { kToiUnknown, kToiFailed, kToiOverlapped, kToiTouching, kToiSeparated };
+inline int timeOfImpact(float* tOut, const Proxy& pA, const Proxy& pB, Sweep sA, Sweep sB, float tMax) {
+    int state = kToiUnknown;
+    *tOut = tMax;
+    sweepNormalize(sA);
+    sweepNormalize(sB);
+    const float totalRadius = pA.radius + pB.radius;
+    const float target = vmax(kLinearSlop, totalRadius - 3.0f * kLinearSlop);
+    const float tolerance = 0.25f * kLinearSlop;
+    float t1 = 0.0f;
+    SimplexCache cache;
+    cache.count = 0;
+    cache.metric = 0.0f;
+    for (int iter = 0;;) {
+        const Xf xfA = sweepXf(sA, t1), xfB = sweepXf(sB, t1);
+        V2 wa, wb;
+        float dist;
+        gjk(&wa, &wb, &dist, &cache, pA, xfA, pB, xfB);
+        if (dist <= 0.0f) { state = kToiOverlapped; *tOut = 0.0f; break; }
+        if (dist < target + tolerance) { state = kToiTouching; *tOut = t1; break; }
+        SepFn fcn;
+        fcn.pA = &pA; fcn.pB = &pB; fcn.sA = sA; fcn.sB = sB;
+        fcn.init(cache, t1);
+        bool done = false;
+        float t2 = tMax;
+        for (int pushBack = 0;;) {
+            int iA, iB;
+            float s2 = fcn.findMin(&iA, &iB, t2);
+            if (s2 > target + tolerance) { state = kToiSeparated; *tOut = tMax; done = true; break; }
+            if (s2 > target - tolerance) { t1 = t2; break; }
+            float s1 = fcn.eval(iA, iB, t1);
+            if (s1 < target - tolerance) { state = kToiFailed; *tOut = t1; done = true; break; }
+            if (s1 <= target + tolerance) { state = kToiTouching; *tOut = t1; done = true; break; }
+            float a1 = t1, a2 = t2;
+            for (int rootIters = 0;;) {
+                const float t = (rootIters & 1) ? a1 + (target - s1) * (a2 - a1) / (s2 - s1) : 0.5f * (a1 + a2);
+                ++rootIters;
+                const float sv = fcn.eval(iA, iB, t);
+                if (absf(sv - target) < tolerance) { t2 = t; break; }
+                if (sv > target) { a1 = t; s1 = sv; }
+                else { a2 = t; s2 = sv; }
+                if (rootIters == 50) break;
+            }
+            if (++pushBack == 8) break;
+        }
+        ++iter;
+        if (done) break;
+        if (iter == 20) { state = kToiFailed; *tOut = t1; break; }
+    }
+    return state;
+}
 
 struct Body {
     int type;
@@ -822,6 +1161,7 @@ struct Body {
     Xf xf;
     V2 localCenter, c0, c;
     float a0, a;
+    float alpha0;  // sweep time of c0 / a0 inside a TOI pass
     V2 v;
     float w;
     V2 force;
@@ -855,6 +1195,8 @@ struct Contact {
     int flags;
     float friction, restitution, tangentSpeed;
     b2g_manifold manifold;
+    float toi;       // cached time of impact (valid with cToi)
+    int toiCount;    // TOI events of this contact in the step
     int prev, next;  // world list
     Edge node[2];    // node[0] on body A (other = B), node[1] on body B
     bool alive;
@@ -1025,6 +1367,7 @@ struct World {
         b.localCenter = mk(0.0f, 0.0f);
         b.c0 = b.xf.p; b.c = b.xf.p;
         b.a0 = d.angle; b.a = d.angle;
+        b.alpha0 = 0.0f;
         b.v = mk(d.vx, d.vy);
         b.w = d.w;
         b.linearDamping = d.linearDamping;
@@ -1240,6 +1583,8 @@ struct World {
         c.restitution = fixtures[A].restitution > fixtures[B].restitution ? fixtures[A].restitution
                                                                           : fixtures[B].restitution;
         c.tangentSpeed = 0.0f;
+        c.toi = 1.0f;
+        c.toiCount = 0;
         c.alive = true;
         c.prev = -1;
         c.next = contactHead;
@@ -1436,30 +1781,9 @@ struct World {
         b.xf.p = b.c - mul(b.xf.q, b.localCenter);
     }
 
-    // b2Island::Solve and the b2ContactSolver it drives,
-    // REF/Dynamics/b2Island.cpp:182-383, REF/Dynamics/Contacts/b2ContactSolver.cpp:52-744
-    void solveIsland(float h, float inv_dt, float dtRatio, int velIters, int posIters) {
-        (void)inv_dt;
-        const int nb = (int)islandBodies.size();
+    // b2ContactSolver ctor over the island's contacts, REF/Dynamics/Contacts/b2ContactSolver.cpp:52-140
+    void contactSolverInit(float dtRatio, bool warm) {
         const int nc = (int)islandContacts.size();
-        posC.resize(nb); posA.resize(nb); velV.resize(nb); velW.resize(nb);
-        for (int i = 0; i < nb; ++i) {
-            Body& b = bodies[islandBodies[i]];
-            V2 c = b.c;
-            float a = b.a;
-            V2 vv = b.v;
-            float w = b.w;
-            b.c0 = b.c;
-            b.a0 = b.a;
-            if (b.type == kDynamic) {
-                vv += h * (b.gravityScale * gravity + b.invMass * b.force);
-                w += h * b.invI * b.torque;
-                vv *= 1.0f / (1.0f + h * b.linearDamping);
-                w *= 1.0f / (1.0f + h * b.angularDamping);
-            }
-            posC[i] = c; posA[i] = a; velV[i] = vv; velW[i] = w;
-        }
-        // b2ContactSolver ctor (REF b2ContactSolver.cpp:52-140)
         vcs.resize(nc); pcs.resize(nc);
         for (int i = 0; i < nc; ++i) {
             Contact& ct = contacts[islandContacts[i]];
@@ -1492,7 +1816,7 @@ struct World {
             pc.type = m.type;
             for (int j = 0; j < m.pointCount; ++j) {
                 VelPoint& vp = vc.points[j];
-                if (warmStarting) {
+                if (warm) {
                     vp.normalImpulse = dtRatio * m.points[j].normalImpulse;
                     vp.tangentImpulse = dtRatio * m.points[j].tangentImpulse;
                 } else {
@@ -1504,7 +1828,11 @@ struct World {
                 pc.localPoints[j] = v(m.points[j].localPoint);
             }
         }
-        // InitializeVelocityConstraints (REF :143-252)
+    }
+
+    // b2ContactSolver::InitializeVelocityConstraints, REF/Dynamics/Contacts/b2ContactSolver.cpp:143-252
+    void initVelocityConstraints() {
+        const int nc = (int)islandContacts.size();
         for (int i = 0; i < nc; ++i) {
             VelConstraint& vc = vcs[i];
             PosConstraint& pc = pcs[i];
@@ -1564,6 +1892,33 @@ struct World {
                 }
             }
         }
+    }
+
+    // b2Island::Solve and the b2ContactSolver it drives,
+    // REF/Dynamics/b2Island.cpp:182-383, REF/Dynamics/Contacts/b2ContactSolver.cpp:52-744
+    void solveIsland(float h, float inv_dt, float dtRatio, int velIters, int posIters) {
+        (void)inv_dt;
+        const int nb = (int)islandBodies.size();
+        const int nc = (int)islandContacts.size();
+        posC.resize(nb); posA.resize(nb); velV.resize(nb); velW.resize(nb);
+        for (int i = 0; i < nb; ++i) {
+            Body& b = bodies[islandBodies[i]];
+            V2 c = b.c;
+            float a = b.a;
+            V2 vv = b.v;
+            float w = b.w;
+            b.c0 = b.c;
+            b.a0 = b.a;
+            if (b.type == kDynamic) {
+                vv += h * (b.gravityScale * gravity + b.invMass * b.force);
+                w += h * b.invI * b.torque;
+                vv *= 1.0f / (1.0f + h * b.linearDamping);
+                w *= 1.0f / (1.0f + h * b.angularDamping);
+            }
+            posC[i] = c; posA[i] = a; velV[i] = vv; velW[i] = w;
+        }
+        contactSolverInit(dtRatio, warmStarting);
+        initVelocityConstraints();
         // WarmStart (REF :254-292)
         if (warmStarting) {
             for (int i = 0; i < nc; ++i) {
@@ -1925,6 +2280,37 @@ struct World {
         velV[iB] = vB; velW[iB] = wB;
     }
 
+    // b2PositionSolverManifold::Initialize, REF/Dynamics/Contacts/b2ContactSolver.cpp:622-671
+    static void positionManifold(const PosConstraint& pc, const Xf& xfA, const Xf& xfB, int j, V2* normalOut,
+                                 V2* pointOut, float* separationOut) {
+        V2 normal, point;
+        float separation;
+        if (pc.type == B2G_MANIFOLD_CIRCLES) {
+            V2 pointA = mul(xfA, pc.localPoint);
+            V2 pointB = mul(xfB, pc.localPoints[0]);
+            normal = pointB - pointA;
+            normalize(normal);
+            point = 0.5f * (pointA + pointB);
+            separation = dot(pointB - pointA, normal) - pc.radiusA - pc.radiusB;
+        } else if (pc.type == B2G_MANIFOLD_FACE_A) {
+            normal = mul(xfA.q, pc.localNormal);
+            V2 planePoint = mul(xfA, pc.localPoint);
+            V2 clipPoint = mul(xfB, pc.localPoints[j]);
+            separation = dot(clipPoint - planePoint, normal) - pc.radiusA - pc.radiusB;
+            point = clipPoint;
+        } else {
+            normal = mul(xfB.q, pc.localNormal);
+            V2 planePoint = mul(xfB, pc.localPoint);
+            V2 clipPoint = mul(xfA, pc.localPoints[j]);
+            separation = dot(clipPoint - planePoint, normal) - pc.radiusA - pc.radiusB;
+            point = clipPoint;
+            normal = -normal;
+        }
+        *normalOut = normal;
+        *pointOut = point;
+        *separationOut = separation;
+    }
+
     // SolvePositionConstraints + b2PositionSolverManifold (REF :614-744)
     bool solvePositions() {
         float minSeparation = 0.0f;
@@ -1943,27 +2329,7 @@ struct World {
                 xfB.p = cB - mul(xfB.q, lcB);
                 V2 normal, point;
                 float separation;
-                if (pc.type == B2G_MANIFOLD_CIRCLES) {
-                    V2 pointA = mul(xfA, pc.localPoint);
-                    V2 pointB = mul(xfB, pc.localPoints[0]);
-                    normal = pointB - pointA;
-                    normalize(normal);
-                    point = 0.5f * (pointA + pointB);
-                    separation = dot(pointB - pointA, normal) - pc.radiusA - pc.radiusB;
-                } else if (pc.type == B2G_MANIFOLD_FACE_A) {
-                    normal = mul(xfA.q, pc.localNormal);
-                    V2 planePoint = mul(xfA, pc.localPoint);
-                    V2 clipPoint = mul(xfB, pc.localPoints[j]);
-                    separation = dot(clipPoint - planePoint, normal) - pc.radiusA - pc.radiusB;
-                    point = clipPoint;
-                } else {
-                    normal = mul(xfB.q, pc.localNormal);
-                    V2 planePoint = mul(xfB, pc.localPoint);
-                    V2 clipPoint = mul(xfA, pc.localPoints[j]);
-                    separation = dot(clipPoint - planePoint, normal) - pc.radiusA - pc.radiusB;
-                    point = clipPoint;
-                    normal = -normal;
-                }
+                positionManifold(pc, xfA, xfB, j, &normal, &point, &separation);
                 V2 rA = point - cA;
                 V2 rB = point - cB;
                 minSeparation = vmin(minSeparation, separation);
@@ -2050,7 +2416,252 @@ struct World {
         findNewContacts();
     }
 
-    // b2World::Step, REF/Dynamics/b2World.cpp:899-964 (SolveTOI not restated)
+    Sweep sweepOf(const Body& b) const {
+        Sweep sw;
+        sw.localCenter = b.localCenter; sw.c0 = b.c0; sw.c = b.c; sw.a0 = b.a0; sw.a = b.a; sw.alpha0 = b.alpha0;
+        return sw;
+    }
+    // b2Sweep::Advance on a body's sweep, REF/Common/b2Math.h:705-712
+    void advanceSweep(Body& b, float alpha) {
+        Sweep sw = sweepOf(b);
+        sweepAdvance(sw, alpha);
+        b.c0 = sw.c0; b.a0 = sw.a0; b.alpha0 = sw.alpha0;
+    }
+    // b2Body::Advance, REF/Dynamics/b2Body.h:842-850
+    void advanceBody(Body& b, float alpha) {
+        advanceSweep(b, alpha);
+        b.c = b.c0;
+        b.a = b.a0;
+        synchronizeTransform(b);
+    }
+
+    // b2ContactSolver::SolveTOIPositionConstraints, REF/Dynamics/Contacts/b2ContactSolver.cpp:753-841:
+    // only the two TOI bodies move, with the TOI Baumgarte factor.
+    bool solveTOIPositions(int toiA, int toiB) {
+        float minSeparation = 0.0f;
+        for (size_t i = 0; i < pcs.size(); ++i) {
+            PosConstraint& pc = pcs[i];
+            const int iA = pc.indexA, iB = pc.indexB;
+            const V2 lcA = pc.localCenterA, lcB = pc.localCenterB;
+            float mA = 0.0f, IA = 0.0f, mB = 0.0f, IB = 0.0f;
+            if (iA == toiA || iA == toiB) { mA = pc.invMassA; IA = pc.invIA; }
+            if (iB == toiA || iB == toiB) { mB = pc.invMassB; IB = pc.invIB; }
+            V2 cA = posC[iA]; float aA = posA[iA];
+            V2 cB = posC[iB]; float aB = posA[iB];
+            for (int j = 0; j < pc.pointCount; ++j) {
+                Xf xfA, xfB;
+                xfA.q = stepRot(aA);
+                xfB.q = stepRot(aB);
+                xfA.p = cA - mul(xfA.q, lcA);
+                xfB.p = cB - mul(xfB.q, lcB);
+                V2 normal, point;
+                float separation;
+                positionManifold(pc, xfA, xfB, j, &normal, &point, &separation);
+                const V2 rA = point - cA;
+                const V2 rB = point - cB;
+                minSeparation = vmin(minSeparation, separation);
+                const float C = clampf(kToiBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, 0.0f);
+                const float rnA = cross(rA, normal);
+                const float rnB = cross(rB, normal);
+                const float K = mA + mB + IA * rnA * rnA + IB * rnB * rnB;
+                const float impulse = K > 0.0f ? -C / K : 0.0f;
+                const V2 P = impulse * normal;
+                cA -= mA * P;
+                aA -= IA * cross(rA, P);
+                cB += mB * P;
+                aB += IB * cross(rB, P);
+            }
+            posC[iA] = cA; posA[iA] = aA;
+            posC[iB] = cB; posA[iB] = aB;
+        }
+        return minSeparation >= -1.5f * kLinearSlop;
+    }
+
+    // b2Island::SolveTOI, REF/Dynamics/b2Island.cpp:385-515 (no warm start,
+    // impulses not stored; the TOI pair is islandBodies[0] and [1]).
+    void solveTOIIsland(float h, int velIters) {
+        const int nb = (int)islandBodies.size();
+        posC.resize(nb); posA.resize(nb); velV.resize(nb); velW.resize(nb);
+        for (int i = 0; i < nb; ++i) {
+            const Body& b = bodies[islandBodies[i]];
+            posC[i] = b.c; posA[i] = b.a; velV[i] = b.v; velW[i] = b.w;
+        }
+        contactSolverInit(1.0f, false);
+        for (int i = 0; i < 20; ++i) {
+            if (solveTOIPositions(0, 1)) break;
+        }
+        for (int k = 0; k < 2; ++k) {
+            Body& b = bodies[islandBodies[k]];
+            b.c0 = posC[k];
+            b.a0 = posA[k];
+        }
+        initVelocityConstraints();
+        for (int it = 0; it < velIters; ++it)
+            for (size_t i = 0; i < vcs.size(); ++i) solveVelocity(vcs[i]);
+        for (int i = 0; i < nb; ++i) {
+            V2 c = posC[i]; float a = posA[i];
+            V2 vv = velV[i]; float w = velW[i];
+            const V2 translation = h * vv;
+            if (dot(translation, translation) > kMaxTranslationSq) {
+                const float ratio = kMaxTranslation / length(translation);
+                vv *= ratio;
+            }
+            const float rotation = h * w;
+            if (rotation * rotation > kMaxRotationSq) {
+                const float ratio = kMaxRotation / absf(rotation);
+                w *= ratio;
+            }
+            c += h * vv;
+            a += h * w;
+            Body& b = bodies[islandBodies[i]];
+            b.c = c; b.a = a; b.v = vv; b.w = w;
+            synchronizeTransform(b);
+        }
+    }
+
+    // TOI of one contact over the rest of the step, the two sweeps first
+    // brought to a common start (REF/Dynamics/b2World.cpp:645-689).
+    float contactTOI(Contact& c) {
+        Body& bA = bodies[fixtures[c.fA].body];
+        Body& bB = bodies[fixtures[c.fB].body];
+        float alpha0 = bA.alpha0;
+        if (bA.alpha0 < bB.alpha0) {
+            alpha0 = bB.alpha0;
+            advanceSweep(bA, alpha0);
+        } else if (bB.alpha0 < bA.alpha0) {
+            alpha0 = bA.alpha0;
+            advanceSweep(bB, alpha0);
+        }
+        const Proxy pA = proxyOf(&fixtures[c.fA].shape), pB = proxyOf(&fixtures[c.fB].shape);
+        float beta;
+        const int state = timeOfImpact(&beta, pA, pB, sweepOf(bA), sweepOf(bB), 1.0f);
+        return state == kToiTouching ? vmin(alpha0 + (1.0f - alpha0) * beta, 1.0f) : 1.0f;
+    }
+
+    // b2World::SolveTOI, REF/Dynamics/b2World.cpp:579-897: repeatedly take
+    // the earliest TOI over the contact list, move its two bodies there,
+    // solve a small island of them and the static, kinematic and bullet
+    // bodies they touch for the rest of the step, and find new pairs.
+    void solveTOI(float dt, int velIters) {
+        if (stepComplete) {
+            for (auto& b : bodies) { b.flags &= ~kIsland; b.alpha0 = 0.0f; }
+            for (int ci = contactHead; ci != -1; ci = contacts[ci].next) {
+                Contact& c = contacts[ci];
+                c.flags &= ~(cToi | cIsland);
+                c.toiCount = 0;
+                c.toi = 1.0f;
+            }
+        }
+        for (;;) {
+            int minContact = -1;
+            float minAlpha = 1.0f;
+            for (int ci = contactHead; ci != -1; ci = contacts[ci].next) {
+                Contact& c = contacts[ci];
+                if ((c.flags & cEnabled) == 0) continue;
+                if (c.toiCount > kMaxSubSteps) continue;
+                float alpha = 1.0f;
+                if (c.flags & cToi) {
+                    alpha = c.toi;
+                } else {
+                    const Fixture& fA = fixtures[c.fA];
+                    const Fixture& fB = fixtures[c.fB];
+                    if (fA.sensor || fB.sensor) continue;
+                    const Body& bA = bodies[fA.body];
+                    const Body& bB = bodies[fB.body];
+                    const bool activeA = (bA.flags & kAwake) && bA.type != kStatic;
+                    const bool activeB = (bB.flags & kAwake) && bB.type != kStatic;
+                    if (!activeA && !activeB) continue;
+                    const bool collideA = (bA.flags & kBullet) || bA.type != kDynamic;
+                    const bool collideB = (bB.flags & kBullet) || bB.type != kDynamic;
+                    if (!collideA && !collideB) continue;
+                    alpha = contactTOI(c);
+                    c.toi = alpha;
+                    c.flags |= cToi;
+                }
+                if (alpha < minAlpha) { minContact = ci; minAlpha = alpha; }
+            }
+            if (minContact == -1 || 1.0f - 10.0f * FLT_EPSILON < minAlpha) {
+                stepComplete = true;
+                break;
+            }
+            Contact& mc = contacts[minContact];
+            const int ia = fixtures[mc.fA].body, ib = fixtures[mc.fB].body;
+            Body& bA = bodies[ia];
+            Body& bB = bodies[ib];
+            const Sweep backupA = sweepOf(bA), backupB = sweepOf(bB);
+            advanceBody(bA, minAlpha);
+            advanceBody(bB, minAlpha);
+            updateContact(minContact);
+            mc.flags &= ~cToi;
+            ++mc.toiCount;
+            if ((mc.flags & cEnabled) == 0 || (mc.flags & cTouching) == 0) {
+                mc.flags &= ~cEnabled;
+                restoreSweep(bA, backupA);
+                restoreSweep(bB, backupB);
+                continue;
+            }
+            setAwake(bA, true);
+            setAwake(bB, true);
+            islandBodies.clear();
+            islandContacts.clear();
+            bA.islandIndex = 0; islandBodies.push_back(ia);
+            bB.islandIndex = 1; islandBodies.push_back(ib);
+            islandContacts.push_back(minContact);
+            bA.flags |= kIsland;
+            bB.flags |= kIsland;
+            mc.flags |= cIsland;
+            const int pair[2] = {ia, ib};
+            for (int k = 0; k < 2; ++k) {
+                Body& body = bodies[pair[k]];
+                if (body.type != kDynamic) continue;
+                for (int e = body.edgeHead; e != -1; e = edgeRef(e).next) {
+                    if ((int)islandBodies.size() == 2 * kMaxTOIContacts) break;
+                    if ((int)islandContacts.size() == kMaxTOIContacts) break;
+                    const int ci = e >> 1;
+                    Contact& c = contacts[ci];
+                    if (c.flags & cIsland) continue;
+                    const int oi = edgeOther(e);
+                    Body& other = bodies[oi];
+                    if (other.type == kDynamic && (body.flags & kBullet) == 0 && (other.flags & kBullet) == 0) continue;
+                    if (fixtures[c.fA].sensor || fixtures[c.fB].sensor) continue;
+                    const Sweep backup = sweepOf(other);
+                    if ((other.flags & kIsland) == 0) advanceBody(other, minAlpha);
+                    updateContact(ci);
+                    if ((c.flags & cEnabled) == 0 || (c.flags & cTouching) == 0) {
+                        restoreSweep(other, backup);
+                        continue;
+                    }
+                    c.flags |= cIsland;
+                    islandContacts.push_back(ci);
+                    if (other.flags & kIsland) continue;
+                    other.flags |= kIsland;
+                    if (other.type != kStatic) setAwake(other, true);
+                    other.islandIndex = (int)islandBodies.size();
+                    islandBodies.push_back(oi);
+                }
+            }
+            solveTOIIsland((1.0f - minAlpha) * dt, velIters);
+            for (int bi : islandBodies) {
+                Body& b = bodies[bi];
+                b.flags &= ~kIsland;
+                if (b.type != kDynamic) continue;
+                synchronizeFixtures(b);
+                for (int e = b.edgeHead; e != -1; e = edgeRef(e).next) contacts[e >> 1].flags &= ~(cToi | cIsland);
+            }
+            findNewContacts();
+            if (subStepping) {
+                stepComplete = false;
+                break;
+            }
+        }
+    }
+    // Restores a body's sweep from a copy and re-derives its transform.
+    void restoreSweep(Body& b, const Sweep& saved) {
+        b.c0 = saved.c0; b.c = saved.c; b.a0 = saved.a0; b.a = saved.a; b.alpha0 = saved.alpha0;
+        synchronizeTransform(b);
+    }
+
+    // b2World::Step, REF/Dynamics/b2World.cpp:899-964
     void step(float dt, int velIters, int posIters) {
         if (newFixture) {
             findNewContacts();
@@ -2060,7 +2671,7 @@ struct World {
         float dtRatio = inv_dt0 * dt;
         collideAll();
         if (stepComplete && dt > 0.0f) solve(dt, inv_dt, dtRatio, velIters, posIters);
-        if (continuous && dt > 0.0f) unsupported |= 1;
+        if (continuous && dt > 0.0f) solveTOI(dt, velIters);
         if (dt > 0.0f) inv_dt0 = inv_dt;
         if (autoClearForces) {
             for (auto& b : bodies) { b.force = mk(0.0f, 0.0f); b.torque = 0.0f; }
@@ -2328,6 +2939,42 @@ void orc_shape_aabb(const b2g_shape* s, const b2g_transform* xf, b2g_aabb* out) 
 void orc_set_trig_mode(int mode) { g_trigMode = mode; }
 
 // glibc sinf/cosf, the functions b2Rot::Set calls (REF b2Math.h:302-311).
+// GJK distance without radii from an empty cache; out = pointA(2),
+// pointB(2), distance, (unused), simplex count.
+void orc_distance(const b2g_shape* a, const b2g_transform* xfA, const b2g_shape* b, const b2g_transform* xfB,
+                  float* out) {
+    const Proxy pa = proxyOf(a), pb = proxyOf(b);
+    Xf ta, tb;
+    ta.p = mk(xfA->px, xfA->py); ta.q.s = xfA->s; ta.q.c = xfA->c;
+    tb.p = mk(xfB->px, xfB->py); tb.q.s = xfB->s; tb.q.c = xfB->c;
+    SimplexCache cache;
+    cache.count = 0;
+    cache.metric = 0.0f;
+    V2 wa, wb;
+    float d;
+    gjk(&wa, &wb, &d, &cache, pa, ta, pb, tb);
+    out[0] = wa.x; out[1] = wa.y; out[2] = wb.x; out[3] = wb.y;
+    out[4] = d; out[5] = 0.0f; out[6] = (float)cache.count;
+}
+
+// Time of impact of two swept shapes; returns the state (unknown, failed,
+// overlapped, touching, separated), *t the time. Trig as set by
+// orc_set_trig_mode.
+int orc_time_of_impact(const b2g_shape* a, const b2g_sweep* swA, const b2g_shape* b, const b2g_sweep* swB,
+                       float tMax, float* t) {
+    Sweep s[2];
+    const b2g_sweep* in[2] = {swA, swB};
+    for (int k = 0; k < 2; ++k) {
+        s[k].localCenter = mk(in[k]->localCenter[0], in[k]->localCenter[1]);
+        s[k].c0 = mk(in[k]->c0[0], in[k]->c0[1]);
+        s[k].c = mk(in[k]->c[0], in[k]->c[1]);
+        s[k].a0 = in[k]->a0;
+        s[k].a = in[k]->a;
+        s[k].alpha0 = in[k]->alpha0;
+    }
+    return timeOfImpact(t, proxyOf(a), proxyOf(b), s[0], s[1], tMax);
+}
+
 void orc_sincos(const float* a, int n, float* s, float* c) {
     for (int i = 0; i < n; ++i) { s[i] = sinf(a[i]); c[i] = cosf(a[i]); }
 }

@@ -468,6 +468,62 @@ int ref_collide(const b2g_shape* a, const b2g_transform* xfA, const b2g_shape* b
     return 0;
 }
 
+// Shape record -> the reference's shape object, kept alive for a proxy.
+struct RefShape {
+    b2PolygonShape poly;
+    b2CircleShape circle;
+    b2EdgeShape edge;
+    const b2Shape* set(const b2g_shape* s) {
+        if (s->type == B2G_POLYGON) { recToPolygon(s, &poly); return &poly; }
+        if (s->type == B2G_CIRCLE) { recToCircle(s, &circle); return &circle; }
+        recToEdge(s, &edge);
+        return &edge;
+    }
+};
+
+// b2Distance without radii from an empty cache; out = pointA(2), pointB(2),
+// distance, iterations, simplex count.
+void ref_distance(const b2g_shape* a, const b2g_transform* xfA, const b2g_shape* b, const b2g_transform* xfB,
+                  float* out) {
+    RefShape sa, sb;
+    b2DistanceInput in;
+    in.proxyA.Set(sa.set(a), 0);
+    in.proxyB.Set(sb.set(b), 0);
+    in.transformA = toXf(xfA);
+    in.transformB = toXf(xfB);
+    in.useRadii = false;
+    b2SimplexCache cache;
+    cache.count = 0;
+    b2DistanceOutput o;
+    b2Distance(&o, &cache, &in);
+    out[0] = o.pointA.x; out[1] = o.pointA.y; out[2] = o.pointB.x; out[3] = o.pointB.y;
+    out[4] = o.distance; out[5] = (float)o.iterations; out[6] = (float)cache.count;
+}
+
+// b2TimeOfImpact; returns the output state (b2TOIOutput::State order), *t the time.
+int ref_time_of_impact(const b2g_shape* a, const b2g_sweep* swA, const b2g_shape* b, const b2g_sweep* swB,
+                       float tMax, float* t) {
+    RefShape sa, sb;
+    b2TOIInput in;
+    in.proxyA.Set(sa.set(a), 0);
+    in.proxyB.Set(sb.set(b), 0);
+    const b2g_sweep* sw[2] = {swA, swB};
+    b2Sweep* dst[2] = {&in.sweepA, &in.sweepB};
+    for (int k = 0; k < 2; ++k) {
+        dst[k]->localCenter.Set(sw[k]->localCenter[0], sw[k]->localCenter[1]);
+        dst[k]->c0.Set(sw[k]->c0[0], sw[k]->c0[1]);
+        dst[k]->c.Set(sw[k]->c[0], sw[k]->c[1]);
+        dst[k]->a0 = sw[k]->a0;
+        dst[k]->a = sw[k]->a;
+        dst[k]->alpha0 = sw[k]->alpha0;
+    }
+    in.tMax = tMax;
+    b2TOIOutput o;
+    b2TimeOfImpact(&o, &in);
+    *t = o.t;
+    return (int)o.state;
+}
+
 // b2WorldManifold::Initialize; out = normal(2), points(4), separations(2).
 void ref_world_manifold(const b2g_manifold* mf, const b2g_transform* xfA, float radiusA,
                         const b2g_transform* xfB, float radiusB, float* out) {

@@ -146,3 +146,52 @@ def collide_all(lib, pre: str, A, XA, B, XB) -> np.ndarray:
         f(E.ptr(a), E.ptr(xa), E.ptr(b), E.ptr(xb), E.ptr(o))
         out[i] = o[0]
     return out
+
+
+def sweep_corpus(lib, pre: str, n_per_type: int, seed: int):
+    """Arrays (A, sweepA, B, sweepB) of 5 * n_per_type swept pairs for the
+    time of impact: B (and A unless it is an edge) moves up to several
+    extents per step with rotations up to 6 rad, towards, past or away from
+    the other shape, some sweeps starting at alpha0 > 0."""
+    rng = np.random.default_rng(seed)
+    As, Bs, SA, SB = [], [], [], []
+
+    def sweep(c0, c, a0, a, lc, alpha0):
+        sw = np.zeros((), E.SWEEP)
+        sw["localCenter"] = lc
+        sw["c0"] = c0
+        sw["c"] = c
+        sw["a0"], sw["a"], sw["alpha0"] = a0, a, alpha0
+        return sw
+
+    for t in range(5):
+        for i in range(n_per_type):
+            if t == 0:
+                A, B = _rand_polygon(lib, pre, rng), _rand_polygon(lib, pre, rng)
+            elif t == 1:
+                A, B = E.circle(rng.uniform(0.05, 1.0)), E.circle(rng.uniform(0.05, 1.0))
+            elif t == 2:
+                A, B = _rand_polygon(lib, pre, rng), E.circle(rng.uniform(0.05, 1.0))
+            elif t == 3:
+                A, B = _rand_edge(rng), E.circle(rng.uniform(0.05, 1.0))
+            else:
+                A, B = _rand_edge(rng), _rand_polygon(lib, pre, rng)
+            reach = _extent(A) + _extent(B)
+            pa = rng.uniform(-20, 20, 2)
+            aa = rng.uniform(-4, 4)
+            if t >= 3:
+                sa = sweep(pa, pa, aa, aa, (0.0, 0.0), 0.0)
+            else:
+                lc = rng.uniform(-0.2, 0.2, 2) * (rng.random() < 0.5)
+                sa = sweep(pa, pa + rng.uniform(-1, 1, 2) * reach, aa, aa + rng.uniform(-3, 3), lc, 0.0)
+            th = rng.uniform(0, 2 * np.pi)
+            start = pa + reach * rng.uniform(1.05, 4.0) * np.array([np.cos(th), np.sin(th)])
+            aim = pa + rng.uniform(-0.8, 0.8, 2) * reach
+            end = start + (aim - start) * rng.uniform(0.5, 3.0)
+            if rng.random() < 0.15:
+                end = start + (start - pa) * 0.5  # moving away
+            ab = rng.uniform(-4, 4)
+            alpha0 = float(rng.uniform(0.0, 0.6)) if rng.random() < 0.2 else 0.0
+            sb = sweep(start, end, ab, ab + rng.uniform(-6, 6) * (rng.random() < 0.7), rng.uniform(-0.2, 0.2, 2), alpha0)
+            As.append(A); Bs.append(B); SA.append(sa); SB.append(sb)
+    return (np.array(As, E.SHAPE), np.array(SA, E.SWEEP), np.array(Bs, E.SHAPE), np.array(SB, E.SWEEP))

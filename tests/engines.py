@@ -31,6 +31,8 @@ SHAPE = np.dtype([
 ])
 assert SHAPE.itemsize == 176
 XF = np.dtype([("px", "<f4"), ("py", "<f4"), ("s", "<f4"), ("c", "<f4")])
+SWEEP = np.dtype([("localCenter", "<f4", 2), ("c0", "<f4", 2), ("c", "<f4", 2), ("a0", "<f4"), ("a", "<f4"),
+                  ("alpha0", "<f4")])
 MPOINT = np.dtype([("lp", VEC2), ("ni", "<f4"), ("ti", "<f4"), ("id", "<u4")])
 MANIFOLD = np.dtype([("points", MPOINT, (2,)), ("ln", VEC2), ("lpt", VEC2), ("type", "<i4"), ("count", "<i4")])
 assert MANIFOLD.itemsize == 64
@@ -82,15 +84,16 @@ class Engine:
         return getattr(self.lib, self.pre + n)
 
     def world(self, kind: str, n: int = 0, seed: int = 0, sleep: bool = True, warm: bool = True,
-              gravity=(0.0, -10.0)) -> "World":
-        return World(self, kind, n, seed, sleep, warm, gravity)
+              gravity=(0.0, -10.0), ccd: bool = False, substep: bool = False) -> "World":
+        return World(self, kind, n, seed, sleep, warm, gravity, ccd, substep)
 
 
 class World:
-    def __init__(self, eng: Engine, kind: str, n: int, seed: int, sleep: bool, warm: bool, gravity):
+    def __init__(self, eng: Engine, kind: str, n: int, seed: int, sleep: bool, warm: bool, gravity,
+                 ccd: bool = False, substep: bool = False):
         self.e = eng
         self.h = VP(eng.fn("world_create")(gravity[0], gravity[1]))
-        eng.fn("world_set_flags")(self.h, int(sleep), int(warm), 0, 0)
+        eng.fn("world_set_flags")(self.h, int(sleep), int(warm), int(ccd), int(substep))
         if kind is None:
             return
         rc = eng.fn("world_load_scene")(self.h, kind.encode(), int(n), ctypes.c_uint(seed))

@@ -364,3 +364,150 @@ def test_one_sided_platform_listener(ref):
         assert ((ev[:, 1] == k) & plat).sum() == 8 * per_box, k
     assert ((ev[:, 1] == 3) & (ev[:, 4] == 0)).sum() > 8
     w.close()
+
+
+# ------------------------------------------------------------ continuous collision
+@pytest.mark.parametrize("kind,n,seed,steps,sleep,substep", [
+    ("bullets", 24, 1, 120, True, False),      # bullets and fast bodies into a column and a thin wall
+    ("bullets", 48, 2, 200, False, False),
+    ("bullets", 24, 3, 120, True, True),       # sub-stepping: one TOI event per step
+    ("pyramid", 20, 0, 300, True, False),      # the reference defaults (sleep and CCD on)
+    ("pile", 400, 3, 300, False, False),       # fast bodies against the static container
+])
+def test_ccd_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, substep):
+    a = ref.world(kind, n, seed, sleep=sleep, ccd=True, substep=substep)
+    b = orc.world(kind, n, seed, sleep=sleep, ccd=True, substep=substep)
+    for s in range(steps):
+        a.step(1)
+        b.step(1)
+        ba, bb = a.bodies(), b.bodies()
+        assert np.array_equal(ba.view(np.uint32), bb.view(np.uint32)), f"{kind} bodies differ at step {s + 1}"
+    ma, na = a.contacts()
+    mb, nb = b.contacts()
+    assert np.array_equal(ma, mb), "contact lists differ"
+    assert np.array_equal(E.manifold_bits(na), E.manifold_bits(nb)), "manifolds differ"
+    a.close(); b.close()
+
+
+def test_ccd_stops_tunnelling(orc):
+    """The bullets scene's wall is 0.2 m thick and bodies travel 1.3-3.3 m
+    per step: without TOI many end up behind it, with TOI none does."""
+    past = {}
+    for ccd in (False, True):
+        w = orc.world("bullets", 48, 2, sleep=False, ccd=ccd)
+        worst = 0
+        for _ in range(200):
+            w.step(1)
+            worst = max(worst, int((w.bodies()[7:, E.PX] > 12.0).sum()))
+        past[ccd] = worst
+        w.close()
+    assert past[False] >= 10 and past[True] == 0, past
+
+
+def _seg_dist(p, a, b):
+    ab = b - a
+    t = np.clip(np.dot(p - a, ab) / max(np.dot(ab, ab), 1e-300), 0.0, 1.0)
+    return np.hypot(*(p - (a + t * ab)))
+
+
+def _world_verts(s, xf):
+    n = 1 if s["type"] == 0 else (2 if s["type"] == 1 else int(s["count"]))
+    if s["type"] == 0:
+        v = np.array([[s["centroid"]["x"], s["centroid"]["y"]]], np.float64)
+    else:
+        v = np.array([[s["verts"][i]["x"], s["verts"][i]["y"]] for i in range(n)], np.float64)
+    c, sn = float(xf["c"]), float(xf["s"])
+    return np.stack([c * v[:, 0] - sn * v[:, 1] + float(xf["px"]), sn * v[:, 0] + c * v[:, 1] + float(xf["py"])], 1)
+
+
+def _cross2(u, v):
+    return u[0] * v[1] - u[1] * v[0]
+
+
+def _edges(P):
+    return [(P[i], P[(i + 1) % len(P)]) for i in range(len(P))] if len(P) > 1 else [(P[0], P[0])]
+
+
+def _brute_distance(va, vb):
+    """Distance between two convex point sets' hulls (segments, vertices):
+    0 when a vertex of one lies inside the other or two edges cross, else
+    the smallest vertex-to-edge distance either way."""
+    def inside(p, poly):
+        if len(poly) < 3:
+            return False
+        cr = [_cross2(poly[(i + 1) % len(poly)] - poly[i], p - poly[i]) for i in range(len(poly))]
+        return all(c >= 0 for c in cr) or all(c <= 0 for c in cr)
+
+    def crosses(a, b, c, d):
+        d1, d2 = _cross2(b - a, c - a), _cross2(b - a, d - a)
+        d3, d4 = _cross2(d - c, a - c), _cross2(d - c, b - c)
+        return d1 * d2 < 0 and d3 * d4 < 0
+    if any(inside(p, vb) for p in va) or any(inside(p, va) for p in vb):
+        return 0.0
+    if any(crosses(a, b, c, d) for a, b in _edges(va) for c, d in _edges(vb)):
+        return 0.0
+    best = np.inf
+    for P, Q in ((va, vb), (vb, va)):
+        for p in P:
+            for a, b in _edges(Q):
+                best = min(best, _seg_dist(p, a, b))
+    return best
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_distance_matches_reference(reflib, orclib, seed):
+    """GJK distance (b2Distance, no radii, empty cache) equals the reference
+    build bit for bit, and the brute-force hull distance in FP64."""
+    A, XA, B, XB = corpus.random_corpus(orclib, "orc_", 60, seed)
+    out_o, out_r = np.zeros(7, np.float32), np.zeros(7, np.float32)
+    checked = 0
+    for i in range(len(A)):
+        orclib.orc_distance(E.ptr(A[i:i + 1]), E.ptr(XA[i:i + 1]), E.ptr(B[i:i + 1]), E.ptr(XB[i:i + 1]), E.ptr(out_o))
+        reflib.ref_distance(E.ptr(A[i:i + 1]), E.ptr(XA[i:i + 1]), E.ptr(B[i:i + 1]), E.ptr(XB[i:i + 1]), E.ptr(out_r))
+        keep = [0, 1, 2, 3, 4, 6]
+        assert np.array_equal(out_o[keep].view(np.uint32), out_r[keep].view(np.uint32)), f"case {i}"
+        d = _brute_distance(_world_verts(A[i], XA[i]), _world_verts(B[i], XB[i]))
+        scale = 1.0 + np.abs(_world_verts(A[i], XA[i])).max()
+        if d > 1e-3:
+            assert abs(out_o[4] - d) <= 2e-5 * scale, (i, out_o[4], d)
+            checked += 1
+        else:
+            assert out_o[4] <= 1e-3 * scale, (i, out_o[4], d)
+    assert checked > 100
+
+
+@pytest.mark.parametrize("seed", [7, 8])
+def test_toi_matches_reference(reflib, orclib, seed):
+    """b2TimeOfImpact state and time equal the reference build's bit for
+    bit over swept pairs of every shape combination."""
+    A, SA, B, SB = corpus.sweep_corpus(orclib, "orc_", 60, seed)
+    t_o, t_r = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    states = []
+    for i in range(len(A)):
+        so = orclib.orc_time_of_impact(E.ptr(A[i:i + 1]), E.ptr(SA[i:i + 1]), E.ptr(B[i:i + 1]), E.ptr(SB[i:i + 1]),
+                                       F(1.0), E.ptr(t_o))
+        sr = reflib.ref_time_of_impact(E.ptr(A[i:i + 1]), E.ptr(SA[i:i + 1]), E.ptr(B[i:i + 1]), E.ptr(SB[i:i + 1]),
+                                       F(1.0), E.ptr(t_r))
+        assert so == sr and t_o.view(np.uint32)[0] == t_r.view(np.uint32)[0], (i, so, sr, t_o[0], t_r[0])
+        states.append(so)
+    states = np.array(states)
+    assert (states == 3).sum() > 50 and (states == 4).sum() > 20  # touching and separated both occur
+
+
+def test_toi_closed_form(orclib):
+    """Circle B translating head-on towards circle A: the time returned is
+    where the centre distance reaches max(linearSlop, rA + rB - 3 linearSlop)
+    within the root finder's tolerance of linearSlop / 4."""
+    slop = 0.005
+    t = np.zeros(1, np.float32)
+    for rA, rB, d0, L in [(0.5, 0.5, 5.0, 8.0), (0.1, 0.3, 3.0, 40.0), (1.0, 0.05, 2.5, 2.0), (0.02, 0.02, 1.0, 3.0)]:
+        A, B = np.array([E.circle(rA)], E.SHAPE), np.array([E.circle(rB)], E.SHAPE)
+        sa = np.zeros(1, E.SWEEP)
+        sb = np.zeros(1, E.SWEEP)
+        sb["c0"] = (d0, 0.0)
+        sb["c"] = (d0 - L, 0.0)
+        state = orclib.orc_time_of_impact(E.ptr(A), E.ptr(sa), E.ptr(B), E.ptr(sb), F(1.0), E.ptr(t))
+        target = max(slop, rA + rB - 3 * slop)
+        assert state == 3, state
+        dist = d0 - float(t[0]) * L
+        assert abs(dist - target) < 0.25 * slop + 1e-6, (rA, rB, d0, L, dist, target)
