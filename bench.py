@@ -66,7 +66,7 @@ class StepStats(ctypes.Structure):
         "bodies", "fixtures", "contacts", "constraints", "points", "colours", "colouringRounds", "newPairs",
         "positionIterations", "beginEvents", "endEvents", "overflow")] + [
         ("minSeparation", ctypes.c_float), ("cellSize", ctypes.c_float), ("freshPoints", ctypes.c_int32),
-        ("fallback", ctypes.c_int32), ("residual", ctypes.c_float)]
+        ("fallback", ctypes.c_int32), ("residual", ctypes.c_float), ("toiEvents", ctypes.c_int32)]
 
 
 def sched_dict(sc):

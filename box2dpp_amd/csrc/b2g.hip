@@ -30,6 +30,7 @@
 #include "collide.h"
 #include "prims.h"
 #include "mlp.h"
+#include "toi.h"
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
@@ -460,6 +461,8 @@ struct ContactArrays {
     float2* cmat;
     b2g_manifold* cman;
     int* ccolour;
+    float* ctoi;
+    int* ctoiCount;
 };
 
 __global__ void k_compact(ContactArrays src, ContactArrays dst, const int* keep, const int* idx, int n,
@@ -475,6 +478,8 @@ __global__ void k_compact(ContactArrays src, ContactArrays dst, const int* keep,
     dst.cmat[d] = src.cmat[c];
     dst.cman[d] = src.cman[c];
     dst.ccolour[d] = src.ccolour[c];
+    dst.ctoi[d] = src.ctoi[c];
+    dst.ctoiCount[d] = src.ctoiCount[c];
 }
 
 template <typename T>
@@ -1749,12 +1754,12 @@ __device__ inline bool fatRule(float4& fat, float4 tight, V2 disp) {
 }
 
 // b2Body::SynchronizeFixtures + b2Fixture::Synchronize,
-// REF/Dynamics/b2Body.cpp:443-454, REF/Dynamics/b2Fixture.cpp:154-176.
-__global__ void k_sync_fixtures(WorldView w) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= w.nFixtures) return;
-    const int b = w.fbody[f];
-    if (!islandBody(w, b)) return;
+// REF/Dynamics/b2Body.cpp:443-454, REF/Dynamics/b2Fixture.cpp:154-176: the
+// swept box from the sweep start to the current transform, run through the
+// fat-box rule. Removed proxies (inverted fat boxes) stay removed.
+__device__ inline bool syncFixtureOne(const WorldView& w, int f, int b) {
+    float4 fat = w.fat[f];
+    if (!(fat.x <= fat.z && fat.y <= fat.w)) return false;
     const float4 p0 = w.pos0[b];
     const float2 lc = w.lcenter[b];
     Xf xf1;
@@ -1765,11 +1770,18 @@ __global__ void k_sync_fixtures(WorldView w) {
     const float4 a1 = shapeAABB(s, xf1);
     const float4 a2 = shapeAABB(s, xf2);
     const float4 tight = make_float4(fminB(a1.x, a2.x), fminB(a1.y, a2.y), fmaxB(a1.z, a2.z), fmaxB(a1.w, a2.w));
-    float4 fat = w.fat[f];
-    if (fatRule(fat, tight, xf2.p - xf1.p)) {
-        w.fat[f] = fat;
-        w.moved[f] = 1;
-    }
+    if (!fatRule(fat, tight, xf2.p - xf1.p)) return false;
+    w.fat[f] = fat;
+    w.moved[f] = 1;
+    return true;
+}
+
+__global__ void k_sync_fixtures(WorldView w) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= w.nFixtures) return;
+    const int b = w.fbody[f];
+    if (!islandBody(w, b)) return;
+    syncFixtureOne(w, f, b);
 }
 
 // ====================================================================
@@ -1996,6 +2008,8 @@ __global__ void k_append_contacts(WorldView w, const unsigned long long* keys, i
     zeroManifold(m);
     w.cman[c] = m;
     w.ccolour[c] = -1;
+    w.ctoi[c] = 1.0f;
+    w.ctoiCount[c] = 0;
     if (ma.w == 0.0f && mb.w == 0.0f) {
         wakeBody(w, w.fbody[a]);
         wakeBody(w, w.fbody[b]);
@@ -2316,6 +2330,41 @@ __global__ void k_static_awake(WorldView w, RefView r) {
 // ====================================================================
 // Batch kernels for per-function parity
 // ====================================================================
+// Per-pair distance and time of impact for the C ABI's batch entry points.
+__global__ void k_distance_batch(const b2g_shape* a, const b2g_transform* xfA, const b2g_shape* b,
+                                 const b2g_transform* xfB, int n, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Proxy pA = proxyOf(a[i]), pB = proxyOf(b[i]);
+    Xf ta, tb;
+    ta.p = mk(xfA[i].px, xfA[i].py); ta.q.s = xfA[i].s; ta.q.c = xfA[i].c;
+    tb.p = mk(xfB[i].px, xfB[i].py); tb.q.s = xfB[i].s; tb.q.c = xfB[i].c;
+    SimplexCache cache;
+    cache.count = 0;
+    cache.metric = 0.0f;
+    out[2 * i] = gjkDistance(cache, pA, ta, pB, tb);
+    out[2 * i + 1] = (float)cache.count;
+}
+
+__global__ void k_toi_batch(const b2g_shape* a, const b2g_sweep* sa, const b2g_shape* b, const b2g_sweep* sb, int n,
+                            float tMax, int32_t* state, float* t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Sweep s[2];
+    const b2g_sweep* in[2] = {sa + i, sb + i};
+    for (int k = 0; k < 2; ++k) {
+        s[k].localCenter = mk(in[k]->localCenter[0], in[k]->localCenter[1]);
+        s[k].c0 = mk(in[k]->c0[0], in[k]->c0[1]);
+        s[k].c = mk(in[k]->c[0], in[k]->c[1]);
+        s[k].a0 = in[k]->a0;
+        s[k].a = in[k]->a;
+        s[k].alpha0 = in[k]->alpha0;
+    }
+    float ti;
+    state[i] = timeOfImpact(&ti, proxyOf(a[i]), proxyOf(b[i]), s[0], s[1], tMax);
+    t[i] = ti;
+}
+
 __global__ void k_collide_batch(const b2g_shape* A, const b2g_transform* xa, const b2g_shape* B,
                                 const b2g_transform* xb, int n, b2g_manifold* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2379,6 +2428,459 @@ __global__ void k_penetration(WorldView w, double* acc, int* npts, unsigned long
         atomicMax(maxBits, (unsigned long long)__double_as_longlong(p));
         atomicAdd(npts, 1);
     }
+}
+
+// ====================================================================
+// Continuous collision: b2World::SolveTOI, REF/Dynamics/b2World.cpp:579-897.
+// One event per pass of the host loop: the earliest time of impact over the
+// contact list (list order = reverse table order breaks ties), the two
+// bodies moved to it, a mini-island of them and the static, kinematic and
+// bullet bodies they touch (b2Island::SolveTOI, REF/Dynamics/b2Island.cpp:
+// 385-515) solved for the rest of the step, fixtures synchronised and new
+// pairs found. Times of impact are computed in parallel when every pending
+// contact's bodies share a sweep start (always on the first scan of a
+// step); otherwise in list order by one lane, since aligning two sweeps
+// moves the later-starting one for the contacts that follow.
+// ====================================================================
+constexpr int kToiGatherCap = 2048;  // edges of the TOI pair sorted in LDS
+
+struct ToiState {
+    unsigned long long minKey;  // (time bits << 32) | list position of the earliest TOI
+    int mismatch;               // a pending contact's sweeps start at different times
+    int active;                 // the event built an island
+    int done;                   // no event left below 1 - 10 eps
+    int moved;                  // proxies the event moved
+    int bodyA, bodyB, contact;
+    float alpha;
+    int nGather;
+    int nBodies, nContacts;     // island sizes
+    int bodies[2 * kMaxTOIContacts];
+    int contacts[kMaxTOIContacts];
+    float4 backup[4];           // pos0 / pos of body A and B before the advance
+    int events;                 // events solved in the pass
+    Counters scratch;           // counters the constraint set-up writes
+};
+
+__device__ inline bool toiPending(const WorldView& w, int c, int flags) {
+    if (!(flags & CF_ENABLED)) return false;
+    if (w.ctoiCount[c] > kMaxSubSteps) return false;
+    if (flags & CF_TOI) return false;
+    const int fA = w.cfA[c], fB = w.cfB[c];
+    if (sensorOf(w, fA) || sensorOf(w, fB)) return false;
+    const int bA = w.fbody[fA], bB = w.fbody[fB];
+    const int tA = w.btype[bA], tB = w.btype[bB];
+    const bool activeA = (w.bflags[bA] & BF_AWAKE) && tA != BT_STATIC;
+    const bool activeB = (w.bflags[bB] & BF_AWAKE) && tB != BT_STATIC;
+    if (!activeA && !activeB) return false;
+    const bool collideA = (w.bflags[bA] & BF_BULLET) || tA != BT_DYNAMIC;
+    const bool collideB = (w.bflags[bB] & BF_BULLET) || tB != BT_DYNAMIC;
+    return collideA || collideB;
+}
+
+__device__ inline Sweep sweepOf(const WorldView& w, int b) {
+    const float4 p0 = w.pos0[b], p = w.pos[b];
+    const float2 lc = w.lcenter[b];
+    Sweep sw;
+    sw.localCenter = mk(lc.x, lc.y);
+    sw.c0 = mk(p0.x, p0.y);
+    sw.a0 = p0.z;
+    sw.alpha0 = p0.w;
+    sw.c = mk(p.x, p.y);
+    sw.a = p.z;
+    return sw;
+}
+
+// b2Sweep::Advance on a body's sweep start.
+__device__ inline void advanceSweepStart(const WorldView& w, int b, float alpha) {
+    Sweep sw = sweepOf(w, b);
+    sweepAdvance(sw, alpha);
+    w.pos0[b] = make_float4(sw.c0.x, sw.c0.y, sw.a0, sw.alpha0);
+}
+
+// b2Body::Advance, REF/Dynamics/b2Body.h:842-850: the body to its sweep at alpha.
+__device__ inline void advanceBodyTo(const WorldView& w, int b, float alpha) {
+    advanceSweepStart(w, b, alpha);
+    const float4 p0 = w.pos0[b];
+    w.pos[b] = make_float4(p0.x, p0.y, p0.z, 0.0f);
+    syncTransformOne(w, b);
+}
+
+// TOI of a contact over the rest of the step, REF/Dynamics/b2World.cpp:645-689.
+__device__ inline float contactToi(const WorldView& w, int c) {
+    const int fA = w.cfA[c], fB = w.cfB[c];
+    const int bA = w.fbody[fA], bB = w.fbody[fB];
+    const float alphaA = w.pos0[bA].w, alphaB = w.pos0[bB].w;
+    float alpha0 = alphaA;
+    if (alphaA < alphaB) {
+        alpha0 = alphaB;
+        advanceSweepStart(w, bA, alpha0);
+    } else if (alphaB < alphaA) {
+        alpha0 = alphaA;
+        advanceSweepStart(w, bB, alpha0);
+    }
+    const Proxy pA = proxyOf(w.shape[fA]), pB = proxyOf(w.shape[fB]);
+    float beta;
+    const int state = timeOfImpact(&beta, pA, pB, sweepOf(w, bA), sweepOf(w, bB), 1.0f);
+    return state == TOI_TOUCHING ? fminB(alpha0 + (1.0f - alpha0) * beta, 1.0f) : 1.0f;
+}
+
+// b2Contact::Update without a listener (REF/Dynamics/Contacts/b2Contact.cpp:
+// 171-251) for one contact on one lane.
+__device__ inline void toiUpdateContact(const WorldView& w, int c) {
+    const int fA = w.cfA[c], fB = w.cfB[c];
+    const int bA = w.fbody[fA], bB = w.fbody[fB];
+    int flags = w.cflags[c] | CF_ENABLED;
+    const b2g_manifold old = w.cman[c];
+    b2g_manifold m;
+    zeroManifold(m);
+    collideShapes(m, w.shape[fA], xfOf(w.xf[bA]), w.shape[fB], xfOf(w.xf[bB]));
+    for (int i = 0; i < m.pointCount; ++i) {
+        m.points[i].normalImpulse = 0.0f;
+        m.points[i].tangentImpulse = 0.0f;
+        for (int j = 0; j < old.pointCount; ++j) {
+            if (old.points[j].id == m.points[i].id) {
+                m.points[i].normalImpulse = old.points[j].normalImpulse;
+                m.points[i].tangentImpulse = old.points[j].tangentImpulse;
+                break;
+            }
+        }
+    }
+    const bool wasTouching = (flags & CF_TOUCHING) != 0;
+    const bool touching = m.pointCount > 0;
+    if (touching != wasTouching) {
+        wakeBody(w, bA);
+        wakeBody(w, bB);
+    }
+    if (touching) flags |= CF_TOUCHING;
+    else flags &= ~CF_TOUCHING;
+    w.cman[c] = m;
+    w.cflags[c] = flags;
+}
+
+// b2ContactSolver::SolveTOIPositionConstraints, REF/Dynamics/Contacts/
+// b2ContactSolver.cpp:753-841, one constraint: only the TOI pair moves.
+__device__ inline float solveTOIPositionOne(const WorldView& w, const ConView& cv, int k, int toiA, int toiB) {
+    const int2 bi = cv.bodies[k];
+    const float4 ms = cv.mass[k];
+    const bool movesA = bi.x == toiA || bi.x == toiB, movesB = bi.y == toiA || bi.y == toiB;
+    const float mA = movesA ? ms.x : 0.0f, iA = movesA ? ms.y : 0.0f;
+    const float mB = movesB ? ms.z : 0.0f, iB = movesB ? ms.w : 0.0f;
+    const int info = cv.info[k];
+    const int pc = (info >> 2) & 3;
+    const int type = (info >> 8) & 3;
+    const float4 lc = cv.lc[k];
+    const V2 lcA = mk(lc.x, lc.y), lcB = mk(lc.z, lc.w);
+    const float4 lp = cv.lp[k];
+    const float4 lnp = cv.lnp[k];
+    const float2 rad = cv.rad[k];
+    const float4 pA = w.pos[bi.x], pB = w.pos[bi.y];
+    V2 cA = mk(pA.x, pA.y), cB = mk(pB.x, pB.y);
+    float aA = pA.z, aB = pB.z;
+    float minSep = 0.0f;
+    for (int j = 0; j < pc; ++j) {
+        Xf xfA, xfB;
+        xfA.q = rotOf(aA);
+        xfB.q = rotOf(aB);
+        xfA.p = cA - mul(xfA.q, lcA);
+        xfB.p = cB - mul(xfB.q, lcB);
+        V2 normal, point;
+        float separation;
+        if (type == B2G_MANIFOLD_CIRCLES) {
+            const V2 pointA = mul(xfA, mk(lnp.z, lnp.w));
+            const V2 pointB = mul(xfB, mk(lp.x, lp.y));
+            normal = pointB - pointA;
+            normalizeV(normal);
+            point = 0.5f * (pointA + pointB);
+            separation = dot(pointB - pointA, normal) - rad.x - rad.y;
+        } else if (type == B2G_MANIFOLD_FACE_A) {
+            normal = mul(xfA.q, mk(lnp.x, lnp.y));
+            const V2 planePoint = mul(xfA, mk(lnp.z, lnp.w));
+            const V2 clipPoint = mul(xfB, j == 0 ? mk(lp.x, lp.y) : mk(lp.z, lp.w));
+            separation = dot(clipPoint - planePoint, normal) - rad.x - rad.y;
+            point = clipPoint;
+        } else {
+            normal = mul(xfB.q, mk(lnp.x, lnp.y));
+            const V2 planePoint = mul(xfB, mk(lnp.z, lnp.w));
+            const V2 clipPoint = mul(xfA, j == 0 ? mk(lp.x, lp.y) : mk(lp.z, lp.w));
+            separation = dot(clipPoint - planePoint, normal) - rad.x - rad.y;
+            point = clipPoint;
+            normal = -normal;
+        }
+        const V2 rA = point - cA;
+        const V2 rB = point - cB;
+        minSep = fminB(minSep, separation);
+        const float C = clampB(kToiBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, 0.0f);
+        const float rnA = cross(rA, normal);
+        const float rnB = cross(rB, normal);
+        const float Kn = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+        const float impulse = Kn > 0.0f ? -C / Kn : 0.0f;
+        const V2 P = impulse * normal;
+        cA -= mA * P;
+        aA -= iA * cross(rA, P);
+        cB += mB * P;
+        aB += iB * cross(rB, P);
+    }
+    if (movesA) w.pos[bi.x] = make_float4(cA.x, cA.y, aA, 0.0f);
+    if (movesB) w.pos[bi.y] = make_float4(cB.x, cB.y, aB, 0.0f);
+    return minSep;
+}
+
+// Start of a TOI pass after a completed step: island flags off, sweeps
+// start at 0, cached TOIs dropped.
+__global__ void k_toi_reset_bodies(WorldView w) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies) return;
+    w.bflags[b] &= ~BF_ISLAND;
+    w.pos0[b].w = 0.0f;
+}
+__global__ void k_toi_reset_contacts(WorldView w) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    w.cflags[c] &= ~(CF_TOI | CF_ISLAND);
+    w.ctoiCount[c] = 0;
+    w.ctoi[c] = 1.0f;
+}
+
+__global__ void k_toi_begin_event(ToiState* st) {
+    if (threadIdx.x != 0) return;
+    st->minKey = ~0ull;
+    st->mismatch = 0;
+    st->active = 0;
+    st->moved = 0;
+    st->nGather = 0;
+    st->nBodies = 0;
+    st->nContacts = 0;
+}
+
+// Contacts whose TOI the scan computes, and whether their sweeps need aligning.
+__global__ void k_toi_classify(WorldView w, int* pending, ToiState* st) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    const bool p = toiPending(w, c, w.cflags[c]);
+    pending[c] = p ? 1 : 0;
+    if (p && w.pos0[w.fbody[w.cfA[c]]].w != w.pos0[w.fbody[w.cfB[c]]].w) atomicOr(&st->mismatch, 1);
+}
+
+__global__ void k_toi_compute(WorldView w, const int* pending, const ToiState* st) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts || st->mismatch || !pending[c]) return;
+    w.ctoi[c] = contactToi(w, c);
+    w.cflags[c] |= CF_TOI;
+}
+
+__global__ void k_toi_compute_serial(WorldView w, const int* pending, const ToiState* st) {
+    if (blockIdx.x != 0 || threadIdx.x != 0 || !st->mismatch) return;
+    for (int c = w.nContacts - 1; c >= 0; --c) {
+        if (!pending[c]) continue;
+        w.ctoi[c] = contactToi(w, c);
+        w.cflags[c] |= CF_TOI;
+    }
+}
+
+// Earliest cached TOI below 1, first in list order among equal times.
+__global__ void k_toi_min(WorldView w, ToiState* st) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long key = ~0ull;
+    if (c < w.nContacts) {
+        const int flags = w.cflags[c];
+        if ((flags & CF_ENABLED) && (flags & CF_TOI) && w.ctoiCount[c] <= kMaxSubSteps) {
+            const float alpha = w.ctoi[c];
+            if (alpha < 1.0f)
+                key = ((unsigned long long)__float_as_uint(alpha) << 32) | (unsigned)(w.nContacts - 1 - c);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_down(key, off);
+        key = o < key ? o : key;
+    }
+    if ((threadIdx.x & 63) == 0 && key != ~0ull) atomicMin(&st->minKey, key);
+}
+
+// The event's TOI pair: advance both bodies, update the contact; an island
+// starts when it touches.
+__global__ void k_toi_event(WorldView w, ToiState* st) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const unsigned long long key = st->minKey;
+    const float alpha = key == ~0ull ? 1.0f : __uint_as_float((unsigned)(key >> 32));
+    if (key == ~0ull || 1.0f - 10.0f * FLT_EPSILON < alpha) {
+        st->done = 1;
+        return;
+    }
+    const int c = w.nContacts - 1 - (int)(key & 0xFFFFFFFFu);
+    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
+    st->backup[0] = w.pos0[bA];
+    st->backup[1] = w.pos[bA];
+    st->backup[2] = w.pos0[bB];
+    st->backup[3] = w.pos[bB];
+    advanceBodyTo(w, bA, alpha);
+    advanceBodyTo(w, bB, alpha);
+    toiUpdateContact(w, c);
+    int flags = w.cflags[c] & ~CF_TOI;
+    w.ctoiCount[c] += 1;
+    if (!(flags & CF_ENABLED) || !(flags & CF_TOUCHING)) {
+        w.cflags[c] = flags & ~CF_ENABLED;
+        w.pos0[bA] = st->backup[0];
+        w.pos[bA] = st->backup[1];
+        w.pos0[bB] = st->backup[2];
+        w.pos[bB] = st->backup[3];
+        syncTransformOne(w, bA);
+        syncTransformOne(w, bB);
+        return;
+    }
+    w.cflags[c] = flags | CF_ISLAND;
+    wakeBody(w, bA);
+    wakeBody(w, bB);
+    w.bflags[bA] |= BF_ISLAND;
+    w.bflags[bB] |= BF_ISLAND;
+    st->bodyA = bA;
+    st->bodyB = bB;
+    st->contact = c;
+    st->alpha = alpha;
+    st->bodies[0] = bA;
+    st->bodies[1] = bB;
+    st->nBodies = 2;
+    st->contacts[0] = c;
+    st->nContacts = 1;
+    st->active = 1;
+    st->events += 1;
+}
+
+// Contact edges of the TOI pair's dynamic bodies: (0 for A / 1 for B) << 32 | list position.
+__global__ void k_toi_gather(WorldView w, ToiState* st, unsigned long long* gather) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts || !st->active) return;
+    const int x = w.fbody[w.cfA[c]], y = w.fbody[w.cfB[c]];
+    const unsigned pos = (unsigned)(w.nContacts - 1 - c);
+    for (int k = 0; k < 2; ++k) {
+        const int body = k == 0 ? st->bodyA : st->bodyB;
+        if (w.btype[body] != BT_DYNAMIC || (x != body && y != body)) continue;
+        const int slot = atomicAdd(&st->nGather, 1);
+        if (slot < kToiGatherCap) gather[slot] = ((unsigned long long)k << 32) | pos;
+    }
+}
+
+// One contact edge of the island search (REF/Dynamics/b2World.cpp:760-843).
+__device__ inline void toiVisitEdge(const WorldView& w, ToiState* st, int body, int c) {
+    if (w.cflags[c] & CF_ISLAND) return;
+    const int x = w.fbody[w.cfA[c]], y = w.fbody[w.cfB[c]];
+    const int other = x == body ? y : x;
+    if (w.btype[other] == BT_DYNAMIC && !(w.bflags[body] & BF_BULLET) && !(w.bflags[other] & BF_BULLET)) return;
+    if (sensorOf(w, w.cfA[c]) || sensorOf(w, w.cfB[c])) return;
+    const float4 save0 = w.pos0[other], save = w.pos[other];
+    if (!(w.bflags[other] & BF_ISLAND)) advanceBodyTo(w, other, st->alpha);
+    toiUpdateContact(w, c);
+    const int flags = w.cflags[c];
+    if (!(flags & CF_ENABLED) || !(flags & CF_TOUCHING)) {
+        w.pos0[other] = save0;
+        w.pos[other] = save;
+        syncTransformOne(w, other);
+        return;
+    }
+    w.cflags[c] = flags | CF_ISLAND;
+    st->contacts[st->nContacts++] = c;
+    if (w.bflags[other] & BF_ISLAND) return;
+    w.bflags[other] |= BF_ISLAND;
+    if (w.btype[other] != BT_STATIC) wakeBody(w, other);
+    st->bodies[st->nBodies++] = other;
+}
+
+// Island search over the TOI pair's edges in list order, then
+// b2Island::SolveTOI on lane 0 (Gauss-Seidel order is the island's).
+__global__ __launch_bounds__(256) void k_toi_island(WorldView w, ConView cv, ToiState* st,
+                                                    const unsigned long long* gather, float dt, int velIters) {
+    __shared__ unsigned long long keys[kToiGatherCap];
+    if (!st->active) return;
+    const int n = st->nGather;
+    const bool sorted = n <= kToiGatherCap;
+    if (sorted) {
+        int size = 1;
+        while (size < n) size <<= 1;
+        for (int i = threadIdx.x; i < size; i += blockDim.x) keys[i] = i < n ? gather[i] : ~0ull;
+        __syncthreads();
+        for (int k = 2; k <= size; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = threadIdx.x; i < size; i += blockDim.x) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const unsigned long long a = keys[i], b = keys[l];
+                        const bool up = (i & k) == 0;
+                        if ((a > b) == up) {
+                            keys[i] = b;
+                            keys[l] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    if (threadIdx.x != 0) return;
+    const int bA = st->bodyA, bB = st->bodyB;
+    const int nc = w.nContacts;
+    if (sorted) {
+        for (int i = 0; i < n; ++i) {
+            if (st->nBodies == 2 * kMaxTOIContacts || st->nContacts == kMaxTOIContacts) break;
+            const int k = (int)(keys[i] >> 32);
+            const int c = nc - 1 - (int)(keys[i] & 0xFFFFFFFFu);
+            toiVisitEdge(w, st, k == 0 ? bA : bB, c);
+        }
+    } else {
+        // more edges than the sort holds: walk the list once per body
+        for (int k = 0; k < 2; ++k) {
+            const int body = k == 0 ? bA : bB;
+            if (w.btype[body] != BT_DYNAMIC) continue;
+            for (int c = nc - 1; c >= 0; --c) {
+                if (st->nBodies == 2 * kMaxTOIContacts || st->nContacts == kMaxTOIContacts) break;
+                if (w.fbody[w.cfA[c]] != body && w.fbody[w.cfB[c]] != body) continue;
+                toiVisitEdge(w, st, body, c);
+            }
+        }
+    }
+    // b2Island::SolveTOI
+    const FreshView noFresh{nullptr, nullptr, nullptr, 0};
+    const int ncons = st->nContacts;
+    for (int k = 0; k < ncons; ++k) initConstraintOne(w, cv, k, st->contacts[k], 1.0f, 0, &st->scratch, noFresh);
+    for (int it = 0; it < 20; ++it) {
+        float minSep = 0.0f;
+        for (int k = 0; k < ncons; ++k) minSep = fminB(minSep, solveTOIPositionOne(w, cv, k, bA, bB));
+        if (minSep >= -1.5f * kLinearSlop) break;
+    }
+    const float4 pa = w.pos[bA], pb = w.pos[bB];
+    w.pos0[bA] = make_float4(pa.x, pa.y, pa.z, w.pos0[bA].w);
+    w.pos0[bB] = make_float4(pb.x, pb.y, pb.z, w.pos0[bB].w);
+    for (int k = 0; k < ncons; ++k) initConstraintOne(w, cv, k, st->contacts[k], 1.0f, 0, &st->scratch, noFresh);
+    for (int it = 0; it < velIters; ++it)
+        for (int k = 0; k < ncons; ++k) solveVelocityOne(w, cv, k);
+    const float h = (1.0f - st->alpha) * dt;
+    for (int i = 0; i < st->nBodies; ++i) {
+        const int b = st->bodies[i];
+        integratePositionOne(w, b, h);
+        syncTransformOne(w, b);
+    }
+}
+
+// Fixtures of the island's dynamic bodies to their swept boxes.
+__global__ void k_toi_sync_fixtures(WorldView w, ToiState* st) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= w.nFixtures || !st->active) return;
+    const int b = w.fbody[f];
+    if (!(w.bflags[b] & BF_ISLAND) || w.btype[b] != BT_DYNAMIC) return;
+    if (syncFixtureOne(w, f, b)) atomicAdd(&st->moved, 1);
+}
+
+// Cached TOIs and island marks of every contact on a displaced body dropped.
+__global__ void k_toi_invalidate(WorldView w, const ToiState* st) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts || !st->active) return;
+    const int x = w.fbody[w.cfA[c]], y = w.fbody[w.cfB[c]];
+    const bool hit = ((w.bflags[x] & BF_ISLAND) && w.btype[x] == BT_DYNAMIC) ||
+                     ((w.bflags[y] & BF_ISLAND) && w.btype[y] == BT_DYNAMIC);
+    if (hit) w.cflags[c] &= ~(CF_TOI | CF_ISLAND);
+}
+
+__global__ void k_toi_end_event(WorldView w, ToiState* st) {
+    if (!st->active) return;
+    for (int i = threadIdx.x; i < st->nBodies; i += blockDim.x) w.bflags[st->bodies[i]] &= ~BF_ISLAND;
 }
 
 inline int blocks(int n, int bs = 256) { return (n + bs - 1) / bs; }
@@ -2521,6 +3023,12 @@ struct b2g_context {
     long long graphClock = 0;
     int* dRanges = nullptr;  // colour ranges of the step (kColourSlots + 1 ints)
     int* hRanges = nullptr;  // pinned host staging of dRanges
+    // continuous collision (b2World::SolveTOI)
+    bool stepComplete = true;       // the last TOI pass ran to the end (sub-stepping may stop it early)
+    ToiState* toiState = nullptr;   // device
+    ToiState* hToi = nullptr;       // pinned host mirror
+    unsigned long long* toiGather = nullptr;
+    ConView toiCv{};                // constraints of one TOI island
     // split step (b2g_step_collide / b2g_step_solve) for the host's PreSolve
     bool keepOldManifolds = false;
     b2g_manifold* oldMan = nullptr;
@@ -2574,6 +3082,8 @@ struct b2g_context {
         w.nContacts = nContacts;
         w.cfA = cur.cfA; w.cfB = cur.cfB; w.cflags = cur.cflags; w.cmat = cur.cmat; w.cman = cur.cman;
         w.ccolour = cur.ccolour;
+        w.ctoi = cur.ctoi;
+        w.ctoiCount = cur.ctoiCount;
         return w;
     }
 };
@@ -2723,12 +3233,16 @@ int growContacts(b2g_context* ctx, int need) {
     HIPCHK(ctx, growArray(ctx->cur.cmat, n, cap, s));
     HIPCHK(ctx, growArray(ctx->cur.cman, n, cap, s));
     HIPCHK(ctx, growArray(ctx->cur.ccolour, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->cur.ctoi, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->cur.ctoiCount, n, cap, s));
     HIPCHK(ctx, growArray(ctx->alt.cfA, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->alt.cfB, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->alt.cflags, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->alt.cmat, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->alt.cman, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->alt.ccolour, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->alt.ctoi, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->alt.ctoiCount, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->keep, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->idx, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->newKeys, 0, cap, s));
@@ -2898,13 +3412,13 @@ int readCounters(b2g_context* ctx) {
 enum Phase {
     PH_COLLIDE, PH_COMPACT, PH_COLOUR, PH_ORDER, PH_INTEGRATE_V, PH_INIT, PH_WARM, PH_SOLVE_V, PH_STORE,
     PH_INTEGRATE_X, PH_SOLVE_X, PH_SYNC, PH_BROADPHASE, PH_ISLANDS, PH_LEARNED, PH_K_SOLVE_V, PH_K_SOLVE_X,
-    PH_K_MLP_INFER, PH_K_MLP_GRAD, PH_COUNT
+    PH_K_MLP_INFER, PH_K_MLP_GRAD, PH_TOI, PH_COUNT
 };
 const char* kPhaseNames[PH_COUNT] = {"collide", "compact", "colour", "order", "integrate_velocities",
                                      "init_constraints", "warm_start", "solve_velocity", "store_impulses",
                                      "integrate_positions", "solve_position", "sync", "broadphase", "islands",
                                      "fresh_points", "k_solve_velocity", "k_solve_position", "k_mlp_infer",
-                                     "k_mlp_grad"};
+                                     "k_mlp_grad", "toi"};
 
 int profStart(b2g_context* ctx) {
     if (!ctx->profiling) return -1;
@@ -3430,6 +3944,8 @@ int labelIslands(b2g_context* ctx, const WorldView& w) {
 // holds the contacts that Collide destroys (flagged dead) until the second
 // half compacts it. With ctx->keepOldManifolds the manifolds from before the
 // update are kept for PreSolve.
+int solveTOI(b2g_context* ctx, const b2g_step_params* p);
+
 int stepColouredCollide(b2g_context* ctx, const b2g_step_params* p) {
     (void)p;
     hipStream_t s = ctx->stream;
@@ -3503,7 +4019,7 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
     }
     ctx->stats.beginEvents = ctx->hcnt->beginEvents;
     ctx->stats.endEvents = ctx->hcnt->endEvents;
-    if (dt > 0.0f) {
+    if (dt > 0.0f && ctx->stepComplete) {
         const int posIters = std::min(p->positionIterations, kMaxPosIters);
         if (posIters > ctx->islandIterCap) {
             freeArray(ctx->islandMinSep);
@@ -3860,8 +4376,9 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         ctx->stats.positionIterations = ran;
         ctx->stats.points = ctx->hcnt->points;
         ctx->stats.minSeparation = lastMin;
-        ctx->inv_dt0 = inv_dt;
     }
+    if (ctx->continuous && dt > 0.0f && (rc = solveTOI(ctx, p)) != B2G_OK) return rc;
+    if (dt > 0.0f) ctx->inv_dt0 = inv_dt;
     if (ctx->autoClearForces && ctx->nBodies > 0)
         hipLaunchKernelGGL(k_fill<float4>, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, ctx->force, ctx->nBodies,
                            make_float4(0.0f, 0.0f, 0.0f, 0.0f));
@@ -3918,14 +4435,131 @@ int ensureRefBuffers(b2g_context* ctx) {
     return B2G_OK;
 }
 
-// b2World::Step in the reference's visiting order (see the reference-order
-// kernels above). Same collide/broad-phase as the coloured path; islands
-// and constraint order follow Box2D exactly.
 int stepColoured(b2g_context* ctx, const b2g_step_params* p) {
     const int rc = stepColouredCollide(ctx, p);
     return rc != B2G_OK ? rc : stepColouredSolve(ctx, p);
 }
 
+int ensureToi(b2g_context* ctx) {
+    if (ctx->toiState) return B2G_OK;
+    HIPCHK(ctx, hipMalloc((void**)&ctx->toiState, sizeof(ToiState)));
+    HIPCHK(ctx, hipHostMalloc((void**)&ctx->hToi, sizeof(ToiState), hipHostMallocDefault));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->toiGather, sizeof(unsigned long long) * kToiGatherCap));
+    ConView& c = ctx->toiCv;
+    const size_t n = kMaxTOIContacts;
+    HIPCHK(ctx, hipMalloc((void**)&c.contact, sizeof(int) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.bodies, sizeof(int2) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.mass, sizeof(float4) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.normal, sizeof(float4) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.r0, sizeof(float4) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.r1, sizeof(float4) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.pmass, sizeof(float4) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.bias, sizeof(float2) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.imp, sizeof(float4) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.K, sizeof(float4) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.NM, sizeof(float4) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.info, sizeof(int) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.lp, sizeof(float4) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.lnp, sizeof(float4) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.lc, sizeof(float4) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.rad, sizeof(float2) * n));
+    HIPCHK(ctx, hipMalloc((void**)&c.island, sizeof(int) * n));
+    return B2G_OK;
+}
+
+void freeToi(b2g_context* ctx) {
+    freeArray(ctx->toiState);
+    if (ctx->hToi) (void)hipHostFree(ctx->hToi);
+    ctx->hToi = nullptr;
+    freeArray(ctx->toiGather);
+    ConView& c = ctx->toiCv;
+    freeArray(c.contact); freeArray(c.bodies); freeArray(c.mass); freeArray(c.normal); freeArray(c.r0);
+    freeArray(c.r1); freeArray(c.pmass); freeArray(c.bias); freeArray(c.imp); freeArray(c.K); freeArray(c.NM);
+    freeArray(c.info); freeArray(c.lp); freeArray(c.lnp); freeArray(c.lc); freeArray(c.rad); freeArray(c.island);
+}
+
+__global__ void k_toi_start(ToiState* st) {
+    if (threadIdx.x != 0) return;
+    st->done = 0;
+    st->events = 0;
+}
+
+// b2World::SolveTOI, REF/Dynamics/b2World.cpp:579-897. One pass of this loop
+// per TOI event; the events themselves run on the device (kernels above),
+// the host reads back one small record per event to decide whether to find
+// new pairs and whether the pass is over.
+int solveTOI(b2g_context* ctx, const b2g_step_params* p) {
+    hipStream_t s = ctx->stream;
+    int rc;
+    if ((rc = ensureToi(ctx)) != B2G_OK) return rc;
+    // the step's broad-phase appended contacts the pair hash does not hold yet
+    if ((rc = rebuildHash(ctx)) != B2G_OK) return rc;
+    ToiState* st = ctx->toiState;
+    if (ctx->stepComplete) {
+        WorldView w = ctx->view();
+        if (ctx->nBodies > 0)
+            hipLaunchKernelGGL(k_toi_reset_bodies, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w);
+        if (ctx->nContacts > 0)
+            hipLaunchKernelGGL(k_toi_reset_contacts, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w);
+    }
+    hipLaunchKernelGGL(k_toi_start, dim3(1), dim3(64), 0, s, st);
+    LAUNCHCHK(ctx);
+    const int pe = profStart(ctx);
+    long long passes = 0;
+    for (;;) {
+        // every event raises some contact's TOI count or disables it, so a
+        // pass ends within kMaxSubSteps + 1 events per contact
+        if (++passes > (long long)(kMaxSubSteps + 2) * ((long long)ctx->nContacts + 1))
+            return fail(ctx, B2G_ERR_STATE, "TOI events did not converge");
+        WorldView w = ctx->view();
+        const int nc = ctx->nContacts;
+        hipLaunchKernelGGL(k_toi_begin_event, dim3(1), dim3(64), 0, s, st);
+        if (nc > 0) {
+            hipLaunchKernelGGL(k_toi_classify, dim3(blocks(nc)), dim3(256), 0, s, w, ctx->keep, st);
+            hipLaunchKernelGGL(k_toi_compute, dim3(blocks(nc, 64)), dim3(64), 0, s, w, ctx->keep, st);
+            hipLaunchKernelGGL(k_toi_compute_serial, dim3(1), dim3(64), 0, s, w, ctx->keep, st);
+            hipLaunchKernelGGL(k_toi_min, dim3(blocks(nc)), dim3(256), 0, s, w, st);
+        }
+        hipLaunchKernelGGL(k_toi_event, dim3(1), dim3(64), 0, s, w, st);
+        if (nc > 0) hipLaunchKernelGGL(k_toi_gather, dim3(blocks(nc)), dim3(256), 0, s, w, st, ctx->toiGather);
+        hipLaunchKernelGGL(k_toi_island, dim3(1), dim3(256), 0, s, w, ctx->toiCv, st, ctx->toiGather, p->dt,
+                           p->velocityIterations);
+        if (ctx->nFixtures > 0)
+            hipLaunchKernelGGL(k_toi_sync_fixtures, dim3(blocks(ctx->nFixtures)), dim3(256), 0, s, w, st);
+        if (nc > 0) hipLaunchKernelGGL(k_toi_invalidate, dim3(blocks(nc)), dim3(256), 0, s, w, st);
+        hipLaunchKernelGGL(k_toi_end_event, dim3(1), dim3(64), 0, s, w, st);
+        LAUNCHCHK(ctx);
+        HIPCHK(ctx, hipMemcpyAsync(ctx->hToi, st, sizeof(ToiState), hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        const ToiState& h = *ctx->hToi;
+        if (h.done) {
+            ctx->stepComplete = true;
+            break;
+        }
+        if (!h.active) continue;  // the pair did not touch: contact disabled for the step
+        if (h.moved > 0) {
+            const int before = ctx->nContacts;
+            if ((rc = runBroadphase(ctx)) != B2G_OK) return rc;
+            const int added = ctx->nContacts - before;
+            if (added > 0) {
+                hipLaunchKernelGGL(k_hash_insert_keys, dim3(blocks(added)), dim3(256), 0, s, ctx->newKeysSorted, added,
+                                   ctx->hashTable, ctx->hashMask);
+                LAUNCHCHK(ctx);
+            }
+        }
+        if (ctx->subStepping) {
+            ctx->stepComplete = false;
+            break;
+        }
+    }
+    profEnd(ctx, PH_TOI, pe);
+    ctx->stats.toiEvents = ctx->hToi->events;
+    return B2G_OK;
+}
+
+// b2World::Step in the reference's visiting order (see the reference-order
+// kernels above). Same collide/broad-phase as the coloured path; islands
+// and constraint order follow Box2D exactly.
 int stepReference(b2g_context* ctx, const b2g_step_params* p) {
     hipStream_t s = ctx->stream;
     const float dt = p->dt;
@@ -3962,7 +4596,7 @@ int stepReference(b2g_context* ctx, const b2g_step_params* p) {
     }
     ctx->stats.beginEvents = ctx->hcnt->beginEvents;
     ctx->stats.endEvents = ctx->hcnt->endEvents;
-    if (dt > 0.0f) {
+    if (dt > 0.0f && ctx->stepComplete) {
         WorldView w = ctx->view();
         const int nb = ctx->nBodies, nc = ctx->nContacts;
         hipLaunchKernelGGL(k_clear_island_flags, dim3(blocks(nb + 1)), dim3(256), 0, s, w, r);
@@ -3996,8 +4630,9 @@ int stepReference(b2g_context* ctx, const b2g_step_params* p) {
         if ((rc = runBroadphase(ctx)) != B2G_OK) return rc;
         ctx->stats.constraints = ctx->hcnt->nConstraints;
         ctx->stats.points = ctx->hcnt->points;
-        ctx->inv_dt0 = inv_dt;
     }
+    if (ctx->continuous && dt > 0.0f && (rc = solveTOI(ctx, p)) != B2G_OK) return rc;
+    if (dt > 0.0f) ctx->inv_dt0 = inv_dt;
     if (ctx->autoClearForces && ctx->nBodies > 0)
         hipLaunchKernelGGL(k_fill<float4>, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, ctx->force, ctx->nBodies,
                            make_float4(0.0f, 0.0f, 0.0f, 0.0f));
@@ -4103,8 +4738,9 @@ int b2g_destroy(b2g_context* ctx) {
     freeArray(ctx->fgroup); freeArray(ctx->fat); freeArray(ctx->moved); freeArray(ctx->range);
     for (ContactArrays* a : {&ctx->cur, &ctx->alt}) {
         freeArray(a->cfA); freeArray(a->cfB); freeArray(a->cflags); freeArray(a->cmat); freeArray(a->cman);
-        freeArray(a->ccolour);
+        freeArray(a->ccolour); freeArray(a->ctoi); freeArray(a->ctoiCount);
     }
+    freeToi(ctx);
     freeArray(ctx->keep); freeArray(ctx->idx); freeArray(ctx->hashTable); freeArray(ctx->newKeys);
     freeArray(ctx->newKeysSorted); freeArray(ctx->consKeys); freeArray(ctx->consKeysSorted);
     freeArray(ctx->consVals); freeArray(ctx->consOrder);
@@ -4146,13 +4782,11 @@ int b2g_reserve(b2g_context* ctx, int maxBodies, int maxFixtures, int maxContact
 
 int b2g_set_world(b2g_context* ctx, const b2g_world_def* def) {
     if (!ctx || !def) return B2G_ERR_ARG;
-    if (def->continuous || def->subStepping)
-        return fail(ctx, B2G_ERR_UNSUPPORTED, "continuous collision (TOI) and sub-stepping are not on the GPU path");
     ctx->gravity = mk(def->gravity[0], def->gravity[1]);
     ctx->allowSleep = def->allowSleep != 0;
     ctx->warmStarting = def->warmStarting != 0;
-    ctx->continuous = false;
-    ctx->subStepping = false;
+    ctx->continuous = def->continuous != 0;
+    ctx->subStepping = def->subStepping != 0;
     ctx->autoClearForces = def->autoClearForces != 0;
     return B2G_OK;
 }
@@ -4630,6 +5264,50 @@ int b2g_collide_batch(b2g_context* ctx, const b2g_shape* a, const b2g_transform*
     hipLaunchKernelGGL(k_collide_batch, dim3(blocks(n, 64)), dim3(64), 0, s, dA.p, dxa.p, dB.p, dxb.p, n, dm.p);
     LAUNCHCHK(ctx);
     HIPCHK(ctx, hipMemcpyAsync(out, dm.p, sizeof(b2g_manifold) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+int b2g_distance_batch(b2g_context* ctx, const b2g_shape* a, const b2g_transform* xfA, const b2g_shape* b,
+                       const b2g_transform* xfB, int n, float* out) {
+    if (!ctx || n < 0 || (n > 0 && (!a || !b || !xfA || !xfB || !out))) return B2G_ERR_ARG;
+    if (n == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<b2g_shape> dA, dB;
+    DevBuf<b2g_transform> dxa, dxb;
+    DevBuf<float> dout;
+    HIPCHK(ctx, dA.alloc(n)); HIPCHK(ctx, dB.alloc(n)); HIPCHK(ctx, dxa.alloc(n)); HIPCHK(ctx, dxb.alloc(n));
+    HIPCHK(ctx, dout.alloc(2 * (size_t)n));
+    HIPCHK(ctx, hipMemcpyAsync(dA.p, a, sizeof(b2g_shape) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dB.p, b, sizeof(b2g_shape) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dxa.p, xfA, sizeof(b2g_transform) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dxb.p, xfB, sizeof(b2g_transform) * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_distance_batch, dim3(blocks(n, 64)), dim3(64), 0, s, dA.p, dxa.p, dB.p, dxb.p, n, dout.p);
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, hipMemcpyAsync(out, dout.p, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+int b2g_toi_batch(b2g_context* ctx, const b2g_shape* a, const b2g_sweep* sweepA, const b2g_shape* b,
+                  const b2g_sweep* sweepB, int n, float tMax, int32_t* state, float* t) {
+    if (!ctx || n < 0 || (n > 0 && (!a || !b || !sweepA || !sweepB || !state || !t))) return B2G_ERR_ARG;
+    if (n == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<b2g_shape> dA, dB;
+    DevBuf<b2g_sweep> dsa, dsb;
+    DevBuf<int32_t> dst;
+    DevBuf<float> dt;
+    HIPCHK(ctx, dA.alloc(n)); HIPCHK(ctx, dB.alloc(n)); HIPCHK(ctx, dsa.alloc(n)); HIPCHK(ctx, dsb.alloc(n));
+    HIPCHK(ctx, dst.alloc(n)); HIPCHK(ctx, dt.alloc(n));
+    HIPCHK(ctx, hipMemcpyAsync(dA.p, a, sizeof(b2g_shape) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dB.p, b, sizeof(b2g_shape) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dsa.p, sweepA, sizeof(b2g_sweep) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dsb.p, sweepB, sizeof(b2g_sweep) * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_toi_batch, dim3(blocks(n, 64)), dim3(64), 0, s, dA.p, dsa.p, dB.p, dsb.p, n, tMax, dst.p, dt.p);
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, hipMemcpyAsync(state, dst.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(t, dt.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
     return B2G_OK;
 }
@@ -5212,7 +5890,7 @@ extern "C" {
 // original would.
 namespace {
 constexpr uint32_t kSnapMagic = 0x53473242u;  // "B2GS"
-constexpr uint32_t kSnapVersion = 1;
+constexpr uint32_t kSnapVersion = 2;
 struct SnapHeader {
     uint32_t magic, version;
     int32_t nBodies, nFixtures, nContacts, nJoints;
@@ -5220,7 +5898,7 @@ struct SnapHeader {
     float inv_dt0, cellSize;
     uint32_t claimTag;
     uint8_t allowSleep, warmStarting, continuous, subStepping, autoClearForces, newFixture, cellSizeSet, subWorlds;
-    uint8_t staticAsleep, pad[3];
+    uint8_t staticAsleep, stepComplete, pad[2];
     int32_t jointColourStart[kColourSlots + 1];
 };
 
@@ -5252,6 +5930,8 @@ void snapArrays(b2g_context* ctx, int nb, int nf, int nc, int nj, bool subWorlds
     visit(ctx->cur.cmat, sizeof(float2) * nc);
     visit(ctx->cur.cman, sizeof(b2g_manifold) * nc);
     visit(ctx->cur.ccolour, sizeof(int) * nc);
+    visit(ctx->cur.ctoi, sizeof(float) * nc);
+    visit(ctx->cur.ctoiCount, sizeof(int) * nc);
     visit(ctx->jBodies, sizeof(int2) * nj);
     visit(ctx->jAnchors, sizeof(float4) * nj);
     visit(ctx->jLimits, sizeof(float4) * nj);
@@ -5286,6 +5966,7 @@ int b2g_snapshot(b2g_context* ctx, void* out, size_t cap, size_t* size) {
     h.allowSleep = ctx->allowSleep; h.warmStarting = ctx->warmStarting; h.continuous = ctx->continuous;
     h.subStepping = ctx->subStepping; h.autoClearForces = ctx->autoClearForces; h.newFixture = ctx->newFixture;
     h.cellSizeSet = ctx->cellSizeSet; h.subWorlds = ctx->subWorlds; h.staticAsleep = ctx->staticAsleep;
+    h.stepComplete = ctx->stepComplete;
     for (int c = 0; c <= kColourSlots; ++c)
         h.jointColourStart[c] = nj > 0 && (int)ctx->jointColourStart.size() > c ? ctx->jointColourStart[c] : 0;
     uint8_t* o = (uint8_t*)out;
@@ -5365,6 +6046,7 @@ int b2g_restore(b2g_context* ctx, const void* in, size_t size) {
     ctx->newFixture = h.newFixture != 0;
     ctx->subWorlds = h.subWorlds != 0;
     ctx->staticAsleep = h.staticAsleep != 0;
+    ctx->stepComplete = h.stepComplete != 0;
     // colour claims only compare with the current tag: zero is older than any
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(ctx->capBodies)), dim3(256), 0, s, ctx->claim,
                        ctx->capBodies, 0ull);

@@ -42,6 +42,7 @@ enum : int {
 // plus the device-only state bits above 0x100.
 enum : int {
     CF_ISLAND = 0x0001, CF_TOUCHING = 0x0002, CF_ENABLED = 0x0004, CF_FILTER = 0x0008,
+    CF_TOI = 0x0020,         // cached time of impact is valid (b2Contact::e_toiFlag)
     CF_DEAD = 0x0100,        // destroyed this step, removed by compaction
     CF_CONSTRAINT = 0x0200,  // solved this step (touching, enabled, not a sensor, awake)
     CF_NOOVERLAP = 0x0400,   // fat AABBs stopped overlapping
@@ -195,6 +196,8 @@ struct WorldView {
     float2* cmat;      // friction restitution
     b2g_manifold* cman;
     int* ccolour;
+    float* ctoi;       // cached time of impact (with CF_TOI)
+    int* ctoiCount;    // TOI events of the contact in the step
 };
 
 }  // namespace b2g

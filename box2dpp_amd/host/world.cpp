@@ -644,8 +644,8 @@ int32_t b2World::Upload() {
         wd.gravity[1] = m_gravity[1];
         wd.allowSleep = m_allowSleep;
         wd.warmStarting = m_warmStarting;
-        wd.continuous = 0;
-        wd.subStepping = 0;
+        wd.continuous = m_continuousPhysics ? 1 : 0;
+        wd.subStepping = m_subStepping ? 1 : 0;
         wd.autoClearForces = m_autoClearForces;
         rc = b2g_set_world(m_ctx, &wd);
         if (rc != B2G_OK) return rc;
@@ -824,10 +824,6 @@ void b2World::Step(float dt, int32_t velocityIterations, int32_t positionIterati
         m_lastStatus = B2G_ERR_HIP;
         m_lastError = "no HIP device";
         return;
-    }
-    if (m_continuousPhysics || m_subStepping) {
-        // The reference runs SolveTOI here; the GPU path has no TOI phase.
-        m_lastError = "continuous physics is not executed on the GPU path; stepping without it";
     }
     m_locked = true;
     int rc = Upload();

@@ -142,8 +142,10 @@ typedef struct b2g_fixture_def {
 
 /* World-level switches (b2World::SetAllowSleeping / SetWarmStarting /
  * SetContinuousPhysics / SetSubStepping / SetAutoClearForces,
- * Box2D/Dynamics/b2World.h:146-217). continuous and subStepping must be 0:
- * the GPU step has no time-of-impact phase. */
+ * Box2D/Dynamics/b2World.h:146-217). continuous runs the time-of-impact
+ * phase after the discrete solve (b2World::SolveTOI,
+ * REF/Dynamics/b2World.cpp:579-897); subStepping stops it after one event
+ * per step. */
 typedef struct b2g_world_def {
     float gravity[2];
     int32_t allowSleep;
@@ -195,6 +197,7 @@ typedef struct b2g_step_stats {
                                   start, the training log or a trace is on) */
     int32_t fallback;          /* 1 if the learned start was replaced by the plain one */
     float residual;            /* RMS residual checked for the fallback, -1 if none */
+    int32_t toiEvents;         /* TOI events solved this step (continuous collision) */
 } b2g_step_stats;
 
 typedef struct b2g_context b2g_context;
@@ -240,9 +243,11 @@ int b2g_remove_fixtures(b2g_context* ctx, const int32_t* ids, int count);
 int b2g_update_fixtures(b2g_context* ctx, const int32_t* ids, const b2g_fixture_def* defs, int count);
 
 /* One time step: b2World::Step(dt, velocityIterations, positionIterations)
- * (Box2D/Dynamics/b2World.cpp:888-954; REF/Dynamics/b2World.cpp:899)
- * without the continuous (TOI) phase. Asynchronous on the context's stream:
- * returns once the work is queued; b2g_get_* and b2g_sync wait for it. */
+ * (Box2D/Dynamics/b2World.cpp:888-954; REF/Dynamics/b2World.cpp:899-964),
+ * with the continuous (TOI) phase when the world def asks for it.
+ * Asynchronous on the context's stream without TOI: returns once the work
+ * is queued; b2g_get_* and b2g_sync wait for it. The TOI phase reads one
+ * small record back per event. */
 int b2g_step(b2g_context* ctx, const b2g_step_params* params);
 /* Blocks until queued work on ctx has finished. */
 int b2g_sync(b2g_context* ctx);
@@ -390,6 +395,18 @@ int b2g_get_trace(b2g_context* ctx, double* curves, int cap, int* iters);
 /* ---------------------------------------------- per-function batch calls
  * Run one hot-path stage over host arrays (copied in and out); used for
  * per-function parity. */
+
+/* GJK distance between the core shapes (no radii) of n pairs from an empty
+ * simplex cache (b2Distance with useRadii = false,
+ * Box2D/Collision/b2Distance.h:99; REF/Collision/b2Distance.cpp:447-606): out[2i] = distance, out[2i+1] = simplex vertex count. */
+int b2g_distance_batch(b2g_context* ctx, const b2g_shape* a, const b2g_transform* xfA, const b2g_shape* b,
+                       const b2g_transform* xfB, int n, float* out);
+/* Time of impact of n swept pairs over [0, tMax] (b2TimeOfImpact,
+ * Box2D/Collision/b2TimeOfImpact.h:70; REF/Collision/b2TimeOfImpact.cpp:
+ * 257-489): state[i] in b2TOIOutput::State order (0 unknown, 1 failed,
+ * 2 overlapped, 3 touching, 4 separated), t[i] the time. */
+int b2g_toi_batch(b2g_context* ctx, const b2g_shape* a, const b2g_sweep* sweepA, const b2g_shape* b,
+                  const b2g_sweep* sweepB, int n, float tMax, int32_t* state, float* t);
 
 /* Narrowphase for n shape pairs already in registry order
  * (b2CollidePolygons / b2CollideCircles / b2CollidePolygonAndCircle /

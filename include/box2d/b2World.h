@@ -6,12 +6,14 @@
 // reference's do).
 //
 // Differences from the reference, by design of the GPU path:
-//  * continuous collision (SetContinuousPhysics(true)) and sub-stepping are
-//    not executed on the GPU; Step reports them through GetLastError().
+//  * continuous collision (SetContinuousPhysics, on by default as in the
+//    reference) runs on the GPU after the discrete solve; the contact
+//    updates of its TOI events call no listener.
 //  * sensors are rejected at creation; of the joint types, revolute joints
 //    run (with the coloured solver).
-//  * b2ContactListener::PreSolve is not called (no mid-step host hook);
-//    BeginContact, EndContact and PostSolve are replayed after Step.
+//  * with a contact listener, Step splits at the narrowphase: BeginContact,
+//    EndContact and PreSolve are called inside the step in list order,
+//    PostSolve after it for the solved contacts.
 #pragma once
 #include <vector>
 #include <string>
@@ -428,9 +430,9 @@ public:
     bool GetAllowSleeping() const { return m_allowSleep; }
     void SetWarmStarting(bool flag) { m_warmStarting = flag; }
     bool GetWarmStarting() const { return m_warmStarting; }
-    void SetContinuousPhysics(bool flag) { m_continuousPhysics = flag; }
+    void SetContinuousPhysics(bool flag) { m_continuousPhysics = flag; m_worldDirty = true; }
     bool GetContinuousPhysics() const { return m_continuousPhysics; }
-    void SetSubStepping(bool flag) { m_subStepping = flag; }
+    void SetSubStepping(bool flag) { m_subStepping = flag; m_worldDirty = true; }
     bool GetSubStepping() const { return m_subStepping; }
     int32_t GetProxyCount() const { return (int32_t)m_fixtures.size(); }
     int32_t GetBodyCount() const { return m_bodyCount; }

@@ -41,7 +41,7 @@ STATS = np.dtype([
     ("bodies", "<i4"), ("fixtures", "<i4"), ("contacts", "<i4"), ("constraints", "<i4"), ("points", "<i4"),
     ("colours", "<i4"), ("colouringRounds", "<i4"), ("newPairs", "<i4"), ("positionIterations", "<i4"),
     ("beginEvents", "<i4"), ("endEvents", "<i4"), ("overflow", "<i4"), ("minSeparation", "<f4"),
-    ("cellSize", "<f4"), ("freshPoints", "<i4"), ("fallback", "<i4"), ("residual", "<f4"),
+    ("cellSize", "<f4"), ("freshPoints", "<i4"), ("fallback", "<i4"), ("residual", "<f4"), ("toiEvents", "<i4"),
 ])
 
 SCHEDULE = np.dtype([("islands", "<i4"), ("collideOrder", "<i4"), ("solverBlock", "<i4"), ("collideBlock", "<i4"),
