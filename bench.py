@@ -447,6 +447,66 @@ def cpu_batched(sizes, args):
                       f"of {args.batch_steps}, one world per thread, {threads} threads, {el:.1f} s"}
 
 
+def continuous(lib, args):
+    """Continuous collision at the reference's default (SetContinuousPhysics
+    on): the random pile of args.ccd_bodies bodies, sleep off, steps
+    1..args.ccd_steps, coloured solver plus the TOI event loop; body-steps/s
+    with the state resident on the device. CPU leg: the reference
+    b2World::Step with its SolveTOI on the same steps of the same pile, one
+    core."""
+    progress(f"continuous collision: {args.ccd_bodies}-body pile, {args.ccd_steps} steps")
+    h = ctypes.c_void_p(lib.b2pp_world_create(0.0, -10.0))
+    lib.b2pp_world_set_flags(h, 0, 1, 1, 0)
+    rc = lib.b2pp_world_load_scene(h, b"pile", args.ccd_bodies, ctypes.c_uint(args.seed))
+    if rc != 0:
+        raise RuntimeError(f"pile load failed rc={rc}")
+    lib.b2pp_world_sync(h)
+    events = 0
+    t0 = time.perf_counter()
+    for _ in range(args.ccd_steps):
+        lib.b2pp_world_step(h, ctypes.c_float(1.0 / 60.0), args.vel, args.pos, 1)
+        events += stats_of(lib, h).toiEvents
+    lib.b2pp_world_sync(h)
+    wall = time.perf_counter() - t0
+    pen = np.zeros(3, np.float64)
+    lib.b2pp_world_penetration(h, pen.ctypes.data_as(ctypes.c_void_p))
+    lib.b2pp_world_destroy(h)
+    out = {"value": args.ccd_bodies * args.ccd_steps / wall, "unit": "body-steps/s",
+           "ms_per_step": 1e3 * wall / args.ccd_steps, "toi_events_per_step": events / args.ccd_steps,
+           "us_per_toi_event": 1e6 * wall / max(events, 1), "mean_penetration_m": float(pen[0]),
+           "workload": f"random pile, {args.ccd_bodies} bodies, seed {args.seed}, steps 1-{args.ccd_steps}, "
+                       f"{args.vel}/{args.pos} iterations, coloured solver, sleep off, CCD on"}
+    if not args.no_cpu:
+        progress("continuous collision: CPU reference on the same steps")
+        ref = os.path.join(ROOT, "oracle", "_ref", "libb2ref.so")
+        kind = "reference"
+        if os.path.exists(ref):
+            rl, pre = ctypes.CDLL(ref), "ref_"
+        else:
+            rl, pre = ctypes.CDLL(os.path.join(ROOT, "oracle", "_build", "liborc.so")), "orc_"
+            kind = "port"
+        f = lambda name: getattr(rl, pre + name)
+        f("world_create").restype = ctypes.c_void_p
+        f("world_create").argtypes = [ctypes.c_float, ctypes.c_float]
+        hc = ctypes.c_void_p(f("world_create")(0.0, -10.0))
+        f("world_set_flags")(hc, 0, 1, 1, 0)
+        f("world_load_scene")(hc, b"pile", args.ccd_bodies, ctypes.c_uint(args.seed))
+        t0 = time.perf_counter()
+        done = 0
+        for k in range(args.ccd_steps):
+            f("world_step")(hc, ctypes.c_float(1.0 / 60.0), args.vel, args.pos, 1)
+            done += 1
+            if time.perf_counter() - t0 > args.ccd_cpu_seconds:
+                break
+        el = time.perf_counter() - t0
+        f("world_destroy")(hc)
+        out["cpu_baseline"] = {"value": args.ccd_bodies * done / el, "unit": "body-steps/s", "cores": 1,
+                               "kind": kind,
+                               "sample": f"steps 1-{done} of the same pile with SolveTOI, one host thread, "
+                                         f"{el:.1f} s"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -472,6 +532,10 @@ def main():
     ap.add_argument("--stress-bodies", type=int, default=125000)
     ap.add_argument("--stress-steps", type=int, default=600)
     ap.add_argument("--stress-seed", type=int, default=1)
+    ap.add_argument("--no-ccd", action="store_true", help="skip the continuous-collision run")
+    ap.add_argument("--ccd-bodies", type=int, default=10000)
+    ap.add_argument("--ccd-steps", type=int, default=500)
+    ap.add_argument("--ccd-cpu-seconds", type=float, default=60.0)
     ap.add_argument("--tune-rounds", type=int, default=3,
                     help="autotuner rounds on the warm-up copy (0: default schedule)")
     ap.add_argument("--tune-at", type=int, default=500, help="step of the warm-up copy at which to autotune")
@@ -565,6 +629,7 @@ def main():
         learned_all, _ = learned_warm_start(lib, args, 1, args.all_points_stride)
     batched = None if args.no_batched else batched_worlds(lib, args)
     stressed = None if args.no_stress else stress(lib, args, params)
+    ccd = None if args.no_ccd else continuous(lib, args)
 
     cpu = None
     if not args.no_cpu:
@@ -599,6 +664,7 @@ def main():
         "learned_warm_start_every_point": learned_all,
         "batched_worlds": batched,
         "stress": stressed,
+        "continuous_collision": ccd,
         "final_state": {"contacts": st.contacts, "constraints": st.constraints, "points": st.points,
                         "colours": st.colours, "mean_penetration_m": float(pen[0]), "max_penetration_m": float(pen[1])},
         "phase_ms_per_step": {k: round(v[1] / max(args.profile_steps, 1), 4) for k, v in phases.items()},

@@ -1988,17 +1988,16 @@ __global__ void k_pairs_wide(const float4* fat, const uint8_t* moved, int n, Gri
 // New contacts from the sorted pair keys (b2Contact::Create with the
 // registry's fixture swap, b2Contact ctor, AddPair's wake-up;
 // REF/Dynamics/Contacts/b2Contact.cpp:64-95, 136-168, REF/Dynamics/b2ContactManager.cpp:284-293).
-__global__ void k_append_contacts(WorldView w, const unsigned long long* keys, int nNew, int base) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nNew) return;
-    const unsigned long long key = keys[k];
+// b2ContactManager::AddPair's contact creation (REF/Dynamics/b2ContactManager.cpp:
+// 179-296): registry order, mixed friction and restitution, no manifold, and
+// both bodies woken unless a fixture is a sensor.
+__device__ inline void appendContactOne(const WorldView& w, unsigned long long key, int c) {
     int a = (int)(key >> 32), b = (int)(key & 0xFFFFFFFFu);
     if (!primaryPair(w.shape[a].type, w.shape[b].type)) {
         const int t = a;
         a = b;
         b = t;
     }
-    const int c = base + k;
     w.cfA[c] = a;
     w.cfB[c] = b;
     w.cflags[c] = CF_ENABLED;
@@ -2014,6 +2013,12 @@ __global__ void k_append_contacts(WorldView w, const unsigned long long* keys, i
         wakeBody(w, w.fbody[a]);
         wakeBody(w, w.fbody[b]);
     }
+}
+
+__global__ void k_append_contacts(WorldView w, const unsigned long long* keys, int nNew, int base) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nNew) return;
+    appendContactOne(w, keys[k], base + k);
 }
 
 // Contacts between bodies that a new joint keeps apart are flagged for
@@ -2443,13 +2448,24 @@ __global__ void k_penetration(WorldView w, double* acc, int* npts, unsigned long
 // moves the later-starting one for the contacts that follow.
 // ====================================================================
 constexpr int kToiGatherCap = 2048;  // edges of the TOI pair sorted in LDS
+constexpr int kToiMovedCap = 256;    // proxies one event may move before the grid broad-phase takes over
+constexpr int kToiNewCap = 1024;     // new pairs one event may create likewise
 
 struct ToiState {
+    // read back after every event
     unsigned long long minKey;  // (time bits << 32) | list position of the earliest TOI
     int mismatch;               // a pending contact's sweeps start at different times
     int active;                 // the event built an island
     int done;                   // no event left below 1 - 10 eps
-    int moved;                  // proxies the event moved
+    int events;                 // events solved in the pass
+    int nMoved;                 // proxies moved by the event (list below, or overflow)
+    int nNew;                   // new pairs found for them
+    int fallback;               // lists overflowed: the host runs the grid broad-phase
+    int live;                   // contacts in the table (new pairs append on the device)
+    int added;                  // contacts appended by the pass
+    int halt;                   // later events of a batch do nothing (done, fallback, sub-step)
+    int subStop;                // sub-stepping: the pass ends after one island event
+    // device only
     int bodyA, bodyB, contact;
     float alpha;
     int nGather;
@@ -2457,9 +2473,11 @@ struct ToiState {
     int bodies[2 * kMaxTOIContacts];
     int contacts[kMaxTOIContacts];
     float4 backup[4];           // pos0 / pos of body A and B before the advance
-    int events;                 // events solved in the pass
+    int movedList[kToiMovedCap];
+    unsigned long long newKeys[kToiNewCap];
     Counters scratch;           // counters the constraint set-up writes
 };
+constexpr size_t kToiReadback = offsetof(ToiState, bodyA);
 
 __device__ inline bool toiPending(const WorldView& w, int c, int flags) {
     if (!(flags & CF_ENABLED)) return false;
@@ -2505,22 +2523,23 @@ __device__ inline void advanceBodyTo(const WorldView& w, int b, float alpha) {
     syncTransformOne(w, b);
 }
 
-// TOI of a contact over the rest of the step, REF/Dynamics/b2World.cpp:645-689.
-__device__ inline float contactToi(const WorldView& w, int c) {
-    const int fA = w.cfA[c], fB = w.cfB[c];
-    const int bA = w.fbody[fA], bB = w.fbody[fB];
+// The two sweeps of a contact brought to a common start: the one that
+// starts earlier is advanced (and stays advanced), REF/Dynamics/b2World.cpp:
+// 645-660.
+__device__ inline void alignSweeps(const WorldView& w, int c) {
+    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
     const float alphaA = w.pos0[bA].w, alphaB = w.pos0[bB].w;
-    float alpha0 = alphaA;
-    if (alphaA < alphaB) {
-        alpha0 = alphaB;
-        advanceSweepStart(w, bA, alpha0);
-    } else if (alphaB < alphaA) {
-        alpha0 = alphaA;
-        advanceSweepStart(w, bB, alpha0);
-    }
-    const Proxy pA = proxyOf(w.shape[fA]), pB = proxyOf(w.shape[fB]);
+    if (alphaA < alphaB) advanceSweepStart(w, bA, alphaB);
+    else if (alphaB < alphaA) advanceSweepStart(w, bB, alphaA);
+}
+
+// TOI of a contact over the rest of the step from aligned sweeps,
+// REF/Dynamics/b2World.cpp:662-689.
+__device__ inline float toiFromSweeps(const WorldView& w, int c, const Sweep& sA, const Sweep& sB) {
+    const Proxy pA = proxyOf(w.shape[w.cfA[c]]), pB = proxyOf(w.shape[w.cfB[c]]);
     float beta;
-    const int state = timeOfImpact(&beta, pA, pB, sweepOf(w, bA), sweepOf(w, bB), 1.0f);
+    const int state = timeOfImpact(&beta, pA, pB, sA, sB, 1.0f);
+    const float alpha0 = sA.alpha0;
     return state == TOI_TOUCHING ? fminB(alpha0 + (1.0f - alpha0) * beta, 1.0f) : 1.0f;
 }
 
@@ -2642,51 +2661,68 @@ __global__ void k_toi_reset_contacts(WorldView w) {
 }
 
 __global__ void k_toi_begin_event(ToiState* st) {
-    if (threadIdx.x != 0) return;
+    if (threadIdx.x != 0 || st->halt) return;
     st->minKey = ~0ull;
     st->mismatch = 0;
     st->active = 0;
-    st->moved = 0;
     st->nGather = 0;
     st->nBodies = 0;
     st->nContacts = 0;
+    st->nMoved = 0;
+    st->nNew = 0;
+    st->fallback = 0;
 }
 
 // Contacts whose TOI the scan computes, and whether their sweeps need aligning.
 __global__ void k_toi_classify(WorldView w, int* pending, ToiState* st) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= w.nContacts) return;
+    if (c >= st->live || st->halt) return;
     const bool p = toiPending(w, c, w.cflags[c]);
     pending[c] = p ? 1 : 0;
     if (p && w.pos0[w.fbody[w.cfA[c]]].w != w.pos0[w.fbody[w.cfB[c]]].w) atomicOr(&st->mismatch, 1);
 }
 
+// With every pending pair's sweeps already aligned, each TOI depends on its
+// own two bodies only: one lane per contact.
 __global__ void k_toi_compute(WorldView w, const int* pending, const ToiState* st) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= w.nContacts || st->mismatch || !pending[c]) return;
-    w.ctoi[c] = contactToi(w, c);
+    if (c >= st->live || st->halt || st->mismatch || !pending[c]) return;
+    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
+    w.ctoi[c] = toiFromSweeps(w, c, sweepOf(w, bA), sweepOf(w, bB));
     w.cflags[c] |= CF_TOI;
 }
 
-__global__ void k_toi_compute_serial(WorldView w, const int* pending, const ToiState* st) {
-    if (blockIdx.x != 0 || threadIdx.x != 0 || !st->mismatch) return;
-    for (int c = w.nContacts - 1; c >= 0; --c) {
+// Otherwise the alignments run in list order on one lane, since an advance
+// moves a body for the contacts after it, and each contact keeps the two
+// sweeps it saw; the TOIs then run in parallel from those.
+__global__ void k_toi_align_serial(WorldView w, const int* pending, const ToiState* st, Sweep* seen) {
+    if (blockIdx.x != 0 || threadIdx.x != 0 || st->halt || !st->mismatch) return;
+    for (int c = st->live - 1; c >= 0; --c) {
         if (!pending[c]) continue;
-        w.ctoi[c] = contactToi(w, c);
-        w.cflags[c] |= CF_TOI;
+        alignSweeps(w, c);
+        seen[2 * c] = sweepOf(w, w.fbody[w.cfA[c]]);
+        seen[2 * c + 1] = sweepOf(w, w.fbody[w.cfB[c]]);
     }
+}
+
+__global__ void k_toi_compute_aligned(WorldView w, const int* pending, const ToiState* st, const Sweep* seen) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= st->live || st->halt || !st->mismatch || !pending[c]) return;
+    w.ctoi[c] = toiFromSweeps(w, c, seen[2 * c], seen[2 * c + 1]);
+    w.cflags[c] |= CF_TOI;
 }
 
 // Earliest cached TOI below 1, first in list order among equal times.
 __global__ void k_toi_min(WorldView w, ToiState* st) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (st->halt) return;
     unsigned long long key = ~0ull;
-    if (c < w.nContacts) {
+    if (c < st->live) {
         const int flags = w.cflags[c];
         if ((flags & CF_ENABLED) && (flags & CF_TOI) && w.ctoiCount[c] <= kMaxSubSteps) {
             const float alpha = w.ctoi[c];
             if (alpha < 1.0f)
-                key = ((unsigned long long)__float_as_uint(alpha) << 32) | (unsigned)(w.nContacts - 1 - c);
+                key = ((unsigned long long)__float_as_uint(alpha) << 32) | (unsigned)(st->live - 1 - c);
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -2699,14 +2735,15 @@ __global__ void k_toi_min(WorldView w, ToiState* st) {
 // The event's TOI pair: advance both bodies, update the contact; an island
 // starts when it touches.
 __global__ void k_toi_event(WorldView w, ToiState* st) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (blockIdx.x != 0 || threadIdx.x != 0 || st->halt) return;
     const unsigned long long key = st->minKey;
     const float alpha = key == ~0ull ? 1.0f : __uint_as_float((unsigned)(key >> 32));
     if (key == ~0ull || 1.0f - 10.0f * FLT_EPSILON < alpha) {
         st->done = 1;
+        st->halt = 1;
         return;
     }
-    const int c = w.nContacts - 1 - (int)(key & 0xFFFFFFFFu);
+    const int c = st->live - 1 - (int)(key & 0xFFFFFFFFu);
     const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
     st->backup[0] = w.pos0[bA];
     st->backup[1] = w.pos[bA];
@@ -2748,9 +2785,9 @@ __global__ void k_toi_event(WorldView w, ToiState* st) {
 // Contact edges of the TOI pair's dynamic bodies: (0 for A / 1 for B) << 32 | list position.
 __global__ void k_toi_gather(WorldView w, ToiState* st, unsigned long long* gather) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= w.nContacts || !st->active) return;
+    if (c >= st->live || st->halt || !st->active) return;
     const int x = w.fbody[w.cfA[c]], y = w.fbody[w.cfB[c]];
-    const unsigned pos = (unsigned)(w.nContacts - 1 - c);
+    const unsigned pos = (unsigned)(st->live - 1 - c);
     for (int k = 0; k < 2; ++k) {
         const int body = k == 0 ? st->bodyA : st->bodyB;
         if (w.btype[body] != BT_DYNAMIC || (x != body && y != body)) continue;
@@ -2789,7 +2826,7 @@ __device__ inline void toiVisitEdge(const WorldView& w, ToiState* st, int body, 
 __global__ __launch_bounds__(256) void k_toi_island(WorldView w, ConView cv, ToiState* st,
                                                     const unsigned long long* gather, float dt, int velIters) {
     __shared__ unsigned long long keys[kToiGatherCap];
-    if (!st->active) return;
+    if (st->halt || !st->active) return;
     const int n = st->nGather;
     const bool sorted = n <= kToiGatherCap;
     if (sorted) {
@@ -2816,7 +2853,7 @@ __global__ __launch_bounds__(256) void k_toi_island(WorldView w, ConView cv, Toi
     }
     if (threadIdx.x != 0) return;
     const int bA = st->bodyA, bB = st->bodyB;
-    const int nc = w.nContacts;
+    const int nc = st->live;
     if (sorted) {
         for (int i = 0; i < n; ++i) {
             if (st->nBodies == 2 * kMaxTOIContacts || st->nContacts == kMaxTOIContacts) break;
@@ -2862,16 +2899,18 @@ __global__ __launch_bounds__(256) void k_toi_island(WorldView w, ConView cv, Toi
 // Fixtures of the island's dynamic bodies to their swept boxes.
 __global__ void k_toi_sync_fixtures(WorldView w, ToiState* st) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= w.nFixtures || !st->active) return;
+    if (f >= w.nFixtures || st->halt || !st->active) return;
     const int b = w.fbody[f];
     if (!(w.bflags[b] & BF_ISLAND) || w.btype[b] != BT_DYNAMIC) return;
-    if (syncFixtureOne(w, f, b)) atomicAdd(&st->moved, 1);
+    if (!syncFixtureOne(w, f, b)) return;
+    const int slot = atomicAdd(&st->nMoved, 1);
+    if (slot < kToiMovedCap) st->movedList[slot] = f;
 }
 
 // Cached TOIs and island marks of every contact on a displaced body dropped.
 __global__ void k_toi_invalidate(WorldView w, const ToiState* st) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= w.nContacts || !st->active) return;
+    if (c >= st->live || st->halt || !st->active) return;
     const int x = w.fbody[w.cfA[c]], y = w.fbody[w.cfB[c]];
     const bool hit = ((w.bflags[x] & BF_ISLAND) && w.btype[x] == BT_DYNAMIC) ||
                      ((w.bflags[y] & BF_ISLAND) && w.btype[y] == BT_DYNAMIC);
@@ -2879,8 +2918,89 @@ __global__ void k_toi_invalidate(WorldView w, const ToiState* st) {
 }
 
 __global__ void k_toi_end_event(WorldView w, ToiState* st) {
-    if (!st->active) return;
+    if (st->halt || !st->active) return;
     for (int i = threadIdx.x; i < st->nBodies; i += blockDim.x) w.bflags[st->bodies[i]] &= ~BF_ISLAND;
+}
+
+// b2BroadPhase::UpdatePairs for the event's moved proxies only: every proxy
+// whose fat box overlaps a moved one's, a pair of two moved proxies found by
+// the lower id's thread, filtered as AddPair filters.
+__global__ void k_toi_pairs(WorldView w, ToiState* st, const unsigned long long* table, uint32_t hmask) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= w.nFixtures || st->halt || !st->active) return;
+    const int nm = st->nMoved;
+    if (nm == 0 || nm > kToiMovedCap) return;
+    const float4 ff = w.fat[f];
+    const bool fMoved = w.moved[f] != 0;
+    for (int i = 0; i < nm; ++i) {
+        const int q = st->movedList[i];
+        if (q == f || (fMoved && f > q)) continue;
+        if (!overlaps(ff, w.fat[q])) continue;
+        const int a = f < q ? f : q, b = f < q ? q : f;
+        if (!acceptPair(w, a, b, table, hmask)) continue;
+        const int slot = atomicAdd(&st->nNew, 1);
+        if (slot < kToiNewCap) st->newKeys[slot] = pairKey(a, b);
+    }
+}
+
+// New contacts of the event in pair order (sorted keys, as the reference
+// sorts its pair buffer), appended to the table and its hash; the moved
+// flags cleared. Lists that overflowed leave everything to the host's grid
+// broad-phase instead.
+__global__ __launch_bounds__(256) void k_toi_append(WorldView w, ToiState* st, unsigned long long* table,
+                                                   uint32_t hmask, int capContacts, int subStepping) {
+    __shared__ unsigned long long keys[kToiNewCap];
+    if (st->halt || !st->active) return;
+    const int nm = st->nMoved, nn = st->nNew, base = st->live;
+    if (nm > kToiMovedCap || nn > kToiNewCap || base + nn > capContacts) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            st->fallback = 1;
+            st->halt = 1;
+        }
+        return;
+    }
+    if (nn > 0) {
+        int size = 1;
+        while (size < nn) size <<= 1;
+        for (int i = threadIdx.x; i < size; i += blockDim.x) keys[i] = i < nn ? st->newKeys[i] : ~0ull;
+        __syncthreads();
+        for (int k = 2; k <= size; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = threadIdx.x; i < size; i += blockDim.x) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const unsigned long long a = keys[i], b = keys[l];
+                        if ((a > b) == ((i & k) == 0)) {
+                            keys[i] = b;
+                            keys[l] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+            appendContactOne(w, keys[i], base + i);
+            const unsigned long long key = keys[i];
+            uint32_t h = hashU64(key) & hmask;
+            for (;;) {
+                const unsigned long long prev = atomicCAS(&table[h], ~0ull, key);
+                if (prev == ~0ull || prev == key) break;
+                h = (h + 1) & hmask;
+            }
+        }
+    }
+    for (int i = threadIdx.x; i < nm; i += blockDim.x) w.moved[st->movedList[i]] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->live = base + nn;
+        st->added += nn;
+        if (subStepping) {
+            st->subStop = 1;
+            st->halt = 1;
+        }
+    }
 }
 
 inline int blocks(int n, int bs = 256) { return (n + bs - 1) / bs; }
@@ -3028,6 +3148,8 @@ struct b2g_context {
     ToiState* toiState = nullptr;   // device
     ToiState* hToi = nullptr;       // pinned host mirror
     unsigned long long* toiGather = nullptr;
+    Sweep* toiSeen = nullptr;       // per contact: the aligned sweeps of a list-order scan
+    int toiSeenCap = 0;
     ConView toiCv{};                // constraints of one TOI island
     // split step (b2g_step_collide / b2g_step_solve) for the host's PreSolve
     bool keepOldManifolds = false;
@@ -4444,6 +4566,7 @@ int ensureToi(b2g_context* ctx) {
     if (ctx->toiState) return B2G_OK;
     HIPCHK(ctx, hipMalloc((void**)&ctx->toiState, sizeof(ToiState)));
     HIPCHK(ctx, hipHostMalloc((void**)&ctx->hToi, sizeof(ToiState), hipHostMallocDefault));
+    memset(ctx->hToi, 0, sizeof(ToiState));
     HIPCHK(ctx, hipMalloc((void**)&ctx->toiGather, sizeof(unsigned long long) * kToiGatherCap));
     ConView& c = ctx->toiCv;
     const size_t n = kMaxTOIContacts;
@@ -4472,28 +4595,54 @@ void freeToi(b2g_context* ctx) {
     if (ctx->hToi) (void)hipHostFree(ctx->hToi);
     ctx->hToi = nullptr;
     freeArray(ctx->toiGather);
+    freeArray(ctx->toiSeen);
+    ctx->toiSeenCap = 0;
     ConView& c = ctx->toiCv;
     freeArray(c.contact); freeArray(c.bodies); freeArray(c.mass); freeArray(c.normal); freeArray(c.r0);
     freeArray(c.r1); freeArray(c.pmass); freeArray(c.bias); freeArray(c.imp); freeArray(c.K); freeArray(c.NM);
     freeArray(c.info); freeArray(c.lp); freeArray(c.lnp); freeArray(c.lc); freeArray(c.rad); freeArray(c.island);
 }
 
-__global__ void k_toi_start(ToiState* st) {
+__global__ void k_toi_start(ToiState* st, int live) {
     if (threadIdx.x != 0) return;
     st->done = 0;
     st->events = 0;
+    st->live = live;
+    st->added = 0;
+    st->halt = 0;
+    st->subStop = 0;
+    st->fallback = 0;
 }
 
-// b2World::SolveTOI, REF/Dynamics/b2World.cpp:579-897. One pass of this loop
-// per TOI event; the events themselves run on the device (kernels above),
-// the host reads back one small record per event to decide whether to find
-// new pairs and whether the pass is over.
+// After the host ran the grid broad-phase for an event: resume the batch.
+__global__ void k_toi_resume(ToiState* st, int live) {
+    if (threadIdx.x != 0) return;
+    st->live = live;
+    st->halt = 0;
+    st->fallback = 0;
+}
+
+// b2World::SolveTOI, REF/Dynamics/b2World.cpp:579-897. The events run on the
+// device, one launch sequence each, in batches that grow geometrically;
+// every kernel of an event does nothing once the pass has halted (no event
+// left, a list overflowed, or the sub-step is over), and the host reads one
+// small record per batch. The contact count lives on the device during the
+// pass, so contact kernels are launched over the table's capacity.
 int solveTOI(b2g_context* ctx, const b2g_step_params* p) {
     hipStream_t s = ctx->stream;
     int rc;
     if ((rc = ensureToi(ctx)) != B2G_OK) return rc;
+    if ((rc = reserveContacts(ctx)) != B2G_OK) return rc;
     // the step's broad-phase appended contacts the pair hash does not hold yet
     if ((rc = rebuildHash(ctx)) != B2G_OK) return rc;
+    // room for the pairs events add; a grown table gets its hash rebuilt
+    auto reserve = [&]() -> int {
+        const int before = ctx->capContacts;
+        int r = reserveContacts(ctx);
+        if (r == B2G_OK && ctx->capContacts != before) r = rebuildHash(ctx);
+        return r;
+    };
+    ctx->hToi->events = 0;
     ToiState* st = ctx->toiState;
     if (ctx->stepComplete) {
         WorldView w = ctx->view();
@@ -4502,58 +4651,84 @@ int solveTOI(b2g_context* ctx, const b2g_step_params* p) {
         if (ctx->nContacts > 0)
             hipLaunchKernelGGL(k_toi_reset_contacts, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w);
     }
-    hipLaunchKernelGGL(k_toi_start, dim3(1), dim3(64), 0, s, st);
+    hipLaunchKernelGGL(k_toi_start, dim3(1), dim3(64), 0, s, st, ctx->nContacts);
     LAUNCHCHK(ctx);
     const int pe = profStart(ctx);
-    long long passes = 0;
+    int batch = 1;
     for (;;) {
         // every event raises some contact's TOI count or disables it, so a
         // pass ends within kMaxSubSteps + 1 events per contact
-        if (++passes > (long long)(kMaxSubSteps + 2) * ((long long)ctx->nContacts + 1))
+        if ((long long)ctx->hToi->events > (long long)(kMaxSubSteps + 2) * ((long long)ctx->nContacts + 1))
             return fail(ctx, B2G_ERR_STATE, "TOI events did not converge");
-        WorldView w = ctx->view();
-        const int nc = ctx->nContacts;
-        hipLaunchKernelGGL(k_toi_begin_event, dim3(1), dim3(64), 0, s, st);
-        if (nc > 0) {
-            hipLaunchKernelGGL(k_toi_classify, dim3(blocks(nc)), dim3(256), 0, s, w, ctx->keep, st);
-            hipLaunchKernelGGL(k_toi_compute, dim3(blocks(nc, 64)), dim3(64), 0, s, w, ctx->keep, st);
-            hipLaunchKernelGGL(k_toi_compute_serial, dim3(1), dim3(64), 0, s, w, ctx->keep, st);
-            hipLaunchKernelGGL(k_toi_min, dim3(blocks(nc)), dim3(256), 0, s, w, st);
+        if (ctx->toiSeenCap < ctx->capContacts) {
+            freeArray(ctx->toiSeen);
+            HIPCHK(ctx, hipMalloc((void**)&ctx->toiSeen, sizeof(Sweep) * 2 * (size_t)ctx->capContacts));
+            ctx->toiSeenCap = ctx->capContacts;
         }
-        hipLaunchKernelGGL(k_toi_event, dim3(1), dim3(64), 0, s, w, st);
-        if (nc > 0) hipLaunchKernelGGL(k_toi_gather, dim3(blocks(nc)), dim3(256), 0, s, w, st, ctx->toiGather);
-        hipLaunchKernelGGL(k_toi_island, dim3(1), dim3(256), 0, s, w, ctx->toiCv, st, ctx->toiGather, p->dt,
-                           p->velocityIterations);
-        if (ctx->nFixtures > 0)
-            hipLaunchKernelGGL(k_toi_sync_fixtures, dim3(blocks(ctx->nFixtures)), dim3(256), 0, s, w, st);
-        if (nc > 0) hipLaunchKernelGGL(k_toi_invalidate, dim3(blocks(nc)), dim3(256), 0, s, w, st);
-        hipLaunchKernelGGL(k_toi_end_event, dim3(1), dim3(64), 0, s, w, st);
+        const WorldView w = ctx->view();
+        const int cap = ctx->capContacts;
+        for (int e = 0; e < batch; ++e) {
+            hipLaunchKernelGGL(k_toi_begin_event, dim3(1), dim3(64), 0, s, st);
+            hipLaunchKernelGGL(k_toi_classify, dim3(blocks(cap)), dim3(256), 0, s, w, ctx->keep, st);
+            hipLaunchKernelGGL(k_toi_compute, dim3(blocks(cap, 64)), dim3(64), 0, s, w, ctx->keep, st);
+            hipLaunchKernelGGL(k_toi_align_serial, dim3(1), dim3(64), 0, s, w, ctx->keep, st, ctx->toiSeen);
+            hipLaunchKernelGGL(k_toi_compute_aligned, dim3(blocks(cap, 64)), dim3(64), 0, s, w, ctx->keep, st,
+                               ctx->toiSeen);
+            hipLaunchKernelGGL(k_toi_min, dim3(blocks(cap)), dim3(256), 0, s, w, st);
+            hipLaunchKernelGGL(k_toi_event, dim3(1), dim3(64), 0, s, w, st);
+            hipLaunchKernelGGL(k_toi_gather, dim3(blocks(cap)), dim3(256), 0, s, w, st, ctx->toiGather);
+            hipLaunchKernelGGL(k_toi_island, dim3(1), dim3(256), 0, s, w, ctx->toiCv, st, ctx->toiGather, p->dt,
+                               p->velocityIterations);
+            if (ctx->nFixtures > 0)
+                hipLaunchKernelGGL(k_toi_sync_fixtures, dim3(blocks(ctx->nFixtures)), dim3(256), 0, s, w, st);
+            hipLaunchKernelGGL(k_toi_invalidate, dim3(blocks(cap)), dim3(256), 0, s, w, st);
+            hipLaunchKernelGGL(k_toi_end_event, dim3(1), dim3(64), 0, s, w, st);
+            if (ctx->nFixtures > 0)
+                hipLaunchKernelGGL(k_toi_pairs, dim3(blocks(ctx->nFixtures)), dim3(256), 0, s, w, st, ctx->hashTable,
+                                   ctx->hashMask);
+            hipLaunchKernelGGL(k_toi_append, dim3(1), dim3(256), 0, s, w, st, ctx->hashTable, ctx->hashMask, cap,
+                               ctx->subStepping ? 1 : 0);
+        }
         LAUNCHCHK(ctx);
-        HIPCHK(ctx, hipMemcpyAsync(ctx->hToi, st, sizeof(ToiState), hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->hToi, st, kToiReadback, hipMemcpyDeviceToHost, s));
         HIPCHK(ctx, hipStreamSynchronize(s));
         const ToiState& h = *ctx->hToi;
+        ctx->nContacts = h.live;
         if (h.done) {
             ctx->stepComplete = true;
             break;
         }
-        if (!h.active) continue;  // the pair did not touch: contact disabled for the step
-        if (h.moved > 0) {
+        if (h.fallback) {
+            // too many moved proxies or new pairs for the event's lists, or
+            // no room: the grid broad-phase over the moved flags, as at the
+            // end of a step
             const int before = ctx->nContacts;
             if ((rc = runBroadphase(ctx)) != B2G_OK) return rc;
             const int added = ctx->nContacts - before;
             if (added > 0) {
                 hipLaunchKernelGGL(k_hash_insert_keys, dim3(blocks(added)), dim3(256), 0, s, ctx->newKeysSorted, added,
                                    ctx->hashTable, ctx->hashMask);
-                LAUNCHCHK(ctx);
             }
+            if ((rc = reserve()) != B2G_OK) return rc;
+            hipLaunchKernelGGL(k_toi_resume, dim3(1), dim3(64), 0, s, st, ctx->nContacts);
+            LAUNCHCHK(ctx);
+            if (ctx->subStepping) {
+                ctx->stepComplete = false;
+                break;
+            }
+            batch = 1;
+            continue;
         }
-        if (ctx->subStepping) {
+        if (h.subStop) {
             ctx->stepComplete = false;
             break;
         }
+        if ((rc = reserve()) != B2G_OK) return rc;
+        batch = std::min(2 * batch, 16);
     }
     profEnd(ctx, PH_TOI, pe);
     ctx->stats.toiEvents = ctx->hToi->events;
+    ctx->stats.newPairs += ctx->hToi->added;
     return B2G_OK;
 }
 

@@ -1016,6 +1016,14 @@ int32_t b2World::CollideWithListener(const void* params) {
     }
     if (!ids.empty())
         rc = b2g_override_contacts(m_ctx, ids.data(), enabled.data(), fr.data(), re.data(), (int)ids.size());
+    // touching set as the listener has seen it; what the TOI phase changes
+    // after this is reported once the step is over
+    m_touchingKeys.clear();
+    for (int i = 0; i < n; ++i) {
+        const int flags = meta[4 * i + 2];
+        if ((flags & 4) == 0 && (flags & 1) != 0) m_touchingKeys.push_back(pairKeyOf(meta[4 * i], meta[4 * i + 1]));
+    }
+    std::sort(m_touchingKeys.begin(), m_touchingKeys.end());
     return rc;
 }
 
@@ -1032,8 +1040,12 @@ void b2World::ReplayEvents(bool collideEventsDone) {
     std::sort(now.begin(), now.end());
     std::unordered_set<uint64_t> before(m_touchingKeys.begin(), m_touchingKeys.end());
     std::unordered_set<uint64_t> after(now.begin(), now.end());
+    // Without the split step every touching change of the step is reported
+    // here; with it, only those of the TOI phase (the narrowphase's were
+    // reported inside the step).
+    (void)collideEventsDone;
     for (uint64_t k : m_touchingKeys) {
-        if (collideEventsDone || after.count(k)) continue;
+        if (after.count(k)) continue;
         b2Contact ended;
         ended.m_fixtureA = m_fixtures[(int)(k >> 32)];
         ended.m_fixtureB = m_fixtures[(int)(k & 0xFFFFFFFFu)];
@@ -1043,7 +1055,7 @@ void b2World::ReplayEvents(bool collideEventsDone) {
     for (b2Contact* c = list; c; c = c->GetNext()) {
         if (!c->IsTouching()) continue;
         const uint64_t k = pairKeyOf(c->m_fixtureA->m_index, c->m_fixtureB->m_index);
-        if (!collideEventsDone && !before.count(k)) m_contactListener->BeginContact(c);
+        if (!before.count(k)) m_contactListener->BeginContact(c);
         if (!solved[nc - 1 - (int)(c - &m_contacts[0])]) continue;  // not solved this step: no PostSolve
         b2ContactImpulse imp{};
         imp.count = c->m_manifold.pointCount;

@@ -93,3 +93,32 @@ def test_split_step_state_errors(gpu):
     p[0]["mode"] = 1
     assert lib.b2g_step_collide(c, E.ptr(p)) != 0          # coloured solver only
     w.close()
+
+
+@pytest.mark.parametrize("ccd", [False, True])
+def test_begin_end_balanced_with_ccd(gpu, ccd):
+    """With continuous collision the TOI events change touching states after
+    the narrowphase; those changes reach the listener after the step. Per
+    fixture pair, BeginContact and EndContact still alternate starting with
+    Begin, and the pairs left open are exactly the touching contacts."""
+    g = gpu.world("bullets", 48, 2, sleep=False, ccd=ccd)
+    g.e.fn("world_set_recorder")(g.h, -1, F(0.0), F(0.0))
+    toi = 0
+    for _ in range(160):
+        g.step(1)
+        toi += int(g.stats()["toiEvents"])
+    ev = _events(g)
+    open_pairs = {}
+    for kind, a, b in ev[np.isin(ev[:, 1], (1, 2))][:, 1:4].tolist():
+        key = (min(a, b), max(a, b))
+        if kind == 1:
+            assert not open_pairs.get(key, False), f"Begin twice for {key}"
+            open_pairs[key] = True
+        else:
+            assert open_pairs.get(key, False), f"End without Begin for {key}"
+            open_pairs[key] = False
+    meta, _ = g.contacts()
+    touching = {(min(a, b), max(a, b)) for a, b, f in meta[:, :3].tolist() if f & 1}
+    assert {k for k, v in open_pairs.items() if v} == touching
+    assert (toi > 20) == ccd
+    g.close()
