@@ -2660,8 +2660,9 @@ __global__ void k_toi_reset_contacts(WorldView w) {
     w.ctoi[c] = 1.0f;
 }
 
-__global__ void k_toi_begin_event(ToiState* st) {
-    if (threadIdx.x != 0 || st->halt) return;
+// Per-event fields cleared for the next event (by the last kernel of an
+// event, and when a pass starts or resumes).
+__device__ inline void toiResetEvent(ToiState* st) {
     st->minKey = ~0ull;
     st->mismatch = 0;
     st->active = 0;
@@ -2682,15 +2683,6 @@ __global__ void k_toi_classify(WorldView w, int* pending, ToiState* st) {
     if (p && w.pos0[w.fbody[w.cfA[c]]].w != w.pos0[w.fbody[w.cfB[c]]].w) atomicOr(&st->mismatch, 1);
 }
 
-// With every pending pair's sweeps already aligned, each TOI depends on its
-// own two bodies only: one lane per contact.
-__global__ void k_toi_compute(WorldView w, const int* pending, const ToiState* st) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= st->live || st->halt || st->mismatch || !pending[c]) return;
-    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
-    w.ctoi[c] = toiFromSweeps(w, c, sweepOf(w, bA), sweepOf(w, bB));
-    w.cflags[c] |= CF_TOI;
-}
 
 // Otherwise the alignments run in list order on one lane, since an advance
 // moves a body for the contacts after it, and each contact keeps the two
@@ -2705,10 +2697,18 @@ __global__ void k_toi_align_serial(WorldView w, const int* pending, const ToiSta
     }
 }
 
-__global__ void k_toi_compute_aligned(WorldView w, const int* pending, const ToiState* st, const Sweep* seen) {
+// One lane per pending contact: from the current sweeps when every pending
+// pair's sweeps already start together (each TOI then depends on its own two
+// bodies only), else from the sweeps the list-order alignment recorded.
+__global__ void k_toi_compute(WorldView w, const int* pending, const ToiState* st, const Sweep* seen) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= st->live || st->halt || !st->mismatch || !pending[c]) return;
-    w.ctoi[c] = toiFromSweeps(w, c, seen[2 * c], seen[2 * c + 1]);
+    if (c >= st->live || st->halt || !pending[c]) return;
+    if (st->mismatch) {
+        w.ctoi[c] = toiFromSweeps(w, c, seen[2 * c], seen[2 * c + 1]);
+    } else {
+        const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
+        w.ctoi[c] = toiFromSweeps(w, c, sweepOf(w, bA), sweepOf(w, bB));
+    }
     w.cflags[c] |= CF_TOI;
 }
 
@@ -2782,7 +2782,10 @@ __global__ void k_toi_event(WorldView w, ToiState* st) {
     st->events += 1;
 }
 
-// Contact edges of the TOI pair's dynamic bodies: (0 for A / 1 for B) << 32 | list position.
+// Contact edges of the TOI pair's dynamic bodies, keyed for list order:
+// (0 for A / 1 for B) << 62 | list position << 11 | gather slot.
+constexpr int kGatherSlotBits = 11;
+static_assert((1 << kGatherSlotBits) >= kToiGatherCap, "gather slot field");
 __global__ void k_toi_gather(WorldView w, ToiState* st, unsigned long long* gather) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= st->live || st->halt || !st->active) return;
@@ -2792,7 +2795,8 @@ __global__ void k_toi_gather(WorldView w, ToiState* st, unsigned long long* gath
         const int body = k == 0 ? st->bodyA : st->bodyB;
         if (w.btype[body] != BT_DYNAMIC || (x != body && y != body)) continue;
         const int slot = atomicAdd(&st->nGather, 1);
-        if (slot < kToiGatherCap) gather[slot] = ((unsigned long long)k << 32) | pos;
+        if (slot < kToiGatherCap)
+            gather[slot] = ((unsigned long long)k << 62) | ((unsigned long long)pos << kGatherSlotBits) | (unsigned)slot;
     }
 }
 
@@ -2821,10 +2825,92 @@ __device__ inline void toiVisitEdge(const WorldView& w, ToiState* st, int body, 
     st->bodies[st->nBodies++] = other;
 }
 
+// The narrowphase of every gathered edge ahead of the island search, one
+// lane each. What b2Contact::Update computes for an edge depends only on the
+// two bodies' transforms at the visit, and those are known in advance: the
+// TOI pair as advanced, any other body as advanced to the event's time
+// (kept if its contact touches, else restored, so a later edge of the same
+// body sees the same advance again); a contact met twice (between the two
+// TOI bodies) is not touching at either visit or is skipped at the second.
+__global__ void k_toi_prep(WorldView w, ToiState* st, const unsigned long long* gather, b2g_manifold* man,
+                           int* touching) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (st->halt || !st->active || i >= st->nGather || i >= kToiGatherCap) return;
+    const unsigned long long key = gather[i];
+    const int k = (int)(key >> 62);
+    const int pos = (int)((key >> kGatherSlotBits) & ((1ull << 51) - 1));
+    const int c = st->live - 1 - pos;
+    const int body = k == 0 ? st->bodyA : st->bodyB;
+    const int fA = w.cfA[c], fB = w.cfB[c];
+    const int x = w.fbody[fA], y = w.fbody[fB];
+    const int other = x == body ? y : x;
+    Xf xfOther;
+    if (w.bflags[other] & BF_ISLAND) {
+        xfOther = xfOf(w.xf[other]);
+    } else {
+        Sweep sw = sweepOf(w, other);
+        sweepAdvance(sw, st->alpha);
+        xfOther.q = rotOf(sw.a0);
+        xfOther.p = sw.c0 - mul(xfOther.q, sw.localCenter);
+    }
+    const Xf xfBody = xfOf(w.xf[body]);
+    const Xf xfX = x == body ? xfBody : xfOther;
+    const Xf xfY = y == body ? xfBody : xfOther;
+    const b2g_manifold old = w.cman[c];
+    b2g_manifold m;
+    zeroManifold(m);
+    collideShapes(m, w.shape[fA], xfX, w.shape[fB], xfY);
+    for (int p = 0; p < m.pointCount; ++p) {
+        m.points[p].normalImpulse = 0.0f;
+        m.points[p].tangentImpulse = 0.0f;
+        for (int j = 0; j < old.pointCount; ++j) {
+            if (old.points[j].id == m.points[p].id) {
+                m.points[p].normalImpulse = old.points[j].normalImpulse;
+                m.points[p].tangentImpulse = old.points[j].tangentImpulse;
+                break;
+            }
+        }
+    }
+    const int slot = (int)(key & ((1u << kGatherSlotBits) - 1));
+    man[slot] = m;
+    touching[slot] = m.pointCount > 0 ? 1 : 0;
+}
+
+// One edge of the island search with its narrowphase done ahead: commits
+// what toiVisitEdge computes.
+__device__ inline void toiCommitEdge(const WorldView& w, ToiState* st, int body, int c, const b2g_manifold& m,
+                                     bool touching) {
+    if (w.cflags[c] & CF_ISLAND) return;
+    const int x = w.fbody[w.cfA[c]], y = w.fbody[w.cfB[c]];
+    const int other = x == body ? y : x;
+    if (w.btype[other] == BT_DYNAMIC && !(w.bflags[body] & BF_BULLET) && !(w.bflags[other] & BF_BULLET)) return;
+    if (sensorOf(w, w.cfA[c]) || sensorOf(w, w.cfB[c])) return;
+    if (touching && !(w.bflags[other] & BF_ISLAND)) advanceBodyTo(w, other, st->alpha);
+    const int flags = w.cflags[c] | CF_ENABLED;
+    const bool wasTouching = (flags & CF_TOUCHING) != 0;
+    if (touching != wasTouching) {
+        wakeBody(w, x);
+        wakeBody(w, y);
+    }
+    w.cman[c] = m;
+    if (!touching) {
+        w.cflags[c] = flags & ~CF_TOUCHING;
+        syncTransformOne(w, other);  // the restored sweep's transform
+        return;
+    }
+    w.cflags[c] = flags | CF_TOUCHING | CF_ISLAND;
+    st->contacts[st->nContacts++] = c;
+    if (w.bflags[other] & BF_ISLAND) return;
+    w.bflags[other] |= BF_ISLAND;
+    if (w.btype[other] != BT_STATIC) wakeBody(w, other);
+    st->bodies[st->nBodies++] = other;
+}
+
 // Island search over the TOI pair's edges in list order, then
 // b2Island::SolveTOI on lane 0 (Gauss-Seidel order is the island's).
 __global__ __launch_bounds__(256) void k_toi_island(WorldView w, ConView cv, ToiState* st,
-                                                    const unsigned long long* gather, float dt, int velIters) {
+                                                    const unsigned long long* gather, const b2g_manifold* man,
+                                                    const int* touching, float dt, int velIters) {
     __shared__ unsigned long long keys[kToiGatherCap];
     if (st->halt || !st->active) return;
     const int n = st->nGather;
@@ -2857,9 +2943,11 @@ __global__ __launch_bounds__(256) void k_toi_island(WorldView w, ConView cv, Toi
     if (sorted) {
         for (int i = 0; i < n; ++i) {
             if (st->nBodies == 2 * kMaxTOIContacts || st->nContacts == kMaxTOIContacts) break;
-            const int k = (int)(keys[i] >> 32);
-            const int c = nc - 1 - (int)(keys[i] & 0xFFFFFFFFu);
-            toiVisitEdge(w, st, k == 0 ? bA : bB, c);
+            const unsigned long long key = keys[i];
+            const int k = (int)(key >> 62);
+            const int c = nc - 1 - (int)((key >> kGatherSlotBits) & ((1ull << 51) - 1));
+            const int slot = (int)(key & ((1u << kGatherSlotBits) - 1));
+            toiCommitEdge(w, st, k == 0 ? bA : bB, c, man[slot], touching[slot] != 0);
         }
     } else {
         // more edges than the sort holds: walk the list once per body
@@ -2896,30 +2984,24 @@ __global__ __launch_bounds__(256) void k_toi_island(WorldView w, ConView cv, Toi
     }
 }
 
-// Fixtures of the island's dynamic bodies to their swept boxes.
-__global__ void k_toi_sync_fixtures(WorldView w, ToiState* st) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= w.nFixtures || st->halt || !st->active) return;
-    const int b = w.fbody[f];
-    if (!(w.bflags[b] & BF_ISLAND) || w.btype[b] != BT_DYNAMIC) return;
-    if (!syncFixtureOne(w, f, b)) return;
-    const int slot = atomicAdd(&st->nMoved, 1);
-    if (slot < kToiMovedCap) st->movedList[slot] = f;
-}
-
-// Cached TOIs and island marks of every contact on a displaced body dropped.
-__global__ void k_toi_invalidate(WorldView w, const ToiState* st) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= st->live || st->halt || !st->active) return;
-    const int x = w.fbody[w.cfA[c]], y = w.fbody[w.cfB[c]];
-    const bool hit = ((w.bflags[x] & BF_ISLAND) && w.btype[x] == BT_DYNAMIC) ||
-                     ((w.bflags[y] & BF_ISLAND) && w.btype[y] == BT_DYNAMIC);
-    if (hit) w.cflags[c] &= ~(CF_TOI | CF_ISLAND);
-}
-
-__global__ void k_toi_end_event(WorldView w, ToiState* st) {
+// Fixtures of the island's dynamic bodies to their swept boxes, and the
+// cached TOIs and island marks of every contact on such a body dropped.
+__global__ void k_toi_displaced(WorldView w, ToiState* st) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (st->halt || !st->active) return;
-    for (int i = threadIdx.x; i < st->nBodies; i += blockDim.x) w.bflags[st->bodies[i]] &= ~BF_ISLAND;
+    if (i < w.nFixtures) {
+        const int b = w.fbody[i];
+        if ((w.bflags[b] & BF_ISLAND) && w.btype[b] == BT_DYNAMIC && syncFixtureOne(w, i, b)) {
+            const int slot = atomicAdd(&st->nMoved, 1);
+            if (slot < kToiMovedCap) st->movedList[slot] = i;
+        }
+    }
+    if (i < st->live) {
+        const int x = w.fbody[w.cfA[i]], y = w.fbody[w.cfB[i]];
+        const bool hit = ((w.bflags[x] & BF_ISLAND) && w.btype[x] == BT_DYNAMIC) ||
+                         ((w.bflags[y] & BF_ISLAND) && w.btype[y] == BT_DYNAMIC);
+        if (hit) w.cflags[i] &= ~(CF_TOI | CF_ISLAND);
+    }
 }
 
 // b2BroadPhase::UpdatePairs for the event's moved proxies only: every proxy
@@ -2950,7 +3032,13 @@ __global__ void k_toi_pairs(WorldView w, ToiState* st, const unsigned long long*
 __global__ __launch_bounds__(256) void k_toi_append(WorldView w, ToiState* st, unsigned long long* table,
                                                    uint32_t hmask, int capContacts, int subStepping) {
     __shared__ unsigned long long keys[kToiNewCap];
-    if (st->halt || !st->active) return;
+    if (st->halt) return;
+    if (!st->active) {
+        __syncthreads();
+        if (threadIdx.x == 0) toiResetEvent(st);
+        return;
+    }
+    for (int i = threadIdx.x; i < st->nBodies; i += blockDim.x) w.bflags[st->bodies[i]] &= ~BF_ISLAND;
     const int nm = st->nMoved, nn = st->nNew, base = st->live;
     if (nm > kToiMovedCap || nn > kToiNewCap || base + nn > capContacts) {
         __syncthreads();
@@ -2996,6 +3084,7 @@ __global__ __launch_bounds__(256) void k_toi_append(WorldView w, ToiState* st, u
     if (threadIdx.x == 0) {
         st->live = base + nn;
         st->added += nn;
+        toiResetEvent(st);
         if (subStepping) {
             st->subStop = 1;
             st->halt = 1;
@@ -3148,6 +3237,8 @@ struct b2g_context {
     ToiState* toiState = nullptr;   // device
     ToiState* hToi = nullptr;       // pinned host mirror
     unsigned long long* toiGather = nullptr;
+    b2g_manifold* toiMan = nullptr; // narrowphase of the gathered edges
+    int* toiTouch = nullptr;
     Sweep* toiSeen = nullptr;       // per contact: the aligned sweeps of a list-order scan
     int toiSeenCap = 0;
     ConView toiCv{};                // constraints of one TOI island
@@ -4568,6 +4659,8 @@ int ensureToi(b2g_context* ctx) {
     HIPCHK(ctx, hipHostMalloc((void**)&ctx->hToi, sizeof(ToiState), hipHostMallocDefault));
     memset(ctx->hToi, 0, sizeof(ToiState));
     HIPCHK(ctx, hipMalloc((void**)&ctx->toiGather, sizeof(unsigned long long) * kToiGatherCap));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->toiMan, sizeof(b2g_manifold) * kToiGatherCap));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->toiTouch, sizeof(int) * kToiGatherCap));
     ConView& c = ctx->toiCv;
     const size_t n = kMaxTOIContacts;
     HIPCHK(ctx, hipMalloc((void**)&c.contact, sizeof(int) * n));
@@ -4595,6 +4688,8 @@ void freeToi(b2g_context* ctx) {
     if (ctx->hToi) (void)hipHostFree(ctx->hToi);
     ctx->hToi = nullptr;
     freeArray(ctx->toiGather);
+    freeArray(ctx->toiMan);
+    freeArray(ctx->toiTouch);
     freeArray(ctx->toiSeen);
     ctx->toiSeenCap = 0;
     ConView& c = ctx->toiCv;
@@ -4611,7 +4706,7 @@ __global__ void k_toi_start(ToiState* st, int live) {
     st->added = 0;
     st->halt = 0;
     st->subStop = 0;
-    st->fallback = 0;
+    toiResetEvent(st);
 }
 
 // After the host ran the grid broad-phase for an event: resume the batch.
@@ -4619,7 +4714,7 @@ __global__ void k_toi_resume(ToiState* st, int live) {
     if (threadIdx.x != 0) return;
     st->live = live;
     st->halt = 0;
-    st->fallback = 0;
+    toiResetEvent(st);
 }
 
 // b2World::SolveTOI, REF/Dynamics/b2World.cpp:579-897. The events run on the
@@ -4668,21 +4763,17 @@ int solveTOI(b2g_context* ctx, const b2g_step_params* p) {
         const WorldView w = ctx->view();
         const int cap = ctx->capContacts;
         for (int e = 0; e < batch; ++e) {
-            hipLaunchKernelGGL(k_toi_begin_event, dim3(1), dim3(64), 0, s, st);
             hipLaunchKernelGGL(k_toi_classify, dim3(blocks(cap)), dim3(256), 0, s, w, ctx->keep, st);
-            hipLaunchKernelGGL(k_toi_compute, dim3(blocks(cap, 64)), dim3(64), 0, s, w, ctx->keep, st);
             hipLaunchKernelGGL(k_toi_align_serial, dim3(1), dim3(64), 0, s, w, ctx->keep, st, ctx->toiSeen);
-            hipLaunchKernelGGL(k_toi_compute_aligned, dim3(blocks(cap, 64)), dim3(64), 0, s, w, ctx->keep, st,
-                               ctx->toiSeen);
+            hipLaunchKernelGGL(k_toi_compute, dim3(blocks(cap, 64)), dim3(64), 0, s, w, ctx->keep, st, ctx->toiSeen);
             hipLaunchKernelGGL(k_toi_min, dim3(blocks(cap)), dim3(256), 0, s, w, st);
             hipLaunchKernelGGL(k_toi_event, dim3(1), dim3(64), 0, s, w, st);
             hipLaunchKernelGGL(k_toi_gather, dim3(blocks(cap)), dim3(256), 0, s, w, st, ctx->toiGather);
-            hipLaunchKernelGGL(k_toi_island, dim3(1), dim3(256), 0, s, w, ctx->toiCv, st, ctx->toiGather, p->dt,
-                               p->velocityIterations);
-            if (ctx->nFixtures > 0)
-                hipLaunchKernelGGL(k_toi_sync_fixtures, dim3(blocks(ctx->nFixtures)), dim3(256), 0, s, w, st);
-            hipLaunchKernelGGL(k_toi_invalidate, dim3(blocks(cap)), dim3(256), 0, s, w, st);
-            hipLaunchKernelGGL(k_toi_end_event, dim3(1), dim3(64), 0, s, w, st);
+            hipLaunchKernelGGL(k_toi_prep, dim3(blocks(kToiGatherCap, 64)), dim3(64), 0, s, w, st, ctx->toiGather,
+                               ctx->toiMan, ctx->toiTouch);
+            hipLaunchKernelGGL(k_toi_island, dim3(1), dim3(256), 0, s, w, ctx->toiCv, st, ctx->toiGather, ctx->toiMan,
+                               ctx->toiTouch, p->dt, p->velocityIterations);
+            hipLaunchKernelGGL(k_toi_displaced, dim3(blocks(std::max(cap, ctx->nFixtures))), dim3(256), 0, s, w, st);
             if (ctx->nFixtures > 0)
                 hipLaunchKernelGGL(k_toi_pairs, dim3(blocks(ctx->nFixtures)), dim3(256), 0, s, w, st, ctx->hashTable,
                                    ctx->hashMask);
