@@ -44,9 +44,15 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_solve_velocity launch on the default workload (100k pile,
 # seed 1, steps 1-1000, 8/3 iterations), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE averaged over all launches and scaled by the counters' readings
-# for a plain 256 MiB copy (x2.000 and x1.000); the summary with the commands
-# is under profiles/.
-DEFAULT_TRAFFIC_BYTES = 4.929e6
+# for a plain 256 MiB copy (x2.000 and x1.000), keyed by the step schedule
+# the counters ran with (the fusion of trailing colours changes which
+# colours the kernel solves); the summary with the commands is under
+# profiles/. Another schedule reports traffic as null.
+DEFAULT_TRAFFIC_BYTES = {(0, 0, 256, 256, 64, 2048): 7.779e6}
+# The pile's step schedule as the autotuner picks it on this workload (step
+# 500 of a warm-up copy): fixed so the HBM traffic above applies; the batched
+# and stress worlds are tuned on every run.
+PILE_SCHEDULE = "0,0,256,256,64,2048"
 DT = 1.0 / 60.0
 # b2g_schedule: island labelling method, collide order, workgroup sizes of
 # the solver, narrowphase and broad-phase query kernels
@@ -533,6 +539,9 @@ def main():
     ap.add_argument("--stress-steps", type=int, default=600)
     ap.add_argument("--stress-seed", type=int, default=1)
     ap.add_argument("--no-ccd", action="store_true", help="skip the continuous-collision run")
+    ap.add_argument("--schedule", type=str, default=PILE_SCHEDULE,
+                    help="pile step schedule (islands,collideOrder,solverBlock,collideBlock,queryBlock,tailColours), "
+                         "or 'auto' to autotune it on the warm-up copy")
     ap.add_argument("--ccd-bodies", type=int, default=10000)
     ap.add_argument("--ccd-steps", type=int, default=500)
     ap.add_argument("--ccd-cpu-seconds", type=float, default=60.0)
@@ -555,8 +564,7 @@ def main():
     if args.profile_steps is None:
         args.profile_steps = args.steps
     default_workload = (args.bodies, args.seed, args.steps, args.vel, args.pos) == (100000, 1, 1000, 8, 3)
-    if args.traffic_bytes is None and default_workload:
-        args.traffic_bytes = DEFAULT_TRAFFIC_BYTES
+    traffic_for_schedule = args.traffic_bytes is None and default_workload
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
                                    "(batched worlds shard across GPUs by world; a single world by slab, "
@@ -569,15 +577,22 @@ def main():
     # changes results, only speed)
     global _schedule
     pile_schedule = None
-    if args.warmup > 0 or args.tune_rounds > 0:
+    if args.schedule and args.schedule != "auto":
+        vals = [int(v) for v in args.schedule.split(",")]
+        pile_schedule = np.zeros(1, SCHEDULE)
+        for name, v in zip(SCHEDULE.names, vals):
+            pile_schedule[0][name] = v
+    if args.warmup > 0 or (args.tune_rounds > 0 and pile_schedule is None):
         hw = make_world(lib, args.bodies, args.seed)
-        if args.tune_rounds > 0:
+        if args.tune_rounds > 0 and pile_schedule is None:
             pile_schedule = autotune(lib, hw, args, max(args.warmup, args.tune_at))
         else:
             step(lib, hw, args.warmup, args.vel, args.pos)
         lib.b2pp_world_sync(hw)
         lib.b2pp_world_destroy(hw)
     _schedule = pile_schedule
+    if traffic_for_schedule and pile_schedule is not None:
+        args.traffic_bytes = DEFAULT_TRAFFIC_BYTES.get(tuple(int(pile_schedule[0][k]) for k in SCHEDULE.names))
 
     h = make_world(lib, args.bodies, args.seed)
     lib.b2pp_world_sync(h)

@@ -6134,7 +6134,7 @@ int b2g_abi_sizes(int32_t* out, int cap) {
         (int32_t)sizeof(b2g_fixture_def), (int32_t)sizeof(b2g_world_def),    (int32_t)sizeof(b2g_step_params),
         (int32_t)sizeof(b2g_step_stats),  (int32_t)sizeof(b2g_phase_profile), (int32_t)sizeof(b2g_mlp_params),
         (int32_t)sizeof(b2g_aabb),        (int32_t)sizeof(b2g_transform),    (int32_t)sizeof(b2g_revolute_def),
-        (int32_t)sizeof(b2g_schedule),
+        (int32_t)sizeof(b2g_schedule),    (int32_t)sizeof(b2g_sweep),
     };
     const int n = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (!out) return n;

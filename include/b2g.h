@@ -521,9 +521,9 @@ int b2g_get_graph_counts(b2g_context* ctx, int64_t* launches, int64_t* captures)
 /* Sizes in bytes of the ABI structs, in this order: b2g_shape,
  * b2g_manifold, b2g_body_state, b2g_fixture_def, b2g_world_def,
  * b2g_step_params, b2g_step_stats, b2g_phase_profile, b2g_mlp_params,
- * b2g_aabb, b2g_transform, b2g_revolute_def, b2g_schedule. Returns the count (out may
- * be null). Needs no
- * device; lets bindings check their struct layouts. */
+ * b2g_aabb, b2g_transform, b2g_revolute_def, b2g_schedule, b2g_sweep.
+ * Returns the count (out may be null). Needs no device; lets bindings check
+ * their struct layouts. */
 int b2g_abi_sizes(int32_t* out, int cap);
 
 /* Checks the colouring of the last coloured step: *conflicts counts dynamic
