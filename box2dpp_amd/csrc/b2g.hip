@@ -1734,6 +1734,7 @@ __device__ inline float4 shapeAABB(const b2g_shape& s, const Xf& xf) {
         lo = vmin(lo, v);
         hi = vmax(hi, v);
     }
+    if (s.type == B2G_EDGE && (s.flags & 4)) return make_float4(lo.x, lo.y, hi.x, hi.y);  // chain child
     const V2 r = mk(s.radius, s.radius);
     const V2 l = lo - r, u = hi + r;
     return make_float4(l.x, l.y, u.x, u.y);
@@ -5166,7 +5167,9 @@ int b2g_add_fixtures(b2g_context* ctx, const b2g_fixture_def* defs, int count, i
                     lo = vmin(lo, v);
                     hi = vmax(hi, v);
                 }
-                const V2 r = mk(s.radius, s.radius);
+                // chain children have no skin in their box (b2ChainShape::ComputeAABB)
+                const float rad = (s.type == B2G_EDGE && (s.flags & 4)) ? 0.0f : s.radius;
+                const V2 r = mk(rad, rad);
                 const V2 l = lo - r, u = hi + r;
                 lx = l.x; ly = l.y; ux = u.x; uy = u.y;
             }

@@ -80,8 +80,20 @@ int loadScene(Runner* r, const b2scene::BodyDef* bodies, int nb, const b2scene::
             b2PolygonShape poly;
             b2CircleShape circle;
             b2EdgeShape edge;
+            b2ChainShape chain;
             const b2Shape* shape = nullptr;
-            if (f.shape == b2scene::kPolygon) {
+            if (f.shape == b2scene::kChain) {
+                b2Vec2 p[8];
+                for (int j = 0; j < f.n && j < 8; ++j) p[j] = b2Vec2(f.pts[2 * j], f.pts[2 * j + 1]);
+                if (f.polyMode == 1) {
+                    chain.CreateLoop(p, f.n);
+                } else {
+                    chain.CreateChain(p, f.n);
+                    if (f.has0) chain.SetPrevVertex(b2Vec2(f.e[0], f.e[1]));
+                    if (f.has3) chain.SetNextVertex(b2Vec2(f.e[6], f.e[7]));
+                }
+                shape = &chain;
+            } else if (f.shape == b2scene::kPolygon) {
                 if (f.polyMode == b2scene::kPolyBox) poly.SetAsBox(f.pts[0], f.pts[1]);
                 else if (f.polyMode == b2scene::kPolyBoxRotated)
                     poly.SetAsBox(f.pts[0], f.pts[1], b2Vec2(f.pts[2], f.pts[3]), f.pts[4]);
@@ -112,7 +124,9 @@ int loadScene(Runner* r, const b2scene::BodyDef* bodies, int nb, const b2scene::
             fd.filter.categoryBits = (uint16_t)f.categoryBits;
             fd.filter.maskBits = (uint16_t)f.maskBits;
             fd.filter.groupIndex = (int16_t)f.groupIndex;
-            r->fixtures.push_back(body->CreateFixture(&fd));
+            b2Fixture* fixture = body->CreateFixture(&fd);
+            // one entry per device fixture (a chain has one per child edge)
+            for (int c = 0; c < shape->GetChildCount(); ++c) r->fixtures.push_back(fixture);
         }
     }
     return flush ? r->world.Flush() : 0;

@@ -189,6 +189,62 @@ void b2EdgeShape::ComputeMass(b2MassData* md, float density) const {
     md->I = 0.0f;
 }
 
+// b2ChainShape, REF/Collision/Shapes/b2ChainShape.cpp:39-196
+void b2ChainShape::CreateLoop(const b2Vec2* vertices, int32_t count) {
+    m_vertices.assign(vertices, vertices + count);
+    m_vertices.push_back(vertices[0]);
+    m_count = count + 1;
+    m_prevVertex = m_vertices[m_count - 2];
+    m_nextVertex = m_vertices[1];
+    m_hasPrevVertex = true;
+    m_hasNextVertex = true;
+}
+
+void b2ChainShape::CreateChain(const b2Vec2* vertices, int32_t count) {
+    m_vertices.assign(vertices, vertices + count);
+    m_count = count;
+    m_hasPrevVertex = false;
+    m_hasNextVertex = false;
+    m_prevVertex = b2Vec2(0.0f, 0.0f);
+    m_nextVertex = b2Vec2(0.0f, 0.0f);
+}
+
+void b2ChainShape::GetChildEdge(b2EdgeShape* edge, int32_t index) const {
+    edge->m_radius = m_radius;
+    edge->m_vertex1 = m_vertices[index];
+    edge->m_vertex2 = m_vertices[index + 1];
+    if (index > 0) {
+        edge->m_vertex0 = m_vertices[index - 1];
+        edge->m_hasVertex0 = true;
+    } else {
+        edge->m_vertex0 = m_prevVertex;
+        edge->m_hasVertex0 = m_hasPrevVertex;
+    }
+    if (index < m_count - 2) {
+        edge->m_vertex3 = m_vertices[index + 2];
+        edge->m_hasVertex3 = true;
+    } else {
+        edge->m_vertex3 = m_nextVertex;
+        edge->m_hasVertex3 = m_hasNextVertex;
+    }
+}
+
+void b2ChainShape::ComputeAABB(b2AABB* aabb, const b2Transform& xf, int32_t childIndex) const {
+    const int32_t i1 = childIndex;
+    const int32_t i2 = childIndex + 1 == m_count ? 0 : childIndex + 1;
+    const b2Vec2 v1 = b2Mul(xf, m_vertices[i1]);
+    const b2Vec2 v2 = b2Mul(xf, m_vertices[i2]);
+    aabb->lowerBound = b2Min(v1, v2);
+    aabb->upperBound = b2Max(v1, v2);
+}
+
+void b2ChainShape::ComputeMass(b2MassData* md, float density) const {
+    B2_NOT_USED(density);
+    md->mass = 0.0f;
+    md->center = b2Vec2(0.0f, 0.0f);
+    md->I = 0.0f;
+}
+
 // b2WorldManifold::Initialize, REF/Collision/b2Collision.cpp:22-86
 void b2WorldManifold::Initialize(const b2Manifold* m, const b2Transform& xfA, float radiusA, const b2Transform& xfB,
                                  float radiusB) {

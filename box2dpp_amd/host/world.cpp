@@ -20,7 +20,15 @@ namespace box2d {
 
 namespace {
 
-b2g_shape toDeviceShape(const b2Shape* s) {
+b2g_shape toDeviceShape(const b2Shape* s, int child) {
+    if (s->GetType() == b2Shape::e_chain) {
+        // a chain fixture is one device fixture per child edge
+        b2EdgeShape e;
+        static_cast<const b2ChainShape*>(s)->GetChildEdge(&e, child);
+        b2g_shape r = toDeviceShape(&e, 0);
+        r.flags |= 4;
+        return r;
+    }
     b2g_shape r{};
     r.type = (int32_t)s->GetType();
     r.radius = s->m_radius;
@@ -64,17 +72,21 @@ bool b2ContactFilter::ShouldCollide(b2Fixture* fixtureA, b2Fixture* fixtureB) {
 }
 
 // ------------------------------------------------------------ fixture
+// Device fixtures of this fixture: one, or one per child of a chain.
+void b2Fixture::MarkUpdated() {
+    for (int32_t c = 0; c < m_shape->GetChildCount(); ++c) m_body->m_world->m_updatedFixtures.push_back(m_index + c);
+}
 void b2Fixture::SetFilterData(const b2Filter& filter) {
     m_filter = filter;
-    m_body->m_world->m_updatedFixtures.push_back(m_index);
+    MarkUpdated();
 }
 void b2Fixture::SetFriction(float f) {
     m_friction = f;
-    m_body->m_world->m_updatedFixtures.push_back(m_index);
+    MarkUpdated();
 }
 void b2Fixture::SetRestitution(float r) {
     m_restitution = r;
-    m_body->m_world->m_updatedFixtures.push_back(m_index);
+    MarkUpdated();
 }
 
 // ------------------------------------------------------------ body
@@ -139,7 +151,7 @@ b2Fixture* b2Body::CreateFixture(const b2FixtureDef* def) {
     f->m_isSensor = def->isSensor;
     f->m_filter = def->filter;
     f->m_index = (int32_t)m_world->m_fixtures.size();
-    m_world->m_fixtures.push_back(f);
+    for (int32_t c = 0; c < f->m_shape->GetChildCount(); ++c) m_world->m_fixtures.push_back(f);
     f->m_next = m_fixtureList;
     m_fixtureList = f;
     ++m_fixtureCount;
@@ -395,7 +407,8 @@ void b2World::DestroyBody(b2Body* b) {
     b->m_invMass = 0.0f;
     b->m_invI = 0.0f;
     b->MarkDirty();
-    for (b2Fixture* f = b->m_fixtureList; f; f = f->m_next) m_removedFixtures.push_back(f->m_index);
+    for (b2Fixture* f = b->m_fixtureList; f; f = f->m_next)
+        for (int32_t c = 0; c < f->m_shape->GetChildCount(); ++c) m_removedFixtures.push_back(f->m_index + c);
     --m_bodyCount;
     m_contactsStale = true;
 }
@@ -707,7 +720,7 @@ int32_t b2World::Upload() {
             d.categoryBits = f->m_filter.categoryBits;
             d.maskBits = f->m_filter.maskBits;
             d.groupIndex = f->m_filter.groupIndex;
-            d.shape = toDeviceShape(f->m_shape.get());
+            d.shape = toDeviceShape(f->m_shape.get(), i - f->m_index);
             defs.push_back(d);
         }
         int first = 0;
@@ -905,6 +918,8 @@ b2Contact* b2World::GetContactList() {
             b2Contact& c = m_contacts[i];
             c.m_fixtureA = m_fixtures[meta[4 * src]];
             c.m_fixtureB = m_fixtures[meta[4 * src + 1]];
+            c.m_indexA = meta[4 * src] - c.m_fixtureA->m_index;
+            c.m_indexB = meta[4 * src + 1] - c.m_fixtureB->m_index;
             c.m_touching = (meta[4 * src + 2] & 1) != 0;
             c.m_enabled = (meta[4 * src + 2] & 2) != 0;
             c.m_friction = haveMat ? mat[2 * src] : sqrtf(c.m_fixtureA->m_friction * c.m_fixtureB->m_friction);
@@ -988,6 +1003,8 @@ int32_t b2World::CollideWithListener(const void* params) {
         b2Contact c;
         c.m_fixtureA = m_fixtures[meta[4 * i]];
         c.m_fixtureB = m_fixtures[meta[4 * i + 1]];
+        c.m_indexA = meta[4 * i] - c.m_fixtureA->m_index;
+        c.m_indexB = meta[4 * i + 1] - c.m_fixtureB->m_index;
         c.m_friction = mat[2 * i];
         c.m_restitution = mat[2 * i + 1];
         c.m_enabled = true;
@@ -1036,7 +1053,8 @@ void b2World::ReplayEvents(bool collideEventsDone) {
     if (nc > 0) b2g_get_contact_solved(m_ctx, solved.data(), nc);
     std::vector<uint64_t> now;
     for (b2Contact* c = list; c; c = c->GetNext())
-        if (c->IsTouching()) now.push_back(pairKeyOf(c->m_fixtureA->m_index, c->m_fixtureB->m_index));
+        if (c->IsTouching())
+            now.push_back(pairKeyOf(c->m_fixtureA->m_index + c->m_indexA, c->m_fixtureB->m_index + c->m_indexB));
     std::sort(now.begin(), now.end());
     std::unordered_set<uint64_t> before(m_touchingKeys.begin(), m_touchingKeys.end());
     std::unordered_set<uint64_t> after(now.begin(), now.end());
@@ -1049,12 +1067,14 @@ void b2World::ReplayEvents(bool collideEventsDone) {
         b2Contact ended;
         ended.m_fixtureA = m_fixtures[(int)(k >> 32)];
         ended.m_fixtureB = m_fixtures[(int)(k & 0xFFFFFFFFu)];
+        ended.m_indexA = (int)(k >> 32) - ended.m_fixtureA->m_index;
+        ended.m_indexB = (int)(k & 0xFFFFFFFFu) - ended.m_fixtureB->m_index;
         ended.m_touching = false;
         m_contactListener->EndContact(&ended);
     }
     for (b2Contact* c = list; c; c = c->GetNext()) {
         if (!c->IsTouching()) continue;
-        const uint64_t k = pairKeyOf(c->m_fixtureA->m_index, c->m_fixtureB->m_index);
+        const uint64_t k = pairKeyOf(c->m_fixtureA->m_index + c->m_indexA, c->m_fixtureB->m_index + c->m_indexB);
         if (!before.count(k)) m_contactListener->BeginContact(c);
         if (!solved[nc - 1 - (int)(c - &m_contacts[0])]) continue;  // not solved this step: no PostSolve
         b2ContactImpulse imp{};

@@ -33,7 +33,7 @@
 
 namespace b2scene {
 
-enum ShapeKind { kCircle = 0, kEdge = 1, kPolygon = 2 };
+enum ShapeKind { kCircle = 0, kEdge = 1, kPolygon = 2, kChain = 3 };
 enum PolyMode { kPolySet = 0, kPolyBox = 1, kPolyBoxRotated = 2 };
 
 struct BodyDef {
@@ -51,8 +51,8 @@ struct FixtureDef {
     int n;                 // polygon: number of input points
     float pts[16];         // polygon input points (x0,y0,x1,y1,...) or box params
     float radius, cx, cy;  // circle: radius and local centre
-    float e[8];            // edge: v0, v1, v2, v3 (x,y pairs)
-    int has0, has3;        // edge: ghost vertex flags
+    float e[8];            // edge: v0, v1, v2, v3 (x,y pairs); chain: prev vertex e[0..1], next e[6..7]
+    int has0, has3;        // edge: ghost vertex flags; chain: prev / next vertex set
     float density, friction, restitution;
     int sensor;
     int categoryBits, maskBits, groupIndex;
@@ -390,12 +390,70 @@ inline Scene bullets(int n, uint32_t seed) {
     return s;
 }
 
+// b2ChainShape of n <= 8 points: CreateLoop when `loop`, else CreateChain,
+// with SetPrevVertex / SetNextVertex when the ghosts are given.
+inline FixtureDef chainFixture(const float* xy, int n, bool loop, const float* prev = nullptr,
+                               const float* next = nullptr) {
+    FixtureDef f = defaultFixture(kChain);
+    f.n = n;
+    f.polyMode = loop ? 1 : 0;
+    for (int i = 0; i < 2 * n; ++i) f.pts[i] = xy[i];
+    if (prev) { f.e[0] = prev[0]; f.e[1] = prev[1]; f.has0 = 1; }
+    if (next) { f.e[6] = next[0]; f.e[7] = next[1]; f.has3 = 1; }
+    f.friction = 0.6f;
+    return f;
+}
+
+// Bodies on chain shapes: a bumpy open chain of 8 points as the floor (with
+// ghost vertices beyond both ends), a second open chain without ghosts as a
+// ramp, and a hexagonal loop around both; n circles and polygons dropped on
+// them from a grid, seeded.
+inline Scene terrain(int n, uint32_t seed) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    float floor[16];
+    for (int i = 0; i < 8; ++i) {
+        floor[2 * i] = -14.0f + 4.0f * (float)i;
+        floor[2 * i + 1] = (i & 1) ? 0.8f : 0.0f;
+    }
+    const float prev[2] = {-18.0f, 1.5f}, next[2] = {18.0f, 1.5f};
+    addFixture(s, chainFixture(floor, 8, false, prev, next));
+    const float ramp[8] = {6.0f, 6.0f, 9.0f, 5.0f, 12.0f, 4.5f, 15.0f, 4.6f};
+    addFixture(s, chainFixture(ramp, 4, false));
+    float hex[12];
+    for (int i = 0; i < 6; ++i) {
+        const double a = 3.14159265358979323846 / 3.0 * (double)i + 0.5235987755982988;
+        hex[2 * i] = (float)(24.0 * cos(a));
+        hex[2 * i + 1] = (float)(10.0 + 24.0 * sin(a));
+    }
+    addFixture(s, chainFixture(hex, 6, true));
+    Uniform U(seed);
+    for (int i = 0; i < n; ++i) {
+        BodyDef b = defaultBody(2, -12.0f + 1.5f * (float)(i % 17), 3.0f + 1.5f * (float)(i / 17));
+        b.vx = -2.0f + 4.0f * U();
+        b.w = -1.0f + 2.0f * U();
+        addBody(s, b);
+        FixtureDef f;
+        if (i & 1) {
+            f = defaultFixture(kCircle);
+            f.radius = 0.3f + 0.3f * U();
+        } else {
+            const float h = 0.3f + 0.3f * U();
+            f = boxFixture(h, 0.8f * h, 1.0f, 0.6f);
+        }
+        f.density = 1.0f; f.friction = 0.6f;
+        addFixture(s, f);
+    }
+    return s;
+}
+
 // Scene by name: "hello", "pyramid" (n = rows, 0 -> 20), "pile" (n bodies,
 // seed), "world" (one batched-scene world of n bodies, seed), "pendulums"
 // (n pendulums), "chains" (n chains of `seed` links), "stress" (pile of n
 // bodies, seed, on a mat of n / 50 chains of 32 links), "plank" (n circles
 // on a plank), "oneway" (n boxes through a one-sided platform), "bullets"
-// (n fast bodies, seed). Unknown names give an empty scene.
+// (n fast bodies, seed), "terrain" (n bodies on chain shapes, seed). Unknown
+// names give an empty scene.
 inline Scene byName(const char* kind, int n, uint32_t seed) {
     const std::string k(kind ? kind : "");
     if (k == "hello") return hello();
@@ -408,6 +466,7 @@ inline Scene byName(const char* kind, int n, uint32_t seed) {
     if (k == "plank") return plank(n > 0 ? n : 100);
     if (k == "oneway") return oneway(n > 0 ? n : 8);
     if (k == "bullets") return bullets(n > 0 ? n : 24, seed);
+    if (k == "terrain") return terrain(n > 0 ? n : 60, seed);
     return Scene();
 }
 

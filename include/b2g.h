@@ -63,7 +63,11 @@ typedef struct b2g_sweep {
  * circle : centroid = m_p, radius (b2CircleShape.h:55-62)
  * edge   : verts[0] = m_vertex1, verts[1] = m_vertex2, v0 = m_vertex0,
  *          v3 = m_vertex3, flags bit0 = m_hasVertex0, bit1 = m_hasVertex3
- *          (b2EdgeShape.h:50-62) */
+ *          (b2EdgeShape.h:50-62)
+ * chain  : one edge record per child (b2ChainShape::GetChildEdge,
+ *          REF/Collision/Shapes/b2ChainShape.cpp:112-143) with flags bit2
+ *          set: its AABB has no skin (b2ChainShape::ComputeAABB, :171-187).
+ *          A chain fixture is added as its children, in child order. */
 typedef struct b2g_shape {
     int32_t type;
     int32_t count;

@@ -2,6 +2,7 @@
 // Box2D/Collision/b2Collision.h and Shapes/*.h surface): AABBs, contact ids,
 // manifolds, world manifolds and the circle, edge and polygon shapes.
 #pragma once
+#include <vector>
 #include "b2Math.h"
 
 namespace box2d {
@@ -108,6 +109,42 @@ public:
     b2Vec2 m_vertex1, m_vertex2;
     b2Vec2 m_vertex0, m_vertex3;
     bool m_hasVertex0 = false, m_hasVertex3 = false;
+};
+
+// Box2D/Collision/Shapes/b2ChainShape.h:32-99: a sequence of segments,
+// collided two-sided as edges with their neighbours as ghost vertices. Each
+// segment is a child with its own broad-phase proxy.
+class b2ChainShape : public b2Shape {
+public:
+    b2ChainShape() { m_type = e_chain; m_radius = POLYGON_RADIUS; }
+    b2Shape* Clone() const override { return new b2ChainShape(*this); }
+    void Clear() {
+        m_vertices.clear();
+        m_count = 0;
+        m_hasPrevVertex = m_hasNextVertex = false;
+    }
+    // A closed loop: the first vertex repeated at the end, and the
+    // neighbours across the seam as ghosts.
+    void CreateLoop(const b2Vec2* vertices, int32_t count);
+    // An open chain with free ends (ghosts set with SetPrevVertex / SetNextVertex).
+    void CreateChain(const b2Vec2* vertices, int32_t count);
+    void SetPrevVertex(const b2Vec2& prevVertex) {
+        m_prevVertex = prevVertex;
+        m_hasPrevVertex = true;
+    }
+    void SetNextVertex(const b2Vec2& nextVertex) {
+        m_nextVertex = nextVertex;
+        m_hasNextVertex = true;
+    }
+    int32_t GetChildCount() const override { return m_count - 1; }
+    void GetChildEdge(b2EdgeShape* edge, int32_t index) const;
+    // Child AABB: the segment's box without the skin (b2ChainShape::ComputeAABB).
+    void ComputeAABB(b2AABB* aabb, const b2Transform& xf, int32_t childIndex) const override;
+    void ComputeMass(b2MassData* massData, float density) const override;
+    std::vector<b2Vec2> m_vertices;
+    int32_t m_count = 0;
+    b2Vec2 m_prevVertex, m_nextVertex;
+    bool m_hasPrevVertex = false, m_hasNextVertex = false;
 };
 
 class b2PolygonShape : public b2Shape {

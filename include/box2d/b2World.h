@@ -133,12 +133,14 @@ public:
     float GetRestitution() const { return m_restitution; }
     void SetRestitution(float r);
     void GetMassData(b2MassData* md) const { m_shape->ComputeMass(md, m_density); }
-    // Broad-phase proxy index of this fixture (its creation order in the world).
+    // Broad-phase proxy index of this fixture (its creation order in the
+    // world); a chain's children follow it in child order.
     int32_t GetProxyIndex() const { return m_index; }
 
 private:
     friend class b2Body;
     friend class b2World;
+    void MarkUpdated();
     std::unique_ptr<b2Shape> m_shape;
     b2Body* m_body = nullptr;
     b2Fixture* m_next = nullptr;
@@ -253,10 +255,10 @@ public:
     const b2Contact* GetNext() const { return m_next; }
     b2Fixture* GetFixtureA() { return m_fixtureA; }
     const b2Fixture* GetFixtureA() const { return m_fixtureA; }
-    int32_t GetChildIndexA() const { return 0; }
+    int32_t GetChildIndexA() const { return m_indexA; }
     b2Fixture* GetFixtureB() { return m_fixtureB; }
     const b2Fixture* GetFixtureB() const { return m_fixtureB; }
-    int32_t GetChildIndexB() const { return 0; }
+    int32_t GetChildIndexB() const { return m_indexB; }
     float GetFriction() const { return m_friction; }
     float GetRestitution() const { return m_restitution; }
     float GetTangentSpeed() const { return 0.0f; }
@@ -273,6 +275,7 @@ private:
     friend class b2World;
     b2Fixture* m_fixtureA = nullptr;
     b2Fixture* m_fixtureB = nullptr;
+    int32_t m_indexA = 0, m_indexB = 0;  // child of a chain fixture
     b2Manifold m_manifold;
     bool m_touching = false, m_enabled = true;
     float m_friction = 0.0f, m_restitution = 0.0f;

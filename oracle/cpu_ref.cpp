@@ -251,7 +251,7 @@ void edgeShape(b2g_shape* s, V2 v0, V2 v1, V2 v2, V2 v3, bool has0, bool has3) {
 }
 
 // ComputeAABB: polygon b2PolygonShape.cpp:335-352, circle b2CircleShape.cpp:85-94,
-// edge b2EdgeShape.cpp:118-131.
+// edge b2EdgeShape.cpp:118-131, chain child (flags bit2, no skin) b2ChainShape.cpp:171-187.
 AABB shapeAABB(const b2g_shape* s, const Xf& xf) {
     AABB a;
     if (s->type == B2G_CIRCLE) {
@@ -267,6 +267,11 @@ AABB shapeAABB(const b2g_shape* s, const Xf& xf) {
         V2 w = mul(xf, v(s->verts[i]));
         lo = vmin(lo, w);
         hi = vmax(hi, w);
+    }
+    if (s->type == B2G_EDGE && (s->flags & 4)) {
+        a.lo = lo;
+        a.hi = hi;
+        return a;
     }
     V2 r = mk(s->radius, s->radius);
     a.lo = lo - r;
@@ -2699,6 +2704,27 @@ struct World {
                     }
                 } else if (f.shape == b2scene::kCircle) {
                     circleShape(&s, mk(f.cx, f.cy), f.radius);
+                } else if (f.shape == b2scene::kChain) {
+                    // b2ChainShape: one proxy per child edge, in child order
+                    // (REF/Collision/Shapes/b2ChainShape.cpp:39-143)
+                    std::vector<V2> vs;
+                    for (int j = 0; j < f.n && j < 8; ++j) vs.push_back(mk(f.pts[2 * j], f.pts[2 * j + 1]));
+                    V2 prev = mk(f.e[0], f.e[1]), next = mk(f.e[6], f.e[7]);
+                    bool hasPrev = f.has0 != 0, hasNext = f.has3 != 0;
+                    if (f.polyMode == 1) {
+                        vs.push_back(vs[0]);
+                        prev = vs[vs.size() - 2];
+                        next = vs[1];
+                        hasPrev = hasNext = true;
+                    }
+                    const int m = (int)vs.size();
+                    for (int c = 0; c + 1 < m; ++c) {
+                        edgeShape(&s, c > 0 ? vs[c - 1] : prev, vs[c], vs[c + 1], c < m - 2 ? vs[c + 2] : next,
+                                  c > 0 || hasPrev, c < m - 2 || hasNext);
+                        s.flags |= 4;
+                        createFixture(bi, s, f);
+                    }
+                    continue;
                 } else {
                     edgeShape(&s, mk(f.e[0], f.e[1]), mk(f.e[2], f.e[3]), mk(f.e[4], f.e[5]), mk(f.e[6], f.e[7]),
                               f.has0 != 0, f.has3 != 0);

@@ -173,8 +173,19 @@ void manifoldOut(const b2Manifold& m, b2g_manifold* out) {
 
 // Builds one fixture's shape from a scene definition through the public API.
 void buildShape(const b2scene::FixtureDef& fd, b2PolygonShape* poly, b2CircleShape* circle,
-                b2EdgeShape* edge, const b2Shape** shape) {
-    if (fd.shape == b2scene::kPolygon) {
+                b2EdgeShape* edge, b2ChainShape* chain, const b2Shape** shape) {
+    if (fd.shape == b2scene::kChain) {
+        b2Vec2 pts[8];
+        for (int i = 0; i < fd.n && i < 8; ++i) pts[i].Set(fd.pts[2 * i], fd.pts[2 * i + 1]);
+        if (fd.polyMode == 1) {
+            chain->CreateLoop(pts, fd.n);
+        } else {
+            chain->CreateChain(pts, fd.n);
+            if (fd.has0) chain->SetPrevVertex(b2Vec2(fd.e[0], fd.e[1]));
+            if (fd.has3) chain->SetNextVertex(b2Vec2(fd.e[6], fd.e[7]));
+        }
+        *shape = chain;
+    } else if (fd.shape == b2scene::kPolygon) {
         if (fd.polyMode == b2scene::kPolyBox) {
             poly->SetAsBox(fd.pts[0], fd.pts[1]);
         } else if (fd.polyMode == b2scene::kPolyBoxRotated) {
@@ -257,8 +268,9 @@ int ref_world_load(void* h, const b2scene::BodyDef* bodies, int nb, const b2scen
             b2PolygonShape poly;
             b2CircleShape circle;
             b2EdgeShape edge;
+            b2ChainShape chain;
             const b2Shape* shape = nullptr;
-            buildShape(fd, &poly, &circle, &edge, &shape);
+            buildShape(fd, &poly, &circle, &edge, &chain, &shape);
             b2FixtureDef fdef;
             fdef.shape = shape;
             fdef.density = fd.density;
@@ -269,7 +281,9 @@ int ref_world_load(void* h, const b2scene::BodyDef* bodies, int nb, const b2scen
             fdef.filter.maskBits = (uint16)fd.maskBits;
             fdef.filter.groupIndex = (int16)fd.groupIndex;
             fdef.userData = (void*)(intptr_t)w->fixtures.size();
-            w->fixtures.push_back(body->CreateFixture(&fdef));
+            b2Fixture* fixture = body->CreateFixture(&fdef);
+            // one entry per proxy (a chain has one per child edge)
+            for (int c = 0; c < fixture->GetShape()->GetChildCount(); ++c) w->fixtures.push_back(fixture);
         }
     }
     return 0;
@@ -379,8 +393,8 @@ int ref_world_get_contacts(void* h, int32_t* meta, b2g_manifold* manifolds, int 
     int n = 0;
     for (b2Contact* c = w->world->GetContactList(); c; c = c->GetNext()) {
         if (n < cap) {
-            meta[4 * n + 0] = fixtureIndex(c->GetFixtureA());
-            meta[4 * n + 1] = fixtureIndex(c->GetFixtureB());
+            meta[4 * n + 0] = fixtureIndex(c->GetFixtureA()) + c->GetChildIndexA();
+            meta[4 * n + 1] = fixtureIndex(c->GetFixtureB()) + c->GetChildIndexB();
             meta[4 * n + 2] = (c->IsTouching() ? 1 : 0) | (c->IsEnabled() ? 2 : 0);
             meta[4 * n + 3] = 0;
             manifoldOut(*c->GetManifold(), manifolds + n);
@@ -395,14 +409,25 @@ void ref_world_get_fat_aabbs(void* h, b2g_aabb* out) {
     RefWorld* w = (RefWorld*)h;
     const b2BroadPhase& bp = w->world->GetContactManager().m_broadPhase;
     for (size_t i = 0; i < w->fixtures.size(); ++i) {
-        const b2AABB& a = bp.GetFatAABB(w->fixtures[i]->m_proxies[0].proxyId);
+        const int child = (int)i - fixtureIndex(w->fixtures[i]);
+        const b2AABB& a = bp.GetFatAABB(w->fixtures[i]->m_proxies[child].proxyId);
         out[i] = {a.lowerBound.x, a.lowerBound.y, a.upperBound.x, a.upperBound.y};
     }
 }
 
 void ref_world_get_shapes(void* h, b2g_shape* out) {
     RefWorld* w = (RefWorld*)h;
-    for (size_t i = 0; i < w->fixtures.size(); ++i) shapeToRec(w->fixtures[i]->GetShape(), out + i);
+    for (size_t i = 0; i < w->fixtures.size(); ++i) {
+        const b2Shape* sh = w->fixtures[i]->GetShape();
+        if (sh->GetType() == b2Shape::e_chain) {
+            b2EdgeShape e;
+            static_cast<const b2ChainShape*>(sh)->GetChildEdge(&e, (int)i - fixtureIndex(w->fixtures[i]));
+            shapeToRec(&e, out + i);
+            out[i].flags |= 4;
+        } else {
+            shapeToRec(sh, out + i);
+        }
+    }
 }
 
 // Mean and max penetration (metres) over touching manifold points, computed

@@ -82,6 +82,7 @@ def test_toi_batch_matches_oracle(ctx, orclib, wide_trig, seed):
     ("bullets", 24, 3, 120, True, True),
     ("pyramid", 20, 0, 300, True, False),
     ("pile", 400, 3, 250, False, False),
+    ("terrain", 60, 3, 200, True, False),
 ])
 def test_reference_mode_ccd_bitexact(gpu, orc, wide_trig, kind, n, seed, steps, sleep, substep):
     g = gpu.world(kind, n, seed, sleep=sleep, ccd=True, substep=substep)

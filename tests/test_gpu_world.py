@@ -187,3 +187,38 @@ def test_overflow_colour_plank(gpu, orc):
     assert np.abs(ga[:, E.CX] - ob[:, E.CX]).max() < 0.005
     assert np.abs(ga[2:, E.VY]).max() < 0.05
     a.close(); b.close(); o.close()
+
+
+def _floor_y(x):
+    """The terrain scene's floor chain: 8 points from x = -14 to 14, 4 m
+    apart, alternating y = 0 and 0.8."""
+    xs = -14.0 + 4.0 * np.arange(8)
+    ys = np.where(np.arange(8) % 2 == 1, 0.8, 0.0)
+    return np.interp(x, xs, ys)
+
+
+def test_chain_terrain_coloured(gpu, orc):
+    """Chain shapes through the box2d:: API (b2ChainShape children as device
+    fixtures): the fat boxes at load equal the oracle's bit for bit, and with
+    the coloured solver no body crosses the bumpy floor chain (bodies that
+    roll off its ends slide down the hexagonal loop beneath it) or leaves the
+    loop."""
+    g = gpu.world("terrain", 120, 4, sleep=True)
+    o = orc.world("terrain", 120, 4, sleep=True)
+    g.step(1)
+    o.step(1)
+    assert np.array_equal(g.fat_aabbs().view(np.uint32), o.fat_aabbs().view(np.uint32))
+    prev = g.bodies()[1:]
+    for _ in range(400):
+        g.step(1)
+        b = g.bodies()[1:]
+        assert np.all(np.isfinite(b))
+        inside = (np.abs(prev[:, E.PX]) < 13.5) & (np.abs(b[:, E.PX]) < 13.5)
+        above_then = prev[:, E.PY] > _floor_y(prev[:, E.PX])
+        below_now = b[:, E.PY] < _floor_y(b[:, E.PX])
+        assert not np.any(inside & above_then & below_now), "a body crossed the floor chain"
+        assert np.hypot(b[:, E.PX], b[:, E.PY] - 10.0).max() < 24.0
+        prev = b
+    meta, _ = g.contacts()
+    assert np.any((meta[:, 0] < 16) & (meta[:, 2] & 1 == 1))
+    g.close(); o.close()

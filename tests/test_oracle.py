@@ -28,6 +28,8 @@ F = ctypes.c_float
     ("chains", 2, 32, 250, True, 4, 2),
     ("stress", 1000, 3, 200, False, 8, 3),     # pile dropped on a mat of 20 chains of 32 links
     ("plank", 100, 0, 200, True, 8, 3),        # one body in 101 contacts
+    ("terrain", 60, 1, 400, True, 8, 3),       # chain shapes: open chains with and without ghosts, a loop
+    ("terrain", 100, 2, 300, False, 8, 3),
 ])
 def test_world_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, vel, pos):
     a = ref.world(kind, n, seed, sleep=sleep)
@@ -373,6 +375,7 @@ def test_one_sided_platform_listener(ref):
     ("bullets", 24, 3, 120, True, True),       # sub-stepping: one TOI event per step
     ("pyramid", 20, 0, 300, True, False),      # the reference defaults (sleep and CCD on)
     ("pile", 400, 3, 300, False, False),       # fast bodies against the static container
+    ("terrain", 60, 3, 300, True, False),      # chain shapes
 ])
 def test_ccd_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, substep):
     a = ref.world(kind, n, seed, sleep=sleep, ccd=True, substep=substep)
@@ -511,3 +514,36 @@ def test_toi_closed_form(orclib):
         assert state == 3, state
         dist = d0 - float(t[0]) * L
         assert abs(dist - target) < 0.25 * slop + 1e-6, (rA, rB, d0, L, dist, target)
+
+
+def test_chain_children(ref, orc):
+    """A chain fixture is one proxy per child edge (b2ChainShape): the child
+    edges' ghost vertices (open chain with and without SetPrev/NextVertex, a
+    loop closed across its seam) equal the reference's GetChildEdge, and each
+    child's fat box is its segment's box without the skin, widened by the
+    AABB extension (0.1 m), as the reference's broad-phase holds it."""
+    a = ref.world("terrain", 10, 1)
+    b = orc.world("terrain", 10, 1)
+    n = a.e.fn("world_fixture_count")(a.h)
+    assert n == b.e.fn("world_fixture_count")(b.h) == 7 + 3 + 6 + 10
+    sa, sb = np.zeros(n, E.SHAPE), np.zeros(n, E.SHAPE)
+    a.e.fn("world_get_shapes")(a.h, E.ptr(sa))
+    b.e.fn("world_get_shapes")(b.h, E.ptr(sb))
+    assert np.array_equal(sa[:16].view(np.uint8), sb[:16].view(np.uint8))
+    flags = sa["flags"][:16]
+    assert np.all(flags & 4)
+    assert flags[0] & 1 and flags[6] & 2                 # ghosts given beyond the floor's ends
+    assert not (flags[7] & 1) and not (flags[9] & 2)     # the ramp has free ends
+    assert np.all(flags[10:16] & 3 == 3)                 # a loop is smooth all round
+    fa, fb = a.fat_aabbs(), b.fat_aabbs()
+    assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32))
+    for i in range(16):
+        v = np.array([[sa[i]["verts"][k]["x"], sa[i]["verts"][k]["y"]] for k in range(2)], np.float32)
+        lo, hi = v.min(0) - np.float32(0.1), v.max(0) + np.float32(0.1)
+        assert np.array_equal(np.array([fa[i]["lx"], fa[i]["ly"]], np.float32), lo), i
+        assert np.array_equal(np.array([fa[i]["ux"], fa[i]["uy"]], np.float32), hi), i
+    a.step(100); b.step(100)
+    assert np.array_equal(a.fat_aabbs().view(np.uint32), b.fat_aabbs().view(np.uint32))
+    meta, _ = b.contacts()
+    assert np.any((meta[:, 0] < 16) & (meta[:, 2] & 1 == 1))  # bodies rest on chain children
+    a.close(); b.close()
