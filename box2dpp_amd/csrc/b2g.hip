@@ -106,7 +106,9 @@ struct ConView {
 enum : int {
     JF_LIMIT = 1, JF_MOTOR = 2, JF_COLLIDE = 4, JF_REMOVED = 8,
     JL_INACTIVE = 0, JL_LOWER = 1, JL_UPPER = 2, JL_EQUAL = 3,  // limit state in bits 8-9
+    JT_REVOLUTE = 0, JT_DISTANCE = 1,                            // joint type in bits 12-15
 };
+__host__ __device__ inline int jointType(int flags) { return (flags >> 12) & 15; }
 struct JointView {
     int n;
     const int* order;  // joint per slot, colour by colour
@@ -121,7 +123,12 @@ struct JointView {
     float4* K0;        // mass matrix m11 m12 m13 m22
     float2* K1;        // m23 m33
     float* motorMass;
+    float h;           // time step of the solve (soft joints)
 };
+// Per-type use of the arrays. Revolute: limits = referenceAngle, lower,
+// upper, maxMotorTorque; impulse = (x, y, limit, motor). Distance: limits =
+// length, frequencyHz, dampingRatio; impulse.x; K0 = (u.x, u.y, mass, bias),
+// K1.x = gamma.
 
 __device__ inline int encodeOrdered(float f) {
     int i = __float_as_int(f);
@@ -1297,10 +1304,128 @@ __device__ inline V2 solve22(const Sym33& m, V2 b) {
     return mk(det * (a22 * b.x - a12 * b.y), det * (a11 * b.y - a21 * b.x));
 }
 
+// b2DistanceJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2DistanceJoint.cpp:63-154
+__device__ inline void distanceInitOne(const WorldView& w, const JointView& jv, int j, int2 b, float dtRatio,
+                                       int warm) {
+    const float4 msA = w.mass[b.x], msB = w.mass[b.y];
+    const float mA = msA.x, iA = msA.y, mB = msB.x, iB = msB.y;
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const Rot qA = rotOf(pA.z), qB = rotOf(pB.z);
+    const float4 an = jv.anchors[j];
+    const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    V2 u = mk(pB.x, pB.y) + rB - mk(pA.x, pA.y) - rA;
+    const float length = lengthOf(u);
+    if (length > kLinearSlop) u *= 1.0f / length;
+    else u = mk(0.0f, 0.0f);
+    const float crAu = cross(rA, u), crBu = cross(rB, u);
+    float invMass = mA + iA * crAu * crAu + mB + iB * crBu * crBu;
+    float mass = invMass != 0.0f ? 1.0f / invMass : 0.0f;
+    const float4 par = jv.limits[j];  // length, frequencyHz, dampingRatio
+    float gamma = 0.0f, bias = 0.0f;
+    if (par.y > 0.0f) {
+        const float C = length - par.x;
+        const float omega = 2.0f * kPi * par.y;
+        const float d = 2.0f * mass * par.z * omega;
+        const float k = mass * omega * omega;
+        const float h = jv.h;
+        gamma = h * (d + h * k);
+        gamma = gamma != 0.0f ? 1.0f / gamma : 0.0f;
+        bias = C * h * k * gamma;
+        invMass += gamma;
+        mass = invMass != 0.0f ? 1.0f / invMass : 0.0f;
+    }
+    float4 imp = jv.impulse[j];
+    if (warm) {
+        imp.x *= dtRatio;
+        const V2 P = imp.x * u;
+        vA -= mA * P;
+        wA -= iA * cross(rA, P);
+        vB += mB * P;
+        wB += iB * cross(rB, P);
+    } else {
+        imp.x = 0.0f;
+    }
+    imp.y = u.x;  // the step's direction, for b2DistanceJoint::GetReactionForce
+    imp.z = u.y;
+    jv.impulse[j] = imp;
+    jv.r[j] = make_float4(rA.x, rA.y, rB.x, rB.y);
+    jv.masses[j] = make_float4(mA, iA, mB, iB);
+    jv.K0[j] = make_float4(u.x, u.y, mass, bias);
+    jv.K1[j] = make_float2(gamma, 0.0f);
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2DistanceJoint::SolveVelocityConstraints, REF b2DistanceJoint.cpp:156-181
+__device__ inline void distanceSolveVelocityOne(const WorldView& w, const JointView& jv, int j, int2 b) {
+    const float4 ms = jv.masses[j];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const float4 r = jv.r[j];
+    const V2 rA = mk(r.x, r.y), rB = mk(r.z, r.w);
+    const float4 k0 = jv.K0[j];
+    const V2 u = mk(k0.x, k0.y);
+    const V2 vpA = vA + cross(wA, rA);
+    const V2 vpB = vB + cross(wB, rB);
+    const float Cdot = dot(u, vpB - vpA);
+    float4 imp = jv.impulse[j];
+    const float impulse = -k0.z * (Cdot + k0.w + jv.K1[j].x * imp.x);
+    imp.x += impulse;
+    const V2 P = impulse * u;
+    vA -= mA * P;
+    wA -= iA * cross(rA, P);
+    vB += mB * P;
+    wB += iB * cross(rB, P);
+    jv.impulse[j] = imp;
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2DistanceJoint::SolvePositionConstraints, REF b2DistanceJoint.cpp:183-221:
+// none for springs; rods report an error of |C| against the linear slop.
+__device__ inline bool distanceSolvePositionOne(const WorldView& w, const JointView& jv, int j, int2 b) {
+    const float4 par = jv.limits[j];
+    if (par.y > 0.0f) return true;
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    V2 cA = mk(pA.x, pA.y), cB = mk(pB.x, pB.y);
+    float aA = pA.z, aB = pB.z;
+    const float4 ms = jv.masses[j];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const Rot qA = rotOf(aA), qB = rotOf(aB);
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float4 an = jv.anchors[j];
+    const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    V2 u = cB + rB - cA - rA;
+    const float length = normalizeV(u);
+    float C = length - par.x;
+    C = clampB(C, -kMaxLinearCorrection, kMaxLinearCorrection);
+    const float impulse = -jv.K0[j].z * C;
+    const V2 P = impulse * u;
+    cA -= mA * P;
+    aA -= iA * cross(rA, P);
+    cB += mB * P;
+    aB += iB * cross(rB, P);
+    if (w.btype[b.x] == BT_DYNAMIC) w.pos[b.x] = make_float4(cA.x, cA.y, aA, pA.w);
+    if (w.btype[b.y] == BT_DYNAMIC) w.pos[b.y] = make_float4(cB.x, cB.y, aB, pB.w);
+    return absB(C) < kLinearSlop;
+}
+
 // b2RevoluteJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2RevoluteJoint.cpp:66-198
 __device__ inline void jointInitOne(const WorldView& w, const JointView& jv, int j, float dtRatio, int warm) {
     const int2 b = jv.bodies[j];
     if (!jointActive(w, b)) return;
+    if (jointType(jv.flags[j]) == JT_DISTANCE) {
+        distanceInitOne(w, jv, j, b, dtRatio, warm);
+        return;
+    }
     const float4 msA = w.mass[b.x], msB = w.mass[b.y];
     const float mA = msA.x, iA = msA.y, mB = msB.x, iB = msB.y;
     const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
@@ -1357,7 +1482,7 @@ __device__ inline void jointInitOne(const WorldView& w, const JointView& jv, int
     } else {
         imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    jv.flags[j] = (flags & 0xFF) | (state << 8);
+    jv.flags[j] = (flags & ~0x300) | (state << 8);
     jv.impulse[j] = imp;
     jv.r[j] = make_float4(rA.x, rA.y, rB.x, rB.y);
     jv.masses[j] = make_float4(mA, iA, mB, iB);
@@ -1382,6 +1507,10 @@ __global__ void k_joint_init_serial(WorldView w, JointView jv, int begin, int en
 __device__ inline void jointSolveVelocityOne(const WorldView& w, const JointView& jv, int j, float h) {
     const int2 b = jv.bodies[j];
     if (!jointActive(w, b)) return;
+    if (jointType(jv.flags[j]) == JT_DISTANCE) {
+        distanceSolveVelocityOne(w, jv, j, b);
+        return;
+    }
     const float4 ms = jv.masses[j];
     const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
     const float4 va = w.vel[b.x], vbv = w.vel[b.y];
@@ -1474,6 +1603,13 @@ __device__ inline void jointSolvePositionOne(const WorldView& w, const JointView
     if (!jointActive(w, b)) return;
     const int root = w.islandOf[w.btype[b.x] != BT_STATIC ? b.x : b.y];
     if (it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0) return;  // island done
+    if (jointType(jv.flags[j]) == JT_DISTANCE) {
+        if (!distanceSolvePositionOne(w, jv, j, b)) {
+            int* f = &w.islandMinSep[it * w.nBodies + root];
+            if (*f == 0) *f = 1;
+        }
+        return;
+    }
     const float4 pA = w.pos[b.x], pB = w.pos[b.y];
     V2 cA = mk(pA.x, pA.y), cB = mk(pB.x, pB.y);
     float aA = pA.z, aB = pB.z;
@@ -1572,7 +1708,13 @@ __global__ void k_joint_drift(WorldView w, JointView jv, float* out) {
     const float4 an = jv.anchors[j];
     const V2 rA = mul(rotOf(pA.z), mk(an.x, an.y) - mk(lcA.x, lcA.y));
     const V2 rB = mul(rotOf(pB.z), mk(an.z, an.w) - mk(lcB.x, lcB.y));
-    out[j] = lengthOf(mk(pB.x, pB.y) + rB - mk(pA.x, pA.y) - rA);
+    const float gap = lengthOf(mk(pB.x, pB.y) + rB - mk(pA.x, pA.y) - rA);
+    if (jointType(jv.flags[j]) == JT_DISTANCE) {
+        // rods: the error in length; springs are meant to stretch
+        out[j] = jv.limits[j].y > 0.0f ? -1.0f : absB(gap - jv.limits[j].x);
+        return;
+    }
+    out[j] = gap;
 }
 
 // Kinetic (0.5 m v.v + 0.5 I w^2) and potential (-m g.c) energy per body.
@@ -3275,6 +3417,7 @@ struct b2g_context {
         j.bodies = jBodies; j.anchors = jAnchors; j.limits = jLimits; j.motorSpeed = jMotorSpeed;
         j.flags = jFlags; j.impulse = jImpulse; j.r = jR; j.masses = jMasses; j.K0 = jK0; j.K1 = jK1;
         j.motorMass = jMotorMass;
+        j.h = 0.0f;
         return j;
     }
 
@@ -4371,7 +4514,8 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
             }
         };
         // Joints: colour by colour; the overflow colour by one lane in order.
-        const JointView jv = ctx->jview();
+        JointView jv = ctx->jview();
+        jv.h = dt;
         auto perJointColour = [&](auto kernelLaunch) {
             const int nc = ctx->nJoints > 0 ? kColourSlots : 0;
             for (int c = 0; c < nc; ++c) {
@@ -6137,7 +6281,7 @@ int b2g_abi_sizes(int32_t* out, int cap) {
         (int32_t)sizeof(b2g_fixture_def), (int32_t)sizeof(b2g_world_def),    (int32_t)sizeof(b2g_step_params),
         (int32_t)sizeof(b2g_step_stats),  (int32_t)sizeof(b2g_phase_profile), (int32_t)sizeof(b2g_mlp_params),
         (int32_t)sizeof(b2g_aabb),        (int32_t)sizeof(b2g_transform),    (int32_t)sizeof(b2g_revolute_def),
-        (int32_t)sizeof(b2g_schedule),    (int32_t)sizeof(b2g_sweep),
+        (int32_t)sizeof(b2g_schedule),    (int32_t)sizeof(b2g_sweep),        (int32_t)sizeof(b2g_distance_def),
     };
     const int n = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (!out) return n;
@@ -6481,6 +6625,7 @@ int b2g_set_joint(b2g_context* ctx, int joint, const b2g_revolute_def* d) {
     HIPCHK(ctx, hipMemcpyAsync(&lim0, ctx->jLimits + joint, sizeof(float4), hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipMemcpyAsync(&imp, ctx->jImpulse + joint, sizeof(float4), hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
+    if (jointType(flags) != JT_REVOLUTE) return fail(ctx, B2G_ERR_ARG, "not a revolute joint");
     // EnableLimit and SetLimits drop the limit impulse when they change
     // something (REF/Dynamics/Joints/b2RevoluteJoint.cpp:440-471).
     const bool limitSwitched = ((flags & JF_LIMIT) != 0) != (d->enableLimit != 0);
@@ -6492,6 +6637,62 @@ int b2g_set_joint(b2g_context* ctx, int joint, const b2g_revolute_def* d) {
     HIPCHK(ctx, hipMemcpyAsync(ctx->jLimits + joint, &lim, sizeof(float4), hipMemcpyHostToDevice, s));
     HIPCHK(ctx, hipMemcpyAsync(ctx->jMotorSpeed + joint, &d->motorSpeed, sizeof(float), hipMemcpyHostToDevice, s));
     HIPCHK(ctx, hipMemcpyAsync(ctx->jImpulse + joint, &imp, sizeof(float4), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+int b2g_add_distance_joints(b2g_context* ctx, const b2g_distance_def* defs, int count, int* first) {
+    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
+    for (int i = 0; i < count; ++i) {
+        const b2g_distance_def& d = defs[i];
+        if (d.bodyA < 0 || d.bodyB < 0 || d.bodyA >= ctx->nBodies || d.bodyB >= ctx->nBodies || d.bodyA == d.bodyB)
+            return fail(ctx, B2G_ERR_ARG, "joint bodies out of range or equal");
+    }
+    if (first) *first = ctx->nJoints;
+    if (count == 0) return B2G_OK;
+    int rc = growJoints(ctx, ctx->nJoints + count);
+    if (rc != B2G_OK) return rc;
+    std::vector<int2> b(count);
+    std::vector<float4> anchors(count), params(count), imp(count, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    std::vector<float> speed(count, 0.0f);
+    std::vector<int> flags(count);
+    bool keepApart = false;
+    for (int i = 0; i < count; ++i) {
+        const b2g_distance_def& d = defs[i];
+        b[i] = make_int2(d.bodyA, d.bodyB);
+        anchors[i] = make_float4(d.localAnchorA[0], d.localAnchorA[1], d.localAnchorB[0], d.localAnchorB[1]);
+        params[i] = make_float4(d.length, d.frequencyHz, d.dampingRatio, 0.0f);
+        flags[i] = (JT_DISTANCE << 12) | (d.collideConnected ? JF_COLLIDE : 0);
+        ctx->hJointBodies.push_back(b[i]);
+        ctx->hJointKeepApart.push_back(d.collideConnected ? 0 : 1);
+        ctx->hJointRemoved.push_back(0);
+        keepApart = keepApart || !d.collideConnected;
+    }
+    hipStream_t s = ctx->stream;
+    const int n0 = ctx->nJoints;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jBodies + n0, b.data(), sizeof(int2) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jAnchors + n0, anchors.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jLimits + n0, params.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jMotorSpeed + n0, speed.data(), sizeof(float) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jFlags + n0, flags.data(), sizeof(int) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jImpulse + n0, imp.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    ctx->nJoints += count;
+    ctx->jointsDirty = true;
+    if (keepApart) return rebuildJointPairs(ctx);
+    return B2G_OK;
+}
+
+int b2g_set_distance_joint(b2g_context* ctx, int joint, float length, float frequencyHz, float dampingRatio) {
+    if (!ctx || joint < 0 || joint >= ctx->nJoints) return B2G_ERR_ARG;
+    if (ctx->hJointRemoved[joint]) return fail(ctx, B2G_ERR_ARG, "joint was removed");
+    hipStream_t s = ctx->stream;
+    int flags = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&flags, ctx->jFlags + joint, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    if (jointType(flags) != JT_DISTANCE) return fail(ctx, B2G_ERR_ARG, "not a distance joint");
+    const float4 par = make_float4(length, frequencyHz, dampingRatio, 0.0f);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jLimits + joint, &par, sizeof(float4), hipMemcpyHostToDevice, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
     return B2G_OK;
 }

@@ -49,7 +49,7 @@ struct Runner {
     b2World world;
     std::vector<b2Body*> bodies;
     std::vector<b2Fixture*> fixtures;
-    std::vector<b2RevoluteJoint*> joints;
+    std::vector<b2Joint*> joints;
     Recorder recorder;
     explicit Runner(float gx, float gy) : world(b2Vec2(gx, gy)) {}
 };
@@ -138,6 +138,21 @@ int loadJoints(Runner* r, int base, const b2scene::JointDef* jd, int nj) {
     for (int k = 0; k < nj; ++k) {
         const b2scene::JointDef& d = jd[k];
         if (base + d.bodyA >= (int)r->bodies.size() || base + d.bodyB >= (int)r->bodies.size()) return B2G_ERR_ARG;
+        if (d.type == 1) {
+            b2DistanceJointDef dd;
+            dd.bodyA = r->bodies[base + d.bodyA];
+            dd.bodyB = r->bodies[base + d.bodyB];
+            dd.collideConnected = d.collideConnected != 0;
+            dd.localAnchorA = b2Vec2(d.ax, d.ay);
+            dd.localAnchorB = b2Vec2(d.bx, d.by);
+            dd.length = d.length;
+            dd.frequencyHz = d.frequencyHz;
+            dd.dampingRatio = d.dampingRatio;
+            b2Joint* j = r->world.CreateJoint(&dd);
+            if (j == nullptr) return B2G_ERR_ARG;
+            r->joints.push_back(j);
+            continue;
+        }
         b2RevoluteJointDef def;
         def.bodyA = r->bodies[base + d.bodyA];
         def.bodyB = r->bodies[base + d.bodyB];
@@ -153,7 +168,7 @@ int loadJoints(Runner* r, int base, const b2scene::JointDef* jd, int nj) {
         def.maxMotorTorque = d.maxMotorTorque;
         b2Joint* j = r->world.CreateJoint(&def);
         if (j == nullptr) return B2G_ERR_ARG;
-        r->joints.push_back(static_cast<b2RevoluteJoint*>(j));
+        r->joints.push_back(j);
     }
     return 0;
 }
@@ -210,7 +225,7 @@ int b2pp_world_joint_count(void* h) { return (int)((Runner*)h)->joints.size(); }
 void b2pp_world_get_joints(void* h, float* out) {
     Runner* r = (Runner*)h;
     for (size_t i = 0; i < r->joints.size(); ++i) {
-        const b2RevoluteJoint* j = r->joints[i];
+        const b2Joint* j = r->joints[i];
         if (j == nullptr) {  // destroyed
             for (int k = 0; k < 4; ++k) out[4 * i + k] = 0.0f;
             continue;
@@ -219,8 +234,23 @@ void b2pp_world_get_joints(void* h, float* out) {
         out[4 * i] = f[0];
         out[4 * i + 1] = f[1];
         out[4 * i + 2] = j->GetReactionTorque(1.0f);
-        out[4 * i + 3] = j->GetMotorTorque(1.0f);
+        out[4 * i + 3] = j->GetType() == b2JointType::REVOLUTE_JOINT
+                             ? static_cast<const b2RevoluteJoint*>(j)->GetMotorTorque(1.0f)
+                             : 0.0f;
     }
+}
+
+// Distance joint parameters through b2DistanceJoint::SetLength,
+// SetFrequency and SetDampingRatio.
+int b2pp_world_set_distance(void* h, int joint, float length, float hz, float ratio) {
+    Runner* r = (Runner*)h;
+    if (joint < 0 || joint >= (int)r->joints.size() || r->joints[joint] == nullptr) return B2G_ERR_ARG;
+    if (r->joints[joint]->GetType() != b2JointType::DISTANCE_JOINT) return B2G_ERR_ARG;
+    b2DistanceJoint* j = static_cast<b2DistanceJoint*>(r->joints[joint]);
+    j->SetLength(length);
+    j->SetFrequency(hz);
+    j->SetDampingRatio(ratio);
+    return 0;
 }
 
 // Joint switches through the box2d:: setters (b2RevoluteJoint::EnableMotor,
@@ -229,7 +259,8 @@ int b2pp_world_set_joint(void* h, int joint, int enableLimit, float lower, float
                          float maxTorque) {
     Runner* r = (Runner*)h;
     if (joint < 0 || joint >= (int)r->joints.size() || r->joints[joint] == nullptr) return B2G_ERR_ARG;
-    b2RevoluteJoint* j = r->joints[joint];
+    if (r->joints[joint]->GetType() != b2JointType::REVOLUTE_JOINT) return B2G_ERR_ARG;
+    b2RevoluteJoint* j = static_cast<b2RevoluteJoint*>(r->joints[joint]);
     j->EnableLimit(enableLimit != 0);
     j->SetLimits(lower, upper);
     j->EnableMotor(enableMotor != 0);

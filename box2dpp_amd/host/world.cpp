@@ -456,10 +456,51 @@ float b2RevoluteJoint::GetJointSpeed() const {
     return m_bodyB->GetAngularVelocity() - m_bodyA->GetAngularVelocity();
 }
 
-void b2RevoluteJoint::MarkDirty() {
+void b2Joint::MarkDirty() {
     if (m_dirty) return;
     m_dirty = true;
     m_world->m_dirtyJoints.push_back(this);
+}
+
+// b2DistanceJointDef::Initialize, REF/Dynamics/Joints/b2DistanceJoint.cpp:40-49
+void b2DistanceJointDef::Initialize(b2Body* bA, b2Body* bB, const b2Vec2& anchorA, const b2Vec2& anchorB) {
+    bodyA = bA;
+    bodyB = bB;
+    localAnchorA = bodyA->GetLocalPoint(anchorA);
+    localAnchorB = bodyB->GetLocalPoint(anchorB);
+    const b2Vec2 d = anchorB - anchorA;
+    length = sqrtf(d[0] * d[0] + d[1] * d[1]);
+}
+
+b2DistanceJoint::b2DistanceJoint(const b2DistanceJointDef* def, b2World* world, int32_t index)
+    : b2Joint(def, world, index), m_localAnchorA(def->localAnchorA), m_localAnchorB(def->localAnchorB),
+      m_length(def->length), m_frequencyHz(def->frequencyHz), m_dampingRatio(def->dampingRatio) {}
+
+b2Vec2 b2DistanceJoint::GetAnchorA() const { return m_bodyA->GetWorldPoint(m_localAnchorA); }
+b2Vec2 b2DistanceJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localAnchorB); }
+// REF/Dynamics/Joints/b2DistanceJoint.cpp:233-244: the impulse along the
+// direction u of the last step.
+b2Vec2 b2DistanceJoint::GetReactionForce(float inv_dt) const {
+    m_world->DownloadJoints();
+    const float s = inv_dt * m_impulse[0];
+    return b2Vec2(s * m_impulse[1], s * m_impulse[2]);
+}
+float b2DistanceJoint::GetReactionTorque(float inv_dt) const {
+    B2_NOT_USED(inv_dt);
+    return 0.0f;
+}
+// Plain setters, as the reference's (b2DistanceJoint.h:144-170)
+void b2DistanceJoint::SetLength(float length) {
+    m_length = length;
+    MarkDirty();
+}
+void b2DistanceJoint::SetFrequency(float hz) {
+    m_frequencyHz = hz;
+    MarkDirty();
+}
+void b2DistanceJoint::SetDampingRatio(float ratio) {
+    m_dampingRatio = ratio;
+    MarkDirty();
 }
 
 // The setters wake both bodies and, where the limit changes, drop the limit
@@ -507,8 +548,8 @@ void b2RevoluteJoint::SetLimits(float lower, float upper) {
 // that a collideConnected = false joint filters out end at the next Step.
 b2Joint* b2World::CreateJoint(const b2JointDef* def) {
     if (IsLocked() || def == nullptr || def->bodyA == nullptr || def->bodyB == nullptr) return nullptr;
-    if (def->type != b2JointType::REVOLUTE_JOINT) {
-        m_lastError = "only revolute joints run on the GPU path";
+    if (def->type != b2JointType::REVOLUTE_JOINT && def->type != b2JointType::DISTANCE_JOINT) {
+        m_lastError = "only revolute and distance joints run on the GPU path";
         return nullptr;
     }
     if (def->bodyA == def->bodyB || def->bodyA->m_world != this || def->bodyB->m_world != this) return nullptr;
@@ -516,7 +557,11 @@ b2Joint* b2World::CreateJoint(const b2JointDef* def) {
         m_lastError = "a joint connects bodies of one sub-world";
         return nullptr;
     }
-    b2Joint* j = new b2RevoluteJoint(static_cast<const b2RevoluteJointDef*>(def), this, (int32_t)m_joints.size());
+    b2Joint* j = nullptr;
+    if (def->type == b2JointType::DISTANCE_JOINT)
+        j = new b2DistanceJoint(static_cast<const b2DistanceJointDef*>(def), this, (int32_t)m_joints.size());
+    else
+        j = new b2RevoluteJoint(static_cast<const b2RevoluteJointDef*>(def), this, (int32_t)m_joints.size());
     m_joints.push_back(j);
     j->m_prev = nullptr;
     j->m_next = m_jointList;
@@ -559,7 +604,7 @@ void b2World::DestroyJoint(b2Joint* j) {
     --m_jointCount;
     m_removedJoints.push_back(j->m_index);
     auto& dirty = m_dirtyJoints;
-    dirty.erase(std::remove(dirty.begin(), dirty.end(), static_cast<b2RevoluteJoint*>(j)), dirty.end());
+    dirty.erase(std::remove(dirty.begin(), dirty.end(), j), dirty.end());
     // A joint not yet on the device keeps its slot until the upload adds it
     // (device indices are creation order) and removes it again.
     if (j->m_index < m_uploadedJoints) {
@@ -581,10 +626,24 @@ void b2World::DownloadJoints() const {
     for (int i = 0; i < n; ++i) {
         b2Joint* j = m_joints[i];
         if (j == nullptr) continue;
-        b2RevoluteJoint* r = static_cast<b2RevoluteJoint*>(j);
-        if (r->m_dirty) continue;
-        for (int k = 0; k < 4; ++k) r->m_impulse[k] = st[4 * (size_t)i + k];
+        if (j->m_dirty) continue;
+        for (int k = 0; k < 4; ++k) j->m_impulse[k] = st[4 * (size_t)i + k];
     }
+}
+
+static b2g_distance_def distanceDef(const b2DistanceJoint* d, int32_t bodyA, int32_t bodyB) {
+    b2g_distance_def r{};
+    r.bodyA = bodyA;
+    r.bodyB = bodyB;
+    r.localAnchorA[0] = d->GetLocalAnchorA()[0];
+    r.localAnchorA[1] = d->GetLocalAnchorA()[1];
+    r.localAnchorB[0] = d->GetLocalAnchorB()[0];
+    r.localAnchorB[1] = d->GetLocalAnchorB()[1];
+    r.length = d->GetLength();
+    r.frequencyHz = d->GetFrequency();
+    r.dampingRatio = d->GetDampingRatio();
+    r.collideConnected = d->GetCollideConnected() ? 1 : 0;
+    return r;
 }
 
 static b2g_revolute_def revoluteDef(const b2RevoluteJoint* r, int32_t bodyA, int32_t bodyB) {
@@ -760,15 +819,30 @@ int32_t b2World::Upload() {
     }
     const int nj = (int)m_joints.size();
     if (nj > m_uploadedJoints) {
-        std::vector<b2g_revolute_def> defs;
-        defs.reserve(nj - m_uploadedJoints);
-        for (int i = m_uploadedJoints; i < nj; ++i) {
-            const b2RevoluteJoint* r = static_cast<const b2RevoluteJoint*>(m_joints[i]);
-            defs.push_back(revoluteDef(r, r->m_bodyA->m_index, r->m_bodyB->m_index));
+        // runs of one type at a time, so device indices follow creation order
+        for (int i = m_uploadedJoints; i < nj;) {
+            const b2JointType type = m_joints[i]->m_type;
+            int k = i;
+            while (k < nj && m_joints[k]->m_type == type) ++k;
+            int first = 0;
+            if (type == b2JointType::DISTANCE_JOINT) {
+                std::vector<b2g_distance_def> defs;
+                for (int q = i; q < k; ++q) {
+                    const b2DistanceJoint* d = static_cast<const b2DistanceJoint*>(m_joints[q]);
+                    defs.push_back(distanceDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index));
+                }
+                rc = b2g_add_distance_joints(m_ctx, defs.data(), (int)defs.size(), &first);
+            } else {
+                std::vector<b2g_revolute_def> defs;
+                for (int q = i; q < k; ++q) {
+                    const b2RevoluteJoint* r = static_cast<const b2RevoluteJoint*>(m_joints[q]);
+                    defs.push_back(revoluteDef(r, r->m_bodyA->m_index, r->m_bodyB->m_index));
+                }
+                rc = b2g_add_joints(m_ctx, defs.data(), (int)defs.size(), &first);
+            }
+            if (rc != B2G_OK) return rc;
+            i = k;
         }
-        int first = 0;
-        rc = b2g_add_joints(m_ctx, defs.data(), (int)defs.size(), &first);
-        if (rc != B2G_OK) return rc;
         for (int i = m_uploadedJoints; i < nj; ++i) {
             if (m_joints[i]->m_world == nullptr) {  // destroyed before its upload
                 delete m_joints[i];
@@ -783,11 +857,17 @@ int32_t b2World::Upload() {
         m_removedJoints.clear();
     }
     if (!m_dirtyJoints.empty()) {
-        for (b2RevoluteJoint* r : m_dirtyJoints) {
-            r->m_dirty = false;
-            if (r->m_index >= m_uploadedJoints) continue;
-            const b2g_revolute_def d = revoluteDef(r, r->m_bodyA->m_index, r->m_bodyB->m_index);
-            rc = b2g_set_joint(m_ctx, r->m_index, &d);
+        for (b2Joint* j : m_dirtyJoints) {
+            j->m_dirty = false;
+            if (j->m_index >= m_uploadedJoints) continue;
+            if (j->m_type == b2JointType::DISTANCE_JOINT) {
+                const b2DistanceJoint* d = static_cast<const b2DistanceJoint*>(j);
+                rc = b2g_set_distance_joint(m_ctx, j->m_index, d->m_length, d->m_frequencyHz, d->m_dampingRatio);
+            } else {
+                const b2RevoluteJoint* r = static_cast<const b2RevoluteJoint*>(j);
+                const b2g_revolute_def d = revoluteDef(r, r->m_bodyA->m_index, r->m_bodyB->m_index);
+                rc = b2g_set_joint(m_ctx, j->m_index, &d);
+            }
             if (rc != B2G_OK) return rc;
         }
         m_dirtyJoints.clear();

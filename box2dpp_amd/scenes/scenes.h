@@ -67,6 +67,9 @@ struct JointDef {
     float lowerAngle, upperAngle;
     float maxMotorTorque, motorSpeed;
     int enableLimit, enableMotor, collideConnected;
+    // b2DistanceJointDef (Box2D/Dynamics/Joints/b2DistanceJoint.h:32-63) when type is 1
+    int type;              // 0 revolute, 1 distance
+    float length, frequencyHz, dampingRatio;
 };
 
 struct Scene {
@@ -447,13 +450,73 @@ inline Scene terrain(int n, uint32_t seed) {
     return s;
 }
 
+// Distance joints: n boxes hung from the ground by rods or springs (every
+// other one soft, at 1.5-2.5 Hz with damping 0-0.6), each (when `chained`)
+// with a smaller box hanging from it on a rod, over a ground edge. Lengths
+// are the anchors' distances at creation (b2DistanceJointDef::Initialize).
+inline Scene distanceJoints(int n, bool chained = true) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-200.0f, 0.0f, 200.0f, 0.0f));
+    for (int i = 0; i < n; ++i) {
+        const float x = 6.0f * (float)i - 3.0f * (float)n;
+        const float ax = x + 1.0f + 0.2f * (float)(i % 5), ay = 14.0f;
+        addBody(s, defaultBody(2, ax, ay));
+        addFixture(s, boxFixture(0.4f, 0.4f, 1.0f, 0.6f));
+        const int a = (int)s.bodies.size() - 1;
+        JointDef up{};
+        up.type = 1;
+        up.bodyA = 0; up.bodyB = a;
+        up.ax = x; up.ay = 20.0f;         // ground anchor (ground body at the origin)
+        up.bx = 0.3f; up.by = 0.3f;       // a corner of box A
+        const float dx = (ax + 0.3f) - x, dy = (ay + 0.3f) - 20.0f;
+        up.length = sqrtf(dx * dx + dy * dy);
+        up.frequencyHz = (i & 1) ? 1.5f + 0.5f * (float)(i % 3) : 0.0f;
+        up.dampingRatio = 0.2f * (float)(i % 4);
+        s.joints.push_back(up);
+        if (!chained) continue;
+        const float bx = x + 2.5f, by = 12.5f;
+        addBody(s, defaultBody(2, bx, by));
+        addFixture(s, boxFixture(0.3f, 0.3f, 1.0f, 0.6f));
+        const int b = a + 1;
+        JointDef rod{};
+        rod.type = 1;
+        rod.bodyA = a; rod.bodyB = b;
+        const float ex = bx - ax, ey = by - ay;
+        rod.length = sqrtf(ex * ex + ey * ey);
+        s.joints.push_back(rod);
+    }
+    return s;
+}
+
+// One box on a soft distance joint to a ground anchor 4 m above it, of rest
+// length 3 m, so it starts stretched by 1 m: a spring of `hz` Hz and
+// damping ratio `ratio` (for runs without gravity).
+inline Scene spring(float hz, float ratio) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addBody(s, defaultBody(2, 0.0f, 6.0f));
+    addFixture(s, boxFixture(0.25f, 0.25f, 1.0f, 0.6f));
+    JointDef j{};
+    j.type = 1;
+    j.bodyA = 0; j.bodyB = 1;
+    j.ax = 0.0f; j.ay = 10.0f;
+    j.length = 3.0f;
+    j.frequencyHz = hz;
+    j.dampingRatio = ratio;
+    s.joints.push_back(j);
+    return s;
+}
+
 // Scene by name: "hello", "pyramid" (n = rows, 0 -> 20), "pile" (n bodies,
 // seed), "world" (one batched-scene world of n bodies, seed), "pendulums"
 // (n pendulums), "chains" (n chains of `seed` links), "stress" (pile of n
 // bodies, seed, on a mat of n / 50 chains of 32 links), "plank" (n circles
 // on a plank), "oneway" (n boxes through a one-sided platform), "bullets"
-// (n fast bodies, seed), "terrain" (n bodies on chain shapes, seed). Unknown
-// names give an empty scene.
+// (n fast bodies, seed), "terrain" (n bodies on chain shapes, seed),
+// "distance" (n rods and springs with boxes; seed 0 hangs a second box on
+// each), "spring" (one box on a spring of n / 10 Hz and damping ratio
+// seed / 10). Unknown names give an empty scene.
 inline Scene byName(const char* kind, int n, uint32_t seed) {
     const std::string k(kind ? kind : "");
     if (k == "hello") return hello();
@@ -467,6 +530,8 @@ inline Scene byName(const char* kind, int n, uint32_t seed) {
     if (k == "oneway") return oneway(n > 0 ? n : 8);
     if (k == "bullets") return bullets(n > 0 ? n : 24, seed);
     if (k == "terrain") return terrain(n > 0 ? n : 60, seed);
+    if (k == "distance") return distanceJoints(n > 0 ? n : 16, seed == 0);
+    if (k == "spring") return spring(0.1f * (float)(n > 0 ? n : 10), 0.1f * (float)seed);
     return Scene();
 }
 

@@ -316,6 +316,24 @@ int b2g_get_joints(b2g_context* ctx, int first, int count, float* out);
  * the reference angle keep their creation values. A changed limit switch or
  * limit range drops the limit impulse, as those setters do. */
 int b2g_set_joint(b2g_context* ctx, int joint, const b2g_revolute_def* def);
+
+/* b2DistanceJointDef (Box2D/Dynamics/Joints/b2DistanceJoint.h:32-63) with
+ * body indices: a rod (frequencyHz 0) or a soft spring-damper. 40 bytes. */
+typedef struct b2g_distance_def {
+    int32_t bodyA, bodyB;
+    float localAnchorA[2];
+    float localAnchorB[2];
+    float length, frequencyHz, dampingRatio;
+    int32_t collideConnected;
+} b2g_distance_def;
+/* Appends distance joints to the same joint table as the revolute ones;
+ * b2g_get_joints then reports the impulse along the rod and the rod's unit
+ * direction of the last step (impulse, u.x, u.y, 0).
+ * They run with the coloured solver (b2DistanceJoint,
+ * REF/Dynamics/Joints/b2DistanceJoint.cpp:63-221). */
+int b2g_add_distance_joints(b2g_context* ctx, const b2g_distance_def* defs, int count, int* first);
+/* b2DistanceJoint::SetLength / SetFrequency / SetDampingRatio. */
+int b2g_set_distance_joint(b2g_context* ctx, int joint, float length, float frequencyHz, float dampingRatio);
 /* Removes joints (b2World::DestroyJoint, Box2D/Dynamics/b2World.cpp:267):
  * they stop constraining and connecting islands at the next step; indices of
  * the other joints do not change. */
@@ -525,7 +543,8 @@ int b2g_get_graph_counts(b2g_context* ctx, int64_t* launches, int64_t* captures)
 /* Sizes in bytes of the ABI structs, in this order: b2g_shape,
  * b2g_manifold, b2g_body_state, b2g_fixture_def, b2g_world_def,
  * b2g_step_params, b2g_step_stats, b2g_phase_profile, b2g_mlp_params,
- * b2g_aabb, b2g_transform, b2g_revolute_def, b2g_schedule, b2g_sweep.
+ * b2g_aabb, b2g_transform, b2g_revolute_def, b2g_schedule, b2g_sweep,
+ * b2g_distance_def.
  * Returns the count (out may be null). Needs no device; lets bindings check
  * their struct layouts. */
 int b2g_abi_sizes(int32_t* out, int cap);

@@ -348,6 +348,11 @@ protected:
     friend class b2World;
     friend class b2Body;
     b2Joint(const b2JointDef* def, b2World* world, int32_t index);
+    void MarkDirty();  // parameters to upload before the next Step
+    // device state of the last Step: revolute impulse x, y, limit, motor;
+    // distance impulse and direction u
+    mutable float m_impulse[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    bool m_dirty = false;
     b2JointType m_type;
     b2Joint* m_prev = nullptr;
     b2Joint* m_next = nullptr;
@@ -388,15 +393,48 @@ public:
 private:
     friend class b2World;
     b2RevoluteJoint(const b2RevoluteJointDef* def, b2World* world, int32_t index);
-    void MarkDirty();  // switches to upload before the next Step
     b2Vec2 m_localAnchorA, m_localAnchorB;
     float m_referenceAngle;
     bool m_enableLimit;
     float m_lowerAngle, m_upperAngle;
     bool m_enableMotor;
     float m_motorSpeed, m_maxMotorTorque;
-    mutable float m_impulse[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // x, y, limit, motor
-    bool m_dirty = false;
+};
+
+// Box2D/Dynamics/Joints/b2DistanceJoint.h:32-63
+struct b2DistanceJointDef : public b2JointDef {
+    b2DistanceJointDef() { type = b2JointType::DISTANCE_JOINT; }
+    // Bodies, local anchors and length from two world anchor points.
+    void Initialize(b2Body* bodyA, b2Body* bodyB, const b2Vec2& anchorA, const b2Vec2& anchorB);
+    b2Vec2 localAnchorA{0.0f, 0.0f};
+    b2Vec2 localAnchorB{0.0f, 0.0f};
+    float length = 1.0f;
+    float frequencyHz = 0.0f;   // 0: a rigid rod
+    float dampingRatio = 0.0f;
+};
+
+// Box2D/Dynamics/Joints/b2DistanceJoint.h:65-130: keeps two anchor points at
+// a distance, rigidly or as a spring-damper.
+class b2DistanceJoint : public b2Joint {
+public:
+    b2Vec2 GetAnchorA() const override;
+    b2Vec2 GetAnchorB() const override;
+    const b2Vec2& GetLocalAnchorA() const { return m_localAnchorA; }
+    const b2Vec2& GetLocalAnchorB() const { return m_localAnchorB; }
+    void SetLength(float length);
+    float GetLength() const { return m_length; }
+    void SetFrequency(float hz);
+    float GetFrequency() const { return m_frequencyHz; }
+    void SetDampingRatio(float ratio);
+    float GetDampingRatio() const { return m_dampingRatio; }
+    b2Vec2 GetReactionForce(float inv_dt) const override;
+    float GetReactionTorque(float inv_dt) const override;
+
+private:
+    friend class b2World;
+    b2DistanceJoint(const b2DistanceJointDef* def, b2World* world, int32_t index);
+    b2Vec2 m_localAnchorA, m_localAnchorB;
+    float m_length, m_frequencyHz, m_dampingRatio;
 };
 
 // Per-step solver options of the GPU path (no reference counterpart).
@@ -418,8 +456,8 @@ public:
     void SetContactListener(b2ContactListener* listener) { m_contactListener = listener; }
     b2Body* CreateBody(const b2BodyDef* def);
     void DestroyBody(b2Body* body);
-    // Box2D/Dynamics/b2World.h:85-92. Revolute joints only; other types
-    // return nullptr with GetLastError() set.
+    // Box2D/Dynamics/b2World.h:85-92. Revolute and distance joints; other
+    // types return nullptr with GetLastError() set.
     b2Joint* CreateJoint(const b2JointDef* def);
     void DestroyJoint(b2Joint* joint);
     b2Joint* GetJointList() { return m_jointList; }
@@ -476,7 +514,9 @@ public:
 private:
     friend class b2Body;
     friend class b2Fixture;
+    friend class b2Joint;
     friend class b2RevoluteJoint;
+    friend class b2DistanceJoint;
     int32_t Upload();
     void Download() const;
     void DownloadJoints() const;
@@ -518,7 +558,7 @@ private:
     std::vector<b2Joint*> m_joints;        // creation order = device index
     int32_t m_uploadedJoints = 0;
     std::vector<int32_t> m_removedJoints;  // device joints to drop at the next upload
-    std::vector<b2RevoluteJoint*> m_dirtyJoints;
+    std::vector<b2Joint*> m_dirtyJoints;
     mutable bool m_jointsStale = false;
 };
 

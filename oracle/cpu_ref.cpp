@@ -1269,6 +1269,11 @@ struct Joint {
     float maxMotorTorque, motorSpeed, lowerAngle, upperAngle;
     int limitState;
     bool removed;
+    // b2DistanceJoint (type 1), REF/Dynamics/Joints/b2DistanceJoint.h:88-113
+    int type;
+    float length, frequencyHz, dampingRatio;
+    V2 u;
+    float distMass, gamma, bias;
     // solver temporaries
     int indexA, indexB;
     V2 rA, rB, localCenterA, localCenterB;
@@ -1452,6 +1457,10 @@ struct World {
         j.maxMotorTorque = d.maxMotorTorque; j.motorSpeed = d.motorSpeed;
         j.enableLimit = d.enableLimit != 0; j.enableMotor = d.enableMotor != 0;
         j.limitState = kInactiveLimit;
+        j.type = d.type;
+        j.length = d.length; j.frequencyHz = d.frequencyHz; j.dampingRatio = d.dampingRatio;
+        j.u = mk(0.0f, 0.0f);
+        j.distMass = j.gamma = j.bias = 0.0f;
         const int ji = (int)joints.size();
         j.prev = -1;
         j.next = jointHead;
@@ -2029,6 +2038,10 @@ struct World {
         j.localCenterA = bA.localCenter; j.localCenterB = bB.localCenter;
         j.invMassA = bA.invMass; j.invMassB = bB.invMass;
         j.invIA = bA.invI; j.invIB = bB.invI;
+        if (j.type == 1) {
+            distanceInitVelocity(j, h, dtRatio);
+            return;
+        }
         const float aA = posA[j.indexA];
         V2 vA = velV[j.indexA]; float wA = velW[j.indexA];
         const float aB = posA[j.indexB];
@@ -2085,6 +2098,10 @@ struct World {
 
     // b2RevoluteJoint::SolveVelocityConstraints, REF b2RevoluteJoint.cpp:200-292
     void jointSolveVelocity(Joint& j, float h) {
+        if (j.type == 1) {
+            distanceSolveVelocity(j);
+            return;
+        }
         V2 vA = velV[j.indexA]; float wA = velW[j.indexA];
         V2 vB = velV[j.indexB]; float wB = velW[j.indexB];
         const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
@@ -2143,7 +2160,93 @@ struct World {
     }
 
     // b2RevoluteJoint::SolvePositionConstraints, REF b2RevoluteJoint.cpp:294-378
+    // b2DistanceJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2DistanceJoint.cpp:63-154
+    void distanceInitVelocity(Joint& j, float h, float dtRatio) {
+        const V2 cA = posC[j.indexA], cB = posC[j.indexB];
+        const float aA = posA[j.indexA], aB = posA[j.indexB];
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        j.rA = mul(qA, j.localAnchorA - j.localCenterA);
+        j.rB = mul(qB, j.localAnchorB - j.localCenterB);
+        j.u = cB + j.rB - cA - j.rA;
+        const float len = length(j.u);
+        if (len > kLinearSlop) j.u *= 1.0f / len;
+        else j.u = mk(0.0f, 0.0f);
+        const float crAu = cross(j.rA, j.u), crBu = cross(j.rB, j.u);
+        float invMass = j.invMassA + j.invIA * crAu * crAu + j.invMassB + j.invIB * crBu * crBu;
+        j.distMass = invMass != 0.0f ? 1.0f / invMass : 0.0f;
+        if (j.frequencyHz > 0.0f) {
+            const float C = len - j.length;
+            const float omega = 2.0f * kPi * j.frequencyHz;
+            const float d = 2.0f * j.distMass * j.dampingRatio * omega;
+            const float k = j.distMass * omega * omega;
+            j.gamma = h * (d + h * k);
+            j.gamma = j.gamma != 0.0f ? 1.0f / j.gamma : 0.0f;
+            j.bias = C * h * k * j.gamma;
+            invMass += j.gamma;
+            j.distMass = invMass != 0.0f ? 1.0f / invMass : 0.0f;
+        } else {
+            j.gamma = 0.0f;
+            j.bias = 0.0f;
+        }
+        if (warmStarting) {
+            j.impulse.x *= dtRatio;
+            const V2 P = j.impulse.x * j.u;
+            vA -= j.invMassA * P;
+            wA -= j.invIA * cross(j.rA, P);
+            vB += j.invMassB * P;
+            wB += j.invIB * cross(j.rB, P);
+        } else {
+            j.impulse.x = 0.0f;
+        }
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2DistanceJoint::SolveVelocityConstraints, REF b2DistanceJoint.cpp:156-181
+    void distanceSolveVelocity(Joint& j) {
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const V2 vpA = vA + cross(wA, j.rA);
+        const V2 vpB = vB + cross(wB, j.rB);
+        const float Cdot = dot(j.u, vpB - vpA);
+        const float impulse = -j.distMass * (Cdot + j.bias + j.gamma * j.impulse.x);
+        j.impulse.x += impulse;
+        const V2 P = impulse * j.u;
+        vA -= j.invMassA * P;
+        wA -= j.invIA * cross(j.rA, P);
+        vB += j.invMassB * P;
+        wB += j.invIB * cross(j.rB, P);
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2DistanceJoint::SolvePositionConstraints, REF b2DistanceJoint.cpp:183-221
+    bool distanceSolvePosition(Joint& j) {
+        if (j.frequencyHz > 0.0f) return true;
+        V2 cA = posC[j.indexA]; float aA = posA[j.indexA];
+        V2 cB = posC[j.indexB]; float aB = posA[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        const V2 rA = mul(qA, j.localAnchorA - j.localCenterA);
+        const V2 rB = mul(qB, j.localAnchorB - j.localCenterB);
+        V2 u = cB + rB - cA - rA;
+        const float len = normalize(u);
+        float C = len - j.length;
+        C = clampf(C, -kMaxLinearCorrection, kMaxLinearCorrection);
+        const float impulse = -j.distMass * C;
+        const V2 P = impulse * u;
+        cA -= j.invMassA * P;
+        aA -= j.invIA * cross(rA, P);
+        cB += j.invMassB * P;
+        aB += j.invIB * cross(rB, P);
+        posC[j.indexA] = cA; posA[j.indexA] = aA;
+        posC[j.indexB] = cB; posA[j.indexB] = aB;
+        return absf(C) < kLinearSlop;
+    }
+
     bool jointSolvePosition(Joint& j) {
+        if (j.type == 1) return distanceSolvePosition(j);
         V2 cA = posC[j.indexA]; float aA = posA[j.indexA];
         V2 cB = posC[j.indexB]; float aB = posA[j.indexB];
         float angularError = 0.0f, positionError = 0.0f;
@@ -2799,12 +2902,31 @@ int orc_world_destroy_joint(void* h, int joint) {
 }
 
 // Per joint in creation order: impulse x, y, z and motor impulse.
+// b2DistanceJoint::SetLength / SetFrequency / SetDampingRatio.
+int orc_world_set_distance(void* h, int joint, float length, float hz, float ratio) {
+    World* w = (World*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint].removed || w->joints[joint].type != 1)
+        return B2G_ERR_ARG;
+    Joint& j = w->joints[joint];
+    j.length = length;
+    j.frequencyHz = hz;
+    j.dampingRatio = ratio;
+    return B2G_OK;
+}
+
 void orc_world_get_joints(void* h, float* out) {
     World* w = (World*)h;
     for (size_t i = 0; i < w->joints.size(); ++i) {
         const Joint& j = w->joints[i];
         if (j.removed) {  // destroyed
             for (int k = 0; k < 4; ++k) out[4 * i + k] = 0.0f;
+            continue;
+        }
+        if (j.type == 1) {  // reaction force at inv_dt = 1: impulse along u
+            out[4 * i] = j.impulse.x * j.u.x;
+            out[4 * i + 1] = j.impulse.x * j.u.y;
+            out[4 * i + 2] = 0.0f;
+            out[4 * i + 3] = 0.0f;
             continue;
         }
         out[4 * i] = j.impulse.x;

@@ -67,7 +67,7 @@ struct RefWorld {
     b2World* world;
     std::vector<b2Body*> bodies;
     std::vector<b2Fixture*> fixtures;
-    std::vector<b2RevoluteJoint*> joints;
+    std::vector<b2Joint*> joints;
     Recorder recorder;
 };
 
@@ -77,6 +77,19 @@ struct RefWorld {
 int loadJoints(RefWorld* w, int base, const b2scene::JointDef* jd, int nj) {
     for (int k = 0; k < nj; ++k) {
         const b2scene::JointDef& d = jd[k];
+        if (d.type == 1) {
+            b2DistanceJointDef dd;
+            dd.bodyA = w->bodies[base + d.bodyA];
+            dd.bodyB = w->bodies[base + d.bodyB];
+            dd.collideConnected = d.collideConnected != 0;
+            dd.localAnchorA.Set(d.ax, d.ay);
+            dd.localAnchorB.Set(d.bx, d.by);
+            dd.length = d.length;
+            dd.frequencyHz = d.frequencyHz;
+            dd.dampingRatio = d.dampingRatio;
+            w->joints.push_back(w->world->CreateJoint(&dd));
+            continue;
+        }
         b2RevoluteJointDef def;
         def.bodyA = w->bodies[base + d.bodyA];
         def.bodyB = w->bodies[base + d.bodyB];
@@ -90,7 +103,7 @@ int loadJoints(RefWorld* w, int base, const b2scene::JointDef* jd, int nj) {
         def.enableMotor = d.enableMotor != 0;
         def.motorSpeed = d.motorSpeed;
         def.maxMotorTorque = d.maxMotorTorque;
-        w->joints.push_back((b2RevoluteJoint*)w->world->CreateJoint(&def));
+        w->joints.push_back(w->world->CreateJoint(&def));
     }
     return 0;
 }
@@ -323,12 +336,24 @@ int ref_world_set_joint(void* h, int joint, int enableLimit, float lower, float 
                         float maxTorque) {
     RefWorld* w = (RefWorld*)h;
     if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
-    b2RevoluteJoint* j = w->joints[joint];
+    if (w->joints[joint]->GetType() != e_revoluteJoint) return -1;
+    b2RevoluteJoint* j = static_cast<b2RevoluteJoint*>(w->joints[joint]);
     j->EnableLimit(enableLimit != 0);
     j->SetLimits(lower, upper);
     j->EnableMotor(enableMotor != 0);
     j->SetMotorSpeed(speed);
     j->SetMaxMotorTorque(maxTorque);
+    return 0;
+}
+
+int ref_world_set_distance(void* h, int joint, float length, float hz, float ratio) {
+    RefWorld* w = (RefWorld*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
+    if (w->joints[joint]->GetType() != e_distanceJoint) return -1;
+    b2DistanceJoint* j = static_cast<b2DistanceJoint*>(w->joints[joint]);
+    j->SetLength(length);
+    j->SetFrequency(hz);
+    j->SetDampingRatio(ratio);
     return 0;
 }
 
@@ -344,11 +369,19 @@ int ref_world_destroy_joint(void* h, int joint) {
 void ref_world_get_joints(void* h, float* out) {
     RefWorld* w = (RefWorld*)h;
     for (size_t i = 0; i < w->joints.size(); ++i) {
-        const b2RevoluteJoint* j = w->joints[i];
-        if (j == nullptr) {  // destroyed
+        if (w->joints[i] == nullptr) {  // destroyed
             for (int k = 0; k < 4; ++k) out[4 * i + k] = 0.0f;
             continue;
         }
+        if (w->joints[i]->GetType() == e_distanceJoint) {
+            const b2Vec2 f = w->joints[i]->GetReactionForce(1.0f);
+            out[4 * i] = f.x;
+            out[4 * i + 1] = f.y;
+            out[4 * i + 2] = 0.0f;
+            out[4 * i + 3] = 0.0f;
+            continue;
+        }
+        const b2RevoluteJoint* j = static_cast<const b2RevoluteJoint*>(w->joints[i]);
         out[4 * i] = j->m_impulse.x;
         out[4 * i + 1] = j->m_impulse.y;
         out[4 * i + 2] = j->m_impulse.z;

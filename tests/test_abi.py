@@ -36,7 +36,8 @@ def test_struct_layouts_match_the_library():
     n = lib.b2g_abi_sizes(None, 0)
     sizes = np.zeros(n, np.int32)
     lib.b2g_abi_sizes(E.ptr(sizes), n)
-    shape, manifold, body, fixture, world, params, stats, profile, mlp, aabb, xf, revolute, schedule, sweep = sizes[:14]
+    shape, manifold, body, fixture, world, params, stats, profile, mlp, aabb, xf, revolute, schedule, sweep, distance = \
+        sizes[:15]
     assert E.SHAPE.itemsize == shape
     assert E.MANIFOLD.itemsize == manifold
     assert E.STATS.itemsize == stats
@@ -50,4 +51,5 @@ def test_struct_layouts_match_the_library():
     assert revolute == 56  # 2 body indices, 9 floats, 3 switches
     assert schedule == E.SCHEDULE.itemsize
     assert sweep == E.SWEEP.itemsize == 36
+    assert distance == 40  # 2 body indices, 7 floats, 1 switch
     assert params == E.STEP_PARAMS.itemsize

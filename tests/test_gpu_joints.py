@@ -252,3 +252,47 @@ def test_stress_worlds_small(gpu, orc):
     assert pen[2] > 0 and pen[0] < 0.1
     g.close()
     o.close()
+
+
+def test_distance_joints_bitexact(gpu, orc, wide_trig):
+    """Distance joints, one per island (rods and springs from the ground):
+    the coloured solver's order is the reference's there, so bodies and the
+    joints' reaction forces equal the oracle's bit for bit, also after
+    SetLength / SetFrequency / SetDampingRatio through the box2d:: API."""
+    g = gpu.world("distance", 16, 1, sleep=True)
+    o = orc.world("distance", 16, 1, sleep=True)
+    for s in range(300):
+        if s == 150:
+            for w in (g, o):
+                assert w.e.fn("world_set_distance")(w.h, 0, F(5.0), F(1.0), F(0.5)) == 0
+                assert w.e.fn("world_set_distance")(w.h, 1, F(6.0), F(0.0), F(0.0)) == 0
+                assert w.e.fn("world_set_distance")(w.h, 2, F(3.0), F(0.0), F(0.0)) == 0
+        g.step(1)
+        o.step(1)
+        if s % 30 == 29 or s < 3:
+            gb, ob = g.bodies(), o.bodies()
+            bad = np.where(~np.all(_bits(gb[:, :10]) == _bits(ob[:, :10]), axis=1))[0]
+            assert len(bad) == 0, f"step {s + 1}: bodies {bad[:8]} differ"
+    assert np.array_equal(_bits(_joints(g)), _bits(_joints(o)))
+    assert np.abs(_joints(g)[:, :2]).max() > 0.05  # the joints carry load
+    g.close()
+    o.close()
+
+
+def test_distance_rods_hold(gpu):
+    """Boxes on rods hanging from rods (two joints per island, solved by
+    colour): every rod keeps its length within 2 cm, as b2g_joint_drift
+    measures it (springs are left out of that measure)."""
+    lib = _b2g(gpu)
+    g = gpu.world("distance", 16, 0, sleep=False)
+    ctx = g.device_context()
+    out = np.zeros(3, np.float64)
+    worst = 0.0
+    for _ in range(10):
+        g.step(30)
+        assert lib.b2g_joint_drift(ctx, E.ptr(out)) == 0
+        assert out[2] == 8 + 16  # rods from the ground (even i) and the hanging rods
+        worst = max(worst, out[1])
+    assert np.all(np.isfinite(g.bodies()))
+    assert worst < 0.02, worst
+    g.close()

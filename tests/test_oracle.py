@@ -29,6 +29,8 @@ F = ctypes.c_float
     ("stress", 1000, 3, 200, False, 8, 3),     # pile dropped on a mat of 20 chains of 32 links
     ("plank", 100, 0, 200, True, 8, 3),        # one body in 101 contacts
     ("terrain", 60, 1, 400, True, 8, 3),       # chain shapes: open chains with and without ghosts, a loop
+    ("distance", 16, 0, 400, True, 8, 3),      # distance joints: rods and springs, a second box on a rod
+    ("distance", 12, 1, 300, False, 8, 3),
     ("terrain", 100, 2, 300, False, 8, 3),
 ])
 def test_world_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, vel, pos):
@@ -547,3 +549,51 @@ def test_chain_children(ref, orc):
     meta, _ = b.contacts()
     assert np.any((meta[:, 0] < 16) & (meta[:, 2] & 1 == 1))  # bodies rest on chain children
     a.close(); b.close()
+
+
+def test_distance_joint_edits_bitexact(ref, orc):
+    """b2DistanceJoint::SetLength, SetFrequency and SetDampingRatio mid-run
+    (a rod made a spring, a spring made a rod, a rod shortened) against the
+    reference's own setters, every step, with the joints' reaction forces."""
+    a = ref.world("distance", 8, 1, sleep=True)
+    b = orc.world("distance", 8, 1, sleep=True)
+    a.step(80); b.step(80)
+    for w in (a, b):
+        assert w.e.fn("world_set_distance")(w.h, 0, F(5.0), F(1.0), F(0.5)) == 0
+        assert w.e.fn("world_set_distance")(w.h, 1, F(6.0), F(0.0), F(0.0)) == 0
+        assert w.e.fn("world_set_distance")(w.h, 2, F(3.0), F(0.0), F(0.0)) == 0
+    for s in range(200):
+        a.step(1); b.step(1)
+        assert np.array_equal(a.bodies().view(np.uint32), b.bodies().view(np.uint32)), f"step {s + 81}"
+    ja, jb = np.zeros(32, np.float32), np.zeros(32, np.float32)
+    a.e.fn("world_get_joints")(a.h, E.ptr(ja))
+    b.e.fn("world_get_joints")(b.h, E.ptr(jb))
+    assert np.array_equal(ja.view(np.uint32), jb.view(np.uint32))
+    a.close(); b.close()
+
+
+@pytest.mark.parametrize("hz", [1.0, 2.0])
+def test_distance_spring_closed_form(orc, hz):
+    """A box on an undamped distance spring without gravity oscillates along
+    the spring at the frequency it was given: the period between upward
+    crossings of the rest position is 1 / hz within 3%; with damping ratio 1
+    (critical) it creeps back without crossing."""
+    w = orc.world("spring", int(round(10 * hz)), 0, gravity=(0.0, 0.0))
+    ys, t = [], []
+    for s in range(300):
+        w.step(1)
+        ys.append(float(w.bodies()[1, E.CY]))
+        t.append((s + 1) / 60.0)
+    y = np.array(ys) - 7.0  # rest: 3 m below the anchor at y = 10
+    up = [t[i] + (t[i + 1] - t[i]) * (-y[i]) / (y[i + 1] - y[i]) for i in range(len(y) - 1) if y[i] < 0 <= y[i + 1]]
+    assert len(up) >= 3
+    period = np.diff(up).mean()
+    assert abs(period - 1.0 / hz) < 0.03 / hz, (period, hz)
+    w.close()
+    c = orc.world("spring", int(round(10 * hz)), 10, gravity=(0.0, 0.0))
+    yc = []
+    for _ in range(300):
+        c.step(1)
+        yc.append(float(c.bodies()[1, E.CY]) - 7.0)
+    assert max(yc) < 1e-3 and yc[-1] < 0.0 and yc[-1] > -0.05, (max(yc), yc[-1])
+    c.close()
