@@ -106,7 +106,7 @@ struct ConView {
 enum : int {
     JF_LIMIT = 1, JF_MOTOR = 2, JF_COLLIDE = 4, JF_REMOVED = 8,
     JL_INACTIVE = 0, JL_LOWER = 1, JL_UPPER = 2, JL_EQUAL = 3,  // limit state in bits 8-9
-    JT_REVOLUTE = 0, JT_DISTANCE = 1,                            // joint type in bits 12-15
+    JT_REVOLUTE = 0, JT_DISTANCE = 1, JT_ROPE = 2, JT_WELD = 3,  // joint type in bits 12-15
 };
 __host__ __device__ inline int jointType(int flags) { return (flags >> 12) & 15; }
 struct JointView {
@@ -128,7 +128,11 @@ struct JointView {
 // Per-type use of the arrays. Revolute: limits = referenceAngle, lower,
 // upper, maxMotorTorque; impulse = (x, y, limit, motor). Distance: limits =
 // length, frequencyHz, dampingRatio; impulse.x; K0 = (u.x, u.y, mass, bias),
-// K1.x = gamma.
+// K1.x = gamma. Rope: limits.x = maxLength; impulse.x; K0 = (u.x, u.y, mass,
+// length of the step); impulse = (impulse, u.x, u.y, limit state), the state
+// also in flags bits 8-9. Weld: limits =
+// referenceAngle, frequencyHz, dampingRatio; impulse x y z; K0, K1 = the
+// effective mass matrix; motorMass = gamma, motorSpeed = bias.
 
 __device__ inline int encodeOrdered(float f) {
     int i = __float_as_int(f);
@@ -1418,12 +1422,328 @@ __device__ inline bool distanceSolvePositionOne(const WorldView& w, const JointV
     return absB(C) < kLinearSlop;
 }
 
+// b2RopeJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2RopeJoint.cpp:48-126
+__device__ inline void ropeInitOne(const WorldView& w, const JointView& jv, int j, int2 b, float dtRatio, int warm) {
+    const float4 msA = w.mass[b.x], msB = w.mass[b.y];
+    const float mA = msA.x, iA = msA.y, mB = msB.x, iB = msB.y;
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const Rot qA = rotOf(pA.z), qB = rotOf(pB.z);
+    const float4 an = jv.anchors[j];
+    const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    V2 u = mk(pB.x, pB.y) + rB - mk(pA.x, pA.y) - rA;
+    const float length = lengthOf(u);
+    const int state = length - jv.limits[j].x > 0.0f ? JL_UPPER : JL_INACTIVE;
+    jv.flags[j] = (jv.flags[j] & ~0x300) | (state << 8);
+    jv.r[j] = make_float4(rA.x, rA.y, rB.x, rB.y);
+    jv.masses[j] = make_float4(mA, iA, mB, iB);
+    float4 imp = jv.impulse[j];
+    if (!(length > kLinearSlop)) {
+        // anchors together: no direction, no mass, no impulse this step
+        jv.impulse[j] = make_float4(0.0f, 0.0f, 0.0f, (float)state);
+        jv.K0[j] = make_float4(0.0f, 0.0f, 0.0f, length);
+        return;
+    }
+    u *= 1.0f / length;
+    const float crA = cross(rA, u), crB = cross(rB, u);
+    const float invMass = mA + iA * crA * crA + mB + iB * crB * crB;
+    const float mass = invMass != 0.0f ? 1.0f / invMass : 0.0f;
+    if (warm) {
+        imp.x *= dtRatio;
+        const V2 P = imp.x * u;
+        vA -= mA * P;
+        wA -= iA * cross(rA, P);
+        vB += mB * P;
+        wB += iB * cross(rB, P);
+    } else {
+        imp.x = 0.0f;
+    }
+    // u for b2RopeJoint::GetReactionForce, the state for GetLimitState
+    jv.impulse[j] = make_float4(imp.x, u.x, u.y, (float)state);
+    jv.K0[j] = make_float4(u.x, u.y, mass, length);
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2RopeJoint::SolveVelocityConstraints, REF b2RopeJoint.cpp:128-162: a
+// one-sided impulse, predictive while the rope is slack.
+__device__ inline void ropeSolveVelocityOne(const WorldView& w, const JointView& jv, int j, int2 b, float h) {
+    const float4 ms = jv.masses[j];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const float4 r = jv.r[j];
+    const V2 rA = mk(r.x, r.y), rB = mk(r.z, r.w);
+    const float4 k0 = jv.K0[j];
+    const V2 u = mk(k0.x, k0.y);
+    const V2 vpA = vA + cross(wA, rA);
+    const V2 vpB = vB + cross(wB, rB);
+    const float C = k0.w - jv.limits[j].x;
+    float Cdot = dot(u, vpB - vpA);
+    if (C < 0.0f) Cdot += (1.0f / h) * C;
+    float impulse = -k0.z * Cdot;
+    float4 imp = jv.impulse[j];
+    const float oldImpulse = imp.x;
+    const float sum = imp.x + impulse;
+    imp.x = 0.0f < sum ? 0.0f : sum;  // b2Min(0, sum)
+    impulse = imp.x - oldImpulse;
+    const V2 P = impulse * u;
+    vA -= mA * P;
+    wA -= iA * cross(rA, P);
+    vB += mB * P;
+    wB += iB * cross(rB, P);
+    jv.impulse[j] = imp;
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2RopeJoint::SolvePositionConstraints, REF b2RopeJoint.cpp:164-196
+__device__ inline bool ropeSolvePositionOne(const WorldView& w, const JointView& jv, int j, int2 b) {
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    V2 cA = mk(pA.x, pA.y), cB = mk(pB.x, pB.y);
+    float aA = pA.z, aB = pB.z;
+    const float4 ms = jv.masses[j];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const Rot qA = rotOf(aA), qB = rotOf(aB);
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float4 an = jv.anchors[j];
+    const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    V2 u = cB + rB - cA - rA;
+    const float length = normalizeV(u);
+    const float maxLength = jv.limits[j].x;
+    const float C = clampB(length - maxLength, 0.0f, kMaxLinearCorrection);
+    const float impulse = -jv.K0[j].z * C;
+    const V2 P = impulse * u;
+    cA -= mA * P;
+    aA -= iA * cross(rA, P);
+    cB += mB * P;
+    aB += iB * cross(rB, P);
+    if (w.btype[b.x] == BT_DYNAMIC) w.pos[b.x] = make_float4(cA.x, cA.y, aA, pA.w);
+    if (w.btype[b.y] == BT_DYNAMIC) w.pos[b.y] = make_float4(cB.x, cB.y, aB, pB.w);
+    return length - maxLength < kLinearSlop;
+}
+
+// The weld's 3x3 K (REF/Dynamics/Joints/b2WeldJoint.cpp:95-104; the same
+// matrix b2RevoluteJoint builds).
+__device__ inline Sym33 weldK(float mA, float iA, float mB, float iB, V2 rA, V2 rB) {
+    Sym33 K;
+    K.m11 = mA + mB + rA.y * rA.y * iA + rB.y * rB.y * iB;
+    K.m12 = -rA.y * rA.x * iA - rB.y * rB.x * iB;
+    K.m13 = -rA.y * iA - rB.y * iB;
+    K.m22 = mA + mB + rA.x * rA.x * iA + rB.x * rB.x * iB;
+    K.m23 = rA.x * iA + rB.x * iB;
+    K.m33 = iA + iB;
+    return K;
+}
+
+// b2Mat33::GetInverse22 (REF/Common/b2Math.cpp:58-71): the inverse of the
+// upper-left 2x2 block, zeros in the third row and column.
+__device__ inline Sym33 inverse22(const Sym33& K) {
+    const float a = K.m11, bb = K.m12, c = K.m12, d = K.m22;
+    float det = a * d - bb * c;
+    if (det != 0.0f) det = 1.0f / det;
+    return Sym33{det * d, -det * c, 0.0f, det * a, 0.0f, 0.0f};
+}
+
+// b2Mat33::GetSymInverse33 (REF/Common/b2Math.cpp:73-97)
+__device__ inline Sym33 symInverse33(const Sym33& K) {
+    const float c1x = K.m22 * K.m33 - K.m23 * K.m23, c1y = K.m23 * K.m13 - K.m12 * K.m33,
+                c1z = K.m12 * K.m23 - K.m22 * K.m13;
+    float det = dot3(K.m11, K.m12, K.m13, c1x, c1y, c1z);
+    if (det != 0.0f) det = 1.0f / det;
+    const float a11 = K.m11, a12 = K.m12, a13 = K.m13, a22 = K.m22, a23 = K.m23, a33 = K.m33;
+    Sym33 M;
+    M.m11 = det * (a22 * a33 - a23 * a23);
+    M.m12 = det * (a13 * a23 - a12 * a33);
+    M.m13 = det * (a12 * a23 - a13 * a22);
+    M.m22 = det * (a11 * a33 - a13 * a13);
+    M.m23 = det * (a13 * a12 - a11 * a23);
+    M.m33 = det * (a11 * a22 - a12 * a12);
+    return M;
+}
+
+// b2WeldJoint::InitVelocityConstraints, REF b2WeldJoint.cpp:60-169
+__device__ inline void weldInitOne(const WorldView& w, const JointView& jv, int j, int2 b, float dtRatio, int warm) {
+    const float4 msA = w.mass[b.x], msB = w.mass[b.y];
+    const float mA = msA.x, iA = msA.y, mB = msB.x, iB = msB.y;
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float aA = w.pos[b.x].z, aB = w.pos[b.y].z;
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const Rot qA = rotOf(aA), qB = rotOf(aB);
+    const float4 an = jv.anchors[j];
+    const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    const Sym33 K = weldK(mA, iA, mB, iB, rA, rB);
+    const float4 par = jv.limits[j];  // referenceAngle, frequencyHz, dampingRatio
+    Sym33 M;
+    float gamma = 0.0f, bias = 0.0f;
+    if (par.y > 0.0f) {
+        M = inverse22(K);
+        float invM = iA + iB;
+        const float m = invM > 0.0f ? 1.0f / invM : 0.0f;
+        const float C = aB - aA - par.x;
+        const float omega = 2.0f * kPi * par.y;
+        const float d = 2.0f * m * par.z * omega;
+        const float k = m * omega * omega;
+        const float h = jv.h;
+        gamma = h * (d + h * k);
+        gamma = gamma != 0.0f ? 1.0f / gamma : 0.0f;
+        bias = C * h * k * gamma;
+        invM += gamma;
+        M.m33 = invM != 0.0f ? 1.0f / invM : 0.0f;
+    } else if (K.m33 == 0.0f) {
+        M = inverse22(K);
+    } else {
+        M = symInverse33(K);
+    }
+    float4 imp = jv.impulse[j];
+    if (warm) {
+        imp.x *= dtRatio;
+        imp.y *= dtRatio;
+        imp.z *= dtRatio;
+        const V2 P = mk(imp.x, imp.y);
+        vA -= mA * P;
+        wA -= iA * (cross(rA, P) + imp.z);
+        vB += mB * P;
+        wB += iB * (cross(rB, P) + imp.z);
+    } else {
+        imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    jv.impulse[j] = imp;
+    jv.r[j] = make_float4(rA.x, rA.y, rB.x, rB.y);
+    jv.masses[j] = make_float4(mA, iA, mB, iB);
+    jv.K0[j] = make_float4(M.m11, M.m12, M.m13, M.m22);
+    jv.K1[j] = make_float2(M.m23, M.m33);
+    jv.motorMass[j] = gamma;
+    jv.motorSpeed[j] = bias;
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2WeldJoint::SolveVelocityConstraints, REF b2WeldJoint.cpp:171-227
+__device__ inline void weldSolveVelocityOne(const WorldView& w, const JointView& jv, int j, int2 b) {
+    const float4 ms = jv.masses[j];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const float4 r = jv.r[j];
+    const V2 rA = mk(r.x, r.y), rB = mk(r.z, r.w);
+    const float4 k0 = jv.K0[j];
+    const float2 k1 = jv.K1[j];
+    float4 imp = jv.impulse[j];
+    if (jv.limits[j].y > 0.0f) {
+        const float Cdot2 = wB - wA;
+        const float impulse2 = -k1.y * (Cdot2 + jv.motorSpeed[j] + jv.motorMass[j] * imp.z);
+        imp.z += impulse2;
+        wA -= iA * impulse2;
+        wB += iB * impulse2;
+        const V2 Cdot1 = vB + cross(wB, rB) - vA - cross(wA, rA);
+        // -b2Mul22(mass, Cdot1)
+        const V2 impulse1 = -mk(k0.x * Cdot1.x + k0.y * Cdot1.y, k0.y * Cdot1.x + k0.w * Cdot1.y);
+        imp.x += impulse1.x;
+        imp.y += impulse1.y;
+        const V2 P = impulse1;
+        vA -= mA * P;
+        wA -= iA * cross(rA, P);
+        vB += mB * P;
+        wB += iB * cross(rB, P);
+    } else {
+        const V2 Cdot1 = vB + cross(wB, rB) - vA - cross(wA, rA);
+        const float Cdot2 = wB - wA;
+        // -b2Mul(mass, Cdot): Cdot.x * ex + Cdot.y * ey + Cdot.z * ez
+        const float ix = -(Cdot1.x * k0.x + Cdot1.y * k0.y + Cdot2 * k0.z);
+        const float iy = -(Cdot1.x * k0.y + Cdot1.y * k0.w + Cdot2 * k1.x);
+        const float iz = -(Cdot1.x * k0.z + Cdot1.y * k1.x + Cdot2 * k1.y);
+        imp.x += ix;
+        imp.y += iy;
+        imp.z += iz;
+        const V2 P = mk(ix, iy);
+        vA -= mA * P;
+        wA -= iA * (cross(rA, P) + iz);
+        vB += mB * P;
+        wB += iB * (cross(rB, P) + iz);
+    }
+    jv.impulse[j] = imp;
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2WeldJoint::SolvePositionConstraints, REF b2WeldJoint.cpp:229-308: the
+// spring leaves the angle to the velocity pass.
+__device__ inline bool weldSolvePositionOne(const WorldView& w, const JointView& jv, int j, int2 b) {
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    V2 cA = mk(pA.x, pA.y), cB = mk(pB.x, pB.y);
+    float aA = pA.z, aB = pB.z;
+    const float4 ms = jv.masses[j];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const Rot qA = rotOf(aA), qB = rotOf(aB);
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float4 an = jv.anchors[j];
+    const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    const Sym33 K = weldK(mA, iA, mB, iB, rA, rB);
+    const float4 par = jv.limits[j];
+    float positionError, angularError;
+    const V2 C1 = cB + rB - cA - rA;
+    if (par.y > 0.0f) {
+        positionError = lengthOf(C1);
+        angularError = 0.0f;
+        const V2 P = -solve22(K, C1);
+        cA -= mA * P;
+        aA -= iA * cross(rA, P);
+        cB += mB * P;
+        aB += iB * cross(rB, P);
+    } else {
+        const float C2 = aB - aA - par.x;
+        positionError = lengthOf(C1);
+        angularError = absB(C2);
+        float ix, iy, iz;
+        if (K.m33 > 0.0f) {
+            solve33(K, C1.x, C1.y, C2, ix, iy, iz);
+            ix = -ix;
+            iy = -iy;
+            iz = -iz;
+        } else {
+            const V2 i2 = -solve22(K, C1);
+            ix = i2.x;
+            iy = i2.y;
+            iz = 0.0f;
+        }
+        const V2 P = mk(ix, iy);
+        cA -= mA * P;
+        aA -= iA * (cross(rA, P) + iz);
+        cB += mB * P;
+        aB += iB * (cross(rB, P) + iz);
+    }
+    if (w.btype[b.x] == BT_DYNAMIC) w.pos[b.x] = make_float4(cA.x, cA.y, aA, pA.w);
+    if (w.btype[b.y] == BT_DYNAMIC) w.pos[b.y] = make_float4(cB.x, cB.y, aB, pB.w);
+    return positionError <= kLinearSlop && angularError <= kAngularSlop;
+}
+
 // b2RevoluteJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2RevoluteJoint.cpp:66-198
 __device__ inline void jointInitOne(const WorldView& w, const JointView& jv, int j, float dtRatio, int warm) {
     const int2 b = jv.bodies[j];
     if (!jointActive(w, b)) return;
-    if (jointType(jv.flags[j]) == JT_DISTANCE) {
+    const int type = jointType(jv.flags[j]);
+    if (type == JT_DISTANCE) {
         distanceInitOne(w, jv, j, b, dtRatio, warm);
+        return;
+    }
+    if (type == JT_ROPE) {
+        ropeInitOne(w, jv, j, b, dtRatio, warm);
+        return;
+    }
+    if (type == JT_WELD) {
+        weldInitOne(w, jv, j, b, dtRatio, warm);
         return;
     }
     const float4 msA = w.mass[b.x], msB = w.mass[b.y];
@@ -1507,8 +1827,17 @@ __global__ void k_joint_init_serial(WorldView w, JointView jv, int begin, int en
 __device__ inline void jointSolveVelocityOne(const WorldView& w, const JointView& jv, int j, float h) {
     const int2 b = jv.bodies[j];
     if (!jointActive(w, b)) return;
-    if (jointType(jv.flags[j]) == JT_DISTANCE) {
+    const int type = jointType(jv.flags[j]);
+    if (type == JT_DISTANCE) {
         distanceSolveVelocityOne(w, jv, j, b);
+        return;
+    }
+    if (type == JT_ROPE) {
+        ropeSolveVelocityOne(w, jv, j, b, h);
+        return;
+    }
+    if (type == JT_WELD) {
+        weldSolveVelocityOne(w, jv, j, b);
         return;
     }
     const float4 ms = jv.masses[j];
@@ -1603,8 +1932,12 @@ __device__ inline void jointSolvePositionOne(const WorldView& w, const JointView
     if (!jointActive(w, b)) return;
     const int root = w.islandOf[w.btype[b.x] != BT_STATIC ? b.x : b.y];
     if (it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0) return;  // island done
-    if (jointType(jv.flags[j]) == JT_DISTANCE) {
-        if (!distanceSolvePositionOne(w, jv, j, b)) {
+    const int type = jointType(jv.flags[j]);
+    if (type != JT_REVOLUTE) {
+        const bool ok = type == JT_DISTANCE ? distanceSolvePositionOne(w, jv, j, b)
+                        : type == JT_ROPE   ? ropeSolvePositionOne(w, jv, j, b)
+                                            : weldSolvePositionOne(w, jv, j, b);
+        if (!ok) {
             int* f = &w.islandMinSep[it * w.nBodies + root];
             if (*f == 0) *f = 1;
         }
@@ -1709,12 +2042,17 @@ __global__ void k_joint_drift(WorldView w, JointView jv, float* out) {
     const V2 rA = mul(rotOf(pA.z), mk(an.x, an.y) - mk(lcA.x, lcA.y));
     const V2 rB = mul(rotOf(pB.z), mk(an.z, an.w) - mk(lcB.x, lcB.y));
     const float gap = lengthOf(mk(pB.x, pB.y) + rB - mk(pA.x, pA.y) - rA);
-    if (jointType(jv.flags[j]) == JT_DISTANCE) {
+    const int type = jointType(jv.flags[j]);
+    if (type == JT_DISTANCE) {
         // rods: the error in length; springs are meant to stretch
         out[j] = jv.limits[j].y > 0.0f ? -1.0f : absB(gap - jv.limits[j].x);
         return;
     }
-    out[j] = gap;
+    if (type == JT_ROPE) {  // stretch beyond the maximum length
+        out[j] = fmaxf(gap - jv.limits[j].x, 0.0f);
+        return;
+    }
+    out[j] = gap;  // revolute and weld: the anchors' separation
 }
 
 // Kinetic (0.5 m v.v + 0.5 I w^2) and potential (-m g.c) energy per body.
@@ -6282,16 +6620,13 @@ int b2g_abi_sizes(int32_t* out, int cap) {
         (int32_t)sizeof(b2g_step_stats),  (int32_t)sizeof(b2g_phase_profile), (int32_t)sizeof(b2g_mlp_params),
         (int32_t)sizeof(b2g_aabb),        (int32_t)sizeof(b2g_transform),    (int32_t)sizeof(b2g_revolute_def),
         (int32_t)sizeof(b2g_schedule),    (int32_t)sizeof(b2g_sweep),        (int32_t)sizeof(b2g_distance_def),
+        (int32_t)sizeof(b2g_rope_def),    (int32_t)sizeof(b2g_weld_def),
     };
     const int n = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (!out) return n;
     for (int i = 0; i < n && i < cap; ++i) out[i] = sizes[i];
     return n;
 }
-
-}  // extern "C"
-
-extern "C" {
 
 }  // extern "C"
 
@@ -6641,10 +6976,20 @@ int b2g_set_joint(b2g_context* ctx, int joint, const b2g_revolute_def* d) {
     return B2G_OK;
 }
 
-int b2g_add_distance_joints(b2g_context* ctx, const b2g_distance_def* defs, int count, int* first) {
-    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
-    for (int i = 0; i < count; ++i) {
-        const b2g_distance_def& d = defs[i];
+}  // extern "C"
+
+// One new row of the joint table: bodies, anchors, per-type parameters
+// (limits), motor speed and flags.
+struct JointRow {
+    int bodyA, bodyB;
+    float4 anchors, params;
+    int flags;
+};
+
+// Appends rows to the joint table (impulses start at zero).
+static int appendJointRows(b2g_context* ctx, const std::vector<JointRow>& rows, int* first) {
+    const int count = (int)rows.size();
+    for (const JointRow& d : rows) {
         if (d.bodyA < 0 || d.bodyB < 0 || d.bodyA >= ctx->nBodies || d.bodyB >= ctx->nBodies || d.bodyA == d.bodyB)
             return fail(ctx, B2G_ERR_ARG, "joint bodies out of range or equal");
     }
@@ -6658,15 +7003,16 @@ int b2g_add_distance_joints(b2g_context* ctx, const b2g_distance_def* defs, int 
     std::vector<int> flags(count);
     bool keepApart = false;
     for (int i = 0; i < count; ++i) {
-        const b2g_distance_def& d = defs[i];
+        const JointRow& d = rows[i];
         b[i] = make_int2(d.bodyA, d.bodyB);
-        anchors[i] = make_float4(d.localAnchorA[0], d.localAnchorA[1], d.localAnchorB[0], d.localAnchorB[1]);
-        params[i] = make_float4(d.length, d.frequencyHz, d.dampingRatio, 0.0f);
-        flags[i] = (JT_DISTANCE << 12) | (d.collideConnected ? JF_COLLIDE : 0);
+        anchors[i] = d.anchors;
+        params[i] = d.params;
+        flags[i] = d.flags;
+        const bool collide = (d.flags & JF_COLLIDE) != 0;
         ctx->hJointBodies.push_back(b[i]);
-        ctx->hJointKeepApart.push_back(d.collideConnected ? 0 : 1);
+        ctx->hJointKeepApart.push_back(collide ? 0 : 1);
         ctx->hJointRemoved.push_back(0);
-        keepApart = keepApart || !d.collideConnected;
+        keepApart = keepApart || !collide;
     }
     hipStream_t s = ctx->stream;
     const int n0 = ctx->nJoints;
@@ -6683,18 +7029,71 @@ int b2g_add_distance_joints(b2g_context* ctx, const b2g_distance_def* defs, int 
     return B2G_OK;
 }
 
-int b2g_set_distance_joint(b2g_context* ctx, int joint, float length, float frequencyHz, float dampingRatio) {
+// Replaces the parameters (limits row) of one joint of the given type.
+static int setJointParams(b2g_context* ctx, int joint, int type, const char* what, float4 par) {
     if (!ctx || joint < 0 || joint >= ctx->nJoints) return B2G_ERR_ARG;
     if (ctx->hJointRemoved[joint]) return fail(ctx, B2G_ERR_ARG, "joint was removed");
     hipStream_t s = ctx->stream;
     int flags = 0;
     HIPCHK(ctx, hipMemcpyAsync(&flags, ctx->jFlags + joint, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
-    if (jointType(flags) != JT_DISTANCE) return fail(ctx, B2G_ERR_ARG, "not a distance joint");
-    const float4 par = make_float4(length, frequencyHz, dampingRatio, 0.0f);
+    if (jointType(flags) != type) return fail(ctx, B2G_ERR_ARG, what);
     HIPCHK(ctx, hipMemcpyAsync(ctx->jLimits + joint, &par, sizeof(float4), hipMemcpyHostToDevice, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
     return B2G_OK;
+}
+
+template <class Def>
+static JointRow jointRow(const Def& d, int type, float4 params) {
+    return JointRow{d.bodyA, d.bodyB,
+                    make_float4(d.localAnchorA[0], d.localAnchorA[1], d.localAnchorB[0], d.localAnchorB[1]), params,
+                    (type << 12) | (d.collideConnected ? JF_COLLIDE : 0)};
+}
+
+extern "C" {
+
+int b2g_add_distance_joints(b2g_context* ctx, const b2g_distance_def* defs, int count, int* first) {
+    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
+    std::vector<JointRow> rows;
+    for (int i = 0; i < count; ++i)
+        rows.push_back(jointRow(defs[i], JT_DISTANCE,
+                                make_float4(defs[i].length, defs[i].frequencyHz, defs[i].dampingRatio, 0.0f)));
+    return appendJointRows(ctx, rows, first);
+}
+
+int b2g_add_rope_joints(b2g_context* ctx, const b2g_rope_def* defs, int count, int* first) {
+    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
+    std::vector<JointRow> rows;
+    for (int i = 0; i < count; ++i)
+        rows.push_back(jointRow(defs[i], JT_ROPE, make_float4(defs[i].maxLength, 0.0f, 0.0f, 0.0f)));
+    return appendJointRows(ctx, rows, first);
+}
+
+int b2g_set_rope_joint(b2g_context* ctx, int joint, float maxLength) {
+    return setJointParams(ctx, joint, JT_ROPE, "not a rope joint", make_float4(maxLength, 0.0f, 0.0f, 0.0f));
+}
+
+int b2g_add_weld_joints(b2g_context* ctx, const b2g_weld_def* defs, int count, int* first) {
+    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
+    std::vector<JointRow> rows;
+    for (int i = 0; i < count; ++i)
+        rows.push_back(jointRow(defs[i], JT_WELD,
+                                make_float4(defs[i].referenceAngle, defs[i].frequencyHz, defs[i].dampingRatio, 0.0f)));
+    return appendJointRows(ctx, rows, first);
+}
+
+int b2g_set_weld_joint(b2g_context* ctx, int joint, float frequencyHz, float dampingRatio) {
+    if (!ctx || joint < 0 || joint >= ctx->nJoints) return B2G_ERR_ARG;
+    if (ctx->hJointRemoved[joint]) return fail(ctx, B2G_ERR_ARG, "joint was removed");
+    float4 par;
+    HIPCHK(ctx, hipMemcpyAsync(&par, ctx->jLimits + joint, sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return setJointParams(ctx, joint, JT_WELD, "not a weld joint", make_float4(par.x, frequencyHz, dampingRatio, 0.0f));
+}
+
+int b2g_set_distance_joint(b2g_context* ctx, int joint, float length, float frequencyHz, float dampingRatio) {
+    return setJointParams(ctx, joint, JT_DISTANCE, "not a distance joint",
+                          make_float4(length, frequencyHz, dampingRatio, 0.0f));
 }
 
 int b2g_remove_joints(b2g_context* ctx, const int* ids, int count) {

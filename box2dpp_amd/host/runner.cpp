@@ -153,6 +153,30 @@ int loadJoints(Runner* r, int base, const b2scene::JointDef* jd, int nj) {
             r->joints.push_back(j);
             continue;
         }
+        if (d.type == 2 || d.type == 3) {
+            b2RopeJointDef rd;
+            b2WeldJointDef wd;
+            b2JointDef* def = &rd;
+            if (d.type == 2) {
+                rd.localAnchorA = b2Vec2(d.ax, d.ay);
+                rd.localAnchorB = b2Vec2(d.bx, d.by);
+                rd.maxLength = d.length;
+            } else {
+                wd.localAnchorA = b2Vec2(d.ax, d.ay);
+                wd.localAnchorB = b2Vec2(d.bx, d.by);
+                wd.referenceAngle = d.referenceAngle;
+                wd.frequencyHz = d.frequencyHz;
+                wd.dampingRatio = d.dampingRatio;
+                def = &wd;
+            }
+            def->bodyA = r->bodies[base + d.bodyA];
+            def->bodyB = r->bodies[base + d.bodyB];
+            def->collideConnected = d.collideConnected != 0;
+            b2Joint* j = r->world.CreateJoint(def);
+            if (j == nullptr) return B2G_ERR_ARG;
+            r->joints.push_back(j);
+            continue;
+        }
         b2RevoluteJointDef def;
         def.bodyA = r->bodies[base + d.bodyA];
         def.bodyB = r->bodies[base + d.bodyB];
@@ -248,6 +272,25 @@ int b2pp_world_set_distance(void* h, int joint, float length, float hz, float ra
     if (r->joints[joint]->GetType() != b2JointType::DISTANCE_JOINT) return B2G_ERR_ARG;
     b2DistanceJoint* j = static_cast<b2DistanceJoint*>(r->joints[joint]);
     j->SetLength(length);
+    j->SetFrequency(hz);
+    j->SetDampingRatio(ratio);
+    return 0;
+}
+
+// b2RopeJoint::SetMaxLength and b2WeldJoint::SetFrequency / SetDampingRatio.
+int b2pp_world_set_rope(void* h, int joint, float maxLength) {
+    Runner* r = (Runner*)h;
+    if (joint < 0 || joint >= (int)r->joints.size() || r->joints[joint] == nullptr) return B2G_ERR_ARG;
+    if (r->joints[joint]->GetType() != b2JointType::ROPE_JOINT) return B2G_ERR_ARG;
+    static_cast<b2RopeJoint*>(r->joints[joint])->SetMaxLength(maxLength);
+    return 0;
+}
+
+int b2pp_world_set_weld(void* h, int joint, float hz, float ratio) {
+    Runner* r = (Runner*)h;
+    if (joint < 0 || joint >= (int)r->joints.size() || r->joints[joint] == nullptr) return B2G_ERR_ARG;
+    if (r->joints[joint]->GetType() != b2JointType::WELD_JOINT) return B2G_ERR_ARG;
+    b2WeldJoint* j = static_cast<b2WeldJoint*>(r->joints[joint]);
     j->SetFrequency(hz);
     j->SetDampingRatio(ratio);
     return 0;

@@ -503,6 +503,66 @@ void b2DistanceJoint::SetDampingRatio(float ratio) {
     MarkDirty();
 }
 
+b2RopeJoint::b2RopeJoint(const b2RopeJointDef* def, b2World* world, int32_t index)
+    : b2Joint(def, world, index), m_localAnchorA(def->localAnchorA), m_localAnchorB(def->localAnchorB),
+      m_maxLength(def->maxLength) {}
+
+b2Vec2 b2RopeJoint::GetAnchorA() const { return m_bodyA->GetWorldPoint(m_localAnchorA); }
+b2Vec2 b2RopeJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localAnchorB); }
+// REF/Dynamics/Joints/b2RopeJoint.cpp:208-218: the impulse along the rope's
+// direction u of the last step.
+b2Vec2 b2RopeJoint::GetReactionForce(float inv_dt) const {
+    m_world->DownloadJoints();
+    const float s = inv_dt * m_impulse[0];
+    return b2Vec2(s * m_impulse[1], s * m_impulse[2]);
+}
+float b2RopeJoint::GetReactionTorque(float inv_dt) const {
+    B2_NOT_USED(inv_dt);
+    return 0.0f;
+}
+// The state the last step's InitVelocityConstraints chose (taut or slack).
+b2LimitState b2RopeJoint::GetLimitState() const {
+    m_world->DownloadJoints();
+    return m_impulse[3] > 0.0f ? b2LimitState::AT_UPPER_LIMIT : b2LimitState::INACTIVE_LIMIT;
+}
+void b2RopeJoint::SetMaxLength(float length) {
+    m_maxLength = length;
+    MarkDirty();
+}
+
+// b2WeldJointDef::Initialize, REF/Dynamics/Joints/b2WeldJoint.cpp:39-46
+void b2WeldJointDef::Initialize(b2Body* bA, b2Body* bB, const b2Vec2& anchor) {
+    bodyA = bA;
+    bodyB = bB;
+    localAnchorA = bodyA->GetLocalPoint(anchor);
+    localAnchorB = bodyB->GetLocalPoint(anchor);
+    referenceAngle = bodyB->GetAngle() - bodyA->GetAngle();
+}
+
+b2WeldJoint::b2WeldJoint(const b2WeldJointDef* def, b2World* world, int32_t index)
+    : b2Joint(def, world, index), m_localAnchorA(def->localAnchorA), m_localAnchorB(def->localAnchorB),
+      m_referenceAngle(def->referenceAngle), m_frequencyHz(def->frequencyHz), m_dampingRatio(def->dampingRatio) {}
+
+b2Vec2 b2WeldJoint::GetAnchorA() const { return m_bodyA->GetWorldPoint(m_localAnchorA); }
+b2Vec2 b2WeldJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localAnchorB); }
+// REF/Dynamics/Joints/b2WeldJoint.cpp:320-329
+b2Vec2 b2WeldJoint::GetReactionForce(float inv_dt) const {
+    m_world->DownloadJoints();
+    return b2Vec2(inv_dt * m_impulse[0], inv_dt * m_impulse[1]);
+}
+float b2WeldJoint::GetReactionTorque(float inv_dt) const {
+    m_world->DownloadJoints();
+    return inv_dt * m_impulse[2];
+}
+void b2WeldJoint::SetFrequency(float hz) {
+    m_frequencyHz = hz;
+    MarkDirty();
+}
+void b2WeldJoint::SetDampingRatio(float ratio) {
+    m_dampingRatio = ratio;
+    MarkDirty();
+}
+
 // The setters wake both bodies and, where the limit changes, drop the limit
 // impulse (REF/Dynamics/Joints/b2RevoluteJoint.cpp:419-471).
 void b2RevoluteJoint::EnableMotor(bool flag) {
@@ -548,8 +608,9 @@ void b2RevoluteJoint::SetLimits(float lower, float upper) {
 // that a collideConnected = false joint filters out end at the next Step.
 b2Joint* b2World::CreateJoint(const b2JointDef* def) {
     if (IsLocked() || def == nullptr || def->bodyA == nullptr || def->bodyB == nullptr) return nullptr;
-    if (def->type != b2JointType::REVOLUTE_JOINT && def->type != b2JointType::DISTANCE_JOINT) {
-        m_lastError = "only revolute and distance joints run on the GPU path";
+    if (def->type != b2JointType::REVOLUTE_JOINT && def->type != b2JointType::DISTANCE_JOINT &&
+        def->type != b2JointType::ROPE_JOINT && def->type != b2JointType::WELD_JOINT) {
+        m_lastError = "only revolute, distance, rope and weld joints run on the GPU path";
         return nullptr;
     }
     if (def->bodyA == def->bodyB || def->bodyA->m_world != this || def->bodyB->m_world != this) return nullptr;
@@ -558,8 +619,13 @@ b2Joint* b2World::CreateJoint(const b2JointDef* def) {
         return nullptr;
     }
     b2Joint* j = nullptr;
+    const int32_t index = (int32_t)m_joints.size();
     if (def->type == b2JointType::DISTANCE_JOINT)
-        j = new b2DistanceJoint(static_cast<const b2DistanceJointDef*>(def), this, (int32_t)m_joints.size());
+        j = new b2DistanceJoint(static_cast<const b2DistanceJointDef*>(def), this, index);
+    else if (def->type == b2JointType::ROPE_JOINT)
+        j = new b2RopeJoint(static_cast<const b2RopeJointDef*>(def), this, index);
+    else if (def->type == b2JointType::WELD_JOINT)
+        j = new b2WeldJoint(static_cast<const b2WeldJointDef*>(def), this, index);
     else
         j = new b2RevoluteJoint(static_cast<const b2RevoluteJointDef*>(def), this, (int32_t)m_joints.size());
     m_joints.push_back(j);
@@ -640,6 +706,34 @@ static b2g_distance_def distanceDef(const b2DistanceJoint* d, int32_t bodyA, int
     r.localAnchorB[0] = d->GetLocalAnchorB()[0];
     r.localAnchorB[1] = d->GetLocalAnchorB()[1];
     r.length = d->GetLength();
+    r.frequencyHz = d->GetFrequency();
+    r.dampingRatio = d->GetDampingRatio();
+    r.collideConnected = d->GetCollideConnected() ? 1 : 0;
+    return r;
+}
+
+static b2g_rope_def ropeDef(const b2RopeJoint* d, int32_t bodyA, int32_t bodyB) {
+    b2g_rope_def r{};
+    r.bodyA = bodyA;
+    r.bodyB = bodyB;
+    r.localAnchorA[0] = d->GetLocalAnchorA()[0];
+    r.localAnchorA[1] = d->GetLocalAnchorA()[1];
+    r.localAnchorB[0] = d->GetLocalAnchorB()[0];
+    r.localAnchorB[1] = d->GetLocalAnchorB()[1];
+    r.maxLength = d->GetMaxLength();
+    r.collideConnected = d->GetCollideConnected() ? 1 : 0;
+    return r;
+}
+
+static b2g_weld_def weldDef(const b2WeldJoint* d, int32_t bodyA, int32_t bodyB) {
+    b2g_weld_def r{};
+    r.bodyA = bodyA;
+    r.bodyB = bodyB;
+    r.localAnchorA[0] = d->GetLocalAnchorA()[0];
+    r.localAnchorA[1] = d->GetLocalAnchorA()[1];
+    r.localAnchorB[0] = d->GetLocalAnchorB()[0];
+    r.localAnchorB[1] = d->GetLocalAnchorB()[1];
+    r.referenceAngle = d->GetReferenceAngle();
     r.frequencyHz = d->GetFrequency();
     r.dampingRatio = d->GetDampingRatio();
     r.collideConnected = d->GetCollideConnected() ? 1 : 0;
@@ -832,6 +926,20 @@ int32_t b2World::Upload() {
                     defs.push_back(distanceDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index));
                 }
                 rc = b2g_add_distance_joints(m_ctx, defs.data(), (int)defs.size(), &first);
+            } else if (type == b2JointType::ROPE_JOINT) {
+                std::vector<b2g_rope_def> defs;
+                for (int q = i; q < k; ++q) {
+                    const b2RopeJoint* d = static_cast<const b2RopeJoint*>(m_joints[q]);
+                    defs.push_back(ropeDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index));
+                }
+                rc = b2g_add_rope_joints(m_ctx, defs.data(), (int)defs.size(), &first);
+            } else if (type == b2JointType::WELD_JOINT) {
+                std::vector<b2g_weld_def> defs;
+                for (int q = i; q < k; ++q) {
+                    const b2WeldJoint* d = static_cast<const b2WeldJoint*>(m_joints[q]);
+                    defs.push_back(weldDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index));
+                }
+                rc = b2g_add_weld_joints(m_ctx, defs.data(), (int)defs.size(), &first);
             } else {
                 std::vector<b2g_revolute_def> defs;
                 for (int q = i; q < k; ++q) {
@@ -863,6 +971,11 @@ int32_t b2World::Upload() {
             if (j->m_type == b2JointType::DISTANCE_JOINT) {
                 const b2DistanceJoint* d = static_cast<const b2DistanceJoint*>(j);
                 rc = b2g_set_distance_joint(m_ctx, j->m_index, d->m_length, d->m_frequencyHz, d->m_dampingRatio);
+            } else if (j->m_type == b2JointType::ROPE_JOINT) {
+                rc = b2g_set_rope_joint(m_ctx, j->m_index, static_cast<const b2RopeJoint*>(j)->m_maxLength);
+            } else if (j->m_type == b2JointType::WELD_JOINT) {
+                const b2WeldJoint* d = static_cast<const b2WeldJoint*>(j);
+                rc = b2g_set_weld_joint(m_ctx, j->m_index, d->m_frequencyHz, d->m_dampingRatio);
             } else {
                 const b2RevoluteJoint* r = static_cast<const b2RevoluteJoint*>(j);
                 const b2g_revolute_def d = revoluteDef(r, r->m_bodyA->m_index, r->m_bodyB->m_index);

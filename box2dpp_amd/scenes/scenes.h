@@ -67,8 +67,11 @@ struct JointDef {
     float lowerAngle, upperAngle;
     float maxMotorTorque, motorSpeed;
     int enableLimit, enableMotor, collideConnected;
-    // b2DistanceJointDef (Box2D/Dynamics/Joints/b2DistanceJoint.h:32-63) when type is 1
-    int type;              // 0 revolute, 1 distance
+    // b2DistanceJointDef (Box2D/Dynamics/Joints/b2DistanceJoint.h:32-63) when
+    // type is 1; b2RopeJointDef (b2RopeJoint.h:30-50, length = maxLength) when
+    // 2; b2WeldJointDef (b2WeldJoint.h:29-60, referenceAngle, frequencyHz,
+    // dampingRatio) when 3
+    int type;              // 0 revolute, 1 distance, 2 rope, 3 weld
     float length, frequencyHz, dampingRatio;
 };
 
@@ -508,6 +511,87 @@ inline Scene spring(float hz, float ratio) {
     return s;
 }
 
+// Rope joints: n boxes on ropes from anchors 16 m up, over a ground edge,
+// started with sideways speeds and 0-20% slack so the ropes go slack and
+// taut; every fourth box starts on its anchor (a rope of length zero), and
+// (when `chained`) a smaller box hangs from each on a second rope.
+inline Scene ropes(int n, bool chained = true) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-200.0f, 0.0f, 200.0f, 0.0f));
+    for (int i = 0; i < n; ++i) {
+        const float x = 6.0f * (float)i - 3.0f * (float)n;
+        const bool atAnchor = (i % 4) == 3;
+        const float ax = atAnchor ? x : x + 1.5f + 0.3f * (float)(i % 4), ay = atAnchor ? 16.0f : 12.0f;
+        BodyDef bd = defaultBody(2, ax, ay);
+        bd.vx = 3.0f * (float)(i % 3 - 1);
+        addBody(s, bd);
+        addFixture(s, boxFixture(0.4f, 0.4f, 1.0f, 0.6f));
+        const int a = (int)s.bodies.size() - 1;
+        JointDef up{};
+        up.type = 2;
+        up.bodyA = 0; up.bodyB = a;
+        up.ax = x; up.ay = 16.0f;
+        up.bx = atAnchor ? 0.0f : 0.4f; up.by = atAnchor ? 0.0f : 0.4f;
+        const float dx = (ax + up.bx) - x, dy = (ay + up.by) - 16.0f;
+        up.length = atAnchor ? 4.0f : sqrtf(dx * dx + dy * dy) * (1.0f + 0.1f * (float)(i % 3));
+        s.joints.push_back(up);
+        if (!chained) continue;
+        const float bx = ax + 0.5f, by = ay - 3.0f;
+        addBody(s, defaultBody(2, bx, by));
+        addFixture(s, boxFixture(0.3f, 0.3f, 1.0f, 0.6f));
+        JointDef low{};
+        low.type = 2;
+        low.bodyA = a; low.bodyB = a + 1;
+        low.length = 1.1f * sqrtf(0.25f + 9.0f);
+        s.joints.push_back(low);
+    }
+    return s;
+}
+
+// Weld joints: n cantilever beams of 4-7 boxes (1 x 0.25 m) welded end to
+// end from an anchor on the ground body, a box dropped on each (when
+// `drop`). Every third
+// beam is rigid, the others have soft welds (4 or 8 Hz, damping 0-0.6);
+// beams 4, 9, 14, ... start with a fixed-rotation link (the weld then has
+// no angular mass when rigid) and odd beams carry a small reference angle.
+// `single` makes every beam one link (one weld per island).
+inline Scene welds(int n, bool drop = true, bool single = false) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-200.0f, 0.0f, 200.0f, 0.0f));
+    for (int i = 0; i < n; ++i) {
+        const float x = 10.0f * (float)i - 5.0f * (float)n;
+        const int links = single ? 1 : 4 + i % 4;
+        const float hz = (i % 3 == 0) ? 0.0f : 4.0f + 4.0f * (float)(i % 2);
+        const float ratio = 0.3f * (float)(i % 3);
+        const float ref = (i & 1) ? 0.02f : 0.0f;
+        int prev = 0;
+        for (int k = 0; k < links; ++k) {
+            BodyDef bd = defaultBody(2, x + 0.5f + (float)k, 10.0f);
+            bd.fixedRotation = (k == 0 && i % 5 == 4) ? 1 : 0;
+            addBody(s, bd);
+            addFixture(s, boxFixture(0.5f, 0.125f, 1.0f, 0.6f));
+            const int b = (int)s.bodies.size() - 1;
+            JointDef j{};
+            j.type = 3;
+            j.bodyA = prev; j.bodyB = b;
+            if (prev == 0) { j.ax = x; j.ay = 10.0f; }
+            else { j.ax = 0.5f; j.ay = 0.0f; }
+            j.bx = -0.5f; j.by = 0.0f;
+            j.referenceAngle = k == 0 ? 0.0f : ref;
+            j.frequencyHz = hz;
+            j.dampingRatio = ratio;
+            s.joints.push_back(j);
+            prev = b;
+        }
+        if (!drop) continue;
+        addBody(s, defaultBody(2, x + (float)links - 0.5f, 13.0f));
+        addFixture(s, boxFixture(0.3f, 0.3f, 1.0f, 0.6f));
+    }
+    return s;
+}
+
 // Scene by name: "hello", "pyramid" (n = rows, 0 -> 20), "pile" (n bodies,
 // seed), "world" (one batched-scene world of n bodies, seed), "pendulums"
 // (n pendulums), "chains" (n chains of `seed` links), "stress" (pile of n
@@ -516,7 +600,9 @@ inline Scene spring(float hz, float ratio) {
 // (n fast bodies, seed), "terrain" (n bodies on chain shapes, seed),
 // "distance" (n rods and springs with boxes; seed 0 hangs a second box on
 // each), "spring" (one box on a spring of n / 10 Hz and damping ratio
-// seed / 10). Unknown names give an empty scene.
+// seed / 10), "ropes" (n boxes on ropes; seed 0 hangs a second box on each),
+// "welds" (n welded beams; seed 0 drops a box on each, seed 2 makes the
+// beams single links). Unknown names give an empty scene.
 inline Scene byName(const char* kind, int n, uint32_t seed) {
     const std::string k(kind ? kind : "");
     if (k == "hello") return hello();
@@ -532,6 +618,8 @@ inline Scene byName(const char* kind, int n, uint32_t seed) {
     if (k == "terrain") return terrain(n > 0 ? n : 60, seed);
     if (k == "distance") return distanceJoints(n > 0 ? n : 16, seed == 0);
     if (k == "spring") return spring(0.1f * (float)(n > 0 ? n : 10), 0.1f * (float)seed);
+    if (k == "ropes") return ropes(n > 0 ? n : 16, seed == 0);
+    if (k == "welds") return welds(n > 0 ? n : 12, seed == 0, seed == 2);
     return Scene();
 }
 

@@ -267,10 +267,13 @@ int b2g_get_fat_aabbs(b2g_context* ctx, b2g_aabb* out, int cap);
 /* Mean and max penetration over touching manifold points (from
  * b2WorldManifold separations), and the point count: out[0..2]. */
 int b2g_penetration(b2g_context* ctx, double* out);
-/* Joint drift: mean and max position error |cB + rB - cA - rA| over the
- * joints (the error b2RevoluteJoint::SolvePositionConstraints measures,
- * Box2D/Dynamics/Joints/b2RevoluteJoint.cpp:336-343), and the joint count.
- * out[3], FP64 sums on the host; no counterpart in the reference. */
+/* Joint drift: mean and max position error over the joints, and the count
+ * of joints measured: |cB + rB - cA - rA| for revolute and weld joints (the
+ * error b2RevoluteJoint::SolvePositionConstraints measures,
+ * Box2D/Dynamics/Joints/b2RevoluteJoint.cpp:336-343), the error in length
+ * for rigid distance joints, the stretch beyond the maximum length for rope
+ * joints; soft distance joints are not measured. out[3], FP64 sums on the
+ * host; no counterpart in the reference. */
 int b2g_joint_drift(b2g_context* ctx, double* out);
 /* Kinetic (0.5 m v.v + 0.5 I w^2) and gravitational potential (-m g.c)
  * energy summed over the bodies in FP64; out[2]. No counterpart in the
@@ -334,6 +337,38 @@ typedef struct b2g_distance_def {
 int b2g_add_distance_joints(b2g_context* ctx, const b2g_distance_def* defs, int count, int* first);
 /* b2DistanceJoint::SetLength / SetFrequency / SetDampingRatio. */
 int b2g_set_distance_joint(b2g_context* ctx, int joint, float length, float frequencyHz, float dampingRatio);
+
+/* b2RopeJointDef (Box2D/Dynamics/Joints/b2RopeJoint.h:30-50) with body
+ * indices: the anchors may come no further apart than maxLength. 32 bytes. */
+typedef struct b2g_rope_def {
+    int32_t bodyA, bodyB;
+    float localAnchorA[2];
+    float localAnchorB[2];
+    float maxLength;
+    int32_t collideConnected;
+} b2g_rope_def;
+/* Appends rope joints (b2RopeJoint, REF/Dynamics/Joints/b2RopeJoint.cpp:48-196)
+ * to the joint table; b2g_get_joints reports the impulse along the rope, the
+ * rope's unit direction of the last step and its limit state (impulse, u.x,
+ * u.y, state: 0 slack, 2 taut as b2LimitState counts). */
+int b2g_add_rope_joints(b2g_context* ctx, const b2g_rope_def* defs, int count, int* first);
+/* b2RopeJoint::SetMaxLength. */
+int b2g_set_rope_joint(b2g_context* ctx, int joint, float maxLength);
+
+/* b2WeldJointDef (Box2D/Dynamics/Joints/b2WeldJoint.h:29-60) with body
+ * indices: rigid (frequencyHz 0) or with a soft angular spring. 40 bytes. */
+typedef struct b2g_weld_def {
+    int32_t bodyA, bodyB;
+    float localAnchorA[2];
+    float localAnchorB[2];
+    float referenceAngle, frequencyHz, dampingRatio;
+    int32_t collideConnected;
+} b2g_weld_def;
+/* Appends weld joints (b2WeldJoint, REF/Dynamics/Joints/b2WeldJoint.cpp:60-308);
+ * b2g_get_joints reports the linear and angular impulse (x, y, z, 0). */
+int b2g_add_weld_joints(b2g_context* ctx, const b2g_weld_def* defs, int count, int* first);
+/* b2WeldJoint::SetFrequency / SetDampingRatio. */
+int b2g_set_weld_joint(b2g_context* ctx, int joint, float frequencyHz, float dampingRatio);
 /* Removes joints (b2World::DestroyJoint, Box2D/Dynamics/b2World.cpp:267):
  * they stop constraining and connecting islands at the next step; indices of
  * the other joints do not change. */
@@ -544,7 +579,7 @@ int b2g_get_graph_counts(b2g_context* ctx, int64_t* launches, int64_t* captures)
  * b2g_manifold, b2g_body_state, b2g_fixture_def, b2g_world_def,
  * b2g_step_params, b2g_step_stats, b2g_phase_profile, b2g_mlp_params,
  * b2g_aabb, b2g_transform, b2g_revolute_def, b2g_schedule, b2g_sweep,
- * b2g_distance_def.
+ * b2g_distance_def, b2g_rope_def, b2g_weld_def.
  * Returns the count (out may be null). Needs no device; lets bindings check
  * their struct layouts. */
 int b2g_abi_sizes(int32_t* out, int cap);

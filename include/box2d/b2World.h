@@ -437,6 +437,69 @@ private:
     float m_length, m_frequencyHz, m_dampingRatio;
 };
 
+// Box2D/Dynamics/Joints/b2RopeJoint.h:30-50
+struct b2RopeJointDef : public b2JointDef {
+    b2RopeJointDef() { type = b2JointType::ROPE_JOINT; }
+    b2Vec2 localAnchorA{-1.0f, 0.0f};
+    b2Vec2 localAnchorB{1.0f, 0.0f};
+    float maxLength = 0.0f;     // must exceed the linear slop to act
+};
+
+// Box2D/Dynamics/Joints/b2RopeJoint.h:60-122: bounds the distance between
+// two anchor points from above.
+class b2RopeJoint : public b2Joint {
+public:
+    b2Vec2 GetAnchorA() const override;
+    b2Vec2 GetAnchorB() const override;
+    const b2Vec2& GetLocalAnchorA() const { return m_localAnchorA; }
+    const b2Vec2& GetLocalAnchorB() const { return m_localAnchorB; }
+    void SetMaxLength(float length);
+    float GetMaxLength() const { return m_maxLength; }
+    b2LimitState GetLimitState() const;
+    b2Vec2 GetReactionForce(float inv_dt) const override;
+    float GetReactionTorque(float inv_dt) const override;
+
+private:
+    friend class b2World;
+    b2RopeJoint(const b2RopeJointDef* def, b2World* world, int32_t index);
+    b2Vec2 m_localAnchorA, m_localAnchorB;
+    float m_maxLength;
+};
+
+// Box2D/Dynamics/Joints/b2WeldJoint.h:29-60
+struct b2WeldJointDef : public b2JointDef {
+    b2WeldJointDef() { type = b2JointType::WELD_JOINT; }
+    // Bodies, local anchors and reference angle from a world anchor point.
+    void Initialize(b2Body* bodyA, b2Body* bodyB, const b2Vec2& anchor);
+    b2Vec2 localAnchorA{0.0f, 0.0f};
+    b2Vec2 localAnchorB{0.0f, 0.0f};
+    float referenceAngle = 0.0f;
+    float frequencyHz = 0.0f;   // 0: rigid; else a soft angular spring
+    float dampingRatio = 0.0f;
+};
+
+// Box2D/Dynamics/Joints/b2WeldJoint.h:64-146: glues two bodies together.
+class b2WeldJoint : public b2Joint {
+public:
+    b2Vec2 GetAnchorA() const override;
+    b2Vec2 GetAnchorB() const override;
+    const b2Vec2& GetLocalAnchorA() const { return m_localAnchorA; }
+    const b2Vec2& GetLocalAnchorB() const { return m_localAnchorB; }
+    float GetReferenceAngle() const { return m_referenceAngle; }
+    void SetFrequency(float hz);
+    float GetFrequency() const { return m_frequencyHz; }
+    void SetDampingRatio(float ratio);
+    float GetDampingRatio() const { return m_dampingRatio; }
+    b2Vec2 GetReactionForce(float inv_dt) const override;
+    float GetReactionTorque(float inv_dt) const override;
+
+private:
+    friend class b2World;
+    b2WeldJoint(const b2WeldJointDef* def, b2World* world, int32_t index);
+    b2Vec2 m_localAnchorA, m_localAnchorB;
+    float m_referenceAngle, m_frequencyHz, m_dampingRatio;
+};
+
 // Per-step solver options of the GPU path (no reference counterpart).
 struct b2SolverOptions {
     int32_t mode = 0;                 // 0 coloured, 1 reference order
@@ -517,6 +580,8 @@ private:
     friend class b2Joint;
     friend class b2RevoluteJoint;
     friend class b2DistanceJoint;
+    friend class b2RopeJoint;
+    friend class b2WeldJoint;
     int32_t Upload();
     void Download() const;
     void DownloadJoints() const;

@@ -1248,6 +1248,42 @@ inline V3 solve33(const Mat33& m, V3 b) {
     x.z = det * dot3(m.ex, cross3(m.ey, b));
     return x;
 }
+// b2Mat33::GetInverse22 and GetSymInverse33, REF/Common/b2Math.cpp:58-97
+inline Mat33 inverse22(const Mat33& K) {
+    const float a = K.ex.x, b = K.ey.x, c = K.ex.y, d = K.ey.y;
+    float det = a * d - b * c;
+    if (det != 0.0f) det = 1.0f / det;
+    Mat33 M;
+    M.ex = V3{det * d, -det * c, 0.0f};
+    M.ey = V3{-det * b, det * a, 0.0f};
+    M.ez = V3{0.0f, 0.0f, 0.0f};
+    return M;
+}
+inline Mat33 symInverse33(const Mat33& K) {
+    float det = dot3(K.ex, cross3(K.ey, K.ez));
+    if (det != 0.0f) det = 1.0f / det;
+    const float a11 = K.ex.x, a12 = K.ey.x, a13 = K.ez.x;
+    const float a22 = K.ey.y, a23 = K.ez.y;
+    const float a33 = K.ez.z;
+    Mat33 M;
+    M.ex.x = det * (a22 * a33 - a23 * a23);
+    M.ex.y = det * (a13 * a23 - a12 * a33);
+    M.ex.z = det * (a12 * a23 - a13 * a22);
+    M.ey.x = M.ex.y;
+    M.ey.y = det * (a11 * a33 - a13 * a13);
+    M.ey.z = det * (a13 * a12 - a11 * a23);
+    M.ez.x = M.ex.z;
+    M.ez.y = M.ey.z;
+    M.ez.z = det * (a11 * a22 - a12 * a12);
+    return M;
+}
+// b2Mul(b2Mat33, b2Vec3) and b2Mul22, REF/Common/b2Math.h:531-540
+inline V3 mul33(const Mat33& A, V3 v) {
+    return V3{v.x * A.ex.x + v.y * A.ey.x + v.z * A.ez.x, v.x * A.ex.y + v.y * A.ey.y + v.z * A.ez.y,
+              v.x * A.ex.z + v.y * A.ey.z + v.z * A.ez.z};
+}
+inline V2 mul22(const Mat33& A, V2 v) { return V2{A.ex.x * v.x + A.ey.x * v.y, A.ex.y * v.x + A.ey.y * v.y}; }
+
 inline V2 solve22(const Mat33& m, V2 b) {
     float a11 = m.ex.x, a12 = m.ey.x, a21 = m.ex.y, a22 = m.ey.y;
     float det = a11 * a22 - a12 * a21;
@@ -1269,11 +1305,16 @@ struct Joint {
     float maxMotorTorque, motorSpeed, lowerAngle, upperAngle;
     int limitState;
     bool removed;
-    // b2DistanceJoint (type 1), REF/Dynamics/Joints/b2DistanceJoint.h:88-113
+    // b2DistanceJoint (type 1), REF/Dynamics/Joints/b2DistanceJoint.h:88-113;
+    // b2RopeJoint (type 2: length is maxLength, distMass the rope's mass,
+    // limitState its state), REF/Dynamics/Joints/b2RopeJoint.h:85-110;
+    // b2WeldJoint (type 3: referenceAngle, frequencyHz, dampingRatio, mass,
+    // gamma, bias), REF/Dynamics/Joints/b2WeldJoint.h:99-124
     int type;
     float length, frequencyHz, dampingRatio;
     V2 u;
     float distMass, gamma, bias;
+    float ropeLength;
     // solver temporaries
     int indexA, indexB;
     V2 rA, rB, localCenterA, localCenterB;
@@ -1461,6 +1502,7 @@ struct World {
         j.length = d.length; j.frequencyHz = d.frequencyHz; j.dampingRatio = d.dampingRatio;
         j.u = mk(0.0f, 0.0f);
         j.distMass = j.gamma = j.bias = 0.0f;
+        j.ropeLength = 0.0f;
         const int ji = (int)joints.size();
         j.prev = -1;
         j.next = jointHead;
@@ -2042,6 +2084,14 @@ struct World {
             distanceInitVelocity(j, h, dtRatio);
             return;
         }
+        if (j.type == 2) {
+            ropeInitVelocity(j, dtRatio);
+            return;
+        }
+        if (j.type == 3) {
+            weldInitVelocity(j, h, dtRatio);
+            return;
+        }
         const float aA = posA[j.indexA];
         V2 vA = velV[j.indexA]; float wA = velW[j.indexA];
         const float aB = posA[j.indexB];
@@ -2100,6 +2150,14 @@ struct World {
     void jointSolveVelocity(Joint& j, float h) {
         if (j.type == 1) {
             distanceSolveVelocity(j);
+            return;
+        }
+        if (j.type == 2) {
+            ropeSolveVelocity(j, 1.0f / h);
+            return;
+        }
+        if (j.type == 3) {
+            weldSolveVelocity(j);
             return;
         }
         V2 vA = velV[j.indexA]; float wA = velW[j.indexA];
@@ -2245,8 +2303,232 @@ struct World {
         return absf(C) < kLinearSlop;
     }
 
+    // b2RopeJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2RopeJoint.cpp:48-126
+    void ropeInitVelocity(Joint& j, float dtRatio) {
+        const V2 cA = posC[j.indexA], cB = posC[j.indexB];
+        const float aA = posA[j.indexA], aB = posA[j.indexB];
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        j.rA = mul(qA, j.localAnchorA - j.localCenterA);
+        j.rB = mul(qB, j.localAnchorB - j.localCenterB);
+        j.u = cB + j.rB - cA - j.rA;
+        j.ropeLength = length(j.u);
+        const float C = j.ropeLength - j.length;
+        j.limitState = C > 0.0f ? kAtUpperLimit : kInactiveLimit;
+        if (j.ropeLength > kLinearSlop) {
+            j.u *= 1.0f / j.ropeLength;
+        } else {
+            j.u = mk(0.0f, 0.0f);
+            j.distMass = 0.0f;
+            j.impulse.x = 0.0f;
+            return;
+        }
+        const float crA = cross(j.rA, j.u), crB = cross(j.rB, j.u);
+        const float invMass = j.invMassA + j.invIA * crA * crA + j.invMassB + j.invIB * crB * crB;
+        j.distMass = invMass != 0.0f ? 1.0f / invMass : 0.0f;
+        if (warmStarting) {
+            j.impulse.x *= dtRatio;
+            const V2 P = j.impulse.x * j.u;
+            vA -= j.invMassA * P;
+            wA -= j.invIA * cross(j.rA, P);
+            vB += j.invMassB * P;
+            wB += j.invIB * cross(j.rB, P);
+        } else {
+            j.impulse.x = 0.0f;
+        }
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2RopeJoint::SolveVelocityConstraints, REF b2RopeJoint.cpp:128-162
+    void ropeSolveVelocity(Joint& j, float invDt) {
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const V2 vpA = vA + cross(wA, j.rA);
+        const V2 vpB = vB + cross(wB, j.rB);
+        const float C = j.ropeLength - j.length;
+        float Cdot = dot(j.u, vpB - vpA);
+        if (C < 0.0f) Cdot += invDt * C;
+        float impulse = -j.distMass * Cdot;
+        const float oldImpulse = j.impulse.x;
+        j.impulse.x = vmin(0.0f, j.impulse.x + impulse);
+        impulse = j.impulse.x - oldImpulse;
+        const V2 P = impulse * j.u;
+        vA -= j.invMassA * P;
+        wA -= j.invIA * cross(j.rA, P);
+        vB += j.invMassB * P;
+        wB += j.invIB * cross(j.rB, P);
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2RopeJoint::SolvePositionConstraints, REF b2RopeJoint.cpp:164-196
+    bool ropeSolvePosition(Joint& j) {
+        V2 cA = posC[j.indexA]; float aA = posA[j.indexA];
+        V2 cB = posC[j.indexB]; float aB = posA[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        const V2 rA = mul(qA, j.localAnchorA - j.localCenterA);
+        const V2 rB = mul(qB, j.localAnchorB - j.localCenterB);
+        V2 u = cB + rB - cA - rA;
+        const float len = normalize(u);
+        float C = len - j.length;
+        C = clampf(C, 0.0f, kMaxLinearCorrection);
+        const float impulse = -j.distMass * C;
+        const V2 P = impulse * u;
+        cA -= j.invMassA * P;
+        aA -= j.invIA * cross(rA, P);
+        cB += j.invMassB * P;
+        aB += j.invIB * cross(rB, P);
+        posC[j.indexA] = cA; posA[j.indexA] = aA;
+        posC[j.indexB] = cB; posA[j.indexB] = aB;
+        return len - j.length < kLinearSlop;
+    }
+
+    // The 3x3 K of b2WeldJoint (REF/Dynamics/Joints/b2WeldJoint.cpp:95-104)
+    static Mat33 weldK(float mA, float iA, float mB, float iB, V2 rA, V2 rB) {
+        Mat33 K;
+        K.ex.x = mA + mB + rA.y * rA.y * iA + rB.y * rB.y * iB;
+        K.ey.x = -rA.y * rA.x * iA - rB.y * rB.x * iB;
+        K.ez.x = -rA.y * iA - rB.y * iB;
+        K.ex.y = K.ey.x;
+        K.ey.y = mA + mB + rA.x * rA.x * iA + rB.x * rB.x * iB;
+        K.ez.y = rA.x * iA + rB.x * iB;
+        K.ex.z = K.ez.x;
+        K.ey.z = K.ez.y;
+        K.ez.z = iA + iB;
+        return K;
+    }
+
+    // b2WeldJoint::InitVelocityConstraints, REF b2WeldJoint.cpp:60-169
+    void weldInitVelocity(Joint& j, float h, float dtRatio) {
+        const float aA = posA[j.indexA], aB = posA[j.indexB];
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        j.rA = mul(qA, j.localAnchorA - j.localCenterA);
+        j.rB = mul(qB, j.localAnchorB - j.localCenterB);
+        const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+        const Mat33 K = weldK(mA, iA, mB, iB, j.rA, j.rB);
+        if (j.frequencyHz > 0.0f) {
+            j.mass = inverse22(K);
+            float invM = iA + iB;
+            const float m = invM > 0.0f ? 1.0f / invM : 0.0f;
+            const float C = aB - aA - j.referenceAngle;
+            const float omega = 2.0f * kPi * j.frequencyHz;
+            const float d = 2.0f * m * j.dampingRatio * omega;
+            const float k = m * omega * omega;
+            j.gamma = h * (d + h * k);
+            j.gamma = j.gamma != 0.0f ? 1.0f / j.gamma : 0.0f;
+            j.bias = C * h * k * j.gamma;
+            invM += j.gamma;
+            j.mass.ez.z = invM != 0.0f ? 1.0f / invM : 0.0f;
+        } else if (K.ez.z == 0.0f) {
+            j.mass = inverse22(K);
+            j.gamma = 0.0f;
+            j.bias = 0.0f;
+        } else {
+            j.mass = symInverse33(K);
+            j.gamma = 0.0f;
+            j.bias = 0.0f;
+        }
+        if (warmStarting) {
+            j.impulse.x *= dtRatio; j.impulse.y *= dtRatio; j.impulse.z *= dtRatio;
+            const V2 P = mk(j.impulse.x, j.impulse.y);
+            vA -= mA * P;
+            wA -= iA * (cross(j.rA, P) + j.impulse.z);
+            vB += mB * P;
+            wB += iB * (cross(j.rB, P) + j.impulse.z);
+        } else {
+            j.impulse = V3{0.0f, 0.0f, 0.0f};
+        }
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2WeldJoint::SolveVelocityConstraints, REF b2WeldJoint.cpp:171-227
+    void weldSolveVelocity(Joint& j) {
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+        if (j.frequencyHz > 0.0f) {
+            const float Cdot2 = wB - wA;
+            const float impulse2 = -j.mass.ez.z * (Cdot2 + j.bias + j.gamma * j.impulse.z);
+            j.impulse.z += impulse2;
+            wA -= iA * impulse2;
+            wB += iB * impulse2;
+            const V2 Cdot1 = vB + cross(wB, j.rB) - vA - cross(wA, j.rA);
+            const V2 impulse1 = -mul22(j.mass, Cdot1);
+            j.impulse.x += impulse1.x;
+            j.impulse.y += impulse1.y;
+            const V2 P = impulse1;
+            vA -= mA * P;
+            wA -= iA * cross(j.rA, P);
+            vB += mB * P;
+            wB += iB * cross(j.rB, P);
+        } else {
+            const V2 Cdot1 = vB + cross(wB, j.rB) - vA - cross(wA, j.rA);
+            const float Cdot2 = wB - wA;
+            V3 impulse = mul33(j.mass, V3{Cdot1.x, Cdot1.y, Cdot2});
+            impulse = V3{-impulse.x, -impulse.y, -impulse.z};
+            j.impulse.x += impulse.x; j.impulse.y += impulse.y; j.impulse.z += impulse.z;
+            const V2 P = mk(impulse.x, impulse.y);
+            vA -= mA * P;
+            wA -= iA * (cross(j.rA, P) + impulse.z);
+            vB += mB * P;
+            wB += iB * (cross(j.rB, P) + impulse.z);
+        }
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2WeldJoint::SolvePositionConstraints, REF b2WeldJoint.cpp:229-308
+    bool weldSolvePosition(Joint& j) {
+        V2 cA = posC[j.indexA]; float aA = posA[j.indexA];
+        V2 cB = posC[j.indexB]; float aB = posA[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+        const V2 rA = mul(qA, j.localAnchorA - j.localCenterA);
+        const V2 rB = mul(qB, j.localAnchorB - j.localCenterB);
+        const Mat33 K = weldK(mA, iA, mB, iB, rA, rB);
+        float positionError, angularError;
+        if (j.frequencyHz > 0.0f) {
+            const V2 C1 = cB + rB - cA - rA;
+            positionError = length(C1);
+            angularError = 0.0f;
+            const V2 P = -solve22(K, C1);
+            cA -= mA * P;
+            aA -= iA * cross(rA, P);
+            cB += mB * P;
+            aB += iB * cross(rB, P);
+        } else {
+            const V2 C1 = cB + rB - cA - rA;
+            const float C2 = aB - aA - j.referenceAngle;
+            positionError = length(C1);
+            angularError = absf(C2);
+            V3 impulse;
+            if (K.ez.z > 0.0f) {
+                impulse = solve33(K, V3{C1.x, C1.y, C2});
+                impulse = V3{-impulse.x, -impulse.y, -impulse.z};
+            } else {
+                const V2 impulse2 = -solve22(K, C1);
+                impulse = V3{impulse2.x, impulse2.y, 0.0f};
+            }
+            const V2 P = mk(impulse.x, impulse.y);
+            cA -= mA * P;
+            aA -= iA * (cross(rA, P) + impulse.z);
+            cB += mB * P;
+            aB += iB * (cross(rB, P) + impulse.z);
+        }
+        posC[j.indexA] = cA; posA[j.indexA] = aA;
+        posC[j.indexB] = cB; posA[j.indexB] = aB;
+        return positionError <= kLinearSlop && angularError <= kAngularSlop;
+    }
+
     bool jointSolvePosition(Joint& j) {
         if (j.type == 1) return distanceSolvePosition(j);
+        if (j.type == 2) return ropeSolvePosition(j);
+        if (j.type == 3) return weldSolvePosition(j);
         V2 cA = posC[j.indexA]; float aA = posA[j.indexA];
         V2 cB = posC[j.indexB]; float aB = posA[j.indexB];
         float angularError = 0.0f, positionError = 0.0f;
@@ -2903,6 +3185,25 @@ int orc_world_destroy_joint(void* h, int joint) {
 
 // Per joint in creation order: impulse x, y, z and motor impulse.
 // b2DistanceJoint::SetLength / SetFrequency / SetDampingRatio.
+// b2RopeJoint::SetMaxLength.
+int orc_world_set_rope(void* h, int joint, float maxLength) {
+    World* w = (World*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint].removed || w->joints[joint].type != 2)
+        return B2G_ERR_ARG;
+    w->joints[joint].length = maxLength;
+    return B2G_OK;
+}
+
+// b2WeldJoint::SetFrequency / SetDampingRatio.
+int orc_world_set_weld(void* h, int joint, float hz, float ratio) {
+    World* w = (World*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint].removed || w->joints[joint].type != 3)
+        return B2G_ERR_ARG;
+    w->joints[joint].frequencyHz = hz;
+    w->joints[joint].dampingRatio = ratio;
+    return B2G_OK;
+}
+
 int orc_world_set_distance(void* h, int joint, float length, float hz, float ratio) {
     World* w = (World*)h;
     if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint].removed || w->joints[joint].type != 1)
@@ -2922,7 +3223,7 @@ void orc_world_get_joints(void* h, float* out) {
             for (int k = 0; k < 4; ++k) out[4 * i + k] = 0.0f;
             continue;
         }
-        if (j.type == 1) {  // reaction force at inv_dt = 1: impulse along u
+        if (j.type == 1 || j.type == 2) {  // reaction force at inv_dt = 1: impulse along u
             out[4 * i] = j.impulse.x * j.u.x;
             out[4 * i + 1] = j.impulse.x * j.u.y;
             out[4 * i + 2] = 0.0f;

@@ -90,6 +90,30 @@ int loadJoints(RefWorld* w, int base, const b2scene::JointDef* jd, int nj) {
             w->joints.push_back(w->world->CreateJoint(&dd));
             continue;
         }
+        if (d.type == 2) {
+            b2RopeJointDef rd;
+            rd.bodyA = w->bodies[base + d.bodyA];
+            rd.bodyB = w->bodies[base + d.bodyB];
+            rd.collideConnected = d.collideConnected != 0;
+            rd.localAnchorA.Set(d.ax, d.ay);
+            rd.localAnchorB.Set(d.bx, d.by);
+            rd.maxLength = d.length;
+            w->joints.push_back(w->world->CreateJoint(&rd));
+            continue;
+        }
+        if (d.type == 3) {
+            b2WeldJointDef wd;
+            wd.bodyA = w->bodies[base + d.bodyA];
+            wd.bodyB = w->bodies[base + d.bodyB];
+            wd.collideConnected = d.collideConnected != 0;
+            wd.localAnchorA.Set(d.ax, d.ay);
+            wd.localAnchorB.Set(d.bx, d.by);
+            wd.referenceAngle = d.referenceAngle;
+            wd.frequencyHz = d.frequencyHz;
+            wd.dampingRatio = d.dampingRatio;
+            w->joints.push_back(w->world->CreateJoint(&wd));
+            continue;
+        }
         b2RevoluteJointDef def;
         def.bodyA = w->bodies[base + d.bodyA];
         def.bodyB = w->bodies[base + d.bodyB];
@@ -357,6 +381,24 @@ int ref_world_set_distance(void* h, int joint, float length, float hz, float rat
     return 0;
 }
 
+int ref_world_set_rope(void* h, int joint, float maxLength) {
+    RefWorld* w = (RefWorld*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
+    if (w->joints[joint]->GetType() != e_ropeJoint) return -1;
+    static_cast<b2RopeJoint*>(w->joints[joint])->SetMaxLength(maxLength);
+    return 0;
+}
+
+int ref_world_set_weld(void* h, int joint, float hz, float ratio) {
+    RefWorld* w = (RefWorld*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
+    if (w->joints[joint]->GetType() != e_weldJoint) return -1;
+    b2WeldJoint* j = static_cast<b2WeldJoint*>(w->joints[joint]);
+    j->SetFrequency(hz);
+    j->SetDampingRatio(ratio);
+    return 0;
+}
+
 int ref_world_destroy_joint(void* h, int joint) {
     RefWorld* w = (RefWorld*)h;
     if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
@@ -373,11 +415,12 @@ void ref_world_get_joints(void* h, float* out) {
             for (int k = 0; k < 4; ++k) out[4 * i + k] = 0.0f;
             continue;
         }
-        if (w->joints[i]->GetType() == e_distanceJoint) {
+        const b2JointType type = w->joints[i]->GetType();
+        if (type == e_distanceJoint || type == e_ropeJoint || type == e_weldJoint) {
             const b2Vec2 f = w->joints[i]->GetReactionForce(1.0f);
             out[4 * i] = f.x;
             out[4 * i + 1] = f.y;
-            out[4 * i + 2] = 0.0f;
+            out[4 * i + 2] = w->joints[i]->GetReactionTorque(1.0f);
             out[4 * i + 3] = 0.0f;
             continue;
         }

@@ -36,8 +36,8 @@ def test_struct_layouts_match_the_library():
     n = lib.b2g_abi_sizes(None, 0)
     sizes = np.zeros(n, np.int32)
     lib.b2g_abi_sizes(E.ptr(sizes), n)
-    shape, manifold, body, fixture, world, params, stats, profile, mlp, aabb, xf, revolute, schedule, sweep, distance = \
-        sizes[:15]
+    shape, manifold, body, fixture, world, params, stats, profile, mlp, aabb, xf, revolute, schedule, sweep, distance, \
+        rope, weld = sizes[:17]
     assert E.SHAPE.itemsize == shape
     assert E.MANIFOLD.itemsize == manifold
     assert E.STATS.itemsize == stats
@@ -52,4 +52,6 @@ def test_struct_layouts_match_the_library():
     assert schedule == E.SCHEDULE.itemsize
     assert sweep == E.SWEEP.itemsize == 36
     assert distance == 40  # 2 body indices, 7 floats, 1 switch
+    assert rope == 32      # 2 body indices, 5 floats, 1 switch
+    assert weld == 40      # 2 body indices, 7 floats, 1 switch
     assert params == E.STEP_PARAMS.itemsize

@@ -296,3 +296,95 @@ def test_distance_rods_hold(gpu):
     assert np.all(np.isfinite(g.bodies()))
     assert worst < 0.02, worst
     g.close()
+
+
+@pytest.mark.parametrize("kind,n,seed,edit", [
+    ("ropes", 16, 1, ("world_set_rope", 0, F(2.0))),
+    ("welds", 12, 2, ("world_set_weld", 1, F(0.0), F(0.0))),
+])
+def test_rope_and_weld_joints_bitexact(gpu, orc, wide_trig, kind, n, seed, edit):
+    """Rope and weld joints, one per island: boxes on ropes from the ground
+    (slack, taut, one of length zero) and single-link beams on rigid and soft
+    welds (one with a fixed-rotation link): bodies and the joints' reaction
+    forces and torques equal the oracle's bit for bit, also after
+    SetMaxLength / SetFrequency through the box2d:: API."""
+    g = gpu.world(kind, n, seed, sleep=True)
+    o = orc.world(kind, n, seed, sleep=True)
+    for s in range(300):
+        if s == 150:
+            for w in (g, o):
+                assert w.e.fn(edit[0])(w.h, *edit[1:]) == 0
+        g.step(1)
+        o.step(1)
+        if s % 30 == 29 or s < 3:
+            gb, ob = g.bodies(), o.bodies()
+            bad = np.where(~np.all(_bits(gb[:, :10]) == _bits(ob[:, :10]), axis=1))[0]
+            assert len(bad) == 0, f"step {s + 1}: bodies {bad[:8]} differ"
+    jg, jo = _joints(g), _joints(o)
+    assert np.array_equal(_bits(jg), _bits(jo))
+    assert np.abs(jg[:, :2]).max() > 0.03  # the joints carry load
+    g.close()
+    o.close()
+
+
+def _weld_gaps(bodies, n):
+    """Anchor gap |cB + rB - cA - rA| of every weld of the welds scene (n
+    beams of 4 + i % 4 links from x = 10 i - 5 n, a dropped box after each)."""
+    def world_point(i, local):
+        p = bodies[i, [E.PX, E.PY]].astype(np.float64)
+        s, c = float(bodies[i, E.QS]), float(bodies[i, E.QC])
+        return p + np.array([c * local[0] - s * local[1], s * local[0] + c * local[1]])
+    gaps, body = [], 1
+    for i in range(n):
+        x, prev = 10.0 * i - 5.0 * n, 0
+        for _ in range(4 + i % 4):
+            anchor = (x, 10.0) if prev == 0 else (0.5, 0.0)
+            gaps.append(np.hypot(*(world_point(body, (-0.5, 0.0)) - world_point(prev, anchor))))
+            prev, body = body, body + 1
+        body += 1
+    return np.array(gaps)
+
+
+def test_ropes_hold(gpu):
+    """Ropes with a second box on a rope (two joints per island, solved by
+    colour) hold: every rope stays within 2 cm of its maximum length, as
+    b2g_joint_drift measures the stretch."""
+    lib = _b2g(gpu)
+    g = gpu.world("ropes", 16, 0, sleep=False)
+    ctx = g.device_context()
+    out = np.zeros(3, np.float64)
+    worst = 0.0
+    for _ in range(10):
+        g.step(30)
+        assert lib.b2g_joint_drift(ctx, E.ptr(out)) == 0
+        assert out[2] == 32
+        worst = max(worst, out[1])
+    assert np.all(np.isfinite(g.bodies()))
+    assert worst < 0.02, worst
+    g.close()
+
+
+def test_welds_hold(gpu, orc):
+    """Welded beams of 4-7 links under dropped boxes (solved by colour):
+    b2g_joint_drift's anchor gap equals the gap computed from the bodies, and
+    the worst gap over the run stays within 1.5x (+1 cm) of the oracle's on
+    the same scene, which sags too (soft welds, long beams)."""
+    lib = _b2g(gpu)
+    g = gpu.world("welds", 12, 0, sleep=False)
+    o = orc.world("welds", 12, 0, sleep=False)
+    ctx = g.device_context()
+    out = np.zeros(3, np.float64)
+    worst_g = worst_o = 0.0
+    for _ in range(10):
+        g.step(30)
+        o.step(30)
+        assert lib.b2g_joint_drift(ctx, E.ptr(out)) == 0
+        assert out[2] == 66
+        gaps = _weld_gaps(g.bodies(), 12)
+        assert abs(out[1] - gaps.max()) < 1e-4 and abs(out[0] - gaps.mean()) < 1e-4, (out, gaps.max())
+        worst_g = max(worst_g, out[1])
+        worst_o = max(worst_o, _weld_gaps(o.bodies(), 12).max())
+    assert np.all(np.isfinite(g.bodies()))
+    assert worst_g <= 1.5 * worst_o + 0.01, (worst_g, worst_o)
+    g.close()
+    o.close()

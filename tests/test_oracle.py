@@ -32,6 +32,11 @@ F = ctypes.c_float
     ("distance", 16, 0, 400, True, 8, 3),      # distance joints: rods and springs, a second box on a rod
     ("distance", 12, 1, 300, False, 8, 3),
     ("terrain", 100, 2, 300, False, 8, 3),
+    ("ropes", 16, 0, 400, True, 8, 3),         # rope joints: slack and taut, zero length, a second box on a rope
+    ("ropes", 12, 1, 300, False, 8, 3),
+    ("welds", 12, 0, 400, True, 8, 3),         # weld joints: rigid and soft beams, a fixed-rotation link
+    ("welds", 10, 0, 250, False, 4, 2),
+    ("welds", 12, 2, 300, True, 8, 3),         # single-link beams: soft welds swing, a fixed-rotation rigid weld
 ])
 def test_world_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, vel, pos):
     a = ref.world(kind, n, seed, sleep=sleep)
@@ -597,3 +602,72 @@ def test_distance_spring_closed_form(orc, hz):
         yc.append(float(c.bodies()[1, E.CY]) - 7.0)
     assert max(yc) < 1e-3 and yc[-1] < 0.0 and yc[-1] > -0.05, (max(yc), yc[-1])
     c.close()
+
+
+def test_rope_and_weld_edits_bitexact(ref, orc):
+    """b2RopeJoint::SetMaxLength and b2WeldJoint::SetFrequency /
+    SetDampingRatio mid-run (a rope shortened and lengthened, a rigid weld
+    made soft, a soft one made rigid) against the reference's setters, with
+    the joints' reaction forces and torques."""
+    for kind, n, edits in (("ropes", 8, lambda w: (w.e.fn("world_set_rope")(w.h, 0, F(2.0)),
+                                                    w.e.fn("world_set_rope")(w.h, 3, F(6.0)))),
+                           ("welds", 6, lambda w: (w.e.fn("world_set_weld")(w.h, 0, F(3.0), F(0.2)),
+                                                   w.e.fn("world_set_weld")(w.h, 5, F(0.0), F(0.0))))):
+        a = ref.world(kind, n, 0, sleep=True)
+        b = orc.world(kind, n, 0, sleep=True)
+        a.step(80); b.step(80)
+        for w in (a, b):
+            assert edits(w) == (0, 0)
+        assert b.e.fn("world_set_rope" if kind == "welds" else "world_set_weld")(b.h, 0, F(1.0), F(0.0)) != 0
+        for s in range(200):
+            a.step(1); b.step(1)
+            assert np.array_equal(a.bodies().view(np.uint32), b.bodies().view(np.uint32)), f"{kind} step {s + 81}"
+        nj = a.e.fn("world_joint_count")(a.h)
+        ja, jb = np.zeros(4 * nj, np.float32), np.zeros(4 * nj, np.float32)
+        a.e.fn("world_get_joints")(a.h, E.ptr(ja))
+        b.e.fn("world_get_joints")(b.h, E.ptr(jb))
+        assert np.array_equal(ja.view(np.uint32), jb.view(np.uint32))
+        a.close(); b.close()
+
+
+def test_rope_slack_free_fall_and_bound(orc):
+    """A box that starts on its rope's anchor (rope of 4 m, box 3 of the
+    unchained rope scene, moving at -3 m/s) falls freely while the rope is
+    slack: no impulse, and the positions of symplectic Euler under gravity,
+    x_k = x0 - 3 k h and y_k = 16 - g h^2 k (k + 1) / 2. Afterwards the
+    rope holds it: the centre never gets further than 4 m + 1 cm from the
+    anchor."""
+    w = orc.world("ropes", 4, 1, sleep=False)
+    b0 = w.bodies()
+    x0 = float(b0[4, E.CX])
+    h = 1.0 / 60.0
+    anchor = np.array([6.0 * 3 - 12.0, 16.0])
+    worst = 0.0
+    for k in range(1, 301):
+        w.step(1)
+        c = w.bodies()[4, [E.CX, E.CY]].astype(np.float64)
+        if k <= 30:
+            assert abs(c[0] - (x0 - 3.0 * k * h)) < 1e-4, (k, c[0])
+            assert abs(c[1] - (16.0 - 10.0 * h * h * k * (k + 1) / 2)) < 1e-4, (k, c[1])
+            assert w.joints()[3, 0] == 0.0
+        worst = max(worst, float(np.hypot(*(c - anchor))))
+    assert 3.99 < worst < 4.01, worst
+    w.close()
+
+
+def test_weld_cantilever_statics(orc):
+    """A rigid cantilever of four 1 x 0.25 m links (density 1: 0.25 kg
+    each) welded to the ground at its left end comes to rest, and the
+    wall weld then carries the statics of the beam: linear impulse
+    (0, W h) with W = 1 kg x 10 m/s^2 and angular impulse M h with the
+    moment M = 0.25 x 10 x (0.5 + 1.5 + 2.5 + 3.5) = 20 N m about the wall,
+    within 2%. The welds in between carry the load of the links beyond them."""
+    w = orc.world("welds", 1, 1, sleep=False)
+    w.step(600)
+    j = w.joints() * 60.0  # reaction force and torque at inv_dt = 60
+    assert abs(j[0, 0]) < 0.05 and abs(j[0, 1] - 10.0) < 0.2 and abs(j[0, 2] - 20.0) < 0.4, j[0]
+    for k in range(1, 4):  # weld k holds links k..3: force 2.5 (4 - k), moment about its anchor
+        arm = sum(0.5 + q for q in range(4 - k))
+        assert abs(j[k, 1] - 2.5 * (4 - k)) < 0.05 * 2.5 * (4 - k), (k, j[k])
+        assert abs(j[k, 2] - 2.5 * arm) < 0.05 * 2.5 * arm, (k, j[k], arm)
+    w.close()
