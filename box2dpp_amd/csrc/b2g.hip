@@ -107,6 +107,7 @@ enum : int {
     JF_LIMIT = 1, JF_MOTOR = 2, JF_COLLIDE = 4, JF_REMOVED = 8,
     JL_INACTIVE = 0, JL_LOWER = 1, JL_UPPER = 2, JL_EQUAL = 3,  // limit state in bits 8-9
     JT_REVOLUTE = 0, JT_DISTANCE = 1, JT_ROPE = 2, JT_WELD = 3,  // joint type in bits 12-15
+    JT_FRICTION = 4, JT_MOTOR = 5,
 };
 __host__ __device__ inline int jointType(int flags) { return (flags >> 12) & 15; }
 struct JointView {
@@ -132,7 +133,11 @@ struct JointView {
 // length of the step); impulse = (impulse, u.x, u.y, limit state), the state
 // also in flags bits 8-9. Weld: limits =
 // referenceAngle, frequencyHz, dampingRatio; impulse x y z; K0, K1 = the
-// effective mass matrix; motorMass = gamma, motorSpeed = bias.
+// effective mass matrix; motorMass = gamma, motorSpeed = bias. Friction:
+// limits = maxForce, maxTorque; impulse = linear x y, angular; K0 = linear
+// mass m11 m12 m22, angular mass. Motor: anchors = linearOffset x y,
+// angularOffset, correctionFactor; limits, impulse and K0 as friction; K1 =
+// linear error, motorMass = angular error.
 
 __device__ inline int encodeOrdered(float f) {
     int i = __float_as_int(f);
@@ -1729,6 +1734,126 @@ __device__ inline bool weldSolvePositionOne(const WorldView& w, const JointView&
     return positionError <= kLinearSlop && angularError <= kAngularSlop;
 }
 
+// The 2x2 point-to-point K of the friction and motor joints, inverted
+// (b2Mat22::GetInverse, REF/Common/b2Math.h:226-238): m11, m12, m22.
+__device__ inline float3 linearMass22(float mA, float iA, float mB, float iB, V2 rA, V2 rB) {
+    const float a = mA + mB + iA * rA.y * rA.y + iB * rB.y * rB.y;
+    const float c = -iA * rA.x * rA.y - iB * rB.x * rB.y;
+    const float bb = c;
+    const float d = mA + mB + iA * rA.x * rA.x + iB * rB.x * rB.x;
+    float det = a * d - bb * c;
+    if (det != 0.0f) det = 1.0f / det;
+    return make_float3(det * d, -det * bb, det * a);
+}
+
+// b2FrictionJoint::InitVelocityConstraints and b2MotorJoint's
+// (REF/Dynamics/Joints/b2FrictionJoint.cpp:59-132,
+// REF/Dynamics/Joints/b2MotorJoint.cpp:63-141): the same rows, the motor
+// anchored at the centres with an error against its offsets.
+__device__ inline void frictionMotorInitOne(const WorldView& w, const JointView& jv, int j, int2 b, float dtRatio,
+                                            int warm, bool motor) {
+    const float4 msA = w.mass[b.x], msB = w.mass[b.y];
+    const float mA = msA.x, iA = msA.y, mB = msB.x, iB = msB.y;
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const Rot qA = rotOf(pA.z), qB = rotOf(pB.z);
+    const float4 an = jv.anchors[j];
+    V2 rA, rB;
+    if (motor) {
+        rA = mul(qA, -mk(lcA.x, lcA.y));
+        rB = mul(qB, -mk(lcB.x, lcB.y));
+    } else {
+        rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+        rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    }
+    const float3 lm = linearMass22(mA, iA, mB, iB, rA, rB);
+    float angularMass = iA + iB;
+    if (angularMass > 0.0f) angularMass = 1.0f / angularMass;
+    if (motor) {
+        const V2 linearError = mk(pB.x, pB.y) + rB - mk(pA.x, pA.y) - rA - mul(qA, mk(an.x, an.y));
+        jv.K1[j] = make_float2(linearError.x, linearError.y);
+        jv.motorMass[j] = pB.z - pA.z - an.z;
+    }
+    float4 imp = jv.impulse[j];
+    if (warm) {
+        imp.x *= dtRatio;
+        imp.y *= dtRatio;
+        imp.z *= dtRatio;
+        const V2 P = mk(imp.x, imp.y);
+        vA -= mA * P;
+        wA -= iA * (cross(rA, P) + imp.z);
+        vB += mB * P;
+        wB += iB * (cross(rB, P) + imp.z);
+    } else {
+        imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    jv.impulse[j] = imp;
+    jv.r[j] = make_float4(rA.x, rA.y, rB.x, rB.y);
+    jv.masses[j] = make_float4(mA, iA, mB, iB);
+    jv.K0[j] = make_float4(lm.x, lm.y, lm.z, angularMass);
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2FrictionJoint::SolveVelocityConstraints and b2MotorJoint's
+// (REF b2FrictionJoint.cpp:134-189, REF b2MotorJoint.cpp:143-199): an
+// angular impulse within h maxTorque, then a linear one within h maxForce.
+__device__ inline void frictionMotorSolveVelocityOne(const WorldView& w, const JointView& jv, int j, int2 b, float h,
+                                                     bool motor) {
+    const float4 ms = jv.masses[j];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const float4 r = jv.r[j];
+    const V2 rA = mk(r.x, r.y), rB = mk(r.z, r.w);
+    const float4 k0 = jv.K0[j];
+    const float4 lim = jv.limits[j];  // maxForce, maxTorque
+    float4 imp = jv.impulse[j];
+    const float invH = 1.0f / h;
+    const float cf = jv.anchors[j].w;
+    {
+        float Cdot = wB - wA;
+        if (motor) Cdot = Cdot + invH * cf * jv.motorMass[j];
+        float impulse = -k0.w * Cdot;
+        const float oldImpulse = imp.z;
+        const float maxImpulse = h * lim.y;
+        imp.z = clampB(imp.z + impulse, -maxImpulse, maxImpulse);
+        impulse = imp.z - oldImpulse;
+        wA -= iA * impulse;
+        wB += iB * impulse;
+    }
+    {
+        V2 Cdot = vB + cross(wB, rB) - vA - cross(wA, rA);
+        if (motor) {
+            const float2 e = jv.K1[j];
+            Cdot = Cdot + (invH * cf) * mk(e.x, e.y);
+        }
+        // -b2Mul(linearMass, Cdot)
+        V2 impulse = -mk(k0.x * Cdot.x + k0.y * Cdot.y, k0.y * Cdot.x + k0.z * Cdot.y);
+        const V2 oldImpulse = mk(imp.x, imp.y);
+        V2 acc = oldImpulse + impulse;
+        const float maxImpulse = h * lim.x;
+        if (lengthSq(acc) > maxImpulse * maxImpulse) {
+            normalizeV(acc);
+            acc *= maxImpulse;
+        }
+        impulse = acc - oldImpulse;
+        imp.x = acc.x;
+        imp.y = acc.y;
+        vA -= mA * impulse;
+        wA -= iA * cross(rA, impulse);
+        vB += mB * impulse;
+        wB += iB * cross(rB, impulse);
+    }
+    jv.impulse[j] = imp;
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
 // b2RevoluteJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2RevoluteJoint.cpp:66-198
 __device__ inline void jointInitOne(const WorldView& w, const JointView& jv, int j, float dtRatio, int warm) {
     const int2 b = jv.bodies[j];
@@ -1744,6 +1869,10 @@ __device__ inline void jointInitOne(const WorldView& w, const JointView& jv, int
     }
     if (type == JT_WELD) {
         weldInitOne(w, jv, j, b, dtRatio, warm);
+        return;
+    }
+    if (type == JT_FRICTION || type == JT_MOTOR) {
+        frictionMotorInitOne(w, jv, j, b, dtRatio, warm, type == JT_MOTOR);
         return;
     }
     const float4 msA = w.mass[b.x], msB = w.mass[b.y];
@@ -1840,6 +1969,10 @@ __device__ inline void jointSolveVelocityOne(const WorldView& w, const JointView
         weldSolveVelocityOne(w, jv, j, b);
         return;
     }
+    if (type == JT_FRICTION || type == JT_MOTOR) {
+        frictionMotorSolveVelocityOne(w, jv, j, b, h, type == JT_MOTOR);
+        return;
+    }
     const float4 ms = jv.masses[j];
     const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
     const float4 va = w.vel[b.x], vbv = w.vel[b.y];
@@ -1933,6 +2066,7 @@ __device__ inline void jointSolvePositionOne(const WorldView& w, const JointView
     const int root = w.islandOf[w.btype[b.x] != BT_STATIC ? b.x : b.y];
     if (it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0) return;  // island done
     const int type = jointType(jv.flags[j]);
+    if (type == JT_FRICTION || type == JT_MOTOR) return;  // no position pass (returns true)
     if (type != JT_REVOLUTE) {
         const bool ok = type == JT_DISTANCE ? distanceSolvePositionOne(w, jv, j, b)
                         : type == JT_ROPE   ? ropeSolvePositionOne(w, jv, j, b)
@@ -2050,6 +2184,10 @@ __global__ void k_joint_drift(WorldView w, JointView jv, float* out) {
     }
     if (type == JT_ROPE) {  // stretch beyond the maximum length
         out[j] = fmaxf(gap - jv.limits[j].x, 0.0f);
+        return;
+    }
+    if (type == JT_FRICTION || type == JT_MOTOR) {  // no positional constraint
+        out[j] = -1.0f;
         return;
     }
     out[j] = gap;  // revolute and weld: the anchors' separation
@@ -6620,7 +6758,8 @@ int b2g_abi_sizes(int32_t* out, int cap) {
         (int32_t)sizeof(b2g_step_stats),  (int32_t)sizeof(b2g_phase_profile), (int32_t)sizeof(b2g_mlp_params),
         (int32_t)sizeof(b2g_aabb),        (int32_t)sizeof(b2g_transform),    (int32_t)sizeof(b2g_revolute_def),
         (int32_t)sizeof(b2g_schedule),    (int32_t)sizeof(b2g_sweep),        (int32_t)sizeof(b2g_distance_def),
-        (int32_t)sizeof(b2g_rope_def),    (int32_t)sizeof(b2g_weld_def),
+        (int32_t)sizeof(b2g_rope_def),    (int32_t)sizeof(b2g_weld_def),     (int32_t)sizeof(b2g_friction_def),
+        (int32_t)sizeof(b2g_motor_def),
     };
     const int n = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (!out) return n;
@@ -7080,6 +7219,42 @@ int b2g_add_weld_joints(b2g_context* ctx, const b2g_weld_def* defs, int count, i
         rows.push_back(jointRow(defs[i], JT_WELD,
                                 make_float4(defs[i].referenceAngle, defs[i].frequencyHz, defs[i].dampingRatio, 0.0f)));
     return appendJointRows(ctx, rows, first);
+}
+
+int b2g_add_friction_joints(b2g_context* ctx, const b2g_friction_def* defs, int count, int* first) {
+    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
+    std::vector<JointRow> rows;
+    for (int i = 0; i < count; ++i)
+        rows.push_back(jointRow(defs[i], JT_FRICTION, make_float4(defs[i].maxForce, defs[i].maxTorque, 0.0f, 0.0f)));
+    return appendJointRows(ctx, rows, first);
+}
+
+int b2g_set_friction_joint(b2g_context* ctx, int joint, float maxForce, float maxTorque) {
+    return setJointParams(ctx, joint, JT_FRICTION, "not a friction joint",
+                          make_float4(maxForce, maxTorque, 0.0f, 0.0f));
+}
+
+int b2g_add_motor_joints(b2g_context* ctx, const b2g_motor_def* defs, int count, int* first) {
+    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
+    std::vector<JointRow> rows;
+    for (int i = 0; i < count; ++i) {
+        const b2g_motor_def& d = defs[i];
+        rows.push_back(JointRow{d.bodyA, d.bodyB,
+                                make_float4(d.linearOffset[0], d.linearOffset[1], d.angularOffset, d.correctionFactor),
+                                make_float4(d.maxForce, d.maxTorque, 0.0f, 0.0f),
+                                (JT_MOTOR << 12) | (d.collideConnected ? JF_COLLIDE : 0)});
+    }
+    return appendJointRows(ctx, rows, first);
+}
+
+int b2g_set_motor_joint(b2g_context* ctx, int joint, const b2g_motor_def* d) {
+    if (!ctx || !d) return B2G_ERR_ARG;
+    int rc = setJointParams(ctx, joint, JT_MOTOR, "not a motor joint", make_float4(d->maxForce, d->maxTorque, 0.0f, 0.0f));
+    if (rc != B2G_OK) return rc;
+    const float4 an = make_float4(d->linearOffset[0], d->linearOffset[1], d->angularOffset, d->correctionFactor);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jAnchors + joint, &an, sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
 }
 
 int b2g_set_weld_joint(b2g_context* ctx, int joint, float frequencyHz, float dampingRatio) {

@@ -153,6 +153,31 @@ int loadJoints(Runner* r, int base, const b2scene::JointDef* jd, int nj) {
             r->joints.push_back(j);
             continue;
         }
+        if (d.type == 4 || d.type == 5) {
+            b2FrictionJointDef fd;
+            b2MotorJointDef md;
+            b2JointDef* def = &fd;
+            if (d.type == 4) {
+                fd.localAnchorA = b2Vec2(d.ax, d.ay);
+                fd.localAnchorB = b2Vec2(d.bx, d.by);
+                fd.maxForce = d.maxForce;
+                fd.maxTorque = d.maxTorque;
+            } else {
+                md.linearOffset = b2Vec2(d.ax, d.ay);
+                md.angularOffset = d.referenceAngle;
+                md.maxForce = d.maxForce;
+                md.maxTorque = d.maxTorque;
+                md.correctionFactor = d.correctionFactor;
+                def = &md;
+            }
+            def->bodyA = r->bodies[base + d.bodyA];
+            def->bodyB = r->bodies[base + d.bodyB];
+            def->collideConnected = d.collideConnected != 0;
+            b2Joint* j = r->world.CreateJoint(def);
+            if (j == nullptr) return B2G_ERR_ARG;
+            r->joints.push_back(j);
+            continue;
+        }
         if (d.type == 2 || d.type == 3) {
             b2RopeJointDef rd;
             b2WeldJointDef wd;
@@ -274,6 +299,32 @@ int b2pp_world_set_distance(void* h, int joint, float length, float hz, float ra
     j->SetLength(length);
     j->SetFrequency(hz);
     j->SetDampingRatio(ratio);
+    return 0;
+}
+
+// b2FrictionJoint::SetMaxForce / SetMaxTorque.
+int b2pp_world_set_friction(void* h, int joint, float maxForce, float maxTorque) {
+    Runner* r = (Runner*)h;
+    if (joint < 0 || joint >= (int)r->joints.size() || r->joints[joint] == nullptr) return B2G_ERR_ARG;
+    if (r->joints[joint]->GetType() != b2JointType::FRICTION_JOINT) return B2G_ERR_ARG;
+    b2FrictionJoint* j = static_cast<b2FrictionJoint*>(r->joints[joint]);
+    j->SetMaxForce(maxForce);
+    j->SetMaxTorque(maxTorque);
+    return 0;
+}
+
+// b2MotorJoint's setters.
+int b2pp_world_set_motor(void* h, int joint, float ox, float oy, float angle, float maxForce, float maxTorque,
+                         float factor) {
+    Runner* r = (Runner*)h;
+    if (joint < 0 || joint >= (int)r->joints.size() || r->joints[joint] == nullptr) return B2G_ERR_ARG;
+    if (r->joints[joint]->GetType() != b2JointType::MOTOR_JOINT) return B2G_ERR_ARG;
+    b2MotorJoint* j = static_cast<b2MotorJoint*>(r->joints[joint]);
+    j->SetLinearOffset(b2Vec2(ox, oy));
+    j->SetAngularOffset(angle);
+    j->SetMaxForce(maxForce);
+    j->SetMaxTorque(maxTorque);
+    j->SetCorrectionFactor(factor);
     return 0;
 }
 

@@ -563,6 +563,92 @@ void b2WeldJoint::SetDampingRatio(float ratio) {
     MarkDirty();
 }
 
+// b2FrictionJointDef::Initialize, REF/Dynamics/Joints/b2FrictionJoint.cpp:38-44
+void b2FrictionJointDef::Initialize(b2Body* bA, b2Body* bB, const b2Vec2& anchor) {
+    bodyA = bA;
+    bodyB = bB;
+    localAnchorA = bodyA->GetLocalPoint(anchor);
+    localAnchorB = bodyB->GetLocalPoint(anchor);
+}
+
+b2FrictionJoint::b2FrictionJoint(const b2FrictionJointDef* def, b2World* world, int32_t index)
+    : b2Joint(def, world, index), m_localAnchorA(def->localAnchorA), m_localAnchorB(def->localAnchorB),
+      m_maxForce(def->maxForce), m_maxTorque(def->maxTorque) {}
+
+b2Vec2 b2FrictionJoint::GetAnchorA() const { return m_bodyA->GetWorldPoint(m_localAnchorA); }
+b2Vec2 b2FrictionJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localAnchorB); }
+// REF/Dynamics/Joints/b2FrictionJoint.cpp:208-216
+b2Vec2 b2FrictionJoint::GetReactionForce(float inv_dt) const {
+    m_world->DownloadJoints();
+    return b2Vec2(inv_dt * m_impulse[0], inv_dt * m_impulse[1]);
+}
+float b2FrictionJoint::GetReactionTorque(float inv_dt) const {
+    m_world->DownloadJoints();
+    return inv_dt * m_impulse[2];
+}
+void b2FrictionJoint::SetMaxForce(float force) {
+    m_maxForce = force;
+    MarkDirty();
+}
+void b2FrictionJoint::SetMaxTorque(float torque) {
+    m_maxTorque = torque;
+    MarkDirty();
+}
+
+// b2MotorJointDef::Initialize, REF/Dynamics/Joints/b2MotorJoint.cpp:37-47
+void b2MotorJointDef::Initialize(b2Body* bA, b2Body* bB) {
+    bodyA = bA;
+    bodyB = bB;
+    linearOffset = bodyA->GetLocalPoint(bodyB->GetPosition());
+    angularOffset = bodyB->GetAngle() - bodyA->GetAngle();
+}
+
+b2MotorJoint::b2MotorJoint(const b2MotorJointDef* def, b2World* world, int32_t index)
+    : b2Joint(def, world, index), m_linearOffset(def->linearOffset), m_angularOffset(def->angularOffset),
+      m_maxForce(def->maxForce), m_maxTorque(def->maxTorque), m_correctionFactor(def->correctionFactor) {}
+
+b2Vec2 b2MotorJoint::GetAnchorA() const { return m_bodyA->GetPosition(); }
+b2Vec2 b2MotorJoint::GetAnchorB() const { return m_bodyB->GetPosition(); }
+// REF/Dynamics/Joints/b2MotorJoint.cpp:218-226
+b2Vec2 b2MotorJoint::GetReactionForce(float inv_dt) const {
+    m_world->DownloadJoints();
+    return b2Vec2(inv_dt * m_impulse[0], inv_dt * m_impulse[1]);
+}
+float b2MotorJoint::GetReactionTorque(float inv_dt) const {
+    m_world->DownloadJoints();
+    return inv_dt * m_impulse[2];
+}
+// The offset setters wake both bodies when the offset changes
+// (REF/Dynamics/Joints/b2MotorJoint.cpp:261-284).
+void b2MotorJoint::SetLinearOffset(const b2Vec2& linearOffset) {
+    if (linearOffset[0] != m_linearOffset[0] || linearOffset[1] != m_linearOffset[1]) {
+        m_bodyA->SetAwake(true);
+        m_bodyB->SetAwake(true);
+        m_linearOffset = linearOffset;
+        MarkDirty();
+    }
+}
+void b2MotorJoint::SetAngularOffset(float angularOffset) {
+    if (angularOffset != m_angularOffset) {
+        m_bodyA->SetAwake(true);
+        m_bodyB->SetAwake(true);
+        m_angularOffset = angularOffset;
+        MarkDirty();
+    }
+}
+void b2MotorJoint::SetMaxForce(float force) {
+    m_maxForce = force;
+    MarkDirty();
+}
+void b2MotorJoint::SetMaxTorque(float torque) {
+    m_maxTorque = torque;
+    MarkDirty();
+}
+void b2MotorJoint::SetCorrectionFactor(float factor) {
+    m_correctionFactor = factor;
+    MarkDirty();
+}
+
 // The setters wake both bodies and, where the limit changes, drop the limit
 // impulse (REF/Dynamics/Joints/b2RevoluteJoint.cpp:419-471).
 void b2RevoluteJoint::EnableMotor(bool flag) {
@@ -609,8 +695,9 @@ void b2RevoluteJoint::SetLimits(float lower, float upper) {
 b2Joint* b2World::CreateJoint(const b2JointDef* def) {
     if (IsLocked() || def == nullptr || def->bodyA == nullptr || def->bodyB == nullptr) return nullptr;
     if (def->type != b2JointType::REVOLUTE_JOINT && def->type != b2JointType::DISTANCE_JOINT &&
-        def->type != b2JointType::ROPE_JOINT && def->type != b2JointType::WELD_JOINT) {
-        m_lastError = "only revolute, distance, rope and weld joints run on the GPU path";
+        def->type != b2JointType::ROPE_JOINT && def->type != b2JointType::WELD_JOINT &&
+        def->type != b2JointType::FRICTION_JOINT && def->type != b2JointType::MOTOR_JOINT) {
+        m_lastError = "only revolute, distance, rope, weld, friction and motor joints run on the GPU path";
         return nullptr;
     }
     if (def->bodyA == def->bodyB || def->bodyA->m_world != this || def->bodyB->m_world != this) return nullptr;
@@ -626,6 +713,10 @@ b2Joint* b2World::CreateJoint(const b2JointDef* def) {
         j = new b2RopeJoint(static_cast<const b2RopeJointDef*>(def), this, index);
     else if (def->type == b2JointType::WELD_JOINT)
         j = new b2WeldJoint(static_cast<const b2WeldJointDef*>(def), this, index);
+    else if (def->type == b2JointType::FRICTION_JOINT)
+        j = new b2FrictionJoint(static_cast<const b2FrictionJointDef*>(def), this, index);
+    else if (def->type == b2JointType::MOTOR_JOINT)
+        j = new b2MotorJoint(static_cast<const b2MotorJointDef*>(def), this, index);
     else
         j = new b2RevoluteJoint(static_cast<const b2RevoluteJointDef*>(def), this, (int32_t)m_joints.size());
     m_joints.push_back(j);
@@ -736,6 +827,34 @@ static b2g_weld_def weldDef(const b2WeldJoint* d, int32_t bodyA, int32_t bodyB) 
     r.referenceAngle = d->GetReferenceAngle();
     r.frequencyHz = d->GetFrequency();
     r.dampingRatio = d->GetDampingRatio();
+    r.collideConnected = d->GetCollideConnected() ? 1 : 0;
+    return r;
+}
+
+static b2g_friction_def frictionDef(const b2FrictionJoint* d, int32_t bodyA, int32_t bodyB) {
+    b2g_friction_def r{};
+    r.bodyA = bodyA;
+    r.bodyB = bodyB;
+    r.localAnchorA[0] = d->GetLocalAnchorA()[0];
+    r.localAnchorA[1] = d->GetLocalAnchorA()[1];
+    r.localAnchorB[0] = d->GetLocalAnchorB()[0];
+    r.localAnchorB[1] = d->GetLocalAnchorB()[1];
+    r.maxForce = d->GetMaxForce();
+    r.maxTorque = d->GetMaxTorque();
+    r.collideConnected = d->GetCollideConnected() ? 1 : 0;
+    return r;
+}
+
+static b2g_motor_def motorDef(const b2MotorJoint* d, int32_t bodyA, int32_t bodyB) {
+    b2g_motor_def r{};
+    r.bodyA = bodyA;
+    r.bodyB = bodyB;
+    r.linearOffset[0] = d->GetLinearOffset()[0];
+    r.linearOffset[1] = d->GetLinearOffset()[1];
+    r.angularOffset = d->GetAngularOffset();
+    r.maxForce = d->GetMaxForce();
+    r.maxTorque = d->GetMaxTorque();
+    r.correctionFactor = d->GetCorrectionFactor();
     r.collideConnected = d->GetCollideConnected() ? 1 : 0;
     return r;
 }
@@ -940,6 +1059,20 @@ int32_t b2World::Upload() {
                     defs.push_back(weldDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index));
                 }
                 rc = b2g_add_weld_joints(m_ctx, defs.data(), (int)defs.size(), &first);
+            } else if (type == b2JointType::FRICTION_JOINT) {
+                std::vector<b2g_friction_def> defs;
+                for (int q = i; q < k; ++q) {
+                    const b2FrictionJoint* d = static_cast<const b2FrictionJoint*>(m_joints[q]);
+                    defs.push_back(frictionDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index));
+                }
+                rc = b2g_add_friction_joints(m_ctx, defs.data(), (int)defs.size(), &first);
+            } else if (type == b2JointType::MOTOR_JOINT) {
+                std::vector<b2g_motor_def> defs;
+                for (int q = i; q < k; ++q) {
+                    const b2MotorJoint* d = static_cast<const b2MotorJoint*>(m_joints[q]);
+                    defs.push_back(motorDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index));
+                }
+                rc = b2g_add_motor_joints(m_ctx, defs.data(), (int)defs.size(), &first);
             } else {
                 std::vector<b2g_revolute_def> defs;
                 for (int q = i; q < k; ++q) {
@@ -976,6 +1109,13 @@ int32_t b2World::Upload() {
             } else if (j->m_type == b2JointType::WELD_JOINT) {
                 const b2WeldJoint* d = static_cast<const b2WeldJoint*>(j);
                 rc = b2g_set_weld_joint(m_ctx, j->m_index, d->m_frequencyHz, d->m_dampingRatio);
+            } else if (j->m_type == b2JointType::FRICTION_JOINT) {
+                const b2FrictionJoint* d = static_cast<const b2FrictionJoint*>(j);
+                rc = b2g_set_friction_joint(m_ctx, j->m_index, d->m_maxForce, d->m_maxTorque);
+            } else if (j->m_type == b2JointType::MOTOR_JOINT) {
+                const b2MotorJoint* m = static_cast<const b2MotorJoint*>(j);
+                const b2g_motor_def d = motorDef(m, m->m_bodyA->m_index, m->m_bodyB->m_index);
+                rc = b2g_set_motor_joint(m_ctx, j->m_index, &d);
             } else {
                 const b2RevoluteJoint* r = static_cast<const b2RevoluteJoint*>(j);
                 const b2g_revolute_def d = revoluteDef(r, r->m_bodyA->m_index, r->m_bodyB->m_index);

@@ -71,8 +71,13 @@ struct JointDef {
     // type is 1; b2RopeJointDef (b2RopeJoint.h:30-50, length = maxLength) when
     // 2; b2WeldJointDef (b2WeldJoint.h:29-60, referenceAngle, frequencyHz,
     // dampingRatio) when 3
-    int type;              // 0 revolute, 1 distance, 2 rope, 3 weld
+    // b2FrictionJointDef (b2FrictionJoint.h:27-53, maxForce, maxTorque) when
+    // 4; b2MotorJointDef (b2MotorJoint.h:27-56, linearOffset = (ax, ay),
+    // angularOffset = referenceAngle, maxForce, maxTorque, correctionFactor)
+    // when 5
+    int type;              // 0 revolute, 1 distance, 2 rope, 3 weld, 4 friction, 5 motor
     float length, frequencyHz, dampingRatio;
+    float maxForce, maxTorque, correctionFactor;
 };
 
 struct Scene {
@@ -592,6 +597,64 @@ inline Scene welds(int n, bool drop = true, bool single = false) {
     return s;
 }
 
+// Friction joints: n boxes (0.5 m, 1 kg) tied to the ground body by
+// friction joints at their centres (maxForce 2-6 N, maxTorque 0.5-1.5 N m),
+// launched with speeds of 2-6 m/s and spins, over a ground edge 2 m below;
+// pairs of neighbours are thrown at each other. Run without gravity it is a
+// top-down world.
+inline Scene frictionJoints(int n) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-400.0f, 0.0f, 400.0f, 0.0f));
+    for (int i = 0; i < n; ++i) {
+        BodyDef bd = defaultBody(2, 4.0f * (float)i - 2.0f * (float)n, 2.5f);
+        bd.vx = ((i & 1) ? -1.0f : 1.0f) * (2.0f + (float)(i % 5));
+        bd.vy = 0.5f * (float)(i % 3);
+        bd.w = 1.0f + (float)(i % 4);
+        addBody(s, bd);
+        addFixture(s, boxFixture(0.5f, 0.5f, 1.0f, 0.4f));
+        JointDef j{};
+        j.type = 4;
+        j.bodyA = 0; j.bodyB = (int)s.bodies.size() - 1;
+        j.ax = bd.px; j.ay = bd.py;   // the box centre, on the ground body and the box
+        j.maxForce = 2.0f + (float)(i % 5);
+        j.maxTorque = 0.5f + 0.5f * (float)(i % 3);
+        s.joints.push_back(j);
+    }
+    return s;
+}
+
+// Motor joints: n boxes driven by motor joints from the ground body towards
+// targets 1.5-3 m away (linear offsets), turned by 0.5-1.5 rad (angular
+// offsets), with maxForce 20-40 N (above the 10 N weight), maxTorque 5-15
+// N m and correction factors 0.1-0.5; when `chained`, every third box is
+// driven from the previous box instead of the ground.
+inline Scene motorJoints(int n, bool chained = true) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-400.0f, 0.0f, 400.0f, 0.0f));
+    for (int i = 0; i < n; ++i) {
+        const float x = 6.0f * (float)i - 3.0f * (float)n, y = 4.0f;
+        addBody(s, defaultBody(2, x, y));
+        addFixture(s, boxFixture(0.5f, 0.5f, 1.0f, 0.4f));
+        const int b = (int)s.bodies.size() - 1;
+        JointDef j{};
+        j.type = 5;
+        const bool fromPrev = chained && (i % 3) == 2;
+        j.bodyA = fromPrev ? b - 1 : 0;
+        j.bodyB = b;
+        const float dx = 1.5f * (float)(1 + i % 2), dy = 1.0f + 0.5f * (float)(i % 3);
+        j.ax = fromPrev ? 6.0f : x + dx;          // offset in body A's frame
+        j.ay = fromPrev ? 0.5f : y + dy;
+        j.referenceAngle = 0.5f + 0.5f * (float)(i % 3);
+        j.maxForce = 20.0f + 10.0f * (float)(i % 3);
+        j.maxTorque = 5.0f + 5.0f * (float)(i % 3);
+        j.correctionFactor = 0.1f + 0.2f * (float)(i % 3);
+        s.joints.push_back(j);
+    }
+    return s;
+}
+
 // Scene by name: "hello", "pyramid" (n = rows, 0 -> 20), "pile" (n bodies,
 // seed), "world" (one batched-scene world of n bodies, seed), "pendulums"
 // (n pendulums), "chains" (n chains of `seed` links), "stress" (pile of n
@@ -602,7 +665,9 @@ inline Scene welds(int n, bool drop = true, bool single = false) {
 // each), "spring" (one box on a spring of n / 10 Hz and damping ratio
 // seed / 10), "ropes" (n boxes on ropes; seed 0 hangs a second box on each),
 // "welds" (n welded beams; seed 0 drops a box on each, seed 2 makes the
-// beams single links). Unknown names give an empty scene.
+// beams single links), "friction" (n boxes on friction joints), "motor" (n
+// boxes on motor joints; seed 0 drives every third from the previous box).
+// Unknown names give an empty scene.
 inline Scene byName(const char* kind, int n, uint32_t seed) {
     const std::string k(kind ? kind : "");
     if (k == "hello") return hello();
@@ -620,6 +685,8 @@ inline Scene byName(const char* kind, int n, uint32_t seed) {
     if (k == "spring") return spring(0.1f * (float)(n > 0 ? n : 10), 0.1f * (float)seed);
     if (k == "ropes") return ropes(n > 0 ? n : 16, seed == 0);
     if (k == "welds") return welds(n > 0 ? n : 12, seed == 0, seed == 2);
+    if (k == "friction") return frictionJoints(n > 0 ? n : 16);
+    if (k == "motor") return motorJoints(n > 0 ? n : 12, seed == 0);
     return Scene();
 }
 

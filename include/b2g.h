@@ -272,7 +272,7 @@ int b2g_penetration(b2g_context* ctx, double* out);
  * error b2RevoluteJoint::SolvePositionConstraints measures,
  * Box2D/Dynamics/Joints/b2RevoluteJoint.cpp:336-343), the error in length
  * for rigid distance joints, the stretch beyond the maximum length for rope
- * joints; soft distance joints are not measured. out[3], FP64 sums on the
+ * joints; soft distance, friction and motor joints are not measured. out[3], FP64 sums on the
  * host; no counterpart in the reference. */
 int b2g_joint_drift(b2g_context* ctx, double* out);
 /* Kinetic (0.5 m v.v + 0.5 I w^2) and gravitational potential (-m g.c)
@@ -369,6 +369,41 @@ typedef struct b2g_weld_def {
 int b2g_add_weld_joints(b2g_context* ctx, const b2g_weld_def* defs, int count, int* first);
 /* b2WeldJoint::SetFrequency / SetDampingRatio. */
 int b2g_set_weld_joint(b2g_context* ctx, int joint, float frequencyHz, float dampingRatio);
+
+/* b2FrictionJointDef (Box2D/Dynamics/Joints/b2FrictionJoint.h:27-53) with
+ * body indices: top-down friction, a force and a torque up to the limits
+ * against relative motion. 36 bytes. */
+typedef struct b2g_friction_def {
+    int32_t bodyA, bodyB;
+    float localAnchorA[2];
+    float localAnchorB[2];
+    float maxForce, maxTorque;
+    int32_t collideConnected;
+} b2g_friction_def;
+/* Appends friction joints (b2FrictionJoint,
+ * REF/Dynamics/Joints/b2FrictionJoint.cpp:59-195); b2g_get_joints reports
+ * the linear and angular impulse (x, y, angular, 0). */
+int b2g_add_friction_joints(b2g_context* ctx, const b2g_friction_def* defs, int count, int* first);
+/* b2FrictionJoint::SetMaxForce / SetMaxTorque. */
+int b2g_set_friction_joint(b2g_context* ctx, int joint, float maxForce, float maxTorque);
+
+/* b2MotorJointDef (Box2D/Dynamics/Joints/b2MotorJoint.h:27-56) with body
+ * indices: drives body B to an offset from body A (in A's frame) with a
+ * force and torque up to the limits. 36 bytes. */
+typedef struct b2g_motor_def {
+    int32_t bodyA, bodyB;
+    float linearOffset[2];
+    float angularOffset;
+    float maxForce, maxTorque, correctionFactor;
+    int32_t collideConnected;
+} b2g_motor_def;
+/* Appends motor joints (b2MotorJoint, REF/Dynamics/Joints/b2MotorJoint.cpp:63-205);
+ * b2g_get_joints reports the linear and angular impulse (x, y, angular, 0). */
+int b2g_add_motor_joints(b2g_context* ctx, const b2g_motor_def* defs, int count, int* first);
+/* b2MotorJoint::SetLinearOffset, SetAngularOffset, SetMaxForce, SetMaxTorque
+ * and SetCorrectionFactor in one call (bodies and collideConnected of `def`
+ * are ignored). */
+int b2g_set_motor_joint(b2g_context* ctx, int joint, const b2g_motor_def* def);
 /* Removes joints (b2World::DestroyJoint, Box2D/Dynamics/b2World.cpp:267):
  * they stop constraining and connecting islands at the next step; indices of
  * the other joints do not change. */
@@ -579,7 +614,7 @@ int b2g_get_graph_counts(b2g_context* ctx, int64_t* launches, int64_t* captures)
  * b2g_manifold, b2g_body_state, b2g_fixture_def, b2g_world_def,
  * b2g_step_params, b2g_step_stats, b2g_phase_profile, b2g_mlp_params,
  * b2g_aabb, b2g_transform, b2g_revolute_def, b2g_schedule, b2g_sweep,
- * b2g_distance_def, b2g_rope_def, b2g_weld_def.
+ * b2g_distance_def, b2g_rope_def, b2g_weld_def, b2g_friction_def, b2g_motor_def.
  * Returns the count (out may be null). Needs no device; lets bindings check
  * their struct layouts. */
 int b2g_abi_sizes(int32_t* out, int cap);

@@ -500,6 +500,76 @@ private:
     float m_referenceAngle, m_frequencyHz, m_dampingRatio;
 };
 
+// Box2D/Dynamics/Joints/b2FrictionJoint.h:27-53
+struct b2FrictionJointDef : public b2JointDef {
+    b2FrictionJointDef() { type = b2JointType::FRICTION_JOINT; }
+    // Bodies and local anchors from a world anchor point.
+    void Initialize(b2Body* bodyA, b2Body* bodyB, const b2Vec2& anchor);
+    b2Vec2 localAnchorA{0.0f, 0.0f};
+    b2Vec2 localAnchorB{0.0f, 0.0f};
+    float maxForce = 0.0f;
+    float maxTorque = 0.0f;
+};
+
+// Box2D/Dynamics/Joints/b2FrictionJoint.h:57-124: top-down friction.
+class b2FrictionJoint : public b2Joint {
+public:
+    b2Vec2 GetAnchorA() const override;
+    b2Vec2 GetAnchorB() const override;
+    const b2Vec2& GetLocalAnchorA() const { return m_localAnchorA; }
+    const b2Vec2& GetLocalAnchorB() const { return m_localAnchorB; }
+    void SetMaxForce(float force);
+    float GetMaxForce() const { return m_maxForce; }
+    void SetMaxTorque(float torque);
+    float GetMaxTorque() const { return m_maxTorque; }
+    b2Vec2 GetReactionForce(float inv_dt) const override;
+    float GetReactionTorque(float inv_dt) const override;
+
+private:
+    friend class b2World;
+    b2FrictionJoint(const b2FrictionJointDef* def, b2World* world, int32_t index);
+    b2Vec2 m_localAnchorA, m_localAnchorB;
+    float m_maxForce, m_maxTorque;
+};
+
+// Box2D/Dynamics/Joints/b2MotorJoint.h:27-56
+struct b2MotorJointDef : public b2JointDef {
+    b2MotorJointDef() { type = b2JointType::MOTOR_JOINT; }
+    // Bodies and offsets from the bodies' current transforms.
+    void Initialize(b2Body* bodyA, b2Body* bodyB);
+    b2Vec2 linearOffset{0.0f, 0.0f};   // position of body B in body A's frame
+    float angularOffset = 0.0f;        // angle of body B minus angle of body A
+    float maxForce = 1.0f;
+    float maxTorque = 1.0f;
+    float correctionFactor = 0.3f;     // position correction in [0, 1]
+};
+
+// Box2D/Dynamics/Joints/b2MotorJoint.h:61-132: drives body B to an offset
+// from body A.
+class b2MotorJoint : public b2Joint {
+public:
+    b2Vec2 GetAnchorA() const override;
+    b2Vec2 GetAnchorB() const override;
+    void SetLinearOffset(const b2Vec2& linearOffset);
+    const b2Vec2& GetLinearOffset() const { return m_linearOffset; }
+    void SetAngularOffset(float angularOffset);
+    float GetAngularOffset() const { return m_angularOffset; }
+    void SetMaxForce(float force);
+    float GetMaxForce() const { return m_maxForce; }
+    void SetMaxTorque(float torque);
+    float GetMaxTorque() const { return m_maxTorque; }
+    void SetCorrectionFactor(float factor);
+    float GetCorrectionFactor() const { return m_correctionFactor; }
+    b2Vec2 GetReactionForce(float inv_dt) const override;
+    float GetReactionTorque(float inv_dt) const override;
+
+private:
+    friend class b2World;
+    b2MotorJoint(const b2MotorJointDef* def, b2World* world, int32_t index);
+    b2Vec2 m_linearOffset;
+    float m_angularOffset, m_maxForce, m_maxTorque, m_correctionFactor;
+};
+
 // Per-step solver options of the GPU path (no reference counterpart).
 struct b2SolverOptions {
     int32_t mode = 0;                 // 0 coloured, 1 reference order
@@ -582,6 +652,8 @@ private:
     friend class b2DistanceJoint;
     friend class b2RopeJoint;
     friend class b2WeldJoint;
+    friend class b2FrictionJoint;
+    friend class b2MotorJoint;
     int32_t Upload();
     void Download() const;
     void DownloadJoints() const;

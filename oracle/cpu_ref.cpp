@@ -1248,6 +1248,19 @@ inline V3 solve33(const Mat33& m, V3 b) {
     x.z = det * dot3(m.ex, cross3(m.ey, b));
     return x;
 }
+// b2Mat22 with GetInverse and b2Mul(b2Mat22, b2Vec2), REF/Common/b2Math.h:226-238, 436-439
+struct Mat22 { V2 ex, ey; };
+inline Mat22 inverse(const Mat22& K) {
+    const float a = K.ex.x, b = K.ey.x, c = K.ex.y, d = K.ey.y;
+    float det = a * d - b * c;
+    if (det != 0.0f) det = 1.0f / det;
+    Mat22 B;
+    B.ex.x = det * d; B.ey.x = -det * b;
+    B.ex.y = -det * c; B.ey.y = det * a;
+    return B;
+}
+inline V2 mul(const Mat22& A, V2 v) { return V2{A.ex.x * v.x + A.ey.x * v.y, A.ex.y * v.x + A.ey.y * v.y}; }
+
 // b2Mat33::GetInverse22 and GetSymInverse33, REF/Common/b2Math.cpp:58-97
 inline Mat33 inverse22(const Mat33& K) {
     const float a = K.ex.x, b = K.ey.x, c = K.ex.y, d = K.ey.y;
@@ -1315,6 +1328,16 @@ struct Joint {
     V2 u;
     float distMass, gamma, bias;
     float ropeLength;
+    // b2FrictionJoint (type 4) and b2MotorJoint (type 5: linearOffset in
+    // localAnchorA, angularOffset in referenceAngle),
+    // REF/Dynamics/Joints/b2FrictionJoint.h:88-106, b2MotorJoint.h:98-119
+    float maxForce, maxTorque, correctionFactor;
+    V2 linearImpulse;
+    float angularImpulse;
+    Mat22 linearMass;
+    float angularMass;
+    V2 linearError;
+    float angularError;
     // solver temporaries
     int indexA, indexB;
     V2 rA, rB, localCenterA, localCenterB;
@@ -1503,6 +1526,9 @@ struct World {
         j.u = mk(0.0f, 0.0f);
         j.distMass = j.gamma = j.bias = 0.0f;
         j.ropeLength = 0.0f;
+        j.maxForce = d.maxForce; j.maxTorque = d.maxTorque; j.correctionFactor = d.correctionFactor;
+        j.linearImpulse = mk(0.0f, 0.0f);
+        j.angularImpulse = 0.0f;
         const int ji = (int)joints.size();
         j.prev = -1;
         j.next = jointHead;
@@ -2092,6 +2118,10 @@ struct World {
             weldInitVelocity(j, h, dtRatio);
             return;
         }
+        if (j.type == 4 || j.type == 5) {
+            frictionMotorInitVelocity(j, dtRatio);
+            return;
+        }
         const float aA = posA[j.indexA];
         V2 vA = velV[j.indexA]; float wA = velW[j.indexA];
         const float aB = posA[j.indexB];
@@ -2158,6 +2188,10 @@ struct World {
         }
         if (j.type == 3) {
             weldSolveVelocity(j);
+            return;
+        }
+        if (j.type == 4 || j.type == 5) {
+            frictionMotorSolveVelocity(j, h, 1.0f / h);
             return;
         }
         V2 vA = velV[j.indexA]; float wA = velW[j.indexA];
@@ -2385,6 +2419,88 @@ struct World {
         return len - j.length < kLinearSlop;
     }
 
+    // b2FrictionJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2FrictionJoint.cpp:59-132,
+    // and b2MotorJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2MotorJoint.cpp:63-141
+    void frictionMotorInitVelocity(Joint& j, float dtRatio) {
+        const V2 cA = posC[j.indexA], cB = posC[j.indexB];
+        const float aA = posA[j.indexA], aB = posA[j.indexB];
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        if (j.type == 5) {
+            j.rA = mul(qA, -j.localCenterA);
+            j.rB = mul(qB, -j.localCenterB);
+        } else {
+            j.rA = mul(qA, j.localAnchorA - j.localCenterA);
+            j.rB = mul(qB, j.localAnchorB - j.localCenterB);
+        }
+        const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+        Mat22 K;
+        K.ex.x = mA + mB + iA * j.rA.y * j.rA.y + iB * j.rB.y * j.rB.y;
+        K.ex.y = -iA * j.rA.x * j.rA.y - iB * j.rB.x * j.rB.y;
+        K.ey.x = K.ex.y;
+        K.ey.y = mA + mB + iA * j.rA.x * j.rA.x + iB * j.rB.x * j.rB.x;
+        j.linearMass = inverse(K);
+        j.angularMass = iA + iB;
+        if (j.angularMass > 0.0f) j.angularMass = 1.0f / j.angularMass;
+        if (j.type == 5) {
+            j.linearError = cB + j.rB - cA - j.rA - mul(qA, j.localAnchorA);
+            j.angularError = aB - aA - j.referenceAngle;
+        }
+        if (warmStarting) {
+            j.linearImpulse *= dtRatio;
+            j.angularImpulse *= dtRatio;
+            const V2 P = j.linearImpulse;
+            vA -= mA * P;
+            wA -= iA * (cross(j.rA, P) + j.angularImpulse);
+            vB += mB * P;
+            wB += iB * (cross(j.rB, P) + j.angularImpulse);
+        } else {
+            j.linearImpulse = mk(0.0f, 0.0f);
+            j.angularImpulse = 0.0f;
+        }
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2FrictionJoint::SolveVelocityConstraints, REF b2FrictionJoint.cpp:134-189,
+    // and b2MotorJoint::SolveVelocityConstraints, REF b2MotorJoint.cpp:143-199
+    void frictionMotorSolveVelocity(Joint& j, float h, float invH) {
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+        const bool motor = j.type == 5;
+        {
+            const float Cdot = motor ? wB - wA + invH * j.correctionFactor * j.angularError : wB - wA;
+            float impulse = -j.angularMass * Cdot;
+            const float oldImpulse = j.angularImpulse;
+            const float maxImpulse = h * j.maxTorque;
+            j.angularImpulse = clampf(j.angularImpulse + impulse, -maxImpulse, maxImpulse);
+            impulse = j.angularImpulse - oldImpulse;
+            wA -= iA * impulse;
+            wB += iB * impulse;
+        }
+        {
+            V2 Cdot = vB + cross(wB, j.rB) - vA - cross(wA, j.rA);
+            if (motor) Cdot = Cdot + invH * j.correctionFactor * j.linearError;
+            V2 impulse = -mul(j.linearMass, Cdot);
+            const V2 oldImpulse = j.linearImpulse;
+            j.linearImpulse += impulse;
+            const float maxImpulse = h * j.maxForce;
+            if (lengthSq(j.linearImpulse) > maxImpulse * maxImpulse) {
+                normalize(j.linearImpulse);
+                j.linearImpulse *= maxImpulse;
+            }
+            impulse = j.linearImpulse - oldImpulse;
+            vA -= mA * impulse;
+            wA -= iA * cross(j.rA, impulse);
+            vB += mB * impulse;
+            wB += iB * cross(j.rB, impulse);
+        }
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
     // The 3x3 K of b2WeldJoint (REF/Dynamics/Joints/b2WeldJoint.cpp:95-104)
     static Mat33 weldK(float mA, float iA, float mB, float iB, V2 rA, V2 rB) {
         Mat33 K;
@@ -2529,6 +2645,7 @@ struct World {
         if (j.type == 1) return distanceSolvePosition(j);
         if (j.type == 2) return ropeSolvePosition(j);
         if (j.type == 3) return weldSolvePosition(j);
+        if (j.type == 4 || j.type == 5) return true;  // b2FrictionJoint / b2MotorJoint: no position pass
         V2 cA = posC[j.indexA]; float aA = posA[j.indexA];
         V2 cB = posC[j.indexB]; float aB = posA[j.indexB];
         float angularError = 0.0f, positionError = 0.0f;
@@ -3185,6 +3302,41 @@ int orc_world_destroy_joint(void* h, int joint) {
 
 // Per joint in creation order: impulse x, y, z and motor impulse.
 // b2DistanceJoint::SetLength / SetFrequency / SetDampingRatio.
+// b2FrictionJoint::SetMaxForce / SetMaxTorque.
+int orc_world_set_friction(void* h, int joint, float maxForce, float maxTorque) {
+    World* w = (World*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint].removed || w->joints[joint].type != 4)
+        return B2G_ERR_ARG;
+    w->joints[joint].maxForce = maxForce;
+    w->joints[joint].maxTorque = maxTorque;
+    return B2G_OK;
+}
+
+// b2MotorJoint::SetLinearOffset / SetAngularOffset (waking both bodies
+// when the offset changes, REF/Dynamics/Joints/b2MotorJoint.cpp:261-284),
+// SetMaxForce, SetMaxTorque, SetCorrectionFactor.
+int orc_world_set_motor(void* h, int joint, float ox, float oy, float angle, float maxForce, float maxTorque,
+                        float factor) {
+    World* w = (World*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint].removed || w->joints[joint].type != 5)
+        return B2G_ERR_ARG;
+    Joint& j = w->joints[joint];
+    if (ox != j.localAnchorA.x || oy != j.localAnchorA.y) {
+        w->setAwake(w->bodies[j.bodyA], true);
+        w->setAwake(w->bodies[j.bodyB], true);
+        j.localAnchorA = mk(ox, oy);
+    }
+    if (angle != j.referenceAngle) {
+        w->setAwake(w->bodies[j.bodyA], true);
+        w->setAwake(w->bodies[j.bodyB], true);
+        j.referenceAngle = angle;
+    }
+    j.maxForce = maxForce;
+    j.maxTorque = maxTorque;
+    j.correctionFactor = factor;
+    return B2G_OK;
+}
+
 // b2RopeJoint::SetMaxLength.
 int orc_world_set_rope(void* h, int joint, float maxLength) {
     World* w = (World*)h;
@@ -3221,6 +3373,13 @@ void orc_world_get_joints(void* h, float* out) {
         const Joint& j = w->joints[i];
         if (j.removed) {  // destroyed
             for (int k = 0; k < 4; ++k) out[4 * i + k] = 0.0f;
+            continue;
+        }
+        if (j.type == 4 || j.type == 5) {  // reaction force and torque at inv_dt = 1
+            out[4 * i] = j.linearImpulse.x;
+            out[4 * i + 1] = j.linearImpulse.y;
+            out[4 * i + 2] = j.angularImpulse;
+            out[4 * i + 3] = 0.0f;
             continue;
         }
         if (j.type == 1 || j.type == 2) {  // reaction force at inv_dt = 1: impulse along u

@@ -101,6 +101,31 @@ int loadJoints(RefWorld* w, int base, const b2scene::JointDef* jd, int nj) {
             w->joints.push_back(w->world->CreateJoint(&rd));
             continue;
         }
+        if (d.type == 4) {
+            b2FrictionJointDef fd;
+            fd.bodyA = w->bodies[base + d.bodyA];
+            fd.bodyB = w->bodies[base + d.bodyB];
+            fd.collideConnected = d.collideConnected != 0;
+            fd.localAnchorA.Set(d.ax, d.ay);
+            fd.localAnchorB.Set(d.bx, d.by);
+            fd.maxForce = d.maxForce;
+            fd.maxTorque = d.maxTorque;
+            w->joints.push_back(w->world->CreateJoint(&fd));
+            continue;
+        }
+        if (d.type == 5) {
+            b2MotorJointDef md;
+            md.bodyA = w->bodies[base + d.bodyA];
+            md.bodyB = w->bodies[base + d.bodyB];
+            md.collideConnected = d.collideConnected != 0;
+            md.linearOffset.Set(d.ax, d.ay);
+            md.angularOffset = d.referenceAngle;
+            md.maxForce = d.maxForce;
+            md.maxTorque = d.maxTorque;
+            md.correctionFactor = d.correctionFactor;
+            w->joints.push_back(w->world->CreateJoint(&md));
+            continue;
+        }
         if (d.type == 3) {
             b2WeldJointDef wd;
             wd.bodyA = w->bodies[base + d.bodyA];
@@ -381,6 +406,30 @@ int ref_world_set_distance(void* h, int joint, float length, float hz, float rat
     return 0;
 }
 
+int ref_world_set_friction(void* h, int joint, float maxForce, float maxTorque) {
+    RefWorld* w = (RefWorld*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
+    if (w->joints[joint]->GetType() != e_frictionJoint) return -1;
+    b2FrictionJoint* j = static_cast<b2FrictionJoint*>(w->joints[joint]);
+    j->SetMaxForce(maxForce);
+    j->SetMaxTorque(maxTorque);
+    return 0;
+}
+
+int ref_world_set_motor(void* h, int joint, float ox, float oy, float angle, float maxForce, float maxTorque,
+                        float factor) {
+    RefWorld* w = (RefWorld*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
+    if (w->joints[joint]->GetType() != e_motorJoint) return -1;
+    b2MotorJoint* j = static_cast<b2MotorJoint*>(w->joints[joint]);
+    j->SetLinearOffset(b2Vec2(ox, oy));
+    j->SetAngularOffset(angle);
+    j->SetMaxForce(maxForce);
+    j->SetMaxTorque(maxTorque);
+    j->SetCorrectionFactor(factor);
+    return 0;
+}
+
 int ref_world_set_rope(void* h, int joint, float maxLength) {
     RefWorld* w = (RefWorld*)h;
     if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
@@ -416,7 +465,8 @@ void ref_world_get_joints(void* h, float* out) {
             continue;
         }
         const b2JointType type = w->joints[i]->GetType();
-        if (type == e_distanceJoint || type == e_ropeJoint || type == e_weldJoint) {
+        if (type == e_distanceJoint || type == e_ropeJoint || type == e_weldJoint || type == e_frictionJoint ||
+            type == e_motorJoint) {
             const b2Vec2 f = w->joints[i]->GetReactionForce(1.0f);
             out[4 * i] = f.x;
             out[4 * i + 1] = f.y;

@@ -37,7 +37,7 @@ def test_struct_layouts_match_the_library():
     sizes = np.zeros(n, np.int32)
     lib.b2g_abi_sizes(E.ptr(sizes), n)
     shape, manifold, body, fixture, world, params, stats, profile, mlp, aabb, xf, revolute, schedule, sweep, distance, \
-        rope, weld = sizes[:17]
+        rope, weld, friction, motor = sizes[:19]
     assert E.SHAPE.itemsize == shape
     assert E.MANIFOLD.itemsize == manifold
     assert E.STATS.itemsize == stats
@@ -54,4 +54,6 @@ def test_struct_layouts_match_the_library():
     assert distance == 40  # 2 body indices, 7 floats, 1 switch
     assert rope == 32      # 2 body indices, 5 floats, 1 switch
     assert weld == 40      # 2 body indices, 7 floats, 1 switch
+    assert friction == 36  # 2 body indices, 6 floats, 1 switch
+    assert motor == 36     # 2 body indices, 6 floats, 1 switch
     assert params == E.STEP_PARAMS.itemsize

@@ -301,13 +301,17 @@ def test_distance_rods_hold(gpu):
 @pytest.mark.parametrize("kind,n,seed,edit", [
     ("ropes", 16, 1, ("world_set_rope", 0, F(2.0))),
     ("welds", 12, 2, ("world_set_weld", 1, F(0.0), F(0.0))),
+    ("friction", 16, 0, ("world_set_friction", 2, F(0.0), F(0.0))),
+    ("motor", 12, 1, ("world_set_motor", 0, F(-8.0), F(6.0), F(2.0), F(30.0), F(8.0), F(0.4))),
 ])
-def test_rope_and_weld_joints_bitexact(gpu, orc, wide_trig, kind, n, seed, edit):
-    """Rope and weld joints, one per island: boxes on ropes from the ground
-    (slack, taut, one of length zero) and single-link beams on rigid and soft
-    welds (one with a fixed-rotation link): bodies and the joints' reaction
-    forces and torques equal the oracle's bit for bit, also after
-    SetMaxLength / SetFrequency through the box2d:: API."""
+def test_rope_weld_friction_motor_joints_bitexact(gpu, orc, wide_trig, kind, n, seed, edit):
+    """Rope, weld, friction and motor joints, one per island (or joints on
+    disjoint bodies where boxes collide): boxes on ropes from the ground
+    (slack, taut, one of length zero), single-link beams on rigid and soft
+    welds (one with a fixed-rotation link), boxes thrown against friction
+    joints and boxes driven to offsets by motor joints: bodies and the
+    joints' reaction forces and torques equal the oracle's bit for bit, also
+    after the setters through the box2d:: API."""
     g = gpu.world(kind, n, seed, sleep=True)
     o = orc.world(kind, n, seed, sleep=True)
     for s in range(300):

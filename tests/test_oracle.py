@@ -37,6 +37,9 @@ F = ctypes.c_float
     ("welds", 12, 0, 400, True, 8, 3),         # weld joints: rigid and soft beams, a fixed-rotation link
     ("welds", 10, 0, 250, False, 4, 2),
     ("welds", 12, 2, 300, True, 8, 3),         # single-link beams: soft welds swing, a fixed-rotation rigid weld
+    ("friction", 16, 0, 300, True, 8, 3),      # friction joints: boxes thrown along the ground, some colliding
+    ("motor", 12, 0, 400, True, 8, 3),         # motor joints: boxes driven to offsets from the ground and each other
+    ("motor", 9, 1, 300, False, 4, 2),
 ])
 def test_world_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, vel, pos):
     a = ref.world(kind, n, seed, sleep=sleep)
@@ -670,4 +673,76 @@ def test_weld_cantilever_statics(orc):
         arm = sum(0.5 + q for q in range(4 - k))
         assert abs(j[k, 1] - 2.5 * (4 - k)) < 0.05 * 2.5 * (4 - k), (k, j[k])
         assert abs(j[k, 2] - 2.5 * arm) < 0.05 * 2.5 * arm, (k, j[k], arm)
+    w.close()
+
+
+def test_friction_and_motor_edits_bitexact(ref, orc):
+    """b2FrictionJoint::SetMaxForce / SetMaxTorque and b2MotorJoint's
+    offset, force, torque and correction setters mid-run (a box set free, a
+    box held harder, a target moved and turned) against the reference's
+    setters, with the joints' reaction forces and torques."""
+    for kind, n, edits in (
+            ("friction", 8, lambda w: (w.e.fn("world_set_friction")(w.h, 0, F(0.0), F(0.0)),
+                                       w.e.fn("world_set_friction")(w.h, 3, F(20.0), F(4.0)))),
+            ("motor", 6, lambda w: (w.e.fn("world_set_motor")(w.h, 0, F(-8.0), F(6.0), F(2.0), F(30.0), F(8.0),
+                                                               F(0.4)),
+                                    w.e.fn("world_set_motor")(w.h, 2, F(6.0), F(0.5), F(1.5), F(5.0), F(1.0),
+                                                               F(0.2))))):
+        a = ref.world(kind, n, 0, sleep=True)
+        b = orc.world(kind, n, 0, sleep=True)
+        a.step(90); b.step(90)
+        for w in (a, b):
+            assert edits(w) == (0, 0)
+        for s in range(200):
+            a.step(1); b.step(1)
+            assert np.array_equal(a.bodies().view(np.uint32), b.bodies().view(np.uint32)), f"{kind} step {s + 91}"
+        nj = a.e.fn("world_joint_count")(a.h)
+        ja, jb = np.zeros(4 * nj, np.float32), np.zeros(4 * nj, np.float32)
+        a.e.fn("world_get_joints")(a.h, E.ptr(ja))
+        b.e.fn("world_get_joints")(b.h, E.ptr(jb))
+        assert np.array_equal(ja.view(np.uint32), jb.view(np.uint32))
+        a.close(); b.close()
+
+
+def test_friction_joint_closed_form(orc):
+    """Without gravity, box 0 of the friction scene (1 kg, 1 x 1 m, I = 1/6,
+    thrown at 2 m/s with 1 rad/s, maxForce 2 N, maxTorque 0.5 N m) slows at
+    exactly F / m and T / I: speed 2 - 2 k h and spin 1 - 3 k h after k steps,
+    until it stops (at k = 60 and k = 20) and stays stopped."""
+    w = orc.world("friction", 16, 0, sleep=False, gravity=(0.0, 0.0))
+    h = 1.0 / 60.0
+    for k in range(1, 91):
+        w.step(1)
+        b = w.bodies()[1]
+        speed, spin = float(np.hypot(b[E.VX], b[E.VY])), float(b[E.W])
+        want_v, want_w = max(2.0 - 2.0 * k * h, 0.0), max(1.0 - 3.0 * k * h, 0.0)
+        assert abs(speed - want_v) < 1e-4 and abs(spin - want_w) < 1e-4, (k, speed, want_v, spin, want_w)
+    w.close()
+
+
+def test_motor_joint_reaches_offsets(orc):
+    """Without gravity every box of the motor scene ends at its target: the
+    linear offset from its body A (in A's frame) within 1 mm and the angular
+    offset within 1e-3 rad, whether driven from the ground or from the
+    previous box."""
+    n = 12
+    w = orc.world("motor", n, 0, sleep=False, gravity=(0.0, 0.0))
+    w.step(900)
+    bodies = w.bodies()
+    for i in range(n):
+        b = bodies[1 + i]
+        from_prev = i % 3 == 2
+        x = 6.0 * i - 3.0 * n
+        if from_prev:
+            a = bodies[i]
+            s, c = float(a[E.QS]), float(a[E.QC])
+            off = (6.0, 0.5)
+            target = np.array([a[E.PX] + c * off[0] - s * off[1], a[E.PY] + s * off[0] + c * off[1]])
+            base_angle = float(a[E.ANG])
+        else:
+            target = np.array([x + 1.5 * (1 + i % 2), 4.0 + 1.0 + 0.5 * (i % 3)])
+            base_angle = 0.0
+        angle = base_angle + 0.5 + 0.5 * (i % 3)
+        assert np.hypot(*(b[[E.PX, E.PY]] - target)) < 1e-3, (i, b[[E.PX, E.PY]], target)
+        assert abs(float(b[E.ANG]) - angle) < 1e-3, (i, b[E.ANG], angle)
     w.close()
