@@ -107,7 +107,7 @@ enum : int {
     JF_LIMIT = 1, JF_MOTOR = 2, JF_COLLIDE = 4, JF_REMOVED = 8,
     JL_INACTIVE = 0, JL_LOWER = 1, JL_UPPER = 2, JL_EQUAL = 3,  // limit state in bits 8-9
     JT_REVOLUTE = 0, JT_DISTANCE = 1, JT_ROPE = 2, JT_WELD = 3,  // joint type in bits 12-15
-    JT_FRICTION = 4, JT_MOTOR = 5,
+    JT_FRICTION = 4, JT_MOTOR = 5, JT_PULLEY = 6, JT_MOUSE = 7,
 };
 __host__ __device__ inline int jointType(int flags) { return (flags >> 12) & 15; }
 struct JointView {
@@ -137,7 +137,12 @@ struct JointView {
 // limits = maxForce, maxTorque; impulse = linear x y, angular; K0 = linear
 // mass m11 m12 m22, angular mass. Motor: anchors = linearOffset x y,
 // angularOffset, correctionFactor; limits, impulse and K0 as friction; K1 =
-// linear error, motorMass = angular error.
+// linear error, motorMass = angular error. Pulley: limits = ground anchors A,
+// B; motorSpeed = ratio, motorMass = the constant lengthA + ratio lengthB;
+// impulse = (impulse, uB.x, uB.y); K0 = (uA, uB), K1.x = mass. Mouse:
+// anchors = target, localAnchorB; limits = maxForce, frequencyHz,
+// dampingRatio, mass of body B; impulse x y; K0 = mass m11 m12 m22, gamma;
+// K1 = beta C.
 
 __device__ inline int encodeOrdered(float f) {
     int i = __float_as_int(f);
@@ -1854,6 +1859,202 @@ __device__ inline void frictionMotorSolveVelocityOne(const WorldView& w, const J
     if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
 }
 
+// Unit vector from a ground anchor to a body anchor, zero when closer than
+// 10 linearSlop, and the distance (b2PulleyJoint,
+// REF/Dynamics/Joints/b2PulleyJoint.cpp:103-126).
+__device__ inline float pulleyDir(V2& u) {
+    const float len = lengthOf(u);
+    if (len > 10.0f * kLinearSlop) u *= 1.0f / len;
+    else u = mk(0.0f, 0.0f);
+    return len;
+}
+
+// b2PulleyJoint::InitVelocityConstraints, REF b2PulleyJoint.cpp:76-164
+__device__ inline void pulleyInitOne(const WorldView& w, const JointView& jv, int j, int2 b, float dtRatio, int warm) {
+    const float4 msA = w.mass[b.x], msB = w.mass[b.y];
+    const float imA = msA.x, iiA = msA.y, imB = msB.x, iiB = msB.y;
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const Rot qA = rotOf(pA.z), qB = rotOf(pB.z);
+    const float4 an = jv.anchors[j];
+    const float4 g = jv.limits[j];  // ground anchors A, B
+    const float ratio = jv.motorSpeed[j];
+    const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    V2 uA = mk(pA.x, pA.y) + rA - mk(g.x, g.y);
+    V2 uB = mk(pB.x, pB.y) + rB - mk(g.z, g.w);
+    pulleyDir(uA);
+    pulleyDir(uB);
+    const float ruA = cross(rA, uA), ruB = cross(rB, uB);
+    const float mA = imA + iiA * ruA * ruA;
+    const float mB = imB + iiB * ruB * ruB;
+    float mass = mA + ratio * ratio * mB;
+    if (mass > 0.0f) mass = 1.0f / mass;
+    float4 imp = jv.impulse[j];
+    if (warm) {
+        imp.x *= dtRatio;
+        const V2 PA = -(imp.x) * uA;
+        const V2 PB = (-ratio * imp.x) * uB;
+        vA += imA * PA;
+        wA += iiA * cross(rA, PA);
+        vB += imB * PB;
+        wB += iiB * cross(rB, PB);
+    } else {
+        imp.x = 0.0f;
+    }
+    jv.impulse[j] = make_float4(imp.x, uB.x, uB.y, 0.0f);  // uB for b2PulleyJoint::GetReactionForce
+    jv.r[j] = make_float4(rA.x, rA.y, rB.x, rB.y);
+    jv.masses[j] = make_float4(imA, iiA, imB, iiB);
+    jv.K0[j] = make_float4(uA.x, uA.y, uB.x, uB.y);
+    jv.K1[j] = make_float2(mass, 0.0f);
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2PulleyJoint::SolveVelocityConstraints, REF b2PulleyJoint.cpp:166-191
+__device__ inline void pulleySolveVelocityOne(const WorldView& w, const JointView& jv, int j, int2 b) {
+    const float4 ms = jv.masses[j];
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const float4 r = jv.r[j];
+    const V2 rA = mk(r.x, r.y), rB = mk(r.z, r.w);
+    const float4 k0 = jv.K0[j];
+    const V2 uA = mk(k0.x, k0.y), uB = mk(k0.z, k0.w);
+    const float ratio = jv.motorSpeed[j];
+    const V2 vpA = vA + cross(wA, rA);
+    const V2 vpB = vB + cross(wB, rB);
+    const float Cdot = -dot(uA, vpA) - ratio * dot(uB, vpB);
+    const float impulse = -jv.K1[j].x * Cdot;
+    float4 imp = jv.impulse[j];
+    imp.x += impulse;
+    const V2 PA = -impulse * uA;
+    const V2 PB = -ratio * impulse * uB;
+    vA += ms.x * PA;
+    wA += ms.y * cross(rA, PA);
+    vB += ms.z * PB;
+    wB += ms.w * cross(rB, PB);
+    jv.impulse[j] = imp;
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2PulleyJoint::SolvePositionConstraints, REF b2PulleyJoint.cpp:193-263:
+// the constant is lengthA + ratio lengthB at creation.
+__device__ inline bool pulleySolvePositionOne(const WorldView& w, const JointView& jv, int j, int2 b) {
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    V2 cA = mk(pA.x, pA.y), cB = mk(pB.x, pB.y);
+    float aA = pA.z, aB = pB.z;
+    const float4 ms = jv.masses[j];
+    const Rot qA = rotOf(aA), qB = rotOf(aB);
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float4 an = jv.anchors[j];
+    const float4 g = jv.limits[j];
+    const float ratio = jv.motorSpeed[j];
+    const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    V2 uA = cA + rA - mk(g.x, g.y);
+    V2 uB = cB + rB - mk(g.z, g.w);
+    const float lengthA = pulleyDir(uA);
+    const float lengthB = pulleyDir(uB);
+    const float ruA = cross(rA, uA), ruB = cross(rB, uB);
+    const float mA = ms.x + ms.y * ruA * ruA;
+    const float mB = ms.z + ms.w * ruB * ruB;
+    float mass = mA + ratio * ratio * mB;
+    if (mass > 0.0f) mass = 1.0f / mass;
+    const float C = jv.motorMass[j] - lengthA - ratio * lengthB;
+    const float linearError = absB(C);
+    const float impulse = -mass * C;
+    const V2 PA = -impulse * uA;
+    const V2 PB = -ratio * impulse * uB;
+    cA += ms.x * PA;
+    aA += ms.y * cross(rA, PA);
+    cB += ms.z * PB;
+    aB += ms.w * cross(rB, PB);
+    if (w.btype[b.x] == BT_DYNAMIC) w.pos[b.x] = make_float4(cA.x, cA.y, aA, pA.w);
+    if (w.btype[b.y] == BT_DYNAMIC) w.pos[b.y] = make_float4(cB.x, cB.y, aB, pB.w);
+    return linearError < kLinearSlop;
+}
+
+// b2MouseJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2MouseJoint.cpp:99-169:
+// a soft point constraint pulling body B's anchor to the target; body A is
+// not touched.
+__device__ inline void mouseInitOne(const WorldView& w, const JointView& jv, int j, int2 b, float dtRatio, int warm) {
+    const float4 msB = w.mass[b.y];
+    const float imB = msB.x, iiB = msB.y;
+    const float2 lcB = w.lcenter[b.y];
+    const float4 pB = w.pos[b.y];
+    const float4 vbv = w.vel[b.y];
+    V2 vB = mk(vbv.x, vbv.y);
+    float wB = vbv.z;
+    const Rot qB = rotOf(pB.z);
+    const float4 an = jv.anchors[j];   // target, localAnchorB
+    const float4 par = jv.limits[j];   // maxForce, frequencyHz, dampingRatio, mass of body B
+    const float mass = par.w;
+    const float omega = 2.0f * kPi * par.y;
+    const float d = 2.0f * mass * par.z * omega;
+    const float k = mass * (omega * omega);
+    const float h = jv.h;
+    float gamma = h * (d + h * k);
+    if (gamma != 0.0f) gamma = 1.0f / gamma;
+    const float beta = h * k * gamma;
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    const float a = imB + iiB * rB.y * rB.y + gamma;
+    const float c = -iiB * rB.x * rB.y;
+    const float bb = c;
+    const float dd = imB + iiB * rB.x * rB.x + gamma;
+    float det = a * dd - bb * c;
+    if (det != 0.0f) det = 1.0f / det;
+    V2 C = mk(pB.x, pB.y) + rB - mk(an.x, an.y);
+    C *= beta;
+    wB *= 0.98f;
+    float4 imp = jv.impulse[j];
+    if (warm) {
+        imp.x *= dtRatio;
+        imp.y *= dtRatio;
+        vB += imB * mk(imp.x, imp.y);
+        wB += iiB * cross(rB, mk(imp.x, imp.y));
+    } else {
+        imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    jv.impulse[j] = imp;
+    jv.r[j] = make_float4(0.0f, 0.0f, rB.x, rB.y);
+    jv.masses[j] = make_float4(0.0f, 0.0f, imB, iiB);
+    jv.K0[j] = make_float4(det * dd, -det * bb, det * a, gamma);
+    jv.K1[j] = make_float2(C.x, C.y);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2MouseJoint::SolveVelocityConstraints, REF b2MouseJoint.cpp:171-194
+__device__ inline void mouseSolveVelocityOne(const WorldView& w, const JointView& jv, int j, int2 b, float h) {
+    const float4 ms = jv.masses[j];
+    const float4 vbv = w.vel[b.y];
+    V2 vB = mk(vbv.x, vbv.y);
+    float wB = vbv.z;
+    const float4 r = jv.r[j];
+    const V2 rB = mk(r.z, r.w);
+    const float4 k0 = jv.K0[j];
+    const float2 C = jv.K1[j];
+    float4 imp = jv.impulse[j];
+    const V2 Cdot = vB + cross(wB, rB);
+    const V2 rhs = -(Cdot + mk(C.x, C.y) + k0.w * mk(imp.x, imp.y));
+    V2 impulse = mk(k0.x * rhs.x + k0.y * rhs.y, k0.y * rhs.x + k0.z * rhs.y);  // b2Mul(mass, rhs)
+    const V2 oldImpulse = mk(imp.x, imp.y);
+    V2 acc = oldImpulse + impulse;
+    const float maxImpulse = h * jv.limits[j].x;
+    if (lengthSq(acc) > maxImpulse * maxImpulse) acc *= maxImpulse / lengthOf(acc);
+    impulse = acc - oldImpulse;
+    imp.x = acc.x;
+    imp.y = acc.y;
+    vB += ms.z * impulse;
+    wB += ms.w * cross(rB, impulse);
+    jv.impulse[j] = imp;
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
 // b2RevoluteJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2RevoluteJoint.cpp:66-198
 __device__ inline void jointInitOne(const WorldView& w, const JointView& jv, int j, float dtRatio, int warm) {
     const int2 b = jv.bodies[j];
@@ -1873,6 +2074,14 @@ __device__ inline void jointInitOne(const WorldView& w, const JointView& jv, int
     }
     if (type == JT_FRICTION || type == JT_MOTOR) {
         frictionMotorInitOne(w, jv, j, b, dtRatio, warm, type == JT_MOTOR);
+        return;
+    }
+    if (type == JT_PULLEY) {
+        pulleyInitOne(w, jv, j, b, dtRatio, warm);
+        return;
+    }
+    if (type == JT_MOUSE) {
+        mouseInitOne(w, jv, j, b, dtRatio, warm);
         return;
     }
     const float4 msA = w.mass[b.x], msB = w.mass[b.y];
@@ -1973,6 +2182,14 @@ __device__ inline void jointSolveVelocityOne(const WorldView& w, const JointView
         frictionMotorSolveVelocityOne(w, jv, j, b, h, type == JT_MOTOR);
         return;
     }
+    if (type == JT_PULLEY) {
+        pulleySolveVelocityOne(w, jv, j, b);
+        return;
+    }
+    if (type == JT_MOUSE) {
+        mouseSolveVelocityOne(w, jv, j, b, h);
+        return;
+    }
     const float4 ms = jv.masses[j];
     const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
     const float4 va = w.vel[b.x], vbv = w.vel[b.y];
@@ -2066,10 +2283,11 @@ __device__ inline void jointSolvePositionOne(const WorldView& w, const JointView
     const int root = w.islandOf[w.btype[b.x] != BT_STATIC ? b.x : b.y];
     if (it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0) return;  // island done
     const int type = jointType(jv.flags[j]);
-    if (type == JT_FRICTION || type == JT_MOTOR) return;  // no position pass (returns true)
+    if (type == JT_FRICTION || type == JT_MOTOR || type == JT_MOUSE) return;  // no position pass (returns true)
     if (type != JT_REVOLUTE) {
         const bool ok = type == JT_DISTANCE ? distanceSolvePositionOne(w, jv, j, b)
                         : type == JT_ROPE   ? ropeSolvePositionOne(w, jv, j, b)
+                        : type == JT_PULLEY ? pulleySolvePositionOne(w, jv, j, b)
                                             : weldSolvePositionOne(w, jv, j, b);
         if (!ok) {
             int* f = &w.islandMinSep[it * w.nBodies + root];
@@ -2186,8 +2404,15 @@ __global__ void k_joint_drift(WorldView w, JointView jv, float* out) {
         out[j] = fmaxf(gap - jv.limits[j].x, 0.0f);
         return;
     }
-    if (type == JT_FRICTION || type == JT_MOTOR) {  // no positional constraint
+    if (type == JT_FRICTION || type == JT_MOTOR || type == JT_MOUSE) {  // no positional constraint
         out[j] = -1.0f;
+        return;
+    }
+    if (type == JT_PULLEY) {  // |lengthA + ratio lengthB - constant|
+        const float4 g = jv.limits[j];
+        const float la = lengthOf(mk(pA.x, pA.y) + rA - mk(g.x, g.y));
+        const float lb = lengthOf(mk(pB.x, pB.y) + rB - mk(g.z, g.w));
+        out[j] = absB(la + jv.motorSpeed[j] * lb - jv.motorMass[j]);
         return;
     }
     out[j] = gap;  // revolute and weld: the anchors' separation
@@ -6759,7 +6984,7 @@ int b2g_abi_sizes(int32_t* out, int cap) {
         (int32_t)sizeof(b2g_aabb),        (int32_t)sizeof(b2g_transform),    (int32_t)sizeof(b2g_revolute_def),
         (int32_t)sizeof(b2g_schedule),    (int32_t)sizeof(b2g_sweep),        (int32_t)sizeof(b2g_distance_def),
         (int32_t)sizeof(b2g_rope_def),    (int32_t)sizeof(b2g_weld_def),     (int32_t)sizeof(b2g_friction_def),
-        (int32_t)sizeof(b2g_motor_def),
+        (int32_t)sizeof(b2g_motor_def),   (int32_t)sizeof(b2g_pulley_def),   (int32_t)sizeof(b2g_mouse_def),
     };
     const int n = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (!out) return n;
@@ -7252,6 +7477,62 @@ int b2g_set_motor_joint(b2g_context* ctx, int joint, const b2g_motor_def* d) {
     int rc = setJointParams(ctx, joint, JT_MOTOR, "not a motor joint", make_float4(d->maxForce, d->maxTorque, 0.0f, 0.0f));
     if (rc != B2G_OK) return rc;
     const float4 an = make_float4(d->linearOffset[0], d->linearOffset[1], d->angularOffset, d->correctionFactor);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jAnchors + joint, &an, sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
+int b2g_add_pulley_joints(b2g_context* ctx, const b2g_pulley_def* defs, int count, int* first) {
+    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
+    std::vector<JointRow> rows;
+    for (int i = 0; i < count; ++i) {
+        const b2g_pulley_def& d = defs[i];
+        if (!(d.ratio > 0.0f)) return fail(ctx, B2G_ERR_ARG, "pulley ratio must be positive");
+        rows.push_back(jointRow(d, JT_PULLEY,
+                                make_float4(d.groundAnchorA[0], d.groundAnchorA[1], d.groundAnchorB[0],
+                                            d.groundAnchorB[1])));
+    }
+    const int n0 = ctx->nJoints;
+    int rc = appendJointRows(ctx, rows, first);
+    if (rc != B2G_OK || count == 0) return rc;
+    // ratio and the constant lengthA + ratio lengthB (b2PulleyJoint ctor,
+    // REF/Dynamics/Joints/b2PulleyJoint.cpp:57-75)
+    std::vector<float> ratio(count), constant(count);
+    for (int i = 0; i < count; ++i) {
+        ratio[i] = defs[i].ratio;
+        constant[i] = defs[i].lengthA + defs[i].ratio * defs[i].lengthB;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jMotorSpeed + n0, ratio.data(), sizeof(float) * count, hipMemcpyHostToDevice,
+                               ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jMotorMass + n0, constant.data(), sizeof(float) * count, hipMemcpyHostToDevice,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
+int b2g_add_mouse_joints(b2g_context* ctx, const b2g_mouse_def* defs, int count, int* first) {
+    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
+    std::vector<JointRow> rows;
+    for (int i = 0; i < count; ++i) {
+        const b2g_mouse_def& d = defs[i];
+        rows.push_back(JointRow{d.bodyA, d.bodyB,
+                                make_float4(d.target[0], d.target[1], d.localAnchorB[0], d.localAnchorB[1]),
+                                make_float4(d.maxForce, d.frequencyHz, d.dampingRatio, d.massB),
+                                (JT_MOUSE << 12) | (d.collideConnected ? JF_COLLIDE : 0)});
+    }
+    return appendJointRows(ctx, rows, first);
+}
+
+int b2g_set_mouse_joint(b2g_context* ctx, int joint, const b2g_mouse_def* d) {
+    if (!ctx || !d) return B2G_ERR_ARG;
+    int rc = setJointParams(ctx, joint, JT_MOUSE, "not a mouse joint",
+                            make_float4(d->maxForce, d->frequencyHz, d->dampingRatio, d->massB));
+    if (rc != B2G_OK) return rc;
+    float4 an;
+    HIPCHK(ctx, hipMemcpyAsync(&an, ctx->jAnchors + joint, sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    an.x = d->target[0];
+    an.y = d->target[1];
     HIPCHK(ctx, hipMemcpyAsync(ctx->jAnchors + joint, &an, sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return B2G_OK;

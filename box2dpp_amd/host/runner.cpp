@@ -153,6 +153,38 @@ int loadJoints(Runner* r, int base, const b2scene::JointDef* jd, int nj) {
             r->joints.push_back(j);
             continue;
         }
+        if (d.type == 6) {
+            b2PulleyJointDef pd;
+            pd.bodyA = r->bodies[base + d.bodyA];
+            pd.bodyB = r->bodies[base + d.bodyB];
+            pd.collideConnected = d.collideConnected != 0;
+            pd.groundAnchorA = b2Vec2(d.gax, d.gay);
+            pd.groundAnchorB = b2Vec2(d.gbx, d.gby);
+            pd.localAnchorA = b2Vec2(d.ax, d.ay);
+            pd.localAnchorB = b2Vec2(d.bx, d.by);
+            pd.lengthA = d.length;
+            pd.lengthB = d.lengthB;
+            pd.ratio = d.ratio;
+            b2Joint* j = r->world.CreateJoint(&pd);
+            if (j == nullptr) return B2G_ERR_ARG;
+            r->joints.push_back(j);
+            continue;
+        }
+        if (d.type == 7) {  // created on the grab point, then pulled to the target
+            b2MouseJointDef md;
+            md.bodyA = r->bodies[base + d.bodyA];
+            md.bodyB = r->bodies[base + d.bodyB];
+            md.collideConnected = d.collideConnected != 0;
+            md.target = md.bodyB->GetWorldPoint(b2Vec2(d.bx, d.by));
+            md.maxForce = d.maxForce;
+            md.frequencyHz = d.frequencyHz;
+            md.dampingRatio = d.dampingRatio;
+            b2Joint* j = r->world.CreateJoint(&md);
+            if (j == nullptr) return B2G_ERR_ARG;
+            static_cast<b2MouseJoint*>(j)->SetTarget(b2Vec2(d.ax, d.ay));
+            r->joints.push_back(j);
+            continue;
+        }
         if (d.type == 4 || d.type == 5) {
             b2FrictionJointDef fd;
             b2MotorJointDef md;
@@ -297,6 +329,19 @@ int b2pp_world_set_distance(void* h, int joint, float length, float hz, float ra
     if (r->joints[joint]->GetType() != b2JointType::DISTANCE_JOINT) return B2G_ERR_ARG;
     b2DistanceJoint* j = static_cast<b2DistanceJoint*>(r->joints[joint]);
     j->SetLength(length);
+    j->SetFrequency(hz);
+    j->SetDampingRatio(ratio);
+    return 0;
+}
+
+// b2MouseJoint::SetTarget, SetMaxForce, SetFrequency, SetDampingRatio.
+int b2pp_world_set_mouse(void* h, int joint, float tx, float ty, float maxForce, float hz, float ratio) {
+    Runner* r = (Runner*)h;
+    if (joint < 0 || joint >= (int)r->joints.size() || r->joints[joint] == nullptr) return B2G_ERR_ARG;
+    if (r->joints[joint]->GetType() != b2JointType::MOUSE_JOINT) return B2G_ERR_ARG;
+    b2MouseJoint* j = static_cast<b2MouseJoint*>(r->joints[joint]);
+    j->SetTarget(b2Vec2(tx, ty));
+    j->SetMaxForce(maxForce);
     j->SetFrequency(hz);
     j->SetDampingRatio(ratio);
     return 0;

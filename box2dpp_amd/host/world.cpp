@@ -209,6 +209,9 @@ void b2Body::ResetMassData() {
     m_sweep.c0 = m_sweep.c = b2Mul(m_xf, m_sweep.localCenter);
     m_linearVelocity += b2Cross(m_angularVelocity, m_sweep.c - oldCenter);
     MarkDirty();
+    // a mouse joint's spring uses the mass of its body B at step time
+    for (b2JointEdge* e = m_jointList; e; e = e->next)
+        if (e->joint->GetType() == b2JointType::MOUSE_JOINT && e->joint->GetBodyB() == this) e->joint->MarkDirty();
 }
 
 const b2Transform& b2Body::GetTransform() const { Touch(); return m_xf; }
@@ -563,6 +566,81 @@ void b2WeldJoint::SetDampingRatio(float ratio) {
     MarkDirty();
 }
 
+// b2PulleyJointDef::Initialize, REF/Dynamics/Joints/b2PulleyJoint.cpp:38-55
+void b2PulleyJointDef::Initialize(b2Body* bA, b2Body* bB, const b2Vec2& groundA, const b2Vec2& groundB,
+                                  const b2Vec2& anchorA, const b2Vec2& anchorB, float r) {
+    bodyA = bA;
+    bodyB = bB;
+    groundAnchorA = groundA;
+    groundAnchorB = groundB;
+    localAnchorA = bodyA->GetLocalPoint(anchorA);
+    localAnchorB = bodyB->GetLocalPoint(anchorB);
+    const b2Vec2 dA = anchorA - groundA;
+    lengthA = sqrtf(dA[0] * dA[0] + dA[1] * dA[1]);
+    const b2Vec2 dB = anchorB - groundB;
+    lengthB = sqrtf(dB[0] * dB[0] + dB[1] * dB[1]);
+    ratio = r;
+}
+
+b2PulleyJoint::b2PulleyJoint(const b2PulleyJointDef* def, b2World* world, int32_t index)
+    : b2Joint(def, world, index), m_groundAnchorA(def->groundAnchorA), m_groundAnchorB(def->groundAnchorB),
+      m_localAnchorA(def->localAnchorA), m_localAnchorB(def->localAnchorB), m_lengthA(def->lengthA),
+      m_lengthB(def->lengthB), m_ratio(def->ratio) {}
+
+b2Vec2 b2PulleyJoint::GetAnchorA() const { return m_bodyA->GetWorldPoint(m_localAnchorA); }
+b2Vec2 b2PulleyJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localAnchorB); }
+float b2PulleyJoint::GetCurrentLengthA() const {
+    const b2Vec2 d = GetAnchorA() - m_groundAnchorA;
+    return sqrtf(d[0] * d[0] + d[1] * d[1]);
+}
+float b2PulleyJoint::GetCurrentLengthB() const {
+    const b2Vec2 d = GetAnchorB() - m_groundAnchorB;
+    return sqrtf(d[0] * d[0] + d[1] * d[1]);
+}
+// REF/Dynamics/Joints/b2PulleyJoint.cpp:275-285: the impulse along body B's
+// rope direction of the last step.
+b2Vec2 b2PulleyJoint::GetReactionForce(float inv_dt) const {
+    m_world->DownloadJoints();
+    const b2Vec2 P(m_impulse[0] * m_impulse[1], m_impulse[0] * m_impulse[2]);
+    return b2Vec2(inv_dt * P[0], inv_dt * P[1]);
+}
+float b2PulleyJoint::GetReactionTorque(float inv_dt) const {
+    B2_NOT_USED(inv_dt);
+    return 0.0f;
+}
+
+// b2MouseJoint ctor, REF/Dynamics/Joints/b2MouseJoint.cpp:34-53: the anchor
+// on body B is the target at creation.
+b2MouseJoint::b2MouseJoint(const b2MouseJointDef* def, b2World* world, int32_t index)
+    : b2Joint(def, world, index), m_targetA(def->target),
+      m_localAnchorB(b2MulT(def->bodyB->GetTransform(), def->target)), m_maxForce(def->maxForce),
+      m_frequencyHz(def->frequencyHz), m_dampingRatio(def->dampingRatio) {}
+
+b2Vec2 b2MouseJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localAnchorB); }
+b2Vec2 b2MouseJoint::GetReactionForce(float inv_dt) const {
+    m_world->DownloadJoints();
+    return b2Vec2(inv_dt * m_impulse[0], inv_dt * m_impulse[1]);
+}
+float b2MouseJoint::GetReactionTorque(float inv_dt) const { return inv_dt * 0.0f; }
+// REF/Dynamics/Joints/b2MouseJoint.cpp:55-97
+void b2MouseJoint::SetTarget(const b2Vec2& target) {
+    if (!m_bodyB->IsAwake()) m_bodyB->SetAwake(true);
+    m_targetA = target;
+    MarkDirty();
+}
+void b2MouseJoint::SetMaxForce(float force) {
+    m_maxForce = force;
+    MarkDirty();
+}
+void b2MouseJoint::SetFrequency(float hz) {
+    m_frequencyHz = hz;
+    MarkDirty();
+}
+void b2MouseJoint::SetDampingRatio(float ratio) {
+    m_dampingRatio = ratio;
+    MarkDirty();
+}
+
 // b2FrictionJointDef::Initialize, REF/Dynamics/Joints/b2FrictionJoint.cpp:38-44
 void b2FrictionJointDef::Initialize(b2Body* bA, b2Body* bB, const b2Vec2& anchor) {
     bodyA = bA;
@@ -696,8 +774,9 @@ b2Joint* b2World::CreateJoint(const b2JointDef* def) {
     if (IsLocked() || def == nullptr || def->bodyA == nullptr || def->bodyB == nullptr) return nullptr;
     if (def->type != b2JointType::REVOLUTE_JOINT && def->type != b2JointType::DISTANCE_JOINT &&
         def->type != b2JointType::ROPE_JOINT && def->type != b2JointType::WELD_JOINT &&
-        def->type != b2JointType::FRICTION_JOINT && def->type != b2JointType::MOTOR_JOINT) {
-        m_lastError = "only revolute, distance, rope, weld, friction and motor joints run on the GPU path";
+        def->type != b2JointType::FRICTION_JOINT && def->type != b2JointType::MOTOR_JOINT &&
+        def->type != b2JointType::PULLEY_JOINT && def->type != b2JointType::MOUSE_JOINT) {
+        m_lastError = "prismatic, wheel and gear joints do not run on the GPU path";
         return nullptr;
     }
     if (def->bodyA == def->bodyB || def->bodyA->m_world != this || def->bodyB->m_world != this) return nullptr;
@@ -717,6 +796,10 @@ b2Joint* b2World::CreateJoint(const b2JointDef* def) {
         j = new b2FrictionJoint(static_cast<const b2FrictionJointDef*>(def), this, index);
     else if (def->type == b2JointType::MOTOR_JOINT)
         j = new b2MotorJoint(static_cast<const b2MotorJointDef*>(def), this, index);
+    else if (def->type == b2JointType::PULLEY_JOINT)
+        j = new b2PulleyJoint(static_cast<const b2PulleyJointDef*>(def), this, index);
+    else if (def->type == b2JointType::MOUSE_JOINT)
+        j = new b2MouseJoint(static_cast<const b2MouseJointDef*>(def), this, index);
     else
         j = new b2RevoluteJoint(static_cast<const b2RevoluteJointDef*>(def), this, (int32_t)m_joints.size());
     m_joints.push_back(j);
@@ -855,6 +938,41 @@ static b2g_motor_def motorDef(const b2MotorJoint* d, int32_t bodyA, int32_t body
     r.maxForce = d->GetMaxForce();
     r.maxTorque = d->GetMaxTorque();
     r.correctionFactor = d->GetCorrectionFactor();
+    r.collideConnected = d->GetCollideConnected() ? 1 : 0;
+    return r;
+}
+
+static b2g_pulley_def pulleyDef(const b2PulleyJoint* d, int32_t bodyA, int32_t bodyB) {
+    b2g_pulley_def r{};
+    r.bodyA = bodyA;
+    r.bodyB = bodyB;
+    r.groundAnchorA[0] = d->GetGroundAnchorA()[0];
+    r.groundAnchorA[1] = d->GetGroundAnchorA()[1];
+    r.groundAnchorB[0] = d->GetGroundAnchorB()[0];
+    r.groundAnchorB[1] = d->GetGroundAnchorB()[1];
+    r.localAnchorA[0] = d->GetLocalAnchorA()[0];
+    r.localAnchorA[1] = d->GetLocalAnchorA()[1];
+    r.localAnchorB[0] = d->GetLocalAnchorB()[0];
+    r.localAnchorB[1] = d->GetLocalAnchorB()[1];
+    r.lengthA = d->GetLengthA();
+    r.lengthB = d->GetLengthB();
+    r.ratio = d->GetRatio();
+    r.collideConnected = d->GetCollideConnected() ? 1 : 0;
+    return r;
+}
+
+static b2g_mouse_def mouseDef(const b2MouseJoint* d, int32_t bodyA, int32_t bodyB, float massB) {
+    b2g_mouse_def r{};
+    r.bodyA = bodyA;
+    r.bodyB = bodyB;
+    r.target[0] = d->GetTarget()[0];
+    r.target[1] = d->GetTarget()[1];
+    r.localAnchorB[0] = d->GetLocalAnchorB()[0];
+    r.localAnchorB[1] = d->GetLocalAnchorB()[1];
+    r.maxForce = d->GetMaxForce();
+    r.frequencyHz = d->GetFrequency();
+    r.dampingRatio = d->GetDampingRatio();
+    r.massB = massB;
     r.collideConnected = d->GetCollideConnected() ? 1 : 0;
     return r;
 }
@@ -1073,6 +1191,20 @@ int32_t b2World::Upload() {
                     defs.push_back(motorDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index));
                 }
                 rc = b2g_add_motor_joints(m_ctx, defs.data(), (int)defs.size(), &first);
+            } else if (type == b2JointType::PULLEY_JOINT) {
+                std::vector<b2g_pulley_def> defs;
+                for (int q = i; q < k; ++q) {
+                    const b2PulleyJoint* d = static_cast<const b2PulleyJoint*>(m_joints[q]);
+                    defs.push_back(pulleyDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index));
+                }
+                rc = b2g_add_pulley_joints(m_ctx, defs.data(), (int)defs.size(), &first);
+            } else if (type == b2JointType::MOUSE_JOINT) {
+                std::vector<b2g_mouse_def> defs;
+                for (int q = i; q < k; ++q) {
+                    const b2MouseJoint* d = static_cast<const b2MouseJoint*>(m_joints[q]);
+                    defs.push_back(mouseDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index, d->m_bodyB->m_mass));
+                }
+                rc = b2g_add_mouse_joints(m_ctx, defs.data(), (int)defs.size(), &first);
             } else {
                 std::vector<b2g_revolute_def> defs;
                 for (int q = i; q < k; ++q) {
@@ -1116,6 +1248,12 @@ int32_t b2World::Upload() {
                 const b2MotorJoint* m = static_cast<const b2MotorJoint*>(j);
                 const b2g_motor_def d = motorDef(m, m->m_bodyA->m_index, m->m_bodyB->m_index);
                 rc = b2g_set_motor_joint(m_ctx, j->m_index, &d);
+            } else if (j->m_type == b2JointType::MOUSE_JOINT) {
+                const b2MouseJoint* m = static_cast<const b2MouseJoint*>(j);
+                const b2g_mouse_def d = mouseDef(m, m->m_bodyA->m_index, m->m_bodyB->m_index, m->m_bodyB->m_mass);
+                rc = b2g_set_mouse_joint(m_ctx, j->m_index, &d);
+            } else if (j->m_type == b2JointType::PULLEY_JOINT) {
+                rc = B2G_OK;  // no setters
             } else {
                 const b2RevoluteJoint* r = static_cast<const b2RevoluteJoint*>(j);
                 const b2g_revolute_def d = revoluteDef(r, r->m_bodyA->m_index, r->m_bodyB->m_index);

@@ -75,9 +75,13 @@ struct JointDef {
     // 4; b2MotorJointDef (b2MotorJoint.h:27-56, linearOffset = (ax, ay),
     // angularOffset = referenceAngle, maxForce, maxTorque, correctionFactor)
     // when 5
-    int type;              // 0 revolute, 1 distance, 2 rope, 3 weld, 4 friction, 5 motor
+    // b2PulleyJointDef (b2PulleyJoint.h:30-70, ground anchors, lengthA =
+    // length, lengthB, ratio) when 6; b2MouseJointDef (b2MouseJoint.h:28-53,
+    // target = (ax, ay), maxForce, frequencyHz, dampingRatio) when 7
+    int type;              // 0 revolute, 1 distance, 2 rope, 3 weld, 4 friction, 5 motor, 6 pulley, 7 mouse
     float length, frequencyHz, dampingRatio;
     float maxForce, maxTorque, correctionFactor;
+    float gax, gay, gbx, gby, lengthB, ratio;
 };
 
 struct Scene {
@@ -655,6 +659,66 @@ inline Scene motorJoints(int n, bool chained = true) {
     return s;
 }
 
+// Pulley joints: n pulleys, each a box A (1 kg) and a box B (0.64-1.69 kg)
+// hanging 10 m and 8 m below ground anchors 4 m apart at y = 20, with
+// ratios 1, 1.5 and 2; odd pulleys hold their boxes by a top corner (the
+// boxes turn), even ones by their centres, over a ground edge.
+inline Scene pulleys(int n) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-400.0f, 0.0f, 400.0f, 0.0f));
+    for (int i = 0; i < n; ++i) {
+        const float x = 10.0f * (float)i - 5.0f * (float)n;
+        const float hb = 0.4f + 0.15f * (float)(i % 4);
+        addBody(s, defaultBody(2, x - 2.0f, 10.0f));
+        addFixture(s, boxFixture(0.5f, 0.5f, 1.0f, 0.4f));
+        addBody(s, defaultBody(2, x + 2.0f, 12.0f));
+        addFixture(s, boxFixture(hb, hb, 1.0f, 0.4f));
+        const int b = (int)s.bodies.size() - 1;
+        JointDef j{};
+        j.type = 6;
+        j.bodyA = b - 1; j.bodyB = b;
+        const bool corner = (i & 1) != 0;
+        j.ax = corner ? 0.5f : 0.0f; j.ay = corner ? 0.5f : 0.0f;
+        j.bx = corner ? -hb : 0.0f; j.by = corner ? hb : 0.0f;
+        j.gax = x - 2.0f; j.gay = 20.0f;
+        j.gbx = x + 2.0f; j.gby = 20.0f;
+        const float dax = x - 2.0f + j.ax - j.gax, day = 10.0f + j.ay - j.gay;
+        const float dbx = x + 2.0f + j.bx - j.gbx, dby = 12.0f + j.by - j.gby;
+        j.length = sqrtf(dax * dax + day * day);
+        j.lengthB = sqrtf(dbx * dbx + dby * dby);
+        j.ratio = 1.0f + 0.5f * (float)(i % 3);
+        s.joints.push_back(j);
+    }
+    return s;
+}
+
+// Mouse joints: n boxes (1 kg) on the ground edge, each grabbed near a
+// corner by a mouse joint (body A the ground; created with its target on the
+// grab point (bx, by) of body B, as the testbed's mouse-down does) and then
+// pulled by SetTarget to (ax, ay), 3-4 m above; maxForce 1000 N, 2-6 Hz,
+// damping 0.7.
+inline Scene mouseJoints(int n) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-400.0f, 0.0f, 400.0f, 0.0f));
+    for (int i = 0; i < n; ++i) {
+        const float x = 4.0f * (float)i - 2.0f * (float)n;
+        addBody(s, defaultBody(2, x, 0.5f));
+        addFixture(s, boxFixture(0.5f, 0.5f, 1.0f, 0.6f));
+        JointDef j{};
+        j.type = 7;
+        j.bodyA = 0; j.bodyB = (int)s.bodies.size() - 1;
+        j.ax = x + 0.4f; j.ay = 3.5f + 0.5f * (float)(i % 3);   // target
+        j.maxForce = 1000.0f;
+        j.frequencyHz = 2.0f + 2.0f * (float)(i % 3);
+        j.dampingRatio = 0.7f;
+        j.bx = 0.4f; j.by = 0.4f;   // grab point on body B
+        s.joints.push_back(j);
+    }
+    return s;
+}
+
 // Scene by name: "hello", "pyramid" (n = rows, 0 -> 20), "pile" (n bodies,
 // seed), "world" (one batched-scene world of n bodies, seed), "pendulums"
 // (n pendulums), "chains" (n chains of `seed` links), "stress" (pile of n
@@ -666,8 +730,8 @@ inline Scene motorJoints(int n, bool chained = true) {
 // seed / 10), "ropes" (n boxes on ropes; seed 0 hangs a second box on each),
 // "welds" (n welded beams; seed 0 drops a box on each, seed 2 makes the
 // beams single links), "friction" (n boxes on friction joints), "motor" (n
-// boxes on motor joints; seed 0 drives every third from the previous box).
-// Unknown names give an empty scene.
+// boxes on motor joints; seed 0 drives every third from the previous box),
+// "pulley" (n pulleys), "mouse" (n boxes on mouse joints). Unknown names give an empty scene.
 inline Scene byName(const char* kind, int n, uint32_t seed) {
     const std::string k(kind ? kind : "");
     if (k == "hello") return hello();
@@ -687,6 +751,8 @@ inline Scene byName(const char* kind, int n, uint32_t seed) {
     if (k == "welds") return welds(n > 0 ? n : 12, seed == 0, seed == 2);
     if (k == "friction") return frictionJoints(n > 0 ? n : 16);
     if (k == "motor") return motorJoints(n > 0 ? n : 12, seed == 0);
+    if (k == "pulley") return pulleys(n > 0 ? n : 8);
+    if (k == "mouse") return mouseJoints(n > 0 ? n : 8);
     return Scene();
 }
 

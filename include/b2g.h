@@ -272,7 +272,8 @@ int b2g_penetration(b2g_context* ctx, double* out);
  * error b2RevoluteJoint::SolvePositionConstraints measures,
  * Box2D/Dynamics/Joints/b2RevoluteJoint.cpp:336-343), the error in length
  * for rigid distance joints, the stretch beyond the maximum length for rope
- * joints; soft distance, friction and motor joints are not measured. out[3], FP64 sums on the
+ * joints, |lengthA + ratio lengthB - constant| for pulley joints; soft
+ * distance, friction, motor and mouse joints are not measured. out[3], FP64 sums on the
  * host; no counterpart in the reference. */
 int b2g_joint_drift(b2g_context* ctx, double* out);
 /* Kinetic (0.5 m v.v + 0.5 I w^2) and gravitational potential (-m g.c)
@@ -404,6 +405,43 @@ int b2g_add_motor_joints(b2g_context* ctx, const b2g_motor_def* defs, int count,
  * and SetCorrectionFactor in one call (bodies and collideConnected of `def`
  * are ignored). */
 int b2g_set_motor_joint(b2g_context* ctx, int joint, const b2g_motor_def* def);
+
+/* b2PulleyJointDef (Box2D/Dynamics/Joints/b2PulleyJoint.h:30-70) with body
+ * indices: lengthA + ratio * lengthB stays at its value at creation. 56
+ * bytes. */
+typedef struct b2g_pulley_def {
+    int32_t bodyA, bodyB;
+    float groundAnchorA[2];
+    float groundAnchorB[2];
+    float localAnchorA[2];
+    float localAnchorB[2];
+    float lengthA, lengthB, ratio;
+    int32_t collideConnected;
+} b2g_pulley_def;
+/* Appends pulley joints (b2PulleyJoint, REF/Dynamics/Joints/b2PulleyJoint.cpp:57-263);
+ * b2g_get_joints reports the impulse and body B's rope direction of the last
+ * step (impulse, uB.x, uB.y, 0); ratio must be positive. */
+int b2g_add_pulley_joints(b2g_context* ctx, const b2g_pulley_def* defs, int count, int* first);
+
+/* b2MouseJointDef (Box2D/Dynamics/Joints/b2MouseJoint.h:28-53) with body
+ * indices, plus what b2MouseJoint takes from body B at creation and step
+ * time: the local anchor (b2MulT of body B's transform and the target) and
+ * body B's mass (b2Body::GetMass). 44 bytes. */
+typedef struct b2g_mouse_def {
+    int32_t bodyA, bodyB;
+    float target[2];
+    float localAnchorB[2];
+    float maxForce, frequencyHz, dampingRatio;
+    float massB;
+    int32_t collideConnected;
+} b2g_mouse_def;
+/* Appends mouse joints (b2MouseJoint, REF/Dynamics/Joints/b2MouseJoint.cpp:34-200);
+ * b2g_get_joints reports the impulse (x, y, 0, 0). */
+int b2g_add_mouse_joints(b2g_context* ctx, const b2g_mouse_def* defs, int count, int* first);
+/* b2MouseJoint::SetTarget, SetMaxForce, SetFrequency, SetDampingRatio, and
+ * body B's mass when it changed (bodies, localAnchorB and collideConnected
+ * of `def` are ignored). */
+int b2g_set_mouse_joint(b2g_context* ctx, int joint, const b2g_mouse_def* def);
 /* Removes joints (b2World::DestroyJoint, Box2D/Dynamics/b2World.cpp:267):
  * they stop constraining and connecting islands at the next step; indices of
  * the other joints do not change. */
@@ -614,7 +652,8 @@ int b2g_get_graph_counts(b2g_context* ctx, int64_t* launches, int64_t* captures)
  * b2g_manifold, b2g_body_state, b2g_fixture_def, b2g_world_def,
  * b2g_step_params, b2g_step_stats, b2g_phase_profile, b2g_mlp_params,
  * b2g_aabb, b2g_transform, b2g_revolute_def, b2g_schedule, b2g_sweep,
- * b2g_distance_def, b2g_rope_def, b2g_weld_def, b2g_friction_def, b2g_motor_def.
+ * b2g_distance_def, b2g_rope_def, b2g_weld_def, b2g_friction_def, b2g_motor_def,
+ * b2g_pulley_def, b2g_mouse_def.
  * Returns the count (out may be null). Needs no device; lets bindings check
  * their struct layouts. */
 int b2g_abi_sizes(int32_t* out, int cap);

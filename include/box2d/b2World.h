@@ -570,6 +570,81 @@ private:
     float m_angularOffset, m_maxForce, m_maxTorque, m_correctionFactor;
 };
 
+// Box2D/Dynamics/Joints/b2PulleyJoint.h:30-70
+struct b2PulleyJointDef : public b2JointDef {
+    b2PulleyJointDef() { type = b2JointType::PULLEY_JOINT; collideConnected = true; }
+    // Bodies, anchors, lengths and ratio from ground and world anchor points.
+    void Initialize(b2Body* bodyA, b2Body* bodyB, const b2Vec2& groundAnchorA, const b2Vec2& groundAnchorB,
+                    const b2Vec2& anchorA, const b2Vec2& anchorB, float ratio);
+    b2Vec2 groundAnchorA{-1.0f, 1.0f};
+    b2Vec2 groundAnchorB{1.0f, 1.0f};
+    b2Vec2 localAnchorA{-1.0f, 0.0f};
+    b2Vec2 localAnchorB{1.0f, 0.0f};
+    float lengthA = 0.0f;
+    float lengthB = 0.0f;
+    float ratio = 1.0f;
+};
+
+// Box2D/Dynamics/Joints/b2PulleyJoint.h:80-150: lengthA + ratio * lengthB
+// stays constant.
+class b2PulleyJoint : public b2Joint {
+public:
+    b2Vec2 GetAnchorA() const override;
+    b2Vec2 GetAnchorB() const override;
+    b2Vec2 GetGroundAnchorA() const { return m_groundAnchorA; }
+    b2Vec2 GetGroundAnchorB() const { return m_groundAnchorB; }
+    const b2Vec2& GetLocalAnchorA() const { return m_localAnchorA; }
+    const b2Vec2& GetLocalAnchorB() const { return m_localAnchorB; }
+    float GetLengthA() const { return m_lengthA; }
+    float GetLengthB() const { return m_lengthB; }
+    float GetRatio() const { return m_ratio; }
+    float GetCurrentLengthA() const;
+    float GetCurrentLengthB() const;
+    b2Vec2 GetReactionForce(float inv_dt) const override;
+    float GetReactionTorque(float inv_dt) const override;
+
+private:
+    friend class b2World;
+    b2PulleyJoint(const b2PulleyJointDef* def, b2World* world, int32_t index);
+    b2Vec2 m_groundAnchorA, m_groundAnchorB, m_localAnchorA, m_localAnchorB;
+    float m_lengthA, m_lengthB, m_ratio;
+};
+
+// Box2D/Dynamics/Joints/b2MouseJoint.h:28-53
+struct b2MouseJointDef : public b2JointDef {
+    b2MouseJointDef() { type = b2JointType::MOUSE_JOINT; }
+    b2Vec2 target{0.0f, 0.0f};   // world point; also fixes the anchor on body B
+    float maxForce = 0.0f;
+    float frequencyHz = 5.0f;
+    float dampingRatio = 0.7f;
+};
+
+// Box2D/Dynamics/Joints/b2MouseJoint.h:62-131: pulls a point of body B to a
+// world target with a soft, force-limited spring.
+class b2MouseJoint : public b2Joint {
+public:
+    b2Vec2 GetAnchorA() const override { return m_targetA; }
+    b2Vec2 GetAnchorB() const override;
+    void SetTarget(const b2Vec2& target);
+    const b2Vec2& GetTarget() const { return m_targetA; }
+    const b2Vec2& GetLocalAnchorB() const { return m_localAnchorB; }
+    void SetMaxForce(float force);
+    float GetMaxForce() const { return m_maxForce; }
+    void SetFrequency(float hz);
+    float GetFrequency() const { return m_frequencyHz; }
+    void SetDampingRatio(float ratio);
+    float GetDampingRatio() const { return m_dampingRatio; }
+    b2Vec2 GetReactionForce(float inv_dt) const override;
+    float GetReactionTorque(float inv_dt) const override;
+
+private:
+    friend class b2World;
+    friend class b2Body;
+    b2MouseJoint(const b2MouseJointDef* def, b2World* world, int32_t index);
+    b2Vec2 m_targetA, m_localAnchorB;
+    float m_maxForce, m_frequencyHz, m_dampingRatio;
+};
+
 // Per-step solver options of the GPU path (no reference counterpart).
 struct b2SolverOptions {
     int32_t mode = 0;                 // 0 coloured, 1 reference order
@@ -654,6 +729,8 @@ private:
     friend class b2WeldJoint;
     friend class b2FrictionJoint;
     friend class b2MotorJoint;
+    friend class b2PulleyJoint;
+    friend class b2MouseJoint;
     int32_t Upload();
     void Download() const;
     void DownloadJoints() const;

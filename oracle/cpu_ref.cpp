@@ -1338,6 +1338,12 @@ struct Joint {
     float angularMass;
     V2 linearError;
     float angularError;
+    // b2PulleyJoint (type 6: length = lengthA), REF/Dynamics/Joints/b2PulleyJoint.h:125-149;
+    // b2MouseJoint (type 7: target in localAnchorA, maxForce, frequencyHz,
+    // dampingRatio, gamma), REF/Dynamics/Joints/b2MouseJoint.h:110-130
+    V2 groundAnchorA, groundAnchorB, uA, uB;
+    float lengthB, ratio, constant, pulleyMass, beta;
+    V2 mouseC;
     // solver temporaries
     int indexA, indexB;
     V2 rA, rB, localCenterA, localCenterB;
@@ -1529,6 +1535,20 @@ struct World {
         j.maxForce = d.maxForce; j.maxTorque = d.maxTorque; j.correctionFactor = d.correctionFactor;
         j.linearImpulse = mk(0.0f, 0.0f);
         j.angularImpulse = 0.0f;
+        j.groundAnchorA = mk(d.gax, d.gay); j.groundAnchorB = mk(d.gbx, d.gby);
+        j.lengthB = d.lengthB; j.ratio = d.ratio;
+        j.constant = d.length + d.ratio * d.lengthB;   // b2PulleyJoint ctor, REF b2PulleyJoint.cpp:57-75
+        j.uA = j.uB = mk(0.0f, 0.0f);
+        j.pulleyMass = j.beta = 0.0f;
+        j.mouseC = mk(0.0f, 0.0f);
+        if (d.type == 7) {
+            // b2MouseJoint ctor (REF/Dynamics/Joints/b2MouseJoint.cpp:34-53) with
+            // the target on the grab point, then SetTarget (:55-62)
+            const Body& bB = bodies[d.bodyB];
+            const V2 grab = mul(bB.xf, mk(d.bx, d.by));
+            j.localAnchorB = mulT(bB.xf, grab);
+            j.localAnchorA = mk(d.ax, d.ay);
+        }
         const int ji = (int)joints.size();
         j.prev = -1;
         j.next = jointHead;
@@ -2122,6 +2142,14 @@ struct World {
             frictionMotorInitVelocity(j, dtRatio);
             return;
         }
+        if (j.type == 6) {
+            pulleyInitVelocity(j, dtRatio);
+            return;
+        }
+        if (j.type == 7) {
+            mouseInitVelocity(j, h, dtRatio);
+            return;
+        }
         const float aA = posA[j.indexA];
         V2 vA = velV[j.indexA]; float wA = velW[j.indexA];
         const float aB = posA[j.indexB];
@@ -2192,6 +2220,14 @@ struct World {
         }
         if (j.type == 4 || j.type == 5) {
             frictionMotorSolveVelocity(j, h, 1.0f / h);
+            return;
+        }
+        if (j.type == 6) {
+            pulleySolveVelocity(j);
+            return;
+        }
+        if (j.type == 7) {
+            mouseSolveVelocity(j, h);
             return;
         }
         V2 vA = velV[j.indexA]; float wA = velW[j.indexA];
@@ -2501,6 +2537,144 @@ struct World {
         velV[j.indexB] = vB; velW[j.indexB] = wB;
     }
 
+    // b2PulleyJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2PulleyJoint.cpp:76-164
+    void pulleyInitVelocity(Joint& j, float dtRatio) {
+        const V2 cA = posC[j.indexA], cB = posC[j.indexB];
+        const float aA = posA[j.indexA], aB = posA[j.indexB];
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        j.rA = mul(qA, j.localAnchorA - j.localCenterA);
+        j.rB = mul(qB, j.localAnchorB - j.localCenterB);
+        j.uA = cA + j.rA - j.groundAnchorA;
+        j.uB = cB + j.rB - j.groundAnchorB;
+        const float lengthA = length(j.uA), lengthB = length(j.uB);
+        if (lengthA > 10.0f * kLinearSlop) j.uA *= 1.0f / lengthA;
+        else j.uA = mk(0.0f, 0.0f);
+        if (lengthB > 10.0f * kLinearSlop) j.uB *= 1.0f / lengthB;
+        else j.uB = mk(0.0f, 0.0f);
+        const float ruA = cross(j.rA, j.uA), ruB = cross(j.rB, j.uB);
+        const float mA = j.invMassA + j.invIA * ruA * ruA;
+        const float mB = j.invMassB + j.invIB * ruB * ruB;
+        j.pulleyMass = mA + j.ratio * j.ratio * mB;
+        if (j.pulleyMass > 0.0f) j.pulleyMass = 1.0f / j.pulleyMass;
+        if (warmStarting) {
+            j.impulse.x *= dtRatio;
+            const V2 PA = -(j.impulse.x) * j.uA;
+            const V2 PB = (-j.ratio * j.impulse.x) * j.uB;
+            vA += j.invMassA * PA;
+            wA += j.invIA * cross(j.rA, PA);
+            vB += j.invMassB * PB;
+            wB += j.invIB * cross(j.rB, PB);
+        } else {
+            j.impulse.x = 0.0f;
+        }
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2PulleyJoint::SolveVelocityConstraints, REF b2PulleyJoint.cpp:166-191
+    void pulleySolveVelocity(Joint& j) {
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const V2 vpA = vA + cross(wA, j.rA);
+        const V2 vpB = vB + cross(wB, j.rB);
+        const float Cdot = -dot(j.uA, vpA) - j.ratio * dot(j.uB, vpB);
+        const float impulse = -j.pulleyMass * Cdot;
+        j.impulse.x += impulse;
+        const V2 PA = -impulse * j.uA;
+        const V2 PB = -j.ratio * impulse * j.uB;
+        vA += j.invMassA * PA;
+        wA += j.invIA * cross(j.rA, PA);
+        vB += j.invMassB * PB;
+        wB += j.invIB * cross(j.rB, PB);
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2PulleyJoint::SolvePositionConstraints, REF b2PulleyJoint.cpp:193-263
+    bool pulleySolvePosition(Joint& j) {
+        V2 cA = posC[j.indexA]; float aA = posA[j.indexA];
+        V2 cB = posC[j.indexB]; float aB = posA[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        const V2 rA = mul(qA, j.localAnchorA - j.localCenterA);
+        const V2 rB = mul(qB, j.localAnchorB - j.localCenterB);
+        V2 uA = cA + rA - j.groundAnchorA;
+        V2 uB = cB + rB - j.groundAnchorB;
+        const float lengthA = length(uA), lengthB = length(uB);
+        if (lengthA > 10.0f * kLinearSlop) uA *= 1.0f / lengthA;
+        else uA = mk(0.0f, 0.0f);
+        if (lengthB > 10.0f * kLinearSlop) uB *= 1.0f / lengthB;
+        else uB = mk(0.0f, 0.0f);
+        const float ruA = cross(rA, uA), ruB = cross(rB, uB);
+        const float mA = j.invMassA + j.invIA * ruA * ruA;
+        const float mB = j.invMassB + j.invIB * ruB * ruB;
+        float mass = mA + j.ratio * j.ratio * mB;
+        if (mass > 0.0f) mass = 1.0f / mass;
+        const float C = j.constant - lengthA - j.ratio * lengthB;
+        const float linearError = absf(C);
+        const float impulse = -mass * C;
+        const V2 PA = -impulse * uA;
+        const V2 PB = -j.ratio * impulse * uB;
+        cA += j.invMassA * PA;
+        aA += j.invIA * cross(rA, PA);
+        cB += j.invMassB * PB;
+        aB += j.invIB * cross(rB, PB);
+        posC[j.indexA] = cA; posA[j.indexA] = aA;
+        posC[j.indexB] = cB; posA[j.indexB] = aB;
+        return linearError < kLinearSlop;
+    }
+
+    // b2MouseJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2MouseJoint.cpp:99-169
+    void mouseInitVelocity(Joint& j, float h, float dtRatio) {
+        const V2 cB = posC[j.indexB];
+        const float aB = posA[j.indexB];
+        V2 vB = velV[j.indexB];
+        float wB = velW[j.indexB];
+        const Rot qB = stepRot(aB);
+        const float mass = bodies[j.bodyB].mass;
+        const float omega = 2.0f * kPi * j.frequencyHz;
+        const float d = 2.0f * mass * j.dampingRatio * omega;
+        const float k = mass * (omega * omega);
+        j.gamma = h * (d + h * k);
+        if (j.gamma != 0.0f) j.gamma = 1.0f / j.gamma;
+        j.beta = h * k * j.gamma;
+        j.rB = mul(qB, j.localAnchorB - j.localCenterB);
+        Mat22 K;
+        K.ex.x = j.invMassB + j.invIB * j.rB.y * j.rB.y + j.gamma;
+        K.ex.y = -j.invIB * j.rB.x * j.rB.y;
+        K.ey.x = K.ex.y;
+        K.ey.y = j.invMassB + j.invIB * j.rB.x * j.rB.x + j.gamma;
+        j.linearMass = inverse(K);
+        j.mouseC = cB + j.rB - j.localAnchorA;
+        j.mouseC *= j.beta;
+        wB *= 0.98f;
+        if (warmStarting) {
+            j.linearImpulse *= dtRatio;
+            vB += j.invMassB * j.linearImpulse;
+            wB += j.invIB * cross(j.rB, j.linearImpulse);
+        } else {
+            j.linearImpulse = mk(0.0f, 0.0f);
+        }
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2MouseJoint::SolveVelocityConstraints, REF b2MouseJoint.cpp:171-194
+    void mouseSolveVelocity(Joint& j, float h) {
+        V2 vB = velV[j.indexB];
+        float wB = velW[j.indexB];
+        const V2 Cdot = vB + cross(wB, j.rB);
+        V2 impulse = mul(j.linearMass, -(Cdot + j.mouseC + j.gamma * j.linearImpulse));
+        const V2 oldImpulse = j.linearImpulse;
+        j.linearImpulse += impulse;
+        const float maxImpulse = h * j.maxForce;
+        if (lengthSq(j.linearImpulse) > maxImpulse * maxImpulse) j.linearImpulse *= maxImpulse / length(j.linearImpulse);
+        impulse = j.linearImpulse - oldImpulse;
+        vB += j.invMassB * impulse;
+        wB += j.invIB * cross(j.rB, impulse);
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
     // The 3x3 K of b2WeldJoint (REF/Dynamics/Joints/b2WeldJoint.cpp:95-104)
     static Mat33 weldK(float mA, float iA, float mB, float iB, V2 rA, V2 rB) {
         Mat33 K;
@@ -2645,7 +2819,8 @@ struct World {
         if (j.type == 1) return distanceSolvePosition(j);
         if (j.type == 2) return ropeSolvePosition(j);
         if (j.type == 3) return weldSolvePosition(j);
-        if (j.type == 4 || j.type == 5) return true;  // b2FrictionJoint / b2MotorJoint: no position pass
+        if (j.type == 4 || j.type == 5 || j.type == 7) return true;  // friction, motor, mouse: no position pass
+        if (j.type == 6) return pulleySolvePosition(j);
         V2 cA = posC[j.indexA]; float aA = posA[j.indexA];
         V2 cB = posC[j.indexB]; float aB = posA[j.indexB];
         float angularError = 0.0f, positionError = 0.0f;
@@ -3302,6 +3477,22 @@ int orc_world_destroy_joint(void* h, int joint) {
 
 // Per joint in creation order: impulse x, y, z and motor impulse.
 // b2DistanceJoint::SetLength / SetFrequency / SetDampingRatio.
+// b2MouseJoint::SetTarget (waking body B), SetMaxForce, SetFrequency,
+// SetDampingRatio, REF/Dynamics/Joints/b2MouseJoint.cpp:55-97.
+int orc_world_set_mouse(void* h, int joint, float tx, float ty, float maxForce, float hz, float ratio) {
+    World* w = (World*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint].removed || w->joints[joint].type != 7)
+        return B2G_ERR_ARG;
+    Joint& j = w->joints[joint];
+    Body& bB = w->bodies[j.bodyB];
+    if (!(bB.flags & kAwake)) w->setAwake(bB, true);
+    j.localAnchorA = mk(tx, ty);
+    j.maxForce = maxForce;
+    j.frequencyHz = hz;
+    j.dampingRatio = ratio;
+    return B2G_OK;
+}
+
 // b2FrictionJoint::SetMaxForce / SetMaxTorque.
 int orc_world_set_friction(void* h, int joint, float maxForce, float maxTorque) {
     World* w = (World*)h;
@@ -3375,7 +3566,15 @@ void orc_world_get_joints(void* h, float* out) {
             for (int k = 0; k < 4; ++k) out[4 * i + k] = 0.0f;
             continue;
         }
-        if (j.type == 4 || j.type == 5) {  // reaction force and torque at inv_dt = 1
+        if (j.type == 6) {  // b2PulleyJoint::GetReactionForce at inv_dt = 1: impulse along uB
+            const V2 P = j.impulse.x * j.uB;
+            out[4 * i] = P.x;
+            out[4 * i + 1] = P.y;
+            out[4 * i + 2] = 0.0f;
+            out[4 * i + 3] = 0.0f;
+            continue;
+        }
+        if (j.type == 4 || j.type == 5 || j.type == 7) {  // reaction force and torque at inv_dt = 1
             out[4 * i] = j.linearImpulse.x;
             out[4 * i + 1] = j.linearImpulse.y;
             out[4 * i + 2] = j.angularImpulse;

@@ -101,6 +101,36 @@ int loadJoints(RefWorld* w, int base, const b2scene::JointDef* jd, int nj) {
             w->joints.push_back(w->world->CreateJoint(&rd));
             continue;
         }
+        if (d.type == 6) {
+            b2PulleyJointDef pd;
+            pd.bodyA = w->bodies[base + d.bodyA];
+            pd.bodyB = w->bodies[base + d.bodyB];
+            pd.collideConnected = d.collideConnected != 0;
+            pd.groundAnchorA.Set(d.gax, d.gay);
+            pd.groundAnchorB.Set(d.gbx, d.gby);
+            pd.localAnchorA.Set(d.ax, d.ay);
+            pd.localAnchorB.Set(d.bx, d.by);
+            pd.lengthA = d.length;
+            pd.lengthB = d.lengthB;
+            pd.ratio = d.ratio;
+            w->joints.push_back(w->world->CreateJoint(&pd));
+            continue;
+        }
+        if (d.type == 7) {
+            // created on the grab point, then pulled to the target
+            b2MouseJointDef md;
+            md.bodyA = w->bodies[base + d.bodyA];
+            md.bodyB = w->bodies[base + d.bodyB];
+            md.collideConnected = d.collideConnected != 0;
+            md.target = md.bodyB->GetWorldPoint(b2Vec2(d.bx, d.by));
+            md.maxForce = d.maxForce;
+            md.frequencyHz = d.frequencyHz;
+            md.dampingRatio = d.dampingRatio;
+            b2MouseJoint* j = static_cast<b2MouseJoint*>(w->world->CreateJoint(&md));
+            j->SetTarget(b2Vec2(d.ax, d.ay));
+            w->joints.push_back(j);
+            continue;
+        }
         if (d.type == 4) {
             b2FrictionJointDef fd;
             fd.bodyA = w->bodies[base + d.bodyA];
@@ -406,6 +436,18 @@ int ref_world_set_distance(void* h, int joint, float length, float hz, float rat
     return 0;
 }
 
+int ref_world_set_mouse(void* h, int joint, float tx, float ty, float maxForce, float hz, float ratio) {
+    RefWorld* w = (RefWorld*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
+    if (w->joints[joint]->GetType() != e_mouseJoint) return -1;
+    b2MouseJoint* j = static_cast<b2MouseJoint*>(w->joints[joint]);
+    j->SetTarget(b2Vec2(tx, ty));
+    j->SetMaxForce(maxForce);
+    j->SetFrequency(hz);
+    j->SetDampingRatio(ratio);
+    return 0;
+}
+
 int ref_world_set_friction(void* h, int joint, float maxForce, float maxTorque) {
     RefWorld* w = (RefWorld*)h;
     if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
@@ -466,7 +508,7 @@ void ref_world_get_joints(void* h, float* out) {
         }
         const b2JointType type = w->joints[i]->GetType();
         if (type == e_distanceJoint || type == e_ropeJoint || type == e_weldJoint || type == e_frictionJoint ||
-            type == e_motorJoint) {
+            type == e_motorJoint || type == e_pulleyJoint || type == e_mouseJoint) {
             const b2Vec2 f = w->joints[i]->GetReactionForce(1.0f);
             out[4 * i] = f.x;
             out[4 * i + 1] = f.y;

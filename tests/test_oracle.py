@@ -40,6 +40,8 @@ F = ctypes.c_float
     ("friction", 16, 0, 300, True, 8, 3),      # friction joints: boxes thrown along the ground, some colliding
     ("motor", 12, 0, 400, True, 8, 3),         # motor joints: boxes driven to offsets from the ground and each other
     ("motor", 9, 1, 300, False, 4, 2),
+    ("pulley", 8, 0, 300, True, 8, 3),         # pulley joints: ratios 1-2, anchors at centres and corners
+    ("mouse", 8, 0, 300, True, 8, 3),          # mouse joints: boxes pulled up by soft springs
 ])
 def test_world_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, vel, pos):
     a = ref.world(kind, n, seed, sleep=sleep)
@@ -745,4 +747,69 @@ def test_motor_joint_reaches_offsets(orc):
         angle = base_angle + 0.5 + 0.5 * (i % 3)
         assert np.hypot(*(b[[E.PX, E.PY]] - target)) < 1e-3, (i, b[[E.PX, E.PY]], target)
         assert abs(float(b[E.ANG]) - angle) < 1e-3, (i, b[E.ANG], angle)
+    w.close()
+
+
+def test_pulley_and_mouse_edits_bitexact(ref, orc):
+    """b2MouseJoint::SetTarget, SetMaxForce, SetFrequency and SetDampingRatio
+    mid-run (a target moved sideways, a spring stiffened, a force limit cut)
+    on the mouse scene, and the pulley scene run long enough for a box to
+    reach its pulley, against the reference, with the reaction forces."""
+    for kind, n, edits in (
+            ("mouse", 6, lambda w: (w.e.fn("world_set_mouse")(w.h, 0, F(-9.0), F(5.0), F(1000.0), F(8.0), F(0.3)),
+                                    w.e.fn("world_set_mouse")(w.h, 3, F(2.5), F(1.0), F(3.0), F(2.0), F(0.7)))),
+            ("pulley", 6, lambda w: (0, 0))):
+        a = ref.world(kind, n, 0, sleep=True)
+        b = orc.world(kind, n, 0, sleep=True)
+        a.step(90); b.step(90)
+        for w in (a, b):
+            assert edits(w) == (0, 0)
+        for s in range(300):
+            a.step(1); b.step(1)
+            assert np.array_equal(a.bodies().view(np.uint32), b.bodies().view(np.uint32)), f"{kind} step {s + 91}"
+        nj = a.e.fn("world_joint_count")(a.h)
+        ja, jb = np.zeros(4 * nj, np.float32), np.zeros(4 * nj, np.float32)
+        a.e.fn("world_get_joints")(a.h, E.ptr(ja))
+        b.e.fn("world_get_joints")(b.h, E.ptr(jb))
+        assert np.array_equal(ja.view(np.uint32), jb.view(np.uint32))
+        a.close(); b.close()
+
+
+def test_pulley_atwood_closed_form(orc):
+    """Boxes hung by their centres on a pulley of ratio r accelerate as an
+    Atwood machine: body B at a_B = g (r m_A - m_B) / (m_B + r^2 m_A) and body
+    A at -r a_B, so y_k = y_0 + a h^2 k (k + 1) / 2 after k steps (1 mm over
+    30 steps), with lengthA + r lengthB held. Pulley 0: r = 1, m_A = 1,
+    m_B = 0.64; pulley 2: r = 2, m_A = 1, m_B = 1.96."""
+    w = orc.world("pulley", 8, 0, sleep=False)
+    h, g = 1.0 / 60.0, 10.0
+    y0 = w.bodies()[:, E.CY].copy()
+    for k in range(1, 31):
+        w.step(1)
+        y = w.bodies()[:, E.CY]
+        for i, r, mb in ((0, 1.0, 0.64), (2, 2.0, 1.96)):
+            ab = g * (r * 1.0 - mb) / (mb + r * r * 1.0)
+            for body, a in ((1 + 2 * i, -r * ab), (2 + 2 * i, ab)):
+                want = y0[body] + a * h * h * k * (k + 1) / 2
+                assert abs(y[body] - want) < 1e-3, (i, body, k, y[body], want)
+    w.close()
+
+
+def test_mouse_joint_hooke_closed_form(orc):
+    """A box hanging at rest from a mouse joint of frequency f sags below the
+    target by g / (2 pi f)^2 (Hooke's law with k = m (2 pi f)^2), and the
+    joint carries its weight: reaction force (0, m g) within 1%."""
+    w = orc.world("mouse", 6, 0, sleep=False)
+    w.step(2400)  # the box also swings about the grab point, damped slowly
+    bodies, j = w.bodies(), w.joints() * 60.0
+    for i in range(3):
+        hz = 2.0 + 2.0 * i
+        b = bodies[1 + i]
+        s, c = float(b[E.QS]), float(b[E.QC])
+        # grab point (0.4, 0.4) on the box, at its local anchor
+        anchor = np.array([b[E.PX] + c * 0.4 - s * 0.4, b[E.PY] + s * 0.4 + c * 0.4])
+        target = np.array([4.0 * i - 12.0 + 0.4, 3.5 + 0.5 * i])
+        sag = 10.0 / (2.0 * np.pi * hz) ** 2
+        assert abs(anchor[0] - target[0]) < 2e-3 and abs(target[1] - anchor[1] - sag) < 2e-3, (i, anchor, target, sag)
+        assert abs(j[i, 0]) < 0.02 and abs(j[i, 1] - 10.0) < 0.1, (i, j[i])
     w.close()
