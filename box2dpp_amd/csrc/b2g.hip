@@ -138,8 +138,8 @@ struct JointView {
 // mass m11 m12 m22, angular mass. Motor: anchors = linearOffset x y,
 // angularOffset, correctionFactor; limits, impulse and K0 as friction; K1 =
 // linear error, motorMass = angular error. Pulley: limits = ground anchors A,
-// B; motorSpeed = ratio, motorMass = the constant lengthA + ratio lengthB;
-// impulse = (impulse, uB.x, uB.y); K0 = (uA, uB), K1.x = mass. Mouse:
+// B; motorSpeed = ratio; impulse = (impulse, uB.x, uB.y, the constant
+// lengthA + ratio lengthB); K0 = (uA, uB), K1.x = mass. Mouse:
 // anchors = target, localAnchorB; limits = maxForce, frequencyHz,
 // dampingRatio, mass of body B; impulse x y; K0 = mass m11 m12 m22, gamma;
 // K1 = beta C.
@@ -1905,7 +1905,7 @@ __device__ inline void pulleyInitOne(const WorldView& w, const JointView& jv, in
     } else {
         imp.x = 0.0f;
     }
-    jv.impulse[j] = make_float4(imp.x, uB.x, uB.y, 0.0f);  // uB for b2PulleyJoint::GetReactionForce
+    jv.impulse[j] = make_float4(imp.x, uB.x, uB.y, imp.w);  // uB for b2PulleyJoint::GetReactionForce
     jv.r[j] = make_float4(rA.x, rA.y, rB.x, rB.y);
     jv.masses[j] = make_float4(imA, iiA, imB, iiB);
     jv.K0[j] = make_float4(uA.x, uA.y, uB.x, uB.y);
@@ -1965,7 +1965,7 @@ __device__ inline bool pulleySolvePositionOne(const WorldView& w, const JointVie
     const float mB = ms.z + ms.w * ruB * ruB;
     float mass = mA + ratio * ratio * mB;
     if (mass > 0.0f) mass = 1.0f / mass;
-    const float C = jv.motorMass[j] - lengthA - ratio * lengthB;
+    const float C = jv.impulse[j].w - lengthA - ratio * lengthB;
     const float linearError = absB(C);
     const float impulse = -mass * C;
     const V2 PA = -impulse * uA;
@@ -2412,7 +2412,7 @@ __global__ void k_joint_drift(WorldView w, JointView jv, float* out) {
         const float4 g = jv.limits[j];
         const float la = lengthOf(mk(pA.x, pA.y) + rA - mk(g.x, g.y));
         const float lb = lengthOf(mk(pB.x, pB.y) + rB - mk(g.z, g.w));
-        out[j] = absB(la + jv.motorSpeed[j] * lb - jv.motorMass[j]);
+        out[j] = absB(la + jv.motorSpeed[j] * lb - jv.impulse[j].w);
         return;
     }
     out[j] = gap;  // revolute and weld: the anchors' separation
@@ -7497,14 +7497,15 @@ int b2g_add_pulley_joints(b2g_context* ctx, const b2g_pulley_def* defs, int coun
     if (rc != B2G_OK || count == 0) return rc;
     // ratio and the constant lengthA + ratio lengthB (b2PulleyJoint ctor,
     // REF/Dynamics/Joints/b2PulleyJoint.cpp:57-75)
-    std::vector<float> ratio(count), constant(count);
+    std::vector<float> ratio(count);
+    std::vector<float4> imp(count);
     for (int i = 0; i < count; ++i) {
         ratio[i] = defs[i].ratio;
-        constant[i] = defs[i].lengthA + defs[i].ratio * defs[i].lengthB;
+        imp[i] = make_float4(0.0f, 0.0f, 0.0f, defs[i].lengthA + defs[i].ratio * defs[i].lengthB);
     }
     HIPCHK(ctx, hipMemcpyAsync(ctx->jMotorSpeed + n0, ratio.data(), sizeof(float) * count, hipMemcpyHostToDevice,
                                ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->jMotorMass + n0, constant.data(), sizeof(float) * count, hipMemcpyHostToDevice,
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jImpulse + n0, imp.data(), sizeof(float4) * count, hipMemcpyHostToDevice,
                                ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return B2G_OK;

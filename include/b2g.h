@@ -419,8 +419,9 @@ typedef struct b2g_pulley_def {
     int32_t collideConnected;
 } b2g_pulley_def;
 /* Appends pulley joints (b2PulleyJoint, REF/Dynamics/Joints/b2PulleyJoint.cpp:57-263);
- * b2g_get_joints reports the impulse and body B's rope direction of the last
- * step (impulse, uB.x, uB.y, 0); ratio must be positive. */
+ * b2g_get_joints reports the impulse, body B's rope direction of the last
+ * step and the pulley's constant lengthA + ratio * lengthB (impulse, uB.x,
+ * uB.y, constant); ratio must be positive. */
 int b2g_add_pulley_joints(b2g_context* ctx, const b2g_pulley_def* defs, int count, int* first);
 
 /* b2MouseJointDef (Box2D/Dynamics/Joints/b2MouseJoint.h:28-53) with body

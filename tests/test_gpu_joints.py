@@ -395,3 +395,20 @@ def test_welds_hold(gpu, orc):
     assert worst_g <= 1.5 * worst_o + 0.01, (worst_g, worst_o)
     g.close()
     o.close()
+
+
+def test_pulleys_hold_after_the_joint_table_grows(gpu):
+    """Pulleys keep lengthA + ratio lengthB (their constant, kept in the
+    joint rows) when a second scene's joints grow the joint table past its
+    first capacity: b2g_joint_drift's pulley error stays under 1 cm."""
+    lib = _b2g(gpu)
+    g = gpu.world("pulley", 8, 0, sleep=False)
+    g.step(30)
+    assert g.e.fn("world_load_scene")(g.h, b"friction", 100, ctypes.c_uint(0)) == 0
+    ctx = g.device_context()
+    out = np.zeros(3, np.float64)
+    for _ in range(4):
+        g.step(15)
+        assert lib.b2g_joint_drift(ctx, E.ptr(out)) == 0
+        assert out[2] == 8 and out[1] < 0.01, out
+    g.close()

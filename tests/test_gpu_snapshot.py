@@ -82,6 +82,7 @@ def _edit_world(lib, w, ctx, kind, p):
     ("pyramid", 20, 0, True, 0),
     ("chains", 4, 14, True, 0),
     ("stress", 600, 11, False, 2),
+    ("pulley", 8, 0, True, 0),      # pulley constants live in the joint rows
 ])
 def test_restored_world_steps_identically(gpu, kind, n, arg, sleep, worlds):
     lib = E.b2g_lib()
