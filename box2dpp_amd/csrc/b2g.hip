@@ -107,7 +107,7 @@ enum : int {
     JF_LIMIT = 1, JF_MOTOR = 2, JF_COLLIDE = 4, JF_REMOVED = 8,
     JL_INACTIVE = 0, JL_LOWER = 1, JL_UPPER = 2, JL_EQUAL = 3,  // limit state in bits 8-9
     JT_REVOLUTE = 0, JT_DISTANCE = 1, JT_ROPE = 2, JT_WELD = 3,  // joint type in bits 12-15
-    JT_FRICTION = 4, JT_MOTOR = 5, JT_PULLEY = 6, JT_MOUSE = 7,
+    JT_FRICTION = 4, JT_MOTOR = 5, JT_PULLEY = 6, JT_MOUSE = 7, JT_PRISMATIC = 8, JT_WHEEL = 9,
 };
 __host__ __device__ inline int jointType(int flags) { return (flags >> 12) & 15; }
 struct JointView {
@@ -124,6 +124,8 @@ struct JointView {
     float4* K0;        // mass matrix m11 m12 m13 m22
     float2* K1;        // m23 m33
     float* motorMass;
+    float4* axis;      // localXAxisA (prismatic, wheel)
+    float4* extra;     // two rows per joint: step directions kept across steps
     float h;           // time step of the solve (soft joints)
 };
 // Per-type use of the arrays. Revolute: limits = referenceAngle, lower,
@@ -142,7 +144,13 @@ struct JointView {
 // lengthA + ratio lengthB); K0 = (uA, uB), K1.x = mass. Mouse:
 // anchors = target, localAnchorB; limits = maxForce, frequencyHz,
 // dampingRatio, mass of body B; impulse x y; K0 = mass m11 m12 m22, gamma;
-// K1 = beta C.
+// K1 = beta C. Prismatic: limits = referenceAngle, lowerTranslation,
+// upperTranslation, maxMotorForce; axis = localXAxisA; impulse x y z, motor;
+// extra[2j] = (axis, perp); r = (s1, s2, a1, a2); K0, K1 = K; limit state in
+// flags bits 8-9. Wheel: limits = frequencyHz, dampingRatio, maxMotorTorque;
+// axis = localXAxisA; impulse = (impulse, spring, motor); extra[2j] = (ax,
+// sAx, sBx), extra[2j+1] = (ay, sAy, sBy); K0 = (mass, springMass, bias,
+// gamma).
 
 __device__ inline int encodeOrdered(float f) {
     int i = __float_as_int(f);
@@ -2055,6 +2063,438 @@ __device__ inline void mouseSolveVelocityOne(const WorldView& w, const JointView
     if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
 }
 
+// b2Vec2::Normalize and b2Cross(1, v) for a joint's local axes, as the
+// b2PrismaticJoint constructor sets them (REF/Dynamics/Joints/b2PrismaticJoint.cpp:107-109).
+__device__ inline V2 perpOf(V2 x) { return mk(-1.0f * x.y, 1.0f * x.x); }
+
+// b2PrismaticJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2PrismaticJoint.cpp:128-266
+__device__ inline void prismaticInitOne(const WorldView& w, const JointView& jv, int j, int2 b, float dtRatio,
+                                        int warm) {
+    const float4 msA = w.mass[b.x], msB = w.mass[b.y];
+    const float mA = msA.x, iA = msA.y, mB = msB.x, iB = msB.y;
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const Rot qA = rotOf(pA.z), qB = rotOf(pB.z);
+    const float4 an = jv.anchors[j];
+    const float4 ax4 = jv.axis[j];
+    const V2 lx = mk(ax4.x, ax4.y);
+    const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    const V2 d = (mk(pB.x, pB.y) - mk(pA.x, pA.y)) + rB - rA;
+    const V2 axis = mul(qA, lx);
+    const float a1 = cross(d + rA, axis), a2 = cross(rB, axis);
+    float motorMass = mA + mB + iA * a1 * a1 + iB * a2 * a2;
+    if (motorMass > 0.0f) motorMass = 1.0f / motorMass;
+    const V2 perp = mul(qA, perpOf(lx));
+    const float s1 = cross(d + rA, perp), s2 = cross(rB, perp);
+    Sym33 K;
+    K.m11 = mA + mB + iA * s1 * s1 + iB * s2 * s2;
+    K.m12 = iA * s1 + iB * s2;
+    K.m13 = iA * s1 * a1 + iB * s2 * a2;
+    K.m22 = iA + iB;
+    if (K.m22 == 0.0f) K.m22 = 1.0f;
+    K.m23 = iA * a1 + iB * a2;
+    K.m33 = mA + mB + iA * a1 * a1 + iB * a2 * a2;
+    int flags = jv.flags[j];
+    float4 imp = jv.impulse[j];
+    int state = (flags >> 8) & 3;
+    const float4 lim = jv.limits[j];  // referenceAngle, lower, upper, maxMotorForce
+    if (flags & JF_LIMIT) {
+        const float jointTranslation = dot(axis, d);
+        if (absB(lim.z - lim.y) < 2.0f * kLinearSlop) {
+            state = JL_EQUAL;
+        } else if (jointTranslation <= lim.y) {
+            if (state != JL_LOWER) {
+                state = JL_LOWER;
+                imp.z = 0.0f;
+            }
+        } else if (jointTranslation >= lim.z) {
+            if (state != JL_UPPER) {
+                state = JL_UPPER;
+                imp.z = 0.0f;
+            }
+        } else {
+            state = JL_INACTIVE;
+            imp.z = 0.0f;
+        }
+    } else {
+        state = JL_INACTIVE;
+        imp.z = 0.0f;
+    }
+    if (!(flags & JF_MOTOR)) imp.w = 0.0f;
+    if (warm) {
+        imp.x *= dtRatio;
+        imp.y *= dtRatio;
+        imp.z *= dtRatio;
+        imp.w *= dtRatio;
+        const V2 P = imp.x * perp + (imp.w + imp.z) * axis;
+        const float LA = imp.x * s1 + imp.y + (imp.w + imp.z) * a1;
+        const float LB = imp.x * s2 + imp.y + (imp.w + imp.z) * a2;
+        vA -= mA * P;
+        wA -= iA * LA;
+        vB += mB * P;
+        wB += iB * LB;
+    } else {
+        imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    jv.flags[j] = (flags & ~0x300) | (state << 8);
+    jv.impulse[j] = imp;
+    jv.extra[2 * j] = make_float4(axis.x, axis.y, perp.x, perp.y);
+    jv.r[j] = make_float4(s1, s2, a1, a2);
+    jv.masses[j] = make_float4(mA, iA, mB, iB);
+    jv.K0[j] = make_float4(K.m11, K.m12, K.m13, K.m22);
+    jv.K1[j] = make_float2(K.m23, K.m33);
+    jv.motorMass[j] = motorMass;
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2PrismaticJoint::SolveVelocityConstraints, REF b2PrismaticJoint.cpp:268-363
+__device__ inline void prismaticSolveVelocityOne(const WorldView& w, const JointView& jv, int j, int2 b, float h) {
+    const float4 ms = jv.masses[j];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const float4 e = jv.extra[2 * j];
+    const V2 axis = mk(e.x, e.y), perp = mk(e.z, e.w);
+    const float4 sa = jv.r[j];
+    const float s1 = sa.x, s2 = sa.y, a1 = sa.z, a2 = sa.w;
+    const int flags = jv.flags[j];
+    const int state = (flags >> 8) & 3;
+    float4 imp = jv.impulse[j];
+    if ((flags & JF_MOTOR) && state != JL_EQUAL) {
+        const float Cdot = dot(axis, vB - vA) + a2 * wB - a1 * wA;
+        float impulse = jv.motorMass[j] * (jv.motorSpeed[j] - Cdot);
+        const float oldImpulse = imp.w;
+        const float maxImpulse = h * jv.limits[j].w;
+        imp.w = clampB(imp.w + impulse, -maxImpulse, maxImpulse);
+        impulse = imp.w - oldImpulse;
+        const V2 P = impulse * axis;
+        const float LA = impulse * a1, LB = impulse * a2;
+        vA -= mA * P;
+        wA -= iA * LA;
+        vB += mB * P;
+        wB += iB * LB;
+    }
+    const float c1x = dot(perp, vB - vA) + s2 * wB - s1 * wA;
+    const float c1y = wB - wA;
+    const float4 k0 = jv.K0[j];
+    const float2 k1 = jv.K1[j];
+    const Sym33 K{k0.x, k0.y, k0.z, k0.w, k1.x, k1.y};
+    if ((flags & JF_LIMIT) && state != JL_INACTIVE) {
+        const float Cdot2 = dot(axis, vB - vA) + a2 * wB - a1 * wA;
+        const float4 f1 = imp;
+        float dx, dy, dz;
+        solve33(K, -c1x, -c1y, -Cdot2, dx, dy, dz);
+        imp.x += dx;
+        imp.y += dy;
+        imp.z += dz;
+        if (state == JL_LOWER) imp.z = imp.z > 0.0f ? imp.z : 0.0f;       // b2Max
+        else if (state == JL_UPPER) imp.z = imp.z < 0.0f ? imp.z : 0.0f;  // b2Min
+        const float sz = imp.z - f1.z;
+        const V2 rhs = mk(-c1x - sz * K.m13, -c1y - sz * K.m23);
+        const V2 f2r = solve22(K, rhs) + mk(f1.x, f1.y);
+        imp.x = f2r.x;
+        imp.y = f2r.y;
+        const float ddx = imp.x - f1.x, ddy = imp.y - f1.y, ddz = imp.z - f1.z;
+        const V2 P = ddx * perp + ddz * axis;
+        const float LA = ddx * s1 + ddy + ddz * a1;
+        const float LB = ddx * s2 + ddy + ddz * a2;
+        vA -= mA * P;
+        wA -= iA * LA;
+        vB += mB * P;
+        wB += iB * LB;
+    } else {
+        const V2 df = solve22(K, mk(-c1x, -c1y));
+        imp.x += df.x;
+        imp.y += df.y;
+        const V2 P = df.x * perp;
+        const float LA = df.x * s1 + df.y;
+        const float LB = df.x * s2 + df.y;
+        vA -= mA * P;
+        wA -= iA * LA;
+        vB += mB * P;
+        wB += iB * LB;
+    }
+    jv.impulse[j] = imp;
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2PrismaticJoint::SolvePositionConstraints, REF b2PrismaticJoint.cpp:365-488
+__device__ inline bool prismaticSolvePositionOne(const WorldView& w, const JointView& jv, int j, int2 b) {
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    V2 cA = mk(pA.x, pA.y), cB = mk(pB.x, pB.y);
+    float aA = pA.z, aB = pB.z;
+    const float4 ms = jv.masses[j];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const Rot qA = rotOf(aA), qB = rotOf(aB);
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float4 an = jv.anchors[j];
+    const float4 ax4 = jv.axis[j];
+    const V2 lx = mk(ax4.x, ax4.y);
+    const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    const V2 d = cB + rB - cA - rA;
+    const V2 axis = mul(qA, lx);
+    const float a1 = cross(d + rA, axis), a2 = cross(rB, axis);
+    const V2 perp = mul(qA, perpOf(lx));
+    const float s1 = cross(d + rA, perp), s2 = cross(rB, perp);
+    const float4 lim = jv.limits[j];
+    const float C1x = dot(perp, d), C1y = aB - aA - lim.x;
+    float linearError = absB(C1x);
+    const float angularError = absB(C1y);
+    bool active = false;
+    float C2 = 0.0f;
+    if (jv.flags[j] & JF_LIMIT) {
+        const float translation = dot(axis, d);
+        if (absB(lim.z - lim.y) < 2.0f * kLinearSlop) {
+            C2 = clampB(translation, -kMaxLinearCorrection, kMaxLinearCorrection);
+            const float e = absB(translation);
+            linearError = linearError > e ? linearError : e;
+            active = true;
+        } else if (translation <= lim.y) {
+            C2 = clampB(translation - lim.y + kLinearSlop, -kMaxLinearCorrection, 0.0f);
+            const float e = lim.y - translation;
+            linearError = linearError > e ? linearError : e;
+            active = true;
+        } else if (translation >= lim.z) {
+            C2 = clampB(translation - lim.z - kLinearSlop, 0.0f, kMaxLinearCorrection);
+            const float e = translation - lim.z;
+            linearError = linearError > e ? linearError : e;
+            active = true;
+        }
+    }
+    Sym33 K;
+    K.m11 = mA + mB + iA * s1 * s1 + iB * s2 * s2;
+    K.m12 = iA * s1 + iB * s2;
+    K.m22 = iA + iB;
+    if (K.m22 == 0.0f) K.m22 = 1.0f;
+    float ix, iy, iz;
+    if (active) {
+        K.m13 = iA * s1 * a1 + iB * s2 * a2;
+        K.m23 = iA * a1 + iB * a2;
+        K.m33 = mA + mB + iA * a1 * a1 + iB * a2 * a2;
+        solve33(K, -C1x, -C1y, -C2, ix, iy, iz);
+    } else {
+        K.m13 = K.m23 = K.m33 = 0.0f;
+        const V2 i1 = solve22(K, mk(-C1x, -C1y));  // b2Mat22::Solve
+        ix = i1.x;
+        iy = i1.y;
+        iz = 0.0f;
+    }
+    const V2 P = ix * perp + iz * axis;
+    const float LA = ix * s1 + iy + iz * a1;
+    const float LB = ix * s2 + iy + iz * a2;
+    cA -= mA * P;
+    aA -= iA * LA;
+    cB += mB * P;
+    aB += iB * LB;
+    if (w.btype[b.x] == BT_DYNAMIC) w.pos[b.x] = make_float4(cA.x, cA.y, aA, pA.w);
+    if (w.btype[b.y] == BT_DYNAMIC) w.pos[b.y] = make_float4(cB.x, cB.y, aB, pB.w);
+    return linearError <= kLinearSlop && angularError <= kAngularSlop;
+}
+
+// b2WheelJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2WheelJoint.cpp:79-216:
+// a point-on-line row along the perpendicular axis, a soft spring along the
+// axis and a motor on the relative rotation.
+__device__ inline void wheelInitOne(const WorldView& w, const JointView& jv, int j, int2 b, float dtRatio, int warm) {
+    const float4 msA = w.mass[b.x], msB = w.mass[b.y];
+    const float mA = msA.x, iA = msA.y, mB = msB.x, iB = msB.y;
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const Rot qA = rotOf(pA.z), qB = rotOf(pB.z);
+    const float4 an = jv.anchors[j];
+    const float4 ax4 = jv.axis[j];
+    const V2 lx = mk(ax4.x, ax4.y);
+    const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    const V2 d = mk(pB.x, pB.y) + rB - mk(pA.x, pA.y) - rA;
+    const V2 ay = mul(qA, perpOf(lx));
+    const float sAy = cross(d + rA, ay), sBy = cross(rB, ay);
+    float mass = mA + mB + iA * sAy * sAy + iB * sBy * sBy;
+    if (mass > 0.0f) mass = 1.0f / mass;
+    const float4 par = jv.limits[j];  // frequencyHz, dampingRatio, maxMotorTorque
+    float4 ex = jv.extra[2 * j];     // ax, sAx, sBx: kept while the spring is off
+    float springMass = 0.0f, bias = 0.0f, gamma = 0.0f;
+    float4 imp = jv.impulse[j];      // impulse, springImpulse, motorImpulse
+    if (par.x > 0.0f) {
+        const V2 ax = mul(qA, lx);
+        const float sAx = cross(d + rA, ax), sBx = cross(rB, ax);
+        ex = make_float4(ax.x, ax.y, sAx, sBx);
+        const float invMass = mA + mB + iA * sAx * sAx + iB * sBx * sBx;
+        if (invMass > 0.0f) {
+            springMass = 1.0f / invMass;
+            const float C = dot(d, ax);
+            const float omega = 2.0f * kPi * par.x;
+            const float dd = 2.0f * springMass * par.y * omega;
+            const float k = springMass * omega * omega;
+            const float h = jv.h;
+            gamma = h * (dd + h * k);
+            if (gamma > 0.0f) gamma = 1.0f / gamma;
+            bias = C * h * k * gamma;
+            springMass = invMass + gamma;
+            if (springMass > 0.0f) springMass = 1.0f / springMass;
+        }
+    } else {
+        imp.y = 0.0f;
+    }
+    float motorMass = 0.0f;
+    if (jv.flags[j] & JF_MOTOR) {
+        motorMass = iA + iB;
+        if (motorMass > 0.0f) motorMass = 1.0f / motorMass;
+    } else {
+        imp.z = 0.0f;
+    }
+    const V2 ax = mk(ex.x, ex.y);
+    if (warm) {
+        imp.x *= dtRatio;
+        imp.y *= dtRatio;
+        imp.z *= dtRatio;
+        const V2 P = imp.x * ay + imp.y * ax;
+        const float LA = imp.x * sAy + imp.y * ex.z + imp.z;
+        const float LB = imp.x * sBy + imp.y * ex.w + imp.z;
+        vA -= mA * P;
+        wA -= iA * LA;
+        vB += mB * P;
+        wB += iB * LB;
+    } else {
+        imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    jv.impulse[j] = imp;
+    jv.extra[2 * j] = ex;
+    jv.extra[2 * j + 1] = make_float4(ay.x, ay.y, sAy, sBy);
+    jv.masses[j] = make_float4(mA, iA, mB, iB);
+    jv.K0[j] = make_float4(mass, springMass, bias, gamma);
+    jv.motorMass[j] = motorMass;
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2WheelJoint::SolveVelocityConstraints, REF b2WheelJoint.cpp:218-280
+__device__ inline void wheelSolveVelocityOne(const WorldView& w, const JointView& jv, int j, int2 b, float h) {
+    const float4 ms = jv.masses[j];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const float4 va = w.vel[b.x], vbv = w.vel[b.y];
+    V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
+    float wA = va.z, wB = vbv.z;
+    const float4 ex = jv.extra[2 * j], ey = jv.extra[2 * j + 1];
+    const V2 ax = mk(ex.x, ex.y), ay = mk(ey.x, ey.y);
+    const float4 k0 = jv.K0[j];  // mass, springMass, bias, gamma
+    float4 imp = jv.impulse[j];
+    {
+        const float Cdot = dot(ax, vB - vA) + ex.w * wB - ex.z * wA;
+        const float impulse = -k0.y * (Cdot + k0.z + k0.w * imp.y);
+        imp.y += impulse;
+        const V2 P = impulse * ax;
+        const float LA = impulse * ex.z, LB = impulse * ex.w;
+        vA -= mA * P;
+        wA -= iA * LA;
+        vB += mB * P;
+        wB += iB * LB;
+    }
+    {
+        const float Cdot = wB - wA - jv.motorSpeed[j];
+        float impulse = -jv.motorMass[j] * Cdot;
+        const float oldImpulse = imp.z;
+        const float maxImpulse = h * jv.limits[j].z;
+        imp.z = clampB(imp.z + impulse, -maxImpulse, maxImpulse);
+        impulse = imp.z - oldImpulse;
+        wA -= iA * impulse;
+        wB += iB * impulse;
+    }
+    {
+        const float Cdot = dot(ay, vB - vA) + ey.w * wB - ey.z * wA;
+        const float impulse = -k0.x * Cdot;
+        imp.x += impulse;
+        const V2 P = impulse * ay;
+        const float LA = impulse * ey.z, LB = impulse * ey.w;
+        vA -= mA * P;
+        wA -= iA * LA;
+        vB += mB * P;
+        wB += iB * LB;
+    }
+    jv.impulse[j] = imp;
+    if (w.btype[b.x] == BT_DYNAMIC) w.vel[b.x] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (w.btype[b.y] == BT_DYNAMIC) w.vel[b.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2WheelJoint::SolvePositionConstraints, REF b2WheelJoint.cpp:282-327; its
+// effective mass uses the sAy, sBy of the velocity pass, as the reference's.
+__device__ inline bool wheelSolvePositionOne(const WorldView& w, const JointView& jv, int j, int2 b) {
+    const float4 pA = w.pos[b.x], pB = w.pos[b.y];
+    V2 cA = mk(pA.x, pA.y), cB = mk(pB.x, pB.y);
+    float aA = pA.z, aB = pB.z;
+    const float4 ms = jv.masses[j];
+    const Rot qA = rotOf(aA), qB = rotOf(aB);
+    const float2 lcA = w.lcenter[b.x], lcB = w.lcenter[b.y];
+    const float4 an = jv.anchors[j];
+    const float4 ax4 = jv.axis[j];
+    const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+    const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+    const V2 d = (cB - cA) + rB - rA;
+    const V2 ay = mul(qA, perpOf(mk(ax4.x, ax4.y)));
+    const float sAy = cross(d + rA, ay), sBy = cross(rB, ay);
+    const float C = dot(d, ay);
+    const float4 ey = jv.extra[2 * j + 1];
+    const float k = ms.x + ms.z + ms.y * ey.z * ey.z + ms.w * ey.w * ey.w;
+    const float impulse = k != 0.0f ? -C / k : 0.0f;
+    const V2 P = impulse * ay;
+    const float LA = impulse * sAy, LB = impulse * sBy;
+    cA -= ms.x * P;
+    aA -= ms.y * LA;
+    cB += ms.z * P;
+    aB += ms.w * LB;
+    if (w.btype[b.x] == BT_DYNAMIC) w.pos[b.x] = make_float4(cA.x, cA.y, aA, pA.w);
+    if (w.btype[b.y] == BT_DYNAMIC) w.pos[b.y] = make_float4(cB.x, cB.y, aB, pB.w);
+    return absB(C) <= kLinearSlop;
+}
+
+// b2Joint::GetReactionForce and GetReactionTorque at inv_dt = 1 for every
+// joint type, from the last step's impulses and directions
+// (REF/Dynamics/Joints/b2*Joint.cpp, GetReactionForce / GetReactionTorque).
+__global__ void k_joint_reactions(JointView jv, int first, int count, float4* out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int j = first + k;
+    const int flags = jv.flags[j];
+    const float4 imp = jv.impulse[j];
+    float fx = 0.0f, fy = 0.0f, t = 0.0f;
+    if (!(flags & JF_REMOVED)) {
+        switch (jointType(flags)) {
+            case JT_REVOLUTE: fx = imp.x; fy = imp.y; t = imp.z; break;
+            case JT_DISTANCE: case JT_ROPE: case JT_PULLEY: {
+                const V2 P = imp.x * mk(imp.y, imp.z);
+                fx = P.x; fy = P.y;
+                break;
+            }
+            case JT_WELD: case JT_FRICTION: case JT_MOTOR: fx = imp.x; fy = imp.y; t = imp.z; break;
+            case JT_MOUSE: fx = imp.x; fy = imp.y; t = 1.0f * 0.0f; break;
+            case JT_PRISMATIC: {
+                const float4 e = jv.extra[2 * j];
+                const V2 P = imp.x * mk(e.z, e.w) + (imp.w + imp.z) * mk(e.x, e.y);
+                fx = P.x; fy = P.y; t = imp.y;
+                break;
+            }
+            case JT_WHEEL: {
+                const float4 ex = jv.extra[2 * j], ey = jv.extra[2 * j + 1];
+                const V2 P = imp.x * mk(ey.x, ey.y) + imp.y * mk(ex.x, ex.y);
+                fx = P.x; fy = P.y; t = imp.z;
+                break;
+            }
+            default: break;
+        }
+    }
+    out[k] = make_float4(fx, fy, t, 0.0f);
+}
+
 // b2RevoluteJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2RevoluteJoint.cpp:66-198
 __device__ inline void jointInitOne(const WorldView& w, const JointView& jv, int j, float dtRatio, int warm) {
     const int2 b = jv.bodies[j];
@@ -2082,6 +2522,14 @@ __device__ inline void jointInitOne(const WorldView& w, const JointView& jv, int
     }
     if (type == JT_MOUSE) {
         mouseInitOne(w, jv, j, b, dtRatio, warm);
+        return;
+    }
+    if (type == JT_PRISMATIC) {
+        prismaticInitOne(w, jv, j, b, dtRatio, warm);
+        return;
+    }
+    if (type == JT_WHEEL) {
+        wheelInitOne(w, jv, j, b, dtRatio, warm);
         return;
     }
     const float4 msA = w.mass[b.x], msB = w.mass[b.y];
@@ -2190,6 +2638,14 @@ __device__ inline void jointSolveVelocityOne(const WorldView& w, const JointView
         mouseSolveVelocityOne(w, jv, j, b, h);
         return;
     }
+    if (type == JT_PRISMATIC) {
+        prismaticSolveVelocityOne(w, jv, j, b, h);
+        return;
+    }
+    if (type == JT_WHEEL) {
+        wheelSolveVelocityOne(w, jv, j, b, h);
+        return;
+    }
     const float4 ms = jv.masses[j];
     const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
     const float4 va = w.vel[b.x], vbv = w.vel[b.y];
@@ -2288,6 +2744,8 @@ __device__ inline void jointSolvePositionOne(const WorldView& w, const JointView
         const bool ok = type == JT_DISTANCE ? distanceSolvePositionOne(w, jv, j, b)
                         : type == JT_ROPE   ? ropeSolvePositionOne(w, jv, j, b)
                         : type == JT_PULLEY ? pulleySolvePositionOne(w, jv, j, b)
+                        : type == JT_PRISMATIC ? prismaticSolvePositionOne(w, jv, j, b)
+                        : type == JT_WHEEL  ? wheelSolvePositionOne(w, jv, j, b)
                                             : weldSolvePositionOne(w, jv, j, b);
         if (!ok) {
             int* f = &w.islandMinSep[it * w.nBodies + root];
@@ -2406,6 +2864,12 @@ __global__ void k_joint_drift(WorldView w, JointView jv, float* out) {
     }
     if (type == JT_FRICTION || type == JT_MOTOR || type == JT_MOUSE) {  // no positional constraint
         out[j] = -1.0f;
+        return;
+    }
+    if (type == JT_PRISMATIC || type == JT_WHEEL) {  // distance of anchor B from the line through anchor A
+        const V2 axis = mul(rotOf(pA.z), mk(jv.axis[j].x, jv.axis[j].y));
+        const V2 d = mk(pB.x, pB.y) + rB - mk(pA.x, pA.y) - rA;
+        out[j] = absB(cross(axis, d)) / lengthOf(axis);
         return;
     }
     if (type == JT_PULLEY) {  // |lengthA + ratio lengthB - constant|
@@ -4033,6 +4497,8 @@ struct b2g_context {
            *jK0 = nullptr, *jImpulseSnap = nullptr;
     float2* jK1 = nullptr;
     float *jMotorSpeed = nullptr, *jMotorMass = nullptr;
+    float4* jAxis = nullptr;   // persistent: local axis of prismatic and wheel joints
+    float4* jExtra = nullptr;  // persistent: two rows of step directions per joint
     int *jFlags = nullptr, *jFlagsSnap = nullptr, *jOrder = nullptr;
     std::vector<int2> hJointBodies;
     std::vector<int> hJointKeepApart;
@@ -4118,6 +4584,8 @@ struct b2g_context {
         j.bodies = jBodies; j.anchors = jAnchors; j.limits = jLimits; j.motorSpeed = jMotorSpeed;
         j.flags = jFlags; j.impulse = jImpulse; j.r = jR; j.masses = jMasses; j.K0 = jK0; j.K1 = jK1;
         j.motorMass = jMotorMass;
+        j.axis = jAxis;
+        j.extra = jExtra;
         j.h = 0.0f;
         return j;
     }
@@ -4545,6 +5013,8 @@ int growJoints(b2g_context* ctx, int need) {
     HIPCHK(ctx, growArray(ctx->jMotorSpeed, n, cap, s));
     HIPCHK(ctx, growArray(ctx->jFlags, n, cap, s));
     HIPCHK(ctx, growArray(ctx->jImpulse, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->jAxis, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->jExtra, 2 * n, 2 * cap, s));
     HIPCHK(ctx, growArray(ctx->jR, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->jMasses, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->jK0, 0, cap, s));
@@ -5868,7 +6338,7 @@ int b2g_destroy(b2g_context* ctx) {
     freeArray(ctx->velSnap); freeArray(ctx->logFeat); freeArray(ctx->logTarget);
     freeArray(ctx->jBodies); freeArray(ctx->jAnchors); freeArray(ctx->jLimits); freeArray(ctx->jImpulse);
     freeArray(ctx->jR); freeArray(ctx->jMasses); freeArray(ctx->jK0); freeArray(ctx->jK1);
-    freeArray(ctx->jMotorSpeed); freeArray(ctx->jMotorMass); freeArray(ctx->jFlags); freeArray(ctx->jOrder);
+    freeArray(ctx->jMotorSpeed); freeArray(ctx->jMotorMass); freeArray(ctx->jAxis); freeArray(ctx->jExtra); freeArray(ctx->jFlags); freeArray(ctx->jOrder);
     freeArray(ctx->jImpulseSnap); freeArray(ctx->jFlagsSnap); freeArray(ctx->jointPairs);
     if (ctx->trainScratch) (void)hipFree(ctx->trainScratch);
     if (ctx->mlpAllocated) b2mlp::mlpFree(ctx->mlp);
@@ -6985,6 +7455,7 @@ int b2g_abi_sizes(int32_t* out, int cap) {
         (int32_t)sizeof(b2g_schedule),    (int32_t)sizeof(b2g_sweep),        (int32_t)sizeof(b2g_distance_def),
         (int32_t)sizeof(b2g_rope_def),    (int32_t)sizeof(b2g_weld_def),     (int32_t)sizeof(b2g_friction_def),
         (int32_t)sizeof(b2g_motor_def),   (int32_t)sizeof(b2g_pulley_def),   (int32_t)sizeof(b2g_mouse_def),
+        (int32_t)sizeof(b2g_prismatic_def), (int32_t)sizeof(b2g_wheel_def),
     };
     const int n = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (!out) return n;
@@ -7002,7 +7473,7 @@ int b2g_abi_sizes(int32_t* out, int cap) {
 // original would.
 namespace {
 constexpr uint32_t kSnapMagic = 0x53473242u;  // "B2GS"
-constexpr uint32_t kSnapVersion = 2;
+constexpr uint32_t kSnapVersion = 3;
 struct SnapHeader {
     uint32_t magic, version;
     int32_t nBodies, nFixtures, nContacts, nJoints;
@@ -7050,6 +7521,8 @@ void snapArrays(b2g_context* ctx, int nb, int nf, int nc, int nj, bool subWorlds
     visit(ctx->jMotorSpeed, sizeof(float) * nj);
     visit(ctx->jFlags, sizeof(int) * nj);
     visit(ctx->jImpulse, sizeof(float4) * nj);
+    visit(ctx->jAxis, sizeof(float4) * nj);
+    visit(ctx->jExtra, sizeof(float4) * 2 * nj);
     visit(ctx->jOrder, sizeof(int) * nj);
 }
 }  // namespace
@@ -7348,6 +7821,8 @@ struct JointRow {
     int bodyA, bodyB;
     float4 anchors, params;
     int flags;
+    float4 axis = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float speed = 0.0f;
 };
 
 // Appends rows to the joint table (impulses start at zero).
@@ -7362,7 +7837,8 @@ static int appendJointRows(b2g_context* ctx, const std::vector<JointRow>& rows, 
     int rc = growJoints(ctx, ctx->nJoints + count);
     if (rc != B2G_OK) return rc;
     std::vector<int2> b(count);
-    std::vector<float4> anchors(count), params(count), imp(count, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    std::vector<float4> anchors(count), params(count), imp(count, make_float4(0.0f, 0.0f, 0.0f, 0.0f)), axis(count);
+    std::vector<float4> extra(2 * (size_t)count, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     std::vector<float> speed(count, 0.0f);
     std::vector<int> flags(count);
     bool keepApart = false;
@@ -7372,6 +7848,8 @@ static int appendJointRows(b2g_context* ctx, const std::vector<JointRow>& rows, 
         anchors[i] = d.anchors;
         params[i] = d.params;
         flags[i] = d.flags;
+        axis[i] = d.axis;
+        speed[i] = d.speed;
         const bool collide = (d.flags & JF_COLLIDE) != 0;
         ctx->hJointBodies.push_back(b[i]);
         ctx->hJointKeepApart.push_back(collide ? 0 : 1);
@@ -7386,11 +7864,27 @@ static int appendJointRows(b2g_context* ctx, const std::vector<JointRow>& rows, 
     HIPCHK(ctx, hipMemcpyAsync(ctx->jMotorSpeed + n0, speed.data(), sizeof(float) * count, hipMemcpyHostToDevice, s));
     HIPCHK(ctx, hipMemcpyAsync(ctx->jFlags + n0, flags.data(), sizeof(int) * count, hipMemcpyHostToDevice, s));
     HIPCHK(ctx, hipMemcpyAsync(ctx->jImpulse + n0, imp.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jAxis + n0, axis.data(), sizeof(float4) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jExtra + 2 * (size_t)n0, extra.data(), sizeof(float4) * 2 * count,
+                               hipMemcpyHostToDevice, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
     ctx->nJoints += count;
     ctx->jointsDirty = true;
     if (keepApart) return rebuildJointPairs(ctx);
     return B2G_OK;
+}
+
+// b2Vec2::Normalize (REF/Common/b2Math.h:117-129), for the local axis of a
+// prismatic joint.
+static float4 normalizedAxis(const float* a) {
+    float x = a[0], y = a[1];
+    const float len = sqrtf(x * x + y * y);
+    if (!(len < FLT_EPSILON)) {
+        const float inv = 1.0f / len;
+        x *= inv;
+        y *= inv;
+    }
+    return make_float4(x, y, 0.0f, 0.0f);
 }
 
 // Replaces the parameters (limits row) of one joint of the given type.
@@ -7536,6 +8030,96 @@ int b2g_set_mouse_joint(b2g_context* ctx, int joint, const b2g_mouse_def* d) {
     an.y = d->target[1];
     HIPCHK(ctx, hipMemcpyAsync(ctx->jAnchors + joint, &an, sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
+int b2g_add_prismatic_joints(b2g_context* ctx, const b2g_prismatic_def* defs, int count, int* first) {
+    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
+    std::vector<JointRow> rows;
+    for (int i = 0; i < count; ++i) {
+        const b2g_prismatic_def& d = defs[i];
+        JointRow r = jointRow(d, JT_PRISMATIC,
+                              make_float4(d.referenceAngle, d.lowerTranslation, d.upperTranslation, d.maxMotorForce));
+        r.flags |= (d.enableLimit ? JF_LIMIT : 0) | (d.enableMotor ? JF_MOTOR : 0);
+        r.axis = normalizedAxis(d.localAxisA);
+        r.speed = d.motorSpeed;
+        rows.push_back(r);
+    }
+    return appendJointRows(ctx, rows, first);
+}
+
+// Shared by the prismatic and wheel setters: motor speed, switches, the
+// limits row; a changed limit switch or range drops the limit impulse.
+static int setAxisJoint(b2g_context* ctx, int joint, int type, const char* what, float4 par, int enableLimit,
+                        int enableMotor, float speed) {
+    if (!ctx || joint < 0 || joint >= ctx->nJoints) return B2G_ERR_ARG;
+    if (ctx->hJointRemoved[joint]) return fail(ctx, B2G_ERR_ARG, "joint was removed");
+    hipStream_t s = ctx->stream;
+    int flags = 0;
+    float4 lim0, imp;
+    HIPCHK(ctx, hipMemcpyAsync(&flags, ctx->jFlags + joint, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(&lim0, ctx->jLimits + joint, sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(&imp, ctx->jImpulse + joint, sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    if (jointType(flags) != type) return fail(ctx, B2G_ERR_ARG, what);
+    if (type == JT_PRISMATIC) {
+        // b2PrismaticJoint::EnableLimit / SetLimits, REF/Dynamics/Joints/b2PrismaticJoint.cpp:546-578
+        const bool limitSwitched = ((flags & JF_LIMIT) != 0) != (enableLimit != 0);
+        const bool limitsMoved = lim0.y != par.y || lim0.z != par.z;
+        if (limitSwitched || limitsMoved) imp.z = 0.0f;
+        flags = (flags & ~JF_LIMIT) | (enableLimit ? JF_LIMIT : 0);
+    }
+    flags = (flags & ~JF_MOTOR) | (enableMotor ? JF_MOTOR : 0);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jFlags + joint, &flags, sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jLimits + joint, &par, sizeof(float4), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jMotorSpeed + joint, &speed, sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jImpulse + joint, &imp, sizeof(float4), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+int b2g_set_prismatic_joint(b2g_context* ctx, int joint, const b2g_prismatic_def* d) {
+    if (!d) return B2G_ERR_ARG;
+    return setAxisJoint(ctx, joint, JT_PRISMATIC, "not a prismatic joint",
+                        make_float4(d->referenceAngle, d->lowerTranslation, d->upperTranslation, d->maxMotorForce),
+                        d->enableLimit, d->enableMotor, d->motorSpeed);
+}
+
+int b2g_add_wheel_joints(b2g_context* ctx, const b2g_wheel_def* defs, int count, int* first) {
+    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
+    std::vector<JointRow> rows;
+    for (int i = 0; i < count; ++i) {
+        const b2g_wheel_def& d = defs[i];
+        JointRow r = jointRow(d, JT_WHEEL, make_float4(d.frequencyHz, d.dampingRatio, d.maxMotorTorque, 0.0f));
+        r.flags |= d.enableMotor ? JF_MOTOR : 0;
+        r.axis = make_float4(d.localAxisA[0], d.localAxisA[1], 0.0f, 0.0f);
+        r.speed = d.motorSpeed;
+        rows.push_back(r);
+    }
+    return appendJointRows(ctx, rows, first);
+}
+
+int b2g_set_wheel_joint(b2g_context* ctx, int joint, const b2g_wheel_def* d) {
+    if (!d) return B2G_ERR_ARG;
+    return setAxisJoint(ctx, joint, JT_WHEEL, "not a wheel joint",
+                        make_float4(d->frequencyHz, d->dampingRatio, d->maxMotorTorque, 0.0f), 0, d->enableMotor,
+                        d->motorSpeed);
+}
+
+int b2g_get_joint_reactions(b2g_context* ctx, int first, int count, float* out) {
+    if (!ctx || first < 0 || count < 0 || first + count > ctx->nJoints || (count > 0 && !out)) return B2G_ERR_ARG;
+    if (count == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    float4* d = nullptr;
+    HIPCHK(ctx, hipMalloc((void**)&d, sizeof(float4) * count));
+    hipLaunchKernelGGL(k_joint_reactions, dim3(blocks(count)), dim3(256), 0, s, ctx->jview(), first, count, d);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipMemcpyAsync(out, d, sizeof(float4) * count, hipMemcpyDeviceToHost, s);
+    const hipError_t e3 = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    HIPCHK(ctx, e1);
+    HIPCHK(ctx, e2);
+    HIPCHK(ctx, e3);
     return B2G_OK;
 }
 

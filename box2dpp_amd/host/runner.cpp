@@ -153,6 +153,44 @@ int loadJoints(Runner* r, int base, const b2scene::JointDef* jd, int nj) {
             r->joints.push_back(j);
             continue;
         }
+        if (d.type == 8) {
+            b2PrismaticJointDef pd;
+            pd.bodyA = r->bodies[base + d.bodyA];
+            pd.bodyB = r->bodies[base + d.bodyB];
+            pd.collideConnected = d.collideConnected != 0;
+            pd.localAnchorA = b2Vec2(d.ax, d.ay);
+            pd.localAnchorB = b2Vec2(d.bx, d.by);
+            pd.localAxisA = b2Vec2(d.axisX, d.axisY);
+            pd.referenceAngle = d.referenceAngle;
+            pd.enableLimit = d.enableLimit != 0;
+            pd.lowerTranslation = d.lowerAngle;
+            pd.upperTranslation = d.upperAngle;
+            pd.enableMotor = d.enableMotor != 0;
+            pd.maxMotorForce = d.maxMotorTorque;
+            pd.motorSpeed = d.motorSpeed;
+            b2Joint* j = r->world.CreateJoint(&pd);
+            if (j == nullptr) return B2G_ERR_ARG;
+            r->joints.push_back(j);
+            continue;
+        }
+        if (d.type == 9) {
+            b2WheelJointDef wd;
+            wd.bodyA = r->bodies[base + d.bodyA];
+            wd.bodyB = r->bodies[base + d.bodyB];
+            wd.collideConnected = d.collideConnected != 0;
+            wd.localAnchorA = b2Vec2(d.ax, d.ay);
+            wd.localAnchorB = b2Vec2(d.bx, d.by);
+            wd.localAxisA = b2Vec2(d.axisX, d.axisY);
+            wd.enableMotor = d.enableMotor != 0;
+            wd.maxMotorTorque = d.maxMotorTorque;
+            wd.motorSpeed = d.motorSpeed;
+            wd.frequencyHz = d.frequencyHz;
+            wd.dampingRatio = d.dampingRatio;
+            b2Joint* j = r->world.CreateJoint(&wd);
+            if (j == nullptr) return B2G_ERR_ARG;
+            r->joints.push_back(j);
+            continue;
+        }
         if (d.type == 6) {
             b2PulleyJointDef pd;
             pd.bodyA = r->bodies[base + d.bodyA];
@@ -331,6 +369,35 @@ int b2pp_world_set_distance(void* h, int joint, float length, float hz, float ra
     j->SetLength(length);
     j->SetFrequency(hz);
     j->SetDampingRatio(ratio);
+    return 0;
+}
+
+// b2PrismaticJoint's limit and motor setters.
+int b2pp_world_set_prismatic(void* h, int joint, int enableLimit, float lower, float upper, int enableMotor,
+                             float speed, float maxForce) {
+    Runner* r = (Runner*)h;
+    if (joint < 0 || joint >= (int)r->joints.size() || r->joints[joint] == nullptr) return B2G_ERR_ARG;
+    if (r->joints[joint]->GetType() != b2JointType::PRISMATIC_JOINT) return B2G_ERR_ARG;
+    b2PrismaticJoint* j = static_cast<b2PrismaticJoint*>(r->joints[joint]);
+    j->EnableLimit(enableLimit != 0);
+    j->SetLimits(lower, upper);
+    j->EnableMotor(enableMotor != 0);
+    j->SetMotorSpeed(speed);
+    j->SetMaxMotorForce(maxForce);
+    return 0;
+}
+
+// b2WheelJoint's motor and spring setters.
+int b2pp_world_set_wheel(void* h, int joint, int enableMotor, float speed, float maxTorque, float hz, float ratio) {
+    Runner* r = (Runner*)h;
+    if (joint < 0 || joint >= (int)r->joints.size() || r->joints[joint] == nullptr) return B2G_ERR_ARG;
+    if (r->joints[joint]->GetType() != b2JointType::WHEEL_JOINT) return B2G_ERR_ARG;
+    b2WheelJoint* j = static_cast<b2WheelJoint*>(r->joints[joint]);
+    j->EnableMotor(enableMotor != 0);
+    j->SetMotorSpeed(speed);
+    j->SetMaxMotorTorque(maxTorque);
+    j->SetSpringFrequencyHz(hz);
+    j->SetSpringDampingRatio(ratio);
     return 0;
 }
 

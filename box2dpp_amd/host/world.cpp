@@ -566,6 +566,158 @@ void b2WeldJoint::SetDampingRatio(float ratio) {
     MarkDirty();
 }
 
+// Reaction force and torque of one joint at inv_dt = 1 from the device.
+static b2g_vec2 deviceReaction(b2g_context* ctx, int32_t index, float* torque) {
+    float r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (ctx) (void)b2g_get_joint_reactions(ctx, index, 1, r);
+    if (torque) *torque = r[2];
+    b2g_vec2 f;
+    f.x = r[0];
+    f.y = r[1];
+    return f;
+}
+
+// b2PrismaticJointDef::Initialize, REF/Dynamics/Joints/b2PrismaticJoint.cpp:90-99
+void b2PrismaticJointDef::Initialize(b2Body* bA, b2Body* bB, const b2Vec2& anchor, const b2Vec2& axis) {
+    bodyA = bA;
+    bodyB = bB;
+    localAnchorA = bodyA->GetLocalPoint(anchor);
+    localAnchorB = bodyB->GetLocalPoint(anchor);
+    localAxisA = bodyA->GetLocalVector(axis);
+    referenceAngle = bodyB->GetAngle() - bodyA->GetAngle();
+}
+
+// The constructor normalizes the axis (REF b2PrismaticJoint.cpp:107-109).
+b2PrismaticJoint::b2PrismaticJoint(const b2PrismaticJointDef* def, b2World* world, int32_t index)
+    : b2Joint(def, world, index), m_localAnchorA(def->localAnchorA), m_localAnchorB(def->localAnchorB),
+      m_localXAxisA(def->localAxisA), m_referenceAngle(def->referenceAngle), m_enableLimit(def->enableLimit),
+      m_lowerTranslation(def->lowerTranslation), m_upperTranslation(def->upperTranslation),
+      m_enableMotor(def->enableMotor), m_maxMotorForce(def->maxMotorForce), m_motorSpeed(def->motorSpeed) {
+    const float len = sqrtf(m_localXAxisA[0] * m_localXAxisA[0] + m_localXAxisA[1] * m_localXAxisA[1]);
+    if (!(len < FLT_EPSILON)) {
+        const float inv = 1.0f / len;
+        m_localXAxisA = b2Vec2(m_localXAxisA[0] * inv, m_localXAxisA[1] * inv);
+    }
+}
+
+b2Vec2 b2PrismaticJoint::GetAnchorA() const { return m_bodyA->GetWorldPoint(m_localAnchorA); }
+b2Vec2 b2PrismaticJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localAnchorB); }
+// REF/Dynamics/Joints/b2PrismaticJoint.cpp:509-518
+float b2PrismaticJoint::GetJointTranslation() const {
+    const b2Vec2 d = GetAnchorB() - GetAnchorA();
+    const b2Vec2 axis = m_bodyA->GetWorldVector(m_localXAxisA);
+    return d[0] * axis[0] + d[1] * axis[1];
+}
+// REF/Dynamics/Joints/b2PrismaticJoint.cpp:499-507 (the force along the
+// last step's axes, from the device), and GetMotorForce.
+b2Vec2 b2PrismaticJoint::GetReactionForce(float inv_dt) const {
+    if (m_dirty) return b2Vec2(0.0f, 0.0f);
+    const b2g_vec2 f = deviceReaction(m_world->m_ctx, m_index, nullptr);
+    return b2Vec2(inv_dt * f.x, inv_dt * f.y);
+}
+float b2PrismaticJoint::GetReactionTorque(float inv_dt) const {
+    m_world->DownloadJoints();
+    return inv_dt * m_impulse[1];
+}
+float b2PrismaticJoint::GetMotorForce(float inv_dt) const {
+    m_world->DownloadJoints();
+    return inv_dt * m_impulse[3];
+}
+// REF/Dynamics/Joints/b2PrismaticJoint.cpp:546-609
+void b2PrismaticJoint::EnableLimit(bool flag) {
+    if (flag != m_enableLimit) {
+        m_bodyA->SetAwake(true);
+        m_bodyB->SetAwake(true);
+        m_enableLimit = flag;
+        MarkDirty();
+    }
+}
+void b2PrismaticJoint::SetLimits(float lower, float upper) {
+    if (lower != m_lowerTranslation || upper != m_upperTranslation) {
+        m_bodyA->SetAwake(true);
+        m_bodyB->SetAwake(true);
+        m_lowerTranslation = lower;
+        m_upperTranslation = upper;
+        MarkDirty();
+    }
+}
+void b2PrismaticJoint::EnableMotor(bool flag) {
+    m_bodyA->SetAwake(true);
+    m_bodyB->SetAwake(true);
+    m_enableMotor = flag;
+    MarkDirty();
+}
+void b2PrismaticJoint::SetMotorSpeed(float speed) {
+    m_bodyA->SetAwake(true);
+    m_bodyB->SetAwake(true);
+    m_motorSpeed = speed;
+    MarkDirty();
+}
+void b2PrismaticJoint::SetMaxMotorForce(float force) {
+    m_bodyA->SetAwake(true);
+    m_bodyB->SetAwake(true);
+    m_maxMotorForce = force;
+    MarkDirty();
+}
+
+// b2WheelJointDef::Initialize, REF/Dynamics/Joints/b2WheelJoint.cpp:41-48
+void b2WheelJointDef::Initialize(b2Body* bA, b2Body* bB, const b2Vec2& anchor, const b2Vec2& axis) {
+    bodyA = bA;
+    bodyB = bB;
+    localAnchorA = bodyA->GetLocalPoint(anchor);
+    localAnchorB = bodyB->GetLocalPoint(anchor);
+    localAxisA = bodyA->GetLocalVector(axis);
+}
+
+b2WheelJoint::b2WheelJoint(const b2WheelJointDef* def, b2World* world, int32_t index)
+    : b2Joint(def, world, index), m_localAnchorA(def->localAnchorA), m_localAnchorB(def->localAnchorB),
+      m_localXAxisA(def->localAxisA), m_enableMotor(def->enableMotor), m_maxMotorTorque(def->maxMotorTorque),
+      m_motorSpeed(def->motorSpeed), m_frequencyHz(def->frequencyHz), m_dampingRatio(def->dampingRatio) {}
+
+b2Vec2 b2WheelJoint::GetAnchorA() const { return m_bodyA->GetWorldPoint(m_localAnchorA); }
+b2Vec2 b2WheelJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localAnchorB); }
+// REF/Dynamics/Joints/b2WheelJoint.cpp:341-349
+b2Vec2 b2WheelJoint::GetReactionForce(float inv_dt) const {
+    if (m_dirty) return b2Vec2(0.0f, 0.0f);
+    const b2g_vec2 f = deviceReaction(m_world->m_ctx, m_index, nullptr);
+    return b2Vec2(inv_dt * f.x, inv_dt * f.y);
+}
+float b2WheelJoint::GetReactionTorque(float inv_dt) const {
+    m_world->DownloadJoints();
+    return inv_dt * m_impulse[2];
+}
+float b2WheelJoint::GetMotorTorque(float inv_dt) const {
+    m_world->DownloadJoints();
+    return inv_dt * m_impulse[2];
+}
+// REF/Dynamics/Joints/b2WheelJoint.cpp:377-401
+void b2WheelJoint::EnableMotor(bool flag) {
+    m_bodyA->SetAwake(true);
+    m_bodyB->SetAwake(true);
+    m_enableMotor = flag;
+    MarkDirty();
+}
+void b2WheelJoint::SetMotorSpeed(float speed) {
+    m_bodyA->SetAwake(true);
+    m_bodyB->SetAwake(true);
+    m_motorSpeed = speed;
+    MarkDirty();
+}
+void b2WheelJoint::SetMaxMotorTorque(float torque) {
+    m_bodyA->SetAwake(true);
+    m_bodyB->SetAwake(true);
+    m_maxMotorTorque = torque;
+    MarkDirty();
+}
+void b2WheelJoint::SetSpringFrequencyHz(float hz) {
+    m_frequencyHz = hz;
+    MarkDirty();
+}
+void b2WheelJoint::SetSpringDampingRatio(float ratio) {
+    m_dampingRatio = ratio;
+    MarkDirty();
+}
+
 // b2PulleyJointDef::Initialize, REF/Dynamics/Joints/b2PulleyJoint.cpp:38-55
 void b2PulleyJointDef::Initialize(b2Body* bA, b2Body* bB, const b2Vec2& groundA, const b2Vec2& groundB,
                                   const b2Vec2& anchorA, const b2Vec2& anchorB, float r) {
@@ -775,8 +927,9 @@ b2Joint* b2World::CreateJoint(const b2JointDef* def) {
     if (def->type != b2JointType::REVOLUTE_JOINT && def->type != b2JointType::DISTANCE_JOINT &&
         def->type != b2JointType::ROPE_JOINT && def->type != b2JointType::WELD_JOINT &&
         def->type != b2JointType::FRICTION_JOINT && def->type != b2JointType::MOTOR_JOINT &&
-        def->type != b2JointType::PULLEY_JOINT && def->type != b2JointType::MOUSE_JOINT) {
-        m_lastError = "prismatic, wheel and gear joints do not run on the GPU path";
+        def->type != b2JointType::PULLEY_JOINT && def->type != b2JointType::MOUSE_JOINT &&
+        def->type != b2JointType::PRISMATIC_JOINT && def->type != b2JointType::WHEEL_JOINT) {
+        m_lastError = "gear joints do not run on the GPU path";
         return nullptr;
     }
     if (def->bodyA == def->bodyB || def->bodyA->m_world != this || def->bodyB->m_world != this) return nullptr;
@@ -800,6 +953,10 @@ b2Joint* b2World::CreateJoint(const b2JointDef* def) {
         j = new b2PulleyJoint(static_cast<const b2PulleyJointDef*>(def), this, index);
     else if (def->type == b2JointType::MOUSE_JOINT)
         j = new b2MouseJoint(static_cast<const b2MouseJointDef*>(def), this, index);
+    else if (def->type == b2JointType::PRISMATIC_JOINT)
+        j = new b2PrismaticJoint(static_cast<const b2PrismaticJointDef*>(def), this, index);
+    else if (def->type == b2JointType::WHEEL_JOINT)
+        j = new b2WheelJoint(static_cast<const b2WheelJointDef*>(def), this, index);
     else
         j = new b2RevoluteJoint(static_cast<const b2RevoluteJointDef*>(def), this, (int32_t)m_joints.size());
     m_joints.push_back(j);
@@ -973,6 +1130,46 @@ static b2g_mouse_def mouseDef(const b2MouseJoint* d, int32_t bodyA, int32_t body
     r.frequencyHz = d->GetFrequency();
     r.dampingRatio = d->GetDampingRatio();
     r.massB = massB;
+    r.collideConnected = d->GetCollideConnected() ? 1 : 0;
+    return r;
+}
+
+static b2g_prismatic_def prismaticDef(const b2PrismaticJoint* d, int32_t bodyA, int32_t bodyB) {
+    b2g_prismatic_def r{};
+    r.bodyA = bodyA;
+    r.bodyB = bodyB;
+    r.localAnchorA[0] = d->GetLocalAnchorA()[0];
+    r.localAnchorA[1] = d->GetLocalAnchorA()[1];
+    r.localAnchorB[0] = d->GetLocalAnchorB()[0];
+    r.localAnchorB[1] = d->GetLocalAnchorB()[1];
+    r.localAxisA[0] = d->GetLocalAxisA()[0];
+    r.localAxisA[1] = d->GetLocalAxisA()[1];
+    r.referenceAngle = d->GetReferenceAngle();
+    r.lowerTranslation = d->GetLowerLimit();
+    r.upperTranslation = d->GetUpperLimit();
+    r.maxMotorForce = d->GetMaxMotorForce();
+    r.motorSpeed = d->GetMotorSpeed();
+    r.enableLimit = d->IsLimitEnabled() ? 1 : 0;
+    r.enableMotor = d->IsMotorEnabled() ? 1 : 0;
+    r.collideConnected = d->GetCollideConnected() ? 1 : 0;
+    return r;
+}
+
+static b2g_wheel_def wheelDef(const b2WheelJoint* d, int32_t bodyA, int32_t bodyB) {
+    b2g_wheel_def r{};
+    r.bodyA = bodyA;
+    r.bodyB = bodyB;
+    r.localAnchorA[0] = d->GetLocalAnchorA()[0];
+    r.localAnchorA[1] = d->GetLocalAnchorA()[1];
+    r.localAnchorB[0] = d->GetLocalAnchorB()[0];
+    r.localAnchorB[1] = d->GetLocalAnchorB()[1];
+    r.localAxisA[0] = d->GetLocalAxisA()[0];
+    r.localAxisA[1] = d->GetLocalAxisA()[1];
+    r.maxMotorTorque = d->GetMaxMotorTorque();
+    r.motorSpeed = d->GetMotorSpeed();
+    r.frequencyHz = d->GetSpringFrequencyHz();
+    r.dampingRatio = d->GetSpringDampingRatio();
+    r.enableMotor = d->IsMotorEnabled() ? 1 : 0;
     r.collideConnected = d->GetCollideConnected() ? 1 : 0;
     return r;
 }
@@ -1205,6 +1402,20 @@ int32_t b2World::Upload() {
                     defs.push_back(mouseDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index, d->m_bodyB->m_mass));
                 }
                 rc = b2g_add_mouse_joints(m_ctx, defs.data(), (int)defs.size(), &first);
+            } else if (type == b2JointType::PRISMATIC_JOINT) {
+                std::vector<b2g_prismatic_def> defs;
+                for (int q = i; q < k; ++q) {
+                    const b2PrismaticJoint* d = static_cast<const b2PrismaticJoint*>(m_joints[q]);
+                    defs.push_back(prismaticDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index));
+                }
+                rc = b2g_add_prismatic_joints(m_ctx, defs.data(), (int)defs.size(), &first);
+            } else if (type == b2JointType::WHEEL_JOINT) {
+                std::vector<b2g_wheel_def> defs;
+                for (int q = i; q < k; ++q) {
+                    const b2WheelJoint* d = static_cast<const b2WheelJoint*>(m_joints[q]);
+                    defs.push_back(wheelDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index));
+                }
+                rc = b2g_add_wheel_joints(m_ctx, defs.data(), (int)defs.size(), &first);
             } else {
                 std::vector<b2g_revolute_def> defs;
                 for (int q = i; q < k; ++q) {
@@ -1254,6 +1465,14 @@ int32_t b2World::Upload() {
                 rc = b2g_set_mouse_joint(m_ctx, j->m_index, &d);
             } else if (j->m_type == b2JointType::PULLEY_JOINT) {
                 rc = B2G_OK;  // no setters
+            } else if (j->m_type == b2JointType::PRISMATIC_JOINT) {
+                const b2PrismaticJoint* p = static_cast<const b2PrismaticJoint*>(j);
+                const b2g_prismatic_def d = prismaticDef(p, p->m_bodyA->m_index, p->m_bodyB->m_index);
+                rc = b2g_set_prismatic_joint(m_ctx, j->m_index, &d);
+            } else if (j->m_type == b2JointType::WHEEL_JOINT) {
+                const b2WheelJoint* p = static_cast<const b2WheelJoint*>(j);
+                const b2g_wheel_def d = wheelDef(p, p->m_bodyA->m_index, p->m_bodyB->m_index);
+                rc = b2g_set_wheel_joint(m_ctx, j->m_index, &d);
             } else {
                 const b2RevoluteJoint* r = static_cast<const b2RevoluteJoint*>(j);
                 const b2g_revolute_def d = revoluteDef(r, r->m_bodyA->m_index, r->m_bodyB->m_index);

@@ -82,6 +82,11 @@ struct JointDef {
     float length, frequencyHz, dampingRatio;
     float maxForce, maxTorque, correctionFactor;
     float gax, gay, gbx, gby, lengthB, ratio;
+    // b2PrismaticJointDef (b2PrismaticJoint.h:30-92: localAxisA, referenceAngle,
+    // lower / upper translation in lowerAngle / upperAngle, maxMotorForce in
+    // maxMotorTorque, motorSpeed) when 8; b2WheelJointDef (b2WheelJoint.h:30-72:
+    // localAxisA, frequencyHz, dampingRatio, maxMotorTorque, motorSpeed) when 9
+    float axisX, axisY;
 };
 
 struct Scene {
@@ -719,6 +724,81 @@ inline Scene mouseJoints(int n) {
     return s;
 }
 
+// Prismatic joints: n boxes (1 kg) sliding on prismatic joints from the
+// ground body along axes at 0-75 degrees (some above 45, so they slide down
+// under gravity), started at rest or moving; every third has a translation
+// limit of [-1, 2] m, every fourth a motor (2 m/s, 15-30 N), one an equal
+// limit, over a ground edge 6 m below.
+inline Scene prismatics(int n) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-400.0f, 0.0f, 400.0f, 0.0f));
+    for (int i = 0; i < n; ++i) {
+        const float x = 8.0f * (float)i - 4.0f * (float)n, y = 8.0f;
+        BodyDef bd = defaultBody(2, x, y);
+        bd.vx = 0.5f * (float)(i % 3);
+        bd.w = 0.3f * (float)(i % 2);
+        addBody(s, bd);
+        addFixture(s, boxFixture(0.5f, 0.5f, 1.0f, 0.4f));
+        JointDef j{};
+        j.type = 8;
+        j.bodyA = 0; j.bodyB = (int)s.bodies.size() - 1;
+        j.ax = x; j.ay = y;
+        const float ang = 0.2618f * (float)(i % 6);
+        j.axisX = 2.0f * cosf(ang); j.axisY = 2.0f * sinf(ang);   // normalized on creation
+        j.enableLimit = (i % 3) == 0;
+        j.lowerAngle = -1.0f; j.upperAngle = (i == 6) ? -1.0f : 2.0f;
+        j.enableMotor = (i % 4) == 1;
+        j.motorSpeed = 2.0f;
+        j.maxMotorTorque = 15.0f + 15.0f * (float)(i % 2);
+        s.joints.push_back(j);
+    }
+    return s;
+}
+
+// Wheel joints: n carts, each a chassis (2 x 0.5 m box) on two circle wheels
+// (r 0.4) held by wheel joints along the vertical axis (4 Hz, damping 0.7);
+// the rear wheel of every other cart is driven (-10 rad/s, 20 N m) over a
+// ground edge with bumps. `single` gives each chassis one wheel on a flat
+// ground (one joint and one contact per wheel).
+inline Scene wheels(int n, bool single = false) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-400.0f, 0.0f, 400.0f, 0.0f));
+    for (int i = -10; i < 10 && !single; ++i) {   // bumps
+        FixtureDef f = boxFixture(0.3f, 0.1f, 0.0f, 0.6f);
+        f.polyMode = kPolyBoxRotated;
+        f.pts[2] = 6.0f * (float)i + 1.0f; f.pts[3] = 0.05f; f.pts[4] = 0.0f;
+        addFixture(s, f);
+    }
+    for (int i = 0; i < n; ++i) {
+        const float x = 12.0f * (float)i - 6.0f * (float)n, y = 1.4f;
+        addBody(s, defaultBody(2, x, y));
+        addFixture(s, boxFixture(1.0f, 0.25f, 1.0f, 0.6f));
+        const int chassis = (int)s.bodies.size() - 1;
+        for (int k = 0; k < (single ? 1 : 2); ++k) {
+            const float wx = x + (k == 0 ? -0.7f : 0.7f), wy = 0.6f;
+            addBody(s, defaultBody(2, wx, wy));
+            FixtureDef wf = defaultFixture(kCircle);
+            wf.radius = 0.4f;
+            wf.density = 1.0f; wf.friction = 0.9f;
+            addFixture(s, wf);
+            JointDef j{};
+            j.type = 9;
+            j.bodyA = chassis; j.bodyB = (int)s.bodies.size() - 1;
+            j.ax = wx - x; j.ay = wy - y;
+            j.axisX = 0.0f; j.axisY = 1.0f;
+            j.frequencyHz = 4.0f;
+            j.dampingRatio = 0.7f;
+            j.enableMotor = (k == 0 && (i & 1) == 0);
+            j.motorSpeed = -10.0f;
+            j.maxMotorTorque = 20.0f;
+            s.joints.push_back(j);
+        }
+    }
+    return s;
+}
+
 // Scene by name: "hello", "pyramid" (n = rows, 0 -> 20), "pile" (n bodies,
 // seed), "world" (one batched-scene world of n bodies, seed), "pendulums"
 // (n pendulums), "chains" (n chains of `seed` links), "stress" (pile of n
@@ -731,7 +811,9 @@ inline Scene mouseJoints(int n) {
 // "welds" (n welded beams; seed 0 drops a box on each, seed 2 makes the
 // beams single links), "friction" (n boxes on friction joints), "motor" (n
 // boxes on motor joints; seed 0 drives every third from the previous box),
-// "pulley" (n pulleys), "mouse" (n boxes on mouse joints). Unknown names give an empty scene.
+// "pulley" (n pulleys), "mouse" (n boxes on mouse joints), "prismatic" (n
+// boxes on prismatic joints), "wheels" (n carts; seed 1 gives each one
+// wheel on flat ground). Unknown names give an empty scene.
 inline Scene byName(const char* kind, int n, uint32_t seed) {
     const std::string k(kind ? kind : "");
     if (k == "hello") return hello();
@@ -753,6 +835,8 @@ inline Scene byName(const char* kind, int n, uint32_t seed) {
     if (k == "motor") return motorJoints(n > 0 ? n : 12, seed == 0);
     if (k == "pulley") return pulleys(n > 0 ? n : 8);
     if (k == "mouse") return mouseJoints(n > 0 ? n : 8);
+    if (k == "prismatic") return prismatics(n > 0 ? n : 12);
+    if (k == "wheels") return wheels(n > 0 ? n : 6, seed == 1);
     return Scene();
 }
 

@@ -272,7 +272,8 @@ int b2g_penetration(b2g_context* ctx, double* out);
  * error b2RevoluteJoint::SolvePositionConstraints measures,
  * Box2D/Dynamics/Joints/b2RevoluteJoint.cpp:336-343), the error in length
  * for rigid distance joints, the stretch beyond the maximum length for rope
- * joints, |lengthA + ratio lengthB - constant| for pulley joints; soft
+ * joints, |lengthA + ratio lengthB - constant| for pulley joints, the
+ * distance of anchor B from the axis line for prismatic and wheel joints; soft
  * distance, friction, motor and mouse joints are not measured. out[3], FP64 sums on the
  * host; no counterpart in the reference. */
 int b2g_joint_drift(b2g_context* ctx, double* out);
@@ -443,6 +444,53 @@ int b2g_add_mouse_joints(b2g_context* ctx, const b2g_mouse_def* defs, int count,
  * body B's mass when it changed (bodies, localAnchorB and collideConnected
  * of `def` are ignored). */
 int b2g_set_mouse_joint(b2g_context* ctx, int joint, const b2g_mouse_def* def);
+
+/* b2PrismaticJointDef (Box2D/Dynamics/Joints/b2PrismaticJoint.h:30-92) with
+ * body indices: body B slides along localAxisA of body A (normalized on
+ * creation), with an optional translation limit and motor. 64 bytes. */
+typedef struct b2g_prismatic_def {
+    int32_t bodyA, bodyB;
+    float localAnchorA[2];
+    float localAnchorB[2];
+    float localAxisA[2];
+    float referenceAngle;
+    float lowerTranslation, upperTranslation;
+    float maxMotorForce, motorSpeed;
+    int32_t enableLimit, enableMotor, collideConnected;
+} b2g_prismatic_def;
+/* Appends prismatic joints (b2PrismaticJoint,
+ * REF/Dynamics/Joints/b2PrismaticJoint.cpp:102-488); b2g_get_joints reports
+ * the impulses (x, y, limit, motor), b2g_get_joint_reactions the force. */
+int b2g_add_prismatic_joints(b2g_context* ctx, const b2g_prismatic_def* defs, int count, int* first);
+/* b2PrismaticJoint::EnableLimit, SetLimits, EnableMotor, SetMotorSpeed,
+ * SetMaxMotorForce (bodies, anchors, axis and reference angle keep their
+ * creation values); a changed limit drops the limit impulse. */
+int b2g_set_prismatic_joint(b2g_context* ctx, int joint, const b2g_prismatic_def* def);
+
+/* b2WheelJointDef (Box2D/Dynamics/Joints/b2WheelJoint.h:30-72) with body
+ * indices: body B (the wheel) rides along localAxisA of body A on a soft
+ * spring, with an optional motor. 56 bytes. */
+typedef struct b2g_wheel_def {
+    int32_t bodyA, bodyB;
+    float localAnchorA[2];
+    float localAnchorB[2];
+    float localAxisA[2];
+    float maxMotorTorque, motorSpeed;
+    float frequencyHz, dampingRatio;
+    int32_t enableMotor, collideConnected;
+} b2g_wheel_def;
+/* Appends wheel joints (b2WheelJoint, REF/Dynamics/Joints/b2WheelJoint.cpp:50-327);
+ * b2g_get_joints reports (impulse, spring impulse, motor impulse, 0),
+ * b2g_get_joint_reactions the force and torque. */
+int b2g_add_wheel_joints(b2g_context* ctx, const b2g_wheel_def* defs, int count, int* first);
+/* b2WheelJoint::EnableMotor, SetMotorSpeed, SetMaxMotorTorque,
+ * SetSpringFrequencyHz, SetSpringDampingRatio. */
+int b2g_set_wheel_joint(b2g_context* ctx, int joint, const b2g_wheel_def* def);
+
+/* b2Joint::GetReactionForce and GetReactionTorque at inv_dt = 1 for joints
+ * [first, first + count), every type: force x, y, torque, 0 (4 floats each);
+ * zeros for removed joints. Multiply by inv_dt for other steps. */
+int b2g_get_joint_reactions(b2g_context* ctx, int first, int count, float* out);
 /* Removes joints (b2World::DestroyJoint, Box2D/Dynamics/b2World.cpp:267):
  * they stop constraining and connecting islands at the next step; indices of
  * the other joints do not change. */
@@ -654,7 +702,7 @@ int b2g_get_graph_counts(b2g_context* ctx, int64_t* launches, int64_t* captures)
  * b2g_step_params, b2g_step_stats, b2g_phase_profile, b2g_mlp_params,
  * b2g_aabb, b2g_transform, b2g_revolute_def, b2g_schedule, b2g_sweep,
  * b2g_distance_def, b2g_rope_def, b2g_weld_def, b2g_friction_def, b2g_motor_def,
- * b2g_pulley_def, b2g_mouse_def.
+ * b2g_pulley_def, b2g_mouse_def, b2g_prismatic_def, b2g_wheel_def.
  * Returns the count (out may be null). Needs no device; lets bindings check
  * their struct layouts. */
 int b2g_abi_sizes(int32_t* out, int cap);

@@ -645,6 +645,106 @@ private:
     float m_maxForce, m_frequencyHz, m_dampingRatio;
 };
 
+// Box2D/Dynamics/Joints/b2PrismaticJoint.h:30-92
+struct b2PrismaticJointDef : public b2JointDef {
+    b2PrismaticJointDef() { type = b2JointType::PRISMATIC_JOINT; }
+    // Bodies, anchors, axis and reference angle from a world anchor and axis.
+    void Initialize(b2Body* bodyA, b2Body* bodyB, const b2Vec2& anchor, const b2Vec2& axis);
+    b2Vec2 localAnchorA{0.0f, 0.0f};
+    b2Vec2 localAnchorB{0.0f, 0.0f};
+    b2Vec2 localAxisA{1.0f, 0.0f};
+    float referenceAngle = 0.0f;
+    bool enableLimit = false;
+    float lowerTranslation = 0.0f;
+    float upperTranslation = 0.0f;
+    bool enableMotor = false;
+    float maxMotorForce = 0.0f;
+    float motorSpeed = 0.0f;
+};
+
+// Box2D/Dynamics/Joints/b2PrismaticJoint.h:95-200: body B slides along an
+// axis fixed in body A.
+class b2PrismaticJoint : public b2Joint {
+public:
+    b2Vec2 GetAnchorA() const override;
+    b2Vec2 GetAnchorB() const override;
+    const b2Vec2& GetLocalAnchorA() const { return m_localAnchorA; }
+    const b2Vec2& GetLocalAnchorB() const { return m_localAnchorB; }
+    const b2Vec2& GetLocalAxisA() const { return m_localXAxisA; }
+    float GetReferenceAngle() const { return m_referenceAngle; }
+    float GetJointTranslation() const;
+    bool IsLimitEnabled() const { return m_enableLimit; }
+    void EnableLimit(bool flag);
+    float GetLowerLimit() const { return m_lowerTranslation; }
+    float GetUpperLimit() const { return m_upperTranslation; }
+    void SetLimits(float lower, float upper);
+    bool IsMotorEnabled() const { return m_enableMotor; }
+    void EnableMotor(bool flag);
+    void SetMotorSpeed(float speed);
+    float GetMotorSpeed() const { return m_motorSpeed; }
+    void SetMaxMotorForce(float force);
+    float GetMaxMotorForce() const { return m_maxMotorForce; }
+    float GetMotorForce(float inv_dt) const;
+    b2Vec2 GetReactionForce(float inv_dt) const override;
+    float GetReactionTorque(float inv_dt) const override;
+
+private:
+    friend class b2World;
+    b2PrismaticJoint(const b2PrismaticJointDef* def, b2World* world, int32_t index);
+    b2Vec2 m_localAnchorA, m_localAnchorB, m_localXAxisA;
+    float m_referenceAngle;
+    bool m_enableLimit;
+    float m_lowerTranslation, m_upperTranslation;
+    bool m_enableMotor;
+    float m_maxMotorForce, m_motorSpeed;
+};
+
+// Box2D/Dynamics/Joints/b2WheelJoint.h:30-72
+struct b2WheelJointDef : public b2JointDef {
+    b2WheelJointDef() { type = b2JointType::WHEEL_JOINT; }
+    // Bodies, anchors and axis from a world anchor and axis.
+    void Initialize(b2Body* bodyA, b2Body* bodyB, const b2Vec2& anchor, const b2Vec2& axis);
+    b2Vec2 localAnchorA{0.0f, 0.0f};
+    b2Vec2 localAnchorB{0.0f, 0.0f};
+    b2Vec2 localAxisA{1.0f, 0.0f};
+    bool enableMotor = false;
+    float maxMotorTorque = 0.0f;
+    float motorSpeed = 0.0f;
+    float frequencyHz = 2.0f;
+    float dampingRatio = 0.7f;
+};
+
+// Box2D/Dynamics/Joints/b2WheelJoint.h:75-190: a wheel on a suspension
+// spring along an axis of body A, with a motor.
+class b2WheelJoint : public b2Joint {
+public:
+    b2Vec2 GetAnchorA() const override;
+    b2Vec2 GetAnchorB() const override;
+    const b2Vec2& GetLocalAnchorA() const { return m_localAnchorA; }
+    const b2Vec2& GetLocalAnchorB() const { return m_localAnchorB; }
+    const b2Vec2& GetLocalAxisA() const { return m_localXAxisA; }
+    bool IsMotorEnabled() const { return m_enableMotor; }
+    void EnableMotor(bool flag);
+    void SetMotorSpeed(float speed);
+    float GetMotorSpeed() const { return m_motorSpeed; }
+    void SetMaxMotorTorque(float torque);
+    float GetMaxMotorTorque() const { return m_maxMotorTorque; }
+    float GetMotorTorque(float inv_dt) const;
+    void SetSpringFrequencyHz(float hz);
+    float GetSpringFrequencyHz() const { return m_frequencyHz; }
+    void SetSpringDampingRatio(float ratio);
+    float GetSpringDampingRatio() const { return m_dampingRatio; }
+    b2Vec2 GetReactionForce(float inv_dt) const override;
+    float GetReactionTorque(float inv_dt) const override;
+
+private:
+    friend class b2World;
+    b2WheelJoint(const b2WheelJointDef* def, b2World* world, int32_t index);
+    b2Vec2 m_localAnchorA, m_localAnchorB, m_localXAxisA;
+    bool m_enableMotor;
+    float m_maxMotorTorque, m_motorSpeed, m_frequencyHz, m_dampingRatio;
+};
+
 // Per-step solver options of the GPU path (no reference counterpart).
 struct b2SolverOptions {
     int32_t mode = 0;                 // 0 coloured, 1 reference order
@@ -731,6 +831,8 @@ private:
     friend class b2MotorJoint;
     friend class b2PulleyJoint;
     friend class b2MouseJoint;
+    friend class b2PrismaticJoint;
+    friend class b2WheelJoint;
     int32_t Upload();
     void Download() const;
     void DownloadJoints() const;

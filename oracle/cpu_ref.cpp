@@ -1344,6 +1344,13 @@ struct Joint {
     V2 groundAnchorA, groundAnchorB, uA, uB;
     float lengthB, ratio, constant, pulleyMass, beta;
     V2 mouseC;
+    // b2PrismaticJoint (type 8: lowerAngle / upperAngle hold the
+    // translations, maxMotorTorque the force), REF/Dynamics/Joints/b2PrismaticJoint.h:150-178;
+    // b2WheelJoint (type 9), REF/Dynamics/Joints/b2WheelJoint.h:150-190
+    V2 localXAxisA, localYAxisA, axis, perp;
+    float s1, s2, a1, a2;
+    V2 ax, ay;
+    float sAx, sBx, sAy, sBy, wheelMass, springMass, springImpulse;
     // solver temporaries
     int indexA, indexB;
     V2 rA, rB, localCenterA, localCenterB;
@@ -1541,6 +1548,12 @@ struct World {
         j.uA = j.uB = mk(0.0f, 0.0f);
         j.pulleyMass = j.beta = 0.0f;
         j.mouseC = mk(0.0f, 0.0f);
+        j.axis = j.perp = j.ax = j.ay = mk(0.0f, 0.0f);
+        j.s1 = j.s2 = j.a1 = j.a2 = 0.0f;
+        j.sAx = j.sBx = j.sAy = j.sBy = j.wheelMass = j.springMass = j.springImpulse = 0.0f;
+        j.localXAxisA = mk(d.axisX, d.axisY);
+        if (d.type == 8) normalize(j.localXAxisA);  // b2PrismaticJoint ctor, REF b2PrismaticJoint.cpp:107-109
+        j.localYAxisA = cross(1.0f, j.localXAxisA);
         if (d.type == 7) {
             // b2MouseJoint ctor (REF/Dynamics/Joints/b2MouseJoint.cpp:34-53) with
             // the target on the grab point, then SetTarget (:55-62)
@@ -2150,6 +2163,14 @@ struct World {
             mouseInitVelocity(j, h, dtRatio);
             return;
         }
+        if (j.type == 8) {
+            prismaticInitVelocity(j, dtRatio);
+            return;
+        }
+        if (j.type == 9) {
+            wheelInitVelocity(j, h, dtRatio);
+            return;
+        }
         const float aA = posA[j.indexA];
         V2 vA = velV[j.indexA]; float wA = velW[j.indexA];
         const float aB = posA[j.indexB];
@@ -2228,6 +2249,14 @@ struct World {
         }
         if (j.type == 7) {
             mouseSolveVelocity(j, h);
+            return;
+        }
+        if (j.type == 8) {
+            prismaticSolveVelocity(j, h);
+            return;
+        }
+        if (j.type == 9) {
+            wheelSolveVelocity(j, h);
             return;
         }
         V2 vA = velV[j.indexA]; float wA = velW[j.indexA];
@@ -2675,6 +2704,338 @@ struct World {
         velV[j.indexB] = vB; velW[j.indexB] = wB;
     }
 
+    // b2PrismaticJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2PrismaticJoint.cpp:128-266
+    void prismaticInitVelocity(Joint& j, float dtRatio) {
+        const V2 cA = posC[j.indexA], cB = posC[j.indexB];
+        const float aA = posA[j.indexA], aB = posA[j.indexB];
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        const V2 rA = mul(qA, j.localAnchorA - j.localCenterA);
+        const V2 rB = mul(qB, j.localAnchorB - j.localCenterB);
+        const V2 d = (cB - cA) + rB - rA;
+        const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+        j.axis = mul(qA, j.localXAxisA);
+        j.a1 = cross(d + rA, j.axis);
+        j.a2 = cross(rB, j.axis);
+        j.motorMass = mA + mB + iA * j.a1 * j.a1 + iB * j.a2 * j.a2;
+        if (j.motorMass > 0.0f) j.motorMass = 1.0f / j.motorMass;
+        j.perp = mul(qA, j.localYAxisA);
+        j.s1 = cross(d + rA, j.perp);
+        j.s2 = cross(rB, j.perp);
+        const float k11 = mA + mB + iA * j.s1 * j.s1 + iB * j.s2 * j.s2;
+        const float k12 = iA * j.s1 + iB * j.s2;
+        const float k13 = iA * j.s1 * j.a1 + iB * j.s2 * j.a2;
+        float k22 = iA + iB;
+        if (k22 == 0.0f) k22 = 1.0f;
+        const float k23 = iA * j.a1 + iB * j.a2;
+        const float k33 = mA + mB + iA * j.a1 * j.a1 + iB * j.a2 * j.a2;
+        j.mass.ex = V3{k11, k12, k13};
+        j.mass.ey = V3{k12, k22, k23};
+        j.mass.ez = V3{k13, k23, k33};
+        if (j.enableLimit) {
+            const float jointTranslation = dot(j.axis, d);
+            if (absf(j.upperAngle - j.lowerAngle) < 2.0f * kLinearSlop) {
+                j.limitState = kEqualLimits;
+            } else if (jointTranslation <= j.lowerAngle) {
+                if (j.limitState != kAtLowerLimit) {
+                    j.limitState = kAtLowerLimit;
+                    j.impulse.z = 0.0f;
+                }
+            } else if (jointTranslation >= j.upperAngle) {
+                if (j.limitState != kAtUpperLimit) {
+                    j.limitState = kAtUpperLimit;
+                    j.impulse.z = 0.0f;
+                }
+            } else {
+                j.limitState = kInactiveLimit;
+                j.impulse.z = 0.0f;
+            }
+        } else {
+            j.limitState = kInactiveLimit;
+            j.impulse.z = 0.0f;
+        }
+        if (!j.enableMotor) j.motorImpulse = 0.0f;
+        if (warmStarting) {
+            j.impulse.x *= dtRatio; j.impulse.y *= dtRatio; j.impulse.z *= dtRatio;
+            j.motorImpulse *= dtRatio;
+            const V2 P = j.impulse.x * j.perp + (j.motorImpulse + j.impulse.z) * j.axis;
+            const float LA = j.impulse.x * j.s1 + j.impulse.y + (j.motorImpulse + j.impulse.z) * j.a1;
+            const float LB = j.impulse.x * j.s2 + j.impulse.y + (j.motorImpulse + j.impulse.z) * j.a2;
+            vA -= mA * P;
+            wA -= iA * LA;
+            vB += mB * P;
+            wB += iB * LB;
+        } else {
+            j.impulse = V3{0.0f, 0.0f, 0.0f};
+            j.motorImpulse = 0.0f;
+        }
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2PrismaticJoint::SolveVelocityConstraints, REF b2PrismaticJoint.cpp:268-363
+    void prismaticSolveVelocity(Joint& j, float h) {
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+        if (j.enableMotor && j.limitState != kEqualLimits) {
+            const float Cdot = dot(j.axis, vB - vA) + j.a2 * wB - j.a1 * wA;
+            float impulse = j.motorMass * (j.motorSpeed - Cdot);
+            const float oldImpulse = j.motorImpulse;
+            const float maxImpulse = h * j.maxMotorTorque;
+            j.motorImpulse = clampf(j.motorImpulse + impulse, -maxImpulse, maxImpulse);
+            impulse = j.motorImpulse - oldImpulse;
+            const V2 P = impulse * j.axis;
+            const float LA = impulse * j.a1, LB = impulse * j.a2;
+            vA -= mA * P;
+            wA -= iA * LA;
+            vB += mB * P;
+            wB += iB * LB;
+        }
+        V2 Cdot1;
+        Cdot1.x = dot(j.perp, vB - vA) + j.s2 * wB - j.s1 * wA;
+        Cdot1.y = wB - wA;
+        if (j.enableLimit && j.limitState != kInactiveLimit) {
+            const float Cdot2 = dot(j.axis, vB - vA) + j.a2 * wB - j.a1 * wA;
+            const V3 f1 = j.impulse;
+            const V3 df0 = solve33(j.mass, V3{-Cdot1.x, -Cdot1.y, -Cdot2});
+            j.impulse = V3{j.impulse.x + df0.x, j.impulse.y + df0.y, j.impulse.z + df0.z};
+            if (j.limitState == kAtLowerLimit) j.impulse.z = vmax(j.impulse.z, 0.0f);
+            else if (j.limitState == kAtUpperLimit) j.impulse.z = vmin(j.impulse.z, 0.0f);
+            const V2 b = -Cdot1 - (j.impulse.z - f1.z) * mk(j.mass.ez.x, j.mass.ez.y);
+            const V2 f2r = solve22(j.mass, b) + mk(f1.x, f1.y);
+            j.impulse.x = f2r.x;
+            j.impulse.y = f2r.y;
+            const V3 df = V3{j.impulse.x - f1.x, j.impulse.y - f1.y, j.impulse.z - f1.z};
+            const V2 P = df.x * j.perp + df.z * j.axis;
+            const float LA = df.x * j.s1 + df.y + df.z * j.a1;
+            const float LB = df.x * j.s2 + df.y + df.z * j.a2;
+            vA -= mA * P;
+            wA -= iA * LA;
+            vB += mB * P;
+            wB += iB * LB;
+        } else {
+            const V2 df = solve22(j.mass, -Cdot1);
+            j.impulse.x += df.x;
+            j.impulse.y += df.y;
+            const V2 P = df.x * j.perp;
+            const float LA = df.x * j.s1 + df.y;
+            const float LB = df.x * j.s2 + df.y;
+            vA -= mA * P;
+            wA -= iA * LA;
+            vB += mB * P;
+            wB += iB * LB;
+        }
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2PrismaticJoint::SolvePositionConstraints, REF b2PrismaticJoint.cpp:365-488
+    bool prismaticSolvePosition(Joint& j) {
+        V2 cA = posC[j.indexA]; float aA = posA[j.indexA];
+        V2 cB = posC[j.indexB]; float aB = posA[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+        const V2 rA = mul(qA, j.localAnchorA - j.localCenterA);
+        const V2 rB = mul(qB, j.localAnchorB - j.localCenterB);
+        const V2 d = cB + rB - cA - rA;
+        const V2 axis = mul(qA, j.localXAxisA);
+        const float a1 = cross(d + rA, axis), a2 = cross(rB, axis);
+        const V2 perp = mul(qA, j.localYAxisA);
+        const float s1 = cross(d + rA, perp), s2 = cross(rB, perp);
+        V3 impulse;
+        V2 C1;
+        C1.x = dot(perp, d);
+        C1.y = aB - aA - j.referenceAngle;
+        float linearError = absf(C1.x);
+        const float angularError = absf(C1.y);
+        bool active = false;
+        float C2 = 0.0f;
+        if (j.enableLimit) {
+            const float translation = dot(axis, d);
+            if (absf(j.upperAngle - j.lowerAngle) < 2.0f * kLinearSlop) {
+                C2 = clampf(translation, -kMaxLinearCorrection, kMaxLinearCorrection);
+                linearError = vmax(linearError, absf(translation));
+                active = true;
+            } else if (translation <= j.lowerAngle) {
+                C2 = clampf(translation - j.lowerAngle + kLinearSlop, -kMaxLinearCorrection, 0.0f);
+                linearError = vmax(linearError, j.lowerAngle - translation);
+                active = true;
+            } else if (translation >= j.upperAngle) {
+                C2 = clampf(translation - j.upperAngle - kLinearSlop, 0.0f, kMaxLinearCorrection);
+                linearError = vmax(linearError, translation - j.upperAngle);
+                active = true;
+            }
+        }
+        if (active) {
+            const float k11 = mA + mB + iA * s1 * s1 + iB * s2 * s2;
+            const float k12 = iA * s1 + iB * s2;
+            const float k13 = iA * s1 * a1 + iB * s2 * a2;
+            float k22 = iA + iB;
+            if (k22 == 0.0f) k22 = 1.0f;
+            const float k23 = iA * a1 + iB * a2;
+            const float k33 = mA + mB + iA * a1 * a1 + iB * a2 * a2;
+            Mat33 K;
+            K.ex = V3{k11, k12, k13};
+            K.ey = V3{k12, k22, k23};
+            K.ez = V3{k13, k23, k33};
+            impulse = solve33(K, V3{-C1.x, -C1.y, -C2});
+        } else {
+            const float k11 = mA + mB + iA * s1 * s1 + iB * s2 * s2;
+            const float k12 = iA * s1 + iB * s2;
+            float k22 = iA + iB;
+            if (k22 == 0.0f) k22 = 1.0f;
+            // b2Mat22::Solve, REF/Common/b2Math.h:242-254
+            float det = k11 * k22 - k12 * k12;
+            if (det != 0.0f) det = 1.0f / det;
+            const V2 b = -C1;
+            impulse = V3{det * (k22 * b.x - k12 * b.y), det * (k11 * b.y - k12 * b.x), 0.0f};
+        }
+        const V2 P = impulse.x * perp + impulse.z * axis;
+        const float LA = impulse.x * s1 + impulse.y + impulse.z * a1;
+        const float LB = impulse.x * s2 + impulse.y + impulse.z * a2;
+        cA -= mA * P;
+        aA -= iA * LA;
+        cB += mB * P;
+        aB += iB * LB;
+        posC[j.indexA] = cA; posA[j.indexA] = aA;
+        posC[j.indexB] = cB; posA[j.indexB] = aB;
+        return linearError <= kLinearSlop && angularError <= kAngularSlop;
+    }
+
+    // b2WheelJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2WheelJoint.cpp:79-216
+    void wheelInitVelocity(Joint& j, float h, float dtRatio) {
+        const V2 cA = posC[j.indexA], cB = posC[j.indexB];
+        const float aA = posA[j.indexA], aB = posA[j.indexB];
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+        const V2 rA = mul(qA, j.localAnchorA - j.localCenterA);
+        const V2 rB = mul(qB, j.localAnchorB - j.localCenterB);
+        const V2 d = cB + rB - cA - rA;
+        j.ay = mul(qA, j.localYAxisA);
+        j.sAy = cross(d + rA, j.ay);
+        j.sBy = cross(rB, j.ay);
+        j.wheelMass = mA + mB + iA * j.sAy * j.sAy + iB * j.sBy * j.sBy;
+        if (j.wheelMass > 0.0f) j.wheelMass = 1.0f / j.wheelMass;
+        j.springMass = 0.0f;
+        j.bias = 0.0f;
+        j.gamma = 0.0f;
+        if (j.frequencyHz > 0.0f) {
+            j.ax = mul(qA, j.localXAxisA);
+            j.sAx = cross(d + rA, j.ax);
+            j.sBx = cross(rB, j.ax);
+            const float invMass = mA + mB + iA * j.sAx * j.sAx + iB * j.sBx * j.sBx;
+            if (invMass > 0.0f) {
+                j.springMass = 1.0f / invMass;
+                const float C = dot(d, j.ax);
+                const float omega = 2.0f * kPi * j.frequencyHz;
+                const float dd = 2.0f * j.springMass * j.dampingRatio * omega;
+                const float k = j.springMass * omega * omega;
+                j.gamma = h * (dd + h * k);
+                if (j.gamma > 0.0f) j.gamma = 1.0f / j.gamma;
+                j.bias = C * h * k * j.gamma;
+                j.springMass = invMass + j.gamma;
+                if (j.springMass > 0.0f) j.springMass = 1.0f / j.springMass;
+            }
+        } else {
+            j.springImpulse = 0.0f;
+        }
+        if (j.enableMotor) {
+            j.motorMass = iA + iB;
+            if (j.motorMass > 0.0f) j.motorMass = 1.0f / j.motorMass;
+        } else {
+            j.motorMass = 0.0f;
+            j.motorImpulse = 0.0f;
+        }
+        if (warmStarting) {
+            j.impulse.x *= dtRatio;
+            j.springImpulse *= dtRatio;
+            j.motorImpulse *= dtRatio;
+            const V2 P = j.impulse.x * j.ay + j.springImpulse * j.ax;
+            const float LA = j.impulse.x * j.sAy + j.springImpulse * j.sAx + j.motorImpulse;
+            const float LB = j.impulse.x * j.sBy + j.springImpulse * j.sBx + j.motorImpulse;
+            vA -= j.invMassA * P;
+            wA -= j.invIA * LA;
+            vB += j.invMassB * P;
+            wB += j.invIB * LB;
+        } else {
+            j.impulse.x = 0.0f;
+            j.springImpulse = 0.0f;
+            j.motorImpulse = 0.0f;
+        }
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2WheelJoint::SolveVelocityConstraints, REF b2WheelJoint.cpp:218-280
+    void wheelSolveVelocity(Joint& j, float h) {
+        V2 vA = velV[j.indexA], vB = velV[j.indexB];
+        float wA = velW[j.indexA], wB = velW[j.indexB];
+        const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+        {
+            const float Cdot = dot(j.ax, vB - vA) + j.sBx * wB - j.sAx * wA;
+            const float impulse = -j.springMass * (Cdot + j.bias + j.gamma * j.springImpulse);
+            j.springImpulse += impulse;
+            const V2 P = impulse * j.ax;
+            const float LA = impulse * j.sAx, LB = impulse * j.sBx;
+            vA -= mA * P;
+            wA -= iA * LA;
+            vB += mB * P;
+            wB += iB * LB;
+        }
+        {
+            const float Cdot = wB - wA - j.motorSpeed;
+            float impulse = -j.motorMass * Cdot;
+            const float oldImpulse = j.motorImpulse;
+            const float maxImpulse = h * j.maxMotorTorque;
+            j.motorImpulse = clampf(j.motorImpulse + impulse, -maxImpulse, maxImpulse);
+            impulse = j.motorImpulse - oldImpulse;
+            wA -= iA * impulse;
+            wB += iB * impulse;
+        }
+        {
+            const float Cdot = dot(j.ay, vB - vA) + j.sBy * wB - j.sAy * wA;
+            const float impulse = -j.wheelMass * Cdot;
+            j.impulse.x += impulse;
+            const V2 P = impulse * j.ay;
+            const float LA = impulse * j.sAy, LB = impulse * j.sBy;
+            vA -= mA * P;
+            wA -= iA * LA;
+            vB += mB * P;
+            wB += iB * LB;
+        }
+        velV[j.indexA] = vA; velW[j.indexA] = wA;
+        velV[j.indexB] = vB; velW[j.indexB] = wB;
+    }
+
+    // b2WheelJoint::SolvePositionConstraints, REF b2WheelJoint.cpp:282-327
+    // (its effective mass uses the sAy, sBy of the velocity pass)
+    bool wheelSolvePosition(Joint& j) {
+        V2 cA = posC[j.indexA]; float aA = posA[j.indexA];
+        V2 cB = posC[j.indexB]; float aB = posA[j.indexB];
+        const Rot qA = stepRot(aA), qB = stepRot(aB);
+        const V2 rA = mul(qA, j.localAnchorA - j.localCenterA);
+        const V2 rB = mul(qB, j.localAnchorB - j.localCenterB);
+        const V2 d = (cB - cA) + rB - rA;
+        const V2 ay = mul(qA, j.localYAxisA);
+        const float sAy = cross(d + rA, ay), sBy = cross(rB, ay);
+        const float C = dot(d, ay);
+        const float k = j.invMassA + j.invMassB + j.invIA * j.sAy * j.sAy + j.invIB * j.sBy * j.sBy;
+        const float impulse = k != 0.0f ? -C / k : 0.0f;
+        const V2 P = impulse * ay;
+        const float LA = impulse * sAy, LB = impulse * sBy;
+        cA -= j.invMassA * P;
+        aA -= j.invIA * LA;
+        cB += j.invMassB * P;
+        aB += j.invIB * LB;
+        posC[j.indexA] = cA; posA[j.indexA] = aA;
+        posC[j.indexB] = cB; posA[j.indexB] = aB;
+        return absf(C) <= kLinearSlop;
+    }
+
     // The 3x3 K of b2WeldJoint (REF/Dynamics/Joints/b2WeldJoint.cpp:95-104)
     static Mat33 weldK(float mA, float iA, float mB, float iB, V2 rA, V2 rB) {
         Mat33 K;
@@ -2821,6 +3182,8 @@ struct World {
         if (j.type == 3) return weldSolvePosition(j);
         if (j.type == 4 || j.type == 5 || j.type == 7) return true;  // friction, motor, mouse: no position pass
         if (j.type == 6) return pulleySolvePosition(j);
+        if (j.type == 8) return prismaticSolvePosition(j);
+        if (j.type == 9) return wheelSolvePosition(j);
         V2 cA = posC[j.indexA]; float aA = posA[j.indexA];
         V2 cB = posC[j.indexB]; float aB = posA[j.indexB];
         float angularError = 0.0f, positionError = 0.0f;
@@ -3477,6 +3840,56 @@ int orc_world_destroy_joint(void* h, int joint) {
 
 // Per joint in creation order: impulse x, y, z and motor impulse.
 // b2DistanceJoint::SetLength / SetFrequency / SetDampingRatio.
+// b2PrismaticJoint::EnableLimit, SetLimits (dropping the limit impulse when
+// they change something), EnableMotor, SetMotorSpeed, SetMaxMotorForce, all
+// waking both bodies, REF/Dynamics/Joints/b2PrismaticJoint.cpp:546-609.
+int orc_world_set_prismatic(void* h, int joint, int enableLimit, float lower, float upper, int enableMotor,
+                            float speed, float maxForce) {
+    World* w = (World*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint].removed || w->joints[joint].type != 8)
+        return B2G_ERR_ARG;
+    Joint& j = w->joints[joint];
+    Body& bA = w->bodies[j.bodyA];
+    Body& bB = w->bodies[j.bodyB];
+    if ((enableLimit != 0) != j.enableLimit) {
+        w->setAwake(bA, true);
+        w->setAwake(bB, true);
+        j.enableLimit = enableLimit != 0;
+        j.impulse.z = 0.0f;
+    }
+    if (lower != j.lowerAngle || upper != j.upperAngle) {
+        w->setAwake(bA, true);
+        w->setAwake(bB, true);
+        j.lowerAngle = lower;
+        j.upperAngle = upper;
+        j.impulse.z = 0.0f;
+    }
+    w->setAwake(bA, true);
+    w->setAwake(bB, true);
+    j.enableMotor = enableMotor != 0;
+    j.motorSpeed = speed;
+    j.maxMotorTorque = maxForce;
+    return B2G_OK;
+}
+
+// b2WheelJoint::EnableMotor, SetMotorSpeed, SetMaxMotorTorque (waking both
+// bodies), SetSpringFrequencyHz, SetSpringDampingRatio,
+// REF/Dynamics/Joints/b2WheelJoint.cpp:377-401.
+int orc_world_set_wheel(void* h, int joint, int enableMotor, float speed, float maxTorque, float hz, float ratio) {
+    World* w = (World*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint].removed || w->joints[joint].type != 9)
+        return B2G_ERR_ARG;
+    Joint& j = w->joints[joint];
+    w->setAwake(w->bodies[j.bodyA], true);
+    w->setAwake(w->bodies[j.bodyB], true);
+    j.enableMotor = enableMotor != 0;
+    j.motorSpeed = speed;
+    j.maxMotorTorque = maxTorque;
+    j.frequencyHz = hz;
+    j.dampingRatio = ratio;
+    return B2G_OK;
+}
+
 // b2MouseJoint::SetTarget (waking body B), SetMaxForce, SetFrequency,
 // SetDampingRatio, REF/Dynamics/Joints/b2MouseJoint.cpp:55-97.
 int orc_world_set_mouse(void* h, int joint, float tx, float ty, float maxForce, float hz, float ratio) {
@@ -3564,6 +3977,22 @@ void orc_world_get_joints(void* h, float* out) {
         const Joint& j = w->joints[i];
         if (j.removed) {  // destroyed
             for (int k = 0; k < 4; ++k) out[4 * i + k] = 0.0f;
+            continue;
+        }
+        if (j.type == 8) {  // b2PrismaticJoint::GetReactionForce / Torque at inv_dt = 1
+            const V2 P = j.impulse.x * j.perp + (j.motorImpulse + j.impulse.z) * j.axis;
+            out[4 * i] = P.x;
+            out[4 * i + 1] = P.y;
+            out[4 * i + 2] = j.impulse.y;
+            out[4 * i + 3] = 0.0f;
+            continue;
+        }
+        if (j.type == 9) {  // b2WheelJoint::GetReactionForce / Torque at inv_dt = 1
+            const V2 P = j.impulse.x * j.ay + j.springImpulse * j.ax;
+            out[4 * i] = P.x;
+            out[4 * i + 1] = P.y;
+            out[4 * i + 2] = j.motorImpulse;
+            out[4 * i + 3] = 0.0f;
             continue;
         }
         if (j.type == 6) {  // b2PulleyJoint::GetReactionForce at inv_dt = 1: impulse along uB

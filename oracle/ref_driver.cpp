@@ -101,6 +101,40 @@ int loadJoints(RefWorld* w, int base, const b2scene::JointDef* jd, int nj) {
             w->joints.push_back(w->world->CreateJoint(&rd));
             continue;
         }
+        if (d.type == 8) {
+            b2PrismaticJointDef pd;
+            pd.bodyA = w->bodies[base + d.bodyA];
+            pd.bodyB = w->bodies[base + d.bodyB];
+            pd.collideConnected = d.collideConnected != 0;
+            pd.localAnchorA.Set(d.ax, d.ay);
+            pd.localAnchorB.Set(d.bx, d.by);
+            pd.localAxisA.Set(d.axisX, d.axisY);
+            pd.referenceAngle = d.referenceAngle;
+            pd.enableLimit = d.enableLimit != 0;
+            pd.lowerTranslation = d.lowerAngle;
+            pd.upperTranslation = d.upperAngle;
+            pd.enableMotor = d.enableMotor != 0;
+            pd.maxMotorForce = d.maxMotorTorque;
+            pd.motorSpeed = d.motorSpeed;
+            w->joints.push_back(w->world->CreateJoint(&pd));
+            continue;
+        }
+        if (d.type == 9) {
+            b2WheelJointDef wd;
+            wd.bodyA = w->bodies[base + d.bodyA];
+            wd.bodyB = w->bodies[base + d.bodyB];
+            wd.collideConnected = d.collideConnected != 0;
+            wd.localAnchorA.Set(d.ax, d.ay);
+            wd.localAnchorB.Set(d.bx, d.by);
+            wd.localAxisA.Set(d.axisX, d.axisY);
+            wd.enableMotor = d.enableMotor != 0;
+            wd.maxMotorTorque = d.maxMotorTorque;
+            wd.motorSpeed = d.motorSpeed;
+            wd.frequencyHz = d.frequencyHz;
+            wd.dampingRatio = d.dampingRatio;
+            w->joints.push_back(w->world->CreateJoint(&wd));
+            continue;
+        }
         if (d.type == 6) {
             b2PulleyJointDef pd;
             pd.bodyA = w->bodies[base + d.bodyA];
@@ -436,6 +470,33 @@ int ref_world_set_distance(void* h, int joint, float length, float hz, float rat
     return 0;
 }
 
+int ref_world_set_prismatic(void* h, int joint, int enableLimit, float lower, float upper, int enableMotor,
+                            float speed, float maxForce) {
+    RefWorld* w = (RefWorld*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
+    if (w->joints[joint]->GetType() != e_prismaticJoint) return -1;
+    b2PrismaticJoint* j = static_cast<b2PrismaticJoint*>(w->joints[joint]);
+    j->EnableLimit(enableLimit != 0);
+    j->SetLimits(lower, upper);
+    j->EnableMotor(enableMotor != 0);
+    j->SetMotorSpeed(speed);
+    j->SetMaxMotorForce(maxForce);
+    return 0;
+}
+
+int ref_world_set_wheel(void* h, int joint, int enableMotor, float speed, float maxTorque, float hz, float ratio) {
+    RefWorld* w = (RefWorld*)h;
+    if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
+    if (w->joints[joint]->GetType() != e_wheelJoint) return -1;
+    b2WheelJoint* j = static_cast<b2WheelJoint*>(w->joints[joint]);
+    j->EnableMotor(enableMotor != 0);
+    j->SetMotorSpeed(speed);
+    j->SetMaxMotorTorque(maxTorque);
+    j->SetSpringFrequencyHz(hz);
+    j->SetSpringDampingRatio(ratio);
+    return 0;
+}
+
 int ref_world_set_mouse(void* h, int joint, float tx, float ty, float maxForce, float hz, float ratio) {
     RefWorld* w = (RefWorld*)h;
     if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
@@ -508,7 +569,8 @@ void ref_world_get_joints(void* h, float* out) {
         }
         const b2JointType type = w->joints[i]->GetType();
         if (type == e_distanceJoint || type == e_ropeJoint || type == e_weldJoint || type == e_frictionJoint ||
-            type == e_motorJoint || type == e_pulleyJoint || type == e_mouseJoint) {
+            type == e_motorJoint || type == e_pulleyJoint || type == e_mouseJoint || type == e_prismaticJoint ||
+            type == e_wheelJoint) {
             const b2Vec2 f = w->joints[i]->GetReactionForce(1.0f);
             out[4 * i] = f.x;
             out[4 * i + 1] = f.y;

@@ -37,7 +37,7 @@ def test_struct_layouts_match_the_library():
     sizes = np.zeros(n, np.int32)
     lib.b2g_abi_sizes(E.ptr(sizes), n)
     shape, manifold, body, fixture, world, params, stats, profile, mlp, aabb, xf, revolute, schedule, sweep, distance, \
-        rope, weld, friction, motor, pulley, mouse = sizes[:21]
+        rope, weld, friction, motor, pulley, mouse, prismatic, wheel = sizes[:23]
     assert E.SHAPE.itemsize == shape
     assert E.MANIFOLD.itemsize == manifold
     assert E.STATS.itemsize == stats
@@ -58,4 +58,6 @@ def test_struct_layouts_match_the_library():
     assert motor == 36     # 2 body indices, 6 floats, 1 switch
     assert pulley == 56    # 2 body indices, 11 floats, 1 switch
     assert mouse == 44     # 2 body indices, 8 floats, 1 switch
+    assert prismatic == 64 # 2 body indices, 11 floats, 3 switches
+    assert wheel == 56     # 2 body indices, 10 floats, 2 switches
     assert params == E.STEP_PARAMS.itemsize

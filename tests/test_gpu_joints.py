@@ -305,6 +305,8 @@ def test_distance_rods_hold(gpu):
     ("motor", 12, 1, ("world_set_motor", 0, F(-8.0), F(6.0), F(2.0), F(30.0), F(8.0), F(0.4))),
     ("pulley", 8, 0, ("world_joint_count",)),
     ("mouse", 8, 0, ("world_set_mouse", 0, F(-9.0), F(5.0), F(1000.0), F(8.0), F(0.3))),
+    ("prismatic", 12, 0, ("world_set_prismatic", 2, 1, F(-0.5), F(0.5), 1, F(-1.0), F(50.0))),
+    ("wheels", 6, 1, ("world_set_wheel", 0, 1, F(5.0), F(40.0), F(2.0), F(0.2))),
 ])
 def test_rope_weld_friction_motor_joints_bitexact(gpu, orc, wide_trig, kind, n, seed, edit):
     """Rope, weld, friction and motor joints, one per island (or joints on
@@ -411,4 +413,23 @@ def test_pulleys_hold_after_the_joint_table_grows(gpu):
         g.step(15)
         assert lib.b2g_joint_drift(ctx, E.ptr(out)) == 0
         assert out[2] == 8 and out[1] < 0.01, out
+    g.close()
+
+
+def test_wheels_hold(gpu):
+    """Carts on two wheel joints each (two joints per island, solved by
+    colour) over bumps: every wheel stays within 1 cm of its axis line, as
+    b2g_joint_drift measures it, and the driven carts move forwards."""
+    lib = _b2g(gpu)
+    g = gpu.world("wheels", 6, 0, sleep=False)
+    ctx = g.device_context()
+    out = np.zeros(3, np.float64)
+    x0 = g.bodies()[:, E.PX].copy()
+    for _ in range(8):
+        g.step(30)
+        assert lib.b2g_joint_drift(ctx, E.ptr(out)) == 0
+        assert out[2] == 12 and out[1] < 0.01, out
+    b = g.bodies()
+    chassis = [1 + 3 * i for i in range(6)]
+    assert all(b[c, E.PX] - x0[c] > 2.0 for c in chassis[::2]), [b[c, E.PX] - x0[c] for c in chassis]
     g.close()

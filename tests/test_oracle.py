@@ -42,6 +42,9 @@ F = ctypes.c_float
     ("motor", 9, 1, 300, False, 4, 2),
     ("pulley", 8, 0, 300, True, 8, 3),         # pulley joints: ratios 1-2, anchors at centres and corners
     ("mouse", 8, 0, 300, True, 8, 3),          # mouse joints: boxes pulled up by soft springs
+    ("prismatic", 12, 0, 300, True, 8, 3),     # prismatic joints: free, limited (one equal limit), motorised
+    ("wheels", 6, 0, 400, True, 8, 3),         # wheel joints: carts on suspension springs over bumps, driven
+    ("wheels", 6, 1, 300, False, 4, 2),
 ])
 def test_world_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, vel, pos):
     a = ref.world(kind, n, seed, sleep=sleep)
@@ -812,4 +815,59 @@ def test_mouse_joint_hooke_closed_form(orc):
         sag = 10.0 / (2.0 * np.pi * hz) ** 2
         assert abs(anchor[0] - target[0]) < 2e-3 and abs(target[1] - anchor[1] - sag) < 2e-3, (i, anchor, target, sag)
         assert abs(j[i, 0]) < 0.02 and abs(j[i, 1] - 10.0) < 0.1, (i, j[i])
+    w.close()
+
+
+def test_prismatic_and_wheel_edits_bitexact(ref, orc):
+    """b2PrismaticJoint's EnableLimit, SetLimits, EnableMotor, SetMotorSpeed,
+    SetMaxMotorForce and b2WheelJoint's EnableMotor, SetMotorSpeed,
+    SetMaxMotorTorque, SetSpringFrequencyHz, SetSpringDampingRatio mid-run,
+    against the reference's setters, with the reaction forces and torques."""
+    for kind, n, edits in (
+            ("prismatic", 8, lambda w: (w.e.fn("world_set_prismatic")(w.h, 2, 1, F(-0.5), F(0.5), 1, F(-1.0), F(50.0)),
+                                        w.e.fn("world_set_prismatic")(w.h, 3, 0, F(-1.0), F(2.0), 0, F(0.0), F(0.0)))),
+            ("wheels", 4, lambda w: (w.e.fn("world_set_wheel")(w.h, 0, 1, F(5.0), F(40.0), F(2.0), F(0.2)),
+                                     w.e.fn("world_set_wheel")(w.h, 3, 1, F(-20.0), F(10.0), F(8.0), F(1.0))))):
+        a = ref.world(kind, n, 0, sleep=True)
+        b = orc.world(kind, n, 0, sleep=True)
+        a.step(60); b.step(60)
+        for w in (a, b):
+            assert edits(w) == (0, 0)
+        for s in range(240):
+            a.step(1); b.step(1)
+            assert np.array_equal(a.bodies().view(np.uint32), b.bodies().view(np.uint32)), f"{kind} step {s + 61}"
+        nj = a.e.fn("world_joint_count")(a.h)
+        ja, jb = np.zeros(4 * nj, np.float32), np.zeros(4 * nj, np.float32)
+        a.e.fn("world_get_joints")(a.h, E.ptr(ja))
+        b.e.fn("world_get_joints")(b.h, E.ptr(jb))
+        assert np.array_equal(ja.view(np.uint32), jb.view(np.uint32))
+        a.close(); b.close()
+
+
+def test_prismatic_closed_forms(orc):
+    """Prismatic joints of the prismatic scene (no contact with the ground,
+    which is body A): box 2 (axis at 30 degrees, free, started at 1 m/s
+    along x) slides as a frictionless incline, s_k = 0.866 k h - 5 h^2
+    k (k + 1) / 2 along the axis with no drift across it; box 3 (45 degrees,
+    limit [-1, 2]) stops at -1 m; box 1 (15 degrees, motor 2 m/s, 30 N >
+    2.6 N of gravity along the axis) runs at 2 m/s."""
+    w = orc.world("prismatic", 12, 0, sleep=False)
+    h = 1.0 / 60.0
+    p0 = w.bodies()[:, [E.PX, E.PY]].astype(np.float64)
+    axis = lambda i: np.array([np.cos(0.2618 * (i % 6)), np.sin(0.2618 * (i % 6))])
+    lowest = 0.0
+    for k in range(1, 181):
+        w.step(1)
+        p = w.bodies()[:, [E.PX, E.PY]].astype(np.float64)
+        d2 = p[3] - p0[3]
+        s2 = float(np.dot(d2, axis(2)))
+        if k <= 60:
+            want = np.cos(0.5236) * k * h - 5.0 * h * h * k * (k + 1) / 2
+            assert abs(s2 - want) < 1e-3, (k, s2, want)
+            ax2 = axis(2)
+            assert abs(float(ax2[0] * d2[1] - ax2[1] * d2[0])) < 1e-4
+        lowest = min(lowest, float(np.dot(p[4] - p0[4], axis(3))))
+    assert -1.0 - 0.01 < lowest < -0.99, lowest
+    v1 = w.bodies()[2, [E.VX, E.VY]].astype(np.float64)
+    assert abs(float(np.dot(v1, axis(1))) - 2.0) < 1e-3, v1
     w.close()
