@@ -107,7 +107,8 @@ enum : int {
     JF_LIMIT = 1, JF_MOTOR = 2, JF_COLLIDE = 4, JF_REMOVED = 8,
     JL_INACTIVE = 0, JL_LOWER = 1, JL_UPPER = 2, JL_EQUAL = 3,  // limit state in bits 8-9
     JT_REVOLUTE = 0, JT_DISTANCE = 1, JT_ROPE = 2, JT_WELD = 3,  // joint type in bits 12-15
-    JT_FRICTION = 4, JT_MOTOR = 5, JT_PULLEY = 6, JT_MOUSE = 7, JT_PRISMATIC = 8, JT_WHEEL = 9,
+    JT_FRICTION = 4, JT_MOTOR = 5, JT_PULLEY = 6, JT_MOUSE = 7, JT_PRISMATIC = 8, JT_WHEEL = 9, JT_GEAR = 10,
+    JF_GEAR_PRISM_A = 1 << 16, JF_GEAR_PRISM_B = 1 << 17,  // a gear's joints: prismatic when set
 };
 __host__ __device__ inline int jointType(int flags) { return (flags >> 12) & 15; }
 struct JointView {
@@ -2483,6 +2484,11 @@ __global__ void k_joint_reactions(JointView jv, int first, int count, float4* ou
                 fx = P.x; fy = P.y; t = imp.y;
                 break;
             }
+            case JT_GEAR: {
+                const V2 P = imp.x * mk(imp.y, imp.z);
+                fx = P.x; fy = P.y; t = imp.x * imp.w;
+                break;
+            }
             case JT_WHEEL: {
                 const float4 ex = jv.extra[2 * j], ey = jv.extra[2 * j + 1];
                 const V2 P = imp.x * mk(ey.x, ey.y) + imp.y * mk(ex.x, ex.y);
@@ -2493,6 +2499,178 @@ __global__ void k_joint_reactions(JointView jv, int first, int count, float4* ou
         }
     }
     out[k] = make_float4(fx, fy, t, 0.0f);
+}
+
+// Gear joint (type 10): bodies A, B in the joint row, C, D (the other bodies
+// of its two joints) as integer bits in axis.x / axis.y, the two joints'
+// kinds in flags (JF_GEAR_PRISM_A / _B set for prismatic, else revolute);
+// anchors = localAnchorA, B; extra[2j] = localAnchorC, D; extra[2j+1] =
+// localAxisC, D; limits = referenceAngleA, referenceAngleB, ratio, constant.
+// Per step: r = (JvAC, JvBD), K0 = (JwA, JwB, JwC, JwD), motorMass = mass;
+// impulse = (impulse, JvAC.x, JvAC.y, JwA) for the reaction.
+struct GearRows {
+    V2 JvAC, JvBD;
+    float JwA, JwB, JwC, JwD, mass, coordinateA, coordinateB;
+};
+
+// The Jacobian rows, effective mass and coordinates of a gear joint at the
+// given poses (b2GearJoint::InitVelocityConstraints and
+// SolvePositionConstraints, REF/Dynamics/Joints/b2GearJoint.cpp:147-393).
+__device__ inline GearRows gearRows(const WorldView& w, const JointView& jv, int j, int4 q, const float4* p,
+                                    bool coords) {
+    const float4 an = jv.anchors[j], lcd = jv.extra[2 * j], axes = jv.extra[2 * j + 1], par = jv.limits[j];
+    const int flags = jv.flags[j];
+    const float2 lcA = w.lcenter[q.x], lcB = w.lcenter[q.y], lcC = w.lcenter[q.z], lcDv = w.lcenter[q.w];
+    const float4 mA = w.mass[q.x], mB = w.mass[q.y], mC = w.mass[q.z], mD = w.mass[q.w];
+    const Rot qA = rotOf(p[0].z), qB = rotOf(p[1].z), qC = rotOf(p[2].z), qD = rotOf(p[3].z);
+    const float ratio = par.z;
+    GearRows g;
+    g.mass = 0.0f;
+    g.coordinateA = g.coordinateB = 0.0f;
+    if (!(flags & JF_GEAR_PRISM_A)) {
+        g.JvAC = mk(0.0f, 0.0f);
+        g.JwA = 1.0f;
+        g.JwC = 1.0f;
+        g.mass += mA.y + mC.y;
+        if (coords) g.coordinateA = p[0].z - p[2].z - par.x;
+    } else {
+        const V2 axisC = mk(axes.x, axes.y);
+        const V2 u = mul(qC, axisC);
+        const V2 rC = mul(qC, mk(lcd.x, lcd.y) - mk(lcC.x, lcC.y));
+        const V2 rA = mul(qA, mk(an.x, an.y) - mk(lcA.x, lcA.y));
+        g.JvAC = u;
+        g.JwC = cross(rC, u);
+        g.JwA = cross(rA, u);
+        g.mass += mC.x + mA.x + mC.y * g.JwC * g.JwC + mA.y * g.JwA * g.JwA;
+        if (coords) {
+            const V2 pC = mk(lcd.x, lcd.y) - mk(lcC.x, lcC.y);
+            const V2 pA = mulT(qC, rA + (mk(p[0].x, p[0].y) - mk(p[2].x, p[2].y)));
+            g.coordinateA = dot(pA - pC, axisC);
+        }
+    }
+    if (!(flags & JF_GEAR_PRISM_B)) {
+        g.JvBD = mk(0.0f, 0.0f);
+        g.JwB = ratio;
+        g.JwD = ratio;
+        g.mass += ratio * ratio * (mB.y + mD.y);
+        if (coords) g.coordinateB = p[1].z - p[3].z - par.y;
+    } else {
+        const V2 axisD = mk(axes.z, axes.w);
+        const V2 u = mul(qD, axisD);
+        const V2 rD = mul(qD, mk(lcd.z, lcd.w) - mk(lcDv.x, lcDv.y));
+        const V2 rB = mul(qB, mk(an.z, an.w) - mk(lcB.x, lcB.y));
+        g.JvBD = ratio * u;
+        g.JwD = ratio * cross(rD, u);
+        g.JwB = ratio * cross(rB, u);
+        g.mass += ratio * ratio * (mD.x + mB.x) + mD.y * g.JwD * g.JwD + mB.y * g.JwB * g.JwB;
+        if (coords) {
+            const V2 pD = mk(lcd.z, lcd.w) - mk(lcDv.x, lcDv.y);
+            const V2 pB = mulT(qD, rB + (mk(p[1].x, p[1].y) - mk(p[3].x, p[3].y)));
+            g.coordinateB = dot(pB - pD, axisD);
+        }
+    }
+    return g;
+}
+
+__device__ inline int4 gearBodies(const JointView& jv, int j, int2 b) {
+    const float4 ax = jv.axis[j];
+    return make_int4(b.x, b.y, __float_as_int(ax.x), __float_as_int(ax.y));
+}
+
+// Writes the four bodies' velocities in the reference's order A, B, C, D
+// (a body met twice keeps the last write, as b2GearJoint's does).
+__device__ inline void gearStoreVel(const WorldView& w, int4 q, const float4* v) {
+    const int ids[4] = {q.x, q.y, q.z, q.w};
+    for (int k = 0; k < 4; ++k)
+        if (w.btype[ids[k]] == BT_DYNAMIC) w.vel[ids[k]] = v[k];
+}
+
+// b2GearJoint::InitVelocityConstraints, REF b2GearJoint.cpp:147-240
+__device__ inline void gearInitOne(const WorldView& w, const JointView& jv, int j, int2 b, float dtRatio, int warm) {
+    const int4 q = gearBodies(jv, j, b);
+    const float4 p[4] = {w.pos[q.x], w.pos[q.y], w.pos[q.z], w.pos[q.w]};
+    const GearRows g = gearRows(w, jv, j, q, p, false);
+    const float mass = g.mass > 0.0f ? 1.0f / g.mass : 0.0f;
+    float4 v[4] = {w.vel[q.x], w.vel[q.y], w.vel[q.z], w.vel[q.w]};
+    float4 imp = jv.impulse[j];
+    if (warm) {
+        const float4 mA = w.mass[q.x], mB = w.mass[q.y], mC = w.mass[q.z], mD = w.mass[q.w];
+        const float im = imp.x;
+        V2 vA = mk(v[0].x, v[0].y), vB = mk(v[1].x, v[1].y), vC = mk(v[2].x, v[2].y), vD = mk(v[3].x, v[3].y);
+        vA += (mA.x * im) * g.JvAC;
+        v[0].z += mA.y * im * g.JwA;
+        vB += (mB.x * im) * g.JvBD;
+        v[1].z += mB.y * im * g.JwB;
+        vC -= (mC.x * im) * g.JvAC;
+        v[2].z -= mC.y * im * g.JwC;
+        vD -= (mD.x * im) * g.JvBD;
+        v[3].z -= mD.y * im * g.JwD;
+        v[0].x = vA.x; v[0].y = vA.y; v[1].x = vB.x; v[1].y = vB.y;
+        v[2].x = vC.x; v[2].y = vC.y; v[3].x = vD.x; v[3].y = vD.y;
+    } else {
+        imp.x = 0.0f;
+    }
+    jv.impulse[j] = make_float4(imp.x, g.JvAC.x, g.JvAC.y, g.JwA);
+    jv.r[j] = make_float4(g.JvAC.x, g.JvAC.y, g.JvBD.x, g.JvBD.y);
+    jv.K0[j] = make_float4(g.JwA, g.JwB, g.JwC, g.JwD);
+    jv.motorMass[j] = mass;
+    for (int k = 0; k < 4; ++k) v[k].w = 0.0f;
+    gearStoreVel(w, q, v);
+}
+
+// b2GearJoint::SolveVelocityConstraints, REF b2GearJoint.cpp:242-274
+__device__ inline void gearSolveVelocityOne(const WorldView& w, const JointView& jv, int j, int2 b) {
+    const int4 q = gearBodies(jv, j, b);
+    float4 v[4] = {w.vel[q.x], w.vel[q.y], w.vel[q.z], w.vel[q.w]};
+    const float4 r = jv.r[j], Jw = jv.K0[j];
+    const V2 JvAC = mk(r.x, r.y), JvBD = mk(r.z, r.w);
+    V2 vA = mk(v[0].x, v[0].y), vB = mk(v[1].x, v[1].y), vC = mk(v[2].x, v[2].y), vD = mk(v[3].x, v[3].y);
+    float Cdot = dot(JvAC, vA - vC) + dot(JvBD, vB - vD);
+    Cdot += (Jw.x * v[0].z - Jw.z * v[2].z) + (Jw.y * v[1].z - Jw.w * v[3].z);
+    const float impulse = -jv.motorMass[j] * Cdot;
+    float4 imp = jv.impulse[j];
+    imp.x += impulse;
+    const float4 mA = w.mass[q.x], mB = w.mass[q.y], mC = w.mass[q.z], mD = w.mass[q.w];
+    vA += (mA.x * impulse) * JvAC;
+    v[0].z += mA.y * impulse * Jw.x;
+    vB += (mB.x * impulse) * JvBD;
+    v[1].z += mB.y * impulse * Jw.y;
+    vC -= (mC.x * impulse) * JvAC;
+    v[2].z -= mC.y * impulse * Jw.z;
+    vD -= (mD.x * impulse) * JvBD;
+    v[3].z -= mD.y * impulse * Jw.w;
+    v[0].x = vA.x; v[0].y = vA.y; v[1].x = vB.x; v[1].y = vB.y;
+    v[2].x = vC.x; v[2].y = vC.y; v[3].x = vD.x; v[3].y = vD.y;
+    jv.impulse[j] = imp;
+    gearStoreVel(w, q, v);
+}
+
+// b2GearJoint::SolvePositionConstraints, REF b2GearJoint.cpp:276-393 (its
+// linear error stays 0, so it never holds back the island's early exit).
+__device__ inline bool gearSolvePositionOne(const WorldView& w, const JointView& jv, int j, int2 b) {
+    const int4 q = gearBodies(jv, j, b);
+    float4 p[4] = {w.pos[q.x], w.pos[q.y], w.pos[q.z], w.pos[q.w]};
+    const GearRows g = gearRows(w, jv, j, q, p, true);
+    const float4 par = jv.limits[j];
+    const float C = (g.coordinateA + par.z * g.coordinateB) - par.w;
+    float impulse = 0.0f;
+    if (g.mass > 0.0f) impulse = -C / g.mass;
+    const float4 mA = w.mass[q.x], mB = w.mass[q.y], mC = w.mass[q.z], mD = w.mass[q.w];
+    V2 cA = mk(p[0].x, p[0].y), cB = mk(p[1].x, p[1].y), cC = mk(p[2].x, p[2].y), cD = mk(p[3].x, p[3].y);
+    cA += mA.x * impulse * g.JvAC;
+    p[0].z += mA.y * impulse * g.JwA;
+    cB += mB.x * impulse * g.JvBD;
+    p[1].z += mB.y * impulse * g.JwB;
+    cC -= mC.x * impulse * g.JvAC;
+    p[2].z -= mC.y * impulse * g.JwC;
+    cD -= mD.x * impulse * g.JvBD;
+    p[3].z -= mD.y * impulse * g.JwD;
+    p[0].x = cA.x; p[0].y = cA.y; p[1].x = cB.x; p[1].y = cB.y;
+    p[2].x = cC.x; p[2].y = cC.y; p[3].x = cD.x; p[3].y = cD.y;
+    const int ids[4] = {q.x, q.y, q.z, q.w};
+    for (int k = 0; k < 4; ++k)
+        if (w.btype[ids[k]] == BT_DYNAMIC) w.pos[ids[k]] = p[k];
+    return true;
 }
 
 // b2RevoluteJoint::InitVelocityConstraints, REF/Dynamics/Joints/b2RevoluteJoint.cpp:66-198
@@ -2526,6 +2704,10 @@ __device__ inline void jointInitOne(const WorldView& w, const JointView& jv, int
     }
     if (type == JT_PRISMATIC) {
         prismaticInitOne(w, jv, j, b, dtRatio, warm);
+        return;
+    }
+    if (type == JT_GEAR) {
+        gearInitOne(w, jv, j, b, dtRatio, warm);
         return;
     }
     if (type == JT_WHEEL) {
@@ -2642,6 +2824,10 @@ __device__ inline void jointSolveVelocityOne(const WorldView& w, const JointView
         prismaticSolveVelocityOne(w, jv, j, b, h);
         return;
     }
+    if (type == JT_GEAR) {
+        gearSolveVelocityOne(w, jv, j, b);
+        return;
+    }
     if (type == JT_WHEEL) {
         wheelSolveVelocityOne(w, jv, j, b, h);
         return;
@@ -2746,6 +2932,7 @@ __device__ inline void jointSolvePositionOne(const WorldView& w, const JointView
                         : type == JT_PULLEY ? pulleySolvePositionOne(w, jv, j, b)
                         : type == JT_PRISMATIC ? prismaticSolvePositionOne(w, jv, j, b)
                         : type == JT_WHEEL  ? wheelSolvePositionOne(w, jv, j, b)
+                        : type == JT_GEAR   ? gearSolvePositionOne(w, jv, j, b)
                                             : weldSolvePositionOne(w, jv, j, b);
         if (!ok) {
             int* f = &w.islandMinSep[it * w.nBodies + root];
@@ -2864,6 +3051,13 @@ __global__ void k_joint_drift(WorldView w, JointView jv, float* out) {
     }
     if (type == JT_FRICTION || type == JT_MOTOR || type == JT_MOUSE) {  // no positional constraint
         out[j] = -1.0f;
+        return;
+    }
+    if (type == JT_GEAR) {  // |coordinateA + ratio coordinateB - constant|
+        const int4 q = gearBodies(jv, j, b);
+        const float4 p[4] = {w.pos[q.x], w.pos[q.y], w.pos[q.z], w.pos[q.w]};
+        const GearRows g = gearRows(w, jv, j, q, p, true);
+        out[j] = absB((g.coordinateA + jv.limits[j].z * g.coordinateB) - jv.limits[j].w);
         return;
     }
     if (type == JT_PRISMATIC || type == JT_WHEEL) {  // distance of anchor B from the line through anchor A
@@ -4501,6 +4695,7 @@ struct b2g_context {
     float4* jExtra = nullptr;  // persistent: two rows of step directions per joint
     int *jFlags = nullptr, *jFlagsSnap = nullptr, *jOrder = nullptr;
     std::vector<int2> hJointBodies;
+    std::vector<int2> hJointBodies2;   // a gear joint's bodies C, D (-1 otherwise)
     std::vector<int> hJointKeepApart;
     std::vector<int> hJointRemoved;
     std::vector<int> jointColourStart;  // colour c: slots [start[c], start[c + 1]) of jOrder
@@ -5033,6 +5228,14 @@ int growJoints(b2g_context* ctx, int need) {
 // or removed, so this runs on the host then). Colour 64 collects joints
 // whose bodies have no free colour left; it is solved one joint at a time.
 // Removed joints take no slot.
+// The bits of a float as an int, on the host (a gear joint keeps its bodies
+// C and D as such bits in its axis row).
+static int floatBits(float f) {
+    int i;
+    memcpy(&i, &f, sizeof(i));
+    return i;
+}
+
 int recolourJoints(b2g_context* ctx) {
     const int nj = ctx->nJoints, nb = ctx->nBodies;
     std::vector<int> btype(nb);
@@ -5047,13 +5250,17 @@ int recolourJoints(b2g_context* ctx) {
         colour[j] = -1;
         if (ctx->hJointRemoved[j]) continue;
         const int2 b = ctx->hJointBodies[j];
-        const bool dA = btype[b.x] == BT_DYNAMIC, dB = btype[b.y] == BT_DYNAMIC;
-        const unsigned long long u = (dA ? used[b.x] : 0ull) | (dB ? used[b.y] : 0ull);
+        const int2 b2 = j < (int)ctx->hJointBodies2.size() ? ctx->hJointBodies2[j] : make_int2(-1, -1);
+        // a gear joint also writes bodies C and D
+        const int ids[4] = {b.x, b.y, b2.x, b2.y};
+        unsigned long long u = 0ull;
+        for (int k = 0; k < 4; ++k)
+            if (ids[k] >= 0 && btype[ids[k]] == BT_DYNAMIC) u |= used[ids[k]];
         int c = kOverflowColour;
         if (~u != 0ull) {
             c = __builtin_ctzll(~u);
-            if (dA) used[b.x] |= 1ull << c;
-            if (dB) used[b.y] |= 1ull << c;
+            for (int k = 0; k < 4; ++k)
+                if (ids[k] >= 0 && btype[ids[k]] == BT_DYNAMIC) used[ids[k]] |= 1ull << c;
         }
         colour[j] = c;
         ++count[c];
@@ -7455,7 +7662,7 @@ int b2g_abi_sizes(int32_t* out, int cap) {
         (int32_t)sizeof(b2g_schedule),    (int32_t)sizeof(b2g_sweep),        (int32_t)sizeof(b2g_distance_def),
         (int32_t)sizeof(b2g_rope_def),    (int32_t)sizeof(b2g_weld_def),     (int32_t)sizeof(b2g_friction_def),
         (int32_t)sizeof(b2g_motor_def),   (int32_t)sizeof(b2g_pulley_def),   (int32_t)sizeof(b2g_mouse_def),
-        (int32_t)sizeof(b2g_prismatic_def), (int32_t)sizeof(b2g_wheel_def),
+        (int32_t)sizeof(b2g_prismatic_def), (int32_t)sizeof(b2g_wheel_def), (int32_t)sizeof(b2g_gear_def),
     };
     const int n = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (!out) return n;
@@ -7612,8 +7819,17 @@ int b2g_restore(b2g_context* ctx, const void* in, size_t size) {
     });
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     HIPCHK(ctx, e);
-    if (nj > 0)
+    ctx->hJointBodies2.assign(nj, make_int2(-1, -1));
+    if (nj > 0) {
         HIPCHK(ctx, hipMemcpy(ctx->hJointBodies.data(), ctx->jBodies, sizeof(int2) * nj, hipMemcpyDeviceToHost));
+        std::vector<int> fl(nj);
+        std::vector<float4> ax(nj);
+        HIPCHK(ctx, hipMemcpy(fl.data(), ctx->jFlags, sizeof(int) * nj, hipMemcpyDeviceToHost));
+        HIPCHK(ctx, hipMemcpy(ax.data(), ctx->jAxis, sizeof(float4) * nj, hipMemcpyDeviceToHost));
+        for (int j = 0; j < nj; ++j)
+            if (jointType(fl[j]) == JT_GEAR)
+                ctx->hJointBodies2[j] = make_int2(floatBits(ax[j].x), floatBits(ax[j].y));
+    }
     ctx->nBodies = nb;
     ctx->nFixtures = nf;
     ctx->nContacts = nc;
@@ -7852,6 +8068,9 @@ static int appendJointRows(b2g_context* ctx, const std::vector<JointRow>& rows, 
         speed[i] = d.speed;
         const bool collide = (d.flags & JF_COLLIDE) != 0;
         ctx->hJointBodies.push_back(b[i]);
+        ctx->hJointBodies2.push_back(jointType(d.flags) == JT_GEAR
+                                         ? make_int2(floatBits(d.axis.x), floatBits(d.axis.y))
+                                         : make_int2(-1, -1));
         ctx->hJointKeepApart.push_back(collide ? 0 : 1);
         ctx->hJointRemoved.push_back(0);
         keepApart = keepApart || !collide;
@@ -8104,6 +8323,48 @@ int b2g_set_wheel_joint(b2g_context* ctx, int joint, const b2g_wheel_def* d) {
     return setAxisJoint(ctx, joint, JT_WHEEL, "not a wheel joint",
                         make_float4(d->frequencyHz, d->dampingRatio, d->maxMotorTorque, 0.0f), 0, d->enableMotor,
                         d->motorSpeed);
+}
+
+int b2g_add_gear_joints(b2g_context* ctx, const b2g_gear_def* defs, int count, int* first) {
+    if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
+    std::vector<JointRow> rows;
+    for (int i = 0; i < count; ++i) {
+        const b2g_gear_def& d = defs[i];
+        if (d.bodyC < 0 || d.bodyD < 0 || d.bodyC >= ctx->nBodies || d.bodyD >= ctx->nBodies)
+            return fail(ctx, B2G_ERR_ARG, "gear bodies C, D out of range");
+        JointRow r{d.bodyA, d.bodyB,
+                   make_float4(d.localAnchorA[0], d.localAnchorA[1], d.localAnchorB[0], d.localAnchorB[1]),
+                   make_float4(d.referenceAngleA, d.referenceAngleB, d.ratio, d.constant),
+                   (JT_GEAR << 12) | (d.collideConnected ? JF_COLLIDE : 0) | (d.prismaticA ? JF_GEAR_PRISM_A : 0) |
+                       (d.prismaticB ? JF_GEAR_PRISM_B : 0)};
+        union { int i; float f; } c, dd;
+        c.i = d.bodyC;
+        dd.i = d.bodyD;
+        r.axis = make_float4(c.f, dd.f, 0.0f, 0.0f);
+        rows.push_back(r);
+    }
+    const int n0 = ctx->nJoints;
+    int rc = appendJointRows(ctx, rows, first);
+    if (rc != B2G_OK || count == 0) return rc;
+    std::vector<float4> extra(2 * (size_t)count);
+    for (int i = 0; i < count; ++i) {
+        const b2g_gear_def& d = defs[i];
+        extra[2 * i] = make_float4(d.localAnchorC[0], d.localAnchorC[1], d.localAnchorD[0], d.localAnchorD[1]);
+        extra[2 * i + 1] = make_float4(d.localAxisC[0], d.localAxisC[1], d.localAxisD[0], d.localAxisD[1]);
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->jExtra + 2 * (size_t)n0, extra.data(), sizeof(float4) * 2 * count,
+                               hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
+int b2g_set_gear_joint(b2g_context* ctx, int joint, float ratio) {
+    if (!ctx || joint < 0 || joint >= ctx->nJoints) return B2G_ERR_ARG;
+    float4 par;
+    HIPCHK(ctx, hipMemcpyAsync(&par, ctx->jLimits + joint, sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    par.z = ratio;  // b2GearJoint::SetRatio leaves the constant
+    return setJointParams(ctx, joint, JT_GEAR, "not a gear joint", par);
 }
 
 int b2g_get_joint_reactions(b2g_context* ctx, int first, int count, float* out) {

@@ -135,9 +135,25 @@ int loadScene(Runner* r, const b2scene::BodyDef* bodies, int nb, const b2scene::
 // Scene joints through b2World::CreateJoint (local anchors and reference
 // angle given directly; body indices relative to `base`).
 int loadJoints(Runner* r, int base, const b2scene::JointDef* jd, int nj) {
+    const int jbase = (int)r->joints.size();
     for (int k = 0; k < nj; ++k) {
         const b2scene::JointDef& d = jd[k];
         if (base + d.bodyA >= (int)r->bodies.size() || base + d.bodyB >= (int)r->bodies.size()) return B2G_ERR_ARG;
+        if (d.type == 10) {
+            if (jbase + d.joint1 >= (int)r->joints.size() || jbase + d.joint2 >= (int)r->joints.size())
+                return B2G_ERR_ARG;
+            b2GearJointDef gd;
+            gd.bodyA = r->bodies[base + d.bodyA];
+            gd.bodyB = r->bodies[base + d.bodyB];
+            gd.collideConnected = d.collideConnected != 0;
+            gd.joint1 = r->joints[jbase + d.joint1];
+            gd.joint2 = r->joints[jbase + d.joint2];
+            gd.ratio = d.ratio;
+            b2Joint* j = r->world.CreateJoint(&gd);
+            if (j == nullptr) return B2G_ERR_ARG;
+            r->joints.push_back(j);
+            continue;
+        }
         if (d.type == 1) {
             b2DistanceJointDef dd;
             dd.bodyA = r->bodies[base + d.bodyA];

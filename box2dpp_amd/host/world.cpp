@@ -718,6 +718,29 @@ void b2WheelJoint::SetSpringDampingRatio(float ratio) {
     MarkDirty();
 }
 
+b2GearJoint::b2GearJoint(const b2GearJointDef* def, b2World* world, int32_t index)
+    : b2Joint(def, world, index), m_joint1(def->joint1), m_joint2(def->joint2), m_ratio(def->ratio), m_def{} {}
+
+b2Vec2 b2GearJoint::GetAnchorA() const { return m_bodyA->GetWorldPoint(m_localAnchorA); }
+b2Vec2 b2GearJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localAnchorB); }
+// REF/Dynamics/Joints/b2GearJoint.cpp:406-416, from the device
+b2Vec2 b2GearJoint::GetReactionForce(float inv_dt) const {
+    if (m_dirty) return b2Vec2(0.0f, 0.0f);
+    const b2g_vec2 f = deviceReaction(m_world->m_ctx, m_index, nullptr);
+    return b2Vec2(inv_dt * f.x, inv_dt * f.y);
+}
+float b2GearJoint::GetReactionTorque(float inv_dt) const {
+    if (m_dirty) return 0.0f;
+    float t = 0.0f;
+    (void)deviceReaction(m_world->m_ctx, m_index, &t);
+    return inv_dt * t;
+}
+void b2GearJoint::SetRatio(float ratio) {
+    m_ratio = ratio;
+    m_def.ratio = ratio;
+    MarkDirty();
+}
+
 // b2PulleyJointDef::Initialize, REF/Dynamics/Joints/b2PulleyJoint.cpp:38-55
 void b2PulleyJointDef::Initialize(b2Body* bA, b2Body* bB, const b2Vec2& groundA, const b2Vec2& groundB,
                                   const b2Vec2& anchorA, const b2Vec2& anchorB, float r) {
@@ -928,9 +951,20 @@ b2Joint* b2World::CreateJoint(const b2JointDef* def) {
         def->type != b2JointType::ROPE_JOINT && def->type != b2JointType::WELD_JOINT &&
         def->type != b2JointType::FRICTION_JOINT && def->type != b2JointType::MOTOR_JOINT &&
         def->type != b2JointType::PULLEY_JOINT && def->type != b2JointType::MOUSE_JOINT &&
-        def->type != b2JointType::PRISMATIC_JOINT && def->type != b2JointType::WHEEL_JOINT) {
-        m_lastError = "gear joints do not run on the GPU path";
+        def->type != b2JointType::PRISMATIC_JOINT && def->type != b2JointType::WHEEL_JOINT &&
+        def->type != b2JointType::GEAR_JOINT) {
+        m_lastError = "unknown joint type";
         return nullptr;
+    }
+    if (def->type == b2JointType::GEAR_JOINT) {
+        const b2GearJointDef* g = static_cast<const b2GearJointDef*>(def);
+        for (const b2Joint* jj : {g->joint1, g->joint2}) {
+            if (jj == nullptr || jj->m_world != this ||
+                (jj->m_type != b2JointType::REVOLUTE_JOINT && jj->m_type != b2JointType::PRISMATIC_JOINT)) {
+                m_lastError = "a gear joint joins two revolute or prismatic joints of this world";
+                return nullptr;
+            }
+        }
     }
     if (def->bodyA == def->bodyB || def->bodyA->m_world != this || def->bodyB->m_world != this) return nullptr;
     if (def->bodyA->m_subWorld != def->bodyB->m_subWorld) {
@@ -957,6 +991,11 @@ b2Joint* b2World::CreateJoint(const b2JointDef* def) {
         j = new b2PrismaticJoint(static_cast<const b2PrismaticJointDef*>(def), this, index);
     else if (def->type == b2JointType::WHEEL_JOINT)
         j = new b2WheelJoint(static_cast<const b2WheelJointDef*>(def), this, index);
+    else if (def->type == b2JointType::GEAR_JOINT) {
+        b2GearJoint* g = new b2GearJoint(static_cast<const b2GearJointDef*>(def), this, index);
+        ResolveGear(g);
+        j = g;
+    }
     else
         j = new b2RevoluteJoint(static_cast<const b2RevoluteJointDef*>(def), this, (int32_t)m_joints.size());
     m_joints.push_back(j);
@@ -1172,6 +1211,62 @@ static b2g_wheel_def wheelDef(const b2WheelJoint* d, int32_t bodyA, int32_t body
     r.enableMotor = d->IsMotorEnabled() ? 1 : 0;
     r.collideConnected = d->GetCollideConnected() ? 1 : 0;
     return r;
+}
+
+// b2GearJoint's constructor, REF/Dynamics/Joints/b2GearJoint.cpp:45-128: the
+// four bodies, the two joints' anchors, axes and reference angles, and the
+// constant from the bodies' current transforms.
+void b2World::ResolveGear(b2GearJoint* g) {
+    b2g_gear_def& d = g->m_def;
+    d = b2g_gear_def{};
+    float coordinate[2];
+    for (int side = 0; side < 2; ++side) {
+        b2Joint* jj = side == 0 ? g->m_joint1 : g->m_joint2;
+        b2Body* other = jj->m_bodyA;   // body C or D
+        b2Body* body = jj->m_bodyB;    // body A or B
+        b2Vec2 anchorOther, anchorBody, axis(0.0f, 0.0f);
+        float ref;
+        const bool prismatic = jj->m_type == b2JointType::PRISMATIC_JOINT;
+        if (!prismatic) {
+            const b2RevoluteJoint* r = static_cast<const b2RevoluteJoint*>(jj);
+            anchorOther = r->m_localAnchorA;
+            anchorBody = r->m_localAnchorB;
+            ref = r->m_referenceAngle;
+            coordinate[side] = body->m_sweep.a - other->m_sweep.a - ref;
+        } else {
+            const b2PrismaticJoint* p = static_cast<const b2PrismaticJoint*>(jj);
+            anchorOther = p->m_localAnchorA;
+            anchorBody = p->m_localAnchorB;
+            ref = p->m_referenceAngle;
+            axis = p->m_localXAxisA;
+            const b2Transform& xfB = body->GetTransform();
+            const b2Transform& xfO = other->GetTransform();
+            const b2Vec2 pO = anchorOther;
+            const b2Vec2 pB = b2MulT(xfO.q, b2Mul(xfB.q, anchorBody) + (xfB.p - xfO.p));
+            const b2Vec2 dd = pB - pO;
+            coordinate[side] = dd[0] * axis[0] + dd[1] * axis[1];
+        }
+        if (side == 0) {
+            d.bodyA = body->m_index; d.bodyC = other->m_index; d.prismaticA = prismatic ? 1 : 0;
+            d.localAnchorA[0] = anchorBody[0]; d.localAnchorA[1] = anchorBody[1];
+            d.localAnchorC[0] = anchorOther[0]; d.localAnchorC[1] = anchorOther[1];
+            d.localAxisC[0] = axis[0]; d.localAxisC[1] = axis[1];
+            d.referenceAngleA = ref;
+            g->m_bodyA = body;
+            g->m_localAnchorA = anchorBody;
+        } else {
+            d.bodyB = body->m_index; d.bodyD = other->m_index; d.prismaticB = prismatic ? 1 : 0;
+            d.localAnchorB[0] = anchorBody[0]; d.localAnchorB[1] = anchorBody[1];
+            d.localAnchorD[0] = anchorOther[0]; d.localAnchorD[1] = anchorOther[1];
+            d.localAxisD[0] = axis[0]; d.localAxisD[1] = axis[1];
+            d.referenceAngleB = ref;
+            g->m_bodyB = body;
+            g->m_localAnchorB = anchorBody;
+        }
+    }
+    d.ratio = g->m_ratio;
+    d.constant = coordinate[0] + g->m_ratio * coordinate[1];
+    d.collideConnected = g->m_collideConnected ? 1 : 0;
 }
 
 static b2g_revolute_def revoluteDef(const b2RevoluteJoint* r, int32_t bodyA, int32_t bodyB) {
@@ -1409,6 +1504,16 @@ int32_t b2World::Upload() {
                     defs.push_back(prismaticDef(d, d->m_bodyA->m_index, d->m_bodyB->m_index));
                 }
                 rc = b2g_add_prismatic_joints(m_ctx, defs.data(), (int)defs.size(), &first);
+            } else if (type == b2JointType::GEAR_JOINT) {
+                std::vector<b2g_gear_def> defs;
+                for (int q = i; q < k; ++q) {
+                    const b2GearJoint* g = static_cast<const b2GearJoint*>(m_joints[q]);
+                    b2g_gear_def d = g->m_def;
+                    d.bodyA = g->m_joint1->m_bodyB->m_index; d.bodyC = g->m_joint1->m_bodyA->m_index;
+                    d.bodyB = g->m_joint2->m_bodyB->m_index; d.bodyD = g->m_joint2->m_bodyA->m_index;
+                    defs.push_back(d);
+                }
+                rc = b2g_add_gear_joints(m_ctx, defs.data(), (int)defs.size(), &first);
             } else if (type == b2JointType::WHEEL_JOINT) {
                 std::vector<b2g_wheel_def> defs;
                 for (int q = i; q < k; ++q) {
@@ -1469,6 +1574,8 @@ int32_t b2World::Upload() {
                 const b2PrismaticJoint* p = static_cast<const b2PrismaticJoint*>(j);
                 const b2g_prismatic_def d = prismaticDef(p, p->m_bodyA->m_index, p->m_bodyB->m_index);
                 rc = b2g_set_prismatic_joint(m_ctx, j->m_index, &d);
+            } else if (j->m_type == b2JointType::GEAR_JOINT) {
+                rc = b2g_set_gear_joint(m_ctx, j->m_index, static_cast<const b2GearJoint*>(j)->m_ratio);
             } else if (j->m_type == b2JointType::WHEEL_JOINT) {
                 const b2WheelJoint* p = static_cast<const b2WheelJoint*>(j);
                 const b2g_wheel_def d = wheelDef(p, p->m_bodyA->m_index, p->m_bodyB->m_index);

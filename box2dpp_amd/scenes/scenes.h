@@ -87,6 +87,9 @@ struct JointDef {
     // maxMotorTorque, motorSpeed) when 8; b2WheelJointDef (b2WheelJoint.h:30-72:
     // localAxisA, frequencyHz, dampingRatio, maxMotorTorque, motorSpeed) when 9
     float axisX, axisY;
+    // b2GearJointDef (b2GearJoint.h:24-42: joint1, joint2 as indices of this
+    // scene's joints, ratio) when 10
+    int joint1, joint2;
 };
 
 struct Scene {
@@ -799,6 +802,60 @@ inline Scene wheels(int n, bool single = false) {
     return s;
 }
 
+// Gear joints: n gear trains (as the reference testbed's gears): a disc of
+// radius 1 and one of radius 2 on revolute joints to the ground, geared at
+// ratio 1/2, and a rack (0.5 x 3 m box) on a vertical prismatic joint
+// (limits +-3 m) geared to the large disc at ratio -1/2; the rack's weight
+// drives the train. Odd trains brake the small disc with a motor.
+inline Scene gears(int n) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-400.0f, 0.0f, 400.0f, 0.0f));
+    for (int i = 0; i < n; ++i) {
+        const float x = 14.0f * (float)i - 7.0f * (float)n, y = 8.0f;
+        const int j0 = (int)s.joints.size();
+        const float cx[2] = {x, x + 3.5f}, rad[2] = {1.0f, 2.0f};
+        for (int k = 0; k < 2; ++k) {
+            addBody(s, defaultBody(2, cx[k], y));
+            FixtureDef f = defaultFixture(kCircle);
+            f.radius = rad[k];
+            f.density = 5.0f;
+            addFixture(s, f);
+            JointDef r{};
+            r.bodyA = 0; r.bodyB = (int)s.bodies.size() - 1;
+            r.ax = cx[k]; r.ay = y;
+            r.enableMotor = (k == 0 && (i & 1)) ? 1 : 0;
+            r.motorSpeed = 0.0f;
+            r.maxMotorTorque = 20.0f;
+            s.joints.push_back(r);
+        }
+        const int disc1 = (int)s.bodies.size() - 2, disc2 = disc1 + 1;
+        JointDef g{};
+        g.type = 10;
+        g.bodyA = disc1; g.bodyB = disc2;
+        g.joint1 = j0; g.joint2 = j0 + 1;
+        g.ratio = 1.0f / 2.0f;
+        s.joints.push_back(g);
+        addBody(s, defaultBody(2, x + 6.0f, y));
+        addFixture(s, boxFixture(0.5f, 3.0f, 1.0f, 0.4f));
+        JointDef p{};
+        p.type = 8;
+        p.bodyA = 0; p.bodyB = (int)s.bodies.size() - 1;
+        p.ax = x + 6.0f; p.ay = y;
+        p.axisX = 0.0f; p.axisY = 1.0f;
+        p.enableLimit = 1;
+        p.lowerAngle = -3.0f; p.upperAngle = 3.0f;
+        s.joints.push_back(p);
+        JointDef g2{};
+        g2.type = 10;
+        g2.bodyA = disc2; g2.bodyB = p.bodyB;
+        g2.joint1 = j0 + 1; g2.joint2 = (int)s.joints.size() - 1;
+        g2.ratio = -1.0f / 2.0f;
+        s.joints.push_back(g2);
+    }
+    return s;
+}
+
 // Scene by name: "hello", "pyramid" (n = rows, 0 -> 20), "pile" (n bodies,
 // seed), "world" (one batched-scene world of n bodies, seed), "pendulums"
 // (n pendulums), "chains" (n chains of `seed` links), "stress" (pile of n
@@ -813,7 +870,7 @@ inline Scene wheels(int n, bool single = false) {
 // boxes on motor joints; seed 0 drives every third from the previous box),
 // "pulley" (n pulleys), "mouse" (n boxes on mouse joints), "prismatic" (n
 // boxes on prismatic joints), "wheels" (n carts; seed 1 gives each one
-// wheel on flat ground). Unknown names give an empty scene.
+// wheel on flat ground), "gears" (n gear trains). Unknown names give an empty scene.
 inline Scene byName(const char* kind, int n, uint32_t seed) {
     const std::string k(kind ? kind : "");
     if (k == "hello") return hello();
@@ -837,6 +894,7 @@ inline Scene byName(const char* kind, int n, uint32_t seed) {
     if (k == "mouse") return mouseJoints(n > 0 ? n : 8);
     if (k == "prismatic") return prismatics(n > 0 ? n : 12);
     if (k == "wheels") return wheels(n > 0 ? n : 6, seed == 1);
+    if (k == "gears") return gears(n > 0 ? n : 4);
     return Scene();
 }
 

@@ -273,7 +273,8 @@ int b2g_penetration(b2g_context* ctx, double* out);
  * Box2D/Dynamics/Joints/b2RevoluteJoint.cpp:336-343), the error in length
  * for rigid distance joints, the stretch beyond the maximum length for rope
  * joints, |lengthA + ratio lengthB - constant| for pulley joints, the
- * distance of anchor B from the axis line for prismatic and wheel joints; soft
+ * distance of anchor B from the axis line for prismatic and wheel joints,
+ * |coordinateA + ratio coordinateB - constant| for gear joints; soft
  * distance, friction, motor and mouse joints are not measured. out[3], FP64 sums on the
  * host; no counterpart in the reference. */
 int b2g_joint_drift(b2g_context* ctx, double* out);
@@ -486,6 +487,34 @@ int b2g_add_wheel_joints(b2g_context* ctx, const b2g_wheel_def* defs, int count,
 /* b2WheelJoint::EnableMotor, SetMotorSpeed, SetMaxMotorTorque,
  * SetSpringFrequencyHz, SetSpringDampingRatio. */
 int b2g_set_wheel_joint(b2g_context* ctx, int joint, const b2g_wheel_def* def);
+
+/* b2GearJointDef (Box2D/Dynamics/Joints/b2GearJoint.h:24-42) resolved the
+ * way the b2GearJoint constructor resolves it
+ * (REF/Dynamics/Joints/b2GearJoint.cpp:45-128): joint1 joins bodies C and A
+ * (its bodyA and bodyB), joint2 bodies D and B; each is revolute (prismatic
+ * flag 0, coordinate = angle difference) or prismatic (coordinate along
+ * localAxis); constant = coordinateA + ratio * coordinateB at creation.
+ * 92 bytes. */
+typedef struct b2g_gear_def {
+    int32_t bodyA, bodyB, bodyC, bodyD;
+    int32_t prismaticA, prismaticB;
+    float localAnchorA[2];
+    float localAnchorB[2];
+    float localAnchorC[2];
+    float localAnchorD[2];
+    float localAxisC[2];
+    float localAxisD[2];
+    float referenceAngleA, referenceAngleB;
+    float ratio, constant;
+    int32_t collideConnected;
+} b2g_gear_def;
+/* Appends gear joints (b2GearJoint, REF/Dynamics/Joints/b2GearJoint.cpp:147-393);
+ * a gear is coloured with all four of its bodies; b2g_get_joints reports
+ * (impulse, JvAC.x, JvAC.y, JwA), b2g_get_joint_reactions the force and
+ * torque. */
+int b2g_add_gear_joints(b2g_context* ctx, const b2g_gear_def* defs, int count, int* first);
+/* b2GearJoint::SetRatio (the constant keeps its creation value). */
+int b2g_set_gear_joint(b2g_context* ctx, int joint, float ratio);
 
 /* b2Joint::GetReactionForce and GetReactionTorque at inv_dt = 1 for joints
  * [first, first + count), every type: force x, y, torque, 0 (4 floats each);
@@ -702,7 +731,7 @@ int b2g_get_graph_counts(b2g_context* ctx, int64_t* launches, int64_t* captures)
  * b2g_step_params, b2g_step_stats, b2g_phase_profile, b2g_mlp_params,
  * b2g_aabb, b2g_transform, b2g_revolute_def, b2g_schedule, b2g_sweep,
  * b2g_distance_def, b2g_rope_def, b2g_weld_def, b2g_friction_def, b2g_motor_def,
- * b2g_pulley_def, b2g_mouse_def, b2g_prismatic_def, b2g_wheel_def.
+ * b2g_pulley_def, b2g_mouse_def, b2g_prismatic_def, b2g_wheel_def, b2g_gear_def.
  * Returns the count (out may be null). Needs no device; lets bindings check
  * their struct layouts. */
 int b2g_abi_sizes(int32_t* out, int cap);

@@ -20,6 +20,7 @@
 #include <memory>
 #include "b2Math.h"
 #include "b2Collision.h"
+#include "../b2g.h"
 
 struct b2g_context;
 
@@ -745,6 +746,37 @@ private:
     float m_maxMotorTorque, m_motorSpeed, m_frequencyHz, m_dampingRatio;
 };
 
+// Box2D/Dynamics/Joints/b2GearJoint.h:24-42
+struct b2GearJointDef : public b2JointDef {
+    b2GearJointDef() { type = b2JointType::GEAR_JOINT; }
+    b2Joint* joint1 = nullptr;   // a revolute or prismatic joint
+    b2Joint* joint2 = nullptr;   // a revolute or prismatic joint
+    float ratio = 1.0f;
+};
+
+// Box2D/Dynamics/Joints/b2GearJoint.h:55-120: coordinate1 + ratio *
+// coordinate2 = constant, over the four bodies of its two joints.
+class b2GearJoint : public b2Joint {
+public:
+    b2Vec2 GetAnchorA() const override;
+    b2Vec2 GetAnchorB() const override;
+    b2Joint* GetJoint1() { return m_joint1; }
+    b2Joint* GetJoint2() { return m_joint2; }
+    void SetRatio(float ratio);
+    float GetRatio() const { return m_ratio; }
+    b2Vec2 GetReactionForce(float inv_dt) const override;
+    float GetReactionTorque(float inv_dt) const override;
+
+private:
+    friend class b2World;
+    b2GearJoint(const b2GearJointDef* def, b2World* world, int32_t index);
+    b2Joint* m_joint1;
+    b2Joint* m_joint2;
+    float m_ratio;
+    b2Vec2 m_localAnchorA, m_localAnchorB;
+    b2g_gear_def m_def;   // resolved at creation, as b2GearJoint's constructor does
+};
+
 // Per-step solver options of the GPU path (no reference counterpart).
 struct b2SolverOptions {
     int32_t mode = 0;                 // 0 coloured, 1 reference order
@@ -833,9 +865,11 @@ private:
     friend class b2MouseJoint;
     friend class b2PrismaticJoint;
     friend class b2WheelJoint;
+    friend class b2GearJoint;
     int32_t Upload();
     void Download() const;
     void DownloadJoints() const;
+    void ResolveGear(b2GearJoint* g);
     int32_t CollideWithListener(const void* params);
     // PostSolve after the Step; Begin/End too unless the split step
     // already delivered them

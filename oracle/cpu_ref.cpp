@@ -1351,6 +1351,11 @@ struct Joint {
     float s1, s2, a1, a2;
     V2 ax, ay;
     float sAx, sBx, sAy, sBy, wheelMass, springMass, springImpulse;
+    // b2GearJoint (type 10; bodyA / bodyB, ratio and constant above),
+    // REF/Dynamics/Joints/b2GearJoint.h:80-120
+    int bodyC, bodyD, typeA, typeB;
+    V2 localAnchorC, localAnchorD, localAxisC, localAxisD, JvAC, JvBD;
+    float referenceAngleA, referenceAngleB, JwA, JwB, JwC, JwD, gearMass;
     // solver temporaries
     int indexA, indexB;
     V2 rA, rB, localCenterA, localCenterB;
@@ -1519,6 +1524,205 @@ struct World {
         return true;
     }
 
+    int jointBase = 0;  // index of the loading scene's first joint
+
+    // b2GearJoint ctor, REF/Dynamics/Joints/b2GearJoint.cpp:45-128: the
+    // coordinates of its two joints at creation fix the constant.
+    void gearCreate(Joint& j, const b2scene::JointDef& d) {
+        const Joint& j1 = joints[jointBase + d.joint1];
+        const Joint& j2 = joints[jointBase + d.joint2];
+        j.typeA = j1.type;
+        j.typeB = j2.type;
+        j.bodyC = j1.bodyA;
+        j.bodyA = j1.bodyB;
+        const Body& bA = bodies[j.bodyA];
+        const Body& bC = bodies[j.bodyC];
+        float coordinateA, coordinateB;
+        j.localAnchorC = j1.localAnchorA;
+        j.localAnchorA = j1.localAnchorB;
+        j.referenceAngleA = j1.referenceAngle;
+        if (j.typeA == 0) {
+            j.localAxisC = mk(0.0f, 0.0f);
+            coordinateA = bA.a - bC.a - j.referenceAngleA;
+        } else {
+            j.localAxisC = j1.localXAxisA;
+            const V2 pC = j.localAnchorC;
+            const V2 pA = mulT(bC.xf.q, mul(bA.xf.q, j.localAnchorA) + (bA.xf.p - bC.xf.p));
+            coordinateA = dot(pA - pC, j.localAxisC);
+        }
+        j.bodyD = j2.bodyA;
+        j.bodyB = j2.bodyB;
+        const Body& bB = bodies[j.bodyB];
+        const Body& bD = bodies[j.bodyD];
+        j.localAnchorD = j2.localAnchorA;
+        j.localAnchorB = j2.localAnchorB;
+        j.referenceAngleB = j2.referenceAngle;
+        if (j.typeB == 0) {
+            j.localAxisD = mk(0.0f, 0.0f);
+            coordinateB = bB.a - bD.a - j.referenceAngleB;
+        } else {
+            j.localAxisD = j2.localXAxisA;
+            const V2 pD = j.localAnchorD;
+            const V2 pB = mulT(bD.xf.q, mul(bB.xf.q, j.localAnchorB) + (bB.xf.p - bD.xf.p));
+            coordinateB = dot(pB - pD, j.localAxisD);
+        }
+        j.ratio = d.ratio;
+        j.constant = coordinateA + j.ratio * coordinateB;
+        j.impulse.x = 0.0f;
+        j.JvAC = j.JvBD = mk(0.0f, 0.0f);
+        j.JwA = j.JwB = j.JwC = j.JwD = j.gearMass = 0.0f;
+    }
+
+    // b2GearJoint::InitVelocityConstraints, REF b2GearJoint.cpp:147-240
+    void gearInitVelocity(Joint& j) {
+        const Body &bA = bodies[j.bodyA], &bB = bodies[j.bodyB], &bC = bodies[j.bodyC], &bD = bodies[j.bodyD];
+        const int iA = bA.islandIndex, iB = bB.islandIndex, iC = bC.islandIndex, iD = bD.islandIndex;
+        const float mA = bA.invMass, mB = bB.invMass, mC = bC.invMass, mD = bD.invMass;
+        const float IA = bA.invI, IB = bB.invI, IC = bC.invI, ID = bD.invI;
+        const float aA = posA[iA], aB = posA[iB], aC = posA[iC], aD = posA[iD];
+        V2 vA = velV[iA], vB = velV[iB], vC = velV[iC], vD = velV[iD];
+        float wA = velW[iA], wB = velW[iB], wC = velW[iC], wD = velW[iD];
+        const Rot qA = stepRot(aA), qB = stepRot(aB), qC = stepRot(aC), qD = stepRot(aD);
+        j.gearMass = 0.0f;
+        if (j.typeA == 0) {
+            j.JvAC = mk(0.0f, 0.0f);
+            j.JwA = 1.0f;
+            j.JwC = 1.0f;
+            j.gearMass += IA + IC;
+        } else {
+            const V2 u = mul(qC, j.localAxisC);
+            const V2 rC = mul(qC, j.localAnchorC - bC.localCenter);
+            const V2 rA = mul(qA, j.localAnchorA - bA.localCenter);
+            j.JvAC = u;
+            j.JwC = cross(rC, u);
+            j.JwA = cross(rA, u);
+            j.gearMass += mC + mA + IC * j.JwC * j.JwC + IA * j.JwA * j.JwA;
+        }
+        if (j.typeB == 0) {
+            j.JvBD = mk(0.0f, 0.0f);
+            j.JwB = j.ratio;
+            j.JwD = j.ratio;
+            j.gearMass += j.ratio * j.ratio * (IB + ID);
+        } else {
+            const V2 u = mul(qD, j.localAxisD);
+            const V2 rD = mul(qD, j.localAnchorD - bD.localCenter);
+            const V2 rB = mul(qB, j.localAnchorB - bB.localCenter);
+            j.JvBD = j.ratio * u;
+            j.JwD = j.ratio * cross(rD, u);
+            j.JwB = j.ratio * cross(rB, u);
+            j.gearMass += j.ratio * j.ratio * (mD + mB) + ID * j.JwD * j.JwD + IB * j.JwB * j.JwB;
+        }
+        j.gearMass = j.gearMass > 0.0f ? 1.0f / j.gearMass : 0.0f;
+        if (warmStarting) {
+            const float im = j.impulse.x;
+            vA += (mA * im) * j.JvAC;
+            wA += IA * im * j.JwA;
+            vB += (mB * im) * j.JvBD;
+            wB += IB * im * j.JwB;
+            vC -= (mC * im) * j.JvAC;
+            wC -= IC * im * j.JwC;
+            vD -= (mD * im) * j.JvBD;
+            wD -= ID * im * j.JwD;
+        } else {
+            j.impulse.x = 0.0f;
+        }
+        velV[iA] = vA; velW[iA] = wA;
+        velV[iB] = vB; velW[iB] = wB;
+        velV[iC] = vC; velW[iC] = wC;
+        velV[iD] = vD; velW[iD] = wD;
+    }
+
+    // b2GearJoint::SolveVelocityConstraints, REF b2GearJoint.cpp:242-274
+    void gearSolveVelocity(Joint& j) {
+        const Body &bA = bodies[j.bodyA], &bB = bodies[j.bodyB], &bC = bodies[j.bodyC], &bD = bodies[j.bodyD];
+        const int iA = bA.islandIndex, iB = bB.islandIndex, iC = bC.islandIndex, iD = bD.islandIndex;
+        V2 vA = velV[iA], vB = velV[iB], vC = velV[iC], vD = velV[iD];
+        float wA = velW[iA], wB = velW[iB], wC = velW[iC], wD = velW[iD];
+        float Cdot = dot(j.JvAC, vA - vC) + dot(j.JvBD, vB - vD);
+        Cdot += (j.JwA * wA - j.JwC * wC) + (j.JwB * wB - j.JwD * wD);
+        const float impulse = -j.gearMass * Cdot;
+        j.impulse.x += impulse;
+        vA += (bA.invMass * impulse) * j.JvAC;
+        wA += bA.invI * impulse * j.JwA;
+        vB += (bB.invMass * impulse) * j.JvBD;
+        wB += bB.invI * impulse * j.JwB;
+        vC -= (bC.invMass * impulse) * j.JvAC;
+        wC -= bC.invI * impulse * j.JwC;
+        vD -= (bD.invMass * impulse) * j.JvBD;
+        wD -= bD.invI * impulse * j.JwD;
+        velV[iA] = vA; velW[iA] = wA;
+        velV[iB] = vB; velW[iB] = wB;
+        velV[iC] = vC; velW[iC] = wC;
+        velV[iD] = vD; velW[iD] = wD;
+    }
+
+    // b2GearJoint::SolvePositionConstraints, REF b2GearJoint.cpp:276-393
+    bool gearSolvePosition(Joint& j) {
+        const Body &bA = bodies[j.bodyA], &bB = bodies[j.bodyB], &bC = bodies[j.bodyC], &bD = bodies[j.bodyD];
+        const int iA = bA.islandIndex, iB = bB.islandIndex, iC = bC.islandIndex, iD = bD.islandIndex;
+        V2 cA = posC[iA], cB = posC[iB], cC = posC[iC], cD = posC[iD];
+        float aA = posA[iA], aB = posA[iB], aC = posA[iC], aD = posA[iD];
+        const Rot qA = stepRot(aA), qB = stepRot(aB), qC = stepRot(aC), qD = stepRot(aD);
+        const float mA = bA.invMass, mB = bB.invMass, mC = bC.invMass, mD = bD.invMass;
+        const float IA = bA.invI, IB = bB.invI, IC = bC.invI, ID = bD.invI;
+        float coordinateA, coordinateB;
+        V2 JvAC, JvBD;
+        float JwA, JwB, JwC, JwD;
+        float mass = 0.0f;
+        if (j.typeA == 0) {
+            JvAC = mk(0.0f, 0.0f);
+            JwA = 1.0f;
+            JwC = 1.0f;
+            mass += IA + IC;
+            coordinateA = aA - aC - j.referenceAngleA;
+        } else {
+            const V2 u = mul(qC, j.localAxisC);
+            const V2 rC = mul(qC, j.localAnchorC - bC.localCenter);
+            const V2 rA = mul(qA, j.localAnchorA - bA.localCenter);
+            JvAC = u;
+            JwC = cross(rC, u);
+            JwA = cross(rA, u);
+            mass += mC + mA + IC * JwC * JwC + IA * JwA * JwA;
+            const V2 pC = j.localAnchorC - bC.localCenter;
+            const V2 pA = mulT(qC, rA + (cA - cC));
+            coordinateA = dot(pA - pC, j.localAxisC);
+        }
+        if (j.typeB == 0) {
+            JvBD = mk(0.0f, 0.0f);
+            JwB = j.ratio;
+            JwD = j.ratio;
+            mass += j.ratio * j.ratio * (IB + ID);
+            coordinateB = aB - aD - j.referenceAngleB;
+        } else {
+            const V2 u = mul(qD, j.localAxisD);
+            const V2 rD = mul(qD, j.localAnchorD - bD.localCenter);
+            const V2 rB = mul(qB, j.localAnchorB - bB.localCenter);
+            JvBD = j.ratio * u;
+            JwD = j.ratio * cross(rD, u);
+            JwB = j.ratio * cross(rB, u);
+            mass += j.ratio * j.ratio * (mD + mB) + ID * JwD * JwD + IB * JwB * JwB;
+            const V2 pD = j.localAnchorD - bD.localCenter;
+            const V2 pB = mulT(qD, rB + (cB - cD));
+            coordinateB = dot(pB - pD, j.localAxisD);
+        }
+        const float C = (coordinateA + j.ratio * coordinateB) - j.constant;
+        float impulse = 0.0f;
+        if (mass > 0.0f) impulse = -C / mass;
+        cA += mA * impulse * JvAC;
+        aA += IA * impulse * JwA;
+        cB += mB * impulse * JvBD;
+        aB += IB * impulse * JwB;
+        cC -= mC * impulse * JvAC;
+        aC -= IC * impulse * JwC;
+        cD -= mD * impulse * JvBD;
+        aD -= ID * impulse * JwD;
+        posC[iA] = cA; posA[iA] = aA;
+        posC[iB] = cB; posA[iB] = aB;
+        posC[iC] = cC; posA[iC] = aC;
+        posC[iD] = cD; posA[iD] = aD;
+        return true;
+    }
+
     // b2World::CreateJoint + b2RevoluteJoint ctor,
     // REF/Dynamics/b2World.cpp:212-268, REF/Dynamics/Joints/b2RevoluteJoint.cpp:47-64
     int createJoint(const b2scene::JointDef& d) {
@@ -1554,6 +1758,9 @@ struct World {
         j.localXAxisA = mk(d.axisX, d.axisY);
         if (d.type == 8) normalize(j.localXAxisA);  // b2PrismaticJoint ctor, REF b2PrismaticJoint.cpp:107-109
         j.localYAxisA = cross(1.0f, j.localXAxisA);
+        j.bodyC = j.bodyD = -1;
+        j.typeA = j.typeB = 0;
+        if (d.type == 10) gearCreate(j, d);
         if (d.type == 7) {
             // b2MouseJoint ctor (REF/Dynamics/Joints/b2MouseJoint.cpp:34-53) with
             // the target on the grab point, then SetTarget (:55-62)
@@ -2167,6 +2374,10 @@ struct World {
             prismaticInitVelocity(j, dtRatio);
             return;
         }
+        if (j.type == 10) {
+            gearInitVelocity(j);
+            return;
+        }
         if (j.type == 9) {
             wheelInitVelocity(j, h, dtRatio);
             return;
@@ -2253,6 +2464,10 @@ struct World {
         }
         if (j.type == 8) {
             prismaticSolveVelocity(j, h);
+            return;
+        }
+        if (j.type == 10) {
+            gearSolveVelocity(j);
             return;
         }
         if (j.type == 9) {
@@ -3183,6 +3398,7 @@ struct World {
         if (j.type == 4 || j.type == 5 || j.type == 7) return true;  // friction, motor, mouse: no position pass
         if (j.type == 6) return pulleySolvePosition(j);
         if (j.type == 8) return prismaticSolvePosition(j);
+        if (j.type == 10) return gearSolvePosition(j);
         if (j.type == 9) return wheelSolvePosition(j);
         V2 cA = posC[j.indexA]; float aA = posA[j.indexA];
         V2 cB = posC[j.indexB]; float aB = posA[j.indexB];
@@ -3772,6 +3988,7 @@ struct World {
                 createFixture(bi, s, f);
             }
         }
+        jointBase = (int)joints.size();
         for (int k = 0; k < nj; ++k) {
             b2scene::JointDef d = jd[k];
             if (d.bodyA < 0 || d.bodyB < 0 || d.bodyA >= nb || d.bodyB >= nb || d.bodyA == d.bodyB)
@@ -3992,6 +4209,14 @@ void orc_world_get_joints(void* h, float* out) {
             out[4 * i] = P.x;
             out[4 * i + 1] = P.y;
             out[4 * i + 2] = j.motorImpulse;
+            out[4 * i + 3] = 0.0f;
+            continue;
+        }
+        if (j.type == 10) {  // b2GearJoint::GetReactionForce / Torque at inv_dt = 1
+            const V2 P = j.impulse.x * j.JvAC;
+            out[4 * i] = P.x;
+            out[4 * i + 1] = P.y;
+            out[4 * i + 2] = j.impulse.x * j.JwA;
             out[4 * i + 3] = 0.0f;
             continue;
         }

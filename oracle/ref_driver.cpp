@@ -75,8 +75,20 @@ struct RefWorld {
 // directly, as b2RevoluteJointDef::Initialize computes them for bodies at
 // angle zero).
 int loadJoints(RefWorld* w, int base, const b2scene::JointDef* jd, int nj) {
+    const int jbase = (int)w->joints.size();
     for (int k = 0; k < nj; ++k) {
         const b2scene::JointDef& d = jd[k];
+        if (d.type == 10) {
+            b2GearJointDef gd;
+            gd.bodyA = w->bodies[base + d.bodyA];
+            gd.bodyB = w->bodies[base + d.bodyB];
+            gd.collideConnected = d.collideConnected != 0;
+            gd.joint1 = w->joints[jbase + d.joint1];
+            gd.joint2 = w->joints[jbase + d.joint2];
+            gd.ratio = d.ratio;
+            w->joints.push_back(w->world->CreateJoint(&gd));
+            continue;
+        }
         if (d.type == 1) {
             b2DistanceJointDef dd;
             dd.bodyA = w->bodies[base + d.bodyA];
@@ -570,7 +582,7 @@ void ref_world_get_joints(void* h, float* out) {
         const b2JointType type = w->joints[i]->GetType();
         if (type == e_distanceJoint || type == e_ropeJoint || type == e_weldJoint || type == e_frictionJoint ||
             type == e_motorJoint || type == e_pulleyJoint || type == e_mouseJoint || type == e_prismaticJoint ||
-            type == e_wheelJoint) {
+            type == e_wheelJoint || type == e_gearJoint) {
             const b2Vec2 f = w->joints[i]->GetReactionForce(1.0f);
             out[4 * i] = f.x;
             out[4 * i + 1] = f.y;

@@ -37,7 +37,7 @@ def test_struct_layouts_match_the_library():
     sizes = np.zeros(n, np.int32)
     lib.b2g_abi_sizes(E.ptr(sizes), n)
     shape, manifold, body, fixture, world, params, stats, profile, mlp, aabb, xf, revolute, schedule, sweep, distance, \
-        rope, weld, friction, motor, pulley, mouse, prismatic, wheel = sizes[:23]
+        rope, weld, friction, motor, pulley, mouse, prismatic, wheel, gear = sizes[:24]
     assert E.SHAPE.itemsize == shape
     assert E.MANIFOLD.itemsize == manifold
     assert E.STATS.itemsize == stats
@@ -60,4 +60,5 @@ def test_struct_layouts_match_the_library():
     assert mouse == 44     # 2 body indices, 8 floats, 1 switch
     assert prismatic == 64 # 2 body indices, 11 floats, 3 switches
     assert wheel == 56     # 2 body indices, 10 floats, 2 switches
+    assert gear == 92      # 4 body indices, 2 kinds, 16 floats, 1 switch
     assert params == E.STEP_PARAMS.itemsize

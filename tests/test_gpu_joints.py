@@ -433,3 +433,21 @@ def test_wheels_hold(gpu):
     chassis = [1 + 3 * i for i in range(6)]
     assert all(b[c, E.PX] - x0[c] > 2.0 for c in chassis[::2]), [b[c, E.PX] - x0[c] for c in chassis]
     g.close()
+
+
+def test_gear_trains_coloured(gpu):
+    """Gear trains (a gear joint writes four bodies; it is coloured with all
+    of them): the GPU keeps the gear relations and the rack's closed-form
+    fall (2%), and b2g_joint_drift stays under 1 cm / 0.01 rad over every
+    revolute, prismatic and gear joint of the four trains."""
+    from test_oracle import gear_train_check
+    lib = _b2g(gpu)
+    g = gpu.world("gears", 4, 0, sleep=False)
+    gear_train_check(g, 60, 0.02)
+    ctx = g.device_context()
+    out = np.zeros(3, np.float64)
+    for _ in range(4):
+        g.step(30)
+        assert lib.b2g_joint_drift(ctx, E.ptr(out)) == 0
+        assert out[2] == 20 and out[1] < 0.01, out
+    g.close()

@@ -45,6 +45,7 @@ F = ctypes.c_float
     ("prismatic", 12, 0, 300, True, 8, 3),     # prismatic joints: free, limited (one equal limit), motorised
     ("wheels", 6, 0, 400, True, 8, 3),         # wheel joints: carts on suspension springs over bumps, driven
     ("wheels", 6, 1, 300, False, 4, 2),
+    ("gears", 4, 0, 300, True, 8, 3),          # gear joints: revolute-revolute and revolute-prismatic (rack) trains
 ])
 def test_world_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, vel, pos):
     a = ref.world(kind, n, seed, sleep=sleep)
@@ -870,4 +871,32 @@ def test_prismatic_closed_forms(orc):
     assert -1.0 - 0.01 < lowest < -0.99, lowest
     v1 = w.bodies()[2, [E.VX, E.VY]].astype(np.float64)
     assert abs(float(np.dot(v1, axis(1))) - 2.0) < 1e-3, v1
+    w.close()
+
+
+def gear_train_check(w, steps, tol):
+    """Shared by the oracle and GPU gear tests: train 0 of the gears scene
+    (disc 1: r 1, density 5; disc 2: r 2, density 5; rack 1 x 6 m, density
+    1) keeps its gear relations, angle1 = -angle2 / 2 and angle2 = y / 2 for
+    the rack's drop y, and while free the rack falls at
+    a = -m g / (m + I2 / 4 + I1 / 16) (one degree of freedom, the energy of
+    the train), so y_k = a h^2 k (k + 1) / 2 within `tol` of its value."""
+    h = 1.0 / 60.0
+    m, I1, I2 = 6.0, 0.5 * 5.0 * np.pi * 1.0, 0.5 * 20.0 * np.pi * 4.0
+    a = -m * 10.0 / (m + I2 / 4.0 + I1 / 16.0)
+    y0 = float(w.bodies()[3, E.CY])
+    for k in range(1, steps + 1):
+        w.step(1)
+        b = w.bodies()
+        y = float(b[3, E.CY]) - y0
+        a1, a2 = float(b[1, E.ANG]), float(b[2, E.ANG])
+        assert abs(a1 + 0.5 * a2) < 1e-3 and abs(a2 - 0.5 * y) < 1e-3, (k, a1, a2, y)
+        want = a * h * h * k * (k + 1) / 2
+        if k >= 10:
+            assert abs(y - want) < tol * abs(want), (k, y, want)
+
+
+def test_gear_train_closed_form(orc):
+    w = orc.world("gears", 4, 0, sleep=False)
+    gear_train_check(w, 60, 0.02)
     w.close()
