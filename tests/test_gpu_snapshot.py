@@ -83,6 +83,8 @@ def _edit_world(lib, w, ctx, kind, p):
     ("chains", 4, 14, True, 0),
     ("stress", 600, 11, False, 2),
     ("pulley", 8, 0, True, 0),      # pulley constants live in the joint rows
+    ("gears", 4, 0, False, 0),      # gear, prismatic and revolute joints: axis and direction rows
+    ("wheels", 6, 0, False, 0),     # wheel springs keep their axis rows across steps
 ])
 def test_restored_world_steps_identically(gpu, kind, n, arg, sleep, worlds):
     lib = E.b2g_lib()
@@ -114,7 +116,7 @@ def test_restored_world_steps_identically(gpu, kind, n, arg, sleep, worlds):
                 assert np.array_equal(ma, mb), f"step {s + 1}: contact tables differ"
                 assert np.array_equal(na, nbm), f"step {s + 1}: manifolds differ"
                 assert np.array_equal(_joints(lib, a, nj), _joints(lib, b, nj)), f"step {s + 1}: joints differ"
-        assert len(ma) > 0
+        assert len(ma) > 0 or kind == "gears"  # the gear trains touch nothing
     finally:
         lib.b2g_destroy(b)
         w.close()
