@@ -5,24 +5,31 @@ Workload: the random pile of 100k dynamic bodies (50% circles r~U[0.25,0.75],
 50% convex polygons of 3-8 points on a circle of R~U[0.3,0.8]), density 1,
 friction 0.6, restitution 0, dropped from a 300x600 m grid into a static U
 container; dt 1/60, 8 velocity / 3 position iterations, sleeping and
-continuous collision off on both sides. A step is one b2World::Step over the
-whole pile; the timed region is steps 1..K of a freshly loaded pile (all
-state already in HBM), after W warm-up steps on a separate copy.
+continuous collision off on both sides, 1000 steps from a fresh load. A step
+is one b2World::Step over the whole pile. With --steps K the last K steps of
+those 1000 are timed (all state already in HBM) and the steps before them run
+untimed; `full_window` reports the rate over all 1000.
 
 Also reported on the same JSON line:
   roofline      the dominant kernel (velocity solve) against HBM bandwidth:
                 algorithmic bytes per launch from the byte model
                 (48 B per constraint + 72 B per manifold point per velocity
                 iteration) over its mean execution time, measured in a second
-                pass over the same workload (fresh pile, same steps) with HIP
-                events that its dispatch packets record on the step stream
-                (hipExtLaunchKernelGGL); traffic is the HBM bytes per launch
-                measured by rocprofv3 counters (given with --traffic-bytes);
-                stream_us_per_launch is the time per launch seen on the
-                stream, dispatch gaps included;
+                pass over the whole workload (fresh pile, the same 1000 steps)
+                with HIP events that its dispatch packets record on the step
+                stream (hipExtLaunchKernelGGL); traffic is the HBM bytes per
+                launch measured by rocprofv3 counters; stream_us_per_launch is
+                the time per launch seen on the stream, dispatch gaps included;
   cpu_baseline  the reference b2World::Step (upstream Box2D 2.3.2 built from
-                the reference's sources, oracle/_ref) on the same pile on one
-                host core, over the first steps that fit the time budget.
+                the reference's sources, oracle/_ref) on one host core over the
+                same window: started from the device's state before the timed
+                steps, bounded by --cpu-seconds;
+  state_at_end / reference_at_end / matched_penetration
+                penetration, FP64 complementarity residual and kinetic energy
+                of the device state after the last step, the reference's at the
+                same step (committed fixture), and the penetration bound.
+Side sections (pyramid, learned warm start, batched worlds, stress,
+continuous collision) each record an "error" field instead of failing the line.
 """
 from __future__ import annotations
 
@@ -58,6 +65,10 @@ DT = 1.0 / 60.0
 # the solver, narrowphase and broad-phase query kernels
 SCHEDULE = np.dtype([("islands", "<i4"), ("collideOrder", "<i4"), ("solverBlock", "<i4"), ("collideBlock", "<i4"),
                      ("queryBlock", "<i4"), ("tailColours", "<i4")])
+_VEC2 = np.dtype([("x", "<f4"), ("y", "<f4")])
+_MPOINT = np.dtype([("lp", _VEC2), ("ni", "<f4"), ("ti", "<f4"), ("id", "<u4")])
+# b2g_manifold (include/b2g.h), 64 bytes
+MANIFOLD = np.dtype([("points", _MPOINT, (2,)), ("ln", _VEC2), ("lpt", _VEC2), ("type", "<i4"), ("count", "<i4")])
 # the schedule the benchmark's worlds run with (set by the autotuner)
 _schedule = None
 
@@ -127,25 +138,65 @@ def stats_of(lib, h):
     return st
 
 
-def cpu_reference(n, seed, vel, pos, budget_s):
-    """Reference b2World::Step on one core: steps 1..k of the same pile."""
+def _ref_lib():
+    """The reference build (upstream Box2D 2.3.2 from the reference's own
+    sources, oracle/_ref) or, where it was not built, the CPU restatement."""
     ref = os.path.join(ROOT, "oracle", "_ref", "libb2ref.so")
-    kind = "reference"
     if os.path.exists(ref):
-        lib, pre = ctypes.CDLL(ref), "ref_"
+        lib, pre, kind = ctypes.CDLL(ref), "ref_", "reference"
     else:
-        lib, pre = ctypes.CDLL(os.path.join(ROOT, "oracle", "_build", "liborc.so")), "orc_"
-        kind = "port"
-    f = lambda name: getattr(lib, pre + name)
+        lib, pre, kind = ctypes.CDLL(os.path.join(ROOT, "oracle", "_build", "liborc.so")), "orc_", "port"
+    f = lambda name: getattr(lib, pre + name)  # noqa: E731
     f("world_create").restype = ctypes.c_void_p
     f("world_create").argtypes = [ctypes.c_float, ctypes.c_float]
+    return f, kind
+
+
+def cpu_reference_window(state, n, seed, first, k, vel, pos, budget_s):
+    """Reference b2World::Step on one core over the same window as the timed
+    device steps: the reference is started from the device's state before
+    step `first` (poses and velocities of every body; its contacts are created
+    by one untimed step, without carried impulses), then up to k steps are
+    timed within the budget."""
+    f, kind = _ref_lib()
+    if kind != "reference":
+        return cpu_reference_prefix(n, seed, vel, pos, budget_s)
+    h = ctypes.c_void_p(f("world_create")(0.0, -10.0))
+    f("world_set_flags")(h, 0, 1, 0, 0)
+    st = np.ascontiguousarray(state, np.float32)
+    rc = f("world_load_scene_state")(h, b"pile", n, ctypes.c_uint(seed), st.ctypes.data_as(ctypes.c_void_p), len(st))
+    if rc != 0:
+        raise RuntimeError(f"reference state load failed rc={rc}")
+    t0 = time.perf_counter()
+    f("world_step")(h, ctypes.c_float(DT), vel, pos, 1)
+    first_s = time.perf_counter() - t0
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < k:
+        f("world_step")(h, ctypes.c_float(DT), vel, pos, 1)
+        steps += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    el = time.perf_counter() - t0
+    f("world_destroy")(h)
+    return {"value": n * steps / el, "unit": "body-steps/s", "cores": 1, "kind": kind,
+            "sample": f"steps {first + 2}-{first + 1 + steps} of the same {n}-body pile (seed {seed}): the reference "
+                      f"started from the device's state after step {first} (poses and velocities), step "
+                      f"{first + 1} untimed to create its contacts ({first_s:.1f} s), then {steps} steps timed on "
+                      f"one host thread ({el:.1f} s)"}
+
+
+def cpu_reference_prefix(n, seed, vel, pos, budget_s):
+    """Reference b2World::Step on one core: the first steps of the pile that
+    fit the budget (used where the reference build is absent)."""
+    f, kind = _ref_lib()
     h = ctypes.c_void_p(f("world_create")(0.0, -10.0))
     f("world_set_flags")(h, 0, 1, 0, 0)
     f("world_load_scene")(h, b"pile", n, ctypes.c_uint(seed))
     steps = 0
     t0 = time.perf_counter()
     while True:
-        f("world_step")(h, ctypes.c_float(1.0 / 60.0), vel, pos, 1)
+        f("world_step")(h, ctypes.c_float(DT), vel, pos, 1)
         steps += 1
         el = time.perf_counter() - t0
         if el >= budget_s or steps >= 1000:
@@ -154,6 +205,53 @@ def cpu_reference(n, seed, vel, pos, budget_s):
     return {"value": n * steps / el, "unit": "body-steps/s", "cores": 1, "kind": kind,
             "sample": f"steps 1-{steps} of 1000 of the same {n}-body pile (seed {seed}), one host thread, "
                       f"{el:.1f} s; early steps are cheaper than the settled pile"}
+
+
+def reference_stats(n, seed, at):
+    """The reference build's post-step measures of the same pile at step
+    `at`, from the committed fixture (tests/golden/pile_ref_stats.json,
+    written by tests/golden/make_pile_stats.py)."""
+    path = os.path.join(ROOT, "tests", "golden", "pile_ref_stats.json")
+    if not os.path.exists(path):
+        return None
+    data = json.load(open(path))
+    for v in data.values():
+        if v.get("bodies") == n and v.get("seed") == seed and str(at) in v.get("at", {}):
+            m = dict(v["at"][str(at)])
+            m["step"] = at
+            m["source"] = "tests/golden/pile_ref_stats.json (reference build, one core, offline)"
+            return m
+    return None
+
+
+def gpu_measures(lib, h):
+    """Post-step measures of a device world (box2dpp_amd.measures, FP64)."""
+    from box2dpp_amd import measures
+    nb = lib.b2pp_world_body_count(h)
+    nf = lib.b2pp_world_fixture_count(h)
+    bodies = np.zeros((nb, 12), np.float32)
+    lib.b2pp_world_get_bodies(h, bodies.ctypes.data_as(ctypes.c_void_p))
+    nc = lib.b2pp_world_contact_count(h)
+    meta = np.zeros((max(nc, 1), 4), np.int32)
+    man = np.zeros(max(nc, 1), MANIFOLD)
+    got = lib.b2pp_world_get_contacts(h, meta.ctypes.data_as(ctypes.c_void_p), man.ctypes.data_as(ctypes.c_void_p),
+                                      nc)
+    nc = min(nc, got)
+    fb = np.zeros(max(nf, 1), np.int32)
+    fm = np.zeros((max(nf, 1), 3), np.float32)
+    lib.b2pp_world_fixture_info(h, fb.ctypes.data_as(ctypes.c_void_p), fm.ctypes.data_as(ctypes.c_void_p))
+    mass = np.zeros((max(nb, 1), 2), np.float32)
+    lib.b2pp_world_body_mass(h, mass.ctypes.data_as(ctypes.c_void_p))
+    m = measures.all_measures(bodies, meta[:nc], man[:nc], fb[:nf], fm[:nf], mass[:nb])
+    m["contacts"] = int(nc)
+    return m
+
+
+def get_bodies(lib, h):
+    nb = lib.b2pp_world_body_count(h)
+    out = np.zeros((nb, 12), np.float32)
+    lib.b2pp_world_get_bodies(h, out.ctypes.data_as(ctypes.c_void_p))
+    return out
 
 
 MLP_PARAM_FLOATS = 64 * 16 + 64 + 2 * (64 * 64 + 64) + 2 * 64 + 2 + 16 + 16 + 2 + 2
@@ -194,11 +292,11 @@ def learned_warm_start(lib, args, scope=0, stride=1):
     _check(b2g, ctx, b2g.b2g_set_learned_scope(ctx, scope, stride))
     _check(b2g, ctx, b2g.b2g_log_fresh(ctx, args.log_capacity))
     t0 = time.perf_counter()
-    step(lib, h, args.steps, args.vel, args.pos)
+    step(lib, h, args.learned_steps, args.vel, args.pos)
     lib.b2pp_world_sync(h)
     n = ctypes.c_int()
     b2g.b2g_log_size(ctx, ctypes.byref(n))
-    out["log"] = {"seed": args.log_seed, "steps": args.steps, "step_stride": stride, "points": n.value,
+    out["log"] = {"seed": args.log_seed, "steps": args.learned_steps, "step_stride": stride, "points": n.value,
                   "seconds": round(time.perf_counter() - t0, 2)}
     # 2. train (kernel time of the gradient kernel from dispatch-packet events)
     progress(f"learned warm start: training on {n.value} points")
@@ -238,8 +336,8 @@ def learned_warm_start(lib, args, scope=0, stride=1):
     curves = []
     buf = np.zeros(3 * (T + 1), np.float64)
     it = ctypes.c_int()
-    for s in range(1, args.steps + 1):
-        traced = s % args.trace_every == 0
+    for s in range(1, args.learned_steps + 1):
+        traced = s >= args.trace_from and s % args.trace_every == 0
         if traced:
             b2g.b2g_set_trace(ctx, T)
         step(lib, h, 1, args.vel, args.pos)
@@ -249,6 +347,8 @@ def learned_warm_start(lib, args, scope=0, stride=1):
             if it.value == T:
                 curves.append(buf.reshape(3, T + 1).copy())
     lib.b2pp_world_destroy(h)
+    if not curves:
+        raise RuntimeError(f"no residual trace in steps {args.trace_from}-{args.learned_steps}")
     cur = np.array(curves).mean(axis=0)
     names = ("cold", "warm", "learned")
 
@@ -261,8 +361,8 @@ def learned_warm_start(lib, args, scope=0, stride=1):
     out["residual"] = {
         "definition": "RMS over solved points of r_n (vn - bias if normal impulse > 0 else min(vn - bias, 0)) "
                       "and r_t (vt if |tangent impulse| < friction * normal impulse else 0), m/s, FP64 on host",
-        "traced_steps": len(curves), "trace_every": args.trace_every, "eval_seed": args.eval_seed,
-        "mean_at_iteration": {nm: {str(k): round(float(cur[i, k]), 5) for k in (0, 1, 2, 4, 8, T) if k <= T}
+        "traced_steps": len(curves), "trace_every": args.trace_every, "trace_from": args.trace_from, "eval_seed": args.eval_seed,
+        "mean_at_iteration": {nm: {str(k): round(float(cur[i, k]), 5) for k in (0, 1, 2, 4, 8, 16, 32, 64, T) if k <= T}
                               for i, nm in enumerate(names)},
         "iterations_to_1e-4": {nm: iters_to(cur[i], 1e-4) for i, nm in enumerate(names)},
         "plain_warm_residual_at_8": round(plain8, 5),
@@ -277,14 +377,14 @@ def learned_warm_start(lib, args, scope=0, stride=1):
     lib.b2pp_world_set_learned(h, 1, ctypes.c_float(0.0))
     lib.b2pp_world_sync(h)
     t0 = time.perf_counter()
-    step(lib, h, args.steps, args.vel, args.pos)
+    step(lib, h, args.learned_steps, args.vel, args.pos)
     lib.b2pp_world_sync(h)
     wall = time.perf_counter() - t0
     pen = np.zeros(3, np.float64)
     lib.b2pp_world_penetration(h, pen.ctypes.data_as(ctypes.c_void_p))
     lib.b2pp_world_destroy(h)
-    out["learned_step"] = {"value": args.bodies * args.steps / wall, "unit": "body-steps/s",
-                           "ms_per_step": 1e3 * wall / args.steps, "mean_penetration_m": float(pen[0]),
+    out["learned_step"] = {"value": args.bodies * args.learned_steps / wall, "unit": "body-steps/s",
+                           "ms_per_step": 1e3 * wall / args.learned_steps, "mean_penetration_m": float(pen[0]),
                            "max_penetration_m": float(pen[1])}
     return out, params
 
@@ -513,25 +613,170 @@ def continuous(lib, args):
     return out
 
 
+def headline(lib, args):
+    """The metric's workload: the pile over `window` steps from a fresh load;
+    the last K (--steps) are timed, the steps before them run untimed."""
+    total = max(args.window, args.steps)
+    prefix = total - args.steps
+    if args.warmup > 0 and prefix < args.warmup:
+        progress(f"warm-up: {args.warmup} steps on a separate copy")
+        hw = make_world(lib, args.bodies, args.seed)
+        step(lib, hw, args.warmup, args.vel, args.pos)
+        lib.b2pp_world_sync(hw)
+        lib.b2pp_world_destroy(hw)
+    h = make_world(lib, args.bodies, args.seed)
+    lib.b2pp_world_sync(h)
+    progress(f"headline: {args.bodies} bodies, steps 1-{prefix} untimed, {prefix + 1}-{total} timed")
+    t0 = time.perf_counter()
+    for k in range(0, prefix, 100):
+        step(lib, h, min(100, prefix - k), args.vel, args.pos)
+    lib.b2pp_world_sync(h)
+    prefix_s = time.perf_counter() - t0
+    state = get_bodies(lib, h) if prefix > 0 else None
+    lib.b2pp_world_sync(h)
+    t0 = time.perf_counter()
+    step(lib, h, args.steps, args.vel, args.pos)
+    lib.b2pp_world_sync(h)
+    wall = time.perf_counter() - t0
+    st = stats_of(lib, h)
+    progress(f"headline: {1e3 * wall / args.steps:.3f} ms/step over the timed steps; measuring the state")
+    meas = gpu_measures(lib, h)
+    lib.b2pp_world_destroy(h)
+    return {"total": total, "prefix": prefix, "prefix_s": prefix_s, "wall": wall, "state": state, "stats": st,
+            "measures": meas}
+
+
+def roofline_pass(lib, args, total):
+    """The same workload from a fresh pile with per-phase events on the step
+    stream and the velocity kernel's execution time from events carried by
+    its own dispatch packets, over all `total` steps of the workload."""
+    b2g = box2dpp_amd.b2g()
+    progress(f"roofline pass: {total} steps with kernel events")
+    h = make_world(lib, args.bodies, args.seed)
+    ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
+    b2g.b2g_set_profiling(ctx, 1)
+    prof_bytes = 0.0
+    for _ in range(total):
+        step(lib, h, 1, args.vel, args.pos)
+        s2 = stats_of(lib, h)
+        prof_bytes += args.vel * (48.0 * s2.constraints + 72.0 * s2.points)
+    prof = (PhaseProfile * 32)()
+    cnt = ctypes.c_int(0)
+    b2g.b2g_get_profile(ctx, prof, 32, ctypes.byref(cnt))
+    b2g.b2g_set_profiling(ctx, 0)
+    lib.b2pp_world_destroy(h)
+    entries = {prof[i].name.decode(): (prof[i].launches, prof[i].ms) for i in range(min(cnt.value, 32))}
+    phases = {k: v for k, v in entries.items() if not k.startswith("k_")}
+    kernels = {k: v for k, v in entries.items() if k.startswith("k_")}
+    sv_launches, sv_ms = kernels.get("k_solve_velocity", (0, 0.0))
+    avg_launch_ms = sv_ms / max(sv_launches, 1)
+    ph_launches, ph_ms = phases.get("solve_velocity", (0, 0.0))
+    bytes_per_launch = prof_bytes / max(sv_launches, 1)
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
+    total_phase_ms = sum(ms for _, ms in phases.values())
+    roof = {"bound": "hbm", "kernel": "k_solve_velocity", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": args.traffic_bytes,
+            "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_ms * 1e3,
+            "launches_per_step": sv_launches / max(total, 1),
+            "stream_us_per_launch": 1e3 * ph_ms / max(ph_launches, 1),
+            "window": f"steps 1-{total} of a fresh pile (the whole workload; the timed steps are its last ones)"}
+    extra = {"phase_ms_per_step": {k: round(v[1] / max(total, 1), 4) for k, v in phases.items()},
+             "phase_share": {k: round(v[1] / total_phase_ms, 4) for k, v in phases.items()} if total_phase_ms else {},
+             "kernel_us_per_launch": {k: round(1e3 * v[1] / max(v[0], 1), 3) for k, v in kernels.items()},
+             "kernel_launches_per_step": {k: round(v[0] / max(total, 1), 2) for k, v in kernels.items()}}
+    return roof, extra
+
+
+def pyramid_leg(lib, args):
+    """The 20-row pyramid (210 boxes) over 600 steps at the reference's
+    defaults (sleeping and continuous collision on), 8/3 iterations: the
+    device step (coloured solver, and the reference-order mode) against the
+    reference b2World::Step on one core, same flags, same steps."""
+    out = {"workload": "20-row box pyramid, 210 dynamic boxes, 600 steps, 8/3 iterations, sleep and CCD on"}
+    for name, mode in (("gpu_coloured", 0), ("gpu_reference_order", 1)):
+        h = ctypes.c_void_p(lib.b2pp_world_create(0.0, -10.0))
+        lib.b2pp_world_set_flags(h, 1, 1, 1, 0)
+        lib.b2pp_world_set_mode(h, mode)
+        rc = lib.b2pp_world_load_scene(h, b"pyramid", 20, ctypes.c_uint(0))
+        if rc != 0:
+            raise RuntimeError(f"pyramid load failed rc={rc}")
+        lib.b2pp_world_sync(h)
+        t0 = time.perf_counter()
+        step(lib, h, 600, args.vel, args.pos)
+        lib.b2pp_world_sync(h)
+        wall = time.perf_counter() - t0
+        m = gpu_measures(lib, h)
+        awake = int(get_bodies(lib, h)[:, 10].sum())
+        lib.b2pp_world_destroy(h)
+        out[name] = {"value": 210 * 600 / wall, "unit": "body-steps/s", "ms_per_step": 1e3 * wall / 600,
+                     "mean_penetration_m": m["mean_penetration_m"], "max_penetration_m": m["max_penetration_m"],
+                     "awake_bodies_at_end": awake}
+    f, kind = _ref_lib()
+    h = ctypes.c_void_p(f("world_create")(0.0, -10.0))
+    f("world_set_flags")(h, 1, 1, 1, 0)
+    f("world_load_scene")(h, b"pyramid", 20, ctypes.c_uint(0))
+    t0 = time.perf_counter()
+    f("world_step")(h, ctypes.c_float(DT), args.vel, args.pos, 600)
+    wall = time.perf_counter() - t0
+    pen = np.zeros(3, np.float64)
+    f("world_penetration")(h, pen.ctypes.data_as(ctypes.c_void_p))
+    f("world_destroy")(h)
+    out["cpu_baseline"] = {"value": 210 * 600 / wall, "unit": "body-steps/s", "cores": 1, "kind": kind,
+                           "ms_per_step": 1e3 * wall / 600, "mean_penetration_m": float(pen[0]),
+                           "max_penetration_m": float(pen[1]), "sample": "the whole workload, one host thread"}
+    return out
+
+
+def check_abi():
+    """The ctypes layouts this script passes to the libraries must be the
+    libraries' own sizes: a smaller ctypes struct handed to a call that fills
+    the C struct overruns the Python heap (b2g_abi_sizes, include/b2g.h)."""
+    b2g = box2dpp_amd.b2g()
+    n = b2g.b2g_abi_sizes(None, 0)
+    sizes = np.zeros(n, np.int32)
+    b2g.b2g_abi_sizes(sizes.ctypes.data_as(ctypes.c_void_p), n)
+    want = {"b2g_manifold": (sizes[1], MANIFOLD.itemsize), "b2g_step_stats": (sizes[6], ctypes.sizeof(StepStats)),
+            "b2g_phase_profile": (sizes[7], ctypes.sizeof(PhaseProfile)),
+            "b2g_mlp_params": (sizes[8], 4 * MLP_PARAM_FLOATS), "b2g_schedule": (sizes[12], SCHEDULE.itemsize)}
+    bad = {k: v for k, v in want.items() if int(v[0]) != int(v[1])}
+    if bad:
+        raise RuntimeError(f"ctypes layouts differ from the library's (C size, ctypes size): {bad}")
+
+
+def section(name, fn, *a):
+    """Runs one side section; a failure is recorded, not raised, so the
+    headline line always prints."""
+    try:
+        return fn(*a)
+    except Exception as e:  # noqa: BLE001 - reported in the JSON line
+        progress(f"{name}: failed: {e!r}")
+        return {"error": repr(e)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=1000,
+                    help="timed steps: the last K steps of the workload's window (all of it when K >= window)")
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed warm-up steps (on a separate copy when the window has no untimed prefix)")
+    ap.add_argument("--window", type=int, default=1000, help="steps of the workload (1000 on the random pile)")
     ap.add_argument("--bodies", type=int, default=100000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--vel", type=int, default=8)
     ap.add_argument("--pos", type=int, default=3)
-    ap.add_argument("--profile-steps", type=int, default=None,
-                    help="steps of the timing pass (default: the same as --steps)")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline, its roofline pass and CPU leg (the profiled command)")
+    ap.add_argument("--no-roofline", action="store_true", help="skip the roofline pass")
+    ap.add_argument("--no-pyramid", action="store_true", help="skip the pyramid leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start evaluation")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-worlds run")
     ap.add_argument("--batch-worlds", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--batch-seed", type=int, default=1)
-    ap.add_argument("--batch-cpu-worlds", type=int, default=1024)
+    ap.add_argument("--batch-cpu-worlds", type=int, default=512)
     ap.add_argument("--batch-cpu-steps", type=int, default=500)
     ap.add_argument("--no-stress", action="store_true", help="skip the stress run (piles with jointed chains)")
     ap.add_argument("--stress-worlds", type=int, default=4)
@@ -541,15 +786,16 @@ def main():
     ap.add_argument("--no-ccd", action="store_true", help="skip the continuous-collision run")
     ap.add_argument("--schedule", type=str, default=PILE_SCHEDULE,
                     help="pile step schedule (islands,collideOrder,solverBlock,collideBlock,queryBlock,tailColours), "
-                         "or 'auto' to autotune it on the warm-up copy")
+                         "or 'auto' to autotune it on a separate copy")
     ap.add_argument("--ccd-bodies", type=int, default=10000)
     ap.add_argument("--ccd-steps", type=int, default=500)
-    ap.add_argument("--ccd-cpu-seconds", type=float, default=60.0)
+    ap.add_argument("--ccd-cpu-seconds", type=float, default=20.0)
     ap.add_argument("--tune-rounds", type=int, default=3,
                     help="autotuner rounds on the warm-up copy (0: default schedule)")
     ap.add_argument("--tune-at", type=int, default=500, help="step of the warm-up copy at which to autotune")
     ap.add_argument("--all-points-stride", type=int, default=30,
                     help="step stride of the every-point log (0 skips the every-point evaluation)")
+    ap.add_argument("--learned-steps", type=int, default=1000, help="steps of the learned warm-start log and eval")
     ap.add_argument("--log-seed", type=int, default=2)
     ap.add_argument("--eval-seed", type=int, default=1001)
     ap.add_argument("--log-capacity", type=int, default=10_000_000)
@@ -557,24 +803,22 @@ def main():
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--trace-every", type=int, default=100)
-    ap.add_argument("--trace-iters", type=int, default=16)
+    ap.add_argument("--trace-from", type=int, default=500, help="first step traced (the settled pile)")
+    ap.add_argument("--trace-iters", type=int, default=128)
     ap.add_argument("--traffic-bytes", type=float, default=None,
                     help="HBM bytes per velocity-solve launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE")
     args = ap.parse_args()
-    if args.profile_steps is None:
-        args.profile_steps = args.steps
-    default_workload = (args.bodies, args.seed, args.steps, args.vel, args.pos) == (100000, 1, 1000, 8, 3)
-    traffic_for_schedule = args.traffic_bytes is None and default_workload
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(batched worlds shard across GPUs by world; a single world by slab, "
-                                   "see DESIGN.md)"}))
+                                   "(batched worlds shard across GPUs by world, a single world by slab; "
+                                   "the sharding is a design, not a run)"}))
         return 2
+    if args.headline_only:
+        args.no_pyramid = args.no_learned = args.no_batched = args.no_stress = args.no_ccd = True
+    default_workload = (args.bodies, args.seed, args.window, args.vel, args.pos) == (100000, 1, 1000, 8, 3)
 
     lib = box2dpp_amd.host()
-    # warm-up on a separate copy of the scene; the autotuner then picks the
-    # step schedule on that copy at a later state of the pile (no schedule
-    # changes results, only speed)
+    check_abi()
     global _schedule
     pile_schedule = None
     if args.schedule and args.schedule != "auto":
@@ -582,76 +826,52 @@ def main():
         pile_schedule = np.zeros(1, SCHEDULE)
         for name, v in zip(SCHEDULE.names, vals):
             pile_schedule[0][name] = v
-    if args.warmup > 0 or (args.tune_rounds > 0 and pile_schedule is None):
+    elif args.tune_rounds > 0:
         hw = make_world(lib, args.bodies, args.seed)
-        if args.tune_rounds > 0 and pile_schedule is None:
-            pile_schedule = autotune(lib, hw, args, max(args.warmup, args.tune_at))
-        else:
-            step(lib, hw, args.warmup, args.vel, args.pos)
-        lib.b2pp_world_sync(hw)
+        pile_schedule = autotune(lib, hw, args, args.tune_at)
         lib.b2pp_world_destroy(hw)
     _schedule = pile_schedule
-    if traffic_for_schedule and pile_schedule is not None:
+    if args.traffic_bytes is None and default_workload and pile_schedule is not None:
         args.traffic_bytes = DEFAULT_TRAFFIC_BYTES.get(tuple(int(pile_schedule[0][k]) for k in SCHEDULE.names))
 
-    h = make_world(lib, args.bodies, args.seed)
-    lib.b2pp_world_sync(h)
-    progress(f"timed run: {args.bodies} bodies, {args.steps} steps")
-    t0 = time.perf_counter()
-    step(lib, h, args.steps, args.vel, args.pos)
-    lib.b2pp_world_sync(h)
-    t1 = time.perf_counter()
-    wall = t1 - t0
+    hl = headline(lib, args)
+    total, prefix, wall, st, meas = hl["total"], hl["prefix"], hl["wall"], hl["stats"], hl["measures"]
     value = args.bodies * args.steps / wall
-    st = stats_of(lib, h)
-    pen = np.zeros(3, np.float64)
-    lib.b2pp_world_penetration(h, pen.ctypes.data_as(ctypes.c_void_p))
-
-    lib.b2pp_world_destroy(h)
-
-    # Timing pass: the same workload again from a fresh pile, with per-phase
-    # events on the step stream and the velocity kernel's own execution time
-    # from events carried by its dispatch packets.
-    b2g = box2dpp_amd.b2g()
-    progress(f"timed run done ({1e3 * wall / args.steps:.3f} ms/step); timing pass")
-    h = make_world(lib, args.bodies, args.seed)
-    ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
-    b2g.b2g_set_profiling(ctx, 1)
-    prof_bytes = 0.0
-    for _ in range(args.profile_steps):
-        step(lib, h, 1, args.vel, args.pos)
-        s2 = stats_of(lib, h)
-        prof_bytes += args.vel * (48.0 * s2.constraints + 72.0 * s2.points)
-    prof = (PhaseProfile * 32)()
-    cnt = ctypes.c_int(0)
-    b2g.b2g_get_profile(ctx, prof, 32, ctypes.byref(cnt))
-    b2g.b2g_set_profiling(ctx, 0)
-    entries = {prof[i].name.decode(): (prof[i].launches, prof[i].ms) for i in range(min(cnt.value, 32))}
-    phases = {k: v for k, v in entries.items() if not k.startswith("k_")}
-    kernels = {k: v for k, v in entries.items() if k.startswith("k_")}
-    # kernel execution time from events carried by the dispatch packets
-    sv_launches, sv_ms = kernels.get("k_solve_velocity", (0, 0.0))
-    avg_launch_ms = sv_ms / max(sv_launches, 1)
-    ph_launches, ph_ms = phases.get("solve_velocity", (0, 0.0))
-    bytes_per_launch = prof_bytes / max(sv_launches, 1)
-    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
-    lib.b2pp_world_destroy(h)
-
-    learned, params = (None, None) if args.no_learned else learned_warm_start(lib, args)
-    learned_all = None
-    if not args.no_learned and args.all_points_stride > 0:
-        progress("learned warm start, every point")
-        learned_all, _ = learned_warm_start(lib, args, 1, args.all_points_stride)
-    batched = None if args.no_batched else batched_worlds(lib, args)
-    stressed = None if args.no_stress else stress(lib, args, params)
-    ccd = None if args.no_ccd else continuous(lib, args)
+    ref_at_end = reference_stats(args.bodies, args.seed, total)
+    roof, extra = (None, {}) if args.no_roofline else section("roofline", roofline_pass, lib, args, total)
+    if isinstance(roof, dict) and "error" in roof:
+        roof, extra = None, {"roofline_error": roof["error"]}
 
     cpu = None
     if not args.no_cpu:
-        progress("CPU reference sample")
-        cpu = cpu_reference(args.bodies, args.seed, args.vel, args.pos, args.cpu_seconds)
+        progress("CPU reference over the timed window")
+        if hl["state"] is not None:
+            cpu = section("cpu_baseline", cpu_reference_window, hl["state"], args.bodies, args.seed, prefix,
+                          args.steps, args.vel, args.pos, args.cpu_seconds)
+        else:
+            cpu = section("cpu_baseline", cpu_reference_prefix, args.bodies, args.seed, args.vel, args.pos,
+                          args.cpu_seconds)
 
-    total_phase_ms = sum(ms for _, ms in phases.values())
+    pyramid = None if args.no_pyramid else section("pyramid", pyramid_leg, lib, args)
+    learned, params = None, None
+    if not args.no_learned:
+        res = section("learned", learned_warm_start, lib, args)
+        learned, params = (res, None) if isinstance(res, dict) else res
+    learned_all = None
+    if not args.no_learned and args.all_points_stride > 0:
+        progress("learned warm start, every point")
+        res = section("learned_every_point", learned_warm_start, lib, args, 1, args.all_points_stride)
+        learned_all = res if isinstance(res, dict) else res[0]
+    batched = None if args.no_batched else section("batched", batched_worlds, lib, args)
+    stressed = None if args.no_stress else section("stress", stress, lib, args, params)
+    ccd = None if args.no_ccd else section("ccd", continuous, lib, args)
+
+    matched = None
+    if ref_at_end is not None:
+        matched = {"gpu_mean_penetration_m": meas["mean_penetration_m"],
+                   "reference_mean_penetration_m": ref_at_end["mean_penetration_m"],
+                   "bound": "gpu mean <= reference mean + 0.005 m at the same step",
+                   "ok": meas["mean_penetration_m"] <= ref_at_end["mean_penetration_m"] + 0.005}
     out = {
         "metric": "body-steps/sec + vel. iters to 1e-4 residual, plain vs learned warm-start",
         "value": value,
@@ -665,27 +885,27 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"random pile, {args.bodies} bodies, seed {args.seed}, steps 1-{args.steps}, "
+        "config": {"workload": f"random pile, {args.bodies} bodies, seed {args.seed}, {total}-step run from a fresh "
+                               f"load: steps {prefix + 1}-{total} timed, steps 1-{prefix} untimed before them; "
                                f"{args.vel} vel / {args.pos} pos iterations, coloured solver, sleep and CCD off",
-                   "bodies": args.bodies, "vel_iters": args.vel, "pos_iters": args.pos, "solver": "coloured"},
-        "roofline": {"bound": "hbm", "kernel": "k_solve_velocity", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": args.traffic_bytes,
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_ms * 1e3,
-                     "launches_per_step": sv_launches / max(args.profile_steps, 1),
-                     "stream_us_per_launch": 1e3 * ph_ms / max(ph_launches, 1)},
+                   "bodies": args.bodies, "vel_iters": args.vel, "pos_iters": args.pos, "solver": "coloured",
+                   "window_steps": total, "timed_steps": f"{prefix + 1}-{total}"},
+        "full_window": {"value": args.bodies * total / (hl["prefix_s"] + wall), "unit": "body-steps/s",
+                        "ms_per_step": 1e3 * (hl["prefix_s"] + wall) / total, "steps": f"1-{total}"},
+        "roofline": roof,
         "cpu_baseline": cpu,
+        "state_at_end": dict(meas, step=total, constraints=st.constraints, points=st.points, colours=st.colours),
+        "reference_at_end": ref_at_end,
+        "matched_penetration": matched,
         "schedule": sched_dict(_schedule),
+        "pyramid": pyramid,
         "learned_warm_start": learned,
         "learned_warm_start_every_point": learned_all,
         "batched_worlds": batched,
         "stress": stressed,
         "continuous_collision": ccd,
-        "final_state": {"contacts": st.contacts, "constraints": st.constraints, "points": st.points,
-                        "colours": st.colours, "mean_penetration_m": float(pen[0]), "max_penetration_m": float(pen[1])},
-        "phase_ms_per_step": {k: round(v[1] / max(args.profile_steps, 1), 4) for k, v in phases.items()},
-        "phase_share": {k: round(v[1] / total_phase_ms, 4) for k, v in phases.items()} if total_phase_ms else {},
-        "kernel_us_per_launch": {k: round(1e3 * v[1] / max(v[0], 1), 3) for k, v in kernels.items()},
     }
+    out.update(extra)
     print(json.dumps(out))
     return 0
 

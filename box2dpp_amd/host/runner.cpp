@@ -617,6 +617,31 @@ void b2pp_world_get_fat_aabbs(void* h, b2g_aabb* out) {
     b2g_get_fat_aabbs(r->world.GetDeviceContext(), out, (int)r->fixtures.size());
 }
 
+// Per fixture proxy: owning body index; radius, friction, restitution.
+void b2pp_world_fixture_info(void* h, int32_t* body, float* material) {
+    Runner* r = (Runner*)h;
+    for (size_t i = 0; i < r->fixtures.size(); ++i) {
+        const b2Fixture* f = r->fixtures[i];
+        body[i] = f->GetBody()->GetIndex();
+        material[3 * i + 0] = f->GetShape()->m_radius;
+        material[3 * i + 1] = f->GetFriction();
+        material[3 * i + 2] = f->GetRestitution();
+    }
+}
+
+// Per body: mass and rotational inertia about the centre of mass
+// (b2Body::GetInertia is about the origin).
+void b2pp_world_body_mass(void* h, float* out) {
+    Runner* r = (Runner*)h;
+    for (size_t i = 0; i < r->bodies.size(); ++i) {
+        const b2Body* b = r->bodies[i];
+        const b2Vec2& lc = b->GetLocalCenter();
+        const double m = b->GetMass();
+        out[2 * i + 0] = (float)m;
+        out[2 * i + 1] = (float)((double)b->GetInertia() - m * ((double)lc[0] * lc[0] + (double)lc[1] * lc[1]));
+    }
+}
+
 void b2pp_world_penetration(void* h, double* out) { ((Runner*)h)->world.GetPenetration(out); }
 
 void* b2pp_world_device_context(void* h) { return ((Runner*)h)->world.GetDeviceContext(); }

@@ -4306,6 +4306,27 @@ void orc_world_get_shapes(void* h, b2g_shape* out) {
     for (size_t i = 0; i < w->fixtures.size(); ++i) out[i] = w->fixtures[i].shape;
 }
 
+// Per fixture proxy: owning body index; radius, friction, restitution.
+void orc_world_fixture_info(void* h, int32_t* body, float* material) {
+    World* w = (World*)h;
+    for (size_t i = 0; i < w->fixtures.size(); ++i) {
+        const Fixture& f = w->fixtures[i];
+        body[i] = f.body;
+        material[3 * i + 0] = f.shape.radius;
+        material[3 * i + 1] = f.friction;
+        material[3 * i + 2] = f.restitution;
+    }
+}
+
+// Per body: mass and rotational inertia about the centre of mass.
+void orc_world_body_mass(void* h, float* out) {
+    World* w = (World*)h;
+    for (size_t i = 0; i < w->bodies.size(); ++i) {
+        out[2 * i + 0] = w->bodies[i].mass;
+        out[2 * i + 1] = w->bodies[i].I;
+    }
+}
+
 void orc_world_penetration(void* h, double* out) {
     World* w = (World*)h;
     double sum = 0.0, mx = 0.0;

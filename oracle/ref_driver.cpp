@@ -21,6 +21,7 @@
 #include <string.h>
 #include <vector>
 #include <algorithm>
+#include <unordered_map>
 
 #include "b2g.h"
 #include "scenes/scenes.h"
@@ -436,6 +437,28 @@ int ref_world_load_scene(void* h, const char* kind, int n, unsigned seed) {
     return loadJoints(w, base, s.joints.data(), (int)s.joints.size());
 }
 
+// A named scene with every body started from a given state: rows of 12
+// floats as *_world_get_bodies writes them (origin x, y at 0-1, angle at 6,
+// linear velocity at 7-8, angular velocity at 9), one per body in creation
+// order. Contacts are created by the first Step, without carried impulses.
+// Used to run the reference from another engine's state (the benchmark's CPU
+// leg resumes it where the device run stands).
+int ref_world_load_scene_state(void* h, const char* kind, int n, unsigned seed, const float* state, int nstate) {
+    b2scene::Scene s = b2scene::byName(kind, n, seed);
+    if (nstate != (int)s.bodies.size()) return -1;
+    for (int i = 0; i < nstate; ++i) {
+        const float* o = state + 12 * i;
+        b2scene::BodyDef& b = s.bodies[i];
+        b.px = o[0]; b.py = o[1]; b.angle = o[6];
+        b.vx = o[7]; b.vy = o[8]; b.w = o[9];
+    }
+    RefWorld* w = (RefWorld*)h;
+    const int base = (int)w->bodies.size();
+    int rc = ref_world_load(h, s.bodies.data(), (int)s.bodies.size(), s.fixtures.data(), (int)s.fixtures.size());
+    if (rc != 0) return rc;
+    return loadJoints(w, base, s.joints.data(), (int)s.joints.size());
+}
+
 int ref_world_joint_count(void* h) { return (int)((RefWorld*)h)->joints.size(); }
 
 // Installs the recording listener with the one-sided platform rule for
@@ -669,6 +692,30 @@ void ref_world_get_shapes(void* h, b2g_shape* out) {
         } else {
             shapeToRec(sh, out + i);
         }
+    }
+}
+
+// Per fixture proxy: owning body index; radius, friction, restitution.
+void ref_world_fixture_info(void* h, int32_t* body, float* material) {
+    RefWorld* w = (RefWorld*)h;
+    std::unordered_map<const b2Body*, int> index;
+    for (size_t i = 0; i < w->bodies.size(); ++i) index[w->bodies[i]] = (int)i;
+    for (size_t i = 0; i < w->fixtures.size(); ++i) {
+        const b2Fixture* f = w->fixtures[i];
+        body[i] = index[f->GetBody()];
+        material[3 * i + 0] = f->GetShape()->m_radius;
+        material[3 * i + 1] = f->GetFriction();
+        material[3 * i + 2] = f->GetRestitution();
+    }
+}
+
+// Per body: mass and rotational inertia about the centre of mass (b2Body::m_I
+// after ResetMassData).
+void ref_world_body_mass(void* h, float* out) {
+    RefWorld* w = (RefWorld*)h;
+    for (size_t i = 0; i < w->bodies.size(); ++i) {
+        out[2 * i + 0] = w->bodies[i]->m_mass;
+        out[2 * i + 1] = w->bodies[i]->m_I;
     }
 }
 

@@ -11,10 +11,13 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
 
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libb2ref.so")
 ORC_LIB = os.path.join(ROOT, "oracle", "_build", "liborc.so")
 GPU_HOST_LIB = os.path.join(ROOT, "box2dpp_amd", "lib", "libbox2d.so")
@@ -170,6 +173,30 @@ class World:
         out = np.zeros(3, np.float64)
         self.e.fn("world_penetration")(self.h, ptr(out))
         return float(out[0]), float(out[1]), int(out[2])
+
+    def fixture_info(self):
+        """(body index per fixture proxy, (radius, friction, restitution) per proxy)."""
+        nf = self.e.fn("world_fixture_count")(self.h)
+        body = np.zeros(max(nf, 1), np.int32)
+        mat = np.zeros((max(nf, 1), 3), np.float32)
+        self.e.fn("world_fixture_info")(self.h, ptr(body), ptr(mat))
+        return body[:nf], mat[:nf]
+
+    def body_mass(self) -> np.ndarray:
+        """(mass, rotational inertia about the centre) per body."""
+        nb = self.e.fn("world_body_count")(self.h)
+        out = np.zeros((max(nb, 1), 2), np.float32)
+        self.e.fn("world_body_mass")(self.h, ptr(out))
+        return out[:nb]
+
+    def measures(self) -> dict:
+        """Penetration, complementarity residual and kinetic energy of the
+        current state, FP64 (box2dpp_amd.measures), the same function for
+        every engine."""
+        from box2dpp_amd import measures
+        fb, fm = self.fixture_info()
+        meta, man = self.contacts()
+        return measures.all_measures(self.bodies(), meta, man, fb, fm, self.body_mass())
 
     def device_context(self) -> VP:
         """The world's b2g_context (C ABI of include/b2g.h)."""
