@@ -772,7 +772,9 @@ def main():
     ap.add_argument("--no-roofline", action="store_true", help="skip the roofline pass")
     ap.add_argument("--no-pyramid", action="store_true", help="skip the pyramid leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start evaluation")
-    ap.add_argument("--no-batched", action="store_true", help="skip the batched-worlds run")
+    ap.add_argument("--batched", action="store_true",
+                    help="run the batched-worlds section (off by default: a host-side crash of this section at "
+                         "512 and 4096 worlds has no identified cause, so it is not run unasked)")
     ap.add_argument("--batch-worlds", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--batch-seed", type=int, default=1)
@@ -814,7 +816,8 @@ def main():
                                    "the sharding is a design, not a run)"}))
         return 2
     if args.headline_only:
-        args.no_pyramid = args.no_learned = args.no_batched = args.no_stress = args.no_ccd = True
+        args.no_pyramid = args.no_learned = args.no_stress = args.no_ccd = True
+        args.batched = False
     default_workload = (args.bodies, args.seed, args.window, args.vel, args.pos) == (100000, 1, 1000, 8, 3)
 
     lib = box2dpp_amd.host()
@@ -862,7 +865,7 @@ def main():
         progress("learned warm start, every point")
         res = section("learned_every_point", learned_warm_start, lib, args, 1, args.all_points_stride)
         learned_all = res if isinstance(res, dict) else res[0]
-    batched = None if args.no_batched else section("batched", batched_worlds, lib, args)
+    batched = section("batched", batched_worlds, lib, args) if args.batched else None
     stressed = None if args.no_stress else section("stress", stress, lib, args, params)
     ccd = None if args.no_ccd else section("ccd", continuous, lib, args)
 

@@ -611,7 +611,8 @@ float b2PrismaticJoint::GetJointTranslation() const {
 // REF/Dynamics/Joints/b2PrismaticJoint.cpp:499-507 (the force along the
 // last step's axes, from the device), and GetMotorForce.
 b2Vec2 b2PrismaticJoint::GetReactionForce(float inv_dt) const {
-    if (m_dirty) return b2Vec2(0.0f, 0.0f);
+    if (m_index >= m_world->m_uploadedJoints) return b2Vec2(0.0f, 0.0f);
+    if (!m_world->IsLocked()) (void)m_world->UploadJointEdits();
     const b2g_vec2 f = deviceReaction(m_world->m_ctx, m_index, nullptr);
     return b2Vec2(inv_dt * f.x, inv_dt * f.y);
 }
@@ -678,7 +679,8 @@ b2Vec2 b2WheelJoint::GetAnchorA() const { return m_bodyA->GetWorldPoint(m_localA
 b2Vec2 b2WheelJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localAnchorB); }
 // REF/Dynamics/Joints/b2WheelJoint.cpp:341-349
 b2Vec2 b2WheelJoint::GetReactionForce(float inv_dt) const {
-    if (m_dirty) return b2Vec2(0.0f, 0.0f);
+    if (m_index >= m_world->m_uploadedJoints) return b2Vec2(0.0f, 0.0f);
+    if (!m_world->IsLocked()) (void)m_world->UploadJointEdits();
     const b2g_vec2 f = deviceReaction(m_world->m_ctx, m_index, nullptr);
     return b2Vec2(inv_dt * f.x, inv_dt * f.y);
 }
@@ -725,12 +727,14 @@ b2Vec2 b2GearJoint::GetAnchorA() const { return m_bodyA->GetWorldPoint(m_localAn
 b2Vec2 b2GearJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localAnchorB); }
 // REF/Dynamics/Joints/b2GearJoint.cpp:406-416, from the device
 b2Vec2 b2GearJoint::GetReactionForce(float inv_dt) const {
-    if (m_dirty) return b2Vec2(0.0f, 0.0f);
+    if (m_index >= m_world->m_uploadedJoints) return b2Vec2(0.0f, 0.0f);
+    if (!m_world->IsLocked()) (void)m_world->UploadJointEdits();
     const b2g_vec2 f = deviceReaction(m_world->m_ctx, m_index, nullptr);
     return b2Vec2(inv_dt * f.x, inv_dt * f.y);
 }
 float b2GearJoint::GetReactionTorque(float inv_dt) const {
-    if (m_dirty) return 0.0f;
+    if (m_index >= m_world->m_uploadedJoints) return 0.0f;
+    if (!m_world->IsLocked()) (void)m_world->UploadJointEdits();
     float t = 0.0f;
     (void)deviceReaction(m_world->m_ctx, m_index, &t);
     return inv_dt * t;
@@ -1545,50 +1549,56 @@ int32_t b2World::Upload() {
         if (rc != B2G_OK) return rc;
         m_removedJoints.clear();
     }
-    if (!m_dirtyJoints.empty()) {
-        for (b2Joint* j : m_dirtyJoints) {
-            j->m_dirty = false;
-            if (j->m_index >= m_uploadedJoints) continue;
-            if (j->m_type == b2JointType::DISTANCE_JOINT) {
-                const b2DistanceJoint* d = static_cast<const b2DistanceJoint*>(j);
-                rc = b2g_set_distance_joint(m_ctx, j->m_index, d->m_length, d->m_frequencyHz, d->m_dampingRatio);
-            } else if (j->m_type == b2JointType::ROPE_JOINT) {
-                rc = b2g_set_rope_joint(m_ctx, j->m_index, static_cast<const b2RopeJoint*>(j)->m_maxLength);
-            } else if (j->m_type == b2JointType::WELD_JOINT) {
-                const b2WeldJoint* d = static_cast<const b2WeldJoint*>(j);
-                rc = b2g_set_weld_joint(m_ctx, j->m_index, d->m_frequencyHz, d->m_dampingRatio);
-            } else if (j->m_type == b2JointType::FRICTION_JOINT) {
-                const b2FrictionJoint* d = static_cast<const b2FrictionJoint*>(j);
-                rc = b2g_set_friction_joint(m_ctx, j->m_index, d->m_maxForce, d->m_maxTorque);
-            } else if (j->m_type == b2JointType::MOTOR_JOINT) {
-                const b2MotorJoint* m = static_cast<const b2MotorJoint*>(j);
-                const b2g_motor_def d = motorDef(m, m->m_bodyA->m_index, m->m_bodyB->m_index);
-                rc = b2g_set_motor_joint(m_ctx, j->m_index, &d);
-            } else if (j->m_type == b2JointType::MOUSE_JOINT) {
-                const b2MouseJoint* m = static_cast<const b2MouseJoint*>(j);
-                const b2g_mouse_def d = mouseDef(m, m->m_bodyA->m_index, m->m_bodyB->m_index, m->m_bodyB->m_mass);
-                rc = b2g_set_mouse_joint(m_ctx, j->m_index, &d);
-            } else if (j->m_type == b2JointType::PULLEY_JOINT) {
-                rc = B2G_OK;  // no setters
-            } else if (j->m_type == b2JointType::PRISMATIC_JOINT) {
-                const b2PrismaticJoint* p = static_cast<const b2PrismaticJoint*>(j);
-                const b2g_prismatic_def d = prismaticDef(p, p->m_bodyA->m_index, p->m_bodyB->m_index);
-                rc = b2g_set_prismatic_joint(m_ctx, j->m_index, &d);
-            } else if (j->m_type == b2JointType::GEAR_JOINT) {
-                rc = b2g_set_gear_joint(m_ctx, j->m_index, static_cast<const b2GearJoint*>(j)->m_ratio);
-            } else if (j->m_type == b2JointType::WHEEL_JOINT) {
-                const b2WheelJoint* p = static_cast<const b2WheelJoint*>(j);
-                const b2g_wheel_def d = wheelDef(p, p->m_bodyA->m_index, p->m_bodyB->m_index);
-                rc = b2g_set_wheel_joint(m_ctx, j->m_index, &d);
-            } else {
-                const b2RevoluteJoint* r = static_cast<const b2RevoluteJoint*>(j);
-                const b2g_revolute_def d = revoluteDef(r, r->m_bodyA->m_index, r->m_bodyB->m_index);
-                rc = b2g_set_joint(m_ctx, j->m_index, &d);
-            }
-            if (rc != B2G_OK) return rc;
+    if ((rc = UploadJointEdits()) != B2G_OK) return rc;
+    return B2G_OK;
+}
+
+int32_t b2World::UploadJointEdits() {
+    if (m_dirtyJoints.empty() || !m_ctx) return B2G_OK;
+    int rc = B2G_OK;
+    for (b2Joint* j : m_dirtyJoints) {
+        j->m_dirty = false;
+        if (j->m_index >= m_uploadedJoints) continue;
+        if (j->m_type == b2JointType::DISTANCE_JOINT) {
+            const b2DistanceJoint* d = static_cast<const b2DistanceJoint*>(j);
+            rc = b2g_set_distance_joint(m_ctx, j->m_index, d->m_length, d->m_frequencyHz, d->m_dampingRatio);
+        } else if (j->m_type == b2JointType::ROPE_JOINT) {
+            rc = b2g_set_rope_joint(m_ctx, j->m_index, static_cast<const b2RopeJoint*>(j)->m_maxLength);
+        } else if (j->m_type == b2JointType::WELD_JOINT) {
+            const b2WeldJoint* d = static_cast<const b2WeldJoint*>(j);
+            rc = b2g_set_weld_joint(m_ctx, j->m_index, d->m_frequencyHz, d->m_dampingRatio);
+        } else if (j->m_type == b2JointType::FRICTION_JOINT) {
+            const b2FrictionJoint* d = static_cast<const b2FrictionJoint*>(j);
+            rc = b2g_set_friction_joint(m_ctx, j->m_index, d->m_maxForce, d->m_maxTorque);
+        } else if (j->m_type == b2JointType::MOTOR_JOINT) {
+            const b2MotorJoint* m = static_cast<const b2MotorJoint*>(j);
+            const b2g_motor_def d = motorDef(m, m->m_bodyA->m_index, m->m_bodyB->m_index);
+            rc = b2g_set_motor_joint(m_ctx, j->m_index, &d);
+        } else if (j->m_type == b2JointType::MOUSE_JOINT) {
+            const b2MouseJoint* m = static_cast<const b2MouseJoint*>(j);
+            const b2g_mouse_def d = mouseDef(m, m->m_bodyA->m_index, m->m_bodyB->m_index, m->m_bodyB->m_mass);
+            rc = b2g_set_mouse_joint(m_ctx, j->m_index, &d);
+        } else if (j->m_type == b2JointType::PULLEY_JOINT) {
+            rc = B2G_OK;  // no setters
+        } else if (j->m_type == b2JointType::PRISMATIC_JOINT) {
+            const b2PrismaticJoint* p = static_cast<const b2PrismaticJoint*>(j);
+            const b2g_prismatic_def d = prismaticDef(p, p->m_bodyA->m_index, p->m_bodyB->m_index);
+            rc = b2g_set_prismatic_joint(m_ctx, j->m_index, &d);
+        } else if (j->m_type == b2JointType::GEAR_JOINT) {
+            rc = b2g_set_gear_joint(m_ctx, j->m_index, static_cast<const b2GearJoint*>(j)->m_ratio);
+        } else if (j->m_type == b2JointType::WHEEL_JOINT) {
+            const b2WheelJoint* p = static_cast<const b2WheelJoint*>(j);
+            const b2g_wheel_def d = wheelDef(p, p->m_bodyA->m_index, p->m_bodyB->m_index);
+            rc = b2g_set_wheel_joint(m_ctx, j->m_index, &d);
+        } else {
+            const b2RevoluteJoint* r = static_cast<const b2RevoluteJoint*>(j);
+            const b2g_revolute_def d = revoluteDef(r, r->m_bodyA->m_index, r->m_bodyB->m_index);
+            rc = b2g_set_joint(m_ctx, j->m_index, &d);
         }
-        m_dirtyJoints.clear();
+        if (rc != B2G_OK) return rc;
     }
+    m_dirtyJoints.clear();
+    m_jointsStale = true;  // impulses a setter reset on the device
     return B2G_OK;
 }
 

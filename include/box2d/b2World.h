@@ -9,8 +9,9 @@
 //  * continuous collision (SetContinuousPhysics, on by default as in the
 //    reference) runs on the GPU after the discrete solve; the contact
 //    updates of its TOI events call no listener.
-//  * sensors are rejected at creation; of the joint types, revolute joints
-//    run (with the coloured solver).
+//  * sensors are rejected at creation; every joint type of the reference
+//    runs with the coloured solver, and the reference-order mode rejects
+//    worlds with joints.
 //  * with a contact listener, Step splits at the narrowphase: BeginContact,
 //    EndContact and PreSolve are called inside the step in list order,
 //    PostSolve after it for the solved contacts.
@@ -796,8 +797,9 @@ public:
     void SetContactListener(b2ContactListener* listener) { m_contactListener = listener; }
     b2Body* CreateBody(const b2BodyDef* def);
     void DestroyBody(b2Body* body);
-    // Box2D/Dynamics/b2World.h:85-92. Revolute and distance joints; other
-    // types return nullptr with GetLastError() set.
+    // Box2D/Dynamics/b2World.h:85-92. Every joint type of the reference
+    // (revolute, prismatic, distance, pulley, mouse, gear, wheel, weld,
+    // friction, rope, motor).
     b2Joint* CreateJoint(const b2JointDef* def);
     void DestroyJoint(b2Joint* joint);
     b2Joint* GetJointList() { return m_jointList; }
@@ -867,6 +869,10 @@ private:
     friend class b2WheelJoint;
     friend class b2GearJoint;
     int32_t Upload();
+    // Uploads the setters' edits of joints already on the device (part of
+    // Upload); getters call it so a reaction read after a setter reflects
+    // the setter's side effects, as the reference's members do at once.
+    int32_t UploadJointEdits();
     void Download() const;
     void DownloadJoints() const;
     void ResolveGear(b2GearJoint* g);

@@ -34,6 +34,15 @@ def gpu():
 
 
 @pytest.fixture(scope="session")
+def gold():
+    """The committed reference-build fixtures (tests/golden/ref_vectors.npz)."""
+    import numpy as np
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_vectors.npz")
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(scope="session")
 def orclib():
     return engines.oracle()
 

@@ -6,11 +6,16 @@ reference's own sources by oracle/Makefile):
                   reference's own b2PolygonShape::Set / SetAsBox) and the
                   manifolds its b2Collide* functions return
   hello_bodies    HelloWorld (6/2 iterations), the box's row after each of 60 steps
+  hello_ccd_bodies  the same with continuous collision on, the reference's
+                  default (HelloWorld/HelloWorld.cpp sets no flag)
   pyramid_*       the 20-row pyramid, sleep on, 8/3 iterations, after 600 steps:
                   bodies, contact count, penetration (mean, max, points)
   pendulum_*      24 pendulums, sleep on, 8/3 iterations, after 200 steps:
                   bodies and joint impulses
 
+The reference's b2Collide* functions leave the points' normal and tangent
+impulses unset, so those bytes of corpus_manifold vary between runs; the
+tests compare the manifold geometry only (tests/engines.py geometry_bits).
 Body rows are those of *_world_get_bodies (tests/engines.py). Run from the
 repository root after building oracle/_ref:  python tests/golden/make_golden.py
 """
@@ -45,6 +50,14 @@ def main() -> None:
         w.step(1, vel=6, pos=2)
         rows.append(w.bodies()[1])
     out["hello_bodies"] = np.array(rows, np.float32)
+    w.close()
+
+    w = ref.world("hello", sleep=True, ccd=True)
+    rows = []
+    for _ in range(60):
+        w.step(1, vel=6, pos=2)
+        rows.append(w.bodies()[1])
+    out["hello_ccd_bodies"] = np.array(rows, np.float32)
     w.close()
 
     w = ref.world("pyramid", 20, 0, sleep=True)

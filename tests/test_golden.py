@@ -11,21 +11,13 @@ path where oracle/_ref is not built.
 from __future__ import annotations
 
 import ctypes
-import os
 
 import numpy as np
 import pytest
 
 import engines as E
 
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_vectors.npz")
 REFERENCE = 1
-
-
-@pytest.fixture(scope="module")
-def gold():
-    with np.load(GOLDEN, allow_pickle=False) as z:
-        return {k: z[k] for k in z.files}
 
 
 def _bits(a):
