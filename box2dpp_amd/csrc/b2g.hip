@@ -43,6 +43,7 @@
 #include <random>
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 
 using namespace b2g;
 
@@ -6476,6 +6477,10 @@ int uploadBodies(b2g_context* ctx, int first, int count, const b2g_body_state* b
 // ====================================================================
 // C ABI
 // ====================================================================
+namespace {
+void noallocInit(int device);
+}  // namespace
+
 extern "C" {
 
 int b2g_create(b2g_context** out, int device) {
@@ -6487,6 +6492,7 @@ int b2g_create(b2g_context** out, int device) {
         delete ctx;
         return B2G_ERR_HIP;
     }
+    noallocInit(device);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return B2G_ERR_HIP;
@@ -8416,6 +8422,165 @@ int b2g_remove_joints(b2g_context* ctx, const int* ids, int count) {
     HIPCHK(ctx, hipStreamSynchronize(s));
     ctx->jointsDirty = true;
     if (keptApart) return rebuildJointPairs(ctx);
+    return B2G_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ pair set without allocation
+// b2g_pairs_batch_noalloc: b2g_pairs_batch with every device buffer of the
+// call (staged inputs, grid, pair keys, sort and scan temporaries, counters)
+// carved out of one file-scope array per GPU, for callers that cannot
+// allocate device memory. Limits: kNoAllocMaxProxies proxies and
+// kNoAllocMaxPairs pairs per call.
+namespace {
+
+constexpr int kNoAllocMaxProxies = 65536;
+constexpr int kNoAllocMaxPairs = 1 << 20;
+constexpr int kNoAllocMaxBuckets = 2 * kNoAllocMaxProxies;  // power of two >= 2n, at least 1024
+constexpr size_t kNoAllocSortBytes = size_t(24) << 20;
+constexpr size_t noallocAlign(size_t b) { return (b + 255) & ~size_t(255); }
+// byte offsets into the array
+constexpr size_t kNaFat = 0;
+constexpr size_t kNaMoved = kNaFat + noallocAlign(sizeof(float4) * kNoAllocMaxProxies);
+constexpr size_t kNaCounts = kNaMoved + noallocAlign(kNoAllocMaxProxies);
+constexpr size_t kNaStarts = kNaCounts + noallocAlign(sizeof(int) * (kNoAllocMaxBuckets + 1));
+constexpr size_t kNaCursor = kNaStarts + noallocAlign(sizeof(int) * (kNoAllocMaxBuckets + 1));
+constexpr size_t kNaEntries = kNaCursor + noallocAlign(sizeof(int) * (kNoAllocMaxBuckets + 1));
+constexpr size_t kNaCell = kNaEntries + noallocAlign(sizeof(int) * kMaxCellsPerProxy * kNoAllocMaxProxies);
+constexpr size_t kNaBig = kNaCell + noallocAlign(sizeof(unsigned long long) * kMaxCellsPerProxy * kNoAllocMaxProxies);
+constexpr size_t kNaRange = kNaBig + noallocAlign(sizeof(unsigned long long) * kNoAllocMaxProxies);
+constexpr size_t kNaKeys = kNaRange + noallocAlign(sizeof(int4) * kNoAllocMaxProxies);
+constexpr size_t kNaSorted = kNaKeys + noallocAlign(sizeof(unsigned long long) * kNoAllocMaxPairs);
+constexpr size_t kNaSort = kNaSorted + noallocAlign(sizeof(unsigned long long) * kNoAllocMaxPairs);
+constexpr size_t kNaCounters = kNaSort + noallocAlign(kNoAllocSortBytes);
+constexpr size_t kNoAllocBytes = kNaCounters + noallocAlign(sizeof(Counters));
+
+}  // namespace
+
+// One copy per GPU, held until the process ends (about 56 MiB). Host code
+// never passes this name to a kernel: it uses the device address that
+// hipGetSymbolAddress returned for the GPU (noallocBase).
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noallocMutex;
+void* g_noallocBase[kNoAllocMaxDevices] = {};
+bool g_noallocTried[kNoAllocMaxDevices] = {};
+
+// Resolves this GPU's copy of the scratch array; called by b2g_create with
+// the device current, once per device.
+void noallocInit(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_noallocMutex);
+    if (g_noallocTried[device]) return;
+    g_noallocTried[device] = true;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_noallocBase[device] = p;
+}
+
+void* noallocBase(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_noallocMutex);
+    return g_noallocBase[device];
+}
+
+// Sorted pair keys to (min, max) proxy index pairs.
+__global__ void k_unpack_pairs(const unsigned long long* keys, int n, int32_t* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long k = keys[i];
+    out[2 * i] = (int32_t)(k >> 32);
+    out[2 * i + 1] = (int32_t)(k & 0xFFFFFFFFu);
+}
+
+}  // namespace
+
+extern "C" {
+
+int b2g_pairs_batch_noalloc(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved, int n, int32_t* pairs,
+                            int cap, int* count) {
+    if (!ctx || n < 0 || cap < 0 || (n > 0 && (!fat || !moved)) || (cap > 0 && !pairs)) return B2G_ERR_ARG;
+    if (count) *count = 0;
+    if (n > kNoAllocMaxProxies)
+        return fail(ctx, B2G_ERR_CAPACITY, "b2g_pairs_batch_noalloc takes at most 65536 proxies");
+    if (n == 0) return B2G_OK;
+    unsigned char* base = (unsigned char*)noallocBase(ctx->device);
+    if (!base) return fail(ctx, B2G_ERR_HIP, "no scratch address for this device");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    // Cell size: median extent of up to 1024 evenly spaced proxies (any
+    // cell size gives the same pair set; this one keeps cells near the
+    // typical proxy).
+    float ext[1024];
+    const int ns = std::min(n, 1024);
+    for (int i = 0; i < ns; ++i) {
+        const b2g_aabb& a = fat[(long long)i * n / ns];
+        ext[i] = std::max(a.ux - a.lx, a.uy - a.ly);
+    }
+    std::nth_element(ext, ext + ns / 2, ext + ns);
+    const float cell = std::max(ext[ns / 2], 1e-3f);
+    int nb = 1024;
+    while (nb < 2 * n) nb <<= 1;
+    const size_t sortBytes = prims_scratch_bytes(std::max(std::max(nb + 1, kNoAllocMaxPairs), n));
+    if (sortBytes > kNoAllocSortBytes) return fail(ctx, B2G_ERR_CAPACITY, "sort temporaries exceed the scratch");
+    float4* dfat = (float4*)(base + kNaFat);
+    uint8_t* dmoved = base + kNaMoved;
+    int* counts = (int*)(base + kNaCounts);
+    int* starts = (int*)(base + kNaStarts);
+    int* cursor = (int*)(base + kNaCursor);
+    unsigned long long* keys = (unsigned long long*)(base + kNaKeys);
+    unsigned long long* sorted = (unsigned long long*)(base + kNaSorted);
+    void* sortTmp = base + kNaSort;
+    Counters* dc = (Counters*)(base + kNaCounters);
+    HIPCHK(ctx, hipMemcpyAsync(dfat, fat, sizeof(float4) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dmoved, moved, n, hipMemcpyHostToDevice, s));
+    // what the previous call left: counters and bucket counts start at zero
+    hipLaunchKernelGGL(k_fill<int>, dim3(blocks(sizeof(Counters) / sizeof(int))), dim3(256), 0, s, (int*)dc,
+                       (int)(sizeof(Counters) / sizeof(int)), 0);
+    hipLaunchKernelGGL(k_fill<int>, dim3(blocks(nb + 1)), dim3(256), 0, s, counts, nb + 1, 0);
+    hipLaunchKernelGGL(k_fill<int>, dim3(blocks(nb + 1)), dim3(256), 0, s, cursor, nb + 1, 0);
+    GridView g;
+    g.invCell = 1.0f / cell;
+    g.bucketMask = (uint32_t)(nb - 1);
+    g.counts = counts; g.starts = starts; g.cursor = cursor;
+    g.entries = (int*)(base + kNaEntries);
+    g.ecell = (unsigned long long*)(base + kNaCell);
+    g.big = (unsigned long long*)(base + kNaBig);
+    g.bigSorted = 0;
+    g.range = (int4*)(base + kNaRange);
+    g.fbody = nullptr;
+    g.bworld = nullptr;
+    hipLaunchKernelGGL(k_grid_count, dim3(blocks(n)), dim3(256), 0, s, dfat, n, (const uint8_t*)nullptr, g, dc);
+    LAUNCHCHK(ctx);
+    HIPCHK(ctx, prims_exclusive_sum(sortTmp, kNoAllocSortBytes, counts, starts, nb + 1, s));
+    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(n)), dim3(256), 0, s, n, g);
+    WorldView w;
+    memset(&w, 0, sizeof(w));
+    PairSink sink{keys, kNoAllocMaxPairs};
+    hipLaunchKernelGGL(k_pairs_query, dim3(blocks(n)), dim3(256), 0, s, dfat, dmoved, n, g, w, false,
+                       (const unsigned long long*)nullptr, 0u, sink, dc);
+    hipLaunchKernelGGL(k_pairs_wide, dim3(blocks(n)), dim3(256), 0, s, dfat, dmoved, n, g, w, false,
+                       (const unsigned long long*)nullptr, 0u, sink, dc);
+    LAUNCHCHK(ctx);
+    Counters hc;
+    HIPCHK(ctx, hipMemcpyAsync(&hc, dc, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    const int np = hc.nNewPairs;
+    if (count) *count = np;
+    if (hc.overflow || np > kNoAllocMaxPairs)
+        return fail(ctx, B2G_ERR_CAPACITY, "b2g_pairs_batch_noalloc holds at most 1048576 pairs");
+    const int out = std::min(np, cap);
+    if (out > 0) {
+        HIPCHK(ctx, prims_sort_keys_u64(sortTmp, kNoAllocSortBytes, keys, sorted, np, s));
+        // the unsorted keys are dead: their slots take the unpacked pairs
+        hipLaunchKernelGGL(k_unpack_pairs, dim3(blocks(out)), dim3(256), 0, s, sorted, out, (int32_t*)keys);
+        LAUNCHCHK(ctx);
+        HIPCHK(ctx, hipMemcpyAsync(pairs, keys, sizeof(int32_t) * 2 * (size_t)out, hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+    }
     return B2G_OK;
 }
 

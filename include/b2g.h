@@ -633,6 +633,16 @@ int b2g_world_manifold_batch(b2g_context* ctx, const b2g_manifold* m, const b2g_
  * unique (min, max) proxy index pairs. *count may exceed cap. */
 int b2g_pairs_batch(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved, int n, int32_t* pairs, int cap,
                     int* count);
+/* b2g_pairs_batch without allocation: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the same pair set,
+ * computed in a fixed scratch array that each GPU holds for the life of the
+ * process (about 56 MiB), with at most 65536 proxies and 1048576 pairs per
+ * call (B2G_ERR_CAPACITY beyond). Inputs and outputs are the caller's host
+ * buffers, as for b2g_pairs_batch. Calls that overlap on one GPU share its
+ * scratch and race: serialise them. */
+int b2g_pairs_batch_noalloc(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved, int n, int32_t* pairs,
+                            int cap, int* count);
 /* b2DynamicTree::MoveProxy's fat-AABB rule (Box2D/Collision/b2DynamicTree.cpp:137-181):
  * per proxy, fatOut = fat unchanged and moved = 0 when fat contains tight,
  * else the re-fattened, displacement-extended box and moved = 1. */
