@@ -55,16 +55,17 @@ HBM_PEAK_GBS = 8000.0
 # the counters ran with (the fusion of trailing colours changes which
 # colours the kernel solves); the summary with the commands is under
 # profiles/. Another schedule reports traffic as null.
-DEFAULT_TRAFFIC_BYTES = {(0, 0, 256, 256, 64, 2048): 7.779e6}
+DEFAULT_TRAFFIC_BYTES = {(0, 0, 256, 256, 64, 2048, 0): 7.779e6}
 # The pile's step schedule as the autotuner picks it on this workload (step
 # 500 of a warm-up copy): fixed so the HBM traffic above applies; the batched
 # and stress worlds are tuned on every run.
-PILE_SCHEDULE = "0,0,256,256,64,2048"
+PILE_SCHEDULE = "0,0,256,256,64,2048,0"
 DT = 1.0 / 60.0
 # b2g_schedule: island labelling method, collide order, workgroup sizes of
-# the solver, narrowphase and broad-phase query kernels
+# the solver, narrowphase and broad-phase query kernels, trailing-colour
+# fusion, cluster size of the clustered solve
 SCHEDULE = np.dtype([("islands", "<i4"), ("collideOrder", "<i4"), ("solverBlock", "<i4"), ("collideBlock", "<i4"),
-                     ("queryBlock", "<i4"), ("tailColours", "<i4")])
+                     ("queryBlock", "<i4"), ("tailColours", "<i4"), ("clusterBodies", "<i4")])
 _VEC2 = np.dtype([("x", "<f4"), ("y", "<f4")])
 _MPOINT = np.dtype([("lp", _VEC2), ("ni", "<f4"), ("ti", "<f4"), ("id", "<u4")])
 # b2g_manifold (include/b2g.h), 64 bytes
@@ -787,7 +788,8 @@ def main():
     ap.add_argument("--stress-seed", type=int, default=1)
     ap.add_argument("--no-ccd", action="store_true", help="skip the continuous-collision run")
     ap.add_argument("--schedule", type=str, default=PILE_SCHEDULE,
-                    help="pile step schedule (islands,collideOrder,solverBlock,collideBlock,queryBlock,tailColours), "
+                    help="pile step schedule (islands,collideOrder,solverBlock,collideBlock,queryBlock,tailColours,"
+                         "clusterBodies), "
                          "or 'auto' to autotune it on a separate copy")
     ap.add_argument("--ccd-bodies", type=int, default=10000)
     ap.add_argument("--ccd-steps", type=int, default=500)

@@ -76,6 +76,9 @@ struct Counters {
     float residualSum;     // sum of squared complementarity residuals (k_residual)
     int colourCount[kColourSlots + 3];
     int minSep[kMaxPosIters];
+    int nClustered;        // bodies placed in clusters (clustered solve)
+    int boundaryStart;     // first constraint between clusters (clustered solve)
+    int maxColour;         // highest constraint colour of the step
 };
 
 // Constraint arrays (b2ContactVelocityConstraint / b2ContactPositionConstraint,
@@ -100,6 +103,7 @@ struct ConView {
     float4* lc;       // localCenterA, localCenterB
     float2* rad;      // radiusA radiusB
     int* island;      // island root of the constraint's non-static bodies
+    int2* slot;       // clustered solve: LDS slot of body A, B in the cluster (-1: not staged)
 };
 
 // Revolute joints (b2RevoluteJoint, REF/Dynamics/Joints/b2RevoluteJoint.h),
@@ -847,12 +851,11 @@ __global__ void k_residual(WorldView w, ConView cv, int n, Counters* cnt) {
 }
 
 // b2ContactSolver::WarmStart, REF/Dynamics/Contacts/b2ContactSolver.cpp:254-292.
-__device__ inline void warmStartOne(const WorldView& w, const ConView& cv, int k) {
-    const int2 bi = cv.bodies[k];
+// The body velocities are passed in and returned (w = 0), so the same
+// arithmetic serves global and LDS-staged bodies.
+__device__ inline void warmStartCore(const ConView& cv, int k, int info, float4& va, float4& vbv) {
     const float4 ms = cv.mass[k];
-    const int info = cv.info[k];
     const int pc = info & 3;
-    const float4 va = w.vel[bi.x], vbv = w.vel[bi.y];
     V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
     float wA = va.z, wB = vbv.z;
     const float4 nf = cv.normal[k];
@@ -870,8 +873,17 @@ __device__ inline void warmStartOne(const WorldView& w, const ConView& cv, int k
         wB += ms.w * cross(mk(r.z, r.w), P);
         vB += ms.z * P;
     }
-    if (info & 16) w.vel[bi.x] = make_float4(vA.x, vA.y, wA, 0.0f);
-    if (info & 32) w.vel[bi.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+    va = make_float4(vA.x, vA.y, wA, 0.0f);
+    vbv = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+__device__ inline void warmStartOne(const WorldView& w, const ConView& cv, int k) {
+    const int2 bi = cv.bodies[k];
+    const int info = cv.info[k];
+    float4 va = w.vel[bi.x], vb = w.vel[bi.y];
+    warmStartCore(cv, k, info, va, vb);
+    if (info & 16) w.vel[bi.x] = va;
+    if (info & 32) w.vel[bi.y] = vb;
 }
 
 __global__ void k_warm_start(WorldView w, ConView cv, int begin, int end) {
@@ -889,13 +901,10 @@ __global__ void k_warm_start_serial(WorldView w, ConView cv, int begin, int end)
 // b2ContactSolver::SolveVelocityConstraints, REF/Dynamics/Contacts/b2ContactSolver.cpp:294-597:
 // friction per point clamped to the friction cone, then the normal impulse,
 // by a 2x2 linear complementarity solve (total enumeration) for two points.
-__device__ inline void solveVelocityOne(const WorldView& w, const ConView& cv, int k) {
-    const int2 bi = cv.bodies[k];
+__device__ inline void solveVelocityCore(const ConView& cv, int k, int info, float4& va, float4& vbv) {
     const float4 ms = cv.mass[k];
     const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
-    const int info = cv.info[k];
     const int pc = info & 3;
-    const float4 va = w.vel[bi.x], vbv = w.vel[bi.y];
     V2 vA = mk(va.x, va.y), vB = mk(vbv.x, vbv.y);
     float wA = va.z, wB = vbv.z;
     const float4 nf = cv.normal[k];
@@ -1004,8 +1013,17 @@ __device__ inline void solveVelocityOne(const WorldView& w, const ConView& cv, i
         }
     }
     cv.imp[k] = imp;
-    if (info & 16) w.vel[bi.x] = make_float4(vA.x, vA.y, wA, 0.0f);
-    if (info & 32) w.vel[bi.y] = make_float4(vB.x, vB.y, wB, 0.0f);
+    va = make_float4(vA.x, vA.y, wA, 0.0f);
+    vbv = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+__device__ inline void solveVelocityOne(const WorldView& w, const ConView& cv, int k) {
+    const int2 bi = cv.bodies[k];
+    const int info = cv.info[k];
+    float4 va = w.vel[bi.x], vb = w.vel[bi.y];
+    solveVelocityCore(cv, k, info, va, vb);
+    if (info & 16) w.vel[bi.x] = va;
+    if (info & 32) w.vel[bi.y] = vb;
 }
 
 __global__ void k_solve_velocity(WorldView w, ConView cv, int begin, int end) {
@@ -1100,11 +1118,9 @@ __global__ void k_integrate_positions(WorldView w, float h) {
 // b2ContactSolver::SolvePositionConstraints with b2PositionSolverManifold,
 // REF/Dynamics/Contacts/b2ContactSolver.cpp:614-744. Skips itself when the
 // previous iteration already met the early-exit test of b2Island::Solve.
-__device__ inline float solvePositionOne(const WorldView& w, const ConView& cv, int k) {
-    const int2 bi = cv.bodies[k];
+__device__ inline float solvePositionCore(const ConView& cv, int k, int info, float4& pA, float4& pB) {
     const float4 ms = cv.mass[k];
     const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
-    const int info = cv.info[k];
     const int pc = (info >> 2) & 3;
     const int type = (info >> 8) & 3;
     const float4 lc = cv.lc[k];
@@ -1112,7 +1128,6 @@ __device__ inline float solvePositionOne(const WorldView& w, const ConView& cv, 
     const float4 lp = cv.lp[k];
     const float4 lnp = cv.lnp[k];
     const float2 rad = cv.rad[k];
-    const float4 pA = w.pos[bi.x], pB = w.pos[bi.y];
     V2 cA = mk(pA.x, pA.y), cB = mk(pB.x, pB.y);
     float aA = pA.z, aB = pB.z;
     float minSep = 0.0f;
@@ -1159,8 +1174,18 @@ __device__ inline float solvePositionOne(const WorldView& w, const ConView& cv, 
         cB += mB * P;
         aB += iB * cross(rB, P);
     }
-    if (info & 16) w.pos[bi.x] = make_float4(cA.x, cA.y, aA, 0.0f);
-    if (info & 32) w.pos[bi.y] = make_float4(cB.x, cB.y, aB, 0.0f);
+    pA = make_float4(cA.x, cA.y, aA, 0.0f);
+    pB = make_float4(cB.x, cB.y, aB, 0.0f);
+    return minSep;
+}
+
+__device__ inline float solvePositionOne(const WorldView& w, const ConView& cv, int k) {
+    const int2 bi = cv.bodies[k];
+    const int info = cv.info[k];
+    float4 pA = w.pos[bi.x], pB = w.pos[bi.y];
+    const float minSep = solvePositionCore(cv, k, info, pA, pB);
+    if (info & 16) w.pos[bi.x] = pA;
+    if (info & 32) w.pos[bi.y] = pB;
     return minSep;
 }
 
@@ -1279,6 +1304,224 @@ __global__ void k_solve_position_tail(WorldView w, ConView cv, ColourRanges r, i
     }
     if (lowest < 0) atomicMin(&blockMin, lowest);
     __syncthreads();
+    if (threadIdx.x == 0 && blockMin < 0) atomicMin(&cnt->minSep[it], blockMin);
+}
+
+// ====================================================================
+// Clustered solve. Awake island bodies are sorted by (sub-world, Morton code
+// of their 1 m cell) and cut into clusters of a fixed size. A constraint
+// whose dynamic bodies lie in one cluster is interior: one workgroup per
+// cluster solves its interior constraints with the cluster's body states
+// staged in LDS, colour after colour with a workgroup barrier between
+// colours. The constraints between clusters (the boundary set) are solved
+// after them by colour, as in the unclustered solve. Each constraint is
+// still visited once per pass and no two constraints solved at once share a
+// dynamic body, so a pass remains a Gauss-Seidel sweep in another order.
+// ====================================================================
+constexpr int kBoundaryCluster = 0xFFFFFF;  // cluster field of a boundary constraint's order key
+constexpr int kMaxClusterBodies = 1 << 21;  // body index bits of the cluster sort key
+
+// Sort key per body: (sub-world, Morton code of its cell, body index); bodies
+// outside the islands sort last.
+__device__ inline uint32_t spreadBits15(uint32_t v) {
+    v &= 0x7FFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+__global__ void k_body_cluster_keys(WorldView w, unsigned long long* keys, int2* cluster, Counters* cnt) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nBodies) return;
+    cluster[b] = make_int2(-1, -1);
+    if (w.btype[b] != BT_DYNAMIC || !(w.bflags[b] & BF_ISLAND)) {
+        keys[b] = ~0ull;
+        return;
+    }
+    atomicAdd(&cnt->nClustered, 1);
+    const float4 p = w.pos[b];
+    const int cx = min(max((int)floorf(p.x) + 16384, 0), 32767);
+    const int cy = min(max((int)floorf(p.y) + 16384, 0), 32767);
+    const uint32_t morton = spreadBits15((uint32_t)cx) | (spreadBits15((uint32_t)cy) << 1);
+    const unsigned long long world = w.bworld ? (unsigned long long)(w.bworld[b] & 0xFFF) : 0ull;
+    keys[b] = (world << 51) | ((unsigned long long)morton << 21) | (unsigned long long)b;
+}
+
+// Cluster and LDS slot of each clustered body, and the bodies by rank.
+__global__ void k_body_clusters(const unsigned long long* sorted, int n, int size, const Counters* cnt,
+                                int2* cluster, int* bodies) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n || r >= cnt->nClustered) return;
+    const int b = (int)(sorted[r] & (unsigned long long)(kMaxClusterBodies - 1));
+    cluster[b] = make_int2(r / size, r % size);
+    bodies[r] = b;
+}
+
+// Constraint order key of the clustered solve: (cluster or the boundary set,
+// colour).
+__global__ void k_cons_gather_clustered(WorldView w, const int* flag, const int* idx, const int2* cluster,
+                                        uint32_t* keys, int* vals, Counters* cnt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    if (c == w.nContacts - 1) cnt->nConstraints = idx[c] + flag[c];
+    if (!flag[c]) return;
+    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
+    const bool dA = w.btype[bA] == BT_DYNAMIC, dB = w.btype[bB] == BT_DYNAMIC;
+    const int ca = dA ? cluster[bA].x : -1, cb = dB ? cluster[bB].x : -1;
+    const bool boundary = (dA && ca < 0) || (dB && cb < 0) || (dA && dB && ca != cb);
+    const uint32_t q = boundary ? (uint32_t)kBoundaryCluster : (uint32_t)max(ca, cb);
+    const int colour = w.ccolour[c];
+    atomicMax(&cnt->maxColour, colour);
+    keys[idx[c]] = (q << 7) | (uint32_t)colour;
+    vals[idx[c]] = c;
+}
+
+// start[q * 66 + c]: first constraint of cluster q with colour >= c (q =
+// nClusters: the boundary set); start[q * 66 + 65] ends the cluster. The
+// boundary set's colour sizes and start go to the counters for the host.
+__global__ void k_cluster_ranges(const uint32_t* keys, int n, int nClusters, int* start, Counters* cnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = t / (kColourSlots + 1), c = t % (kColourSlots + 1);
+    if (q > nClusters) return;
+    const uint32_t cl = q == nClusters ? (uint32_t)kBoundaryCluster : (uint32_t)q;
+    auto lower = [&](uint32_t key) {
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (keys[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    const int s0 = lower((cl << 7) | (uint32_t)c);
+    start[t] = s0;
+    if (q == nClusters) {
+        if (c == 0) cnt->boundaryStart = s0;
+        if (c < kColourSlots) cnt->colourCount[c] = lower((cl << 7) | (uint32_t)(c + 1)) - s0;
+    }
+}
+
+struct ClusterView {
+    const int* start;   // per cluster: kColourSlots + 1 constraint offsets
+    const int* bodies;  // clustered bodies by rank
+    const int* count;   // number of clustered bodies (device counter)
+    int size;           // bodies per cluster
+};
+
+// Shared set-up of the cluster kernels: the cluster's body rows (velocities
+// or positions) into LDS and its colour offsets; returns the number of bodies
+// staged, or -1 for a workgroup past the last cluster.
+__device__ inline int stageCluster(const ClusterView& cl, const float4* rows, float4* st, int* rs) {
+    const int base = blockIdx.x * cl.size;
+    const int total = *cl.count;
+    if (base >= total) return -1;
+    const int nb = min(cl.size, total - base);
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) st[i] = rows[cl.bodies[base + i]];
+    for (int i = threadIdx.x; i <= kColourSlots; i += blockDim.x)
+        rs[i] = cl.start[blockIdx.x * (kColourSlots + 1) + i];
+    __syncthreads();
+    return nb;
+}
+
+__device__ inline void unstageCluster(const ClusterView& cl, float4* rows, const float4* st, int nb) {
+    __syncthreads();
+    const int base = blockIdx.x * cl.size;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) rows[cl.bodies[base + i]] = st[i];
+}
+
+// Runs fn(k) over the cluster's interior constraints colour by colour; the
+// overflow colour by lane 0 in order.
+template <typename F>
+__device__ inline void clusterColours(const int* rs, F fn) {
+    for (int c = 0; c < kColourSlots; ++c) {
+        const int b0 = rs[c], b1 = rs[c + 1];
+        if (b1 <= b0) continue;
+        if (c == kOverflowColour) {
+            if (threadIdx.x == 0)
+                for (int k = b0; k < b1; ++k) fn(k);
+        } else {
+            for (int k = b0 + threadIdx.x; k < b1; k += blockDim.x) fn(k);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ inline float4 stagedRow(const float4* st, const float4* rows, int slot, int body) {
+    return slot >= 0 ? st[slot] : rows[body];
+}
+
+// LDS slots of each constraint's dynamic bodies (used by interior constraints).
+__global__ void k_cons_slots(ConView cv, int n, const int2* cluster) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int2 bi = cv.bodies[k];
+    const int info = cv.info[k];
+    cv.slot[k] = make_int2((info & 16) ? cluster[bi.x].y : -1, (info & 32) ? cluster[bi.y].y : -1);
+}
+
+__global__ void k_warm_start_cluster(WorldView w, ConView cv, ClusterView cl) {
+    extern __shared__ float4 st[];
+    __shared__ int rs[kColourSlots + 1];
+    const int nb = stageCluster(cl, w.vel, st, rs);
+    if (nb < 0) return;
+    clusterColours(rs, [&](int k) {
+        const int2 bi = cv.bodies[k];
+        const int2 sl = cv.slot[k];
+        const int info = cv.info[k];
+        float4 va = stagedRow(st, w.vel, sl.x, bi.x), vb = stagedRow(st, w.vel, sl.y, bi.y);
+        warmStartCore(cv, k, info, va, vb);
+        if ((info & 16) && sl.x >= 0) st[sl.x] = va;
+        if ((info & 32) && sl.y >= 0) st[sl.y] = vb;
+    });
+    unstageCluster(cl, w.vel, st, nb);
+}
+
+__global__ void k_solve_velocity_cluster(WorldView w, ConView cv, ClusterView cl) {
+    extern __shared__ float4 st[];
+    __shared__ int rs[kColourSlots + 1];
+    const int nb = stageCluster(cl, w.vel, st, rs);
+    if (nb < 0) return;
+    clusterColours(rs, [&](int k) {
+        const int2 bi = cv.bodies[k];
+        const int2 sl = cv.slot[k];
+        const int info = cv.info[k];
+        float4 va = stagedRow(st, w.vel, sl.x, bi.x), vb = stagedRow(st, w.vel, sl.y, bi.y);
+        solveVelocityCore(cv, k, info, va, vb);
+        if ((info & 16) && sl.x >= 0) st[sl.x] = va;
+        if ((info & 32) && sl.y >= 0) st[sl.y] = vb;
+    });
+    unstageCluster(cl, w.vel, st, nb);
+}
+
+// Position pass with the per-island early exit of k_solve_position.
+__global__ void k_solve_position_cluster(WorldView w, ConView cv, ClusterView cl, Counters* cnt, int it) {
+    extern __shared__ float4 st[];
+    __shared__ int rs[kColourSlots + 1];
+    __shared__ int blockMin;
+    if (threadIdx.x == 0) blockMin = 0;
+    const int nb = stageCluster(cl, w.pos, st, rs);
+    if (nb < 0) return;
+    int lowest = 0;
+    clusterColours(rs, [&](int k) {
+        const int root = cv.island[k];
+        if (it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0) return;
+        const int2 bi = cv.bodies[k];
+        const int2 sl = cv.slot[k];
+        const int info = cv.info[k];
+        float4 pA = stagedRow(st, w.pos, sl.x, bi.x), pB = stagedRow(st, w.pos, sl.y, bi.y);
+        const float minSep = solvePositionCore(cv, k, info, pA, pB);
+        if ((info & 16) && sl.x >= 0) st[sl.x] = pA;
+        if ((info & 32) && sl.y >= 0) st[sl.y] = pB;
+        if (minSep < -3.0f * kLinearSlop) {
+            int* f = &w.islandMinSep[it * w.nBodies + root];
+            if (*f == 0) *f = 1;
+        }
+        if (minSep < 0.0f) lowest = min(lowest, encodeOrdered(minSep));
+    });
+    if (lowest < 0) atomicMin(&blockMin, lowest);
+    unstageCluster(cl, w.pos, st, nb);
     if (threadIdx.x == 0 && blockMin < 0) atomicMin(&cnt->minSep[it], blockMin);
 }
 
@@ -4604,7 +4847,7 @@ inline int blocks(int n, int bs = 256) { return (n + bs - 1) / bs; }
 // ====================================================================
 struct b2g_context {
     int device = 0;
-    b2g_schedule sched{0, 0, 256, 256, 256, 1024};
+    b2g_schedule sched{0, 0, 256, 256, 256, 1024, 0};
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -4630,6 +4873,12 @@ struct b2g_context {
     int islandIterCap = 0;
     int* bworld = nullptr;          // sub-world of each body (batched worlds)
     bool subWorlds = false;         // some body is outside sub-world 0
+    // clustered solve: bodies sorted by (sub-world, Morton code of the cell)
+    unsigned long long *bodyKey = nullptr, *bodyKeySorted = nullptr;
+    int2* bodyCluster = nullptr;    // per body: cluster, slot (-1, -1: not clustered)
+    int* clusterBodies = nullptr;   // bodies by rank
+    int* clusterStart = nullptr;    // per cluster (and the boundary set): 66 constraint offsets
+    int clusterStartCap = 0;
 
     // fixture arrays
     int* fbody = nullptr;
@@ -4853,6 +5102,8 @@ void freeArray(T*& p) {
     p = nullptr;
 }
 
+int sizeScratch(b2g_context* ctx);
+
 int growBodies(b2g_context* ctx, int need) {
     if (need <= ctx->capBodies) return B2G_OK;
     const int cap = std::max(need, ctx->capBodies * 2 + 64);
@@ -4885,6 +5136,12 @@ int growBodies(b2g_context* ctx, int need) {
         }
     }
     HIPCHK(ctx, growArray(ctx->islandMinSep, 0, cap * std::max(ctx->islandIterCap, 4), s));
+    HIPCHK(ctx, growArray(ctx->bodyKey, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->bodyKeySorted, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->bodyCluster, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->clusterBodies, 0, cap, s));
+    ctx->clusterStartCap = (cap / 64 + 2) * (kColourSlots + 1);  // clusters of at least 64 bodies, plus the boundary
+    HIPCHK(ctx, growArray(ctx->clusterStart, 0, ctx->clusterStartCap, s));
     ctx->islandIterCap = std::max(ctx->islandIterCap, 4);
     // claims start at zero (tag 0 never wins)
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(cap - n)), dim3(256), 0, s, ctx->claim + n, cap - n,
@@ -4894,14 +5151,14 @@ int growBodies(b2g_context* ctx, int need) {
     hipLaunchKernelGGL(k_fill<int>, dim3(blocks(cap - n)), dim3(256), 0, s, ctx->bworld + n, cap - n, 0);
     LAUNCHCHK(ctx);
     ctx->capBodies = cap;
-    return B2G_OK;
+    return ctx->scratch ? sizeScratch(ctx) : B2G_OK;  // the clustered solve sorts the bodies
 }
 
 int growContacts(b2g_context* ctx, int need);
 
 // Scan/sort scratch for the largest array the step scans or sorts.
 int sizeScratch(b2g_context* ctx) {
-    const int sn = std::max(std::max(ctx->capContacts, ctx->nBuckets + 1), 1024);
+    const int sn = std::max(std::max(std::max(ctx->capContacts, ctx->nBuckets + 1), ctx->capBodies), 1024);
     const size_t need = prims_scratch_bytes(sn);
     if (need > ctx->scratchBytes) {
         freeArray(ctx->scratch);
@@ -4991,6 +5248,7 @@ int growContacts(b2g_context* ctx, int need) {
     HIPCHK(ctx, growArray(c.lc, 0, cap, s));
     HIPCHK(ctx, growArray(c.rad, 0, cap, s));
     HIPCHK(ctx, growArray(c.island, 0, cap, s));
+    HIPCHK(ctx, growArray(c.slot, 0, cap, s));
     int hc = 1024;
     while (hc < 2 * cap) hc <<= 1;
     freeArray(ctx->hashTable);
@@ -5134,13 +5392,14 @@ int readCounters(b2g_context* ctx) {
 enum Phase {
     PH_COLLIDE, PH_COMPACT, PH_COLOUR, PH_ORDER, PH_INTEGRATE_V, PH_INIT, PH_WARM, PH_SOLVE_V, PH_STORE,
     PH_INTEGRATE_X, PH_SOLVE_X, PH_SYNC, PH_BROADPHASE, PH_ISLANDS, PH_LEARNED, PH_K_SOLVE_V, PH_K_SOLVE_X,
-    PH_K_MLP_INFER, PH_K_MLP_GRAD, PH_TOI, PH_COUNT
+    PH_K_MLP_INFER, PH_K_MLP_GRAD, PH_TOI, PH_K_SOLVE_V_CLUSTER, PH_K_SOLVE_X_CLUSTER, PH_COUNT
 };
 const char* kPhaseNames[PH_COUNT] = {"collide", "compact", "colour", "order", "integrate_velocities",
                                      "init_constraints", "warm_start", "solve_velocity", "store_impulses",
                                      "integrate_positions", "solve_position", "sync", "broadphase", "islands",
                                      "fresh_points", "k_solve_velocity", "k_solve_position", "k_mlp_infer",
-                                     "k_mlp_grad", "toi"};
+                                     "k_mlp_grad", "toi", "k_solve_velocity_cluster",
+                                     "k_solve_position_cluster"};
 
 int profStart(b2g_context* ctx) {
     if (!ctx->profiling) return -1;
@@ -5802,28 +6061,52 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
             if (++rounds > 100000) return fail(ctx, B2G_ERR_STATE, "colouring did not converge");
         }
         ctx->stats.colouringRounds = rounds;
-        // Constraint order by colour.
+        // Constraint order by colour; in the clustered solve by (cluster or
+        // the boundary set, colour), and colourStart then holds the boundary
+        // set's colour ranges.
+        const int clusterSize = ctx->sched.clusterBodies;
+        const bool clustered = clusterSize > 0 && ctx->nContacts > 0 && ctx->nBodies < kMaxClusterBodies;
+        const int nClusters = clustered ? (ctx->nBodies + clusterSize - 1) / clusterSize : 0;
         int nCons = 0;
         int colourStart[kColourSlots + 1];
         memset(colourStart, 0, sizeof(colourStart));
         if (ctx->nContacts > 0) {
             const int pe = profStart(ctx);
+            if (clustered) {
+                if ((nClusters + 1) * (kColourSlots + 1) > ctx->clusterStartCap)
+                    return fail(ctx, B2G_ERR_STATE, "cluster table too small");
+                hipLaunchKernelGGL(k_body_cluster_keys, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, ctx->bodyKey,
+                                   ctx->bodyCluster, ctx->cnt);
+                HIPCHK(ctx, prims_sort_keys_u64(ctx->scratch, ctx->scratchBytes, ctx->bodyKey, ctx->bodyKeySorted,
+                                                ctx->nBodies, s));
+                hipLaunchKernelGGL(k_body_clusters, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, ctx->bodyKeySorted,
+                                   ctx->nBodies, clusterSize, ctx->cnt, ctx->bodyCluster, ctx->clusterBodies);
+            }
             hipLaunchKernelGGL(k_cons_flags, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->keep);
             HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->keep, ctx->idx, ctx->nContacts, s));
-            hipLaunchKernelGGL(k_cons_gather, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->keep, ctx->idx,
-                               ctx->consKeys, ctx->consVals, ctx->cnt);
+            if (clustered)
+                hipLaunchKernelGGL(k_cons_gather_clustered, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->keep,
+                                   ctx->idx, ctx->bodyCluster, ctx->consKeys, ctx->consVals, ctx->cnt);
+            else
+                hipLaunchKernelGGL(k_cons_gather, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->keep, ctx->idx,
+                                   ctx->consKeys, ctx->consVals, ctx->cnt);
             LAUNCHCHK(ctx);
             if ((rc = readCounters(ctx)) != B2G_OK) return rc;
             nCons = ctx->hcnt->nConstraints;
             if (nCons > 0) {
                 HIPCHK(ctx, prims_sort_pairs_u32(ctx->scratch, ctx->scratchBytes, ctx->consKeys, ctx->consKeysSorted,
-                                                 ctx->consVals, ctx->consOrder, nCons, 7, s));
-                hipLaunchKernelGGL(k_colour_hist, dim3(blocks(nCons)), dim3(256), 0, s, ctx->consKeysSorted, nCons,
-                                   ctx->cnt);
+                                                 ctx->consVals, ctx->consOrder, nCons, clustered ? 31 : 7, s));
+                if (clustered)
+                    hipLaunchKernelGGL(k_cluster_ranges, dim3(blocks((nClusters + 1) * (kColourSlots + 1))), dim3(256),
+                                       0, s, ctx->consKeysSorted, nCons, nClusters, ctx->clusterStart, ctx->cnt);
+                else
+                    hipLaunchKernelGGL(k_colour_hist, dim3(blocks(nCons)), dim3(256), 0, s, ctx->consKeysSorted, nCons,
+                                       ctx->cnt);
                 profEnd(ctx, PH_ORDER, pe);
                 LAUNCHCHK(ctx);
                 if ((rc = readCounters(ctx)) != B2G_OK) return rc;
             }
+            if (clustered && nCons > 0) colourStart[0] = ctx->hcnt->boundaryStart;
             for (int c = 0; c < kColourSlots; ++c)
                 colourStart[c + 1] = colourStart[c] + (nCons > 0 ? ctx->hcnt->colourCount[c] : 0);
         }
@@ -5833,7 +6116,10 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         memcpy(ctx->lastColourStart, colourStart, sizeof(colourStart));
         ctx->lastConstraints = nCons;
         ctx->stats.constraints = nCons;
-        ctx->stats.colours = nColours;
+        ctx->stats.colours = clustered ? (nCons > 0 ? ctx->hcnt->maxColour + 1 : 0) : nColours;
+        const ClusterView cview{ctx->clusterStart, ctx->clusterBodies, &ctx->cnt->nClustered, clusterSize};
+        const size_t clusterLds = sizeof(float4) * (size_t)std::max(clusterSize, 1);
+        const bool clusterLaunch = clustered && nCons > 0 && colourStart[0] > 0;  // some interior constraint
 
         // Island solve.
         if (ctx->jointsDirty && (rc = recolourJoints(ctx)) != B2G_OK) return rc;
@@ -5850,6 +6136,8 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
             pe = profStart(ctx);
             hipLaunchKernelGGL(k_init_constraints, dim3(blocks(nCons)), dim3(256), 0, s, w, ctx->cv, ctx->consOrder,
                                nCons, dtRatio, ctx->warmStarting ? 1 : 0, ctx->cnt, fv);
+            if (clusterLaunch)
+                hipLaunchKernelGGL(k_cons_slots, dim3(blocks(nCons)), dim3(256), 0, s, ctx->cv, nCons, ctx->bodyCluster);
             profEnd(ctx, PH_INIT, pe);
         }
         LAUNCHCHK(ctx);
@@ -5905,6 +6193,11 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         // b2Island::Solve's start of the velocity phase (REF/Dynamics/b2Island.cpp:246-255):
         // contact warm start, then the joints' InitVelocityConstraints.
         auto startSolve = [&](bool contactWarm) {
+            if (contactWarm && clusterLaunch) {
+                const int pc = profStart(ctx);
+                hipLaunchKernelGGL(k_warm_start_cluster, dim3(nClusters), dim3(sb), clusterLds, s, w, ctx->cv, cview);
+                profEnd(ctx, PH_WARM, pc);
+            }
             if (contactWarm) {
                 perColour(
                     PH_WARM,
@@ -5936,6 +6229,15 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
                     else
                         hipLaunchKernelGGL(k_joint_solve_velocity, sgrid(b1 - b0), dim3(sb), 0, s, w, jv, b0, b1, dt);
                 });
+                if (clusterLaunch) {
+                    const int pc = profStart(ctx);
+                    hipEvent_t e0, e1;
+                    const int pk = profKernelEvents(ctx, &e0, &e1);
+                    hipExtLaunchKernelGGL(k_solve_velocity_cluster, dim3(nClusters), dim3(sb), clusterLds, s, e0, e1, 0,
+                                          w, ctx->cv, cview);
+                    profKernel(ctx, PH_K_SOLVE_V_CLUSTER, pk);
+                    profEnd(ctx, PH_SOLVE_V, pc);
+                }
                 perColour(
                     PH_SOLVE_V,
                     [&](int b0, int b1, bool serial) {
@@ -5982,7 +6284,7 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         // The plain solve (no learned start, trace, log or profiling) runs as
         // a captured graph once its layout has repeated.
         bool graphRan = false;
-        if (ctx->graphs && !ctx->profiling && !learned && ctx->traceIters == 0 && ctx->logCap == 0) {
+        if (ctx->graphs && !ctx->profiling && !learned && ctx->traceIters == 0 && ctx->logCap == 0 && !clustered) {
             rc = solveByGraph(ctx, w, jv, colourStart, nColours, nCons, dt, dtRatio, p->velocityIterations, posIters,
                               &graphRan);
             if (rc != B2G_OK) return rc;
@@ -6045,6 +6347,15 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         LAUNCHCHK(ctx);
         // Each position iteration: contacts, then joints (REF/Dynamics/b2Island.cpp:310-328).
         for (int it = 0; it < posIters && (nCons > 0 || ctx->nJoints > 0); ++it) {
+            if (clusterLaunch) {
+                const int pc = profStart(ctx);
+                hipEvent_t e0, e1;
+                const int pk = profKernelEvents(ctx, &e0, &e1);
+                hipExtLaunchKernelGGL(k_solve_position_cluster, dim3(nClusters), dim3(sb), clusterLds, s, e0, e1, 0, w,
+                                      ctx->cv, cview, ctx->cnt, it);
+                profKernel(ctx, PH_K_SOLVE_X_CLUSTER, pk);
+                profEnd(ctx, PH_SOLVE_X, pc);
+            }
             perColour(
                 PH_SOLVE_X,
                 [&](int b0, int b1, bool serial) {
@@ -6523,6 +6834,8 @@ int b2g_destroy(b2g_context* ctx) {
     freeArray(ctx->bworld);
     freeArray(ctx->islandAwake); freeArray(ctx->islandSleep); freeArray(ctx->islandMinSep);
     freeArray(ctx->islandMax); freeArray(ctx->staticKey);
+    freeArray(ctx->bodyKey); freeArray(ctx->bodyKeySorted); freeArray(ctx->bodyCluster);
+    freeArray(ctx->clusterBodies); freeArray(ctx->clusterStart);
     for (auto& g : ctx->solveGraphs) (void)hipGraphExecDestroy(g.exec);
     ctx->solveGraphs.clear();
     freeArray(ctx->dRanges);
@@ -6543,6 +6856,7 @@ int b2g_destroy(b2g_context* ctx) {
     freeArray(c.contact); freeArray(c.bodies); freeArray(c.mass); freeArray(c.normal); freeArray(c.r0);
     freeArray(c.r1); freeArray(c.pmass); freeArray(c.bias); freeArray(c.imp); freeArray(c.K); freeArray(c.NM);
     freeArray(c.info); freeArray(c.lp); freeArray(c.lnp); freeArray(c.lc); freeArray(c.rad); freeArray(c.island);
+    freeArray(c.slot);
     freeArray(ctx->gCounts); freeArray(ctx->gStarts); freeArray(ctx->gCursor); freeArray(ctx->gEntries);
     freeArray(ctx->gBig); freeArray(ctx->gBigSorted); freeArray(ctx->gCell);
     freeRef(ctx);
@@ -7614,17 +7928,16 @@ int b2g_check_colouring(b2g_context* ctx, int* conflicts, int* maskErrors) {
         HIPCHK(ctx, hipMemcpyAsync(colour.data(), ctx->cur.ccolour, sizeof(int) * ctx->nContacts,
                                    hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
-    // no dynamic body twice in one colour
-    std::vector<int> seen(ctx->nBodies, -1);
+    // no dynamic body twice in one colour (each constraint's colour is its
+    // contact's persistent colour, whatever order the solve used)
     std::vector<unsigned long long> rebuilt(ctx->nBodies, 0ull);
-    for (int c = 0; c < kMaxColours; ++c) {
-        for (int k = ctx->lastColourStart[c]; k < ctx->lastColourStart[c + 1] && k < n; ++k) {
-            for (int b : {bodies[k].x, bodies[k].y}) {
-                if (b < 0 || b >= ctx->nBodies || btype[b] != BT_DYNAMIC) continue;
-                if (seen[b] == c) ++*conflicts;
-                seen[b] = c;
-                rebuilt[b] |= 1ull << c;
-            }
+    for (int k = 0; k < n; ++k) {
+        const int c = colour[contact[k]];
+        if (c < 0 || c >= kMaxColours) continue;
+        for (int b : {bodies[k].x, bodies[k].y}) {
+            if (b < 0 || b >= ctx->nBodies || btype[b] != BT_DYNAMIC) continue;
+            if (rebuilt[b] & (1ull << c)) ++*conflicts;
+            rebuilt[b] |= 1ull << c;
         }
     }
     for (int b = 0; b < ctx->nBodies; ++b)
@@ -7884,7 +8197,9 @@ int b2g_set_schedule(b2g_context* ctx, const b2g_schedule* sc) {
     auto blockOk = [](int b) { return b >= 64 && b <= 1024 && b % 64 == 0; };
     if (sc->islands < 0 || sc->islands > 2 || sc->collideOrder < 0 || sc->collideOrder > 1 ||
         !blockOk(sc->solverBlock) || !blockOk(sc->collideBlock) || !blockOk(sc->queryBlock) ||
-        sc->tailColours < 0 || sc->tailColours > 8192)
+        sc->tailColours < 0 || sc->tailColours > 8192 ||
+        (sc->clusterBodies != 0 &&
+         (sc->clusterBodies < 64 || sc->clusterBodies > 2048 || (sc->clusterBodies & (sc->clusterBodies - 1)) != 0)))
         return fail(ctx, B2G_ERR_ARG, "schedule field out of range");
     ctx->sched = *sc;
     return B2G_OK;

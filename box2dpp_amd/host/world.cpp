@@ -612,7 +612,7 @@ float b2PrismaticJoint::GetJointTranslation() const {
 // last step's axes, from the device), and GetMotorForce.
 b2Vec2 b2PrismaticJoint::GetReactionForce(float inv_dt) const {
     if (m_index >= m_world->m_uploadedJoints) return b2Vec2(0.0f, 0.0f);
-    if (!m_world->IsLocked()) (void)m_world->UploadJointEdits();
+    m_world->DownloadJoints();  // uploads pending setter edits
     const b2g_vec2 f = deviceReaction(m_world->m_ctx, m_index, nullptr);
     return b2Vec2(inv_dt * f.x, inv_dt * f.y);
 }
@@ -680,7 +680,7 @@ b2Vec2 b2WheelJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localA
 // REF/Dynamics/Joints/b2WheelJoint.cpp:341-349
 b2Vec2 b2WheelJoint::GetReactionForce(float inv_dt) const {
     if (m_index >= m_world->m_uploadedJoints) return b2Vec2(0.0f, 0.0f);
-    if (!m_world->IsLocked()) (void)m_world->UploadJointEdits();
+    m_world->DownloadJoints();  // uploads pending setter edits
     const b2g_vec2 f = deviceReaction(m_world->m_ctx, m_index, nullptr);
     return b2Vec2(inv_dt * f.x, inv_dt * f.y);
 }
@@ -728,13 +728,13 @@ b2Vec2 b2GearJoint::GetAnchorB() const { return m_bodyB->GetWorldPoint(m_localAn
 // REF/Dynamics/Joints/b2GearJoint.cpp:406-416, from the device
 b2Vec2 b2GearJoint::GetReactionForce(float inv_dt) const {
     if (m_index >= m_world->m_uploadedJoints) return b2Vec2(0.0f, 0.0f);
-    if (!m_world->IsLocked()) (void)m_world->UploadJointEdits();
+    m_world->DownloadJoints();  // uploads pending setter edits
     const b2g_vec2 f = deviceReaction(m_world->m_ctx, m_index, nullptr);
     return b2Vec2(inv_dt * f.x, inv_dt * f.y);
 }
 float b2GearJoint::GetReactionTorque(float inv_dt) const {
     if (m_index >= m_world->m_uploadedJoints) return 0.0f;
-    if (!m_world->IsLocked()) (void)m_world->UploadJointEdits();
+    m_world->DownloadJoints();  // uploads pending setter edits
     float t = 0.0f;
     (void)deviceReaction(m_world->m_ctx, m_index, &t);
     return inv_dt * t;
@@ -1057,6 +1057,9 @@ void b2World::DestroyJoint(b2Joint* j) {
 }
 
 void b2World::DownloadJoints() const {
+    // setters' edits first, so the impulses read back carry their side
+    // effects (a dropped limit impulse) as the reference's members do at once
+    if (!m_locked && !m_dirtyJoints.empty()) (void)const_cast<b2World*>(this)->UploadJointEdits();
     if (!m_jointsStale || !m_ctx) return;
     m_jointsStale = false;
     const int n = m_uploadedJoints;

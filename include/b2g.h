@@ -673,6 +673,14 @@ typedef struct b2g_schedule {
     int32_t tailColours;   /* trailing colours of at most this many constraints are
                               solved by one 1024-lane workgroup, colour after colour
                               (0: every colour its own launch) */
+    int32_t clusterBodies; /* clustered solve: awake bodies are grouped into clusters of
+                              this many (a power of two, 64-2048) by the Morton order of
+                              their positions; constraints inside a cluster are solved by
+                              one workgroup with the cluster's bodies in LDS, colour after
+                              colour, then the constraints between clusters by colour.
+                              0: off. Unlike the other fields this changes the order in
+                              which a pass visits constraints (each still once per pass),
+                              so results differ from the unclustered solve. */
 } b2g_schedule;
 /* Workgroup sizes are multiples of 64 from 64 to 1024; tailColours 0 to 8192. */
 int b2g_set_schedule(b2g_context* ctx, const b2g_schedule* s);

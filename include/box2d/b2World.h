@@ -870,8 +870,9 @@ private:
     friend class b2GearJoint;
     int32_t Upload();
     // Uploads the setters' edits of joints already on the device (part of
-    // Upload); getters call it so a reaction read after a setter reflects
-    // the setter's side effects, as the reference's members do at once.
+    // Upload); DownloadJoints calls it first, so a reaction read after a
+    // setter reflects the setter's side effects, as the reference's members
+    // do at once.
     int32_t UploadJointEdits();
     void Download() const;
     void DownloadJoints() const;

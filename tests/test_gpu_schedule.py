@@ -1,7 +1,14 @@
 """The step schedule (island labelling method, collide order, workgroup
-sizes) changes how the coloured step runs, not what it computes: worlds
-stepped under different schedules stay identical bit for bit, and the
-autotuner's trial steps are ordinary steps."""
+sizes, trailing-colour fusion) changes how the coloured step runs, not what
+it computes: worlds stepped under different schedules stay identical bit for
+bit, and the autotuner's trial steps are ordinary steps.
+
+The clustered solve (schedule field clusterBodies) changes the order of a
+pass: constraints inside a body cluster first, by one workgroup per cluster
+with the bodies in LDS, then those between clusters. With one cluster holding
+every awake body the order is the unclustered one, and the steps are
+identical bit for bit; with many clusters the steps are reproducible and stay
+within bounds of the unclustered solve."""
 from __future__ import annotations
 
 import ctypes
@@ -13,9 +20,9 @@ import engines as E
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = (0, 0, 256, 256, 256, 0)
-SCHEDULES = [(1, 1, 64, 128, 64, 1024), (2, 1, 512, 64, 128, 8192), (2, 0, 128, 1024, 256, 512),
-             (0, 1, 192, 64, 512, 2048)]
+DEFAULT = (0, 0, 256, 256, 256, 0, 0)
+SCHEDULES = [(1, 1, 64, 128, 64, 1024, 0), (2, 1, 512, 64, 128, 8192, 0), (2, 0, 128, 1024, 256, 512, 0),
+             (0, 1, 192, 64, 512, 2048, 0)]
 
 
 def _world(gpu):
@@ -75,9 +82,10 @@ def test_autotune_steps_are_ordinary_steps(gpu):
 
 def test_set_schedule_rejects_bad_values(gpu):
     w = gpu.world("pyramid", 4, 0)
-    for bad in [(3, 0, 256, 256, 256, 0), (0, 2, 256, 256, 256, 0), (0, 0, 100, 256, 256, 0),
-                (0, 0, 256, 2048, 256, 0), (0, 0, 256, 256, 0, 0), (0, 0, 256, 256, 256, -1),
-                (0, 0, 256, 256, 256, 9000)]:
+    for bad in [(3, 0, 256, 256, 256, 0, 0), (0, 2, 256, 256, 256, 0, 0), (0, 0, 100, 256, 256, 0, 0),
+                (0, 0, 256, 2048, 256, 0, 0), (0, 0, 256, 256, 0, 0, 0), (0, 0, 256, 256, 256, -1, 0),
+                (0, 0, 256, 256, 256, 9000, 0), (0, 0, 256, 256, 256, 0, 32), (0, 0, 256, 256, 256, 0, 100),
+                (0, 0, 256, 256, 256, 0, 4096)]:
         s = np.array([bad], E.SCHEDULE)
         assert w.e.lib.b2pp_world_set_schedule(w.h, E.ptr(s)) != 0
     w.close()
@@ -112,3 +120,45 @@ def test_graph_solve_matches_direct(gpu, scene):
     E.b2g_check(lib, cb, lib.b2g_get_graph_counts(cb, ctypes.byref(launches), None))
     assert launches.value == 0
     a.close(); b.close()
+
+
+def _pen_energy(w):
+    m = w.measures()
+    return m["mean_penetration_m"], m["kinetic_energy_J"]
+
+
+def test_single_cluster_is_the_unclustered_solve(gpu):
+    a = gpu.world("pile", 1200, 9, sleep=False)
+    b = gpu.world("pile", 1200, 9, sleep=False)
+    _set(a, DEFAULT)
+    _set(b, (0, 0, 256, 256, 256, 1024, 2048))
+    for _ in range(6):
+        a.step(50)
+        b.step(50)
+        _same(a, b)
+    assert a.stats()["constraints"] > 1000
+    a.close(); b.close()
+
+
+@pytest.mark.parametrize("size", [64, 256])
+def test_many_clusters(gpu, size):
+    lib = E.b2g_lib()
+    plain, a, b = _world(gpu), _world(gpu), _world(gpu)
+    _set(plain, DEFAULT)
+    _set(a, (0, 0, 256, 256, 256, 1024, size))
+    _set(b, (0, 0, 128, 256, 256, 0, size))   # other workgroup size and no fusion: same order
+    conflicts, mask = ctypes.c_int(-1), ctypes.c_int(-1)
+    for _ in range(4):
+        plain.step(50)
+        a.step(50)
+        b.step(50)
+        _same(a, b)
+        E.b2g_check(lib, a.device_context(), lib.b2g_check_colouring(a.device_context(), ctypes.byref(conflicts),
+                                                                     ctypes.byref(mask)))
+        assert conflicts.value == 0 and mask.value == 0
+    pp, kp = _pen_energy(plain)
+    pa, ka = _pen_energy(a)
+    assert np.isfinite(a.bodies()).all()
+    assert pa <= pp + 0.005, (pa, pp)
+    assert abs(ka / kp - 1.0) <= 0.2, (ka, kp)
+    plain.close(); a.close(); b.close()

@@ -144,3 +144,68 @@ def test_snapshot_errors(gpu):
     finally:
         lib.b2g_destroy(b)
         w.close()
+
+
+@pytest.fixture
+def wide_trig(orclib):
+    orclib.orc_set_trig_mode(1)
+    yield
+    orclib.orc_set_trig_mode(0)
+
+
+def _rows(state):
+    """Oracle body rows (px py s c cx cy a vx vy w awake) from b2g_body_state
+    words: xf at 2-5, sweep c at 11-12, a at 13, v at 14-15, w at 16, the
+    awake flag in bit 1 of the flags word."""
+    awake = ((state[:, 1] & 2) != 0).astype(np.float32).view(np.uint32)
+    return np.concatenate([state[:, 2:6], state[:, 11:14], state[:, 14:17], awake[:, None]], axis=1)
+
+
+@pytest.mark.parametrize("kind,n,seed,sleep,ccd,k,m", [
+    ("pyramid", 20, 0, True, False, 150, 250),   # sleeps after the resume (near step 280)
+    ("pile", 1200, 9, False, False, 100, 100),
+    ("bullets", 24, 1, True, True, 40, 80),      # time-of-impact events after the resume
+])
+def test_resumed_reference_mode_matches_oracle(gpu, orc, wide_trig, kind, n, seed, sleep, ccd, k, m):
+    """Reference-order mode, the counterpart of b2World::Dump
+    (Box2D/Dynamics/b2World.cpp:1271-1328): a snapshot taken after k steps
+    and restored into a new context continues exactly as the oracle that
+    stepped k + m times without a break (bodies after every step, the
+    contact table and manifolds at the end)."""
+    lib = E.b2g_lib()
+    g = gpu.world(kind, n, seed, sleep=sleep, ccd=ccd)
+    g.set_mode(1)
+    o = orc.world(kind, n, seed, sleep=sleep, ccd=ccd)
+    g.step(k)
+    o.step(k)
+    nb = g.e.fn("world_body_count")(g.h)
+    snap = _snapshot(lib, g.device_context())
+    g.close()
+    b = _fresh(lib)
+    p = _params()
+    p[0]["mode"] = 1
+    events = 0
+    try:
+        E.b2g_check(lib, b, lib.b2g_restore(b, E.ptr(snap), ctypes.c_size_t(snap.size)))
+        st = np.zeros(1, E.STATS)
+        for s in range(m):
+            E.b2g_check(lib, b, lib.b2g_step(b, E.ptr(p)))
+            o.step(1)
+            E.b2g_check(lib, b, lib.b2g_get_stats(b, E.ptr(st)))
+            events += int(st[0]["toiEvents"])
+            got = _rows(_bodies(lib, b, nb))
+            want = (o.bodies()[:, :11] + np.float32(0)).view(np.uint32)
+            bad = np.where(~np.all(got == want, axis=1))[0]
+            assert len(bad) == 0, f"step {k + s + 1}: {len(bad)} bodies differ, first {bad[:5]}"
+        meta, man = _state(lib, b)
+        gm, gman = E.contact_table(meta.reshape(-1, 4), man.view(E.MANIFOLD).reshape(-1))
+        om, oman = E.contact_table(*o.contacts())
+        assert np.array_equal(gm[:, :3], om[:, :3])
+        assert np.array_equal(E.manifold_bits(gman), E.manifold_bits(oman))
+        if ccd:
+            assert events > 5, events
+        if sleep and kind == "pyramid":
+            assert not got[:, 10].any()  # the resumed pyramid fell asleep as the oracle's did
+    finally:
+        lib.b2g_destroy(b)
+        o.close()
