@@ -57,6 +57,8 @@ constexpr int kOverflowColour = 64;   // solved serially when masks are full
 constexpr int kColourSlots = 65;
 constexpr int kMaxPosIters = 32;
 constexpr int kMaxCellsPerProxy = 16;
+constexpr int kColourRoundsUnchecked = 2;  // colouring rounds launched before the first readback
+constexpr uint32_t kNotConstraintKey = 0x7Fu;  // order key of a contact outside the constraint set
 
 struct Counters {
     int nContacts;
@@ -65,6 +67,7 @@ struct Counters {
     int remaining;
     int overflow;
     int nBig;
+    int nQueries;          // moved proxies binned in the grid (their queries)
     int beginEvents;
     int endEvents;
     int points;
@@ -525,9 +528,10 @@ __global__ void k_fill(T* p, int n, T v) {
 
 // Open-addressing set of live contact pair keys (the existence test of
 // b2ContactManager::AddPair, REF/Dynamics/b2ContactManager.cpp:196-221).
-__global__ void k_hash_insert(WorldView w, unsigned long long* table, uint32_t mask) {
+// live: the contact count when the host holds only a bound (w.nContacts).
+__global__ void k_hash_insert(WorldView w, unsigned long long* table, uint32_t mask, const Counters* live) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= w.nContacts) return;
+    if (c >= w.nContacts || (live && c >= live->nContacts)) return;
     const unsigned long long key = pairKey(w.cfA[c], w.cfB[c]);
     uint32_t h = hashU64(key) & mask;
     for (;;) {
@@ -550,8 +554,10 @@ __device__ inline unsigned long long claimValue(uint32_t tag, const WorldView& w
     return ((unsigned long long)tag << 48) | (0xFFFFFFFFFFFFull - prio);
 }
 
-__global__ void k_colour_claim(WorldView w, unsigned long long* claim, uint32_t tag) {
+// Also clears the count of losers that k_colour_commit then takes.
+__global__ void k_colour_claim(WorldView w, unsigned long long* claim, uint32_t tag, Counters* cnt) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) cnt->remaining = 0;
     if (c >= w.nContacts) return;
     if (!(w.cflags[c] & CF_CONSTRAINT) || w.ccolour[c] >= 0) return;
     const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
@@ -585,20 +591,15 @@ __global__ void k_colour_commit(WorldView w, const unsigned long long* claim, ui
     w.ccolour[c] = col;
 }
 
-// Gather constraint contacts with their colour as sort key.
-__global__ void k_cons_flags(WorldView w, int* flag) {
+// Order key of every contact: a constraint's colour; contacts outside the
+// constraint set sort after every colour.
+// Also clears the colour counts that k_colour_hist then takes.
+__global__ void k_cons_keys(WorldView w, uint32_t* keys, int* vals, Counters* cnt) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < kColourSlots + 3) cnt->colourCount[c] = 0;
     if (c >= w.nContacts) return;
-    flag[c] = (w.cflags[c] & CF_CONSTRAINT) ? 1 : 0;
-}
-
-__global__ void k_cons_gather(WorldView w, const int* flag, const int* idx, uint32_t* keys, int* vals, Counters* cnt) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= w.nContacts) return;
-    if (c == w.nContacts - 1) cnt->nConstraints = idx[c] + flag[c];
-    if (!flag[c]) return;
-    keys[idx[c]] = (uint32_t)w.ccolour[c];
-    vals[idx[c]] = c;
+    keys[c] = (w.cflags[c] & CF_CONSTRAINT) ? (uint32_t)w.ccolour[c] : kNotConstraintKey;
+    vals[c] = c;
 }
 
 __global__ void k_colour_hist(const uint32_t* keys, int n, Counters* cnt) {
@@ -606,7 +607,7 @@ __global__ void k_colour_hist(const uint32_t* keys, int n, Counters* cnt) {
     for (int i = threadIdx.x; i < kColourSlots; i += blockDim.x) h[i] = 0;
     __syncthreads();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) atomicAdd(&h[keys[k]], 1);
+    if (k < n && keys[k] < (uint32_t)kColourSlots) atomicAdd(&h[keys[k]], 1);
     __syncthreads();
     for (int i = threadIdx.x; i < kColourSlots; i += blockDim.x)
         if (h[i]) atomicAdd(&cnt->colourCount[i], h[i]);
@@ -1359,14 +1360,17 @@ __global__ void k_body_clusters(const unsigned long long* sorted, int n, int siz
     bodies[r] = b;
 }
 
-// Constraint order key of the clustered solve: (cluster or the boundary set,
-// colour).
-__global__ void k_cons_gather_clustered(WorldView w, const int* flag, const int* idx, const int2* cluster,
-                                        uint32_t* keys, int* vals, Counters* cnt) {
+// Order key of the clustered solve: (cluster or the boundary set, colour)
+// for a constraint; contacts outside the constraint set sort last.
+__global__ void k_cons_keys_clustered(WorldView w, const int2* cluster, uint32_t* keys, int* vals, Counters* cnt) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < kColourSlots + 3) cnt->colourCount[c] = 0;
     if (c >= w.nContacts) return;
-    if (c == w.nContacts - 1) cnt->nConstraints = idx[c] + flag[c];
-    if (!flag[c]) return;
+    vals[c] = c;
+    if (!(w.cflags[c] & CF_CONSTRAINT)) {
+        keys[c] = 0x7FFFFFFFu;
+        return;
+    }
     const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
     const bool dA = w.btype[bA] == BT_DYNAMIC, dB = w.btype[bB] == BT_DYNAMIC;
     const int ca = dA ? cluster[bA].x : -1, cb = dB ? cluster[bB].x : -1;
@@ -1374,8 +1378,7 @@ __global__ void k_cons_gather_clustered(WorldView w, const int* flag, const int*
     const uint32_t q = boundary ? (uint32_t)kBoundaryCluster : (uint32_t)max(ca, cb);
     const int colour = w.ccolour[c];
     atomicMax(&cnt->maxColour, colour);
-    keys[idx[c]] = (q << 7) | (uint32_t)colour;
-    vals[idx[c]] = c;
+    keys[c] = (q << 7) | (uint32_t)colour;
 }
 
 // start[q * 66 + c]: first constraint of cluster q with colour >= c (q =
@@ -3546,7 +3549,9 @@ struct GridView {
     int* counts;     // nBuckets + 1
     int* starts;     // exclusive scan of counts
     int* cursor;     // fill cursor per bucket
-    int* entries;    // proxy ids by bucket
+    int* entries;    // proxy ids by bucket; bit 31 set: the proxy moved
+    float4* efat;    // each entry's fat box, next to its id
+    int* queries;    // moved proxies binned in the grid, in no fixed order
     unsigned long long* ecell;  // the cell each entry was binned for (buckets mix cells)
     unsigned long long* big;  // wide proxies: (sub-world << 32) | proxy id, sorted when bigSorted
     int bigSorted;
@@ -3594,45 +3599,67 @@ __device__ inline uint32_t cellHash(int ix, int iy, int world) {
     return ((uint32_t)ix * 0x8DA6B343u) ^ ((uint32_t)iy * 0xD8163841u) ^ ((uint32_t)world * 0xCB1AB31Fu);
 }
 
-__global__ void k_grid_count(const float4* fat, int n, const uint8_t* hasProxy, GridView g, Counters* cnt) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= n) return;
-    if (hasProxy && !hasProxy[f]) {
-        g.range[f] = make_int4(1, 0, 0, 0);
-        return;
-    }
-    const float4 a = fat[f];
-    if (!(a.x <= a.z && a.y <= a.w)) {
-        g.range[f] = make_int4(1, 0, 0, -1);  // removed proxy: not binned, never queried
-        return;
-    }
-    const int x0 = cellCoord(a.x, g.invCell), y0 = cellCoord(a.y, g.invCell);
-    const int x1 = cellCoord(a.z, g.invCell), y1 = cellCoord(a.w, g.invCell);
-    const long long cells = (long long)(x1 - x0 + 1) * (long long)(y1 - y0 + 1);
-    if (cells > kMaxCellsPerProxy) {
-        g.range[f] = make_int4(1, 0, 0, 0);
-        const int slot = atomicAdd(&cnt->nBig, 1);
-        g.big[slot] = ((unsigned long long)proxyWorld(g, f) << 32) | (uint32_t)f;
-        return;
-    }
-    g.range[f] = make_int4(x0, y0, x1, y1);
-    const int wf = proxyWorld(g, f);
-    for (int iy = y0; iy <= y1; ++iy)
-        for (int ix = x0; ix <= x1; ++ix) atomicAdd(&g.counts[cellHash(ix, iy, wf) & g.bucketMask], 1);
+// Slot of this lane in a list that every lane with `want` appends to: one
+// atomic per wavefront, lanes in lane order.
+__device__ inline int waveAppend(int* counter, bool want) {
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0ull) return -1;
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    return want ? base + __popcll(mask & ((1ull << lane) - 1ull)) : -1;
 }
 
-__global__ void k_grid_scatter(int n, GridView g) {
+__global__ void k_grid_count(const float4* fat, const uint8_t* moved, int n, const uint8_t* hasProxy, GridView g,
+                             Counters* cnt) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    bool query = false;
+    if (f < n) {
+        if (hasProxy && !hasProxy[f]) {
+            g.range[f] = make_int4(1, 0, 0, 0);
+        } else {
+            const float4 a = fat[f];
+            if (!(a.x <= a.z && a.y <= a.w)) {
+                g.range[f] = make_int4(1, 0, 0, -1);  // removed proxy: not binned, never queried
+            } else {
+                const int x0 = cellCoord(a.x, g.invCell), y0 = cellCoord(a.y, g.invCell);
+                const int x1 = cellCoord(a.z, g.invCell), y1 = cellCoord(a.w, g.invCell);
+                const long long cells = (long long)(x1 - x0 + 1) * (long long)(y1 - y0 + 1);
+                if (cells > kMaxCellsPerProxy) {
+                    g.range[f] = make_int4(1, 0, 0, 0);
+                    const int slot = atomicAdd(&cnt->nBig, 1);
+                    g.big[slot] = ((unsigned long long)proxyWorld(g, f) << 32) | (uint32_t)f;
+                } else {
+                    g.range[f] = make_int4(x0, y0, x1, y1);
+                    query = moved && moved[f];
+                    const int wf = proxyWorld(g, f);
+                    for (int iy = y0; iy <= y1; ++iy)
+                        for (int ix = x0; ix <= x1; ++ix) atomicAdd(&g.counts[cellHash(ix, iy, wf) & g.bucketMask], 1);
+                }
+            }
+        }
+    }
+    const int slot = waveAppend(&cnt->nQueries, query);
+    if (query) g.queries[slot] = f;
+}
+
+__global__ void k_grid_scatter(int n, GridView g, const float4* fat, const uint8_t* moved) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= n) return;
     const int4 r = g.range[f];
     if (r.x > r.z) return;
     const int wf = proxyWorld(g, f);
+    const float4 box = fat[f];
+    const int id = f | (moved[f] ? (int)0x80000000 : 0);
     for (int iy = r.y; iy <= r.w; ++iy)
         for (int ix = r.x; ix <= r.z; ++ix) {
             const uint32_t bkt = cellHash(ix, iy, wf) & g.bucketMask;
-            const int slot = atomicAdd(&g.cursor[bkt], 1);
-            g.entries[g.starts[bkt] + slot] = f;
-            g.ecell[g.starts[bkt] + slot] = cellKey(ix, iy, wf);
+            const int e = g.starts[bkt] + atomicAdd(&g.cursor[bkt], 1);
+            g.entries[e] = id;
+            g.efat[e] = box;
+            g.ecell[e] = cellKey(ix, iy, wf);
         }
 }
 
@@ -3640,10 +3667,12 @@ __global__ void k_grid_scatter(int n, GridView g) {
 // distinct bodies, no existing contact, at least one dynamic body, filter
 // pass, and a contact type for the two shapes.
 __device__ inline bool acceptPair(const WorldView& w, int a, int b, const unsigned long long* table, uint32_t hmask) {
+    // the existing-contact test first: it needs only the proxy ids and
+    // rejects most overlapping pairs of a settled scene
+    if (table && hashContains(table, hmask, pairKey(a, b))) return false;
     const int ba = w.fbody[a], bb = w.fbody[b];
     if (ba == bb) return false;
     if (w.bworld && w.bworld[ba] != w.bworld[bb]) return false;
-    if (table && hashContains(table, hmask, pairKey(a, b))) return false;
     if (jointKeepsApart(w, ba, bb)) return false;
     if (w.btype[ba] != BT_DYNAMIC && w.btype[bb] != BT_DYNAMIC) return false;
     if (!filterPass(w, a, b)) return false;
@@ -3662,19 +3691,27 @@ __device__ inline void emitPair(PairSink sink, Counters* cnt, int a, int b) {
     else atomicOr(&cnt->overflow, 1);
 }
 
-// One thread per proxy; moved proxies query. `world` is null for the bare
-// pair-set computation (no contact filtering).
+// Queries of the moved proxies binned in the grid (the list k_grid_count
+// builds, in no fixed order): kQueryLanes lanes per query, lane k scanning
+// the query's cells k, k + kQueryLanes, ... and a share of the wide proxies.
+// A pair is reported from the one cell holding the corner of the two boxes'
+// intersection, so lanes never report a pair twice. Each candidate's id,
+// move flag and fat box come from its grid entry. `world` is false for the
+// bare pair-set computation (no contact filtering).
+constexpr int kQueryLanes = 16;  // >= kMaxCellsPerProxy, a divisor of 64
+
 __global__ void k_pairs_query(const float4* fat, const uint8_t* moved, int n, GridView g, WorldView w, bool world,
                               const unsigned long long* table, uint32_t hmask, PairSink sink, Counters* cnt) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n || !moved[q]) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / kQueryLanes, lane = t % kQueryLanes;
+    if (i >= n || i >= cnt->nQueries) return;
+    const int q = g.queries[i];
     const float4 fq = fat[q];
     const int4 r = g.range[q];
-    if (r.w < 0 && r.x > r.z) return;  // removed proxy
-    auto consider = [&](int p, bool checkCell, int ix, int iy) {
+    const int nBig = cnt->nBig;
+    auto consider = [&](int p, bool movedP, const float4& fp, bool checkCell, int ix, int iy) {
         if (p == q) return;
-        if (moved[p] && p < q) return;  // reported by p's own query
-        const float4 fp = fat[p];
+        if (movedP && p < q) return;  // reported by p's own query
         if (!overlaps(fq, fp)) return;
         if (checkCell) {
             const int ox = cellCoord(fmaxB(fq.x, fp.x), g.invCell);
@@ -3685,22 +3722,26 @@ __global__ void k_pairs_query(const float4* fat, const uint8_t* moved, int n, Gr
         if (world && !acceptPair(w, a, b, table, hmask)) return;
         emitPair(sink, cnt, a, b);
     };
-    const int nBig = cnt->nBig;
-    if (r.x > r.z) return;  // a wide proxy's own query runs in k_pairs_wide
     const int wq = proxyWorld(g, q);
-    for (int iy = r.y; iy <= r.w; ++iy)
-        for (int ix = r.x; ix <= r.z; ++ix) {
-            const uint32_t bkt = cellHash(ix, iy, wq) & g.bucketMask;
-            const int s0 = g.starts[bkt], s1 = g.starts[bkt + 1];
-            const unsigned long long ck = cellKey(ix, iy, wq);
-            for (int e = s0; e < s1; ++e)
-                if (g.ecell[e] == ck) consider(g.entries[e], true, ix, iy);
+    const int width = r.z - r.x + 1, ncell = width * (r.w - r.y + 1);
+    for (int k = lane; k < ncell; k += kQueryLanes) {
+        const int ix = r.x + k % width, iy = r.y + k / width;
+        const uint32_t bkt = cellHash(ix, iy, wq) & g.bucketMask;
+        const int s0 = g.starts[bkt], s1 = g.starts[bkt + 1];
+        const unsigned long long ck = cellKey(ix, iy, wq);
+        for (int e = s0; e < s1; ++e) {
+            if (g.ecell[e] != ck) continue;
+            const int id = g.entries[e];
+            consider(id & 0x7FFFFFFF, id < 0, g.efat[e], true, ix, iy);
         }
+    }
     int b0, b1;
     bigRange(g, nBig, wq, b0, b1);
-    for (int i = b0; i < b1; ++i) {
-        const unsigned long long e = g.big[i];
-        if ((int)(e >> 32) == wq) consider((int)(e & 0xFFFFFFFFu), false, 0, 0);
+    for (int k = b0 + lane; k < b1; k += kQueryLanes) {
+        const unsigned long long e = g.big[k];
+        if ((int)(e >> 32) != wq) continue;
+        const int p = (int)(e & 0xFFFFFFFFu);
+        consider(p, moved[p] != 0, fat[p], false, 0, 0);
     }
 }
 
@@ -4909,7 +4950,8 @@ struct b2g_context {
 
     // grid
     int nBuckets = 0, capEntries = 0;
-    int *gCounts = nullptr, *gStarts = nullptr, *gCursor = nullptr, *gEntries = nullptr;
+    int *gCounts = nullptr, *gStarts = nullptr, *gCursor = nullptr, *gEntries = nullptr, *gQueries = nullptr;
+    float4* gEfat = nullptr;
     unsigned long long *gBig = nullptr, *gBigSorted = nullptr;
     unsigned long long* gCell = nullptr;
     float cellSize = 2.0f;
@@ -5012,6 +5054,8 @@ struct b2g_context {
     std::vector<double> trace;      // [3][traceIters + 1]
     int traceStepIters = 0;         // iterations of the last traced step
     int lastColourStart[kColourSlots + 1] = {};  // colour ranges of the last coloured step
+    int compactedFrom = -1;  // table length before the last coloured step's compaction (-1: none)
+    int lastColourRounds = 0;  // colouring rounds the last coloured step launched
     int lastConstraints = 0;
 
     // profiling (HIP events around launches)
@@ -5186,7 +5230,8 @@ int growFixtures(b2g_context* ctx, int need) {
     int nb = 4096;
     while (nb < 4 * cap) nb <<= 1;
     freeArray(ctx->gCounts); freeArray(ctx->gStarts); freeArray(ctx->gCursor); freeArray(ctx->gEntries);
-    freeArray(ctx->gBig); freeArray(ctx->gBigSorted); freeArray(ctx->gCell);
+    freeArray(ctx->gBig); freeArray(ctx->gBigSorted); freeArray(ctx->gCell); freeArray(ctx->gEfat);
+    freeArray(ctx->gQueries);
     ctx->nBuckets = nb;
     ctx->capEntries = kMaxCellsPerProxy * cap;
     HIPCHK(ctx, hipMalloc((void**)&ctx->gCounts, sizeof(int) * (nb + 1)));
@@ -5194,6 +5239,8 @@ int growFixtures(b2g_context* ctx, int need) {
     HIPCHK(ctx, hipMalloc((void**)&ctx->gCursor, sizeof(int) * (nb + 1)));
     HIPCHK(ctx, hipMalloc((void**)&ctx->gEntries, sizeof(int) * ctx->capEntries));
     HIPCHK(ctx, hipMalloc((void**)&ctx->gCell, sizeof(unsigned long long) * ctx->capEntries));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->gEfat, sizeof(float4) * ctx->capEntries));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->gQueries, sizeof(int) * cap));
     HIPCHK(ctx, hipMalloc((void**)&ctx->gBig, sizeof(unsigned long long) * cap));
     HIPCHK(ctx, hipMalloc((void**)&ctx->gBigSorted, sizeof(unsigned long long) * cap));
     const int rc = growContacts(ctx, 8 * cap + 4096);
@@ -5285,7 +5332,7 @@ void chooseCellSize(b2g_context* ctx, const b2g_fixture_def* defs, int n, const 
     }
 }
 
-int rebuildHash(b2g_context* ctx);
+int rebuildHash(b2g_context* ctx, bool liveOnDevice = false);
 
 // Keeps a third of the contact capacity free ahead of the step.
 int reserveContacts(b2g_context* ctx) {
@@ -5296,15 +5343,28 @@ int reserveContacts(b2g_context* ctx) {
 
 // Runs the broad-phase over the current fat boxes and appends the new
 // contacts. Requires the hash of live contacts to be current.
-int runBroadphase(b2g_context* ctx) {
+int readCounters(b2g_context* ctx);
+
+// liveOnDevice: the contact table was compacted without a readback
+// (compactContacts); the query's readback brings the live count.
+int runBroadphase(b2g_context* ctx, bool liveOnDevice = false) {
     hipStream_t s = ctx->stream;
     const int nf = ctx->nFixtures;
-    if (nf == 0) return B2G_OK;
+    if (nf == 0) {
+        if (liveOnDevice) {
+            const int rc = readCounters(ctx);
+            if (rc != B2G_OK) return rc;
+            ctx->nContacts = ctx->hcnt->nContacts;
+        }
+        return B2G_OK;
+    }
     GridView g;
     g.invCell = 1.0f / ctx->cellSize;
     g.bucketMask = (uint32_t)(ctx->nBuckets - 1);
     g.counts = ctx->gCounts; g.starts = ctx->gStarts; g.cursor = ctx->gCursor; g.entries = ctx->gEntries;
     g.ecell = ctx->gCell;
+    g.efat = ctx->gEfat;
+    g.queries = ctx->gQueries;
     g.big = ctx->gBig; g.range = ctx->range;
     g.bigSorted = 0;
     g.fbody = ctx->fbody;
@@ -5313,13 +5373,13 @@ int runBroadphase(b2g_context* ctx) {
                        ctx->nBuckets + 1, 0);
     hipLaunchKernelGGL(k_fill<int>, dim3(blocks(ctx->nBuckets + 1)), dim3(256), 0, s, ctx->gCursor,
                        ctx->nBuckets + 1, 0);
-    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nBig, 1, 0);
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nBig, 2, 0);  // nBig, nQueries
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nNewPairs, 1, 0);
-    hipLaunchKernelGGL(k_grid_count, dim3(blocks(nf)), dim3(256), 0, s, ctx->fat, nf, (const uint8_t*)nullptr, g,
-                       ctx->cnt);
+    hipLaunchKernelGGL(k_grid_count, dim3(blocks(nf)), dim3(256), 0, s, ctx->fat, ctx->moved, nf,
+                       (const uint8_t*)nullptr, g, ctx->cnt);
     LAUNCHCHK(ctx);
     HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->gCounts, ctx->gStarts, ctx->nBuckets + 1, s));
-    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(nf)), dim3(256), 0, s, nf, g);
+    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(nf)), dim3(256), 0, s, nf, g, ctx->fat, ctx->moved);
     if (ctx->subWorlds) {
         // Wide proxies sorted by sub-world, so a query scans its own world's only.
         int nBig = 0;
@@ -5340,13 +5400,15 @@ int runBroadphase(b2g_context* ctx) {
         hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->overflow, 1, 0);
         PairSink sink{ctx->newKeys, ctx->capContacts};
         const int qb = ctx->sched.queryBlock;
-        hipLaunchKernelGGL(k_pairs_query, dim3((nf + qb - 1) / qb), dim3(qb), 0, s, ctx->fat, ctx->moved, nf, g,
+        hipLaunchKernelGGL(k_pairs_query, dim3((unsigned)(((long long)nf * kQueryLanes + qb - 1) / qb)), dim3(qb), 0, s,
+                           ctx->fat, ctx->moved, nf, g,
                            ctx->view(), true, ctx->hashTable, ctx->hashMask, sink, ctx->cnt);
         hipLaunchKernelGGL(k_pairs_wide, dim3(blocks(nf)), dim3(256), 0, s, ctx->fat, ctx->moved, nf, g, ctx->view(),
                            true, ctx->hashTable, ctx->hashMask, sink, ctx->cnt);
         LAUNCHCHK(ctx);
         HIPCHK(ctx, hipMemcpyAsync(ctx->hcnt, ctx->cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
         HIPCHK(ctx, hipStreamSynchronize(s));
+        if (liveOnDevice && attempt == 0) ctx->nContacts = ctx->hcnt->nContacts;
         nNew = ctx->hcnt->nNewPairs;
         if (!ctx->hcnt->overflow && ctx->nContacts + nNew <= ctx->capContacts) break;
         if (attempt >= 2) return fail(ctx, B2G_ERR_CAPACITY, "contact capacity exceeded in the broad-phase");
@@ -5367,16 +5429,20 @@ int runBroadphase(b2g_context* ctx) {
     return B2G_OK;
 }
 
-int rebuildHash(b2g_context* ctx) {
+// liveOnDevice: ctx->nContacts only bounds the table; the live count is the
+// device's (set by k_compact).
+int rebuildHash(b2g_context* ctx, bool liveOnDevice) {
     hipStream_t s = ctx->stream;
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(ctx->hashCap)), dim3(256), 0, s, ctx->hashTable,
                        ctx->hashCap, ~0ull);
     if (ctx->nContacts > 0)
         hipLaunchKernelGGL(k_hash_insert, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), ctx->hashTable,
-                           ctx->hashMask);
+                           ctx->hashMask, liveOnDevice ? (const Counters*)ctx->cnt : (const Counters*)nullptr);
     LAUNCHCHK(ctx);
     return B2G_OK;
 }
+
+
 
 int readCounters(b2g_context* ctx) {
     HIPCHK(ctx, hipMemcpyAsync(ctx->hcnt, ctx->cnt, sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
@@ -5454,6 +5520,28 @@ void profCollect(b2g_context* ctx) {
     }
     ctx->evMarks.clear();
     ctx->evUsed = 0;
+}
+
+// Removes the contacts flagged dead, keeping the others in table order; the
+// live count is left in the device counters (no readback).
+int compactContacts(b2g_context* ctx) {
+    hipStream_t s = ctx->stream;
+    if (ctx->nContacts == 0) {
+        ctx->compactedFrom = 0;
+        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nContacts, 1, 0);
+        LAUNCHCHK(ctx);
+        return B2G_OK;
+    }
+    const int pe = profStart(ctx);
+    ctx->compactedFrom = ctx->nContacts;  // idx[] maps the step's contact indices to the compacted table
+    hipLaunchKernelGGL(k_mark_dead, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), ctx->keep);
+    HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->keep, ctx->idx, ctx->nContacts, s));
+    hipLaunchKernelGGL(k_compact, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->cur, ctx->alt, ctx->keep, ctx->idx,
+                       ctx->nContacts, ctx->cnt);
+    profEnd(ctx, PH_COMPACT, pe);
+    LAUNCHCHK(ctx);
+    std::swap(ctx->cur, ctx->alt);
+    return B2G_OK;
 }
 
 // ---------------------------------------------------------------- joints
@@ -5906,11 +5994,17 @@ int labelIslands(b2g_context* ctx, const WorldView& w) {
     const int method = ctx->sched.islands;
     if (nc > 0 && method == 0) {
         for (int pass = 0;; ++pass) {
-            hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->islandChanged, 1, 0);
-            for (int r = 0; r < 2; ++r) {
-                hipLaunchKernelGGL(k_island_hook, dim3(blocks(nc)), dim3(256), 0, s, w, &ctx->cnt->islandChanged);
+            // rounds before the checked one: four on the first pass (most
+            // scenes settle within them), one after; a checked round that
+            // lowers no label means the labels are final
+            const int unchecked = pass == 0 ? 3 : 1;
+            for (int r = 0; r < unchecked; ++r) {
+                hipLaunchKernelGGL(k_island_hook, dim3(blocks(nc)), dim3(256), 0, s, w, (int*)nullptr);
                 hipLaunchKernelGGL(k_island_jump, dim3(blocks(nb)), dim3(256), 0, s, w);
             }
+            hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->islandChanged, 1, 0);
+            hipLaunchKernelGGL(k_island_hook, dim3(blocks(nc)), dim3(256), 0, s, w, &ctx->cnt->islandChanged);
+            hipLaunchKernelGGL(k_island_jump, dim3(blocks(nb)), dim3(256), 0, s, w);
             LAUNCHCHK(ctx);
             HIPCHK(ctx, hipMemcpyAsync(&ctx->hcnt->islandChanged, &ctx->cnt->islandChanged, sizeof(int),
                                        hipMemcpyDeviceToHost, s));
@@ -5997,23 +6091,16 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
     const float inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
     const float dtRatio = ctx->inv_dt0 * dt;
     int rc;
+    // The contacts Collide destroyed stay in the table, flagged dead (every
+    // stage below skips them), until the compaction before the broad-phase;
+    // their colours go back to the bodies now. ctx->nContacts stays the
+    // table's length until the broad-phase reads back the live count.
     if (ctx->nContacts > 0) {
-        int pe = profStart(ctx);
+        const int pe = profStart(ctx);
         hipLaunchKernelGGL(k_mark_dead, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), ctx->keep);
-        LAUNCHCHK(ctx);
-        HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->keep, ctx->idx, ctx->nContacts, s));
-        hipLaunchKernelGGL(k_compact, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->cur, ctx->alt, ctx->keep,
-                           ctx->idx, ctx->nContacts, ctx->cnt);
         profEnd(ctx, PH_COMPACT, pe);
         LAUNCHCHK(ctx);
-        std::swap(ctx->cur, ctx->alt);
-        if ((rc = readCounters(ctx)) != B2G_OK) return rc;
-        ctx->nContacts = ctx->hcnt->nContacts;
-    } else {
-        if ((rc = readCounters(ctx)) != B2G_OK) return rc;
     }
-    ctx->stats.beginEvents = ctx->hcnt->beginEvents;
-    ctx->stats.endEvents = ctx->hcnt->endEvents;
     if (dt > 0.0f && ctx->stepComplete) {
         const int posIters = std::min(p->positionIterations, kMaxPosIters);
         if (posIters > ctx->islandIterCap) {
@@ -6037,12 +6124,12 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
                 hipLaunchKernelGGL(k_mark_constraints, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->cnt);
             profEnd(ctx, PH_ISLANDS, pe);
             LAUNCHCHK(ctx);
-            if ((rc = readCounters(ctx)) != B2G_OK) return rc;
         }
-        // Colour new constraints.
-        int remaining = ctx->nContacts > 0 ? ctx->hcnt->remaining : 0;
+        // Colour new constraints: rounds are launched without a readback
+        // (a round with nothing left to colour changes nothing), and the count
+        // still uncoloured comes back with the order's counts below.
         int rounds = 0;
-        while (remaining > 0) {
+        auto colourRound = [&]() -> int {
             if (ctx->claimTag >= 0xFFFF) {
                 hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(ctx->nBodies)), dim3(256), 0, s,
                                    ctx->claim, ctx->nBodies, 0ull);
@@ -6050,17 +6137,20 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
             }
             const uint32_t tag = ++ctx->claimTag;
             const int pe = profStart(ctx);
-            hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->remaining, 1, 0);
-            hipLaunchKernelGGL(k_colour_claim, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->claim, tag);
+            hipLaunchKernelGGL(k_colour_claim, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->claim, tag,
+                               ctx->cnt);
             hipLaunchKernelGGL(k_colour_commit, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->claim, tag,
                                ctx->cnt);
             profEnd(ctx, PH_COLOUR, pe);
             LAUNCHCHK(ctx);
-            if ((rc = readCounters(ctx)) != B2G_OK) return rc;
-            remaining = ctx->hcnt->remaining;
             if (++rounds > 100000) return fail(ctx, B2G_ERR_STATE, "colouring did not converge");
-        }
-        ctx->stats.colouringRounds = rounds;
+            return B2G_OK;
+        };
+        // as many unchecked rounds as the last step needed (at least two)
+        const int unchecked = std::min(std::max(ctx->lastColourRounds, kColourRoundsUnchecked), 16);
+        if (ctx->nContacts > 0)
+            for (int r = 0; r < unchecked; ++r)
+                if ((rc = colourRound()) != B2G_OK) return rc;
         // Constraint order by colour; in the clustered solve by (cluster or
         // the boundary set, colour), and colourStart then holds the boundary
         // set's colour ranges.
@@ -6070,46 +6160,55 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         int nCons = 0;
         int colourStart[kColourSlots + 1];
         memset(colourStart, 0, sizeof(colourStart));
-        if (ctx->nContacts > 0) {
+        // Every contact gets an order key (constraints by colour, or by
+        // cluster and colour; the others after them) and the table is
+        // sorted; one readback brings the counts and what the colouring left.
+        auto order = [&]() -> int {
             const int pe = profStart(ctx);
             if (clustered) {
                 if ((nClusters + 1) * (kColourSlots + 1) > ctx->clusterStartCap)
                     return fail(ctx, B2G_ERR_STATE, "cluster table too small");
+                hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nClustered, 1, 0);
                 hipLaunchKernelGGL(k_body_cluster_keys, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, ctx->bodyKey,
                                    ctx->bodyCluster, ctx->cnt);
                 HIPCHK(ctx, prims_sort_keys_u64(ctx->scratch, ctx->scratchBytes, ctx->bodyKey, ctx->bodyKeySorted,
                                                 ctx->nBodies, s));
                 hipLaunchKernelGGL(k_body_clusters, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, ctx->bodyKeySorted,
                                    ctx->nBodies, clusterSize, ctx->cnt, ctx->bodyCluster, ctx->clusterBodies);
+                hipLaunchKernelGGL(k_cons_keys_clustered, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w,
+                                   ctx->bodyCluster, ctx->consKeys, ctx->consVals, ctx->cnt);
+            } else {
+                hipLaunchKernelGGL(k_cons_keys, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->consKeys,
+                                   ctx->consVals, ctx->cnt);
             }
-            hipLaunchKernelGGL(k_cons_flags, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->keep);
-            HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->keep, ctx->idx, ctx->nContacts, s));
+            HIPCHK(ctx, prims_sort_pairs_u32(ctx->scratch, ctx->scratchBytes, ctx->consKeys, ctx->consKeysSorted,
+                                             ctx->consVals, ctx->consOrder, ctx->nContacts, clustered ? 31 : 7, s));
             if (clustered)
-                hipLaunchKernelGGL(k_cons_gather_clustered, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->keep,
-                                   ctx->idx, ctx->bodyCluster, ctx->consKeys, ctx->consVals, ctx->cnt);
+                hipLaunchKernelGGL(k_cluster_ranges, dim3(blocks((nClusters + 1) * (kColourSlots + 1))), dim3(256), 0,
+                                   s, ctx->consKeysSorted, ctx->nContacts, nClusters, ctx->clusterStart, ctx->cnt);
             else
-                hipLaunchKernelGGL(k_cons_gather, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->keep, ctx->idx,
-                                   ctx->consKeys, ctx->consVals, ctx->cnt);
+                hipLaunchKernelGGL(k_colour_hist, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->consKeysSorted,
+                                   ctx->nContacts, ctx->cnt);
+            profEnd(ctx, PH_ORDER, pe);
             LAUNCHCHK(ctx);
-            if ((rc = readCounters(ctx)) != B2G_OK) return rc;
-            nCons = ctx->hcnt->nConstraints;
-            if (nCons > 0) {
-                HIPCHK(ctx, prims_sort_pairs_u32(ctx->scratch, ctx->scratchBytes, ctx->consKeys, ctx->consKeysSorted,
-                                                 ctx->consVals, ctx->consOrder, nCons, clustered ? 31 : 7, s));
-                if (clustered)
-                    hipLaunchKernelGGL(k_cluster_ranges, dim3(blocks((nClusters + 1) * (kColourSlots + 1))), dim3(256),
-                                       0, s, ctx->consKeysSorted, nCons, nClusters, ctx->clusterStart, ctx->cnt);
-                else
-                    hipLaunchKernelGGL(k_colour_hist, dim3(blocks(nCons)), dim3(256), 0, s, ctx->consKeysSorted, nCons,
-                                       ctx->cnt);
-                profEnd(ctx, PH_ORDER, pe);
-                LAUNCHCHK(ctx);
-                if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+            return readCounters(ctx);
+        };
+        if (ctx->nContacts > 0) {
+            if ((rc = order()) != B2G_OK) return rc;
+            if (ctx->hcnt->remaining > 0) {
+                // the colouring needed more rounds: finish it, then order again
+                while (ctx->hcnt->remaining > 0) {
+                    if ((rc = colourRound()) != B2G_OK) return rc;
+                    if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+                }
+                if ((rc = order()) != B2G_OK) return rc;
             }
-            if (clustered && nCons > 0) colourStart[0] = ctx->hcnt->boundaryStart;
-            for (int c = 0; c < kColourSlots; ++c)
-                colourStart[c + 1] = colourStart[c] + (nCons > 0 ? ctx->hcnt->colourCount[c] : 0);
+            if (clustered) colourStart[0] = ctx->hcnt->boundaryStart;
+            for (int c = 0; c < kColourSlots; ++c) colourStart[c + 1] = colourStart[c] + ctx->hcnt->colourCount[c];
+            nCons = colourStart[kColourSlots];
         }
+        ctx->stats.colouringRounds = rounds;
+        ctx->lastColourRounds = rounds;
         int nColours = 0;
         for (int c = 0; c < kColourSlots; ++c)
             if (colourStart[c + 1] > colourStart[c]) nColours = c + 1;
@@ -6408,10 +6507,12 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         hipLaunchKernelGGL(k_sync_fixtures, dim3(blocks(ctx->nFixtures)), dim3(256), 0, s, w);
         profEnd(ctx, PH_SYNC, pe);
         LAUNCHCHK(ctx);
-        // Broad-phase against the compacted contact set.
+        // Broad-phase against the compacted contact set (its readback brings
+        // the live count).
+        if ((rc = compactContacts(ctx)) != B2G_OK) return rc;
         pe = profStart(ctx);
-        if ((rc = rebuildHash(ctx)) != B2G_OK) return rc;
-        if ((rc = runBroadphase(ctx)) != B2G_OK) return rc;
+        if ((rc = rebuildHash(ctx, true)) != B2G_OK) return rc;
+        if ((rc = runBroadphase(ctx, true)) != B2G_OK) return rc;
         profEnd(ctx, PH_BROADPHASE, pe);
         // Stats of the position iterations.
         int ran = 0;
@@ -6424,7 +6525,13 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         ctx->stats.positionIterations = ran;
         ctx->stats.points = ctx->hcnt->points;
         ctx->stats.minSeparation = lastMin;
+    } else {
+        if ((rc = compactContacts(ctx)) != B2G_OK) return rc;
+        if ((rc = readCounters(ctx)) != B2G_OK) return rc;
+        ctx->nContacts = ctx->hcnt->nContacts;
     }
+    ctx->stats.beginEvents = ctx->hcnt->beginEvents;
+    ctx->stats.endEvents = ctx->hcnt->endEvents;
     if (ctx->continuous && dt > 0.0f && (rc = solveTOI(ctx, p)) != B2G_OK) return rc;
     if (dt > 0.0f) ctx->inv_dt0 = inv_dt;
     if (ctx->autoClearForces && ctx->nBodies > 0)
@@ -6858,7 +6965,8 @@ int b2g_destroy(b2g_context* ctx) {
     freeArray(c.info); freeArray(c.lp); freeArray(c.lnp); freeArray(c.lc); freeArray(c.rad); freeArray(c.island);
     freeArray(c.slot);
     freeArray(ctx->gCounts); freeArray(ctx->gStarts); freeArray(ctx->gCursor); freeArray(ctx->gEntries);
-    freeArray(ctx->gBig); freeArray(ctx->gBigSorted); freeArray(ctx->gCell);
+    freeArray(ctx->gBig); freeArray(ctx->gBigSorted); freeArray(ctx->gCell); freeArray(ctx->gEfat);
+    freeArray(ctx->gQueries);
     freeRef(ctx);
     freeArray(ctx->freshMask); freeArray(ctx->freshNum); freeArray(ctx->freshOff); freeArray(ctx->freshAll);
     freeArray(ctx->freshFeat); freeArray(ctx->freshRef); freeArray(ctx->impSnap); freeArray(ctx->impAlt);
@@ -7465,6 +7573,8 @@ int b2g_pairs_batch(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved,
     DevBuf<int> counts, starts, cursor, entries;
     DevBuf<unsigned long long> big;
     DevBuf<unsigned long long> ecell;
+    DevBuf<float4> efat;
+    DevBuf<int> queries;
     DevBuf<int4> range;
     DevBuf<unsigned long long> keys, sorted;
     DevBuf<char> scratch;
@@ -7474,6 +7584,8 @@ int b2g_pairs_batch(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved,
     HIPCHK(ctx, starts.alloc(nb + 1)); HIPCHK(ctx, cursor.alloc(nb + 1));
     HIPCHK(ctx, entries.alloc((size_t)kMaxCellsPerProxy * n)); HIPCHK(ctx, big.alloc(n)); HIPCHK(ctx, range.alloc(n));
     HIPCHK(ctx, ecell.alloc((size_t)kMaxCellsPerProxy * n));
+    HIPCHK(ctx, efat.alloc((size_t)kMaxCellsPerProxy * n));
+    HIPCHK(ctx, queries.alloc(n));
     HIPCHK(ctx, keys.alloc(capPairs)); HIPCHK(ctx, sorted.alloc(capPairs)); HIPCHK(ctx, scratch.alloc(sb));
     HIPCHK(ctx, dc.alloc(1));
     HIPCHK(ctx, hipMemcpyAsync(dfat.p, fat, sizeof(float4) * n, hipMemcpyHostToDevice, s));
@@ -7487,17 +7599,20 @@ int b2g_pairs_batch(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved,
     g.counts = counts.p; g.starts = starts.p; g.cursor = cursor.p; g.entries = entries.p; g.big = big.p;
     g.bigSorted = 0;
     g.ecell = ecell.p;
+    g.efat = efat.p;
+    g.queries = queries.p;
     g.range = range.p;
     g.fbody = nullptr;
     g.bworld = nullptr;
-    hipLaunchKernelGGL(k_grid_count, dim3(blocks(n)), dim3(256), 0, s, dfat.p, n, (const uint8_t*)nullptr, g, dc.p);
+    hipLaunchKernelGGL(k_grid_count, dim3(blocks(n)), dim3(256), 0, s, dfat.p, dmoved.p, n, (const uint8_t*)nullptr, g,
+                       dc.p);
     LAUNCHCHK(ctx);
     HIPCHK(ctx, prims_exclusive_sum(scratch.p, sb, counts.p, starts.p, nb + 1, s));
-    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(n)), dim3(256), 0, s, n, g);
+    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(n)), dim3(256), 0, s, n, g, dfat.p, dmoved.p);
     WorldView w;
     memset(&w, 0, sizeof(w));
     PairSink sink{keys.p, capPairs};
-    hipLaunchKernelGGL(k_pairs_query, dim3(blocks(n)), dim3(256), 0, s, dfat.p, dmoved.p, n, g, w, false,
+    hipLaunchKernelGGL(k_pairs_query, dim3(blocks(n * kQueryLanes)), dim3(256), 0, s, dfat.p, dmoved.p, n, g, w, false,
                        (const unsigned long long*)nullptr, 0u, sink, dc.p);
     hipLaunchKernelGGL(k_pairs_wide, dim3(blocks(n)), dim3(256), 0, s, dfat.p, dmoved.p, n, g, w, false,
                        (const unsigned long long*)nullptr, 0u, sink, dc.p);
@@ -7915,9 +8030,14 @@ int b2g_check_colouring(b2g_context* ctx, int* conflicts, int* maskErrors) {
     std::vector<int2> bodies(n);
     std::vector<int> contact(n), btype(ctx->nBodies), colour(ctx->nContacts);
     std::vector<unsigned long long> mask(ctx->nBodies);
+    // the constraints index the table as it was during the solve; the
+    // compaction after it moved contact c to idx[c]
+    std::vector<int> moved(std::max(ctx->compactedFrom, 0));
     if (n > 0) {
         HIPCHK(ctx, hipMemcpyAsync(bodies.data(), ctx->cv.bodies, sizeof(int2) * n, hipMemcpyDeviceToHost, s));
         HIPCHK(ctx, hipMemcpyAsync(contact.data(), ctx->cv.contact, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+        if (!moved.empty())
+            HIPCHK(ctx, hipMemcpyAsync(moved.data(), ctx->idx, sizeof(int) * moved.size(), hipMemcpyDeviceToHost, s));
     }
     if (ctx->nBodies > 0) {
         HIPCHK(ctx, hipMemcpyAsync(btype.data(), ctx->btype, sizeof(int) * ctx->nBodies, hipMemcpyDeviceToHost, s));
@@ -7932,7 +8052,9 @@ int b2g_check_colouring(b2g_context* ctx, int* conflicts, int* maskErrors) {
     // contact's persistent colour, whatever order the solve used)
     std::vector<unsigned long long> rebuilt(ctx->nBodies, 0ull);
     for (int k = 0; k < n; ++k) {
-        const int c = colour[contact[k]];
+        const int ck = moved.empty() ? contact[k] : moved[contact[k]];
+        if (ck < 0 || ck >= ctx->nContacts) continue;
+        const int c = colour[ck];
         if (c < 0 || c >= kMaxColours) continue;
         for (int b : {bodies[k].x, bodies[k].y}) {
             if (b < 0 || b >= ctx->nBodies || btype[b] != BT_DYNAMIC) continue;
@@ -8763,7 +8885,9 @@ constexpr size_t kNaStarts = kNaCounts + noallocAlign(sizeof(int) * (kNoAllocMax
 constexpr size_t kNaCursor = kNaStarts + noallocAlign(sizeof(int) * (kNoAllocMaxBuckets + 1));
 constexpr size_t kNaEntries = kNaCursor + noallocAlign(sizeof(int) * (kNoAllocMaxBuckets + 1));
 constexpr size_t kNaCell = kNaEntries + noallocAlign(sizeof(int) * kMaxCellsPerProxy * kNoAllocMaxProxies);
-constexpr size_t kNaBig = kNaCell + noallocAlign(sizeof(unsigned long long) * kMaxCellsPerProxy * kNoAllocMaxProxies);
+constexpr size_t kNaEfat = kNaCell + noallocAlign(sizeof(unsigned long long) * kMaxCellsPerProxy * kNoAllocMaxProxies);
+constexpr size_t kNaQueries = kNaEfat + noallocAlign(sizeof(float4) * kMaxCellsPerProxy * kNoAllocMaxProxies);
+constexpr size_t kNaBig = kNaQueries + noallocAlign(sizeof(int) * kNoAllocMaxProxies);
 constexpr size_t kNaRange = kNaBig + noallocAlign(sizeof(unsigned long long) * kNoAllocMaxProxies);
 constexpr size_t kNaKeys = kNaRange + noallocAlign(sizeof(int4) * kNoAllocMaxProxies);
 constexpr size_t kNaSorted = kNaKeys + noallocAlign(sizeof(unsigned long long) * kNoAllocMaxPairs);
@@ -8773,7 +8897,7 @@ constexpr size_t kNoAllocBytes = kNaCounters + noallocAlign(sizeof(Counters));
 
 }  // namespace
 
-// One copy per GPU, held until the process ends (about 56 MiB). Host code
+// One copy per GPU, held until the process ends (about 72 MiB). Host code
 // never passes this name to a kernel: it uses the device address that
 // hipGetSymbolAddress returned for the GPU (noallocBase).
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
@@ -8863,19 +8987,21 @@ int b2g_pairs_batch_noalloc(b2g_context* ctx, const b2g_aabb* fat, const uint8_t
     g.counts = counts; g.starts = starts; g.cursor = cursor;
     g.entries = (int*)(base + kNaEntries);
     g.ecell = (unsigned long long*)(base + kNaCell);
+    g.efat = (float4*)(base + kNaEfat);
+    g.queries = (int*)(base + kNaQueries);
     g.big = (unsigned long long*)(base + kNaBig);
     g.bigSorted = 0;
     g.range = (int4*)(base + kNaRange);
     g.fbody = nullptr;
     g.bworld = nullptr;
-    hipLaunchKernelGGL(k_grid_count, dim3(blocks(n)), dim3(256), 0, s, dfat, n, (const uint8_t*)nullptr, g, dc);
+    hipLaunchKernelGGL(k_grid_count, dim3(blocks(n)), dim3(256), 0, s, dfat, dmoved, n, (const uint8_t*)nullptr, g, dc);
     LAUNCHCHK(ctx);
     HIPCHK(ctx, prims_exclusive_sum(sortTmp, kNoAllocSortBytes, counts, starts, nb + 1, s));
-    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(n)), dim3(256), 0, s, n, g);
+    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(n)), dim3(256), 0, s, n, g, dfat, dmoved);
     WorldView w;
     memset(&w, 0, sizeof(w));
     PairSink sink{keys, kNoAllocMaxPairs};
-    hipLaunchKernelGGL(k_pairs_query, dim3(blocks(n)), dim3(256), 0, s, dfat, dmoved, n, g, w, false,
+    hipLaunchKernelGGL(k_pairs_query, dim3(blocks(n * kQueryLanes)), dim3(256), 0, s, dfat, dmoved, n, g, w, false,
                        (const unsigned long long*)nullptr, 0u, sink, dc);
     hipLaunchKernelGGL(k_pairs_wide, dim3(blocks(n)), dim3(256), 0, s, dfat, dmoved, n, g, w, false,
                        (const unsigned long long*)nullptr, 0u, sink, dc);

@@ -637,7 +637,7 @@ int b2g_pairs_batch(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved,
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc): the same pair set,
  * computed in a fixed scratch array that each GPU holds for the life of the
- * process (about 56 MiB), with at most 65536 proxies and 1048576 pairs per
+ * process (about 72 MiB), with at most 65536 proxies and 1048576 pairs per
  * call (B2G_ERR_CAPACITY beyond). Inputs and outputs are the caller's host
  * buffers, as for b2g_pairs_batch. Calls that overlap on one GPU share its
  * scratch and race: serialise them. */
