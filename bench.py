@@ -119,7 +119,7 @@ def autotune(lib, h, args, at):
     """Schedule chosen by the autotuner on world h after `at` steps."""
     out = np.zeros(1, SCHEDULE)
     step(lib, h, at, args.vel, args.pos)
-    progress(f"autotuning at step {at} ({19 * args.tune_rounds} steps)")
+    progress(f"autotuning at step {at} ({20 * args.tune_rounds} steps)")
     rc = lib.b2pp_world_autotune(h, ctypes.c_float(DT), args.vel, args.pos, args.tune_rounds,
                                  out.ctypes.data_as(ctypes.c_void_p))
     if rc != 0:

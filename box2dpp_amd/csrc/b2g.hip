@@ -57,7 +57,8 @@ constexpr int kOverflowColour = 64;   // solved serially when masks are full
 constexpr int kColourSlots = 65;
 constexpr int kMaxPosIters = 32;
 constexpr int kMaxCellsPerProxy = 16;
-constexpr int kColourRoundsUnchecked = 2;  // colouring rounds launched before the first readback
+constexpr int kColourRoundsUnchecked = 2;  // fewest colouring rounds launched before the first readback
+constexpr int kColourRoundSlots = 16;      // rounds whose leftovers are recorded (later rounds share the last)
 constexpr uint32_t kNotConstraintKey = 0x7Fu;  // order key of a contact outside the constraint set
 
 struct Counters {
@@ -68,6 +69,8 @@ struct Counters {
     int overflow;
     int nBig;
     int nQueries;          // moved proxies binned in the grid (their queries)
+    int nTiles;            // polygon-polygon contacts deferred to k_collide_tiles
+    int roundLeft[kColourRoundSlots];  // constraints left uncoloured by colouring round r of the step
     int beginEvents;
     int endEvents;
     int points;
@@ -266,47 +269,23 @@ __global__ void k_override_contacts(WorldView w, const int* ids, const int* enab
     w.cmat[c] = mat[i];
 }
 
-__global__ void k_collide(WorldView w, Counters* cnt, const int* order) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= w.nContacts) return;
-    const int c = order ? order[i] : i;
-    int flags = w.cflags[c] & ~CF_UPDATED;
-    const int fA = w.cfA[c], fB = w.cfB[c];
-    const int bA = w.fbody[fA], bB = w.fbody[fB];
-    if (flags & CF_FILTER) {
-        const bool dynAny = w.btype[bA] == BT_DYNAMIC || w.btype[bB] == BT_DYNAMIC;
-        if (!dynAny || jointKeepsApart(w, bA, bB) || !filterPass(w, fA, fB)) {
-            if (flags & CF_TOUCHING) atomicAdd(&cnt->endEvents, 1);
-            if (w.cman[c].pointCount > 0 && !sensorOf(w, fA) && !sensorOf(w, fB)) {
-                wakeBody(w, bA);
-                wakeBody(w, bB);
-            }
-            w.cflags[c] = flags | CF_DEAD;
-            return;
-        }
-        flags &= ~CF_FILTER;
-    }
-    const bool activeA = (w.bflags[bA] & BF_AWAKE) && w.btype[bA] != BT_STATIC;
-    const bool activeB = (w.bflags[bB] & BF_AWAKE) && w.btype[bB] != BT_STATIC;
-    if (!activeA && !activeB) {
-        w.cflags[c] = flags;
-        return;
-    }
-    if (!overlaps(w.fat[fA], w.fat[fB])) {
-        if (flags & CF_TOUCHING) atomicAdd(&cnt->endEvents, 1);
-        if (w.cman[c].pointCount > 0 && !sensorOf(w, fA) && !sensorOf(w, fB)) {
-            wakeBody(w, bA);
-            wakeBody(w, bB);
-        }
-        w.cflags[c] = flags | CF_DEAD;
-        return;
-    }
-    const b2g_manifold old = w.cman[c];
-    b2g_manifold m;
-    zeroManifold(m);
-    const Xf xfA = xfOf(w.xf[bA]);
-    const Xf xfB = xfOf(w.xf[bB]);
-    collideShapes(m, w.shape[fA], xfA, w.shape[fB], xfB);
+// Slot of this lane in a list that every lane with `want` appends to: one
+// atomic per wavefront, lanes in lane order.
+__device__ inline int waveAppend(int* counter, bool want) {
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0ull) return -1;
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    return want ? base + __popcll(mask & ((1ull << lane) - 1ull)) : -1;
+}
+
+// b2Contact::Update after the manifold is known: impulses carried over by
+// feature id, touching flag, wakes and begin/end counts.
+__device__ inline void collideFinish(const WorldView& w, Counters* cnt, int c, int flags, b2g_manifold& m,
+                                     const b2g_manifold& old, int bA, int bB) {
     const bool wasTouching = (flags & CF_TOUCHING) != 0;
     const bool touching = m.pointCount > 0;
     for (int i = 0; i < 2; ++i) {
@@ -334,6 +313,118 @@ __global__ void k_collide(WorldView w, Counters* cnt, const int* order) {
     w.cflags[c] = flags;
 }
 
+// One contact of b2ContactManager::Collide: filtering, the activity test,
+// the fat-box test, then the narrowphase. With `deferPolygons` a
+// polygon-polygon contact stops after the tests (flags written) and returns
+// true: k_collide_tiles takes its narrowphase.
+__device__ inline bool collideOne(const WorldView& w, Counters* cnt, int c, bool deferPolygons) {
+    int flags = w.cflags[c] & ~CF_UPDATED;
+    const int fA = w.cfA[c], fB = w.cfB[c];
+    const int bA = w.fbody[fA], bB = w.fbody[fB];
+    if (flags & CF_FILTER) {
+        const bool dynAny = w.btype[bA] == BT_DYNAMIC || w.btype[bB] == BT_DYNAMIC;
+        if (!dynAny || jointKeepsApart(w, bA, bB) || !filterPass(w, fA, fB)) {
+            if (flags & CF_TOUCHING) atomicAdd(&cnt->endEvents, 1);
+            if (w.cman[c].pointCount > 0 && !sensorOf(w, fA) && !sensorOf(w, fB)) {
+                wakeBody(w, bA);
+                wakeBody(w, bB);
+            }
+            w.cflags[c] = flags | CF_DEAD;
+            return false;
+        }
+        flags &= ~CF_FILTER;
+    }
+    const bool activeA = (w.bflags[bA] & BF_AWAKE) && w.btype[bA] != BT_STATIC;
+    const bool activeB = (w.bflags[bB] & BF_AWAKE) && w.btype[bB] != BT_STATIC;
+    if (!activeA && !activeB) {
+        w.cflags[c] = flags;
+        return false;
+    }
+    if (!overlaps(w.fat[fA], w.fat[fB])) {
+        if (flags & CF_TOUCHING) atomicAdd(&cnt->endEvents, 1);
+        if (w.cman[c].pointCount > 0 && !sensorOf(w, fA) && !sensorOf(w, fB)) {
+            wakeBody(w, bA);
+            wakeBody(w, bB);
+        }
+        w.cflags[c] = flags | CF_DEAD;
+        return false;
+    }
+    if (deferPolygons && w.shape[fA].type == B2G_POLYGON && w.shape[fB].type == B2G_POLYGON) {
+        w.cflags[c] = flags;
+        return true;
+    }
+    const b2g_manifold old = w.cman[c];
+    b2g_manifold m;
+    zeroManifold(m);
+    const Xf xfA = xfOf(w.xf[bA]);
+    const Xf xfB = xfOf(w.xf[bB]);
+    collideShapes(m, w.shape[fA], xfA, w.shape[fB], xfB);
+    collideFinish(w, cnt, c, flags, m, old, bA, bB);
+    return false;
+}
+
+// tiles: null computes every contact here; else polygon-polygon contacts go
+// to that list (count in cnt->nTiles) for k_collide_tiles.
+__global__ void k_collide(WorldView w, Counters* cnt, const int* order, int* tiles) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = -1;
+    bool deferred = false;
+    if (i < w.nContacts) {
+        c = order ? order[i] : i;
+        deferred = collideOne(w, cnt, c, tiles != nullptr);
+    }
+    if (tiles) {
+        const int slot = waveAppend(&cnt->nTiles, deferred);
+        if (deferred) tiles[slot] = c;
+    }
+}
+
+// The shapes of one tile in LDS: kTileLanes lanes stage the two b2g_shape
+// records of their contact with 16-byte loads.
+constexpr int kShapeWords = (int)(sizeof(b2g_shape) / sizeof(float4));
+static_assert(sizeof(b2g_shape) % sizeof(float4) == 0, "b2g_shape is a whole number of float4");
+
+__device__ inline void stageShapes(b2g_shape* dst, const b2g_shape& a, const b2g_shape& b, int lane) {
+    float4* d = (float4*)dst;
+    const float4* sa = (const float4*)&a;
+    const float4* sb = (const float4*)&b;
+    for (int k = lane; k < 2 * kShapeWords; k += kTileLanes) d[k] = k < kShapeWords ? sa[k] : sb[k - kShapeWords];
+}
+
+// Narrowphase of the deferred polygon-polygon contacts: a group of
+// kTileLanes lanes per contact (collidePolygonsTile), the two shapes staged
+// in LDS; the workgroups walk the list in steps of their tile count.
+constexpr int kTileBlock = 64;
+__global__ void __launch_bounds__(kTileBlock) k_collide_tiles(WorldView w, Counters* cnt, const int* tiles) {
+    __shared__ b2g_shape sh[kTileBlock / kTileLanes][2];
+    const int grp = threadIdx.x / kTileLanes, lane = threadIdx.x % kTileLanes;
+    const int perBlock = kTileBlock / kTileLanes;
+    const int n = cnt->nTiles;
+    for (int base = blockIdx.x * perBlock; base < n; base += gridDim.x * perBlock) {
+        const int t = base + grp;
+        int c = -1, fA = 0, fB = 0;
+        if (t < n) {
+            c = tiles[t];
+            fA = w.cfA[c];
+            fB = w.cfB[c];
+            stageShapes(sh[grp], w.shape[fA], w.shape[fB], lane);
+        }
+        __syncthreads();
+        if (c >= 0) {
+            const int bA = w.fbody[fA], bB = w.fbody[fB];
+            const Xf xfA = xfOf(w.xf[bA]);
+            const Xf xfB = xfOf(w.xf[bB]);
+            b2g_manifold m;
+            zeroManifold(m);
+            collidePolygonsTile(m, sh[grp][0], xfA, sh[grp][1], xfB, lane);
+            if (lane == 0) {
+                const b2g_manifold old = w.cman[c];
+                collideFinish(w, cnt, c, w.cflags[c], m, old, bA, bB);
+            }
+        }
+        __syncthreads();
+    }
+}
 // Constraint set of the step: touching, enabled, non-sensor contacts with an
 // awake non-static body (the contacts b2World::Solve's island search adds,
 // REF/Dynamics/b2World.cpp:451-475). Contacts leaving the set give their
@@ -555,9 +646,9 @@ __device__ inline unsigned long long claimValue(uint32_t tag, const WorldView& w
 }
 
 // Also clears the count of losers that k_colour_commit then takes.
-__global__ void k_colour_claim(WorldView w, unsigned long long* claim, uint32_t tag, Counters* cnt) {
+__global__ void k_colour_claim(WorldView w, unsigned long long* claim, uint32_t tag, Counters* cnt, int round) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c == 0) cnt->remaining = 0;
+    if (c == 0) cnt->roundLeft[round] = 0;
     if (c >= w.nContacts) return;
     if (!(w.cflags[c] & CF_CONSTRAINT) || w.ccolour[c] >= 0) return;
     const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
@@ -566,7 +657,7 @@ __global__ void k_colour_claim(WorldView w, unsigned long long* claim, uint32_t 
     if (w.btype[bB] == BT_DYNAMIC) atomicMax(&claim[bB], v);
 }
 
-__global__ void k_colour_commit(WorldView w, const unsigned long long* claim, uint32_t tag, Counters* cnt) {
+__global__ void k_colour_commit(WorldView w, const unsigned long long* claim, uint32_t tag, Counters* cnt, int round) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= w.nContacts) return;
     if (!(w.cflags[c] & CF_CONSTRAINT) || w.ccolour[c] >= 0) return;
@@ -575,7 +666,7 @@ __global__ void k_colour_commit(WorldView w, const unsigned long long* claim, ui
     const unsigned long long v = claimValue(tag, w, c);
     const bool won = (!dA || claim[bA] == v) && (!dB || claim[bB] == v);
     if (!won) {
-        atomicAdd(&cnt->remaining, 1);
+        atomicAdd(&cnt->roundLeft[round], 1);
         return;
     }
     const unsigned long long used = (dA ? w.colourMask[bA] : 0ull) | (dB ? w.colourMask[bB] : 0ull);
@@ -3599,18 +3690,6 @@ __device__ inline uint32_t cellHash(int ix, int iy, int world) {
     return ((uint32_t)ix * 0x8DA6B343u) ^ ((uint32_t)iy * 0xD8163841u) ^ ((uint32_t)world * 0xCB1AB31Fu);
 }
 
-// Slot of this lane in a list that every lane with `want` appends to: one
-// atomic per wavefront, lanes in lane order.
-__device__ inline int waveAppend(int* counter, bool want) {
-    const unsigned long long mask = __ballot(want);
-    if (mask == 0ull) return -1;
-    const int lane = (int)__lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader);
-    return want ? base + __popcll(mask & ((1ull << lane) - 1ull)) : -1;
-}
 
 __global__ void k_grid_count(const float4* fat, const uint8_t* moved, int n, const uint8_t* hasProxy, GridView g,
                              Counters* cnt) {
@@ -4166,9 +4245,31 @@ __global__ void k_collide_batch(const b2g_shape* A, const b2g_transform* xa, con
     Xf xfA, xfB;
     xfA.p = mk(xa[i].px, xa[i].py); xfA.q.s = xa[i].s; xfA.q.c = xa[i].c;
     xfB.p = mk(xb[i].px, xb[i].py); xfB.q.s = xb[i].s; xfB.q.c = xb[i].c;
+    if (A[i].type == B2G_POLYGON && B[i].type == B2G_POLYGON) return;  // k_collide_batch_tiles
     collideShapes(m, A[i], xfA, B[i], xfB);
     if (m.pointCount == 0) zeroManifold(m);
     out[i] = m;
+}
+
+// The polygon pairs of a batch, eight lanes per pair (collidePolygonsTile).
+__global__ void __launch_bounds__(kTileBlock) k_collide_batch_tiles(const b2g_shape* A, const b2g_transform* xa,
+                                                                    const b2g_shape* B, const b2g_transform* xb, int n,
+                                                                    b2g_manifold* out) {
+    __shared__ b2g_shape sh[kTileBlock / kTileLanes][2];
+    const int grp = threadIdx.x / kTileLanes, lane = threadIdx.x % kTileLanes;
+    const int i = blockIdx.x * (kTileBlock / kTileLanes) + grp;
+    const bool mine = i < n && A[i].type == B2G_POLYGON && B[i].type == B2G_POLYGON;
+    if (mine) stageShapes(sh[grp], A[i], B[i], lane);
+    __syncthreads();
+    if (!mine) return;
+    Xf xfA, xfB;
+    xfA.p = mk(xa[i].px, xa[i].py); xfA.q.s = xa[i].s; xfA.q.c = xa[i].c;
+    xfB.p = mk(xb[i].px, xb[i].py); xfB.q.s = xb[i].s; xfB.q.c = xb[i].c;
+    b2g_manifold m;
+    zeroManifold(m);
+    collidePolygonsTile(m, sh[grp][0], xfA, sh[grp][1], xfB, lane);
+    if (m.pointCount == 0) zeroManifold(m);
+    if (lane == 0) out[i] = m;
 }
 
 __global__ void k_world_manifold_batch(const b2g_manifold* m, const b2g_transform* xa, const float* ra,
@@ -4947,6 +5048,7 @@ struct b2g_context {
     uint32_t* consKeysSorted = nullptr;
     int* consVals = nullptr;
     int* consOrder = nullptr;
+    int* tileList = nullptr;   // polygon-polygon contacts of the step's narrowphase
 
     // grid
     int nBuckets = 0, capEntries = 0;
@@ -5055,7 +5157,7 @@ struct b2g_context {
     int traceStepIters = 0;         // iterations of the last traced step
     int lastColourStart[kColourSlots + 1] = {};  // colour ranges of the last coloured step
     int compactedFrom = -1;  // table length before the last coloured step's compaction (-1: none)
-    int lastColourRounds = 0;  // colouring rounds the last coloured step launched
+    int colourRoundsNeeded = 0;  // colouring rounds the last coloured step needed
     int lastConstraints = 0;
 
     // profiling (HIP events around launches)
@@ -5277,6 +5379,7 @@ int growContacts(b2g_context* ctx, int need) {
     HIPCHK(ctx, growArray(ctx->consKeysSorted, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->consVals, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->consOrder, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->tileList, 0, cap, s));
     ConView& c = ctx->cv;
     HIPCHK(ctx, growArray(c.contact, 0, cap, s));
     HIPCHK(ctx, growArray(c.bodies, 0, cap, s));
@@ -6076,8 +6179,18 @@ int stepColouredCollide(b2g_context* ctx, const b2g_step_params* p) {
             order = ctx->consOrder;
         }
         const int cb = ctx->sched.collideBlock;
+        const bool tiles = ctx->sched.collideOrder == 2;
         hipLaunchKernelGGL(k_collide, dim3((ctx->nContacts + cb - 1) / cb), dim3(cb), 0, s, ctx->view(), ctx->cnt,
-                           order);
+                           order, tiles ? ctx->tileList : (int*)nullptr);
+        if (tiles) {
+            // the deferred polygon pairs, eight lanes each: one pass of the grid
+            // covers the last step's count (a longer list takes more passes)
+            const int perBlock = kTileBlock / kTileLanes;
+            const int expect = std::min(ctx->nContacts, ctx->hcnt->nTiles + ctx->hcnt->nTiles / 4 + 1024);
+            const int tileBlocks = std::max((expect + perBlock - 1) / perBlock, 1);
+            hipLaunchKernelGGL(k_collide_tiles, dim3(tileBlocks), dim3(kTileBlock), 0, s, ctx->view(), ctx->cnt,
+                               ctx->tileList);
+        }
         profEnd(ctx, PH_COLLIDE, pe);
         LAUNCHCHK(ctx);
     }
@@ -6136,21 +6249,24 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
                 ctx->claimTag = 0;
             }
             const uint32_t tag = ++ctx->claimTag;
+            const int slot = std::min(rounds, kColourRoundSlots - 1);
             const int pe = profStart(ctx);
             hipLaunchKernelGGL(k_colour_claim, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->claim, tag,
-                               ctx->cnt);
+                               ctx->cnt, slot);
             hipLaunchKernelGGL(k_colour_commit, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->claim, tag,
-                               ctx->cnt);
+                               ctx->cnt, slot);
             profEnd(ctx, PH_COLOUR, pe);
             LAUNCHCHK(ctx);
             if (++rounds > 100000) return fail(ctx, B2G_ERR_STATE, "colouring did not converge");
             return B2G_OK;
         };
-        // as many unchecked rounds as the last step needed (at least two)
-        const int unchecked = std::min(std::max(ctx->lastColourRounds, kColourRoundsUnchecked), 16);
+        // as many unchecked rounds as the last step needed, at least two
+        const int unchecked =
+            std::min(std::max(ctx->colourRoundsNeeded, kColourRoundsUnchecked), kColourRoundSlots - 1);
         if (ctx->nContacts > 0)
             for (int r = 0; r < unchecked; ++r)
                 if ((rc = colourRound()) != B2G_OK) return rc;
+        auto leftAfterLast = [&]() { return ctx->hcnt->roundLeft[std::min(rounds - 1, kColourRoundSlots - 1)]; };
         // Constraint order by colour; in the clustered solve by (cluster or
         // the boundary set, colour), and colourStart then holds the boundary
         // set's colour ranges.
@@ -6195,20 +6311,26 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         };
         if (ctx->nContacts > 0) {
             if ((rc = order()) != B2G_OK) return rc;
-            if (ctx->hcnt->remaining > 0) {
+            if (leftAfterLast() > 0) {
                 // the colouring needed more rounds: finish it, then order again
-                while (ctx->hcnt->remaining > 0) {
+                while (leftAfterLast() > 0) {
                     if ((rc = colourRound()) != B2G_OK) return rc;
                     if ((rc = readCounters(ctx)) != B2G_OK) return rc;
                 }
                 if ((rc = order()) != B2G_OK) return rc;
             }
+            int needed = rounds;
+            for (int r = 0; r < std::min(rounds, kColourRoundSlots); ++r)
+                if (ctx->hcnt->roundLeft[r] == 0) {
+                    needed = r + 1;
+                    break;
+                }
+            ctx->colourRoundsNeeded = needed;
             if (clustered) colourStart[0] = ctx->hcnt->boundaryStart;
             for (int c = 0; c < kColourSlots; ++c) colourStart[c + 1] = colourStart[c] + ctx->hcnt->colourCount[c];
             nCons = colourStart[kColourSlots];
         }
         ctx->stats.colouringRounds = rounds;
-        ctx->lastColourRounds = rounds;
         int nColours = 0;
         for (int c = 0; c < kColourSlots; ++c)
             if (colourStart[c + 1] > colourStart[c]) nColours = c + 1;
@@ -6958,7 +7080,7 @@ int b2g_destroy(b2g_context* ctx) {
     freeToi(ctx);
     freeArray(ctx->keep); freeArray(ctx->idx); freeArray(ctx->hashTable); freeArray(ctx->newKeys);
     freeArray(ctx->newKeysSorted); freeArray(ctx->consKeys); freeArray(ctx->consKeysSorted);
-    freeArray(ctx->consVals); freeArray(ctx->consOrder);
+    freeArray(ctx->consVals); freeArray(ctx->consOrder); freeArray(ctx->tileList);
     ConView& c = ctx->cv;
     freeArray(c.contact); freeArray(c.bodies); freeArray(c.mass); freeArray(c.normal); freeArray(c.r0);
     freeArray(c.r1); freeArray(c.pmass); freeArray(c.bias); freeArray(c.imp); freeArray(c.K); freeArray(c.NM);
@@ -7481,6 +7603,8 @@ int b2g_collide_batch(b2g_context* ctx, const b2g_shape* a, const b2g_transform*
     HIPCHK(ctx, hipMemcpyAsync(dxa.p, xfA, sizeof(b2g_transform) * n, hipMemcpyHostToDevice, s));
     HIPCHK(ctx, hipMemcpyAsync(dxb.p, xfB, sizeof(b2g_transform) * n, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_collide_batch, dim3(blocks(n, 64)), dim3(64), 0, s, dA.p, dxa.p, dB.p, dxb.p, n, dm.p);
+    hipLaunchKernelGGL(k_collide_batch_tiles, dim3((n + kTileBlock / kTileLanes - 1) / (kTileBlock / kTileLanes)),
+                       dim3(kTileBlock), 0, s, dA.p, dxa.p, dB.p, dxb.p, n, dm.p);
     LAUNCHCHK(ctx);
     HIPCHK(ctx, hipMemcpyAsync(out, dm.p, sizeof(b2g_manifold) * n, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
@@ -8317,7 +8441,7 @@ int b2g_get_graph_counts(b2g_context* ctx, int64_t* launches, int64_t* captures)
 int b2g_set_schedule(b2g_context* ctx, const b2g_schedule* sc) {
     if (!ctx || !sc) return B2G_ERR_ARG;
     auto blockOk = [](int b) { return b >= 64 && b <= 1024 && b % 64 == 0; };
-    if (sc->islands < 0 || sc->islands > 2 || sc->collideOrder < 0 || sc->collideOrder > 1 ||
+    if (sc->islands < 0 || sc->islands > 2 || sc->collideOrder < 0 || sc->collideOrder > 2 ||
         !blockOk(sc->solverBlock) || !blockOk(sc->collideBlock) || !blockOk(sc->queryBlock) ||
         sc->tailColours < 0 || sc->tailColours > 8192 ||
         (sc->clusterBodies != 0 &&
@@ -8347,7 +8471,7 @@ int b2g_autotune(b2g_context* ctx, const b2g_step_params* p, int rounds, b2g_sch
     };
     const Knob knobs[] = {
         {&b2g_schedule::islands, {0, 1, 2}, {PH_ISLANDS}},
-        {&b2g_schedule::collideOrder, {0, 1}, {PH_COLLIDE}},
+        {&b2g_schedule::collideOrder, {0, 1, 2}, {PH_COLLIDE}},
         {&b2g_schedule::solverBlock, {64, 128, 256, 512}, {PH_WARM, PH_SOLVE_V, PH_SOLVE_X}},
         {&b2g_schedule::collideBlock, {64, 128, 256}, {PH_COLLIDE}},
         {&b2g_schedule::queryBlock, {64, 128, 256}, {PH_BROADPHASE}},

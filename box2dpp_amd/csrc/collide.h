@@ -59,6 +59,9 @@ __device__ inline float maxSeparation(int* edge, const b2g_shape& p1, const Xf& 
     return maxSep;
 }
 
+__device__ inline void collidePolygonsFaces(b2g_manifold& m, const b2g_shape& A, const Xf& xfA, const b2g_shape& B,
+                                            const Xf& xfB, float sepA, int edgeA, float sepB, int edgeB);
+
 // b2CollidePolygons with b2FindIncidentEdge inlined.
 __device__ inline void collidePolygons(b2g_manifold& m, const b2g_shape& A, const Xf& xfA, const b2g_shape& B,
                                        const Xf& xfB) {
@@ -70,7 +73,14 @@ __device__ inline void collidePolygons(b2g_manifold& m, const b2g_shape& A, cons
     int edgeB = 0;
     const float sepB = maxSeparation(&edgeB, B, xfB, A, xfA);
     if (sepB > totalRadius) return;
+    collidePolygonsFaces(m, A, xfA, B, xfB, sepA, edgeA, sepB, edgeB);
+}
 
+// The rest of b2CollidePolygons once both separations are known: reference
+// face, incident edge (b2FindIncidentEdge) and the two clips.
+__device__ inline void collidePolygonsFaces(b2g_manifold& m, const b2g_shape& A, const Xf& xfA, const b2g_shape& B,
+                                            const Xf& xfB, float sepA, int edgeA, float sepB, int edgeB) {
+    const float totalRadius = A.radius + B.radius;
     const bool flip = sepB > sepA + 0.1f * kLinearSlop;
     const b2g_shape& p1 = flip ? B : A;
     const b2g_shape& p2 = flip ? A : B;
@@ -123,6 +133,63 @@ __device__ inline void collidePolygons(b2g_manifold& m, const b2g_shape& A, cons
         }
     }
     m.pointCount = pc;
+}
+
+// ---------------------------------------------------------------- tiles
+// b2CollidePolygons by a group of kTileLanes lanes of one wavefront (the
+// group's lanes are consecutive and aligned). Every lane of the group calls
+// it with the same shapes (staged in LDS) and transforms; lane i of the
+// group evaluates face normal i of each b2FindMaxSeparation, and the group
+// reduces to the deepest separation, the lowest face index winning a tie,
+// which is the face the reference's sequential scan keeps (it replaces the
+// running maximum only on a strictly greater value). The rest of the
+// routine (incident edge, clipping) is evaluated identically on every lane,
+// so all lanes end with the same manifold.
+constexpr int kTileLanes = 8;  // polygons have at most 8 vertices
+
+__device__ inline float tileMaxSeparation(int* edge, const b2g_shape& p1, const Xf& xf1, const b2g_shape& p2,
+                                          const Xf& xf2, int lane) {
+    const Xf xf = mulT(xf2, xf1);
+    // this lane's face: the minimum over poly2's vertices, as the scan forms it
+    float si = -FLT_MAX;
+    int index = kTileLanes;  // no face
+    if (lane < p1.count) {
+        const V2 n = mul(xf.q, ld(p1.normals[lane]));
+        const V2 v1 = mul(xf, ld(p1.verts[lane]));
+        float s = FLT_MAX;
+        for (int j = 0; j < p2.count; ++j) {
+            const float sij = dot(n, ld(p2.verts[j]) - v1);
+            if (sij < s) s = sij;
+        }
+        // faces at or below the scan's starting value (or NaN) never win
+        if (s > -FLT_MAX) {
+            si = s;
+            index = lane;
+        }
+    }
+    for (int mask = 1; mask < kTileLanes; mask <<= 1) {
+        const float os = __shfl_xor(si, mask, kTileLanes);
+        const int oi = __shfl_xor(index, mask, kTileLanes);
+        if (os > si || (os == si && oi < index)) {
+            si = os;
+            index = oi;
+        }
+    }
+    *edge = index < kTileLanes ? index : 0;
+    return si;
+}
+
+__device__ inline void collidePolygonsTile(b2g_manifold& m, const b2g_shape& A, const Xf& xfA, const b2g_shape& B,
+                                           const Xf& xfB, int lane) {
+    m.pointCount = 0;
+    const float totalRadius = A.radius + B.radius;
+    int edgeA = 0;
+    const float sepA = tileMaxSeparation(&edgeA, A, xfA, B, xfB, lane);
+    if (sepA > totalRadius) return;
+    int edgeB = 0;
+    const float sepB = tileMaxSeparation(&edgeB, B, xfB, A, xfA, lane);
+    if (sepB > totalRadius) return;
+    collidePolygonsFaces(m, A, xfA, B, xfB, sepA, edgeA, sepB, edgeB);
 }
 
 __device__ inline void collideCircles(b2g_manifold& m, const b2g_shape& A, const Xf& xfA, const b2g_shape& B,

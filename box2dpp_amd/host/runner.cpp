@@ -567,7 +567,7 @@ int b2pp_world_events(void* h, int32_t* out, int cap) {
     return n;
 }
 
-// Autotunes the world's step schedule on 19 * rounds steps; *sched
+// Autotunes the world's step schedule on 20 * rounds steps; *sched
 // (b2g_schedule) receives the choice when not null.
 int b2pp_world_autotune(void* h, float dt, int vel, int pos, int rounds, b2g_schedule* sched) {
     Runner* r = (Runner*)h;

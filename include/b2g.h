@@ -666,7 +666,8 @@ int b2g_set_body_worlds(b2g_context* ctx, int first, int count, const int32_t* w
 typedef struct b2g_schedule {
     int32_t islands;       /* island labelling: 0 label-propagation rounds, 1 one round
                               then union-find, 2 union-find */
-    int32_t collideOrder;  /* 0 contact-table order, 1 grouped by shape-pair class */
+    int32_t collideOrder;  /* 0 contact-table order, 1 grouped by shape-pair class,
+                              2 table order with polygon pairs by eight-lane tiles */
     int32_t solverBlock;   /* workgroup size of the per-colour solver kernels */
     int32_t collideBlock;  /* workgroup size of the narrowphase kernel */
     int32_t queryBlock;    /* workgroup size of the broad-phase query */

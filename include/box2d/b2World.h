@@ -848,7 +848,7 @@ public:
     int32_t BeginSubWorld() { return ++m_subWorldCount, m_currentSubWorld = m_subWorldCount; }
     int32_t GetSubWorldCount() const { return m_subWorldCount + 1; }
     // Autotuner (b2g_autotune): times the step's schedule choices on
-    // 19 * rounds ordinary Steps of this world and keeps the fastest. No
+    // 20 * rounds ordinary Steps of this world and keeps the fastest. No
     // choice changes results. Listener events of these steps are not
     // replayed. Returns a b2g.h status.
     int32_t Autotune(float timeStep, int32_t velocityIterations, int32_t positionIterations, int32_t rounds);
