@@ -684,7 +684,7 @@ __global__ void k_colour_commit(WorldView w, const unsigned long long* claim, ui
 
 // Order key of every contact: a constraint's colour; contacts outside the
 // constraint set sort after every colour.
-// Also clears the colour counts that k_colour_hist then takes.
+// Also clears the colour counts that the order's counting sort then takes.
 __global__ void k_cons_keys(WorldView w, uint32_t* keys, int* vals, Counters* cnt) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < kColourSlots + 3) cnt->colourCount[c] = 0;
@@ -693,16 +693,6 @@ __global__ void k_cons_keys(WorldView w, uint32_t* keys, int* vals, Counters* cn
     vals[c] = c;
 }
 
-__global__ void k_colour_hist(const uint32_t* keys, int n, Counters* cnt) {
-    __shared__ int h[kColourSlots];
-    for (int i = threadIdx.x; i < kColourSlots; i += blockDim.x) h[i] = 0;
-    __syncthreads();
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n && keys[k] < (uint32_t)kColourSlots) atomicAdd(&h[keys[k]], 1);
-    __syncthreads();
-    for (int i = threadIdx.x; i < kColourSlots; i += blockDim.x)
-        if (h[i]) atomicAdd(&cnt->colourCount[i], h[i]);
-}
 
 // ====================================================================
 // Island solve stages (b2Island::Solve, REF/Dynamics/b2Island.cpp:182-383)
@@ -3759,14 +3749,19 @@ __device__ inline bool acceptPair(const WorldView& w, int a, int b, const unsign
     return primaryPair(ta, tb) || primaryPair(tb, ta);
 }
 
+// Where queries put the pairs they find. With packBase = n (the proxy
+// count) a pair a < b is stored as a * n + b, which orders pairs as
+// pairKey does in fewer bits (fewer radix passes); with 0 as pairKey.
 struct PairSink {
     unsigned long long* keys;
     int cap;
+    unsigned long long packBase;
 };
 
 __device__ inline void emitPair(PairSink sink, Counters* cnt, int a, int b) {
     const int slot = atomicAdd(&cnt->nNewPairs, 1);
-    if (slot < sink.cap) sink.keys[slot] = pairKey(a, b);
+    if (slot < sink.cap)
+        sink.keys[slot] = sink.packBase ? (unsigned long long)a * sink.packBase + (unsigned long long)b : pairKey(a, b);
     else atomicOr(&cnt->overflow, 1);
 }
 
@@ -3881,10 +3876,13 @@ __device__ inline void appendContactOne(const WorldView& w, unsigned long long k
     }
 }
 
-__global__ void k_append_contacts(WorldView w, const unsigned long long* keys, int nNew, int base) {
+// keys packed as a * packBase + b (PairSink).
+__global__ void k_append_contacts(WorldView w, const unsigned long long* keys, int nNew, int base,
+                                  unsigned long long packBase) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nNew) return;
-    appendContactOne(w, keys[k], base + k);
+    const unsigned long long key = keys[k];
+    appendContactOne(w, pairKey((int)(key / packBase), (int)(key % packBase)), base + k);
 }
 
 // Contacts between bodies that a new joint keeps apart are flagged for
@@ -5501,7 +5499,7 @@ int runBroadphase(b2g_context* ctx, bool liveOnDevice = false) {
     for (int attempt = 0;; ++attempt) {
         hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nNewPairs, 1, 0);
         hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->overflow, 1, 0);
-        PairSink sink{ctx->newKeys, ctx->capContacts};
+        PairSink sink{ctx->newKeys, ctx->capContacts, (unsigned long long)nf};
         const int qb = ctx->sched.queryBlock;
         hipLaunchKernelGGL(k_pairs_query, dim3((unsigned)(((long long)nf * kQueryLanes + qb - 1) / qb)), dim3(qb), 0, s,
                            ctx->fat, ctx->moved, nf, g,
@@ -5522,9 +5520,13 @@ int runBroadphase(b2g_context* ctx, bool liveOnDevice = false) {
     hipLaunchKernelGGL(k_fill<uint8_t>, dim3(blocks(nf)), dim3(256), 0, s, ctx->moved, nf, (uint8_t)0);
     WorldView w = ctx->view();
     if (nNew > 0) {
-        HIPCHK(ctx, prims_sort_keys_u64(ctx->scratch, ctx->scratchBytes, ctx->newKeys, ctx->newKeysSorted, nNew, s));
+        // packed keys a * nf + b < nf * nf
+        int endBit = 1;
+        while (endBit < 64 && (1ull << endBit) < (unsigned long long)nf * (unsigned long long)nf) ++endBit;
+        HIPCHK(ctx, prims_sort_keys_u64(ctx->scratch, ctx->scratchBytes, ctx->newKeys, ctx->newKeysSorted, nNew, s,
+                                        endBit));
         hipLaunchKernelGGL(k_append_contacts, dim3(blocks(nNew)), dim3(256), 0, s, w, ctx->newKeysSorted, nNew,
-                           ctx->nContacts);
+                           ctx->nContacts, (unsigned long long)nf);
         LAUNCHCHK(ctx);
         ctx->nContacts += nNew;
     }
@@ -6174,8 +6176,8 @@ int stepColouredCollide(b2g_context* ctx, const b2g_step_params* p) {
             // the constraint-order buffers are free until the order phase
             hipLaunchKernelGGL(k_collide_keys, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(),
                                ctx->consKeys, ctx->consVals);
-            HIPCHK(ctx, prims_sort_pairs_u32(ctx->scratch, ctx->scratchBytes, ctx->consKeys, ctx->consKeysSorted,
-                                             ctx->consVals, ctx->consOrder, ctx->nContacts, 4, s));
+            HIPCHK(ctx, prims_count_sort_pairs(ctx->scratch, ctx->scratchBytes, ctx->consKeys, ctx->consKeysSorted,
+                                               ctx->consVals, ctx->consOrder, ctx->nContacts, nullptr, 0, s));
             order = ctx->consOrder;
         }
         const int cb = ctx->sched.collideBlock;
@@ -6297,14 +6299,17 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
                 hipLaunchKernelGGL(k_cons_keys, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->consKeys,
                                    ctx->consVals, ctx->cnt);
             }
-            HIPCHK(ctx, prims_sort_pairs_u32(ctx->scratch, ctx->scratchBytes, ctx->consKeys, ctx->consKeysSorted,
-                                             ctx->consVals, ctx->consOrder, ctx->nContacts, clustered ? 31 : 7, s));
-            if (clustered)
+            if (clustered) {
+                HIPCHK(ctx, prims_sort_pairs_u32(ctx->scratch, ctx->scratchBytes, ctx->consKeys, ctx->consKeysSorted,
+                                                 ctx->consVals, ctx->consOrder, ctx->nContacts, 31, s));
                 hipLaunchKernelGGL(k_cluster_ranges, dim3(blocks((nClusters + 1) * (kColourSlots + 1))), dim3(256), 0,
                                    s, ctx->consKeysSorted, ctx->nContacts, nClusters, ctx->clusterStart, ctx->cnt);
-            else
-                hipLaunchKernelGGL(k_colour_hist, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->consKeysSorted,
-                                   ctx->nContacts, ctx->cnt);
+            } else {
+                // colours and the key of the contacts outside the set fit 7 bits
+                HIPCHK(ctx, prims_count_sort_pairs(ctx->scratch, ctx->scratchBytes, ctx->consKeys, ctx->consKeysSorted,
+                                                   ctx->consVals, ctx->consOrder, ctx->nContacts,
+                                                   ctx->cnt->colourCount, kColourSlots, s));
+            }
             profEnd(ctx, PH_ORDER, pe);
             LAUNCHCHK(ctx);
             return readCounters(ctx);
@@ -7735,7 +7740,7 @@ int b2g_pairs_batch(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved,
     hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(n)), dim3(256), 0, s, n, g, dfat.p, dmoved.p);
     WorldView w;
     memset(&w, 0, sizeof(w));
-    PairSink sink{keys.p, capPairs};
+    PairSink sink{keys.p, capPairs, 0ull};
     hipLaunchKernelGGL(k_pairs_query, dim3(blocks(n * kQueryLanes)), dim3(256), 0, s, dfat.p, dmoved.p, n, g, w, false,
                        (const unsigned long long*)nullptr, 0u, sink, dc.p);
     hipLaunchKernelGGL(k_pairs_wide, dim3(blocks(n)), dim3(256), 0, s, dfat.p, dmoved.p, n, g, w, false,
@@ -9124,7 +9129,7 @@ int b2g_pairs_batch_noalloc(b2g_context* ctx, const b2g_aabb* fat, const uint8_t
     hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(n)), dim3(256), 0, s, n, g, dfat, dmoved);
     WorldView w;
     memset(&w, 0, sizeof(w));
-    PairSink sink{keys, kNoAllocMaxPairs};
+    PairSink sink{keys, kNoAllocMaxPairs, 0ull};
     hipLaunchKernelGGL(k_pairs_query, dim3(blocks(n * kQueryLanes)), dim3(256), 0, s, dfat, dmoved, n, g, w, false,
                        (const unsigned long long*)nullptr, 0u, sink, dc);
     hipLaunchKernelGGL(k_pairs_wide, dim3(blocks(n)), dim3(256), 0, s, dfat, dmoved, n, g, w, false,

@@ -1,6 +1,7 @@
 // Device-wide scan and sort used by the step (contact compaction, grid
-// build, colour ordering, new-pair ordering). Backed by hipCUB's radix sort
-// and scan; all calls are asynchronous on the given stream and take a
+// build, colour ordering, new-pair ordering). Large inputs use hipCUB's
+// radix sort and scan, small 64-bit key sorts a one-workgroup bitonic
+// network; all calls are asynchronous on the given stream and take a
 // caller-owned scratch buffer sized by prims_scratch_bytes().
 #pragma once
 #include <hip/hip_runtime.h>
@@ -19,8 +20,16 @@ hipError_t prims_exclusive_sum(void* scratch, size_t bytes, const int* in, int* 
 hipError_t prims_sort_pairs_u32(void* scratch, size_t bytes, const uint32_t* kin, uint32_t* kout, const int* vin,
                                 int* vout, int n, int bits, hipStream_t s);
 
-// Sort of 64-bit keys.
+// Stable sort of (key, value) pairs by the low 7 bits of key (a counting
+// sort: four launches); totals[k] (if not null) gains the count of key k for
+// k < totalsKeys.
+size_t prims_count_sort_bytes(int n);
+hipError_t prims_count_sort_pairs(void* scratch, size_t bytes, const uint32_t* kin, uint32_t* kout, const int* vin,
+                                  int* vout, int n, int* totals, int totalsKeys, hipStream_t s);
+
+// Ascending sort of 64-bit keys whose bits at and above endBit are equal
+// (all keys, for the default 64). Up to 4096 keys: one workgroup in LDS.
 hipError_t prims_sort_keys_u64(void* scratch, size_t bytes, const unsigned long long* kin, unsigned long long* kout,
-                               int n, hipStream_t s);
+                               int n, hipStream_t s, int endBit = 64);
 
 }  // namespace b2g
