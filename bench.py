@@ -13,13 +13,16 @@ untimed; `full_window` reports the rate over all 1000.
 Also reported on the same JSON line:
   roofline      the dominant kernel (velocity solve) against HBM bandwidth:
                 algorithmic bytes per launch from the byte model
-                (48 B per constraint + 72 B per manifold point per velocity
+                (132 B per constraint + 48 B per manifold point per velocity
                 iteration) over its mean execution time, measured in a second
                 pass over the whole workload (fresh pile, the same 1000 steps)
                 with HIP events that its dispatch packets record on the step
                 stream (hipExtLaunchKernelGGL); traffic is the HBM bytes per
                 launch measured by rocprofv3 counters; stream_us_per_launch is
                 the time per launch seen on the stream, dispatch gaps included;
+  kernel_rooflines
+                the same figures for the next kernels (position solve,
+                broad-phase query, narrowphase), each from its own byte model;
   cpu_baseline  the reference b2World::Step (upstream Box2D 2.3.2 built from
                 the reference's sources, oracle/_ref) on one host core over the
                 same window: started from the device's state before the timed
@@ -48,18 +51,18 @@ import numpy as np  # noqa: E402
 import box2dpp_amd  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_solve_velocity launch on the default workload (100k pile,
-# seed 1, steps 1-1000, 8/3 iterations), from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE averaged over all launches and scaled by the counters' readings
-# for a plain 256 MiB copy (x2.000 and x1.000), keyed by the step schedule
-# the counters ran with (the fusion of trailing colours changes which
-# colours the kernel solves); the summary with the commands is under
-# profiles/. Another schedule reports traffic as null.
-DEFAULT_TRAFFIC_BYTES = {(0, 0, 256, 256, 64, 2048, 0): 7.779e6}
+# HBM bytes per launch of the main kernels on the default workload (100k
+# pile, seed 1, steps 1-1000, 8/3 iterations), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE averaged over each kernel's launches and scaled by the counters'
+# readings for a plain 256 MiB copy (x2.000 and x1.000), keyed by the step
+# schedule the counters ran with (the fusion of trailing colours changes
+# which colours the solver kernels see); the summaries with the commands are
+# under profiles/. Another schedule reports traffic as null.
+DEFAULT_TRAFFIC_BYTES = {(0, 1, 256, 256, 64, 2048, 0): {}}
 # The pile's step schedule as the autotuner picks it on this workload (step
 # 500 of a warm-up copy): fixed so the HBM traffic above applies; the batched
 # and stress worlds are tuned on every run.
-PILE_SCHEDULE = "0,0,256,256,64,2048,0"
+PILE_SCHEDULE = "0,1,256,256,64,2048,0"
 DT = 1.0 / 60.0
 # b2g_schedule: island labelling method, collide order, workgroup sizes of
 # the solver, narrowphase and broad-phase query kernels, trailing-colour
@@ -84,7 +87,8 @@ class StepStats(ctypes.Structure):
         "bodies", "fixtures", "contacts", "constraints", "points", "colours", "colouringRounds", "newPairs",
         "positionIterations", "beginEvents", "endEvents", "overflow")] + [
         ("minSeparation", ctypes.c_float), ("cellSize", ctypes.c_float), ("freshPoints", ctypes.c_int32),
-        ("fallback", ctypes.c_int32), ("residual", ctypes.c_float), ("toiEvents", ctypes.c_int32)]
+        ("fallback", ctypes.c_int32), ("residual", ctypes.c_float), ("toiEvents", ctypes.c_int32),
+        ("queries", ctypes.c_int32), ("scanned", ctypes.c_int32), ("wideConstraints", ctypes.c_int32)]
 
 
 def sched_dict(sc):
@@ -647,20 +651,64 @@ def headline(lib, args):
             "measures": meas}
 
 
+# Algorithmic bytes of the step's main kernels, per unit (the loads and
+# stores each unit's code names, every one counted once per unit):
+#   k_solve_velocity  per constraint and iteration 132 B (body indices 8,
+#                     info 4, masses 16, normal and friction 16, point masses
+#                     16, impulses read 16 and written 16, velocity biases 8,
+#                     the two bodies' (v, w) rows read 32 and written 32) + per
+#                     manifold point 48 B (anchor offsets 16, and for the
+#                     second point the block matrix and its inverse 32, taken
+#                     as 16 per point): 180 B for one point, 228 B for two;
+#   k_solve_position  per constraint and iteration 156 B (island and its
+#                     flag 8, body indices 8, info 4, masses 16, local
+#                     centres 16, local points 16, normal and plane point
+#                     16, radii 8, body positions read 32 and written 32);
+#   k_collide         per contact 360 B (order 4, flags read and written 8,
+#                     fixture ids 8, body ids 8, body type and flags 4, fat
+#                     boxes 32, transforms 32, manifold read 64 and written
+#                     64, shapes 2 x 68: a 24 B header plus 16 B per polygon
+#                     vertex, 5.5 vertices on half the pile's fixtures);
+#   k_pairs_query     36 B per query (proxy id 4, fat box 16, cell range 16)
+#                     + 28 B per grid entry scanned (cell key 8, id 4, fat box
+#                     16) + 8 B per new pair written.
+BYTES_PER_CONSTRAINT_V, BYTES_PER_POINT_V = 132.0, 48.0
+BYTES_PER_CONSTRAINT_X = 156.0
+BYTES_PER_CONTACT_COLLIDE = 360.0
+BYTES_PER_QUERY, BYTES_PER_SCANNED, BYTES_PER_NEW_PAIR = 36.0, 28.0, 8.0
+
+
+def kernel_bytes(s2, contacts_before, vel):
+    """Algorithmic bytes of one step's launches of each main kernel, from the
+    step's counters (the per-colour solver kernels see the constraints of the
+    colours launched one by one; their points are taken in the step's mean
+    ratio of points to constraints)."""
+    wide = s2.wideConstraints
+    pts = s2.points * wide / s2.constraints if s2.constraints else 0.0
+    return {"k_solve_velocity": vel * (BYTES_PER_CONSTRAINT_V * wide + BYTES_PER_POINT_V * pts),
+            "k_solve_position": s2.positionIterations * BYTES_PER_CONSTRAINT_X * wide,
+            "k_collide": BYTES_PER_CONTACT_COLLIDE * contacts_before,
+            "k_pairs_query": BYTES_PER_QUERY * s2.queries + BYTES_PER_SCANNED * s2.scanned
+            + BYTES_PER_NEW_PAIR * s2.newPairs}
+
+
 def roofline_pass(lib, args, total):
     """The same workload from a fresh pile with per-phase events on the step
-    stream and the velocity kernel's execution time from events carried by
-    its own dispatch packets, over all `total` steps of the workload."""
+    stream and the main kernels' execution times from events carried by
+    their own dispatch packets, over all `total` steps of the workload."""
     b2g = box2dpp_amd.b2g()
     progress(f"roofline pass: {total} steps with kernel events")
     h = make_world(lib, args.bodies, args.seed)
     ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
     b2g.b2g_set_profiling(ctx, 1)
-    prof_bytes = 0.0
+    alg = {}
+    contacts_before = 0
     for _ in range(total):
         step(lib, h, 1, args.vel, args.pos)
         s2 = stats_of(lib, h)
-        prof_bytes += args.vel * (48.0 * s2.constraints + 72.0 * s2.points)
+        for k, v in kernel_bytes(s2, contacts_before, args.vel).items():
+            alg[k] = alg.get(k, 0.0) + v
+        contacts_before = s2.contacts
     prof = (PhaseProfile * 32)()
     cnt = ctypes.c_int(0)
     b2g.b2g_get_profile(ctx, prof, 32, ctypes.byref(cnt))
@@ -669,19 +717,25 @@ def roofline_pass(lib, args, total):
     entries = {prof[i].name.decode(): (prof[i].launches, prof[i].ms) for i in range(min(cnt.value, 32))}
     phases = {k: v for k, v in entries.items() if not k.startswith("k_")}
     kernels = {k: v for k, v in entries.items() if k.startswith("k_")}
-    sv_launches, sv_ms = kernels.get("k_solve_velocity", (0, 0.0))
-    avg_launch_ms = sv_ms / max(sv_launches, 1)
+    traffic = args.traffic_bytes or {}
+
+    def line(name):
+        launches, ms = kernels.get(name, (0, 0.0))
+        avg_ms = ms / max(launches, 1)
+        per_launch = alg.get(name, 0.0) / max(launches, 1)
+        achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+        return {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic.get(name), "bytes_per_launch": per_launch,
+                "avg_launch_us": avg_ms * 1e3, "launches_per_step": launches / max(total, 1),
+                "kernel_ms_per_step": ms / max(total, 1)}
+
+    roof = line("k_solve_velocity")
     ph_launches, ph_ms = phases.get("solve_velocity", (0, 0.0))
-    bytes_per_launch = prof_bytes / max(sv_launches, 1)
-    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
+    roof["stream_us_per_launch"] = 1e3 * ph_ms / max(ph_launches, 1)
+    roof["window"] = f"steps 1-{total} of a fresh pile (the whole workload; the timed steps are its last ones)"
     total_phase_ms = sum(ms for _, ms in phases.values())
-    roof = {"bound": "hbm", "kernel": "k_solve_velocity", "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": args.traffic_bytes,
-            "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_ms * 1e3,
-            "launches_per_step": sv_launches / max(total, 1),
-            "stream_us_per_launch": 1e3 * ph_ms / max(ph_launches, 1),
-            "window": f"steps 1-{total} of a fresh pile (the whole workload; the timed steps are its last ones)"}
-    extra = {"phase_ms_per_step": {k: round(v[1] / max(total, 1), 4) for k, v in phases.items()},
+    extra = {"kernel_rooflines": {k: line(k) for k in ("k_solve_position", "k_pairs_query", "k_collide")},
+             "phase_ms_per_step": {k: round(v[1] / max(total, 1), 4) for k, v in phases.items()},
              "phase_share": {k: round(v[1] / total_phase_ms, 4) for k, v in phases.items()} if total_phase_ms else {},
              "kernel_us_per_launch": {k: round(1e3 * v[1] / max(v[0], 1), 3) for k, v in kernels.items()},
              "kernel_launches_per_step": {k: round(v[0] / max(total, 1), 2) for k, v in kernels.items()}}
@@ -810,7 +864,8 @@ def main():
     ap.add_argument("--trace-from", type=int, default=500, help="first step traced (the settled pile)")
     ap.add_argument("--trace-iters", type=int, default=128)
     ap.add_argument("--traffic-bytes", type=float, default=None,
-                    help="HBM bytes per velocity-solve launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE")
+                    help="HBM bytes per velocity-solve launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                         "(replaces the committed figures for the velocity solve)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -836,16 +891,22 @@ def main():
         pile_schedule = autotune(lib, hw, args, args.tune_at)
         lib.b2pp_world_destroy(hw)
     _schedule = pile_schedule
-    if args.traffic_bytes is None and default_workload and pile_schedule is not None:
+    if args.traffic_bytes is not None:
+        args.traffic_bytes = {"k_solve_velocity": args.traffic_bytes}
+    elif default_workload and pile_schedule is not None:
         args.traffic_bytes = DEFAULT_TRAFFIC_BYTES.get(tuple(int(pile_schedule[0][k]) for k in SCHEDULE.names))
 
     hl = headline(lib, args)
     total, prefix, wall, st, meas = hl["total"], hl["prefix"], hl["wall"], hl["stats"], hl["measures"]
     value = args.bodies * args.steps / wall
     ref_at_end = reference_stats(args.bodies, args.seed, total)
-    roof, extra = (None, {}) if args.no_roofline else section("roofline", roofline_pass, lib, args, total)
-    if isinstance(roof, dict) and "error" in roof:
-        roof, extra = None, {"roofline_error": roof["error"]}
+    roof, extra = None, {}
+    if not args.no_roofline:
+        res = section("roofline", roofline_pass, lib, args, total)
+        if isinstance(res, dict):  # the section failed
+            extra = {"roofline_error": res["error"]}
+        else:
+            roof, extra = res
 
     cpu = None
     if not args.no_cpu:

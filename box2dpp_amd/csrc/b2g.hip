@@ -70,6 +70,7 @@ struct Counters {
     int nBig;
     int nQueries;          // moved proxies binned in the grid (their queries)
     int nTiles;            // polygon-polygon contacts deferred to k_collide_tiles
+    int nScanned;          // grid entries and wide proxies the queries examined
     int roundLeft[kColourRoundSlots];  // constraints left uncoloured by colouring round r of the step
     int beginEvents;
     int endEvents;
@@ -269,6 +270,15 @@ __global__ void k_override_contacts(WorldView w, const int* ids, const int* enab
     w.cmat[c] = mat[i];
 }
 
+// b2Contact::SetTangentSpeed (REF/Dynamics/Contacts/b2Contact.h:346-349): the
+// speed stays with the contact until changed or the contact ends.
+__global__ void k_set_tangent_speeds(WorldView w, const int* ids, const float* speeds, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = ids[i];
+    if (c >= 0 && c < w.nContacts) w.ctan[c] = speeds[i];
+}
+
 // Slot of this lane in a list that every lane with `want` appends to: one
 // atomic per wavefront, lanes in lane order.
 __device__ inline int waveAppend(int* counter, bool want) {
@@ -307,9 +317,41 @@ __device__ inline void collideFinish(const WorldView& w, Counters* cnt, int c, i
     flags |= CF_ENABLED | CF_UPDATED;
     if (touching) flags |= CF_TOUCHING;
     else flags &= ~CF_TOUCHING;
+    if (wasTouching) flags |= CF_WAS_TOUCHING;
+    else flags &= ~CF_WAS_TOUCHING;
     if (!wasTouching && touching) atomicAdd(&cnt->beginEvents, 1);
     if (wasTouching && !touching) atomicAdd(&cnt->endEvents, 1);
     w.cman[c] = m;
+    w.cflags[c] = flags;
+}
+
+// b2TestOverlap (REF/Collision/b2Collision.cpp:237-255): GJK distance from an
+// empty cache, the radii taken off as b2Distance's useRadii does
+// (REF/Collision/b2Distance.cpp:579-602), under 10 epsilon.
+__device__ inline bool testOverlap(const b2g_shape& a, const Xf& xfA, const b2g_shape& b, const Xf& xfB) {
+    const Proxy pA = proxyOf(a), pB = proxyOf(b);
+    SimplexCache cache;
+    cache.count = 0;
+    cache.metric = 0.0f;
+    float d = gjkDistance(cache, pA, xfA, pB, xfB);
+    if (d > pA.radius + pB.radius && d > FLT_EPSILON) d -= pA.radius + pB.radius;
+    else d = 0.0f;
+    return d < 10.0f * FLT_EPSILON;
+}
+
+// b2Contact::Update's sensor branch (REF/Dynamics/Contacts/b2Contact.cpp:189-197):
+// touching is the shapes' overlap, the manifold keeps no points, and no
+// body is woken.
+__device__ inline void sensorFinish(const WorldView& w, Counters* cnt, int c, int flags, bool touching) {
+    const bool wasTouching = (flags & CF_TOUCHING) != 0;
+    flags |= CF_ENABLED | CF_UPDATED;
+    if (touching) flags |= CF_TOUCHING;
+    else flags &= ~CF_TOUCHING;
+    if (wasTouching) flags |= CF_WAS_TOUCHING;
+    else flags &= ~CF_WAS_TOUCHING;
+    if (!wasTouching && touching) atomicAdd(&cnt->beginEvents, 1);
+    if (wasTouching && !touching) atomicAdd(&cnt->endEvents, 1);
+    w.cman[c].pointCount = 0;
     w.cflags[c] = flags;
 }
 
@@ -347,6 +389,10 @@ __device__ inline bool collideOne(const WorldView& w, Counters* cnt, int c, bool
             wakeBody(w, bB);
         }
         w.cflags[c] = flags | CF_DEAD;
+        return false;
+    }
+    if (sensorOf(w, fA) || sensorOf(w, fB)) {
+        sensorFinish(w, cnt, c, flags, testOverlap(w.shape[fA], xfOf(w.xf[bA]), w.shape[fB], xfOf(w.xf[bB])));
         return false;
     }
     if (deferPolygons && w.shape[fA].type == B2G_POLYGON && w.shape[fB].type == B2G_POLYGON) {
@@ -588,6 +634,7 @@ struct ContactArrays {
     int* cfB;
     int* cflags;
     float2* cmat;
+    float* ctan;
     b2g_manifold* cman;
     int* ccolour;
     float* ctoi;
@@ -605,6 +652,7 @@ __global__ void k_compact(ContactArrays src, ContactArrays dst, const int* keep,
     dst.cfB[d] = src.cfB[c];
     dst.cflags[d] = src.cflags[c];
     dst.cmat[d] = src.cmat[c];
+    dst.ctan[d] = src.ctan[c];
     dst.cman[d] = src.cman[c];
     dst.ccolour[d] = src.ccolour[c];
     dst.ctoi[d] = src.ctoi[c];
@@ -807,7 +855,7 @@ __device__ inline void initConstraintOne(const WorldView& w, const ConView& cv, 
     cv.contact[k] = c;
     cv.bodies[k] = make_int2(bA, bB);
     cv.mass[k] = make_float4(mA, iA, mB, iB);
-    cv.normal[k] = make_float4(normal.x, normal.y, mat.x, 0.0f);
+    cv.normal[k] = make_float4(normal.x, normal.y, mat.x, w.ctan[c]);
     cv.r0[k] = make_float4(rA[0].x, rA[0].y, rB[0].x, rB[0].y);
     cv.r1[k] = make_float4(rA[1].x, rA[1].y, rB[1].x, rB[1].y);
     cv.pmass[k] = make_float4(nm[0], tm[0], nm[1], tm[1]);
@@ -3244,30 +3292,9 @@ __global__ void k_joint_solve_velocity_serial(WorldView w, JointView jv, int beg
     for (int k = begin; k < end; ++k) jointSolveVelocityOne(w, jv, jv.order[k], h);
 }
 
-// b2RevoluteJoint::SolvePositionConstraints, REF b2RevoluteJoint.cpp:294-378.
-// A joint left with an error above the slops marks its island's pass as
-// failed (the jointsOkay of b2Island::Solve, REF/Dynamics/b2Island.cpp:310-328).
-__device__ inline void jointSolvePositionOne(const WorldView& w, const JointView& jv, int j, int it) {
-    const int2 b = jv.bodies[j];
-    if (!jointActive(w, b)) return;
-    const int root = w.islandOf[w.btype[b.x] != BT_STATIC ? b.x : b.y];
-    if (it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0) return;  // island done
-    const int type = jointType(jv.flags[j]);
-    if (type == JT_FRICTION || type == JT_MOTOR || type == JT_MOUSE) return;  // no position pass (returns true)
-    if (type != JT_REVOLUTE) {
-        const bool ok = type == JT_DISTANCE ? distanceSolvePositionOne(w, jv, j, b)
-                        : type == JT_ROPE   ? ropeSolvePositionOne(w, jv, j, b)
-                        : type == JT_PULLEY ? pulleySolvePositionOne(w, jv, j, b)
-                        : type == JT_PRISMATIC ? prismaticSolvePositionOne(w, jv, j, b)
-                        : type == JT_WHEEL  ? wheelSolvePositionOne(w, jv, j, b)
-                        : type == JT_GEAR   ? gearSolvePositionOne(w, jv, j, b)
-                                            : weldSolvePositionOne(w, jv, j, b);
-        if (!ok) {
-            int* f = &w.islandMinSep[it * w.nBodies + root];
-            if (*f == 0) *f = 1;
-        }
-        return;
-    }
+// b2RevoluteJoint::SolvePositionConstraints, REF b2RevoluteJoint.cpp:294-378:
+// true when the errors are within the slops.
+__device__ inline bool revoluteSolvePositionOne(const WorldView& w, const JointView& jv, int j, int2 b) {
     const float4 pA = w.pos[b.x], pB = w.pos[b.y];
     V2 cA = mk(pA.x, pA.y), cB = mk(pB.x, pB.y);
     float aA = pA.z, aB = pB.z;
@@ -3323,7 +3350,36 @@ __device__ inline void jointSolvePositionOne(const WorldView& w, const JointView
     }
     if (w.btype[b.x] == BT_DYNAMIC) w.pos[b.x] = make_float4(cA.x, cA.y, aA, pA.w);
     if (w.btype[b.y] == BT_DYNAMIC) w.pos[b.y] = make_float4(cB.x, cB.y, aB, pB.w);
-    if (!(positionError <= kLinearSlop && angularError <= kAngularSlop)) {
+    return positionError <= kLinearSlop && angularError <= kAngularSlop;
+}
+
+// One joint's SolvePositionConstraints by type; friction, motor and mouse
+// joints have no position pass and report true.
+__device__ inline bool jointPositionCore(const WorldView& w, const JointView& jv, int j, int2 b) {
+    const int type = jointType(jv.flags[j]);
+    switch (type) {
+    case JT_FRICTION:
+    case JT_MOTOR:
+    case JT_MOUSE: return true;
+    case JT_DISTANCE: return distanceSolvePositionOne(w, jv, j, b);
+    case JT_ROPE: return ropeSolvePositionOne(w, jv, j, b);
+    case JT_PULLEY: return pulleySolvePositionOne(w, jv, j, b);
+    case JT_PRISMATIC: return prismaticSolvePositionOne(w, jv, j, b);
+    case JT_WHEEL: return wheelSolvePositionOne(w, jv, j, b);
+    case JT_GEAR: return gearSolvePositionOne(w, jv, j, b);
+    case JT_WELD: return weldSolvePositionOne(w, jv, j, b);
+    default: return revoluteSolvePositionOne(w, jv, j, b);
+    }
+}
+
+// A joint left with an error above the slops marks its island's pass as
+// failed (the jointsOkay of b2Island::Solve, REF/Dynamics/b2Island.cpp:310-328).
+__device__ inline void jointSolvePositionOne(const WorldView& w, const JointView& jv, int j, int it) {
+    const int2 b = jv.bodies[j];
+    if (!jointActive(w, b)) return;
+    const int root = w.islandOf[w.btype[b.x] != BT_STATIC ? b.x : b.y];
+    if (it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0) return;  // island done
+    if (!jointPositionCore(w, jv, j, b)) {
         int* f = &w.islandMinSep[it * w.nBodies + root];
         if (*f == 0) *f = 1;
     }
@@ -3766,13 +3822,15 @@ __device__ inline void emitPair(PairSink sink, Counters* cnt, int a, int b) {
 }
 
 // Queries of the moved proxies binned in the grid (the list k_grid_count
-// builds, in no fixed order): kQueryLanes lanes per query, lane k scanning
-// the query's cells k, k + kQueryLanes, ... and a share of the wide proxies.
+// builds, in no fixed order): kQueryLanes lanes per query, visiting the
+// query's cells one after another with the lanes striding over each cell's
+// entries (a cell holds about as many entries as a group has lanes), and a
+// share of the wide proxies each.
 // A pair is reported from the one cell holding the corner of the two boxes'
 // intersection, so lanes never report a pair twice. Each candidate's id,
 // move flag and fat box come from its grid entry. `world` is false for the
 // bare pair-set computation (no contact filtering).
-constexpr int kQueryLanes = 16;  // >= kMaxCellsPerProxy, a divisor of 64
+constexpr int kQueryLanes = 16;  // a divisor of 64
 
 __global__ void k_pairs_query(const float4* fat, const uint8_t* moved, int n, GridView g, WorldView w, bool world,
                               const unsigned long long* table, uint32_t hmask, PairSink sink, Counters* cnt) {
@@ -3798,12 +3856,13 @@ __global__ void k_pairs_query(const float4* fat, const uint8_t* moved, int n, Gr
     };
     const int wq = proxyWorld(g, q);
     const int width = r.z - r.x + 1, ncell = width * (r.w - r.y + 1);
-    for (int k = lane; k < ncell; k += kQueryLanes) {
+    // cell after cell, the group's lanes striding over the cell's entries
+    for (int k = 0; k < ncell; ++k) {
         const int ix = r.x + k % width, iy = r.y + k / width;
         const uint32_t bkt = cellHash(ix, iy, wq) & g.bucketMask;
         const int s0 = g.starts[bkt], s1 = g.starts[bkt + 1];
         const unsigned long long ck = cellKey(ix, iy, wq);
-        for (int e = s0; e < s1; ++e) {
+        for (int e = s0 + lane; e < s1; e += kQueryLanes) {
             if (g.ecell[e] != ck) continue;
             const int id = g.entries[e];
             consider(id & 0x7FFFFFFF, id < 0, g.efat[e], true, ix, iy);
@@ -3817,6 +3876,29 @@ __global__ void k_pairs_query(const float4* fat, const uint8_t* moved, int n, Gr
         const int p = (int)(e & 0xFFFFFFFFu);
         consider(p, moved[p] != 0, fat[p], false, 0, 0);
     }
+}
+
+// Grid entries and wide proxies that k_pairs_query examines (the buckets of
+// each query's cells and its sub-world's wide list), one thread per query:
+// the unit count of the query's byte model, for the timing pass only.
+__global__ void k_count_scanned(int n, GridView g, Counters* cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int scanned = 0;
+    if (i < n && i < cnt->nQueries) {
+        const int q = g.queries[i];
+        const int4 r = g.range[q];
+        const int wq = proxyWorld(g, q);
+        const int width = r.z - r.x + 1, ncell = width * (r.w - r.y + 1);
+        for (int k = 0; k < ncell; ++k) {
+            const uint32_t bkt = cellHash(r.x + k % width, r.y + k / width, wq) & g.bucketMask;
+            scanned += g.starts[bkt + 1] - g.starts[bkt];
+        }
+        int b0, b1;
+        bigRange(g, cnt->nBig, wq, b0, b1);
+        scanned += b1 - b0;
+    }
+    for (int off = 32; off > 0; off >>= 1) scanned += __shfl_down(scanned, off);
+    if ((threadIdx.x & 63) == 0 && scanned) atomicAdd(&cnt->nScanned, scanned);
 }
 
 // Queries of moved wide proxies (those not binned in the grid), one thread
@@ -3864,6 +3946,7 @@ __device__ inline void appendContactOne(const WorldView& w, unsigned long long k
     w.cflags[c] = CF_ENABLED;
     const float4 ma = w.fmat[a], mb = w.fmat[b];
     w.cmat[c] = make_float2(sqrtf(ma.x * mb.x), ma.y > mb.y ? ma.y : mb.y);
+    w.ctan[c] = 0.0f;
     b2g_manifold m;
     zeroManifold(m);
     w.cman[c] = m;
@@ -3939,7 +4022,7 @@ struct RefView {
     int* edgeCursor;
     unsigned long long* edgeKeys;
     unsigned long long* edgeSorted;
-    int* edgeContact;
+    int* edgeContact;       // contact c, or ~j for joint j
     int* bvisited;
     int* cvisited;
     int* stack;
@@ -3948,6 +4031,9 @@ struct RefView {
     int4* islandRange;      // bodyStart bodyCount contactStart contactCount
     int* islandSlept;       // per island: went to sleep this step
     int* lastIsland;        // per static body: last island (in search order) holding it
+    int* jvisited;          // per joint: in an island this step
+    int* islandJoints;
+    int2* islandJointRange; // per island: jointStart jointCount
 };
 
 __global__ void k_collide_compute(WorldView w, RefView r) {
@@ -3959,7 +4045,11 @@ __global__ void k_collide_compute(WorldView w, RefView r) {
     r.overlap[c] = ov ? 1 : 0;
     b2g_manifold m;
     zeroManifold(m);
-    if (ov) collideShapes(m, w.shape[fA], xfOf(w.xf[bA]), w.shape[fB], xfOf(w.xf[bB]));
+    if (ov && (sensorOf(w, fA) || sensorOf(w, fB))) {
+        if (testOverlap(w.shape[fA], xfOf(w.xf[bA]), w.shape[fB], xfOf(w.xf[bB]))) r.overlap[c] = 3;
+    } else if (ov) {
+        collideShapes(m, w.shape[fA], xfOf(w.xf[bA]), w.shape[fB], xfOf(w.xf[bB]));
+    }
     r.tmpMan[c] = m;
 }
 
@@ -3992,6 +4082,10 @@ __global__ void k_collide_serial(WorldView w, RefView r, Counters* cnt) {
             if (flags & CF_TOUCHING) cnt->endEvents++;
             if (destroyWake) { wakeBody(w, bA); wakeBody(w, bB); }
             w.cflags[c] = flags | CF_DEAD;
+            continue;
+        }
+        if (sensorOf(w, fA) || sensorOf(w, fB)) {
+            sensorFinish(w, cnt, c, flags, r.overlap[c] == 3);
             continue;
         }
         const b2g_manifold old = w.cman[c];
@@ -4046,26 +4140,58 @@ __global__ void k_edge_fill(WorldView w, RefView r) {
         const int b = bs[k];
         if (w.btype[b] == BT_STATIC) continue;
         const int slot = r.edgeStart[b] + atomicAdd(&r.edgeCursor[b], 1);
-        r.edgeKeys[slot] = ((unsigned long long)(unsigned)b << 32) | (0xFFFFFFFFu - (unsigned)c);
+        r.edgeKeys[slot] = ((unsigned long long)(unsigned)b << 32) | (0x7FFFFFFFu - (unsigned)c);
+    }
+}
+
+// Joint edges of the non-static bodies (b2Body::m_jointList): a body's
+// edges sort after its contact edges, newest joint first, as the island
+// search of b2World::Solve walks them (REF/Dynamics/b2World.cpp:470-531).
+__device__ inline bool islandJoint(const WorldView& w, const JointView& jv, int j) {
+    return (jv.flags[j] & JF_REMOVED) == 0;
+}
+
+__global__ void k_joint_edge_count(WorldView w, JointView jv, RefView r, Counters* cnt) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= jv.n) return;
+    r.jvisited[j] = 0;
+    if (!islandJoint(w, jv, j)) return;
+    const int2 b = jv.bodies[j];
+    if (w.btype[b.x] != BT_STATIC) { atomicAdd(&r.edgeCount[b.x], 1); atomicAdd(&cnt->nEdges, 1); }
+    if (w.btype[b.y] != BT_STATIC) { atomicAdd(&r.edgeCount[b.y], 1); atomicAdd(&cnt->nEdges, 1); }
+}
+
+__global__ void k_joint_edge_fill(WorldView w, JointView jv, RefView r) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= jv.n || !islandJoint(w, jv, j)) return;
+    const int2 bb = jv.bodies[j];
+    const int bs[2] = {bb.x, bb.y};
+    for (int k = 0; k < 2; ++k) {
+        const int b = bs[k];
+        if (w.btype[b] == BT_STATIC) continue;
+        const int slot = r.edgeStart[b] + atomicAdd(&r.edgeCursor[b], 1);
+        r.edgeKeys[slot] = ((unsigned long long)(unsigned)b << 32) | (0xFFFFFFFFu - (unsigned)j);
     }
 }
 
 __global__ void k_edge_extract(RefView r, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    r.edgeContact[i] = (int)(0xFFFFFFFFu - (unsigned)(r.edgeSorted[i] & 0xFFFFFFFFull));
+    const unsigned low = (unsigned)(r.edgeSorted[i] & 0xFFFFFFFFull);
+    r.edgeContact[i] = low < 0x80000000u ? (int)(0x7FFFFFFFu - low) : ~(int)(0xFFFFFFFFu - low);
 }
 
-// Island search of b2World::Solve, one thread.
-__global__ void k_dfs(WorldView w, RefView r, Counters* cnt) {
+// Island search of b2World::Solve (REF/Dynamics/b2World.cpp:429-539), one
+// thread: contact edges, then joint edges to active bodies.
+__global__ void k_dfs(WorldView w, JointView jv, RefView r, Counters* cnt) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    int nI = 0, nb = 0, nc = 0;
+    int nI = 0, nb = 0, nc = 0, nj = 0;
     for (int seed = w.nBodies - 1; seed >= 0; --seed) {
         if (r.bvisited[seed]) continue;
         const int sf = w.bflags[seed];
         if (!(sf & BF_AWAKE) || !(sf & BF_ACTIVE)) continue;
         if (w.btype[seed] == BT_STATIC) continue;
-        const int bStart = nb, cStart = nc;
+        const int bStart = nb, cStart = nc, jStart = nj;
         int sp = 0;
         r.stack[sp++] = seed;
         r.bvisited[seed] = 1;
@@ -4080,6 +4206,19 @@ __global__ void k_dfs(WorldView w, RefView r, Counters* cnt) {
             w.bflags[b] |= BF_ISLAND;
             for (int e = r.edgeStart[b]; e < r.edgeStart[b + 1]; ++e) {
                 const int c = r.edgeContact[e];
+                if (c < 0) {
+                    const int j = ~c;
+                    if (r.jvisited[j]) continue;
+                    const int2 jb = jv.bodies[j];
+                    const int other = jb.x == b ? jb.y : jb.x;
+                    if ((w.bflags[other] & BF_ACTIVE) == 0) continue;
+                    r.islandJoints[nj++] = j;
+                    r.jvisited[j] = 1;
+                    if (r.bvisited[other]) continue;
+                    r.stack[sp++] = other;
+                    r.bvisited[other] = 1;
+                    continue;
+                }
                 if (r.cvisited[c]) continue;
                 r.islandContacts[nc++] = c;
                 r.cvisited[c] = 1;
@@ -4090,6 +4229,7 @@ __global__ void k_dfs(WorldView w, RefView r, Counters* cnt) {
                 r.bvisited[other] = 1;
             }
         }
+        r.islandJointRange[nI] = make_int2(jStart, nj - jStart);
         r.islandRange[nI++] = make_int4(bStart, nb - bStart, cStart, nc - cStart);
         for (int i = bStart; i < nb; ++i)
             if (w.btype[r.islandBodies[i]] == BT_STATIC) r.bvisited[r.islandBodies[i]] = 0;
@@ -4100,9 +4240,15 @@ __global__ void k_dfs(WorldView w, RefView r, Counters* cnt) {
 
 // One wavefront per island: parallel per-body and per-constraint set-up,
 // the sequential passes of b2ContactSolver on lane 0.
-__global__ void k_island_solve(WorldView w, ConView cv, RefView r, V2 gravity, float h, float dtRatio, int warm,
-                               int velIters, int posIters, int allowSleep, Counters* cnt) {
+// Joints as b2Island::Solve orders them (REF/Dynamics/b2Island.cpp:246-328):
+// their InitVelocityConstraints after the contact warm start, each velocity
+// iteration joints then contacts, each position iteration contacts then
+// joints, and the early exit when both are within tolerance.
+__global__ void k_island_solve(WorldView w, ConView cv, JointView jv, RefView r, V2 gravity, float h, float dtRatio,
+                               int warm, int velIters, int posIters, int allowSleep, Counters* cnt) {
     const int4 rg = r.islandRange[blockIdx.x];
+    const int2 jr = r.islandJointRange[blockIdx.x];
+    const int* joints = r.islandJoints + jr.x;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int* bodies = r.islandBodies + rg.x;
     const int k0 = rg.z, k1 = rg.z + rg.w;
@@ -4115,8 +4261,11 @@ __global__ void k_island_solve(WorldView w, ConView cv, RefView r, V2 gravity, f
     if (tid == 0) {
         if (warm)
             for (int k = k0; k < k1; ++k) warmStartOne(w, cv, k);
-        for (int it = 0; it < velIters; ++it)
+        for (int i = 0; i < jr.y; ++i) jointInitOne(w, jv, joints[i], dtRatio, warm);
+        for (int it = 0; it < velIters; ++it) {
+            for (int i = 0; i < jr.y; ++i) jointSolveVelocityOne(w, jv, joints[i], h);
             for (int k = k0; k < k1; ++k) solveVelocityOne(w, cv, k);
+        }
     }
     __syncthreads();
     for (int k = k0 + tid; k < k1; k += nt) storeImpulsesOne(w, cv, k);
@@ -4131,7 +4280,12 @@ __global__ void k_island_solve(WorldView w, ConView cv, RefView r, V2 gravity, f
         for (int it = 0; it < posIters; ++it) {
             float minSep = 0.0f;
             for (int k = k0; k < k1; ++k) minSep = fminB(minSep, solvePositionOne(w, cv, k));
-            if (minSep >= -3.0f * kLinearSlop) {
+            bool jointsOkay = true;
+            for (int i = 0; i < jr.y; ++i) {
+                const bool ok = jointPositionCore(w, jv, joints[i], jv.bodies[joints[i]]);
+                jointsOkay = jointsOkay && ok;
+            }
+            if (minSep >= -3.0f * kLinearSlop && jointsOkay) {
                 solved = 1;
                 break;
             }
@@ -5074,6 +5228,7 @@ struct b2g_context {
 
     // reference-order mode buffers
     RefView ref{};
+    int refCapJoints = 0;
     int refCapBodies = 0, refCapContacts = 0;
 
     // revolute joints
@@ -5195,7 +5350,7 @@ struct b2g_context {
         w.fbody = fbody; w.shape = shape; w.fmat = fmat; w.ffilter = ffilter; w.fgroup = fgroup; w.fat = fat;
         w.moved = moved;
         w.nContacts = nContacts;
-        w.cfA = cur.cfA; w.cfB = cur.cfB; w.cflags = cur.cflags; w.cmat = cur.cmat; w.cman = cur.cman;
+        w.cfA = cur.cfA; w.cfB = cur.cfB; w.cflags = cur.cflags; w.cmat = cur.cmat; w.ctan = cur.ctan; w.cman = cur.cman;
         w.ccolour = cur.ccolour;
         w.ctoi = cur.ctoi;
         w.ctoiCount = cur.ctoiCount;
@@ -5357,6 +5512,7 @@ int growContacts(b2g_context* ctx, int need) {
     HIPCHK(ctx, growArray(ctx->cur.cfB, n, cap, s));
     HIPCHK(ctx, growArray(ctx->cur.cflags, n, cap, s));
     HIPCHK(ctx, growArray(ctx->cur.cmat, n, cap, s));
+    HIPCHK(ctx, growArray(ctx->cur.ctan, n, cap, s));
     HIPCHK(ctx, growArray(ctx->cur.cman, n, cap, s));
     HIPCHK(ctx, growArray(ctx->cur.ccolour, n, cap, s));
     HIPCHK(ctx, growArray(ctx->cur.ctoi, n, cap, s));
@@ -5365,6 +5521,7 @@ int growContacts(b2g_context* ctx, int need) {
     HIPCHK(ctx, growArray(ctx->alt.cfB, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->alt.cflags, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->alt.cmat, 0, cap, s));
+    HIPCHK(ctx, growArray(ctx->alt.ctan, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->alt.cman, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->alt.ccolour, 0, cap, s));
     HIPCHK(ctx, growArray(ctx->alt.ctoi, 0, cap, s));
@@ -5444,7 +5601,26 @@ int reserveContacts(b2g_context* ctx) {
 
 // Runs the broad-phase over the current fat boxes and appends the new
 // contacts. Requires the hash of live contacts to be current.
+// Phases are timed with events recorded on the step stream around their
+// launches (so they include dispatch gaps); entries named after a kernel
+// ("k_...") are that kernel's own execution time, from events that the
+// dispatch packet itself records (hipExtLaunchKernelGGL).
+enum Phase {
+    PH_COLLIDE, PH_COMPACT, PH_COLOUR, PH_ORDER, PH_INTEGRATE_V, PH_INIT, PH_WARM, PH_SOLVE_V, PH_STORE,
+    PH_INTEGRATE_X, PH_SOLVE_X, PH_SYNC, PH_BROADPHASE, PH_ISLANDS, PH_LEARNED, PH_K_SOLVE_V, PH_K_SOLVE_X,
+    PH_K_MLP_INFER, PH_K_MLP_GRAD, PH_TOI, PH_K_SOLVE_V_CLUSTER, PH_K_SOLVE_X_CLUSTER, PH_K_COLLIDE,
+    PH_K_PAIRS_QUERY, PH_COUNT
+};
+const char* kPhaseNames[PH_COUNT] = {"collide", "compact", "colour", "order", "integrate_velocities",
+                                     "init_constraints", "warm_start", "solve_velocity", "store_impulses",
+                                     "integrate_positions", "solve_position", "sync", "broadphase", "islands",
+                                     "fresh_points", "k_solve_velocity", "k_solve_position", "k_mlp_infer",
+                                     "k_mlp_grad", "toi", "k_solve_velocity_cluster",
+                                     "k_solve_position_cluster", "k_collide", "k_pairs_query"};
+
 int readCounters(b2g_context* ctx);
+int profKernelEvents(b2g_context* ctx, hipEvent_t* start, hipEvent_t* stop);
+void profKernel(b2g_context* ctx, int phase, int i);
 
 // liveOnDevice: the contact table was compacted without a readback
 // (compactContacts); the query's readback brings the live count.
@@ -5501,9 +5677,14 @@ int runBroadphase(b2g_context* ctx, bool liveOnDevice = false) {
         hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->overflow, 1, 0);
         PairSink sink{ctx->newKeys, ctx->capContacts, (unsigned long long)nf};
         const int qb = ctx->sched.queryBlock;
-        hipLaunchKernelGGL(k_pairs_query, dim3((unsigned)(((long long)nf * kQueryLanes + qb - 1) / qb)), dim3(qb), 0, s,
-                           ctx->fat, ctx->moved, nf, g,
-                           ctx->view(), true, ctx->hashTable, ctx->hashMask, sink, ctx->cnt);
+        hipEvent_t e0, e1;
+        const int pk = profKernelEvents(ctx, &e0, &e1);
+        hipExtLaunchKernelGGL(k_pairs_query, dim3((unsigned)(((long long)nf * kQueryLanes + qb - 1) / qb)), dim3(qb), 0,
+                              s, e0, e1, 0, ctx->fat, ctx->moved, nf, g, ctx->view(), true, ctx->hashTable,
+                              ctx->hashMask, sink, ctx->cnt);
+        if (ctx->profiling)
+            hipLaunchKernelGGL(k_count_scanned, dim3(blocks(nf)), dim3(256), 0, s, nf, g, ctx->cnt);
+        profKernel(ctx, PH_K_PAIRS_QUERY, pk);
         hipLaunchKernelGGL(k_pairs_wide, dim3(blocks(nf)), dim3(256), 0, s, ctx->fat, ctx->moved, nf, g, ctx->view(),
                            true, ctx->hashTable, ctx->hashMask, sink, ctx->cnt);
         LAUNCHCHK(ctx);
@@ -5531,6 +5712,8 @@ int runBroadphase(b2g_context* ctx, bool liveOnDevice = false) {
         ctx->nContacts += nNew;
     }
     ctx->stats.newPairs += nNew;
+    ctx->stats.queries = ctx->hcnt->nQueries;
+    ctx->stats.scanned = ctx->hcnt->nScanned;
     return B2G_OK;
 }
 
@@ -5556,22 +5739,6 @@ int readCounters(b2g_context* ctx) {
 }
 
 // ---------------------------------------------------------------- profiling
-// Phases are timed with events recorded on the step stream around their
-// launches (so they include dispatch gaps); entries named after a kernel
-// ("k_...") are that kernel's own execution time, from events that the
-// dispatch packet itself records (hipExtLaunchKernelGGL).
-enum Phase {
-    PH_COLLIDE, PH_COMPACT, PH_COLOUR, PH_ORDER, PH_INTEGRATE_V, PH_INIT, PH_WARM, PH_SOLVE_V, PH_STORE,
-    PH_INTEGRATE_X, PH_SOLVE_X, PH_SYNC, PH_BROADPHASE, PH_ISLANDS, PH_LEARNED, PH_K_SOLVE_V, PH_K_SOLVE_X,
-    PH_K_MLP_INFER, PH_K_MLP_GRAD, PH_TOI, PH_K_SOLVE_V_CLUSTER, PH_K_SOLVE_X_CLUSTER, PH_COUNT
-};
-const char* kPhaseNames[PH_COUNT] = {"collide", "compact", "colour", "order", "integrate_velocities",
-                                     "init_constraints", "warm_start", "solve_velocity", "store_impulses",
-                                     "integrate_positions", "solve_position", "sync", "broadphase", "islands",
-                                     "fresh_points", "k_solve_velocity", "k_solve_position", "k_mlp_infer",
-                                     "k_mlp_grad", "toi", "k_solve_velocity_cluster",
-                                     "k_solve_position_cluster"};
-
 int profStart(b2g_context* ctx) {
     if (!ctx->profiling) return -1;
     while ((int)ctx->evPool.size() < ctx->evUsed + 2) {
@@ -5957,7 +6124,7 @@ int solveByGraph(b2g_context* ctx, const WorldView& w, const JointView& jv, cons
     ptr(w.mass); ptr(w.gsleep); ptr(w.colourMask); ptr(w.islandOf); ptr(w.islandAwake); ptr(w.islandSleep);
     ptr(w.islandMinSep); ptr(w.bworld); ptr(w.jointPairs); put(w.jointPairMask);
     ptr(w.fbody); ptr(w.shape); ptr(w.fmat); ptr(w.ffilter); ptr(w.fgroup); ptr(w.fat); ptr(w.moved);
-    ptr(w.cfA); ptr(w.cfB); ptr(w.cflags); ptr(w.cmat); ptr(w.cman); ptr(w.ccolour);
+    ptr(w.cfA); ptr(w.cfB); ptr(w.cflags); ptr(w.cmat); ptr(w.ctan); ptr(w.cman); ptr(w.ccolour);
     const ConView& cv = ctx->cv;
     ptr(cv.contact); ptr(cv.bodies); ptr(cv.mass); ptr(cv.normal); ptr(cv.r0); ptr(cv.r1); ptr(cv.pmass);
     ptr(cv.bias); ptr(cv.imp); ptr(cv.K); ptr(cv.NM); ptr(cv.info); ptr(cv.lp); ptr(cv.lnp); ptr(cv.lc);
@@ -6182,8 +6349,11 @@ int stepColouredCollide(b2g_context* ctx, const b2g_step_params* p) {
         }
         const int cb = ctx->sched.collideBlock;
         const bool tiles = ctx->sched.collideOrder == 2;
-        hipLaunchKernelGGL(k_collide, dim3((ctx->nContacts + cb - 1) / cb), dim3(cb), 0, s, ctx->view(), ctx->cnt,
-                           order, tiles ? ctx->tileList : (int*)nullptr);
+        hipEvent_t e0, e1;
+        const int pk = profKernelEvents(ctx, &e0, &e1);
+        hipExtLaunchKernelGGL(k_collide, dim3((ctx->nContacts + cb - 1) / cb), dim3(cb), 0, s, e0, e1, 0, ctx->view(),
+                              ctx->cnt, order, tiles ? ctx->tileList : (int*)nullptr);
+        profKernel(ctx, PH_K_COLLIDE, pk);
         if (tiles) {
             // the deferred polygon pairs, eight lanes each: one pass of the grid
             // covers the last step's count (a longer list takes more passes)
@@ -6390,6 +6560,9 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
                 --tailBegin;
             if (nColours - tailBegin < 2) tailBegin = nColours;  // one colour: no gain
         }
+        ctx->stats.wideConstraints = 0;
+        for (int c = 0; c < tailBegin; ++c)
+            if (c != kOverflowColour) ctx->stats.wideConstraints += colourStart[c + 1] - colourStart[c];
         ColourRanges ranges;
         memcpy(ranges.s, colourStart, sizeof(ranges.s));
         auto perColour = [&](int phase, auto kernelLaunch, auto tailLaunch) {
@@ -6680,35 +6853,44 @@ void freeRef(b2g_context* ctx) {
     freeArray(r.edgeCursor); freeArray(r.edgeKeys); freeArray(r.edgeSorted); freeArray(r.edgeContact);
     freeArray(r.bvisited); freeArray(r.cvisited); freeArray(r.stack); freeArray(r.islandBodies);
     freeArray(r.islandContacts); freeArray(r.islandRange); freeArray(r.islandSlept); freeArray(r.lastIsland);
-    ctx->refCapBodies = ctx->refCapContacts = 0;
+    freeArray(r.jvisited); freeArray(r.islandJoints); freeArray(r.islandJointRange);
+    ctx->refCapBodies = ctx->refCapContacts = ctx->refCapJoints = 0;
 }
 
 int ensureRefBuffers(b2g_context* ctx) {
-    if (ctx->refCapBodies >= ctx->capBodies && ctx->refCapContacts >= ctx->capContacts) return B2G_OK;
+    if (ctx->refCapBodies >= ctx->capBodies && ctx->refCapContacts >= ctx->capContacts &&
+        ctx->refCapJoints >= ctx->capJoints)
+        return B2G_OK;
     freeRef(ctx);
     RefView& r = ctx->ref;
     const size_t nb = (size_t)ctx->capBodies, nc = (size_t)ctx->capContacts;
+    const size_t nj = (size_t)std::max(ctx->capJoints, 1);
     HIPCHK(ctx, hipMalloc((void**)&r.tmpMan, sizeof(b2g_manifold) * nc));
     HIPCHK(ctx, hipMalloc((void**)&r.overlap, nc));
     HIPCHK(ctx, hipMalloc((void**)&r.edgeCount, sizeof(int) * (nb + 1)));
     HIPCHK(ctx, hipMalloc((void**)&r.edgeStart, sizeof(int) * (nb + 1)));
     HIPCHK(ctx, hipMalloc((void**)&r.edgeCursor, sizeof(int) * nb));
-    HIPCHK(ctx, hipMalloc((void**)&r.edgeKeys, sizeof(unsigned long long) * 2 * nc));
-    HIPCHK(ctx, hipMalloc((void**)&r.edgeSorted, sizeof(unsigned long long) * 2 * nc));
-    HIPCHK(ctx, hipMalloc((void**)&r.edgeContact, sizeof(int) * 2 * nc));
+    HIPCHK(ctx, hipMalloc((void**)&r.edgeKeys, sizeof(unsigned long long) * 2 * (nc + nj)));
+    HIPCHK(ctx, hipMalloc((void**)&r.edgeSorted, sizeof(unsigned long long) * 2 * (nc + nj)));
+    HIPCHK(ctx, hipMalloc((void**)&r.edgeContact, sizeof(int) * 2 * (nc + nj)));
     HIPCHK(ctx, hipMalloc((void**)&r.bvisited, sizeof(int) * nb));
     HIPCHK(ctx, hipMalloc((void**)&r.cvisited, sizeof(int) * nc));
     HIPCHK(ctx, hipMalloc((void**)&r.stack, sizeof(int) * nb));
-    HIPCHK(ctx, hipMalloc((void**)&r.islandBodies, sizeof(int) * (nb + nc)));
+    // a static body joins each island that reaches it through a contact or a joint
+    HIPCHK(ctx, hipMalloc((void**)&r.islandBodies, sizeof(int) * (nb + nc + nj)));
     HIPCHK(ctx, hipMalloc((void**)&r.islandContacts, sizeof(int) * nc));
     HIPCHK(ctx, hipMalloc((void**)&r.islandRange, sizeof(int4) * nb));
     HIPCHK(ctx, hipMalloc((void**)&r.islandSlept, sizeof(int) * nb));
     HIPCHK(ctx, hipMalloc((void**)&r.lastIsland, sizeof(int) * nb));
+    HIPCHK(ctx, hipMalloc((void**)&r.jvisited, sizeof(int) * nj));
+    HIPCHK(ctx, hipMalloc((void**)&r.islandJoints, sizeof(int) * nj));
+    HIPCHK(ctx, hipMalloc((void**)&r.islandJointRange, sizeof(int2) * nb));
     hipLaunchKernelGGL(k_fill<int>, dim3(blocks((int)nb)), dim3(256), 0, ctx->stream, r.lastIsland, (int)nb, -1);
     ctx->refCapBodies = (int)nb;
     ctx->refCapContacts = (int)nc;
-    // sorting 2 edges per contact
-    const size_t need = prims_scratch_bytes((int)std::max<size_t>(2 * nc, 1024));
+    ctx->refCapJoints = (int)nj;
+    // sorting 2 edges per contact and joint
+    const size_t need = prims_scratch_bytes((int)std::max<size_t>(2 * (nc + nj), 1024));
     if (need > ctx->scratchBytes) {
         freeArray(ctx->scratch);
         HIPCHK(ctx, hipMalloc(&ctx->scratch, need));
@@ -6934,11 +7116,16 @@ int stepReference(b2g_context* ctx, const b2g_step_params* p) {
     if (dt > 0.0f && ctx->stepComplete) {
         WorldView w = ctx->view();
         const int nb = ctx->nBodies, nc = ctx->nContacts;
+        JointView jv = ctx->jview();
+        jv.h = dt;
+        const int nj = ctx->nJoints;
         hipLaunchKernelGGL(k_clear_island_flags, dim3(blocks(nb + 1)), dim3(256), 0, s, w, r);
         if (nc > 0) hipLaunchKernelGGL(k_edge_count, dim3(blocks(nc)), dim3(256), 0, s, w, r, ctx->cnt);
+        if (nj > 0) hipLaunchKernelGGL(k_joint_edge_count, dim3(blocks(nj)), dim3(256), 0, s, w, jv, r, ctx->cnt);
         LAUNCHCHK(ctx);
         HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, r.edgeCount, r.edgeStart, nb + 1, s));
         if (nc > 0) hipLaunchKernelGGL(k_edge_fill, dim3(blocks(nc)), dim3(256), 0, s, w, r);
+        if (nj > 0) hipLaunchKernelGGL(k_joint_edge_fill, dim3(blocks(nj)), dim3(256), 0, s, w, jv, r);
         LAUNCHCHK(ctx);
         if ((rc = readCounters(ctx)) != B2G_OK) return rc;
         const int nEdges = ctx->hcnt->nEdges;
@@ -6946,15 +7133,15 @@ int stepReference(b2g_context* ctx, const b2g_step_params* p) {
             HIPCHK(ctx, prims_sort_keys_u64(ctx->scratch, ctx->scratchBytes, r.edgeKeys, r.edgeSorted, nEdges, s));
             hipLaunchKernelGGL(k_edge_extract, dim3(blocks(nEdges)), dim3(256), 0, s, r, nEdges);
         }
-        hipLaunchKernelGGL(k_dfs, dim3(1), dim3(64), 0, s, w, r, ctx->cnt);
+        hipLaunchKernelGGL(k_dfs, dim3(1), dim3(64), 0, s, w, jv, r, ctx->cnt);
         LAUNCHCHK(ctx);
         if ((rc = readCounters(ctx)) != B2G_OK) return rc;
         const int nIslands = ctx->hcnt->nIslands;
         const int posIters = std::min(p->positionIterations, kMaxPosIters);
         if (nIslands > 0)
-            hipLaunchKernelGGL(k_island_solve, dim3(nIslands), dim3(64), 0, s, w, ctx->cv, r, ctx->gravity, dt, dtRatio,
-                               ctx->warmStarting ? 1 : 0, p->velocityIterations, posIters, ctx->allowSleep ? 1 : 0,
-                               ctx->cnt);
+            hipLaunchKernelGGL(k_island_solve, dim3(nIslands), dim3(64), 0, s, w, ctx->cv, jv, r, ctx->gravity, dt,
+                               dtRatio, ctx->warmStarting ? 1 : 0, p->velocityIterations, posIters,
+                               ctx->allowSleep ? 1 : 0, ctx->cnt);
         if (nIslands > 0 && ctx->allowSleep) {
             hipLaunchKernelGGL(k_static_last_island, dim3(nIslands), dim3(64), 0, s, w, r, nIslands);
             hipLaunchKernelGGL(k_static_awake, dim3(blocks(nb)), dim3(256), 0, s, w, r);
@@ -7079,7 +7266,7 @@ int b2g_destroy(b2g_context* ctx) {
     freeArray(ctx->fbody); freeArray(ctx->shape); freeArray(ctx->fmat); freeArray(ctx->ffilter);
     freeArray(ctx->fgroup); freeArray(ctx->fat); freeArray(ctx->moved); freeArray(ctx->range);
     for (ContactArrays* a : {&ctx->cur, &ctx->alt}) {
-        freeArray(a->cfA); freeArray(a->cfB); freeArray(a->cflags); freeArray(a->cmat); freeArray(a->cman);
+        freeArray(a->cfA); freeArray(a->cfB); freeArray(a->cflags); freeArray(a->cmat); freeArray(a->ctan); freeArray(a->cman);
         freeArray(a->ccolour); freeArray(a->ctoi); freeArray(a->ctoiCount);
     }
     freeToi(ctx);
@@ -7196,7 +7383,6 @@ int b2g_add_fixtures(b2g_context* ctx, const b2g_fixture_def* defs, int count, i
     if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
     for (int i = 0; i < count; ++i) {
         if (defs[i].body < 0 || defs[i].body >= ctx->nBodies) return fail(ctx, B2G_ERR_ARG, "fixture body index");
-        if (defs[i].sensor) return fail(ctx, B2G_ERR_UNSUPPORTED, "sensor fixtures are not on the GPU path");
         const int t = defs[i].shape.type;
         if (t != B2G_CIRCLE && t != B2G_EDGE && t != B2G_POLYGON)
             return fail(ctx, B2G_ERR_UNSUPPORTED, "shape type");
@@ -7335,8 +7521,6 @@ int b2g_step(b2g_context* ctx, const b2g_step_params* p) {
         return fail(ctx, B2G_ERR_ARG, "solver mode");
     if (p->learnedWarmStart && p->mode == B2G_MODE_REFERENCE)
         return fail(ctx, B2G_ERR_UNSUPPORTED, "the learned warm start runs with the coloured solver");
-    if (ctx->nJoints > 0 && p->mode == B2G_MODE_REFERENCE)
-        return fail(ctx, B2G_ERR_UNSUPPORTED, "joints run with the coloured solver");
     if (ctx->stepOpen) return fail(ctx, B2G_ERR_STATE, "a split step is open: finish it with b2g_step_solve");
     (void)hipSetDevice(ctx->device);
     return p->mode == B2G_MODE_REFERENCE ? stepReference(ctx, p) : stepColoured(ctx, p);
@@ -7420,6 +7604,38 @@ int b2g_override_contacts(b2g_context* ctx, const int32_t* ids, const int32_t* e
     return B2G_OK;
 }
 
+int b2g_get_contact_tangent_speeds(b2g_context* ctx, float* out, int cap) {
+    if (!ctx || cap < 0 || (cap > 0 && !out)) return B2G_ERR_ARG;
+    const int m = std::min(ctx->nContacts, cap);
+    if (m == 0) return B2G_OK;
+    HIPCHK(ctx, hipMemcpyAsync(out, ctx->cur.ctan, sizeof(float) * m, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
+int b2g_set_contact_tangent_speeds(b2g_context* ctx, const int32_t* ids, const float* speeds, int n) {
+    if (!ctx || n < 0 || (n > 0 && (!ids || !speeds))) return B2G_ERR_ARG;
+    for (int i = 0; i < n; ++i)
+        if (ids[i] < 0 || ids[i] >= ctx->nContacts) return fail(ctx, B2G_ERR_ARG, "contact index out of range");
+    if (n == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    int* dIds = nullptr;
+    float* dSp = nullptr;
+    hipError_t e = hipMalloc((void**)&dIds, sizeof(int) * n);
+    if (e == hipSuccess) e = hipMalloc((void**)&dSp, sizeof(float) * n);
+    if (e == hipSuccess) e = hipMemcpyAsync(dIds, ids, sizeof(int) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dSp, speeds, sizeof(float) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_set_tangent_speeds, dim3(blocks(n)), dim3(256), 0, s, ctx->view(), dIds, dSp, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(dIds);
+    (void)hipFree(dSp);
+    HIPCHK(ctx, e);
+    return B2G_OK;
+}
+
 int b2g_step_solve(b2g_context* ctx) {
     if (!ctx) return B2G_ERR_ARG;
     if (!ctx->stepOpen) return fail(ctx, B2G_ERR_STATE, "no split step is open");
@@ -7485,7 +7701,9 @@ int b2g_get_contacts(b2g_context* ctx, int32_t* meta, b2g_manifold* manifolds, i
             meta[4 * i + 0] = fa[i];
             meta[4 * i + 1] = fb[i];
             meta[4 * i + 2] = ((fl[i] & CF_TOUCHING) ? 1 : 0) | ((fl[i] & CF_ENABLED) ? 2 : 0);
-            if (ctx->stepOpen) meta[4 * i + 2] |= ((fl[i] & CF_DEAD) ? 4 : 0) | ((fl[i] & CF_UPDATED) ? 8 : 0);
+            if (ctx->stepOpen)
+                meta[4 * i + 2] |= ((fl[i] & CF_DEAD) ? 4 : 0) | ((fl[i] & CF_UPDATED) ? 8 : 0) |
+                                   ((fl[i] & CF_WAS_TOUCHING) ? 16 : 0);
             meta[4 * i + 3] = col[i];
         }
     }
@@ -8250,7 +8468,7 @@ int b2g_abi_sizes(int32_t* out, int cap) {
 // original would.
 namespace {
 constexpr uint32_t kSnapMagic = 0x53473242u;  // "B2GS"
-constexpr uint32_t kSnapVersion = 3;
+constexpr uint32_t kSnapVersion = 4;
 struct SnapHeader {
     uint32_t magic, version;
     int32_t nBodies, nFixtures, nContacts, nJoints;
@@ -8288,6 +8506,7 @@ void snapArrays(b2g_context* ctx, int nb, int nf, int nc, int nj, bool subWorlds
     visit(ctx->cur.cfB, sizeof(int) * nc);
     visit(ctx->cur.cflags, sizeof(int) * nc);
     visit(ctx->cur.cmat, sizeof(float2) * nc);
+    visit(ctx->cur.ctan, sizeof(float) * nc);
     visit(ctx->cur.cman, sizeof(b2g_manifold) * nc);
     visit(ctx->cur.ccolour, sizeof(int) * nc);
     visit(ctx->cur.ctoi, sizeof(float) * nc);

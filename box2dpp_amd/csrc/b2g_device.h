@@ -47,6 +47,7 @@ enum : int {
     CF_CONSTRAINT = 0x0200,  // solved this step (touching, enabled, not a sensor, awake)
     CF_NOOVERLAP = 0x0400,   // fat AABBs stopped overlapping
     CF_UPDATED = 0x0800,     // b2Contact::Update ran on it this step
+    CF_WAS_TOUCHING = 0x1000,  // touching before this step's update
 };
 enum : int { BT_STATIC = 0, BT_KINEMATIC = 1, BT_DYNAMIC = 2 };
 
@@ -194,6 +195,7 @@ struct WorldView {
     int* cfB;
     int* cflags;
     float2* cmat;      // friction restitution
+    float* ctan;       // tangent speed (b2Contact::m_tangentSpeed)
     b2g_manifold* cman;
     int* ccolour;
     float* ctoi;       // cached time of impact (with CF_TOI)

@@ -18,10 +18,14 @@ namespace {
 // enabled) and disables the contacts of a one-sided platform fixture while
 // the other body's origin is less than `clear` above the platform's top (a
 // one-sided platform after the reference testbed's OneSidedPlatform).
+// With a conveyor fixture set, PreSolve also gives its contacts the belt's
+// tangent speed (b2Contact::SetTangentSpeed).
 struct Recorder : public b2ContactListener {
     const std::vector<b2Fixture*>* fixtures = nullptr;
     int platform = -1;
     float top = 0.0f, clear = 0.0f;
+    int conveyor = -1;  // fixture whose contacts get the belt speed
+    float belt = 0.0f;
     int step = 0;
     std::vector<int32_t> events;  // step, kind, fixture A, fixture B, flag
     int indexOf(const b2Fixture* f) const {
@@ -40,6 +44,10 @@ struct Recorder : public b2ContactListener {
             const b2Body* other = (a == platform ? c->GetFixtureB() : c->GetFixtureA())->GetBody();
             if (other->GetPosition()[1] < top + clear) c->SetEnabled(false);
         }
+        // a conveyor belt after the testbed's ConveyorBelt: the tangent speed
+        // of the belt's contacts, its sign by the belt's side of the contact
+        if (a == conveyor) c->SetTangentSpeed(belt);
+        if (b == conveyor) c->SetTangentSpeed(-belt);
         record(3, c, c->IsEnabled() ? 1 : 0);
     }
     void PostSolve(b2Contact* c, const b2ContactImpulse*) override { record(4, c, 0); }
@@ -555,6 +563,16 @@ void b2pp_world_set_recorder(void* h, int platform, float top, float clear) {
     r->recorder.platform = platform;
     r->recorder.top = top;
     r->recorder.clear = clear;
+    r->world.SetContactListener(&r->recorder);
+}
+
+// Installs the recording listener (record only, unless set before) with
+// `fixture` as a conveyor belt of tangent speed `speed`.
+void b2pp_world_set_conveyor(void* h, int fixture, float speed) {
+    Runner* r = (Runner*)h;
+    r->recorder.fixtures = &r->fixtures;
+    r->recorder.conveyor = fixture;
+    r->recorder.belt = speed;
     r->world.SetContactListener(&r->recorder);
 }
 

@@ -88,6 +88,13 @@ void b2Fixture::SetRestitution(float r) {
     m_restitution = r;
     MarkUpdated();
 }
+// REF/Dynamics/b2Fixture.cpp:223-230
+void b2Fixture::SetSensor(bool sensor) {
+    if (sensor == m_isSensor) return;
+    m_body->SetAwake(true);
+    m_isSensor = sensor;
+    MarkUpdated();
+}
 
 // ------------------------------------------------------------ body
 // b2Body::b2Body, REF/Dynamics/b2Body.cpp:25-90
@@ -1719,8 +1726,9 @@ b2Contact* b2World::GetContactList() {
         std::vector<int32_t> meta(4 * (size_t)n);
         std::vector<b2g_manifold> man(n);
         b2g_get_contacts(m_ctx, meta.data(), man.data(), n, &n);
-        std::vector<float> mat(2 * (size_t)n);
+        std::vector<float> mat(2 * (size_t)n), tangent(n);
         const bool haveMat = n > 0 && b2g_get_contact_materials(m_ctx, mat.data(), n) == B2G_OK;
+        const bool haveTangent = n > 0 && b2g_get_contact_tangent_speeds(m_ctx, tangent.data(), n) == B2G_OK;
         m_contacts.assign(n, b2Contact());
         // Newest contacts first, as the reference's contact list.
         for (int i = 0; i < n; ++i) {
@@ -1735,6 +1743,7 @@ b2Contact* b2World::GetContactList() {
             c.m_friction = haveMat ? mat[2 * src] : sqrtf(c.m_fixtureA->m_friction * c.m_fixtureB->m_friction);
             c.m_restitution = haveMat ? mat[2 * src + 1]
                                       : std::max(c.m_fixtureA->m_restitution, c.m_fixtureB->m_restitution);
+            c.m_tangentSpeed = haveTangent ? tangent[src] : 0.0f;
             const b2g_manifold& m = man[src];
             c.m_manifold.pointCount = m.pointCount;
             c.m_manifold.type = (b2Manifold::Type)m.type;
@@ -1786,11 +1795,12 @@ int32_t b2World::CollideWithListener(const void* params) {
     b2g_get_contacts(m_ctx, nullptr, nullptr, 0, &n);
     std::vector<int32_t> meta(4 * (size_t)n);
     std::vector<b2g_manifold> man(n), old(n);
-    std::vector<float> mat(2 * (size_t)n);
+    std::vector<float> mat(2 * (size_t)n), tangent(n);
     if (n > 0) {
         if ((rc = b2g_get_contacts(m_ctx, meta.data(), man.data(), n, &n)) != B2G_OK) return rc;
         if ((rc = b2g_get_old_manifolds(m_ctx, old.data(), n, nullptr)) != B2G_OK) return rc;
         if ((rc = b2g_get_contact_materials(m_ctx, mat.data(), n)) != B2G_OK) return rc;
+        if ((rc = b2g_get_contact_tangent_speeds(m_ctx, tangent.data(), n)) != B2G_OK) return rc;
     }
     auto toHost = [](const b2g_manifold& m, b2Manifold& out) {
         out.pointCount = m.pointCount;
@@ -1804,8 +1814,8 @@ int32_t b2World::CollideWithListener(const void* params) {
             out.points[k].id.key = m.points[k].id;
         }
     };
-    std::vector<int32_t> ids, enabled;
-    std::vector<float> fr, re;
+    std::vector<int32_t> ids, enabled, tIds;
+    std::vector<float> fr, re, tSpeeds;
     for (int i = n - 1; i >= 0; --i) {
         const int flags = meta[4 * i + 2];
         const bool dead = (flags & 4) != 0, updated = (flags & 8) != 0, touching = (flags & 1) != 0;
@@ -1817,6 +1827,7 @@ int32_t b2World::CollideWithListener(const void* params) {
         c.m_indexB = meta[4 * i + 1] - c.m_fixtureB->m_index;
         c.m_friction = mat[2 * i];
         c.m_restitution = mat[2 * i + 1];
+        c.m_tangentSpeed = tangent[i];
         c.m_enabled = true;
         if (dead) {
             c.m_touching = touching;
@@ -1824,12 +1835,13 @@ int32_t b2World::CollideWithListener(const void* params) {
             if (touching) m_contactListener->EndContact(&c);
             continue;
         }
-        const bool wasTouching = old[i].pointCount > 0;
+        const bool wasTouching = (flags & 16) != 0;
+        const bool sensor = c.m_fixtureA->m_isSensor || c.m_fixtureB->m_isSensor;
         c.m_touching = touching;
         toHost(man[i], c.m_manifold);
         if (!wasTouching && touching) m_contactListener->BeginContact(&c);
         if (wasTouching && !touching) m_contactListener->EndContact(&c);
-        if (touching) {
+        if (touching && !sensor) {
             b2Manifold oldManifold;
             toHost(old[i], oldManifold);
             m_contactListener->PreSolve(&c, &oldManifold);
@@ -1839,10 +1851,16 @@ int32_t b2World::CollideWithListener(const void* params) {
                 fr.push_back(c.m_friction);
                 re.push_back(c.m_restitution);
             }
+            if (c.m_tangentSpeed != tangent[i]) {
+                tIds.push_back(i);
+                tSpeeds.push_back(c.m_tangentSpeed);
+            }
         }
     }
     if (!ids.empty())
         rc = b2g_override_contacts(m_ctx, ids.data(), enabled.data(), fr.data(), re.data(), (int)ids.size());
+    if (rc == B2G_OK && !tIds.empty())
+        rc = b2g_set_contact_tangent_speeds(m_ctx, tIds.data(), tSpeeds.data(), (int)tIds.size());
     // touching set as the listener has seen it; what the TOI phase changes
     // after this is reported once the step is over
     m_touchingKeys.clear();

@@ -376,6 +376,70 @@ inline Scene oneway(int n) {
     return s;
 }
 
+// Sensors: n boxes and circles (seeded sizes and spin) dropped through a
+// static sensor slab onto a ground edge; every third body carries a second,
+// sensor, circle of radius 1.2 m (density 0), so sensors meet sensors,
+// dynamic bodies and the ground. A sensor only reports touching
+// (b2Contact::Update's sensor branch) and never joins a solve. Seed 0: one
+// row of bodies 3 m apart, of fixed rotation, that land apart (one solid
+// contact per island).
+inline Scene sensors(int n, uint32_t seed) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-30.0f, 0.0f, 30.0f, 0.0f));
+    addBody(s, defaultBody(0, 0.0f, 8.0f));
+    FixtureDef slab = boxFixture(14.0f, 1.5f, 0.0f, 0.6f);
+    slab.sensor = 1;
+    addFixture(s, slab);
+    Uniform U(seed);
+    for (int i = 0; i < n; ++i) {
+        BodyDef b = defaultBody(2, -11.0f + 2.4f * (float)(i % 10) + 0.4f * U(), 14.0f + 2.4f * (float)(i / 10));
+        b.w = -2.0f + 4.0f * U();
+        if (seed == 0) {
+            b.px = -13.5f + 3.0f * (float)i;
+            b.py = 14.0f;
+            b.w = 0.0f;
+            b.fixedRotation = 1;
+        }
+        addBody(s, b);
+        FixtureDef f;
+        if (i & 1) {
+            f = defaultFixture(kCircle);
+            f.radius = 0.35f + 0.3f * U();
+        } else {
+            const float h = 0.35f + 0.3f * U();
+            f = boxFixture(h, h, 1.0f, 0.6f);
+        }
+        f.density = 1.0f; f.friction = 0.6f;
+        addFixture(s, f);
+        if (i % 3 == 0) {
+            FixtureDef halo = defaultFixture(kCircle);
+            halo.radius = 1.2f;
+            halo.sensor = 1;
+            addFixture(s, halo);
+        }
+    }
+    return s;
+}
+
+// A conveyor belt: a static platform 20 m long at y = 5 (fixture 1) under n
+// boxes of fixed rotation dropped 2.5 m apart, which the belt (a listener
+// setting the contacts' tangent speed) carries off its end onto the ground.
+inline Scene conveyor(int n) {
+    Scene s;
+    addBody(s, defaultBody(0, 0.0f, 0.0f));
+    addFixture(s, edgeFixture(-40.0f, 0.0f, 40.0f, 0.0f));
+    addBody(s, defaultBody(0, 0.0f, 5.0f));
+    addFixture(s, boxFixture(10.0f, 0.25f, 0.0f, 0.8f));
+    for (int i = 0; i < n; ++i) {
+        BodyDef b = defaultBody(2, -9.0f + 2.5f * (float)i, 6.0f);
+        b.fixedRotation = 1;
+        addBody(s, b);
+        addFixture(s, boxFixture(0.4f, 0.4f, 1.0f, 0.8f));
+    }
+    return s;
+}
+
 // Fast bodies for continuous collision: n small circles and boxes fired at
 // 80-200 m/s from the left, over a ground edge, through a column of five
 // dynamic boxes, at a static wall 0.2 m thick (3 m of travel per step would
@@ -870,7 +934,9 @@ inline Scene gears(int n) {
 // boxes on motor joints; seed 0 drives every third from the previous box),
 // "pulley" (n pulleys), "mouse" (n boxes on mouse joints), "prismatic" (n
 // boxes on prismatic joints), "wheels" (n carts; seed 1 gives each one
-// wheel on flat ground), "gears" (n gear trains). Unknown names give an empty scene.
+// wheel on flat ground), "gears" (n gear trains), "sensors" (n bodies through
+// a sensor slab, seed), "conveyor" (n boxes on a belt). Unknown names give an
+// empty scene.
 inline Scene byName(const char* kind, int n, uint32_t seed) {
     const std::string k(kind ? kind : "");
     if (k == "hello") return hello();
@@ -895,6 +961,8 @@ inline Scene byName(const char* kind, int n, uint32_t seed) {
     if (k == "prismatic") return prismatics(n > 0 ? n : 12);
     if (k == "wheels") return wheels(n > 0 ? n : 6, seed == 1);
     if (k == "gears") return gears(n > 0 ? n : 4);
+    if (k == "sensors") return sensors(n > 0 ? n : 30, seed);
+    if (k == "conveyor") return conveyor(n > 0 ? n : 6);
     return Scene();
 }
 

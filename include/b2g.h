@@ -202,6 +202,11 @@ typedef struct b2g_step_stats {
     int32_t fallback;          /* 1 if the learned start was replaced by the plain one */
     float residual;            /* RMS residual checked for the fallback, -1 if none */
     int32_t toiEvents;         /* TOI events solved this step (continuous collision) */
+    int32_t queries;           /* moved proxies whose broad-phase queries ran */
+    int32_t scanned;           /* grid entries and wide proxies those queries examined */
+    int32_t wideConstraints;   /* constraints in the colours that get a solver launch
+                                  each (not the fused trailing colours, not the
+                                  overflow colour) */
 } b2g_step_stats;
 
 typedef struct b2g_context b2g_context;
@@ -717,6 +722,15 @@ int b2g_get_contact_solved(b2g_context* ctx, int32_t* out, int cap);
  * order of b2g_get_contacts (b2Contact::GetFriction / GetRestitution). */
 int b2g_get_contact_materials(b2g_context* ctx, float* out, int cap);
 int b2g_get_old_manifolds(b2g_context* ctx, b2g_manifold* out, int cap, int* count);
+/* Tangent speeds of contacts [0, cap) in the order of b2g_get_contacts
+ * (b2Contact::GetTangentSpeed, Box2D/Dynamics/Contacts/b2Contact.h:142,
+ * 351-354); 0 for a new contact. */
+int b2g_get_contact_tangent_speeds(b2g_context* ctx, float* out, int cap);
+/* b2Contact::SetTangentSpeed (b2Contact.h:139, 346-349) per contact index:
+ * the speed (m/s along the contact tangent, a conveyor belt) holds until
+ * changed or the contact ends. Between steps, or inside a split step before
+ * b2g_step_solve (from PreSolve). Both solvers read it. */
+int b2g_set_contact_tangent_speeds(b2g_context* ctx, const int32_t* ids, const float* speeds, int n);
 int b2g_override_contacts(b2g_context* ctx, const int32_t* ids, const int32_t* enabled, const float* friction,
                           const float* restitution, int n);
 int b2g_step_solve(b2g_context* ctx);

@@ -121,6 +121,9 @@ public:
     b2Shape* GetShape() { return m_shape.get(); }
     const b2Shape* GetShape() const { return m_shape.get(); }
     bool IsSensor() const { return m_isSensor; }
+    // b2Fixture::SetSensor (Box2D/Dynamics/b2Fixture.cpp:223-230): wakes the body
+    // when the flag changes.
+    void SetSensor(bool sensor);
     const b2Filter& GetFilterData() const { return m_filter; }
     void SetFilterData(const b2Filter& filter);
     b2Body* GetBody() { return m_body; }
@@ -263,15 +266,17 @@ public:
     int32_t GetChildIndexB() const { return m_indexB; }
     float GetFriction() const { return m_friction; }
     float GetRestitution() const { return m_restitution; }
-    float GetTangentSpeed() const { return 0.0f; }
-    // Box2D/Dynamics/Contacts/b2Contact.h:98-131. Edits made inside
+    float GetTangentSpeed() const { return m_tangentSpeed; }
+    // Box2D/Dynamics/Contacts/b2Contact.h:98-142. Edits made inside
     // b2ContactListener::PreSolve reach the step: SetEnabled for this step,
-    // friction and restitution for the rest of the contact's life.
+    // friction, restitution and the tangent speed (a conveyor belt) for the
+    // rest of the contact's life.
     void SetEnabled(bool flag) { m_enabled = flag; }
     void SetFriction(float friction) { m_friction = friction; }
     void ResetFriction();
     void SetRestitution(float restitution) { m_restitution = restitution; }
     void ResetRestitution();
+    void SetTangentSpeed(float speed) { m_tangentSpeed = speed; }
 
 private:
     friend class b2World;
@@ -280,7 +285,7 @@ private:
     int32_t m_indexA = 0, m_indexB = 0;  // child of a chain fixture
     b2Manifold m_manifold;
     bool m_touching = false, m_enabled = true;
-    float m_friction = 0.0f, m_restitution = 0.0f;
+    float m_friction = 0.0f, m_restitution = 0.0f, m_tangentSpeed = 0.0f;
     b2Contact* m_next = nullptr;
 };
 

@@ -1493,7 +1493,6 @@ struct World {
         f.category = (uint16_t)d.categoryBits;
         f.mask = (uint16_t)d.maskBits;
         f.group = (int16_t)d.groupIndex;
-        if (f.sensor) unsupported |= 2;
         Body& b = bodies[bi];
         f.proxy = (b.flags & kActive) != 0;
         if (f.proxy) {
@@ -1942,6 +1941,22 @@ struct World {
         --contactCount;
     }
 
+    // b2TestOverlap, REF/Collision/b2Collision.cpp:237-255: GJK distance from
+    // an empty cache, with the radii taken off as b2Distance's useRadii does
+    // (REF/Collision/b2Distance.cpp:579-602), under 10 epsilon.
+    static bool testOverlap(const b2g_shape* a, const Xf& xfA, const b2g_shape* b, const Xf& xfB) {
+        const Proxy pA = proxyOf(a), pB = proxyOf(b);
+        SimplexCache cache;
+        cache.count = 0;
+        cache.metric = 0.0f;
+        V2 wA, wB;
+        float d;
+        gjk(&wA, &wB, &d, &cache, pA, xfA, pB, xfB);
+        if (d > pA.radius + pB.radius && d > FLT_EPSILON) d -= pA.radius + pB.radius;
+        else d = 0.0f;
+        return d < 10.0f * FLT_EPSILON;
+    }
+
     // b2Contact::Update, REF/Dynamics/Contacts/b2Contact.cpp:171-251
     void updateContact(int ci) {
         Contact& c = contacts[ci];
@@ -1950,6 +1965,16 @@ struct World {
         bool wasTouching = (c.flags & cTouching) != 0;
         Body& bA = bodies[fixtures[c.fA].body];
         Body& bB = bodies[fixtures[c.fB].body];
+        if (fixtures[c.fA].sensor || fixtures[c.fB].sensor) {
+            // the sensor branch (:189-197): overlap only, no manifold, no wake-up
+            const bool touching = testOverlap(&fixtures[c.fA].shape, bA.xf, &fixtures[c.fB].shape, bB.xf);
+            c.manifold.pointCount = 0;
+            if (touching) c.flags |= cTouching;
+            else c.flags &= ~cTouching;
+            if (!wasTouching && touching) events.begin++;
+            if (wasTouching && !touching) events.end++;
+            return;
+        }
         collide(&c.manifold, &fixtures[c.fA].shape, bA.xf, &fixtures[c.fB].shape, bB.xf);
         bool touching = c.manifold.pointCount > 0;
         for (int i = 0; i < c.manifold.pointCount; ++i) {

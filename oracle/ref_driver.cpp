@@ -36,10 +36,14 @@ namespace {
 // the other body's origin is less than `clear` above the platform's top (a
 // one-sided platform after the testbed's OneSidedPlatform,
 // Testbed/Tests/OneSidedPlatform.h).
+// With a conveyor fixture set, PreSolve also gives its contacts the belt's
+// tangent speed (b2Contact::SetTangentSpeed).
 struct Recorder : public b2ContactListener {
     std::vector<b2Fixture*>* fixtures = nullptr;
     int platform = -1;
     float top = 0.0f, clear = 0.0f;
+    int conveyor = -1;  // fixture whose contacts get the belt speed
+    float belt = 0.0f;
     int step = 0;
     std::vector<int32_t> events;  // step, kind, fixture A, fixture B, flag
     int indexOf(const b2Fixture* f) const {
@@ -59,6 +63,10 @@ struct Recorder : public b2ContactListener {
             const b2Body* other = (a == platform ? c->GetFixtureB() : c->GetFixtureA())->GetBody();
             if (other->GetPosition().y < top + clear) c->SetEnabled(false);
         }
+        // a conveyor belt after the testbed's ConveyorBelt: the tangent speed
+        // of the belt's contacts, its sign by the belt's side of the contact
+        if (a == conveyor) c->SetTangentSpeed(belt);
+        if (b == conveyor) c->SetTangentSpeed(-belt);
         record(3, c, c->IsEnabled() ? 1 : 0);
     }
     void PostSolve(b2Contact* c, const b2ContactImpulse*) override { record(4, c, 0); }
@@ -469,6 +477,16 @@ void ref_world_set_recorder(void* h, int platform, float top, float clear) {
     w->recorder.platform = platform;
     w->recorder.top = top;
     w->recorder.clear = clear;
+    w->world->SetContactListener(&w->recorder);
+}
+
+// Installs the recording listener (record only, unless set before) with
+// `fixture` as a conveyor belt of tangent speed `speed`.
+void ref_world_set_conveyor(void* h, int fixture, float speed) {
+    RefWorld* w = (RefWorld*)h;
+    w->recorder.fixtures = &w->fixtures;
+    w->recorder.conveyor = fixture;
+    w->recorder.belt = speed;
     w->world->SetContactListener(&w->recorder);
 }
 

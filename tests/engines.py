@@ -45,6 +45,7 @@ STATS = np.dtype([
     ("colours", "<i4"), ("colouringRounds", "<i4"), ("newPairs", "<i4"), ("positionIterations", "<i4"),
     ("beginEvents", "<i4"), ("endEvents", "<i4"), ("overflow", "<i4"), ("minSeparation", "<f4"),
     ("cellSize", "<f4"), ("freshPoints", "<i4"), ("fallback", "<i4"), ("residual", "<f4"), ("toiEvents", "<i4"),
+    ("queries", "<i4"), ("scanned", "<i4"), ("wideConstraints", "<i4"),
 ])
 
 SCHEDULE = np.dtype([("islands", "<i4"), ("collideOrder", "<i4"), ("solverBlock", "<i4"), ("collideBlock", "<i4"),

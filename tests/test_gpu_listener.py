@@ -122,3 +122,54 @@ def test_begin_end_balanced_with_ccd(gpu, ccd):
     assert {k for k, v in open_pairs.items() if v} == touching
     assert (toi > 20) == ccd
     g.close()
+
+
+def test_sensors_match_reference(gpu, ref):
+    """Sensor fixtures through the listener: a row of bodies of fixed
+    rotation falling through a sensor slab, every third with a sensor halo,
+    landing apart (one solid contact per island). Poses equal the reference
+    build's bit for bit, and so does the sequence of Begin / End / PreSolve
+    calls (sensor contacts report Begin and End, never PreSolve); PostSolve
+    per step as a set."""
+    g = gpu.world("sensors", 8, 0, sleep=True)
+    r = ref.world("sensors", 8, 0, sleep=True)
+    for w in (g, r):
+        w.e.fn("world_set_recorder")(w.h, -1, F(0.0), F(0.0))
+    for s in range(200):
+        g.step(1)
+        r.step(1)
+        assert np.array_equal(_bits(g.bodies()[:, :11]), _bits(r.bodies()[:, :11])), f"step {s + 1}"
+    ge, re_ = _events(g), _events(r)
+    inside = lambda e: e[e[:, 1] != 4]
+    assert np.array_equal(inside(ge), inside(re_))
+    post = lambda e: sorted(map(tuple, e[e[:, 1] == 4].tolist()))
+    assert post(ge) == post(re_)
+    slab = ((ge[:, 2] == 1) | (ge[:, 3] == 1))
+    assert ((ge[:, 1] == 1) & slab).sum() == 8 + 3 and not ((ge[:, 1] == 3) & slab).any()
+    g.close()
+    r.close()
+
+
+def test_conveyor_matches_reference(gpu, ref):
+    """b2Contact::SetTangentSpeed from PreSolve (a conveyor belt carrying
+    boxes of fixed rotation off its end). While each box has one solid
+    contact (its island's only one: the first 180 steps, until boxes that
+    fell off the end land on each other) poses equal the reference build's
+    bit for bit, and so do the listener calls; all boxes end up beyond the
+    belt's end."""
+    g = gpu.world("conveyor", 6, 0, sleep=True)
+    r = ref.world("conveyor", 6, 0, sleep=True)
+    for w in (g, r):
+        w.e.fn("world_set_conveyor")(w.h, 1, F(5.0))
+    exact = 180
+    for s in range(exact):
+        g.step(1)
+        r.step(1)
+        assert np.array_equal(_bits(g.bodies()[:, :11]), _bits(r.bodies()[:, :11])), f"step {s + 1}"
+    ge, re_ = _events(g), _events(r)
+    inside = lambda e: e[e[:, 1] != 4]
+    assert np.array_equal(inside(ge), inside(re_))
+    g.step(220)
+    assert np.all(g.bodies()[2:, E.CX] > 10.0)
+    g.close()
+    r.close()

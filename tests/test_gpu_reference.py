@@ -38,6 +38,8 @@ def _canon(b):
     ("pile", 400, 3, 300, True, 8, 3),
     ("pile", 1200, 9, 200, False, 8, 3),
     ("terrain", 60, 1, 300, True, 8, 3),     # chain shapes (children as edges without skin in their boxes)
+    ("sensors", 30, 1, 300, True, 8, 3),     # sensor fixtures: a slab and halos, overlap-only contacts
+    ("sensors", 40, 2, 200, False, 8, 3),
 ])
 def test_reference_mode_bitexact(gpu, orc, wide_trig, kind, n, seed, steps, sleep, vel, pos):
     g = gpu.world(kind, n, seed, sleep=sleep)

@@ -49,6 +49,7 @@ def _contacts_equal(ga, oa, impulses=True):
     ("pyramid", 20, 0, 150),
     ("pile", 400, 3, 250),
     ("pile", 3000, 5, 120),
+    ("sensors", 40, 2, 150),   # sensor contacts: overlap tests, no manifold points
 ])
 def test_zero_iteration_trajectory_bitexact(gpu, orc, wide_trig, kind, n, seed, steps):
     g = gpu.world(kind, n, seed, sleep=False)

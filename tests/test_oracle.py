@@ -46,6 +46,9 @@ F = ctypes.c_float
     ("wheels", 6, 0, 400, True, 8, 3),         # wheel joints: carts on suspension springs over bumps, driven
     ("wheels", 6, 1, 300, False, 4, 2),
     ("gears", 4, 0, 300, True, 8, 3),          # gear joints: revolute-revolute and revolute-prismatic (rack) trains
+    ("sensors", 30, 1, 300, True, 8, 3),       # sensors: a slab bodies fall through, sensor halos on bodies
+    ("sensors", 40, 2, 200, False, 8, 3),
+    ("sensors", 8, 0, 200, True, 8, 3),
 ])
 def test_world_trajectory_bitexact(ref, orc, kind, n, seed, steps, sleep, vel, pos):
     a = ref.world(kind, n, seed, sleep=sleep)
@@ -381,6 +384,69 @@ def test_one_sided_platform_listener(ref):
     for k, per_box in ((1, 2), (2, 1)):
         assert ((ev[:, 1] == k) & plat).sum() == 8 * per_box, k
     assert ((ev[:, 1] == 3) & (ev[:, 4] == 0)).sum() > 8
+    w.close()
+
+
+def test_sensor_facts(ref, orc):
+    """What b2Contact::Update's sensor branch fixes, on the reference build
+    (listener record) and the oracle (contact table): the row of bodies falls
+    through the sensor slab (fixture 1) to the ground, each solid fixture
+    and each halo (the sensor circle after every third body's solid)
+    beginning and ending touching the slab once; no PreSolve or PostSolve names a
+    sensor; every sensor contact keeps no manifold points; the bodies rest
+    on the ground."""
+    n = 8
+    w = ref.world("sensors", n, 0, sleep=True)
+    w.e.fn("world_set_recorder")(w.h, -1, F(0.0), F(0.0))
+    o = orc.world("sensors", n, 0, sleep=True)
+    for _ in range(200):
+        w.step(1)
+        o.step(1)
+    solid, halo = [], []
+    f = 2
+    for i in range(n):
+        solid.append(f)
+        f += 1
+        if i % 3 == 0:
+            halo.append(f)
+            f += 1
+    ev = _events(w)
+    with_slab = (ev[:, 2] == 1) | (ev[:, 3] == 1)
+    for k in (1, 2):
+        other = np.where(ev[:, 2] == 1, ev[:, 3], ev[:, 2])[(ev[:, 1] == k) & with_slab]
+        assert sorted(other.tolist()) == sorted(solid + halo), (k, other)
+    sensors = set(halo) | {1}
+    for kind in (3, 4):
+        named = ev[ev[:, 1] == kind][:, 2:4]
+        assert not (set(named.ravel().tolist()) & sensors), kind
+    meta, man = o.contacts()
+    sens = np.isin(meta[:, 0], list(sensors)) | np.isin(meta[:, 1], list(sensors))
+    assert sens.any() and np.all(man[sens]["count"] == 0)
+    b = w.bodies()[2:]
+    assert np.all(b[:, E.CY] < 1.0), b[:, E.CY]
+    w.close()
+    o.close()
+
+
+def test_conveyor_belt_facts(ref):
+    """b2Contact::SetTangentSpeed on the reference build (a listener giving
+    the belt's contacts 5 m/s from PreSolve): a box that lands on the belt
+    slides until it moves with the belt, so while on the belt it reaches
+    v_x = 5 m/s (friction 0.8: 0.6 s at mu g); the boxes leave the belt's
+    right end (x = 10) and come to rest beyond it, below its top."""
+    n = 6
+    w = ref.world("conveyor", n, 0, sleep=True)
+    w.e.fn("world_set_conveyor")(w.h, 1, F(5.0))
+    top = np.full(n, -1e9)
+    for _ in range(600):
+        w.step(1)
+        b = w.bodies()[2:]
+        on = np.abs(b[:, E.CY] - 5.65) < 0.02
+        top = np.where(on, np.maximum(top, b[:, E.VX]), top)
+    assert np.all(np.abs(top - 5.0) < 1e-3), top
+    b = w.bodies()[2:]
+    assert np.all(b[:, E.CX] > 10.0) and np.all(b[:, E.CY] < 5.0), b[:, [E.CX, E.CY]]
+    assert np.all(np.abs(b[:, [E.VX, E.VY]]) < 0.05), b[:, [E.VX, E.VY]]
     w.close()
 
 
