@@ -58,17 +58,20 @@ HBM_PEAK_GBS = 8000.0
 # schedule the counters ran with (the fusion of trailing colours changes
 # which colours the solver kernels see); the summaries with the commands are
 # under profiles/. Another schedule reports traffic as null.
-DEFAULT_TRAFFIC_BYTES = {(0, 1, 256, 256, 64, 2048, 0): {}}
+DEFAULT_TRAFFIC_BYTES = {(0, 1, 256, 256, 64, 2048, 0, 0): {
+    "k_solve_velocity": 7822234.0, "k_solve_position": 6470861.0, "k_collide": 551341773.0,
+    "k_pairs_query": 75245978.0}}
 # The pile's step schedule as the autotuner picks it on this workload (step
 # 500 of a warm-up copy): fixed so the HBM traffic above applies; the batched
 # and stress worlds are tuned on every run.
-PILE_SCHEDULE = "0,1,256,256,64,2048,0"
+PILE_SCHEDULE = "0,1,256,256,64,2048,0,0"
 DT = 1.0 / 60.0
 # b2g_schedule: island labelling method, collide order, workgroup sizes of
 # the solver, narrowphase and broad-phase query kernels, trailing-colour
 # fusion, cluster size of the clustered solve
 SCHEDULE = np.dtype([("islands", "<i4"), ("collideOrder", "<i4"), ("solverBlock", "<i4"), ("collideBlock", "<i4"),
-                     ("queryBlock", "<i4"), ("tailColours", "<i4"), ("clusterBodies", "<i4")])
+                     ("queryBlock", "<i4"), ("tailColours", "<i4"), ("clusterBodies", "<i4"),
+                     ("persistentSolve", "<i4")])
 _VEC2 = np.dtype([("x", "<f4"), ("y", "<f4")])
 _MPOINT = np.dtype([("lp", _VEC2), ("ni", "<f4"), ("ti", "<f4"), ("id", "<u4")])
 # b2g_manifold (include/b2g.h), 64 bytes
@@ -635,6 +638,8 @@ def headline(lib, args):
     t0 = time.perf_counter()
     for k in range(0, prefix, 100):
         step(lib, h, min(100, prefix - k), args.vel, args.pos)
+        if k % 500 == 400:
+            progress(f"headline: step {k + 100} of the untimed prefix")
     lib.b2pp_world_sync(h)
     prefix_s = time.perf_counter() - t0
     state = get_bodies(lib, h) if prefix > 0 else None
@@ -671,8 +676,22 @@ def headline(lib, args):
 #                     vertex, 5.5 vertices on half the pile's fixtures);
 #   k_pairs_query     36 B per query (proxy id 4, fat box 16, cell range 16)
 #                     + 28 B per grid entry scanned (cell key 8, id 4, fat box
-#                     16) + 8 B per new pair written.
+#                     16) + 8 B per new pair written;
+#   k_solve_persistent (the one-launch solve: every pass of the step's solve)
+#                     the warm start, 124 B per constraint (body indices 8,
+#                     info 4, masses 16, normal 16, impulses 16, the two
+#                     bodies' (v, w) rows read and written 64) + 16 B per point
+#                     (anchor offsets); the velocity passes as above; the
+#                     impulse store, 24 B per constraint (slot 4, info 4,
+#                     impulses 16) + 8 B per point (manifold impulses); the
+#                     position integration, 56 B per body (flags and island 8,
+#                     (v, w) 16, (c, a) read and written 32); the position
+#                     passes as above; the transform sync, 48 B per body (flags
+#                     and island 8, (c, a) 16, local centre 8, transform 16).
 BYTES_PER_CONSTRAINT_V, BYTES_PER_POINT_V = 132.0, 48.0
+BYTES_PER_CONSTRAINT_WARM, BYTES_PER_POINT_WARM = 124.0, 16.0
+BYTES_PER_CONSTRAINT_STORE, BYTES_PER_POINT_STORE = 24.0, 8.0
+BYTES_PER_BODY_INTEGRATE_X, BYTES_PER_BODY_SYNC = 56.0, 48.0
 BYTES_PER_CONSTRAINT_X = 156.0
 BYTES_PER_CONTACT_COLLIDE = 360.0
 BYTES_PER_QUERY, BYTES_PER_SCANNED, BYTES_PER_NEW_PAIR = 36.0, 28.0, 8.0
@@ -685,7 +704,14 @@ def kernel_bytes(s2, contacts_before, vel):
     ratio of points to constraints)."""
     wide = s2.wideConstraints
     pts = s2.points * wide / s2.constraints if s2.constraints else 0.0
-    return {"k_solve_velocity": vel * (BYTES_PER_CONSTRAINT_V * wide + BYTES_PER_POINT_V * pts),
+    cons, allpts = s2.constraints, s2.points
+    solve = (BYTES_PER_CONSTRAINT_WARM * cons + BYTES_PER_POINT_WARM * allpts
+             + vel * (BYTES_PER_CONSTRAINT_V * cons + BYTES_PER_POINT_V * allpts)
+             + BYTES_PER_CONSTRAINT_STORE * cons + BYTES_PER_POINT_STORE * allpts
+             + (BYTES_PER_BODY_INTEGRATE_X + BYTES_PER_BODY_SYNC) * s2.bodies
+             + s2.positionIterations * BYTES_PER_CONSTRAINT_X * cons)
+    return {"k_solve_persistent": solve,
+            "k_solve_velocity": vel * (BYTES_PER_CONSTRAINT_V * wide + BYTES_PER_POINT_V * pts),
             "k_solve_position": s2.positionIterations * BYTES_PER_CONSTRAINT_X * wide,
             "k_collide": BYTES_PER_CONTACT_COLLIDE * contacts_before,
             "k_pairs_query": BYTES_PER_QUERY * s2.queries + BYTES_PER_SCANNED * s2.scanned
@@ -718,6 +744,9 @@ def roofline_pass(lib, args, total):
     phases = {k: v for k, v in entries.items() if not k.startswith("k_")}
     kernels = {k: v for k, v in entries.items() if k.startswith("k_")}
     traffic = args.traffic_bytes or {}
+    # the dominant kernel: the one-launch solve when the schedule runs it,
+    # else the per-colour velocity solve
+    dominant = "k_solve_persistent" if kernels.get("k_solve_persistent", (0, 0.0))[0] > 0 else "k_solve_velocity"
 
     def line(name):
         launches, ms = kernels.get(name, (0, 0.0))
@@ -725,16 +754,20 @@ def roofline_pass(lib, args, total):
         per_launch = alg.get(name, 0.0) / max(launches, 1)
         achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
         return {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic.get(name), "bytes_per_launch": per_launch,
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": traffic.get(name, traffic.get("dominant") if name == dominant else None),
+                "bytes_per_launch": per_launch,
                 "avg_launch_us": avg_ms * 1e3, "launches_per_step": launches / max(total, 1),
                 "kernel_ms_per_step": ms / max(total, 1)}
 
-    roof = line("k_solve_velocity")
+    roof = line(dominant)
     ph_launches, ph_ms = phases.get("solve_velocity", (0, 0.0))
     roof["stream_us_per_launch"] = 1e3 * ph_ms / max(ph_launches, 1)
     roof["window"] = f"steps 1-{total} of a fresh pile (the whole workload; the timed steps are its last ones)"
     total_phase_ms = sum(ms for _, ms in phases.values())
-    extra = {"kernel_rooflines": {k: line(k) for k in ("k_solve_position", "k_pairs_query", "k_collide")},
+    others = [k for k in ("k_solve_velocity", "k_solve_position", "k_pairs_query", "k_collide")
+              if k != dominant and kernels.get(k, (0, 0.0))[0] > 0]
+    extra = {"kernel_rooflines": {k: line(k) for k in others},
              "phase_ms_per_step": {k: round(v[1] / max(total, 1), 4) for k, v in phases.items()},
              "phase_share": {k: round(v[1] / total_phase_ms, 4) for k, v in phases.items()} if total_phase_ms else {},
              "kernel_us_per_launch": {k: round(1e3 * v[1] / max(v[0], 1), 3) for k, v in kernels.items()},
@@ -780,6 +813,75 @@ def pyramid_leg(lib, args):
                            "ms_per_step": 1e3 * wall / 600, "mean_penetration_m": float(pen[0]),
                            "max_penetration_m": float(pen[1]), "sample": "the whole workload, one host thread"}
     return out
+
+
+def _kernel_short(name):
+    """A kernel's own name out of a rocprofv3 row (namespaces, template
+    arguments beyond the first and the parameter list dropped)."""
+    import re
+    m = re.search(r"(k_[A-Za-z0-9_]+(?:<[^>]*>)?)", name)
+    if m:
+        return m.group(1)
+    if "rocprim" in name or "hipcub" in name:
+        return "rocprim sort/scan"
+    return name.split("(")[0][-48:]
+
+
+def rocprof_summary(paths, steps, top):
+    """Summarises rocprofv3 CSV output (--output-format csv): from a kernel
+    trace, per kernel the dispatches, mean duration and mean idle time of the
+    queue before each dispatch (the gap since the previous kernel ended), and
+    the traced span per step; from counter collections, per kernel the mean
+    of each counter per dispatch (rows of one dispatch summed). Prints a
+    markdown table."""
+    import csv
+    from collections import defaultdict
+    out = []
+    for path in paths:
+        with open(path, newline="") as f:
+            rows = list(csv.DictReader(f))
+        if not rows:
+            continue
+        if "Counter_Name" in rows[0]:
+            per = defaultdict(float)
+            names = {}
+            for r in rows:
+                key = (r["Dispatch_Id"], r["Counter_Name"])
+                per[key] += float(r["Counter_Value"])
+                names[r["Dispatch_Id"]] = _kernel_short(r["Kernel_Name"])
+            agg = defaultdict(lambda: [0, 0.0])
+            for (d, c), v in per.items():
+                a = agg[(names[d], c)]
+                a[0] += 1
+                a[1] += v
+            out.append(f"counters from {os.path.basename(path)} (mean per dispatch):")
+            out.append("| kernel | counter | dispatches | mean |")
+            out.append("|---|---|---:|---:|")
+            for (k, c), (n, v) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+                out.append(f"| {k} | {c} | {n} | {v / n:.1f} |")
+        else:
+            ev = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), _kernel_short(r["Kernel_Name"]))
+                        for r in rows)
+            agg = defaultdict(lambda: [0, 0, 0])
+            prev_end = None
+            for t0, t1, k in ev:
+                a = agg[k]
+                a[0] += 1
+                a[1] += t1 - t0
+                if prev_end is not None:
+                    a[2] += max(0, t0 - prev_end)
+                prev_end = t1 if prev_end is None else max(prev_end, t1)
+            span = ev[-1][1] - ev[0][0]
+            busy = sum(a[1] for a in agg.values())
+            out.append(f"kernel trace {os.path.basename(path)}: {len(ev)} dispatches, span {span / 1e6:.1f} ms "
+                       f"({span / 1e6 / max(steps, 1):.3f} ms per step over {steps} steps), kernels busy "
+                       f"{100.0 * busy / span:.1f}% of it")
+            out.append("| kernel | dispatches | per step | mean us | idle before, mean us | share of kernel time % |")
+            out.append("|---|---:|---:|---:|---:|---:|")
+            for k, (n, d, g) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+                out.append(f"| {k} | {n} | {n / max(steps, 1):.2f} | {d / n / 1e3:.2f} | {g / n / 1e3:.2f} | "
+                           f"{100.0 * d / busy:.2f} |")
+    print("\n".join(out))
 
 
 def check_abi():
@@ -864,9 +966,16 @@ def main():
     ap.add_argument("--trace-from", type=int, default=500, help="first step traced (the settled pile)")
     ap.add_argument("--trace-iters", type=int, default=128)
     ap.add_argument("--traffic-bytes", type=float, default=None,
-                    help="HBM bytes per velocity-solve launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE "
-                         "(replaces the committed figures for the velocity solve)")
+                    help="HBM bytes per launch of the dominant kernel (the one-launch solve, or the per-colour "
+                         "velocity solve) measured with rocprofv3 FETCH_SIZE + WRITE_SIZE (replaces the committed "
+                         "figures)")
+    ap.add_argument("--rocprof-summary", nargs="+", default=None, metavar="CSV",
+                    help="summarise rocprofv3 CSV files (kernel trace, counter collection) and exit")
+    ap.add_argument("--summary-steps", type=int, default=1, help="steps the summarised trace covers")
     args = ap.parse_args()
+    if args.rocprof_summary:
+        rocprof_summary(args.rocprof_summary, args.summary_steps, 40)
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
                                    "(batched worlds shard across GPUs by world, a single world by slab; "
@@ -892,7 +1001,7 @@ def main():
         lib.b2pp_world_destroy(hw)
     _schedule = pile_schedule
     if args.traffic_bytes is not None:
-        args.traffic_bytes = {"k_solve_velocity": args.traffic_bytes}
+        args.traffic_bytes = {"dominant": args.traffic_bytes}
     elif default_workload and pile_schedule is not None:
         args.traffic_bytes = DEFAULT_TRAFFIC_BYTES.get(tuple(int(pile_schedule[0][k]) for k in SCHEDULE.names))
 

@@ -86,6 +86,9 @@ struct Counters {
     int nClustered;        // bodies placed in clusters (clustered solve)
     int boundaryStart;     // first constraint between clusters (clustered solve)
     int maxColour;         // highest constraint colour of the step
+    unsigned barrierCount; // one-launch solve: workgroups arrived at the grid barrier
+    unsigned barrierGen;   // one-launch solve: barrier generation
+    int barrierAbort;      // one-launch solve: a barrier wait passed its bound
 };
 
 // Constraint arrays (b2ContactVelocityConstraint / b2ContactPositionConstraint,
@@ -3491,6 +3494,165 @@ __global__ void k_sync_transforms(WorldView w) {
     syncTransformOne(w, b);
 }
 
+// ====================================================================
+// One-launch solve: the passes of b2Island::Solve (warm start, velocity
+// iterations, impulse store and position integration, position
+// iterations, transform sync) in one cooperative launch whose workgroups
+// are all resident, the colours separated by a grid-wide barrier instead
+// of kernel boundaries. Every constraint is solved by one lane with the
+// per-colour kernels' functions, colour after colour, so the results are
+// theirs bit for bit; what goes is the dispatch gap between two colours.
+// ====================================================================
+enum : int { SP_WARM = 1, SP_VELOCITY = 2, SP_POSITION = 4 };
+
+struct SolvePlan {
+    int cs[kColourSlots + 1];  // contact colour c holds constraints [cs[c], cs[c + 1])
+    int js[kColourSlots + 1];  // joint colour c holds order slots [js[c], js[c + 1])
+    int nColours;              // contact colour slots in use
+    int tailBegin;             // colours [tailBegin, nColours) by workgroup 0, a workgroup barrier apart
+    int nJointColours;         // joint colour slots to visit (0 without joints)
+    int stages;                // SP_* passes of this launch
+    int contactWarm;           // warm start the contacts (SP_WARM)
+    int jointWarm;             // the joints' warm starting flag (SP_WARM)
+    float dtRatio;
+    float h;                   // time step
+    int velIters, posIters;
+    int nCons;
+    int pad;
+};
+
+// Spin bound of the grid barrier (wall clock at 100 MHz: 2 s). A workgroup
+// that waits longer sets the abort word, after which every barrier of the
+// launch returns at once, so the launch always ends; the host reports it.
+constexpr long long kBarrierSpinTicks = 200000000ll;
+
+// Grid-wide barrier. The workgroup barrier first waits for every lane's
+// stores; lane 0 then arrives with an agent-scope release (L2 written back
+// for the other XCDs) and leaves with an agent-scope acquire (stale lines
+// invalidated), which the workgroup's lanes share through the closing
+// workgroup barrier. The last workgroup to arrive resets the count and
+// opens the next generation.
+__device__ inline void gridBarrier(Counters* cnt) {
+    __syncthreads();
+    if (threadIdx.x == 0 && __hip_atomic_load(&cnt->barrierAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        const unsigned g = __hip_atomic_load(&cnt->barrierGen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned arrived =
+            __hip_atomic_fetch_add(&cnt->barrierCount, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(&cnt->barrierCount, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&cnt->barrierGen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const long long t0 = wall_clock64();
+            while (__hip_atomic_load(&cnt->barrierGen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(1);
+                if (wall_clock64() - t0 > kBarrierSpinTicks) {
+                    __hip_atomic_store(&cnt->barrierAbort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                if (__hip_atomic_load(&cnt->barrierAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__device__ inline int waveMin(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+
+// kJoints: the joint passes are compiled in (worlds with joints), with at
+// most kPersistJointBlock lanes per workgroup so the joint code keeps its
+// registers; without them the kernel keeps the contact passes' budget.
+constexpr int kPersistJointBlock = 256;
+template <bool kJoints>
+__global__ void __launch_bounds__(kJoints ? kPersistJointBlock : 1024) k_solve_persistent(WorldView w, ConView cv, JointView jv, SolvePlan p,
+                                                           Counters* cnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    // Contact colours: the wide ones shared by the grid (the overflow colour
+    // by one lane in order), then the trailing ones by workgroup 0.
+    auto contactColours = [&](auto one) {
+        for (int c = 0; c < p.tailBegin; ++c) {
+            const int b0 = p.cs[c], b1 = p.cs[c + 1];
+            if (b1 <= b0) continue;
+            if (c == kOverflowColour) {
+                if (t == 0)
+                    for (int k = b0; k < b1; ++k) one(k);
+            } else {
+                for (int k = b0 + t; k < b1; k += stride) one(k);
+            }
+            gridBarrier(cnt);
+        }
+        if (p.tailBegin < p.nColours) {
+            if (blockIdx.x == 0) {
+                for (int c = p.tailBegin; c < p.nColours; ++c) {
+                    if (c == kOverflowColour) {
+                        if (threadIdx.x == 0)
+                            for (int k = p.cs[c]; k < p.cs[c + 1]; ++k) one(k);
+                    } else {
+                        for (int k = p.cs[c] + (int)threadIdx.x; k < p.cs[c + 1]; k += blockDim.x) one(k);
+                    }
+                    __syncthreads();
+                }
+            }
+            gridBarrier(cnt);
+        }
+    };
+    auto jointColours = [&](auto one) {
+        if (!kJoints) return;
+        for (int c = 0; c < p.nJointColours; ++c) {
+            const int b0 = p.js[c], b1 = p.js[c + 1];
+            if (b1 <= b0) continue;
+            if (c == kOverflowColour) {
+                if (t == 0)
+                    for (int k = b0; k < b1; ++k) one(jv.order[k]);
+            } else {
+                for (int k = b0 + t; k < b1; k += stride) one(jv.order[k]);
+            }
+            gridBarrier(cnt);
+        }
+    };
+    // b2Island::Solve, REF/Dynamics/b2Island.cpp:246-270: contact warm
+    // start, the joints' init, then per iteration the joints and the contacts.
+    if (p.stages & SP_WARM) {
+        if (p.contactWarm) contactColours([&](int k) { warmStartOne(w, cv, k); });
+        jointColours([&](int j) { jointInitOne(w, jv, j, p.dtRatio, p.jointWarm); });
+    }
+    if (p.stages & SP_VELOCITY) {
+        for (int it = 0; it < p.velIters; ++it) {
+            jointColours([&](int j) { jointSolveVelocityOne(w, jv, j, p.h); });
+            contactColours([&](int k) { solveVelocityOne(w, cv, k); });
+        }
+    }
+    if (!(p.stages & SP_POSITION)) return;
+    // StoreImpulses and the position integration (b2Island.cpp:272-306).
+    for (int k = t; k < p.nCons; k += stride) storeImpulsesOne(w, cv, k);
+    for (int b = t; b < w.nBodies; b += stride)
+        if (islandBody(w, b)) integratePositionOne(w, b, p.h);
+    gridBarrier(cnt);
+    // Position iterations: contacts, then joints (b2Island.cpp:310-328), each
+    // skipped for islands the previous pass left solved.
+    for (int it = 0; it < p.posIters; ++it) {
+        int lowest = 0;
+        contactColours([&](int k) {
+            const int root = cv.island[k];
+            if (it > 0 && w.islandMinSep[(it - 1) * w.nBodies + root] == 0) return;
+            const float minSep = solvePositionOne(w, cv, k);
+            if (minSep < -3.0f * kLinearSlop) {
+                int* f = &w.islandMinSep[it * w.nBodies + root];
+                if (*f == 0) *f = 1;
+            }
+            if (minSep < 0.0f) lowest = min(lowest, encodeOrdered(minSep));
+        });
+        lowest = waveMin(lowest);
+        if ((threadIdx.x & 63) == 0 && lowest < 0) atomicMin(&cnt->minSep[it], lowest);
+        jointColours([&](int j) { jointSolvePositionOne(w, jv, j, it); });
+    }
+    for (int b = t; b < w.nBodies; b += stride)
+        if (islandBody(w, b)) syncTransformOne(w, b);
+}
+
 // Sleep (b2Island::Solve, REF/Dynamics/b2Island.cpp:345-382): per island,
 // the minimum over its bodies of the time spent below the sleep tolerances;
 // an island that has rested for b2_timeToSleep and whose positions were
@@ -5141,9 +5303,12 @@ inline int blocks(int n, int bs = 256) { return (n + bs - 1) / bs; }
 // ====================================================================
 struct b2g_context {
     int device = 0;
-    b2g_schedule sched{0, 0, 256, 256, 256, 1024, 0};
+    b2g_schedule sched{0, 0, 256, 256, 256, 1024, 0, 0};
     hipStream_t stream = nullptr;
     std::string err;
+    int cuCount = 0;            // compute units of the device (one-launch solve)
+    int persistOcc[2][17] = {};  // resident one-launch solve workgroups per CU, without / with joints, by
+                                 // workgroup size / 64
 
     // capacities
     int capBodies = 0, capFixtures = 0, capContacts = 0;
@@ -5609,14 +5774,15 @@ enum Phase {
     PH_COLLIDE, PH_COMPACT, PH_COLOUR, PH_ORDER, PH_INTEGRATE_V, PH_INIT, PH_WARM, PH_SOLVE_V, PH_STORE,
     PH_INTEGRATE_X, PH_SOLVE_X, PH_SYNC, PH_BROADPHASE, PH_ISLANDS, PH_LEARNED, PH_K_SOLVE_V, PH_K_SOLVE_X,
     PH_K_MLP_INFER, PH_K_MLP_GRAD, PH_TOI, PH_K_SOLVE_V_CLUSTER, PH_K_SOLVE_X_CLUSTER, PH_K_COLLIDE,
-    PH_K_PAIRS_QUERY, PH_COUNT
+    PH_K_PAIRS_QUERY, PH_K_SOLVE_PERSIST, PH_COUNT
 };
 const char* kPhaseNames[PH_COUNT] = {"collide", "compact", "colour", "order", "integrate_velocities",
                                      "init_constraints", "warm_start", "solve_velocity", "store_impulses",
                                      "integrate_positions", "solve_position", "sync", "broadphase", "islands",
                                      "fresh_points", "k_solve_velocity", "k_solve_position", "k_mlp_infer",
                                      "k_mlp_grad", "toi", "k_solve_velocity_cluster",
-                                     "k_solve_position_cluster", "k_collide", "k_pairs_query"};
+                                     "k_solve_position_cluster", "k_collide", "k_pairs_query",
+                                     "k_solve_persistent"};
 
 int readCounters(b2g_context* ctx);
 int profKernelEvents(b2g_context* ctx, hipEvent_t* start, hipEvent_t* stop);
@@ -5735,6 +5901,23 @@ int rebuildHash(b2g_context* ctx, bool liveOnDevice) {
 int readCounters(b2g_context* ctx) {
     HIPCHK(ctx, hipMemcpyAsync(ctx->hcnt, ctx->cnt, sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
+// Workgroups of a one-launch solve of `block` lanes each: sched.persistentSolve
+// per compute unit, at most as many as can be resident together (a
+// cooperative launch needs every workgroup resident, and is refused
+// otherwise).
+int persistentBlocks(b2g_context* ctx, int block, int* out) {
+    if (ctx->cuCount == 0)
+        HIPCHK(ctx, hipDeviceGetAttribute(&ctx->cuCount, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const bool joints = ctx->nJoints > 0;
+    int& occ = ctx->persistOcc[joints ? 1 : 0][block / 64];
+    if (occ == 0)
+        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &occ, joints ? k_solve_persistent<true> : k_solve_persistent<false>, block, 0));
+    if (occ <= 0 || ctx->cuCount <= 0) return fail(ctx, B2G_ERR_STATE, "one-launch solve: no resident workgroup");
+    *out = ctx->cuCount * std::min(occ, ctx->sched.persistentSolve);
     return B2G_OK;
 }
 
@@ -6680,10 +6863,55 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
                                        hipMemcpyDeviceToDevice, s));
             return B2G_OK;
         };
+        // One-launch solve (sched.persistentSolve): the passes below in one
+        // cooperative launch, or two where the host acts between the velocity
+        // and the position passes (the residual fallback, the impulse log).
+        const bool persist = ctx->sched.persistentSolve > 0 && !clusterLaunch && ctx->traceIters == 0 &&
+                             (nCons > 0 || ctx->nJoints > 0);
+        auto persistentLaunch = [&](int stages, bool contactWarm) -> int {
+            SolvePlan plan;
+            memset(&plan, 0, sizeof(plan));
+            memcpy(plan.cs, colourStart, sizeof(plan.cs));
+            if (ctx->nJoints > 0) {
+                for (int c = 0; c <= kColourSlots; ++c) plan.js[c] = ctx->jointColourStart[c];
+                plan.nJointColours = kColourSlots;
+            }
+            plan.nColours = nColours;
+            plan.tailBegin = tailBegin;
+            plan.stages = stages;
+            plan.contactWarm = contactWarm ? 1 : 0;
+            plan.jointWarm = ctx->warmStarting ? 1 : 0;
+            plan.dtRatio = dtRatio;
+            plan.h = dt;
+            plan.velIters = p->velocityIterations;
+            plan.posIters = posIters;
+            plan.nCons = nCons;
+            int nb = 0;
+            const int pb = ctx->nJoints > 0 ? std::min(sb, kPersistJointBlock) : sb;
+            const int brc = persistentBlocks(ctx, pb, &nb);
+            if (brc != B2G_OK) return brc;
+            WorldView wa = w;
+            ConView ca = ctx->cv;
+            JointView ja = jv;
+            Counters* cp = ctx->cnt;
+            void* args[] = {(void*)&wa, (void*)&ca, (void*)&ja, (void*)&plan, (void*)&cp};
+            const int pc = profStart(ctx);
+            hipEvent_t e0, e1;
+            const int pk = profKernelEvents(ctx, &e0, &e1);
+            if (e0) (void)hipEventRecord(e0, s);
+            const void* kern = ctx->nJoints > 0 ? (const void*)k_solve_persistent<true>
+                                                : (const void*)k_solve_persistent<false>;
+            HIPCHK(ctx, hipLaunchCooperativeKernel(kern, dim3((unsigned)nb), dim3(pb), args, 0, s));
+            if (e1) (void)hipEventRecord(e1, s);
+            profKernel(ctx, PH_K_SOLVE_PERSIST, pk);
+            profEnd(ctx, (stages & SP_VELOCITY) ? PH_SOLVE_V : PH_SOLVE_X, pc);
+            return B2G_OK;
+        };
         // The plain solve (no learned start, trace, log or profiling) runs as
         // a captured graph once its layout has repeated.
         bool graphRan = false;
-        if (ctx->graphs && !ctx->profiling && !learned && ctx->traceIters == 0 && ctx->logCap == 0 && !clustered) {
+        if (ctx->graphs && !persist && !ctx->profiling && !learned && ctx->traceIters == 0 && ctx->logCap == 0 &&
+            !clustered) {
             rc = solveByGraph(ctx, w, jv, colourStart, nColours, nCons, dt, dtRatio, p->velocityIterations, posIters,
                               &graphRan);
             if (rc != B2G_OK) return rc;
@@ -6707,8 +6935,16 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
             if ((rc = saveJoints()) != B2G_OK) return rc;
         }
         if (usePrediction && (rc = predict(ctx->cv.imp)) != B2G_OK) return rc;
-        startSolve(nCons > 0 && (ctx->warmStarting || usePrediction));
-        solveVelocity(p->velocityIterations);
+        // With nothing for the host between the velocity and the position
+        // passes, the one-launch solve runs them all in one launch below.
+        const bool velocityDeferred = persist && !checkResidual && ctx->logCap == 0;
+        const bool contactWarm = nCons > 0 && (ctx->warmStarting || usePrediction);
+        if (!persist) {
+            startSolve(contactWarm);
+            solveVelocity(p->velocityIterations);
+        } else if (!velocityDeferred && (rc = persistentLaunch(SP_WARM | SP_VELOCITY, contactWarm)) != B2G_OK) {
+            return rc;
+        }
         if (checkResidual) {
             double rms = 0.0;
             if ((rc = deviceResidual(ctx, w, nCons, &rms)) != B2G_OK) return rc;
@@ -6720,8 +6956,12 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
                 HIPCHK(ctx, hipMemcpyAsync(ctx->cv.imp, ctx->impSnap, sizeof(float4) * nCons, hipMemcpyDeviceToDevice,
                                            s));
                 if ((rc = restoreJoints()) != B2G_OK) return rc;
-                startSolve(ctx->warmStarting);
-                solveVelocity(p->velocityIterations);
+                if (!persist) {
+                    startSolve(ctx->warmStarting);
+                    solveVelocity(p->velocityIterations);
+                } else if ((rc = persistentLaunch(SP_WARM | SP_VELOCITY, ctx->warmStarting)) != B2G_OK) {
+                    return rc;
+                }
                 ctx->stats.fallback = 1;
             }
         }
@@ -6735,6 +6975,12 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
             LAUNCHCHK(ctx);
             ctx->logCount = std::min(ctx->logCap, ctx->logCount + nFresh);
         }
+        if (persist) {
+            // store, integrate, position passes and transforms (and the
+            // velocity passes when deferred) in one launch
+            const int stages = SP_POSITION | (velocityDeferred ? SP_WARM | SP_VELOCITY : 0);
+            if ((rc = persistentLaunch(stages, contactWarm)) != B2G_OK) return rc;
+        } else {
         if (nCons > 0) {
             pe = profStart(ctx);
             hipLaunchKernelGGL(k_store_impulses, dim3(blocks(nCons)), dim3(256), 0, s, w, ctx->cv, nCons);
@@ -6780,9 +7026,10 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
                     hipLaunchKernelGGL(k_joint_solve_position, sgrid(b1 - b0), dim3(sb), 0, s, w, jv, b0, b1, it);
             });
         }
+        }  // !persist
         }  // !graphRan
         pe = profStart(ctx);
-        if (!graphRan) hipLaunchKernelGGL(k_sync_transforms, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w);
+        if (!graphRan && !persist) hipLaunchKernelGGL(k_sync_transforms, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w);
         if (ctx->allowSleep) {
             hipLaunchKernelGGL(k_sleep_accum, dim3(blocks(ctx->nBodies)), dim3(256), 0, s, w, dt);
         }
@@ -6825,6 +7072,8 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
         ctx->stats.positionIterations = ran;
         ctx->stats.points = ctx->hcnt->points;
         ctx->stats.minSeparation = lastMin;
+        if (ctx->hcnt->barrierAbort)
+            return fail(ctx, B2G_ERR_STATE, "one-launch solve: a grid barrier wait passed its bound");
     } else {
         if ((rc = compactContacts(ctx)) != B2G_OK) return rc;
         if ((rc = readCounters(ctx)) != B2G_OK) return rc;
@@ -8669,7 +8918,8 @@ int b2g_set_schedule(b2g_context* ctx, const b2g_schedule* sc) {
         !blockOk(sc->solverBlock) || !blockOk(sc->collideBlock) || !blockOk(sc->queryBlock) ||
         sc->tailColours < 0 || sc->tailColours > 8192 ||
         (sc->clusterBodies != 0 &&
-         (sc->clusterBodies < 64 || sc->clusterBodies > 2048 || (sc->clusterBodies & (sc->clusterBodies - 1)) != 0)))
+         (sc->clusterBodies < 64 || sc->clusterBodies > 2048 || (sc->clusterBodies & (sc->clusterBodies - 1)) != 0)) ||
+        sc->persistentSolve < 0 || sc->persistentSolve > 16)
         return fail(ctx, B2G_ERR_ARG, "schedule field out of range");
     ctx->sched = *sc;
     return B2G_OK;

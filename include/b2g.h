@@ -687,16 +687,30 @@ typedef struct b2g_schedule {
                               0: off. Unlike the other fields this changes the order in
                               which a pass visits constraints (each still once per pass),
                               so results differ from the unclustered solve. */
+    int32_t persistentSolve; /* one-launch solve: the warm start, velocity and position
+                              passes of the step in one cooperative launch of this many
+                              workgroups per compute unit (at most what fits), colours
+                              separated by a grid-wide barrier instead of a launch each;
+                              results are those of the per-colour launches bit for bit.
+                              0: a launch per colour. Not used by the clustered solve or
+                              a residual trace. On MI355X a grid barrier with the
+                              cross-XCD cache write-back and invalidation it needs costs
+                              more than a kernel boundary (the 100k pile steps in 4.9 ms
+                              against 1.7 ms), so it is opt-in and not autotuned. */
 } b2g_schedule;
-/* Workgroup sizes are multiples of 64 from 64 to 1024; tailColours 0 to 8192. */
+/* Workgroup sizes are multiples of 64 from 64 to 1024; tailColours 0 to 8192;
+   persistentSolve 0 to 16. */
 int b2g_set_schedule(b2g_context* ctx, const b2g_schedule* s);
 int b2g_get_schedule(b2g_context* ctx, b2g_schedule* s);
-/* Autotuner: picks each schedule field in turn by timing the candidates on
- * real steps of this world (wall time of b2g_step, candidates interleaved
- * and their order rotated each of `rounds` rounds), keeps the fastest, and
- * returns the result in *out. The steps advance the world as ordinary
- * steps do; since no choice changes results, the trajectory is the one the
- * world would have had anyway. (3 + 2 + 4 + 3 + 3 + 4) * rounds steps. */
+/* Autotuner: picks each schedule field in turn (islands, collideOrder,
+ * solverBlock, collideBlock, queryBlock, tailColours; the clustered and the
+ * one-launch solve, both measured slower on the piles, are not tried) by
+ * timing the candidates on real steps of this world (the event time of the
+ * phases the field affects, candidates interleaved and their order rotated
+ * each of `rounds` rounds), keeps the fastest, and returns the result in
+ * *out. The steps advance the world as ordinary steps do; since no choice
+ * changes results, the trajectory is the one the world would have had
+ * anyway. (3 + 3 + 4 + 3 + 3 + 4) * rounds steps. */
 int b2g_autotune(b2g_context* ctx, const b2g_step_params* p, int rounds, b2g_schedule* out);
 
 /* ------------------------------------------------------------- split step

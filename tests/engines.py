@@ -49,7 +49,8 @@ STATS = np.dtype([
 ])
 
 SCHEDULE = np.dtype([("islands", "<i4"), ("collideOrder", "<i4"), ("solverBlock", "<i4"), ("collideBlock", "<i4"),
-                     ("queryBlock", "<i4"), ("tailColours", "<i4"), ("clusterBodies", "<i4")])
+                     ("queryBlock", "<i4"), ("tailColours", "<i4"), ("clusterBodies", "<i4"),
+                     ("persistentSolve", "<i4")])
 STEP_PARAMS = np.dtype([("dt", "<f4"), ("velocityIterations", "<i4"), ("positionIterations", "<i4"), ("mode", "<i4"),
                         ("learnedWarmStart", "<i4"), ("residualThreshold", "<f4")])
 

@@ -345,3 +345,31 @@ def test_every_point_zero_network_is_the_warm_start(lib, gpu):
     assert abs(cur[2, 0] - cur[1, 0]) <= 0.05 * cur[1, 0], cur
     assert np.allclose(cur[2, 1:], cur[1, 1:], rtol=2e-2, atol=1e-4), cur
     w.close()
+
+
+@pytest.mark.parametrize("threshold", [1e-20, 1e9])
+def test_persistent_solve_with_learned_start(lib, gpu, threshold):
+    """The one-launch solve splits at the residual check and the impulse log
+    (velocity passes, host check, position passes): with the learned start,
+    falling back (threshold 1e-20) or not (1e9) and logging fresh points, it
+    steps as the per-colour launches do, bit for bit, and logs the same."""
+    rng = np.random.default_rng(5)
+    p = M.random_params(rng, norm=False)
+    ws = [gpu.world("world", 1200, 8, sleep=False) for _ in range(2)]
+    for w, persistent in zip(ws, (0, 2)):
+        s = np.array([(0, 0, 256, 256, 256, 1024, 0, persistent)], E.SCHEDULE)
+        assert w.e.lib.b2pp_world_set_schedule(w.h, E.ptr(s)) == 0, w.error()
+        c = w.device_context()
+        _set(lib, c, p)
+        E.b2g_check(lib, c, lib.b2g_log_fresh(c, 100000))
+        w.set_learned(True, threshold)
+    for _ in range(40):
+        for w in ws:
+            w.step(1)
+    a, b = ws
+    assert np.array_equal(a.bodies().view(np.uint32), b.bodies().view(np.uint32))
+    na, nb = ctypes.c_int(), ctypes.c_int()
+    lib.b2g_log_size(a.device_context(), ctypes.byref(na))
+    lib.b2g_log_size(b.device_context(), ctypes.byref(nb))
+    assert na.value == nb.value > 0
+    a.close(); b.close()

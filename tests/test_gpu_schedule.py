@@ -20,9 +20,11 @@ import engines as E
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = (0, 0, 256, 256, 256, 0, 0)
-SCHEDULES = [(1, 1, 64, 128, 64, 1024, 0), (2, 1, 512, 64, 128, 8192, 0), (2, 0, 128, 1024, 256, 512, 0),
-             (0, 1, 192, 64, 512, 2048, 0), (1, 2, 256, 256, 64, 2048, 0)]
+DEFAULT = (0, 0, 256, 256, 256, 0, 0, 0)
+SCHEDULES = [(1, 1, 64, 128, 64, 1024, 0, 0), (2, 1, 512, 64, 128, 8192, 0, 0), (2, 0, 128, 1024, 256, 512, 0, 0),
+             (0, 1, 192, 64, 512, 2048, 0, 0), (1, 2, 256, 256, 64, 2048, 0, 0),
+             # one-launch solve: grid barriers between colours, workgroup sizes, trailing colours
+             (0, 0, 256, 256, 256, 0, 0, 1), (2, 1, 1024, 256, 64, 2048, 0, 2), (0, 0, 64, 256, 256, 512, 0, 16)]
 
 
 def _world(gpu):
@@ -74,6 +76,7 @@ def test_autotune_steps_are_ordinary_steps(gpu):
     assert o["islands"] in (0, 1, 2) and o["collideOrder"] in (0, 1, 2)
     assert o["solverBlock"] in (64, 128, 256, 512) and o["collideBlock"] in (64, 128, 256)
     assert o["queryBlock"] in (64, 128, 256) and o["tailColours"] in (0, 512, 1024, 2048)
+    assert o["persistentSolve"] == 0 and o["clusterBodies"] == 0  # opt-in only
     a.step(60)
     b.step(60)
     _same(a, b)
@@ -82,10 +85,10 @@ def test_autotune_steps_are_ordinary_steps(gpu):
 
 def test_set_schedule_rejects_bad_values(gpu):
     w = gpu.world("pyramid", 4, 0)
-    for bad in [(3, 0, 256, 256, 256, 0, 0), (0, 3, 256, 256, 256, 0, 0), (0, 0, 100, 256, 256, 0, 0),
-                (0, 0, 256, 2048, 256, 0, 0), (0, 0, 256, 256, 0, 0, 0), (0, 0, 256, 256, 256, -1, 0),
-                (0, 0, 256, 256, 256, 9000, 0), (0, 0, 256, 256, 256, 0, 32), (0, 0, 256, 256, 256, 0, 100),
-                (0, 0, 256, 256, 256, 0, 4096)]:
+    for bad in [(3, 0, 256, 256, 256, 0, 0, 0), (0, 3, 256, 256, 256, 0, 0, 0), (0, 0, 100, 256, 256, 0, 0, 0),
+                (0, 0, 256, 2048, 256, 0, 0, 0), (0, 0, 256, 256, 0, 0, 0, 0), (0, 0, 256, 256, 256, -1, 0, 0),
+                (0, 0, 256, 256, 256, 9000, 0, 0), (0, 0, 256, 256, 256, 0, 32, 0), (0, 0, 256, 256, 256, 0, 100, 0),
+                (0, 0, 256, 256, 256, 0, 4096, 0), (0, 0, 256, 256, 256, 0, 0, -1), (0, 0, 256, 256, 256, 0, 0, 17)]:
         s = np.array([bad], E.SCHEDULE)
         assert w.e.lib.b2pp_world_set_schedule(w.h, E.ptr(s)) != 0
     w.close()
@@ -131,7 +134,7 @@ def test_single_cluster_is_the_unclustered_solve(gpu):
     a = gpu.world("pile", 1200, 9, sleep=False)
     b = gpu.world("pile", 1200, 9, sleep=False)
     _set(a, DEFAULT)
-    _set(b, (0, 0, 256, 256, 256, 1024, 2048))
+    _set(b, (0, 0, 256, 256, 256, 1024, 2048, 0))
     for _ in range(6):
         a.step(50)
         b.step(50)
@@ -145,8 +148,8 @@ def test_many_clusters(gpu, size):
     lib = E.b2g_lib()
     plain, a, b = _world(gpu), _world(gpu), _world(gpu)
     _set(plain, DEFAULT)
-    _set(a, (0, 0, 256, 256, 256, 1024, size))
-    _set(b, (0, 0, 128, 256, 256, 0, size))   # other workgroup size and no fusion: same order
+    _set(a, (0, 0, 256, 256, 256, 1024, size, 0))
+    _set(b, (0, 0, 128, 256, 256, 0, size, 0))   # other workgroup size and no fusion: same order
     conflicts, mask = ctypes.c_int(-1), ctypes.c_int(-1)
     for _ in range(4):
         plain.step(50)
@@ -162,3 +165,25 @@ def test_many_clusters(gpu, size):
     assert pa <= pp + 0.005, (pa, pp)
     assert abs(ka / kp - 1.0) <= 0.2, (ka, kp)
     plain.close(); a.close(); b.close()
+
+
+FUSED = (0, 0, 256, 256, 256, 1024, 0, 0)
+PERSIST = (0, 0, 256, 256, 256, 1024, 0, 2)
+
+
+@pytest.mark.parametrize("scene,n,seed,sleep", [("plank", 100, 0, True), ("pendulums", 24, 0, True),
+                                                ("chains", 6, 16, True), ("pile", 3000, 4, False)])
+def test_persistent_solve_matches_per_colour(gpu, scene, n, seed, sleep):
+    """The one-launch solve (grid barriers between colours) against a launch
+    per colour, bit for bit: a plank whose contacts overflow into the serial
+    colour, pendulums (joints only), chains on the ground (joints and
+    contacts) and a pile."""
+    a = gpu.world(scene, n, seed, sleep=sleep)
+    b = gpu.world(scene, n, seed, sleep=sleep)
+    _set(a, FUSED)
+    _set(b, PERSIST)
+    for _ in range(4):
+        a.step(40)
+        b.step(40)
+        _same(a, b)
+    a.close(); b.close()
