@@ -78,6 +78,8 @@ _MPOINT = np.dtype([("lp", _VEC2), ("ni", "<f4"), ("ti", "<f4"), ("id", "<u4")])
 MANIFOLD = np.dtype([("points", _MPOINT, (2,)), ("ln", _VEC2), ("lpt", _VEC2), ("type", "<i4"), ("count", "<i4")])
 # the schedule the benchmark's worlds run with (set by the autotuner)
 _schedule = None
+# colour demotion rounds of the pile's worlds (None: the library's default)
+_demotion = None
 
 
 class PhaseProfile(ctypes.Structure):
@@ -118,6 +120,10 @@ def make_world(lib, n, seed):
     if rc != 0:
         raise RuntimeError(f"scene load failed: {lib.b2pp_world_last_error(h)}")
     apply_schedule(lib, h)
+    if _demotion is not None:
+        b2g = box2dpp_amd.b2g()
+        ctx = ctypes.c_void_p(lib.b2pp_world_device_context(h))
+        _check(b2g, ctx, b2g.b2g_set_colour_demotion(ctx, _demotion))
     lib.b2pp_world_sync(h)
     return h
 
@@ -252,6 +258,11 @@ def gpu_measures(lib, h):
     lib.b2pp_world_body_mass(h, mass.ctypes.data_as(ctypes.c_void_p))
     m = measures.all_measures(bodies, meta[:nc], man[:nc], fb[:nf], fm[:nf], mass[:nb])
     m["contacts"] = int(nc)
+    b2g = box2dpp_amd.b2g()
+    counts = np.zeros(65, np.int32)
+    b2g.b2g_get_colour_counts(ctypes.c_void_p(lib.b2pp_world_device_context(h)), counts.ctypes.data_as(ctypes.c_void_p),
+                              65, None)
+    m["colour_sizes"] = [int(v) for v in counts[:int(np.nonzero(counts)[0].max()) + 1]] if counts.any() else []
     return m
 
 
@@ -923,6 +934,8 @@ def main():
     ap.add_argument("--vel", type=int, default=8)
     ap.add_argument("--pos", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=30.0)
+    ap.add_argument("--demotion", type=int, default=None,
+                    help="colour demotion rounds per step of the pile's worlds (default: the library's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline, its roofline pass and CPU leg (the profiled command)")
@@ -988,7 +1001,8 @@ def main():
 
     lib = box2dpp_amd.host()
     check_abi()
-    global _schedule
+    global _schedule, _demotion
+    _demotion = args.demotion
     pile_schedule = None
     if args.schedule and args.schedule != "auto":
         vals = [int(v) for v in args.schedule.split(",")]

@@ -733,6 +733,57 @@ __global__ void k_colour_commit(WorldView w, const unsigned long long* claim, ui
     w.ccolour[c] = col;
 }
 
+// Demotion: a constraint keeps its colour for its life, so when the
+// contacts around it end, lower colours free up at both of its bodies while
+// it stays high. Per round, a constraint above the lowest colour free at
+// both of its dynamic bodies (any colour, from the overflow slot) claims
+// them, each body admits its highest-priority claimant as in the colouring,
+// and the winners move down. A winner is the only constraint changing colour
+// at its bodies that round, so no colour ever holds two constraints on one
+// dynamic body. The pass visits fewer, larger colours.
+__device__ inline unsigned long long demotionBits(const WorldView& w, int c, int bA, int bB, bool dA, bool dB) {
+    const int col = w.ccolour[c];
+    const unsigned long long used = (dA ? w.colourMask[bA] : 0ull) | (dB ? w.colourMask[bB] : 0ull);
+    const unsigned long long below = col >= kMaxColours ? ~0ull : ((1ull << col) - 1ull);
+    return ~used & below;
+}
+
+__global__ void k_demote_claim(WorldView w, unsigned long long* claim, uint32_t tag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    if (!(w.cflags[c] & CF_CONSTRAINT) || w.ccolour[c] <= 0) return;
+    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
+    const bool dA = w.btype[bA] == BT_DYNAMIC, dB = w.btype[bB] == BT_DYNAMIC;
+    if (demotionBits(w, c, bA, bB, dA, dB) == 0ull) return;
+    const unsigned long long v = claimValue(tag, w, c);
+    if (dA) atomicMax(&claim[bA], v);
+    if (dB) atomicMax(&claim[bB], v);
+}
+
+__global__ void k_demote_commit(WorldView w, const unsigned long long* claim, uint32_t tag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    if (!(w.cflags[c] & CF_CONSTRAINT) || w.ccolour[c] <= 0) return;
+    const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
+    const bool dA = w.btype[bA] == BT_DYNAMIC, dB = w.btype[bB] == BT_DYNAMIC;
+    const unsigned long long v = claimValue(tag, w, c);
+    if ((dA && claim[bA] != v) || (dB && claim[bB] != v)) return;
+    const unsigned long long freeBits = demotionBits(w, c, bA, bB, dA, dB);
+    if (freeBits == 0ull) return;
+    const int col = w.ccolour[c];
+    const int to = __ffsll((long long)freeBits) - 1;
+    const unsigned long long keep = col >= kMaxColours ? ~0ull : ~(1ull << col), bit = 1ull << to;
+    if (dA) {
+        atomicAnd(&w.colourMask[bA], keep);
+        atomicOr(&w.colourMask[bA], bit);
+    }
+    if (dB) {
+        atomicAnd(&w.colourMask[bB], keep);
+        atomicOr(&w.colourMask[bB], bit);
+    }
+    w.ccolour[c] = to;
+}
+
 // Order key of every contact: a constraint's colour; contacts outside the
 // constraint set sort after every colour.
 // Also clears the colour counts that the order's counting sort then takes.
@@ -5474,6 +5525,7 @@ struct b2g_context {
     std::vector<double> trace;      // [3][traceIters + 1]
     int traceStepIters = 0;         // iterations of the last traced step
     int lastColourStart[kColourSlots + 1] = {};  // colour ranges of the last coloured step
+    int demoteRounds = 1;           // colour demotion rounds per coloured step
     int compactedFrom = -1;  // table length before the last coloured step's compaction (-1: none)
     int colourRoundsNeeded = 0;  // colouring rounds the last coloured step needed
     int lastConstraints = 0;
@@ -6591,6 +6643,21 @@ int stepColouredSolve(b2g_context* ctx, const b2g_step_params* p) {
             if (ctx->nContacts > 0)
                 hipLaunchKernelGGL(k_mark_constraints, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->cnt);
             profEnd(ctx, PH_ISLANDS, pe);
+            LAUNCHCHK(ctx);
+        }
+        // Constraints move down to lower colours freed since they were
+        // coloured (rounds launched without a readback).
+        for (int r = 0; r < ctx->demoteRounds && ctx->nContacts > 0; ++r) {
+            if (ctx->claimTag >= 0xFFFF) {
+                hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(ctx->nBodies)), dim3(256), 0, s,
+                                   ctx->claim, ctx->nBodies, 0ull);
+                ctx->claimTag = 0;
+            }
+            const uint32_t tag = ++ctx->claimTag;
+            const int pe = profStart(ctx);
+            hipLaunchKernelGGL(k_demote_claim, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->claim, tag);
+            hipLaunchKernelGGL(k_demote_commit, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->claim, tag);
+            profEnd(ctx, PH_COLOUR, pe);
             LAUNCHCHK(ctx);
         }
         // Colour new constraints: rounds are launched without a readback
@@ -8616,6 +8683,19 @@ int b2g_get_trace(b2g_context* ctx, double* curves, int cap, int* iters) {
 }  // extern "C"
 
 extern "C" {
+
+int b2g_set_colour_demotion(b2g_context* ctx, int rounds) {
+    if (!ctx || rounds < 0 || rounds > 64) return B2G_ERR_ARG;
+    ctx->demoteRounds = rounds;
+    return B2G_OK;
+}
+
+int b2g_get_colour_counts(b2g_context* ctx, int32_t* out, int cap, int* count) {
+    if (!ctx || cap < 0 || (cap > 0 && !out)) return B2G_ERR_ARG;
+    for (int c = 0; c < kColourSlots && c < cap; ++c) out[c] = ctx->lastColourStart[c + 1] - ctx->lastColourStart[c];
+    if (count) *count = kColourSlots;
+    return B2G_OK;
+}
 
 int b2g_check_colouring(b2g_context* ctx, int* conflicts, int* maskErrors) {
     if (!ctx || !conflicts || !maskErrors) return B2G_ERR_ARG;

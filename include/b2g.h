@@ -790,6 +790,21 @@ int b2g_abi_sizes(int32_t* out, int cap);
  * sequentially). */
 int b2g_check_colouring(b2g_context* ctx, int* conflicts, int* maskErrors);
 
+/* Colour demotion rounds per coloured step (0-64, default 1). A constraint
+ * keeps its colour for its life; each round moves constraints down to the
+ * lowest colour that has since become free at both of their dynamic bodies
+ * (each body admitting one mover per round, so a colour still never holds
+ * two constraints on one dynamic body). Fewer, larger colours mean fewer
+ * launches per pass. This changes the order in which a pass visits
+ * constraints, so results differ with the number of rounds. */
+int b2g_set_colour_demotion(b2g_context* ctx, int rounds);
+
+/* Constraints per colour slot of the last coloured step (slots 0-63, then
+ * the overflow slot 64): out[c] for c < min(cap, 65); *count = 65. No
+ * reference counterpart (the reference solves sequentially); it tells how
+ * many launches a pass takes. */
+int b2g_get_colour_counts(b2g_context* ctx, int32_t* out, int cap, int* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -187,3 +187,49 @@ def test_persistent_solve_matches_per_colour(gpu, scene, n, seed, sleep):
         b.step(40)
         _same(a, b)
     a.close(); b.close()
+
+
+def _colour_counts(lib, w):
+    counts = np.zeros(65, np.int32)
+    n = ctypes.c_int()
+    E.b2g_check(lib, w.device_context(), lib.b2g_get_colour_counts(w.device_context(), E.ptr(counts), 65,
+                                                                   ctypes.byref(n)))
+    assert n.value == 65
+    return counts
+
+
+def test_colour_demotion(gpu):
+    """Demotion rounds (b2g_set_colour_demotion) move constraints down to
+    colours freed since they were coloured: on a settling pile the pass has
+    fewer colours than without them, the colouring invariant (no dynamic body
+    twice in a colour, masks equal to the constraints' colours) holds after
+    every step, the colour counts add up to the constraints, and the state
+    stays within the bounds of the solve without demotion (the visit order
+    differs)."""
+    lib = E.b2g_lib()
+    plain = gpu.world("pile", 3000, 4, sleep=False)
+    dem = gpu.world("pile", 3000, 4, sleep=False)
+    E.b2g_check(lib, plain.device_context(), lib.b2g_set_colour_demotion(plain.device_context(), 0))
+    E.b2g_check(lib, dem.device_context(), lib.b2g_set_colour_demotion(dem.device_context(), 1))
+    assert lib.b2g_set_colour_demotion(dem.device_context(), -1) != 0
+    conf, mask = ctypes.c_int(-1), ctypes.c_int(-1)
+    for s in range(500):
+        plain.step(1)
+        dem.step(1)
+        if s % 5 == 4:
+            E.b2g_check(lib, dem.device_context(), lib.b2g_check_colouring(dem.device_context(), ctypes.byref(conf),
+                                                                           ctypes.byref(mask)))
+            assert conf.value == 0 and mask.value == 0, (s, conf.value, mask.value)
+    cp, cd = _colour_counts(lib, plain), _colour_counts(lib, dem)
+    assert cd.sum() == dem.stats()["constraints"] and cp.sum() == plain.stats()["constraints"]
+    assert cd.sum() > 1000
+    # constraints sit in lower colours: the mean colour drops, the count does not grow
+    k = np.arange(65)
+    assert (cd * k).sum() / cd.sum() < (cp * k).sum() / cp.sum(), (cp[:16], cd[:16])
+    assert np.count_nonzero(cd) <= np.count_nonzero(cp), (cp[:16], cd[:16])
+    pp, kp = _pen_energy(plain)
+    pd, kd = _pen_energy(dem)
+    assert np.isfinite(dem.bodies()).all()
+    assert pd <= pp + 0.005, (pd, pp)
+    assert abs(kd / kp - 1.0) <= 0.25, (kd, kp)
+    plain.close(); dem.close()
