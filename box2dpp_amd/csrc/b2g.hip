@@ -4271,7 +4271,7 @@ __global__ void k_collide_compute(WorldView w, RefView r) {
 __global__ void k_collide_serial(WorldView w, RefView r, Counters* cnt) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     for (int c = w.nContacts - 1; c >= 0; --c) {
-        int flags = w.cflags[c];
+        int flags = w.cflags[c] & ~CF_UPDATED;
         const int fA = w.cfA[c], fB = w.cfB[c];
         const int bA = w.fbody[fA], bB = w.fbody[fB];
         const bool destroyWake = w.cman[c].pointCount > 0 && !sensorOf(w, fA) && !sensorOf(w, fB);
@@ -4315,9 +4315,11 @@ __global__ void k_collide_serial(WorldView w, RefView r, Counters* cnt) {
             }
         }
         if (touching != wasTouching) { wakeBody(w, bA); wakeBody(w, bB); }
-        flags |= CF_ENABLED;
+        flags |= CF_ENABLED | CF_UPDATED;
         if (touching) flags |= CF_TOUCHING;
         else flags &= ~CF_TOUCHING;
+        if (wasTouching) flags |= CF_WAS_TOUCHING;
+        else flags &= ~CF_WAS_TOUCHING;
         if (!wasTouching && touching) cnt->beginEvents++;
         if (wasTouching && !touching) cnt->endEvents++;
         w.cman[c] = m;
@@ -4339,6 +4341,7 @@ __global__ void k_edge_count(WorldView w, RefView r, Counters* cnt) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= w.nContacts) return;
     r.cvisited[c] = 0;
+    w.cflags[c] &= ~CF_CONSTRAINT;  // set again for the island contacts by k_dfs
     if (!islandContact(w, c)) return;
     const int bA = w.fbody[w.cfA[c]], bB = w.fbody[w.cfB[c]];
     if (w.btype[bA] != BT_STATIC) { atomicAdd(&r.edgeCount[bA], 1); atomicAdd(&cnt->nEdges, 1); }
@@ -4435,6 +4438,7 @@ __global__ void k_dfs(WorldView w, JointView jv, RefView r, Counters* cnt) {
                 if (r.cvisited[c]) continue;
                 r.islandContacts[nc++] = c;
                 r.cvisited[c] = 1;
+                w.cflags[c] |= CF_CONSTRAINT;
                 const int bA = w.fbody[w.cfA[c]];
                 const int other = bA == b ? w.fbody[w.cfB[c]] : bA;
                 if (r.bvisited[other]) continue;
@@ -7393,11 +7397,14 @@ int solveTOI(b2g_context* ctx, const b2g_step_params* p) {
 // b2World::Step in the reference's visiting order (see the reference-order
 // kernels above). Same collide/broad-phase as the coloured path; islands
 // and constraint order follow Box2D exactly.
-int stepReference(b2g_context* ctx, const b2g_step_params* p) {
+// First half of a reference-order step: counters, pending broad-phase and
+// b2ContactManager::Collide in list order (the contacts it destroys stay in
+// the table, flagged dead, until the second half compacts it). With
+// ctx->keepOldManifolds the manifolds from before the update are kept for
+// PreSolve.
+int stepReferenceCollide(b2g_context* ctx, const b2g_step_params* p) {
+    (void)p;
     hipStream_t s = ctx->stream;
-    const float dt = p->dt;
-    const float inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
-    const float dtRatio = ctx->inv_dt0 * dt;
     int rc;
     if ((rc = reserveContacts(ctx)) != B2G_OK) return rc;
     if ((rc = ensureRefBuffers(ctx)) != B2G_OK) return rc;
@@ -7413,8 +7420,29 @@ int stepReference(b2g_context* ctx, const b2g_step_params* p) {
     RefView r = ctx->ref;
     if (ctx->nContacts > 0) {
         WorldView w = ctx->view();
+        if (ctx->keepOldManifolds) {
+            if ((rc = ensureOldManifolds(ctx)) != B2G_OK) return rc;
+            HIPCHK(ctx, hipMemcpyAsync(ctx->oldMan, ctx->cur.cman, sizeof(b2g_manifold) * ctx->nContacts,
+                                       hipMemcpyDeviceToDevice, s));
+        }
         hipLaunchKernelGGL(k_collide_compute, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, r);
         hipLaunchKernelGGL(k_collide_serial, dim3(1), dim3(64), 0, s, w, r, ctx->cnt);
+        LAUNCHCHK(ctx);
+    }
+    return B2G_OK;
+}
+
+// Second half: compaction, Box2D's island search and island solves, the
+// broad-phase and continuous collision.
+int stepReferenceSolve(b2g_context* ctx, const b2g_step_params* p) {
+    hipStream_t s = ctx->stream;
+    const float dt = p->dt;
+    const float inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
+    const float dtRatio = ctx->inv_dt0 * dt;
+    int rc;
+    RefView r = ctx->ref;
+    if (ctx->nContacts > 0) {
+        WorldView w = ctx->view();
         hipLaunchKernelGGL(k_mark_dead, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, w, ctx->keep);
         LAUNCHCHK(ctx);
         HIPCHK(ctx, prims_exclusive_sum(ctx->scratch, ctx->scratchBytes, ctx->keep, ctx->idx, ctx->nContacts, s));
@@ -7481,6 +7509,11 @@ int stepReference(b2g_context* ctx, const b2g_step_params* p) {
     ctx->stats.overflow = ctx->hcnt->overflow;
     ctx->stats.cellSize = ctx->cellSize;
     return B2G_OK;
+}
+
+int stepReference(b2g_context* ctx, const b2g_step_params* p) {
+    const int rc = stepReferenceCollide(ctx, p);
+    return rc != B2G_OK ? rc : stepReferenceSolve(ctx, p);
 }
 
 void bodyToDevice(const b2g_body_state& b, int& type, int& flags, float4& xf, float4& pos, float4& pos0,
@@ -7865,11 +7898,12 @@ int b2g_step_collide(b2g_context* ctx, const b2g_step_params* p) {
     if (!ctx || !p) return B2G_ERR_ARG;
     if (ctx->stepOpen) return fail(ctx, B2G_ERR_STATE, "a split step is already open");
     if (p->velocityIterations < 0 || p->positionIterations < 0) return fail(ctx, B2G_ERR_ARG, "iterations");
-    if (p->mode != B2G_MODE_COLOURED)
-        return fail(ctx, B2G_ERR_UNSUPPORTED, "the split step runs with the coloured solver");
+    if (p->mode != B2G_MODE_COLOURED && p->mode != B2G_MODE_REFERENCE) return fail(ctx, B2G_ERR_ARG, "solver mode");
+    if (p->learnedWarmStart && p->mode == B2G_MODE_REFERENCE)
+        return fail(ctx, B2G_ERR_UNSUPPORTED, "the learned warm start runs with the coloured solver");
     (void)hipSetDevice(ctx->device);
     ctx->keepOldManifolds = true;
-    const int rc = stepColouredCollide(ctx, p);
+    const int rc = p->mode == B2G_MODE_REFERENCE ? stepReferenceCollide(ctx, p) : stepColouredCollide(ctx, p);
     ctx->keepOldManifolds = false;
     if (rc != B2G_OK) return rc;
     ctx->openParams = *p;
@@ -7957,7 +7991,8 @@ int b2g_step_solve(b2g_context* ctx) {
     if (!ctx->stepOpen) return fail(ctx, B2G_ERR_STATE, "no split step is open");
     ctx->stepOpen = false;
     (void)hipSetDevice(ctx->device);
-    return stepColouredSolve(ctx, &ctx->openParams);
+    return ctx->openParams.mode == B2G_MODE_REFERENCE ? stepReferenceSolve(ctx, &ctx->openParams)
+                                                       : stepColouredSolve(ctx, &ctx->openParams);
 }
 
 int b2g_sync(b2g_context* ctx) {

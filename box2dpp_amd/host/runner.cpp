@@ -6,6 +6,7 @@
 #include "b2g.h"
 #include "scenes/scenes.h"
 
+#include <string.h>
 #include <vector>
 #include <string>
 
@@ -50,7 +51,12 @@ struct Recorder : public b2ContactListener {
         if (b == conveyor) c->SetTangentSpeed(-belt);
         record(3, c, c->IsEnabled() ? 1 : 0);
     }
-    void PostSolve(b2Contact* c, const b2ContactImpulse*) override { record(4, c, 0); }
+    // PostSolve's flag: the bits of the first point's normal impulse
+    void PostSolve(b2Contact* c, const b2ContactImpulse* imp) override {
+        int32_t bits = 0;
+        if (imp->count > 0) memcpy(&bits, &imp->normalImpulses[0], sizeof(bits));
+        record(4, c, bits);
+    }
 };
 
 struct Runner {

@@ -1665,7 +1665,7 @@ void b2World::Step(float dt, int32_t velocityIterations, int32_t positionIterati
         p.mode = m_options.mode;
         p.learnedWarmStart = m_options.learnedWarmStart ? 1 : 0;
         p.residualThreshold = m_options.residualThreshold;
-        if (m_contactListener && p.mode == B2G_MODE_COLOURED) {
+        if (m_contactListener) {
             // Begin/End/PreSolve inside the step, as Collide calls them
             rc = CollideWithListener(&p);
             if (rc == B2G_OK) rc = b2g_step_solve(m_ctx);
@@ -1685,7 +1685,7 @@ void b2World::Step(float dt, int32_t velocityIterations, int32_t positionIterati
     b2g_step_stats st{};
     b2g_get_stats(m_ctx, &st);
     m_contactCount = st.contacts;
-    if (m_contactListener) ReplayEvents(m_options.mode == B2G_MODE_COLOURED);
+    if (m_contactListener) ReplayEvents(true);
     const auto t1 = std::chrono::steady_clock::now();
     m_profile.step = std::chrono::duration<float, std::milli>(t1 - t0).count();
 }
@@ -1861,6 +1861,18 @@ int32_t b2World::CollideWithListener(const void* params) {
         rc = b2g_override_contacts(m_ctx, ids.data(), enabled.data(), fr.data(), re.data(), (int)ids.size());
     if (rc == B2G_OK && !tIds.empty())
         rc = b2g_set_contact_tangent_speeds(m_ctx, tIds.data(), tSpeeds.data(), (int)tIds.size());
+    // The order in which the solve's Report will call PostSolve: the island
+    // search over the contacts as the collide and PreSolve left them, with
+    // indices as the table has them after the dead contacts leave it.
+    {
+        std::vector<uint8_t> on(n, 0);
+        for (int i = 0; i < n; ++i) on[i] = (meta[4 * i + 2] & 2) != 0;
+        for (size_t k = 0; k < ids.size(); ++k) on[ids[k]] = (uint8_t)enabled[k];
+        IslandContactOrder(meta, on, &m_postSolveOrder);
+        std::vector<int32_t> live(n + 1, 0);
+        for (int i = 0; i < n; ++i) live[i + 1] = live[i] + ((meta[4 * i + 2] & 4) == 0 ? 1 : 0);
+        for (int32_t& t : m_postSolveOrder) t = live[t];
+    }
     // touching set as the listener has seen it; what the TOI phase changes
     // after this is reported once the step is over
     m_touchingKeys.clear();
@@ -1870,6 +1882,84 @@ int32_t b2World::CollideWithListener(const void* params) {
     }
     std::sort(m_touchingKeys.begin(), m_touchingKeys.end());
     return rc;
+}
+
+// b2World::Solve's island search (REF/Dynamics/b2World.cpp:388-565) over the
+// state after the narrowphase: seeds in body-list order (newest first) that
+// are awake, active and not static; a depth-first walk over each body's
+// contact edges (newest first) and joint edges (newest first), taking the
+// touching, enabled, non-sensor contacts and the joints whose other body is
+// active, never walking on from a static body, whose island flag is cleared
+// after its island so that it can join the next. The island contacts in the
+// order the walk adds them are the order of b2Island::Report's PostSolve
+// calls (REF/Dynamics/b2Island.cpp:517-540), island after island. meta is
+// b2g_get_contacts' table (creation order); on[t] whether contact t is
+// enabled; order receives table indices.
+void b2World::IslandContactOrder(const std::vector<int32_t>& meta, const std::vector<uint8_t>& on,
+                                 std::vector<int32_t>* order) const {
+    order->clear();
+    Download();  // awake flags as the narrowphase left them
+    const int n = (int)on.size();
+    const int nb = (int)m_bodies.size();
+    const int nj = (int)m_joints.size();
+    auto bodyOf = [&](int f) { return m_fixtures[f]->m_body->m_index; };
+    // contact edges per body, newest first
+    std::vector<int32_t> start(nb + 1, 0), edge;
+    for (int t = 0; t < n; ++t) {
+        if (meta[4 * t + 2] & 4) continue;
+        ++start[bodyOf(meta[4 * t]) + 1];
+        ++start[bodyOf(meta[4 * t + 1]) + 1];
+    }
+    for (int b = 0; b < nb; ++b) start[b + 1] += start[b];
+    edge.resize(start[nb]);
+    std::vector<int32_t> cursor(start.begin(), start.end() - 1);
+    for (int t = n - 1; t >= 0; --t) {
+        if (meta[4 * t + 2] & 4) continue;
+        edge[cursor[bodyOf(meta[4 * t])]++] = t;
+        edge[cursor[bodyOf(meta[4 * t + 1])]++] = t;
+    }
+    std::vector<uint8_t> bodyFlag(nb, 0), contactFlag(n, 0), jointFlag(nj, 0);
+    std::vector<int32_t> stack, island;
+    for (const b2Body* seed = m_bodyList; seed; seed = seed->m_next) {
+        const int s0 = seed->m_index;
+        if (bodyFlag[s0] || !(seed->m_flags & b2Body::e_awakeFlag) || !seed->IsActive() ||
+            seed->m_type == b2BodyType::STATIC_BODY)
+            continue;
+        island.clear();
+        stack.assign(1, s0);
+        bodyFlag[s0] = 1;
+        while (!stack.empty()) {
+            const int b = stack.back();
+            stack.pop_back();
+            island.push_back(b);
+            const b2Body* body = m_bodies[b];
+            if (body->m_type == b2BodyType::STATIC_BODY) continue;
+            for (int e = start[b]; e < start[b + 1]; ++e) {
+                const int t = edge[e];
+                if (contactFlag[t] || !on[t] || !(meta[4 * t + 2] & 1)) continue;
+                const b2Fixture* fa = m_fixtures[meta[4 * t]];
+                const b2Fixture* fb = m_fixtures[meta[4 * t + 1]];
+                if (fa->m_isSensor || fb->m_isSensor) continue;
+                order->push_back(t);
+                contactFlag[t] = 1;
+                const int other = fa->m_body->m_index == b ? fb->m_body->m_index : fa->m_body->m_index;
+                if (bodyFlag[other]) continue;
+                stack.push_back(other);
+                bodyFlag[other] = 1;
+            }
+            for (const b2JointEdge* je = body->m_jointList; je; je = je->next) {
+                const int j = je->joint->m_index;
+                if (jointFlag[j] || !je->other->IsActive()) continue;
+                jointFlag[j] = 1;
+                const int other = je->other->m_index;
+                if (bodyFlag[other]) continue;
+                stack.push_back(other);
+                bodyFlag[other] = 1;
+            }
+        }
+        for (int b : island)
+            if (m_bodies[b]->m_type == b2BodyType::STATIC_BODY) bodyFlag[b] = 0;
+    }
 }
 
 void b2World::ReplayEvents(bool collideEventsDone) {
@@ -1886,10 +1976,30 @@ void b2World::ReplayEvents(bool collideEventsDone) {
     std::sort(now.begin(), now.end());
     std::unordered_set<uint64_t> before(m_touchingKeys.begin(), m_touchingKeys.end());
     std::unordered_set<uint64_t> after(now.begin(), now.end());
+    // PostSolve first, as b2Island::Report calls it at the end of each
+    // island's solve (REF/Dynamics/b2Island.cpp:517-540), in the island
+    // search's order when the split step computed it, else in list order;
+    // then what the TOI phase changed, as SolveTOI runs after the islands.
+    auto postSolve = [&](b2Contact* c) {
+        b2ContactImpulse imp{};
+        imp.count = c->m_manifold.pointCount;
+        for (int j = 0; j < imp.count; ++j) {
+            imp.normalImpulses[j] = c->m_manifold.points[j].normalImpulse;
+            imp.tangentImpulses[j] = c->m_manifold.points[j].tangentImpulse;
+        }
+        m_contactListener->PostSolve(c, &imp);
+    };
+    if (collideEventsDone) {
+        for (int32_t t : m_postSolveOrder)
+            if (t >= 0 && t < nc && solved[t]) postSolve(&m_contacts[nc - 1 - t]);
+    } else {
+        for (b2Contact* c = list; c; c = c->GetNext())
+            if (c->IsTouching() && solved[nc - 1 - (int)(c - &m_contacts[0])]) postSolve(c);
+    }
+    m_postSolveOrder.clear();
     // Without the split step every touching change of the step is reported
     // here; with it, only those of the TOI phase (the narrowphase's were
     // reported inside the step).
-    (void)collideEventsDone;
     for (uint64_t k : m_touchingKeys) {
         if (after.count(k)) continue;
         b2Contact ended;
@@ -1904,14 +2014,6 @@ void b2World::ReplayEvents(bool collideEventsDone) {
         if (!c->IsTouching()) continue;
         const uint64_t k = pairKeyOf(c->m_fixtureA->m_index + c->m_indexA, c->m_fixtureB->m_index + c->m_indexB);
         if (!before.count(k)) m_contactListener->BeginContact(c);
-        if (!solved[nc - 1 - (int)(c - &m_contacts[0])]) continue;  // not solved this step: no PostSolve
-        b2ContactImpulse imp{};
-        imp.count = c->m_manifold.pointCount;
-        for (int j = 0; j < imp.count; ++j) {
-            imp.normalImpulses[j] = c->m_manifold.points[j].normalImpulse;
-            imp.tangentImpulses[j] = c->m_manifold.points[j].tangentImpulse;
-        }
-        m_contactListener->PostSolve(c, &imp);
     }
     m_touchingKeys.swap(now);
 }

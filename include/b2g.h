@@ -717,11 +717,13 @@ int b2g_autotune(b2g_context* ctx, const b2g_step_params* p, int rounds, b2g_sch
  * b2g_step in two halves, so that the host can run b2ContactListener's
  * BeginContact, EndContact and PreSolve inside the step as
  * b2ContactManager::Collide does (Box2D/Dynamics/b2ContactManager.cpp:110-175,
- * Contacts/b2Contact.cpp:170-256). Coloured solver only.
+ * Contacts/b2Contact.cpp:170-256). Either solver (the reference-order
+ * narrowphase commits in list order before the split).
  *   b2g_step_collide: the narrowphase over the contact table. Until
  *     b2g_step_solve, b2g_get_contacts returns the updated table including
  *     the contacts Collide destroys, with meta[2] bits 1 touching,
- *     2 enabled, 4 destroyed this step, 8 updated this step, and
+ *     2 enabled, 4 destroyed this step, 8 updated this step, 16 touching
+ *     before the update, and
  *     b2g_get_old_manifolds the manifolds from before the update (in the
  *     same order);
  *   b2g_override_contacts: PreSolve's edits per contact index: enabled for

@@ -8,13 +8,15 @@
 // Differences from the reference, by design of the GPU path:
 //  * continuous collision (SetContinuousPhysics, on by default as in the
 //    reference) runs on the GPU after the discrete solve; the contact
-//    updates of its TOI events call no listener.
-//  * sensors are rejected at creation; every joint type of the reference
-//    runs with the coloured solver, and the reference-order mode rejects
-//    worlds with joints.
+//    updates of its TOI events call no listener during the step (their
+//    touching changes are reported after it).
 //  * with a contact listener, Step splits at the narrowphase: BeginContact,
 //    EndContact and PreSolve are called inside the step in list order,
-//    PostSolve after it for the solved contacts.
+//    PostSolve after the solve for the island contacts, in the reference's
+//    island order.
+//  * DestroyBody and DestroyFixture keep device slots; fixtures created
+//    afterwards take new proxy ids where the reference reuses freed ones,
+//    which can change the order of new contacts (not the pair set).
 #pragma once
 #include <vector>
 #include <string>
@@ -883,6 +885,10 @@ private:
     void DownloadJoints() const;
     void ResolveGear(b2GearJoint* g);
     int32_t CollideWithListener(const void* params);
+    // b2World::Solve's island search over the contact table after the
+    // narrowphase: the island contacts in the order Report reaches them.
+    void IslandContactOrder(const std::vector<int32_t>& meta, const std::vector<uint8_t>& on,
+                            std::vector<int32_t>* order) const;
     // PostSolve after the Step; Begin/End too unless the split step
     // already delivered them
     void ReplayEvents(bool collideEventsDone);
@@ -911,6 +917,7 @@ private:
     std::vector<b2Contact> m_contacts;
     bool m_contactsStale = true;
     std::vector<uint64_t> m_touchingKeys;  // touching fixture pairs after the last Step
+    std::vector<int32_t> m_postSolveOrder; // PostSolve order of the open step (table indices after it)
     std::vector<int32_t> m_removedFixtures;  // proxies to drop at the next upload
     std::vector<int32_t> m_updatedFixtures;  // material/filter changes to upload
     int32_t m_subWorldCount = 0;

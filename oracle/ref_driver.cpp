@@ -32,7 +32,7 @@ namespace {
 
 // Records the listener calls (kind 1 BeginContact, 2 EndContact, 3 PreSolve,
 // 4 PostSolve; fixture indices; for PreSolve whether the contact stays
-// enabled) and disables the contacts of a one-sided platform fixture while
+// enabled, for PostSolve the bits of the first normal impulse) and disables the contacts of a one-sided platform fixture while
 // the other body's origin is less than `clear` above the platform's top (a
 // one-sided platform after the testbed's OneSidedPlatform,
 // Testbed/Tests/OneSidedPlatform.h).
@@ -69,7 +69,12 @@ struct Recorder : public b2ContactListener {
         if (b == conveyor) c->SetTangentSpeed(-belt);
         record(3, c, c->IsEnabled() ? 1 : 0);
     }
-    void PostSolve(b2Contact* c, const b2ContactImpulse*) override { record(4, c, 0); }
+    // PostSolve's flag: the bits of the first point's normal impulse
+    void PostSolve(b2Contact* c, const b2ContactImpulse* imp) override {
+        int32_t bits = 0;
+        if (imp->count > 0) memcpy(&bits, &imp->normalImpulses[0], sizeof(bits));
+        record(4, c, bits);
+    }
 };
 
 struct RefWorld {

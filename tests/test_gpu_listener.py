@@ -1,18 +1,21 @@
 """b2ContactListener on the GPU path against the reference build.
 
 With a listener set, b2World::Step splits at the narrowphase
-(b2g_step_collide / b2g_step_solve) and makes the reference's calls inside
-the step: over the contact list, newest first, EndContact for destroyed
-touching contacts, BeginContact / EndContact on touching changes and
-PreSolve with the old manifold; PreSolve's SetEnabled(false) keeps the
-contact out of the step's solve. PostSolve follows the step for the solved
-contacts.
+(b2g_step_collide / b2g_step_solve), with either solver, and makes the
+reference's calls inside the step: over the contact list, newest first,
+EndContact for destroyed touching contacts, BeginContact / EndContact on
+touching changes and PreSolve with the old manifold; PreSolve's
+SetEnabled(false) keeps the contact out of the step's solve. PostSolve
+follows for the island contacts, in the order of b2World::Solve's island
+search (island after island, each in the order the search adds them), the
+recorded flag being the bits of the first normal impulse.
 
 The one-sided platform scene has one contact per island and bodies of fixed
 rotation, so the coloured solver's order is the reference's and no in-step
-trig rounds differently: poses and the sequence of listener calls must equal
-the reference's exactly. PostSolve calls are compared per step as a set (the
-reference reports them island by island, the GPU in list order).
+trig rounds differently: poses and the whole sequence of listener calls must
+equal the reference's exactly. The reference-order solver keeps them equal
+with several contacts per island too (the conveyor's boxes landing on each
+other).
 """
 from __future__ import annotations
 
@@ -38,9 +41,11 @@ def _bits(a):
     return (a + np.float32(0)).view(np.uint32)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("sleep", [True, False])
-def test_one_sided_platform_matches_reference(gpu, ref, sleep):
+def test_one_sided_platform_matches_reference(gpu, ref, sleep, mode):
     g = gpu.world("oneway", 8, 0, sleep=sleep)
+    g.set_mode(mode)
     r = ref.world("oneway", 8, 0, sleep=sleep)
     for w in (g, r):
         w.e.fn("world_set_recorder")(w.h, 1, F(5.25), F(0.1))
@@ -50,11 +55,9 @@ def test_one_sided_platform_matches_reference(gpu, ref, sleep):
         gb, rb = g.bodies(), r.bodies()
         assert np.array_equal(_bits(gb[:, :11]), _bits(rb[:, :11])), f"step {s + 1}"
     ge, re_ = _events(g), _events(r)
-    inside = lambda e: e[e[:, 1] != 4]  # Begin, End, PreSolve: in call order
-    assert np.array_equal(inside(ge), inside(re_))
-    post = lambda e: sorted(map(tuple, e[e[:, 1] == 4].tolist()))
-    assert post(ge) == post(re_)
+    assert np.array_equal(ge, re_)                          # every call, in order, with PostSolve's impulses
     assert ((ge[:, 1] == 3) & (ge[:, 4] == 0)).sum() > 8  # PreSolve did disable contacts
+    assert (ge[:, 1] == 4).sum() > 100
     assert np.all(np.abs(gb[2:, E.CY] - 5.5) < 0.02)       # the boxes landed on the platform
     g.close()
     r.close()
@@ -90,8 +93,11 @@ def test_split_step_state_errors(gpu):
     assert lib.b2g_step(c, E.ptr(p)) != 0                  # must finish first
     E.b2g_check(lib, c, lib.b2g_step_solve(c))
     E.b2g_check(lib, c, lib.b2g_step(c, E.ptr(p)))
-    p[0]["mode"] = 1
-    assert lib.b2g_step_collide(c, E.ptr(p)) != 0          # coloured solver only
+    p[0]["mode"] = 1                                       # the reference-order solver splits too
+    E.b2g_check(lib, c, lib.b2g_step_collide(c, E.ptr(p)))
+    E.b2g_check(lib, c, lib.b2g_step_solve(c))
+    p[0]["mode"] = 2
+    assert lib.b2g_step_collide(c, E.ptr(p)) != 0          # no such solver
     w.close()
 
 
@@ -129,8 +135,8 @@ def test_sensors_match_reference(gpu, ref):
     rotation falling through a sensor slab, every third with a sensor halo,
     landing apart (one solid contact per island). Poses equal the reference
     build's bit for bit, and so does the sequence of Begin / End / PreSolve
-    calls (sensor contacts report Begin and End, never PreSolve); PostSolve
-    per step as a set."""
+    calls (sensor contacts report Begin and End, never PreSolve, and never
+    join an island's PostSolve)."""
     g = gpu.world("sensors", 8, 0, sleep=True)
     r = ref.world("sensors", 8, 0, sleep=True)
     for w in (g, r):
@@ -140,10 +146,7 @@ def test_sensors_match_reference(gpu, ref):
         r.step(1)
         assert np.array_equal(_bits(g.bodies()[:, :11]), _bits(r.bodies()[:, :11])), f"step {s + 1}"
     ge, re_ = _events(g), _events(r)
-    inside = lambda e: e[e[:, 1] != 4]
-    assert np.array_equal(inside(ge), inside(re_))
-    post = lambda e: sorted(map(tuple, e[e[:, 1] == 4].tolist()))
-    assert post(ge) == post(re_)
+    assert np.array_equal(ge, re_)
     slab = ((ge[:, 2] == 1) | (ge[:, 3] == 1))
     assert ((ge[:, 1] == 1) & slab).sum() == 8 + 3 and not ((ge[:, 1] == 3) & slab).any()
     g.close()
@@ -167,9 +170,33 @@ def test_conveyor_matches_reference(gpu, ref):
         r.step(1)
         assert np.array_equal(_bits(g.bodies()[:, :11]), _bits(r.bodies()[:, :11])), f"step {s + 1}"
     ge, re_ = _events(g), _events(r)
-    inside = lambda e: e[e[:, 1] != 4]
-    assert np.array_equal(inside(ge), inside(re_))
+    assert np.array_equal(ge, re_)
     g.step(220)
     assert np.all(g.bodies()[2:, E.CX] > 10.0)
+    g.close()
+    r.close()
+
+
+def test_conveyor_reference_order_islands(gpu, ref):
+    """The conveyor with the reference-order solver over 400 steps, past the
+    boxes landing on each other beyond the belt's end: islands of several
+    contacts, solved in Box2D's order. Poses equal the reference build's bit
+    for bit every step, and so does the whole sequence of listener calls,
+    PostSolve island by island in the island search's order with its
+    impulses."""
+    g = gpu.world("conveyor", 6, 0, sleep=True)
+    g.set_mode(1)
+    r = ref.world("conveyor", 6, 0, sleep=True)
+    for w in (g, r):
+        w.e.fn("world_set_conveyor")(w.h, 1, F(5.0))
+    for s in range(400):
+        g.step(1)
+        r.step(1)
+        assert np.array_equal(_bits(g.bodies()[:, :11]), _bits(r.bodies()[:, :11])), f"step {s + 1}"
+    ge, re_ = _events(g), _events(r)
+    assert np.array_equal(ge, re_)
+    post = ge[ge[:, 1] == 4]
+    steps, counts = np.unique(post[:, 0], return_counts=True)
+    assert counts.max() >= 3  # a step reported islands with several contacts
     g.close()
     r.close()
