@@ -1,12 +1,18 @@
 // Kernels of the warm-start network (layout and parameter vector in mlp.h).
 //
-// One wavefront evaluates 16 samples at a time with the 16x16x16 BF16
-// matrix-core instruction. Lane l = 16*g + c (g = l / 16, c = l % 16):
+// One wavefront evaluates 16 samples at a time on the matrix cores. Lane
+// l = 16*g + c (g = l / 16, c = l % 16). With the 16x16x16 BF16 instruction:
 //   A operand (weights, 16 rows x 16 k): row c, k = 4g..4g+3
 //   B operand (activations, 16 k x 16 samples): k = 4g..4g+3, sample c
 //   accumulator: rows 4g..4g+3, sample c
 // so a layer's accumulator, after the activation and conversion to BF16, is
-// the B operand of the next layer's product over the same 16 rows.
+// the B operand of the next layer's product over the same 16 rows. Products
+// over 64 features use gfx950's 16x16x32 BF16 instruction (two per 16
+// output rows instead of four), whose lane l holds k = 8g..8g+7: its B
+// operand for features [32t, 32t + 32) is the pair of accumulator tiles 2t
+// and 2t + 1 side by side (features 32t + 4g + v, then 32t + 16 + 4g + v),
+// and its A operand takes the weight columns in that same order (two runs of
+// four from LDS). The sum runs over the same features in another order.
 #include "mlp.h"
 
 #include <hip/hip_ext.h>
@@ -24,16 +30,31 @@ constexpr int kMaxGradBlocks = 256; // workgroups of one gradient launch at most
 constexpr int kThreads = 64 * kWaves;
 constexpr int kBiasCount = 3 * kHidden + kOutPad;
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
 __device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// Two 4-element halves side by side as an 8-element operand.
+__device__ inline bf16x8 join8(bf16x4 lo, bf16x4 hi) {
+    const short s[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    bf16x8 r;
+    __builtin_memcpy(&r, s, sizeof(r));
+    return r;
+}
+
+// 16x16x32: features [32t, 32t + 32) of a 64-feature product, as above.
+__device__ inline f32x4 mfma32(bf16x4 aLo, bf16x4 aHi, bf16x4 bLo, bf16x4 bHi, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(join8(aLo, aHi), join8(bLo, bHi), c, 0, 0, 0);
+}
+
+// Round to nearest even BF16 by the hardware conversion (the toBf16 rule for
+// finite inputs).
 __device__ inline bf16x4 pack4(float a, float b, float c, float d) {
+    const __bf16 h[4] = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
     bf16x4 r;
-    r[0] = (short)toBf16(a);
-    r[1] = (short)toBf16(b);
-    r[2] = (short)toBf16(c);
-    r[3] = (short)toBf16(d);
+    __builtin_memcpy(&r, h, sizeof(r));
     return r;
 }
 
@@ -44,12 +65,32 @@ __device__ inline bf16x4 frag(const uint16_t* p) {
     return r;
 }
 
-// GELU with the Gaussian CDF, 0.5 x (1 + erf(x / sqrt 2)), and its derivative.
-__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// GELU with the Gaussian CDF, 0.5 x (1 + erf(x / sqrt 2)), and its
+// derivative cdf(x) + x pdf(x). erf is Abramowitz and Stegun's rational
+// approximation 7.1.26 (Handbook of Mathematical Functions, 1964; absolute
+// error below 1.5e-7, far under the BF16 rounding of the activations), whose
+// exponential exp(-z^2) at z = x / sqrt 2 is also the density's exp(-x^2 / 2),
+// so both come from one hardware exponential and one reciprocal instead of
+// the library's erff and expf.
+struct Cdf {
+    float cdf, pdf;
+};
+__device__ inline Cdf gaussCdf(float x) {
+    const float z = fabsf(x) * 0.70710678118654752f;
+    const float e = __expf(-0.5f * x * x);
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * z);
+    const float poly =
+        t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    const float erfAbs = 1.0f - poly * e;
+    Cdf r;
+    r.cdf = 0.5f * (1.0f + (x < 0.0f ? -erfAbs : erfAbs));
+    r.pdf = 0.39894228040143268f * e;
+    return r;
+}
+__device__ inline float gelu(float x) { return x * gaussCdf(x).cdf; }
 __device__ inline float geluGrad(float x) {
-    const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
-    return cdf + x * pdf;
+    const Cdf c = gaussCdf(x);
+    return c.cdf + x * c.pdf;
 }
 
 struct Smem {
@@ -89,25 +130,35 @@ __device__ inline f32x4 biasTile(const Smem& S, int base, int g) {
     return r;
 }
 
-// Hidden layer: acc[mt] = W[16mt + c][16kt + 4g + j] x in[kt] + bias.
+// The product of weight rows [16mt, 16mt + 16) (row stride ldw, in LDS at
+// wOff) with the four 16-feature tiles in[0..3] of a 64-feature input, added
+// to acc: two 16x16x32 products.
+__device__ inline f32x4 product64(const Smem& S, int wOff, int ldw, int mt, const bf16x4* in, f32x4 acc, int g,
+                                  int c) {
+    const uint16_t* row = &S.w[wOff + (16 * mt + c) * ldw];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+        acc = mfma32(frag(row + 32 * t + 4 * g), frag(row + 32 * t + 16 + 4 * g), in[2 * t], in[2 * t + 1], acc);
+    return acc;
+}
+
+// Hidden layer: acc[mt] = W[16mt + c][.] x in + bias; the 16-feature input
+// layer with one 16x16x16 product, the 64-feature ones with product64.
 template <int KT>
 __device__ inline void hiddenLayer(const Smem& S, int wOff, int ldw, int bOff, const bf16x4* in, f32x4* z,
                                    bf16x4* a, int g, int c) {
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
         f32x4 acc = biasTile(S, bOff + 16 * mt, g);
-#pragma unroll
-        for (int kt = 0; kt < KT; ++kt) acc = mfma(frag(&S.w[wOff + (16 * mt + c) * ldw + 16 * kt + 4 * g]), in[kt], acc);
+        if (KT == 1) acc = mfma(frag(&S.w[wOff + (16 * mt + c) * ldw + 4 * g]), in[0], acc);
+        else acc = product64(S, wOff, ldw, mt, in, acc, g, c);
         z[mt] = acc;
         a[mt] = pack4(gelu(acc[0]), gelu(acc[1]), gelu(acc[2]), gelu(acc[3]));
     }
 }
 
 __device__ inline f32x4 outputLayer(const Smem& S, const bf16x4* a3, int g, int c) {
-    f32x4 acc = biasTile(S, 3 * kHidden, g);
-#pragma unroll
-    for (int kt = 0; kt < 4; ++kt) acc = mfma(frag(&S.w[hW4 + c * kHidden + 16 * kt + 4 * g]), a3[kt], acc);
-    return acc;
+    return product64(S, hW4, kHidden, 0, a3, biasTile(S, 3 * kHidden, g), g, c);
 }
 
 // ------------------------------------------------------------ inference
@@ -282,10 +333,7 @@ __global__ __launch_bounds__(kThreads) void k_mlp_grad(const uint16_t* wh, const
             f32x4 d2[4];
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
-                f32x4 acc = zero4;
-#pragma unroll
-                for (int kt = 0; kt < 4; ++kt)
-                    acc = mfma(frag(&S.w[hW3t + (16 * mt + c) * kHidden + 16 * kt + 4 * g]), dB[kt], acc);
+                f32x4 acc = product64(S, hW3t, kHidden, mt, dB, zero4, g, c);
                 for (int j = 0; j < 4; ++j) acc[j] *= geluGrad(z2[mt][j]);
                 d2[mt] = acc;
             }
@@ -303,10 +351,7 @@ __global__ __launch_bounds__(kThreads) void k_mlp_grad(const uint16_t* wh, const
             const bf16x4 xT = transposeH(TH, x0[0], g, c);
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
-                f32x4 acc = zero4;
-#pragma unroll
-                for (int kt = 0; kt < 4; ++kt)
-                    acc = mfma(frag(&S.w[hW2t + (16 * mt + c) * kHidden + 16 * kt + 4 * g]), dB[kt], acc);
+                f32x4 acc = product64(S, hW2t, kHidden, mt, dB, zero4, g, c);
                 for (int j = 0; j < 4; ++j) acc[j] *= geluGrad(z1[mt][j]);
                 const bf16x4 dT = transposeF(TF, acc, g, c);
                 gW1[mt] = mfma(dT, xT, gW1[mt]);
@@ -356,15 +401,27 @@ __global__ __launch_bounds__(kThreads) void k_mlp_grad(const uint16_t* wh, const
     if (threadIdx.x == 0) row[kParams] = lossShared;
 }
 
-// Sum of the workgroups' rows, in workgroup order, added to the gradient
-// and the loss.
-__global__ void k_mlp_reduce(const float* partial, int rows, float* grad, float* loss) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > kParams) return;
+// Sum of the workgroups' rows added to the gradient and the loss, in a
+// fixed order: a workgroup takes kReduceCols consecutive entries; its
+// kReduceSplit lane groups each sum every kReduceSplit-th row in row order
+// (coalesced across the entries), then the groups' sums are added in group
+// order.
+constexpr int kReduceCols = 64, kReduceSplit = 4;
+__global__ __launch_bounds__(kReduceCols * kReduceSplit) void k_mlp_reduce(const float* partial, int rows,
+                                                                            float* grad, float* loss) {
+    __shared__ float part[kReduceSplit][kReduceCols];
+    const int col = threadIdx.x % kReduceCols, grp = threadIdx.x / kReduceCols;
+    const int i = blockIdx.x * kReduceCols + col;
     float acc = 0.0f;
-    for (int r = 0; r < rows; ++r) acc += partial[(size_t)r * (kParams + 1) + i];
-    if (i < kParams) grad[i] += acc;
-    else *loss += acc;
+    if (i <= kParams)
+        for (int r = grp; r < rows; r += kReduceSplit) acc += partial[(size_t)r * (kParams + 1) + i];
+    part[grp][col] = acc;
+    __syncthreads();
+    if (grp != 0 || i > kParams) return;
+    float sum = part[0][col];
+    for (int k = 1; k < kReduceSplit; ++k) sum += part[k][col];
+    if (i < kParams) grad[i] += sum;
+    else *loss += sum;
 }
 
 // BF16 image entries of parameter i (weights only; biases stay FP32).
@@ -479,8 +536,8 @@ hipError_t mlpGradient(const Net& n, const float* x, const float* t, const int* 
     if (grid < 1) grid = 1;
     hipExtLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(kThreads), 0, s, e0, e1, 0, n.wh, n.params, n.norm, x, t,
                           order, first, count, 1.0f / (float)count, n.partial);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((kParams + 1 + 255) / 256), dim3(256), 0, s, n.partial, grid, n.grad,
-                       n.loss);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((kParams + 1 + kReduceCols - 1) / kReduceCols),
+                       dim3(kReduceCols * kReduceSplit), 0, s, n.partial, grid, n.grad, n.loss);
     return hipGetLastError();
 }
 
