@@ -1,7 +1,8 @@
 // Warm-start network of the learned warm start: a 16-64-64-64-2 multilayer
 // perceptron with GELU activations, BF16 weights and activations on the
-// matrix cores (v_mfma_f32_16x16x16_bf16), FP32 accumulation, FP32 master
-// weights trained with Adam by the kernels in mlp.hip.
+// matrix cores (v_mfma_f32_16x16x32_bf16 for the 64-feature products,
+// v_mfma_f32_16x16x16_bf16 for the 16-wide ones), FP32 accumulation, FP32
+// master weights trained with Adam by the kernels in mlp.hip.
 //
 // Activations are kept in a "feature x sample" layout: a wavefront owns 16
 // samples, and the accumulator tile of one layer (lane = 16*g + c holds
