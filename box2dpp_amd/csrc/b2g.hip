@@ -8732,6 +8732,70 @@ int b2g_get_colour_counts(b2g_context* ctx, int32_t* out, int cap, int* count) {
     return B2G_OK;
 }
 
+int b2g_prim_exclusive_sum(b2g_context* ctx, const int32_t* in, int n, int32_t* out) {
+    if (!ctx || n < 0 || (n > 0 && (!in || !out))) return B2G_ERR_ARG;
+    if (n == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<int> din, dout;
+    DevBuf<char> scratch;
+    const size_t sb = prims_scratch_bytes(n);
+    HIPCHK(ctx, din.alloc(n)); HIPCHK(ctx, dout.alloc(n)); HIPCHK(ctx, scratch.alloc(sb));
+    HIPCHK(ctx, hipMemcpyAsync(din.p, in, sizeof(int) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, prims_exclusive_sum(scratch.p, sb, din.p, dout.p, n, s));
+    HIPCHK(ctx, hipMemcpyAsync(out, dout.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+int b2g_prim_sort_pairs(b2g_context* ctx, const uint32_t* keys, const int32_t* vals, int n, int bits, int method,
+                        uint32_t* outKeys, int32_t* outVals, int32_t* totals, int totalsKeys) {
+    if (!ctx || n < 0 || (n > 0 && (!keys || !vals || !outKeys || !outVals))) return B2G_ERR_ARG;
+    if (method != 0 && method != 1) return fail(ctx, B2G_ERR_ARG, "sort method: 0 (radix) or 1 (7-bit)");
+    if (method == 0 && (bits < 1 || bits > 32)) return fail(ctx, B2G_ERR_ARG, "sort bits: 1-32");
+    if (method == 1 && (totalsKeys < 0 || totalsKeys > 128 || (totalsKeys > 0 && !totals)))
+        return fail(ctx, B2G_ERR_ARG, "totals: 0-128 keys");
+    if (n == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<uint32_t> dk, dko;
+    DevBuf<int> dv, dvo, dt;
+    DevBuf<char> scratch;
+    const size_t sb = prims_scratch_bytes(n);
+    HIPCHK(ctx, dk.alloc(n)); HIPCHK(ctx, dko.alloc(n)); HIPCHK(ctx, dv.alloc(n)); HIPCHK(ctx, dvo.alloc(n));
+    HIPCHK(ctx, dt.alloc(128)); HIPCHK(ctx, scratch.alloc(sb));
+    HIPCHK(ctx, hipMemcpyAsync(dk.p, keys, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dv.p, vals, sizeof(int) * n, hipMemcpyHostToDevice, s));
+    if (method == 0) {
+        HIPCHK(ctx, prims_sort_pairs_u32(scratch.p, sb, dk.p, dko.p, dv.p, dvo.p, n, bits, s));
+    } else {
+        if (totalsKeys > 0)
+            HIPCHK(ctx, hipMemcpyAsync(dt.p, totals, sizeof(int) * totalsKeys, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, prims_count_sort_pairs(scratch.p, sb, dk.p, dko.p, dv.p, dvo.p, n, totalsKeys > 0 ? dt.p : nullptr,
+                                           totalsKeys, s));
+        if (totalsKeys > 0)
+            HIPCHK(ctx, hipMemcpyAsync(totals, dt.p, sizeof(int) * totalsKeys, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(ctx, hipMemcpyAsync(outKeys, dko.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(outVals, dvo.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
+int b2g_prim_sort_keys_u64(b2g_context* ctx, const uint64_t* keys, int n, int endBit, uint64_t* out) {
+    if (!ctx || n < 0 || (n > 0 && (!keys || !out))) return B2G_ERR_ARG;
+    if (endBit < 1 || endBit > 64) return fail(ctx, B2G_ERR_ARG, "sort endBit: 1-64");
+    if (n == 0) return B2G_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<unsigned long long> dk, dko;
+    DevBuf<char> scratch;
+    const size_t sb = prims_scratch_bytes(n);
+    HIPCHK(ctx, dk.alloc(n)); HIPCHK(ctx, dko.alloc(n)); HIPCHK(ctx, scratch.alloc(sb));
+    HIPCHK(ctx, hipMemcpyAsync(dk.p, keys, sizeof(uint64_t) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, prims_sort_keys_u64(scratch.p, sb, dk.p, dko.p, n, s, endBit));
+    HIPCHK(ctx, hipMemcpyAsync(out, dko.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return B2G_OK;
+}
+
 int b2g_check_colouring(b2g_context* ctx, int* conflicts, int* maskErrors) {
     if (!ctx || !conflicts || !maskErrors) return B2G_ERR_ARG;
     *conflicts = 0;

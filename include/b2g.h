@@ -807,6 +807,23 @@ int b2g_set_colour_demotion(b2g_context* ctx, int rounds);
  * many launches a pass takes. */
 int b2g_get_colour_counts(b2g_context* ctx, int32_t* out, int cap, int* count);
 
+/* The device scan and sorts the step is built on, run once on host buffers
+ * (device buffers allocated per call). No reference counterpart: the
+ * reference sorts its pair buffer on the host with std::sort
+ * (Box2D/Collision/b2BroadPhase.h:220) and needs no scan; these let tests
+ * check the primitives themselves against a host sort.
+ *  - b2g_prim_exclusive_sum: out[i] = in[0] + ... + in[i-1].
+ *  - b2g_prim_sort_pairs: stable sort of (key, value) pairs; method 0 by the
+ *    low `bits` bits of key (1-32, higher bits ignored), method 1 by the low
+ *    7 bits (one pass) with totals[k] += count of key k for k < totalsKeys
+ *    (0-128; totals may be null when 0).
+ *  - b2g_prim_sort_keys_u64: ascending sort of 64-bit keys whose bits at and
+ *    above endBit (1-64) are all equal. */
+int b2g_prim_exclusive_sum(b2g_context* ctx, const int32_t* in, int n, int32_t* out);
+int b2g_prim_sort_pairs(b2g_context* ctx, const uint32_t* keys, const int32_t* vals, int n, int bits, int method,
+                        uint32_t* outKeys, int32_t* outVals, int32_t* totals, int totalsKeys);
+int b2g_prim_sort_keys_u64(b2g_context* ctx, const uint64_t* keys, int n, int endBit, uint64_t* out);
+
 #ifdef __cplusplus
 }
 #endif
