@@ -834,8 +834,6 @@ def _kernel_short(name):
     m = re.search(r"(k_[A-Za-z0-9_]+(?:<[^>]*>)?)", name)
     if m:
         return m.group(1)
-    if "rocprim" in name or "hipcub" in name:
-        return "rocprim sort/scan"
     return name.split("(")[0][-48:]
 
 
