@@ -196,6 +196,10 @@ __device__ inline unsigned long long pairKey(int a, int b) {
     return ((unsigned long long)(unsigned)lo << 32) | (unsigned)hi;
 }
 
+// A proxy's broad-phase order key and back (WorldView::ford / ordf).
+__device__ inline int orderOf(const WorldView& w, int f) { return w.ford ? w.ford[f] : f; }
+__device__ inline int fixtureOf(const WorldView& w, int k) { return w.ordf ? w.ordf[k] : k; }
+
 __device__ inline uint32_t hashU64(unsigned long long k) {
     k ^= k >> 33;
     k *= 0xff51afd7ed558ccdULL;
@@ -230,6 +234,10 @@ __device__ inline void wakeBody(const WorldView& w, int b) {
 }
 
 __device__ inline bool sensorOf(const WorldView& w, int f) { return w.fmat[f].w != 0.0f; }
+
+// A removed proxy's fat box is inverted (k_remove_fixtures); a live one
+// never is.
+__device__ inline bool removedFixture(const WorldView& w, int f) { return w.fat[f].x > w.fat[f].z; }
 
 // b2ContactFilter::ShouldCollide, REF/Dynamics/b2WorldCallbacks.cpp:26-38.
 __device__ inline bool filterPass(const WorldView& w, int fa, int fb) {
@@ -366,6 +374,12 @@ __device__ inline bool collideOne(const WorldView& w, Counters* cnt, int c, bool
     int flags = w.cflags[c] & ~CF_UPDATED;
     const int fA = w.cfA[c], fB = w.cfB[c];
     const int bA = w.fbody[fA], bB = w.fbody[fB];
+    if (removedFixture(w, fA) || removedFixture(w, fB)) {
+        // destroyed with its body (its wake-ups done by k_wake_removed_contacts)
+        if (flags & CF_TOUCHING) atomicAdd(&cnt->endEvents, 1);
+        w.cflags[c] = flags | CF_DEAD;
+        return false;
+    }
     if (flags & CF_FILTER) {
         const bool dynAny = w.btype[bA] == BT_DYNAMIC || w.btype[bB] == BT_DYNAMIC;
         if (!dynAny || jointKeepsApart(w, bA, bB) || !filterPass(w, fA, fB)) {
@@ -4019,18 +4033,32 @@ __device__ inline bool acceptPair(const WorldView& w, int a, int b, const unsign
 }
 
 // Where queries put the pairs they find. With packBase = n (the proxy
-// count) a pair a < b is stored as a * n + b, which orders pairs as
-// pairKey does in fewer bits (fewer radix passes); with 0 as pairKey.
+// count, or the order-key span with `ord`) a pair is stored as lo * n + hi,
+// lo < hi its two fixtures' order keys (`ord`, or the fixture indices when
+// null), which orders pairs as pairKey does in fewer bits (fewer radix
+// passes); with packBase 0 as pairKey of the fixture indices.
 struct PairSink {
     unsigned long long* keys;
     int cap;
     unsigned long long packBase;
+    const int* ord;
 };
 
 __device__ inline void emitPair(PairSink sink, Counters* cnt, int a, int b) {
+    unsigned long long key;
+    if (sink.packBase) {
+        if (sink.ord) {
+            const int ka = sink.ord[a], kb = sink.ord[b];
+            if (ka < 0 || kb < 0) return;  // no proxy (only inverted boxes, which overlap nothing)
+            a = ka < kb ? ka : kb;
+            b = ka < kb ? kb : ka;
+        }
+        key = (unsigned long long)a * sink.packBase + (unsigned long long)b;
+    } else {
+        key = pairKey(a, b);
+    }
     const int slot = atomicAdd(&cnt->nNewPairs, 1);
-    if (slot < sink.cap)
-        sink.keys[slot] = sink.packBase ? (unsigned long long)a * sink.packBase + (unsigned long long)b : pairKey(a, b);
+    if (slot < sink.cap) sink.keys[slot] = key;
     else atomicOr(&cnt->overflow, 1);
 }
 
@@ -4147,8 +4175,8 @@ __global__ void k_pairs_wide(const float4* fat, const uint8_t* moved, int n, Gri
 // b2ContactManager::AddPair's contact creation (REF/Dynamics/b2ContactManager.cpp:
 // 179-296): registry order, mixed friction and restitution, no manifold, and
 // both bodies woken unless a fixture is a sensor.
-__device__ inline void appendContactOne(const WorldView& w, unsigned long long key, int c) {
-    int a = (int)(key >> 32), b = (int)(key & 0xFFFFFFFFu);
+// a: the fixture of the pair's smaller order key (AddPair's fixture A).
+__device__ inline void appendContactOne(const WorldView& w, int a, int b, int c) {
     if (!primaryPair(w.shape[a].type, w.shape[b].type)) {
         const int t = a;
         a = b;
@@ -4172,13 +4200,13 @@ __device__ inline void appendContactOne(const WorldView& w, unsigned long long k
     }
 }
 
-// keys packed as a * packBase + b (PairSink).
+// keys packed as lo * packBase + hi of the two order keys (PairSink).
 __global__ void k_append_contacts(WorldView w, const unsigned long long* keys, int nNew, int base,
                                   unsigned long long packBase) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nNew) return;
     const unsigned long long key = keys[k];
-    appendContactOne(w, pairKey((int)(key / packBase), (int)(key % packBase)), base + k);
+    appendContactOne(w, fixtureOf(w, (int)(key / packBase)), fixtureOf(w, (int)(key % packBase)), base + k);
 }
 
 // Contacts between bodies that a new joint keeps apart are flagged for
@@ -4187,6 +4215,19 @@ __global__ void k_flag_joint_contacts(WorldView w) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= w.nContacts) return;
     if (jointKeepsApart(w, w.fbody[w.cfA[c]], w.fbody[w.cfB[c]])) w.cflags[c] |= CF_FILTER;
+}
+
+// The pair keys of contacts first..first+n-1 into the live-contact set.
+__global__ void k_hash_insert_range(WorldView w, int first, int n, unsigned long long* table, uint32_t mask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long key = pairKey(w.cfA[first + i], w.cfB[first + i]);
+    uint32_t h = hashU64(key) & mask;
+    for (;;) {
+        const unsigned long long prev = atomicCAS(&table[h], ~0ull, key);
+        if (prev == ~0ull || prev == key) return;
+        h = (h + 1) & mask;
+    }
 }
 
 __global__ void k_hash_insert_keys(const unsigned long long* keys, int n, unsigned long long* table, uint32_t mask) {
@@ -4208,6 +4249,22 @@ __global__ void k_remove_fixtures(WorldView w, const int* ids, int n) {
     const int f = ids[i];
     w.fat[f] = make_float4(FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX);
     w.moved[f] = 0;
+}
+
+// b2World::DestroyBody destroys the body's contacts at once, and destroying
+// a contact with manifold points wakes both bodies (b2ContactManager::Destroy,
+// b2Contact::Destroy; REF/Dynamics/b2World.cpp:159-167,
+// b2ContactManager.cpp:39-101, Contacts/b2Contact.cpp:104-118). The wake-ups
+// happen here, as the proxies go; the next step's collide drops the
+// contacts before the activity test reads any body.
+__global__ void k_wake_removed_contacts(WorldView w) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    const int fA = w.cfA[c], fB = w.cfB[c];
+    if (!removedFixture(w, fA) && !removedFixture(w, fB)) return;
+    if ((w.cflags[c] & CF_DEAD) || w.cman[c].pointCount == 0 || sensorOf(w, fA) || sensorOf(w, fB)) return;
+    wakeBody(w, w.fbody[fA]);
+    wakeBody(w, w.fbody[fB]);
 }
 
 // b2Fixture::Refilter (REF/Dynamics/b2Fixture.cpp:183-213): flag contacts of
@@ -4275,6 +4332,11 @@ __global__ void k_collide_serial(WorldView w, RefView r, Counters* cnt) {
         const int fA = w.cfA[c], fB = w.cfB[c];
         const int bA = w.fbody[fA], bB = w.fbody[fB];
         const bool destroyWake = w.cman[c].pointCount > 0 && !sensorOf(w, fA) && !sensorOf(w, fB);
+        if (removedFixture(w, fA) || removedFixture(w, fB)) {
+            if (flags & CF_TOUCHING) cnt->endEvents++;
+            w.cflags[c] = flags | CF_DEAD;
+            continue;
+        }
         if (flags & CF_FILTER) {
             const bool dynAny = w.btype[bA] == BT_DYNAMIC || w.btype[bB] == BT_DYNAMIC;
             if (!dynAny || jointKeepsApart(w, bA, bB) || !filterPass(w, fA, fB)) {
@@ -5277,8 +5339,9 @@ __global__ void k_toi_pairs(WorldView w, ToiState* st, const unsigned long long*
         if (!overlaps(ff, w.fat[q])) continue;
         const int a = f < q ? f : q, b = f < q ? q : f;
         if (!acceptPair(w, a, b, table, hmask)) continue;
+        if (orderOf(w, a) < 0 || orderOf(w, b) < 0) continue;  // no proxy
         const int slot = atomicAdd(&st->nNew, 1);
-        if (slot < kToiNewCap) st->newKeys[slot] = pairKey(a, b);
+        if (slot < kToiNewCap) st->newKeys[slot] = pairKey(orderOf(w, a), orderOf(w, b));
     }
 }
 
@@ -5326,8 +5389,9 @@ __global__ __launch_bounds__(256) void k_toi_append(WorldView w, ToiState* st, u
             }
         }
         for (int i = threadIdx.x; i < nn; i += blockDim.x) {
-            appendContactOne(w, keys[i], base + i);
-            const unsigned long long key = keys[i];
+            const int a = fixtureOf(w, (int)(keys[i] >> 32)), b = fixtureOf(w, (int)(keys[i] & 0xFFFFFFFFu));
+            appendContactOne(w, a, b, base + i);
+            const unsigned long long key = pairKey(a, b);
             uint32_t h = hashU64(key) & hmask;
             for (;;) {
                 const unsigned long long prev = atomicCAS(&table[h], ~0ull, key);
@@ -5413,6 +5477,12 @@ struct b2g_context {
     int hashCap = 0;
     unsigned long long* newKeys = nullptr;
     unsigned long long* newKeysSorted = nullptr;
+    // broad-phase order keys (b2g_set_proxy_ids): key per fixture for the
+    // first orderCount fixtures, fixture per key below orderSpan; orderSpan
+    // 0 while unset (the fixture index is the key)
+    int* fixOrder = nullptr;
+    int* orderFix = nullptr;
+    int orderCount = 0, orderSpan = 0;
 
     // constraints
     ConView cv{};
@@ -5570,6 +5640,8 @@ struct b2g_context {
         w.nFixtures = nFixtures;
         w.fbody = fbody; w.shape = shape; w.fmat = fmat; w.ffilter = ffilter; w.fgroup = fgroup; w.fat = fat;
         w.moved = moved;
+        w.ford = orderSpan ? fixOrder : nullptr;
+        w.ordf = orderSpan ? orderFix : nullptr;
         w.nContacts = nContacts;
         w.cfA = cur.cfA; w.cfB = cur.cfB; w.cflags = cur.cflags; w.cmat = cur.cmat; w.ctan = cur.ctan; w.cman = cur.cman;
         w.ccolour = cur.ccolour;
@@ -5849,6 +5921,9 @@ void profKernel(b2g_context* ctx, int phase, int i);
 int runBroadphase(b2g_context* ctx, bool liveOnDevice = false) {
     hipStream_t s = ctx->stream;
     const int nf = ctx->nFixtures;
+    if (ctx->orderSpan > 0 && ctx->orderCount != nf)
+        return fail(ctx, B2G_ERR_STATE,
+                    "proxy ids cover " + std::to_string(ctx->orderCount) + " of " + std::to_string(nf) + " fixtures");
     if (nf == 0) {
         if (liveOnDevice) {
             const int rc = readCounters(ctx);
@@ -5893,11 +5968,14 @@ int runBroadphase(b2g_context* ctx, bool liveOnDevice = false) {
     }
     // The query only reads the grid, the fat boxes and the move flags, so it
     // is re-run after growing the contact arrays when its pairs do not fit.
+    // Pairs are packed by the proxies' order keys when the host set them.
+    const bool ordered = ctx->orderSpan > 0;
+    const int packBase = ordered ? ctx->orderSpan : nf;
     int nNew = 0;
     for (int attempt = 0;; ++attempt) {
         hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nNewPairs, 1, 0);
         hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->overflow, 1, 0);
-        PairSink sink{ctx->newKeys, ctx->capContacts, (unsigned long long)nf};
+        PairSink sink{ctx->newKeys, ctx->capContacts, (unsigned long long)packBase, ordered ? ctx->fixOrder : nullptr};
         const int qb = ctx->sched.queryBlock;
         hipEvent_t e0, e1;
         const int pk = profKernelEvents(ctx, &e0, &e1);
@@ -5923,13 +6001,13 @@ int runBroadphase(b2g_context* ctx, bool liveOnDevice = false) {
     hipLaunchKernelGGL(k_fill<uint8_t>, dim3(blocks(nf)), dim3(256), 0, s, ctx->moved, nf, (uint8_t)0);
     WorldView w = ctx->view();
     if (nNew > 0) {
-        // packed keys a * nf + b < nf * nf
+        // packed keys lo * packBase + hi < packBase^2
         int endBit = 1;
-        while (endBit < 64 && (1ull << endBit) < (unsigned long long)nf * (unsigned long long)nf) ++endBit;
+        while (endBit < 64 && (1ull << endBit) < (unsigned long long)packBase * (unsigned long long)packBase) ++endBit;
         HIPCHK(ctx, prims_sort_keys_u64(ctx->scratch, ctx->scratchBytes, ctx->newKeys, ctx->newKeysSorted, nNew, s,
                                         endBit));
         hipLaunchKernelGGL(k_append_contacts, dim3(blocks(nNew)), dim3(256), 0, s, w, ctx->newKeysSorted, nNew,
-                           ctx->nContacts, (unsigned long long)nf);
+                           ctx->nContacts, (unsigned long long)packBase);
         LAUNCHCHK(ctx);
         ctx->nContacts += nNew;
     }
@@ -7368,8 +7446,8 @@ int solveTOI(b2g_context* ctx, const b2g_step_params* p) {
             if ((rc = runBroadphase(ctx)) != B2G_OK) return rc;
             const int added = ctx->nContacts - before;
             if (added > 0) {
-                hipLaunchKernelGGL(k_hash_insert_keys, dim3(blocks(added)), dim3(256), 0, s, ctx->newKeysSorted, added,
-                                   ctx->hashTable, ctx->hashMask);
+                hipLaunchKernelGGL(k_hash_insert_range, dim3(blocks(added)), dim3(256), 0, s, ctx->view(), before,
+                                   added, ctx->hashTable, ctx->hashMask);
             }
             if ((rc = reserve()) != B2G_OK) return rc;
             hipLaunchKernelGGL(k_toi_resume, dim3(1), dim3(64), 0, s, st, ctx->nContacts);
@@ -7621,6 +7699,7 @@ int b2g_destroy(b2g_context* ctx) {
     freeToi(ctx);
     freeArray(ctx->keep); freeArray(ctx->idx); freeArray(ctx->hashTable); freeArray(ctx->newKeys);
     freeArray(ctx->newKeysSorted); freeArray(ctx->consKeys); freeArray(ctx->consKeysSorted);
+    freeArray(ctx->fixOrder); freeArray(ctx->orderFix);
     freeArray(ctx->consVals); freeArray(ctx->consOrder); freeArray(ctx->tileList);
     ConView& c = ctx->cv;
     freeArray(c.contact); freeArray(c.bodies); freeArray(c.mass); freeArray(c.normal); freeArray(c.r0);
@@ -7786,6 +7865,9 @@ int b2g_add_fixtures(b2g_context* ctx, const b2g_fixture_def* defs, int count, i
                 lx = l.x; ly = l.y; ux = u.x; uy = u.y;
             }
             fat[i] = make_float4(lx - kAabbExtension, ly - kAabbExtension, ux + kAabbExtension, uy + kAabbExtension);
+            // an inactive body's fixtures have no proxy (b2Body::CreateFixture,
+            // REF/Dynamics/b2Body.cpp:182-186): the inverted box of a removed one
+            if (!active) fat[i] = make_float4(FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX);
             moved[i] = active ? 1 : 0;
         }
         std::vector<b2g_shape> shapes(count);
@@ -7819,11 +7901,45 @@ int b2g_remove_fixtures(b2g_context* ctx, const int32_t* ids, int count) {
     hipError_t e = hipMemcpyAsync(d, ids, sizeof(int) * count, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_remove_fixtures, dim3(blocks(count)), dim3(256), 0, ctx->stream, ctx->view(), d, count);
+        if (ctx->nContacts > 0)
+            hipLaunchKernelGGL(k_wake_removed_contacts, dim3(blocks(ctx->nContacts)), dim3(256), 0, ctx->stream,
+                               ctx->view());
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(ctx, B2G_ERR_HIP, hipGetErrorString(e));
+    return B2G_OK;
+}
+
+int b2g_set_proxy_ids(b2g_context* ctx, const int32_t* ids, int count) {
+    if (!ctx || count < 0 || (count > 0 && !ids)) return B2G_ERR_ARG;
+    if (ctx->subWorlds) return fail(ctx, B2G_ERR_STATE, "proxy ids: one world per context only");
+    if (count != ctx->nFixtures)
+        return fail(ctx, B2G_ERR_ARG, "proxy ids: one per fixture (" + std::to_string(ctx->nFixtures) + ")");
+    int span = 0;
+    for (int i = 0; i < count; ++i) {
+        if (ids[i] < -1 || ids[i] >= (1 << 30)) return fail(ctx, B2G_ERR_ARG, "proxy id out of range");
+        span = std::max(span, ids[i] + 1);
+    }
+    std::vector<int> inverse(std::max(span, 1), -1);
+    for (int i = 0; i < count; ++i) {
+        if (ids[i] < 0) continue;
+        if (inverse[ids[i]] >= 0) return fail(ctx, B2G_ERR_ARG, "proxy id " + std::to_string(ids[i]) + " repeated");
+        inverse[ids[i]] = i;
+    }
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipStreamSynchronize(s));  // earlier steps may still read the old keys
+    freeArray(ctx->fixOrder);
+    freeArray(ctx->orderFix);
+    ctx->orderCount = ctx->orderSpan = 0;
+    HIPCHK(ctx, hipMalloc((void**)&ctx->fixOrder, sizeof(int) * std::max(count, 1)));
+    HIPCHK(ctx, hipMalloc((void**)&ctx->orderFix, sizeof(int) * inverse.size()));
+    if (count > 0) HIPCHK(ctx, hipMemcpyAsync(ctx->fixOrder, ids, sizeof(int) * count, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->orderFix, inverse.data(), sizeof(int) * inverse.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    ctx->orderCount = count;
+    ctx->orderSpan = (int)inverse.size();
     return B2G_OK;
 }
 
@@ -8309,7 +8425,7 @@ int b2g_pairs_batch(b2g_context* ctx, const b2g_aabb* fat, const uint8_t* moved,
     hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(n)), dim3(256), 0, s, n, g, dfat.p, dmoved.p);
     WorldView w;
     memset(&w, 0, sizeof(w));
-    PairSink sink{keys.p, capPairs, 0ull};
+    PairSink sink{keys.p, capPairs, 0ull, nullptr};
     hipLaunchKernelGGL(k_pairs_query, dim3(blocks(n * kQueryLanes)), dim3(256), 0, s, dfat.p, dmoved.p, n, g, w, false,
                        (const unsigned long long*)nullptr, 0u, sink, dc.p);
     hipLaunchKernelGGL(k_pairs_wide, dim3(blocks(n)), dim3(256), 0, s, dfat.p, dmoved.p, n, g, w, false,
@@ -8896,7 +9012,7 @@ int b2g_abi_sizes(int32_t* out, int cap) {
 // original would.
 namespace {
 constexpr uint32_t kSnapMagic = 0x53473242u;  // "B2GS"
-constexpr uint32_t kSnapVersion = 4;
+constexpr uint32_t kSnapVersion = 5;
 struct SnapHeader {
     uint32_t magic, version;
     int32_t nBodies, nFixtures, nContacts, nJoints;
@@ -8906,11 +9022,13 @@ struct SnapHeader {
     uint8_t allowSleep, warmStarting, continuous, subStepping, autoClearForces, newFixture, cellSizeSet, subWorlds;
     uint8_t staticAsleep, stepComplete, pad[2];
     int32_t jointColourStart[kColourSlots + 1];
+    int32_t orderCount, orderSpan;  // broad-phase order keys (0, 0: unset)
 };
 
 // Walks the snapshot layout: `visit(devicePointer, bytes)` for each array.
 template <typename F>
-void snapArrays(b2g_context* ctx, int nb, int nf, int nc, int nj, bool subWorlds, F visit) {
+void snapArrays(b2g_context* ctx, int nb, int nf, int nc, int nj, bool subWorlds, int orderCount, int orderSpan,
+                F visit) {
     visit(ctx->btype, sizeof(int) * nb);
     visit(ctx->bflags, sizeof(int) * nb);
     visit(ctx->xf, sizeof(float4) * nb);
@@ -8948,6 +9066,10 @@ void snapArrays(b2g_context* ctx, int nb, int nf, int nc, int nj, bool subWorlds
     visit(ctx->jAxis, sizeof(float4) * nj);
     visit(ctx->jExtra, sizeof(float4) * 2 * nj);
     visit(ctx->jOrder, sizeof(int) * nj);
+    if (orderSpan > 0) {
+        visit(ctx->fixOrder, sizeof(int) * orderCount);
+        visit(ctx->orderFix, sizeof(int) * orderSpan);
+    }
 }
 }  // namespace
 
@@ -8962,7 +9084,8 @@ int b2g_snapshot(b2g_context* ctx, void* out, size_t cap, size_t* size) {
     }
     const int nb = ctx->nBodies, nf = ctx->nFixtures, nc = ctx->nContacts, nj = ctx->nJoints;
     size_t total = sizeof(SnapHeader) + 3 * sizeof(int32_t) * (size_t)nj;  // host joint vectors
-    snapArrays(ctx, nb, nf, nc, nj, ctx->subWorlds, [&](const void*, size_t bytes) { total += bytes; });
+    snapArrays(ctx, nb, nf, nc, nj, ctx->subWorlds, ctx->orderCount, ctx->orderSpan,
+               [&](const void*, size_t bytes) { total += bytes; });
     *size = total;
     if (!out) return B2G_OK;
     if (cap < total) return fail(ctx, B2G_ERR_CAPACITY, "snapshot buffer too small");
@@ -8976,6 +9099,8 @@ int b2g_snapshot(b2g_context* ctx, void* out, size_t cap, size_t* size) {
     h.subStepping = ctx->subStepping; h.autoClearForces = ctx->autoClearForces; h.newFixture = ctx->newFixture;
     h.cellSizeSet = ctx->cellSizeSet; h.subWorlds = ctx->subWorlds; h.staticAsleep = ctx->staticAsleep;
     h.stepComplete = ctx->stepComplete;
+    h.orderCount = ctx->orderCount;
+    h.orderSpan = ctx->orderSpan;
     for (int c = 0; c <= kColourSlots; ++c)
         h.jointColourStart[c] = nj > 0 && (int)ctx->jointColourStart.size() > c ? ctx->jointColourStart[c] : 0;
     uint8_t* o = (uint8_t*)out;
@@ -8988,7 +9113,7 @@ int b2g_snapshot(b2g_context* ctx, void* out, size_t cap, size_t* size) {
     }
     hipStream_t s = ctx->stream;
     hipError_t e = hipSuccess;
-    snapArrays(ctx, nb, nf, nc, nj, ctx->subWorlds, [&](const void* p, size_t bytes) {
+    snapArrays(ctx, nb, nf, nc, nj, ctx->subWorlds, ctx->orderCount, ctx->orderSpan, [&](const void* p, size_t bytes) {
         if (e == hipSuccess && bytes > 0) e = hipMemcpyAsync(o, p, bytes, hipMemcpyDeviceToHost, s);
         o += bytes;
     });
@@ -9005,11 +9130,13 @@ int b2g_restore(b2g_context* ctx, const void* in, size_t size) {
     if (size < sizeof(h)) return fail(ctx, B2G_ERR_ARG, "snapshot too short");
     memcpy(&h, in, sizeof(h));
     if (h.magic != kSnapMagic || h.version != kSnapVersion) return fail(ctx, B2G_ERR_ARG, "not a snapshot");
-    if (h.nBodies < 0 || h.nFixtures < 0 || h.nContacts < 0 || h.nJoints < 0)
+    if (h.nBodies < 0 || h.nFixtures < 0 || h.nContacts < 0 || h.nJoints < 0 || h.orderSpan < 0 ||
+        (h.orderSpan > 0 && h.orderCount != h.nFixtures))
         return fail(ctx, B2G_ERR_ARG, "snapshot counts");
     const int nb = h.nBodies, nf = h.nFixtures, nc = h.nContacts, nj = h.nJoints;
     size_t total = sizeof(SnapHeader) + 3 * sizeof(int32_t) * (size_t)nj;
-    snapArrays(ctx, nb, nf, nc, nj, h.subWorlds != 0, [&](const void*, size_t bytes) { total += bytes; });
+    snapArrays(ctx, nb, nf, nc, nj, h.subWorlds != 0, h.orderCount, h.orderSpan,
+               [&](const void*, size_t bytes) { total += bytes; });
     if (size != total) return fail(ctx, B2G_ERR_ARG, "snapshot size does not match its counts");
     int rc;
     if ((rc = growBodies(ctx, std::max(nb, 1))) != B2G_OK) return rc;
@@ -9017,6 +9144,12 @@ int b2g_restore(b2g_context* ctx, const void* in, size_t size) {
     if ((rc = growContacts(ctx, std::max(2 * nc, 1024))) != B2G_OK) return rc;
     if (nj > 0 && (rc = growJoints(ctx, nj)) != B2G_OK) return rc;
     if (h.subWorlds && !ctx->bworld) HIPCHK(ctx, hipMalloc((void**)&ctx->bworld, sizeof(int) * ctx->capBodies));
+    if (h.orderSpan > 0) {
+        freeArray(ctx->fixOrder);
+        freeArray(ctx->orderFix);
+        HIPCHK(ctx, hipMalloc((void**)&ctx->fixOrder, sizeof(int) * std::max(h.orderCount, 1)));
+        HIPCHK(ctx, hipMalloc((void**)&ctx->orderFix, sizeof(int) * h.orderSpan));
+    }
     const uint8_t* q = (const uint8_t*)in + sizeof(h);
     ctx->hJointBodies.assign(nj, make_int2(0, 0));
     ctx->hJointKeepApart.assign(nj, 0);
@@ -9030,7 +9163,7 @@ int b2g_restore(b2g_context* ctx, const void* in, size_t size) {
     }
     hipStream_t s = ctx->stream;
     hipError_t e = hipSuccess;
-    snapArrays(ctx, nb, nf, nc, nj, h.subWorlds != 0, [&](const void* p, size_t bytes) {
+    snapArrays(ctx, nb, nf, nc, nj, h.subWorlds != 0, h.orderCount, h.orderSpan, [&](const void* p, size_t bytes) {
         if (e == hipSuccess && bytes > 0) e = hipMemcpyAsync((void*)p, q, bytes, hipMemcpyHostToDevice, s);
         q += bytes;
     });
@@ -9065,6 +9198,8 @@ int b2g_restore(b2g_context* ctx, const void* in, size_t size) {
     ctx->subWorlds = h.subWorlds != 0;
     ctx->staticAsleep = h.staticAsleep != 0;
     ctx->stepComplete = h.stepComplete != 0;
+    ctx->orderCount = h.orderSpan > 0 ? h.orderCount : 0;
+    ctx->orderSpan = h.orderSpan;
     // colour claims only compare with the current tag: zero is older than any
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(ctx->capBodies)), dim3(256), 0, s, ctx->claim,
                        ctx->capBodies, 0ull);
@@ -9777,7 +9912,7 @@ int b2g_pairs_batch_noalloc(b2g_context* ctx, const b2g_aabb* fat, const uint8_t
     hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(n)), dim3(256), 0, s, n, g, dfat, dmoved);
     WorldView w;
     memset(&w, 0, sizeof(w));
-    PairSink sink{keys, kNoAllocMaxPairs, 0ull};
+    PairSink sink{keys, kNoAllocMaxPairs, 0ull, nullptr};
     hipLaunchKernelGGL(k_pairs_query, dim3(blocks(n * kQueryLanes)), dim3(256), 0, s, dfat, dmoved, n, g, w, false,
                        (const unsigned long long*)nullptr, 0u, sink, dc);
     hipLaunchKernelGGL(k_pairs_wide, dim3(blocks(n)), dim3(256), 0, s, dfat, dmoved, n, g, w, false,

@@ -189,6 +189,11 @@ struct WorldView {
     int* fgroup;
     float4* fat;       // fat AABB lower.x lower.y upper.x upper.y
     uint8_t* moved;    // proxy is in the move buffer
+    // Broad-phase order of the proxies: new pairs are (smaller, larger) key
+    // and sorted by key, as the reference sorts its pair buffer by tree proxy
+    // id; null: the fixture index is the key.
+    const int* ford;   // key per fixture
+    const int* ordf;   // fixture per key (-1: none)
     // contacts
     int nContacts;
     int* cfA;

@@ -62,6 +62,7 @@ struct Recorder : public b2ContactListener {
 struct Runner {
     b2World world;
     std::vector<b2Body*> bodies;
+    std::vector<uint8_t> destroyed;  // per body: DestroyBody called (the slot stays)
     std::vector<b2Fixture*> fixtures;
     std::vector<b2Joint*> joints;
     Recorder recorder;
@@ -503,6 +504,30 @@ int b2pp_world_set_joint(void* h, int joint, int enableLimit, float lower, float
     j->SetMotorSpeed(speed);
     j->SetMaxMotorTorque(maxTorque);
     return 0;
+}
+
+// b2World::DestroyBody; the runner forgets the body's joints (the body's
+// slot stays, so indices keep their creation order).
+int b2pp_world_destroy_body(void* h, int body) {
+    Runner* r = (Runner*)h;
+    if (body < 0 || body >= (int)r->bodies.size()) return B2G_ERR_ARG;
+    r->destroyed.resize(r->bodies.size(), 0);
+    if (r->destroyed[body]) return B2G_ERR_ARG;
+    b2Body* b = r->bodies[body];
+    for (auto& j : r->joints)
+        if (j && (j->GetBodyA() == b || j->GetBodyB() == b)) j = nullptr;
+    r->world.DestroyBody(b);
+    r->destroyed[body] = 1;
+    return 0;
+}
+
+// Per fixture proxy in creation order: its broad-phase tree proxy id, -1
+// once destroyed.
+int b2pp_world_proxy_ids(void* h, int32_t* out, int cap) {
+    const std::vector<int32_t>& ids = ((Runner*)h)->world.GetProxyIds();
+    const int n = (int)ids.size();
+    for (int i = 0; i < n && i < cap; ++i) out[i] = ids[i];
+    return n;
 }
 
 int b2pp_world_destroy_joint(void* h, int joint) {

@@ -3,7 +3,9 @@
 // C ABI of b2g.h:
 //   * bodies and fixtures created since the last Step are appended to the
 //     device world in creation order (body index = creation order, proxy
-//     index = fixture creation order, as the reference's proxy ids grow);
+//     index = fixture creation order); the proxies' broad-phase order is the
+//     reference tree's proxy ids, modelled here, which grow with creation
+//     order until a body is destroyed and are uploaded from then on;
 //   * host edits (SetTransform, velocities, forces, SetAwake, mass) mark a
 //     body dirty and are uploaded before the next Step;
 //   * after Step the mirrors are refreshed from the device on first read,
@@ -158,7 +160,12 @@ b2Fixture* b2Body::CreateFixture(const b2FixtureDef* def) {
     f->m_isSensor = def->isSensor;
     f->m_filter = def->filter;
     f->m_index = (int32_t)m_world->m_fixtures.size();
-    for (int32_t c = 0; c < f->m_shape->GetChildCount(); ++c) m_world->m_fixtures.push_back(f);
+    for (int32_t c = 0; c < f->m_shape->GetChildCount(); ++c) {
+        m_world->m_fixtures.push_back(f);
+        // b2Fixture::CreateProxies, active bodies only (REF/Dynamics/b2Body.cpp:182-186)
+        m_world->m_proxyIds.push_back(IsActive() ? m_world->CreateProxyId() : -1);
+    }
+    if (m_world->m_proxyIdsReused) m_world->m_proxyIdsDirty = true;
     f->m_next = m_fixtureList;
     m_fixtureList = f;
     ++m_fixtureCount;
@@ -403,7 +410,8 @@ b2Body* b2World::CreateBody(const b2BodyDef* def) {
 // Destroyed bodies become inactive static bodies on the device; their
 // proxies leave the broad-phase, so their contacts end at the next Step.
 void b2World::DestroyBody(b2Body* b) {
-    if (IsLocked() || b == nullptr || b->m_world != this) return;
+    if (IsLocked() || b == nullptr || b->m_world != this || b->m_destroyed) return;
+    b->m_destroyed = true;
     // Attached joints go first (REF/Dynamics/b2World.cpp:134-147).
     while (b->m_jointList) DestroyJoint(b->m_jointList->joint);
     b->Touch();
@@ -417,10 +425,55 @@ void b2World::DestroyBody(b2Body* b) {
     b->m_invMass = 0.0f;
     b->m_invI = 0.0f;
     b->MarkDirty();
+    // fixtures newest first, each one's proxies in child order
+    // (REF/Dynamics/b2World.cpp:170-188, b2Fixture.cpp:141-152)
     for (b2Fixture* f = b->m_fixtureList; f; f = f->m_next)
-        for (int32_t c = 0; c < f->m_shape->GetChildCount(); ++c) m_removedFixtures.push_back(f->m_index + c);
+        for (int32_t c = 0; c < f->m_shape->GetChildCount(); ++c) {
+            m_removedFixtures.push_back(f->m_index + c);
+            int32_t& id = m_proxyIds[f->m_index + c];
+            if (id >= 0) {
+                DestroyProxyId(id);
+                id = -1;
+                m_proxyIdsReused = m_proxyIdsDirty = true;
+            }
+        }
     --m_bodyCount;
     m_contactsStale = true;
+}
+
+// The reference's broad-phase tree takes node ids from a free list that
+// starts as 0..15 and doubles when it runs out (the new ids in ascending
+// order), and returns freed nodes to its head (b2DynamicTree::AllocateNode /
+// FreeNode, REF/Collision/b2DynamicTree.cpp:25-53, 56-106). A proxy takes a
+// leaf node, then a parent node unless the tree was empty (CreateProxy,
+// InsertLeaf :108-121, 179-265); destroying one frees its parent unless it
+// was the root, then its leaf (DestroyProxy, RemoveLeaf :124-131, 319-376).
+// A move frees the leaf's parent and takes the same node straight back, and
+// nothing else in a step allocates, so the free list changes only here.
+// Which internal node a leaf hangs from depends on the tree's shape; such a
+// node is kept as -1 and is only ever taken again as a parent, since a freed
+// leaf always lies above its parent in the list.
+int32_t b2World::CreateProxyId() {
+    auto take = [this]() {
+        if (m_treeFree.empty()) {
+            const int32_t old = m_treeCapacity;
+            m_treeCapacity = old == 0 ? 16 : 2 * old;
+            for (int32_t id = m_treeCapacity - 1; id >= old; --id) m_treeFree.push_back(id);
+        }
+        const int32_t id = m_treeFree.back();
+        m_treeFree.pop_back();
+        return id;
+    };
+    const int32_t leaf = take();
+    if (m_treeLeaves > 0) (void)take();
+    ++m_treeLeaves;
+    return leaf;
+}
+
+void b2World::DestroyProxyId(int32_t id) {
+    if (m_treeLeaves > 1) m_treeFree.push_back(-1);
+    m_treeFree.push_back(id);
+    --m_treeLeaves;
 }
 
 // ------------------------------------------------------------ joints
@@ -1453,6 +1506,11 @@ int32_t b2World::Upload() {
         rc = b2g_remove_fixtures(m_ctx, ids.data(), (int)ids.size());
         if (rc != B2G_OK) return rc;
         m_removedFixtures.clear();
+    }
+    if (m_proxyIdsDirty && m_subWorldCount == 0) {
+        rc = b2g_set_proxy_ids(m_ctx, m_proxyIds.data(), (int)m_proxyIds.size());
+        if (rc != B2G_OK) return rc;
+        m_proxyIdsDirty = false;
     }
     const int nj = (int)m_joints.size();
     if (nj > m_uploadedJoints) {

@@ -231,6 +231,21 @@ inline Scene pile(int n, uint32_t seed, float width = 300.0f, float height = 600
     return s;
 }
 
+// Bodies to add to a running world (no static body): the pile's shape mix
+// on a grid `width` metres wide, centred on x = 0, from height `base` up, at
+// the batched worlds' spacing.
+inline Scene rain(int n, uint32_t seed, float width = 12.0f, float base = 25.0f) {
+    Scene p = pile(n, seed, width, 1.3416f * (float)((n + 8) / 9), base);
+    Scene s;
+    for (size_t i = 1; i < p.bodies.size(); ++i) {
+        BodyDef b = p.bodies[i];
+        b.firstFixture = (int)s.fixtures.size();
+        for (int k = 0; k < p.bodies[i].fixtureCount; ++k) s.fixtures.push_back(p.fixtures[p.bodies[i].firstFixture + k]);
+        s.bodies.push_back(b);
+    }
+    return s;
+}
+
 // Body count of world w in the batched scene: floor(16 * 64^U), U from seed.
 inline std::vector<int> worldSizes(int nWorlds, uint32_t seed, int lo = 16, int hi = 1024) {
     Uniform U(seed);
@@ -943,6 +958,7 @@ inline Scene byName(const char* kind, int n, uint32_t seed) {
     if (k == "pyramid") return pyramid(n > 0 ? n : 20);
     if (k == "pile") return pile(n, seed);
     if (k == "world") return batchedWorld(n, seed);
+    if (k == "rain") return rain(n > 0 ? n : 20, seed);
     if (k == "pendulums") return pendulums(n > 0 ? n : 16);
     if (k == "chains") return chains(n > 0 ? n : 4, seed > 0 ? (int)seed : 32);
     if (k == "stress") return stress(n, seed, n / 50);

@@ -242,9 +242,22 @@ int b2g_get_bodies(b2g_context* ctx, int first, int count, b2g_body_state* out);
 int b2g_add_fixtures(b2g_context* ctx, const b2g_fixture_def* fixtures, int count, int* first);
 
 /* Removes fixtures' proxies from the broad-phase (b2Body::DestroyFixture /
- * b2World::DestroyBody, Box2D/Dynamics/b2Body.h:156, b2World.h:80): their
- * contacts end at the next step when their fat AABBs no longer overlap. */
+ * b2World::DestroyBody, Box2D/Dynamics/b2Body.h:156, b2World.h:80): the
+ * bodies of their contacts with manifold points wake now, as the reference's
+ * immediate contact destruction wakes them; the contacts themselves end
+ * first thing in the next step's collide (EndContact then). */
 int b2g_remove_fixtures(b2g_context* ctx, const int32_t* ids, int count);
+/* The broad-phase order of the proxies: ids[f] is the reference tree's
+ * proxy id of fixture f (-1 for a removed one) for every fixture (count =
+ * fixtures added so far; call again after adding more). New pairs are then
+ * (smaller id, larger id), created in id order with the smaller id's fixture
+ * as A, as b2BroadPhase::UpdatePairs sorts its pair buffer
+ * (Box2D/Collision/b2BroadPhase.h:193-247, b2BroadPhase.cpp:98-121). Unset,
+ * the fixture index is the id: the reference's ids grow with creation order
+ * until a proxy is destroyed (b2DynamicTree::AllocateNode / FreeNode,
+ * Box2D/Collision/b2DynamicTree.cpp:59-110). One world per context. */
+int b2g_set_proxy_ids(b2g_context* ctx, const int32_t* ids, int count);
+
 /* Updates material and filter of fixtures (b2Fixture::SetFriction /
  * SetRestitution / SetFilterData, Box2D/Dynamics/b2Fixture.h:131-170); a
  * filter change flags the fixture's contacts for filtering and re-queries

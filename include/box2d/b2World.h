@@ -14,9 +14,11 @@
 //    EndContact and PreSolve are called inside the step in list order,
 //    PostSolve after the solve for the island contacts, in the reference's
 //    island order.
-//  * DestroyBody and DestroyFixture keep device slots; fixtures created
-//    afterwards take new proxy ids where the reference reuses freed ones,
-//    which can change the order of new contacts (not the pair set).
+//  * DestroyBody keeps device slots. Its contacts end at the start of the
+//    next Step (EndContact then), the bodies they touched wake at once as in
+//    the reference. Broad-phase proxy ids follow the reference's tree: later
+//    fixtures take the freed ids, so new contacts come in the reference's
+//    order.
 #pragma once
 #include <vector>
 #include <string>
@@ -229,6 +231,7 @@ private:
     b2BodyType m_type;
     int32_t m_flags = 0;
     int32_t m_index = -1;
+    bool m_destroyed = false;  // by DestroyBody (the device slot stays)
     b2Transform m_xf;
     b2Sweep m_sweep;
     b2Vec2 m_linearVelocity;
@@ -847,6 +850,10 @@ public:
     // Mean and max penetration over touching points and the point count,
     // from device world manifolds.
     void GetPenetration(double* meanMax3) const;
+    // Per fixture proxy (chain edges one each) in creation order: the id the
+    // reference's broad-phase tree gives it, -1 once destroyed (or on an
+    // inactive body).
+    const std::vector<int32_t>& GetProxyIds() const { return m_proxyIds; }
     // Batched worlds: bodies created after BeginSubWorld() belong to a new
     // sub-world (ids 1, 2, ...; bodies created before are in sub-world 0).
     // Fixtures of different sub-worlds never collide, so one b2World holds
@@ -892,6 +899,10 @@ private:
     // PostSolve after the Step; Begin/End too unless the split step
     // already delivered them
     void ReplayEvents(bool collideEventsDone);
+    // Node ids of the reference's broad-phase tree pool for a proxy created
+    // or destroyed now (b2DynamicTree::CreateProxy / DestroyProxy).
+    int32_t CreateProxyId();
+    void DestroyProxyId(int32_t id);
 
     b2g_context* m_ctx = nullptr;
     b2Vec2 m_gravity;
@@ -919,6 +930,11 @@ private:
     std::vector<uint64_t> m_touchingKeys;  // touching fixture pairs after the last Step
     std::vector<int32_t> m_postSolveOrder; // PostSolve order of the open step (table indices after it)
     std::vector<int32_t> m_removedFixtures;  // proxies to drop at the next upload
+    std::vector<int32_t> m_proxyIds;       // tree proxy id per fixture proxy, creation order
+    std::vector<int32_t> m_treeFree;       // the tree's free node ids, head at the back (-1: a freed parent)
+    int32_t m_treeCapacity = 0, m_treeLeaves = 0;
+    bool m_proxyIdsReused = false;         // a proxy was destroyed: ids no longer follow creation order
+    bool m_proxyIdsDirty = false;          // ids to upload with the next Upload
     std::vector<int32_t> m_updatedFixtures;  // material/filter changes to upload
     int32_t m_subWorldCount = 0;
     int32_t m_currentSubWorld = 0;

@@ -1191,6 +1191,8 @@ struct Fixture {
     AABB aabb;   // swept tight AABB of the last synchronize
     AABB fat;    // fat AABB in the broad-phase tree
     bool proxy;  // has a broad-phase proxy
+    int proxyId; // its node id in the broad-phase tree (-1 without a proxy)
+    int owner;   // first child fixture of the same b2Fixture (chain edges share one)
 };
 
 struct Edge { int prev, next; };  // contact-edge links (edge ids)
@@ -1375,6 +1377,21 @@ struct World {
     int contactHead = -1;
     int contactCount = 0;
     std::vector<int> moveBuffer;
+    // Node ids of the broad-phase tree's pool, b2DynamicTree::AllocateNode /
+    // FreeNode / CreateProxy / DestroyProxy / InsertLeaf / RemoveLeaf
+    // (REF/Collision/b2DynamicTree.cpp:25-53, 56-131, 179-265, 319-376): a
+    // free list that starts as 0..15 and doubles when it runs out (the new
+    // ids in ascending order), taking freed nodes at its head. A proxy takes
+    // a leaf node, then a parent node unless the tree was empty; destroying
+    // one frees its parent (unless it was the root) and then its leaf.
+    // Moving a proxy frees its parent and takes the same node straight back,
+    // and nothing else in a step allocates, so only creation and destruction
+    // change the free list. Freed parents are kept as -1: which internal node
+    // a leaf hangs from depends on the tree's shape, but such a node is only
+    // ever taken again as a parent (a freed leaf always lies above it).
+    std::vector<int> treeFree;  // free node ids, the head at the back
+    int treeCapacity = 0, treeLeaves = 0;
+    std::vector<int> proxyFixture;  // fixture of each live proxy id
     bool newFixture = false;
     bool allowSleep = true, warmStarting = true, continuous = false, subStepping = false;
     bool autoClearForces = true;
@@ -1444,6 +1461,31 @@ struct World {
         b.v += cross(b.w, b.c - oldCenter);
     }
 
+    int treeAllocate() {
+        if (treeFree.empty()) {
+            const int old = treeCapacity;
+            treeCapacity = old == 0 ? 16 : 2 * old;
+            for (int id = treeCapacity - 1; id >= old; --id) treeFree.push_back(id);
+        }
+        const int id = treeFree.back();
+        treeFree.pop_back();
+        return id;
+    }
+    int createProxyId(int fi) {
+        const int leaf = treeAllocate();
+        if (treeLeaves > 0) (void)treeAllocate();
+        ++treeLeaves;
+        if ((int)proxyFixture.size() <= leaf) proxyFixture.resize(leaf + 1, -1);
+        proxyFixture[leaf] = fi;
+        return leaf;
+    }
+    void destroyProxyId(int leaf) {
+        if (treeLeaves > 1) treeFree.push_back(-1);
+        treeFree.push_back(leaf);
+        --treeLeaves;
+        proxyFixture[leaf] = -1;
+    }
+
     // b2World::CreateBody + b2Body::b2Body, REF/Dynamics/b2World.cpp:109-127,
     // REF/Dynamics/b2Body.cpp:25-90
     int createBody(const b2scene::BodyDef& d) {
@@ -1482,7 +1524,8 @@ struct World {
 
     // b2Body::CreateFixture + b2Fixture::CreateProxies + b2BroadPhase::CreateProxy,
     // REF/Dynamics/b2Body.cpp:164-203, b2Fixture.cpp:124-139, b2DynamicTree.cpp:112-126
-    int createFixture(int bi, const b2g_shape& shape, const b2scene::FixtureDef& d) {
+    // owner: the first child fixture of a chain's b2Fixture (-1: this one).
+    int createFixture(int bi, const b2g_shape& shape, const b2scene::FixtureDef& d, int owner = -1) {
         Fixture f;
         f.body = bi;
         f.shape = shape;
@@ -1501,8 +1544,10 @@ struct World {
             f.fat.lo = f.aabb.lo - r;
             f.fat.hi = f.aabb.hi + r;
         }
+        const int fi = (int)fixtures.size();
+        f.proxyId = f.proxy ? createProxyId(fi) : -1;
+        f.owner = owner < 0 ? fi : owner;
         fixtures.push_back(f);
-        const int fi = (int)fixtures.size() - 1;
         if (f.proxy) moveBuffer.push_back(fi);
         b.fixtures.push_back(fi);
         if (f.density > 0.0f) resetMassData(b);
@@ -1816,6 +1861,38 @@ struct World {
         }
     }
 
+    // b2World::DestroyBody, REF/Dynamics/b2World.cpp:133-215: the body's
+    // joints (each DestroyJoint), its contacts (b2ContactManager::Destroy:
+    // EndContact if touching, the bodies woken when the manifold had points),
+    // then its fixtures newest first, each one's proxies in child order
+    // (b2Fixture::DestroyProxies, REF/Dynamics/b2Fixture.cpp:141-152, and
+    // b2BroadPhase::DestroyProxy, REF/Collision/b2BroadPhase.cpp:50-55, which
+    // takes the proxy out of the move buffer). The slot stays, marked dead,
+    // so body and fixture indices keep their creation order.
+    void destroyBody(int bi) {
+        Body& b = bodies[bi];
+        if (!b.alive) return;
+        while (b.jointHead != -1) destroyJoint(b.jointHead >> 1);
+        while (b.edgeHead != -1) destroyContact(b.edgeHead >> 1);
+        int k = (int)b.fixtures.size();
+        while (k > 0) {
+            const int owner = fixtures[b.fixtures[k - 1]].owner;
+            int first = k - 1;
+            while (first > 0 && fixtures[b.fixtures[first - 1]].owner == owner) --first;
+            for (int i = first; i < k; ++i) {
+                Fixture& f = fixtures[b.fixtures[i]];
+                if (!f.proxy) continue;
+                for (int& m : moveBuffer)
+                    if (m == b.fixtures[i]) m = -1;
+                destroyProxyId(f.proxyId);
+                f.proxy = false;
+                f.proxyId = -1;
+            }
+            k = first;
+        }
+        b.alive = false;
+    }
+
     // b2RevoluteJoint::EnableLimit, SetLimits, EnableMotor, SetMotorSpeed,
     // SetMaxMotorTorque in that order, REF/Dynamics/Joints/b2RevoluteJoint.cpp:419-471.
     void setJoint(int ji, bool enableLimit, float lower, float upper, bool enableMotor, float speed, float torque) {
@@ -2033,9 +2110,8 @@ struct World {
 
     // b2BroadPhase::UpdatePairs + QueryCallback, REF/Collision/b2BroadPhase.h:188-241,
     // b2BroadPhase.cpp:98-119. The pair set is every proxy whose fat AABB overlaps a
-    // moved proxy's fat AABB; pairs are keyed by proxy id. Proxy ids of the dynamic
-    // tree grow with creation order when no proxy is destroyed (0, 1, 3, 5, ...), so
-    // sorting by fixture creation index gives the same order.
+    // moved proxy's fat AABB; a pair is (smaller, larger) tree proxy id, the pairs are
+    // sorted by those ids, and the fixture of the smaller id becomes AddPair's A.
     void findNewContacts() {
         std::vector<std::pair<int, int>> pairs;
         const int nf = (int)fixtures.size();
@@ -2070,7 +2146,9 @@ struct World {
             const AABB& fq = fixtures[q].fat;
             auto consider = [&](int p) {
                 if (p == q) return;
-                if (overlap(fq, fixtures[p].fat)) pairs.push_back(std::make_pair(std::min(p, q), std::max(p, q)));
+                if (!overlap(fq, fixtures[p].fat)) return;
+                const int ip = fixtures[p].proxyId, iq = fixtures[q].proxyId;
+                pairs.push_back(std::make_pair(std::min(ip, iq), std::max(ip, iq)));
             };
             const float qw = fq.hi.x - fq.lo.x;
             if (qw > wide) {
@@ -2085,7 +2163,7 @@ struct World {
         moveBuffer.clear();
         std::sort(pairs.begin(), pairs.end());
         pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
-        for (auto& pr : pairs) addPair(pr.first, pr.second);
+        for (auto& pr : pairs) addPair(proxyFixture[pr.first], proxyFixture[pr.second]);
     }
 
     // b2DynamicTree::MoveProxy fat-AABB rule, REF/Collision/b2DynamicTree.cpp:133-181
@@ -3999,11 +4077,13 @@ struct World {
                         hasPrev = hasNext = true;
                     }
                     const int m = (int)vs.size();
+                    int owner = -1;
                     for (int c = 0; c + 1 < m; ++c) {
                         edgeShape(&s, c > 0 ? vs[c - 1] : prev, vs[c], vs[c + 1], c < m - 2 ? vs[c + 2] : next,
                                   c > 0 || hasPrev, c < m - 2 || hasNext);
                         s.flags |= 4;
-                        createFixture(bi, s, f);
+                        const int fi = createFixture(bi, s, f, owner);
+                        if (owner < 0) owner = fi;
                     }
                     continue;
                 } else {
@@ -4071,6 +4151,22 @@ int orc_world_set_joint(void* h, int joint, int enableLimit, float lower, float 
     if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint].removed) return -1;
     w->setJoint(joint, enableLimit != 0, lower, upper, enableMotor != 0, speed, maxTorque);
     return 0;
+}
+
+int orc_world_destroy_body(void* h, int body) {
+    World* w = (World*)h;
+    if (body < 0 || body >= (int)w->bodies.size() || !w->bodies[body].alive) return -1;
+    w->destroyBody(body);
+    return 0;
+}
+
+// Per fixture (chain edges one each) in creation order: its broad-phase
+// tree proxy id, -1 once destroyed.
+int orc_world_proxy_ids(void* h, int32_t* out, int cap) {
+    World* w = (World*)h;
+    const int n = (int)w->fixtures.size();
+    for (int i = 0; i < n && i < cap; ++i) out[i] = w->fixtures[i].proxyId;
+    return n;
 }
 
 int orc_world_destroy_joint(void* h, int joint) {

@@ -609,6 +609,33 @@ int ref_world_set_weld(void* h, int joint, float hz, float ratio) {
     return 0;
 }
 
+// b2World::DestroyBody; the driver forgets the body, its fixtures and its
+// joints (their slots stay, empty, so indices keep their creation order).
+int ref_world_destroy_body(void* h, int body) {
+    RefWorld* w = (RefWorld*)h;
+    if (body < 0 || body >= (int)w->bodies.size() || w->bodies[body] == nullptr) return -1;
+    b2Body* b = w->bodies[body];
+    for (auto& f : w->fixtures)
+        if (f && f->GetBody() == b) f = nullptr;
+    for (auto& j : w->joints)
+        if (j && (j->GetBodyA() == b || j->GetBodyB() == b)) j = nullptr;
+    w->world->DestroyBody(b);
+    w->bodies[body] = nullptr;
+    return 0;
+}
+
+// Per fixture proxy in creation order: its id in the broad-phase tree, -1
+// once destroyed.
+int ref_world_proxy_ids(void* h, int32_t* out, int cap) {
+    RefWorld* w = (RefWorld*)h;
+    const int n = (int)w->fixtures.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        const b2Fixture* f = w->fixtures[i];
+        out[i] = f ? f->m_proxies[i - fixtureIndex(f)].proxyId : -1;
+    }
+    return n;
+}
+
 int ref_world_destroy_joint(void* h, int joint) {
     RefWorld* w = (RefWorld*)h;
     if (joint < 0 || joint >= (int)w->joints.size() || w->joints[joint] == nullptr) return -1;
@@ -661,8 +688,12 @@ void ref_world_get_bodies(void* h, float* out) {
     RefWorld* w = (RefWorld*)h;
     for (size_t i = 0; i < w->bodies.size(); ++i) {
         const b2Body* b = w->bodies[i];
-        const b2Transform& xf = b->GetTransform();
         float* o = out + 12 * i;
+        if (b == nullptr) {  // destroyed
+            for (int k = 0; k < 12; ++k) o[k] = 0.0f;
+            continue;
+        }
+        const b2Transform& xf = b->GetTransform();
         o[0] = xf.p.x; o[1] = xf.p.y; o[2] = xf.q.s; o[3] = xf.q.c;
         o[4] = b->GetWorldCenter().x; o[5] = b->GetWorldCenter().y; o[6] = b->GetAngle();
         o[7] = b->GetLinearVelocity().x; o[8] = b->GetLinearVelocity().y;
@@ -697,6 +728,10 @@ void ref_world_get_fat_aabbs(void* h, b2g_aabb* out) {
     RefWorld* w = (RefWorld*)h;
     const b2BroadPhase& bp = w->world->GetContactManager().m_broadPhase;
     for (size_t i = 0; i < w->fixtures.size(); ++i) {
+        if (w->fixtures[i] == nullptr) {  // destroyed
+            out[i] = {0.0f, 0.0f, 0.0f, 0.0f};
+            continue;
+        }
         const int child = (int)i - fixtureIndex(w->fixtures[i]);
         const b2AABB& a = bp.GetFatAABB(w->fixtures[i]->m_proxies[child].proxyId);
         out[i] = {a.lowerBound.x, a.lowerBound.y, a.upperBound.x, a.upperBound.y};
@@ -706,6 +741,10 @@ void ref_world_get_fat_aabbs(void* h, b2g_aabb* out) {
 void ref_world_get_shapes(void* h, b2g_shape* out) {
     RefWorld* w = (RefWorld*)h;
     for (size_t i = 0; i < w->fixtures.size(); ++i) {
+        if (w->fixtures[i] == nullptr) {  // destroyed
+            memset(out + i, 0, sizeof(b2g_shape));
+            continue;
+        }
         const b2Shape* sh = w->fixtures[i]->GetShape();
         if (sh->GetType() == b2Shape::e_chain) {
             b2EdgeShape e;
@@ -725,6 +764,11 @@ void ref_world_fixture_info(void* h, int32_t* body, float* material) {
     for (size_t i = 0; i < w->bodies.size(); ++i) index[w->bodies[i]] = (int)i;
     for (size_t i = 0; i < w->fixtures.size(); ++i) {
         const b2Fixture* f = w->fixtures[i];
+        if (f == nullptr) {  // destroyed
+            body[i] = -1;
+            material[3 * i + 0] = material[3 * i + 1] = material[3 * i + 2] = 0.0f;
+            continue;
+        }
         body[i] = index[f->GetBody()];
         material[3 * i + 0] = f->GetShape()->m_radius;
         material[3 * i + 1] = f->GetFriction();
@@ -737,6 +781,10 @@ void ref_world_fixture_info(void* h, int32_t* body, float* material) {
 void ref_world_body_mass(void* h, float* out) {
     RefWorld* w = (RefWorld*)h;
     for (size_t i = 0; i < w->bodies.size(); ++i) {
+        if (w->bodies[i] == nullptr) {  // destroyed
+            out[2 * i + 0] = out[2 * i + 1] = 0.0f;
+            continue;
+        }
         out[2 * i + 0] = w->bodies[i]->m_mass;
         out[2 * i + 1] = w->bodies[i]->m_I;
     }

@@ -59,6 +59,81 @@ def test_reference_mode_bitexact(gpu, orc, wide_trig, kind, n, seed, steps, slee
     g.close(); o.close()
 
 
+@pytest.mark.parametrize("ccd", [False, True])
+def test_destroy_body_reference_order_bitexact(gpu, orc, wide_trig, ccd):
+    """Bodies destroyed mid-run and others added after them (the plan of
+    test_oracle.destroy_plan_run, which pins the oracle against the reference
+    build): the host models the reference tree's proxy ids and the device
+    orders new pairs by them, so contacts are created in the reference's
+    order with its A/B roles. Reference-order mode stays equal to the oracle
+    bit for bit, the contact list in order, with and without continuous
+    collision (whose events find new pairs too)."""
+    from test_oracle import destroy_plan_run, proxy_ids
+    g = gpu.world("world", 150, 5, sleep=True, ccd=ccd)
+    g.set_mode(REFERENCE)
+    o = orc.world("world", 150, 5, sleep=True, ccd=ccd)
+    print(f"case: destroy plan, reference order, ccd={ccd}", flush=True)
+
+    def compare(tag, alive):
+        assert np.array_equal(proxy_ids(g), proxy_ids(o)), f"{tag}: proxy ids differ"
+        if "step" not in tag:
+            return  # the device drops a destroyed body's contacts at the next step
+        gb, ob = g.bodies()[alive], o.bodies()[alive]
+        bad = np.where(~np.all(_canon(gb[:, :11]) == _canon(ob[:, :11]), axis=1))[0]
+        assert len(bad) == 0, f"{tag}: {len(bad)} bodies differ, first {bad[:5]}"
+        gm, gman = g.contacts()
+        om, oman = o.contacts()
+        # the device table is in creation order, the reference's list newest first
+        assert np.array_equal(gm[::-1, :3], om[:, :3]), f"{tag}: contact lists differ"
+        assert np.array_equal(E.manifold_bits(gman[::-1]), E.manifold_bits(oman)), f"{tag}: manifolds differ"
+
+    destroy_plan_run(g, o, compare)
+    ids = proxy_ids(g)
+    assert np.any(np.diff(ids[ids >= 0]) < 0)  # freed ids were taken again
+    g.close(); o.close()
+
+
+def test_destroy_body_coloured(gpu, orc):
+    """The coloured solver through the same plan: the proxy ids equal the
+    oracle's, every contact's fixture A is the one with the smaller proxy id
+    unless the shape pair's registry swaps them, no contact survives its
+    body's next step, and the penetration at the end stays within the
+    oracle's on the same plan plus 0.5 cm (mean) and 5 cm (max)."""
+    from test_oracle import destroy_plan_run, proxy_ids
+    g = gpu.world("world", 150, 5, sleep=True)
+    o = orc.world("world", 150, 5, sleep=True)
+    def compare(tag, alive):
+        ids = proxy_ids(g)
+        assert np.array_equal(ids, proxy_ids(o)), f"{tag}: proxy ids differ"
+        if "step" not in tag:
+            return
+        meta, _ = g.contacts()
+        body, _ = g.fixture_info()
+        assert np.all(alive[body[meta[:, 0]]]) and np.all(alive[body[meta[:, 1]]]), f"{tag}: a destroyed body's contact"
+        fa, fb = meta[:, 0], meta[:, 1]
+        assert np.all(ids[fa] >= 0) and np.all(ids[fb] >= 0)
+        # the same fixtures in the oracle: their shape types
+        nf = len(ids)
+        sh = np.zeros(nf, E.SHAPE)
+        o.e.fn("world_get_shapes")(o.h, E.ptr(sh))
+        ta, tb = sh["type"][fa], sh["type"][fb]
+        # A has the larger id only where the registry swapped a pair whose
+        # smaller-id fixture comes second in b2Contact's table: (circle,
+        # polygon), (circle, edge), (polygon, edge) become (polygon, circle),
+        # (edge, circle), (edge, polygon) (types 0 circle, 1 edge, 2 polygon)
+        swapped = ids[fa] > ids[fb]
+        pair = ta * 3 + tb
+        assert np.all(np.isin(pair[swapped], [2 * 3 + 0, 1 * 3 + 0, 1 * 3 + 2])), tag
+        assert np.all(np.isfinite(g.bodies()[alive]))
+
+    destroy_plan_run(g, o, compare)
+    # matched penetration: the oracle ran the same plan (2.1 cm mean, 19.5 cm max)
+    mean, mx = g.penetration()[:2]
+    omean, omx = o.penetration()[:2]
+    assert mean <= omean + 0.005 and mx <= omx + 0.05, (mean, mx, omean, omx)
+    g.close(); o.close()
+
+
 def test_reference_mode_matches_reference_build(gpu, ref):
     """Against upstream Box2D itself (glibc trig): poses within 1e-4 over 60 steps."""
     g = gpu.world("pyramid", 20, 0, sleep=True)

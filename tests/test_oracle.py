@@ -122,6 +122,77 @@ def test_joint_edits_bitexact(ref, orc, kind, n, seed, sleep):
     a.close(); b.close(); c.close()
 
 
+def _destroy_bodies(w, ids):
+    for i in ids:
+        assert w.e.fn("world_destroy_body")(w.h, int(i)) == 0, i
+
+
+def proxy_ids(w):
+    """Per fixture proxy in creation order: its broad-phase tree id (-1 once
+    destroyed)."""
+    n = w.e.fn("world_fixture_count")(w.h)
+    out = np.zeros(max(n, 1), np.int32)
+    assert w.e.fn("world_proxy_ids")(w.h, E.ptr(out), n) == n
+    return out[:n]
+
+
+# (steps before, bodies destroyed, bodies then added by the "rain" scene, its seed)
+DESTROY_PLAN = [(60, 12, 30, 7), (90, 8, 20, 8), (45, 25, 0, 0), (60, 0, 40, 9), (120, 0, 0, 0)]
+
+
+def destroy_plan_run(a, b, compare, seed=3, n=150):
+    """Drives two worlds loaded with the same 150-body "world" scene through
+    DESTROY_PLAN (the same bodies destroyed in both, picked by `seed`), calling
+    compare(tag, alive) after every step and after every destroy or load."""
+    rng = np.random.default_rng(seed)
+    alive = np.ones(n + 1, bool)  # body 0 is the container
+    for k, (steps, nd, na, s) in enumerate(DESTROY_PLAN):
+        for i in range(steps):
+            a.step(1)
+            b.step(1)
+            compare(f"phase {k} step {i + 1}", alive)
+        if nd:
+            ids = rng.choice(np.nonzero(alive[1:])[0] + 1, nd, replace=False)
+            _destroy_bodies(a, ids)
+            _destroy_bodies(b, ids)
+            alive[ids] = False
+            compare(f"phase {k} destroyed {sorted(ids.tolist())}", alive)
+        if na:
+            for w in (a, b):
+                assert w.e.fn("world_load_scene")(w.h, b"rain", na, ctypes.c_uint(s)) == 0
+            alive = np.concatenate([alive, np.ones(na, bool)])
+            compare(f"phase {k} added {na}", alive)
+    return alive
+
+
+def test_destroy_body_reuses_proxy_ids_bitexact(ref, orc):
+    """b2World::DestroyBody frees the body's tree proxies (REF
+    b2World.cpp:133-215, b2DynamicTree.cpp:56-131); fixtures created later take
+    the freed ids, so new contacts come in another order and with other A/B
+    roles than creation order gives. The oracle's free-list model must give
+    the reference build's proxy ids exactly, and with them the same contact
+    list and trajectories, bit for bit."""
+    a = ref.world("world", 150, 5, sleep=True)
+    b = orc.world("world", 150, 5, sleep=True)
+
+    def compare(tag, alive):
+        pa, pb = proxy_ids(a), proxy_ids(b)
+        assert np.array_equal(pa, pb), f"{tag}: proxy ids differ at {np.nonzero(pa != pb)[0][:8]}"
+        ba, bb = a.bodies()[alive], b.bodies()[alive]
+        assert np.array_equal(ba.view(np.uint32), bb.view(np.uint32)), f"{tag}: bodies differ"
+        ma, _ = a.contacts()
+        mb, _ = b.contacts()
+        assert np.array_equal(ma, mb), f"{tag}: contact lists differ"
+
+    destroy_plan_run(a, b, compare)
+    ids = proxy_ids(a)
+    live = ids[ids >= 0]
+    # freed ids were taken again: the ids no longer grow with creation order
+    assert np.any(np.diff(live) < 0)
+    assert len(np.unique(live)) == len(live)
+    a.close(); b.close()
+
+
 def test_pendulum_closed_forms(orc):
     """Free pendulums: the hinge holds each box at its length from the
     anchor (position error below the linear slop) and the motorised joints
