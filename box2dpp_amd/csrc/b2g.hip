@@ -4032,6 +4032,22 @@ __device__ inline bool acceptPair(const WorldView& w, int a, int b, const unsign
     return primaryPair(ta, tb) || primaryPair(tb, ta);
 }
 
+// The broad-phase pass's zeroing in one launch: bucket counts and cursors,
+// and the pass's counters (wide proxies, queries, new pairs, overflow).
+__global__ void k_grid_reset(int* counts, int* cursor, int n, Counters* cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        counts[i] = 0;
+        cursor[i] = 0;
+    }
+    if (i == 0) {
+        cnt->nBig = 0;
+        cnt->nQueries = 0;
+        cnt->nNewPairs = 0;
+        cnt->overflow = 0;
+    }
+}
+
 // Where queries put the pairs they find. With packBase = n (the proxy
 // count, or the order-key span with `ord`) a pair is stored as lo * n + hi,
 // lo < hi its two fixtures' order keys (`ord`, or the fixture indices when
@@ -5943,12 +5959,8 @@ int runBroadphase(b2g_context* ctx, bool liveOnDevice = false) {
     g.bigSorted = 0;
     g.fbody = ctx->fbody;
     g.bworld = ctx->subWorlds ? ctx->bworld : nullptr;
-    hipLaunchKernelGGL(k_fill<int>, dim3(blocks(ctx->nBuckets + 1)), dim3(256), 0, s, ctx->gCounts,
-                       ctx->nBuckets + 1, 0);
-    hipLaunchKernelGGL(k_fill<int>, dim3(blocks(ctx->nBuckets + 1)), dim3(256), 0, s, ctx->gCursor,
-                       ctx->nBuckets + 1, 0);
-    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nBig, 2, 0);  // nBig, nQueries
-    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nNewPairs, 1, 0);
+    hipLaunchKernelGGL(k_grid_reset, dim3(blocks(ctx->nBuckets + 1)), dim3(256), 0, s, ctx->gCounts, ctx->gCursor,
+                       ctx->nBuckets + 1, ctx->cnt);
     hipLaunchKernelGGL(k_grid_count, dim3(blocks(nf)), dim3(256), 0, s, ctx->fat, ctx->moved, nf,
                        (const uint8_t*)nullptr, g, ctx->cnt);
     LAUNCHCHK(ctx);
@@ -5973,8 +5985,10 @@ int runBroadphase(b2g_context* ctx, bool liveOnDevice = false) {
     const int packBase = ordered ? ctx->orderSpan : nf;
     int nNew = 0;
     for (int attempt = 0;; ++attempt) {
-        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nNewPairs, 1, 0);
-        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->overflow, 1, 0);
+        if (attempt > 0) {  // k_grid_reset cleared them for the first
+            hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->nNewPairs, 1, 0);
+            hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, &ctx->cnt->overflow, 1, 0);
+        }
         PairSink sink{ctx->newKeys, ctx->capContacts, (unsigned long long)packBase, ordered ? ctx->fixOrder : nullptr};
         const int qb = ctx->sched.queryBlock;
         hipEvent_t e0, e1;

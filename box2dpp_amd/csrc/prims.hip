@@ -9,7 +9,8 @@
 //    digits before it in the tile (lanes with an equal digit found by one
 //    ballot per digit bit, then the earlier wavefronts' counts from LDS);
 //  * the same kernels for one pass over 7-bit keys (the colour order);
-//  * up to 4096 64-bit keys: one workgroup's bitonic network in LDS.
+//  * up to 4096 64-bit keys: one workgroup's bitonic network in LDS; up to
+//    2^17, such chunks merged pairwise by merge-path passes.
 #include "prims.h"
 
 #include <algorithm>
@@ -265,32 +266,120 @@ hipError_t radixSort(void* scratch, size_t bytes, const K* kin, K* kout, const i
 constexpr int kSmallSortMax = 4096;
 constexpr int kSmallSortThreads = 1024;
 
-// Bitonic sort of n <= kSmallSortMax keys, padded to a power of two with the
-// largest key (which sorts last and is not written back).
-__global__ void __launch_bounds__(kSmallSortThreads) k_sort_small_u64(const unsigned long long* in,
-                                                                      unsigned long long* out, int n) {
+// Bitonic sort of each kSmallSortMax-key chunk of the input (one workgroup
+// per chunk; a partial chunk padded with the largest key, which sorts last
+// and is not written back). Each lane holds kChunkItems consecutive keys in
+// registers: compare-exchanges between keys of one lane (stride 1, 2) stay in
+// registers, those between lanes of one wavefront (stride 4-128) go through
+// lane shuffles, and only strides of 256 and up go through LDS.
+constexpr int kChunkItems = kSmallSortMax / kSmallSortThreads;
+static_assert(kChunkItems == 4, "four keys per lane");
+
+__device__ inline unsigned long long bitonicPick(unsigned long long x, unsigned long long y, int i, int j, int k) {
+    const bool lower = (i & j) == 0, ascending = (i & k) == 0;
+    const bool keepMin = lower == ascending;
+    return keepMin ? (x < y ? x : y) : (x < y ? y : x);
+}
+
+__global__ void __launch_bounds__(kSmallSortThreads) k_sort_chunks_u64(const unsigned long long* in,
+                                                                       unsigned long long* out, int n) {
     __shared__ unsigned long long a[kSmallSortMax];
-    int m = 1;
-    while (m < n) m <<= 1;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) a[i] = i < n ? in[i] : ~0ull;
-    __syncthreads();
-    for (int k = 2; k <= m; k <<= 1) {
+    const int base = blockIdx.x * kSmallSortMax;
+    const int len = n - base < kSmallSortMax ? n - base : kSmallSortMax;
+    const int t = threadIdx.x;
+    unsigned long long x[kChunkItems];
+#pragma unroll
+    for (int r = 0; r < kChunkItems; ++r) {
+        const int i = kChunkItems * t + r;
+        x[r] = i < len ? in[base + i] : ~0ull;
+    }
+    for (int k = 2; k <= kSmallSortMax; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < m; i += blockDim.x) {
-                const int p = i ^ j;
-                if (p > i) {
-                    const unsigned long long x = a[i], y = a[p];
-                    const bool ascending = (i & k) == 0;
-                    if ((x > y) == ascending) {
-                        a[i] = y;
-                        a[p] = x;
-                    }
+            if (j >= 64 * kChunkItems) {
+#pragma unroll
+                for (int r = 0; r < kChunkItems; ++r) a[kChunkItems * t + r] = x[r];
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < kChunkItems; ++r) {
+                    const int i = kChunkItems * t + r;
+                    x[r] = bitonicPick(x[r], a[i ^ j], i, j, k);
                 }
+                __syncthreads();
+            } else if (j >= kChunkItems) {
+#pragma unroll
+                for (int r = 0; r < kChunkItems; ++r) {
+                    const unsigned long long y = __shfl_xor(x[r], j / kChunkItems);
+                    x[r] = bitonicPick(x[r], y, kChunkItems * t + r, j, k);
+                }
+            } else {
+                unsigned long long y[kChunkItems];
+#pragma unroll
+                for (int r = 0; r < kChunkItems; ++r) y[r] = x[r ^ j];
+#pragma unroll
+                for (int r = 0; r < kChunkItems; ++r) x[r] = bitonicPick(x[r], y[r], kChunkItems * t + r, j, k);
             }
-            __syncthreads();
         }
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) out[i] = a[i];
+#pragma unroll
+    for (int r = 0; r < kChunkItems; ++r) {
+        const int i = kChunkItems * t + r;
+        if (i < len) out[base + i] = x[r];
+    }
+}
+
+// One pass of a merge sort: sorted runs of `width` keys merged in pairs.
+// Each lane writes kMergeItems consecutive outputs of one pair, starting
+// where the merge path crosses its first output (a binary search over the
+// diagonal; ties take the left run first).
+constexpr int kMergeItems = 8;
+constexpr int kMergeThreads = 256;
+__global__ void __launch_bounds__(kMergeThreads) k_merge_u64(const unsigned long long* in, unsigned long long* out,
+                                                            int n, int width) {
+    const long long k0 = ((long long)blockIdx.x * kMergeThreads + threadIdx.x) * kMergeItems;
+    if (k0 >= n) return;
+    const long long pairBase = k0 / (2LL * width) * (2LL * width);
+    const int a = (int)(pairBase + width < n ? width : n - pairBase);
+    const int b = (int)(pairBase + width >= n ? 0 : (pairBase + 2LL * width < n ? width : n - pairBase - width));
+    const unsigned long long* A = in + pairBase;
+    const unsigned long long* B = A + a;
+    const int d = (int)(k0 - pairBase);
+    int lo = d - b > 0 ? d - b : 0, hi = d < a ? d : a;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (A[mid] <= B[d - 1 - mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    int i = lo, j = d - lo;
+    for (int t = 0; t < kMergeItems && d + t < a + b; ++t) {
+        const bool takeA = i < a && (j >= b || A[i] <= B[j]);
+        out[k0 + t] = takeA ? A[i++] : B[j++];
+    }
+}
+
+// Merge sort above one chunk: chunks sorted in LDS, then merge passes
+// ping-ponging between the output and a scratch copy so the last pass
+// writes the output. Fewer launches than the radix sort's four per 8-bit
+// digit for the pair keys of a step.
+constexpr int kMergeSortMax = 1 << 17;
+
+size_t mergeBytes(int n) { return align256(sizeof(unsigned long long) * (size_t)n); }
+
+hipError_t mergeSort(void* scratch, size_t bytes, const unsigned long long* kin, unsigned long long* kout, int n,
+                     hipStream_t s) {
+    if (bytes < mergeBytes(n)) return hipErrorInvalidValue;
+    unsigned long long* tmp = (unsigned long long*)scratch;
+    const int chunks = (n + kSmallSortMax - 1) / kSmallSortMax;
+    int passes = 0;
+    for (int w = kSmallSortMax; w < n; w *= 2) ++passes;
+    unsigned long long* dst = passes % 2 == 0 ? kout : tmp;
+    hipLaunchKernelGGL(k_sort_chunks_u64, dim3(chunks), dim3(kSmallSortThreads), 0, s, kin, dst, n);
+    const int grid = (int)(((long long)n + (long long)kMergeThreads * kMergeItems - 1) / ((long long)kMergeThreads * kMergeItems));
+    for (int w = kSmallSortMax; w < n; w *= 2) {
+        unsigned long long* next = dst == kout ? tmp : kout;
+        hipLaunchKernelGGL(k_merge_u64, dim3(grid), dim3(kMergeThreads), 0, s, dst, next, n, w);
+        dst = next;
+    }
+    return hipGetLastError();
 }
 
 constexpr int kCountBits = 7;
@@ -328,9 +417,10 @@ hipError_t prims_sort_keys_u64(void* scratch, size_t bytes, const unsigned long 
                                int n, hipStream_t s, int endBit) {
     if (n <= 0) return hipSuccess;
     if (n <= kSmallSortMax) {
-        hipLaunchKernelGGL(k_sort_small_u64, dim3(1), dim3(kSmallSortThreads), 0, s, kin, kout, n);
+        hipLaunchKernelGGL(k_sort_chunks_u64, dim3(1), dim3(kSmallSortThreads), 0, s, kin, kout, n);
         return hipGetLastError();
     }
+    if (n <= kMergeSortMax) return mergeSort(scratch, bytes, kin, kout, n, s);
     return radixSort<unsigned long long, false>(scratch, bytes, kin, kout, (const int*)nullptr, (int*)nullptr, n,
                                                endBit, s);
 }

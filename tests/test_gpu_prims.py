@@ -5,7 +5,8 @@ colour order, new-pair order, island edges), called through the C ABI
   exclusive sum      b2g_prim_exclusive_sum               == shifted np.cumsum
   radix pair sort    b2g_prim_sort_pairs method 0         == stable argsort of key & (2^bits - 1)
   7-bit pair sort    b2g_prim_sort_pairs method 1         == stable argsort of key & 127, totals += bincount
-  64-bit key sort    b2g_prim_sort_keys_u64 (bitonic <= 4096 keys, radix above) == np.sort
+  64-bit key sort    b2g_prim_sort_keys_u64 (bitonic <= 4096 keys, 4096-key chunks merged
+                     pairwise up to 2^17, radix above)   == np.sort
 
 All bit-exact. The sizes straddle the scan's 4096-item tiles, the sorts'
 256-item tiles and the small-sort limit; repeated keys check stability.
@@ -92,7 +93,7 @@ def test_count_sort_pairs(b2g, n):
     assert np.array_equal(totals, before + np.bincount(low, minlength=128)[:66].astype(np.int32))
 
 
-@pytest.mark.parametrize("n", [1, 100, 4096, 4097, 200001])
+@pytest.mark.parametrize("n", [1, 100, 4096, 4097, 8192, 12289, 65536, 100003, 131072, 131073, 200001])
 @pytest.mark.parametrize("end_bit", [64, 40])
 def test_sort_keys_u64(b2g, n, end_bit):
     lib, ctx = b2g
