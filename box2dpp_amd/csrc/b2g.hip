@@ -4089,11 +4089,9 @@ __device__ inline void emitPair(PairSink sink, Counters* cnt, int a, int b) {
 // bare pair-set computation (no contact filtering).
 constexpr int kQueryLanes = 16;  // a divisor of 64
 
-__global__ void k_pairs_query(const float4* fat, const uint8_t* moved, int n, GridView g, WorldView w, bool world,
-                              const unsigned long long* table, uint32_t hmask, PairSink sink, Counters* cnt) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = t / kQueryLanes, lane = t % kQueryLanes;
-    if (i >= n || i >= cnt->nQueries) return;
+__device__ inline void queryOne(const float4* fat, const uint8_t* moved, const GridView& g, const WorldView& w,
+                                bool world, const unsigned long long* table, uint32_t hmask, const PairSink& sink,
+                                Counters* cnt, int i, int lane) {
     const int q = g.queries[i];
     const float4 fq = fat[q];
     const int4 r = g.range[q];
@@ -4133,6 +4131,17 @@ __global__ void k_pairs_query(const float4* fat, const uint8_t* moved, int n, Gr
         const int p = (int)(e & 0xFFFFFFFFu);
         consider(p, moved[p] != 0, fat[p], false, 0, 0);
     }
+}
+
+// The grid may be smaller than the query list (a launch sized for every
+// proxy leaves most lanes idle when few moved): groups stride over it.
+__global__ void k_pairs_query(const float4* fat, const uint8_t* moved, int n, GridView g, WorldView w, bool world,
+                              const unsigned long long* table, uint32_t hmask, PairSink sink, Counters* cnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = t % kQueryLanes;
+    const int nq = cnt->nQueries < n ? cnt->nQueries : n;
+    const int groups = (int)(gridDim.x * blockDim.x / kQueryLanes);
+    for (int i = t / kQueryLanes; i < nq; i += groups) queryOne(fat, moved, g, w, world, table, hmask, sink, cnt, i, lane);
 }
 
 // Grid entries and wide proxies that k_pairs_query examines (the buckets of
@@ -5993,7 +6002,10 @@ int runBroadphase(b2g_context* ctx, bool liveOnDevice = false) {
         const int qb = ctx->sched.queryBlock;
         hipEvent_t e0, e1;
         const int pk = profKernelEvents(ctx, &e0, &e1);
-        hipExtLaunchKernelGGL(k_pairs_query, dim3((unsigned)(((long long)nf * kQueryLanes + qb - 1) / qb)), dim3(qb), 0,
+        // at most as many query groups as the chip holds at once (256 CUs x 32
+        // wavefronts); the groups stride over the moved proxies' queries
+        const long long queryThreads = std::min((long long)nf * kQueryLanes, 32768LL * kQueryLanes);
+        hipExtLaunchKernelGGL(k_pairs_query, dim3((unsigned)((queryThreads + qb - 1) / qb)), dim3(qb), 0,
                               s, e0, e1, 0, ctx->fat, ctx->moved, nf, g, ctx->view(), true, ctx->hashTable,
                               ctx->hashMask, sink, ctx->cnt);
         if (ctx->profiling)

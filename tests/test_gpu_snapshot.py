@@ -209,3 +209,40 @@ def test_resumed_reference_mode_matches_oracle(gpu, orc, wide_trig, kind, n, see
     finally:
         lib.b2g_destroy(b)
         o.close()
+
+
+def test_resumed_after_destruction_matches_oracle(gpu, orc, wide_trig):
+    """A snapshot taken after bodies were destroyed and others added (the
+    first two phases of the destroy plan) holds the proxies' tree ids: the
+    restored context creates its new contacts in the reference's order and
+    stays equal to the oracle, bit for bit, over 150 more steps."""
+    from test_oracle import DESTROY_PLAN, destroy_plan_run
+    lib = E.b2g_lib()
+    g = gpu.world("world", 150, 5, sleep=True)
+    g.set_mode(1)
+    o = orc.world("world", 150, 5, sleep=True)
+    alive = destroy_plan_run(g, o, lambda tag, alive: None, plan=DESTROY_PLAN[:2])
+    g.step(1)  # uploads the added bodies; their pairs are found with the reused ids
+    o.step(1)
+    nb = g.e.fn("world_body_count")(g.h)
+    snap = _snapshot(lib, g.device_context())
+    g.close()
+    b = _fresh(lib)
+    p = _params()
+    p[0]["mode"] = 1
+    try:
+        E.b2g_check(lib, b, lib.b2g_restore(b, E.ptr(snap), ctypes.c_size_t(snap.size)))
+        for s in range(150):
+            E.b2g_check(lib, b, lib.b2g_step(b, E.ptr(p)))
+            o.step(1)
+            got = _rows(_bodies(lib, b, nb))[alive]
+            want = (o.bodies()[alive, :11] + np.float32(0)).view(np.uint32)
+            bad = np.where(~np.all(got == want, axis=1))[0]
+            assert len(bad) == 0, f"step {s + 1} after the resume: {len(bad)} bodies differ, first {bad[:5]}"
+        meta, man = _state(lib, b)
+        gm = meta.reshape(-1, 4)
+        om, _ = o.contacts()
+        assert np.array_equal(gm[::-1, :3], om[:, :3])  # creation order against the newest-first list
+    finally:
+        lib.b2g_destroy(b)
+        o.close()

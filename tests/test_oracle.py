@@ -140,13 +140,14 @@ def proxy_ids(w):
 DESTROY_PLAN = [(60, 12, 30, 7), (90, 8, 20, 8), (45, 25, 0, 0), (60, 0, 40, 9), (120, 0, 0, 0)]
 
 
-def destroy_plan_run(a, b, compare, seed=3, n=150):
+def destroy_plan_run(a, b, compare, seed=3, n=150, plan=DESTROY_PLAN):
     """Drives two worlds loaded with the same 150-body "world" scene through
-    DESTROY_PLAN (the same bodies destroyed in both, picked by `seed`), calling
-    compare(tag, alive) after every step and after every destroy or load."""
+    `plan` (the same bodies destroyed in both, picked by `seed`), calling
+    compare(tag, alive) after every step and after every destroy or load;
+    returns which bodies are alive."""
     rng = np.random.default_rng(seed)
     alive = np.ones(n + 1, bool)  # body 0 is the container
-    for k, (steps, nd, na, s) in enumerate(DESTROY_PLAN):
+    for k, (steps, nd, na, s) in enumerate(plan):
         for i in range(steps):
             a.step(1)
             b.step(1)
