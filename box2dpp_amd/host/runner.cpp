@@ -521,6 +521,16 @@ int b2pp_world_destroy_body(void* h, int body) {
     return 0;
 }
 
+// b2Body::DestroyFixture of the fixture whose first proxy slot is `fixture`.
+int b2pp_world_destroy_fixture(void* h, int fixture) {
+    Runner* r = (Runner*)h;
+    if (fixture < 0 || fixture >= (int)r->fixtures.size() || r->fixtures[fixture]->GetProxyIndex() != fixture)
+        return B2G_ERR_ARG;
+    b2Fixture* f = r->fixtures[fixture];  // stays valid (the world keeps it): the slot is kept
+    f->GetBody()->DestroyFixture(f);
+    return 0;
+}
+
 // Per fixture proxy in creation order: its broad-phase tree proxy id, -1
 // once destroyed.
 int b2pp_world_proxy_ids(void* h, int32_t* out, int cap) {

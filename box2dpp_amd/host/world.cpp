@@ -173,6 +173,34 @@ b2Fixture* b2Body::CreateFixture(const b2FixtureDef* def) {
     return f;
 }
 
+// b2Body::DestroyFixture, REF/Dynamics/b2Body.cpp:216-281: off the body's
+// list, its contacts destroyed (on the device: dropped at the next collide,
+// their bodies woken when the proxies go), its proxies in child order, the
+// mass data reset.
+void b2Body::DestroyFixture(b2Fixture* fixture) {
+    if (m_world->IsLocked() || fixture == nullptr || fixture->m_body != this || m_destroyed) return;
+    Touch();
+    b2Fixture** node = &m_fixtureList;
+    while (*node && *node != fixture) node = &(*node)->m_next;
+    if (*node == nullptr) return;
+    *node = fixture->m_next;
+    fixture->m_next = nullptr;
+    --m_fixtureCount;
+    b2World* w = m_world;
+    for (int32_t c = 0; c < fixture->m_shape->GetChildCount(); ++c) {
+        w->m_removedFixtures.push_back(fixture->m_index + c);
+        int32_t& id = w->m_proxyIds[fixture->m_index + c];
+        if (id >= 0) {
+            w->DestroyProxyId(id);
+            id = -1;
+            w->m_proxyIdsReused = w->m_proxyIdsDirty = true;
+        }
+    }
+    w->m_destroyedFixtures.push_back(fixture);
+    w->m_contactsStale = true;
+    ResetMassData();
+}
+
 b2Fixture* b2Body::CreateFixture(const b2Shape* shape, float density) {
     b2FixtureDef def;
     def.shape = shape;
@@ -379,6 +407,7 @@ b2World::b2World(const b2Vec2& gravity) : m_gravity(gravity) {
 
 b2World::~b2World() {
     for (b2Joint* j : m_joints) delete j;
+    for (b2Fixture* f : m_destroyedFixtures) delete f;
     for (b2Body* b : m_bodies) {
         if (!b) continue;
         b2Fixture* f = b->m_fixtureList;

@@ -14,11 +14,11 @@
 //    EndContact and PreSolve are called inside the step in list order,
 //    PostSolve after the solve for the island contacts, in the reference's
 //    island order.
-//  * DestroyBody keeps device slots. Its contacts end at the start of the
-//    next Step (EndContact then), the bodies they touched wake at once as in
-//    the reference. Broad-phase proxy ids follow the reference's tree: later
-//    fixtures take the freed ids, so new contacts come in the reference's
-//    order.
+//  * DestroyBody and DestroyFixture keep device slots. The destroyed
+//    contacts end at the start of the next Step (EndContact then); the
+//    bodies they touched wake at once, as in the reference. Broad-phase
+//    proxy ids follow the reference's tree: later fixtures take the freed
+//    ids, so new contacts come in the reference's order.
 #pragma once
 #include <vector>
 #include <string>
@@ -164,6 +164,11 @@ class b2Body {
 public:
     b2Fixture* CreateFixture(const b2FixtureDef* def);
     b2Fixture* CreateFixture(const b2Shape* shape, float density);
+    // Box2D/Dynamics/b2Body.h:156: the fixture's contacts and broad-phase
+    // proxies go (its contacts end at the start of the next Step, the bodies
+    // they touched wake at once), the mass is recomputed. The fixture object
+    // stays owned by the world until it is destroyed, but must not be used.
+    void DestroyFixture(b2Fixture* fixture);
 
     void SetTransform(const b2Vec2& position, float angle);
     const b2Transform& GetTransform() const;
@@ -930,6 +935,7 @@ private:
     std::vector<uint64_t> m_touchingKeys;  // touching fixture pairs after the last Step
     std::vector<int32_t> m_postSolveOrder; // PostSolve order of the open step (table indices after it)
     std::vector<int32_t> m_removedFixtures;  // proxies to drop at the next upload
+    std::vector<b2Fixture*> m_destroyedFixtures;  // by DestroyFixture, deleted with the world
     std::vector<int32_t> m_proxyIds;       // tree proxy id per fixture proxy, creation order
     std::vector<int32_t> m_treeFree;       // the tree's free node ids, head at the back (-1: a freed parent)
     int32_t m_treeCapacity = 0, m_treeLeaves = 0;

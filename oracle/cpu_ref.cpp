@@ -1893,6 +1893,39 @@ struct World {
         b.alive = false;
     }
 
+    // b2Body::DestroyFixture, REF/Dynamics/b2Body.cpp:216-281: the fixture
+    // (every child of a chain) leaves the body's list, the body's contacts
+    // through it are destroyed in the body's edge-list order, then its
+    // proxies in child order, and the mass data is reset.
+    void destroyFixture(int owner) {
+        Fixture& f0 = fixtures[owner];
+        Body& b = bodies[f0.body];
+        if (!b.alive) return;
+        std::vector<int> children;
+        for (int fi : b.fixtures)
+            if (fixtures[fi].owner == owner) children.push_back(fi);
+        if (children.empty()) return;
+        auto mine = [&](int fi) { return fixtures[fi].owner == owner; };
+        for (int e = b.edgeHead; e != -1;) {
+            const int next = edgeRef(e).next;
+            const Contact& c = contacts[e >> 1];
+            if ((fixtures[c.fA].body == f0.body && mine(c.fA)) || (fixtures[c.fB].body == f0.body && mine(c.fB)))
+                destroyContact(e >> 1);
+            e = next;
+        }
+        for (int fi : children) {
+            Fixture& f = fixtures[fi];
+            if (!f.proxy) continue;
+            for (int& m : moveBuffer)
+                if (m == fi) m = -1;
+            destroyProxyId(f.proxyId);
+            f.proxy = false;
+            f.proxyId = -1;
+        }
+        b.fixtures.erase(std::remove_if(b.fixtures.begin(), b.fixtures.end(), mine), b.fixtures.end());
+        resetMassData(b);
+    }
+
     // b2RevoluteJoint::EnableLimit, SetLimits, EnableMotor, SetMotorSpeed,
     // SetMaxMotorTorque in that order, REF/Dynamics/Joints/b2RevoluteJoint.cpp:419-471.
     void setJoint(int ji, bool enableLimit, float lower, float upper, bool enableMotor, float speed, float torque) {
@@ -4157,6 +4190,19 @@ int orc_world_destroy_body(void* h, int body) {
     World* w = (World*)h;
     if (body < 0 || body >= (int)w->bodies.size() || !w->bodies[body].alive) return -1;
     w->destroyBody(body);
+    return 0;
+}
+
+// fixture: the first device-order slot of the b2Fixture (its first child).
+int orc_world_destroy_fixture(void* h, int fixture) {
+    World* w = (World*)h;
+    if (fixture < 0 || fixture >= (int)w->fixtures.size() || w->fixtures[fixture].owner != fixture) return -1;
+    const orc::Fixture& f = w->fixtures[fixture];
+    if (!w->bodies[f.body].alive) return -1;
+    bool attached = false;
+    for (int fi : w->bodies[f.body].fixtures) attached = attached || fi == fixture;
+    if (!attached) return -1;
+    w->destroyFixture(fixture);
     return 0;
 }
 

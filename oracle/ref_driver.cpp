@@ -624,6 +624,19 @@ int ref_world_destroy_body(void* h, int body) {
     return 0;
 }
 
+// b2Body::DestroyFixture of the fixture whose first proxy slot is `fixture`.
+int ref_world_destroy_fixture(void* h, int fixture) {
+    RefWorld* w = (RefWorld*)h;
+    if (fixture < 0 || fixture >= (int)w->fixtures.size() || w->fixtures[fixture] == nullptr ||
+        fixtureIndex(w->fixtures[fixture]) != fixture)
+        return -1;
+    b2Fixture* f = w->fixtures[fixture];
+    for (auto& g : w->fixtures)
+        if (g == f) g = nullptr;
+    f->GetBody()->DestroyFixture(f);
+    return 0;
+}
+
 // Per fixture proxy in creation order: its id in the broad-phase tree, -1
 // once destroyed.
 int ref_world_proxy_ids(void* h, int32_t* out, int cap) {

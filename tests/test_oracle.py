@@ -140,12 +140,14 @@ def proxy_ids(w):
 DESTROY_PLAN = [(60, 12, 30, 7), (90, 8, 20, 8), (45, 25, 0, 0), (60, 0, 40, 9), (120, 0, 0, 0)]
 
 
-def destroy_plan_run(a, b, compare, seed=3, n=150, plan=DESTROY_PLAN):
+def destroy_plan_run(a, b, compare, seed=3, n=150, plan=DESTROY_PLAN, fixtures=True):
     """Drives two worlds loaded with the same 150-body "world" scene through
-    `plan` (the same bodies destroyed in both, picked by `seed`), calling
-    compare(tag, alive) after every step and after every destroy or load;
-    returns which bodies are alive."""
+    `plan` (the same bodies destroyed in both, picked by `seed`, and with
+    `fixtures` three more bodies' fixtures), calling compare(tag, alive)
+    after every step and after every destroy or load; returns which bodies
+    are alive."""
     rng = np.random.default_rng(seed)
+    gone = set()
     alive = np.ones(n + 1, bool)  # body 0 is the container
     for k, (steps, nd, na, s) in enumerate(plan):
         for i in range(steps):
@@ -158,6 +160,16 @@ def destroy_plan_run(a, b, compare, seed=3, n=150, plan=DESTROY_PLAN):
             _destroy_bodies(b, ids)
             alive[ids] = False
             compare(f"phase {k} destroyed {sorted(ids.tolist())}", alive)
+            if fixtures:
+                # b2Body::DestroyFixture on a few live bodies' only fixture: they
+                # fall on as shapeless bodies of unit mass
+                body, _ = b.fixture_info()
+                live = [f for f in range(len(body)) if body[f] > 0 and alive[body[f]] and f not in gone]
+                for f in rng.choice(live, min(3, len(live)), replace=False):
+                    for w in (a, b):
+                        assert w.e.fn("world_destroy_fixture")(w.h, int(f)) == 0, f
+                    gone.add(int(f))
+                compare(f"phase {k} destroyed fixtures", alive)
         if na:
             for w in (a, b):
                 assert w.e.fn("world_load_scene")(w.h, b"rain", na, ctypes.c_uint(s)) == 0
