@@ -1844,9 +1844,74 @@ b2Contact* b2World::GetContactList() {
             }
             c.m_next = i + 1 < n ? &m_contacts[i + 1] : nullptr;
         }
+        // Each body's contact list is the world's order restricted to the
+        // body (both are newest first in the reference): built oldest first,
+        // each edge prepended.
+        for (b2Body* b : m_bodies)
+            if (b) b->m_contactList = nullptr;
+        for (int i = n - 1; i >= 0; --i) {
+            b2Contact& c = m_contacts[i];
+            b2Body* bA = c.m_fixtureA->m_body;
+            b2Body* bB = c.m_fixtureB->m_body;
+            b2ContactEdge* edges[2] = {&c.m_nodeA, &c.m_nodeB};
+            b2Body* owners[2] = {bA, bB};
+            for (int k = 0; k < 2; ++k) {
+                b2ContactEdge* e = edges[k];
+                e->contact = &c;
+                e->other = owners[1 - k];
+                e->prev = nullptr;
+                e->next = owners[k]->m_contactList;
+                if (e->next) e->next->prev = e;
+                owners[k]->m_contactList = e;
+            }
+        }
         m_contactsStale = false;
     }
     return m_contacts.empty() ? nullptr : &m_contacts[0];
+}
+
+b2ContactEdge* b2Body::GetContactList() {
+    m_world->GetContactList();
+    return m_contactList;
+}
+
+void b2Body::SetBullet(bool flag) {
+    Touch();
+    if (flag) m_flags |= e_bulletFlag;
+    else m_flags &= ~e_bulletFlag;
+    MarkDirty();
+}
+
+// REF/Dynamics/b2Body.cpp:438-457
+void b2Body::SetFixedRotation(bool flag) {
+    Touch();
+    const bool status = (m_flags & e_fixedRotationFlag) != 0;
+    if (status == flag) return;
+    if (flag) m_flags |= e_fixedRotationFlag;
+    else m_flags &= ~e_fixedRotationFlag;
+    m_angularVelocity = 0.0f;
+    ResetMassData();
+}
+
+// REF/Dynamics/b2Body.cpp:354-394
+void b2Body::SetMassData(const b2MassData* data) {
+    if (m_world->IsLocked() || data == nullptr || m_type != b2BodyType::DYNAMIC_BODY) return;
+    Touch();
+    m_invMass = 0.0f;
+    m_I = 0.0f;
+    m_invI = 0.0f;
+    m_mass = data->mass;
+    if (m_mass <= 0.0f) m_mass = 1.0f;
+    m_invMass = 1.0f / m_mass;
+    if (data->I > 0.0f && (m_flags & e_fixedRotationFlag) == 0) {
+        m_I = data->I - m_mass * b2Dot(data->center, data->center);
+        m_invI = 1.0f / m_I;
+    }
+    const b2Vec2 oldCenter = m_sweep.c;
+    m_sweep.localCenter = data->center;
+    m_sweep.c0 = m_sweep.c = b2Mul(m_xf, m_sweep.localCenter);
+    m_linearVelocity += b2Cross(m_angularVelocity, m_sweep.c - oldCenter);
+    MarkDirty();
 }
 
 // b2MixFriction, b2MixRestitution (REF/Dynamics/Contacts/b2Contact.h:30-41)

@@ -37,6 +37,7 @@ class b2Fixture;
 class b2Contact;
 class b2Joint;
 struct b2JointEdge;
+struct b2ContactEdge;
 
 // Box2D/Dynamics/b2Body.h:41-49
 enum class b2BodyType { STATIC_BODY = 0, KINEMATIC_BODY, DYNAMIC_BODY };
@@ -211,6 +212,16 @@ public:
     b2Fixture* GetFixtureList() { return m_fixtureList; }
     const b2Fixture* GetFixtureList() const { return m_fixtureList; }
     b2JointEdge* GetJointList() { return m_jointList; }
+    // Box2D/Dynamics/b2Body.h:301-305: the body's contacts, newest first (the
+    // world's contact list order); valid until the next Step or edit.
+    b2ContactEdge* GetContactList();
+    // Box2D/Dynamics/b2Body.h:260-276, 583-586, 618-628, b2Body.cpp:354-394, 438-457
+    b2Vec2 GetLinearVelocityFromLocalPoint(const b2Vec2& localPoint) const {
+        return GetLinearVelocityFromWorldPoint(GetWorldPoint(localPoint));
+    }
+    void SetBullet(bool flag);
+    void SetFixedRotation(bool flag);
+    void SetMassData(const b2MassData* data);
     const b2JointEdge* GetJointList() const { return m_jointList; }
     b2Body* GetNext() { return m_next; }
     const b2Body* GetNext() const { return m_next; }
@@ -249,6 +260,7 @@ private:
     b2Fixture* m_fixtureList = nullptr;
     int32_t m_fixtureCount = 0;
     b2JointEdge* m_jointList = nullptr;
+    b2ContactEdge* m_contactList = nullptr;  // built with the world's contact mirror
     float m_mass = 0.0f, m_invMass = 0.0f;
     float m_I = 0.0f, m_invI = 0.0f;
     float m_linearDamping = 0.0f, m_angularDamping = 0.0f, m_gravityScale = 1.0f;
@@ -259,6 +271,15 @@ private:
 };
 
 // Host view of one device contact, valid until the next Step.
+// Box2D/Dynamics/Contacts/b2Contact.h:63-73: a contact's entry in a body's
+// contact list.
+struct b2ContactEdge {
+    b2Body* other = nullptr;
+    b2Contact* contact = nullptr;
+    b2ContactEdge* prev = nullptr;
+    b2ContactEdge* next = nullptr;
+};
+
 class b2Contact {
 public:
     b2Manifold* GetManifold() { return &m_manifold; }
@@ -297,6 +318,7 @@ private:
     bool m_touching = false, m_enabled = true;
     float m_friction = 0.0f, m_restitution = 0.0f, m_tangentSpeed = 0.0f;
     b2Contact* m_next = nullptr;
+    b2ContactEdge m_nodeA, m_nodeB;  // entries in body A's and body B's contact lists
 };
 
 // Box2D/Dynamics/Joints/b2Joint.h:31-53
