@@ -374,8 +374,9 @@ __device__ inline bool collideOne(const WorldView& w, Counters* cnt, int c, bool
     int flags = w.cflags[c] & ~CF_UPDATED;
     const int fA = w.cfA[c], fB = w.cfB[c];
     const int bA = w.fbody[fA], bB = w.fbody[fB];
-    if (removedFixture(w, fA) || removedFixture(w, fB)) {
-        // destroyed with its body (its wake-ups done by k_wake_removed_contacts)
+    if (removedFixture(w, fA) || removedFixture(w, fB) || (flags & CF_DESTROY)) {
+        // destroyed with its body or fixture, or by a body type change (its
+        // wake-ups done when that happened)
         if (flags & CF_TOUCHING) atomicAdd(&cnt->endEvents, 1);
         w.cflags[c] = flags | CF_DEAD;
         return false;
@@ -4282,6 +4283,32 @@ __global__ void k_remove_fixtures(WorldView w, const int* ids, int n) {
 // b2ContactManager.cpp:39-101, Contacts/b2Contact.cpp:104-118). The wake-ups
 // happen here, as the proxies go; the next step's collide drops the
 // contacts before the activity test reads any body.
+// b2Body::SetType destroys the body's contacts (the same wake-ups as above)
+// and touches its proxies, so the step's broad-phase recreates the pairs that
+// still overlap (REF/Dynamics/b2Body.cpp:115-172, b2BroadPhase::TouchProxy
+// REF/Collision/b2BroadPhase.cpp:66-69). mark: per body, 1 for the bodies
+// whose type changed.
+__global__ void k_drop_body_contacts(WorldView w, const uint8_t* mark) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w.nContacts) return;
+    const int fA = w.cfA[c], fB = w.cfB[c];
+    const int bA = w.fbody[fA], bB = w.fbody[fB];
+    if (!mark[bA] && !mark[bB]) return;
+    const int flags = w.cflags[c];
+    if (flags & (CF_DEAD | CF_DESTROY)) return;
+    if (w.cman[c].pointCount > 0 && !sensorOf(w, fA) && !sensorOf(w, fB)) {
+        wakeBody(w, bA);
+        wakeBody(w, bB);
+    }
+    w.cflags[c] = flags | CF_DESTROY;
+}
+
+__global__ void k_touch_body_proxies(WorldView w, const uint8_t* mark) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= w.nFixtures || !mark[w.fbody[f]] || removedFixture(w, f)) return;
+    w.moved[f] = 1;
+}
+
 __global__ void k_wake_removed_contacts(WorldView w) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= w.nContacts) return;
@@ -4357,7 +4384,7 @@ __global__ void k_collide_serial(WorldView w, RefView r, Counters* cnt) {
         const int fA = w.cfA[c], fB = w.cfB[c];
         const int bA = w.fbody[fA], bB = w.fbody[fB];
         const bool destroyWake = w.cman[c].pointCount > 0 && !sensorOf(w, fA) && !sensorOf(w, fB);
-        if (removedFixture(w, fA) || removedFixture(w, fB)) {
+        if (removedFixture(w, fA) || removedFixture(w, fB) || (flags & CF_DESTROY)) {
             if (flags & CF_TOUCHING) cnt->endEvents++;
             w.cflags[c] = flags | CF_DEAD;
             continue;
@@ -7933,6 +7960,29 @@ int b2g_remove_fixtures(b2g_context* ctx, const int32_t* ids, int count) {
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(ctx, B2G_ERR_HIP, hipGetErrorString(e));
+    return B2G_OK;
+}
+
+int b2g_drop_body_contacts(b2g_context* ctx, const int32_t* bodies, int count) {
+    if (!ctx || count < 0 || (count > 0 && !bodies)) return B2G_ERR_ARG;
+    if (count == 0) return B2G_OK;
+    std::vector<uint8_t> mark(std::max(ctx->nBodies, 1), 0);
+    for (int i = 0; i < count; ++i) {
+        if (bodies[i] < 0 || bodies[i] >= ctx->nBodies) return fail(ctx, B2G_ERR_ARG, "body index");
+        mark[bodies[i]] = 1;
+    }
+    uint8_t* d = nullptr;
+    HIPCHK(ctx, hipMalloc((void**)&d, mark.size()));
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipMemcpyAsync(d, mark.data(), mark.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && ctx->nContacts > 0)
+        hipLaunchKernelGGL(k_drop_body_contacts, dim3(blocks(ctx->nContacts)), dim3(256), 0, s, ctx->view(), d);
+    if (e == hipSuccess && ctx->nFixtures > 0)
+        hipLaunchKernelGGL(k_touch_body_proxies, dim3(blocks(ctx->nFixtures)), dim3(256), 0, s, ctx->view(), d);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(ctx, B2G_ERR_HIP, hipGetErrorString(e));
     return B2G_OK;

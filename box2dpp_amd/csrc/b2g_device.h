@@ -48,6 +48,7 @@ enum : int {
     CF_NOOVERLAP = 0x0400,   // fat AABBs stopped overlapping
     CF_UPDATED = 0x0800,     // b2Contact::Update ran on it this step
     CF_WAS_TOUCHING = 0x1000,  // touching before this step's update
+    CF_DESTROY = 0x2000,     // destroyed by a body type change; dropped first thing in the next collide
 };
 enum : int { BT_STATIC = 0, BT_KINEMATIC = 1, BT_DYNAMIC = 2 };
 

@@ -521,6 +521,14 @@ int b2pp_world_destroy_body(void* h, int body) {
     return 0;
 }
 
+// b2Body::SetType (0 static, 1 kinematic, 2 dynamic).
+int b2pp_world_set_type(void* h, int body, int type) {
+    Runner* r = (Runner*)h;
+    if (body < 0 || body >= (int)r->bodies.size() || type < 0 || type > 2) return B2G_ERR_ARG;
+    r->bodies[body]->SetType((b2BodyType)type);
+    return 0;
+}
+
 // b2Body::DestroyFixture of the fixture whose first proxy slot is `fixture`.
 int b2pp_world_destroy_fixture(void* h, int fixture) {
     Runner* r = (Runner*)h;

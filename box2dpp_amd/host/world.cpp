@@ -1536,6 +1536,11 @@ int32_t b2World::Upload() {
         if (rc != B2G_OK) return rc;
         m_removedFixtures.clear();
     }
+    if (!m_typeChanged.empty()) {
+        rc = b2g_drop_body_contacts(m_ctx, m_typeChanged.data(), (int)m_typeChanged.size());
+        if (rc != B2G_OK) return rc;
+        m_typeChanged.clear();
+    }
     if (m_proxyIdsDirty && m_subWorldCount == 0) {
         rc = b2g_set_proxy_ids(m_ctx, m_proxyIds.data(), (int)m_proxyIds.size());
         if (rc != B2G_OK) return rc;
@@ -1873,6 +1878,30 @@ b2Contact* b2World::GetContactList() {
 b2ContactEdge* b2Body::GetContactList() {
     m_world->GetContactList();
     return m_contactList;
+}
+
+// REF/Dynamics/b2Body.cpp:115-172. The reference also re-synchronises a
+// body made static from its (now equal) sweep ends: that box lies inside
+// the fat box its last synchronisation left, so no proxy moves and the
+// device has nothing to do for it.
+void b2Body::SetType(b2BodyType type) {
+    if (m_world->IsLocked() || m_destroyed) return;
+    Touch();
+    if (m_type == type) return;
+    m_type = type;
+    ResetMassData();
+    if (m_type == b2BodyType::STATIC_BODY) {
+        m_linearVelocity.SetZero();
+        m_angularVelocity = 0.0f;
+        m_sweep.a0 = m_sweep.a;
+        m_sweep.c0 = m_sweep.c;
+    }
+    SetAwake(true);
+    m_force.SetZero();
+    m_torque = 0.0f;
+    MarkDirty();
+    m_world->m_typeChanged.push_back(m_index);
+    m_world->m_contactsStale = true;
 }
 
 void b2Body::SetBullet(bool flag) {

@@ -247,6 +247,14 @@ int b2g_add_fixtures(b2g_context* ctx, const b2g_fixture_def* fixtures, int coun
  * immediate contact destruction wakes them; the contacts themselves end
  * first thing in the next step's collide (EndContact then). */
 int b2g_remove_fixtures(b2g_context* ctx, const int32_t* ids, int count);
+/* The contact reset of b2Body::SetType (Box2D/Dynamics/b2Body.h:305) for
+ * bodies whose type the host changed (their new state uploaded first): each
+ * of their contacts is destroyed, its bodies woken now when its manifold has
+ * points and the contact dropped first thing in the next step's collide
+ * (EndContact then), and their proxies join the move buffer, so the step's
+ * broad-phase recreates the pairs that still overlap. */
+int b2g_drop_body_contacts(b2g_context* ctx, const int32_t* bodies, int count);
+
 /* The broad-phase order of the proxies: ids[f] is the reference tree's
  * proxy id of fixture f (-1 for a removed one) for every fixture (count =
  * fixtures added so far; call again after adding more). New pairs are then

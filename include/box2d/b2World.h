@@ -220,6 +220,10 @@ public:
         return GetLinearVelocityFromWorldPoint(GetWorldPoint(localPoint));
     }
     void SetBullet(bool flag);
+    // Box2D/Dynamics/b2Body.h:305: mass data reset, the body woken, its
+    // contacts destroyed (ending at the start of the next Step) and its
+    // proxies touched so the next Step's broad-phase recreates the pairs.
+    void SetType(b2BodyType type);
     void SetFixedRotation(bool flag);
     void SetMassData(const b2MassData* data);
     const b2JointEdge* GetJointList() const { return m_jointList; }
@@ -958,6 +962,7 @@ private:
     std::vector<int32_t> m_postSolveOrder; // PostSolve order of the open step (table indices after it)
     std::vector<int32_t> m_removedFixtures;  // proxies to drop at the next upload
     std::vector<b2Fixture*> m_destroyedFixtures;  // by DestroyFixture, deleted with the world
+    std::vector<int32_t> m_typeChanged;    // bodies whose contacts SetType dropped, for the next upload
     std::vector<int32_t> m_proxyIds;       // tree proxy id per fixture proxy, creation order
     std::vector<int32_t> m_treeFree;       // the tree's free node ids, head at the back (-1: a freed parent)
     int32_t m_treeCapacity = 0, m_treeLeaves = 0;

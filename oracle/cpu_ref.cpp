@@ -1893,6 +1893,30 @@ struct World {
         b.alive = false;
     }
 
+    // b2Body::SetType, REF/Dynamics/b2Body.cpp:115-172: mass data reset; a
+    // body made static stops and re-synchronises its fixtures from its equal
+    // sweep ends; woken, forces cleared, its contacts destroyed, its proxies
+    // touched (b2BroadPhase::TouchProxy, REF/Collision/b2BroadPhase.cpp:66-69).
+    void setType(int bi, int type) {
+        Body& b = bodies[bi];
+        if (!b.alive || b.type == type) return;
+        b.type = type;
+        resetMassData(b);
+        if (type == kStatic) {
+            b.v = mk(0.0f, 0.0f);
+            b.w = 0.0f;
+            b.a0 = b.a;
+            b.c0 = b.c;
+            synchronizeFixtures(b);
+        }
+        setAwake(b, true);
+        b.force = mk(0.0f, 0.0f);
+        b.torque = 0.0f;
+        while (b.edgeHead != -1) destroyContact(b.edgeHead >> 1);
+        for (int k = (int)b.fixtures.size() - 1; k >= 0; --k)
+            if (fixtures[b.fixtures[k]].proxy) moveBuffer.push_back(b.fixtures[k]);
+    }
+
     // b2Body::DestroyFixture, REF/Dynamics/b2Body.cpp:216-281: the fixture
     // (every child of a chain) leaves the body's list, the body's contacts
     // through it are destroyed in the body's edge-list order, then its
@@ -4190,6 +4214,14 @@ int orc_world_destroy_body(void* h, int body) {
     World* w = (World*)h;
     if (body < 0 || body >= (int)w->bodies.size() || !w->bodies[body].alive) return -1;
     w->destroyBody(body);
+    return 0;
+}
+
+// type: 0 static, 1 kinematic, 2 dynamic.
+int orc_world_set_type(void* h, int body, int type) {
+    World* w = (World*)h;
+    if (body < 0 || body >= (int)w->bodies.size() || !w->bodies[body].alive || type < 0 || type > 2) return -1;
+    w->setType(body, type);
     return 0;
 }
 

@@ -134,6 +134,32 @@ def test_destroy_body_coloured(gpu, orc):
     g.close(); o.close()
 
 
+def test_set_type_reference_order_bitexact(gpu, orc, wide_trig):
+    """b2Body::SetType mid-run (the plan of test_oracle.type_plan_run, which
+    pins the oracle against the reference build): the body's contacts end at
+    the next step, its neighbours wake at once, its proxies are queried again
+    at the end of the next step; reference-order mode stays equal to the
+    oracle bit for bit, the contact list in order."""
+    from test_oracle import type_plan_run
+    g = gpu.world("world", 150, 5, sleep=True)
+    g.set_mode(REFERENCE)
+    o = orc.world("world", 150, 5, sleep=True)
+    print("case: type plan, reference order", flush=True)
+
+    def compare(tag):
+        if "step" not in tag:
+            return  # the neighbours' wake-ups reach the host copy with the next step
+        gb, ob = g.bodies(), o.bodies()
+        bad = np.where(~np.all(_canon(gb[:, :11]) == _canon(ob[:, :11]), axis=1))[0]
+        assert len(bad) == 0, f"{tag}: {len(bad)} bodies differ, first {bad[:5]}"
+        gm, _ = g.contacts()
+        om, _ = o.contacts()
+        assert np.array_equal(gm[::-1, :3], om[:, :3]), f"{tag}: contact lists differ"
+
+    type_plan_run(g, o, compare)
+    g.close(); o.close()
+
+
 def test_reference_mode_matches_reference_build(gpu, ref):
     """Against upstream Box2D itself (glibc trig): poses within 1e-4 over 60 steps."""
     g = gpu.world("pyramid", 20, 0, sleep=True)

@@ -206,6 +206,43 @@ def test_destroy_body_reuses_proxy_ids_bitexact(ref, orc):
     a.close(); b.close()
 
 
+# (steps before, [(body, new type)]): 0 static, 1 kinematic, 2 dynamic
+TYPE_PLAN = [(60, [(5, 0), (17, 0), (40, 1), (0, 2)]), (90, [(5, 2), (0, 0), (33, 0), (40, 2)]),
+             (60, [(17, 2), (80, 1), (81, 0)]), (120, [])]
+
+
+def type_plan_run(a, b, compare):
+    """b2Body::SetType mid-run on two worlds loaded with the 150-body "world"
+    scene: bodies made static and kinematic and back, the container made
+    dynamic and static again; compare(tag) after every step and change."""
+    for k, (steps, changes) in enumerate(TYPE_PLAN):
+        for i in range(steps):
+            a.step(1)
+            b.step(1)
+            compare(f"phase {k} step {i + 1}")
+        for body, t in changes:
+            for w in (a, b):
+                assert w.e.fn("world_set_type")(w.h, body, t) == 0
+        if changes:
+            compare(f"phase {k} types changed")
+
+
+def test_set_type_bitexact(ref, orc):
+    """The oracle's SetType (REF b2Body.cpp:115-172) against the reference
+    build: bodies and contact lists bit for bit through the type plan."""
+    a = ref.world("world", 150, 5, sleep=True)
+    b = orc.world("world", 150, 5, sleep=True)
+
+    def compare(tag):
+        assert np.array_equal(a.bodies().view(np.uint32), b.bodies().view(np.uint32)), f"{tag}: bodies differ"
+        ma, _ = a.contacts()
+        mb, _ = b.contacts()
+        assert np.array_equal(ma, mb), f"{tag}: contact lists differ"
+
+    type_plan_run(a, b, compare)
+    a.close(); b.close()
+
+
 def test_pendulum_closed_forms(orc):
     """Free pendulums: the hinge holds each box at its length from the
     anchor (position error below the linear slop) and the motorised joints
