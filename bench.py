@@ -60,8 +60,8 @@ HBM_PEAK_GBS = 8000.0
 # demotion round per step; the summaries with the commands are under
 # profiles/. Another schedule or demotion setting reports traffic as null.
 DEFAULT_TRAFFIC_BYTES = {(0, 1, 256, 256, 64, 2048, 0, 0): {
-    "k_solve_velocity": 8765133.0, "k_solve_position": 7213978.0, "k_collide": 552350618.0,
-    "k_pairs_query": 75074970.0}}
+    "k_solve_velocity": 8765133.0, "k_solve_position": 7220736.0, "k_collide": 569667277.0,
+    "k_pairs_query": 75295642.0}}
 # The pile's step schedule as the autotuner picks it on this workload (step
 # 500 of a warm-up copy): fixed so the HBM traffic above applies; the batched
 # and stress worlds are tuned on every run.
