@@ -7860,6 +7860,38 @@ int b2g_get_bodies(b2g_context* ctx, int first, int count, b2g_body_state* out) 
     return B2G_OK;
 }
 
+}  // extern "C"
+
+// b2Fixture::CreateProxies' box: the shape's AABB at the body transform,
+// fattened by the AABB extension (REF/Collision/b2DynamicTree.cpp:112-126);
+// chain children have no skin in theirs (b2ChainShape::ComputeAABB).
+static float4 proxyFatBox(const b2g_shape& s, const float bxf[4]) {
+    Xf xf;
+    xf.p = mk(bxf[0], bxf[1]);
+    xf.q.s = bxf[2];
+    xf.q.c = bxf[3];
+    float lx, ly, ux, uy;
+    if (s.type == B2G_CIRCLE) {
+        const V2 p = xf.p + mul(xf.q, ld(s.centroid));
+        lx = p.x - s.radius; ly = p.y - s.radius; ux = p.x + s.radius; uy = p.y + s.radius;
+    } else {
+        const int nv = s.type == B2G_EDGE ? 2 : s.count;
+        V2 lo = mul(xf, ld(s.verts[0])), hi = lo;
+        for (int k = 1; k < nv; ++k) {
+            const V2 v = mul(xf, ld(s.verts[k]));
+            lo = vmin(lo, v);
+            hi = vmax(hi, v);
+        }
+        const float rad = (s.type == B2G_EDGE && (s.flags & 4)) ? 0.0f : s.radius;
+        const V2 r = mk(rad, rad);
+        const V2 l = lo - r, u = hi + r;
+        lx = l.x; ly = l.y; ux = u.x; uy = u.y;
+    }
+    return make_float4(lx - kAabbExtension, ly - kAabbExtension, ux + kAabbExtension, uy + kAabbExtension);
+}
+
+extern "C" {
+
 int b2g_add_fixtures(b2g_context* ctx, const b2g_fixture_def* defs, int count, int* first) {
     if (!ctx || count < 0 || (count > 0 && !defs)) return B2G_ERR_ARG;
     for (int i = 0; i < count; ++i) {
@@ -7892,32 +7924,7 @@ int b2g_add_fixtures(b2g_context* ctx, const b2g_fixture_def* defs, int count, i
             fmat[i] = make_float4(d.friction, d.restitution, d.density, d.sensor ? 1.0f : 0.0f);
             const b2g_body_state& b = bodies[d.body];
             const bool active = (b.flags & BF_ACTIVE) != 0;
-            // b2Fixture::CreateProxies: AABB at the body transform, fattened
-            // by the AABB extension (REF/Collision/b2DynamicTree.cpp:112-126).
-            Xf xf;
-            xf.p = mk(b.xf[0], b.xf[1]);
-            xf.q.s = b.xf[2];
-            xf.q.c = b.xf[3];
-            const b2g_shape& s = d.shape;
-            float lx, ly, ux, uy;
-            if (s.type == B2G_CIRCLE) {
-                const V2 p = xf.p + mul(xf.q, ld(s.centroid));
-                lx = p.x - s.radius; ly = p.y - s.radius; ux = p.x + s.radius; uy = p.y + s.radius;
-            } else {
-                const int nv = s.type == B2G_EDGE ? 2 : s.count;
-                V2 lo = mul(xf, ld(s.verts[0])), hi = lo;
-                for (int k = 1; k < nv; ++k) {
-                    const V2 v = mul(xf, ld(s.verts[k]));
-                    lo = vmin(lo, v);
-                    hi = vmax(hi, v);
-                }
-                // chain children have no skin in their box (b2ChainShape::ComputeAABB)
-                const float rad = (s.type == B2G_EDGE && (s.flags & 4)) ? 0.0f : s.radius;
-                const V2 r = mk(rad, rad);
-                const V2 l = lo - r, u = hi + r;
-                lx = l.x; ly = l.y; ux = u.x; uy = u.y;
-            }
-            fat[i] = make_float4(lx - kAabbExtension, ly - kAabbExtension, ux + kAabbExtension, uy + kAabbExtension);
+            fat[i] = proxyFatBox(d.shape, b.xf);
             // an inactive body's fixtures have no proxy (b2Body::CreateFixture,
             // REF/Dynamics/b2Body.cpp:182-186): the inverted box of a removed one
             if (!active) fat[i] = make_float4(FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX);
@@ -7941,6 +7948,34 @@ int b2g_add_fixtures(b2g_context* ctx, const b2g_fixture_def* defs, int count, i
         if (rc != B2G_OK) return rc;
     }
     if (first) *first = f;
+    return B2G_OK;
+}
+
+int b2g_restore_fixtures(b2g_context* ctx, const int32_t* ids, int count) {
+    if (!ctx || count < 0 || (count > 0 && !ids)) return B2G_ERR_ARG;
+    if (count == 0) return B2G_OK;
+    for (int i = 0; i < count; ++i)
+        if (ids[i] < 0 || ids[i] >= ctx->nFixtures) return fail(ctx, B2G_ERR_ARG, "fixture index");
+    hipStream_t s = ctx->stream;
+    std::vector<int> body(count);
+    std::vector<b2g_shape> shape(count);
+    for (int i = 0; i < count; ++i) {
+        HIPCHK(ctx, hipMemcpyAsync(&body[i], ctx->fbody + ids[i], sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipMemcpyAsync(&shape[i], ctx->shape + ids[i], sizeof(b2g_shape), hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    std::vector<float4> xf(count), fat(count);
+    for (int i = 0; i < count; ++i)
+        HIPCHK(ctx, hipMemcpyAsync(&xf[i], ctx->xf + body[i], sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    const uint8_t one = 1;
+    for (int i = 0; i < count; ++i) {
+        const float bxf[4] = {xf[i].x, xf[i].y, xf[i].z, xf[i].w};
+        fat[i] = proxyFatBox(shape[i], bxf);
+        HIPCHK(ctx, hipMemcpyAsync(ctx->fat + ids[i], &fat[i], sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->moved + ids[i], &one, 1, hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(s));
     return B2G_OK;
 }
 

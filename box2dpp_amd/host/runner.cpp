@@ -521,6 +521,13 @@ int b2pp_world_destroy_body(void* h, int body) {
     return 0;
 }
 
+int b2pp_world_set_active(void* h, int body, int flag) {
+    Runner* r = (Runner*)h;
+    if (body < 0 || body >= (int)r->bodies.size()) return B2G_ERR_ARG;
+    r->bodies[body]->SetActive(flag != 0);
+    return 0;
+}
+
 // b2Body::SetType (0 static, 1 kinematic, 2 dynamic).
 int b2pp_world_set_type(void* h, int body, int type) {
     Runner* r = (Runner*)h;

@@ -1536,6 +1536,11 @@ int32_t b2World::Upload() {
         if (rc != B2G_OK) return rc;
         m_removedFixtures.clear();
     }
+    if (!m_restoredFixtures.empty()) {
+        rc = b2g_restore_fixtures(m_ctx, m_restoredFixtures.data(), (int)m_restoredFixtures.size());
+        if (rc != B2G_OK) return rc;
+        m_restoredFixtures.clear();
+    }
     if (!m_typeChanged.empty()) {
         rc = b2g_drop_body_contacts(m_ctx, m_typeChanged.data(), (int)m_typeChanged.size());
         if (rc != B2G_OK) return rc;
@@ -1902,6 +1907,34 @@ void b2Body::SetType(b2BodyType type) {
     MarkDirty();
     m_world->m_typeChanged.push_back(m_index);
     m_world->m_contactsStale = true;
+}
+
+// REF/Dynamics/b2Body.cpp:456-496: fixtures in list order (newest first),
+// each one's proxies in child order.
+void b2Body::SetActive(bool flag) {
+    if (m_world->IsLocked() || m_destroyed) return;
+    Touch();
+    if (flag == IsActive()) return;
+    b2World* w = m_world;
+    if (flag) m_flags |= e_activeFlag;
+    else m_flags &= ~e_activeFlag;
+    for (b2Fixture* f = m_fixtureList; f; f = f->m_next)
+        for (int32_t c = 0; c < f->m_shape->GetChildCount(); ++c) {
+            const int32_t slot = f->m_index + c;
+            int32_t& id = w->m_proxyIds[slot];
+            if (flag) {
+                id = w->CreateProxyId();
+                w->m_restoredFixtures.push_back(slot);
+            } else if (id >= 0) {
+                w->DestroyProxyId(id);
+                id = -1;
+                w->m_removedFixtures.push_back(slot);
+            }
+        }
+    // ids taken out of creation order from here on
+    w->m_proxyIdsReused = w->m_proxyIdsDirty = true;
+    w->m_contactsStale = true;
+    MarkDirty();
 }
 
 void b2Body::SetBullet(bool flag) {

@@ -266,6 +266,12 @@ int b2g_drop_body_contacts(b2g_context* ctx, const int32_t* bodies, int count);
  * Box2D/Collision/b2DynamicTree.cpp:59-110). One world per context. */
 int b2g_set_proxy_ids(b2g_context* ctx, const int32_t* ids, int count);
 
+/* Gives removed fixtures their proxies back (b2Body::SetActive(true),
+ * Box2D/Dynamics/b2Body.h:353): each fat box from the shape at its body's
+ * device transform, the proxy in the move buffer, so the next step's
+ * closing broad-phase finds its pairs. */
+int b2g_restore_fixtures(b2g_context* ctx, const int32_t* ids, int count);
+
 /* Updates material and filter of fixtures (b2Fixture::SetFriction /
  * SetRestitution / SetFilterData, Box2D/Dynamics/b2Fixture.h:131-170); a
  * filter change flags the fixture's contacts for filtering and re-queries

@@ -224,6 +224,9 @@ public:
     // contacts destroyed (ending at the start of the next Step) and its
     // proxies touched so the next Step's broad-phase recreates the pairs.
     void SetType(b2BodyType type);
+    // Box2D/Dynamics/b2Body.h:353: proxies created (their pairs found at the
+    // end of the next Step) or destroyed together with the contacts.
+    void SetActive(bool flag);
     void SetFixedRotation(bool flag);
     void SetMassData(const b2MassData* data);
     const b2JointEdge* GetJointList() const { return m_jointList; }
@@ -963,6 +966,7 @@ private:
     std::vector<int32_t> m_removedFixtures;  // proxies to drop at the next upload
     std::vector<b2Fixture*> m_destroyedFixtures;  // by DestroyFixture, deleted with the world
     std::vector<int32_t> m_typeChanged;    // bodies whose contacts SetType dropped, for the next upload
+    std::vector<int32_t> m_restoredFixtures;  // proxies SetActive(true) gave back, for the next upload
     std::vector<int32_t> m_proxyIds;       // tree proxy id per fixture proxy, creation order
     std::vector<int32_t> m_treeFree;       // the tree's free node ids, head at the back (-1: a freed parent)
     int32_t m_treeCapacity = 0, m_treeLeaves = 0;

@@ -1917,6 +1917,45 @@ struct World {
             if (fixtures[b.fixtures[k]].proxy) moveBuffer.push_back(b.fixtures[k]);
     }
 
+    // b2Body::SetActive, REF/Dynamics/b2Body.cpp:456-496: proxies created
+    // (b2Fixture::CreateProxies at the body's transform, REF/Dynamics/
+    // b2Fixture.cpp:124-139) or destroyed and the contacts with them, the
+    // fixtures newest first and each one's children in order.
+    void setActive(int bi, bool flag) {
+        Body& b = bodies[bi];
+        if (!b.alive || ((b.flags & kActive) != 0) == flag) return;
+        if (flag) b.flags |= kActive;
+        else b.flags &= ~kActive;
+        int k = (int)b.fixtures.size();
+        while (k > 0) {
+            const int owner = fixtures[b.fixtures[k - 1]].owner;
+            int first = k - 1;
+            while (first > 0 && fixtures[b.fixtures[first - 1]].owner == owner) --first;
+            for (int i = first; i < k; ++i) {
+                const int fi = b.fixtures[i];
+                Fixture& f = fixtures[fi];
+                if (flag) {
+                    f.proxy = true;
+                    f.aabb = shapeAABB(&f.shape, b.xf);
+                    const V2 r = mk(kAabbExtension, kAabbExtension);
+                    f.fat.lo = f.aabb.lo - r;
+                    f.fat.hi = f.aabb.hi + r;
+                    f.proxyId = createProxyId(fi);
+                    moveBuffer.push_back(fi);
+                } else if (f.proxy) {
+                    for (int& m : moveBuffer)
+                        if (m == fi) m = -1;
+                    destroyProxyId(f.proxyId);
+                    f.proxy = false;
+                    f.proxyId = -1;
+                }
+            }
+            k = first;
+        }
+        if (!flag)
+            while (b.edgeHead != -1) destroyContact(b.edgeHead >> 1);
+    }
+
     // b2Body::DestroyFixture, REF/Dynamics/b2Body.cpp:216-281: the fixture
     // (every child of a chain) leaves the body's list, the body's contacts
     // through it are destroyed in the body's edge-list order, then its
@@ -4214,6 +4253,13 @@ int orc_world_destroy_body(void* h, int body) {
     World* w = (World*)h;
     if (body < 0 || body >= (int)w->bodies.size() || !w->bodies[body].alive) return -1;
     w->destroyBody(body);
+    return 0;
+}
+
+int orc_world_set_active(void* h, int body, int flag) {
+    World* w = (World*)h;
+    if (body < 0 || body >= (int)w->bodies.size() || !w->bodies[body].alive) return -1;
+    w->setActive(body, flag != 0);
     return 0;
 }
 

@@ -624,6 +624,13 @@ int ref_world_destroy_body(void* h, int body) {
     return 0;
 }
 
+int ref_world_set_active(void* h, int body, int flag) {
+    RefWorld* w = (RefWorld*)h;
+    if (body < 0 || body >= (int)w->bodies.size() || w->bodies[body] == nullptr) return -1;
+    w->bodies[body]->SetActive(flag != 0);
+    return 0;
+}
+
 int ref_world_set_type(void* h, int body, int type) {
     RefWorld* w = (RefWorld*)h;
     if (body < 0 || body >= (int)w->bodies.size() || w->bodies[body] == nullptr || type < 0 || type > 2) return -1;

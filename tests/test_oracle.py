@@ -206,15 +206,18 @@ def test_destroy_body_reuses_proxy_ids_bitexact(ref, orc):
     a.close(); b.close()
 
 
-# (steps before, [(body, new type)]): 0 static, 1 kinematic, 2 dynamic
-TYPE_PLAN = [(60, [(5, 0), (17, 0), (40, 1), (0, 2)]), (90, [(5, 2), (0, 0), (33, 0), (40, 2)]),
-             (60, [(17, 2), (80, 1), (81, 0)]), (120, [])]
+# (steps before, [(body, change)]): change 0 static, 1 kinematic, 2 dynamic,
+# 3 deactivate, 4 activate
+TYPE_PLAN = [(60, [(5, 0), (17, 0), (40, 1), (0, 2), (60, 3), (61, 3)]),
+             (90, [(5, 2), (0, 0), (33, 0), (40, 2), (60, 4), (90, 3)]),
+             (60, [(17, 2), (80, 1), (81, 0), (61, 4)]), (120, [(90, 4)]), (60, [])]
 
 
 def type_plan_run(a, b, compare):
-    """b2Body::SetType mid-run on two worlds loaded with the 150-body "world"
-    scene: bodies made static and kinematic and back, the container made
-    dynamic and static again; compare(tag) after every step and change."""
+    """b2Body::SetType and SetActive mid-run on two worlds loaded with the
+    150-body "world" scene: bodies made static and kinematic and back, the
+    container made dynamic and static again, bodies deactivated and
+    reactivated; compare(tag) after every step and change."""
     for k, (steps, changes) in enumerate(TYPE_PLAN):
         for i in range(steps):
             a.step(1)
@@ -222,14 +225,18 @@ def type_plan_run(a, b, compare):
             compare(f"phase {k} step {i + 1}")
         for body, t in changes:
             for w in (a, b):
-                assert w.e.fn("world_set_type")(w.h, body, t) == 0
+                if t < 3:
+                    assert w.e.fn("world_set_type")(w.h, body, t) == 0
+                else:
+                    assert w.e.fn("world_set_active")(w.h, body, t - 3) == 0
         if changes:
             compare(f"phase {k} types changed")
 
 
 def test_set_type_bitexact(ref, orc):
-    """The oracle's SetType (REF b2Body.cpp:115-172) against the reference
-    build: bodies and contact lists bit for bit through the type plan."""
+    """The oracle's SetType and SetActive (REF b2Body.cpp:115-172, 456-496)
+    against the reference build: bodies and contact lists bit for bit
+    through the type plan."""
     a = ref.world("world", 150, 5, sleep=True)
     b = orc.world("world", 150, 5, sleep=True)
 
