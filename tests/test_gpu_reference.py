@@ -135,11 +135,12 @@ def test_destroy_body_coloured(gpu, orc):
 
 
 def test_set_type_reference_order_bitexact(gpu, orc, wide_trig):
-    """b2Body::SetType mid-run (the plan of test_oracle.type_plan_run, which
-    pins the oracle against the reference build): the body's contacts end at
-    the next step, its neighbours wake at once, its proxies are queried again
-    at the end of the next step; reference-order mode stays equal to the
-    oracle bit for bit, the contact list in order."""
+    """b2Body::SetType and SetActive mid-run (the plan of
+    test_oracle.type_plan_run, which pins the oracle against the reference
+    build): the body's contacts end at the next step, its neighbours wake at
+    once, its proxies are queried again (or destroyed, or created with new
+    tree ids) by the end of the next step; reference-order mode stays equal
+    to the oracle bit for bit, the contact list in order."""
     from test_oracle import type_plan_run
     g = gpu.world("world", 150, 5, sleep=True)
     g.set_mode(REFERENCE)
