@@ -528,6 +528,38 @@ int b2pp_world_set_active(void* h, int body, int flag) {
     return 0;
 }
 
+// b2World::QueryAABB: the first proxy slot of each reported fixture, in
+// report order; returns the report count.
+int b2pp_world_query_aabb(void* h, const float* box, int32_t* out, int cap) {
+    struct Q : b2QueryCallback {
+        std::vector<int32_t> ids;
+        bool ReportFixture(b2Fixture* f) override { ids.push_back(f->GetProxyIndex()); return true; }
+    } q;
+    b2AABB a;
+    a.lowerBound = b2Vec2(box[0], box[1]);
+    a.upperBound = b2Vec2(box[2], box[3]);
+    ((Runner*)h)->world.QueryAABB(&q, a);
+    for (size_t i = 0; i < q.ids.size() && (int)i < cap; ++i) out[i] = q.ids[i];
+    return (int)q.ids.size();
+}
+
+// b2World::RayCast with the closest-hit callback: out = fraction, normal,
+// point; returns the hit fixture's first proxy slot, -1 for none.
+int b2pp_world_raycast_closest(void* h, float x1, float y1, float x2, float y2, float* out) {
+    struct C : b2RayCastCallback {
+        int32_t id = -1;
+        float o[5] = {0, 0, 0, 0, 0};
+        float ReportFixture(b2Fixture* f, const b2Vec2& p, const b2Vec2& n, float fraction) override {
+            id = f->GetProxyIndex();
+            o[0] = fraction; o[1] = n[0]; o[2] = n[1]; o[3] = p[0]; o[4] = p[1];
+            return fraction;
+        }
+    } c;
+    ((Runner*)h)->world.RayCast(&c, b2Vec2(x1, y1), b2Vec2(x2, y2));
+    for (int k = 0; k < 5; ++k) out[k] = c.o[k];
+    return c.id;
+}
+
 // b2Body::SetType (0 static, 1 kinematic, 2 dynamic).
 int b2pp_world_set_type(void* h, int body, int type) {
     Runner* r = (Runner*)h;

@@ -285,4 +285,96 @@ void b2WorldManifold::Initialize(const b2Manifold* m, const b2Transform& xfA, fl
     }
 }
 
+// ------------------------------------------------------------ ray casts
+// The ray p1 + t (p2 - p1), t in [0, maxFraction], against each shape; the
+// reported normal points against the ray.
+
+// Circle (Box2D/Collision/Shapes/b2CircleShape.cpp:56-89): the smaller root
+// of |s + t r|^2 = radius^2, s = p1 - centre, r = p2 - p1.
+bool b2CircleShape::RayCast(b2RayCastOutput* output, const b2RayCastInput& input, const b2Transform& xf,
+                            int32_t childIndex) const {
+    (void)childIndex;
+    const b2Vec2 centre = xf.p + b2Mul(xf.q, m_p);
+    const b2Vec2 sv = input.p1 - centre;
+    const float b = b2Dot(sv, sv) - m_radius * m_radius;
+    const b2Vec2 r = input.p2 - input.p1;
+    const float c = b2Dot(sv, r);
+    const float rr = b2Dot(r, r);
+    const float sigma = c * c - rr * b;
+    if (sigma < 0.0f || rr < FLT_EPSILON) return false;
+    float a = -(c + b2Sqrt(sigma));
+    if (0.0f <= a && a <= input.maxFraction * rr) {
+        a /= rr;
+        output->fraction = a;
+        output->normal = sv + a * r;
+        b2Normalize(output->normal);
+        return true;
+    }
+    return false;
+}
+
+// Edge, both sides (Box2D/Collision/Shapes/b2EdgeShape.cpp:88-137): the ray
+// meets the edge's line at t, the hit must lie between the vertices.
+bool b2EdgeShape::RayCast(b2RayCastOutput* output, const b2RayCastInput& input, const b2Transform& xf,
+                          int32_t childIndex) const {
+    (void)childIndex;
+    const b2Vec2 p1 = b2MulT(xf.q, input.p1 - xf.p);
+    const b2Vec2 p2 = b2MulT(xf.q, input.p2 - xf.p);
+    const b2Vec2 d = p2 - p1;
+    const b2Vec2 e = m_vertex2 - m_vertex1;
+    b2Vec2 normal(e[1], -e[0]);
+    b2Normalize(normal);
+    const float numerator = b2Dot(normal, m_vertex1 - p1);
+    const float denominator = b2Dot(normal, d);
+    if (denominator == 0.0f) return false;
+    const float t = numerator / denominator;
+    if (t < 0.0f || input.maxFraction < t) return false;
+    const b2Vec2 q = p1 + t * d;
+    const float ee = b2Dot(e, e);
+    if (ee == 0.0f) return false;
+    const float sPos = b2Dot(q - m_vertex1, e) / ee;
+    if (sPos < 0.0f || 1.0f < sPos) return false;
+    output->fraction = t;
+    output->normal = numerator > 0.0f ? -b2Mul(xf.q, normal) : b2Mul(xf.q, normal);
+    return true;
+}
+
+// A chain child is its edge (Box2D/Collision/Shapes/b2ChainShape.cpp:145-163).
+bool b2ChainShape::RayCast(b2RayCastOutput* output, const b2RayCastInput& input, const b2Transform& xf,
+                           int32_t childIndex) const {
+    b2EdgeShape edge;
+    GetChildEdge(&edge, childIndex);
+    return edge.RayCast(output, input, xf, 0);
+}
+
+// Convex polygon (Box2D/Collision/Shapes/b2PolygonShape.cpp:279-333): the
+// ray clipped against each edge's half-plane in the polygon frame; the last
+// edge that raised the entry bound is the one hit.
+bool b2PolygonShape::RayCast(b2RayCastOutput* output, const b2RayCastInput& input, const b2Transform& xf,
+                             int32_t childIndex) const {
+    (void)childIndex;
+    const b2Vec2 p1 = b2MulT(xf.q, input.p1 - xf.p);
+    const b2Vec2 p2 = b2MulT(xf.q, input.p2 - xf.p);
+    const b2Vec2 d = p2 - p1;
+    float lower = 0.0f, upper = input.maxFraction;
+    int32_t index = -1;
+    for (int32_t i = 0; i < m_count; ++i) {
+        const float numerator = b2Dot(m_normals[i], m_vertices[i] - p1);
+        const float denominator = b2Dot(m_normals[i], d);
+        if (denominator == 0.0f) {
+            if (numerator < 0.0f) return false;
+        } else if (denominator < 0.0f && numerator < lower * denominator) {
+            lower = numerator / denominator;
+            index = i;
+        } else if (denominator > 0.0f && numerator < upper * denominator) {
+            upper = numerator / denominator;
+        }
+        if (upper < lower) return false;
+    }
+    if (index < 0) return false;
+    output->fraction = lower;
+    output->normal = b2Mul(xf.q, m_normals[index]);
+    return true;
+}
+
 }  // namespace box2d

@@ -505,6 +505,64 @@ void b2World::DestroyProxyId(int32_t id) {
     --m_treeLeaves;
 }
 
+void b2World::QueryAABB(b2QueryCallback* callback, const b2AABB& aabb) {
+    if (!callback || !m_ctx) return;
+    if (!IsLocked() && Upload() != B2G_OK) return;
+    const int n = (int)m_fixtures.size();
+    std::vector<b2g_aabb> fat(n);
+    if (n == 0 || b2g_get_fat_aabbs(m_ctx, fat.data(), n) != B2G_OK) return;
+    for (int i = 0; i < n; ++i) {
+        const b2g_aabb& f = fat[i];
+        if (!(f.lx <= f.ux && f.ly <= f.uy)) continue;  // no proxy
+        // b2TestOverlap (REF/Collision/b2Collision.h:241-252): touching counts
+        if (aabb.lowerBound[0] > f.ux || aabb.lowerBound[1] > f.uy || f.lx > aabb.upperBound[0] ||
+            f.ly > aabb.upperBound[1])
+            continue;
+        if (!callback->ReportFixture(m_fixtures[i])) return;
+    }
+}
+
+// b2World::RayCast over b2DynamicTree::RayCast (REF/Dynamics/b2World.cpp:1026-1040,
+// REF/Collision/b2DynamicTree.h:220-290): a proxy is tested when its fat box
+// meets the segment cut at the current maximum fraction; a hit's callback
+// value clips the segment (> 0), stops the cast (0) or is ignored (< 0).
+void b2World::RayCast(b2RayCastCallback* callback, const b2Vec2& point1, const b2Vec2& point2) {
+    if (!callback || !m_ctx) return;
+    if (!IsLocked() && Upload() != B2G_OK) return;
+    const int n = (int)m_fixtures.size();
+    std::vector<b2g_aabb> fat(n);
+    if (n == 0 || b2g_get_fat_aabbs(m_ctx, fat.data(), n) != B2G_OK) return;
+    Download();
+    const b2Vec2 r = point2 - point1;
+    b2Vec2 v = b2Cross(1.0f, r);
+    b2Normalize(v);
+    const b2Vec2 absV(fabsf(v[0]), fabsf(v[1]));
+    float maxFraction = 1.0f;
+    for (int i = 0; i < n; ++i) {
+        const b2g_aabb& f = fat[i];
+        if (!(f.lx <= f.ux && f.ly <= f.uy)) continue;
+        const b2Vec2 t = point1 + maxFraction * r;
+        const b2Vec2 lo = b2Min(point1, t), hi = b2Max(point1, t);
+        if (lo[0] > f.ux || lo[1] > f.uy || f.lx > hi[0] || f.ly > hi[1]) continue;
+        // separating axis along the segment's normal
+        const b2Vec2 c(0.5f * (f.lx + f.ux), 0.5f * (f.ly + f.uy));
+        const b2Vec2 h(0.5f * (f.ux - f.lx), 0.5f * (f.uy - f.ly));
+        if (fabsf(b2Dot(v, point1 - c)) - b2Dot(absV, h) > 0.0f) continue;
+        b2Fixture* fixture = m_fixtures[i];
+        b2RayCastInput input;
+        input.p1 = point1;
+        input.p2 = point2;
+        input.maxFraction = maxFraction;
+        b2RayCastOutput output;
+        if (!fixture->m_shape->RayCast(&output, input, fixture->m_body->m_xf, i - fixture->m_index)) continue;
+        const float fraction = output.fraction;
+        const b2Vec2 point = (1.0f - fraction) * point1 + fraction * point2;
+        const float value = callback->ReportFixture(fixture, point, output.normal, fraction);
+        if (value == 0.0f) return;
+        if (value > 0.0f) maxFraction = value;
+    }
+}
+
 // ------------------------------------------------------------ joints
 b2Joint::b2Joint(const b2JointDef* def, b2World* world, int32_t index)
     : m_type(def->type), m_bodyA(def->bodyA), m_bodyB(def->bodyB), m_world(world), m_index(index),

@@ -72,6 +72,17 @@ struct b2MassData {
     float I = 0.0f;
 };
 
+// Box2D/Collision/b2Collision.h:158-171: a ray from p1 through p2, cut at
+// p1 + maxFraction * (p2 - p1); a hit's normal and fraction along it.
+struct b2RayCastInput {
+    b2Vec2 p1, p2;
+    float maxFraction = 1.0f;
+};
+struct b2RayCastOutput {
+    b2Vec2 normal;
+    float fraction = 0.0f;
+};
+
 class b2Shape {
 public:
     enum Type { e_circle = 0, e_edge = 1, e_polygon = 2, e_chain = 3, e_typeCount = 4 };
@@ -80,6 +91,11 @@ public:
     virtual int32_t GetChildCount() const { return 1; }
     virtual void ComputeAABB(b2AABB* aabb, const b2Transform& xf, int32_t childIndex) const = 0;
     virtual void ComputeMass(b2MassData* massData, float density) const = 0;
+    // Box2D/Collision/Shapes/b2Shape.h:83-89: the ray against the shape's
+    // child at transform xf (the shapes' own RayCast in b2CircleShape.cpp,
+    // b2EdgeShape.cpp, b2PolygonShape.cpp, b2ChainShape.cpp).
+    virtual bool RayCast(b2RayCastOutput* output, const b2RayCastInput& input, const b2Transform& xf,
+                         int32_t childIndex) const = 0;
     Type GetType() const { return m_type; }
     Type m_type = e_circle;
     float m_radius = 0.0f;
@@ -91,6 +107,8 @@ public:
     b2Shape* Clone() const override { return new b2CircleShape(*this); }
     void ComputeAABB(b2AABB* aabb, const b2Transform& xf, int32_t childIndex) const override;
     void ComputeMass(b2MassData* massData, float density) const override;
+    bool RayCast(b2RayCastOutput* output, const b2RayCastInput& input, const b2Transform& xf,
+                 int32_t childIndex) const override;
     b2Vec2 m_p;
 };
 
@@ -106,6 +124,8 @@ public:
     b2Shape* Clone() const override { return new b2EdgeShape(*this); }
     void ComputeAABB(b2AABB* aabb, const b2Transform& xf, int32_t childIndex) const override;
     void ComputeMass(b2MassData* massData, float density) const override;
+    bool RayCast(b2RayCastOutput* output, const b2RayCastInput& input, const b2Transform& xf,
+                 int32_t childIndex) const override;
     b2Vec2 m_vertex1, m_vertex2;
     b2Vec2 m_vertex0, m_vertex3;
     bool m_hasVertex0 = false, m_hasVertex3 = false;
@@ -141,6 +161,8 @@ public:
     // Child AABB: the segment's box without the skin (b2ChainShape::ComputeAABB).
     void ComputeAABB(b2AABB* aabb, const b2Transform& xf, int32_t childIndex) const override;
     void ComputeMass(b2MassData* massData, float density) const override;
+    bool RayCast(b2RayCastOutput* output, const b2RayCastInput& input, const b2Transform& xf,
+                 int32_t childIndex) const override;
     std::vector<b2Vec2> m_vertices;
     int32_t m_count = 0;
     b2Vec2 m_prevVertex, m_nextVertex;
@@ -158,6 +180,8 @@ public:
     void SetAsBox(float hx, float hy, const b2Vec2& center, float angle);
     void ComputeAABB(b2AABB* aabb, const b2Transform& xf, int32_t childIndex) const override;
     void ComputeMass(b2MassData* massData, float density) const override;
+    bool RayCast(b2RayCastOutput* output, const b2RayCastInput& input, const b2Transform& xf,
+                 int32_t childIndex) const override;
     int32_t GetVertexCount() const { return m_count; }
     const b2Vec2& GetVertex(int32_t i) const { return m_vertices[i]; }
     b2Vec2 m_centroid;

@@ -830,6 +830,18 @@ struct b2SolverOptions {
                                       // start leaves a larger RMS residual (m/s); 0: no check
 };
 
+// Box2D/Dynamics/b2WorldCallbacks.h:140-162
+class b2QueryCallback {
+public:
+    virtual ~b2QueryCallback() {}
+    virtual bool ReportFixture(b2Fixture* fixture) = 0;
+};
+class b2RayCastCallback {
+public:
+    virtual ~b2RayCastCallback() {}
+    virtual float ReportFixture(b2Fixture* fixture, const b2Vec2& point, const b2Vec2& normal, float fraction) = 0;
+};
+
 class b2World {
 public:
     explicit b2World(const b2Vec2& gravity);
@@ -884,6 +896,13 @@ public:
     // Mean and max penetration over touching points and the point count,
     // from device world manifolds.
     void GetPenetration(double* meanMax3) const;
+    // Box2D/Dynamics/b2World.h:115-123: every proxy whose fat box overlaps
+    // the box, and every fixture child the segment hits (the callback's
+    // return clips, ignores with -1, or stops with 0). The device's fat
+    // boxes are tested by brute force on the host, so the reports are the
+    // reference's but in proxy order rather than its tree's order.
+    void QueryAABB(b2QueryCallback* callback, const b2AABB& aabb);
+    void RayCast(b2RayCastCallback* callback, const b2Vec2& point1, const b2Vec2& point2);
     // Per fixture proxy (chain edges one each) in creation order: the id the
     // reference's broad-phase tree gives it, -1 once destroyed (or on an
     // inactive body).

@@ -624,6 +624,34 @@ int ref_world_destroy_body(void* h, int body) {
     return 0;
 }
 
+int ref_world_query_aabb(void* h, const float* box, int32_t* out, int cap) {
+    struct Q : b2QueryCallback {
+        std::vector<int32_t> ids;
+        bool ReportFixture(b2Fixture* f) override { ids.push_back(fixtureIndex(f)); return true; }
+    } q;
+    b2AABB a;
+    a.lowerBound.Set(box[0], box[1]);
+    a.upperBound.Set(box[2], box[3]);
+    ((RefWorld*)h)->world->QueryAABB(&q, a);
+    for (size_t i = 0; i < q.ids.size() && (int)i < cap; ++i) out[i] = q.ids[i];
+    return (int)q.ids.size();
+}
+
+int ref_world_raycast_closest(void* h, float x1, float y1, float x2, float y2, float* out) {
+    struct C : b2RayCastCallback {
+        int32_t id = -1;
+        float o[5] = {0, 0, 0, 0, 0};
+        float32 ReportFixture(b2Fixture* f, const b2Vec2& p, const b2Vec2& n, float32 fraction) override {
+            id = fixtureIndex(f);
+            o[0] = fraction; o[1] = n.x; o[2] = n.y; o[3] = p.x; o[4] = p.y;
+            return fraction;
+        }
+    } c;
+    ((RefWorld*)h)->world->RayCast(&c, b2Vec2(x1, y1), b2Vec2(x2, y2));
+    for (int k = 0; k < 5; ++k) out[k] = c.o[k];
+    return c.id;
+}
+
 int ref_world_set_active(void* h, int body, int flag) {
     RefWorld* w = (RefWorld*)h;
     if (body < 0 || body >= (int)w->bodies.size() || w->bodies[body] == nullptr) return -1;
