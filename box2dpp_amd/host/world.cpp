@@ -441,8 +441,13 @@ b2Body* b2World::CreateBody(const b2BodyDef* def) {
 void b2World::DestroyBody(b2Body* b) {
     if (IsLocked() || b == nullptr || b->m_world != this || b->m_destroyed) return;
     b->m_destroyed = true;
-    // Attached joints go first (REF/Dynamics/b2World.cpp:134-147).
-    while (b->m_jointList) DestroyJoint(b->m_jointList->joint);
+    // Attached joints go first (REF/Dynamics/b2World.cpp:142-157), each
+    // announced to the destruction listener.
+    while (b->m_jointList) {
+        b2Joint* j = b->m_jointList->joint;
+        if (m_destructionListener) m_destructionListener->SayGoodbye(j);
+        DestroyJoint(j);
+    }
     b->Touch();
     if (b->m_prev) b->m_prev->m_next = b->m_next;
     if (b->m_next) b->m_next->m_prev = b->m_prev;
@@ -456,7 +461,8 @@ void b2World::DestroyBody(b2Body* b) {
     b->MarkDirty();
     // fixtures newest first, each one's proxies in child order
     // (REF/Dynamics/b2World.cpp:170-188, b2Fixture.cpp:141-152)
-    for (b2Fixture* f = b->m_fixtureList; f; f = f->m_next)
+    for (b2Fixture* f = b->m_fixtureList; f; f = f->m_next) {
+        if (m_destructionListener) m_destructionListener->SayGoodbye(f);
         for (int32_t c = 0; c < f->m_shape->GetChildCount(); ++c) {
             m_removedFixtures.push_back(f->m_index + c);
             int32_t& id = m_proxyIds[f->m_index + c];
@@ -466,6 +472,7 @@ void b2World::DestroyBody(b2Body* b) {
                 m_proxyIdsReused = m_proxyIdsDirty = true;
             }
         }
+    }
     --m_bodyCount;
     m_contactsStale = true;
 }

@@ -830,6 +830,15 @@ struct b2SolverOptions {
                                       // start leaves a larger RMS residual (m/s); 0: no check
 };
 
+// Box2D/Dynamics/b2WorldCallbacks.h:40-57: told of joints and fixtures that
+// DestroyBody destroys implicitly.
+class b2DestructionListener {
+public:
+    virtual ~b2DestructionListener() {}
+    virtual void SayGoodbye(b2Joint* joint) = 0;
+    virtual void SayGoodbye(b2Fixture* fixture) = 0;
+};
+
 // Box2D/Dynamics/b2WorldCallbacks.h:140-162
 class b2QueryCallback {
 public:
@@ -850,6 +859,7 @@ public:
     b2World& operator=(const b2World&) = delete;
 
     void SetContactFilter(b2ContactFilter* filter) { m_contactFilter = filter; }
+    void SetDestructionListener(b2DestructionListener* listener) { m_destructionListener = listener; }
     void SetContactListener(b2ContactListener* listener) { m_contactListener = listener; }
     b2Body* CreateBody(const b2BodyDef* def);
     void DestroyBody(b2Body* body);
@@ -973,6 +983,7 @@ private:
     mutable bool m_hostStale = false;
     bool m_anyDirty = false;
     b2ContactFilter* m_contactFilter = nullptr;
+    b2DestructionListener* m_destructionListener = nullptr;
     b2ContactListener* m_contactListener = nullptr;
     b2Profile m_profile;
     b2SolverOptions m_options;

@@ -1,7 +1,8 @@
 // b2Body calls beyond HelloWorld's, through the box2d:: headers on the GPU
 // path: per-body contact lists (Box2D/Dynamics/b2Body.h:301-305), SetMassData,
 // SetFixedRotation, SetBullet and GetLinearVelocityFromLocalPoint
-// (b2Body.h:260-276, 583-586, 618-628). Prints "FAIL ..." and exits 1 on
+// (b2Body.h:260-276, 583-586, 618-628), and the destruction listener
+// (b2WorldCallbacks.h:40-57). Prints "FAIL ..." and exits 1 on
 // the first broken fact, "ok" otherwise.
 #include <box2d/Box2D.h>
 
@@ -12,6 +13,12 @@
 using namespace box2d;
 
 static int fails = 0;
+
+struct Goodbyes : b2DestructionListener {
+    int joints = 0, fixtures = 0;
+    void SayGoodbye(b2Joint*) override { ++joints; }
+    void SayGoodbye(b2Fixture*) override { ++fixtures; }
+};
 static void check(bool ok, const char* what) {
     if (!ok) {
         std::printf("FAIL %s\n", what);
@@ -104,6 +111,21 @@ int main() {
     check(!boxes[0]->IsBullet(), "SetBullet off");
     for (int s = 0; s < 10; ++s) world.Step(1.0f / 60.0f, 8, 3);
     check(world.GetLastStatus() == 0, "steps after the edits");
+
+    // DestroyBody tells the destruction listener of its joints and fixtures.
+    Goodbyes bye;
+    world.SetDestructionListener(&bye);
+    b2Body* hinged = boxes[1];
+    b2RevoluteJointDef jd;
+    jd.Initialize(ground, hinged, hinged->GetPosition());
+    world.CreateJoint(&jd);
+    b2CircleShape extra;
+    extra.m_radius = 0.2f;
+    hinged->CreateFixture(&extra, 1.0f);
+    world.DestroyBody(hinged);
+    check(bye.joints == 1 && bye.fixtures == 2, "destruction listener goodbyes");
+    for (int s = 0; s < 5; ++s) world.Step(1.0f / 60.0f, 8, 3);
+    check(world.GetLastStatus() == 0, "steps after the destruction");
 
     if (fails) return 1;
     std::printf("ok %d edges\n", edges);
