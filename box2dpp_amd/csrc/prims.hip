@@ -9,8 +9,8 @@
 //    digits before it in the tile (lanes with an equal digit found by one
 //    ballot per digit bit, then the earlier wavefronts' counts from LDS);
 //  * the same kernels for one pass over 7-bit keys (the colour order);
-//  * up to 4096 64-bit keys: one workgroup's bitonic network in LDS; up to
-//    2^17, such chunks merged pairwise by merge-path passes.
+//  * up to 4096 64-bit keys: one workgroup's merge sort in LDS; up to 2^17,
+//    such chunks merged pairwise by merge-path passes.
 #include "prims.h"
 
 #include <algorithm>
@@ -266,65 +266,65 @@ hipError_t radixSort(void* scratch, size_t bytes, const K* kin, K* kout, const i
 constexpr int kSmallSortMax = 4096;
 constexpr int kSmallSortThreads = 1024;
 
-// Bitonic sort of each kSmallSortMax-key chunk of the input (one workgroup
-// per chunk; a partial chunk padded with the largest key, which sorts last
-// and is not written back). Each lane holds kChunkItems consecutive keys in
-// registers: compare-exchanges between keys of one lane (stride 1, 2) stay in
-// registers, those between lanes of one wavefront (stride 4-128) go through
-// lane shuffles, and only strides of 256 and up go through LDS.
+// Sort of each kSmallSortMax-key chunk of the input in one workgroup (a
+// partial chunk padded with the largest key, which sorts last and is not
+// written back): each lane sorts its kChunkItems keys in registers, then
+// runs of 4, 8, ... keys are merged pairwise in LDS, each lane finding where
+// the merge path crosses its first output by a binary search over the
+// diagonal and writing kChunkItems outputs.
 constexpr int kChunkItems = kSmallSortMax / kSmallSortThreads;
 static_assert(kChunkItems == 4, "four keys per lane");
 
-__device__ inline unsigned long long bitonicPick(unsigned long long x, unsigned long long y, int i, int j, int k) {
-    const bool lower = (i & j) == 0, ascending = (i & k) == 0;
-    const bool keepMin = lower == ascending;
-    return keepMin ? (x < y ? x : y) : (x < y ? y : x);
+__device__ inline void sortPair(unsigned long long& a, unsigned long long& b) {
+    const unsigned long long lo = a < b ? a : b, hi = a < b ? b : a;
+    a = lo;
+    b = hi;
 }
 
 __global__ void __launch_bounds__(kSmallSortThreads) k_sort_chunks_u64(const unsigned long long* in,
                                                                        unsigned long long* out, int n) {
-    __shared__ unsigned long long a[kSmallSortMax];
+    __shared__ unsigned long long buf[2][kSmallSortMax];
     const int base = blockIdx.x * kSmallSortMax;
     const int len = n - base < kSmallSortMax ? n - base : kSmallSortMax;
     const int t = threadIdx.x;
+    const int k0 = kChunkItems * t;
     unsigned long long x[kChunkItems];
 #pragma unroll
-    for (int r = 0; r < kChunkItems; ++r) {
-        const int i = kChunkItems * t + r;
-        x[r] = i < len ? in[base + i] : ~0ull;
-    }
-    for (int k = 2; k <= kSmallSortMax; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 64 * kChunkItems) {
+    for (int r = 0; r < kChunkItems; ++r) x[r] = k0 + r < len ? in[base + k0 + r] : ~0ull;
+    sortPair(x[0], x[1]);
+    sortPair(x[2], x[3]);
+    sortPair(x[0], x[2]);
+    sortPair(x[1], x[3]);
+    sortPair(x[1], x[2]);
 #pragma unroll
-                for (int r = 0; r < kChunkItems; ++r) a[kChunkItems * t + r] = x[r];
-                __syncthreads();
-#pragma unroll
-                for (int r = 0; r < kChunkItems; ++r) {
-                    const int i = kChunkItems * t + r;
-                    x[r] = bitonicPick(x[r], a[i ^ j], i, j, k);
-                }
-                __syncthreads();
-            } else if (j >= kChunkItems) {
-#pragma unroll
-                for (int r = 0; r < kChunkItems; ++r) {
-                    const unsigned long long y = __shfl_xor(x[r], j / kChunkItems);
-                    x[r] = bitonicPick(x[r], y, kChunkItems * t + r, j, k);
-                }
-            } else {
-                unsigned long long y[kChunkItems];
-#pragma unroll
-                for (int r = 0; r < kChunkItems; ++r) y[r] = x[r ^ j];
-#pragma unroll
-                for (int r = 0; r < kChunkItems; ++r) x[r] = bitonicPick(x[r], y[r], kChunkItems * t + r, j, k);
-            }
+    for (int r = 0; r < kChunkItems; ++r) buf[0][k0 + r] = x[r];
+    __syncthreads();
+    int cur = 0;
+    for (int w = kChunkItems; w < kSmallSortMax; w *= 2) {
+        const unsigned long long* src = buf[cur];
+        unsigned long long* dst = buf[cur ^ 1];
+        const int pairBase = k0 & ~(2 * w - 1);
+        const unsigned long long* A = src + pairBase;
+        const unsigned long long* B = A + w;
+        const int d = k0 - pairBase;
+        int lo = d - w > 0 ? d - w : 0, hi = d < w ? d : w;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (A[mid] <= B[d - 1 - mid]) lo = mid + 1;
+            else hi = mid;
         }
+        int i = lo, j = d - lo;
+#pragma unroll
+        for (int r = 0; r < kChunkItems; ++r) {
+            const bool takeA = i < w && (j >= w || A[i] <= B[j]);
+            dst[k0 + r] = takeA ? A[i++] : B[j++];
+        }
+        __syncthreads();
+        cur ^= 1;
     }
 #pragma unroll
-    for (int r = 0; r < kChunkItems; ++r) {
-        const int i = kChunkItems * t + r;
-        if (i < len) out[base + i] = x[r];
-    }
+    for (int r = 0; r < kChunkItems; ++r)
+        if (k0 + r < len) out[base + k0 + r] = buf[cur][k0 + r];
 }
 
 // One pass of a merge sort: sorted runs of `width` keys merged in pairs.
